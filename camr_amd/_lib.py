@@ -1,0 +1,135 @@
+"""ctypes binding of ``libcamr_hydro.so`` (the C-ABI in include/camr_hydro.h).
+
+The shared library is built in-tree by ``camr_amd/build.py`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback for the hydro path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcamr_hydro.so")
+
+NVAR, QVAR, NQAUX, NGROW = 10, 11, 5, 4
+
+# status codes (camr_status)
+OK, ERR_PARAM, ERR_SHAPE, ERR_NEG_ENERGY, ERR_ZERO_DENSITY, ERR_DT, ERR_HIP, ERR_ALLOC = range(8)
+
+# physical BC codes (camr_bc)
+INTERIOR, INFLOW, OUTFLOW, SYMMETRY, SLIPWALL, NOSLIPWALL = range(6)
+
+ENGINE_AUTO, ENGINE_STAGED, ENGINE_FUSED = 0, 1, 2
+
+
+class Box(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3)]
+
+
+class Array4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64),
+                ("kstride", C.c_int64), ("nstride", C.c_int64),
+                ("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3),
+                ("ncomp", C.c_int32)]
+
+
+class HydroParams(C.Structure):
+    _fields_ = [
+        ("ppm_type", C.c_int32), ("plm_iorder", C.c_int32),
+        ("use_pslope", C.c_int32), ("use_flattening", C.c_int32),
+        ("transverse_reset_density", C.c_int32),
+        ("allow_negative_energy", C.c_int32),
+        ("allow_small_energy", C.c_int32),
+        ("dual_energy_update_E_from_e", C.c_int32),
+        ("small", C.c_double), ("small_dens", C.c_double),
+        ("small_pres", C.c_double), ("small_temp", C.c_double),
+        ("smallu", C.c_double), ("small_num", C.c_double),
+        ("difmag", C.c_double), ("dual_energy_eta1", C.c_double),
+        ("dual_energy_eta2", C.c_double),
+        ("bclo", C.c_int32 * 3), ("bchi", C.c_int32 * 3),
+        ("domlo", C.c_int32 * 3), ("domhi", C.c_int32 * 3),
+        ("dx", C.c_double * 3),
+    ]
+
+
+class LevelDesc(C.Structure):
+    _fields_ = [
+        ("hp", HydroParams), ("is_periodic", C.c_int32 * 3),
+        ("add_grav_src", C.c_int32), ("const_grav", C.c_double),
+        ("cfl", C.c_double), ("change_max", C.c_double),
+        ("init_shrink", C.c_double), ("max_dt", C.c_double),
+        ("fixed_dt", C.c_double), ("stop_time", C.c_double),
+        ("nbox", C.c_int32), ("box_lohi", C.POINTER(C.c_int32)),
+        ("engine", C.c_int32),
+    ]
+
+
+# Every extern "C" symbol declared in include/camr_hydro.h.
+EXPORTS = [
+    "camr_default_params", "camr_last_error", "camr_version",
+    "camr_hydro_ctoprim", "camr_hydro_srctoprim", "camr_hydro_umdrv",
+    "camr_estdt_hydro", "camr_default_level_desc", "camr_level_create",
+    "camr_level_destroy", "camr_level_set_stream", "camr_level_upload_box",
+    "camr_level_download_box", "camr_level_state_array",
+    "camr_level_init_post", "camr_level_initial_dt", "camr_level_new_dt",
+    "camr_level_advance", "camr_level_run", "camr_level_sync",
+    "camr_level_time",
+]
+
+_lib = None
+
+
+class CamrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"camr status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is not built; run camr_amd/build.py "
+                          "(there is no CPU fallback for the hydro path)")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.camr_default_params.argtypes = [C.POINTER(HydroParams)]
+        L.camr_default_params.restype = None
+        L.camr_default_level_desc.argtypes = [C.POINTER(LevelDesc)]
+        L.camr_default_level_desc.restype = None
+        L.camr_last_error.restype = C.c_char_p
+        L.camr_version.restype = ip
+        A4 = C.POINTER(Array4)
+        L.camr_hydro_ctoprim.argtypes = [C.POINTER(Box), A4, A4, A4, C.POINTER(HydroParams), vp]
+        L.camr_hydro_srctoprim.argtypes = [C.POINTER(Box), A4, A4, A4, A4, vp]
+        L.camr_hydro_umdrv.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4, A4, A4,
+                                       C.c_double, C.POINTER(HydroParams), vp]
+        L.camr_estdt_hydro.argtypes = [C.POINTER(Box), A4, dp, dp, vp]
+        L.camr_level_create.argtypes = [C.POINTER(LevelDesc), C.POINTER(vp)]
+        L.camr_level_destroy.argtypes = [vp]
+        L.camr_level_set_stream.argtypes = [vp, vp]
+        L.camr_level_upload_box.argtypes = [vp, C.c_int32, dp]
+        L.camr_level_download_box.argtypes = [vp, C.c_int32, dp]
+        L.camr_level_state_array.argtypes = [vp, C.c_int32, A4]
+        L.camr_level_init_post.argtypes = [vp]
+        L.camr_level_initial_dt.argtypes = [vp, dp]
+        L.camr_level_new_dt.argtypes = [vp, C.c_double, dp]
+        L.camr_level_advance.argtypes = [vp, C.c_double]
+        L.camr_level_run.argtypes = [vp, C.c_int32, dp]
+        L.camr_level_sync.argtypes = [vp]
+        L.camr_level_time.argtypes = [vp]
+        L.camr_level_time.restype = C.c_double
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if f.restype is C.c_int:  # default restype
+                pass
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = lib().camr_last_error().decode(errors="replace")
+        raise CamrError(rc, msg)
+    return rc

@@ -1,0 +1,744 @@
+// camr_level.cpp -- host side of the MI355X hydro advance: the C-ABI entry
+// points of include/camr_hydro.h and the level driver that mirrors
+// CAMR::CAMR_advance (Source/CAMR_advance.cpp:38-164),
+// CAMR::construct_hydro_source (Hydro/CAMR_construct_hydro_source.cpp:9-267),
+// CAMR::clean_state / expand_state (CAMR.cpp:1243-1262) and the dt control
+// of CAMR::estTimeStep / computeNewDt / computeInitialDt (CAMR.cpp:474-676).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "hydro_state.h"
+
+namespace camr {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+#define CAMR_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            set_error("HIP error %s at %s:%d", hipGetErrorString(e_),         \
+                      __FILE__, __LINE__);                                    \
+            return CAMR_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+Scratch::~Scratch() {
+    if (base_) (void)hipFree(base_);
+}
+bool Scratch::reserve(size_t bytes) {
+    if (bytes <= cap_) return true;
+    if (base_) (void)hipFree(base_);
+    base_ = nullptr;
+    cap_ = 0;
+    if (hipMalloc(&base_, bytes) != hipSuccess) { base_ = nullptr; return false; }
+    cap_ = bytes;
+    return true;
+}
+double *Scratch::take(size_t n) {
+    size_t bytes = ((n * sizeof(double)) + 255) & ~size_t(255);
+    if (used_ + bytes > cap_) return nullptr;
+    double *p = (double *)(base_ + used_);
+    used_ += bytes;
+    return p;
+}
+
+HydroConst make_const(const camr_hydro_params &p) {
+    HydroConst h;
+    h.ppm_type = p.ppm_type;
+    h.plm_iorder = p.plm_iorder;
+    h.use_pslope = p.use_pslope;
+    h.use_flattening = p.use_flattening;
+    h.trd = p.transverse_reset_density;
+    h.allow_negative_energy = p.allow_negative_energy;
+    h.allow_small_energy = p.allow_small_energy;
+    h.update_E_from_e = p.dual_energy_update_E_from_e;
+    h.small = p.small;
+    h.small_dens = p.small_dens;
+    h.small_pres = p.small_pres;
+    h.small_temp = p.small_temp;
+    h.smallu = p.smallu;
+    h.small_num = p.small_num;
+    h.difmag = p.difmag;
+    h.eta1 = p.dual_energy_eta1;
+    h.eta2 = p.dual_energy_eta2;
+    for (int d = 0; d < 3; ++d) {
+        h.bclo[d] = p.bclo[d];
+        h.bchi[d] = p.bchi[d];
+        h.domlo[d] = p.domlo[d];
+        h.domhi[d] = p.domhi[d];
+        h.dx[d] = p.dx[d];
+    }
+    return h;
+}
+
+static int check_params(const camr_hydro_params &p) {
+    if (p.ppm_type != 0 && p.ppm_type != 1) {
+        set_error("ppm_type must be 0 (PLM) or 1 (PPM)");  // Godunov_3D.cpp:174-176
+        return CAMR_ERR_PARAM;
+    }
+    if (p.plm_iorder != 1 && p.plm_iorder != 2 && p.plm_iorder != 4) {
+        set_error("plm_iorder must be 1, 2 or 4");
+        return CAMR_ERR_PARAM;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (!(p.dx[d] > 0.0)) { set_error("dx must be positive"); return CAMR_ERR_PARAM; }
+        if (p.bclo[d] < 0 || p.bclo[d] > 5 || p.bchi[d] < 0 || p.bchi[d] > 5) {
+            set_error("invalid physical BC code");
+            return CAMR_ERR_PARAM;
+        }
+    }
+    return CAMR_OK;
+}
+
+static bool covers(const camr_array4 &a, const Box &b, int nc) {
+    return a.p != nullptr && box_of(a).contains(b) && a.ncomp >= nc;
+}
+
+// ============================================================ level driver
+class HydroLevel {
+  public:
+    explicit HydroLevel(const camr_level_desc &d) : desc_(d) {}
+    ~HydroLevel() { release(); }
+
+    int create();
+    int upload(int b, const double *h);
+    int download(int b, double *h);
+    int state_array(int b, camr_array4 *out);
+    int init_post();
+    int initial_dt(double *dt);
+    int new_dt(double dt_old, double *dt);
+    int advance(double dt);
+    int sync();
+    void set_stream(hipStream_t s) { stream_ = s; }
+    double time() const { return time_; }
+
+  private:
+    int est_time_step(double *est);
+    int clean_state(std::vector<Arr> &S, bool launch_detect);
+    int expand_state();
+    int construct_hydro_source(double dt);
+    int check_status();
+    void release();
+    Arr valid_view(double *p, const Box &b) const { return dense(p, b); }
+
+    camr_level_desc desc_;
+    HydroConst hc_{};
+    DomInfo dom_{};
+    std::vector<Box> box_;
+    std::vector<double *> bufA_, bufB_, sb_, dsdt_, srchyd_;
+    bool a_is_new_ = true;
+    std::vector<Arr> snew_, sold_, sbv_;
+    DevFab *tab_old_ = nullptr, *tab_sb_ = nullptr, *tab_src_ = nullptr;
+    int *owner_ = nullptr;
+    int *status_ = nullptr;              // device: error bits | floor flag
+    unsigned long long *dtmin_ = nullptr;  // device: bit pattern of min dt
+    int *h_status_ = nullptr;            // pinned host mirrors
+    unsigned long long *h_dtmin_ = nullptr;
+    double *q_ = nullptr, *qaux_ = nullptr, *srcq_ = nullptr, *flx_[3] = {};
+    Scratch scratch_;
+    hipStream_t stream_ = nullptr;
+    double time_ = 0.0;
+    bool grav_src_values_ = false;
+};
+
+void HydroLevel::release() {
+    auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    for (auto p : bufA_) fr(p);
+    for (auto p : bufB_) fr(p);
+    for (auto p : sb_) fr(p);
+    for (auto p : dsdt_) fr(p);
+    for (auto p : srchyd_) fr(p);
+    bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear();
+    fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(owner_); fr(status_); fr(dtmin_);
+    fr(q_); fr(qaux_); fr(srcq_);
+    for (auto &f : flx_) { fr(f); f = nullptr; }
+    if (h_status_) (void)hipHostFree(h_status_);
+    if (h_dtmin_) (void)hipHostFree(h_dtmin_);
+    tab_old_ = tab_sb_ = tab_src_ = nullptr;
+    owner_ = status_ = nullptr;
+    dtmin_ = nullptr;
+    h_status_ = nullptr;
+    h_dtmin_ = nullptr;
+    q_ = qaux_ = srcq_ = nullptr;
+}
+
+int HydroLevel::create() {
+    int rc = check_params(desc_.hp);
+    if (rc) return rc;
+    if (desc_.nbox <= 0 || !desc_.box_lohi) { set_error("no boxes"); return CAMR_ERR_PARAM; }
+    if (!(desc_.cfl > 0.0 && desc_.cfl <= 1.0)) {
+        set_error("Invalid CFL factor; must be between zero and one.");  // CAMR.cpp:173-174
+        return CAMR_ERR_PARAM;
+    }
+    hc_ = make_const(desc_.hp);
+    for (int d = 0; d < 3; ++d) {
+        dom_.domlo[d] = desc_.hp.domlo[d];
+        dom_.domhi[d] = desc_.hp.domhi[d];
+        dom_.periodic[d] = desc_.is_periodic[d];
+        dom_.bclo[d] = desc_.hp.bclo[d];
+        dom_.bchi[d] = desc_.hp.bchi[d];
+        if (desc_.is_periodic[d] && (desc_.hp.bclo[d] != 0 || desc_.hp.bchi[d] != 0)) {
+            set_error("periodic direction needs Interior BCs");  // CAMR.cpp:129-143
+            return CAMR_ERR_PARAM;
+        }
+        if (!desc_.is_periodic[d] && (desc_.hp.bclo[d] == 0 || desc_.hp.bchi[d] == 0)) {
+            set_error("Interior BC on a non-periodic direction");
+            return CAMR_ERR_PARAM;
+        }
+    }
+    const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
+    const int nb = desc_.nbox;
+    size_t max_g4 = 0, max_valid = 0, max_nodes = 0;
+    Box maxbox{};
+    for (int b = 0; b < nb; ++b) {
+        Box B = make_box(desc_.box_lohi + 6 * b, desc_.box_lohi + 6 * b + 3);
+        if (!B.ok() || !dom.contains(B)) { set_error("box %d outside the domain", b); return CAMR_ERR_PARAM; }
+        box_.push_back(B);
+        max_g4 = std::max<size_t>(max_g4, grow(B, CAMR_NGROW).npts());
+        max_valid = std::max<size_t>(max_valid, B.npts());
+        for (int d = 0; d < 3; ++d) max_nodes = std::max<size_t>(max_nodes, nodes(B, d).npts());
+        if (B.npts() > maxbox.npts()) maxbox = B;
+    }
+    grav_src_values_ = desc_.add_grav_src && desc_.const_grav != 0.0;
+    auto alloc = [&](double **p, size_t n) -> int {
+        if (hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess) {
+            set_error("device allocation of %zu doubles failed", n);
+            return CAMR_ERR_ALLOC;
+        }
+        return CAMR_OK;
+    };
+    for (int b = 0; b < nb; ++b) {
+        double *p;
+        const Box B = box_[b];
+        if ((rc = alloc(&p, NVAR * B.npts()))) return rc;
+        bufA_.push_back(p);
+        if ((rc = alloc(&p, NVAR * B.npts()))) return rc;
+        bufB_.push_back(p);
+        if ((rc = alloc(&p, NVAR * grow(B, CAMR_NGROW).npts()))) return rc;
+        sb_.push_back(p);
+        if ((rc = alloc(&p, NVAR * B.npts()))) return rc;
+        dsdt_.push_back(p);
+        if (grav_src_values_) {
+            if ((rc = alloc(&p, NVAR * grow(B, CAMR_NGROW).npts()))) return rc;
+            srchyd_.push_back(p);
+        }
+        sbv_.push_back(dense(sb_[b], grow(B, CAMR_NGROW)));
+    }
+    snew_.resize(nb);
+    sold_.resize(nb);
+    for (int b = 0; b < nb; ++b) {
+        snew_[b] = dense(bufA_[b], box_[b]);
+        sold_[b] = dense(bufB_[b], box_[b]);
+    }
+    // owner map over the domain
+    const size_t ncell = dom.npts();
+    std::vector<int> own(ncell, -1);
+    const int64_t n0 = dom.len(0), n1 = dom.len(1);
+    for (int b = 0; b < nb; ++b) {
+        const Box B = box_[b];
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                    int &o = own[(i - dom.lo[0]) + (j - dom.lo[1]) * n0 + (int64_t)(k - dom.lo[2]) * n0 * n1];
+                    if (o >= 0) { set_error("boxes %d and %d overlap", o, b); return CAMR_ERR_PARAM; }
+                    o = b;
+                }
+    }
+    CAMR_HIP(hipMalloc(&owner_, ncell * sizeof(int)));
+    CAMR_HIP(hipMemcpy(owner_, own.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+    CAMR_HIP(hipMalloc(&tab_old_, nb * sizeof(DevFab)));
+    CAMR_HIP(hipMalloc(&tab_sb_, nb * sizeof(DevFab)));
+    if (grav_src_values_) CAMR_HIP(hipMalloc(&tab_src_, nb * sizeof(DevFab)));
+    std::vector<DevFab> t(nb);
+    for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
+    CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    if (grav_src_values_) {
+        for (int b = 0; b < nb; ++b) t[b].a = dense(srchyd_[b], grow(box_[b], CAMR_NGROW));
+        CAMR_HIP(hipMemcpy(tab_src_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    }
+    CAMR_HIP(hipMalloc(&status_, sizeof(int)));
+    CAMR_HIP(hipMalloc(&dtmin_, sizeof(unsigned long long)));
+    CAMR_HIP(hipMemset(status_, 0, sizeof(int)));
+    CAMR_HIP(hipHostMalloc(&h_status_, sizeof(int)));
+    CAMR_HIP(hipHostMalloc(&h_dtmin_, sizeof(unsigned long long)));
+    {
+        if ((rc = alloc(&q_, QVAR * max_g4))) return rc;
+        if ((rc = alloc(&qaux_, NQAUX * max_g4))) return rc;
+        if ((rc = alloc(&srcq_, QVAR * max_g4))) return rc;
+        for (int d = 0; d < 3; ++d)
+            if ((rc = alloc(&flx_[d], NVAR * max_nodes))) return rc;
+        size_t sc = 0;
+        for (const Box &B : box_) sc = std::max(sc, umdrv_staged_scratch_doubles(B));
+        if (!scratch_.reserve(sc * sizeof(double))) {
+            set_error("scratch allocation of %zu bytes failed", sc * sizeof(double));
+            return CAMR_ERR_ALLOC;
+        }
+    }
+    return CAMR_OK;
+}
+
+int HydroLevel::upload(int b, const double *h) {
+    if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    CAMR_HIP(hipMemcpy(snew_[b].p, h, NVAR * box_[b].npts() * sizeof(double), hipMemcpyHostToDevice));
+    return CAMR_OK;
+}
+
+int HydroLevel::download(int b, double *h) {
+    if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    CAMR_HIP(hipMemcpy(h, snew_[b].p, NVAR * box_[b].npts() * sizeof(double), hipMemcpyDeviceToHost));
+    return CAMR_OK;
+}
+
+int HydroLevel::state_array(int b, camr_array4 *out) {
+    if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    const Box B = box_[b];
+    out->p = snew_[b].p;
+    out->jstride = B.len(0);
+    out->kstride = (int64_t)B.len(0) * B.len(1);
+    out->nstride = B.npts();
+    for (int d = 0; d < 3; ++d) { out->lo[d] = B.lo[d]; out->hi[d] = B.hi[d]; }
+    out->ncomp = NVAR;
+    return CAMR_OK;
+}
+
+int HydroLevel::check_status() {
+    CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    const int s = *h_status_ & ~kFloorFlag;
+    if (s) {
+        CAMR_HIP(hipMemsetAsync(status_, 0, sizeof(int), stream_));
+        for (int e = 1; e < 31; ++e)
+            if (s & (1 << e)) {
+                static const char *msg[] = {"", "", "",
+                    "rho E or rho e not positive (Hydro_ctoprim.H:56,68)",
+                    "density exactly zero in enforce_minimum_density (CAMR.cpp:990-992)",
+                    "dt needs to be positive (Timestep.H:59)", "", ""};
+                set_error("%s", (e < 8 && msg[e][0]) ? msg[e] : "device error");
+                return e;
+            }
+    }
+    return CAMR_OK;
+}
+
+int HydroLevel::sync() {
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    return check_status();
+}
+
+int HydroLevel::init_post() {
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_init_post, box_[b], stream_, snew_[b], hc_);
+    CAMR_HIP(hipGetLastError());
+    return sync();
+}
+
+// clean_state on valid cells (CAMR.cpp:1253-1262 with ng = 0):
+// enforce_min_density -> normalize_species -> computeTemp.
+int HydroLevel::clean_state(std::vector<Arr> &S, bool launch_detect) {
+    for (size_t b = 0; b < box_.size(); ++b) {
+        if (launch_detect) launch_on(k_detect, box_[b], stream_, S[b], hc_.small_dens, status_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], S[b], hc_, status_);
+        launch_on(k_clean_point, box_[b], stream_, S[b], hc_);
+    }
+    return CAMR_OK;
+}
+
+// expand_state (CAMR.cpp:1243-1250): FillPatch from S_old, then
+// clean_state(Sborder) with ng = numGrow.
+int HydroLevel::expand_state() {
+    std::vector<DevFab> t(box_.size());
+    for (size_t b = 0; b < box_.size(); ++b) { t[b].a = sold_[b]; t[b].b = box_[b]; }
+    CAMR_HIP(hipMemcpyAsync(tab_old_, t.data(), t.size() * sizeof(DevFab), hipMemcpyHostToDevice, stream_));
+    for (size_t b = 0; b < box_.size(); ++b) {
+        const Box G = grow(box_[b], CAMR_NGROW);
+        launch_on(k_fill_raw, G, stream_, sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_, status_);
+    }
+    for (size_t b = 0; b < box_.size(); ++b) {
+        launch_on(k_detect, box_[b], stream_, sbv_[b], hc_.small_dens, status_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], sbv_[b], hc_, status_);
+        launch_on(k_normalize, box_[b], stream_, sbv_[b]);
+    }
+    for (size_t b = 0; b < box_.size(); ++b) {
+        const Box G = grow(box_[b], CAMR_NGROW);
+        launch_on(k_resync, G, stream_, box_[b], sbv_[b], (const DevFab *)tab_sb_, (const int *)owner_, dom_);
+    }
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_resetT, grow(box_[b], CAMR_NGROW), stream_, sbv_[b], hc_);
+    // hipMemcpyAsync from pageable host memory is synchronous w.r.t. the
+    // host buffer, so `t` may go out of scope here.
+    return CAMR_OK;
+}
+
+// construct_hydro_source (Hydro/CAMR_construct_hydro_source.cpp:9-267):
+// per box ctoprim, srctoprim, hydro_umdrv, then S_new += dt * dSdt.
+int HydroLevel::construct_hydro_source(double dt) {
+    if (grav_src_values_) {
+        for (size_t b = 0; b < box_.size(); ++b)
+            launch_on(k_srchyd, grow(box_[b], CAMR_NGROW), stream_, box_[b],
+                      dense(srchyd_[b], grow(box_[b], CAMR_NGROW)), sold_[b], desc_.const_grav);
+        for (size_t b = 0; b < box_.size(); ++b)
+            launch_on(k_resync, grow(box_[b], CAMR_NGROW), stream_, box_[b],
+                      dense(srchyd_[b], grow(box_[b], CAMR_NGROW)), (const DevFab *)tab_src_,
+                      (const int *)owner_, dom_);
+    }
+    {
+        for (size_t b = 0; b < box_.size(); ++b) {
+            const Box B = box_[b], G = grow(B, CAMR_NGROW);
+            Arr q = dense(q_, G), qa = dense(qaux_, G), sq = dense(srcq_, G);
+            launch_on(k_ctoprim, G, stream_, sbv_[b], q, qa, hc_, status_);
+            if (grav_src_values_)
+                launch_on(k_srctoprim, G, stream_, q, qa, dense(srchyd_[b], G), sq);
+            Arr ds = dense(dsdt_[b], B);
+            launch_on(k_fill_value, B, stream_, ds, (int)NVAR, 0.0);
+            UmdrvArgs A;
+            A.bx = B;
+            A.qbox = G;
+            A.hc = hc_;
+            A.dt = dt;
+            A.uin = sbv_[b];
+            A.dsdt = ds;
+            A.q = q;
+            A.qaux = qa;
+            A.srcq = sq;
+            A.has_src = grav_src_values_ ? 1 : 0;
+            for (int d = 0; d < 3; ++d) {
+                A.flx[d] = dense(flx_[d], nodes(B, d));
+                A.area[d] = A.flx[d];
+            }
+            A.has_area = 0;
+            A.area_c[0] = hc_.dx[1] * hc_.dx[2];
+            A.area_c[1] = hc_.dx[0] * hc_.dx[2];
+            A.area_c[2] = hc_.dx[0] * hc_.dx[1];
+            A.vol = ds;
+            A.has_vol = 0;
+            A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
+            int rc = umdrv_staged(A, scratch_.base(), stream_);
+            if (rc) { set_error("kernel launch failed"); return rc; }
+        }
+    }
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_add_hydro, box_[b], stream_, snew_[b], dense(dsdt_[b], box_[b]), dt,
+                  hc_.small_dens, status_);
+    return CAMR_OK;
+}
+
+int HydroLevel::advance(double dt) {
+    if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    // swapTimeLevels (CAMR_advance.cpp:72-75)
+    std::swap(snew_, sold_);
+    a_is_new_ = !a_is_new_;
+    // clean_state(S_old) (:83)
+    clean_state(sold_, true);
+    // expand_state(Sborder, prev_time, numGrow()) (:99)
+    int rc = expand_state();
+    if (rc) return rc;
+    // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_copy_oldsrc, box_[b], stream_, snew_[b], sbv_[b], sold_[b], dt,
+                  desc_.const_grav, desc_.add_grav_src);
+    // construct_hydro_source + Saxpy (:138-141)
+    scratch_.reset();
+    rc = construct_hydro_source(dt);
+    if (rc) return rc;
+    // clean_state(S_new) (:146)
+    clean_state(snew_, false);
+    if (desc_.add_grav_src) {
+        // new-time source correction + clean_state (:150-154)
+        for (size_t b = 0; b < box_.size(); ++b)
+            launch_on(k_add_newsrc, box_[b], stream_, snew_[b], sold_[b], dt, desc_.const_grav,
+                      hc_.small_dens, status_);
+        clean_state(snew_, false);
+    }
+    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
+    clean_state(snew_, true);
+    CAMR_HIP(hipGetLastError());
+    time_ += dt;
+    return CAMR_OK;
+}
+
+// estTimeStep (CAMR.cpp:490-563)
+int HydroLevel::est_time_step(double *est) {
+    if (desc_.fixed_dt > 0.0) { *est = desc_.fixed_dt; return CAMR_OK; }
+    double estdt = desc_.max_dt;
+    const double max_dt_over_cfl = desc_.max_dt / desc_.cfl;
+    double estdt_hydro = max_dt_over_cfl;
+    const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
+    CAMR_HIP(hipMemcpyAsync(dtmin_, &init, sizeof(init), hipMemcpyHostToDevice, stream_));
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_);
+    CAMR_HIP(hipGetLastError());
+    CAMR_HIP(hipMemcpyAsync(h_dtmin_, dtmin_, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+    int rc = sync();
+    if (rc) return rc;
+    double dt;
+    unsigned long long bits = *h_dtmin_;
+    memcpy(&dt, &bits, sizeof(dt));
+    estdt_hydro = vmin(estdt_hydro, dt);
+    estdt_hydro *= desc_.cfl;
+    if (estdt_hydro < estdt) estdt = estdt_hydro;
+    *est = estdt;
+    return CAMR_OK;
+}
+
+// computeInitialDt (CAMR.cpp:638-676) with initialTimeStep (:474-488)
+int HydroLevel::initial_dt(double *dt) {
+    double est;
+    int rc = est_time_step(&est);
+    if (rc) return rc;
+    double dt_0 = DBL_MAX;
+    dt_0 = vmin(dt_0, 1 * (desc_.init_shrink * est));
+    const double dt_eps = 0.001 * dt_0;
+    if (desc_.stop_time >= 0.0 && (time_ + dt_0) > (desc_.stop_time - dt_eps))
+        dt_0 = desc_.stop_time - time_;
+    *dt = dt_0 / 1;
+    return CAMR_OK;
+}
+
+// computeNewDt (CAMR.cpp:566-635), single level, no regrid
+int HydroLevel::new_dt(double dt_old, double *dt) {
+    double dt_min;
+    int rc = est_time_step(&dt_min);
+    if (rc) return rc;
+    if (desc_.fixed_dt <= 0.0) dt_min = vmin(dt_min, desc_.change_max * dt_old);
+    double dt_0 = DBL_MAX;
+    dt_0 = vmin(dt_0, 1 * dt_min);
+    const double dt_eps = 0.001 * dt_0;
+    if (desc_.stop_time >= 0.0 && (time_ + dt_0) > (desc_.stop_time - dt_eps))
+        dt_0 = desc_.stop_time - time_;
+    *dt = dt_0 / 1;
+    return CAMR_OK;
+}
+
+} // namespace camr
+
+// ================================================================ C-ABI
+using namespace camr;
+
+struct camr_level {
+    HydroLevel impl;
+    explicit camr_level(const camr_level_desc &d) : impl(d) {}
+};
+
+extern "C" {
+
+void camr_default_params(camr_hydro_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->ppm_type = 0;
+    p->plm_iorder = 4;
+    p->use_pslope = 0;
+    p->use_flattening = 1;
+    p->transverse_reset_density = 1;
+    p->allow_negative_energy = 0;
+    p->allow_small_energy = 1;
+    p->dual_energy_update_E_from_e = 1;
+    p->small = 1.e-8;
+    p->small_dens = 1.e-7;
+    p->small_pres = 1.e-200;
+    p->small_temp = 1.e2;
+    p->smallu = 1.e-12;
+    p->small_num = 1.e-8;
+    p->difmag = 0.1;
+    p->dual_energy_eta1 = 1.0e0;
+    p->dual_energy_eta2 = 1.0e-4;
+    for (int d = 0; d < 3; ++d) p->dx[d] = 1.0;
+}
+
+void camr_default_level_desc(camr_level_desc *d) {
+    memset(d, 0, sizeof(*d));
+    camr_default_params(&d->hp);
+    d->add_grav_src = 1;
+    d->const_grav = 0.0;
+    d->cfl = 0.8;
+    d->change_max = 1.1;
+    d->init_shrink = 1.0;
+    d->max_dt = 1.e200;
+    d->fixed_dt = -1.0;
+    d->stop_time = -1.0;
+    d->engine = 0;
+}
+
+const char *camr_last_error(void) { return g_err; }
+int camr_version(void) { return 100; }
+
+int camr_hydro_ctoprim(const camr_box *qbx, const camr_array4 *u, const camr_array4 *q,
+                       const camr_array4 *qaux, const camr_hydro_params *p, void *stream) {
+    if (!qbx || !u || !q || !qaux || !p) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    const Box B = make_box(qbx->lo, qbx->hi);
+    if (!covers(*u, B, NVAR) || !covers(*q, B, QVAR) || !covers(*qaux, B, NQAUX)) {
+        set_error("camr_hydro_ctoprim: arrays do not cover qbx");
+        return CAMR_ERR_SHAPE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int *st = nullptr;
+    CAMR_HIP(hipMallocAsync(&st, sizeof(int), s));
+    CAMR_HIP(hipMemsetAsync(st, 0, sizeof(int), s));
+    launch_on(k_ctoprim, B, s, view(*u), view(*q), view(*qaux), make_const(*p), st);
+    int h = 0;
+    CAMR_HIP(hipMemcpyAsync(&h, st, sizeof(int), hipMemcpyDeviceToHost, s));
+    CAMR_HIP(hipFreeAsync(st, s));
+    CAMR_HIP(hipStreamSynchronize(s));
+    if (h & (1 << CAMR_ERR_NEG_ENERGY)) {
+        set_error("rho E or rho e not positive (Hydro_ctoprim.H:56,68)");
+        return CAMR_ERR_NEG_ENERGY;
+    }
+    return CAMR_OK;
+}
+
+int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q, const camr_array4 *qaux,
+                         const camr_array4 *src, const camr_array4 *srcq, void *stream) {
+    if (!qbx || !q || !qaux || !src || !srcq) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    const Box B = make_box(qbx->lo, qbx->hi);
+    if (!covers(*q, B, QVAR) || !covers(*qaux, B, NQAUX) || !covers(*src, B, NVAR) ||
+        !covers(*srcq, B, QVAR)) {
+        set_error("camr_hydro_srctoprim: arrays do not cover qbx");
+        return CAMR_ERR_SHAPE;
+    }
+    launch_on(k_srctoprim, B, (hipStream_t)stream, view(*q), view(*qaux), view(*src), view(*srcq));
+    CAMR_HIP(hipGetLastError());
+    return CAMR_OK;
+}
+
+int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
+                     const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
+                     const camr_array4 flx[3], const camr_array4 area[3], const camr_array4 *vol,
+                     double dt, const camr_hydro_params *p, void *stream) {
+    if (!bx || !uin || !dsdt || !q || !qaux || !flx || !p) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    int rc = check_params(*p);
+    if (rc) return rc;
+    if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    const Box B = make_box(bx->lo, bx->hi);
+    const Box G = grow(B, CAMR_NGROW);
+    if (!B.ok() || !covers(*q, G, QVAR) || !covers(*qaux, G, NQAUX) || !covers(*uin, grow(B, 1), NVAR) ||
+        !covers(*dsdt, B, NVAR)) {
+        set_error("camr_hydro_umdrv: q/qaux must cover grow(bx,4), uin grow(bx,1), dsdt bx");
+        return CAMR_ERR_SHAPE;
+    }
+    const bool has_src = srcq && srcq->p;
+    if (has_src && !covers(*srcq, G, QVAR)) { set_error("srcq must cover grow(bx,4)"); return CAMR_ERR_SHAPE; }
+    bool has_area = area && area[0].p && area[1].p && area[2].p;
+    for (int d = 0; d < 3; ++d) {
+        if (!covers(flx[d], nodes(B, d), NVAR)) { set_error("flx[%d] must cover the faces of bx", d); return CAMR_ERR_SHAPE; }
+        if (has_area && !covers(area[d], nodes(B, d), 1)) { set_error("area[%d] must cover the faces of bx", d); return CAMR_ERR_SHAPE; }
+    }
+    const bool has_vol = vol && vol->p;
+    if (has_vol && !covers(*vol, B, 1)) { set_error("vol must cover bx"); return CAMR_ERR_SHAPE; }
+    UmdrvArgs A;
+    A.bx = B;
+    A.qbox = box_of(*q);
+    A.hc = make_const(*p);
+    A.dt = dt;
+    A.uin = view(*uin);
+    A.dsdt = view(*dsdt);
+    A.q = view(*q);
+    A.qaux = view(*qaux);
+    A.srcq = has_src ? view(*srcq) : view(*q);
+    A.has_src = has_src ? 1 : 0;
+    for (int d = 0; d < 3; ++d) {
+        A.flx[d] = view(flx[d]);
+        A.area[d] = has_area ? view(area[d]) : A.flx[d];
+    }
+    A.has_area = has_area ? 1 : 0;
+    A.area_c[0] = p->dx[1] * p->dx[2];
+    A.area_c[1] = p->dx[0] * p->dx[2];
+    A.area_c[2] = p->dx[0] * p->dx[1];
+    A.vol = has_vol ? view(*vol) : A.dsdt;
+    A.has_vol = has_vol ? 1 : 0;
+    A.vol_c = p->dx[0] * p->dx[1] * p->dx[2];
+    hipStream_t s = (hipStream_t)stream;
+    double *scr = nullptr;
+    const size_t n = umdrv_staged_scratch_doubles(B);
+    CAMR_HIP(hipMallocAsync((void **)&scr, n * sizeof(double), s));
+    rc = umdrv_staged(A, scr, s);
+    CAMR_HIP(hipFreeAsync(scr, s));
+    if (rc) { set_error("kernel launch failed"); return rc; }
+    return CAMR_OK;
+}
+
+int camr_estdt_hydro(const camr_box *bx, const camr_array4 *u, const double dx[3],
+                     double *dt_out, void *stream) {
+    if (!bx || !u || !dx || !dt_out) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    const Box B = make_box(bx->lo, bx->hi);
+    if (!covers(*u, B, NVAR)) { set_error("u must cover bx"); return CAMR_ERR_SHAPE; }
+    hipStream_t s = (hipStream_t)stream;
+    struct { unsigned long long m; int st; int pad; } h = {0x7fefffffffffffffULL, 0, 0};
+    char *buf = nullptr;
+    CAMR_HIP(hipMallocAsync((void **)&buf, 16, s));
+    CAMR_HIP(hipMemcpyAsync(buf, &h, 16, hipMemcpyHostToDevice, s));
+    launch_on(k_estdt, B, s, view(*u), dx[0], dx[1], dx[2], (unsigned long long *)buf, (int *)(buf + 8));
+    CAMR_HIP(hipMemcpyAsync(&h, buf, 16, hipMemcpyDeviceToHost, s));
+    CAMR_HIP(hipFreeAsync(buf, s));
+    CAMR_HIP(hipStreamSynchronize(s));
+    if (h.st & (1 << CAMR_ERR_DT)) { set_error("ERROR: dt needs to be positive."); return CAMR_ERR_DT; }
+    double v;
+    memcpy(&v, &h.m, sizeof(v));
+    *dt_out = v;
+    return CAMR_OK;
+}
+
+int camr_level_create(const camr_level_desc *d, camr_level **out) {
+    if (!d || !out) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    *out = nullptr;
+    camr_level *L = new camr_level(*d);
+    int rc = L->impl.create();
+    if (rc) { delete L; return rc; }
+    *out = L;
+    return CAMR_OK;
+}
+int camr_level_destroy(camr_level *L) {
+    if (L) { (void)L->impl.sync(); delete L; }
+    return CAMR_OK;
+}
+int camr_level_set_stream(camr_level *L, void *stream) {
+    if (!L) return CAMR_ERR_PARAM;
+    L->impl.set_stream((hipStream_t)stream);
+    return CAMR_OK;
+}
+int camr_level_upload_box(camr_level *L, int32_t b, const double *h) {
+    return L ? L->impl.upload(b, h) : CAMR_ERR_PARAM;
+}
+int camr_level_download_box(camr_level *L, int32_t b, double *h) {
+    return L ? L->impl.download(b, h) : CAMR_ERR_PARAM;
+}
+int camr_level_state_array(camr_level *L, int32_t b, camr_array4 *out) {
+    return (L && out) ? L->impl.state_array(b, out) : CAMR_ERR_PARAM;
+}
+int camr_level_init_post(camr_level *L) { return L ? L->impl.init_post() : CAMR_ERR_PARAM; }
+int camr_level_initial_dt(camr_level *L, double *dt) {
+    return (L && dt) ? L->impl.initial_dt(dt) : CAMR_ERR_PARAM;
+}
+int camr_level_new_dt(camr_level *L, double dt_old, double *dt) {
+    return (L && dt) ? L->impl.new_dt(dt_old, dt) : CAMR_ERR_PARAM;
+}
+int camr_level_advance(camr_level *L, double dt) { return L ? L->impl.advance(dt) : CAMR_ERR_PARAM; }
+int camr_level_run(camr_level *L, int32_t nsteps, double *dts) {
+    if (!L) return CAMR_ERR_PARAM;
+    double dt = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = (s == 0) ? L->impl.initial_dt(&dt) : L->impl.new_dt(dt, &dt);
+        if (rc) return rc;
+        if (dts) dts[s] = dt;
+        rc = L->impl.advance(dt);
+        if (rc) return rc;
+    }
+    return L->impl.sync();
+}
+int camr_level_sync(camr_level *L) { return L ? L->impl.sync() : CAMR_ERR_PARAM; }
+double camr_level_time(const camr_level *L) { return L ? L->impl.time() : 0.0; }
+
+} // extern "C"

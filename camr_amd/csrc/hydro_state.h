@@ -1,0 +1,48 @@
+// hydro_state.h -- declarations of the level-step state kernels.
+#pragma once
+
+#include "hydro_kernels.h"
+
+namespace camr {
+
+constexpr int kFloorFlag = 1 << 30;  // a density below small_dens was seen
+
+struct DevFab {  // device-side box table entry
+    Arr a;
+    Box b;
+};
+
+struct DomInfo {
+    int domlo[3], domhi[3], periodic[3], bclo[3], bchi[3];
+};
+
+__global__ void k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status);
+__global__ void k_srctoprim(Range r, Arr q, Arr qa, Arr src, Arr srcq);
+__global__ void k_estdt(Range r, Arr u, double dx, double dy, double dz,
+                        unsigned long long *out, int *status);
+__global__ void k_detect(Range r, Arr S, double small_dens, int *flags);
+__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags);
+__global__ void k_clean_point(Range r, Arr S, HydroConst hc);
+__global__ void k_normalize(Range r, Arr S);
+__global__ void k_resetT(Range r, Arr S, HydroConst hc);
+__global__ void k_init_post(Range r, Arr S, HydroConst hc);
+__global__ void k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner,
+                           DomInfo D, int *status);
+__global__ void k_resync(Range r, Box valid, Arr dst, const DevFab *src,
+                         const int *owner, DomInfo D);
+__global__ void k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt,
+                              double g, int add_grav);
+__global__ void k_add_hydro(Range r, Arr snew, Arr dsdt, double dt,
+                            double small_dens, int *flags);
+__global__ void k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g,
+                             double small_dens, int *flags);
+__global__ void k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g);
+__global__ void k_fill_value(Range r, Arr a, int nc, double v);
+
+template <typename K, typename... A>
+inline void launch_on(K kern, const Box &b, hipStream_t s, A... args) {
+    Range r = range_of(b);
+    hipLaunchKernelGGL(kern, dim3(grid_for(r.n)), dim3(kBlock), 0, s, r, args...);
+}
+
+} // namespace camr

@@ -1,0 +1,462 @@
+// hydro_state.hip -- pointwise state kernels of the level step on gfx950:
+// conserved -> primitive conversion, primitive sources, clean_state
+// (density floor, species normalisation, internal-energy reset and
+// temperature), single-level ghost fill, source Saxpys and the CFL dt
+// min-reduction.
+#include "hydro_state.h"
+
+namespace camr {
+
+// ---------------------------------------------------------------- ctoprim
+// hydro_ctoprim (Source/Hydro/Hydro_ctoprim.H:11-101), all QVAR / NQAUX
+// components.  Cells with rho <= 0 are left untouched as in the reference.
+__global__ void __launch_bounds__(kBlock)
+k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const double rho = u(i, j, k, URHO);
+    if (!(rho > 0.)) return;
+    const double rhoinv = 1.0 / rho;
+    const double vx = u(i, j, k, UMX) * rhoinv;
+    const double vy = u(i, j, k, UMY) * rhoinv;
+    const double vz = u(i, j, k, UMZ) * rhoinv;
+    const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    q(i, j, k, QRHO) = rho;
+    q(i, j, k, QU) = vx;
+    q(i, j, k, QV) = vy;
+    q(i, j, k, QW) = vz;
+    const double ueden = u(i, j, k, UEDEN);
+    const double ueint = u(i, j, k, UEINT);
+    if (!hc.allow_negative_energy && !(ueden > 0. && ueint > 0.))
+        atomicOr(status, 1 << CAMR_ERR_NEG_ENERGY);
+    const double e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv
+                                                         : ueint * rhoinv;
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) q(i, j, k, QFS + s) = u(i, j, k, UFS + s) * rhoinv;
+    const double p = (kGamma - 1.0) * rho * e;
+    q(i, j, k, QTEMP) = eos_T_from_e(e);
+    q(i, j, k, QREINT) = e * rho;
+    q(i, j, k, QPRES) = p;
+    q(i, j, k, QGAME) = p / (e * rho) + 1.0;
+    qa(i, j, k, QDPDR) = (kGamma - 1.0) * e;
+    qa(i, j, k, QDPDE) = (kGamma - 1.0) * rho;
+    qa(i, j, k, QGAMC) = kGamma;
+    const double cs = sqrt(kGamma * p / rho);
+    qa(i, j, k, QC) = cs;
+    qa(i, j, k, QCSML) = vmax(hc.small_num, hc.small_num * cs);
+}
+
+// hydro_srctoprim (Hydro_ctoprim.H:106-157)
+__global__ void __launch_bounds__(kBlock)
+k_srctoprim(Range r, Arr q, Arr qa, Arr src, Arr srcq) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const double srcrho = src(i, j, k, URHO);
+    if (q(i, j, k, QRHO) > 0.) {
+        const double rhoinv = 1.0 / q(i, j, k, QRHO);
+        srcq(i, j, k, QRHO) = srcrho;
+        srcq(i, j, k, QU) = (src(i, j, k, UMX) - q(i, j, k, QU) * srcrho) * rhoinv;
+        srcq(i, j, k, QV) = (src(i, j, k, UMY) - q(i, j, k, QV) * srcrho) * rhoinv;
+        srcq(i, j, k, QW) = (src(i, j, k, UMZ) - q(i, j, k, QW) * srcrho) * rhoinv;
+        const double sre = src(i, j, k, UEINT);
+        srcq(i, j, k, QREINT) = sre;
+#pragma unroll
+        for (int s = 0; s < NSPEC; ++s)
+            srcq(i, j, k, QFS + s) = (src(i, j, k, UFS + s) - q(i, j, k, QFS + s) * srcrho) * rhoinv;
+        srcq(i, j, k, QPRES) =
+            qa(i, j, k, QDPDE) * (sre - q(i, j, k, QREINT) * srcrho * rhoinv) * rhoinv +
+            qa(i, j, k, QDPDR) * srcrho;
+    } else {
+        srcq(i, j, k, QRHO) = 0.;
+        srcq(i, j, k, QU) = 0.;
+        srcq(i, j, k, QV) = 0.;
+        srcq(i, j, k, QW) = 0.;
+        srcq(i, j, k, QPRES) = 0.;
+        srcq(i, j, k, QREINT) = 0.;
+    }
+}
+
+// ---------------------------------------------------------------- estdt
+// CAMR_estdt_hydro (Source/Utils/Timestep.H:18-61).  The min over cells is
+// order independent; non-negative doubles order like their bit patterns,
+// so the block minimum is combined with a 64-bit integer atomicMin.
+__device__ __forceinline__ double estdt_point(const Arr &u, int i, int j, int k,
+                                              double dx, double dy, double dz) {
+    const double rho = u(i, j, k, URHO);
+    const double rhoInv = 1.0 / rho;
+    const double eint = u(i, j, k, UEINT) / rho;
+    const double pres = (kGamma - 1.0) * rho * eint;
+    const double c = sqrt(kGamma * pres / rho);
+    double dt = 1.7976931348623157e308;
+    const double ux = u(i, j, k, UMX) * rhoInv;
+    dt = vmin(dt, dx / (c + fabs(ux)));
+    const double uy = u(i, j, k, UMY) * rhoInv;
+    dt = vmin(dt, dy / (c + fabs(uy)));
+    const double uz = u(i, j, k, UMZ) * rhoInv;
+    dt = vmin(dt, dz / (c + fabs(uz)));
+    return dt;
+}
+
+__device__ __forceinline__ void block_min_commit(double v, unsigned long long *out,
+                                                 int *status) {
+    __shared__ double red[kBlock / 64];
+    // non-positive (or negative-signed) dt is an error (Timestep.H:59)
+    if (!(v > 0.0) && v == v) atomicOr(status, 1 << CAMR_ERR_DT);
+    if (!(v == v)) v = 1.7976931348623157e308;  // std::min never selects NaN
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = vmin(v, o);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = vmin(m, red[w]);
+        if (m > 0.0) atomicMin(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_estdt(Range r, Arr u, double dx, double dy, double dz, unsigned long long *out,
+        int *status) {
+    int i, j, k;
+    double v = 1.7976931348623157e308;
+    if (range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+        v = estdt_point(u, i, j, k, dx, dy, dz);
+    block_min_commit(v, out, status);
+}
+
+// --------------------------------------------------------- clean_state
+// CAMR_reset_internal_e (Source/Utils/CAMR_reset_internal_e.H:13-129) then
+// the temperature of computeTemp (CAMR.cpp:1141-1185), on a register copy
+// of one cell.
+__device__ __forceinline__ void reset_e_and_T(double *s, const HydroConst &hc) {
+    const double rho = s[URHO];
+    const double rhoInv = 1.0 / rho;
+    const double Up = s[UMX] * rhoInv;
+    const double Vp = s[UMY] * rhoInv;
+    const double Wp = s[UMZ] * rhoInv;
+    const double ke = 0.5 * (Up * Up + Vp * Vp + Wp * Wp);
+    const double eden = s[UEDEN] * rhoInv;
+    const double y0 = s[UFS] * rhoInv, y1 = s[UFS + 1] * rhoInv, y2 = s[UFS + 2] * rhoInv;
+    const double small_T = hc.small_temp;
+    if (hc.allow_small_energy == 0) {
+        const double small_e = eos_E_from_T(small_T, y0, y1, y2);
+        if (eden < small_e) {
+            if (s[UEINT] * rhoInv < small_e) {
+                const double T = vmax(s[UTEMP], small_T);
+                s[UEINT] = rho * eos_E_from_T(T, y0, y1, y2);
+            }
+            s[UEDEN] = s[UEINT] + rho * ke;
+        } else {
+            const double rho_eint = s[UEDEN] - s[URHO] * ke;
+            if (rho_eint > 0. && rho_eint / s[UEDEN] > hc.eta2) s[UEINT] = rho_eint;
+            if (s[UEINT] * rhoInv < small_e) {
+                const double T = vmax(s[UTEMP], small_T);
+                const double e_out = eos_E_from_T(T, y0, y1, y2);
+                if (hc.update_E_from_e == 1) s[UEDEN] += (rho * e_out - s[UEINT]);
+                s[UEINT] = s[URHO] * e_out;
+            }
+        }
+    } else if (hc.allow_negative_energy == 0) {
+        if (s[UEDEN] < 0.) {
+            if (s[UEINT] < 0.) s[UEINT] = rho * eos_E_from_T(small_T, y0, y1, y2);
+            s[UEDEN] = s[UEINT] + rho * ke;
+        } else {
+            const double rho_eint = s[UEDEN] - rho * ke;
+            if (rho_eint > 0. && rho_eint / s[UEDEN] > hc.eta2) {
+                s[UEINT] = rho_eint;
+            } else if (s[UEINT] > 0 && hc.update_E_from_e == 1) {
+                s[UEDEN] = s[UEINT] + rho * ke;
+            } else if (s[UEINT] <= 0.) {
+                const double e_out = eos_E_from_T(small_T, y0, y1, y2);
+                s[UEINT] = rho * e_out;
+                if (hc.update_E_from_e == 1) s[UEDEN] += (rho * e_out - s[UEINT]);
+            }
+        }
+    } else {
+        s[UEINT] = s[UEDEN] - rho * ke;
+    }
+    const double rinv = 1.0 / s[URHO];
+    s[UTEMP] = eos_T_from_e(s[UEINT] * rinv);
+}
+
+// normalize_species (CAMR.cpp:866-908) on a register copy of one cell
+__device__ __forceinline__ void normalize_species(double *s) {
+    double sum = 0.;
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) {
+        s[UFS + n] = vmax(vmin(s[URHO], s[UFS + n]), 0.0);
+        sum += s[UFS + n];
+    }
+    if (s[URHO] > 0.) {
+        sum /= s[URHO];
+#pragma unroll
+        for (int n = 0; n < NSPEC; ++n) s[UFS + n] /= sum;
+    }
+}
+
+__device__ __forceinline__ void floor_detect(double rho, double small_dens, int *flags) {
+    if (rho == 0.) atomicOr(flags, 1 << CAMR_ERR_ZERO_DENSITY);
+    else if (rho < small_dens) atomicOr(flags, kFloorFlag);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_detect(Range r, Arr S, double small_dens, int *flags) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    floor_detect(S(i, j, k, URHO), small_dens, flags);
+}
+
+// enforce_min_density (CAMR.cpp:941-1045), exact sequential order of the
+// reference CPU loop over one box; runs only if a floor was detected.
+__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!(*flags & kFloorFlag)) return;
+    const double sd = hc.small_dens;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        const double r0 = S(i, j, k, URHO);
+        if (r0 == 0. || !(r0 < sd)) continue;
+        double max_dens = r0;
+        int is = i, js = j, ks = k;
+        for (int kk = -1; kk <= 1; kk++)
+        for (int jj = -1; jj <= 1; jj++)
+        for (int ii = -1; ii <= 1; ii++) {
+            if (bx.contains(i + ii, j + jj, k + kk) && S(i + ii, j + jj, k + kk, URHO) > max_dens) {
+                is = i + ii; js = j + jj; ks = k + kk;
+                max_dens = S(is, js, ks, URHO);
+            }
+        }
+        if (max_dens < sd) {
+            const double rho = S(i, j, k, URHO);
+            const double small_e = eos_E_from_T(hc.small_temp, S(i, j, k, UFS) / rho,
+                                                S(i, j, k, UFS + 1) / rho, S(i, j, k, UFS + 2) / rho);
+            S(i, j, k, URHO) = sd;
+            S(i, j, k, UTEMP) = hc.small_temp;
+            S(i, j, k, UMX) = 0.;
+            S(i, j, k, UMY) = 0.;
+            S(i, j, k, UMZ) = 0.;
+            S(i, j, k, UEINT) = sd * small_e;
+            S(i, j, k, UEDEN) = S(i, j, k, UEINT);
+        } else {
+            for (int n = 0; n < NVAR; n++) S(i, j, k, n) = S(is, js, ks, n);
+        }
+    }
+}
+
+// normalize_species + computeTemp on valid cells (after the floor pass).
+__global__ void __launch_bounds__(kBlock)
+k_clean_point(Range r, Arr S, HydroConst hc) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    double s[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
+    normalize_species(s);
+    reset_e_and_T(s, hc);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) S(i, j, k, n) = s[n];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_normalize(Range r, Arr S) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    double s[NVAR];
+    s[URHO] = S(i, j, k, URHO);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) s[UFS + n] = S(i, j, k, UFS + n);
+    normalize_species(s);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) S(i, j, k, UFS + n) = s[UFS + n];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_resetT(Range r, Arr S, HydroConst hc) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    double s[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
+    reset_e_and_T(s, hc);
+    S(i, j, k, UEDEN) = s[UEDEN];
+    S(i, j, k, UEINT) = s[UEINT];
+    S(i, j, k, UTEMP) = s[UTEMP];
+}
+
+// enforce_consistent_e (CAMR.cpp:911-938) + computeTemp
+__global__ void __launch_bounds__(kBlock)
+k_init_post(Range r, Arr S, HydroConst hc) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    double s[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
+    const double rhoInv = 1.0 / s[URHO];
+    const double u = s[UMX] * rhoInv, v = s[UMY] * rhoInv, w = s[UMZ] * rhoInv;
+    s[UEDEN] = s[UEINT] + 0.5 * s[URHO] * (u * u + v * v + w * w);
+    reset_e_and_T(s, hc);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) S(i, j, k, n) = s[n];
+}
+
+// ---------------------------------------------------------- ghost filling
+// Source index and sign of component n for a cell c of the grown box:
+// periodic wrap, foextrap (Outflow; Inflow with the identity bcnormal of
+// the Sod problems), reflect_even / reflect_odd (Symmetry, SlipWall,
+// NoSlipWall) per Source/Utils/CAMR_setup.cpp:19-21.  AMReX fills faces,
+// edges, then corners from already-filled cells, which composes the
+// per-direction maps; returns true if c lies outside the domain in a
+// non-periodic direction.
+__device__ __forceinline__ bool map_cell(const DomInfo &D, int c0, int c1, int c2,
+                                         int *m, unsigned *oddmask) {
+    const int c[3] = {c0, c1, c2};
+    bool phys = false;
+    unsigned odd = 0;  // bit n set -> component n flips sign
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int x = c[d], lo = D.domlo[d], hi = D.domhi[d];
+        if (x >= lo && x <= hi) { m[d] = x; continue; }
+        if (D.periodic[d]) {
+            const int n = hi - lo + 1;
+            int r = (x - lo) % n;
+            if (r < 0) r += n;
+            m[d] = lo + r;
+            continue;
+        }
+        phys = true;
+        const int bc = (x < lo) ? D.bclo[d] : D.bchi[d];
+        if (bc == BC_OUTFLOW || bc == BC_INFLOW || bc == BC_INTERIOR) {
+            m[d] = (x < lo) ? lo : hi;
+        } else {
+            m[d] = (x < lo) ? (2 * lo - x - 1) : (2 * hi - x + 1);
+            odd ^= 1u << (UMX + d);
+            if (bc == BC_NOSLIPWALL)
+                odd ^= ((1u << UMX) | (1u << UMY) | (1u << UMZ)) & ~(1u << (UMX + d));
+        }
+    }
+    *oddmask = odd;
+    return phys;
+}
+
+__device__ __forceinline__ int owner_of(const DomInfo &D, const int *owner, const int *m) {
+    const int64_t n0 = D.domhi[0] - D.domlo[0] + 1, n1 = D.domhi[1] - D.domlo[1] + 1;
+    return owner[(int64_t)(m[0] - D.domlo[0]) + (int64_t)(m[1] - D.domlo[1]) * n0 +
+                 (int64_t)(m[2] - D.domlo[2]) * n0 * n1];
+}
+
+// FillPatchSingleLevel of every cell of the grown box of `dst` from the
+// valid data of the level's boxes (`src` table).
+__global__ void __launch_bounds__(kBlock)
+k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D,
+           int *status) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    int m[3];
+    unsigned odd;
+    map_cell(D, i, j, k, m, &odd);
+    const int ob = owner_of(D, owner, m);
+    if (ob < 0) { atomicOr(status, 1 << CAMR_ERR_SHAPE); return; }
+    const Arr s = src[ob].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        const double v = s(m[0], m[1], m[2], n);
+        dst(i, j, k, n) = ((odd >> n) & 1u) ? -v : v;
+    }
+}
+
+// FillBoundary: ghost cells of box b that are periodic / interior images of
+// valid cells get those cells' values (physical-boundary ghosts untouched).
+__global__ void __launch_bounds__(kBlock)
+k_resync(Range r, Box valid, Arr dst, const DevFab *src, const int *owner, DomInfo D) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (valid.contains(i, j, k)) return;
+    int m[3];
+    unsigned odd;
+    if (map_cell(D, i, j, k, m, &odd)) return;
+    const int ob = owner_of(D, owner, m);
+    if (ob < 0) return;
+    const Arr s = src[ob].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) dst(i, j, k, n) = s(m[0], m[1], m[2], n);
+}
+
+// ----------------------------------------------------- Saxpy with sources
+// S_new = Sborder (MultiFab::Copy, CAMR_advance.cpp:112), then the old
+// gravity source Saxpy (:115-118; GravitySrc.cpp:36-78).  Zero-valued
+// sources still change -0.0 into +0.0, so they are applied literally.
+__global__ void __launch_bounds__(kBlock)
+k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt, double g, int add_grav) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double v = sb(i, j, k, n);
+        if (add_grav) {
+            double src = 0.0;
+            if (n == UMZ) src = g * sold(i, j, k, URHO);
+            else if (n == UEDEN) src = g * sold(i, j, k, UMZ);
+            v += dt * src;
+        }
+        snew(i, j, k, n) = v;
+    }
+}
+
+// S_new += dt * dSdt (CAMR_advance.cpp:141) + floor detection for the
+// following clean_state.
+__global__ void __launch_bounds__(kBlock)
+k_add_hydro(Range r, Arr snew, Arr dsdt, double dt, double small_dens, int *flags) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) += dt * dsdt(i, j, k, n);
+    floor_detect(snew(i, j, k, URHO), small_dens, flags);
+}
+
+// New-time gravity correction source (GravitySrc.cpp:81-125) Saxpy
+// (CAMR_advance.cpp:150-152) + floor detection.
+__global__ void __launch_bounds__(kBlock)
+k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens, int *flags) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const double s_umz = 0.5 * g * (snew(i, j, k, URHO) - sold(i, j, k, URHO));
+    const double s_ued = 0.5 * g * (snew(i, j, k, UMZ) - sold(i, j, k, UMZ));
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
+        snew(i, j, k, n) += dt * src;
+    }
+    floor_detect(snew(i, j, k, URHO), small_dens, flags);
+}
+
+// sources_for_hydro = 0 + 1.0 * old source on valid cells
+// (CAMR_construct_hydro_source.cpp:31-34), ghosts zero; FillBoundary is a
+// separate k_resync pass.
+__global__ void __launch_bounds__(kBlock)
+k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const bool v = valid.contains(i, j, k);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double src = 0.0;
+        if (v) {
+            if (n == UMZ) src = g * sold(i, j, k, URHO);
+            else if (n == UEDEN) src = g * sold(i, j, k, UMZ);
+        }
+        dst(i, j, k, n) = v ? (0.0 + 1.0 * src) : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fill_value(Range r, Arr a, int nc, double v) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    for (int n = 0; n < nc; ++n) a(i, j, k, n) = v;
+}
+
+} // namespace camr

@@ -1,0 +1,208 @@
+"""Host-side mirror of CAMR's level advance over the gfx950 C-ABI.
+
+``Level`` drives ``camr_level_*`` (the CAMR_advance / computeNewDt mirror in
+camr_amd/csrc/camr_level.cpp); ``hydro_umdrv`` and friends call the per-box
+drop-in entry points on torch device tensors.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+# CAMR parameter defaults (Source/Params/_cpp_parameters, CAMR_Constants.H)
+DEFAULTS = dict(ppm_type=0, plm_iorder=4, use_pslope=0, use_flattening=1,
+                transverse_reset_density=1, allow_negative_energy=0,
+                allow_small_energy=1, dual_energy_update_E_from_e=1,
+                small=1.e-8, small_dens=1.e-7, small_pres=1.e-200,
+                small_temp=1.e2, smallu=1.e-12, small_num=1.e-8,
+                difmag=0.1, dual_energy_eta1=1.0, dual_energy_eta2=1.0e-4)
+LEVEL_DEFAULTS = dict(add_grav_src=1, const_grav=0.0, cfl=0.8,
+                      change_max=1.1, init_shrink=1.0, max_dt=1.e200,
+                      fixed_dt=-1.0, stop_time=-1.0)
+
+
+@dataclass
+class Problem:
+    """A single-level problem: domain, BCs, boxes and runtime parameters."""
+    domlo: tuple
+    domhi: tuple
+    dx: tuple
+    bclo: tuple
+    bchi: tuple
+    is_periodic: tuple
+    boxes: list
+    prob_lo: tuple = (0.0, 0.0, 0.0)
+    params: dict = field(default_factory=dict)
+
+    def value(self, k):
+        if k in self.params:
+            return self.params[k]
+        return DEFAULTS.get(k, LEVEL_DEFAULTS.get(k))
+
+
+def hydro_params(prob: Problem) -> L.HydroParams:
+    p = L.HydroParams()
+    L.lib().camr_default_params(C.byref(p))
+    for k in DEFAULTS:
+        setattr(p, k, prob.value(k))
+    for d in range(3):
+        p.bclo[d], p.bchi[d] = prob.bclo[d], prob.bchi[d]
+        p.domlo[d], p.domhi[d] = prob.domlo[d], prob.domhi[d]
+        p.dx[d] = prob.dx[d]
+    return p
+
+
+class Level:
+    """One AMR level advanced on the GPU by the C-ABI level driver."""
+
+    def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO):
+        self.prob = prob
+        d = L.LevelDesc()
+        L.lib().camr_default_level_desc(C.byref(d))
+        d.hp = hydro_params(prob)
+        for dd in range(3):
+            d.is_periodic[dd] = int(prob.is_periodic[dd])
+        for k in LEVEL_DEFAULTS:
+            setattr(d, k, prob.value(k))
+        self._lohi = np.ascontiguousarray(
+            np.array([x for b in prob.boxes for x in b], dtype=np.int32))
+        d.nbox = len(prob.boxes)
+        d.box_lohi = self._lohi.ctypes.data_as(C.POINTER(C.c_int32))
+        d.engine = engine
+        self._desc = d
+        h = C.c_void_p()
+        L.check(L.lib().camr_level_create(C.byref(d), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().camr_level_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, b):
+        lo, hi = self.prob.boxes[b][:3], self.prob.boxes[b][3:]
+        return (L.NVAR, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+
+    def set_box(self, b, data):
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.shape == self.shape(b)
+        L.check(L.lib().camr_level_upload_box(
+            self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def get_box(self, b):
+        a = np.empty(self.shape(b))
+        L.check(L.lib().camr_level_download_box(
+            self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
+        return a
+
+    def state_array(self, b) -> L.Array4:
+        a = L.Array4()
+        L.check(L.lib().camr_level_state_array(self._h, b, C.byref(a)))
+        return a
+
+    def set_stream(self, stream_ptr: int):
+        L.check(L.lib().camr_level_set_stream(self._h, C.c_void_p(stream_ptr)))
+
+    def init_post(self):
+        L.check(L.lib().camr_level_init_post(self._h))
+
+    def initial_dt(self):
+        v = C.c_double()
+        L.check(L.lib().camr_level_initial_dt(self._h, C.byref(v)))
+        return v.value
+
+    def new_dt(self, dt_old):
+        v = C.c_double()
+        L.check(L.lib().camr_level_new_dt(self._h, dt_old, C.byref(v)))
+        return v.value
+
+    def advance(self, dt):
+        L.check(L.lib().camr_level_advance(self._h, dt))
+
+    def sync(self):
+        L.check(L.lib().camr_level_sync(self._h))
+
+    def run(self, nsteps):
+        dts = np.zeros(max(nsteps, 1))
+        L.check(L.lib().camr_level_run(
+            self._h, nsteps, dts.ctypes.data_as(C.POINTER(C.c_double))))
+        return dts[:nsteps]
+
+    @property
+    def time(self):
+        return L.lib().camr_level_time(self._h)
+
+
+# ---------------------------------------------------------------- per box
+def array4_of(t, lo, ncomp=None) -> L.Array4:
+    """Array4 view of a (nc, nz, ny, nx) contiguous float64 torch tensor."""
+    assert t.dtype.is_floating_point and t.is_contiguous() and t.dim() == 4
+    nc, nz, ny, nx = t.shape
+    a = L.Array4()
+    a.p = t.data_ptr()
+    a.jstride, a.kstride, a.nstride = nx, nx * ny, nx * ny * nz
+    for d in range(3):
+        a.lo[d] = int(lo[d])
+    a.hi[0], a.hi[1], a.hi[2] = lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1
+    a.ncomp = nc if ncomp is None else ncomp
+    return a
+
+
+def _box(lo, hi) -> L.Box:
+    b = L.Box()
+    for d in range(3):
+        b.lo[d], b.hi[d] = int(lo[d]), int(hi[d])
+    return b
+
+
+def _stream(stream):
+    return C.c_void_p(stream) if stream else None
+
+
+def hydro_ctoprim(qlo, qhi, u, ulo, q, qa, params, stream=0):
+    L.check(L.lib().camr_hydro_ctoprim(
+        C.byref(_box(qlo, qhi)), C.byref(array4_of(u, ulo)),
+        C.byref(array4_of(q, qlo)), C.byref(array4_of(qa, qlo)),
+        C.byref(params), _stream(stream)))
+
+
+def hydro_srctoprim(qlo, qhi, q, qa, src, srcq, stream=0):
+    L.check(L.lib().camr_hydro_srctoprim(
+        C.byref(_box(qlo, qhi)), C.byref(array4_of(q, qlo)),
+        C.byref(array4_of(qa, qlo)), C.byref(array4_of(src, qlo)),
+        C.byref(array4_of(srcq, qlo)), _stream(stream)))
+
+
+def hydro_umdrv(lo, hi, uin, uin_lo, dsdt, q, qa, srcq, flx, dt, params,
+                area=None, vol=None, stream=0):
+    """hydro_umdrv (Hydro.H:11-35) on device tensors; q/qa/srcq start at
+    lo-4, flx[d] at lo, dsdt at lo.  srcq None = identically +0 sources."""
+    g = [x - L.NGROW for x in lo]
+    fl = (L.Array4 * 3)(*[array4_of(f, lo) for f in flx])
+    ar = (L.Array4 * 3)(*([array4_of(a, lo) for a in area] if area else
+                          [L.Array4() for _ in range(3)]))
+    L.check(L.lib().camr_hydro_umdrv(
+        C.byref(_box(lo, hi)), C.byref(array4_of(uin, uin_lo)),
+        C.byref(array4_of(dsdt, lo)), C.byref(array4_of(q, g)),
+        C.byref(array4_of(qa, g)),
+        C.byref(array4_of(srcq, g)) if srcq is not None else C.byref(L.Array4()),
+        fl, ar, C.byref(array4_of(vol, lo)) if vol is not None else C.byref(L.Array4()),
+        dt, C.byref(params), _stream(stream)))
+
+
+def estdt_hydro(lo, hi, u, ulo, dx, stream=0):
+    out = C.c_double()
+    dxa = (C.c_double * 3)(*dx)
+    L.check(L.lib().camr_estdt_hydro(C.byref(_box(lo, hi)), C.byref(array4_of(u, ulo)),
+                                     dxa, C.byref(out), _stream(stream)))
+    return out.value

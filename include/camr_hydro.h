@@ -1,0 +1,174 @@
+/*
+ * camr_hydro.h -- C-ABI of the MI355X (gfx950) single-level hydro advance.
+ *
+ * Drop-in boundary for CAMR's per-box hydro call chain.  Every entry point
+ * takes plain pointers and sizes (no AMReX or torch types); arrays are
+ * device pointers described by camr_array4, which has the memory layout of
+ * amrex::Array4<Real>: x fastest, then y, then z, component outermost,
+ *     p[(i-lo[0]) + (j-lo[1])*jstride + (k-lo[2])*kstride + n*nstride].
+ *
+ * All functions return a camr_status (0 = success).  Fatal conditions that
+ * abort the reference (amrex::Error / AMREX_ALWAYS_ASSERT) become non-zero
+ * status codes here; camr_last_error() returns the message of the last
+ * failure on the calling thread.  Per-box functions are asynchronous on the
+ * given hipStream_t (NULL = default stream) unless stated otherwise.
+ *
+ * Component maps are CAMR's 3-D maps with NUM_SPECIES = 3
+ * (Source/Utils/IndexDefines.H:17-92):
+ *   U  (NVAR=10):  URHO UMX UMY UMZ UEDEN UEINT UTEMP UFS..UFS+2
+ *   Q  (QVAR=11):  QRHO QU QV QW QGAME QPRES QREINT QTEMP QFS..QFS+2
+ *   QAUX (NQAUX=5): QGAMC QC QCSML QDPDR QDPDE
+ */
+#ifndef CAMR_HYDRO_H
+#define CAMR_HYDRO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAMR_NVAR 10
+#define CAMR_QVAR 11
+#define CAMR_NQAUX 5
+#define CAMR_NGROW 4 /* CAMR::numGrow() without EB, Source/CAMR.H:455-483 */
+
+/* Physical boundary codes, Source/Utils/IndexDefines.H:8-13. */
+enum camr_bc {
+    CAMR_BC_INTERIOR = 0,
+    CAMR_BC_INFLOW = 1,
+    CAMR_BC_OUTFLOW = 2,
+    CAMR_BC_SYMMETRY = 3,
+    CAMR_BC_SLIPWALL = 4,
+    CAMR_BC_NOSLIPWALL = 5
+};
+
+typedef enum camr_status {
+    CAMR_OK = 0,
+    CAMR_ERR_PARAM = 1,        /* invalid parameter (e.g. ppm_type not 0/1) */
+    CAMR_ERR_SHAPE = 2,        /* array does not cover the region required */
+    CAMR_ERR_NEG_ENERGY = 3,   /* rho E or rho e <= 0 (Hydro_ctoprim.H:56,68) */
+    CAMR_ERR_ZERO_DENSITY = 4, /* rho == 0 in enforce_min_density */
+    CAMR_ERR_DT = 5,           /* dt <= 0 (Timestep.H:59) */
+    CAMR_ERR_HIP = 6,          /* HIP runtime failure */
+    CAMR_ERR_ALLOC = 7         /* device allocation failure */
+} camr_status;
+
+typedef struct camr_box {
+    int32_t lo[3], hi[3]; /* inclusive cell-index bounds */
+} camr_box;
+
+typedef struct camr_array4 {
+    double *p; /* device pointer to element (lo, comp 0) */
+    int64_t jstride, kstride, nstride;
+    int32_t lo[3], hi[3];
+    int32_t ncomp;
+} camr_array4;
+
+/* Hydro runtime parameters passed by CAMR::construct_hydro_source to
+ * hydro_umdrv (Source/Hydro/Hydro.H:11-35), plus the ctoprim / clean_state
+ * controls (Source/Params/_cpp_parameters). */
+typedef struct camr_hydro_params {
+    int32_t ppm_type;                 /* 0 PLM, 1 PPM */
+    int32_t plm_iorder;               /* 1, 2 or 4 */
+    int32_t use_pslope;
+    int32_t use_flattening;
+    int32_t transverse_reset_density;
+    int32_t allow_negative_energy;
+    int32_t allow_small_energy;
+    int32_t dual_energy_update_E_from_e;
+    double small, small_dens, small_pres, small_temp;
+    double smallu, small_num, difmag;
+    double dual_energy_eta1, dual_energy_eta2;
+    int32_t bclo[3], bchi[3];   /* physical BC codes (camr_bc) */
+    int32_t domlo[3], domhi[3]; /* problem-domain cell bounds */
+    double dx[3];
+} camr_hydro_params;
+
+/* Defaults of Source/Params/_cpp_parameters and CAMR_Constants.H. */
+void camr_default_params(camr_hydro_params *p);
+const char *camr_last_error(void);
+int camr_version(void);
+
+/*
+ * Per-box entry points (the MFIter loop body of
+ * CAMR::construct_hydro_source, Source/Hydro/CAMR_construct_hydro_source.cpp).
+ */
+
+/* hydro_ctoprim over qbx: the ParallelFor at
+ * CAMR_construct_hydro_source.cpp:108-120 (Hydro_ctoprim.H:11-101).
+ * Writes all QVAR components of q and NQAUX of qaux. */
+int camr_hydro_ctoprim(const camr_box *qbx, const camr_array4 *u,
+                       const camr_array4 *q, const camr_array4 *qaux,
+                       const camr_hydro_params *p, void *stream);
+
+/* hydro_srctoprim over qbx: CAMR_construct_hydro_source.cpp:129-143
+ * (Hydro_ctoprim.H:106-157). */
+int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q,
+                         const camr_array4 *qaux, const camr_array4 *src,
+                         const camr_array4 *srcq, void *stream);
+
+/* hydro_umdrv (Godunov branch), Source/Hydro/Hydro.H:11-35 /
+ * Hydro_umdrv.cpp:11-93.  Same contract: q, qaux, srcq cover grow(bx,4);
+ * uin covers grow(bx,1); dsdt is ACCUMULATED (+=) on bx; flx[d] covers the
+ * d-faces of bx and is overwritten with extensive (area-weighted) fluxes.
+ * area[d].p / vol->p may be NULL for a uniform Cartesian grid (then
+ * area = product of the two other dx, vol = dx*dy*dz as geom.GetFaceArea /
+ * GetVolume give); srcq->p may be NULL for identically +0 sources.
+ * Scratch memory comes from a per-device pool owned by the library. */
+int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin,
+                     const camr_array4 *dsdt, const camr_array4 *q,
+                     const camr_array4 *qaux, const camr_array4 *srcq,
+                     const camr_array4 flx[3], const camr_array4 area[3],
+                     const camr_array4 *vol, double dt,
+                     const camr_hydro_params *p, void *stream);
+
+/* CAMR_estdt_hydro (Source/Utils/Timestep.H:18-61) min-reduced over bx.
+ * Synchronous: *dt_out receives the minimum on return. */
+int camr_estdt_hydro(const camr_box *bx, const camr_array4 *u,
+                     const double dx[3], double *dt_out, void *stream);
+
+/*
+ * Level driver: CAMR::CAMR_advance (Source/CAMR_advance.cpp:38-164) plus
+ * post_timestep's clean_state and computeNewDt (Source/CAMR.cpp:566-718) for
+ * one AMR level made of a list of boxes.  Device memory for the level
+ * (S_new, S_old, Sborder, sources, scratch) is owned by the level object.
+ */
+typedef struct camr_level camr_level;
+
+typedef struct camr_level_desc {
+    camr_hydro_params hp;
+    int32_t is_periodic[3];
+    int32_t add_grav_src; /* default 1 (_cpp_parameters:52) */
+    double const_grav;    /* default 0.0 (_cpp_parameters:55) */
+    double cfl, change_max, init_shrink, max_dt, fixed_dt, stop_time;
+    int32_t nbox;
+    const int32_t *box_lohi; /* host array of nbox*6 ints: lo[3], hi[3] */
+    int32_t engine;          /* 0 = fused (default), 1 = staged per-stage kernels */
+} camr_level_desc;
+
+void camr_default_level_desc(camr_level_desc *d);
+int camr_level_create(const camr_level_desc *d, camr_level **out);
+int camr_level_destroy(camr_level *L);
+int camr_level_set_stream(camr_level *L, void *stream);
+/* Valid data of box b in Array4 order (NVAR components), host memory. */
+int camr_level_upload_box(camr_level *L, int32_t b, const double *host);
+int camr_level_download_box(camr_level *L, int32_t b, double *host);
+/* Device view of the new-time state of box b (valid cells, NVAR comps). */
+int camr_level_state_array(camr_level *L, int32_t b, camr_array4 *out);
+/* CAMR::initData tail: enforce_consistent_e + computeTemp (CAMR.cpp:423-425). */
+int camr_level_init_post(camr_level *L);
+/* computeInitialDt (CAMR.cpp:638-676) / computeNewDt (CAMR.cpp:566-635). */
+int camr_level_initial_dt(camr_level *L, double *dt);
+int camr_level_new_dt(camr_level *L, double dt_old, double *dt);
+/* One CAMR_advance + post_timestep clean_state, asynchronous. */
+int camr_level_advance(camr_level *L, double dt);
+/* nsteps with the reference dt sequence; dts (host, may be NULL). */
+int camr_level_run(camr_level *L, int32_t nsteps, double *dts);
+int camr_level_sync(camr_level *L);
+double camr_level_time(const camr_level *L);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAMR_HYDRO_H */

@@ -1,0 +1,2118 @@
+/*
+ * camr_oracle.c -- TEST INFRASTRUCTURE ONLY (parity oracle + CPU baseline).
+ *
+ * Plain-C99 restatement of CAMR's single-level Godunov hydro advance.  Every
+ * function follows the reference file:line cited above it, keeping the
+ * reference's stage order, per-tile temporaries and floating-point operation
+ * order.  Build with -O3 -ffp-contract=off (no FMA contraction) so that
+ * results are those of the reference's x86-64 CPU build.
+ *
+ * std::min(a,b) == (b < a) ? b : a   and   std::max(a,b) == (a < b) ? b : a
+ * are reproduced exactly by smin/smax (NaN and signed-zero behaviour of the
+ * C++ library templates, which differs from fmin/fmax).
+ */
+#include "camr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* constants                                                                 */
+/* ------------------------------------------------------------------------ */
+#define EOS_GAMMA 1.4                /* Source/EOS/GammaLaw/EOS.H:16       */
+#define PC_RU 8.31446261815324e7     /* Source/EOS/PhysicsConstants.H:8    */
+#define PC_AIRMW 26.93               /* Source/EOS/PhysicsConstants.H:11   */
+#define NSPEC 3
+#define NGROW 4                      /* CAMR::numGrow, Source/CAMR.H:455-483 */
+
+#define ERR_NEG_ENERGY 1
+#define ERR_ZERO_DENS  2
+#define ERR_DT         4
+
+static const int UPASS[NSPEC] = {7, 8, 9};  /* IndexDefines.cpp:4-22 */
+static const int QPASS[NSPEC] = {8, 9, 10};
+
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+#define FA(f, i, j, k, n)                                                   \
+    ((f)->p[(int64_t)((i) - (f)->lo[0]) + (int64_t)((j) - (f)->lo[1]) * (f)->js + \
+            (int64_t)((k) - (f)->lo[2]) * (f)->ks + (int64_t)(n) * (f)->ns])
+#define FV(f, iv, n) FA(f, (iv)[0], (iv)[1], (iv)[2], n)
+
+typedef struct { int lo[3], hi[3]; } obox;
+
+static obox mkbox(const int lo[3], const int hi[3]) {
+    obox b; for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
+    return b;
+}
+static obox bgrow(obox b, int n) {
+    for (int d = 0; d < 3; ++d) { b.lo[d] -= n; b.hi[d] += n; }
+    return b;
+}
+static obox bgrowd(obox b, int d, int n) { b.lo[d] -= n; b.hi[d] += n; return b; }
+static obox bgrowhi(obox b, int d, int n) { b.hi[d] += n; return b; }
+static obox bnodes(obox b, int d) { b.hi[d] += 1; return b; }
+
+static int g_threads = 0;
+void ora_set_threads(int n) { g_threads = n; }
+int ora_get_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ora_default_params(ora_params *p) {
+    memset(p, 0, sizeof(*p));
+    /* Source/Params/_cpp_parameters */
+    p->ppm_type = 0; p->plm_iorder = 4; p->use_pslope = 0;
+    p->use_flattening = 1; p->transverse_reset_density = 1;
+    p->allow_negative_energy = 0; p->allow_small_energy = 1;
+    p->dual_energy_update_E_from_e = 1; p->add_grav_src = 1;
+    p->small = 1.e-8; p->small_dens = 1.e-7; p->small_pres = 1.e-200;
+    p->small_temp = 1.e2;
+    p->smallu = 1.e-12; p->small_num = 1.e-8; /* CAMR_Constants.H:9-10 */
+    p->difmag = 0.1; p->dual_energy_eta1 = 1.0e0; p->dual_energy_eta2 = 1.0e-4;
+    p->const_grav = 0.0;
+    p->cfl = 0.8; p->change_max = 1.1; p->init_shrink = 1.0;
+    p->max_dt = 1.e200; p->fixed_dt = -1.0; p->stop_time = -1.0;
+    p->tile[0] = 1024; p->tile[1] = 16; p->tile[2] = 16; /* CAMR.cpp:29-30 */
+}
+
+int ora_fab_alloc(ora_fab *f, const int lo[3], const int hi[3], int nc) {
+    int64_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1,
+            nz = hi[2] - lo[2] + 1;
+    for (int d = 0; d < 3; ++d) { f->lo[d] = lo[d]; f->hi[d] = hi[d]; }
+    f->js = nx; f->ks = nx * ny; f->ns = nx * ny * nz; f->nc = nc;
+    f->p = (double *)calloc((size_t)(f->ns * nc > 0 ? f->ns * nc : 1),
+                            sizeof(double));
+    return f->p ? 0 : -1;
+}
+static int fab_alloc_box(ora_fab *f, obox b, int nc) {
+    return ora_fab_alloc(f, b.lo, b.hi, nc);
+}
+void ora_fab_free(ora_fab *f) { free(f->p); f->p = NULL; }
+
+static int fab_contains(const ora_fab *f, const int iv[3]) {
+    for (int d = 0; d < 3; ++d)
+        if (iv[d] < f->lo[d] || iv[d] > f->hi[d]) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* EOS (Source/EOS/GammaLaw/EOS.H)                                           */
+/* ------------------------------------------------------------------------ */
+/* EY2T / REY2T, EOS.H:118-131 */
+static inline double eos_rey2t(double e) {
+    double poverrho = (EOS_GAMMA - 1.0) * e;
+    return poverrho * PC_AIRMW / PC_RU;
+}
+/* RTY2E -> TY2E -> T2Ei, EOS.H:27-33,266-281 */
+static inline double eos_rty2e(double T, const double Y[NSPEC]) {
+    double wbar = PC_AIRMW;
+    const double Cv = PC_RU / (wbar * (EOS_GAMMA - 1.0));
+    double ei[NSPEC];
+    for (int n = 0; n < NSPEC; n++) ei[n] = Cv * T;
+    double E = 0.0;
+    for (int n = 0; n < NSPEC; n++) E += Y[n] * ei[n];
+    return E;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_ctoprim, Source/Hydro/Hydro_ctoprim.H:11-101                        */
+/* ------------------------------------------------------------------------ */
+static void ctoprim_cell(int i, int j, int k, const ora_fab *u, ora_fab *q,
+                         ora_fab *qa, double small_num, double eta1,
+                         int allow_neg, int *err) {
+    const double rho = FA(u, i, j, k, O_URHO);
+    if (rho > 0.) {
+        const double rhoinv = 1.0 / rho;
+        const double vx = FA(u, i, j, k, O_UMX) * rhoinv;
+        const double vy = FA(u, i, j, k, O_UMY) * rhoinv;
+        const double vz = FA(u, i, j, k, O_UMZ) * rhoinv;
+        const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+        FA(q, i, j, k, O_QRHO) = rho;
+        FA(q, i, j, k, O_QU) = vx;
+        FA(q, i, j, k, O_QV) = vy;
+        FA(q, i, j, k, O_QW) = vz;
+        for (int ip = 0; ip < NSPEC; ++ip)
+            FA(q, i, j, k, QPASS[ip]) = FA(u, i, j, k, UPASS[ip]) * rhoinv;
+        const double ueden = FA(u, i, j, k, O_UEDEN);
+        if (!allow_neg && !(ueden > 0.)) *err |= ERR_NEG_ENERGY;
+        if ((ueden - kineng) / ueden > eta1)
+            FA(q, i, j, k, O_QREINT) = (ueden - kineng) * rhoinv;
+        else
+            FA(q, i, j, k, O_QREINT) = FA(u, i, j, k, O_UEINT) * rhoinv;
+        const double e = FA(q, i, j, k, O_QREINT);
+        if (!allow_neg && !(FA(u, i, j, k, O_UEINT) > 0.)) *err |= ERR_NEG_ENERGY;
+        for (int sp = 0; sp < NSPEC; ++sp)
+            FA(q, i, j, k, sp + O_QFS) = FA(u, i, j, k, O_UFS + sp) * rhoinv;
+        /* EOS::REY2T, REY2P, REY2Gam, REY2dpde, REY2dpdr_e */
+        const double T = eos_rey2t(e);
+        const double p = (EOS_GAMMA - 1.0) * rho * e;
+        const double gam1 = EOS_GAMMA;
+        const double dpde = (EOS_GAMMA - 1.0) * rho;
+        const double dpdr_e = (EOS_GAMMA - 1.0) * e;
+        FA(q, i, j, k, O_QTEMP) = T;
+        FA(q, i, j, k, O_QREINT) = e * rho;
+        FA(q, i, j, k, O_QPRES) = p;
+        FA(q, i, j, k, O_QGAME) = p / (e * rho) + 1.0;
+        FA(qa, i, j, k, O_QDPDR) = dpdr_e;
+        FA(qa, i, j, k, O_QDPDE) = dpde;
+        FA(qa, i, j, k, O_QGAMC) = gam1;
+        const double cs = sqrt(gam1 * p / rho);
+        FA(qa, i, j, k, O_QC) = cs;
+        FA(qa, i, j, k, O_QCSML) = smax(small_num, small_num * cs);
+    }
+}
+
+void ora_ctoprim_box(const int lo[3], const int hi[3], const ora_fab *u,
+                     ora_fab *q, ora_fab *qa, const ora_params *p, int *err) {
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                ctoprim_cell(i, j, k, u, q, qa, p->small_num,
+                             p->dual_energy_eta1, p->allow_negative_energy, err);
+}
+
+/* hydro_srctoprim, Hydro_ctoprim.H:106-157 */
+static void srctoprim_cell(int i, int j, int k, const ora_fab *q,
+                           const ora_fab *qa, const ora_fab *src,
+                           ora_fab *srcq) {
+    const double srcrho = FA(src, i, j, k, O_URHO);
+    if (FA(q, i, j, k, O_QRHO) > 0.) {
+        const double rhoinv = 1.0 / FA(q, i, j, k, O_QRHO);
+        const double vx = FA(q, i, j, k, O_QU);
+        const double vy = FA(q, i, j, k, O_QV);
+        const double vz = FA(q, i, j, k, O_QW);
+        const double smx = FA(src, i, j, k, O_UMX);
+        const double smy = FA(src, i, j, k, O_UMY);
+        const double smz = FA(src, i, j, k, O_UMZ);
+        FA(srcq, i, j, k, O_QRHO) = srcrho;
+        FA(srcq, i, j, k, O_QU) = (smx - vx * srcrho) * rhoinv;
+        FA(srcq, i, j, k, O_QV) = (smy - vy * srcrho) * rhoinv;
+        FA(srcq, i, j, k, O_QW) = (smz - vz * srcrho) * rhoinv;
+        FA(srcq, i, j, k, O_QREINT) = FA(src, i, j, k, O_UEINT);
+        for (int ip = 0; ip < NSPEC; ++ip) {
+            const int n = UPASS[ip], nq = QPASS[ip];
+            FA(srcq, i, j, k, nq) =
+                (FA(src, i, j, k, n) - FA(q, i, j, k, nq) * srcrho) * rhoinv;
+        }
+        FA(srcq, i, j, k, O_QPRES) =
+            FA(qa, i, j, k, O_QDPDE) *
+                (FA(srcq, i, j, k, O_QREINT) -
+                 FA(q, i, j, k, O_QREINT) * srcrho * rhoinv) * rhoinv +
+            FA(qa, i, j, k, O_QDPDR) * srcrho;
+    } else {
+        FA(srcq, i, j, k, O_QRHO) = 0.;
+        FA(srcq, i, j, k, O_QU) = 0.;
+        FA(srcq, i, j, k, O_QV) = 0.;
+        FA(srcq, i, j, k, O_QW) = 0.;
+        FA(srcq, i, j, k, O_QPRES) = 0.;
+        FA(srcq, i, j, k, O_QREINT) = 0.;
+    }
+}
+
+void ora_srctoprim_box(const int lo[3], const int hi[3], const ora_fab *q,
+                       const ora_fab *qa, const ora_fab *src, ora_fab *srcq) {
+    for (int k = lo[2]; k <= hi[2]; ++k)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                srctoprim_cell(i, j, k, q, qa, src, srcq);
+}
+
+/* ------------------------------------------------------------------------ */
+/* riemannus, Source/Hydro/Hydro_riemann.H:8-173 (3-D)                       */
+/* out: ustar, f_rho, f_u, f_v, f_w, f_eden, f_eint, gd_iu, gd_iv1, gd_iv2,  */
+/*      gd_pres, gd_game                                                     */
+/* ------------------------------------------------------------------------ */
+void ora_riemannus(double rl_in, double ul, double vl, double v2l,
+                   double pl_in, double rel, double rr_in, double ur,
+                   double vr, double v2r, double pr_in, double rer,
+                   double cav, double gamcl, double gamcr, double small,
+                   double smallc, double small_dens, double small_pres,
+                   double smallu, double out[12]) {
+    double wsmall = smallc * small_dens;
+    double rl = smax(rl_in, small_dens);
+    double pl = smax(pl_in, small_pres);
+    double rr = smax(rr_in, small_dens);
+    double pr = smax(pr_in, small_pres);
+
+    double wl = smax(wsmall, sqrt(fabs(gamcl * pl * rl)));
+    double wr = smax(wsmall, sqrt(fabs(gamcr * pr * rr)));
+
+    double pstar = ((wr * pl + wl * pr) + wl * wr * (ul - ur)) / (wl + wr);
+    pstar = smax(pstar, small_pres);
+    double ustar = ((wl * ul + wr * ur) + (pl - pr)) / (wl + wr);
+
+    double ro, uo, po, reo, gamco;
+
+    int mask = fabs(ustar) < smallu * 0.5 * (fabs(ul) + fabs(ur));
+    ustar = mask ? 0.0 : ustar;
+
+    if (ustar > 0.) {
+        ro = rl; uo = ul; po = pl; reo = rel; gamco = gamcl;
+    } else if (ustar < 0) {
+        ro = rr; uo = ur; po = pr; reo = rer; gamco = gamcr;
+    } else {
+        ro = 0.5 * (rl + rr);
+        uo = 0.5 * (ul + ur);
+        po = 0.5 * (pl + pr);
+        reo = 0.5 * (rel + rer);
+        gamco = 0.5 * (gamcl + gamcr);
+    }
+
+    ro = smax(ro, small_dens);
+    double roinv = 1.0 / ro;
+
+    double co, co2inv, rstar, cstar, entho, estar, sgnm, spout, spin, ushock,
+        frac, drho;
+    co = sqrt(fabs(gamco * po * roinv));
+    co = smax(smallc, co);
+    co2inv = 1.0 / (co * co);
+
+    drho = (pstar - po) * co2inv;
+    rstar = ro + drho;
+    rstar = smax(small_dens, rstar);
+
+    entho = (reo + po) * roinv * co2inv;
+    estar = reo + (pstar - po) * entho;
+    cstar = sqrt(fabs(gamco * pstar / rstar));
+    cstar = smax(cstar, smallc);
+
+    sgnm = (ustar < 0.) ? -1.0 : 1.0;
+    spout = co - sgnm * uo;
+    spin = cstar - sgnm * ustar;
+    ushock = 0.5 * (spin + spout);
+
+    if (pstar - po > 0.0) {
+        spin = ushock;
+        spout = ushock;
+    }
+
+    double scr;
+    if (spout - spin == 0.0) scr = small * cav;
+    else scr = spout - spin;
+
+    frac = (1.0 + (spout + spin) / scr) * 0.5;
+    frac = smax(0.0, smin(1.0, frac));
+
+    double qint_iv1, qint_iv2, qint_iu, qint_gdpres, qint_gdgame;
+    mask = ustar > 0.0;
+    qint_iv1 = mask ? vl : vr;
+    qint_iv2 = mask ? v2l : v2r;
+
+    mask = (ustar == 0.0);
+    qint_iv1 = mask ? 0.5 * (vl + vr) : qint_iv1;
+    qint_iv2 = mask ? 0.5 * (v2l + v2r) : qint_iv2;
+
+    double rho_gd = frac * rstar + (1.0 - frac) * ro;
+    qint_iu = frac * ustar + (1.0 - frac) * uo;
+    qint_gdpres = frac * pstar + (1.0 - frac) * po;
+    double re_gd = frac * estar + (1.0 - frac) * reo;
+
+    mask = (spout < 0.0);
+    rho_gd = mask ? ro : rho_gd;
+    qint_iu = mask ? uo : qint_iu;
+    qint_gdpres = mask ? po : qint_gdpres;
+    re_gd = mask ? reo : re_gd;
+
+    mask = (spin >= 0.0);
+    rho_gd = mask ? rstar : rho_gd;
+    qint_iu = mask ? ustar : qint_iu;
+    qint_gdpres = mask ? pstar : qint_gdpres;
+    re_gd = mask ? estar : re_gd;
+
+    qint_gdgame = qint_gdpres / re_gd + 1.0;
+    qint_gdpres = smax(qint_gdpres, small_pres);
+
+    double uflx_rho = rho_gd * qint_iu;
+    double uflx_u = uflx_rho * qint_iu + qint_gdpres;
+    double uflx_v = uflx_rho * qint_iv1;
+    double uflx_w = uflx_rho * qint_iv2;
+    const double rhoetot =
+        re_gd + 0.5 * rho_gd *
+                    (qint_iu * qint_iu + qint_iv1 * qint_iv1 +
+                     qint_iv2 * qint_iv2);
+    double uflx_eden = qint_iu * (rhoetot + qint_gdpres);
+    double uflx_eint = qint_iu * re_gd;
+
+    out[0] = ustar; out[1] = uflx_rho; out[2] = uflx_u; out[3] = uflx_v;
+    out[4] = uflx_w; out[5] = uflx_eden; out[6] = uflx_eint;
+    out[7] = qint_iu; out[8] = qint_iv1; out[9] = qint_iv2;
+    out[10] = qint_gdpres; out[11] = qint_gdgame;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_cmpflx (3-D), Source/Hydro/Hydro_cmpflx.H:153-310                   */
+/* Mutates ql/qr at domain faces exactly as the reference does.              */
+/* ------------------------------------------------------------------------ */
+static void cmpflx(int i, int j, int k, int bclo, int bchi, int domlo,
+                   int domhi, ora_fab *ql, ora_fab *qr, ora_fab *flx,
+                   ora_fab *qg, const ora_fab *qa, int dir, double small,
+                   double small_dens, double small_pres, double smallu) {
+    double cav, smallc, gamcl, gamcr;
+    double ul, ur, vl, vr, v2l, v2r;
+    int IU, IV, IV2, GU, GV, GV2, f_idx[3];
+    int im[3] = {i, j, k};
+    im[dir] -= 1;
+    if (dir == 0) {
+        IU = O_QU; IV = O_QV; IV2 = O_QW;
+        GU = O_GDU; GV = O_GDV; GV2 = O_GDW;
+        f_idx[0] = O_UMX; f_idx[1] = O_UMY; f_idx[2] = O_UMZ;
+    } else if (dir == 1) {
+        IU = O_QV; IV = O_QU; IV2 = O_QW;
+        GU = O_GDV; GV = O_GDU; GV2 = O_GDW;
+        f_idx[0] = O_UMY; f_idx[1] = O_UMX; f_idx[2] = O_UMZ;
+    } else {
+        IU = O_QW; IV = O_QU; IV2 = O_QV;
+        GU = O_GDW; GV = O_GDU; GV2 = O_GDV;
+        f_idx[0] = O_UMZ; f_idx[1] = O_UMX; f_idx[2] = O_UMY;
+    }
+    cav = 0.5 * (FA(qa, i, j, k, O_QC) + FV(qa, im, O_QC));
+    smallc = smax(FA(qa, i, j, k, O_QCSML), FV(qa, im, O_QCSML));
+    gamcl = FV(qa, im, O_QGAMC);
+    gamcr = FA(qa, i, j, k, O_QGAMC);
+
+    ul = FA(ql, i, j, k, IU);
+    vl = FA(ql, i, j, k, IV);
+    v2l = FA(ql, i, j, k, IV2);
+    ur = FA(qr, i, j, k, IU);
+    vr = FA(qr, i, j, k, IV);
+    v2r = FA(qr, i, j, k, IV2);
+
+    int idx = (dir == 2) ? k : ((dir == 0) ? i : j);
+
+    if (idx == domlo) {
+        if (bclo == O_Symmetry || bclo == O_SlipWall || bclo == O_NoSlipWall) {
+            FA(ql, i, j, k, O_QRHO) = FA(qr, i, j, k, O_QRHO);
+            FA(ql, i, j, k, O_QPRES) = FA(qr, i, j, k, O_QPRES);
+            FA(ql, i, j, k, O_QREINT) = FA(qr, i, j, k, O_QREINT);
+            gamcl = gamcr;
+            ul = -ur;
+            vl = vr;
+            v2l = v2r;
+        } else if (bclo == O_Outflow) {
+            ul = ur;
+            vl = vr;
+            v2l = v2r;
+            gamcl = gamcr;
+            FA(ql, i, j, k, O_QRHO) = FA(qr, i, j, k, O_QRHO);
+            FA(ql, i, j, k, O_QPRES) = FA(qr, i, j, k, O_QPRES);
+            FA(ql, i, j, k, O_QREINT) = FA(qr, i, j, k, O_QREINT);
+        }
+    }
+    if (idx == domhi + 1) {
+        if (bchi == O_Symmetry || bchi == O_SlipWall || bchi == O_NoSlipWall) {
+            FA(qr, i, j, k, O_QRHO) = FA(ql, i, j, k, O_QRHO);
+            FA(qr, i, j, k, O_QPRES) = FA(ql, i, j, k, O_QPRES);
+            FA(qr, i, j, k, O_QREINT) = FA(ql, i, j, k, O_QREINT);
+            gamcr = gamcl;
+            ur = -ul;
+            vr = vl;
+            v2r = v2l;
+        } else if (bchi == O_Outflow) {
+            FA(qr, i, j, k, O_QRHO) = FA(ql, i, j, k, O_QRHO);
+            FA(qr, i, j, k, O_QPRES) = FA(ql, i, j, k, O_QPRES);
+            FA(qr, i, j, k, O_QREINT) = FA(ql, i, j, k, O_QREINT);
+            gamcr = gamcl;
+            ur = ul;
+            vr = vl;
+            v2r = v2l;
+        }
+    }
+
+    double o[12];
+    ora_riemannus(FA(ql, i, j, k, O_QRHO), ul, vl, v2l,
+                  FA(ql, i, j, k, O_QPRES), FA(ql, i, j, k, O_QREINT),
+                  FA(qr, i, j, k, O_QRHO), ur, vr, v2r,
+                  FA(qr, i, j, k, O_QPRES), FA(qr, i, j, k, O_QREINT), cav,
+                  gamcl, gamcr, small, smallc, small_dens, small_pres, smallu,
+                  o);
+    const double ustar = o[0];
+    FA(flx, i, j, k, O_URHO) = o[1];
+    FA(flx, i, j, k, f_idx[0]) = o[2];
+    FA(flx, i, j, k, f_idx[1]) = o[3];
+    FA(flx, i, j, k, f_idx[2]) = o[4];
+    FA(flx, i, j, k, O_UEDEN) = o[5];
+    FA(flx, i, j, k, O_UEINT) = o[6];
+    FA(qg, i, j, k, GU) = o[7];
+    FA(qg, i, j, k, GV) = o[8];
+    FA(qg, i, j, k, GV2) = o[9];
+    FA(qg, i, j, k, O_GDPRES) = o[10];
+    FA(qg, i, j, k, O_GDGAME) = o[11];
+
+    const double flxrho = FA(flx, i, j, k, O_URHO);
+    for (int ip = 0; ip < NSPEC; ++ip) {
+        const int n = UPASS[ip], nqp = QPASS[ip];
+        if (ustar > 0.0)
+            FA(flx, i, j, k, n) = flxrho * FA(ql, i, j, k, nqp);
+        else if (ustar < 0.0)
+            FA(flx, i, j, k, n) = flxrho * FA(qr, i, j, k, nqp);
+        else
+            FA(flx, i, j, k, n) =
+                flxrho * 0.50 * (FA(ql, i, j, k, nqp) + FA(qr, i, j, k, nqp));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* flatten, Source/Hydro/Godunov/flatten.H:19-99                             */
+/* ------------------------------------------------------------------------ */
+double ora_flatten(const ora_fab *q, int i, int j, int k, int dir) {
+    int iv[3] = {i, j, k};
+    int ivm3[3] = {i, j, k}, ivp3[3] = {i, j, k};
+    ivm3[dir] -= 3; ivp3[dir] += 3;
+    if (!fab_contains(q, ivp3) || !fab_contains(q, ivm3)) return 1.0;
+
+    const int n = O_QPRES;
+    const double small_pres = 1.e-200;
+    const double shktst = 0.33e0;
+    const double zcut1 = 0.75e0;
+    const double zcut2 = 0.85e0;
+    const double dzcut = 1.0 / (zcut2 - zcut1);
+
+#define QOFF(off, comp) \
+    FA(q, iv[0] + (dir == 0) * (off), iv[1] + (dir == 1) * (off), \
+       iv[2] + (dir == 2) * (off), comp)
+    const double qm = QOFF(-1, n);
+    const double qp = QOFF(+1, n);
+    const double qmun = QOFF(-1, O_QU + dir);
+    const double qpun = QOFF(+1, O_QU + dir);
+    const double qmm = QOFF(-2, n);
+    const double qpp = QOFF(+2, n);
+
+    double dp = qp - qm;
+    const int ishft = (dp > 0.0) ? 1 : -1;
+
+    double denom = smax(small_pres, fabs(qpp - qmm));
+    double zeta = fabs(dp) / denom;
+    const double z = smin(1.0, smax(0.0, dzcut * (zeta - zcut1)));
+
+    double tst = (qmun - qpun >= 0.0) ? 1.0 : 0.0;
+    double tmp = smin(qp, qm);
+    const double chi = (fabs(dp) / tmp) > shktst ? tst : 0.0;
+
+    const double qpshft = QOFF(1 - ishft, n);
+    const double qmshft = QOFF(-(1 + ishft), n);
+    const double qpunshft = QOFF(1 - ishft, O_QU + dir);
+    const double qmunshft = QOFF(-(1 + ishft), O_QU + dir);
+    const double qppshft = QOFF(2 - ishft, n);
+    const double qmmshft = QOFF(-(2 + ishft), n);
+#undef QOFF
+
+    dp = qpshft - qmshft;
+    denom = smax(small_pres, fabs(qppshft - qmmshft));
+    zeta = fabs(dp) / denom;
+    const double z2 = smin(1.0, smax(0.0, dzcut * (zeta - zcut1)));
+
+    tst = (qmunshft - qpunshft >= 0.0) ? 1.0 : 0.0;
+    tmp = smin(qpshft, qmshft);
+    const double chi2 = (fabs(dp) / tmp) > shktst ? tst : 0.0;
+
+    return 1.0 - smax(chi2 * z2, chi * z);
+}
+
+/* ------------------------------------------------------------------------ */
+/* plm_slope / plm_pslope / hydro_plm_d, Source/Hydro/Godunov/PLM.H          */
+/* ------------------------------------------------------------------------ */
+#define QD(qf, iv, dir, off, comp)                                  \
+    FA(qf, (iv)[0] + ((dir) == 0) * (off), (iv)[1] + ((dir) == 1) * (off), \
+       (iv)[2] + ((dir) == 2) * (off), comp)
+
+/* PLM.H:19-79 */
+double ora_plm_slope(const ora_fab *q, int i, int j, int k, int n, int dir,
+                     double flat, int order) {
+    if (order == 1) return 0.;
+    const int iv[3] = {i, j, k};
+    double dlft, drgt, dcen, dfm = 0.0, dfp = 0.0, dlim, dsgn, dtemp;
+    const double qc = QD(q, iv, dir, 0, n);
+    const double qm2 = QD(q, iv, dir, -2, n);
+    const double qm = QD(q, iv, dir, -1, n);
+    const double qp = QD(q, iv, dir, +1, n);
+    const double qp2 = QD(q, iv, dir, +2, n);
+    if (order == 4) {
+        dlft = qm - qm2;
+        drgt = qc - qm;
+        dcen = 0.5 * (dlft + drgt);
+        dsgn = copysign(1.0, dcen);
+        dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+        dfm = dsgn * smin(dlim, fabs(dcen));
+
+        dlft = qp - qc;
+        drgt = (order == 4) ? qp2 - qp : 0.0;
+        dcen = 0.5 * (dlft + drgt);
+        dsgn = copysign(1.0, dcen);
+        dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+        dfp = dsgn * smin(dlim, fabs(dcen));
+    }
+    dlft = qc - qm;
+    drgt = qp - qc;
+    dcen = 0.5 * (dlft + drgt);
+    dsgn = copysign(1.0, dcen);
+    dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+    dtemp = 4.0 / 3.0 * dcen - 1.0 / 6.0 * (dfp + dfm);
+    return flat * dsgn * smin(dlim, fabs(dtemp));
+}
+
+/* PLM.H:84-165 */
+static double plm_pslope(const ora_fab *q, int i, int j, int k, int n,
+                         int dir, double dx, const ora_fab *srcQ, double flat,
+                         int order) {
+    if (order == 1) return 0.;
+    const int iv[3] = {i, j, k};
+    double dlft, drgt, dcen, dfm = 0.0, dfp = 0.0, dlim, dsgn, dtemp;
+    const double qc = QD(q, iv, dir, 0, n);
+    const double qm2 = QD(q, iv, dir, -2, n);
+    const double qm = QD(q, iv, dir, -1, n);
+    const double qp = QD(q, iv, dir, +1, n);
+    const double qp2 = QD(q, iv, dir, +2, n);
+    const int QIU = (dir == 0) ? O_QU : ((dir == 1) ? O_QV : O_QW);
+#define RH(off) QD(q, iv, dir, off, O_QRHO)
+#define SU(off) QD(srcQ, iv, dir, off, QIU)
+    if (order == 4) {
+        dlft = qm - qm2 - 0.25 * dx * (RH(-1) + RH(-2)) * (SU(-1) + SU(-2));
+        drgt = qc - qm - 0.25 * dx * (RH(-1) + RH(0)) * (SU(-1) + SU(0));
+        dcen = 0.5 * (dlft + drgt);
+        dsgn = copysign(1.0, dcen);
+        dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+        dfm = dsgn * smin(dlim, fabs(dcen));
+
+        dlft = qp - qc - 0.25 * dx * (RH(1) + RH(0)) * (SU(1) + SU(0));
+        drgt = qp2 - qp - 0.25 * dx * (RH(1) + RH(2)) * (SU(1) + SU(2));
+        dcen = 0.5 * (dlft + drgt);
+        dsgn = copysign(1.0, dcen);
+        dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+        dfp = dsgn * smin(dlim, fabs(dcen));
+    }
+    dlft = qc - qm - 0.25 * dx * (RH(0) + RH(-1)) * (SU(0) + SU(-1));
+    drgt = qp - qc - 0.25 * dx * (RH(0) + RH(1)) * (SU(0) + SU(1));
+    dcen = 0.5 * (dlft + drgt);
+    dsgn = copysign(1.0, dcen);
+    dlim = (dlft * drgt >= 0.0) ? 2.0 * smin(fabs(dlft), fabs(drgt)) : 0.0;
+    dtemp = 4.0 / 3.0 * dcen - 1.0 / 6.0 * (dfp + dfm);
+    double dp = flat * dsgn * smin(dlim, fabs(dtemp));
+    dp += RH(0) * SU(0) * dx;
+#undef RH
+#undef SU
+    return dp;
+}
+
+/* PLM.H:170-323 */
+static void plm_d(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
+                  const double slope[O_QVAR], const ora_fab *q, double cc,
+                  double dx, double dt, double small_dens, double small_pres) {
+    const int iv[3] = {i, j, k};
+    int ivp[3] = {i, j, k};
+    ivp[dir] += 1;
+    int l_idx[3];
+    l_idx[0] = dir;
+    l_idx[1] = (dir == 0) ? 1 : 0;
+    l_idx[2] = (dir == 2) ? 1 : 2;
+
+    const double dtdx = dt / dx;
+    const double cs = cc * cc;
+    const double rho = FV(q, iv, O_QRHO);
+    const double vel[3] = {FV(q, iv, O_QU), FV(q, iv, O_QV), FV(q, iv, O_QW)};
+    const double p = FV(q, iv, O_QPRES);
+    const double rhoe = FV(q, iv, O_QREINT);
+    const double enth = ((rhoe + p) / rho) / cs;
+    const double drho = slope[O_QRHO];
+    const double dvel[3] = {slope[O_QU], slope[O_QV], slope[O_QW]};
+    const double dp = slope[O_QPRES];
+    const double drhoe = slope[O_QREINT];
+    const double alpham = 0.5 * (dp / (rho * cc) - dvel[dir]) * rho / cc;
+    const double alphap = 0.5 * (dp / (rho * cc) + dvel[dir]) * rho / cc;
+    const double alpha0r = drho - dp / cs;
+    const double alpha0e = drhoe - dp * enth;
+    const double alpha0v = dvel[l_idx[1]];
+    const double alpha0w = dvel[l_idx[2]];
+    const double wv[3] = {vel[dir] - cc, vel[dir], vel[dir] + cc};
+
+    { /* right state on the i-1/2 interface */
+        const double rho_ref =
+            rho - 0.5 * (1.0 + dtdx * smin(wv[0], 0.0)) * drho;
+        double vel_ref[3];
+        for (int dim = 0; dim < 3; dim++)
+            vel_ref[dim] =
+                vel[dim] - 0.5 * (1.0 + dtdx * smin(wv[0], 0.0)) * dvel[dim];
+        const double p_ref = p - 0.5 * (1.0 + dtdx * smin(wv[0], 0.0)) * dp;
+        const double rhoe_ref =
+            rhoe - 0.5 * (1.0 + dtdx * smin(wv[0], 0.0)) * drhoe;
+        const double apright = 0.25 * dtdx * (wv[0] - wv[2]) *
+                               (1.0 - copysign(1.0, wv[2])) * alphap;
+        const double amright = 0.0;
+        const double azrright = 0.25 * dtdx * (wv[0] - wv[1]) *
+                                (1.0 - copysign(1.0, wv[1])) * alpha0r;
+        const double azeright = 0.25 * dtdx * (wv[0] - wv[1]) *
+                                (1.0 - copysign(1.0, wv[1])) * alpha0e;
+        const double azv1rght = 0.25 * dtdx * (wv[0] - wv[1]) *
+                                (1.0 - copysign(1.0, wv[1])) * alpha0v;
+        const double azw1rght = 0.25 * dtdx * (wv[0] - wv[1]) *
+                                (1.0 - copysign(1.0, wv[1])) * alpha0w;
+        FV(qp, iv, O_QRHO) = rho_ref + apright + amright + azrright;
+        FV(qp, iv, O_QRHO) = smax(FV(qp, iv, O_QRHO), small_dens);
+        FV(qp, iv, O_QU + l_idx[0]) =
+            vel_ref[l_idx[0]] + (apright - amright) * cc / rho;
+        FV(qp, iv, O_QU + l_idx[1]) = 0.;
+        FV(qp, iv, O_QU + l_idx[2]) = 0.;
+        FV(qp, iv, O_QU + l_idx[1]) = vel_ref[l_idx[1]] + azv1rght;
+        FV(qp, iv, O_QU + l_idx[2]) = vel_ref[l_idx[2]] + azw1rght;
+        FV(qp, iv, O_QPRES) = p_ref + (apright + amright) * cs;
+        FV(qp, iv, O_QPRES) = smax(FV(qp, iv, O_QPRES), small_pres);
+        FV(qp, iv, O_QREINT) =
+            rhoe_ref + (apright + amright) * enth * cs + azeright;
+    }
+    { /* left state on the i+1/2 interface */
+        const double rho_ref =
+            rho + 0.5 * (1.0 - dtdx * smax(wv[2], 0.0)) * drho;
+        double vel_ref[3];
+        for (int dim = 0; dim < 3; dim++)
+            vel_ref[dim] =
+                vel[dim] + 0.5 * (1.0 - dtdx * smax(wv[2], 0.0)) * dvel[dim];
+        const double p_ref = p + 0.5 * (1.0 - dtdx * smax(wv[2], 0.0)) * dp;
+        const double rhoe_ref =
+            rhoe + 0.5 * (1.0 - dtdx * smax(wv[2], 0.0)) * drhoe;
+        const double apleft = 0.0;
+        const double amleft = 0.25 * dtdx * (wv[2] - wv[0]) *
+                              (1.0 + copysign(1.0, wv[0])) * alpham;
+        const double azrleft = 0.25 * dtdx * (wv[2] - wv[1]) *
+                               (1.0 + copysign(1.0, wv[1])) * alpha0r;
+        const double azeleft = 0.25 * dtdx * (wv[2] - wv[1]) *
+                               (1.0 + copysign(1.0, wv[1])) * alpha0e;
+        const double azv1left = 0.25 * dtdx * (wv[2] - wv[1]) *
+                                (1.0 + copysign(1.0, wv[1])) * alpha0v;
+        const double azw1left = 0.25 * dtdx * (wv[2] - wv[1]) *
+                                (1.0 + copysign(1.0, wv[1])) * alpha0w;
+        FV(qm, ivp, O_QRHO) = rho_ref + apleft + amleft + azrleft;
+        FV(qm, ivp, O_QRHO) = smax(FV(qm, ivp, O_QRHO), small_dens);
+        FV(qm, ivp, O_QU + l_idx[0]) =
+            vel_ref[l_idx[0]] + (apleft - amleft) * cc / rho;
+        FV(qm, ivp, O_QU + l_idx[1]) = 0.;
+        FV(qm, ivp, O_QU + l_idx[2]) = 0.;
+        FV(qm, ivp, O_QU + l_idx[1]) = vel_ref[l_idx[1]] + azv1left;
+        FV(qm, ivp, O_QU + l_idx[2]) = vel_ref[l_idx[2]] + azw1left;
+        FV(qm, ivp, O_QPRES) = p_ref + (apleft + amleft) * cs;
+        FV(qm, ivp, O_QPRES) = smax(FV(qm, ivp, O_QPRES), small_pres);
+        FV(qm, ivp, O_QREINT) =
+            rhoe_ref + (apleft + amleft) * enth * cs + azeleft;
+    }
+    /* upwind the passive variables */
+    for (int ip = 0; ip < NSPEC; ++ip) {
+        const int n = QPASS[ip];
+        const double vel_ad = FV(q, iv, O_QU + l_idx[0]);
+        {
+            const double spzero = (vel_ad > 0) ? -1.0 : vel_ad * dtdx;
+            const double acmprght = 0.5 * (-1.0 - spzero) * slope[n];
+            FV(qp, iv, n) = FV(q, iv, n) + acmprght;
+        }
+        {
+            const double spzero = vel_ad >= 0 ? vel_ad * dtdx : 1.0;
+            const double acmpleft = 0.5 * (1.0 - spzero) * slope[n];
+            FV(qm, ivp, n) = FV(q, iv, n) + acmpleft;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* PPM, Source/Hydro/Godunov/PPM.H:25-183 and PPM.cpp:11-411                 */
+/* ------------------------------------------------------------------------ */
+void ora_ppm_reconstruct(const double *s, double flatn, double *smo,
+                         double *spo) {
+    enum { im2 = 0, im1 = 1, i0 = 2, ip1 = 3, ip2 = 4 };
+    double sm, sp;
+    double dsl = 2.0 * (s[im1] - s[im2]);
+    double dsr = 2.0 * (s[i0] - s[im1]);
+    double dsvl_l = 0.0;
+    if (dsl * dsr > 0.0) {
+        double dsc = 0.5 * (s[i0] - s[im2]);
+        dsvl_l = copysign(1.0, dsc) * smin(fabs(dsc), smin(fabs(dsl), fabs(dsr)));
+    }
+    dsl = 2.0 * (s[i0] - s[im1]);
+    dsr = 2.0 * (s[ip1] - s[i0]);
+    double dsvl_r = 0.0;
+    if (dsl * dsr > 0.0) {
+        double dsc = 0.5 * (s[ip1] - s[im1]);
+        dsvl_r = copysign(1.0, dsc) * smin(fabs(dsc), smin(fabs(dsl), fabs(dsr)));
+    }
+    sm = 0.5 * (s[i0] + s[im1]) - (1.0 / 6.0) * (dsvl_r - dsvl_l);
+    sm = smax(sm, smin(s[i0], s[im1]));
+    sm = smin(sm, smax(s[i0], s[im1]));
+
+    dsl = 2.0 * (s[i0] - s[im1]);
+    dsr = 2.0 * (s[ip1] - s[i0]);
+    dsvl_l = 0.0;
+    if (dsl * dsr > 0.0) {
+        double dsc = 0.5 * (s[ip1] - s[im1]);
+        dsvl_l = copysign(1.0, dsc) * smin(fabs(dsc), smin(fabs(dsl), fabs(dsr)));
+    }
+    dsl = 2.0 * (s[ip1] - s[i0]);
+    dsr = 2.0 * (s[ip2] - s[ip1]);
+    dsvl_r = 0.0;
+    if (dsl * dsr > 0.0) {
+        double dsc = 0.5 * (s[ip2] - s[i0]);
+        dsvl_r = copysign(1.0, dsc) * smin(fabs(dsc), smin(fabs(dsl), fabs(dsr)));
+    }
+    sp = 0.5 * (s[ip1] + s[i0]) - (1.0 / 6.0) * (dsvl_r - dsvl_l);
+    sp = smax(sp, smin(s[ip1], s[i0]));
+    sp = smin(sp, smax(s[ip1], s[i0]));
+
+    sm = flatn * sm + (1.0 - flatn) * s[i0];
+    sp = flatn * sp + (1.0 - flatn) * s[i0];
+
+    if ((sp - s[i0]) * (s[i0] - sm) <= 0.0) {
+        sp = s[i0];
+        sm = s[i0];
+    } else if (fabs(sp - s[i0]) >= 2.0 * fabs(sm - s[i0])) {
+        sp = 3.0 * s[i0] - 2.0 * sm;
+    } else if (fabs(sm - s[i0]) >= 2.0 * fabs(sp - s[i0])) {
+        sm = 3.0 * s[i0] - 2.0 * sp;
+    }
+    *smo = sm;
+    *spo = sp;
+}
+
+void ora_ppm_int_profile(double sm, double sp, double sc, double u, double c,
+                         double dtdx, double *Ip, double *Im) {
+    double s6 = 6.0 * sc - 3.0 * (sm + sp);
+    double speed = u - c;
+    double sigma = fabs(speed) * dtdx;
+    if (speed <= 0.0) {
+        Ip[0] = sp;
+        Im[0] = sm + 0.5 * sigma * (sp - sm + (1.0 - (2.0 / 3.0) * sigma) * s6);
+    } else {
+        Ip[0] = sp - 0.5 * sigma * (sp - sm - (1.0 - (2.0 / 3.0) * sigma) * s6);
+        Im[0] = sm;
+    }
+    speed = u;
+    sigma = fabs(speed) * dtdx;
+    if (speed <= 0.0) {
+        Ip[1] = sp;
+        Im[1] = sm + 0.5 * sigma * (sp - sm + (1.0 - (2.0 / 3.0) * sigma) * s6);
+    } else {
+        Ip[1] = sp - 0.5 * sigma * (sp - sm - (1.0 - (2.0 / 3.0) * sigma) * s6);
+        Im[1] = sm;
+    }
+    speed = u + c;
+    sigma = fabs(speed) * dtdx;
+    if (speed <= 0.0) {
+        Ip[2] = sp;
+        Im[2] = sm + 0.5 * sigma * (sp - sm + (1.0 - (2.0 / 3.0) * sigma) * s6);
+    } else {
+        Ip[2] = sp - 0.5 * sigma * (sp - sm - (1.0 - (2.0 / 3.0) * sigma) * s6);
+        Im[2] = sm;
+    }
+}
+
+/* trace_ppm, PPM.cpp:11-411 */
+static void trace_ppm(obox bx, int idir, const ora_fab *q, const ora_fab *qa,
+                      ora_fab *qm, ora_fab *qp, obox vbx, double dt,
+                      const double dx[3], int use_flattening,
+                      double small_dens, double small_pres) {
+    const double dtdx = dt / dx[idir];
+    int QUN = 0, QUT = 0, QUTT = 0;
+    if (idir == 0) { QUN = O_QU; QUT = O_QV; QUTT = O_QW; }
+    else if (idir == 1) { QUN = O_QV; QUT = O_QW; QUTT = O_QU; }
+    else { QUN = O_QW; QUT = O_QU; QUTT = O_QV; }
+
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        const int iv[3] = {i, j, k};
+        int ivp1[3] = {i, j, k};
+        ivp1[idir] += 1;
+        double cc = sqrt(FV(qa, iv, O_QGAMC) * FV(q, iv, O_QPRES) /
+                         FV(q, iv, O_QRHO));
+        double un = FV(q, iv, QUN);
+        double flat = 1.0;
+        if (use_flattening == 1)
+            for (int dir_flat = 0; dir_flat < 3; dir_flat++)
+                flat = smin(flat, ora_flatten(q, i, j, k, dir_flat));
+
+        double Ip[O_QVAR + 1][3];
+        double Im[O_QVAR + 1][3];
+        for (int n = 0; n < O_QVAR; n++) {
+            double s[5];
+            for (int o = -2; o <= 2; ++o) s[o + 2] = QD(q, iv, idir, o, n);
+            double sm, sp;
+            ora_ppm_reconstruct(s, flat, &sm, &sp);
+            ora_ppm_int_profile(sm, sp, s[2], un, cc, dtdx, Ip[n], Im[n]);
+        }
+        {
+            double s[5];
+            for (int o = -2; o <= 2; ++o) s[o + 2] = QD(qa, iv, idir, o, O_QGAMC);
+            double sm, sp;
+            ora_ppm_reconstruct(s, flat, &sm, &sp);
+            ora_ppm_int_profile(sm, sp, s[2], un, cc, dtdx, Ip[O_QVAR],
+                                Im[O_QVAR]);
+        }
+        for (int ip = 0; ip < NSPEC; ++ip) {
+            const int n = QPASS[ip];
+            if (iv[idir] >= vbx.lo[idir]) FV(qp, iv, n) = Im[n][1];
+            if (iv[idir] <= vbx.hi[idir]) FV(qm, ivp1, n) = Ip[n][1];
+        }
+        if (iv[idir] >= vbx.lo[idir]) {
+            double rho_ref = Im[O_QRHO][0];
+            double un_ref = Im[QUN][0];
+            double p_ref = Im[O_QPRES][0];
+            double rhoe_g_ref = Im[O_QREINT][0];
+            double gam_ref = Im[O_QVAR][0];
+            rho_ref = smax(rho_ref, small_dens);
+            double rho_ref_inv = 1.0 / rho_ref;
+            p_ref = smax(p_ref, small_pres);
+            double cc_ref = sqrt(gam_ref * p_ref / rho_ref);
+            double csq_ref = cc_ref * cc_ref;
+            double cc_ref_inv = 1.0 / cc_ref;
+            double h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv / csq_ref;
+            double dum = un_ref - Im[QUN][0];
+            double dptotm = p_ref - Im[O_QPRES][0];
+            double drho = rho_ref - Im[O_QRHO][1];
+            double dptot = p_ref - Im[O_QPRES][1];
+            double drhoe_g = rhoe_g_ref - Im[O_QREINT][1];
+            double dup = un_ref - Im[QUN][2];
+            double dptotp = p_ref - Im[O_QPRES][2];
+            double alpham =
+                0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+            double alphap =
+                0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+            double alpha0r = drho - dptot / csq_ref;
+            double alpha0e_g = drhoe_g - dptot * h_g_ref;
+            if (un - cc > 0.) alpham = 0.;
+            else if (un - cc < 0.) alpham *= -1.0;
+            else alpham *= -0.5;
+            if (un + cc > 0.) alphap = 0.;
+            else if (un + cc < 0.) alphap *= -1.0;
+            else alphap *= -0.5;
+            if (un > 0.) { alpha0r = 0.; alpha0e_g = 0.; }
+            else if (un < 0.) { alpha0r *= -1.0; alpha0e_g *= -1.0; }
+            else { alpha0r *= -0.5; alpha0e_g *= -0.5; }
+            FV(qp, iv, O_QRHO) = rho_ref + alphap + alpham + alpha0r;
+            FV(qp, iv, QUN) = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
+            FV(qp, iv, O_QPRES) = p_ref + (alphap + alpham) * csq_ref;
+            FV(qp, iv, O_QRHO) = smax(FV(qp, iv, O_QRHO), small_dens);
+            FV(qp, iv, O_QPRES) = smax(FV(qp, iv, O_QPRES), small_pres);
+            FV(qp, iv, QUT) = Im[QUT][1];
+            FV(qp, iv, QUTT) = Im[QUTT][1];
+            FV(qp, iv, O_QREINT) =
+                rhoe_g_ref + (alphap + alpham) * h_g_ref * csq_ref + alpha0e_g;
+        }
+        if (iv[idir] <= vbx.hi[idir]) {
+            double rho_ref = Ip[O_QRHO][2];
+            double un_ref = Ip[QUN][2];
+            double p_ref = Ip[O_QPRES][2];
+            double rhoe_g_ref = Ip[O_QREINT][2];
+            double gam_ref = Ip[O_QVAR][2];
+            rho_ref = smax(rho_ref, small_dens);
+            double rho_ref_inv = 1.0 / rho_ref;
+            p_ref = smax(p_ref, small_pres);
+            double cc_ref = sqrt(gam_ref * p_ref / rho_ref);
+            double csq_ref = cc_ref * cc_ref;
+            double cc_ref_inv = 1.0 / cc_ref;
+            double h_g_ref = (p_ref + rhoe_g_ref) * rho_ref_inv / csq_ref;
+            double dum = un_ref - Ip[QUN][0];
+            double dptotm = p_ref - Ip[O_QPRES][0];
+            double drho = rho_ref - Ip[O_QRHO][1];
+            double dptot = p_ref - Ip[O_QPRES][1];
+            double drhoe_g = rhoe_g_ref - Ip[O_QREINT][1];
+            double dup = un_ref - Ip[QUN][2];
+            double dptotp = p_ref - Ip[O_QPRES][2];
+            double alpham =
+                0.5 * (dptotm * rho_ref_inv * cc_ref_inv - dum) * rho_ref * cc_ref_inv;
+            double alphap =
+                0.5 * (dptotp * rho_ref_inv * cc_ref_inv + dup) * rho_ref * cc_ref_inv;
+            double alpha0r = drho - dptot / csq_ref;
+            double alpha0e_g = drhoe_g - dptot * h_g_ref;
+            if (un - cc > 0.) alpham *= -1.0;
+            else if (un - cc < 0.) alpham = 0.;
+            else alpham *= -0.5;
+            if (un + cc > 0.) alphap *= -1.0;
+            else if (un + cc < 0.) alphap = 0.;
+            else alphap *= -0.5;
+            if (un > 0.) { alpha0r *= -1.0; alpha0e_g *= -1.0; }
+            else if (un < 0.) { alpha0r = 0.; alpha0e_g = 0.; }
+            else { alpha0r *= -0.5; alpha0e_g *= -0.5; }
+            FV(qm, ivp1, O_QRHO) = rho_ref + alphap + alpham + alpha0r;
+            FV(qm, ivp1, QUN) = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
+            FV(qm, ivp1, O_QPRES) = p_ref + (alphap + alpham) * csq_ref;
+            FV(qm, ivp1, O_QRHO) = smax(FV(qm, ivp1, O_QRHO), small_dens);
+            /* PPM.cpp:398 clamps the minus-state pressure with small_dens */
+            FV(qm, ivp1, O_QPRES) = smax(FV(qm, ivp1, O_QPRES), small_dens);
+            FV(qm, ivp1, QUT) = Ip[QUT][1];
+            FV(qm, ivp1, QUTT) = Ip[QUTT][1];
+            FV(qm, ivp1, O_QREINT) =
+                rhoe_g_ref + (alphap + alpham) * h_g_ref * csq_ref + alpha0e_g;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_transdo / hydro_transdd, Godunov_utils_3D.H:22-239, 245-482        */
+/* (non-EB: every face open)                                                  */
+/* ------------------------------------------------------------------------ */
+static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
+                    ora_fab *qp, const ora_fab *qnormm, const ora_fab *qnormp,
+                    const ora_fab *flxx, const ora_fab *qa,
+                    const ora_fab *qint, double cdtdx, int trd,
+                    double small_pres) {
+    const int iv[3] = {i, j, k};
+    int ivpn[3] = {i, j, k}, ivpt[3] = {i, j, k};
+    ivpn[dir] += 1;
+    ivpt[other_dir] += 1;
+    const int qvidx = (other_dir == 0) ? O_GDU : ((other_dir == 1) ? O_GDV : O_GDW);
+
+    const double flxrho = cdtdx * (FV(flxx, ivpt, O_URHO) - FV(flxx, iv, O_URHO));
+    const double flxu = cdtdx * (FV(flxx, ivpt, O_UMX) - FV(flxx, iv, O_UMX));
+    const double flxv = cdtdx * (FV(flxx, ivpt, O_UMY) - FV(flxx, iv, O_UMY));
+    const double flxw = cdtdx * (FV(flxx, ivpt, O_UMZ) - FV(flxx, iv, O_UMZ));
+    const double flxe = cdtdx * (FV(flxx, ivpt, O_UEDEN) - FV(flxx, iv, O_UEDEN));
+    const double c = FV(qa, iv, O_QGAMC);
+
+    for (int ip = 0; ip < NSPEC; ++ip) {
+        const int n = UPASS[ip], nqp = QPASS[ip];
+        const double compn = cdtdx * (FV(flxx, ivpt, n) - FV(flxx, iv, n));
+        double rr, rrnew, compo;
+        rr = FV(qnormp, iv, O_QRHO);
+        rrnew = rr - flxrho;
+        compo = rr * FV(qnormp, iv, nqp) - compn;
+        FV(qp, iv, nqp) = compo / rrnew;
+
+        rr = FV(qnormm, ivpn, O_QRHO);
+        rrnew = rr - flxrho;
+        compo = rr * FV(qnormm, ivpn, nqp) - compn;
+        FV(qm, ivpn, nqp) = compo / rrnew;
+    }
+
+    double dAup, pav, dAu;
+    {
+        const double pggp = FV(qint, ivpt, O_GDPRES);
+        const double pggm = FV(qint, iv, O_GDPRES);
+        const double ugp = FV(qint, ivpt, qvidx);
+        const double ugm = FV(qint, iv, qvidx);
+        dAup = pggp * ugp - pggm * ugm;
+        pav = 0.5 * (pggp + pggm);
+        dAu = ugp - ugm;
+    }
+    { /* QP */
+        const double rrr = FV(qnormp, iv, O_QRHO);
+        double rur = FV(qnormp, iv, O_QU);
+        double rvr = FV(qnormp, iv, O_QV);
+        double rwr = FV(qnormp, iv, O_QW);
+        const double ekinr = 0.5 * rrr * (rur * rur + rvr * rvr + rwr * rwr);
+        rur *= rrr;
+        rvr *= rrr;
+        rwr *= rrr;
+        const double rer = FV(qnormp, iv, O_QREINT) + ekinr;
+        double rrnewr = rrr - flxrho;
+        double runewr = rur - flxu;
+        double rvnewr = rvr - flxv;
+        double rwnewr = rwr - flxw;
+        double renewr = rer - flxe;
+        int reset_state = 0;
+        if (trd == 1 && rrnewr < 0.) {
+            rrnewr = rrr; runewr = rur; rvnewr = rvr; rwnewr = rwr;
+            renewr = rer; reset_state = 1;
+        }
+        FV(qp, iv, O_QRHO) = rrnewr;
+        FV(qp, iv, O_QU) = runewr / rrnewr;
+        FV(qp, iv, O_QV) = rvnewr / rrnewr;
+        FV(qp, iv, O_QW) = rwnewr / rrnewr;
+        const double rhoekinr =
+            0.5 * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+        FV(qp, iv, O_QREINT) = renewr - rhoekinr;
+        if (!reset_state) {
+            double pnewr = FV(qnormp, iv, O_QPRES) -
+                           cdtdx * (dAup + pav * dAu * (c - 1.0));
+            FV(qp, iv, O_QPRES) = smax(pnewr, small_pres);
+        } else {
+            FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES);
+        }
+    }
+    { /* QM */
+        const double rrl = FV(qnormm, ivpn, O_QRHO);
+        double rul = FV(qnormm, ivpn, O_QU);
+        double rvl = FV(qnormm, ivpn, O_QV);
+        double rwl = FV(qnormm, ivpn, O_QW);
+        const double ekinl = 0.5 * rrl * (rul * rul + rvl * rvl + rwl * rwl);
+        rul *= rrl;
+        rvl *= rrl;
+        rwl *= rrl;
+        const double rel = FV(qnormm, ivpn, O_QREINT) + ekinl;
+        double rrnewl = rrl - flxrho;
+        double runewl = rul - flxu;
+        double rvnewl = rvl - flxv;
+        double rwnewl = rwl - flxw;
+        double renewl = rel - flxe;
+        int reset_state = 0;
+        if (trd == 1 && rrnewl < 0.) {
+            rrnewl = rrl; runewl = rul; rvnewl = rvl; rwnewl = rwl;
+            renewl = rel; reset_state = 1;
+        }
+        FV(qm, ivpn, O_QRHO) = rrnewl;
+        FV(qm, ivpn, O_QU) = runewl / rrnewl;
+        FV(qm, ivpn, O_QV) = rvnewl / rrnewl;
+        FV(qm, ivpn, O_QW) = rwnewl / rrnewl;
+        const double rhoekinl =
+            0.5 * (runewl * runewl + rvnewl * rvnewl + rwnewl * rwnewl) / rrnewl;
+        FV(qm, ivpn, O_QREINT) = renewl - rhoekinl;
+        if (!reset_state) {
+            double pnewl = FV(qnormm, ivpn, O_QPRES) -
+                           cdtdx * (dAup + pav * dAu * (c - 1.0));
+            FV(qm, ivpn, O_QPRES) = smax(pnewl, small_pres);
+        } else {
+            FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES);
+        }
+    }
+}
+
+static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
+                    const ora_fab *qnormm, const ora_fab *qnormp,
+                    const ora_fab *flxx, const ora_fab *flxy,
+                    const ora_fab *qx, const ora_fab *qy, const ora_fab *qa,
+                    const ora_fab *srcq, double hdt, double cdtdx0,
+                    double cdtdx1, int trd, double small_pres) {
+    const int iv[3] = {i, j, k};
+    const int qvidx0 = (dir == 0) ? O_GDV : O_GDU;
+    const int qvidx1 = (dir == 2) ? O_GDV : O_GDW;
+    const int l1 = (dir == 0) ? 1 : 0;
+    const int l2 = (dir == 2) ? 1 : 2;
+    int ivpt0[3] = {i, j, k}, ivpt1[3] = {i, j, k}, ivpn[3] = {i, j, k};
+    ivpt0[l1] += 1;
+    ivpt1[l2] += 1;
+    ivpn[dir] += 1;
+
+#define TF(comp) (cdtdx0 * (FV(flxx, ivpt0, comp) - FV(flxx, iv, comp)) + \
+                  cdtdx1 * (FV(flxy, ivpt1, comp) - FV(flxy, iv, comp)))
+    const double flxrho = TF(O_URHO);
+    const double flxu = TF(O_UMX);
+    const double flxv = TF(O_UMY);
+    const double flxw = TF(O_UMZ);
+    const double flxe = TF(O_UEDEN);
+    const double c = FV(qa, iv, O_QGAMC);
+    const double srcrho = FV(srcq, iv, O_QRHO);
+    const double srcu = FV(srcq, iv, O_QU);
+    const double srcv = FV(srcq, iv, O_QV);
+    const double srcw = FV(srcq, iv, O_QW);
+    const double srce = FV(srcq, iv, O_QREINT);
+    const double srcp = FV(srcq, iv, O_QPRES);
+
+    const double rrr = FV(qnormp, iv, O_QRHO);
+    const double rrl = FV(qnormm, ivpn, O_QRHO);
+    double rrnewl = rrl - flxrho;
+    double rrnewr = rrr - flxrho;
+
+    for (int ip = 0; ip < NSPEC; ++ip) {
+        const int n = UPASS[ip], nqp = QPASS[ip];
+        const double srcpass = FV(srcq, iv, nqp);
+        const double compn = TF(n);
+        double compo;
+        compo = rrr * FV(qnormp, iv, nqp) - compn;
+        FV(qp, iv, nqp) = compo / rrnewr + hdt * srcpass;
+        compo = rrl * FV(qnormm, ivpn, nqp) - compn;
+        FV(qm, ivpn, nqp) = compo / rrnewl + hdt * srcpass;
+    }
+#undef TF
+
+    double dAupx, pavx, dAux, dAupy, pavy, dAuy;
+    {
+        const double pggpx = FV(qx, ivpt0, O_GDPRES);
+        const double pggmx = FV(qx, iv, O_GDPRES);
+        const double ugpx = FV(qx, ivpt0, qvidx0);
+        const double ugmx = FV(qx, iv, qvidx0);
+        dAupx = pggpx * ugpx - pggmx * ugmx;
+        pavx = 0.5 * (pggpx + pggmx);
+        dAux = ugpx - ugmx;
+        const double pggpy = FV(qy, ivpt1, O_GDPRES);
+        const double pggmy = FV(qy, iv, O_GDPRES);
+        const double ugpy = FV(qy, ivpt1, qvidx1);
+        const double ugmy = FV(qy, iv, qvidx1);
+        dAupy = pggpy * ugpy - pggmy * ugmy;
+        pavy = 0.5 * (pggpy + pggmy);
+        dAuy = ugpy - ugmy;
+    }
+    const double pxnew = cdtdx0 * (dAupx + pavx * dAux * (c - 1.0));
+    const double pynew = cdtdx1 * (dAupy + pavy * dAuy * (c - 1.0));
+
+    { /* qp */
+        const double rur = rrr * FV(qnormp, iv, O_QU);
+        const double rvr = rrr * FV(qnormp, iv, O_QV);
+        const double rwr = rrr * FV(qnormp, iv, O_QW);
+        const double ekinr = 0.5 * (rur * rur + rvr * rvr + rwr * rwr) / rrr;
+        const double rer = FV(qnormp, iv, O_QREINT) + ekinr;
+        double runewr = rur - flxu;
+        double rvnewr = rvr - flxv;
+        double rwnewr = rwr - flxw;
+        double renewr = rer - flxe;
+        int reset_state = 0;
+        if (trd == 1 && rrnewr < 0.) {
+            rrnewr = rrr; runewr = rur; rvnewr = rvr; rwnewr = rwr;
+            renewr = rer; reset_state = 1;
+        }
+        FV(qp, iv, O_QRHO) = rrnewr + hdt * srcrho;
+        FV(qp, iv, O_QU) = runewr / rrnewr + hdt * srcu;
+        FV(qp, iv, O_QV) = rvnewr / rrnewr + hdt * srcv;
+        FV(qp, iv, O_QW) = rwnewr / rrnewr + hdt * srcw;
+        const double rhoekinr =
+            0.5 * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
+        FV(qp, iv, O_QREINT) = renewr - rhoekinr + hdt * srce;
+        if (!reset_state)
+            FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES) - pxnew - pynew + hdt * srcp;
+        else
+            FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES) + hdt * srcp;
+        FV(qp, iv, O_QPRES) = smax(FV(qp, iv, O_QPRES), small_pres);
+    }
+    { /* qm */
+        const double rul = rrl * FV(qnormm, ivpn, O_QU);
+        const double rvl = rrl * FV(qnormm, ivpn, O_QV);
+        const double rwl = rrl * FV(qnormm, ivpn, O_QW);
+        const double ekinl = 0.5 * (rul * rul + rvl * rvl + rwl * rwl) / rrl;
+        const double rel = FV(qnormm, ivpn, O_QREINT) + ekinl;
+        double runewl = rul - flxu;
+        double rvnewl = rvl - flxv;
+        double rwnewl = rwl - flxw;
+        double renewl = rel - flxe;
+        int reset_state = 0;
+        if (trd == 1 && rrnewl < 0.) {
+            rrnewl = rrl; runewl = rul; rvnewl = rvl; rwnewl = rwl;
+            renewl = rel; reset_state = 1;
+        }
+        FV(qm, ivpn, O_QRHO) = rrnewl + hdt * srcrho;
+        FV(qm, ivpn, O_QU) = runewl / rrnewl + hdt * srcu;
+        FV(qm, ivpn, O_QV) = rvnewl / rrnewl + hdt * srcv;
+        FV(qm, ivpn, O_QW) = rwnewl / rrnewl + hdt * srcw;
+        const double rhoekinl =
+            0.5 * (runewl * runewl + rvnewl * rvnewl + rwnewl * rwnewl) / rrnewl;
+        FV(qm, ivpn, O_QREINT) = renewl - rhoekinl + hdt * srce;
+        if (!reset_state)
+            FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES) - pxnew - pynew + hdt * srcp;
+        else
+            FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES) + hdt * srcp;
+        FV(qm, ivpn, O_QPRES) = smax(FV(qm, ivpn, O_QPRES), small_pres);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hydro_utils_K.H: hydro_divu :12-169, hydro_pdivu :175-202,                */
+/* hydro_norm_spec_flx :207-219, hydro_ext_flx :224-234, hydro_update        */
+/* :239-257; hydro_artif_visc.H:7-64                                          */
+/* ------------------------------------------------------------------------ */
+static void divu_node(int i, int j, int k, const ora_fab *q, double dx,
+                      double dy, double dz, ora_fab *divu, const int domlo[3],
+                      const int domhi[3], const int bclo[3],
+                      const int bchi[3]) {
+#define Q(a, b, c, n) FA(q, a, b, c, n)
+    const int NS = O_NoSlipWall;
+    double ux, vy, wz;
+    ux = 0.25 * (Q(i, j, k, O_QU) - Q(i - 1, j, k, O_QU) +
+                 Q(i, j, k - 1, O_QU) - Q(i - 1, j, k - 1, O_QU) +
+                 Q(i, j - 1, k, O_QU) - Q(i - 1, j - 1, k, O_QU) +
+                 Q(i, j - 1, k - 1, O_QU) - Q(i - 1, j - 1, k - 1, O_QU)) / dx;
+    vy = 0.25 * (Q(i, j, k, O_QV) - Q(i, j - 1, k, O_QV) +
+                 Q(i, j, k - 1, O_QV) - Q(i, j - 1, k - 1, O_QV) +
+                 Q(i - 1, j, k, O_QV) - Q(i - 1, j - 1, k, O_QV) +
+                 Q(i - 1, j, k - 1, O_QV) - Q(i - 1, j - 1, k - 1, O_QV)) / dy;
+    wz = 0.25 * (Q(i, j, k, O_QW) - Q(i, j, k - 1, O_QW) +
+                 Q(i - 1, j, k, O_QW) - Q(i - 1, j, k - 1, O_QW) +
+                 Q(i, j - 1, k, O_QW) - Q(i, j - 1, k - 1, O_QW) +
+                 Q(i - 1, j - 1, k, O_QW) - Q(i - 1, j - 1, k - 1, O_QW)) / dz;
+
+    if (i == domlo[0] && bclo[0] == NS) {
+        vy = 0.5 * (Q(i, j, k, O_QV) - Q(i, j - 1, k, O_QV) +
+                    Q(i, j, k - 1, O_QV) - Q(i, j - 1, k - 1, O_QV)) / dy;
+        wz = 0.5 * (Q(i, j, k, O_QW) - Q(i, j, k - 1, O_QW) +
+                    Q(i, j - 1, k, O_QW) - Q(i, j - 1, k - 1, O_QW)) / dz;
+    } else if (i == domhi[0] + 1 && bchi[0] == NS) {
+        vy = 0.5 * (Q(i - 1, j, k, O_QV) - Q(i - 1, j - 1, k, O_QV) +
+                    Q(i - 1, j, k - 1, O_QV) - Q(i - 1, j - 1, k - 1, O_QV)) / dy;
+        wz = 0.5 * (Q(i - 1, j, k, O_QW) - Q(i - 1, j, k - 1, O_QW) +
+                    Q(i - 1, j - 1, k, O_QW) - Q(i - 1, j - 1, k - 1, O_QW)) / dz;
+    }
+    if (j == domlo[1] && bclo[1] == NS) {
+        ux = 0.5 * (Q(i, j, k, O_QU) - Q(i - 1, j, k, O_QU) +
+                    Q(i, j, k - 1, O_QU) - Q(i - 1, j, k - 1, O_QU)) / dx;
+        wz = 0.5 * (Q(i, j, k, O_QW) - Q(i, j, k - 1, O_QW) +
+                    Q(i - 1, j, k, O_QW) - Q(i - 1, j, k - 1, O_QW)) / dz;
+    } else if (j == domhi[1] + 1 && bchi[1] == NS) {
+        ux = 0.5 * (Q(i, j - 1, k, O_QU) - Q(i - 1, j - 1, k, O_QU) +
+                    Q(i, j - 1, k - 1, O_QU) - Q(i - 1, j - 1, k - 1, O_QU)) / dx;
+        wz = 0.5 * (Q(i, j - 1, k, O_QW) - Q(i, j - 1, k - 1, O_QW) +
+                    Q(i - 1, j - 1, k, O_QW) - Q(i - 1, j - 1, k - 1, O_QW)) / dz;
+    }
+    if (k == domlo[2] && bclo[2] == NS) {
+        ux = 0.5 * (Q(i, j, k, O_QU) - Q(i - 1, j, k, O_QU) +
+                    Q(i, j - 1, k, O_QU) - Q(i - 1, j - 1, k, O_QU)) / dx;
+        vy = 0.5 * (Q(i, j, k, O_QV) - Q(i, j - 1, k, O_QV) +
+                    Q(i - 1, j, k, O_QV) - Q(i - 1, j - 1, k, O_QV)) / dy;
+    } else if (k == domhi[2] + 1 && bchi[2] == NS) {
+        ux = 0.5 * (Q(i, j, k - 1, O_QU) - Q(i - 1, j, k - 1, O_QU) +
+                    Q(i, j - 1, k - 1, O_QU) - Q(i - 1, j - 1, k - 1, O_QU)) / dx;
+        vy = 0.5 * (Q(i, j, k - 1, O_QV) - Q(i, j - 1, k - 1, O_QV) +
+                    Q(i - 1, j, k - 1, O_QV) - Q(i - 1, j - 1, k - 1, O_QV)) / dy;
+    }
+    /* corners */
+    if (j == domlo[1] && bclo[1] == NS && k == domlo[2] && bclo[2] == NS)
+        ux = (Q(i, j, k, O_QU) - Q(i - 1, j, k, O_QU)) / dx;
+    else if ((j == domhi[1] + 1 && bchi[1] == NS) && (k == domlo[2] && bclo[2] == NS))
+        ux = (Q(i, j - 1, k, O_QU) - Q(i - 1, j - 1, k, O_QU)) / dx;
+    else if ((j == domlo[1] && bclo[1] == NS) && (k == domhi[2] + 1 && bchi[2] == NS))
+        ux = (Q(i, j, k - 1, O_QU) - Q(i - 1, j, k - 1, O_QU)) / dx;
+    else if ((j == domhi[1] + 1 && bchi[1] == NS) && (k == domhi[2] + 1 && bchi[2] == NS))
+        ux = (Q(i, j - 1, k - 1, O_QU) - Q(i - 1, j - 1, k - 1, O_QU)) / dx;
+
+    if (i == domlo[0] && bclo[0] == NS && k == domlo[2] && bclo[2] == NS)
+        vy = (Q(i, j, k, O_QV) - Q(i, j - 1, k, O_QV)) / dy;
+    else if ((i == domhi[0] + 1 && bchi[0] == NS) && (k == domlo[2] && bclo[2] == NS))
+        vy = (Q(i - 1, j, k, O_QV) - Q(i - 1, j - 1, k, O_QV)) / dy;
+    else if ((i == domlo[0] && bclo[0] == NS) && (k == domhi[2] + 1 && bchi[2] == NS))
+        vy = (Q(i, j, k - 1, O_QV) - Q(i, j - 1, k - 1, O_QV)) / dy;
+    else if ((i == domhi[0] + 1 && bchi[0] == NS) && (k == domhi[2] + 1 && bchi[2] == NS))
+        vy = (Q(i - 1, j, k - 1, O_QV) - Q(i - 1, j - 1, k - 1, O_QV)) / dy;
+
+    if (i == domlo[0] && bclo[0] == NS && j == domlo[1] && bclo[1] == NS)
+        wz = (Q(i, j, k, O_QW) - Q(i, j, k - 1, O_QW)) / dz;
+    else if ((i == domhi[0] + 1 && bchi[0] == NS) && (j == domlo[1] && bclo[1] == NS))
+        wz = (Q(i - 1, j, k, O_QW) - Q(i - 1, j, k - 1, O_QW)) / dz;
+    else if ((i == domlo[0] && bclo[0] == NS) && (j == domhi[1] + 1 && bchi[1] == NS))
+        wz = (Q(i, j - 1, k, O_QW) - Q(i, j - 1, k - 1, O_QW)) / dz;
+    else if ((i == domhi[0] + 1 && bchi[0] == NS) && (j == domhi[1] + 1 && bchi[1] == NS))
+        wz = (Q(i - 1, j - 1, k, O_QW) - Q(i - 1, j - 1, k - 1, O_QW)) / dz;
+#undef Q
+    FA(divu, i, j, k, 0) = ux + vy + wz;
+}
+
+static void artif_visc(int i, int j, int k, ora_fab *flx, const ora_fab *divu,
+                       const ora_fab *u, double dx, double difmag, int dir,
+                       int domlo, int domhi, int bclo, int bchi) {
+    const int iv[3] = {i, j, k};
+    int ivm[3] = {i, j, k};
+    ivm[dir] -= 1;
+    const int l1 = (dir == 0) ? 1 : 0;
+    const int l2 = (dir == 2) ? 1 : 2;
+    int ivpj[3] = {i, j, k}, ivpk[3] = {i, j, k}, ivpp[3] = {i, j, k};
+    ivpj[l1] += 1;
+    ivpk[l2] += 1;
+    ivpp[l1] += 1;
+    ivpp[l2] += 1;
+    const double div =
+        difmag * smin(0.0, 0.25 * (FV(divu, iv, 0) + FV(divu, ivpj, 0) +
+                                   FV(divu, ivpk, 0) + FV(divu, ivpp, 0)));
+    int at_bndry = ((dir == 0) && (i == domlo) && (bclo == O_NoSlipWall)) ||
+                   ((dir == 0) && (i == domhi + 1) && (bchi == O_NoSlipWall)) ||
+                   ((dir == 1) && (j == domlo) && (bclo == O_NoSlipWall)) ||
+                   ((dir == 1) && (j == domhi + 1) && (bchi == O_NoSlipWall));
+    at_bndry = at_bndry ||
+               ((dir == 2) && (k == domlo) && (bclo == O_NoSlipWall)) ||
+               ((dir == 2) && (k == domhi + 1) && (bchi == O_NoSlipWall));
+    for (int n = 0; n < O_NVAR; ++n) {
+        if (n != O_UTEMP) {
+            int is_tang_vel = (dir == 0 && (n == O_UMY || n == O_UMZ)) ||
+                              (dir == 1 && (n == O_UMX || n == O_UMZ)) ||
+                              (dir == 2 && (n == O_UMX || n == O_UMY));
+            if (!(at_bndry && is_tang_vel))
+                FV(flx, iv, n) += dx * div * (FV(u, iv, n) - FV(u, ivm, n));
+        }
+    }
+    FV(flx, iv, O_UTEMP) = 0.0;
+}
+
+static void norm_spec_flx(int i, int j, int k, ora_fab *flx) {
+    const int num = NSPEC + O_UFS;
+    double summ = 0.0, fac;
+    for (int n = O_UFS; n < num; ++n) summ += FA(flx, i, j, k, n);
+    fac = (summ != 0.0) ? FA(flx, i, j, k, O_URHO) / summ : 1.0;
+    for (int n = O_UFS; n < num; ++n) FA(flx, i, j, k, n) *= fac;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Godunov_umeth (3-D), Source/Hydro/Godunov/Godunov_3D.cpp:12-461           */
+/* ------------------------------------------------------------------------ */
+static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
+                          const ora_fab *qaux, const ora_fab *srcQ,
+                          ora_fab *flx1, ora_fab *flx2, ora_fab *flx3,
+                          ora_fab *q1, ora_fab *q2, ora_fab *q3,
+                          const double area[3], ora_fab *pdivu, double vol,
+                          double dt) {
+    const double dx = P->dx[0], dy = P->dx[1], dz = P->dx[2];
+    const double hdtdx = 0.5 * dt / dx;
+    const double hdtdy = 0.5 * dt / dy;
+    const double hdtdz = 0.5 * dt / dz;
+    const double cdtdx = 1.0 / 3.0 * dt / dx;
+    const double cdtdy = 1.0 / 3.0 * dt / dy;
+    const double cdtdz = 1.0 / 3.0 * dt / dz;
+    const double hdt = 0.5 * dt;
+    const int *bclo = P->bclo, *bchi = P->bchi, *dlo = P->domlo, *dhi = P->domhi;
+    const double small = P->small, sd = P->small_dens, sp = P->small_pres,
+                 su = P->smallu;
+    const int trd = P->transverse_reset_density;
+    (void)hdtdx; (void)hdtdy; (void)hdtdz;
+
+    const obox bxg1 = bgrow(bx, 1);
+    const obox bxg2 = bgrow(bx, 2);
+
+    ora_fab qdm[3], qdp[3];
+    obox flxbx[3];
+    for (int d = 0; d < 3; ++d) {
+        fab_alloc_box(&qdm[d], bgrowhi(bxg2, d, 1), O_QVAR);
+        fab_alloc_box(&qdp[d], bxg2, O_QVAR);
+        flxbx[d] = bnodes(bgrowd(bxg2, d, -1), d);
+    }
+
+    if (P->ppm_type == 0) {
+        const double del[3] = {dx, dy, dz};
+        for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
+        for (int j = bxg2.lo[1]; j <= bxg2.hi[1]; ++j)
+        for (int i = bxg2.lo[0]; i <= bxg2.hi[0]; ++i) {
+            double slope[O_QVAR];
+            double flat = 1.0;
+            if (P->use_flattening == 1)
+                for (int dir_flat = 0; dir_flat < 3; dir_flat++)
+                    flat = smin(flat, ora_flatten(q, i, j, k, dir_flat));
+            for (int idir = 0; idir < 3; ++idir) {
+                for (int n = 0; n < O_QVAR; ++n) {
+                    if (n == O_QPRES && P->use_pslope)
+                        slope[n] = plm_pslope(q, i, j, k, n, idir, del[idir],
+                                              srcQ, flat, P->plm_iorder);
+                    else
+                        slope[n] = ora_plm_slope(q, i, j, k, n, idir, flat,
+                                                 P->plm_iorder);
+                }
+                plm_d(i, j, k, idir, &qdm[idir], &qdp[idir], slope, q,
+                      FA(qaux, i, j, k, O_QC), del[idir], dt, sd, sp);
+            }
+        }
+    } else {
+        for (int idir = 0; idir < 3; ++idir)
+            trace_ppm(bxg2, idir, q, qaux, &qdm[idir], &qdp[idir], bxg2, dt,
+                      P->dx, P->use_flattening, sd, sp);
+    }
+
+    /* first flux estimates */
+    ora_fab f[3], qgd[3];
+    for (int d = 0; d < 3; ++d) {
+        fab_alloc_box(&f[d], flxbx[d], O_NVAR);
+        fab_alloc_box(&qgd[d], flxbx[d], O_NGDNV);
+        obox b = flxbx[d];
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+            cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qdm[d], &qdp[d],
+                   &f[d], &qgd[d], qaux, d, small, sd, sp, su);
+    }
+
+    /* transverse corrections and Riemann solves for the pairs d|t.
+     * Pair order follows the reference: X|Y, X|Z, Y|X, Y|Z, Z|X, Z|Y. */
+    const double cdt[3] = {cdtdx, cdtdy, cdtdz};
+    ora_fab fl2[3][3], qg2[3][3]; /* fl2[d][t]: flux of direction d corrected by t */
+    for (int d = 0; d < 3; ++d) {
+        const int t0 = (d == 0) ? 1 : 0;
+        const int t1 = (d == 2) ? 1 : 2;
+        const obox tbx = bgrowd(bxg1, d, 1);
+        const obox tbxm = bgrowhi(tbx, d, 1);
+        ora_fab qmA, qpA, qmB, qpB;
+        fab_alloc_box(&qmA, tbxm, O_QVAR);
+        fab_alloc_box(&qpA, tbx, O_QVAR);
+        fab_alloc_box(&qmB, tbxm, O_QVAR);
+        fab_alloc_box(&qpB, tbx, O_QVAR);
+        for (int k = tbx.lo[2]; k <= tbx.hi[2]; ++k)
+        for (int j = tbx.lo[1]; j <= tbx.hi[1]; ++j)
+        for (int i = tbx.lo[0]; i <= tbx.hi[0]; ++i) {
+            transdo(i, j, k, d, t0, &qmA, &qpA, &qdm[d], &qdp[d], &f[t0],
+                    qaux, &qgd[t0], cdt[t0], trd, sp);
+            transdo(i, j, k, d, t1, &qmB, &qpB, &qdm[d], &qdp[d], &f[t1],
+                    qaux, &qgd[t1], cdt[t1], trd, sp);
+        }
+        const obox tfx = bnodes(bxg1, d);
+        fab_alloc_box(&fl2[d][t0], tfx, O_NVAR);
+        fab_alloc_box(&fl2[d][t1], tfx, O_NVAR);
+        fab_alloc_box(&qg2[d][t0], tfx, O_NGDNV);
+        fab_alloc_box(&qg2[d][t1], tfx, O_NGDNV);
+        for (int k = tfx.lo[2]; k <= tfx.hi[2]; ++k)
+        for (int j = tfx.lo[1]; j <= tfx.hi[1]; ++j)
+        for (int i = tfx.lo[0]; i <= tfx.hi[0]; ++i) {
+            cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qmA, &qpA,
+                   &fl2[d][t0], &qg2[d][t0], qaux, d, small, sd, sp, su);
+            cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qmB, &qpB,
+                   &fl2[d][t1], &qg2[d][t1], qaux, d, small, sd, sp, su);
+        }
+        ora_fab_free(&qmA); ora_fab_free(&qpA);
+        ora_fab_free(&qmB); ora_fab_free(&qpB);
+    }
+    for (int d = 0; d < 3; ++d) { ora_fab_free(&f[d]); ora_fab_free(&qgd[d]); }
+
+    /* final states and fluxes */
+    ora_fab qmf, qpf;
+    fab_alloc_box(&qmf, bxg2, O_QVAR);
+    fab_alloc_box(&qpf, bxg1, O_QVAR);
+    ora_fab *flxd[3] = {flx1, flx2, flx3};
+    ora_fab *qd[3] = {q1, q2, q3};
+    for (int d = 0; d < 3; ++d) {
+        /* dir d corrected by the fluxes (t0|t1) and (t1|t0) */
+        const int t0 = (d == 0) ? 1 : 0;
+        const int t1 = (d == 2) ? 1 : 2;
+        const double hdtd[3] = {0.5 * dt / dx, 0.5 * dt / dy, 0.5 * dt / dz};
+        const obox tb = bgrowd(bx, d, 1);
+        for (int k = tb.lo[2]; k <= tb.hi[2]; ++k)
+        for (int j = tb.lo[1]; j <= tb.hi[1]; ++j)
+        for (int i = tb.lo[0]; i <= tb.hi[0]; ++i)
+            transdd(i, j, k, d, &qmf, &qpf, &qdm[d], &qdp[d], &fl2[t0][t1],
+                    &fl2[t1][t0], &qg2[t0][t1], &qg2[t1][t0], qaux, srcQ, hdt,
+                    hdtd[t0], hdtd[t1], trd, sp);
+        const obox fb = bnodes(bx, d);
+        for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+        for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+        for (int i = fb.lo[0]; i <= fb.hi[0]; ++i)
+            cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qmf, &qpf,
+                   flxd[d], qd[d], qaux, d, small, sd, sp, su);
+    }
+    ora_fab_free(&qmf); ora_fab_free(&qpf);
+    for (int d = 0; d < 3; ++d)
+        for (int t = 0; t < 3; ++t)
+            if (t != d) { ora_fab_free(&fl2[d][t]); ora_fab_free(&qg2[d][t]); }
+    for (int d = 0; d < 3; ++d) { ora_fab_free(&qdm[d]); ora_fab_free(&qdp[d]); }
+
+    /* p div(u), Hydro_utils_K.H:175-202 */
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        FA(pdivu, i, j, k, 0) =
+            0.5 *
+            ((FA(q1, i + 1, j, k, O_GDPRES) + FA(q1, i, j, k, O_GDPRES)) *
+                 (FA(q1, i + 1, j, k, O_GDU) * area[0] -
+                  FA(q1, i, j, k, O_GDU) * area[0]) +
+             (FA(q2, i, j + 1, k, O_GDPRES) + FA(q2, i, j, k, O_GDPRES)) *
+                 (FA(q2, i, j + 1, k, O_GDV) * area[1] -
+                  FA(q2, i, j, k, O_GDV) * area[1]) +
+             (FA(q3, i, j, k + 1, O_GDPRES) + FA(q3, i, j, k, O_GDPRES)) *
+                 (FA(q3, i, j, k + 1, O_GDW) * area[2] -
+                  FA(q3, i, j, k, O_GDW) * area[2])) /
+            vol;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_umdrv, Source/Hydro/Hydro_umdrv.cpp:11-93                           */
+/* + adjust_fluxes Hydro_adjust_fluxes.cpp:13-44 + hydro_consup              */
+/* Hydro_consup.cpp:13-25                                                     */
+/* ------------------------------------------------------------------------ */
+void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
+                     ora_fab *dsdt, const ora_fab *q, const ora_fab *qa,
+                     const ora_fab *srcq, ora_fab flx[3], double dt,
+                     const ora_params *P) {
+    const obox bx = mkbox(lo, hi);
+    const obox bxg2 = bgrow(bx, 2);
+    ora_fab qec[3];
+    for (int d = 0; d < 3; ++d) fab_alloc_box(&qec[d], bnodes(bxg2, d), O_NGDNV);
+    ora_fab divu, pdivu;
+    fab_alloc_box(&divu, bxg2, 1);
+    fab_alloc_box(&pdivu, bx, 1);
+
+    /* geom.GetVolume / GetFaceArea for a Cartesian grid (CAMR.cpp:337-348) */
+    const double *dx = P->dx;
+    const double vol = dx[0] * dx[1] * dx[2];
+    const double area[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
+
+    godunov_umeth(bx, P, q, qa, srcq, &flx[0], &flx[1], &flx[2], &qec[0],
+                  &qec[1], &qec[2], area, &pdivu, vol, dt);
+
+    for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
+    for (int j = bxg2.lo[1]; j <= bxg2.hi[1]; ++j)
+    for (int i = bxg2.lo[0]; i <= bxg2.hi[0]; ++i)
+        divu_node(i, j, k, q, dx[0], dx[1], dx[2], &divu, P->domlo, P->domhi,
+                  P->bclo, P->bchi);
+
+    for (int d = 0; d < 3; ++d) {
+        const obox fb = bnodes(bx, d);
+        for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+        for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+        for (int i = fb.lo[0]; i <= fb.hi[0]; ++i) {
+            artif_visc(i, j, k, &flx[d], &divu, uin, dx[d], P->difmag, d,
+                       P->domlo[d], P->domhi[d], P->bclo[d], P->bchi[d]);
+            norm_spec_flx(i, j, k, &flx[d]);
+            const double a = area[d];
+            for (int n = 0; n < O_NVAR; ++n) FA(&flx[d], i, j, k, n) *= a;
+        }
+    }
+
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        double volinv = 1.0 / vol;
+        for (int n = 0; n < O_NVAR; ++n) {
+            FA(dsdt, i, j, k, n) +=
+                (FA(&flx[0], i, j, k, n) - FA(&flx[0], i + 1, j, k, n) +
+                 FA(&flx[1], i, j, k, n) - FA(&flx[1], i, j + 1, k, n) +
+                 FA(&flx[2], i, j, k, n) - FA(&flx[2], i, j, k + 1, n)) *
+                volinv;
+        }
+        FA(dsdt, i, j, k, O_UEINT) -= FA(&pdivu, i, j, k, 0);
+    }
+
+    for (int d = 0; d < 3; ++d) ora_fab_free(&qec[d]);
+    ora_fab_free(&divu);
+    ora_fab_free(&pdivu);
+}
+
+/* ------------------------------------------------------------------------ */
+/* CAMR_estdt_hydro, Source/Utils/Timestep.H:18-61                           */
+/* ------------------------------------------------------------------------ */
+double ora_estdt_box(const int lo[3], const int hi[3], const ora_fab *u,
+                     const double dx[3]) {
+    double dt = DBL_MAX;
+    for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+    for (int i = lo[0]; i <= hi[0]; ++i) {
+        const double rho = FA(u, i, j, k, O_URHO);
+        const double rhoInv = 1.0 / rho;
+        double eint = FA(u, i, j, k, O_UEINT) / rho;
+        const double pres = (EOS_GAMMA - 1.0) * rho * eint;
+        const double gam = EOS_GAMMA;
+        const double c = sqrt(gam * pres / rho);
+        const double ux = FA(u, i, j, k, O_UMX) * rhoInv;
+        const double dt1 = dx[0] / (c + fabs(ux));
+        dt = smin(dt, dt1);
+        const double uy = FA(u, i, j, k, O_UMY) * rhoInv;
+        const double dt2 = dx[1] / (c + fabs(uy));
+        dt = smin(dt, dt2);
+        const double uz = FA(u, i, j, k, O_UMZ) * rhoInv;
+        const double dt3 = dx[2] / (c + fabs(uz));
+        dt = smin(dt, dt3);
+    }
+    return dt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Level driver: CAMR_advance (Source/CAMR_advance.cpp:38-164),              */
+/* construct_hydro_source (Hydro/CAMR_construct_hydro_source.cpp:9-267),     */
+/* clean_state family (Source/CAMR.cpp:866-1262), estTimeStep /             */
+/* computeNewDt / computeInitialDt (CAMR.cpp:474-676).                       */
+/* ------------------------------------------------------------------------ */
+struct ora_level {
+    ora_params P;
+    int nbox;
+    obox *box;
+    ora_fab *Snew, *Sold, *Sborder, *hsrc, *oldsrc, *newsrc, *srchyd;
+    int *owner;          /* domain cell -> box index (or -1) */
+    int64_t dn[3];
+    double time;
+    int err;
+};
+
+static int64_t dom_index(const ora_level *L, const int c[3]) {
+    return (int64_t)(c[0] - L->P.domlo[0]) +
+           (int64_t)(c[1] - L->P.domlo[1]) * L->dn[0] +
+           (int64_t)(c[2] - L->P.domlo[2]) * L->dn[0] * L->dn[1];
+}
+
+ora_level *ora_level_create(const ora_params *p, int nbox, const int *lohi) {
+    ora_level *L = (ora_level *)calloc(1, sizeof(ora_level));
+    L->P = *p;
+    L->nbox = nbox;
+    L->box = (obox *)calloc(nbox, sizeof(obox));
+    L->Snew = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->Sold = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->Sborder = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->hsrc = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->oldsrc = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->newsrc = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    L->srchyd = (ora_fab *)calloc(nbox, sizeof(ora_fab));
+    for (int d = 0; d < 3; ++d)
+        L->dn[d] = p->domhi[d] - p->domlo[d] + 1;
+    int64_t ncell = L->dn[0] * L->dn[1] * L->dn[2];
+    L->owner = (int *)malloc(sizeof(int) * (size_t)ncell);
+    for (int64_t c = 0; c < ncell; ++c) L->owner[c] = -1;
+    for (int b = 0; b < nbox; ++b) {
+        obox B;
+        for (int d = 0; d < 3; ++d) { B.lo[d] = lohi[6 * b + d]; B.hi[d] = lohi[6 * b + 3 + d]; }
+        L->box[b] = B;
+        fab_alloc_box(&L->Snew[b], B, O_NVAR);
+        fab_alloc_box(&L->Sold[b], B, O_NVAR);
+        fab_alloc_box(&L->Sborder[b], bgrow(B, NGROW), O_NVAR);
+        fab_alloc_box(&L->hsrc[b], bgrow(B, NGROW), O_NVAR);
+        fab_alloc_box(&L->oldsrc[b], bgrow(B, NGROW), O_NVAR);
+        fab_alloc_box(&L->newsrc[b], B, O_NVAR);
+        fab_alloc_box(&L->srchyd[b], bgrow(B, NGROW), O_NVAR);
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            int c[3] = {i, j, k};
+            L->owner[dom_index(L, c)] = b;
+        }
+    }
+    return L;
+}
+
+void ora_level_destroy(ora_level *L) {
+    if (!L) return;
+    for (int b = 0; b < L->nbox; ++b) {
+        ora_fab_free(&L->Snew[b]); ora_fab_free(&L->Sold[b]);
+        ora_fab_free(&L->Sborder[b]); ora_fab_free(&L->hsrc[b]);
+        ora_fab_free(&L->oldsrc[b]); ora_fab_free(&L->newsrc[b]);
+        ora_fab_free(&L->srchyd[b]);
+    }
+    free(L->Snew); free(L->Sold); free(L->Sborder); free(L->hsrc);
+    free(L->oldsrc); free(L->newsrc); free(L->srchyd); free(L->box);
+    free(L->owner); free(L);
+}
+
+int ora_level_nbox(const ora_level *L) { return L->nbox; }
+double ora_level_time(const ora_level *L) { return L->time; }
+
+void ora_level_set_box(ora_level *L, int b, const double *data) {
+    memcpy(L->Snew[b].p, data, sizeof(double) * (size_t)(L->Snew[b].ns * O_NVAR));
+}
+void ora_level_get_box(const ora_level *L, int b, double *data) {
+    memcpy(data, L->Snew[b].p, sizeof(double) * (size_t)(L->Snew[b].ns * O_NVAR));
+}
+
+/* CAMR_reset_internal_e, Source/Utils/CAMR_reset_internal_e.H:13-129 */
+static void reset_internal_e(ora_fab *S, int i, int j, int k,
+                             const ora_params *P) {
+    const double rho = FA(S, i, j, k, O_URHO);
+    const double rhoInv = 1.0 / rho;
+    const double Up = FA(S, i, j, k, O_UMX) * rhoInv;
+    const double Vp = FA(S, i, j, k, O_UMY) * rhoInv;
+    const double Wp = FA(S, i, j, k, O_UMZ) * rhoInv;
+    const double ke = 0.5 * (Up * Up + Vp * Vp + Wp * Wp);
+    const double eden = FA(S, i, j, k, O_UEDEN) * rhoInv;
+    double massfrac[NSPEC];
+    for (int sp = 0; sp < NSPEC; sp++) massfrac[sp] = FA(S, i, j, k, sp + O_UFS) * rhoInv;
+    double e_out = 0.;
+    double small_e = 0.;
+    const double small_T = P->small_temp;
+    if (P->allow_small_energy == 0) {
+        small_e = eos_rty2e(small_T, massfrac);
+        if (eden < small_e) {
+            if (FA(S, i, j, k, O_UEINT) * rhoInv < small_e) {
+                const double T = smax(FA(S, i, j, k, O_UTEMP), small_T);
+                e_out = eos_rty2e(T, massfrac);
+                FA(S, i, j, k, O_UEINT) = rho * e_out;
+            }
+            FA(S, i, j, k, O_UEDEN) = FA(S, i, j, k, O_UEINT) + rho * ke;
+        } else {
+            const double rho_eint = FA(S, i, j, k, O_UEDEN) - FA(S, i, j, k, O_URHO) * ke;
+            if (rho_eint > 0. && rho_eint / FA(S, i, j, k, O_UEDEN) > P->dual_energy_eta2)
+                FA(S, i, j, k, O_UEINT) = rho_eint;
+            if (FA(S, i, j, k, O_UEINT) * rhoInv < small_e) {
+                const double T = smax(FA(S, i, j, k, O_UTEMP), small_T);
+                e_out = eos_rty2e(T, massfrac);
+                if (P->dual_energy_update_E_from_e == 1)
+                    FA(S, i, j, k, O_UEDEN) += (rho * e_out - FA(S, i, j, k, O_UEINT));
+                FA(S, i, j, k, O_UEINT) = FA(S, i, j, k, O_URHO) * e_out;
+            }
+        }
+    } else if (P->allow_negative_energy == 0) {
+        if (FA(S, i, j, k, O_UEDEN) < 0.) {
+            if (FA(S, i, j, k, O_UEINT) < 0.) {
+                e_out = eos_rty2e(small_T, massfrac);
+                FA(S, i, j, k, O_UEINT) = rho * e_out;
+            }
+            FA(S, i, j, k, O_UEDEN) = FA(S, i, j, k, O_UEINT) + rho * ke;
+        } else {
+            const double rho_eint = FA(S, i, j, k, O_UEDEN) - rho * ke;
+            if (rho_eint > 0. && rho_eint / FA(S, i, j, k, O_UEDEN) > P->dual_energy_eta2) {
+                FA(S, i, j, k, O_UEINT) = rho_eint;
+            } else if (FA(S, i, j, k, O_UEINT) > 0 && P->dual_energy_update_E_from_e == 1) {
+                FA(S, i, j, k, O_UEDEN) = FA(S, i, j, k, O_UEINT) + rho * ke;
+            } else if (FA(S, i, j, k, O_UEINT) <= 0.) {
+                e_out = eos_rty2e(small_T, massfrac);
+                FA(S, i, j, k, O_UEINT) = rho * e_out;
+                if (P->dual_energy_update_E_from_e == 1)
+                    FA(S, i, j, k, O_UEDEN) += (rho * e_out - FA(S, i, j, k, O_UEINT));
+            }
+        }
+    } else {
+        FA(S, i, j, k, O_UEINT) = FA(S, i, j, k, O_UEDEN) - rho * ke;
+    }
+}
+
+/* computeTemp, CAMR.cpp:1141-1185 (reset_internal_energy first) */
+static void compute_temp(ora_fab *S, obox gb, const ora_params *P) {
+    for (int k = gb.lo[2]; k <= gb.hi[2]; ++k)
+    for (int j = gb.lo[1]; j <= gb.hi[1]; ++j)
+    for (int i = gb.lo[0]; i <= gb.hi[0]; ++i)
+        reset_internal_e(S, i, j, k, P);
+    for (int k = gb.lo[2]; k <= gb.hi[2]; ++k)
+    for (int j = gb.lo[1]; j <= gb.hi[1]; ++j)
+    for (int i = gb.lo[0]; i <= gb.hi[0]; ++i) {
+        double rhoInv = 1.0 / FA(S, i, j, k, O_URHO);
+        double e = FA(S, i, j, k, O_UEINT) * rhoInv;
+        FA(S, i, j, k, O_UTEMP) = eos_rey2t(e);
+    }
+}
+
+/* enforce_min_density, CAMR.cpp:941-1045 (untiled box, sequential order) */
+static void enforce_min_density(ora_fab *S, obox bx, const ora_params *P,
+                                int *err) {
+    const double sd = P->small_dens;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        if (FA(S, i, j, k, O_URHO) == 0.) {
+            *err |= ERR_ZERO_DENS;
+        } else if (FA(S, i, j, k, O_URHO) < sd) {
+            double max_dens = FA(S, i, j, k, O_URHO);
+            int is = i, js = j, ks = k;
+            for (int kk = -1; kk <= 1; kk++)
+            for (int jj = -1; jj <= 1; jj++)
+            for (int ii = -1; ii <= 1; ii++) {
+                if ((i + ii >= bx.lo[0]) && (j + jj >= bx.lo[1]) && (k + kk >= bx.lo[2]) &&
+                    (i + ii <= bx.hi[0]) && (j + jj <= bx.hi[1]) && (k + kk <= bx.hi[2])) {
+                    if (FA(S, i + ii, j + jj, k + kk, O_URHO) > max_dens) {
+                        is = i + ii; js = j + jj; ks = k + kk;
+                        max_dens = FA(S, is, js, ks, O_URHO);
+                    }
+                }
+            }
+            if (max_dens < sd) {
+                double massfrac[NSPEC];
+                for (int s = 0; s < NSPEC; s++)
+                    massfrac[s] = FA(S, i, j, k, O_UFS + s) / FA(S, i, j, k, O_URHO);
+                double small_e = eos_rty2e(P->small_temp, massfrac);
+                FA(S, i, j, k, O_URHO) = sd;
+                FA(S, i, j, k, O_UTEMP) = P->small_temp;
+                FA(S, i, j, k, O_UMX) = 0.;
+                FA(S, i, j, k, O_UMY) = 0.;
+                FA(S, i, j, k, O_UMZ) = 0.;
+                FA(S, i, j, k, O_UEINT) = sd * small_e;
+                FA(S, i, j, k, O_UEDEN) = FA(S, i, j, k, O_UEINT);
+            } else {
+                for (int n = 0; n < O_NVAR; n++)
+                    FA(S, i, j, k, n) = FA(S, is, js, ks, n);
+            }
+        }
+    }
+}
+
+/* normalize_species, CAMR.cpp:866-908 (valid cells) */
+static void normalize_species(ora_fab *S, obox bx) {
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        double sum = 0.;
+        for (int n = 0; n < NSPEC; n++) {
+            FA(S, i, j, k, O_UFS + n) =
+                smax(smin(FA(S, i, j, k, O_URHO), FA(S, i, j, k, O_UFS + n)), 0.0);
+            sum += FA(S, i, j, k, O_UFS + n);
+        }
+        if (FA(S, i, j, k, O_URHO) > 0.) {
+            sum /= FA(S, i, j, k, O_URHO);
+            for (int n = 0; n < NSPEC; n++) FA(S, i, j, k, O_UFS + n) /= sum;
+        }
+    }
+}
+
+/* Per-direction ghost-cell source index and sign (AMReX FillPatchSingleLevel
+ * + FilccCell semantics with the BC tables of CAMR_setup.cpp:19-21).
+ * Returns 1 if the cell is outside the domain in a non-periodic direction. */
+static int map_cell(const ora_params *P, const int c[3], int comp, int m[3],
+                    double *sign) {
+    int phys = 0;
+    *sign = 1.0;
+    for (int d = 0; d < 3; ++d) {
+        int x = c[d];
+        const int lo = P->domlo[d], hi = P->domhi[d], n = hi - lo + 1;
+        if (x >= lo && x <= hi) { m[d] = x; continue; }
+        if (P->is_periodic[d]) {
+            int r = (x - lo) % n;
+            if (r < 0) r += n;
+            m[d] = lo + r;
+            continue;
+        }
+        phys = 1;
+        const int bc = (x < lo) ? P->bclo[d] : P->bchi[d];
+        const int is_norm = (comp == O_UMX + d);
+        const int is_tang = (comp >= O_UMX && comp <= O_UMZ && !is_norm);
+        if (bc == O_Outflow || bc == O_Inflow || bc == O_Interior) {
+            m[d] = (x < lo) ? lo : hi; /* foextrap (Inflow: bcnormal == copy) */
+        } else {
+            m[d] = (x < lo) ? (2 * lo - x - 1) : (2 * hi - x + 1);
+            int odd = is_norm || (is_tang && bc == O_NoSlipWall);
+            if (odd) *sign = -(*sign);
+        }
+    }
+    return phys;
+}
+
+/* FillPatch (single level) of all ghost+valid cells of dst from src. */
+static void fill_patch(ora_level *L, ora_fab *dst, int b, ora_fab *src) {
+    obox g = bgrow(L->box[b], NGROW);
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = g.lo[2]; k <= g.hi[2]; ++k)
+    for (int j = g.lo[1]; j <= g.hi[1]; ++j)
+    for (int i = g.lo[0]; i <= g.hi[0]; ++i) {
+        int c[3] = {i, j, k}, m[3];
+        double s;
+        map_cell(&L->P, c, n, m, &s);
+        int ob = L->owner[dom_index(L, m)];
+        double v = FA(&src[ob], m[0], m[1], m[2], n);
+        FA(dst, i, j, k, n) = (s < 0.0) ? -v : v;
+    }
+}
+
+/* FillBoundary: re-copy ghost cells that are periodic/interior images of
+ * valid cells (physical-boundary ghosts untouched). */
+static void fill_boundary(ora_level *L, ora_fab *S, int ng) {
+    for (int b = 0; b < L->nbox; ++b) {
+        obox g = bgrow(L->box[b], ng);
+        obox v = L->box[b];
+        for (int k = g.lo[2]; k <= g.hi[2]; ++k)
+        for (int j = g.lo[1]; j <= g.hi[1]; ++j)
+        for (int i = g.lo[0]; i <= g.hi[0]; ++i) {
+            if (i >= v.lo[0] && i <= v.hi[0] && j >= v.lo[1] && j <= v.hi[1] &&
+                k >= v.lo[2] && k <= v.hi[2]) continue;
+            int c[3] = {i, j, k}, m[3];
+            double s;
+            if (map_cell(&L->P, c, 0, m, &s)) continue;
+            int ob = L->owner[dom_index(L, m)];
+            if (ob < 0) continue;
+            for (int n = 0; n < O_NVAR; ++n)
+                FA(&S[b], i, j, k, n) = FA(&S[ob], m[0], m[1], m[2], n);
+        }
+    }
+}
+
+/* clean_state, CAMR.cpp:1253-1262 */
+static void clean_state(ora_level *L, ora_fab *S, int ng) {
+    for (int b = 0; b < L->nbox; ++b) enforce_min_density(&S[b], L->box[b], &L->P, &L->err);
+    for (int b = 0; b < L->nbox; ++b) normalize_species(&S[b], L->box[b]);
+    if (ng > 0) fill_boundary(L, S, ng);
+    for (int b = 0; b < L->nbox; ++b) compute_temp(&S[b], bgrow(L->box[b], ng), &L->P);
+}
+
+/* enforce_consistent_e, CAMR.cpp:911-938 */
+void ora_level_init_post(ora_level *L) {
+    for (int b = 0; b < L->nbox; ++b) {
+        ora_fab *S = &L->Snew[b];
+        obox B = L->box[b];
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            const double rhoInv = 1.0 / FA(S, i, j, k, O_URHO);
+            const double u = FA(S, i, j, k, O_UMX) * rhoInv;
+            const double v = FA(S, i, j, k, O_UMY) * rhoInv;
+            const double w = FA(S, i, j, k, O_UMZ) * rhoInv;
+            FA(S, i, j, k, O_UEDEN) =
+                FA(S, i, j, k, O_UEINT) + 0.5 * FA(S, i, j, k, O_URHO) * (u * u + v * v + w * w);
+        }
+        compute_temp(S, B, &L->P);
+    }
+}
+
+/* estTimeStep, CAMR.cpp:490-563 */
+static double est_time_step(ora_level *L) {
+    const ora_params *P = &L->P;
+    if (P->fixed_dt > 0.0) return P->fixed_dt;
+    double estdt = P->max_dt;
+    const double max_dt_over_cfl = P->max_dt / P->cfl;
+    double estdt_hydro = max_dt_over_cfl;
+    double dt = DBL_MAX;
+    for (int b = 0; b < L->nbox; ++b)
+        dt = smin(dt, ora_estdt_box(L->box[b].lo, L->box[b].hi, &L->Snew[b], P->dx));
+    if (!(dt > 0.0)) L->err |= ERR_DT;
+    estdt_hydro = smin(estdt_hydro, dt);
+    estdt_hydro *= P->cfl;
+    if (estdt_hydro < estdt) estdt = estdt_hydro;
+    return estdt;
+}
+
+/* computeInitialDt / initialTimeStep, CAMR.cpp:474-488, 638-676 */
+double ora_level_initial_dt(ora_level *L) {
+    double init_dt = L->P.init_shrink * est_time_step(L);
+    double dt_0 = DBL_MAX;
+    dt_0 = smin(dt_0, 1 * init_dt);
+    const double dt_eps = 0.001 * dt_0;
+    if (L->P.stop_time >= 0.0)
+        if ((L->time + dt_0) > (L->P.stop_time - dt_eps)) dt_0 = L->P.stop_time - L->time;
+    return dt_0 / 1;
+}
+
+/* computeNewDt, CAMR.cpp:566-635 (single level, no regrid) */
+double ora_level_new_dt(ora_level *L, double dt_old) {
+    double dt_min = est_time_step(L);
+    if (L->P.fixed_dt <= 0.0) dt_min = smin(dt_min, L->P.change_max * dt_old);
+    double dt_0 = DBL_MAX;
+    dt_0 = smin(dt_0, 1 * dt_min);
+    const double dt_eps = 0.001 * dt_0;
+    if (L->P.stop_time >= 0.0)
+        if ((L->time + dt_0) > (L->P.stop_time - dt_eps)) dt_0 = L->P.stop_time - L->time;
+    return dt_0 / 1;
+}
+
+typedef struct { int b; obox t; } otile;
+
+/* construct_hydro_source, Hydro/CAMR_construct_hydro_source.cpp:9-267 */
+static void construct_hydro_source(ora_level *L, double dt) {
+    const ora_params *P = &L->P;
+    for (int b = 0; b < L->nbox; ++b) {
+        ora_fab *h = &L->hsrc[b];
+        memset(h->p, 0, sizeof(double) * (size_t)(h->ns * h->nc));
+    }
+    /* sources_for_hydro (zeroed at CAMR_advance.cpp:120) += old sources on
+     * valid cells, then FillBoundary */
+    for (int b = 0; b < L->nbox; ++b) {
+        obox B = L->box[b];
+        for (int n = 0; n < O_NVAR; ++n)
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+            FA(&L->srchyd[b], i, j, k, n) += 1.0 * FA(&L->oldsrc[b], i, j, k, n);
+    }
+    fill_boundary(L, L->srchyd, NGROW);
+
+    /* tiles (hydro_tile_size) */
+    int ntile = 0, cap = 64;
+    otile *tiles = (otile *)malloc(sizeof(otile) * cap);
+    for (int b = 0; b < L->nbox; ++b) {
+        obox B = L->box[b];
+        for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += P->tile[2])
+        for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += P->tile[1])
+        for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += P->tile[0]) {
+            otile t; t.b = b;
+            t.t.lo[0] = i0; t.t.lo[1] = j0; t.t.lo[2] = k0;
+            t.t.hi[0] = (i0 + P->tile[0] - 1 < B.hi[0]) ? i0 + P->tile[0] - 1 : B.hi[0];
+            t.t.hi[1] = (j0 + P->tile[1] - 1 < B.hi[1]) ? j0 + P->tile[1] - 1 : B.hi[1];
+            t.t.hi[2] = (k0 + P->tile[2] - 1 < B.hi[2]) ? k0 + P->tile[2] - 1 : B.hi[2];
+            if (ntile == cap) { cap *= 2; tiles = (otile *)realloc(tiles, sizeof(otile) * cap); }
+            tiles[ntile++] = t;
+        }
+    }
+    int err = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(ora_get_threads()) reduction(| : err)
+#endif
+    for (int it = 0; it < ntile; ++it) {
+        const int b = tiles[it].b;
+        const obox bx = tiles[it].t;
+        const obox qbx = bgrow(bx, NGROW);
+        ora_fab flux[3];
+        for (int d = 0; d < 3; ++d) fab_alloc_box(&flux[d], bnodes(bx, d), O_NVAR);
+        ora_fab q, qaux, srcq;
+        fab_alloc_box(&q, qbx, O_QVAR);
+        fab_alloc_box(&qaux, qbx, O_NQAUX);
+        fab_alloc_box(&srcq, qbx, O_QVAR);
+        ora_ctoprim_box(qbx.lo, qbx.hi, &L->Sborder[b], &q, &qaux, P, &err);
+        ora_srctoprim_box(qbx.lo, qbx.hi, &q, &qaux, &L->srchyd[b], &srcq);
+        ora_hydro_umdrv(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
+                        &srcq, flux, dt, P);
+        ora_fab_free(&q); ora_fab_free(&qaux); ora_fab_free(&srcq);
+        for (int d = 0; d < 3; ++d) ora_fab_free(&flux[d]);
+    }
+    L->err |= err;
+    free(tiles);
+}
+
+/* S_new += a * src on valid cells (MultiFab::Saxpy with separate mul/add) */
+static void saxpy_valid(ora_level *L, ora_fab *dst, double a, ora_fab *src) {
+    for (int b = 0; b < L->nbox; ++b) {
+        obox B = L->box[b];
+        for (int n = 0; n < O_NVAR; ++n)
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+            FA(&dst[b], i, j, k, n) += a * FA(&src[b], i, j, k, n);
+    }
+}
+
+static void set_val(ora_fab *f, int nbox, double v) {
+    for (int b = 0; b < nbox; ++b) {
+        int64_t N = f[b].ns * f[b].nc;
+        for (int64_t c = 0; c < N; ++c) f[b].p[c] = v;
+    }
+}
+
+/* CAMR_advance (Source/CAMR_advance.cpp:38-164) followed by post_timestep's
+ * clean_state (CAMR.cpp:698-699). */
+int ora_level_advance(ora_level *L, double dt) {
+    const ora_params *P = &L->P;
+    L->err = 0;
+    /* swapTimeLevels: old <- new */
+    for (int b = 0; b < L->nbox; ++b) {
+        ora_fab t = L->Sold[b]; L->Sold[b] = L->Snew[b]; L->Snew[b] = t;
+    }
+    clean_state(L, L->Sold, 0);
+    /* expand_state: FillPatch + clean_state(Sborder) */
+    for (int b = 0; b < L->nbox; ++b) fill_patch(L, &L->Sborder[b], b, L->Sold);
+    clean_state(L, L->Sborder, NGROW);
+    const int nsrc = P->add_grav_src ? 1 : 0; /* src_list = {grav_src} */
+    if (nsrc) { set_val(L->oldsrc, L->nbox, 0.0); set_val(L->newsrc, L->nbox, 0.0); }
+    /* MultiFab::Copy(S_new, Sborder) on valid cells */
+    for (int b = 0; b < L->nbox; ++b) {
+        obox B = L->box[b];
+        for (int n = 0; n < O_NVAR; ++n)
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+            FA(&L->Snew[b], i, j, k, n) = FA(&L->Sborder[b], i, j, k, n);
+    }
+    if (nsrc) {
+        /* construct_old_grav_source, GravitySrc.cpp:4-17,36-78 */
+        set_val(L->oldsrc, L->nbox, 0.0);
+        for (int b = 0; b < L->nbox; ++b) {
+            obox B = L->box[b];
+            for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+            for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                FA(&L->oldsrc[b], i, j, k, O_UMZ) = P->const_grav * FA(&L->Sold[b], i, j, k, O_URHO);
+                FA(&L->oldsrc[b], i, j, k, O_UEDEN) = P->const_grav * FA(&L->Sold[b], i, j, k, O_UMZ);
+            }
+        }
+        fill_boundary(L, L->oldsrc, NGROW);
+        saxpy_valid(L, L->Snew, dt, L->oldsrc);
+    }
+    set_val(L->srchyd, L->nbox, 0.0);
+    construct_hydro_source(L, dt);
+    saxpy_valid(L, L->Snew, dt, L->hsrc);
+    clean_state(L, L->Snew, 0);
+    if (nsrc) {
+        /* construct_new_grav_source, GravitySrc.cpp:20-33,81-125 */
+        set_val(L->newsrc, L->nbox, 0.0);
+        for (int b = 0; b < L->nbox; ++b) {
+            obox B = L->box[b];
+            for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+            for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                FA(&L->newsrc[b], i, j, k, O_UMZ) =
+                    0.5 * P->const_grav *
+                    (FA(&L->Snew[b], i, j, k, O_URHO) - FA(&L->Sold[b], i, j, k, O_URHO));
+                FA(&L->newsrc[b], i, j, k, O_UEDEN) =
+                    0.5 * P->const_grav *
+                    (FA(&L->Snew[b], i, j, k, O_UMZ) - FA(&L->Sold[b], i, j, k, O_UMZ));
+            }
+        }
+        saxpy_valid(L, L->Snew, dt, L->newsrc);
+        clean_state(L, L->Snew, 0);
+    }
+    /* post_timestep: clean_state(S_new) */
+    clean_state(L, L->Snew, 0);
+    L->time += dt;
+    return L->err;
+}
+
+int ora_level_run(ora_level *L, int nsteps, double *dts) {
+    double dt = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+        dt = (s == 0) ? ora_level_initial_dt(L) : ora_level_new_dt(L, dt);
+        if (dts) dts[s] = dt;
+        int e = ora_level_advance(L, dt);
+        if (e) return e;
+    }
+    return 0;
+}
+
+void ora_level_sums(const ora_level *L, double sums[O_NVAR]) {
+    for (int n = 0; n < O_NVAR; ++n) sums[n] = 0.0;
+    for (int b = 0; b < L->nbox; ++b) {
+        const ora_fab *S = &L->Snew[b];
+        obox B = L->box[b];
+        for (int n = 0; n < O_NVAR; ++n)
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+            sums[n] += FA(S, i, j, k, n);
+    }
+}

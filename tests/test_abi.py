@@ -1,0 +1,66 @@
+"""The C-ABI library loads and exports every entry point include/ declares
+(no compute calls: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+from camr_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(?:int|void|double|const char\s*\*)\s*\*?\s*(camr_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_all_declared_symbols():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) == names
+
+
+def test_struct_layouts_match_header():
+    # sizes of the ABI structs as the C compiler lays them out
+    assert C.sizeof(_lib.Box) == 24
+    assert C.sizeof(_lib.Array4) == 8 + 3 * 8 + 6 * 4 + 4 + 4
+    p = _lib.HydroParams()
+    _lib.lib().camr_default_params(C.byref(p))
+    assert p.plm_iorder == 4 and p.small_dens == 1e-7 and p.difmag == 0.1
+    assert p.small_pres == 1e-200 and p.smallu == 1e-12 and p.small_num == 1e-8
+    d = _lib.LevelDesc()
+    _lib.lib().camr_default_level_desc(C.byref(d))
+    assert d.cfl == 0.8 and d.change_max == 1.1 and d.add_grav_src == 1
+
+
+def test_invalid_parameters_are_rejected_without_gpu_work():
+    # parameter validation happens on the host before any device call
+    from camr_amd import workloads
+    from camr_amd.hydro import Level
+    prob = workloads.sod_tube(nx=16)
+    prob.params["ppm_type"] = 3
+    try:
+        Level(prob)
+    except _lib.CamrError as e:
+        assert e.code == _lib.ERR_PARAM and "ppm_type" in str(e)
+    else:
+        raise AssertionError("ppm_type=3 accepted")
+    prob = workloads.sod_tube(nx=16)
+    prob.params["cfl"] = 1.5
+    try:
+        Level(prob)
+    except _lib.CamrError as e:
+        assert e.code == _lib.ERR_PARAM
+    else:
+        raise AssertionError("cfl=1.5 accepted")

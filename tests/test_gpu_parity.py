@@ -1,0 +1,220 @@
+"""Parity of the gfx950 path with the CPU oracle, through the C-ABI.
+
+Bar: bit-identical FP64 results (the north-star tolerance is 1e-12 relative
+L-inf per conserved field; the kernels reproduce the oracle's operation
+order with FMA contraction disabled, so equality is exact).  Reported on
+failure: max relative difference per component.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from camr_amd import _lib, problems, workloads
+from camr_amd.hydro import Level, Problem, hydro_params
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _rel(a, b):
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(a), np.abs(b))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.where(s > 0, d / s, 0.0)
+    return r.reshape(r.shape[0], -1).max(axis=1)
+
+
+def assert_bitwise(gpu, ref, what):
+    if not np.array_equal(gpu, ref):
+        raise AssertionError(f"{what}: not bit-identical; max rel diff per "
+                             f"component {_rel(gpu, ref)}")
+
+
+def _oracle_params(O, prob):
+    return O.params(
+        ppm_type=prob.value("ppm_type"), plm_iorder=prob.value("plm_iorder"),
+        use_flattening=prob.value("use_flattening"), use_pslope=prob.value("use_pslope"),
+        transverse_reset_density=prob.value("transverse_reset_density"),
+        cfl=prob.value("cfl"), add_grav_src=prob.value("add_grav_src"),
+        const_grav=prob.value("const_grav"), bclo=prob.bclo, bchi=prob.bchi,
+        domlo=prob.domlo, domhi=prob.domhi, is_periodic=prob.is_periodic, dx=prob.dx)
+
+
+def _run_both(O, prob, init, nsteps, engine=_lib.ENGINE_AUTO):
+    ol = O.Level(_oracle_params(O, prob), prob.boxes)
+    gl = Level(prob, engine=engine)
+    for b in range(len(prob.boxes)):
+        U = init(prob, b)
+        ol.set_box(b, U)
+        gl.set_box(b, U)
+    ol.init_post()
+    gl.init_post()
+    err, odts = ol.run(nsteps)
+    assert err == 0
+    gdts = gl.run(nsteps)
+    return ol, gl, odts, gdts
+
+
+# ------------------------------------------------------------ per box
+def _box_state(lo, hi, seed, shock=True):
+    rng = np.random.default_rng(seed)
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    fld = problems.SmoothField(seed=seed, kmax=4, mach=0.6)
+    N = 64
+    U = fld.state(N, [l % N for l in lo], [l % N + n[d] - 1 for d, l in enumerate(lo)])
+    if shock:
+        # planar jumps in pressure and density to switch limiters / flattening
+        z = np.arange(n[2])[:, None, None] + 0.5 * np.arange(n[0])[None, None, :]
+        jump = (z > n[2] / 2).astype(float)
+        rho = U[0] * (1.0 + 1.5 * jump)
+        U[1:4] *= (rho / U[0])[None]
+        U[0] = rho
+        U[5] = U[5] * (1.0 + 4.0 * jump)
+        U[7] = rho
+    U[4] = U[5] + 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0]
+    U[6] = rng.uniform(0, 1, U[6].shape)  # UTEMP is overwritten by ctoprim
+    return np.ascontiguousarray(U)
+
+
+CASES = [
+    dict(ppm=1, order=4, flat=1, bclo=(0, 0, 0), bchi=(0, 0, 0)),
+    dict(ppm=0, order=4, flat=1, bclo=(0, 0, 0), bchi=(0, 0, 0)),
+    dict(ppm=0, order=2, flat=0, bclo=(4, 2, 5), bchi=(3, 4, 2)),
+    dict(ppm=1, order=4, flat=1, bclo=(5, 3, 2), bchi=(2, 5, 4)),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_umdrv_per_box_bitwise(oracle, case):
+    lo, hi = (3, 5, 7), (3 + 15, 5 + 11, 7 + 13)
+    G = 4
+    glo = tuple(l - G for l in lo)
+    ghi = tuple(h + G for h in hi)
+    U = _box_state(glo, ghi, seed=11)
+    dx = (1 / 64,) * 3
+    prob = Problem(domlo=lo, domhi=hi, dx=dx, bclo=case["bclo"], bchi=case["bchi"],
+                   is_periodic=(0, 0, 0), boxes=[lo + hi],
+                   params=dict(ppm_type=case["ppm"], plm_iorder=case["order"],
+                               use_flattening=case["flat"]))
+    hp = hydro_params(prob)
+    op = _oracle_params(oracle, prob)
+    dt = 0.3 * dx[0]
+    # oracle
+    q = np.zeros((11,) + U.shape[1:])
+    qa = np.zeros((5,) + U.shape[1:])
+    srcq = np.zeros_like(q)
+    err = C.c_int(0)
+    L3 = C.c_int * 3
+    oracle.lib().ora_ctoprim_box(L3(*glo), L3(*ghi), C.byref(oracle.fab(U, glo)),
+                                 C.byref(oracle.fab(q, glo)), C.byref(oracle.fab(qa, glo)),
+                                 C.byref(op), C.byref(err))
+    assert err.value == 0
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    dsdt = np.zeros((10, n[2], n[1], n[0]))
+    flx = []
+    for d in range(3):
+        s = [n[2], n[1], n[0]]
+        s[2 - d] += 1
+        flx.append(np.zeros([10] + s))
+    fl = (oracle.OraFab * 3)(*[oracle.fab(f, lo) for f in flx])
+    oracle.lib().ora_hydro_umdrv(L3(*lo), L3(*hi), C.byref(oracle.fab(U, glo)),
+                                 C.byref(oracle.fab(dsdt, lo)), C.byref(oracle.fab(q, glo)),
+                                 C.byref(oracle.fab(qa, glo)), C.byref(oracle.fab(srcq, glo)),
+                                 fl, dt, C.byref(op))
+    # gpu
+    from camr_amd.hydro import hydro_ctoprim, hydro_umdrv
+    dev = torch.device("cuda:0")
+    tU = torch.from_numpy(U).to(dev)
+    tq = torch.zeros((11,) + U.shape[1:], dtype=torch.float64, device=dev)
+    tqa = torch.zeros((5,) + U.shape[1:], dtype=torch.float64, device=dev)
+    hydro_ctoprim(glo, ghi, tU, glo, tq, tqa, hp)
+    torch.cuda.synchronize()
+    assert_bitwise(tq.cpu().numpy(), q, "ctoprim q")
+    assert_bitwise(tqa.cpu().numpy(), qa, "ctoprim qaux")
+    tds = torch.zeros(dsdt.shape, dtype=torch.float64, device=dev)
+    tfl = [torch.full(f.shape, float("nan"), dtype=torch.float64, device=dev) for f in flx]
+    hydro_umdrv(lo, hi, tU, glo, tds, tq, tqa, None, tfl, dt, hp)
+    torch.cuda.synchronize()
+    for d in range(3):
+        assert_bitwise(tfl[d].cpu().numpy(), flx[d], f"flux {d}")
+    assert_bitwise(tds.cpu().numpy(), dsdt, "dsdt")
+
+
+def test_estdt_matches_oracle(oracle):
+    lo, hi = (0, 0, 0), (20, 17, 9)
+    U = _box_state(lo, hi, seed=5)
+    dx = (0.01, 0.01, 0.01)
+    from camr_amd.hydro import estdt_hydro
+    t = torch.from_numpy(U).to("cuda:0")
+    g = estdt_hydro(lo, hi, t, lo, dx)
+    L3 = C.c_int * 3
+    o = oracle.lib().ora_estdt_box(L3(*lo), L3(*hi), C.byref(oracle.fab(U, lo)),
+                                   (C.c_double * 3)(*dx))
+    assert g == o
+
+
+# ------------------------------------------------------------ level steps
+@pytest.mark.parametrize("engine", [_lib.ENGINE_AUTO, _lib.ENGINE_STAGED])
+def test_sod_tube_100_steps_bitwise(oracle, engine):
+    prob = workloads.sod_tube()
+    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.sod_init, 100, engine)
+    assert np.array_equal(odts, gdts), (odts - gdts)
+    assert_bitwise(gl.get_box(0), ol.get_box(0), "Sod state after 100 steps")
+
+
+@pytest.mark.parametrize("ppm", [1, 0])
+def test_smooth_periodic_bitwise(oracle, ppm):
+    prob = workloads.smooth_periodic(n=32, ppm_type=ppm)
+    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 5)
+    assert np.array_equal(odts, gdts)
+    assert_bitwise(gl.get_box(0), ol.get_box(0), "smooth 32^3 after 5 steps")
+
+
+def test_multibox_level_bitwise(oracle):
+    prob = workloads.smooth_periodic(n=32, ppm_type=1, max_grid=16)
+    assert len(prob.boxes) == 8
+    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 3)
+    assert np.array_equal(odts, gdts)
+    for b in range(8):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"box {b}")
+
+
+@pytest.mark.parametrize("grav", [0.0, -0.7])
+def test_walls_and_sources_bitwise(oracle, grav):
+    n = (24, 20, 16)
+    dx = (1 / 24,) * 3
+    prob = Problem(domlo=(0, 0, 0), domhi=tuple(x - 1 for x in n), dx=dx,
+                   bclo=(2, 4, 5), bchi=(3, 5, 4), is_periodic=(0, 0, 0),
+                   boxes=[(0, 0, 0, 11, 19, 15), (12, 0, 0, 23, 19, 15)],
+                   params=dict(ppm_type=0, cfl=0.4, const_grav=grav))
+    full = problems.SmoothField(seed=3, kmax=3, mach=0.4).state(48, (0, 0, 0), (23, 19, 15))
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(full[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 4)
+    assert np.array_equal(odts, gdts)
+    for b in range(2):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"walls box {b}")
+
+
+def test_large_periodic_conservation():
+    # size-independent property at the full 256^3 size: periodic fluxes
+    # telescope, so mass, momentum and total energy are conserved to
+    # round-off; species 0 tracks density.
+    prob = workloads.smooth_periodic(n=256, ppm_type=1)
+    gl = Level(prob)
+    U = workloads.smooth_init(prob, 0)
+    gl.set_box(0, U)
+    gl.init_post()
+    s0 = gl.get_box(0).reshape(10, -1).sum(axis=1)
+    gl.run(2)
+    S = gl.get_box(0)
+    s1 = S.reshape(10, -1).sum(axis=1)
+    scale = np.abs(S).reshape(10, -1).sum(axis=1)
+    for c in (0, 1, 2, 3, 4):
+        assert abs(s1[c] - s0[c]) <= 1e-12 * scale[c], (c, s1[c] - s0[c])
+    assert np.all(np.isfinite(S))
