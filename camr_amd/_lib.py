@@ -74,7 +74,7 @@ EXPORTS = [
     "camr_level_download_box", "camr_level_state_array",
     "camr_level_init_post", "camr_level_initial_dt", "camr_level_new_dt",
     "camr_level_advance", "camr_level_run", "camr_level_sync",
-    "camr_level_time",
+    "camr_level_time", "camr_level_profile", "camr_level_profile_report",
 ]
 
 _lib = None
@@ -120,6 +120,8 @@ def lib():
         L.camr_level_sync.argtypes = [vp]
         L.camr_level_time.argtypes = [vp]
         L.camr_level_time.restype = C.c_double
+        L.camr_level_profile.argtypes = [vp, C.c_int32, C.c_char_p]
+        L.camr_level_profile_report.argtypes = [vp, C.c_char_p, C.c_int64]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default restype

@@ -126,6 +126,7 @@ class HydroLevel {
     int sync();
     void set_stream(hipStream_t s) { stream_ = s; }
     double time() const { return time_; }
+    Prof &prof() { return prof_; }
 
   private:
     int est_time_step(double *est);
@@ -154,6 +155,7 @@ class HydroLevel {
     hipStream_t stream_ = nullptr;
     double time_ = 0.0;
     bool grav_src_values_ = false;
+    Prof prof_;
 };
 
 void HydroLevel::release() {
@@ -428,7 +430,7 @@ int HydroLevel::construct_hydro_source(double dt) {
             A.vol = ds;
             A.has_vol = 0;
             A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
-            int rc = umdrv_staged(A, scratch_.base(), stream_);
+            int rc = umdrv_staged(A, scratch_.base(), stream_, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
     }
@@ -440,6 +442,7 @@ int HydroLevel::construct_hydro_source(double dt) {
 
 int HydroLevel::advance(double dt) {
     if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    Region step(&prof_, "advance", stream_);
     // swapTimeLevels (CAMR_advance.cpp:72-75)
     std::swap(snew_, sold_);
     a_is_new_ = !a_is_new_;
@@ -739,6 +742,19 @@ int camr_level_run(camr_level *L, int32_t nsteps, double *dts) {
     return L->impl.sync();
 }
 int camr_level_sync(camr_level *L) { return L ? L->impl.sync() : CAMR_ERR_PARAM; }
+int camr_level_profile(camr_level *L, int32_t enable, const char *kernel_filter) {
+    if (!L) return CAMR_ERR_PARAM;
+    L->impl.prof().enable(enable != 0, kernel_filter);
+    L->impl.prof().reset();
+    return CAMR_OK;
+}
+int camr_level_profile_report(camr_level *L, char *buf, int64_t buflen) {
+    if (!L || !buf || buflen <= 0) return CAMR_ERR_PARAM;
+    std::string s = L->impl.prof().json();
+    if ((int64_t)s.size() + 1 > buflen) { set_error("report buffer too small"); return CAMR_ERR_PARAM; }
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return CAMR_OK;
+}
 double camr_level_time(const camr_level *L) { return L ? L->impl.time() : 0.0; }
 
 } // extern "C"

@@ -3,6 +3,7 @@
 
 #include "array.h"
 #include "hydro_math.h"
+#include "profiler.h"
 
 namespace camr {
 
@@ -26,7 +27,8 @@ struct UmdrvArgs {
 HydroConst make_const(const camr_hydro_params &p);
 
 size_t umdrv_staged_scratch_doubles(const Box &bx);
-int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st);
+int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st,
+                 Prof *prof = nullptr);
 
 // ---- hydro_divu (Source/Hydro/Hydro_utils_K.H:12-169), node (i,j,k) -------
 __device__ __forceinline__ double divu_node(const Arr &q, int i, int j, int k,

@@ -372,11 +372,16 @@ k_update(Range r, Arr dsdt, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
 
 // ---------------------------------------------------------------- driver
 namespace {
+Prof *g_prof = nullptr;  // set for the duration of one umdrv_staged call
+const char *g_name = "";
+
 template <typename K, typename... A>
 void launch(K kern, const Box &b, hipStream_t s, A... args) {
     Range r = range_of(b);
+    Region reg(g_prof, g_name, s);
     hipLaunchKernelGGL(kern, dim3(grid_for(r.n)), dim3(kBlock), 0, s, r, args...);
 }
+#define NAMED(n) g_name = n
 } // namespace
 
 size_t umdrv_staged_scratch_doubles(const Box &bx) {
@@ -406,7 +411,8 @@ size_t umdrv_staged_scratch_doubles(const Box &bx) {
     return n + 64;
 }
 
-int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
+int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof) {
+    g_prof = prof;
     const HydroConst &hc = A.hc;
     const Box bx = A.bx, bxg1 = grow(bx, 1), bxg2 = grow(bx, 2);
     double *cur = scratch;
@@ -418,6 +424,7 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
     const double dt = A.dt;
 
     Arr flat = take(bxg2, 1);
+    NAMED("flatten");
     launch(k_flat, bxg2, st, A.q, A.qbox, flat, hc.use_flattening);
 
     Arr qdm[3], qdp[3];
@@ -426,9 +433,11 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
         qdp[d] = take(bxg2, NS);
     }
     if (hc.ppm_type == 0) {
+        NAMED("trace_plm");
         launch(k_plm, bxg2, st, A.q, A.qaux, A.srcq, A.has_src, flat, qdm[0],
                qdp[0], qdm[1], qdp[1], qdm[2], qdp[2], hc, dt);
     } else {
+        NAMED("trace_ppm");
         launch(k_ppm<0>, bxg2, st, A.q, A.qaux, flat, qdm[0], qdp[0], hc, dt);
         launch(k_ppm<1>, bxg2, st, A.q, A.qaux, flat, qdm[1], qdp[1], hc, dt);
         launch(k_ppm<2>, bxg2, st, A.q, A.qaux, flat, qdm[2], qdp[2], hc, dt);
@@ -441,6 +450,7 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
         F1[d] = take(fb, NF);
         G1[d] = take(fb, NG);
     }
+    NAMED("riemann_normal");
     launch(k_riemann<0, 0>, nodes(grow_dir(bxg2, 0, -1), 0), st, qdm[0], qdp[0], A.qaux, F1[0], G1[0], hc);
     launch(k_riemann<1, 0>, nodes(grow_dir(bxg2, 1, -1), 1), st, qdm[1], qdp[1], A.qaux, F1[1], G1[1], hc);
     launch(k_riemann<2, 0>, nodes(grow_dir(bxg2, 2, -1), 2), st, qdm[2], qdp[2], A.qaux, F1[2], G1[2], hc);
@@ -456,6 +466,7 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
                 G2[d][t] = take(nodes(bxg1, d), NG);
             }
     double *tq_base = cur;
+    NAMED("transverse_1");
     for (int d = 0; d < 3; ++d) {
         cur = tq_base;
         const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
@@ -483,12 +494,12 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
     const double hdtd[3] = {0.5 * dt / hc.dx[0], 0.5 * dt / hc.dx[1], 0.5 * dt / hc.dx[2]};
     Arr G3[3];
     double *fq_base = cur;
-    for (int d = 0; d < 3; ++d) G3[d] = dense(nullptr, nodes(bx, d));
-    // G3 live until the update: allocate before the reused region
+    // G3 and divu stay live until the update: allocate before the reused region
     cur = fq_base;
     for (int d = 0; d < 3; ++d) G3[d] = take(nodes(bx, d), NG);
     Arr divu = take(nodes(nodes(nodes(bx, 0), 1), 2), 1);
     double *qf_base = cur;
+    NAMED("transverse_2");
     for (int d = 0; d < 3; ++d) {
         cur = qf_base;
         const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
@@ -507,13 +518,17 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st) {
         }
     }
 
+    NAMED("divu");
     launch(k_divu, nodes(nodes(nodes(bx, 0), 1), 2), st, A.q, divu, hc);
+    NAMED("adjust_fluxes");
     launch(k_adjust<0>, nodes(bx, 0), st, A.flx[0], divu, A.uin, A.area[0], A.has_area, A.area_c[0], hc);
     launch(k_adjust<1>, nodes(bx, 1), st, A.flx[1], divu, A.uin, A.area[1], A.has_area, A.area_c[1], hc);
     launch(k_adjust<2>, nodes(bx, 2), st, A.flx[2], divu, A.uin, A.area[2], A.has_area, A.area_c[2], hc);
+    NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G3[0], G3[1], G3[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
            A.area_c[2], A.vol, A.has_vol, A.vol_c);
+    g_prof = nullptr;
     return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
 }
 

@@ -142,6 +142,17 @@ class Level:
     def time(self):
         return L.lib().camr_level_time(self._h)
 
+    def profile(self, enable=True, kernel=None):
+        """Time launches (all, or only those named `kernel`) with hipEvents."""
+        L.check(L.lib().camr_level_profile(
+            self._h, int(enable), kernel.encode() if kernel else None))
+
+    def profile_report(self):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        L.check(L.lib().camr_level_profile_report(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
 
 # ---------------------------------------------------------------- per box
 def array4_of(t, lo, ncomp=None) -> L.Array4:

@@ -168,6 +168,13 @@ int camr_level_run(camr_level *L, int32_t nsteps, double *dts);
 int camr_level_sync(camr_level *L);
 double camr_level_time(const camr_level *L);
 
+/* Named hipEvent timers around kernel launches on the level's stream -- the
+ * counterpart of the reference's BL_PROFILE regions (TinyProfiler).  With a
+ * non-NULL filter only launches of that name are timed.  The report is a
+ * JSON object {"name": {"count": n, "ms": total}, ...}. */
+int camr_level_profile(camr_level *L, int32_t enable, const char *kernel_filter);
+int camr_level_profile_report(camr_level *L, char *buf, int64_t buflen);
+
 #ifdef __cplusplus
 }
 #endif
