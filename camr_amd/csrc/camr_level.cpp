@@ -14,6 +14,7 @@
 #include <memory>
 #include <vector>
 
+#include "hydro_fused.h"
 #include "hydro_state.h"
 
 namespace camr {
@@ -156,6 +157,8 @@ class HydroLevel {
     double time_ = 0.0;
     bool grav_src_values_ = false;
     Prof prof_;
+    bool fused_ = true;
+    FusedEngine engine_;
 };
 
 void HydroLevel::release() {
@@ -278,7 +281,13 @@ int HydroLevel::create() {
     CAMR_HIP(hipMemset(status_, 0, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_status_, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_dtmin_, sizeof(unsigned long long)));
-    {
+    fused_ = (desc_.engine != 1);
+    if (fused_) {
+        if ((rc = engine_.reserve(box_))) {
+            set_error("fused-engine scratch allocation failed");
+            return rc;
+        }
+    } else {
         if ((rc = alloc(&q_, QVAR * max_g4))) return rc;
         if ((rc = alloc(&qaux_, NQAUX * max_g4))) return rc;
         if ((rc = alloc(&srcq_, QVAR * max_g4))) return rc;
@@ -399,7 +408,16 @@ int HydroLevel::construct_hydro_source(double dt) {
                       dense(srchyd_[b], grow(box_[b], CAMR_NGROW)), (const DevFab *)tab_src_,
                       (const int *)owner_, dom_);
     }
-    {
+    if (fused_) {
+        for (size_t b = 0; b < box_.size(); ++b) {
+            const Box G = grow(box_[b], CAMR_NGROW);
+            Arr src = grav_src_values_ ? dense(srchyd_[b], G) : sbv_[b];
+            int rc = engine_.advance_box(box_[b], sbv_[b], src, grav_src_values_ ? 1 : 0,
+                                         snew_[b], hc_, dt, stream_, status_, &prof_);
+            if (rc) return rc;
+        }
+        return CAMR_OK;
+    } else {
         for (size_t b = 0; b < box_.size(); ++b) {
             const Box B = box_[b], G = grow(B, CAMR_NGROW);
             Arr q = dense(q_, G), qa = dense(qaux_, G), sq = dense(srcq_, G);
