@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcamr_hydro.so")
+LIB_PATH = os.environ.get("CAMR_LIB_PATH") or os.path.join(HERE, "libcamr_hydro.so")
 
 NVAR, QVAR, NQAUX, NGROW = 10, 11, 5, 4
 

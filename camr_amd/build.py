@@ -18,6 +18,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcamr_hydro.so")
+# Extra compiler flags (e.g. -D switches) for variant builds; the product
+# library is always built without them.
+EXTRA = os.environ.get("CAMR_EXTRA_FLAGS", "").split()
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CAMR_OFFLOAD_ARCH", "gfx950")
 
@@ -35,20 +38,23 @@ def sources():
                   if f.endswith((".hip", ".cpp")))
 
 
-def _compile(src: str) -> tuple[str, str, int]:
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+def _compile(src: str, build_dir: str = BUILD, extra=()) -> tuple[str, str, int]:
+    obj = os.path.join(build_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *CFLAGS, *extra, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     return obj, (p.stdout + p.stderr), p.returncode
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, out: str = LIB,
+          extra=None) -> str:
+    extra = EXTRA if extra is None else list(extra)
+    build_dir = BUILD if out == LIB else BUILD + "_" + os.path.basename(out).replace(".", "_")
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     objs, failed = [], []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, log, rc in ex.map(_compile, srcs):
+        for obj, log, rc in ex.map(lambda s: _compile(s, build_dir, extra), srcs):
             objs.append(obj)
             if rc != 0:
                 failed.append(log)
@@ -56,14 +62,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
                 print(log, file=sys.stderr)
     if failed:
         raise RuntimeError("hipcc failed:\n" + "\n".join(failed))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    args = [a for a in sys.argv[1:] if a != "-v"]
+    print(build(verbose="-v" in sys.argv, out=args[0] if args else LIB))

@@ -102,6 +102,20 @@ __device__ __forceinline__ bool range_point(const Range &r, int64_t t, int &i,
 }
 
 constexpr int kBlock = 256;
+
+// Workgroups are dealt round-robin to the 8 XCDs of an MI355X, each with its
+// own L2.  Renumber them so that XCD x owns a contiguous run of logical
+// blocks: neighbouring rows / planes of a stencil then share one L2.
+constexpr unsigned kXCD = 8;
+__device__ __forceinline__ int64_t logical_thread() {
+#ifndef CAMR_XCD_REMAP
+    return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#endif
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned q = nb / kXCD, r = nb % kXCD, x = b % kXCD;
+    const unsigned lb = x * q + (x < r ? x : r) + b / kXCD;
+    return (int64_t)lb * blockDim.x + threadIdx.x;
+}
 inline unsigned grid_for(int64_t n, int block = kBlock) {
     int64_t g = (n + block - 1) / block;
     return (unsigned)(g > 0 ? g : 1);

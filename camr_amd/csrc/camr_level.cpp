@@ -131,6 +131,9 @@ class HydroLevel {
 
   private:
     int est_time_step(double *est);
+    int advance_exact(double dt);
+    int advance_fused(double dt);
+    int resolve();
     int clean_state(std::vector<Arr> &S, bool launch_detect);
     int expand_state();
     int construct_hydro_source(double dt);
@@ -159,6 +162,10 @@ class HydroLevel {
     Prof prof_;
     bool fused_ = true;
     FusedEngine engine_;
+    bool pending_ = false;   // a fused step awaits resolve()
+    bool dt_ready_ = false;  // dtmin_ holds the CFL estimate of S_new
+    double last_dt_ = 0.0;
+    DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
 };
 
 void HydroLevel::release() {
@@ -169,7 +176,8 @@ void HydroLevel::release() {
     for (auto p : dsdt_) fr(p);
     for (auto p : srchyd_) fr(p);
     bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear();
-    fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(owner_); fr(status_); fr(dtmin_);
+    fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_);
+    tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
     fr(q_); fr(qaux_); fr(srcq_);
     for (auto &f : flx_) { fr(f); f = nullptr; }
     if (h_status_) (void)hipHostFree(h_status_);
@@ -267,6 +275,19 @@ int HydroLevel::create() {
     CAMR_HIP(hipMalloc(&owner_, ncell * sizeof(int)));
     CAMR_HIP(hipMemcpy(owner_, own.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
     CAMR_HIP(hipMalloc(&tab_old_, nb * sizeof(DevFab)));
+    CAMR_HIP(hipMalloc(&tab_a_, nb * sizeof(DevFab)));
+    CAMR_HIP(hipMalloc(&tab_b_, nb * sizeof(DevFab)));
+    {
+        std::vector<DevFab> ta(nb), tb(nb);
+        for (int b = 0; b < nb; ++b) {
+            ta[b].a = dense(bufA_[b], box_[b]);
+            ta[b].b = box_[b];
+            tb[b].a = dense(bufB_[b], box_[b]);
+            tb[b].b = box_[b];
+        }
+        CAMR_HIP(hipMemcpy(tab_a_, ta.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(tab_b_, tb.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    }
     CAMR_HIP(hipMalloc(&tab_sb_, nb * sizeof(DevFab)));
     if (grav_src_values_) CAMR_HIP(hipMalloc(&tab_src_, nb * sizeof(DevFab)));
     std::vector<DevFab> t(nb);
@@ -305,12 +326,17 @@ int HydroLevel::create() {
 
 int HydroLevel::upload(int b, const double *h) {
     if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
+    dt_ready_ = false;
     CAMR_HIP(hipMemcpy(snew_[b].p, h, NVAR * box_[b].npts() * sizeof(double), hipMemcpyHostToDevice));
     return CAMR_OK;
 }
 
 int HydroLevel::download(int b, double *h) {
     if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
     CAMR_HIP(hipStreamSynchronize(stream_));
     CAMR_HIP(hipMemcpy(h, snew_[b].p, NVAR * box_[b].npts() * sizeof(double), hipMemcpyDeviceToHost));
     return CAMR_OK;
@@ -348,11 +374,14 @@ int HydroLevel::check_status() {
 }
 
 int HydroLevel::sync() {
+    int rc = resolve();
+    if (rc) return rc;
     CAMR_HIP(hipStreamSynchronize(stream_));
     return check_status();
 }
 
 int HydroLevel::init_post() {
+    dt_ready_ = false;
     for (size_t b = 0; b < box_.size(); ++b)
         launch_on(k_init_post, box_[b], stream_, snew_[b], hc_);
     CAMR_HIP(hipGetLastError());
@@ -460,10 +489,29 @@ int HydroLevel::construct_hydro_source(double dt) {
 
 int HydroLevel::advance(double dt) {
     if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    int rc = resolve();
+    if (rc) return rc;
     Region step(&prof_, "advance", stream_);
     // swapTimeLevels (CAMR_advance.cpp:72-75)
     std::swap(snew_, sold_);
     a_is_new_ = !a_is_new_;
+    if (fused_ && !grav_src_values_) {
+        rc = advance_fused(dt);
+        pending_ = true;
+        last_dt_ = dt;
+    } else {
+        rc = advance_exact(dt);
+        dt_ready_ = false;
+    }
+    if (rc) return rc;
+    CAMR_HIP(hipGetLastError());
+    time_ += dt;
+    return CAMR_OK;
+}
+
+// The reference sequence pass by pass (CAMR_advance.cpp:72-164), starting
+// after the swap; S_old holds the raw previous state.
+int HydroLevel::advance_exact(double dt) {
     // clean_state(S_old) (:83)
     clean_state(sold_, true);
     // expand_state(Sborder, prev_time, numGrow()) (:99)
@@ -488,9 +536,64 @@ int HydroLevel::advance(double dt) {
     }
     // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
     clean_state(snew_, true);
-    CAMR_HIP(hipGetLastError());
-    time_ += dt;
     return CAMR_OK;
+}
+
+// Same step with the pointwise passes fused (level_fused.hip); exact unless a
+// density floor fires, which resolve() detects and answers with
+// advance_exact from the untouched raw S_old.
+int HydroLevel::advance_fused(double dt) {
+    const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
+    launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
+    const DevFab *tab = a_is_new_ ? tab_b_ : tab_a_;  // table of the buffers now old
+    for (size_t b = 0; b < box_.size(); ++b) {
+        FillArgs fa;
+        fa.valid = box_[b];
+        fa.sborder = sbv_[b];
+        fa.q = engine_.prim_view(box_[b]);
+        fa.snew = snew_[b];
+        fa.sold_tab = tab;
+        fa.owner = owner_;
+        fa.dom = dom_;
+        fa.hc = hc_;
+        fa.dt = dt;
+        fa.g = desc_.const_grav;
+        fa.add_grav = desc_.add_grav_src;
+        launch_fill_fused(fa, stream_, status_, &prof_);
+        int rc = engine_.advance_box(box_[b], sbv_[b], sbv_[b], 0, snew_[b], hc_, dt, stream_,
+                                     status_, &prof_, true);
+        if (rc) return rc;
+        PostArgs pa;
+        pa.valid = box_[b];
+        pa.snew = snew_[b];
+        pa.sold = sold_[b];
+        pa.hc = hc_;
+        pa.dt = dt;
+        pa.g = desc_.const_grav;
+        pa.add_grav = desc_.add_grav_src;
+        pa.dtmin = dtmin_;
+        launch_post_fused(pa, stream_, status_, &prof_);
+    }
+    dt_ready_ = true;
+    return CAMR_OK;
+}
+
+// Settle a fused step: wait for it and, if a density floor was flagged,
+// recompute it with the exact pass-by-pass sequence.
+int HydroLevel::resolve() {
+    if (!pending_) return CAMR_OK;
+    pending_ = false;
+    CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    if (*h_status_ & kFloorFlag) {
+        *h_status_ &= ~kFloorFlag;
+        CAMR_HIP(hipMemcpyAsync(status_, h_status_, sizeof(int), hipMemcpyHostToDevice, stream_));
+        int rc = advance_exact(last_dt_);
+        dt_ready_ = false;
+        if (rc) return rc;
+        CAMR_HIP(hipStreamSynchronize(stream_));
+    }
+    return check_status();
 }
 
 // estTimeStep (CAMR.cpp:490-563)
@@ -499,11 +602,16 @@ int HydroLevel::est_time_step(double *est) {
     double estdt = desc_.max_dt;
     const double max_dt_over_cfl = desc_.max_dt / desc_.cfl;
     double estdt_hydro = max_dt_over_cfl;
-    const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
-    CAMR_HIP(hipMemcpyAsync(dtmin_, &init, sizeof(init), hipMemcpyHostToDevice, stream_));
-    for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_);
-    CAMR_HIP(hipGetLastError());
+    int rc0 = resolve();
+    if (rc0) return rc0;
+    if (!dt_ready_) {
+        const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
+        launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
+        for (size_t b = 0; b < box_.size(); ++b)
+            launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_);
+        CAMR_HIP(hipGetLastError());
+        dt_ready_ = true;
+    }
     CAMR_HIP(hipMemcpyAsync(h_dtmin_, dtmin_, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
     int rc = sync();
     if (rc) return rc;

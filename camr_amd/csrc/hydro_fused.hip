@@ -23,42 +23,29 @@ namespace {
 
 constexpr int ex_(int D, int d) { return D == d ? 1 : 0; }
 
+// occupancy request for the register-heavy face kernels (waves per SIMD)
+#ifdef CAMR_TRANS_MINW
+#define TRANS_BOUNDS __launch_bounds__(kBlock, CAMR_TRANS_MINW)
+#else
+#define TRANS_BOUNDS __launch_bounds__(kBlock)
+#endif
+
 // ------------------------------------------------------------ primitives
 // hydro_ctoprim (Hydro_ctoprim.H:11-101) into the compact layout, with
 // hydro_srctoprim (:106-157) when sources are non-zero.
 __global__ void __launch_bounds__(kBlock)
 k_prim_c(Range r, Arr u, Arr q, Arr src, Arr srcq, int has_src, HydroConst hc, int *status) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    const double rho = u(i, j, k, URHO);
+    if (!range_point(r, logical_thread(), i, j, k)) return;
+    double v[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) v[n] = u(i, j, k, n);
+    const double rho = v[URHO];
     if (!(rho > 0.)) return;
-    const double rhoinv = 1.0 / rho;
-    const double vx = u(i, j, k, UMX) * rhoinv;
-    const double vy = u(i, j, k, UMY) * rhoinv;
-    const double vz = u(i, j, k, UMZ) * rhoinv;
-    const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-    const double ueden = u(i, j, k, UEDEN);
-    const double ueint = u(i, j, k, UEINT);
-    if (!hc.allow_negative_energy && !(ueden > 0. && ueint > 0.))
-        atomicOr(status, 1 << CAMR_ERR_NEG_ENERGY);
-    const double e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv
-                                                         : ueint * rhoinv;
-    const double p = (kGamma - 1.0) * rho * e;
-    const double cs = sqrt(kGamma * p / rho);
-    double y[NSPEC];
-#pragma unroll
-    for (int s = 0; s < NSPEC; ++s) y[s] = u(i, j, k, UFS + s) * rhoinv;
-    q(i, j, k, SR) = rho;
-    q(i, j, k, SU) = vx;
-    q(i, j, k, SV) = vy;
-    q(i, j, k, SW) = vz;
-    q(i, j, k, SP) = p;
-    q(i, j, k, SE) = e * rho;
-#pragma unroll
-    for (int s = 0; s < NSPEC; ++s) q(i, j, k, SY + s) = y[s];
-    q(i, j, k, SC) = cs;
-    q(i, j, k, SCS) = vmax(hc.small_num, hc.small_num * cs);
+    const double e = prim_compact_point(v, q, i, j, k, hc, status);
     if (has_src) {
+        const double rhoinv = 1.0 / rho;
+        const double vx = v[UMX] * rhoinv, vy = v[UMY] * rhoinv, vz = v[UMZ] * rhoinv;
         const double srcrho = src(i, j, k, URHO);
         srcq(i, j, k, SR) = srcrho;
         srcq(i, j, k, SU) = (src(i, j, k, UMX) - vx * srcrho) * rhoinv;
@@ -68,7 +55,7 @@ k_prim_c(Range r, Arr u, Arr q, Arr src, Arr srcq, int has_src, HydroConst hc, i
         srcq(i, j, k, SE) = sre;
 #pragma unroll
         for (int s = 0; s < NSPEC; ++s)
-            srcq(i, j, k, SY + s) = (src(i, j, k, UFS + s) - y[s] * srcrho) * rhoinv;
+            srcq(i, j, k, SY + s) = (src(i, j, k, UFS + s) - (v[UFS + s] * rhoinv) * srcrho) * rhoinv;
         const double dpde = (kGamma - 1.0) * rho, dpdr = (kGamma - 1.0) * e;
         srcq(i, j, k, SP) = dpde * (sre - (e * rho) * srcrho * rhoinv) * rhoinv + dpdr * srcrho;
     }
@@ -80,7 +67,7 @@ k_prim_c(Range r, Arr u, Arr q, Arr src, Arr srcq, int has_src, HydroConst hc, i
 __global__ void __launch_bounds__(kBlock)
 k_flat_c(Range r, Arr q, Box qbox, Arr flat, int use_flattening) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     double f = 1.0;
     if (use_flattening == 1) {
 #pragma unroll
@@ -110,7 +97,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_ppm_c(Range r, Arr q, Arr flat, Arr qm, Arr qp, HydroConst hc, double dtdx) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
     constexpr int SN = SU + D;
     constexpr int T1 = (D == 0) ? SV : SU;
@@ -170,7 +157,7 @@ __global__ void __launch_bounds__(kBlock)
 k_plm_c(Range r, Arr q, Arr srcq, int has_src, Arr flat, Arr qm, Arr qp,
         HydroConst hc, double dtdx) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
     const double fl = flat(i, j, k, 0);
     double c[NS], sl[NS];
@@ -251,7 +238,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_riemann1(Range r, Arr qm, Arr qp, Arr q, Arr F, Arr G, HydroConst hc) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
     double ql[NS], qr[NS];
 #pragma unroll
@@ -276,13 +263,43 @@ k_riemann1(Range r, Arr qm, Arr qp, Arr q, Arr F, Arr G, HydroConst hc) {
 // ------------------------------------------------ transdo + Riemann (pairs)
 // For D-face f: the plus state comes from transdo at cell f, the minus state
 // from transdo at cell f-e_D (Godunov_utils_3D.H:22-239 writes qp(iv) and
-// qm(iv+e_D)); both pairs D|T0 and D|T1.
+// qm(iv+e_D)); both pairs D|t.
+template <int D, int T>
+__device__ __forceinline__ void trans1_pair(int i, int j, int k, const double *np,
+                                            const double *nm, const FaceAux &a, int idx,
+                                            const Arr &F, const Arr &G, const Arr &O,
+                                            const Arr &OG, double cdt, const HydroConst &hc) {
+    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+    constexpr int tx = ex_(T, 0), ty = ex_(T, 1), tz = ex_(T, 2);
+    double sp[NS], sm[NS];
+    {
+        double fl[NF], fr[NF], gl[NG], gr[NG];
+        TransDiff t;
+        load_tflux(F, G, i, j, k, fl, gl);
+        load_tflux(F, G, i + tx, j + ty, k + tz, fr, gr);
+        trans_diff_one(fl, fr, gl, gr, cdt, t);
+        transverse_one_d(np, t, cdt, kGamma, hc.trd, hc.small_pres, sp);
+    }
+    {
+        double fl[NF], fr[NF], gl[NG], gr[NG];
+        TransDiff t;
+        load_tflux(F, G, i - ex, j - ey, k - ez, fl, gl);
+        load_tflux(F, G, i - ex + tx, j - ey + ty, k - ez + tz, fr, gr);
+        trans_diff_one(fl, fr, gl, gr, cdt, t);
+        transverse_one_d(nm, t, cdt, kGamma, hc.trd, hc.small_pres, sm);
+    }
+    double f[NF], g[NG];
+    int mut;
+    face_flux(D, idx, hc, sm, sp, a, f, g, &mut);
+    store_tflux(O, OG, i, j, k, f, g);
+}
+
 template <int D>
-__global__ void __launch_bounds__(kBlock)
+__global__ void TRANS_BOUNDS
 k_trans1(Range r, Arr qm, Arr qp, Arr q, Arr Fa, Arr Ga, Arr Fb, Arr Gb,
          Arr Oa, Arr OGa, Arr Ob, Arr OGb, double cda, double cdb, HydroConst hc) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     constexpr int T0 = (D == 0) ? 1 : 0;
     constexpr int T1 = (D == 2) ? 1 : 2;
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
@@ -294,79 +311,65 @@ k_trans1(Range r, Arr qm, Arr qp, Arr q, Arr Fa, Arr Ga, Arr Fb, Arr Gb,
     }
     const FaceAux a = face_aux(q, i - ex, j - ey, k - ez, i, j, k);
     const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-#pragma unroll 1
-    for (int pair = 0; pair < 2; ++pair) {
-        const int t = pair ? T1 : T0;
-        const Arr &F = pair ? Fb : Fa;
-        const Arr &G = pair ? Gb : Ga;
-        const double cdt = pair ? cdb : cda;
-        const int tx = (t == 0), ty = (t == 1), tz = (t == 2);
-        double fl[NF], fr[NF], gl[NG], gr[NG];
-        double sp[NS], sm[NS];
-        load_tflux(F, G, i, j, k, fl, gl);
-        load_tflux(F, G, i + tx, j + ty, k + tz, fr, gr);
-        transverse_one(np, fl, fr, gl, gr, cdt, kGamma, hc.trd, hc.small_pres, sp);
-        load_tflux(F, G, i - ex, j - ey, k - ez, fl, gl);
-        load_tflux(F, G, i - ex + tx, j - ey + ty, k - ez + tz, fr, gr);
-        transverse_one(nm, fl, fr, gl, gr, cdt, kGamma, hc.trd, hc.small_pres, sm);
-        double f[NF], g[NG];
-        int mut;
-        face_flux(D, idx, hc, sm, sp, a, f, g, &mut);
-        store_tflux(pair ? Ob : Oa, pair ? OGb : OGa, i, j, k, f, g);
-    }
+    trans1_pair<D, T0>(i, j, k, np, nm, a, idx, Fa, Ga, Oa, OGa, cda, hc);
+    trans1_pair<D, T1>(i, j, k, np, nm, a, idx, Fb, Gb, Ob, OGb, cdb, hc);
+}
+
+// transdd differences of cell (i,j,k) for direction D: Fa is the flux of
+// direction T0 corrected by T1 (faces of T0), Fb the flux of T1 corrected
+// by T0.
+template <int D>
+__device__ __forceinline__ void trans2_diff(int i, int j, int k, const Arr &Fa,
+                                            const Arr &Ga, const Arr &Fb, const Arr &Gb,
+                                            double ca, double cb, TransDiff &t) {
+    constexpr int T0 = (D == 0) ? 1 : 0;
+    constexpr int T1 = (D == 2) ? 1 : 2;
+    constexpr int ax = ex_(T0, 0), ay = ex_(T0, 1), az = ex_(T0, 2);
+    constexpr int bx = ex_(T1, 0), by = ex_(T1, 1), bz = ex_(T1, 2);
+    double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
+    load_tflux(Fa, Ga, i, j, k, fal, gal);
+    load_tflux(Fa, Ga, i + ax, j + ay, k + az, far_, gar);
+    load_tflux(Fb, Gb, i, j, k, fbl, gbl);
+    load_tflux(Fb, Gb, i + bx, j + by, k + bz, fbr, gbr);
+    trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, kGamma, t);
 }
 
 // ---------------------------------- transdd + final Riemann + adjust_fluxes
 // D-face f: plus state from transdd at cell f, minus state from transdd at
 // cell f-e_D (Godunov_utils_3D.H:245-482); then hydro_cmpflx, then
 // hydro_artif_visc / hydro_norm_spec_flx / hydro_ext_flx
-// (Hydro_adjust_fluxes.cpp:13-44).  Fa: flux of direction T0 corrected by
-// T1; Fb: direction T1 corrected by T0.
+// (Hydro_adjust_fluxes.cpp:13-44).
 template <int D>
-__global__ void __launch_bounds__(kBlock)
+__global__ void TRANS_BOUNDS
 k_trans2(Range r, Arr qm, Arr qp, Arr q, Arr srcq, int has_src, Arr Fa, Arr Ga,
          Arr Fb, Arr Gb, Arr divu, Arr u, Arr flx, Arr G3, double ca, double cb,
          double hdt, double area, HydroConst hc) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    constexpr int T0 = (D == 0) ? 1 : 0;
-    constexpr int T1 = (D == 2) ? 1 : 2;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
-    constexpr int ax = ex_(T0, 0), ay = ex_(T0, 1), az = ex_(T0, 2);
-    constexpr int bx = ex_(T1, 0), by = ex_(T1, 1), bz = ex_(T1, 2);
     double sp[NS], sm[NS];
     {
         double n[NS], src[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) n[s] = qp(i, j, k, s);
-        if (has_src) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) src[s] = srcq(i, j, k, s);
+        for (int s = 0; s < NS; ++s) {
+            n[s] = qp(i, j, k, s);
+            src[s] = has_src ? srcq(i, j, k, s) : 0.0;
         }
-        double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
-        load_tflux(Fa, Ga, i, j, k, fal, gal);
-        load_tflux(Fa, Ga, i + ax, j + ay, k + az, far_, gar);
-        load_tflux(Fb, Gb, i, j, k, fbl, gbl);
-        load_tflux(Fb, Gb, i + bx, j + by, k + bz, fbr, gbr);
-        transverse_two(n, fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, hdt,
-                       has_src ? src : nullptr, kGamma, hc.trd, hc.small_pres, sp);
+        TransDiff t;
+        trans2_diff<D>(i, j, k, Fa, Ga, Fb, Gb, ca, cb, t);
+        transverse_two_d(n, t, hdt, src, hc.trd, hc.small_pres, sp);
     }
     {
         const int i0 = i - ex, j0 = j - ey, k0 = k - ez;
         double n[NS], src[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) n[s] = qm(i, j, k, s);
-        if (has_src) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) src[s] = srcq(i0, j0, k0, s);
+        for (int s = 0; s < NS; ++s) {
+            n[s] = qm(i, j, k, s);
+            src[s] = has_src ? srcq(i0, j0, k0, s) : 0.0;
         }
-        double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
-        load_tflux(Fa, Ga, i0, j0, k0, fal, gal);
-        load_tflux(Fa, Ga, i0 + ax, j0 + ay, k0 + az, far_, gar);
-        load_tflux(Fb, Gb, i0, j0, k0, fbl, gbl);
-        load_tflux(Fb, Gb, i0 + bx, j0 + by, k0 + bz, fbr, gbr);
-        transverse_two(n, fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, hdt,
-                       has_src ? src : nullptr, kGamma, hc.trd, hc.small_pres, sm);
+        TransDiff t;
+        trans2_diff<D>(i0, j0, k0, Fa, Ga, Fb, Gb, ca, cb, t);
+        transverse_two_d(n, t, hdt, src, hc.trd, hc.small_pres, sm);
     }
     const FaceAux a = face_aux(q, i - ex, j - ey, k - ez, i, j, k);
     const int idx = (D == 0) ? i : ((D == 1) ? j : k);
@@ -388,7 +391,7 @@ k_trans2(Range r, Arr qm, Arr qp, Arr q, Arr srcq, int has_src, Arr Fa, Arr Ga,
 __global__ void __launch_bounds__(kBlock)
 k_divu_c(Range r, Arr q, Arr divu, HydroConst hc) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     // compact q has u, v, w in the same slots as QU, QV, QW
     divu(i, j, k, 0) = divu_node(q, i, j, k, hc);
 }
@@ -402,7 +405,7 @@ k_update_c(Range r, Arr snew, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
            double a0, double a1, double a2, double vol, double dt,
            double small_dens, int *flags) {
     int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (!range_point(r, logical_thread(), i, j, k)) return;
     const double pdivu =
         0.5 *
         ((g0(i + 1, j, k, GP) + g0(i, j, k, GP)) * (g0(i + 1, j, k, GUN) * a0 - g0(i, j, k, GUN) * a0) +
@@ -459,7 +462,8 @@ int FusedEngine::reserve(const std::vector<Box> &boxes) {
 
 int FusedEngine::advance_box(const Box &bx, const Arr &sborder, const Arr &srchyd,
                              int has_src, const Arr &snew, const HydroConst &hc,
-                             double dt, hipStream_t st, int *status, Prof *prof) {
+                             double dt, hipStream_t st, int *status, Prof *prof,
+                             bool have_prim) {
     const Box bxg1 = grow(bx, 1), bxg2 = grow(bx, 2), g4 = grow(bx, CAMR_NGROW);
     scratch_.reset();
     auto take = [&](const Box &b, int nc) { return dense(scratch_.take((size_t)nc * b.npts()), b); };
@@ -487,7 +491,8 @@ int FusedEngine::advance_box(const Box &bx, const Arr &sborder, const Arr &srchy
         return CAMR_ERR_ALLOC;
     }
 
-    go("prim", prof, k_prim_c, g4, st, sborder, q, srchyd, sq, has_src, hc, status);
+    if (!have_prim || has_src)
+        go("prim", prof, k_prim_c, g4, st, sborder, q, srchyd, sq, has_src, hc, status);
     go("flatten", prof, k_flat_c, bxg2, st, q, g4, flat, hc.use_flattening);
     const double dtdx[3] = {dt / hc.dx[0], dt / hc.dx[1], dt / hc.dx[2]};
     if (hc.ppm_type == 1) {

@@ -520,33 +520,49 @@ CAMR_HD void ppm_project(int side, const PPMWaves &I, double un, double cc,
 }
 
 // ---- transverse corrections (Godunov_utils_3D.H) ---------------------------
-// hydro_transdo (Godunov_utils_3D.H:22-239), non-EB.  fl/fr: compact fluxes
-// on the two t-faces bounding the cell (iv and iv+e_t); gl/gr: their GD
-// (normal velocity, pressure).  qn: normal-predictor state in compact layout;
-// `right` selects the qp update (state at iv) vs the qm update (state at
-// iv+e_d) -- both use the same flux differences.
-CAMR_HD void transverse_one(const double *qn, const double *fl, const double *fr,
-                            const double *gl, const double *gr, double cdtdx,
-                            double gamc, int trd, double small_pres,
-                            double *qo) {
-    const double flxrho = cdtdx * (fr[FR] - fl[FR]);
-    const double flxu = cdtdx * (fr[FMX] - fl[FMX]);
-    const double flxv = cdtdx * (fr[FMY] - fl[FMY]);
-    const double flxw = cdtdx * (fr[FMZ] - fl[FMZ]);
-    const double flxe = cdtdx * (fr[FE] - fl[FE]);
+// Conservative flux differences across a cell in the transverse direction(s)
+// (already multiplied by the dt/dx factors) and the p*u work terms: the
+// only inputs the transverse predictors need from the neighbouring fluxes.
+struct TransDiff {
+    double rho, mu, mv, mw, e, y[NSPEC];
+    double dAup, pav, dAu;  // transdo: p u difference, mean p, u difference
+    double pnew;            // transdd: summed pressure correction
+};
+
+// Differences of one transverse direction for hydro_transdo
+// (Godunov_utils_3D.H:74-85, 112-122): fl/fr are the compact fluxes on the
+// faces iv and iv+e_t, gl/gr their Godunov (normal velocity, pressure).
+CAMR_HD void trans_diff_one(const double *fl, const double *fr, const double *gl,
+                            const double *gr, double cdtdx, TransDiff &t) {
+    t.rho = cdtdx * (fr[FR] - fl[FR]);
+    t.mu = cdtdx * (fr[FMX] - fl[FMX]);
+    t.mv = cdtdx * (fr[FMY] - fl[FMY]);
+    t.mw = cdtdx * (fr[FMZ] - fl[FMZ]);
+    t.e = cdtdx * (fr[FE] - fl[FE]);
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) t.y[s] = cdtdx * (fr[FY + s] - fl[FY + s]);
+    const double pggp = gr[GP], pggm = gl[GP];
+    const double ugp = gr[GUN], ugm = gl[GUN];
+    t.dAup = pggp * ugp - pggm * ugm;
+    t.pav = 0.5 * (pggp + pggm);
+    t.dAu = ugp - ugm;
+}
+
+// hydro_transdo (Godunov_utils_3D.H:22-239), non-EB, for one side: qn is
+// the normal-predictor state (qnormp(iv) for the qp update, qnormm(iv+e_d)
+// for the qm update); both sides use the differences of cell iv.
+CAMR_HD void transverse_one_d(const double *qn, const TransDiff &t, double cdtdx,
+                              double gamc, int trd, double small_pres,
+                              double *qo) {
+    const double flxrho = t.rho, flxu = t.mu, flxv = t.mv, flxw = t.mw, flxe = t.e;
     const double rr = qn[SR];
 #pragma unroll
     for (int s = 0; s < NSPEC; ++s) {
-        const double compn = cdtdx * (fr[FY + s] - fl[FY + s]);
         const double rrnew = rr - flxrho;
-        const double compo = rr * qn[SY + s] - compn;
+        const double compo = rr * qn[SY + s] - t.y[s];
         qo[SY + s] = compo / rrnew;
     }
-    const double pggp = gr[GP], pggm = gl[GP];
-    const double ugp = gr[GUN], ugm = gl[GUN];
-    const double dAup = pggp * ugp - pggm * ugm;
-    const double pav = 0.5 * (pggp + pggm);
-    const double dAu = ugp - ugm;
+    const double dAup = t.dAup, pav = t.pav, dAu = t.dAu;
 
     double ru = qn[SU], rv = qn[SV], rw = qn[SW];
     const double ekin = 0.5 * rr * (ru * ru + rv * rv + rw * rw);
@@ -578,37 +594,30 @@ CAMR_HD void transverse_one(const double *qn, const double *fl, const double *fr
     }
 }
 
-// hydro_transdd (Godunov_utils_3D.H:245-482), non-EB.  Corrections from the
-// two transverse directions a (fluxes fa_l/fa_r, GD ga_l/ga_r, hdt/dx_a) and
-// b, plus hdt * source.  src in compact layout (NULL = identically +0).
-CAMR_HD void transverse_two(const double *qn, const double *fal, const double *far_,
-                            const double *gal, const double *gar,
-                            const double *fbl, const double *fbr,
-                            const double *gbl, const double *gbr, double ca,
-                            double cb, double hdt, const double *src,
+CAMR_HD void transverse_one(const double *qn, const double *fl, const double *fr,
+                            const double *gl, const double *gr, double cdtdx,
                             double gamc, int trd, double small_pres,
                             double *qo) {
+    TransDiff t;
+    trans_diff_one(fl, fr, gl, gr, cdtdx, t);
+    transverse_one_d(qn, t, cdtdx, gamc, trd, small_pres, qo);
+}
+
+// Differences of the two transverse directions a (fluxes fal/far_, GD
+// gal/gar, factor ca = hdt/dx_a) and b for hydro_transdd
+// (Godunov_utils_3D.H:307-316, 362-384).
+CAMR_HD void trans_diff_two(const double *fal, const double *far_, const double *gal,
+                            const double *gar, const double *fbl, const double *fbr,
+                            const double *gbl, const double *gbr, double ca,
+                            double cb, double gamc, TransDiff &t) {
 #define CAMR_TF(c) (ca * (far_[c] - fal[c]) + cb * (fbr[c] - fbl[c]))
-    const double flxrho = CAMR_TF(FR);
-    const double flxu = CAMR_TF(FMX);
-    const double flxv = CAMR_TF(FMY);
-    const double flxw = CAMR_TF(FMZ);
-    const double flxe = CAMR_TF(FE);
-    const double s_r = src ? src[SR] : 0.0;
-    const double s_u = src ? src[SU] : 0.0;
-    const double s_v = src ? src[SV] : 0.0;
-    const double s_w = src ? src[SW] : 0.0;
-    const double s_e = src ? src[SE] : 0.0;
-    const double s_p = src ? src[SP] : 0.0;
-    const double rr = qn[SR];
-    double rrnew = rr - flxrho;
+    t.rho = CAMR_TF(FR);
+    t.mu = CAMR_TF(FMX);
+    t.mv = CAMR_TF(FMY);
+    t.mw = CAMR_TF(FMZ);
+    t.e = CAMR_TF(FE);
 #pragma unroll
-    for (int s = 0; s < NSPEC; ++s) {
-        const double srcpass = src ? src[SY + s] : 0.0;
-        const double compn = CAMR_TF(FY + s);
-        const double compo = rr * qn[SY + s] - compn;
-        qo[SY + s] = compo / rrnew + hdt * srcpass;
-    }
+    for (int s = 0; s < NSPEC; ++s) t.y[s] = CAMR_TF(FY + s);
 #undef CAMR_TF
     const double dAupx = gar[GP] * gar[GUN] - gal[GP] * gal[GUN];
     const double pavx = 0.5 * (gar[GP] + gal[GP]);
@@ -618,6 +627,27 @@ CAMR_HD void transverse_two(const double *qn, const double *fal, const double *f
     const double dAuy = gbr[GUN] - gbl[GUN];
     const double pxnew = ca * (dAupx + pavx * dAux * (gamc - 1.0));
     const double pynew = cb * (dAupy + pavy * dAuy * (gamc - 1.0));
+    // kept separate so that qn[SP] - pxnew - pynew keeps the reference order
+    t.dAup = pxnew;
+    t.pnew = pynew;
+}
+
+// hydro_transdd (Godunov_utils_3D.H:245-482), non-EB, for one side, with
+// hdt * source (src in compact layout; all +0 when the sources vanish).
+CAMR_HD void transverse_two_d(const double *qn, const TransDiff &t, double hdt,
+                              const double *src, int trd, double small_pres,
+                              double *qo) {
+    const double flxrho = t.rho, flxu = t.mu, flxv = t.mv, flxw = t.mw, flxe = t.e;
+    const double s_r = src[SR], s_u = src[SU], s_v = src[SV], s_w = src[SW];
+    const double s_e = src[SE], s_p = src[SP];
+    const double rr = qn[SR];
+    double rrnew = rr - flxrho;
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) {
+        const double compo = rr * qn[SY + s] - t.y[s];
+        qo[SY + s] = compo / rrnew + hdt * src[SY + s];
+    }
+    const double pxnew = t.dAup, pynew = t.pnew;
 
     const double ru = rr * qn[SU];
     const double rv = rr * qn[SV];
@@ -645,6 +675,21 @@ CAMR_HD void transverse_two(const double *qn, const double *fal, const double *f
     else
         pn = qn[SP] + hdt * s_p;
     qo[SP] = vmax(pn, small_pres);
+}
+
+CAMR_HD void transverse_two(const double *qn, const double *fal, const double *far_,
+                            const double *gal, const double *gar,
+                            const double *fbl, const double *fbr,
+                            const double *gbl, const double *gbr, double ca,
+                            double cb, double hdt, const double *src,
+                            double gamc, int trd, double small_pres,
+                            double *qo) {
+    TransDiff t;
+    trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, gamc, t);
+    double z[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) z[s] = src ? src[s] : 0.0;
+    transverse_two_d(qn, t, hdt, z, trd, small_pres, qo);
 }
 
 } // namespace camr

@@ -38,6 +38,8 @@ __global__ void k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g,
                              double small_dens, int *flags);
 __global__ void k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g);
 __global__ void k_fill_value(Range r, Arr a, int nc, double v);
+__global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v);
+inline Box make_box1() { Box b; for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = 0; } return b; }
 
 template <typename K, typename... A>
 inline void launch_on(K kern, const Box &b, hipStream_t s, A... args) {
