@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 cell-updates/s of CAMR's single-level Godunov PPM advance
+on a 256^3 periodic grid (one MI355X).
+
+A step is one full level step as CAMR takes it: computeNewDt (CFL min
+reduction + host read-back), then CAMR_advance (clean_state, ghost fill,
+ctoprim, PPM CTU hydro, flux divergence and update, clean_state).  Inputs
+are resident in HBM before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6       # MI355X FP64 vector, TFLOP/s
+NVAR = 10
+
+# Kernel names of the profiler regions and the per-launch algorithmic HBM
+# bytes of each, per unit (face or cell) it processes: every array element
+# the kernel must read or write, counted once (neighbour re-reads excluded).
+#   transverse_2 (one launch per direction d): per d-face: normal states qm, qp (2 x 9), the two
+#     corrected transverse fluxes + Godunov (u, p) of its cell (2 x 10),
+#     sound speed / floor (2), conserved state of the cell (10, artificial
+#     viscosity), one divu node; writes the NVAR flux and Godunov (u, p)
+#     = 18 + 20 + 2 + 10 + 1 + 12 = 63 doubles.
+#   transverse_1: per d-face: qm, qp (18), two transverse fluxes +
+#     Godunov of its cell (2 x 10), sound speed / floor (2); writes two
+#     compact fluxes + Godunov (2 x 10) = 60 doubles.
+#   trace_ppm: per cell: compact primitives (9 + sound speed) and flat (1);
+#     writes the two face states (2 x 9) = 29 doubles.
+#   riemann_normal: per face: qm, qp (18), sound speed / floor of both
+#     cells (2); writes the compact flux + Godunov (10) = 30 doubles.
+ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
+                "riemann_normal": 30}
+
+
+def unit_count(name: str, n: int) -> int:
+    """Faces or cells one launch of kernel `name` covers for an n^3 box."""
+    if name.startswith("transverse_2"):
+        return (n + 1) * n * n
+    if name.startswith("transverse_1"):
+        return (n + 3) * (n + 2) * (n + 2)
+    if name.startswith("trace_ppm"):
+        return (n + 4) ** 3
+    if name.startswith("riemann_normal"):
+        return (n + 3) * (n + 4) * (n + 4)
+    return n ** 3
+
+
+def cpu_baseline(n: int, cfl: float, U0, nsteps: int = 2):
+    """`nsteps` steps of the same workload with the CPU oracle (the
+    reference's stages restated in C, OpenMP over hydro tiles), on this
+    host's cores."""
+    from oracle import oracle as O
+    O.build()
+    from camr_amd import workloads
+    prob = workloads.smooth_periodic(n=n, ppm_type=1, cfl=cfl)
+    p = O.params(ppm_type=1, cfl=cfl, bclo=prob.bclo, bchi=prob.bchi,
+                 domlo=prob.domlo, domhi=prob.domhi, is_periodic=prob.is_periodic,
+                 dx=prob.dx)
+    L = O.Level(p, prob.boxes)
+    L.set_box(0, U0)
+    L.init_post()
+    dt = L.initial_dt()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        err = L.advance(dt)
+        assert err == 0
+        dt = L.new_dt(dt)  # the dt reduction belongs to the step
+    el = time.perf_counter() - t0
+    L.close()
+    return {"value": n ** 3 * nsteps / el, "unit": "cell-updates/s",
+            "cores": O.lib().ora_get_threads(), "kind": "port",
+            "sample": f"{nsteps} full steps (advance + new dt) of the same "
+                      f"{n}^3 PPM workload from the same initial state, "
+                      f"{el:.1f} s wall on {O.lib().ora_get_threads()} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: multi-GPU runs are not part of "
+              "this build (the level shards by boxes; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from camr_amd import _lib, workloads
+    from camr_amd.hydro import Level
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, cfl = args.n, 0.4
+    prob = workloads.smooth_periodic(n=n, ppm_type=1, cfl=cfl)
+    U0 = workloads.smooth_init(prob, 0)
+    eng = _lib.ENGINE_FUSED if args.engine == "fused" else _lib.ENGINE_STAGED
+    L = Level(prob, engine=eng)
+    L.set_box(0, U0)
+    L.init_post()
+
+    # warm-up steps follow the reference dt sequence
+    dt = L.initial_dt()
+    L.advance(dt)
+    for _ in range(max(args.warmup - 1, 0)):
+        dt = L.new_dt(dt)
+        L.advance(dt)
+    L.sync()
+
+    # find the dominant kernel over one profiled step
+    L.profile(True)
+    dt = L.new_dt(dt)
+    L.advance(dt)
+    L.sync()
+    rep = L.profile_report()
+    kern = max((k for k in rep if k != "advance"), key=lambda k: rep[k]["ms"])
+    L.profile(True, kern)  # time only that kernel during the timed region
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = L.new_dt(dt)
+        L.advance(dt)
+    L.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rep = L.profile_report()[kern]
+    L.profile(False)
+
+    cups = n ** 3 * args.steps / el
+    avg_ms = rep["ms"] / rep["count"]
+    algo = ALGO_DOUBLES.get(kern, 2 * NVAR) * 8 * unit_count(kern, n)
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "FP64 cell-updates/s at 256³ PPM; % HBM peak; adjoint/forward cost ratio",
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded smooth periodic field (rho = 1 + 0.1 x 8 sines, "
+                "k^-2 Gaussian velocity at rms Mach 0.5, p = 1)",
+        "config": {"workload": f"{n}^3 periodic single-level Godunov PPM advance "
+                               f"(CAMR_advance + computeNewDt), CFL {cfl}",
+                   "cells": n ** 3, "ppm_type": 1, "cfl": cfl,
+                   "engine": args.engine},
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "launches": rep["count"]},
+        "step_hbm": {"bytes_per_cell_update": 2 * NVAR * 8,
+                     "achieved_GBs": cups * 2 * NVAR * 8 / 1e9,
+                     "frac": cups * 2 * NVAR * 8 / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(n, cfl, U0)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
