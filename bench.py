@@ -40,6 +40,12 @@ NVAR = 10
 ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
                 "riemann_normal": 30}
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# for the fused engine at 256^3 PPM (profiles/hbm_traffic_fused_256cube_ppm.md,
+# FETCH_SIZE scaled by 1.9692, WRITE_SIZE by 1.0, the factors an 8-byte-lane
+# copy of 1 GiB reads back).  Reported only for that engine and size.
+MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15013282837}
+
 
 def unit_count(name: str, n: int) -> int:
     """Faces or cells one launch of kernel `name` covers for an n^3 box."""
@@ -91,11 +97,15 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run one elementwise copy "
+                         "(torch.neg) of MIB MiB of float64: a kernel of known "
+                         "HBM traffic to scale hardware byte counters with")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X: multi-GPU runs are not part of "
-              "this build (the level shards by boxes; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one MI355X: runs on several GPUs are not "
+              "supported (the level would shard by boxes, with one halo "
+              "exchange and one dt all-reduce per step)", file=sys.stderr)
         return 2
 
     import torch
@@ -138,6 +148,15 @@ def main() -> int:
     el = time.perf_counter() - t0
     rep = L.profile_report()[kern]
     L.profile(False)
+    if args.calib_copy > 0:
+        # aligned: 32-byte vector loads per lane; offset by one element:
+        # 8-byte loads per lane, the access pattern of the hydro kernels
+        src = torch.ones(args.calib_copy << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.neg(src[1:], out=dst[1:])
+        torch.cuda.synchronize()
+        del src, dst
 
     cups = n ** 3 * args.steps / el
     avg_ms = rep["ms"] / rep["count"]
@@ -163,7 +182,8 @@ def main() -> int:
                    "engine": args.engine},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.engine, n, kern)),
                      "avg_launch_ms": avg_ms, "launches": rep["count"]},
         "step_hbm": {"bytes_per_cell_update": 2 * NVAR * 8,
                      "achieved_GBs": cups * 2 * NVAR * 8 / 1e9,
