@@ -61,8 +61,10 @@ class LevelDesc(C.Structure):
         ("init_shrink", C.c_double), ("max_dt", C.c_double),
         ("fixed_dt", C.c_double), ("stop_time", C.c_double),
         ("nbox", C.c_int32), ("box_lohi", C.POINTER(C.c_int32)),
-        ("engine", C.c_int32),
+        ("engine", C.c_int32), ("batched", C.c_int32),
     ]
+
+NSTAGE = 13  # CAMR_NSTAGE: stage ranges of the fused engine
 
 
 # Every extern "C" symbol declared in include/camr_hydro.h.
@@ -75,6 +77,7 @@ EXPORTS = [
     "camr_level_init_post", "camr_level_initial_dt", "camr_level_new_dt",
     "camr_level_advance", "camr_level_run", "camr_level_sync",
     "camr_level_time", "camr_level_profile", "camr_level_profile_report",
+    "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
 ]
 
 _lib = None
@@ -122,6 +125,9 @@ def lib():
         L.camr_level_time.restype = C.c_double
         L.camr_level_profile.argtypes = [vp, C.c_int32, C.c_char_p]
         L.camr_level_profile_report.argtypes = [vp, C.c_char_p, C.c_int64]
+        L.camr_level_set_exterior.argtypes = [vp, dp]
+        L.camr_level_set_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
+        L.camr_level_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default restype

@@ -33,13 +33,13 @@ inline Box make_box(const int32_t lo[3], const int32_t hi[3]) {
     for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
     return b;
 }
-inline Box grow(Box b, int n) {
+__host__ __device__ inline Box grow(Box b, int n) {
     for (int d = 0; d < 3; ++d) { b.lo[d] -= n; b.hi[d] += n; }
     return b;
 }
-inline Box grow_dir(Box b, int d, int n) { b.lo[d] -= n; b.hi[d] += n; return b; }
-inline Box grow_hi(Box b, int d, int n) { b.hi[d] += n; return b; }
-inline Box nodes(Box b, int d) { b.hi[d] += 1; return b; }
+__host__ __device__ inline Box grow_dir(Box b, int d, int n) { b.lo[d] -= n; b.hi[d] += n; return b; }
+__host__ __device__ inline Box grow_hi(Box b, int d, int n) { b.hi[d] += n; return b; }
+__host__ __device__ inline Box nodes(Box b, int d) { b.hi[d] += 1; return b; }
 
 // Device view with Array4 indexing.
 struct Arr {
@@ -81,7 +81,7 @@ struct Range {
     int lo0, lo1, lo2, n0, n1;
     int64_t n;
 };
-inline Range range_of(const Box &b) {
+__host__ __device__ inline Range range_of(const Box &b) {
     Range r;
     r.lo0 = b.lo[0]; r.lo1 = b.lo[1]; r.lo2 = b.lo[2];
     r.n0 = b.len(0); r.n1 = b.len(1);

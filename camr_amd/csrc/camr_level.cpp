@@ -163,10 +163,56 @@ class HydroLevel {
     bool fused_ = true;
     FusedEngine engine_;
     bool pending_ = false;   // a fused step awaits resolve()
+    bool needs_ext_ = false; // some ghost cell has no owning box
     bool dt_ready_ = false;  // dtmin_ holds the CFL estimate of S_new
     double last_dt_ = 0.0;
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
+    double *dt_dev_ = nullptr;  // the step's dt, read by the kernels
+    double *ext_ = nullptr;     // exterior data (whole domain), when set
+    Arr extv_{};
+    bool batched_ = true;       // one launch per stage over all boxes
+    int parity() const { return a_is_new_ ? 0 : 1; }
+    void set_dt_dev(double dt);
+
+  public:
+    int set_exterior(const double *h);
+    int set_tuning(const Tuning &t);
+    int get_tuning(Tuning *t) const;
+    bool fused() const { return fused_; }
 };
+
+void HydroLevel::set_dt_dev(double dt) {
+    unsigned long long bits;
+    memcpy(&bits, &dt, sizeof(bits));
+    launch_on(k_fill_u64, make_box1(), stream_, (unsigned long long *)dt_dev_, bits);
+}
+
+// Exterior data: values of the cells of the domain that no box owns, used
+// as the FillPatch result for ghost cells there (coarse-fine ghosts of a
+// refined level).  `h` holds NVAR components over the whole domain.
+int HydroLevel::set_exterior(const double *h) {
+    if (!h) { set_error("null exterior data"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
+    const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
+    if (!ext_) CAMR_HIP(hipMalloc(&ext_, NVAR * dom.npts() * sizeof(double)));
+    CAMR_HIP(hipMemcpy(ext_, h, NVAR * dom.npts() * sizeof(double), hipMemcpyHostToDevice));
+    extv_ = dense(ext_, dom);
+    return CAMR_OK;
+}
+
+int HydroLevel::set_tuning(const Tuning &t) {
+    if (!fused_) { set_error("tuning applies to the fused engine"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    return engine_.set_tuning(t);
+}
+int HydroLevel::get_tuning(Tuning *t) const {
+    if (!fused_) { set_error("tuning applies to the fused engine"); return CAMR_ERR_PARAM; }
+    *t = engine_.tuning();
+    return CAMR_OK;
+}
 
 void HydroLevel::release() {
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
@@ -178,6 +224,7 @@ void HydroLevel::release() {
     bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear();
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_);
     tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
+    fr(dt_dev_); fr(ext_); dt_dev_ = nullptr; ext_ = nullptr;
     fr(q_); fr(qaux_); fr(srcq_);
     for (auto &f : flx_) { fr(f); f = nullptr; }
     if (h_status_) (void)hipHostFree(h_status_);
@@ -274,6 +321,36 @@ int HydroLevel::create() {
     }
     CAMR_HIP(hipMalloc(&owner_, ncell * sizeof(int)));
     CAMR_HIP(hipMemcpy(owner_, own.data(), ncell * sizeof(int), hipMemcpyHostToDevice));
+    // ghost cells whose FillPatch source no box of the level owns (the
+    // coarse-fine ghosts of a refined level) need exterior data
+    needs_ext_ = false;
+    for (int b = 0; b < nb && !needs_ext_; ++b) {
+        const Box G = grow(box_[b], CAMR_NGROW);
+        for (int k = G.lo[2]; k <= G.hi[2] && !needs_ext_; ++k)
+            for (int j = G.lo[1]; j <= G.hi[1] && !needs_ext_; ++j)
+                for (int i = G.lo[0]; i <= G.hi[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    int m[3];
+                    for (int d = 0; d < 3; ++d) {
+                        const int lo = dom.lo[d], hi = dom.hi[d], n = hi - lo + 1;
+                        int x = c[d];
+                        if (x < lo || x > hi) {
+                            if (desc_.is_periodic[d]) x = lo + (((x - lo) % n) + n) % n;
+                            else {
+                                const int bc = x < lo ? desc_.hp.bclo[d] : desc_.hp.bchi[d];
+                                const bool copy = bc == CAMR_BC_OUTFLOW || bc == CAMR_BC_INFLOW;
+                                x = copy ? (x < lo ? lo : hi) : (x < lo ? 2 * lo - x - 1 : 2 * hi - x + 1);
+                            }
+                        }
+                        m[d] = x;
+                    }
+                    if (own[(m[0] - dom.lo[0]) + (m[1] - dom.lo[1]) * n0 +
+                            (int64_t)(m[2] - dom.lo[2]) * n0 * n1] < 0) {
+                        needs_ext_ = true;
+                        break;
+                    }
+                }
+    }
     CAMR_HIP(hipMalloc(&tab_old_, nb * sizeof(DevFab)));
     CAMR_HIP(hipMalloc(&tab_a_, nb * sizeof(DevFab)));
     CAMR_HIP(hipMalloc(&tab_b_, nb * sizeof(DevFab)));
@@ -299,16 +376,25 @@ int HydroLevel::create() {
     }
     CAMR_HIP(hipMalloc(&status_, sizeof(int)));
     CAMR_HIP(hipMalloc(&dtmin_, sizeof(unsigned long long)));
+    CAMR_HIP(hipMalloc(&dt_dev_, sizeof(double)));
     CAMR_HIP(hipMemset(status_, 0, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_status_, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_dtmin_, sizeof(unsigned long long)));
     fused_ = (desc_.engine != 1);
+    batched_ = desc_.batched != 0;
     if (fused_) {
-        if ((rc = engine_.reserve(box_))) {
-            set_error("fused-engine scratch allocation failed");
-            return rc;
+        std::vector<Arr> va(nb), vb(nb);
+        for (int b = 0; b < nb; ++b) {
+            va[b] = dense(bufA_[b], box_[b]);
+            vb[b] = dense(bufB_[b], box_[b]);
         }
-    } else {
+        rc = engine_.setup(box_, va, vb, default_tuning());
+        // boxes beyond the fused engine's 32-bit layout run on the staged
+        // engine unless the fused one was asked for explicitly
+        if (rc == CAMR_ERR_SHAPE && desc_.engine == 0) fused_ = false;
+        else if (rc) return rc;
+    }
+    if (!fused_) {
         if ((rc = alloc(&q_, QVAR * max_g4))) return rc;
         if ((rc = alloc(&qaux_, NQAUX * max_g4))) return rc;
         if ((rc = alloc(&srcq_, QVAR * max_g4))) return rc;
@@ -407,7 +493,8 @@ int HydroLevel::expand_state() {
     CAMR_HIP(hipMemcpyAsync(tab_old_, t.data(), t.size() * sizeof(DevFab), hipMemcpyHostToDevice, stream_));
     for (size_t b = 0; b < box_.size(); ++b) {
         const Box G = grow(box_[b], CAMR_NGROW);
-        launch_on(k_fill_raw, G, stream_, sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_, status_);
+        launch_on(k_fill_raw, G, stream_, sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_,
+                  extv_, ext_ ? 1 : 0, status_);
     }
     for (size_t b = 0; b < box_.size(); ++b) {
         launch_on(k_detect, box_[b], stream_, sbv_[b], hc_.small_dens, status_);
@@ -438,11 +525,14 @@ int HydroLevel::construct_hydro_source(double dt) {
                       (const int *)owner_, dom_);
     }
     if (fused_) {
+        const int has_src = grav_src_values_ ? 1 : 0;
+        const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
+        if (batched_)
+            return engine_.hydro(-1, parity(), false, tab_sb_, src, has_src, hc_, dt_dev_,
+                                 stream_, status_, &prof_);
         for (size_t b = 0; b < box_.size(); ++b) {
-            const Box G = grow(box_[b], CAMR_NGROW);
-            Arr src = grav_src_values_ ? dense(srchyd_[b], G) : sbv_[b];
-            int rc = engine_.advance_box(box_[b], sbv_[b], src, grav_src_values_ ? 1 : 0,
-                                         snew_[b], hc_, dt, stream_, status_, &prof_);
+            int rc = engine_.hydro((int)b, parity(), false, tab_sb_, src, has_src, hc_, dt_dev_,
+                                   stream_, status_, &prof_);
             if (rc) return rc;
         }
         return CAMR_OK;
@@ -489,12 +579,17 @@ int HydroLevel::construct_hydro_source(double dt) {
 
 int HydroLevel::advance(double dt) {
     if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    if (needs_ext_ && !ext_) {
+        set_error("ghost cells outside every box of the level: set exterior data");
+        return CAMR_ERR_SHAPE;
+    }
     int rc = resolve();
     if (rc) return rc;
     Region step(&prof_, "advance", stream_);
     // swapTimeLevels (CAMR_advance.cpp:72-75)
     std::swap(snew_, sold_);
     a_is_new_ = !a_is_new_;
+    set_dt_dev(dt);
     if (fused_ && !grav_src_values_) {
         rc = advance_fused(dt);
         pending_ = true;
@@ -543,36 +638,33 @@ int HydroLevel::advance_exact(double dt) {
 // density floor fires, which resolve() detects and answers with
 // advance_exact from the untouched raw S_old.
 int HydroLevel::advance_fused(double dt) {
+    (void)dt;
     const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
     launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
-    const DevFab *tab = a_is_new_ ? tab_b_ : tab_a_;  // table of the buffers now old
-    for (size_t b = 0; b < box_.size(); ++b) {
-        FillArgs fa;
-        fa.valid = box_[b];
-        fa.sborder = sbv_[b];
-        fa.q = engine_.prim_view(box_[b]);
-        fa.snew = snew_[b];
-        fa.sold_tab = tab;
-        fa.owner = owner_;
-        fa.dom = dom_;
-        fa.hc = hc_;
-        fa.dt = dt;
-        fa.g = desc_.const_grav;
-        fa.add_grav = desc_.add_grav_src;
-        launch_fill_fused(fa, stream_, status_, &prof_);
-        int rc = engine_.advance_box(box_[b], sbv_[b], sbv_[b], 0, snew_[b], hc_, dt, stream_,
-                                     status_, &prof_, true);
+    FillArgs fa;
+    fa.sold_tab = a_is_new_ ? tab_b_ : tab_a_;  // table of the buffers now old
+    fa.owner = owner_;
+    fa.ext = extv_;
+    fa.has_ext = ext_ ? 1 : 0;
+    fa.dom = dom_;
+    fa.hc = hc_;
+    fa.dtp = dt_dev_;
+    fa.g = desc_.const_grav;
+    fa.add_grav = desc_.add_grav_src;
+    PostArgs pa;
+    pa.hc = hc_;
+    pa.dtp = dt_dev_;
+    pa.g = desc_.const_grav;
+    pa.add_grav = desc_.add_grav_src;
+    pa.dtmin = dtmin_;
+    const int nb = batched_ ? 1 : (int)box_.size();
+    for (int i = 0; i < nb; ++i) {
+        const int b = batched_ ? -1 : i;
+        int rc = engine_.fill(b, parity(), fa, stream_, status_, &prof_);
+        if (!rc) rc = engine_.hydro(b, parity(), true, nullptr, nullptr, 0, hc_, dt_dev_, stream_,
+                                    status_, &prof_);
+        if (!rc) rc = engine_.post(b, parity(), pa, stream_, status_, &prof_);
         if (rc) return rc;
-        PostArgs pa;
-        pa.valid = box_[b];
-        pa.snew = snew_[b];
-        pa.sold = sold_[b];
-        pa.hc = hc_;
-        pa.dt = dt;
-        pa.g = desc_.const_grav;
-        pa.add_grav = desc_.add_grav_src;
-        pa.dtmin = dtmin_;
-        launch_post_fused(pa, stream_, status_, &prof_);
     }
     dt_ready_ = true;
     return CAMR_OK;
@@ -700,6 +792,7 @@ void camr_default_level_desc(camr_level_desc *d) {
     d->fixed_dt = -1.0;
     d->stop_time = -1.0;
     d->engine = 0;
+    d->batched = 1;
 }
 
 const char *camr_last_error(void) { return g_err; }
@@ -868,6 +961,26 @@ int camr_level_run(camr_level *L, int32_t nsteps, double *dts) {
     return L->impl.sync();
 }
 int camr_level_sync(camr_level *L) { return L ? L->impl.sync() : CAMR_ERR_PARAM; }
+int camr_level_set_exterior(camr_level *L, const double *host) {
+    return L ? L->impl.set_exterior(host) : CAMR_ERR_PARAM;
+}
+int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order) {
+    if (!L || !block) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    static_assert(CAMR_NSTAGE == ST_COUNT, "stage kinds");
+    Tuning t;
+    for (int k = 0; k < ST_COUNT; ++k) t.block[k] = block[k];
+    t.order = order;
+    return L->impl.set_tuning(t);
+}
+int camr_level_get_tuning(camr_level *L, int32_t block[CAMR_NSTAGE], int32_t *order) {
+    if (!L || !block || !order) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) block[k] = t.block[k];
+    *order = t.order;
+    return CAMR_OK;
+}
 int camr_level_profile(camr_level *L, int32_t enable, const char *kernel_filter) {
     if (!L) return CAMR_ERR_PARAM;
     L->impl.prof().enable(enable != 0, kernel_filter);

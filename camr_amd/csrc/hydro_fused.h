@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "canon.h"
 #include "hydro_state.h"
 
 namespace camr {
@@ -12,14 +13,13 @@ namespace camr {
 enum { SC = NS, SCS = NS + 1, NQC = NS + 2 };
 
 // hydro_ctoprim (Source/Hydro/Hydro_ctoprim.H:11-101) of the conserved
-// state v (NVAR values) into the compact primitives at (i,j,k); returns the
-// specific internal energy e (needed by hydro_srctoprim).  Cells with
-// rho <= 0 are left untouched as in the reference.
-__device__ __forceinline__ double prim_compact_point(const double *v, const Arr &q,
-                                                     int i, int j, int k,
-                                                     const HydroConst &hc, int *status) {
+// state v (NVAR values) into the NQC compact primitives o; e receives the
+// specific internal energy (needed by hydro_srctoprim).  Returns false and
+// leaves o untouched for rho <= 0, as the reference leaves such cells.
+__device__ __forceinline__ bool prim_compact(const double *v, double *o, double &e,
+                                             const HydroConst &hc, int *status) {
     const double rho = v[URHO];
-    if (!(rho > 0.)) return 0.0;
+    if (!(rho > 0.)) return false;
     const double rhoinv = 1.0 / rho;
     const double vx = v[UMX] * rhoinv;
     const double vy = v[UMY] * rhoinv;
@@ -29,66 +29,233 @@ __device__ __forceinline__ double prim_compact_point(const double *v, const Arr 
     const double ueint = v[UEINT];
     if (!hc.allow_negative_energy && !(ueden > 0. && ueint > 0.))
         atomicOr(status, 1 << CAMR_ERR_NEG_ENERGY);
-    const double e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv
-                                                         : ueint * rhoinv;
+    e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv : ueint * rhoinv;
     const double p = (kGamma - 1.0) * rho * e;
     const double cs = sqrt(kGamma * p / rho);
-    q(i, j, k, SR) = rho;
-    q(i, j, k, SU) = vx;
-    q(i, j, k, SV) = vy;
-    q(i, j, k, SW) = vz;
-    q(i, j, k, SP) = p;
-    q(i, j, k, SE) = e * rho;
+    o[SR] = rho;
+    o[SU] = vx;
+    o[SV] = vy;
+    o[SW] = vz;
+    o[SP] = p;
+    o[SE] = e * rho;
 #pragma unroll
-    for (int s = 0; s < NSPEC; ++s) q(i, j, k, SY + s) = v[UFS + s] * rhoinv;
-    q(i, j, k, SC) = cs;
-    q(i, j, k, SCS) = vmax(hc.small_num, hc.small_num * cs);
-    return e;
+    for (int s = 0; s < NSPEC; ++s) o[SY + s] = v[UFS + s] * rhoinv;
+    o[SC] = cs;
+    o[SCS] = vmax(hc.small_num, hc.small_num * cs);
+    return true;
+}
+
+// The same, stored at (i,j,k) of q.
+__device__ __forceinline__ void prim_compact_point(const double *v, const Arr &q, int i,
+                                                   int j, int k, const HydroConst &hc,
+                                                   int *status) {
+    double o[NQC], e;
+    if (!prim_compact(v, o, e, hc, status)) return;
+#pragma unroll
+    for (int n = 0; n < NQC; ++n) q(i, j, k, n) = o[n];
+}
+
+// ------------------------------------------------------------ box batches
+// The engine runs every stage either over all boxes of the level in one
+// launch, or over one box per launch; both use the same kernels.  A block
+// finds its box by a binary search of the stage's block prefix table.
+
+// Index ranges of the stages (one prefix table per kind).
+enum StageKind {
+    ST_G4 = 0,        // grow(bx, 4): ghost fill, primitives
+    ST_G2 = 1,        // grow(bx, 2): flattening, traces
+    ST_R1 = 2,        // +D: faces of the first Riemann solves
+    ST_T1 = 5,        // +D: faces of the transdo pairs
+    ST_DIVU = 8,      // nodes of bx
+    ST_T2 = 9,        // +D: faces of bx
+    ST_VALID = 12,    // bx
+    ST_COUNT = 13
+};
+
+__host__ __device__ inline Box stage_box(int kind, const Box &bx) {
+    if (kind == ST_G4) return grow(bx, CAMR_NGROW);
+    if (kind == ST_G2) return grow(bx, 2);
+    if (kind >= ST_R1 && kind < ST_R1 + 3) {
+        const int d = kind - ST_R1;
+        return nodes(grow_dir(grow(bx, 2), d, -1), d);
+    }
+    if (kind >= ST_T1 && kind < ST_T1 + 3) return nodes(grow(bx, 1), kind - ST_T1);
+    if (kind == ST_DIVU) return nodes(nodes(nodes(bx, 0), 1), 2);
+    if (kind >= ST_T2 && kind < ST_T2 + 3) return nodes(bx, kind - ST_T2);
+    return bx;
+}
+
+// Canonical arrays of one box, in scratch order.
+// compact transverse flux (one array per flux): FR FMX FMY FMZ FE Y0 Y1 Y2
+// (no rho e), then the Godunov normal velocity and pressure
+enum { TR = 0, TMX = 1, TMY = 2, TMZ = 3, TE = 4, TY = 5, TGU = 8, TGP = 9, NTF = 10 };
+// final flux: the NVAR fluxes, then Godunov (u, p)
+enum { XGU = NVAR, XGP = NVAR + 1, NXF = NVAR + 2 };
+enum Slot {
+    S_U,                 // Sborder copy (NVAR)
+    S_Q,                 // compact primitives (NQC)
+    S_SRCQ,              // compact primitive sources (NS)
+    S_FLAT,              // flattening (1)
+    S_QM0,               // qm per direction (NS)
+    S_QP0 = S_QM0 + 3,   // qp per direction (NS)
+    S_F1 = S_QP0 + 3,    // first fluxes per direction (NTF)
+    S_F2 = S_F1 + 3,     // transdo-corrected fluxes, 6 (NTF)
+    S_FLX = S_F2 + 6,    // final fluxes per direction (NXF)
+    S_DIVU = S_FLX + 3,  // nodal divu (1)
+    S_COUNT
+};
+__host__ __device__ constexpr int slot_comps(int s) {
+    return s == S_U ? NVAR
+         : s == S_Q ? NQC
+         : s == S_SRCQ ? NS
+         : s == S_FLAT ? 1
+         : s < S_F1 ? NS
+         : s < S_FLX ? NTF
+         : s < S_DIVU ? NXF
+         : 1;
+}
+__host__ __device__ constexpr int slot_off(int s) {
+    int o = 0;
+    for (int t = 0; t < s; ++t) o += slot_comps(t);
+    return o;
+}
+// the slot of the d-flux corrected by direction t (t != d)
+__host__ __device__ constexpr int f2_slot(int d, int t) {
+    return S_F2 + 2 * d + (t > d ? t - 1 : t);
+}
+
+// One box of the level as the kernels see it.
+struct BoxEnt {
+    int id;         // box index in the level
+    Box bx;         // valid box
+    Canon L;        // layout of its canonical arrays
+    double *base;   // its canonical arrays, in slot order
+    Arr snew, sold; // level state (valid cells)
+};
+
+struct Batch {
+    const BoxEnt *__restrict__ tab;
+    const int *__restrict__ blk;  // block prefix of this stage kind (nbox + 1)
+    int nbox;
+    int first;                    // block id of this launch's first block
+};
+
+// Box of this block and the block's index within the box's stage range.
+__device__ __forceinline__ int batch_box(const Batch &B, int &lblock) {
+    const int bid = (int)blockIdx.x + B.first;
+    int lo = 0, hi = B.nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (B.blk[mid] <= bid) lo = mid;
+        else hi = mid - 1;
+    }
+    lblock = bid - B.blk[lo];
+    return lo;
+}
+
+// First statements of every kernel: the block's box E (and its layout L)
+// and the point (i,j,k) of the stage range this thread covers.
+#define BATCH_POINT(KIND)                                                      \
+    int lblk_;                                                                 \
+    const BoxEnt &E = B.tab[batch_box(B, lblk_)];                              \
+    const Canon L = E.L;                                                       \
+    int i, j, k;                                                               \
+    if (!range_point(range_of(stage_box((KIND), E.bx)),                        \
+                     (int64_t)lblk_ * blockDim.x + threadIdx.x, i, j, k))      \
+        return;
+
+// Slot S of box E (S a compile-time constant, so the address stays scalar).
+template <int S>
+__device__ __forceinline__ double *slot_ptr(const BoxEnt &E) {
+    constexpr size_t off = (size_t)slot_off(S);
+    return E.base + off * E.L.cs;
 }
 
 // Level-step kernels fusing the pointwise bookkeeping around the hydro
 // source (level_fused.hip).
 struct FillArgs {
-    Box valid;      // box b
-    Arr sborder;    // grow(box, 4)
-    Arr q;          // compact primitives over grow(box, 4)
-    Arr snew;       // valid cells
     const DevFab *sold_tab;  // raw S_old of every box (valid cells)
-    const int *owner;
+    const int *owner;        // owning box of every domain cell (-1: none)
+    Arr ext;                 // exterior data for cells no box owns
+    int has_ext;
     DomInfo dom;
     HydroConst hc;
-    double dt, g;
+    const double *dtp;       // the step's dt (device)
+    double g;
     int add_grav;
 };
-void launch_fill_fused(const FillArgs &a, hipStream_t st, int *status, Prof *prof);
-
 struct PostArgs {
-    Box valid;
-    Arr snew, sold;
     HydroConst hc;
-    double dt, g;
+    const double *dtp;
+    double g;
     int add_grav;
     unsigned long long *dtmin;
 };
-void launch_post_fused(const PostArgs &a, hipStream_t st, int *status, Prof *prof);
+
+// Grid of one stage launch over a box set.
+struct LaunchCfg {
+    Batch B;
+    unsigned grid;
+    int block;
+};
+void launch_fill_fused(const LaunchCfg &c, const FillArgs &a, hipStream_t st, int *status);
+void launch_post_fused(const LaunchCfg &c, const PostArgs &a, hipStream_t st, int *status);
+
+// Per-stage launch configuration (block size per stage kind).
+struct Tuning {
+    int block[ST_COUNT];
+    int order;  // 0: boxes in level order; 1: largest first
+};
 
 class FusedEngine {
   public:
+    FusedEngine() = default;
+    ~FusedEngine();
+    FusedEngine(const FusedEngine &) = delete;
+    FusedEngine &operator=(const FusedEngine &) = delete;
     static size_t scratch_doubles(const Box &bx);
-    int reserve(const std::vector<Box> &boxes);
-    // construct_hydro_source for one box + the S_new Saxpy:
-    // snew += dt * dS/dt(sborder); floor / energy checks into *status.
-    // have_prim: the compact primitives were already written to prim_view(bx)
-    int advance_box(const Box &bx, const Arr &sborder, const Arr &srchyd, int has_src,
-                    const Arr &snew, const HydroConst &hc, double dt, hipStream_t st,
-                    int *status, Prof *prof, bool have_prim = false);
-    // where advance_box expects / writes the compact primitives of bx
-    Arr prim_view(const Box &bx) const {
-        return dense(scratch_.base(), grow(bx, CAMR_NGROW));
-    }
+    static bool fits(const Box &bx);
+    // Canonical arrays for every box and the two box tables (state buffers
+    // A/B as new/old and swapped).
+    int setup(const std::vector<Box> &boxes, const std::vector<Arr> &bufA,
+              const std::vector<Arr> &bufB, const Tuning &t);
+    int set_tuning(const Tuning &t);
+    const Tuning &tuning() const { return tune_; }
+
+    // Box set of one launch: b < 0 = all boxes (one launch per stage).
+    // parity selects the table whose snew is buffer A (0) or B (1).
+    // construct_hydro_source + the S_new Saxpy (snew += dt * dS/dt); floor
+    // and energy checks into *status.  have_prim: fill() already wrote the
+    // primitives and the Sborder copy; else they come from sb_tab (and the
+    // primitive sources from src_tab when has_src).
+    int hydro(int b, int parity, bool have_prim, const DevFab *sb_tab, const DevFab *src_tab,
+              int has_src, const HydroConst &hc, const double *dtp, hipStream_t st,
+              int *status, Prof *prof);
+    // ghost fill + clean + primitives + S_new = Sborder + dt * old source
+    int fill(int b, int parity, const FillArgs &a, hipStream_t st, int *status, Prof *prof);
+    // clean_state passes after the update + next dt estimate
+    int post(int b, int parity, const PostArgs &a, hipStream_t st, int *status, Prof *prof);
+    int nbox() const { return (int)boxes_.size(); }
+
+    LaunchCfg cfg(int kind, int b, int parity) const;
 
   private:
+    template <typename K, typename... A>
+    void launch(const char *name, Prof *prof, K kern, int kind, int b, int parity,
+                hipStream_t st, A... args);
+    int build_tables();
+    std::vector<Box> boxes_;
+    std::vector<Arr> bufA_, bufB_;
+    std::vector<size_t> off_;      // scratch offset of each box (doubles)
+    std::vector<int> order_;       // box order of the tables
+    std::vector<int> blk_host_;    // ST_COUNT x (nbox + 1)
+    std::vector<int> pos_;         // table position of box b
+    Tuning tune_{};
+    BoxEnt *tab_[2] = {nullptr, nullptr};
+    int *blk_dev_ = nullptr;
     Scratch scratch_;
 };
+
+Tuning default_tuning();
 
 } // namespace camr

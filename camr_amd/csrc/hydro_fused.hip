@@ -15,6 +15,11 @@
 // Every face state is computed exactly once, as in the reference; the gamma
 // of the gamma-law EOS is the constant 1.4, which the PPM reconstruction of
 // QGAMC reproduces exactly (a constant profile integrates to itself).
+//
+// All arrays of one box share the canonical layout of canon.h: a thread
+// computes one 32-bit byte offset for its cell and reaches components and
+// neighbours through uniform offsets.
+#include "canon.h"
 #include "hydro_fused.h"
 
 namespace camr {
@@ -30,64 +35,91 @@ constexpr int ex_(int D, int d) { return D == d ? 1 : 0; }
 #define TRANS_BOUNDS __launch_bounds__(kBlock)
 #endif
 
+// byte shift of the unit vector e_D in the canonical layout
+template <int D>
+__device__ __forceinline__ unsigned shift(const Canon &L) {
+    return D == 0 ? 8u : (D == 1 ? L.sy() : L.sz());
+}
+
+
 // ------------------------------------------------------------ primitives
 // hydro_ctoprim (Hydro_ctoprim.H:11-101) into the compact layout, with
-// hydro_srctoprim (:106-157) when sources are non-zero.
+// hydro_srctoprim (:106-157) when sources are non-zero; also copies Sborder
+// into the canonical layout (read back by the artificial viscosity).
 __global__ void __launch_bounds__(kBlock)
-k_prim_c(Range r, Arr u, Arr q, Arr src, Arr srcq, int has_src, HydroConst hc, int *status) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
+k_prim_c(Batch B, const DevFab *sb_tab, const DevFab *src_tab, int has_src, HydroConst hc,
+         int *status) {
+    BATCH_POINT(ST_G4)
+    const Arr u = sb_tab[E.id].a;
+    const Arr src = has_src ? src_tab[E.id].a : u;
+    double *srcq_ = slot_ptr<S_SRCQ>(E);
+    const CView uc = cview(L, slot_ptr<S_U>(E), NVAR), q = cview(L, slot_ptr<S_Q>(E), NQC);
+    const unsigned v0 = L.off(i, j, k);
     double v[NVAR];
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) v[n] = u(i, j, k, n);
+    for (int n = 0; n < NVAR; ++n) {
+        v[n] = u(i, j, k, n);
+        uc.st(v0, n, v[n]);
+    }
     const double rho = v[URHO];
-    if (!(rho > 0.)) return;
-    const double e = prim_compact_point(v, q, i, j, k, hc, status);
+    double o[NQC];
+    double e;
+    if (!prim_compact(v, o, e, hc, status)) return;
+#pragma unroll
+    for (int n = 0; n < NQC; ++n) q.st(v0, n, o[n]);
     if (has_src) {
+        const CView sq = cview(L, srcq_, NS);
         const double rhoinv = 1.0 / rho;
         const double vx = v[UMX] * rhoinv, vy = v[UMY] * rhoinv, vz = v[UMZ] * rhoinv;
         const double srcrho = src(i, j, k, URHO);
-        srcq(i, j, k, SR) = srcrho;
-        srcq(i, j, k, SU) = (src(i, j, k, UMX) - vx * srcrho) * rhoinv;
-        srcq(i, j, k, SV) = (src(i, j, k, UMY) - vy * srcrho) * rhoinv;
-        srcq(i, j, k, SW) = (src(i, j, k, UMZ) - vz * srcrho) * rhoinv;
+        sq.st(v0, SR, srcrho);
+        sq.st(v0, SU, (src(i, j, k, UMX) - vx * srcrho) * rhoinv);
+        sq.st(v0, SV, (src(i, j, k, UMY) - vy * srcrho) * rhoinv);
+        sq.st(v0, SW, (src(i, j, k, UMZ) - vz * srcrho) * rhoinv);
         const double sre = src(i, j, k, UEINT);
-        srcq(i, j, k, SE) = sre;
+        sq.st(v0, SE, sre);
 #pragma unroll
         for (int s = 0; s < NSPEC; ++s)
-            srcq(i, j, k, SY + s) = (src(i, j, k, UFS + s) - (v[UFS + s] * rhoinv) * srcrho) * rhoinv;
+            sq.st(v0, SY + s, (src(i, j, k, UFS + s) - (v[UFS + s] * rhoinv) * srcrho) * rhoinv);
         const double dpde = (kGamma - 1.0) * rho, dpdr = (kGamma - 1.0) * e;
-        srcq(i, j, k, SP) = dpde * (sre - (e * rho) * srcrho * rhoinv) * rhoinv + dpdr * srcrho;
+        sq.st(v0, SP, dpde * (sre - (e * rho) * srcrho * rhoinv) * rhoinv + dpdr * srcrho);
     }
 }
 
 // ------------------------------------------------------------ flattening
 // min over directions of flatten (flatten.H:19-99); 1 when the +-3 stencil
 // leaves q's box.
+template <int D>
+__device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsigned v0,
+                                           int i, int j, int k, const Box &qbox) {
+    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+    if (!(qbox.contains(i + 3 * ex, j + 3 * ey, k + 3 * ez) &&
+          qbox.contains(i - 3 * ex, j - 3 * ey, k - 3 * ez)))
+        return 1.0;
+    const unsigned sd = shift<D>(L);
+    double p[7], un[7];
+#pragma unroll
+    for (int o = -3; o <= 3; ++o) {
+        const unsigned vo = v0 + (unsigned)o * sd;
+        p[o + 3] = q.ld(vo, SP);
+        un[o + 3] = q.ld(vo, SU + D);
+    }
+    return flatten_1d(p, un);
+}
+
 __global__ void __launch_bounds__(kBlock)
-k_flat_c(Range r, Arr q, Box qbox, Arr flat, int use_flattening) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
+k_flat_c(Batch B, int use_flattening) {
+    BATCH_POINT(ST_G2)
+    const Box qbox = grow(E.bx, CAMR_NGROW);
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), flat = cview(L, slot_ptr<S_FLAT>(E), 1);
+    const unsigned v0 = L.off(i, j, k);
     double f = 1.0;
     if (use_flattening == 1) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int ex = (d == 0), ey = (d == 1), ez = (d == 2);
-            double fd = 1.0;
-            if (qbox.contains(i + 3 * ex, j + 3 * ey, k + 3 * ez) &&
-                qbox.contains(i - 3 * ex, j - 3 * ey, k - 3 * ez)) {
-                double p[7], un[7];
-#pragma unroll
-                for (int o = -3; o <= 3; ++o) {
-                    p[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, SP);
-                    un[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, SU + d);
-                }
-                fd = flatten_1d(p, un);
-            }
-            f = vmin(f, fd);
-        }
+        f = vmin(f, flat_dir<0>(q, L, v0, i, j, k, qbox));
+        f = vmin(f, flat_dir<1>(q, L, v0, i, j, k, qbox));
+        f = vmin(f, flat_dir<2>(q, L, v0, i, j, k, qbox));
     }
-    flat(i, j, k, 0) = f;
+    flat.st(v0, 0, f);
 }
 
 // ------------------------------------------------------------ PPM trace
@@ -95,22 +127,24 @@ k_flat_c(Range r, Arr q, Box qbox, Arr flat, int use_flattening) {
 // face i, qm(i+1) the minus state of face i+1.
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_ppm_c(Range r, Arr q, Arr flat, Arr qm, Arr qp, HydroConst hc, double dtdx) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+k_ppm_c(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_G2)
+    const double dtdx = *dtp / hc.dx[D];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), flat = cview(L, slot_ptr<S_FLAT>(E), 1);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
     constexpr int SN = SU + D;
     constexpr int T1 = (D == 0) ? SV : SU;
     constexpr int T2 = (D == 2) ? SV : SW;
-    const double cc = q(i, j, k, SC);
-    const double un = q(i, j, k, SN);
-    const double fl = flat(i, j, k, 0);
+    const unsigned v0 = L.off(i, j, k), sd = shift<D>(L);
+    const double cc = q.ld(v0, SC);
+    const double un = q.ld(v0, SN);
+    const double fl = flat.ld(v0, 0);
     double Ip[NS][3], Im[NS][3];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         double st[5];
 #pragma unroll
-        for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, s);
+        for (int o = -2; o <= 2; ++o) st[o + 2] = q.ld(v0 + (unsigned)o * sd, s);
         double sm, sp;
         ppm_edges(st, fl, sm, sp);
         const double s6 = ppm_s6(sm, sp, st[2]);
@@ -127,11 +161,10 @@ k_ppm_c(Range r, Arr q, Arr flat, Arr qm, Arr qp, HydroConst hc, double dtdx) {
     {
         double ro, uo, po, eo;
         ppm_project(+1, W, un, cc, hc.small_dens, hc.small_pres, ro, uo, po, eo);
-        qp(i, j, k, SR) = ro; qp(i, j, k, SN) = uo; qp(i, j, k, SP) = po;
-        qp(i, j, k, SE) = eo;
-        qp(i, j, k, T1) = Im[T1][1]; qp(i, j, k, T2) = Im[T2][1];
+        qp.st(v0, SR, ro); qp.st(v0, SN, uo); qp.st(v0, SP, po); qp.st(v0, SE, eo);
+        qp.st(v0, T1, Im[T1][1]); qp.st(v0, T2, Im[T2][1]);
 #pragma unroll
-        for (int s = 0; s < NSPEC; ++s) qp(i, j, k, SY + s) = Im[SY + s][1];
+        for (int s = 0; s < NSPEC; ++s) qp.st(v0, SY + s, Im[SY + s][1]);
     }
 #pragma unroll
     for (int w = 0; w < 3; ++w) {
@@ -141,12 +174,11 @@ k_ppm_c(Range r, Arr q, Arr flat, Arr qm, Arr qp, HydroConst hc, double dtdx) {
     {
         double ro, uo, po, eo;
         ppm_project(-1, W, un, cc, hc.small_dens, hc.small_pres, ro, uo, po, eo);
-        const int a = i + ex, b = j + ey, c = k + ez;
-        qm(a, b, c, SR) = ro; qm(a, b, c, SN) = uo; qm(a, b, c, SP) = po;
-        qm(a, b, c, SE) = eo;
-        qm(a, b, c, T1) = Ip[T1][1]; qm(a, b, c, T2) = Ip[T2][1];
+        const unsigned v1 = v0 + sd;
+        qm.st(v1, SR, ro); qm.st(v1, SN, uo); qm.st(v1, SP, po); qm.st(v1, SE, eo);
+        qm.st(v1, T1, Ip[T1][1]); qm.st(v1, T2, Ip[T2][1]);
 #pragma unroll
-        for (int s = 0; s < NSPEC; ++s) qm(a, b, c, SY + s) = Ip[SY + s][1];
+        for (int s = 0; s < NSPEC; ++s) qm.st(v1, SY + s, Ip[SY + s][1]);
     }
 }
 
@@ -154,25 +186,27 @@ k_ppm_c(Range r, Arr q, Arr flat, Arr qm, Arr qp, HydroConst hc, double dtdx) {
 // hydro_plm_d (PLM.H:170-323) for direction D.
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_plm_c(Range r, Arr q, Arr srcq, int has_src, Arr flat, Arr qm, Arr qp,
-        HydroConst hc, double dtdx) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
-    const double fl = flat(i, j, k, 0);
+k_plm_c(Batch B, int has_src, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_G2)
+    const double dtdx = *dtp / hc.dx[D];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), flat = cview(L, slot_ptr<S_FLAT>(E), 1);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const CView sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+    const unsigned v0 = L.off(i, j, k), sd = shift<D>(L);
+    const double fl = flat.ld(v0, 0);
     double c[NS], sl[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
         double st[5];
 #pragma unroll
-        for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, s);
+        for (int o = -2; o <= 2; ++o) st[o + 2] = q.ld(v0 + (unsigned)o * sd, s);
         c[s] = st[2];
         if (s == SP && hc.use_pslope) {
             double rr[5], gg[5];
 #pragma unroll
             for (int o = -2; o <= 2; ++o) {
-                rr[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, SR);
-                gg[o + 2] = has_src ? srcq(i + o * ex, j + o * ey, k + o * ez, SU + D) : 0.0;
+                rr[o + 2] = q.ld(v0 + (unsigned)o * sd, SR);
+                gg[o + 2] = has_src ? sq.ld(v0 + (unsigned)o * sd, SU + D) : 0.0;
             }
             sl[s] = mc_pslope(st, rr, gg, hc.dx[D], fl, hc.plm_iorder);
         } else {
@@ -180,54 +214,50 @@ k_plm_c(Range r, Arr q, Arr srcq, int has_src, Arr flat, Arr qm, Arr qp,
         }
     }
     double op[NS], om[NS];
-    plm_trace(D, c, sl, q(i, j, k, SC), dtdx, hc.small_dens, hc.small_pres, op, om);
+    plm_trace(D, c, sl, q.ld(v0, SC), dtdx, hc.small_dens, hc.small_pres, op, om);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        qp(i, j, k, s) = op[s];
-        qm(i + ex, j + ey, k + ez, s) = om[s];
+        qp.st(v0, s, op[s]);
+        qm.st(v0 + sd, s, om[s]);
     }
 }
 
-__device__ __forceinline__ FaceAux face_aux(const Arr &q, int il, int jl, int kl,
-                                            int i, int j, int k) {
+// sound speeds of the cells left (offset vl) and right (vr) of a face
+__device__ __forceinline__ FaceAux face_aux(const CView &q, unsigned vl, unsigned vr) {
     FaceAux a;
-    a.c_l = q(il, jl, kl, SC);
-    a.c_r = q(i, j, k, SC);
-    a.csml_l = q(il, jl, kl, SCS);
-    a.csml_r = q(i, j, k, SCS);
+    a.c_l = q.ld(vl, SC);
+    a.c_r = q.ld(vr, SC);
+    a.csml_l = q.ld(vl, SCS);
+    a.csml_r = q.ld(vr, SCS);
     a.gam_l = kGamma;
     a.gam_r = kGamma;
     return a;
 }
 
-// compact transverse flux layout: FR FMX FMY FMZ FE Y0 Y1 Y2 (no rho e)
-enum { TR = 0, TMX = 1, TMY = 2, TMZ = 3, TE = 4, TY = 5, NT = 8 };
-
-__device__ __forceinline__ void load_tflux(const Arr &F, const Arr &G, int i, int j,
-                                           int k, double *f, double *g) {
-    f[FR] = F(i, j, k, TR);
-    f[FMX] = F(i, j, k, TMX);
-    f[FMY] = F(i, j, k, TMY);
-    f[FMZ] = F(i, j, k, TMZ);
-    f[FE] = F(i, j, k, TE);
+__device__ __forceinline__ void load_tflux(const CView &F, unsigned v, double *f, double *g) {
+    f[FR] = F.ld(v, TR);
+    f[FMX] = F.ld(v, TMX);
+    f[FMY] = F.ld(v, TMY);
+    f[FMZ] = F.ld(v, TMZ);
+    f[FE] = F.ld(v, TE);
     f[FEI] = 0.0;
 #pragma unroll
-    for (int s = 0; s < NSPEC; ++s) f[FY + s] = F(i, j, k, TY + s);
-    g[GUN] = G(i, j, k, GUN);
-    g[GP] = G(i, j, k, GP);
+    for (int s = 0; s < NSPEC; ++s) f[FY + s] = F.ld(v, TY + s);
+    g[GUN] = F.ld(v, TGU);
+    g[GP] = F.ld(v, TGP);
 }
 
-__device__ __forceinline__ void store_tflux(const Arr &F, const Arr &G, int i, int j,
-                                            int k, const double *f, const double *g) {
-    F(i, j, k, TR) = f[FR];
-    F(i, j, k, TMX) = f[FMX];
-    F(i, j, k, TMY) = f[FMY];
-    F(i, j, k, TMZ) = f[FMZ];
-    F(i, j, k, TE) = f[FE];
+__device__ __forceinline__ void store_tflux(const CView &F, unsigned v, const double *f,
+                                            const double *g) {
+    F.st(v, TR, f[FR]);
+    F.st(v, TMX, f[FMX]);
+    F.st(v, TMY, f[FMY]);
+    F.st(v, TMZ, f[FMZ]);
+    F.st(v, TE, f[FE]);
 #pragma unroll
-    for (int s = 0; s < NSPEC; ++s) F(i, j, k, TY + s) = f[FY + s];
-    G(i, j, k, GUN) = g[GUN];
-    G(i, j, k, GP) = g[GP];
+    for (int s = 0; s < NSPEC; ++s) F.st(v, TY + s, f[FY + s]);
+    F.st(v, TGU, g[GUN]);
+    F.st(v, TGP, g[GP]);
 }
 
 // ------------------------------------------------------------ first fluxes
@@ -236,28 +266,29 @@ __device__ __forceinline__ void store_tflux(const Arr &F, const Arr &G, int i, i
 // state arrays because the transverse stages read them.
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_riemann1(Range r, Arr qm, Arr qp, Arr q, Arr F, Arr G, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+k_riemann1(Batch B, HydroConst hc) {
+    BATCH_POINT(ST_R1 + D)
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), F = cview(L, slot_ptr<S_F1 + D>(E), NTF);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const unsigned v0 = L.off(i, j, k), sd = shift<D>(L);
     double ql[NS], qr[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        ql[s] = qm(i, j, k, s);
-        qr[s] = qp(i, j, k, s);
+        ql[s] = qm.ld(v0, s);
+        qr[s] = qp.ld(v0, s);
     }
-    const FaceAux a = face_aux(q, i - ex, j - ey, k - ez, i, j, k);
+    const FaceAux a = face_aux(q, v0 - sd, v0);
     const int idx = (D == 0) ? i : ((D == 1) ? j : k);
     double f[NF], g[NG];
     int mut;
     face_flux(D, idx, hc, ql, qr, a, f, g, &mut);
     if (mut & 1) {
-        qm(i, j, k, SR) = ql[SR]; qm(i, j, k, SP) = ql[SP]; qm(i, j, k, SE) = ql[SE];
+        qm.st(v0, SR, ql[SR]); qm.st(v0, SP, ql[SP]); qm.st(v0, SE, ql[SE]);
     }
     if (mut & 2) {
-        qp(i, j, k, SR) = qr[SR]; qp(i, j, k, SP) = qr[SP]; qp(i, j, k, SE) = qr[SE];
+        qp.st(v0, SR, qr[SR]); qp.st(v0, SP, qr[SP]); qp.st(v0, SE, qr[SE]);
     }
-    store_tflux(F, G, i, j, k, f, g);
+    store_tflux(F, v0, f, g);
 }
 
 // ------------------------------------------------ transdo + Riemann (pairs)
@@ -265,72 +296,73 @@ k_riemann1(Range r, Arr qm, Arr qp, Arr q, Arr F, Arr G, HydroConst hc) {
 // from transdo at cell f-e_D (Godunov_utils_3D.H:22-239 writes qp(iv) and
 // qm(iv+e_D)); both pairs D|t.
 template <int D, int T>
-__device__ __forceinline__ void trans1_pair(int i, int j, int k, const double *np,
+__device__ __forceinline__ void trans1_pair(unsigned v0, const Canon &L, const double *np,
                                             const double *nm, const FaceAux &a, int idx,
-                                            const Arr &F, const Arr &G, const Arr &O,
-                                            const Arr &OG, double cdt, const HydroConst &hc) {
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
-    constexpr int tx = ex_(T, 0), ty = ex_(T, 1), tz = ex_(T, 2);
+                                            const CView &F, const CView &O, double cdt,
+                                            const HydroConst &hc) {
+    const unsigned sd = shift<D>(L), st_ = shift<T>(L);
     double sp[NS], sm[NS];
     {
         double fl[NF], fr[NF], gl[NG], gr[NG];
         TransDiff t;
-        load_tflux(F, G, i, j, k, fl, gl);
-        load_tflux(F, G, i + tx, j + ty, k + tz, fr, gr);
+        load_tflux(F, v0, fl, gl);
+        load_tflux(F, v0 + st_, fr, gr);
         trans_diff_one(fl, fr, gl, gr, cdt, t);
         transverse_one_d(np, t, cdt, kGamma, hc.trd, hc.small_pres, sp);
     }
     {
         double fl[NF], fr[NF], gl[NG], gr[NG];
         TransDiff t;
-        load_tflux(F, G, i - ex, j - ey, k - ez, fl, gl);
-        load_tflux(F, G, i - ex + tx, j - ey + ty, k - ez + tz, fr, gr);
+        load_tflux(F, v0 - sd, fl, gl);
+        load_tflux(F, v0 - sd + st_, fr, gr);
         trans_diff_one(fl, fr, gl, gr, cdt, t);
         transverse_one_d(nm, t, cdt, kGamma, hc.trd, hc.small_pres, sm);
     }
     double f[NF], g[NG];
     int mut;
     face_flux(D, idx, hc, sm, sp, a, f, g, &mut);
-    store_tflux(O, OG, i, j, k, f, g);
+    store_tflux(O, v0, f, g);
 }
 
 template <int D>
 __global__ void TRANS_BOUNDS
-k_trans1(Range r, Arr qm, Arr qp, Arr q, Arr Fa, Arr Ga, Arr Fb, Arr Gb,
-         Arr Oa, Arr OGa, Arr Ob, Arr OGb, double cda, double cdb, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
+k_trans1(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_T1 + D)
     constexpr int T0 = (D == 0) ? 1 : 0;
     constexpr int T1 = (D == 2) ? 1 : 2;
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+    const double cda = 1.0 / 3.0 * *dtp / hc.dx[T0];
+    const double cdb = 1.0 / 3.0 * *dtp / hc.dx[T1];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const unsigned v0 = L.off(i, j, k), sd = shift<D>(L);
     double np[NS], nm[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-        np[s] = qp(i, j, k, s);
-        nm[s] = qm(i, j, k, s);
+        np[s] = qp.ld(v0, s);
+        nm[s] = qm.ld(v0, s);
     }
-    const FaceAux a = face_aux(q, i - ex, j - ey, k - ez, i, j, k);
+    const FaceAux a = face_aux(q, v0 - sd, v0);
     const int idx = (D == 0) ? i : ((D == 1) ? j : k);
-    trans1_pair<D, T0>(i, j, k, np, nm, a, idx, Fa, Ga, Oa, OGa, cda, hc);
-    trans1_pair<D, T1>(i, j, k, np, nm, a, idx, Fb, Gb, Ob, OGb, cdb, hc);
+    trans1_pair<D, T0>(v0, L, np, nm, a, idx, cview(L, slot_ptr<S_F1 + T0>(E), NTF),
+                       cview(L, slot_ptr<f2_slot(D, T0)>(E), NTF), cda, hc);
+    trans1_pair<D, T1>(v0, L, np, nm, a, idx, cview(L, slot_ptr<S_F1 + T1>(E), NTF),
+                       cview(L, slot_ptr<f2_slot(D, T1)>(E), NTF), cdb, hc);
 }
 
-// transdd differences of cell (i,j,k) for direction D: Fa is the flux of
-// direction T0 corrected by T1 (faces of T0), Fb the flux of T1 corrected
-// by T0.
+// transdd differences of the cell at offset v for direction D: Fa is the
+// flux of direction T0 corrected by T1 (faces of T0), Fb the flux of T1
+// corrected by T0.
 template <int D>
-__device__ __forceinline__ void trans2_diff(int i, int j, int k, const Arr &Fa,
-                                            const Arr &Ga, const Arr &Fb, const Arr &Gb,
-                                            double ca, double cb, TransDiff &t) {
+__device__ __forceinline__ void trans2_diff(unsigned v, const Canon &L, const CView &Fa,
+                                            const CView &Fb, double ca, double cb,
+                                            TransDiff &t) {
     constexpr int T0 = (D == 0) ? 1 : 0;
     constexpr int T1 = (D == 2) ? 1 : 2;
-    constexpr int ax = ex_(T0, 0), ay = ex_(T0, 1), az = ex_(T0, 2);
-    constexpr int bx = ex_(T1, 0), by = ex_(T1, 1), bz = ex_(T1, 2);
     double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
-    load_tflux(Fa, Ga, i, j, k, fal, gal);
-    load_tflux(Fa, Ga, i + ax, j + ay, k + az, far_, gar);
-    load_tflux(Fb, Gb, i, j, k, fbl, gbl);
-    load_tflux(Fb, Gb, i + bx, j + by, k + bz, fbr, gbr);
+    load_tflux(Fa, v, fal, gal);
+    load_tflux(Fa, v + shift<T0>(L), far_, gar);
+    load_tflux(Fb, v, fbl, gbl);
+    load_tflux(Fb, v + shift<T1>(L), fbr, gbr);
     trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, kGamma, t);
 }
 
@@ -339,39 +371,47 @@ __device__ __forceinline__ void trans2_diff(int i, int j, int k, const Arr &Fa,
 // cell f-e_D (Godunov_utils_3D.H:245-482); then hydro_cmpflx, then
 // hydro_artif_visc / hydro_norm_spec_flx / hydro_ext_flx
 // (Hydro_adjust_fluxes.cpp:13-44).
-template <int D>
+template <int D, int HAS_SRC>
 __global__ void TRANS_BOUNDS
-k_trans2(Range r, Arr qm, Arr qp, Arr q, Arr srcq, int has_src, Arr Fa, Arr Ga,
-         Arr Fb, Arr Gb, Arr divu, Arr u, Arr flx, Arr G3, double ca, double cb,
-         double hdt, double area, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
-    constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+k_trans2(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_T2 + D)
+    constexpr int T0 = (D == 0) ? 1 : 0;
+    constexpr int T1 = (D == 2) ? 1 : 2;
+    const double dt = *dtp;
+    const double hdt = 0.5 * dt;
+    const double ca = 0.5 * dt / hc.dx[T0], cb = 0.5 * dt / hc.dx[T1];
+    const double area = (D == 0) ? hc.dx[1] * hc.dx[2]
+                      : (D == 1) ? hc.dx[0] * hc.dx[2] : hc.dx[0] * hc.dx[1];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const CView Fa = cview(L, slot_ptr<f2_slot(T0, T1)>(E), NTF);
+    const CView Fb = cview(L, slot_ptr<f2_slot(T1, T0)>(E), NTF);
+    const unsigned v0 = L.off(i, j, k), sd = shift<D>(L);
     double sp[NS], sm[NS];
     {
         double n[NS], src[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            n[s] = qp(i, j, k, s);
-            src[s] = has_src ? srcq(i, j, k, s) : 0.0;
+            n[s] = qp.ld(v0, s);
+            src[s] = HAS_SRC ? sq.ld(v0, s) : 0.0;
         }
         TransDiff t;
-        trans2_diff<D>(i, j, k, Fa, Ga, Fb, Gb, ca, cb, t);
+        trans2_diff<D>(v0, L, Fa, Fb, ca, cb, t);
         transverse_two_d(n, t, hdt, src, hc.trd, hc.small_pres, sp);
     }
     {
-        const int i0 = i - ex, j0 = j - ey, k0 = k - ez;
+        const unsigned vm = v0 - sd;
         double n[NS], src[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            n[s] = qm(i, j, k, s);
-            src[s] = has_src ? srcq(i0, j0, k0, s) : 0.0;
+            n[s] = qm.ld(v0, s);
+            src[s] = HAS_SRC ? sq.ld(vm, s) : 0.0;
         }
         TransDiff t;
-        trans2_diff<D>(i0, j0, k0, Fa, Ga, Fb, Gb, ca, cb, t);
+        trans2_diff<D>(vm, L, Fa, Fb, ca, cb, t);
         transverse_two_d(n, t, hdt, src, hc.trd, hc.small_pres, sm);
     }
-    const FaceAux a = face_aux(q, i - ex, j - ey, k - ez, i, j, k);
+    const FaceAux a = face_aux(q, v0 - sd, v0);
     const int idx = (D == 0) ? i : ((D == 1) ? j : k);
     double f[NF], g[NG];
     int mut;
@@ -381,19 +421,21 @@ k_trans2(Range r, Arr qm, Arr qp, Arr q, Arr srcq, int has_src, Arr Fa, Arr Ga,
     fu[UEDEN] = f[FE]; fu[UEINT] = f[FEI]; fu[UTEMP] = 0.0;
 #pragma unroll
     for (int s = 0; s < NSPEC; ++s) fu[UFS + s] = f[FY + s];
-    adjust_face<D>(fu, divu, u, i, j, k, area, hc);
+    adjust_face<D>(fu, cview(L, slot_ptr<S_DIVU>(E), 1), cview(L, slot_ptr<S_U>(E), NVAR), i, j,
+                   k, area, hc);
+    const CView flx = cview(L, slot_ptr<S_FLX + D>(E), NXF);
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) flx(i, j, k, n) = fu[n];
-    G3(i, j, k, GUN) = g[GUN];
-    G3(i, j, k, GP) = g[GP];
+    for (int n = 0; n < NVAR; ++n) flx.st(v0, n, fu[n]);
+    flx.st(v0, XGU, g[GUN]);
+    flx.st(v0, XGP, g[GP]);
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_divu_c(Range r, Arr q, Arr divu, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
+k_divu_c(Batch B, HydroConst hc) {
+    BATCH_POINT(ST_DIVU)
     // compact q has u, v, w in the same slots as QU, QV, QW
-    divu(i, j, k, 0) = divu_node(q, i, j, k, hc);
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), divu = cview(L, slot_ptr<S_DIVU>(E), 1);
+    divu.st(L.off(i, j, k), 0, divu_node(q, i, j, k, hc));
 }
 
 // ------------------------------------------------------------ update
@@ -401,23 +443,28 @@ k_divu_c(Range r, Arr q, Arr divu, HydroConst hc) {
 // dS/dt (CAMR_construct_hydro_source.cpp:14), then the Saxpy of
 // CAMR_advance.cpp:141 and the floor check of the following clean_state.
 __global__ void __launch_bounds__(kBlock)
-k_update_c(Range r, Arr snew, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
-           double a0, double a1, double a2, double vol, double dt,
-           double small_dens, int *flags) {
-    int i, j, k;
-    if (!range_point(r, logical_thread(), i, j, k)) return;
+k_update_c(Batch B, HydroConst hc, const double *dtp, int *flags) {
+    BATCH_POINT(ST_VALID)
+    const double dt = *dtp;
+    const double a0 = hc.dx[1] * hc.dx[2], a1 = hc.dx[0] * hc.dx[2], a2 = hc.dx[0] * hc.dx[1];
+    const double vol = hc.dx[0] * hc.dx[1] * hc.dx[2];
+    const double small_dens = hc.small_dens;
+    const Arr snew = E.snew;
+    const CView f0 = cview(L, slot_ptr<S_FLX + 0>(E), NXF), f1 = cview(L, slot_ptr<S_FLX + 1>(E), NXF),
+                f2 = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
+    const unsigned v0 = L.off(i, j, k), sx = 8u, sy = L.sy(), sz = L.sz();
     const double pdivu =
         0.5 *
-        ((g0(i + 1, j, k, GP) + g0(i, j, k, GP)) * (g0(i + 1, j, k, GUN) * a0 - g0(i, j, k, GUN) * a0) +
-         (g1(i, j + 1, k, GP) + g1(i, j, k, GP)) * (g1(i, j + 1, k, GUN) * a1 - g1(i, j, k, GUN) * a1) +
-         (g2(i, j, k + 1, GP) + g2(i, j, k, GP)) * (g2(i, j, k + 1, GUN) * a2 - g2(i, j, k, GUN) * a2)) /
+        ((f0.ld(v0 + sx, XGP) + f0.ld(v0, XGP)) * (f0.ld(v0 + sx, XGU) * a0 - f0.ld(v0, XGU) * a0) +
+         (f1.ld(v0 + sy, XGP) + f1.ld(v0, XGP)) * (f1.ld(v0 + sy, XGU) * a1 - f1.ld(v0, XGU) * a1) +
+         (f2.ld(v0 + sz, XGP) + f2.ld(v0, XGP)) * (f2.ld(v0 + sz, XGU) * a2 - f2.ld(v0, XGU) * a2)) /
         vol;
     const double volinv = 1.0 / vol;
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
         double ds = 0.0;
-        ds += (f0(i, j, k, n) - f0(i + 1, j, k, n) + f1(i, j, k, n) - f1(i, j + 1, k, n) +
-               f2(i, j, k, n) - f2(i, j, k + 1, n)) *
+        ds += (f0.ld(v0, n) - f0.ld(v0 + sx, n) + f1.ld(v0, n) - f1.ld(v0 + sy, n) +
+               f2.ld(v0, n) - f2.ld(v0 + sz, n)) *
               volinv;
         if (n == UEINT) ds -= pdivu;
         snew(i, j, k, n) += dt * ds;
@@ -427,111 +474,207 @@ k_update_c(Range r, Arr snew, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
     else if (rho < small_dens) atomicOr(flags, kFloorFlag);
 }
 
-template <typename K, typename... A>
-void go(const char *name, Prof *prof, K kern, const Box &b, hipStream_t s, A... args) {
-    Range r = range_of(b);
-    Region reg(prof, name, s);
-    hipLaunchKernelGGL(kern, dim3(grid_for(r.n)), dim3(kBlock), 0, s, r, args...);
-}
-
 } // namespace
 
 // --------------------------------------------------------------- engine
+Tuning default_tuning() {
+    Tuning t;
+    for (int k = 0; k < ST_COUNT; ++k) t.block[k] = kBlock;
+    t.order = 0;
+    return t;
+}
+
 size_t FusedEngine::scratch_doubles(const Box &bx) {
-    const Box bxg1 = grow(bx, 1), bxg2 = grow(bx, 2), g4 = grow(bx, CAMR_NGROW);
-    size_t n = 0;
-    n += (size_t)NQC * g4.npts();                             // compact q
-    n += (size_t)NS * g4.npts();                              // compact srcq
-    n += bxg2.npts();                                         // flat
-    for (int d = 0; d < 3; ++d) {
-        n += (size_t)NS * grow_hi(bxg2, d, 1).npts();         // qm
-        n += (size_t)NS * bxg2.npts();                        // qp
-        n += (size_t)(NT + NG) * nodes(grow_dir(bxg2, d, -1), d).npts();  // F1
-        n += (size_t)2 * (NT + NG) * nodes(bxg1, d).npts();   // F2 pairs
-        n += (size_t)(NVAR + NG) * nodes(bx, d).npts();       // flx, G3
-    }
-    n += nodes(nodes(nodes(bx, 0), 1), 2).npts();             // divu
-    return n + 32 * 64;
+    return (size_t)slot_off(S_COUNT) * (size_t)canon_box(bx).npts();
 }
+bool FusedEngine::fits(const Box &bx) { return canon_fits(bx); }
 
-int FusedEngine::reserve(const std::vector<Box> &boxes) {
-    size_t n = 0;
-    for (const Box &b : boxes) n = std::max(n, scratch_doubles(b));
-    return scratch_.reserve(n * sizeof(double)) ? CAMR_OK : CAMR_ERR_ALLOC;
-}
-
-int FusedEngine::advance_box(const Box &bx, const Arr &sborder, const Arr &srchyd,
-                             int has_src, const Arr &snew, const HydroConst &hc,
-                             double dt, hipStream_t st, int *status, Prof *prof,
-                             bool have_prim) {
-    const Box bxg1 = grow(bx, 1), bxg2 = grow(bx, 2), g4 = grow(bx, CAMR_NGROW);
-    scratch_.reset();
-    auto take = [&](const Box &b, int nc) { return dense(scratch_.take((size_t)nc * b.npts()), b); };
-    Arr q = take(g4, NQC);
-    Arr sq = has_src ? take(g4, NS) : q;
-    Arr flat = take(bxg2, 1);
-    Arr qm[3], qp[3], F1[3], G1[3], F2[3][3], G2[3][3], flx[3], G3[3];
-    for (int d = 0; d < 3; ++d) {
-        qm[d] = take(grow_hi(bxg2, d, 1), NS);
-        qp[d] = take(bxg2, NS);
-        const Box fb = nodes(grow_dir(bxg2, d, -1), d);
-        F1[d] = take(fb, NT);
-        G1[d] = take(fb, NG);
-        for (int t = 0; t < 3; ++t)
-            if (t != d) {
-                F2[d][t] = take(nodes(bxg1, d), NT);
-                G2[d][t] = take(nodes(bxg1, d), NG);
-            }
-        flx[d] = take(nodes(bx, d), NVAR);
-        G3[d] = take(nodes(bx, d), NG);
+int FusedEngine::setup(const std::vector<Box> &boxes, const std::vector<Arr> &bufA,
+                       const std::vector<Arr> &bufB, const Tuning &t) {
+    boxes_ = boxes;
+    bufA_ = bufA;
+    bufB_ = bufB;
+    const int n = (int)boxes_.size();
+    off_.assign(n, 0);
+    size_t total = 0;
+    for (int b = 0; b < n; ++b) {
+        if (!canon_fits(boxes_[b])) {
+            set_error("box %d (%lld cells) is too large for the fused engine "
+                      "(32-bit buffer offsets)", b, (long long)boxes_[b].npts());
+            return CAMR_ERR_SHAPE;
+        }
+        off_[b] = total;
+        total += (scratch_doubles(boxes_[b]) + 31) & ~size_t(31);  // 256-byte aligned
     }
-    Arr divu = take(nodes(nodes(nodes(bx, 0), 1), 2), 1);
-    if (!divu.p) {
-        set_error("fused engine scratch too small");
+    if (!scratch_.reserve(total * sizeof(double))) {
+        set_error("fused-engine scratch allocation of %zu bytes failed", total * sizeof(double));
         return CAMR_ERR_ALLOC;
     }
-
-    if (!have_prim || has_src)
-        go("prim", prof, k_prim_c, g4, st, sborder, q, srchyd, sq, has_src, hc, status);
-    go("flatten", prof, k_flat_c, bxg2, st, q, g4, flat, hc.use_flattening);
-    const double dtdx[3] = {dt / hc.dx[0], dt / hc.dx[1], dt / hc.dx[2]};
-    if (hc.ppm_type == 1) {
-        go("trace_ppm", prof, k_ppm_c<0>, bxg2, st, q, flat, qm[0], qp[0], hc, dtdx[0]);
-        go("trace_ppm", prof, k_ppm_c<1>, bxg2, st, q, flat, qm[1], qp[1], hc, dtdx[1]);
-        go("trace_ppm", prof, k_ppm_c<2>, bxg2, st, q, flat, qm[2], qp[2], hc, dtdx[2]);
-    } else {
-        go("trace_plm", prof, k_plm_c<0>, bxg2, st, q, sq, has_src, flat, qm[0], qp[0], hc, dtdx[0]);
-        go("trace_plm", prof, k_plm_c<1>, bxg2, st, q, sq, has_src, flat, qm[1], qp[1], hc, dtdx[1]);
-        go("trace_plm", prof, k_plm_c<2>, bxg2, st, q, sq, has_src, flat, qm[2], qp[2], hc, dtdx[2]);
+    for (int p = 0; p < 2; ++p) {
+        if (tab_[p]) (void)hipFree(tab_[p]);
+        tab_[p] = nullptr;
+        if (hipMalloc(&tab_[p], std::max(n, 1) * sizeof(BoxEnt)) != hipSuccess) {
+            set_error("box table allocation failed");
+            return CAMR_ERR_ALLOC;
+        }
     }
-    go("riemann_normal", prof, k_riemann1<0>, nodes(grow_dir(bxg2, 0, -1), 0), st, qm[0], qp[0], q, F1[0], G1[0], hc);
-    go("riemann_normal", prof, k_riemann1<1>, nodes(grow_dir(bxg2, 1, -1), 1), st, qm[1], qp[1], q, F1[1], G1[1], hc);
-    go("riemann_normal", prof, k_riemann1<2>, nodes(grow_dir(bxg2, 2, -1), 2), st, qm[2], qp[2], q, F1[2], G1[2], hc);
+    if (blk_dev_) (void)hipFree(blk_dev_);
+    blk_dev_ = nullptr;
+    if (hipMalloc(&blk_dev_, (size_t)ST_COUNT * (n + 1) * sizeof(int)) != hipSuccess) {
+        set_error("block table allocation failed");
+        return CAMR_ERR_ALLOC;
+    }
+    return set_tuning(t);
+}
 
-    const double cdt[3] = {1.0 / 3.0 * dt / hc.dx[0], 1.0 / 3.0 * dt / hc.dx[1],
-                           1.0 / 3.0 * dt / hc.dx[2]};
-    go("transverse_1", prof, k_trans1<0>, nodes(bxg1, 0), st, qm[0], qp[0], q, F1[1], G1[1], F1[2], G1[2],
-       F2[0][1], G2[0][1], F2[0][2], G2[0][2], cdt[1], cdt[2], hc);
-    go("transverse_1", prof, k_trans1<1>, nodes(bxg1, 1), st, qm[1], qp[1], q, F1[0], G1[0], F1[2], G1[2],
-       F2[1][0], G2[1][0], F2[1][2], G2[1][2], cdt[0], cdt[2], hc);
-    go("transverse_1", prof, k_trans1<2>, nodes(bxg1, 2), st, qm[2], qp[2], q, F1[0], G1[0], F1[1], G1[1],
-       F2[2][0], G2[2][0], F2[2][1], G2[2][1], cdt[0], cdt[1], hc);
+FusedEngine::~FusedEngine() {
+    for (int p = 0; p < 2; ++p)
+        if (tab_[p]) (void)hipFree(tab_[p]);
+    if (blk_dev_) (void)hipFree(blk_dev_);
+}
 
-    go("divu", prof, k_divu_c, nodes(nodes(nodes(bx, 0), 1), 2), st, q, divu, hc);
-    const double hdt = 0.5 * dt;
-    const double hdtd[3] = {0.5 * dt / hc.dx[0], 0.5 * dt / hc.dx[1], 0.5 * dt / hc.dx[2]};
-    const double area[3] = {hc.dx[1] * hc.dx[2], hc.dx[0] * hc.dx[2], hc.dx[0] * hc.dx[1]};
-    go("transverse_2", prof, k_trans2<0>, nodes(bx, 0), st, qm[0], qp[0], q, sq, has_src,
-       F2[1][2], G2[1][2], F2[2][1], G2[2][1], divu, sborder, flx[0], G3[0], hdtd[1], hdtd[2], hdt, area[0], hc);
-    go("transverse_2", prof, k_trans2<1>, nodes(bx, 1), st, qm[1], qp[1], q, sq, has_src,
-       F2[0][2], G2[0][2], F2[2][0], G2[2][0], divu, sborder, flx[1], G3[1], hdtd[0], hdtd[2], hdt, area[1], hc);
-    go("transverse_2", prof, k_trans2<2>, nodes(bx, 2), st, qm[2], qp[2], q, sq, has_src,
-       F2[0][1], G2[0][1], F2[1][0], G2[1][0], divu, sborder, flx[2], G3[2], hdtd[0], hdtd[1], hdt, area[2], hc);
+int FusedEngine::set_tuning(const Tuning &t) {
+    for (int k = 0; k < ST_COUNT; ++k)
+        if (t.block[k] != 64 && t.block[k] != 128 && t.block[k] != 256) {
+            set_error("block size must be 64, 128 or 256");
+            return CAMR_ERR_PARAM;
+        }
+    if (t.order != 0 && t.order != 1) {
+        set_error("box order must be 0 (level order) or 1 (largest first)");
+        return CAMR_ERR_PARAM;
+    }
+    tune_ = t;
+    return build_tables();
+}
 
-    const double vol = hc.dx[0] * hc.dx[1] * hc.dx[2];
-    go("consup", prof, k_update_c, bx, st, snew, flx[0], flx[1], flx[2], G3[0], G3[1], G3[2],
-       area[0], area[1], area[2], vol, dt, hc.small_dens, status);
+int FusedEngine::build_tables() {
+    const int n = (int)boxes_.size();
+    order_.resize(n);
+    for (int b = 0; b < n; ++b) order_[b] = b;
+    if (tune_.order == 1)
+        std::stable_sort(order_.begin(), order_.end(), [&](int a, int b) {
+            return boxes_[a].npts() > boxes_[b].npts();
+        });
+    pos_.assign(n, 0);
+    for (int p = 0; p < n; ++p) pos_[order_[p]] = p;
+    std::vector<BoxEnt> t0(n), t1(n);
+    for (int p = 0; p < n; ++p) {
+        const int b = order_[p];
+        BoxEnt e;
+        e.id = b;
+        e.bx = boxes_[b];
+        e.L = canon_of(boxes_[b]);
+        e.base = scratch_.base() + off_[b];
+        e.snew = bufA_[b];
+        e.sold = bufB_[b];
+        t0[p] = e;
+        e.snew = bufB_[b];
+        e.sold = bufA_[b];
+        t1[p] = e;
+    }
+    blk_host_.assign((size_t)ST_COUNT * (n + 1), 0);
+    for (int k = 0; k < ST_COUNT; ++k) {
+        int *row = &blk_host_[(size_t)k * (n + 1)];
+        for (int p = 0; p < n; ++p) {
+            const int64_t pts = stage_box(k, boxes_[order_[p]]).npts();
+            const int64_t nb = (pts + tune_.block[k] - 1) / tune_.block[k];
+            if ((int64_t)row[p] + nb > 0x7fffffff) {
+                set_error("stage grid exceeds 2^31 blocks");
+                return CAMR_ERR_SHAPE;
+            }
+            row[p + 1] = row[p] + (int)nb;
+        }
+    }
+    if (n > 0) {
+        if (hipMemcpy(tab_[0], t0.data(), n * sizeof(BoxEnt), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(tab_[1], t1.data(), n * sizeof(BoxEnt), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(blk_dev_, blk_host_.data(), blk_host_.size() * sizeof(int),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("box table upload failed");
+            return CAMR_ERR_HIP;
+        }
+    }
+    return CAMR_OK;
+}
+
+LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {
+    const int n = (int)boxes_.size();
+    const int *row = &blk_host_[(size_t)kind * (n + 1)];
+    LaunchCfg c;
+    c.B.tab = tab_[parity];
+    c.B.blk = blk_dev_ + (size_t)kind * (n + 1);
+    c.B.nbox = n;
+    if (b < 0) {
+        c.B.first = 0;
+        c.grid = (unsigned)row[n];
+    } else {
+        const int p = pos_[b];
+        c.B.first = row[p];
+        c.grid = (unsigned)(row[p + 1] - row[p]);
+    }
+    c.block = tune_.block[kind];
+    return c;
+}
+
+template <typename K, typename... A>
+void FusedEngine::launch(const char *name, Prof *prof, K kern, int kind, int b, int parity,
+                         hipStream_t st, A... args) {
+    const LaunchCfg c = cfg(kind, b, parity);
+    if (c.grid == 0) return;
+    Region reg(prof, name, st);
+    hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.block), 0, st, c.B, args...);
+}
+
+int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
+                       const DevFab *src_tab, int has_src, const HydroConst &hc,
+                       const double *dtp, hipStream_t st, int *status, Prof *prof) {
+    if (!have_prim || has_src)
+        launch("prim", prof, k_prim_c, ST_G4, b, parity, st, sb_tab, src_tab, has_src, hc, status);
+    launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening);
+    if (hc.ppm_type == 1) {
+        launch("trace_ppm", prof, k_ppm_c<0>, ST_G2, b, parity, st, hc, dtp);
+        launch("trace_ppm", prof, k_ppm_c<1>, ST_G2, b, parity, st, hc, dtp);
+        launch("trace_ppm", prof, k_ppm_c<2>, ST_G2, b, parity, st, hc, dtp);
+    } else {
+        launch("trace_plm", prof, k_plm_c<0>, ST_G2, b, parity, st, has_src, hc, dtp);
+        launch("trace_plm", prof, k_plm_c<1>, ST_G2, b, parity, st, has_src, hc, dtp);
+        launch("trace_plm", prof, k_plm_c<2>, ST_G2, b, parity, st, has_src, hc, dtp);
+    }
+    launch("riemann_normal", prof, k_riemann1<0>, ST_R1 + 0, b, parity, st, hc);
+    launch("riemann_normal", prof, k_riemann1<1>, ST_R1 + 1, b, parity, st, hc);
+    launch("riemann_normal", prof, k_riemann1<2>, ST_R1 + 2, b, parity, st, hc);
+    launch("transverse_1", prof, k_trans1<0>, ST_T1 + 0, b, parity, st, hc, dtp);
+    launch("transverse_1", prof, k_trans1<1>, ST_T1 + 1, b, parity, st, hc, dtp);
+    launch("transverse_1", prof, k_trans1<2>, ST_T1 + 2, b, parity, st, hc, dtp);
+    launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
+    if (has_src) {
+        launch("transverse_2", prof, k_trans2<0, 1>, ST_T2 + 0, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2<1, 1>, ST_T2 + 1, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
+    } else {
+        launch("transverse_2", prof, k_trans2<0, 0>, ST_T2 + 0, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
+    }
+    launch("consup", prof, k_update_c, ST_VALID, b, parity, st, hc, dtp, status);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
+
+int FusedEngine::fill(int b, int parity, const FillArgs &a, hipStream_t st, int *status,
+                      Prof *prof) {
+    const LaunchCfg c = cfg(ST_G4, b, parity);
+    Region reg(prof, "fill_clean_prim", st);
+    launch_fill_fused(c, a, st, status);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
+int FusedEngine::post(int b, int parity, const PostArgs &a, hipStream_t st, int *status,
+                      Prof *prof) {
+    const LaunchCfg c = cfg(ST_VALID, b, parity);
+    Region reg(prof, "clean_estdt", st);
+    launch_post_fused(c, a, st, status);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
 
 } // namespace camr

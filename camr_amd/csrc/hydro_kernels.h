@@ -31,7 +31,8 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st,
                  Prof *prof = nullptr);
 
 // ---- hydro_divu (Source/Hydro/Hydro_utils_K.H:12-169), node (i,j,k) -------
-__device__ __forceinline__ double divu_node(const Arr &q, int i, int j, int k,
+template <class QA>
+__device__ __forceinline__ double divu_node(const QA &q, int i, int j, int k,
                                             const HydroConst &hc) {
     const double dx = hc.dx[0], dy = hc.dx[1], dz = hc.dx[2];
     const int NSW = BC_NOSLIPWALL;
@@ -93,9 +94,9 @@ __device__ __forceinline__ double divu_node(const Arr &q, int i, int j, int k,
 // ---- adjust_fluxes for one face: hydro_artif_visc (hydro_artif_visc.H:7-64),
 // hydro_norm_spec_flx (Hydro_utils_K.H:207-219), hydro_ext_flx (:224-234).
 // f holds the NVAR fluxes of the face, u the conserved state (Sborder).
-template <int D>
-__device__ __forceinline__ void adjust_face(double *f, const Arr &divu,
-                                            const Arr &u, int i, int j, int k,
+template <int D, class DA, class UA>
+__device__ __forceinline__ void adjust_face(double *f, const DA &divu,
+                                            const UA &u, int i, int j, int k,
                                             double area, const HydroConst &hc) {
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int L1 = (D == 0) ? 1 : 0;

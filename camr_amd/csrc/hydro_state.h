@@ -27,7 +27,7 @@ __global__ void k_normalize(Range r, Arr S);
 __global__ void k_resetT(Range r, Arr S, HydroConst hc);
 __global__ void k_init_post(Range r, Arr S, HydroConst hc);
 __global__ void k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner,
-                           DomInfo D, int *status);
+                           DomInfo D, Arr ext, int has_ext, int *status);
 __global__ void k_resync(Range r, Box valid, Arr dst, const DevFab *src,
                          const int *owner, DomInfo D);
 __global__ void k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt,

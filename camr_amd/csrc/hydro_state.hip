@@ -195,16 +195,17 @@ k_init_post(Range r, Arr S, HydroConst hc) {
 // FillPatchSingleLevel of every cell of the grown box of `dst` from the
 // valid data of the level's boxes (`src` table).
 __global__ void __launch_bounds__(kBlock)
-k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D,
-           int *status) {
+k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D, Arr ext,
+           int has_ext, int *status) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     int m[3];
     unsigned odd;
     map_cell(D, i, j, k, m, &odd);
     const int ob = owner_of(D, owner, m);
-    if (ob < 0) { atomicOr(status, 1 << CAMR_ERR_SHAPE); return; }
-    const Arr s = src[ob].a;
+    if (ob < 0 && !has_ext) { atomicOr(status, 1 << CAMR_ERR_SHAPE); return; }
+    // cells no box owns take the level's exterior data
+    const Arr s = ob >= 0 ? src[ob].a : ext;
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
         const double v = s(m[0], m[1], m[2], n);

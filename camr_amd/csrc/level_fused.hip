@@ -13,6 +13,10 @@
 //   clean_state (CAMR_advance.cpp:146-154; CAMR.cpp:698-699), then the
 //   CFL estimate of the next computeNewDt (Timestep.H:18-61).
 //
+// Each launch covers a set of boxes (all boxes of the level, or one); ghost
+// cells that no box of the level owns (coarse-fine ghosts of a refined
+// level) take their values from the level's exterior array when one is set.
+//
 // Both are exact compositions of the reference's passes as long as no
 // density floor fires (enforce_min_density is the only non-pointwise piece;
 // normalize_species and the energy reset never change rho, rho*u):
@@ -27,36 +31,54 @@ namespace camr {
 namespace {
 
 __global__ void __launch_bounds__(kBlock)
-k_fill_fused(Range r, FillArgs a, int *status) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+k_fill_fused(Batch B, FillArgs a, int *status) {
+    BATCH_POINT(ST_G4)
     int m[3];
     unsigned odd;
     const bool phys = map_cell(a.dom, i, j, k, m, &odd);
     const int ob = owner_of(a.dom, a.owner, m);
-    if (ob < 0) { atomicOr(status, 1 << CAMR_ERR_SHAPE); return; }
-    const Arr s = a.sold_tab[ob].a;
     double v[NVAR];
+    double rho_old = 0.0, rhow_old = 0.0;
+    const bool valid = E.bx.contains(i, j, k);
+    if (ob >= 0) {
+        const Arr s = a.sold_tab[ob].a;
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) v[n] = s(m[0], m[1], m[2], n);
-    const bool valid = a.valid.contains(i, j, k);
-    if (valid) floor_detect(v[URHO], a.hc.small_dens, status);
-    const double rho_old = v[URHO], rhow_old = v[UMZ];
-    // clean_state(S_old) on the source cell
-    normalize_species(v);
-    reset_e_and_T(v, a.hc);
+        for (int n = 0; n < NVAR; ++n) v[n] = s(m[0], m[1], m[2], n);
+        if (valid) floor_detect(v[URHO], a.hc.small_dens, status);
+        rho_old = v[URHO];
+        rhow_old = v[UMZ];
+        // clean_state(S_old) on the source cell
+        normalize_species(v);
+        reset_e_and_T(v, a.hc);
+    } else if (a.has_ext) {
+        // exterior data (the FillPatch values of a coarse-fine ghost)
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) v[n] = a.ext(m[0], m[1], m[2], n);
+    } else {
+        atomicOr(status, 1 << CAMR_ERR_SHAPE);
+        return;
+    }
     // FillPatch copy with the reflect signs of physical boundaries
 #pragma unroll
     for (int n = 0; n < NVAR; ++n)
         if ((odd >> n) & 1u) v[n] = -v[n];
     // clean_state(Sborder): normalize_species on valid cells, propagated to
     // interior / periodic ghosts by FillBoundary; computeTemp everywhere
-    if (!phys) normalize_species(v);
+    if (!phys && ob >= 0) normalize_species(v);
     reset_e_and_T(v, a.hc);
+    const unsigned v0 = L.off(i, j, k);
+    const CView u = cview(L, slot_ptr<S_U>(E), NVAR);
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) a.sborder(i, j, k, n) = v[n];
-    prim_compact_point(v, a.q, i, j, k, a.hc, status);
+    for (int n = 0; n < NVAR; ++n) u.st(v0, n, v[n]);
+    double o[NQC], e;
+    if (prim_compact(v, o, e, a.hc, status)) {
+        const CView q = cview(L, slot_ptr<S_Q>(E), NQC);
+#pragma unroll
+        for (int n = 0; n < NQC; ++n) q.st(v0, n, o[n]);
+    }
     if (valid) {
+        const double dt = *a.dtp;
+        const Arr snew = E.snew;
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) {
             double x = v[n];
@@ -64,9 +86,9 @@ k_fill_fused(Range r, FillArgs a, int *status) {
                 double src = 0.0;
                 if (n == UMZ) src = a.g * rho_old;
                 else if (n == UEDEN) src = a.g * rhow_old;
-                x += a.dt * src;
+                x += dt * src;
             }
-            a.snew(i, j, k, n) = x;
+            snew(i, j, k, n) = x;
         }
     }
 }
@@ -77,27 +99,32 @@ __device__ __forceinline__ void clean_point(double *v, const HydroConst &hc) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_post_fused(Range r, PostArgs a, int *status) {
+k_post_fused(Batch B, PostArgs a, int *status) {
+    int lblk;
+    const BoxEnt &E = B.tab[batch_box(B, lblk)];
     int i, j, k;
     double dtv = 1.7976931348623157e308;
-    if (range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) {
+    // every thread reaches the block reduction below
+    if (range_point(range_of(E.bx), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k)) {
+        const Arr snew = E.snew, sold = E.sold;
+        const double dt = *a.dtp;
         double v[NVAR];
 #pragma unroll
-        for (int n = 0; n < NVAR; ++n) v[n] = a.snew(i, j, k, n);
+        for (int n = 0; n < NVAR; ++n) v[n] = snew(i, j, k, n);
         clean_point(v, a.hc);
         if (a.add_grav) {
-            const double s_umz = 0.5 * a.g * (v[URHO] - a.sold(i, j, k, URHO));
-            const double s_ued = 0.5 * a.g * (v[UMZ] - a.sold(i, j, k, UMZ));
+            const double s_umz = 0.5 * a.g * (v[URHO] - sold(i, j, k, URHO));
+            const double s_ued = 0.5 * a.g * (v[UMZ] - sold(i, j, k, UMZ));
 #pragma unroll
             for (int n = 0; n < NVAR; ++n) {
                 const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
-                v[n] += a.dt * src;
+                v[n] += dt * src;
             }
             clean_point(v, a.hc);
         }
         clean_point(v, a.hc);
 #pragma unroll
-        for (int n = 0; n < NVAR; ++n) a.snew(i, j, k, n) = v[n];
+        for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) = v[n];
         // estTimeStep on the new state
         const double rho = v[URHO];
         const double rhoInv = 1.0 / rho;
@@ -113,17 +140,14 @@ k_post_fused(Range r, PostArgs a, int *status) {
 
 } // namespace
 
-void launch_fill_fused(const FillArgs &a, hipStream_t st, int *status, Prof *prof) {
-    const Box G = grow(a.valid, CAMR_NGROW);
-    Range r = range_of(G);
-    Region reg(prof, "fill_clean_prim", st);
-    hipLaunchKernelGGL(k_fill_fused, dim3(grid_for(r.n)), dim3(kBlock), 0, st, r, a, status);
+void launch_fill_fused(const LaunchCfg &c, const FillArgs &a, hipStream_t st, int *status) {
+    if (c.grid == 0) return;
+    hipLaunchKernelGGL(k_fill_fused, dim3(c.grid), dim3(c.block), 0, st, c.B, a, status);
 }
 
-void launch_post_fused(const PostArgs &a, hipStream_t st, int *status, Prof *prof) {
-    Range r = range_of(a.valid);
-    Region reg(prof, "clean_estdt", st);
-    hipLaunchKernelGGL(k_post_fused, dim3(grid_for(r.n)), dim3(kBlock), 0, st, r, a, status);
+void launch_post_fused(const LaunchCfg &c, const PostArgs &a, hipStream_t st, int *status) {
+    if (c.grid == 0) return;
+    hipLaunchKernelGGL(k_post_fused, dim3(c.grid), dim3(c.block), 0, st, c.B, a, status);
 }
 
 } // namespace camr

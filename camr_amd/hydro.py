@@ -59,7 +59,7 @@ def hydro_params(prob: Problem) -> L.HydroParams:
 class Level:
     """One AMR level advanced on the GPU by the C-ABI level driver."""
 
-    def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO):
+    def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO, batched: bool = True):
         self.prob = prob
         d = L.LevelDesc()
         L.lib().camr_default_level_desc(C.byref(d))
@@ -73,6 +73,7 @@ class Level:
         d.nbox = len(prob.boxes)
         d.box_lohi = self._lohi.ctypes.data_as(C.POINTER(C.c_int32))
         d.engine = engine
+        d.batched = int(bool(batched))
         self._desc = d
         h = C.c_void_p()
         L.check(L.lib().camr_level_create(C.byref(d), C.byref(h)))
@@ -141,6 +142,27 @@ class Level:
     @property
     def time(self):
         return L.lib().camr_level_time(self._h)
+
+    def set_exterior(self, data):
+        """Values for ghost cells no box owns: (NVAR, nz, ny, nx) over the domain."""
+        n = [self.prob.domhi[d] - self.prob.domlo[d] + 1 for d in range(3)]
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.shape == (L.NVAR, n[2], n[1], n[0])
+        L.check(L.lib().camr_level_set_exterior(self._h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    @property
+    def tuning(self):
+        """(block sizes of the CAMR_NSTAGE stage ranges, box order)."""
+        blk = (C.c_int32 * L.NSTAGE)()
+        order = C.c_int32()
+        L.check(L.lib().camr_level_get_tuning(self._h, blk, C.byref(order)))
+        return list(blk), order.value
+
+    @tuning.setter
+    def tuning(self, t):
+        blk, order = t
+        arr = (C.c_int32 * L.NSTAGE)(*blk)
+        L.check(L.lib().camr_level_set_tuning(self._h, arr, int(order)))
 
     def profile(self, enable=True, kernel=None):
         """Time launches (all, or only those named `kernel`) with hipEvents."""

@@ -5,10 +5,18 @@
   resolution and CFL of the CPU configuration).
 * ``smooth_periodic``: 256^3 (or n^3) periodic PPM FP64 advance of the seeded
   smooth field of ``problems.SmoothField``, CFL 0.4, 20 steps.
+* ``amr_levels``: the two levels of the batched AMR benchmark: level 0 is
+  128^3 periodic cut into 32^3 boxes; level 1 (refinement ratio 2, a 256^3
+  index space) covers the cells whose centre lies within 0.3 of the domain
+  centre, clustered into boxes aligned to 8 cells with sides 8..64
+  (``amr.cluster``, efficiency 0.8: about 300 boxes).  Both use the smooth
+  field at their own cell centres; level-1 ghost cells outside level 1 take
+  the smooth field as exterior data (standing for the coarse-level
+  interpolation).
 """
 from __future__ import annotations
 
-from . import problems
+from . import amr, problems
 from .hydro import Problem
 from ._lib import INTERIOR, OUTFLOW
 
@@ -56,3 +64,28 @@ def smooth_init(prob: Problem, b, seed=problems.SEED_FIELD):
     n = prob.domhi[0] - prob.domlo[0] + 1
     lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
     return smooth_field(seed).state(n, lo, hi)
+
+
+def refined_level(n=256, radius=0.3, blocking_factor=8, max_size=64, efficiency=0.8,
+                  ppm_type=1, cfl=0.4):
+    """Level covering the sphere of `radius` around the centre of an n^3
+    periodic index space (the refined level of ``amr_levels``)."""
+    tags = amr.sphere_tags(n, radius)
+    boxes = amr.cluster(tags, blocking_factor, max_size, efficiency)
+    dx = 1.0 / n
+    return Problem(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(dx,) * 3,
+                   bclo=(INTERIOR,) * 3, bchi=(INTERIOR,) * 3,
+                   is_periodic=(1, 1, 1), boxes=boxes,
+                   params=dict(ppm_type=ppm_type, cfl=cfl))
+
+
+def exterior(prob: Problem, seed=problems.SEED_FIELD):
+    """The smooth field over the whole domain of `prob` (exterior data)."""
+    n = prob.domhi[0] - prob.domlo[0] + 1
+    return smooth_field(seed).state(n, prob.domlo, prob.domhi)
+
+
+def amr_levels(n0=128):
+    """(level 0, level 1) of the batched AMR benchmark."""
+    return (smooth_periodic(n=n0, ppm_type=1, cfl=0.4, max_grid=32),
+            refined_level(n=2 * n0))

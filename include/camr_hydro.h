@@ -144,7 +144,13 @@ typedef struct camr_level_desc {
     double cfl, change_max, init_shrink, max_dt, fixed_dt, stop_time;
     int32_t nbox;
     const int32_t *box_lohi; /* host array of nbox*6 ints: lo[3], hi[3] */
-    int32_t engine;          /* 0 = fused (default), 1 = staged per-stage kernels */
+    int32_t engine;          /* 0 = fused, or staged for boxes too large for the fused
+                                engine's 32-bit layout (default); 1 = staged per-stage
+                                kernels; 2 = fused only */
+    int32_t batched;         /* fused engine: 1 = each stage is one launch over every
+                                box of the level (default); 0 = one launch per box,
+                                the per-box loop of construct_hydro_source
+                                (CAMR_construct_hydro_source.cpp:59-71) */
 } camr_level_desc;
 
 void camr_default_level_desc(camr_level_desc *d);
@@ -167,6 +173,22 @@ int camr_level_advance(camr_level *L, double dt);
 int camr_level_run(camr_level *L, int32_t nsteps, double *dts);
 int camr_level_sync(camr_level *L);
 double camr_level_time(const camr_level *L);
+
+/* Exterior data of a level that does not cover its domain: NVAR components
+ * over the whole domain (host, Array4 order).  Ghost cells that no box of
+ * the level owns take these values where FillPatch would interpolate from
+ * the coarser level (FillPatch, CAMR.cpp:1247; CAMR_setup.cpp:141-144).
+ * Without it such ghost cells are an error (CAMR_ERR_SHAPE). */
+int camr_level_set_exterior(camr_level *L, const double *host);
+
+/* Launch configuration of the fused engine: block size (64, 128 or 256) of
+ * each of the CAMR_NSTAGE stage ranges (grow(bx,4), grow(bx,2), first-flux
+ * faces x/y/z, transdo faces x/y/z, nodes, final faces x/y/z, bx) and the
+ * box order of the batched launches (0 level order, 1 largest first).  No
+ * reference counterpart: set by the autotuner (camr_amd/autotune.py). */
+#define CAMR_NSTAGE 13
+int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order);
+int camr_level_get_tuning(camr_level *L, int32_t block[CAMR_NSTAGE], int32_t *order);
 
 /* Named hipEvent timers around kernel launches on the level's stream -- the
  * counterpart of the reference's BL_PROFILE regions (TinyProfiler).  With a

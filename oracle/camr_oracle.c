@@ -33,6 +33,7 @@
 #define ERR_NEG_ENERGY 1
 #define ERR_ZERO_DENS  2
 #define ERR_DT         4
+#define ERR_UNOWNED    8  /* a ghost cell no box owns and no exterior data */
 
 static const int UPASS[NSPEC] = {7, 8, 9};  /* IndexDefines.cpp:4-22 */
 static const int QPASS[NSPEC] = {8, 9, 10};
@@ -1597,6 +1598,7 @@ struct ora_level {
     obox *box;
     ora_fab *Snew, *Sold, *Sborder, *hsrc, *oldsrc, *newsrc, *srchyd;
     int *owner;          /* domain cell -> box index (or -1) */
+    double *ext;         /* exterior data over the domain (NVAR comps), or NULL */
     int64_t dn[3];
     double time;
     int err;
@@ -1646,6 +1648,12 @@ ora_level *ora_level_create(const ora_params *p, int nbox, const int *lohi) {
     return L;
 }
 
+void ora_level_set_exterior(ora_level *L, const double *data) {
+    const int64_t nc = L->dn[0] * L->dn[1] * L->dn[2] * O_NVAR;
+    if (!L->ext) L->ext = (double *)malloc(sizeof(double) * (size_t)nc);
+    memcpy(L->ext, data, sizeof(double) * (size_t)nc);
+}
+
 void ora_level_destroy(ora_level *L) {
     if (!L) return;
     for (int b = 0; b < L->nbox; ++b) {
@@ -1656,7 +1664,7 @@ void ora_level_destroy(ora_level *L) {
     }
     free(L->Snew); free(L->Sold); free(L->Sborder); free(L->hsrc);
     free(L->oldsrc); free(L->newsrc); free(L->srchyd); free(L->box);
-    free(L->owner); free(L);
+    free(L->owner); free(L->ext); free(L);
 }
 
 int ora_level_nbox(const ora_level *L) { return L->nbox; }
@@ -1848,8 +1856,19 @@ static void fill_patch(ora_level *L, ora_fab *dst, int b, ora_fab *src) {
         int c[3] = {i, j, k}, m[3];
         double s;
         map_cell(&L->P, c, n, m, &s);
-        int ob = L->owner[dom_index(L, m)];
-        double v = FA(&src[ob], m[0], m[1], m[2], n);
+        const int64_t di = dom_index(L, m);
+        int ob = L->owner[di];
+        double v;
+        if (ob >= 0) {
+            v = FA(&src[ob], m[0], m[1], m[2], n);
+        } else if (L->ext) {
+            /* a cell no box owns: the exterior data stands for the
+             * coarse-level FillPatch values */
+            v = L->ext[di + (int64_t)n * L->dn[0] * L->dn[1] * L->dn[2]];
+        } else {
+            L->err |= ERR_UNOWNED;
+            v = 0.0;
+        }
         FA(dst, i, j, k, n) = (s < 0.0) ? -v : v;
     }
 }

@@ -100,6 +100,9 @@ typedef struct ora_level ora_level;
 ora_level *ora_level_create(const ora_params *p, int nbox,
                             const int *box_lohi /* nbox*6 */);
 void   ora_level_destroy(ora_level *L);
+/* Values of the domain cells no box owns (NVAR comps over the domain,
+ * Array4 order): the FillPatch result for such ghost cells. */
+void   ora_level_set_exterior(ora_level *L, const double *data);
 int    ora_level_nbox(const ora_level *L);
 /* Copy a box's valid data in/out (Array4 order, NVAR components). */
 void   ora_level_set_box(ora_level *L, int b, const double *data);

@@ -83,6 +83,7 @@ def lib():
         L.ora_level_create.argtypes = [C.POINTER(OraParams), C.c_int, ip]
         L.ora_level_create.restype = C.c_void_p
         L.ora_level_destroy.argtypes = [C.c_void_p]
+        L.ora_level_set_exterior.argtypes = [C.c_void_p, dp]
         L.ora_level_set_box.argtypes = [C.c_void_p, C.c_int, dp]
         L.ora_level_get_box.argtypes = [C.c_void_p, C.c_int, dp]
         L.ora_level_init_post.argtypes = [C.c_void_p]
@@ -189,6 +190,12 @@ class Level:
         a = np.empty(self.shape(b))
         lib().ora_level_get_box(self._h, b, _dp(a))
         return a
+
+    def set_exterior(self, data):
+        n = [self.p.domhi[d] - self.p.domlo[d] + 1 for d in range(3)]
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.shape == (NVAR, n[2], n[1], n[0])
+        lib().ora_level_set_exterior(self._h, _dp(a))
 
     def init_post(self):
         lib().ora_level_init_post(self._h)

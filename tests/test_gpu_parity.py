@@ -42,9 +42,15 @@ def _oracle_params(O, prob):
         domlo=prob.domlo, domhi=prob.domhi, is_periodic=prob.is_periodic, dx=prob.dx)
 
 
-def _run_both(O, prob, init, nsteps, engine=_lib.ENGINE_AUTO):
+def _run_both(O, prob, init, nsteps, engine=_lib.ENGINE_AUTO, batched=True, ext=None,
+              tuning=None):
     ol = O.Level(_oracle_params(O, prob), prob.boxes)
-    gl = Level(prob, engine=engine)
+    gl = Level(prob, engine=engine, batched=batched)
+    if tuning is not None:
+        gl.tuning = tuning
+    if ext is not None:
+        ol.set_exterior(ext)
+        gl.set_exterior(ext)
     for b in range(len(prob.boxes)):
         U = init(prob, b)
         ol.set_box(b, U)
@@ -172,10 +178,11 @@ def test_smooth_periodic_bitwise(oracle, ppm):
     assert_bitwise(gl.get_box(0), ol.get_box(0), "smooth 32^3 after 5 steps")
 
 
-def test_multibox_level_bitwise(oracle):
+@pytest.mark.parametrize("batched", [True, False])
+def test_multibox_level_bitwise(oracle, batched):
     prob = workloads.smooth_periodic(n=32, ppm_type=1, max_grid=16)
     assert len(prob.boxes) == 8
-    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 3)
+    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 3, batched=batched)
     assert np.array_equal(odts, gdts)
     for b in range(8):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"box {b}")
@@ -240,3 +247,39 @@ def test_density_floor_path_bitwise(oracle, engine):
     assert np.array_equal(odts, gdts)
     for b in range(len(prob.boxes)):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"floor box {b}")
+
+
+# ------------------------------------------------- batched refined level
+@pytest.mark.parametrize("batched,tuning", [
+    (True, None),
+    (False, None),
+    (True, ([64, 128, 256, 64, 128, 256, 64, 128, 64, 256, 128, 64, 128], 1)),
+])
+def test_refined_level_with_exterior_bitwise(oracle, batched, tuning):
+    # uneven boxes (sides 8..32) covering a sphere of a 64^3 periodic index
+    # space; ghost cells outside the level come from exterior data
+    prob = workloads.refined_level(n=64, radius=0.3, blocking_factor=8, max_size=32,
+                                   efficiency=0.8)
+    assert len(prob.boxes) > 10
+    ext = workloads.exterior(prob)
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, batched=batched, ext=ext,
+                                   tuning=tuning)
+    assert np.array_equal(odts, gdts)
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"refined box {b}")
+
+
+def test_unowned_ghosts_without_exterior_are_an_error():
+    prob = workloads.refined_level(n=32, radius=0.3, blocking_factor=8, max_size=16)
+    gl = Level(prob)
+    for b in range(len(prob.boxes)):
+        lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+        gl.set_box(b, workloads.smooth_field().state(32, lo, hi))
+    gl.init_post()
+    with pytest.raises(_lib.CamrError):
+        gl.run(1)
