@@ -89,6 +89,74 @@ def cpu_baseline(n: int, cfl: float, U0, nsteps: int = 2):
                       f"{el:.1f} s wall on {O.lib().ora_get_threads()} threads"}
 
 
+def _amr_level(prob, mode, ext, init_field):
+    """Level in one of the launch modes: per-box, batched, batched + graph."""
+    from camr_amd.hydro import Level
+    L = Level(prob, batched=(mode != "per_box"), graph=(mode == "graph"))
+    if ext is not None:
+        L.set_exterior(ext)
+    for b in range(len(prob.boxes)):
+        L.set_box(b, init_field(b))
+    L.init_post()
+    return L
+
+
+def bench_amr(args) -> int:
+    """Batched AMR levels: per-box launches vs one launch per stage over all
+    boxes vs captured step graphs, on the same two-level hierarchy."""
+    import numpy as np
+    import torch
+    from camr_amd import workloads
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    lev0, lev1 = workloads.amr_levels(args.amr_n0)
+    ext1 = workloads.exterior(lev1)
+    levels = []
+    for name, prob, ext in (("level0", lev0, None), ("level1", lev1, ext1)):
+        n = prob.domhi[0] + 1
+        if ext is not None:
+            def init(b, prob=prob, ext=ext):
+                lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+                return ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+        else:
+            def init(b, prob=prob):
+                return workloads.smooth_init(prob, b)
+        cells = sum((b[3] - b[0] + 1) * (b[4] - b[1] + 1) * (b[5] - b[2] + 1) for b in prob.boxes)
+        res = {"boxes": len(prob.boxes), "cells": cells, "index_space": n}
+        finals = {}
+        for mode in ("per_box", "batched", "graph"):
+            L = _amr_level(prob, mode, ext, init)
+            L.run(args.warmup)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L.run(args.steps)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            res[mode] = {"ms_per_step": el / args.steps * 1e3,
+                         "cell_updates_per_s": cells * args.steps / el}
+            finals[mode] = [L.get_box(b) for b in range(len(prob.boxes))]
+            L.close()
+        res["identical"] = all(np.array_equal(a, b) for m in ("batched", "graph")
+                               for a, b in zip(finals["per_box"], finals[m]))
+        res["speedup_batched"] = res["per_box"]["ms_per_step"] / res["batched"]["ms_per_step"]
+        res["speedup_graph"] = res["per_box"]["ms_per_step"] / res["graph"]["ms_per_step"]
+        levels.append((name, res))
+    out = {"metric": "FP64 cell-updates/s of the batched AMR level step (PPM)",
+           "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"level 0 {args.amr_n0}^3 periodic in 32^3 boxes; level 1 "
+                                  f"(ratio 2) on the sphere r=0.3, boxes 8..64 clustered "
+                                  f"(blocking factor 8, efficiency 0.8), exterior ghosts "
+                                  f"from the smooth field"},
+           "levels": dict(levels)}
+    tot_cells = sum(r["cells"] for _, r in levels)
+    t_graph = sum(r["graph"]["ms_per_step"] for _, r in levels)
+    t_box = sum(r["per_box"]["ms_per_step"] for _, r in levels)
+    out["value"] = tot_cells / (t_graph * 1e-3)
+    out["speedup_vs_per_box"] = t_box / t_graph
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +165,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["c1", "amr"], default="c1",
+                    help="c1: 256^3 PPM level step (default); amr: batched "
+                         "AMR levels against per-box launches")
+    ap.add_argument("--amr-n0", type=int, default=128)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run one elementwise copy "
                          "(torch.neg) of MIB MiB of float64: a kernel of known "
@@ -107,6 +179,8 @@ def main() -> int:
               "supported (the level would shard by boxes, with one halo "
               "exchange and one dt all-reduce per step)", file=sys.stderr)
         return 2
+    if args.workload == "amr":
+        return bench_amr(args)
 
     import torch
     from camr_amd import _lib, workloads

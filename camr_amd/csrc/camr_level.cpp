@@ -174,7 +174,22 @@ class HydroLevel {
     int parity() const { return a_is_new_ ? 0 : 1; }
     void set_dt_dev(double dt);
 
+    // captured level-step graphs (one per buffer parity)
+    StepCtl *ctl_ = nullptr;
+    double *ctl_dts_ = nullptr;
+    int ctl_cap_ = 0;
+    hipStream_t gstream_ = nullptr;
+    hipGraph_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    int capture_graphs();
+    void release_graphs();
+
   public:
+    int run(int nsteps, double *dts);
+    int run_graph(int nsteps, double *dts);
+    bool use_graph() const {
+        return desc_.graph != 0 && fused_ && batched_ && !grav_src_values_;
+    }
     int set_exterior(const double *h);
     int set_tuning(const Tuning &t);
     int get_tuning(Tuning *t) const;
@@ -206,6 +221,7 @@ int HydroLevel::set_tuning(const Tuning &t) {
     int rc = resolve();
     if (rc) return rc;
     CAMR_HIP(hipStreamSynchronize(stream_));
+    release_graphs();
     return engine_.set_tuning(t);
 }
 int HydroLevel::get_tuning(Tuning *t) const {
@@ -225,6 +241,10 @@ void HydroLevel::release() {
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_);
     tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
     fr(dt_dev_); fr(ext_); dt_dev_ = nullptr; ext_ = nullptr;
+    release_graphs();
+    fr(ctl_); fr(ctl_dts_); ctl_ = nullptr; ctl_dts_ = nullptr; ctl_cap_ = 0;
+    if (gstream_) (void)hipStreamDestroy(gstream_);
+    gstream_ = nullptr;
     fr(q_); fr(qaux_); fr(srcq_);
     for (auto &f : flx_) { fr(f); f = nullptr; }
     if (h_status_) (void)hipHostFree(h_status_);
@@ -670,6 +690,169 @@ int HydroLevel::advance_fused(double dt) {
     return CAMR_OK;
 }
 
+void HydroLevel::release_graphs() {
+    for (int p = 0; p < 2; ++p) {
+        if (gexec_[p]) (void)hipGraphExecDestroy(gexec_[p]);
+        if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
+        gexec_[p] = nullptr;
+        graph_[p] = nullptr;
+    }
+}
+
+// Capture one level step per buffer parity: device dt, fill, the hydro
+// stages and the post pass, every stage one launch over all boxes, reading
+// dt from the control block.  No memset or host work inside.
+int HydroLevel::capture_graphs() {
+    if (!gstream_) CAMR_HIP(hipStreamCreateWithFlags(&gstream_, hipStreamNonBlocking));
+    DtParams P;
+    P.fixed_dt = desc_.fixed_dt;
+    P.max_dt = desc_.max_dt;
+    P.cfl = desc_.cfl;
+    P.change_max = desc_.change_max;
+    P.init_shrink = desc_.init_shrink;
+    P.stop_time = desc_.stop_time;
+    const double *dtp = &ctl_->dt;
+    engine_.set_halt(&ctl_->halt);
+    int rc = CAMR_OK;
+    for (int p = 0; p < 2 && !rc; ++p) {
+        FillArgs fa;
+        fa.sold_tab = p == 0 ? tab_b_ : tab_a_;  // parity 0: A is new, B old
+        fa.owner = owner_;
+        fa.ext = extv_;
+        fa.has_ext = ext_ ? 1 : 0;
+        fa.dom = dom_;
+        fa.hc = hc_;
+        fa.dtp = dtp;
+        fa.g = desc_.const_grav;
+        fa.add_grav = desc_.add_grav_src;
+        PostArgs pa;
+        pa.hc = hc_;
+        pa.dtp = dtp;
+        pa.g = desc_.const_grav;
+        pa.add_grav = desc_.add_grav_src;
+        pa.dtmin = dtmin_;
+        if (hipStreamBeginCapture(gstream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_error("stream capture failed to start");
+            rc = CAMR_ERR_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(64), 0, gstream_, ctl_, dtmin_, status_, P);
+        rc = engine_.fill(-1, p, fa, gstream_, status_, nullptr);
+        if (!rc) rc = engine_.hydro(-1, p, true, nullptr, nullptr, 0, hc_, dtp, gstream_, status_, nullptr);
+        if (!rc) rc = engine_.post(-1, p, pa, gstream_, status_, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(gstream_, &g);
+        if (!rc && (ec != hipSuccess || !g)) { set_error("stream capture failed"); rc = CAMR_ERR_HIP; }
+        graph_[p] = g;
+        if (!rc && hipGraphInstantiate(&gexec_[p], g, nullptr, nullptr, 0) != hipSuccess) {
+            set_error("graph instantiation failed");
+            rc = CAMR_ERR_HIP;
+        }
+    }
+    engine_.set_halt(nullptr);
+    if (rc) release_graphs();
+    return rc;
+}
+
+// camr_level_run on the host: the reference dt sequence, one step at a time.
+int HydroLevel::run(int nsteps, double *dts) {
+    if (use_graph()) return run_graph(nsteps, dts);
+    double dt = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = (s == 0) ? initial_dt(&dt) : new_dt(dt, &dt);
+        if (rc) return rc;
+        if (dts) dts[s] = dt;
+        rc = advance(dt);
+        if (rc) return rc;
+    }
+    return sync();
+}
+
+// The same sequence with every step a replay of a captured graph: dt is
+// computed on the device, so the host only queues replays.  A step that
+// trips a density floor halts the remaining replays; the host then redoes
+// that step exactly (resolve) and continues from there.
+int HydroLevel::run_graph(int nsteps, double *dts) {
+    if (nsteps <= 0) return sync();
+    if (needs_ext_ && !ext_) {
+        set_error("ghost cells outside every box of the level: set exterior data");
+        return CAMR_ERR_SHAPE;
+    }
+    if (!ctl_) CAMR_HIP(hipMalloc(&ctl_, sizeof(StepCtl)));
+    if (ctl_cap_ < nsteps) {
+        if (ctl_dts_) (void)hipFree(ctl_dts_);
+        ctl_dts_ = nullptr;
+        CAMR_HIP(hipMalloc(&ctl_dts_, nsteps * sizeof(double)));
+        ctl_cap_ = nsteps;
+    }
+    if (!gexec_[0]) {
+        int rc = capture_graphs();
+        if (rc) return rc;
+    }
+    int done = 0;
+    double dt_old = 0.0;
+    bool first = true;
+    while (done < nsteps) {
+        int rc = resolve();
+        if (rc) return rc;
+        if (!dt_ready_ && desc_.fixed_dt <= 0.0) {
+            const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;
+            launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
+            for (size_t b = 0; b < box_.size(); ++b)
+                launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2],
+                          dtmin_, status_);
+            CAMR_HIP(hipGetLastError());
+        }
+        StepCtl c{};
+        c.dt = dt_old;
+        c.time = time_;
+        c.first = first ? 1 : 0;
+        c.halt = 0;
+        c.nstep = 0;
+        c.cap = ctl_cap_;
+        c.dts = ctl_dts_;
+        // from pageable memory: the copy completes before the call returns
+        CAMR_HIP(hipMemcpyAsync(ctl_, &c, sizeof(c), hipMemcpyHostToDevice, stream_));
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        const bool a0 = a_is_new_;
+        const int todo = nsteps - done;
+        for (int s = 0; s < todo; ++s) {
+            a_is_new_ = !a_is_new_;  // swapTimeLevels
+            CAMR_HIP(hipGraphLaunch(gexec_[parity()], stream_));
+        }
+        CAMR_HIP(hipMemcpyAsync(&c, ctl_, sizeof(c), hipMemcpyDeviceToHost, stream_));
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        // the host buffer roles follow the steps actually taken
+        a_is_new_ = (c.nstep & 1) ? !a0 : a0;
+        snew_.clear();
+        sold_.clear();
+        for (size_t b = 0; b < box_.size(); ++b) {
+            const Arr A = dense(bufA_[b], box_[b]), Bv = dense(bufB_[b], box_[b]);
+            snew_.push_back(a_is_new_ ? A : Bv);
+            sold_.push_back(a_is_new_ ? Bv : A);
+        }
+        if (c.nstep > 0 && dts)
+            CAMR_HIP(hipMemcpy(dts + done, ctl_dts_, c.nstep * sizeof(double), hipMemcpyDeviceToHost));
+        time_ = c.time;
+        done += c.nstep;
+        if (c.nstep > 0) {
+            first = false;
+            dt_old = c.dt;
+            last_dt_ = c.dt;
+            pending_ = true;   // resolve() redoes a floored last step
+            dt_ready_ = true;  // the post pass left the CFL estimate in dtmin_
+        }
+        rc = check_status();
+        if (rc) return rc;
+        if (c.halt && c.nstep == 0) {
+            rc = resolve();
+            if (rc) return rc;
+            if (!pending_) { set_error("captured step made no progress"); return CAMR_ERR_HIP; }
+        }
+    }
+    return sync();
+}
+
 // Settle a fused step: wait for it and, if a density floor was flagged,
 // recompute it with the exact pass-by-pass sequence.
 int HydroLevel::resolve() {
@@ -680,10 +863,19 @@ int HydroLevel::resolve() {
     if (*h_status_ & kFloorFlag) {
         *h_status_ &= ~kFloorFlag;
         CAMR_HIP(hipMemcpyAsync(status_, h_status_, sizeof(int), hipMemcpyHostToDevice, stream_));
+        set_dt_dev(last_dt_);  // graph replays keep dt on the device only
         int rc = advance_exact(last_dt_);
         dt_ready_ = false;
         if (rc) return rc;
+        // the floor bit only gates enforce_min_density within a step: the
+        // redone step leaves it set, the next step starts without it
+        CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
         CAMR_HIP(hipStreamSynchronize(stream_));
+        if (*h_status_ & kFloorFlag) {
+            *h_status_ &= ~kFloorFlag;
+            CAMR_HIP(hipMemcpyAsync(status_, h_status_, sizeof(int), hipMemcpyHostToDevice, stream_));
+            CAMR_HIP(hipStreamSynchronize(stream_));
+        }
     }
     return check_status();
 }
@@ -793,6 +985,7 @@ void camr_default_level_desc(camr_level_desc *d) {
     d->stop_time = -1.0;
     d->engine = 0;
     d->batched = 1;
+    d->graph = 0;
 }
 
 const char *camr_last_error(void) { return g_err; }
@@ -950,15 +1143,7 @@ int camr_level_new_dt(camr_level *L, double dt_old, double *dt) {
 int camr_level_advance(camr_level *L, double dt) { return L ? L->impl.advance(dt) : CAMR_ERR_PARAM; }
 int camr_level_run(camr_level *L, int32_t nsteps, double *dts) {
     if (!L) return CAMR_ERR_PARAM;
-    double dt = 0.0;
-    for (int s = 0; s < nsteps; ++s) {
-        int rc = (s == 0) ? L->impl.initial_dt(&dt) : L->impl.new_dt(dt, &dt);
-        if (rc) return rc;
-        if (dts) dts[s] = dt;
-        rc = L->impl.advance(dt);
-        if (rc) return rc;
-    }
-    return L->impl.sync();
+    return L->impl.run(nsteps, dts);
 }
 int camr_level_sync(camr_level *L) { return L ? L->impl.sync() : CAMR_ERR_PARAM; }
 int camr_level_set_exterior(camr_level *L, const double *host) {

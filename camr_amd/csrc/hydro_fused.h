@@ -138,6 +138,7 @@ struct Batch {
     const int *__restrict__ blk;  // block prefix of this stage kind (nbox + 1)
     int nbox;
     int first;                    // block id of this launch's first block
+    const int *halt;              // graph replays: non-zero = skip (or NULL)
 };
 
 // Box of this block and the block's index within the box's stage range.
@@ -156,6 +157,7 @@ __device__ __forceinline__ int batch_box(const Batch &B, int &lblock) {
 // First statements of every kernel: the block's box E (and its layout L)
 // and the point (i,j,k) of the stage range this thread covers.
 #define BATCH_POINT(KIND)                                                      \
+    if (B.halt && *B.halt) return;                                             \
     int lblk_;                                                                 \
     const BoxEnt &E = B.tab[batch_box(B, lblk_)];                              \
     const Canon L = E.L;                                                       \
@@ -238,6 +240,9 @@ class FusedEngine {
     int nbox() const { return (int)boxes_.size(); }
 
     LaunchCfg cfg(int kind, int b, int parity) const;
+    // launches made while set read this word and skip their work when it is
+    // non-zero (the halt flag of captured step graphs)
+    void set_halt(const int *h) { halt_ = h; }
 
   private:
     template <typename K, typename... A>
@@ -253,6 +258,7 @@ class FusedEngine {
     Tuning tune_{};
     BoxEnt *tab_[2] = {nullptr, nullptr};
     int *blk_dev_ = nullptr;
+    const int *halt_ = nullptr;
     Scratch scratch_;
 };
 

@@ -613,6 +613,7 @@ LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {
         c.grid = (unsigned)(row[p + 1] - row[p]);
     }
     c.block = tune_.block[kind];
+    c.B.halt = halt_;
     return c;
 }
 

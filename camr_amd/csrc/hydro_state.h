@@ -16,6 +16,21 @@ struct DomInfo {
     int domlo[3], domhi[3], periodic[3], bclo[3], bchi[3];
 };
 
+// Device-resident step control of captured level-step graphs.
+struct StepCtl {
+    double dt;      // dt of the step in progress (read by the step kernels)
+    double time;    // level time
+    int first;      // next dt is computeInitialDt's, not computeNewDt's
+    int halt;       // set when a step must be redone on the host
+    int nstep;      // steps taken since the host last reset it
+    int cap;        // capacity of dts
+    double *dts;    // dt of each step taken
+};
+struct DtParams {
+    double fixed_dt, max_dt, cfl, change_max, init_shrink, stop_time;
+};
+__global__ void k_step_dt(StepCtl *c, unsigned long long *dtmin, int *status, DtParams P);
+
 __global__ void k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status);
 __global__ void k_srctoprim(Range r, Arr q, Arr qa, Arr src, Arr srcq);
 __global__ void k_estdt(Range r, Arr u, double dx, double dy, double dz,

@@ -310,4 +310,52 @@ __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v)
     if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
+// ----------------------------------------------------- device dt control
+// First node of a captured level step: estTimeStep + computeNewDt (or
+// computeInitialDt on the first step) of CAMR.cpp:474-676 from the CFL
+// minimum of the previous step, in the host driver's operation order; then
+// re-arms the minimum for this step's estimate.  A density floor flagged by
+// the previous step (or a bad dt) halts the graph: every later kernel
+// returns at once and the host finishes the step.
+__global__ void k_step_dt(StepCtl *c, unsigned long long *dtmin, int *status, DtParams P) {
+    if (threadIdx.x != 0 || c->halt) return;
+    if (*status & kFloorFlag) { c->halt = 1; return; }
+    double est;
+    if (P.fixed_dt > 0.0) {
+        est = P.fixed_dt;
+    } else {
+        double estdt = P.max_dt;
+        const double max_dt_over_cfl = P.max_dt / P.cfl;
+        double estdt_hydro = max_dt_over_cfl;
+        const double red = __longlong_as_double((long long)*dtmin);
+        estdt_hydro = vmin(estdt_hydro, red);
+        estdt_hydro *= P.cfl;
+        if (estdt_hydro < estdt) estdt = estdt_hydro;
+        est = estdt;
+    }
+    double dt_0 = 1.7976931348623157e308;
+    if (c->first) {
+        dt_0 = vmin(dt_0, 1 * (P.init_shrink * est));
+    } else {
+        double dt_min = est;
+        if (P.fixed_dt <= 0.0) dt_min = vmin(dt_min, P.change_max * c->dt);
+        dt_0 = vmin(dt_0, 1 * dt_min);
+    }
+    const double dt_eps = 0.001 * dt_0;
+    if (P.stop_time >= 0.0 && (c->time + dt_0) > (P.stop_time - dt_eps)) dt_0 = P.stop_time - c->time;
+    const double dt = dt_0 / 1;
+    if (!(dt > 0.0)) {
+        atomicOr(status, 1 << CAMR_ERR_DT);
+        c->halt = 1;
+        return;
+    }
+    c->first = 0;
+    c->dt = dt;
+    if (c->nstep < c->cap) c->dts[c->nstep] = dt;
+    c->nstep += 1;
+    c->time += dt;
+    *dtmin = 0x7fefffffffffffffULL;  // DBL_MAX for this step's estimate
+}
+
 } // namespace camr
+

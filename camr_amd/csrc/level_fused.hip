@@ -100,6 +100,7 @@ __device__ __forceinline__ void clean_point(double *v, const HydroConst &hc) {
 
 __global__ void __launch_bounds__(kBlock)
 k_post_fused(Batch B, PostArgs a, int *status) {
+    if (B.halt && *B.halt) return;  // uniform: the whole block leaves
     int lblk;
     const BoxEnt &E = B.tab[batch_box(B, lblk)];
     int i, j, k;

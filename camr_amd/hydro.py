@@ -59,7 +59,8 @@ def hydro_params(prob: Problem) -> L.HydroParams:
 class Level:
     """One AMR level advanced on the GPU by the C-ABI level driver."""
 
-    def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO, batched: bool = True):
+    def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO, batched: bool = True,
+                 graph: bool = False):
         self.prob = prob
         d = L.LevelDesc()
         L.lib().camr_default_level_desc(C.byref(d))
@@ -74,6 +75,7 @@ class Level:
         d.box_lohi = self._lohi.ctypes.data_as(C.POINTER(C.c_int32))
         d.engine = engine
         d.batched = int(bool(batched))
+        d.graph = int(bool(graph))
         self._desc = d
         h = C.c_void_p()
         L.check(L.lib().camr_level_create(C.byref(d), C.byref(h)))

@@ -151,6 +151,10 @@ typedef struct camr_level_desc {
                                 box of the level (default); 0 = one launch per box,
                                 the per-box loop of construct_hydro_source
                                 (CAMR_construct_hydro_source.cpp:59-71) */
+    int32_t graph;           /* 1: camr_level_run replays one captured hipGraph per
+                                step (device-side dt, no host round trip per step);
+                                used with the batched fused engine and zero gravity,
+                                otherwise camr_level_run steps on the host */
 } camr_level_desc;
 
 void camr_default_level_desc(camr_level_desc *d);

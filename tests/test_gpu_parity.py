@@ -43,9 +43,9 @@ def _oracle_params(O, prob):
 
 
 def _run_both(O, prob, init, nsteps, engine=_lib.ENGINE_AUTO, batched=True, ext=None,
-              tuning=None):
+              tuning=None, graph=False):
     ol = O.Level(_oracle_params(O, prob), prob.boxes)
-    gl = Level(prob, engine=engine, batched=batched)
+    gl = Level(prob, engine=engine, batched=batched, graph=graph)
     if tuning is not None:
         gl.tuning = tuning
     if ext is not None:
@@ -178,11 +178,12 @@ def test_smooth_periodic_bitwise(oracle, ppm):
     assert_bitwise(gl.get_box(0), ol.get_box(0), "smooth 32^3 after 5 steps")
 
 
-@pytest.mark.parametrize("batched", [True, False])
-def test_multibox_level_bitwise(oracle, batched):
+@pytest.mark.parametrize("batched,graph", [(True, False), (False, False), (True, True)])
+def test_multibox_level_bitwise(oracle, batched, graph):
     prob = workloads.smooth_periodic(n=32, ppm_type=1, max_grid=16)
     assert len(prob.boxes) == 8
-    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 3, batched=batched)
+    ol, gl, odts, gdts = _run_both(oracle, prob, workloads.smooth_init, 5, batched=batched,
+                                   graph=graph)
     assert np.array_equal(odts, gdts)
     for b in range(8):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"box {b}")
@@ -227,8 +228,9 @@ def test_large_periodic_conservation():
     assert np.all(np.isfinite(S))
 
 
-@pytest.mark.parametrize("engine", [_lib.ENGINE_AUTO, _lib.ENGINE_STAGED])
-def test_density_floor_path_bitwise(oracle, engine):
+@pytest.mark.parametrize("engine,graph", [(_lib.ENGINE_AUTO, False), (_lib.ENGINE_STAGED, False),
+                                          (_lib.ENGINE_AUTO, True)])
+def test_density_floor_path_bitwise(oracle, engine, graph):
     # cells below small_dens (1e-7) trigger enforce_min_density's neighbour
     # search / small-state reset; the fused engine detects them and redoes
     # the step with the exact pass-by-pass sequence
@@ -243,19 +245,20 @@ def test_density_floor_path_bitwise(oracle, engine):
                 U[4, k, j, i] = U[5, k, j, i] + 0.5 * (U[1:4, k, j, i] ** 2).sum() / U[0, k, j, i]
         return U
 
-    ol, gl, odts, gdts = _run_both(oracle, prob, init, 2, engine)
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, engine, graph=graph)
     assert np.array_equal(odts, gdts)
     for b in range(len(prob.boxes)):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"floor box {b}")
 
 
 # ------------------------------------------------- batched refined level
-@pytest.mark.parametrize("batched,tuning", [
-    (True, None),
-    (False, None),
-    (True, ([64, 128, 256, 64, 128, 256, 64, 128, 64, 256, 128, 64, 128], 1)),
+@pytest.mark.parametrize("batched,tuning,graph", [
+    (True, None, False),
+    (False, None, False),
+    (True, ([64, 128, 256, 64, 128, 256, 64, 128, 64, 256, 128, 64, 128], 1), False),
+    (True, None, True),
 ])
-def test_refined_level_with_exterior_bitwise(oracle, batched, tuning):
+def test_refined_level_with_exterior_bitwise(oracle, batched, tuning, graph):
     # uneven boxes (sides 8..32) covering a sphere of a 64^3 periodic index
     # space; ghost cells outside the level come from exterior data
     prob = workloads.refined_level(n=64, radius=0.3, blocking_factor=8, max_size=32,
@@ -268,7 +271,7 @@ def test_refined_level_with_exterior_bitwise(oracle, batched, tuning):
         return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
 
     ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, batched=batched, ext=ext,
-                                   tuning=tuning)
+                                   tuning=tuning, graph=graph)
     assert np.array_equal(odts, gdts)
     for b in range(len(prob.boxes)):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"refined box {b}")
@@ -283,3 +286,32 @@ def test_unowned_ghosts_without_exterior_are_an_error():
     gl.init_post()
     with pytest.raises(_lib.CamrError):
         gl.run(1)
+
+
+def test_graph_replays_continue_after_host_steps(oracle):
+    # host-stepped and graph-replayed runs interleave on one level: the
+    # device dt sequence picks up the host's, and back
+    prob = workloads.smooth_periodic(n=24, ppm_type=1, max_grid=12)
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gh = Level(prob, graph=False)
+    gg = Level(prob, graph=True)
+    for b in range(len(prob.boxes)):
+        U = workloads.smooth_init(prob, b)
+        ol.set_box(b, U)
+        gh.set_box(b, U)
+        gg.set_box(b, U)
+    for L in (ol, gh, gg):
+        L.init_post()
+    err, odts = ol.run(6)
+    assert err == 0
+    hdts = gh.run(6)
+    g1 = gg.run(4)
+    dt = gg.new_dt(g1[-1])
+    gg.advance(dt)
+    g2 = [dt, gg.new_dt(dt)]
+    gg.advance(g2[1])
+    assert np.array_equal(odts, hdts)
+    assert np.array_equal(odts, np.concatenate([g1, g2]))
+    assert gg.time == ol.time
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gg.get_box(b), ol.get_box(b), f"graph box {b}")
