@@ -106,7 +106,7 @@ def bench_amr(args) -> int:
     boxes vs captured step graphs, on the same two-level hierarchy."""
     import numpy as np
     import torch
-    from camr_amd import workloads
+    from camr_amd import autotune, workloads
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     lev0, lev1 = workloads.amr_levels(args.amr_n0)
     ext1 = workloads.exterior(lev1)
@@ -125,6 +125,13 @@ def bench_amr(args) -> int:
         finals = {}
         for mode in ("per_box", "batched", "graph"):
             L = _amr_level(prob, mode, ext, init)
+            if mode != "per_box" and not args.no_tune:
+                # launch shapes from the autotuner (cached per device and box
+                # population); the tuning runs advance a scratch copy
+                T = _amr_level(prob, mode, ext, init)
+                res["tuning"] = autotune.tune(T, steps=2)
+                T.close()
+                L.tuning = res["tuning"]
             L.run(args.warmup)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -169,6 +176,8 @@ def main() -> int:
                     help="c1: 256^3 PPM level step (default); amr: batched "
                          "AMR levels against per-box launches")
     ap.add_argument("--amr-n0", type=int, default=128)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch shapes instead of the autotuner's")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run one elementwise copy "
                          "(torch.neg) of MIB MiB of float64: a kernel of known "
@@ -194,6 +203,12 @@ def main() -> int:
     L = Level(prob, engine=eng)
     L.set_box(0, U0)
     L.init_post()
+    tuning = None
+    if args.engine == "fused" and not args.no_tune:
+        # launch shapes from the autotuner (cached per device and box shape);
+        # its timing steps advance this level before the warm-up
+        from camr_amd import autotune
+        tuning = autotune.tune(L, steps=2)
 
     # warm-up steps follow the reference dt sequence
     dt = L.initial_dt()
@@ -253,7 +268,7 @@ def main() -> int:
         "config": {"workload": f"{n}^3 periodic single-level Godunov PPM advance "
                                f"(CAMR_advance + computeNewDt), CFL {cfl}",
                    "cells": n ** 3, "ppm_type": 1, "cfl": cfl,
-                   "engine": args.engine},
+                   "engine": args.engine, "tuning": tuning},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -756,7 +756,8 @@ int HydroLevel::capture_graphs() {
 
 // camr_level_run on the host: the reference dt sequence, one step at a time.
 int HydroLevel::run(int nsteps, double *dts) {
-    if (use_graph()) return run_graph(nsteps, dts);
+    // profiled runs step on the host so that every launch is timed
+    if (use_graph() && !prof_.enabled()) return run_graph(nsteps, dts);
     double dt = 0.0;
     for (int s = 0; s < nsteps; ++s) {
         int rc = (s == 0) ? initial_dt(&dt) : new_dt(dt, &dt);

@@ -22,6 +22,7 @@ class Prof {
         on_ = on;
         filter_ = filter ? filter : "";
     }
+    bool enabled() const { return on_; }
     bool active(const char *name) const {
         return on_ && (filter_.empty() || filter_ == name);
     }

@@ -315,3 +315,30 @@ def test_graph_replays_continue_after_host_steps(oracle):
     assert gg.time == ol.time
     for b in range(len(prob.boxes)):
         assert_bitwise(gg.get_box(b), ol.get_box(b), f"graph box {b}")
+
+
+def test_autotuned_level_stays_bitwise(oracle, tmp_path, monkeypatch):
+    from camr_amd import autotune
+    monkeypatch.setenv("CAMR_TUNING_CACHE", str(tmp_path / "tuning.json"))
+    prob = workloads.refined_level(n=32, radius=0.3, blocking_factor=8, max_size=16)
+    ext = workloads.exterior(prob)
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    T = Level(prob)
+    T.set_exterior(ext)
+    for b in range(len(prob.boxes)):
+        T.set_box(b, init(prob, b))
+    T.init_post()
+    blocks, order = autotune.tune(T, steps=1)
+    assert len(blocks) == _lib.NSTAGE and all(x in (64, 128, 256) for x in blocks)
+    assert (tmp_path / "tuning.json").exists()
+    # a second level with the same boxes reads the cache
+    assert autotune.tune(Level(prob), steps=1) == (blocks, order)
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, ext=ext, tuning=(blocks, order),
+                                   graph=True)
+    assert np.array_equal(odts, gdts)
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"tuned box {b}")
