@@ -8,9 +8,6 @@
 
 namespace camr {
 
-// compact primitive layout: SR..SY of hydro_math.h, then sound speed and
-// the small-sound-speed floor of QAUX (QC, QCSML)
-enum { SC = NS, SCS = NS + 1, NQC = NS + 2 };
 
 // hydro_ctoprim (Source/Hydro/Hydro_ctoprim.H:11-101) of the conserved
 // state v (NVAR values) into the NQC compact primitives o; e receives the

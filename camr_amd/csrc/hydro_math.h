@@ -27,6 +27,9 @@ enum { BC_INTERIOR = 0, BC_INFLOW = 1, BC_OUTFLOW = 2, BC_SYMMETRY = 3,
 // Compact face-state layout used between kernels: primitive variables that
 // later stages read (QGAME / QTEMP are never read after tracing).
 enum { SR = 0, SU = 1, SV = 2, SW = 3, SP = 4, SE = 5, SY = 6, NS = 9 };
+// Compact primitives of a cell: the state layout, then the sound speed and
+// the small-sound-speed floor of QAUX (QC, QCSML).
+enum { SC = NS, SCS = NS + 1, NQC = NS + 2 };
 // Compact flux layout: the NVAR fluxes without UTEMP (which is always 0).
 enum { FR = 0, FMX = 1, FMY = 2, FMZ = 3, FE = 4, FEI = 5, FY = 6, NF = 9 };
 // Godunov (GD) face quantities kept: normal velocity and pressure.

@@ -52,7 +52,7 @@ __device__ __forceinline__ void block_min_commit(double v, unsigned long long *o
 // CAMR_reset_internal_e (Source/Utils/CAMR_reset_internal_e.H:13-129) then
 // the temperature of computeTemp (CAMR.cpp:1141-1185), on a register copy
 // of one cell.
-__device__ __forceinline__ void reset_e_and_T(double *s, const HydroConst &hc) {
+CAMR_HD void reset_e_and_T(double *s, const HydroConst &hc) {
     const double rho = s[URHO];
     const double rhoInv = 1.0 / rho;
     const double Up = s[UMX] * rhoInv;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void reset_e_and_T(double *s, const HydroConst &hc) {
 }
 
 // normalize_species (CAMR.cpp:866-908) on a register copy of one cell
-__device__ __forceinline__ void normalize_species(double *s) {
+CAMR_HD void normalize_species(double *s) {
     double sum = 0.;
 #pragma unroll
     for (int n = 0; n < NSPEC; ++n) {
