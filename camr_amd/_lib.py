@@ -78,6 +78,8 @@ EXPORTS = [
     "camr_level_advance", "camr_level_run", "camr_level_sync",
     "camr_level_time", "camr_level_profile", "camr_level_profile_report",
     "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
+    "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
+    "camr_level_adjoint_backward",
 ]
 
 _lib = None
@@ -128,6 +130,10 @@ def lib():
         L.camr_level_set_exterior.argtypes = [vp, dp]
         L.camr_level_set_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.camr_level_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.camr_level_adjoint_forward.argtypes = [vp, C.c_int32, dp, C.c_int32]
+        L.camr_level_adj_upload_box.argtypes = [vp, C.c_int32, dp]
+        L.camr_level_adj_download_box.argtypes = [vp, C.c_int32, dp]
+        L.camr_level_adjoint_backward.argtypes = [vp]
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default restype

@@ -14,6 +14,7 @@
 #include <memory>
 #include <vector>
 
+#include "hydro_adjoint.h"
 #include "hydro_fused.h"
 #include "hydro_state.h"
 
@@ -184,7 +185,26 @@ class HydroLevel {
     int capture_graphs();
     void release_graphs();
 
+    // adjoint: state adjoints (two sets per box), device checkpoints
+    std::vector<double *> abar_[2];
+    DevFab *tab_abar_[2] = {nullptr, nullptr};
+    int acur_ = 0;                      // set holding the current adjoint
+    std::vector<double *> ckpt_, blk_;  // whole-level state copies
+    std::vector<double> adj_dts_;
+    int adj_every_ = 0;
+    double adj_time0_ = 0.0;
+    size_t state_doubles_ = 0;
+    int adjoint_scope() const;
+    int adjoint_alloc(int nck, int every);
+    int save_state(double *dst);
+    int load_state(const double *src);
+    int adjoint_step(double dt);
+
   public:
+    int adjoint_forward(int nsteps, const double *dts, int every);
+    int adjoint_backward();
+    int adj_upload(int b, const double *h);
+    int adj_download(int b, double *h);
     int run(int nsteps, double *dts);
     int run_graph(int nsteps, double *dts);
     bool use_graph() const {
@@ -243,6 +263,16 @@ void HydroLevel::release() {
     fr(dt_dev_); fr(ext_); dt_dev_ = nullptr; ext_ = nullptr;
     release_graphs();
     fr(ctl_); fr(ctl_dts_); ctl_ = nullptr; ctl_dts_ = nullptr; ctl_cap_ = 0;
+    for (int q = 0; q < 2; ++q) {
+        for (auto p : abar_[q]) fr(p);
+        abar_[q].clear();
+        fr(tab_abar_[q]);
+        tab_abar_[q] = nullptr;
+    }
+    for (auto p : ckpt_) fr(p);
+    for (auto p : blk_) fr(p);
+    ckpt_.clear();
+    blk_.clear();
     if (gstream_) (void)hipStreamDestroy(gstream_);
     gstream_ = nullptr;
     fr(q_); fr(qaux_); fr(srcq_);
@@ -471,8 +501,10 @@ int HydroLevel::check_status() {
                 static const char *msg[] = {"", "", "",
                     "rho E or rho e not positive (Hydro_ctoprim.H:56,68)",
                     "density exactly zero in enforce_minimum_density (CAMR.cpp:990-992)",
-                    "dt needs to be positive (Timestep.H:59)", "", ""};
-                set_error("%s", (e < 8 && msg[e][0]) ? msg[e] : "device error");
+                    "dt needs to be positive (Timestep.H:59)", "", "",
+                    "adjoint: a state took a branch the adjoint does not differentiate "
+                    "(small-energy reset)"};
+                set_error("%s", (e < 9 && msg[e][0]) ? msg[e] : "device error");
                 return e;
             }
     }
@@ -854,6 +886,204 @@ int HydroLevel::run_graph(int nsteps, double *dts) {
     return sync();
 }
 
+// ------------------------------------------------------------- adjoint
+int HydroLevel::adjoint_scope() const {
+    const char *why = nullptr;
+    if (!fused_) why = "the adjoint runs on the fused engine";
+    else if (desc_.hp.ppm_type != 1) why = "the adjoint differentiates the PPM path";
+    else if (grav_src_values_) why = "the adjoint needs const_grav = 0";
+    else if (desc_.hp.allow_small_energy != 1 || desc_.hp.allow_negative_energy != 0)
+        why = "the adjoint needs allow_small_energy = 1, allow_negative_energy = 0";
+    else if (!(desc_.is_periodic[0] && desc_.is_periodic[1] && desc_.is_periodic[2]))
+        why = "the adjoint needs a periodic domain";
+    else if (ext_ || needs_ext_) why = "the adjoint needs a level covering its domain";
+    if (why) {
+        set_error("%s", why);
+        return CAMR_ERR_ADJOINT;
+    }
+    return CAMR_OK;
+}
+
+int HydroLevel::save_state(double *dst) {
+    size_t o = 0;
+    for (size_t b = 0; b < box_.size(); ++b) {
+        const size_t n = NVAR * box_[b].npts();
+        CAMR_HIP(hipMemcpyAsync(dst + o, snew_[b].p, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        o += n;
+    }
+    return CAMR_OK;
+}
+
+int HydroLevel::load_state(const double *src) {
+    int rc = resolve();
+    if (rc) return rc;
+    size_t o = 0;
+    for (size_t b = 0; b < box_.size(); ++b) {
+        const size_t n = NVAR * box_[b].npts();
+        CAMR_HIP(hipMemcpyAsync(snew_[b].p, src + o, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        o += n;
+    }
+    dt_ready_ = false;
+    return CAMR_OK;
+}
+
+int HydroLevel::adjoint_alloc(int nck, int every) {
+    int rc = engine_.enable_adjoint();
+    if (rc) return rc;
+    state_doubles_ = 0;
+    for (const Box &B : box_) state_doubles_ += NVAR * B.npts();
+    if (abar_[0].empty()) {
+        for (int q = 0; q < 2; ++q) {
+            std::vector<DevFab> t(box_.size());
+            for (size_t b = 0; b < box_.size(); ++b) {
+                double *p = nullptr;
+                CAMR_HIP(hipMalloc(&p, NVAR * box_[b].npts() * sizeof(double)));
+                abar_[q].push_back(p);
+                t[b].a = dense(p, box_[b]);
+                t[b].b = box_[b];
+            }
+            CAMR_HIP(hipMalloc(&tab_abar_[q], box_.size() * sizeof(DevFab)));
+            CAMR_HIP(hipMemcpy(tab_abar_[q], t.data(), box_.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+        }
+    }
+    while ((int)ckpt_.size() < nck) {
+        double *p = nullptr;
+        CAMR_HIP(hipMalloc(&p, state_doubles_ * sizeof(double)));
+        ckpt_.push_back(p);
+    }
+    while ((int)blk_.size() < every) {
+        double *p = nullptr;
+        CAMR_HIP(hipMalloc(&p, state_doubles_ * sizeof(double)));
+        blk_.push_back(p);
+    }
+    return CAMR_OK;
+}
+
+// From the level's state S_0: nsteps steps with frozen dts, checkpoints every
+// `every` steps (AMReX-style restart copies kept on the device).
+int HydroLevel::adjoint_forward(int nsteps, const double *dts, int every) {
+    if (nsteps <= 0 || every <= 0 || !dts) { set_error("bad adjoint run arguments"); return CAMR_ERR_PARAM; }
+    int rc = adjoint_scope();
+    if (rc) return rc;
+    rc = resolve();
+    if (rc) return rc;
+    const int nck = (nsteps + every - 1) / every;
+    rc = adjoint_alloc(nck, every);
+    if (rc) return rc;
+    adj_dts_.assign(dts, dts + nsteps);
+    adj_every_ = every;
+    adj_time0_ = time_;
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % every == 0) {
+            rc = resolve();
+            if (!rc) rc = save_state(ckpt_[s / every]);
+            if (rc) return rc;
+        }
+        rc = advance(dts[s]);
+        if (rc) return rc;
+    }
+    return sync();
+}
+
+// One step's adjoint: recompute the step from the current state S_s
+// (fill + hydro, no post: S_new then holds the updated, uncleaned state),
+// then run the adjoint kernels from abar_[acur_] (dL/dS_{s+1}) into the
+// other set (dL/dS_s).  The level's state is S_s again on return.
+int HydroLevel::adjoint_step(double dt) {
+    int rc = resolve();
+    if (rc) return rc;
+    std::swap(snew_, sold_);
+    a_is_new_ = !a_is_new_;
+    set_dt_dev(dt);
+    FillArgs fa;
+    fa.sold_tab = a_is_new_ ? tab_b_ : tab_a_;
+    fa.owner = owner_;
+    fa.ext = extv_;
+    fa.has_ext = 0;
+    fa.dom = dom_;
+    fa.hc = hc_;
+    fa.dtp = dt_dev_;
+    fa.g = desc_.const_grav;
+    fa.add_grav = desc_.add_grav_src;
+    rc = engine_.fill(-1, parity(), fa, stream_, status_, &prof_);
+    if (!rc) rc = engine_.hydro(-1, parity(), true, nullptr, nullptr, 0, hc_, dt_dev_, stream_,
+                                status_, &prof_);
+    if (!rc) {
+        CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        if (*h_status_ & kFloorFlag) {
+            set_error("adjoint: the step takes a density floor, which is not differentiated");
+            rc = CAMR_ERR_ADJOINT;
+        }
+    }
+    if (!rc) {
+        const int out = 1 - acur_;
+        zero_doubles(engine_.adjoint_base(), engine_.scratch_total(), stream_);
+        for (size_t b = 0; b < box_.size(); ++b)
+            zero_doubles(abar_[out][b], NVAR * box_[b].npts(), stream_);
+        AdjArgs a;
+        a.sbar_in = tab_abar_[acur_];
+        a.sbar_out = tab_abar_[out];
+        a.sold_tab = fa.sold_tab;
+        a.owner = owner_;
+        a.dom = dom_;
+        a.hc = hc_;
+        a.dtp = dt_dev_;
+        a.add_grav = desc_.add_grav_src;
+        a.status = status_;
+        rc = adjoint_step_kernels(engine_, parity(), a, stream_, &prof_);
+        acur_ = out;
+    }
+    std::swap(snew_, sold_);
+    a_is_new_ = !a_is_new_;
+    dt_ready_ = false;
+    if (rc) return rc;
+    return check_status();
+}
+
+int HydroLevel::adjoint_backward() {
+    const int nsteps = (int)adj_dts_.size();
+    if (nsteps == 0 || abar_[0].empty()) { set_error("no adjoint forward run"); return CAMR_ERR_PARAM; }
+    const int every = adj_every_;
+    const int nck = (nsteps + every - 1) / every;
+    int rc = CAMR_OK;
+    for (int c = nck - 1; c >= 0 && !rc; --c) {
+        const int start = c * every;
+        const int len = std::min(every, nsteps - start);
+        rc = load_state(ckpt_[c]);
+        time_ = adj_time0_;
+        // states S_start .. S_start+len-1 of this block
+        for (int k = 1; k < len && !rc; ++k) {
+            rc = advance(adj_dts_[start + k - 1]);
+            if (!rc) rc = resolve();
+            if (!rc) rc = save_state(blk_[k]);
+        }
+        for (int k = len - 1; k >= 0 && !rc; --k) {
+            rc = load_state(k == 0 ? ckpt_[c] : blk_[k]);
+            if (!rc) rc = adjoint_step(adj_dts_[start + k]);
+        }
+    }
+    if (rc) return rc;
+    rc = load_state(ckpt_[0]);
+    time_ = adj_time0_;
+    if (rc) return rc;
+    return sync();
+}
+
+int HydroLevel::adj_upload(int b, const double *h) {
+    if (b < 0 || b >= (int)box_.size() || abar_[0].empty()) { set_error("bad box index or no adjoint run"); return CAMR_ERR_PARAM; }
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    CAMR_HIP(hipMemcpy(abar_[acur_][b], h, NVAR * box_[b].npts() * sizeof(double), hipMemcpyHostToDevice));
+    return CAMR_OK;
+}
+
+int HydroLevel::adj_download(int b, double *h) {
+    if (b < 0 || b >= (int)box_.size() || abar_[0].empty()) { set_error("bad box index or no adjoint run"); return CAMR_ERR_PARAM; }
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    CAMR_HIP(hipMemcpy(h, abar_[acur_][b], NVAR * box_[b].npts() * sizeof(double), hipMemcpyDeviceToHost));
+    return CAMR_OK;
+}
+
 // Settle a fused step: wait for it and, if a density floor was flagged,
 // recompute it with the exact pass-by-pass sequence.
 int HydroLevel::resolve() {
@@ -1147,6 +1377,19 @@ int camr_level_run(camr_level *L, int32_t nsteps, double *dts) {
     return L->impl.run(nsteps, dts);
 }
 int camr_level_sync(camr_level *L) { return L ? L->impl.sync() : CAMR_ERR_PARAM; }
+int camr_level_adjoint_forward(camr_level *L, int32_t nsteps, const double *dts,
+                               int32_t ckpt_every) {
+    return L ? L->impl.adjoint_forward(nsteps, dts, ckpt_every) : CAMR_ERR_PARAM;
+}
+int camr_level_adj_upload_box(camr_level *L, int32_t b, const double *host) {
+    return L ? L->impl.adj_upload(b, host) : CAMR_ERR_PARAM;
+}
+int camr_level_adj_download_box(camr_level *L, int32_t b, double *host) {
+    return L ? L->impl.adj_download(b, host) : CAMR_ERR_PARAM;
+}
+int camr_level_adjoint_backward(camr_level *L) {
+    return L ? L->impl.adjoint_backward() : CAMR_ERR_PARAM;
+}
 int camr_level_set_exterior(camr_level *L, const double *host) {
     return L ? L->impl.set_exterior(host) : CAMR_ERR_PARAM;
 }

@@ -128,6 +128,7 @@ struct BoxEnt {
     Canon L;        // layout of its canonical arrays
     double *base;   // its canonical arrays, in slot order
     Arr snew, sold; // level state (valid cells)
+    double *abase;  // adjoints of the canonical arrays (same layout), or NULL
 };
 
 struct Batch {
@@ -235,6 +236,11 @@ class FusedEngine {
     // clean_state passes after the update + next dt estimate
     int post(int b, int parity, const PostArgs &a, hipStream_t st, int *status, Prof *prof);
     int nbox() const { return (int)boxes_.size(); }
+    // adjoint arrays with the canonical layout of every box (allocated on
+    // first use); total doubles of one scratch set
+    int enable_adjoint();
+    double *adjoint_base() const { return ascratch_.base(); }
+    size_t scratch_total() const { return total_; }
 
     LaunchCfg cfg(int kind, int b, int parity) const;
     // launches made while set read this word and skip their work when it is
@@ -256,7 +262,8 @@ class FusedEngine {
     BoxEnt *tab_[2] = {nullptr, nullptr};
     int *blk_dev_ = nullptr;
     const int *halt_ = nullptr;
-    Scratch scratch_;
+    size_t total_ = 0;
+    Scratch scratch_, ascratch_;
 };
 
 Tuning default_tuning();

@@ -510,6 +510,7 @@ int FusedEngine::setup(const std::vector<Box> &boxes, const std::vector<Arr> &bu
         set_error("fused-engine scratch allocation of %zu bytes failed", total * sizeof(double));
         return CAMR_ERR_ALLOC;
     }
+    total_ = total;
     for (int p = 0; p < 2; ++p) {
         if (tab_[p]) (void)hipFree(tab_[p]);
         tab_[p] = nullptr;
@@ -567,6 +568,7 @@ int FusedEngine::build_tables() {
         e.base = scratch_.base() + off_[b];
         e.snew = bufA_[b];
         e.sold = bufB_[b];
+        e.abase = ascratch_.base() ? ascratch_.base() + off_[b] : nullptr;
         t0[p] = e;
         e.snew = bufB_[b];
         e.sold = bufA_[b];
@@ -595,6 +597,15 @@ int FusedEngine::build_tables() {
         }
     }
     return CAMR_OK;
+}
+
+int FusedEngine::enable_adjoint() {
+    if (ascratch_.base()) return CAMR_OK;
+    if (!ascratch_.reserve(total_ * sizeof(double))) {
+        set_error("adjoint scratch allocation of %zu bytes failed", total_ * sizeof(double));
+        return CAMR_ERR_ALLOC;
+    }
+    return build_tables();
 }
 
 LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {

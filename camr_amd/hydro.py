@@ -145,6 +145,31 @@ class Level:
     def time(self):
         return L.lib().camr_level_time(self._h)
 
+    # ------------------------------------------------------------ adjoint
+    def adjoint_forward(self, dts, every=5):
+        """From the current state S_0, len(dts) steps with these (frozen) dts,
+        keeping device checkpoints every `every` steps; ends at S_T."""
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        L.check(L.lib().camr_level_adjoint_forward(
+            self._h, len(d), d.ctypes.data_as(C.POINTER(C.c_double)), int(every)))
+
+    def set_adjoint(self, b, data):
+        """dL/dS_T of box b, (NVAR, nz, ny, nx)."""
+        a = np.ascontiguousarray(data, dtype=np.float64)
+        assert a.shape == self.shape(b)
+        L.check(L.lib().camr_level_adj_upload_box(
+            self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def get_adjoint(self, b):
+        a = np.empty(self.shape(b))
+        L.check(L.lib().camr_level_adj_download_box(
+            self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
+        return a
+
+    def adjoint_backward(self):
+        """dL/dS_T (set_adjoint) -> dL/dS_0 (get_adjoint); the level ends at S_0."""
+        L.check(L.lib().camr_level_adjoint_backward(self._h))
+
     def set_exterior(self, data):
         """Values for ghost cells no box owns: (NVAR, nz, ny, nx) over the domain."""
         n = [self.prob.domhi[d] - self.prob.domlo[d] + 1 for d in range(3)]

@@ -66,6 +66,30 @@ def smooth_init(prob: Problem, b, seed=problems.SEED_FIELD):
     return smooth_field(seed).state(n, lo, hi)
 
 
+def smooth_init_varp(prob: Problem, b, seed=problems.SEED_FIELD, amp=0.1):
+    """``smooth_init`` with the pressure 1 + amp * (two fixed sine modes of
+    the cell centre) instead of the uniform p = 1.
+
+    Gradient checks need it: with a uniform pressure every PPM pressure
+    stencil is flat to rounding, so the edge limiter of every cell sits on
+    its switching point and the step is non-differentiable at the scale of
+    one ulp there; a finite difference then measures a secant across those
+    kinks, not the derivative."""
+    import numpy as np
+    U = smooth_init(prob, b, seed)
+    n = prob.domhi[0] - prob.domlo[0] + 1
+    lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+    x, y, z = [(np.arange(lo[d], hi[d] + 1) + 0.5) / n for d in range(3)]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    p = (1.0 + amp * np.sin(2 * np.pi * (X + 2 * Y) + 0.3)
+         + 0.5 * amp * np.sin(2 * np.pi * (3 * Z - Y) + 1.1))
+    P = problems
+    ke = 0.5 * (U[P.UMX] ** 2 + U[P.UMY] ** 2 + U[P.UMZ] ** 2) / U[P.URHO]
+    U[P.UEINT] = p / (P.GAMMA - 1.0)
+    U[P.UEDEN] = U[P.UEINT] + ke
+    return U
+
+
 def refined_level(n=256, radius=0.3, blocking_factor=8, max_size=64, efficiency=0.8,
                   ppm_type=1, cfl=0.4):
     """Level covering the sphere of `radius` around the centre of an n^3

@@ -51,7 +51,9 @@ typedef enum camr_status {
     CAMR_ERR_ZERO_DENSITY = 4, /* rho == 0 in enforce_min_density */
     CAMR_ERR_DT = 5,           /* dt <= 0 (Timestep.H:59) */
     CAMR_ERR_HIP = 6,          /* HIP runtime failure */
-    CAMR_ERR_ALLOC = 7         /* device allocation failure */
+    CAMR_ERR_ALLOC = 7,        /* device allocation failure */
+    CAMR_ERR_ADJOINT = 8       /* outside the adjoint's scope (settings, or a state
+                                  taking a branch it does not differentiate) */
 } camr_status;
 
 typedef struct camr_box {
@@ -193,6 +195,25 @@ int camr_level_set_exterior(camr_level *L, const double *host);
 #define CAMR_NSTAGE 13
 int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order);
 int camr_level_get_tuning(camr_level *L, int32_t block[CAMR_NSTAGE], int32_t *order);
+
+/* Adjoint (reverse mode) of level steps, for gradients of a loss on the
+ * final state with respect to the initial state.  No reference counterpart
+ * (CAMR has no adjoint); for inverse and sensitivity problems.  Scope: the
+ * fused engine, periodic (Interior) domain, PPM, const_grav = 0, the
+ * default energy branches; rho, rho u, rho E, rho e carry the adjoint,
+ * species and temperature are passive (zero gradient).  dt is frozen.
+ *
+ * camr_level_adjoint_forward: from the level's state S_0, nsteps steps with
+ * the given dts, keeping device checkpoints every ckpt_every steps; the
+ * level ends at S_T.  Then set dL/dS_T box by box (camr_level_adj_upload_box)
+ * and call camr_level_adjoint_backward, which recomputes the steps from the
+ * checkpoints, leaves dL/dS_0 in the adjoint arrays
+ * (camr_level_adj_download_box) and the level at S_0. */
+int camr_level_adjoint_forward(camr_level *L, int32_t nsteps, const double *dts,
+                               int32_t ckpt_every);
+int camr_level_adj_upload_box(camr_level *L, int32_t b, const double *host);
+int camr_level_adj_download_box(camr_level *L, int32_t b, double *host);
+int camr_level_adjoint_backward(camr_level *L);
 
 /* Named hipEvent timers around kernel launches on the level's stream -- the
  * counterpart of the reference's BL_PROFILE regions (TinyProfiler).  With a
