@@ -164,6 +164,107 @@ def bench_amr(args) -> int:
     return 0
 
 
+def bench_adjoint(args) -> int:
+    """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
+    wall time of the backward pass (recomputation from the checkpoints
+    included) over that of the forward steps, and the gradient checked
+    against central differences along seeded unit directions."""
+    import numpy as np
+    import torch
+    from camr_amd import problems, workloads
+    from camr_amd.hydro import Level
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, nsteps, every = args.adj_n, args.steps, 5
+    prob = workloads.smooth_periodic(n=n, ppm_type=1, cfl=0.4)
+
+    def level_from(seed):
+        L = Level(prob)
+        L.set_box(0, workloads.smooth_init_varp(prob, 0, seed))
+        L.init_post()
+        return L
+
+    L = level_from(problems.SEED_FIELD)
+    S0 = L.get_box(0)
+    dts = L.run(nsteps)  # the primal dt sequence, frozen from here on
+    T = level_from(problems.SEED_TARGET)
+    for dt in dts:
+        T.advance(dt)
+    rho_t = T.get_box(0)[0]
+    T.close()
+
+    def forward(U):
+        L.set_box(0, U)
+        for dt in dts:
+            L.advance(dt)
+        return L.get_box(0)
+
+    def gradient():
+        L.set_box(0, S0)
+        L.adjoint_forward(dts, every)
+        g = np.zeros_like(S0)
+        g[0] = L.get_box(0)[0] - rho_t
+        L.set_adjoint(0, g)
+        L.adjoint_backward()
+        return L.get_adjoint(0)
+
+    gradient()  # warm-up: allocates the checkpoints and adjoint scratch
+    times = {"forward": [], "forward_checkpointed": [], "backward": []}
+    for _ in range(max(1, args.adj_reps)):
+        L.set_box(0, S0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for dt in dts:
+            L.advance(dt)
+        torch.cuda.synchronize()
+        times["forward"].append(time.perf_counter() - t0)
+        L.set_box(0, S0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.adjoint_forward(dts, every)
+        torch.cuda.synchronize()
+        times["forward_checkpointed"].append(time.perf_counter() - t0)
+        g = np.zeros_like(S0)
+        g[0] = L.get_box(0)[0] - rho_t
+        L.set_adjoint(0, g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.adjoint_backward()
+        torch.cuda.synchronize()
+        times["backward"].append(time.perf_counter() - t0)
+    ms = {k: 1e3 * min(v) for k, v in times.items()}
+    grad = L.get_adjoint(0)
+    gnorm = float(np.sqrt((grad * grad).sum()))
+    rng = np.random.default_rng(problems.SEED_DIRECTIONS)
+    eps, fd_rows = 1e-6, []
+    for _ in range(args.adj_dirs):
+        v = rng.standard_normal(S0.shape)
+        v /= np.sqrt((v * v).sum())
+        lp = 0.5 * float(((forward(S0 + eps * v)[0] - rho_t) ** 2).sum())
+        lm = 0.5 * float(((forward(S0 - eps * v)[0] - rho_t) ** 2).sum())
+        fd = (lp - lm) / (2 * eps)
+        ad = float((grad * v).sum())
+        fd_rows.append((fd, ad))
+    err_g = max((abs(f - a) / gnorm for f, a in fd_rows), default=None)
+    err_d = max((abs(f - a) / abs(f) for f, a in fd_rows), default=None)
+    ratio = ms["backward"] / ms["forward"]
+    out = {"metric": "adjoint/forward cost ratio", "value": ratio, "unit": "x",
+           "higher_is_better": False, "n_gpus": 1, "steps": nsteps, "warmup": 1,
+           "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": f"adjoint of {nsteps} periodic PPM level steps at {n}^3, "
+                                  f"checkpoints every {every} steps, frozen dt sequence, "
+                                  "L = 1/2 sum (rho(T) - rho_target)^2 over all initial "
+                                  "conserved components; smooth field with a smooth "
+                                  "pressure variation"},
+           "forward_ms": ms["forward"], "forward_checkpointed_ms": ms["forward_checkpointed"],
+           "backward_ms": ms["backward"],
+           "backward_over_checkpointed_forward": ms["backward"] / ms["forward_checkpointed"],
+           "fd_check": {"directions": args.adj_dirs, "eps": eps, "grad_norm": gnorm,
+                        "max_abs_err_over_grad_norm": err_g,
+                        "max_abs_err_over_directional": err_d}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,10 +273,15 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "amr"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "amr", "adjoint"], default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
-                         "AMR levels against per-box launches")
+                         "AMR levels against per-box launches; adjoint: "
+                         "backward/forward cost of the level-step adjoint")
     ap.add_argument("--amr-n0", type=int, default=128)
+    ap.add_argument("--adj-n", type=int, default=128)
+    ap.add_argument("--adj-dirs", type=int, default=16,
+                    help="finite-difference directions of the adjoint check")
+    ap.add_argument("--adj-reps", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of the autotuner's")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -190,6 +296,8 @@ def main() -> int:
         return 2
     if args.workload == "amr":
         return bench_amr(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     import torch
     from camr_amd import _lib, workloads

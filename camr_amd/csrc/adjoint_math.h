@@ -21,15 +21,19 @@ namespace camr {
 // ---------------------------------------------------------------- helpers
 CAMR_HD double sgn_of(double x) { return (x < 0.0) ? -1.0 : 1.0; }
 
-// ybar of y = vmax(a, b) to abar / bbar
+// ybar of y = vmax(a, b) to abar / bbar.  Both accumulators are updated
+// (one with +0): a branch between the two would become a select between
+// their addresses and keep the callers' locals out of registers.
 CAMR_HD void vmax_adj(double a, double b, double yb, double &ab, double &bb) {
-    if (a < b) bb += yb;
-    else ab += yb;
+    const bool to_b = a < b;
+    ab += to_b ? 0.0 : yb;
+    bb += to_b ? yb : 0.0;
 }
 // ybar of y = vmin(a, b)
 CAMR_HD void vmin_adj(double a, double b, double yb, double &ab, double &bb) {
-    if (b < a) bb += yb;
-    else ab += yb;
+    const bool to_b = b < a;
+    ab += to_b ? 0.0 : yb;
+    bb += to_b ? yb : 0.0;
 }
 
 // ---------------------------------------------------------------- Riemann
@@ -64,15 +68,12 @@ CAMR_HD void riemann_two_shock_adj(double rl_in, double ul, double vl, double v2
     const double ustar0 = ((wl * ul + wr * ur) + (pl - pr)) / wsum;
     const bool uzero = fabs(ustar0) < smallu * 0.5 * (fabs(ul) + fabs(ur));
     const double ustar = uzero ? 0.0 : ustar0;
-    double ro, uo, po, reo, gamco;
-    if (ustar > 0.) {
-        ro = rl; uo = ul; po = pl; reo = rel; gamco = gamcl;
-    } else if (ustar < 0.) {
-        ro = rr; uo = ur; po = pr; reo = rer; gamco = gamcr;
-    } else {
-        ro = 0.5 * (rl + rr); uo = 0.5 * (ul + ur); po = 0.5 * (pl + pr);
-        reo = 0.5 * (rel + rer); gamco = 0.5 * (gamcl + gamcr);
-    }
+    const bool upos = ustar > 0., uneg = ustar < 0.;
+    const double ro = upos ? rl : (uneg ? rr : 0.5 * (rl + rr));
+    const double uo = upos ? ul : (uneg ? ur : 0.5 * (ul + ur));
+    const double po = upos ? pl : (uneg ? pr : 0.5 * (pl + pr));
+    const double reo = upos ? rel : (uneg ? rer : 0.5 * (rel + rer));
+    const double gamco = upos ? gamcl : (uneg ? gamcr : 0.5 * (gamcl + gamcr));
     const double ro2 = vmax(ro, small_dens);
     const double roinv = 1.0 / ro2;
     const double yco = gamco * po * roinv;
@@ -160,15 +161,13 @@ CAMR_HD void riemann_two_shock_adj(double rl_in, double ul, double vl, double v2
         b_pstar += b_pg0 * frac; b_po += b_pg0 * omf; b_frac += b_pg0 * (pstar - po);
         b_estar += b_re_gd * frac; b_reo += b_re_gd * omf; b_frac += b_re_gd * (estar - reo);
     }
+    // upwind weights of the left / right states (1, 0 or 1/2 each: exact
+    // factors); weighted updates instead of branches keep ib in registers
+    const double wgl = upos ? 1.0 : (uneg ? 0.0 : 0.5);
+    const double wgr = uneg ? 1.0 : (upos ? 0.0 : 0.5);
     // transverse velocities
-    if (ustar > 0.0) {
-        ib.vl += b_t1; ib.v2l += b_t2;
-    } else if (ustar < 0.0) {
-        ib.vr += b_t1; ib.v2r += b_t2;
-    } else {
-        ib.vl += 0.5 * b_t1; ib.vr += 0.5 * b_t1;
-        ib.v2l += 0.5 * b_t2; ib.v2r += 0.5 * b_t2;
-    }
+    ib.vl += wgl * b_t1; ib.v2l += wgl * b_t2;
+    ib.vr += wgr * b_t1; ib.v2r += wgr * b_t2;
     // frac = vmax(0, vmin(1, frac0))
     double b_fmin = 0.0, b_dummy = 0.0, b_frac0 = 0.0;
     vmax_adj(0.0, fmin, b_frac, b_dummy, b_fmin);
@@ -240,17 +239,9 @@ CAMR_HD void riemann_two_shock_adj(double rl_in, double ul, double vl, double v2
     double b_ro = 0.0;
     vmax_adj(ro, small_dens, b_ro2, b_ro, b_dummy);
     // outer state choice
-    double b_rl = 0.0, b_pl = 0.0, b_rr = 0.0, b_pr = 0.0;
-    if (ustar > 0.) {
-        b_rl += b_ro; ib.ul += b_uo; b_pl += b_po; ib.rel += b_reo;
-    } else if (ustar < 0.) {
-        b_rr += b_ro; ib.ur += b_uo; b_pr += b_po; ib.rer += b_reo;
-    } else {
-        b_rl += 0.5 * b_ro; b_rr += 0.5 * b_ro;
-        ib.ul += 0.5 * b_uo; ib.ur += 0.5 * b_uo;
-        b_pl += 0.5 * b_po; b_pr += 0.5 * b_po;
-        ib.rel += 0.5 * b_reo; ib.rer += 0.5 * b_reo;
-    }
+    double b_rl = wgl * b_ro, b_pl = wgl * b_po, b_rr = wgr * b_ro, b_pr = wgr * b_po;
+    ib.ul += wgl * b_uo; ib.rel += wgl * b_reo;
+    ib.ur += wgr * b_uo; ib.rer += wgr * b_reo;
     // ustar (zeroed in the dead band)
     const double b_ustar0 = uzero ? 0.0 : b_ustar;
     double b_wl = 0.0, b_wr = 0.0, b_wsum = 0.0;

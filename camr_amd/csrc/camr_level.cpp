@@ -139,6 +139,7 @@ class HydroLevel {
     int expand_state();
     int construct_hydro_source(double dt);
     int check_status();
+    int status_error(int st);
     void release();
     Arr valid_view(double *p, const Box &b) const { return dense(p, b); }
 
@@ -493,7 +494,14 @@ int HydroLevel::state_array(int b, camr_array4 *out) {
 int HydroLevel::check_status() {
     CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
     CAMR_HIP(hipStreamSynchronize(stream_));
-    const int s = *h_status_ & ~kFloorFlag;
+    return status_error(*h_status_);
+}
+
+// The error of the device status word `st` (already read back): the first
+// error bit, which is then cleared on the device; the floor bit is not an
+// error.
+int HydroLevel::status_error(int st) {
+    const int s = st & ~kFloorFlag;
     if (s) {
         CAMR_HIP(hipMemsetAsync(status_, 0, sizeof(int), stream_));
         for (int e = 1; e < 31; ++e)
@@ -1009,18 +1017,9 @@ int HydroLevel::adjoint_step(double dt) {
     if (!rc) rc = engine_.hydro(-1, parity(), true, nullptr, nullptr, 0, hc_, dt_dev_, stream_,
                                 status_, &prof_);
     if (!rc) {
-        CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-        CAMR_HIP(hipStreamSynchronize(stream_));
-        if (*h_status_ & kFloorFlag) {
-            set_error("adjoint: the step takes a density floor, which is not differentiated");
-            rc = CAMR_ERR_ADJOINT;
-        }
-    }
-    if (!rc) {
         const int out = 1 - acur_;
-        zero_doubles(engine_.adjoint_base(), engine_.scratch_total(), stream_);
-        for (size_t b = 0; b < box_.size(); ++b)
-            zero_doubles(abar_[out][b], NVAR * box_[b].npts(), stream_);
+        // the adjoint scratch is all zero here: cleared before the first
+        // step, and each kernel clears the adjoints it consumes
         AdjArgs a;
         a.sbar_in = tab_abar_[acur_];
         a.sbar_out = tab_abar_[out];
@@ -1038,7 +1037,19 @@ int HydroLevel::adjoint_step(double dt) {
     a_is_new_ = !a_is_new_;
     dt_ready_ = false;
     if (rc) return rc;
-    return check_status();
+    // one read of the flags after the whole step: the recompute's floor
+    // flag (the forward step would have been redone with enforced floors,
+    // which the adjoint does not differentiate) and the adjoint's own
+    CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    if (*h_status_ & kFloorFlag) {
+        *h_status_ &= ~kFloorFlag;
+        CAMR_HIP(hipMemcpyAsync(status_, h_status_, sizeof(int), hipMemcpyHostToDevice, stream_));
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        set_error("adjoint: the step takes a density floor, which is not differentiated");
+        return CAMR_ERR_ADJOINT;
+    }
+    return status_error(*h_status_);
 }
 
 int HydroLevel::adjoint_backward() {
@@ -1046,7 +1057,9 @@ int HydroLevel::adjoint_backward() {
     if (nsteps == 0 || abar_[0].empty()) { set_error("no adjoint forward run"); return CAMR_ERR_PARAM; }
     const int every = adj_every_;
     const int nck = (nsteps + every - 1) / every;
-    int rc = CAMR_OK;
+    int rc = resolve();
+    if (rc) return rc;
+    zero_doubles(engine_.adjoint_base(), engine_.scratch_total(), stream_);
     for (int c = nck - 1; c >= 0 && !rc; --c) {
         const int start = c * every;
         const int len = std::min(every, nsteps - start);
