@@ -236,16 +236,21 @@ def bench_adjoint(args) -> int:
     gnorm = float(np.sqrt((grad * grad).sum()))
     rng = np.random.default_rng(problems.SEED_DIRECTIONS)
     eps, fd_rows = 1e-6, []
+    l0 = 0.5 * float(((forward(S0)[0] - rho_t) ** 2).sum()) if args.adj_dirs else 0.0
     for _ in range(args.adj_dirs):
         v = rng.standard_normal(S0.shape)
         v /= np.sqrt((v * v).sum())
         lp = 0.5 * float(((forward(S0 + eps * v)[0] - rho_t) ** 2).sum())
         lm = 0.5 * float(((forward(S0 - eps * v)[0] - rho_t) ** 2).sum())
         fd = (lp - lm) / (2 * eps)
+        # one-sided slopes: they differ where the step crosses a limiter or
+        # upwind switch within +-eps, which the central difference averages
+        fr, fl = (lp - l0) / eps, (l0 - lm) / eps
         ad = float((grad * v).sum())
-        fd_rows.append((fd, ad))
-    err_g = max((abs(f - a) / gnorm for f, a in fd_rows), default=None)
-    err_d = max((abs(f - a) / abs(f) for f, a in fd_rows), default=None)
+        fd_rows.append((fd, ad, abs(fr - fl)))
+    err_g = max((abs(f - a) / gnorm for f, a, _ in fd_rows), default=None)
+    err_d = max((abs(f - a) / abs(f) for f, a, _ in fd_rows), default=None)
+    kink = max((k / abs(f) for f, _, k in fd_rows), default=None)
     ratio = ms["backward"] / ms["forward"]
     out = {"metric": "adjoint/forward cost ratio", "value": ratio, "unit": "x",
            "higher_is_better": False, "n_gpus": 1, "steps": nsteps, "warmup": 1,
@@ -260,7 +265,9 @@ def bench_adjoint(args) -> int:
            "backward_over_checkpointed_forward": ms["backward"] / ms["forward_checkpointed"],
            "fd_check": {"directions": args.adj_dirs, "eps": eps, "grad_norm": gnorm,
                         "max_abs_err_over_grad_norm": err_g,
-                        "max_abs_err_over_directional": err_d}}
+                        "max_abs_err_over_directional": err_d,
+                        "max_one_sided_slope_gap_over_directional": kink,
+                        "directions_fd_adjoint_slope_gap": [[f, a, k] for f, a, k in fd_rows]}}
     print(json.dumps(out), flush=True)
     return 0
 
