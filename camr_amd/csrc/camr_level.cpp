@@ -722,8 +722,7 @@ int HydroLevel::advance_fused(double dt) {
         const int b = batched_ ? -1 : i;
         int rc = engine_.fill(b, parity(), fa, stream_, status_, &prof_);
         if (!rc) rc = engine_.hydro(b, parity(), true, nullptr, nullptr, 0, hc_, dt_dev_, stream_,
-                                    status_, &prof_);
-        if (!rc) rc = engine_.post(b, parity(), pa, stream_, status_, &prof_);
+                                    status_, &prof_, &pa);
         if (rc) return rc;
     }
     dt_ready_ = true;
@@ -778,8 +777,8 @@ int HydroLevel::capture_graphs() {
         }
         hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(64), 0, gstream_, ctl_, dtmin_, status_, P);
         rc = engine_.fill(-1, p, fa, gstream_, status_, nullptr);
-        if (!rc) rc = engine_.hydro(-1, p, true, nullptr, nullptr, 0, hc_, dtp, gstream_, status_, nullptr);
-        if (!rc) rc = engine_.post(-1, p, pa, gstream_, status_, nullptr);
+        if (!rc) rc = engine_.hydro(-1, p, true, nullptr, nullptr, 0, hc_, dtp, gstream_, status_,
+                                    nullptr, &pa);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(gstream_, &g);
         if (!rc && (ec != hipSuccess || !g)) { set_error("stream capture failed"); rc = CAMR_ERR_HIP; }

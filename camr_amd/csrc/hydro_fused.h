@@ -171,6 +171,37 @@ __device__ __forceinline__ double *slot_ptr(const BoxEnt &E) {
     return E.base + off * E.L.cs;
 }
 
+// hydro_pdivu (Hydro_utils_K.H:175-202) and hydro_update (:239-257) of the
+// valid cell (i,j,k) of box E on a zeroed dS/dt
+// (CAMR_construct_hydro_source.cpp:14), then the Saxpy of
+// CAMR_advance.cpp:141: v holds S_new on entry and S_new + dt dS/dt on
+// return.
+__device__ __forceinline__ void update_values(const BoxEnt &E, int i, int j, int k,
+                                              const HydroConst &hc, double dt, double *v) {
+    const Canon L = E.L;
+    const double a0 = hc.dx[1] * hc.dx[2], a1 = hc.dx[0] * hc.dx[2], a2 = hc.dx[0] * hc.dx[1];
+    const double vol = hc.dx[0] * hc.dx[1] * hc.dx[2];
+    const CView f0 = cview(L, slot_ptr<S_FLX + 0>(E), NXF), f1 = cview(L, slot_ptr<S_FLX + 1>(E), NXF),
+                f2 = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
+    const unsigned v0 = L.off(i, j, k), sx = 8u, sy = L.sy(), sz = L.sz();
+    const double pdivu =
+        0.5 *
+        ((f0.ld(v0 + sx, XGP) + f0.ld(v0, XGP)) * (f0.ld(v0 + sx, XGU) * a0 - f0.ld(v0, XGU) * a0) +
+         (f1.ld(v0 + sy, XGP) + f1.ld(v0, XGP)) * (f1.ld(v0 + sy, XGU) * a1 - f1.ld(v0, XGU) * a1) +
+         (f2.ld(v0 + sz, XGP) + f2.ld(v0, XGP)) * (f2.ld(v0 + sz, XGU) * a2 - f2.ld(v0, XGU) * a2)) /
+        vol;
+    const double volinv = 1.0 / vol;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double ds = 0.0;
+        ds += (f0.ld(v0, n) - f0.ld(v0 + sx, n) + f1.ld(v0, n) - f1.ld(v0 + sy, n) +
+               f2.ld(v0, n) - f2.ld(v0 + sz, n)) *
+              volinv;
+        if (n == UEINT) ds -= pdivu;
+        v[n] += dt * ds;
+    }
+}
+
 // Level-step kernels fusing the pointwise bookkeeping around the hydro
 // source (level_fused.hip).
 struct FillArgs {
@@ -200,6 +231,10 @@ struct LaunchCfg {
 };
 void launch_fill_fused(const LaunchCfg &c, const FillArgs &a, hipStream_t st, int *status);
 void launch_post_fused(const LaunchCfg &c, const PostArgs &a, hipStream_t st, int *status);
+// The update of k_update_c and the post pass in one kernel (S_new read and
+// written once).
+void launch_update_post_fused(const LaunchCfg &c, const HydroConst &hc, const double *dtp,
+                              const PostArgs &a, hipStream_t st, int *status);
 
 // Per-stage launch configuration (block size per stage kind).
 struct Tuning {
@@ -228,9 +263,10 @@ class FusedEngine {
     // and energy checks into *status.  have_prim: fill() already wrote the
     // primitives and the Sborder copy; else they come from sb_tab (and the
     // primitive sources from src_tab when has_src).
+    // With `post`, the update and the post pass (post()) run as one kernel.
     int hydro(int b, int parity, bool have_prim, const DevFab *sb_tab, const DevFab *src_tab,
               int has_src, const HydroConst &hc, const double *dtp, hipStream_t st,
-              int *status, Prof *prof);
+              int *status, Prof *prof, const PostArgs *post = nullptr);
     // ghost fill + clean + primitives + S_new = Sborder + dt * old source
     int fill(int b, int parity, const FillArgs &a, hipStream_t st, int *status, Prof *prof);
     // clean_state passes after the update + next dt estimate

@@ -445,33 +445,17 @@ k_divu_c(Batch B, HydroConst hc) {
 __global__ void __launch_bounds__(kBlock)
 k_update_c(Batch B, HydroConst hc, const double *dtp, int *flags) {
     BATCH_POINT(ST_VALID)
-    const double dt = *dtp;
-    const double a0 = hc.dx[1] * hc.dx[2], a1 = hc.dx[0] * hc.dx[2], a2 = hc.dx[0] * hc.dx[1];
-    const double vol = hc.dx[0] * hc.dx[1] * hc.dx[2];
-    const double small_dens = hc.small_dens;
+    (void)L;
     const Arr snew = E.snew;
-    const CView f0 = cview(L, slot_ptr<S_FLX + 0>(E), NXF), f1 = cview(L, slot_ptr<S_FLX + 1>(E), NXF),
-                f2 = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
-    const unsigned v0 = L.off(i, j, k), sx = 8u, sy = L.sy(), sz = L.sz();
-    const double pdivu =
-        0.5 *
-        ((f0.ld(v0 + sx, XGP) + f0.ld(v0, XGP)) * (f0.ld(v0 + sx, XGU) * a0 - f0.ld(v0, XGU) * a0) +
-         (f1.ld(v0 + sy, XGP) + f1.ld(v0, XGP)) * (f1.ld(v0 + sy, XGU) * a1 - f1.ld(v0, XGU) * a1) +
-         (f2.ld(v0 + sz, XGP) + f2.ld(v0, XGP)) * (f2.ld(v0 + sz, XGU) * a2 - f2.ld(v0, XGU) * a2)) /
-        vol;
-    const double volinv = 1.0 / vol;
+    double v[NVAR];
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) {
-        double ds = 0.0;
-        ds += (f0.ld(v0, n) - f0.ld(v0 + sx, n) + f1.ld(v0, n) - f1.ld(v0 + sy, n) +
-               f2.ld(v0, n) - f2.ld(v0 + sz, n)) *
-              volinv;
-        if (n == UEINT) ds -= pdivu;
-        snew(i, j, k, n) += dt * ds;
-    }
-    const double rho = snew(i, j, k, URHO);
+    for (int n = 0; n < NVAR; ++n) v[n] = snew(i, j, k, n);
+    update_values(E, i, j, k, hc, *dtp, v);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) = v[n];
+    const double rho = v[URHO];
     if (rho == 0.) atomicOr(flags, 1 << CAMR_ERR_ZERO_DENSITY);
-    else if (rho < small_dens) atomicOr(flags, kFloorFlag);
+    else if (rho < hc.small_dens) atomicOr(flags, kFloorFlag);
 }
 
 } // namespace
@@ -639,7 +623,8 @@ void FusedEngine::launch(const char *name, Prof *prof, K kern, int kind, int b, 
 
 int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
                        const DevFab *src_tab, int has_src, const HydroConst &hc,
-                       const double *dtp, hipStream_t st, int *status, Prof *prof) {
+                       const double *dtp, hipStream_t st, int *status, Prof *prof,
+                       const PostArgs *post) {
     if (!have_prim || has_src)
         launch("prim", prof, k_prim_c, ST_G4, b, parity, st, sb_tab, src_tab, has_src, hc, status);
     launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening);
@@ -668,7 +653,13 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
         launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
     }
-    launch("consup", prof, k_update_c, ST_VALID, b, parity, st, hc, dtp, status);
+    if (post) {
+        const LaunchCfg c = cfg(ST_VALID, b, parity);
+        Region reg(prof, "consup_clean_estdt", st);
+        launch_update_post_fused(c, hc, dtp, *post, st, status);
+    } else {
+        launch("consup", prof, k_update_c, ST_VALID, b, parity, st, hc, dtp, status);
+    }
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 

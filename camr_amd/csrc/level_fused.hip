@@ -8,7 +8,8 @@
 //                              ghosts after normalize_species
 //   hydro_ctoprim              (CAMR_construct_hydro_source.cpp:108-120)
 //   S_new = Sborder + dt * old source (CAMR_advance.cpp:112-118)
-// k_post_fused, one thread per valid cell, after S_new += dt dS/dt:
+// k_post_fused, one thread per valid cell, after S_new += dt dS/dt
+// (k_update_post: the update itself and then the same):
 //   clean_state(S_new), new-source Saxpy, clean_state, post_timestep's
 //   clean_state (CAMR_advance.cpp:146-154; CAMR.cpp:698-699), then the
 //   CFL estimate of the next computeNewDt (Timestep.H:18-61).
@@ -98,6 +99,37 @@ __device__ __forceinline__ void clean_point(double *v, const HydroConst &hc) {
     reset_e_and_T(v, hc);
 }
 
+// The post pass of one valid cell on v (S_new after the update): the
+// cleaning passes and new-source Saxpy; returns the cell's CFL estimate.
+__device__ __forceinline__ double post_values(const BoxEnt &E, int i, int j, int k,
+                                              const PostArgs &a, double *v) {
+    const Arr sold = E.sold;
+    const double dt = *a.dtp;
+    clean_point(v, a.hc);
+    if (a.add_grav) {
+        const double s_umz = 0.5 * a.g * (v[URHO] - sold(i, j, k, URHO));
+        const double s_ued = 0.5 * a.g * (v[UMZ] - sold(i, j, k, UMZ));
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) {
+            const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
+            v[n] += dt * src;
+        }
+        clean_point(v, a.hc);
+    }
+    clean_point(v, a.hc);
+    // estTimeStep on the new state
+    const double rho = v[URHO];
+    const double rhoInv = 1.0 / rho;
+    const double eint = v[UEINT] / rho;
+    const double pres = (kGamma - 1.0) * rho * eint;
+    const double c = sqrt(kGamma * pres / rho);
+    double dtv = 1.7976931348623157e308;
+    dtv = vmin(dtv, a.hc.dx[0] / (c + fabs(v[UMX] * rhoInv)));
+    dtv = vmin(dtv, a.hc.dx[1] / (c + fabs(v[UMY] * rhoInv)));
+    dtv = vmin(dtv, a.hc.dx[2] / (c + fabs(v[UMZ] * rhoInv)));
+    return dtv;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_post_fused(Batch B, PostArgs a, int *status) {
     if (B.halt && *B.halt) return;  // uniform: the whole block leaves
@@ -107,34 +139,39 @@ k_post_fused(Batch B, PostArgs a, int *status) {
     double dtv = 1.7976931348623157e308;
     // every thread reaches the block reduction below
     if (range_point(range_of(E.bx), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k)) {
-        const Arr snew = E.snew, sold = E.sold;
-        const double dt = *a.dtp;
+        const Arr snew = E.snew;
         double v[NVAR];
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) v[n] = snew(i, j, k, n);
-        clean_point(v, a.hc);
-        if (a.add_grav) {
-            const double s_umz = 0.5 * a.g * (v[URHO] - sold(i, j, k, URHO));
-            const double s_ued = 0.5 * a.g * (v[UMZ] - sold(i, j, k, UMZ));
-#pragma unroll
-            for (int n = 0; n < NVAR; ++n) {
-                const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
-                v[n] += dt * src;
-            }
-            clean_point(v, a.hc);
-        }
-        clean_point(v, a.hc);
+        dtv = post_values(E, i, j, k, a, v);
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) = v[n];
-        // estTimeStep on the new state
+    }
+    block_min_commit(dtv, a.dtmin, status);
+}
+
+// k_update_c followed by k_post_fused, with S_new kept in registers in
+// between (the same arithmetic: S_new is rounded once by the update either
+// way).
+__global__ void __launch_bounds__(kBlock)
+k_update_post(Batch B, HydroConst hc, const double *dtp, PostArgs a, int *status) {
+    if (B.halt && *B.halt) return;  // uniform: the whole block leaves
+    int lblk;
+    const BoxEnt &E = B.tab[batch_box(B, lblk)];
+    int i, j, k;
+    double dtv = 1.7976931348623157e308;
+    if (range_point(range_of(E.bx), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k)) {
+        const Arr snew = E.snew;
+        double v[NVAR];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) v[n] = snew(i, j, k, n);
+        update_values(E, i, j, k, hc, *dtp, v);
         const double rho = v[URHO];
-        const double rhoInv = 1.0 / rho;
-        const double eint = v[UEINT] / rho;
-        const double pres = (kGamma - 1.0) * rho * eint;
-        const double c = sqrt(kGamma * pres / rho);
-        dtv = vmin(dtv, a.hc.dx[0] / (c + fabs(v[UMX] * rhoInv)));
-        dtv = vmin(dtv, a.hc.dx[1] / (c + fabs(v[UMY] * rhoInv)));
-        dtv = vmin(dtv, a.hc.dx[2] / (c + fabs(v[UMZ] * rhoInv)));
+        if (rho == 0.) atomicOr(status, 1 << CAMR_ERR_ZERO_DENSITY);
+        else if (rho < hc.small_dens) atomicOr(status, kFloorFlag);
+        dtv = post_values(E, i, j, k, a, v);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) = v[n];
     }
     block_min_commit(dtv, a.dtmin, status);
 }
@@ -149,6 +186,13 @@ void launch_fill_fused(const LaunchCfg &c, const FillArgs &a, hipStream_t st, in
 void launch_post_fused(const LaunchCfg &c, const PostArgs &a, hipStream_t st, int *status) {
     if (c.grid == 0) return;
     hipLaunchKernelGGL(k_post_fused, dim3(c.grid), dim3(c.block), 0, st, c.B, a, status);
+}
+
+void launch_update_post_fused(const LaunchCfg &c, const HydroConst &hc, const double *dtp,
+                              const PostArgs &a, hipStream_t st, int *status) {
+    if (c.grid == 0) return;
+    hipLaunchKernelGGL(k_update_post, dim3(c.grid), dim3(c.block), 0, st, c.B, hc, dtp, a,
+                       status);
 }
 
 } // namespace camr
