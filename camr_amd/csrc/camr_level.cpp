@@ -193,13 +193,15 @@ class HydroLevel {
     std::vector<double *> ckpt_, blk_;  // whole-level state copies
     std::vector<double> adj_dts_;
     int adj_every_ = 0;
+    bool adj_tail_ready_ = false;  // blk_ holds the last block's states from the forward run
     double adj_time0_ = 0.0;
     size_t state_doubles_ = 0;
     int adjoint_scope() const;
     int adjoint_alloc(int nck, int every);
     int save_state(double *dst);
     int load_state(const double *src);
-    int adjoint_step(double dt);
+    int adjoint_step(double dt, bool check = true);
+    int adjoint_flags();
 
   public:
     int adjoint_forward(int nsteps, const double *dts, int every);
@@ -980,6 +982,9 @@ int HydroLevel::adjoint_forward(int nsteps, const double *dts, int every) {
     adj_dts_.assign(dts, dts + nsteps);
     adj_every_ = every;
     adj_time0_ = time_;
+    // the states of the last block are kept as they are computed, so that
+    // the first block of the backward pass need not recompute them
+    const int tail = (nck - 1) * every;
     for (int s = 0; s < nsteps; ++s) {
         if (s % every == 0) {
             rc = resolve();
@@ -987,16 +992,22 @@ int HydroLevel::adjoint_forward(int nsteps, const double *dts, int every) {
             if (rc) return rc;
         }
         rc = advance(dts[s]);
+        if (!rc && s + 1 > tail && s + 1 < nsteps) {
+            rc = resolve();
+            if (!rc) rc = save_state(blk_[s + 1 - tail]);
+        }
         if (rc) return rc;
     }
+    adj_tail_ready_ = true;
     return sync();
 }
 
 // One step's adjoint: recompute the step from the current state S_s
-// (fill + hydro, no post: S_new then holds the updated, uncleaned state),
+// (fill + hydro up to the final fluxes; the post-pass adjoint applies the
+// update to its cell itself),
 // then run the adjoint kernels from abar_[acur_] (dL/dS_{s+1}) into the
 // other set (dL/dS_s).  The level's state is S_s again on return.
-int HydroLevel::adjoint_step(double dt) {
+int HydroLevel::adjoint_step(double dt, bool check) {
     int rc = resolve();
     if (rc) return rc;
     std::swap(snew_, sold_);
@@ -1013,8 +1024,9 @@ int HydroLevel::adjoint_step(double dt) {
     fa.g = desc_.const_grav;
     fa.add_grav = desc_.add_grav_src;
     rc = engine_.fill(-1, parity(), fa, stream_, status_, &prof_);
+    // the update itself is applied in registers by the post-pass adjoint
     if (!rc) rc = engine_.hydro(-1, parity(), true, nullptr, nullptr, 0, hc_, dt_dev_, stream_,
-                                status_, &prof_);
+                                status_, &prof_, nullptr, true);
     if (!rc) {
         const int out = 1 - acur_;
         // the adjoint scratch is all zero here: cleared before the first
@@ -1036,9 +1048,14 @@ int HydroLevel::adjoint_step(double dt) {
     a_is_new_ = !a_is_new_;
     dt_ready_ = false;
     if (rc) return rc;
-    // one read of the flags after the whole step: the recompute's floor
-    // flag (the forward step would have been redone with enforced floors,
-    // which the adjoint does not differentiate) and the adjoint's own
+    return check ? adjoint_flags() : CAMR_OK;
+}
+
+// The device flags after a stretch of asynchronous adjoint work: a density
+// floor in a recomputed step (the forward step would have been redone with
+// enforced floors, which the adjoint does not differentiate) or a device
+// error of the adjoint kernels.
+int HydroLevel::adjoint_flags() {
     CAMR_HIP(hipMemcpyAsync(h_status_, status_, sizeof(int), hipMemcpyDeviceToHost, stream_));
     CAMR_HIP(hipStreamSynchronize(stream_));
     if (*h_status_ & kFloorFlag) {
@@ -1064,16 +1081,22 @@ int HydroLevel::adjoint_backward() {
         const int len = std::min(every, nsteps - start);
         rc = load_state(ckpt_[c]);
         time_ = adj_time0_;
-        // states S_start .. S_start+len-1 of this block
-        for (int k = 1; k < len && !rc; ++k) {
+        // states S_start .. S_start+len-1 of this block (those of the last
+        // block are still there from the forward run).  No host check per
+        // step: a floor would have refused the run, and the block's flags
+        // are read once at its end (they accumulate on the device).
+        const bool kept = c == nck - 1 && adj_tail_ready_;
+        adj_tail_ready_ = false;
+        for (int k = 1; k < len && !rc && !kept; ++k) {
             rc = advance(adj_dts_[start + k - 1]);
-            if (!rc) rc = resolve();
+            pending_ = false;
             if (!rc) rc = save_state(blk_[k]);
         }
         for (int k = len - 1; k >= 0 && !rc; --k) {
             rc = load_state(k == 0 ? ckpt_[c] : blk_[k]);
-            if (!rc) rc = adjoint_step(adj_dts_[start + k]);
+            if (!rc) rc = adjoint_step(adj_dts_[start + k], false);
         }
+        if (!rc) rc = adjoint_flags();
     }
     if (rc) return rc;
     rc = load_state(ckpt_[0]);

@@ -100,17 +100,22 @@ __device__ __forceinline__ void take_tf_bar(double *Fb, size_t cs, size_t e, dou
 
 // ------------------------------------------------------------ post pass
 // clean_state x2 (x3 with the zero-gravity new-source Saxpy, an identity)
-// after the update: sbar_in <- adjoint of S_new before the cleaning.
+// after the update: sbar_in <- adjoint of S_new before the cleaning.  The
+// update of the cell (and its floor check) is recomputed here.
 __global__ void __launch_bounds__(kBlock)
 k_post_adj(Batch B, AdjArgs a) {
     BATCH_POINT(ST_VALID)
     (void)L;
     const Arr snew = E.snew;
     const Arr bin = a.sbar_in[E.id].a;
-    // inputs of the two (three) energy resets
+    // inputs of the two (three) energy resets: S_new after the update, which
+    // the recompute leaves to this kernel (S_new holds the filled state)
     double x0[NVAR], x1[NVAR], x2[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) x0[n] = snew(i, j, k, n);
+    update_values(E, i, j, k, a.hc, *a.dtp, x0);
+    if (x0[URHO] == 0.) atomicOr(a.status, 1 << CAMR_ERR_ZERO_DENSITY);
+    else if (x0[URHO] < a.hc.small_dens) atomicOr(a.status, kFloorFlag);
     normalize_species(x0);
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) x1[n] = x0[n];

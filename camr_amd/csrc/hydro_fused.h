@@ -263,10 +263,13 @@ class FusedEngine {
     // and energy checks into *status.  have_prim: fill() already wrote the
     // primitives and the Sborder copy; else they come from sb_tab (and the
     // primitive sources from src_tab when has_src).
-    // With `post`, the update and the post pass (post()) run as one kernel.
+    // With `post`, the update and the post pass (post()) run as one kernel;
+    // with `no_update` the final update is left out (the fluxes are ready in
+    // the scratch, S_new is not touched).
     int hydro(int b, int parity, bool have_prim, const DevFab *sb_tab, const DevFab *src_tab,
               int has_src, const HydroConst &hc, const double *dtp, hipStream_t st,
-              int *status, Prof *prof, const PostArgs *post = nullptr);
+              int *status, Prof *prof, const PostArgs *post = nullptr,
+              bool no_update = false);
     // ghost fill + clean + primitives + S_new = Sborder + dt * old source
     int fill(int b, int parity, const FillArgs &a, hipStream_t st, int *status, Prof *prof);
     // clean_state passes after the update + next dt estimate

@@ -624,7 +624,7 @@ void FusedEngine::launch(const char *name, Prof *prof, K kern, int kind, int b, 
 int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
                        const DevFab *src_tab, int has_src, const HydroConst &hc,
                        const double *dtp, hipStream_t st, int *status, Prof *prof,
-                       const PostArgs *post) {
+                       const PostArgs *post, bool no_update) {
     if (!have_prim || has_src)
         launch("prim", prof, k_prim_c, ST_G4, b, parity, st, sb_tab, src_tab, has_src, hc, status);
     launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening);
@@ -653,7 +653,9 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
         launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
     }
-    if (post) {
+    if (no_update) {
+        // the caller applies the update itself
+    } else if (post) {
         const LaunchCfg c = cfg(ST_VALID, b, parity);
         Region reg(prof, "consup_clean_estdt", st);
         launch_update_post_fused(c, hc, dtp, *post, st, status);
