@@ -46,6 +46,12 @@ ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
 # copy of 1 GiB reads back).  Reported only for that engine and size.
 MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15013282837}
 
+# Executed FP64 work of one PPM cell-update (ADD, MUL, TRANS = 1, FMA = 2 per
+# lane; FMAs only from the division / square-root expansions, contraction is
+# off), summed over the step's kernels from rocprofv3 SQ_INSTS_VALU_*_F64
+# counters at 256^3 (profiles/valu_fused_256cube_ppm.md).
+FP64_FLOP_PER_CELL_UPDATE = 9955
+
 
 def unit_count(name: str, n: int) -> int:
     """Faces or cells one launch of kernel `name` covers for an n^3 box."""
@@ -160,6 +166,60 @@ def bench_amr(args) -> int:
     t_box = sum(r["per_box"]["ms_per_step"] for _, r in levels)
     out["value"] = tot_cells / (t_graph * 1e-3)
     out["speedup_vs_per_box"] = t_box / t_graph
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_sod(args) -> int:
+    """The reference's CPU-runnable case: Sod 256x4x4, PLM, 100 steps (dt
+    from the CFL estimate every step), on the GPU as per-step launches and
+    as captured step graphs, beside the oracle on the host cores; the three
+    final states must be bit-identical."""
+    import numpy as np
+    import torch
+    from camr_amd import workloads
+    from camr_amd.hydro import Level
+    from oracle import oracle as O
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    prob = workloads.sod_tube()
+    cells = 256 * 4 * 4
+    nsteps = 100
+    res, finals = {}, {}
+    for mode in ("launches", "graph"):
+        L = Level(prob, graph=(mode == "graph"))
+        L.set_box(0, workloads.sod_init(prob, 0))
+        L.init_post()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.run(nsteps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = {"ms_per_step": el / nsteps * 1e3, "cell_updates_per_s": cells * nsteps / el}
+        finals[mode] = L.get_box(0)
+        L.close()
+    O.build()
+    p = O.params(ppm_type=0, cfl=0.5, bclo=prob.bclo, bchi=prob.bchi, domlo=prob.domlo,
+                 domhi=prob.domhi, is_periodic=prob.is_periodic, dx=prob.dx)
+    OL = O.Level(p, prob.boxes)
+    OL.set_box(0, workloads.sod_init(prob, 0))
+    OL.init_post()
+    t0 = time.perf_counter()
+    err, _ = OL.run(nsteps)
+    el = time.perf_counter() - t0
+    assert err == 0
+    ref = OL.get_box(0)
+    OL.close()
+    out = {"metric": "FP64 cell-updates/s, Sod 256x4x4 PLM, 100 steps", "unit": "cell-updates/s",
+           "value": res["graph"]["cell_updates_per_s"], "higher_is_better": True, "n_gpus": 1,
+           "steps": nsteps, "warmup": 0, "dtype": "f64", "data": "synthetic: Sod states",
+           "vs_baseline": None,
+           "config": {"workload": "Sod shock tube 256x4x4, PLM (plm_iorder 4), CFL 0.5, "
+                                  "outflow in x, periodic in y/z, dt from the CFL estimate"},
+           "modes": res,
+           "identical_to_oracle": all(np.array_equal(f, ref) for f in finals.values()),
+           "cpu_baseline": {"value": cells * nsteps / el, "unit": "cell-updates/s",
+                            "cores": O.lib().ora_get_threads(), "kind": "port",
+                            "sample": f"the same 100 steps, {el * 1e3:.1f} ms wall"}}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -280,10 +340,11 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "amr", "adjoint"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod"], default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
                          "AMR levels against per-box launches; adjoint: "
-                         "backward/forward cost of the level-step adjoint")
+                         "backward/forward cost of the level-step adjoint; sod: "
+                         "the 256x4x4 Sod tube, GPU beside the host oracle")
     ap.add_argument("--amr-n0", type=int, default=128)
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--adj-dirs", type=int, default=16,
@@ -305,6 +366,8 @@ def main() -> int:
         return bench_amr(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "sod":
+        return bench_sod(args)
 
     import torch
     from camr_amd import _lib, workloads
@@ -392,6 +455,10 @@ def main() -> int:
         "step_hbm": {"bytes_per_cell_update": 2 * NVAR * 8,
                      "achieved_GBs": cups * 2 * NVAR * 8 / 1e9,
                      "frac": cups * 2 * NVAR * 8 / 1e9 / HBM_PEAK_GBS},
+        "step_fp64": {"flop_per_cell_update": FP64_FLOP_PER_CELL_UPDATE,
+                      "achieved_TFs": cups * FP64_FLOP_PER_CELL_UPDATE / 1e12,
+                      "peak_TFs": FP64_PEAK_TFS,
+                      "frac": cups * FP64_FLOP_PER_CELL_UPDATE / 1e12 / FP64_PEAK_TFS},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, cfl, U0)
