@@ -44,13 +44,13 @@ ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
 # for the fused engine at 256^3 PPM (profiles/hbm_traffic_fused_256cube_ppm.md,
 # FETCH_SIZE scaled by 1.9692, WRITE_SIZE by 1.0, the factors an 8-byte-lane
 # copy of 1 GiB reads back).  Reported only for that engine and size.
-MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15013282837}
+MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15783877140}
 
 # Executed FP64 work of one PPM cell-update (ADD, MUL, TRANS = 1, FMA = 2 per
 # lane; FMAs only from the division / square-root expansions, contraction is
 # off), summed over the step's kernels from rocprofv3 SQ_INSTS_VALU_*_F64
 # counters at 256^3 (profiles/valu_fused_256cube_ppm.md).
-FP64_FLOP_PER_CELL_UPDATE = 9955
+FP64_FLOP_PER_CELL_UPDATE = 10114
 
 
 def unit_count(name: str, n: int) -> int:
