@@ -113,3 +113,45 @@ def amr_levels(n0=128):
     """(level 0, level 1) of the batched AMR benchmark."""
     return (smooth_periodic(n=n0, ppm_type=1, cfl=0.4, max_grid=32),
             refined_level(n=2 * n0))
+
+
+def sod_sphere(n=128, nx=None, max_grid=32, cfl=0.3, interface=0.3,
+               center=(0.6, 0.5, 0.5), radius=0.15):
+    """Sod shock tube past a sphere: (2n) x n x n cells on [0,2] x [0,1]^2
+    (dx = 1/n), Outflow in x, SlipWall in y and z, PLM, the sphere of
+    `radius` at `center` with the fluid outside it (Exec/SodPlusSphere/
+    inputs:18-19, prob.H:15-49, eb_sphere.cpp:14-44), boxes of `max_grid`."""
+    from ._lib import SLIPWALL
+    nx = nx or 2 * n
+    dx = 1.0 / n
+    boxes = []
+    for k in range(0, n, max_grid):
+        for j in range(0, n, max_grid):
+            for i in range(0, nx, max_grid):
+                boxes.append((i, j, k, min(i + max_grid, nx) - 1, min(j + max_grid, n) - 1,
+                              min(k + max_grid, n) - 1))
+    prob = Problem(domlo=(0, 0, 0), domhi=(nx - 1, n - 1, n - 1), dx=(dx,) * 3,
+                   bclo=(OUTFLOW, SLIPWALL, SLIPWALL), bchi=(OUTFLOW, SLIPWALL, SLIPWALL),
+                   is_periodic=(0, 0, 0), boxes=boxes,
+                   params=dict(ppm_type=0, cfl=cfl))
+    prob.sphere = (tuple(center), float(radius))
+    prob.interface = float(interface)
+    return prob
+
+
+def sod_sphere_geometry(prob):
+    from . import eb
+    c, r = prob.sphere
+    return eb.Geometry(prob, eb.Sphere(c, r))
+
+
+def sod_sphere_init(prob, b, geom=None):
+    """Left state (rho, p) = (1, 1) for x < interface, right (0.125, 0.1),
+    at rest; covered cells zero (EB_set_covered, CAMR.cpp:419-421)."""
+    import numpy as np
+    lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+    U = problems.sod(lo, hi, prob.dx, prob.prob_lo, interface=prob.interface)
+    if geom is not None:
+        cov = geom.covered_mask(lo, hi)
+        U[:, cov] = 0.0
+    return U

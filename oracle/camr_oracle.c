@@ -29,6 +29,7 @@
 #define PC_AIRMW 26.93               /* Source/EOS/PhysicsConstants.H:11   */
 #define NSPEC 3
 #define NGROW 4                      /* CAMR::numGrow, Source/CAMR.H:455-483 */
+#define EB_NGROW 6 /* numGrow for Godunov + FluxRedist, Source/CAMR.H:455-483 */
 
 #define ERR_NEG_ENERGY 1
 #define ERR_ZERO_DENS  2
@@ -47,6 +48,22 @@ static inline double smin(double a, double b) { return (b < a) ? b : a; }
 #define FV(f, iv, n) FA(f, (iv)[0], (iv)[1], (iv)[2], n)
 
 typedef struct { int lo[3], hi[3]; } obox;
+
+/* Embedded-boundary flag word: bits 0..26 connectivity to the neighbour at
+ * offset (ii,jj,kk), bit (ii+1)+3(jj+1)+9(kk+1); bits 27..28 the type (0
+ * regular, 1 single-valued, 2 covered).  AMReX's EBCellFlag meaning in an
+ * encoding of our own. */
+#define FLG(G, i, j, k)                                                      \
+    ((G)->flags[(int64_t)((i) - (G)->flo[0]) +                                \
+                (int64_t)((j) - (G)->flo[1]) * (G)->fjs +                     \
+                (int64_t)((k) - (G)->flo[2]) * (G)->fks])
+static inline int fl_type(int32_t w) { return (w >> 27) & 3; }
+static inline int fl_covered(int32_t w) { return fl_type(w) == 2; }
+static inline int fl_regular(int32_t w) { return fl_type(w) == 0; }
+static inline int fl_single(int32_t w) { return fl_type(w) == 1; }
+static inline int fl_conn(int32_t w, int ii, int jj, int kk) {
+    return (w >> ((ii + 1) + 3 * (jj + 1) + 9 * (kk + 1))) & 1;
+}
 
 static obox mkbox(const int lo[3], const int hi[3]) {
     obox b; for (int d = 0; d < 3; ++d) { b.lo[d] = lo[d]; b.hi[d] = hi[d]; }
@@ -605,13 +622,17 @@ static double plm_pslope(const ora_fab *q, int i, int j, int k, int n,
     return dp;
 }
 
-/* PLM.H:170-323 */
+/* PLM.H:170-323; with `area` (face apertures of direction dir) it is
+ * hydro_plm_d_eb, PLM_eb.H:103-287: the state of a closed face is zero. */
 static void plm_d(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
                   const double slope[O_QVAR], const ora_fab *q, double cc,
-                  double dx, double dt, double small_dens, double small_pres) {
+                  double dx, double dt, double small_dens, double small_pres,
+                  const ora_fab *area) {
     const int iv[3] = {i, j, k};
     int ivp[3] = {i, j, k};
     ivp[dir] += 1;
+    const int open_lo = area ? (FV(area, iv, 0) > 0.0) : 1;
+    const int open_hi = area ? (FV(area, ivp, 0) > 0.0) : 1;
     int l_idx[3];
     l_idx[0] = dir;
     l_idx[1] = (dir == 0) ? 1 : 0;
@@ -636,7 +657,9 @@ static void plm_d(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
     const double alpha0w = dvel[l_idx[2]];
     const double wv[3] = {vel[dir] - cc, vel[dir], vel[dir] + cc};
 
-    { /* right state on the i-1/2 interface */
+    if (!open_lo) {
+        for (int n = 0; n < O_QVAR; n++) FV(qp, iv, n) = 0.0;
+    } else { /* right state on the i-1/2 interface */
         const double rho_ref =
             rho - 0.5 * (1.0 + dtdx * smin(wv[0], 0.0)) * drho;
         double vel_ref[3];
@@ -670,7 +693,9 @@ static void plm_d(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
         FV(qp, iv, O_QREINT) =
             rhoe_ref + (apright + amright) * enth * cs + azeright;
     }
-    { /* left state on the i+1/2 interface */
+    if (!open_hi) {
+        for (int n = 0; n < O_QVAR; n++) FV(qm, ivp, n) = 0.0;
+    } else { /* left state on the i+1/2 interface */
         const double rho_ref =
             rho + 0.5 * (1.0 - dtdx * smax(wv[2], 0.0)) * drho;
         double vel_ref[3];
@@ -708,15 +733,19 @@ static void plm_d(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
     for (int ip = 0; ip < NSPEC; ++ip) {
         const int n = QPASS[ip];
         const double vel_ad = FV(q, iv, O_QU + l_idx[0]);
-        {
+        if (open_lo) {
             const double spzero = (vel_ad > 0) ? -1.0 : vel_ad * dtdx;
             const double acmprght = 0.5 * (-1.0 - spzero) * slope[n];
             FV(qp, iv, n) = FV(q, iv, n) + acmprght;
+        } else {
+            FV(qp, iv, n) = 0.0;
         }
-        {
+        if (open_hi) {
             const double spzero = vel_ad >= 0 ? vel_ad * dtdx : 1.0;
             const double acmpleft = 0.5 * (1.0 - spzero) * slope[n];
             FV(qm, ivp, n) = FV(q, iv, n) + acmpleft;
+        } else {
+            FV(qm, ivp, n) = 0.0;
         }
     }
 }
@@ -954,43 +983,60 @@ static void trace_ppm(obox bx, int idir, const ora_fab *q, const ora_fab *qa,
 
 /* ------------------------------------------------------------------------ */
 /* hydro_transdo / hydro_transdd, Godunov_utils_3D.H:22-239, 245-482        */
-/* (non-EB: every face open)                                                  */
+/* With NULL apertures every face is open (the non-EB build); with
+ * norm_area / trans_area (apertures of the faces normal to dir and to
+ * other_dir) it is the EB form.                                              */
 /* ------------------------------------------------------------------------ */
 static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
                     ora_fab *qp, const ora_fab *qnormm, const ora_fab *qnormp,
                     const ora_fab *flxx, const ora_fab *qa,
                     const ora_fab *qint, double cdtdx, int trd,
-                    double small_pres) {
+                    double small_pres, const ora_fab *norm_area,
+                    const ora_fab *trans_area) {
     const int iv[3] = {i, j, k};
     int ivpn[3] = {i, j, k}, ivpt[3] = {i, j, k};
     ivpn[dir] += 1;
     ivpt[other_dir] += 1;
     const int qvidx = (other_dir == 0) ? O_GDU : ((other_dir == 1) ? O_GDV : O_GDW);
+    int no_cov_face = 1, lo_open = 1, hi_open = 1;
+    if (trans_area)
+        no_cov_face = (FV(trans_area, ivpt, 0) > 0.0) && (FV(trans_area, iv, 0) > 0.0);
+    if (norm_area) {
+        lo_open = FV(norm_area, iv, 0) > 0.0;
+        hi_open = FV(norm_area, ivpn, 0) > 0.0;
+    }
 
-    const double flxrho = cdtdx * (FV(flxx, ivpt, O_URHO) - FV(flxx, iv, O_URHO));
-    const double flxu = cdtdx * (FV(flxx, ivpt, O_UMX) - FV(flxx, iv, O_UMX));
-    const double flxv = cdtdx * (FV(flxx, ivpt, O_UMY) - FV(flxx, iv, O_UMY));
-    const double flxw = cdtdx * (FV(flxx, ivpt, O_UMZ) - FV(flxx, iv, O_UMZ));
-    const double flxe = cdtdx * (FV(flxx, ivpt, O_UEDEN) - FV(flxx, iv, O_UEDEN));
+    const double flxrho = no_cov_face ? cdtdx * (FV(flxx, ivpt, O_URHO) - FV(flxx, iv, O_URHO)) : 0.0;
+    const double flxu = no_cov_face ? cdtdx * (FV(flxx, ivpt, O_UMX) - FV(flxx, iv, O_UMX)) : 0.0;
+    const double flxv = no_cov_face ? cdtdx * (FV(flxx, ivpt, O_UMY) - FV(flxx, iv, O_UMY)) : 0.0;
+    const double flxw = no_cov_face ? cdtdx * (FV(flxx, ivpt, O_UMZ) - FV(flxx, iv, O_UMZ)) : 0.0;
+    const double flxe = no_cov_face ? cdtdx * (FV(flxx, ivpt, O_UEDEN) - FV(flxx, iv, O_UEDEN)) : 0.0;
     const double c = FV(qa, iv, O_QGAMC);
 
     for (int ip = 0; ip < NSPEC; ++ip) {
         const int n = UPASS[ip], nqp = QPASS[ip];
-        const double compn = cdtdx * (FV(flxx, ivpt, n) - FV(flxx, iv, n));
+        const double compn = no_cov_face ? cdtdx * (FV(flxx, ivpt, n) - FV(flxx, iv, n)) : 0.0;
         double rr, rrnew, compo;
-        rr = FV(qnormp, iv, O_QRHO);
-        rrnew = rr - flxrho;
-        compo = rr * FV(qnormp, iv, nqp) - compn;
-        FV(qp, iv, nqp) = compo / rrnew;
-
-        rr = FV(qnormm, ivpn, O_QRHO);
-        rrnew = rr - flxrho;
-        compo = rr * FV(qnormm, ivpn, nqp) - compn;
-        FV(qm, ivpn, nqp) = compo / rrnew;
+        if (lo_open) {
+            rr = FV(qnormp, iv, O_QRHO);
+            rrnew = rr - flxrho;
+            compo = rr * FV(qnormp, iv, nqp) - compn;
+            FV(qp, iv, nqp) = compo / rrnew;
+        } else {
+            FV(qp, iv, nqp) = FV(qnormp, iv, nqp);
+        }
+        if (hi_open) {
+            rr = FV(qnormm, ivpn, O_QRHO);
+            rrnew = rr - flxrho;
+            compo = rr * FV(qnormm, ivpn, nqp) - compn;
+            FV(qm, ivpn, nqp) = compo / rrnew;
+        } else {
+            FV(qm, ivpn, nqp) = FV(qnormm, ivpn, nqp);
+        }
     }
 
-    double dAup, pav, dAu;
-    {
+    double dAup = 0.0, pav = 0.0, dAu = 0.0;
+    if (no_cov_face) {
         const double pggp = FV(qint, ivpt, O_GDPRES);
         const double pggm = FV(qint, iv, O_GDPRES);
         const double ugp = FV(qint, ivpt, qvidx);
@@ -999,7 +1045,16 @@ static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
         pav = 0.5 * (pggp + pggm);
         dAu = ugp - ugm;
     }
-    { /* QP */
+    if (!lo_open) {
+        /* the reference leaves QW unset here; the state of a closed face
+         * is never read */
+        FV(qp, iv, O_QRHO) = FV(qnormp, iv, O_QRHO);
+        FV(qp, iv, O_QU) = FV(qnormp, iv, O_QU);
+        FV(qp, iv, O_QV) = FV(qnormp, iv, O_QV);
+        FV(qp, iv, O_QW) = FV(qnormp, iv, O_QW);
+        FV(qp, iv, O_QREINT) = FV(qnormp, iv, O_QREINT);
+        FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES);
+    } else { /* QP */
         const double rrr = FV(qnormp, iv, O_QRHO);
         double rur = FV(qnormp, iv, O_QU);
         double rvr = FV(qnormp, iv, O_QV);
@@ -1034,7 +1089,14 @@ static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
             FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES);
         }
     }
-    { /* QM */
+    if (!hi_open) {
+        FV(qm, ivpn, O_QRHO) = FV(qnormm, ivpn, O_QRHO);
+        FV(qm, ivpn, O_QU) = FV(qnormm, ivpn, O_QU);
+        FV(qm, ivpn, O_QV) = FV(qnormm, ivpn, O_QV);
+        FV(qm, ivpn, O_QW) = FV(qnormm, ivpn, O_QW);
+        FV(qm, ivpn, O_QREINT) = FV(qnormm, ivpn, O_QREINT);
+        FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES);
+    } else { /* QM */
         const double rrl = FV(qnormm, ivpn, O_QRHO);
         double rul = FV(qnormm, ivpn, O_QU);
         double rvl = FV(qnormm, ivpn, O_QV);
@@ -1076,7 +1138,9 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
                     const ora_fab *flxx, const ora_fab *flxy,
                     const ora_fab *qx, const ora_fab *qy, const ora_fab *qa,
                     const ora_fab *srcq, double hdt, double cdtdx0,
-                    double cdtdx1, int trd, double small_pres) {
+                    double cdtdx1, int trd, double small_pres,
+                    const ora_fab *norm_area, const ora_fab *trans0_area,
+                    const ora_fab *trans1_area) {
     const int iv[3] = {i, j, k};
     const int qvidx0 = (dir == 0) ? O_GDV : O_GDU;
     const int qvidx1 = (dir == 2) ? O_GDV : O_GDW;
@@ -1086,9 +1150,17 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
     ivpt0[l1] += 1;
     ivpt1[l2] += 1;
     ivpn[dir] += 1;
+    int no_cov_face = 1, lo_open = 1, hi_open = 1;
+    if (trans0_area && trans1_area)
+        no_cov_face = (FV(trans0_area, ivpt0, 0) > 0.0) && (FV(trans1_area, ivpt1, 0) > 0.0) &&
+                      (FV(trans0_area, iv, 0) > 0.0) && (FV(trans1_area, iv, 0) > 0.0);
+    if (norm_area) {
+        lo_open = FV(norm_area, iv, 0) > 0.0;
+        hi_open = FV(norm_area, ivpn, 0) > 0.0;
+    }
 
-#define TF(comp) (cdtdx0 * (FV(flxx, ivpt0, comp) - FV(flxx, iv, comp)) + \
-                  cdtdx1 * (FV(flxy, ivpt1, comp) - FV(flxy, iv, comp)))
+#define TF(comp) (no_cov_face ? cdtdx0 * (FV(flxx, ivpt0, comp) - FV(flxx, iv, comp)) + \
+                  cdtdx1 * (FV(flxy, ivpt1, comp) - FV(flxy, iv, comp)) : 0.0)
     const double flxrho = TF(O_URHO);
     const double flxu = TF(O_UMX);
     const double flxv = TF(O_UMY);
@@ -1112,15 +1184,23 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
         const double srcpass = FV(srcq, iv, nqp);
         const double compn = TF(n);
         double compo;
-        compo = rrr * FV(qnormp, iv, nqp) - compn;
-        FV(qp, iv, nqp) = compo / rrnewr + hdt * srcpass;
-        compo = rrl * FV(qnormm, ivpn, nqp) - compn;
-        FV(qm, ivpn, nqp) = compo / rrnewl + hdt * srcpass;
+        if (lo_open) {
+            compo = rrr * FV(qnormp, iv, nqp) - compn;
+            FV(qp, iv, nqp) = compo / rrnewr + hdt * srcpass;
+        } else {
+            FV(qp, iv, nqp) = FV(qnormp, iv, nqp);
+        }
+        if (hi_open) {
+            compo = rrl * FV(qnormm, ivpn, nqp) - compn;
+            FV(qm, ivpn, nqp) = compo / rrnewl + hdt * srcpass;
+        } else {
+            FV(qm, ivpn, nqp) = FV(qnormm, ivpn, nqp);
+        }
     }
 #undef TF
 
-    double dAupx, pavx, dAux, dAupy, pavy, dAuy;
-    {
+    double dAupx = 0.0, pavx = 0.0, dAux = 0.0, dAupy = 0.0, pavy = 0.0, dAuy = 0.0;
+    if (no_cov_face) {
         const double pggpx = FV(qx, ivpt0, O_GDPRES);
         const double pggmx = FV(qx, iv, O_GDPRES);
         const double ugpx = FV(qx, ivpt0, qvidx0);
@@ -1139,7 +1219,14 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
     const double pxnew = cdtdx0 * (dAupx + pavx * dAux * (c - 1.0));
     const double pynew = cdtdx1 * (dAupy + pavy * dAuy * (c - 1.0));
 
-    { /* qp */
+    if (!lo_open) {
+        FV(qp, iv, O_QRHO) = FV(qnormp, iv, O_QRHO);
+        FV(qp, iv, O_QU) = FV(qnormp, iv, O_QU);
+        FV(qp, iv, O_QV) = FV(qnormp, iv, O_QV);
+        FV(qp, iv, O_QW) = FV(qnormp, iv, O_QW);
+        FV(qp, iv, O_QREINT) = FV(qnormp, iv, O_QREINT);
+        FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES);
+    } else { /* qp */
         const double rur = rrr * FV(qnormp, iv, O_QU);
         const double rvr = rrr * FV(qnormp, iv, O_QV);
         const double rwr = rrr * FV(qnormp, iv, O_QW);
@@ -1167,7 +1254,14 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
             FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES) + hdt * srcp;
         FV(qp, iv, O_QPRES) = smax(FV(qp, iv, O_QPRES), small_pres);
     }
-    { /* qm */
+    if (!hi_open) {
+        FV(qm, ivpn, O_QRHO) = FV(qnormm, ivpn, O_QRHO);
+        FV(qm, ivpn, O_QU) = FV(qnormm, ivpn, O_QU);
+        FV(qm, ivpn, O_QV) = FV(qnormm, ivpn, O_QV);
+        FV(qm, ivpn, O_QW) = FV(qnormm, ivpn, O_QW);
+        FV(qm, ivpn, O_QREINT) = FV(qnormm, ivpn, O_QREINT);
+        FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES);
+    } else { /* qm */
         const double rul = rrl * FV(qnormm, ivpn, O_QU);
         const double rvl = rrl * FV(qnormm, ivpn, O_QV);
         const double rwl = rrl * FV(qnormm, ivpn, O_QW);
@@ -1383,7 +1477,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
                                                  P->plm_iorder);
                 }
                 plm_d(i, j, k, idir, &qdm[idir], &qdp[idir], slope, q,
-                      FA(qaux, i, j, k, O_QC), del[idir], dt, sd, sp);
+                      FA(qaux, i, j, k, O_QC), del[idir], dt, sd, sp, NULL);
             }
         }
     } else {
@@ -1423,9 +1517,9 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
         for (int j = tbx.lo[1]; j <= tbx.hi[1]; ++j)
         for (int i = tbx.lo[0]; i <= tbx.hi[0]; ++i) {
             transdo(i, j, k, d, t0, &qmA, &qpA, &qdm[d], &qdp[d], &f[t0],
-                    qaux, &qgd[t0], cdt[t0], trd, sp);
+                    qaux, &qgd[t0], cdt[t0], trd, sp, NULL, NULL);
             transdo(i, j, k, d, t1, &qmB, &qpB, &qdm[d], &qdp[d], &f[t1],
-                    qaux, &qgd[t1], cdt[t1], trd, sp);
+                    qaux, &qgd[t1], cdt[t1], trd, sp, NULL, NULL);
         }
         const obox tfx = bnodes(bxg1, d);
         fab_alloc_box(&fl2[d][t0], tfx, O_NVAR);
@@ -1462,7 +1556,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
         for (int i = tb.lo[0]; i <= tb.hi[0]; ++i)
             transdd(i, j, k, d, &qmf, &qpf, &qdm[d], &qdp[d], &fl2[t0][t1],
                     &fl2[t1][t0], &qg2[t0][t1], &qg2[t1][t0], qaux, srcQ, hdt,
-                    hdtd[t0], hdtd[t1], trd, sp);
+                    hdtd[t0], hdtd[t1], trd, sp, NULL, NULL, NULL);
         const obox fb = bnodes(bx, d);
         for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
         for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
@@ -1561,12 +1655,13 @@ void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
 /* ------------------------------------------------------------------------ */
 /* CAMR_estdt_hydro, Source/Utils/Timestep.H:18-61                           */
 /* ------------------------------------------------------------------------ */
-double ora_estdt_box(const int lo[3], const int hi[3], const ora_fab *u,
-                     const double dx[3]) {
+static double estdt_box_eb(const int lo[3], const int hi[3], const ora_fab *u,
+                           const double dx[3], const ora_ebgeom *G) {
     double dt = DBL_MAX;
     for (int k = lo[2]; k <= hi[2]; ++k)
     for (int j = lo[1]; j <= hi[1]; ++j)
     for (int i = lo[0]; i <= hi[0]; ++i) {
+        if (G && fl_covered(FLG(G, i, j, k))) continue; /* Timestep.H:32-33 */
         const double rho = FA(u, i, j, k, O_URHO);
         const double rhoInv = 1.0 / rho;
         double eint = FA(u, i, j, k, O_UEINT) / rho;
@@ -1586,6 +1681,11 @@ double ora_estdt_box(const int lo[3], const int hi[3], const ora_fab *u,
     return dt;
 }
 
+double ora_estdt_box(const int lo[3], const int hi[3], const ora_fab *u,
+                     const double dx[3]) {
+    return estdt_box_eb(lo, hi, u, dx, NULL);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Level driver: CAMR_advance (Source/CAMR_advance.cpp:38-164),              */
 /* construct_hydro_source (Hydro/CAMR_construct_hydro_source.cpp:9-267),     */
@@ -1602,6 +1702,10 @@ struct ora_level {
     int64_t dn[3];
     double time;
     int err;
+    int ng;              /* ghost width of Sborder: 4, or 6 with EB */
+    int32_t *flags;      /* EB flags over the domain grown by 7 (or NULL) */
+    ora_ebgeom *geom;    /* per box: flags view + cut-cell data on grow(bx,7) */
+    int *kind;           /* per box: 0 regular, 1 EB path, 2 all covered */
 };
 
 static int64_t dom_index(const ora_level *L, const int c[3]) {
@@ -1614,6 +1718,7 @@ ora_level *ora_level_create(const ora_params *p, int nbox, const int *lohi) {
     ora_level *L = (ora_level *)calloc(1, sizeof(ora_level));
     L->P = *p;
     L->nbox = nbox;
+    L->ng = NGROW;
     L->box = (obox *)calloc(nbox, sizeof(obox));
     L->Snew = (ora_fab *)calloc(nbox, sizeof(ora_fab));
     L->Sold = (ora_fab *)calloc(nbox, sizeof(ora_fab));
@@ -1633,11 +1738,11 @@ ora_level *ora_level_create(const ora_params *p, int nbox, const int *lohi) {
         L->box[b] = B;
         fab_alloc_box(&L->Snew[b], B, O_NVAR);
         fab_alloc_box(&L->Sold[b], B, O_NVAR);
-        fab_alloc_box(&L->Sborder[b], bgrow(B, NGROW), O_NVAR);
-        fab_alloc_box(&L->hsrc[b], bgrow(B, NGROW), O_NVAR);
-        fab_alloc_box(&L->oldsrc[b], bgrow(B, NGROW), O_NVAR);
+        fab_alloc_box(&L->Sborder[b], bgrow(B, L->ng), O_NVAR);
+        fab_alloc_box(&L->hsrc[b], bgrow(B, L->ng), O_NVAR);
+        fab_alloc_box(&L->oldsrc[b], bgrow(B, L->ng), O_NVAR);
         fab_alloc_box(&L->newsrc[b], B, O_NVAR);
-        fab_alloc_box(&L->srchyd[b], bgrow(B, NGROW), O_NVAR);
+        fab_alloc_box(&L->srchyd[b], bgrow(B, L->ng), O_NVAR);
         for (int k = B.lo[2]; k <= B.hi[2]; ++k)
         for (int j = B.lo[1]; j <= B.hi[1]; ++j)
         for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
@@ -1656,6 +1761,15 @@ void ora_level_set_exterior(ora_level *L, const double *data) {
 
 void ora_level_destroy(ora_level *L) {
     if (!L) return;
+    if (L->geom) {
+        for (int b = 0; b < L->nbox; ++b) {
+            ora_fab_free(&L->geom[b].vfrac);
+            for (int d = 0; d < 3; ++d) { ora_fab_free(&L->geom[b].ap[d]); ora_fab_free(&L->geom[b].fc[d]); }
+        }
+        free(L->geom);
+    }
+    free(L->kind);
+    free(L->flags);
     for (int b = 0; b < L->nbox; ++b) {
         ora_fab_free(&L->Snew[b]); ora_fab_free(&L->Sold[b]);
         ora_fab_free(&L->Sborder[b]); ora_fab_free(&L->hsrc[b]);
@@ -1681,6 +1795,7 @@ void ora_level_get_box(const ora_level *L, int b, double *data) {
 static void reset_internal_e(ora_fab *S, int i, int j, int k,
                              const ora_params *P) {
     const double rho = FA(S, i, j, k, O_URHO);
+    if (P->eb && !(rho > 0.)) return; /* EB build: CAMR_reset_internal_e.H:36-40 */
     const double rhoInv = 1.0 / rho;
     const double Up = FA(S, i, j, k, O_UMX) * rhoInv;
     const double Vp = FA(S, i, j, k, O_UMY) * rhoInv;
@@ -1747,6 +1862,10 @@ static void compute_temp(ora_fab *S, obox gb, const ora_params *P) {
     for (int k = gb.lo[2]; k <= gb.hi[2]; ++k)
     for (int j = gb.lo[1]; j <= gb.hi[1]; ++j)
     for (int i = gb.lo[0]; i <= gb.hi[0]; ++i) {
+        if (P->eb) { /* EB build, CAMR.cpp:1166-1169 */
+            FA(S, i, j, k, O_UTEMP) = 0.0;
+            if (!(FA(S, i, j, k, O_URHO) > 0.)) continue;
+        }
         double rhoInv = 1.0 / FA(S, i, j, k, O_URHO);
         double e = FA(S, i, j, k, O_UEINT) * rhoInv;
         FA(S, i, j, k, O_UTEMP) = eos_rey2t(e);
@@ -1755,11 +1874,12 @@ static void compute_temp(ora_fab *S, obox gb, const ora_params *P) {
 
 /* enforce_min_density, CAMR.cpp:941-1045 (untiled box, sequential order) */
 static void enforce_min_density(ora_fab *S, obox bx, const ora_params *P,
-                                int *err) {
+                                int *err, const ora_ebgeom *G) {
     const double sd = P->small_dens;
     for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
     for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
     for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        if (G && fl_covered(FLG(G, i, j, k))) continue; /* CAMR.cpp:987-988 */
         if (FA(S, i, j, k, O_URHO) == 0.) {
             *err |= ERR_ZERO_DENS;
         } else if (FA(S, i, j, k, O_URHO) < sd) {
@@ -1797,10 +1917,11 @@ static void enforce_min_density(ora_fab *S, obox bx, const ora_params *P,
 }
 
 /* normalize_species, CAMR.cpp:866-908 (valid cells) */
-static void normalize_species(ora_fab *S, obox bx) {
+static void normalize_species(ora_fab *S, obox bx, const ora_ebgeom *G) {
     for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
     for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
     for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        if (G && fl_covered(FLG(G, i, j, k))) continue; /* CAMR.cpp:886-887 */
         double sum = 0.;
         for (int n = 0; n < NSPEC; n++) {
             FA(S, i, j, k, O_UFS + n) =
@@ -1848,7 +1969,7 @@ static int map_cell(const ora_params *P, const int c[3], int comp, int m[3],
 
 /* FillPatch (single level) of all ghost+valid cells of dst from src. */
 static void fill_patch(ora_level *L, ora_fab *dst, int b, ora_fab *src) {
-    obox g = bgrow(L->box[b], NGROW);
+    obox g = bgrow(L->box[b], L->ng);
     for (int n = 0; n < O_NVAR; ++n)
     for (int k = g.lo[2]; k <= g.hi[2]; ++k)
     for (int j = g.lo[1]; j <= g.hi[1]; ++j)
@@ -1897,8 +2018,10 @@ static void fill_boundary(ora_level *L, ora_fab *S, int ng) {
 
 /* clean_state, CAMR.cpp:1253-1262 */
 static void clean_state(ora_level *L, ora_fab *S, int ng) {
-    for (int b = 0; b < L->nbox; ++b) enforce_min_density(&S[b], L->box[b], &L->P, &L->err);
-    for (int b = 0; b < L->nbox; ++b) normalize_species(&S[b], L->box[b]);
+    for (int b = 0; b < L->nbox; ++b)
+        enforce_min_density(&S[b], L->box[b], &L->P, &L->err, L->geom ? &L->geom[b] : NULL);
+    for (int b = 0; b < L->nbox; ++b)
+        normalize_species(&S[b], L->box[b], L->geom ? &L->geom[b] : NULL);
     if (ng > 0) fill_boundary(L, S, ng);
     for (int b = 0; b < L->nbox; ++b) compute_temp(&S[b], bgrow(L->box[b], ng), &L->P);
 }
@@ -1911,6 +2034,7 @@ void ora_level_init_post(ora_level *L) {
         for (int k = B.lo[2]; k <= B.hi[2]; ++k)
         for (int j = B.lo[1]; j <= B.hi[1]; ++j)
         for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            if (L->P.eb && !(FA(S, i, j, k, O_URHO) > 0.)) continue; /* CAMR.cpp:921-923 */
             const double rhoInv = 1.0 / FA(S, i, j, k, O_URHO);
             const double u = FA(S, i, j, k, O_UMX) * rhoInv;
             const double v = FA(S, i, j, k, O_UMY) * rhoInv;
@@ -1931,7 +2055,8 @@ static double est_time_step(ora_level *L) {
     double estdt_hydro = max_dt_over_cfl;
     double dt = DBL_MAX;
     for (int b = 0; b < L->nbox; ++b)
-        dt = smin(dt, ora_estdt_box(L->box[b].lo, L->box[b].hi, &L->Snew[b], P->dx));
+        dt = smin(dt, estdt_box_eb(L->box[b].lo, L->box[b].hi, &L->Snew[b], P->dx,
+                                   L->geom ? &L->geom[b] : NULL));
     if (!(dt > 0.0)) L->err |= ERR_DT;
     estdt_hydro = smin(estdt_hydro, dt);
     estdt_hydro *= P->cfl;
@@ -1981,16 +2106,18 @@ static void construct_hydro_source(ora_level *L, double dt) {
         for (int i = B.lo[0]; i <= B.hi[0]; ++i)
             FA(&L->srchyd[b], i, j, k, n) += 1.0 * FA(&L->oldsrc[b], i, j, k, n);
     }
-    fill_boundary(L, L->srchyd, NGROW);
+    fill_boundary(L, L->srchyd, L->ng);
 
     /* tiles (hydro_tile_size) */
     int ntile = 0, cap = 64;
     otile *tiles = (otile *)malloc(sizeof(otile) * cap);
     for (int b = 0; b < L->nbox; ++b) {
         obox B = L->box[b];
-        for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += P->tile[2])
-        for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += P->tile[1])
-        for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += P->tile[0]) {
+        if (L->kind && L->kind[b] == 2) continue; /* covered box: dsdt stays 0 */
+        const int untiled = L->kind && L->kind[b] == 1; /* hydro_umdrv_eb runs per box */
+        for (int k0 = B.lo[2]; k0 <= B.hi[2]; k0 += untiled ? (B.hi[2] - B.lo[2] + 1) : P->tile[2])
+        for (int j0 = B.lo[1]; j0 <= B.hi[1]; j0 += untiled ? (B.hi[1] - B.lo[1] + 1) : P->tile[1])
+        for (int i0 = B.lo[0]; i0 <= B.hi[0]; i0 += untiled ? (B.hi[0] - B.lo[0] + 1) : P->tile[0]) {
             otile t; t.b = b;
             t.t.lo[0] = i0; t.t.lo[1] = j0; t.t.lo[2] = k0;
             t.t.hi[0] = (i0 + P->tile[0] - 1 < B.hi[0]) ? i0 + P->tile[0] - 1 : B.hi[0];
@@ -2007,7 +2134,7 @@ static void construct_hydro_source(ora_level *L, double dt) {
     for (int it = 0; it < ntile; ++it) {
         const int b = tiles[it].b;
         const obox bx = tiles[it].t;
-        const obox qbx = bgrow(bx, NGROW);
+        const obox qbx = bgrow(bx, L->ng);
         ora_fab flux[3];
         for (int d = 0; d < 3; ++d) fab_alloc_box(&flux[d], bnodes(bx, d), O_NVAR);
         ora_fab q, qaux, srcq;
@@ -2016,8 +2143,12 @@ static void construct_hydro_source(ora_level *L, double dt) {
         fab_alloc_box(&srcq, qbx, O_QVAR);
         ora_ctoprim_box(qbx.lo, qbx.hi, &L->Sborder[b], &q, &qaux, P, &err);
         ora_srctoprim_box(qbx.lo, qbx.hi, &q, &qaux, &L->srchyd[b], &srcq);
-        ora_hydro_umdrv(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
-                        &srcq, flux, dt, P);
+        if (L->kind && L->kind[b] == 1)
+            ora_hydro_umdrv_eb(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
+                               &srcq, flux, dt, P, &L->geom[b]);
+        else
+            ora_hydro_umdrv(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
+                            &srcq, flux, dt, P);
         ora_fab_free(&q); ora_fab_free(&qaux); ora_fab_free(&srcq);
         for (int d = 0; d < 3; ++d) ora_fab_free(&flux[d]);
     }
@@ -2056,7 +2187,7 @@ int ora_level_advance(ora_level *L, double dt) {
     clean_state(L, L->Sold, 0);
     /* expand_state: FillPatch + clean_state(Sborder) */
     for (int b = 0; b < L->nbox; ++b) fill_patch(L, &L->Sborder[b], b, L->Sold);
-    clean_state(L, L->Sborder, NGROW);
+    clean_state(L, L->Sborder, L->ng);
     const int nsrc = P->add_grav_src ? 1 : 0; /* src_list = {grav_src} */
     if (nsrc) { set_val(L->oldsrc, L->nbox, 0.0); set_val(L->newsrc, L->nbox, 0.0); }
     /* MultiFab::Copy(S_new, Sborder) on valid cells */
@@ -2080,7 +2211,7 @@ int ora_level_advance(ora_level *L, double dt) {
                 FA(&L->oldsrc[b], i, j, k, O_UEDEN) = P->const_grav * FA(&L->Sold[b], i, j, k, O_UMZ);
             }
         }
-        fill_boundary(L, L->oldsrc, NGROW);
+        fill_boundary(L, L->oldsrc, L->ng);
         saxpy_valid(L, L->Snew, dt, L->oldsrc);
     }
     set_val(L->srchyd, L->nbox, 0.0);
@@ -2133,5 +2264,524 @@ void ora_level_sums(const ora_level *L, double sums[O_NVAR]) {
         for (int j = B.lo[1]; j <= B.hi[1]; ++j)
         for (int i = B.lo[0]; i <= B.hi[0]; ++i)
             sums[n] += FA(S, i, j, k, n);
+    }
+}
+
+/* ======================================================================== */
+/* Embedded boundaries: hydro_umdrv_eb (Source/Hydro/Hydro_umdrv_eb.cpp:14-191)*/
+/* ======================================================================== */
+
+/* is_cut_neighborhood, Source/EB/eb_utils.H:7-21, ng = 3 along dir only */
+static int cut_along(const ora_ebgeom *G, int i, int j, int k, int dir, int ng) {
+    int has = fl_single(FLG(G, i, j, k));
+    const int e[3] = {dir == 0, dir == 1, dir == 2};
+    for (int n = 1; n <= ng; n++)
+        has |= fl_single(FLG(G, i - n * e[0], j - n * e[1], k - n * e[2])) ||
+               fl_single(FLG(G, i + n * e[0], j + n * e[1], k + n * e[2]));
+    return has;
+}
+
+/* flatten_eb, Source/Hydro/Godunov/flatten.H:104-134 */
+static double flatten_eb(const ora_fab *q, const ora_ebgeom *G, int i, int j, int k, int dir) {
+    int ivp3[3] = {i, j, k}, ivm3[3] = {i, j, k};
+    ivp3[dir] += 3; ivm3[dir] -= 3;
+    if (!fab_contains(q, ivp3) || !fab_contains(q, ivm3)) return 1.0;
+    if (cut_along(G, i, j, k, dir, 3)) return 1.0;
+    return ora_flatten(q, i, j, k, dir);
+}
+
+/* plm_slope_eb / plm_pslope_eb, Source/Hydro/Godunov/PLM_eb.H:18-100 */
+static double plm_slope_eb(const ora_fab *q, const ora_ebgeom *G, int i, int j, int k,
+                           int n, int dir, double flat, int order, int pslope,
+                           double dx, const ora_fab *srcQ) {
+    const int e[3] = {dir == 0, dir == 1, dir == 2};
+#define CV(o) fl_covered(FLG(G, i + (o) * e[0], j + (o) * e[1], k + (o) * e[2]))
+    int ord = 0;
+    if (order == 4 && !CV(0) && !CV(-1) && !CV(-2) && !CV(1) && !CV(2)) ord = order;
+    else if (order > 1 && !CV(0) && !CV(-1) && !CV(1)) ord = 2;
+#undef CV
+    if (!ord) return 0.0;
+    return pslope ? plm_pslope(q, i, j, k, n, dir, dx, srcQ, flat, ord)
+                  : ora_plm_slope(q, i, j, k, n, dir, flat, ord);
+}
+
+/* Godunov_umeth_eb, Source/Hydro/Godunov/Godunov_3D_eb.cpp:16-567.  `btf` is
+ * bx_to_fill = grow(bx, 2); flx[d] (zeroed by the caller) and qec[d] receive
+ * the final fluxes and Godunov states on faces of btf grown by one
+ * tangentially, where the aperture is > 0. */
+static void godunov_umeth_eb(obox btf, const ora_params *P, const ora_ebgeom *G,
+                             const ora_fab *q, const ora_fab *qaux, const ora_fab *srcQ,
+                             ora_fab *flx, ora_fab *qec, double dt) {
+    const double dx = P->dx[0], dy = P->dx[1], dz = P->dx[2];
+    const double hdt = 0.5 * dt;
+    const double hdtd[3] = {hdt / dx, hdt / dy, hdt / dz};
+    const double cdt[3] = {1.0 / 3.0 * dt / dx, 1.0 / 3.0 * dt / dy, 1.0 / 3.0 * dt / dz};
+    const double del[3] = {dx, dy, dz};
+    const int *bclo = P->bclo, *bchi = P->bchi, *dlo = P->domlo, *dhi = P->domhi;
+    const double small = P->small, sd = P->small_dens, sp = P->small_pres, su = P->smallu;
+    const int trd = P->transverse_reset_density;
+    const ora_fab *ap = G->ap;
+    const obox bxg1 = bgrow(btf, 1), bxg2 = bgrow(btf, 2);
+
+    ora_fab qdm[3], qdp[3];
+    for (int d = 0; d < 3; ++d) {
+        fab_alloc_box(&qdm[d], bgrowhi(bxg2, d, 1), O_QVAR);
+        fab_alloc_box(&qdp[d], bxg2, O_QVAR);
+    }
+    /* PLM slopes and traces on bxg2, covered cells skipped (:94-152) */
+    for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
+    for (int j = bxg2.lo[1]; j <= bxg2.hi[1]; ++j)
+    for (int i = bxg2.lo[0]; i <= bxg2.hi[0]; ++i) {
+        if (fl_covered(FLG(G, i, j, k))) continue;
+        double slope[O_QVAR];
+        double flat = 1.0;
+        if (P->use_flattening == 1)
+            for (int df = 0; df < 3; df++) flat = smin(flat, flatten_eb(q, G, i, j, k, df));
+        for (int idir = 0; idir < 3; ++idir) {
+            for (int n = 0; n < O_QVAR; ++n)
+                slope[n] = plm_slope_eb(q, G, i, j, k, n, idir, flat, P->plm_iorder,
+                                        n == O_QPRES && P->use_pslope, del[idir], srcQ);
+            plm_d(i, j, k, idir, &qdm[idir], &qdp[idir], slope, q,
+                  FA(qaux, i, j, k, O_QC), del[idir], dt, sd, sp, &ap[idir]);
+        }
+    }
+    /* first fluxes where the aperture is open (:158-212) */
+    ora_fab f[3], qgd[3];
+    for (int d = 0; d < 3; ++d) {
+        const obox b = bnodes(bgrowd(bxg2, d, -1), d);
+        fab_alloc_box(&f[d], b, O_NVAR);
+        fab_alloc_box(&qgd[d], b, O_NGDNV);
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+            if (FA(&ap[d], i, j, k, 0) > 0.)
+                cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qdm[d], &qdp[d],
+                       &f[d], &qgd[d], qaux, d, small, sd, sp, su);
+    }
+    /* transverse corrections d|t and their Riemann solves (:214-455) */
+    ora_fab fl2[3][3], qg2[3][3];
+    for (int d = 0; d < 3; ++d) {
+        const int tt[2] = {(d == 0) ? 1 : 0, (d == 2) ? 1 : 2};
+        const obox mbx = bgrowhi(bxg2, d, 1);
+        for (int w = 0; w < 2; ++w) {
+            const int t = tt[w], o = tt[1 - w];
+            ora_fab qmT, qpT;
+            fab_alloc_box(&qmT, mbx, O_QVAR);
+            fab_alloc_box(&qpT, mbx, O_QVAR);
+            const obox tb = bgrowd(bxg2, t, -1);
+            for (int k = tb.lo[2]; k <= tb.hi[2]; ++k)
+            for (int j = tb.lo[1]; j <= tb.hi[1]; ++j)
+            for (int i = tb.lo[0]; i <= tb.hi[0]; ++i)
+                if (!fl_covered(FLG(G, i, j, k)))
+                    transdo(i, j, k, d, t, &qmT, &qpT, &qdm[d], &qdp[d], &f[t], qaux,
+                            &qgd[t], cdt[t], trd, sp, &ap[d], &ap[t]);
+            fab_alloc_box(&fl2[d][t], bnodes(bxg2, d), O_NVAR);
+            fab_alloc_box(&qg2[d][t], bnodes(bxg2, d), O_NGDNV);
+            const obox cb = bgrowd(bnodes(bxg1, d), o, 1);
+            for (int k = cb.lo[2]; k <= cb.hi[2]; ++k)
+            for (int j = cb.lo[1]; j <= cb.hi[1]; ++j)
+            for (int i = cb.lo[0]; i <= cb.hi[0]; ++i)
+                if (FA(&ap[d], i, j, k, 0) > 0.)
+                    cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qmT, &qpT,
+                           &fl2[d][t], &qg2[d][t], qaux, d, small, sd, sp, su);
+            ora_fab_free(&qmT); ora_fab_free(&qpT);
+        }
+    }
+    for (int d = 0; d < 3; ++d) { ora_fab_free(&f[d]); ora_fab_free(&qgd[d]); }
+    /* final states on bxg1 and fluxes on faces of btf grown tangentially
+     * (:457-565) */
+    ora_fab qm, qp;
+    fab_alloc_box(&qm, bxg2, O_QVAR);
+    fab_alloc_box(&qp, bxg1, O_QVAR);
+    for (int d = 0; d < 3; ++d) {
+        const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
+        for (int k = bxg1.lo[2]; k <= bxg1.hi[2]; ++k)
+        for (int j = bxg1.lo[1]; j <= bxg1.hi[1]; ++j)
+        for (int i = bxg1.lo[0]; i <= bxg1.hi[0]; ++i)
+            if (!fl_covered(FLG(G, i, j, k)))
+                transdd(i, j, k, d, &qm, &qp, &qdm[d], &qdp[d], &fl2[t0][t1], &fl2[t1][t0],
+                        &qg2[t0][t1], &qg2[t1][t0], qaux, srcQ, hdt, hdtd[t0], hdtd[t1],
+                        trd, sp, &ap[d], &ap[t0], &ap[t1]);
+        const obox fb = bgrowd(bgrowd(bnodes(btf, d), t0, 1), t1, 1);
+        for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+        for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+        for (int i = fb.lo[0]; i <= fb.hi[0]; ++i)
+            if (FA(&ap[d], i, j, k, 0) > 0.)
+                cmpflx(i, j, k, bclo[d], bchi[d], dlo[d], dhi[d], &qm, &qp, &flx[d],
+                       &qec[d], qaux, d, small, sd, sp, su);
+    }
+    ora_fab_free(&qm); ora_fab_free(&qp);
+    for (int d = 0; d < 3; ++d)
+        for (int t = 0; t < 3; ++t)
+            if (t != d) { ora_fab_free(&fl2[d][t]); ora_fab_free(&qg2[d][t]); }
+    for (int d = 0; d < 3; ++d) { ora_fab_free(&qdm[d]); ora_fab_free(&qdp[d]); }
+}
+
+/* eb_divu, Source/Hydro/Hydro_utils_eb_K.H:31-78 (node i,j,k) */
+static double eb_divu_node(const ora_fab *q, const ora_fab *vf, int i, int j, int k,
+                           const double dxinv[3]) {
+    const double sv = FA(vf, i, j, k, 0) + FA(vf, i - 1, j, k, 0) + FA(vf, i, j - 1, k, 0) +
+                      FA(vf, i - 1, j - 1, k, 0) + FA(vf, i, j, k - 1, 0) +
+                      FA(vf, i - 1, j, k - 1, 0) + FA(vf, i, j - 1, k - 1, 0) +
+                      FA(vf, i - 1, j - 1, k - 1, 0);
+    const double cutoff = 8.0 * (1.0 - 1.e-12);
+    if (!(sv >= cutoff)) return 0.0;
+#define Q(a, b, c, n) FA(q, a, b, c, n)
+    const double ux = 0.25 *
+        (Q(i, j, k, O_QU) - Q(i - 1, j, k, O_QU) + Q(i, j, k - 1, O_QU) - Q(i - 1, j, k - 1, O_QU) +
+         Q(i, j - 1, k, O_QU) - Q(i - 1, j - 1, k, O_QU) + Q(i, j - 1, k - 1, O_QU) -
+         Q(i - 1, j - 1, k - 1, O_QU)) * dxinv[0];
+    const double vy = 0.25 *
+        (Q(i, j, k, O_QV) - Q(i, j - 1, k, O_QV) + Q(i, j, k - 1, O_QV) - Q(i, j - 1, k - 1, O_QV) +
+         Q(i - 1, j, k, O_QV) - Q(i - 1, j - 1, k, O_QV) + Q(i - 1, j, k - 1, O_QV) -
+         Q(i - 1, j - 1, k - 1, O_QV)) * dxinv[1];
+    const double wz = 0.25 *
+        (Q(i, j, k, O_QW) - Q(i, j, k - 1, O_QW) + Q(i - 1, j, k, O_QW) - Q(i - 1, j, k - 1, O_QW) +
+         Q(i, j - 1, k, O_QW) - Q(i, j - 1, k - 1, O_QW) + Q(i - 1, j - 1, k, O_QW) -
+         Q(i - 1, j - 1, k - 1, O_QW)) * dxinv[2];
+#undef Q
+    return ux + vy + wz;
+}
+
+/* compute_hyp_wallflux, Source/Hydro/Hydro_hypflux_eb_K.H:12-68 */
+static void hyp_wallflux(const ora_fab *q, const ora_fab *qa, int i, int j, int k,
+                         double axm, double axp, double aym, double ayp, double azm,
+                         double azp, const ora_params *P, double fluxw[O_NVAR]) {
+    for (int n = 0; n < O_NVAR; ++n) fluxw[n] = 0.;
+    const double u = FA(q, i, j, k, O_QU), v = FA(q, i, j, k, O_QV), w = FA(q, i, j, k, O_QW);
+    const double apnorm = sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) +
+                               (azm - azp) * (azm - azp));
+    const double apnorminv = 1. / apnorm;
+    const double un = u * (axm - axp) * apnorminv + v * (aym - ayp) * apnorminv +
+                      w * (azm - azp) * apnorminv;
+    const double rho = FA(q, i, j, k, O_QRHO), p = FA(q, i, j, k, O_QPRES);
+    const double qrei = FA(q, i, j, k, O_QREINT), gamc = FA(qa, i, j, k, O_QGAMC);
+    const double cav = FA(qa, i, j, k, O_QC), smallc = FA(qa, i, j, k, O_QCSML);
+    double o[12];
+    ora_riemannus(rho, un, 0., 0., p, qrei, rho, -un, 0., 0., p, qrei, cav, gamc, gamc,
+                  P->small, smallc, P->small_dens, P->small_pres, P->smallu, o);
+    fluxw[O_URHO] = o[1];
+    fluxw[O_UEDEN] = o[5];
+    fluxw[O_UEINT] = o[6];
+    fluxw[O_UMX] = (axm - axp) * o[2];
+    fluxw[O_UMY] = (aym - ayp) * o[2];
+    fluxw[O_UMZ] = (azm - azp) * o[2];
+}
+
+/* Face value of direction d at face (i,j,k), moved to the face centroid by
+ * the bilinear interpolation of eb_compute_div / eb_pdivu
+ * (Hydro_utils_eb_K.H:161-179, 437-447): u is the face array, comp its
+ * component, c the cell whose connectivity is consulted. */
+static double face_centroid_value(const ora_fab *u, int comp, const ora_fab *fc,
+                                  const ora_fab *apd, const ora_ebgeom *G, int d,
+                                  int i, int j, int k) {
+    double v = FA(u, i, j, k, comp);
+    const double a = FA(apd, i, j, k, 0);
+    if (a != 0.0 && a != 1.0) {
+        const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
+        int o0[3] = {0, 0, 0}, o1[3] = {0, 0, 0};
+        o0[t0] = (int)copysign(1.0, FA(fc, i, j, k, 0));
+        o1[t1] = (int)copysign(1.0, FA(fc, i, j, k, 1));
+        const int32_t w = FLG(G, i, j, k);
+        const double f0 = fl_conn(w, o0[0], o0[1], o0[2]) ? fabs(FA(fc, i, j, k, 0)) : 0.0;
+        const double f1 = fl_conn(w, o1[0], o1[1], o1[2]) ? fabs(FA(fc, i, j, k, 1)) : 0.0;
+        v = (1.0 - f0) * (1.0 - f1) * v +
+            f0 * (1.0 - f1) * FA(u, i + o0[0], j + o0[1], k + o0[2], comp) +
+            f1 * (1.0 - f0) * FA(u, i + o1[0], j + o1[1], k + o1[2], comp) +
+            f0 * f1 * FA(u, i + o0[0] + o1[0], j + o0[1] + o1[1], k + o0[2] + o1[2], comp);
+    }
+    return v;
+}
+
+/* ApplyMLRedistribution with "FluxRedist" on a single level (AMReX
+ * AMReX_EB_Redistribution.cpp, amrex_flux_redistribute; as_crse = as_fine =
+ * 0, use_wts_in_divnc = false): the published algorithm, restated.  dsdt =
+ * divc + the redistributed mass deficit of the cut cells.  The scatter is
+ * done in the sequential source order of the CPU build. */
+static void flux_redistribute(obox bx, const ora_params *P, const ora_ebgeom *G,
+                              ora_fab *dsdt, const ora_fab *divc, const ora_fab *wt) {
+    const ora_fab *vf = &G->vfrac;
+    const obox g1 = bgrow(bx, 1), g2 = bgrow(bx, 2);
+    obox dbox2;
+    for (int d = 0; d < 3; ++d) {
+        const int gp = P->is_periodic[d] ? 2 : 0;
+        dbox2.lo[d] = P->domlo[d] - gp; dbox2.hi[d] = P->domhi[d] + gp;
+    }
+#define IN2(a, b, c) ((a) >= dbox2.lo[0] && (a) <= dbox2.hi[0] && (b) >= dbox2.lo[1] && \
+                      (b) <= dbox2.hi[1] && (c) >= dbox2.lo[2] && (c) <= dbox2.hi[2])
+    ora_fab optmp, delm;
+    fab_alloc_box(&optmp, g2, O_NVAR);
+    fab_alloc_box(&delm, g1, O_NVAR);
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = g1.lo[2]; k <= g1.hi[2]; ++k)
+    for (int j = g1.lo[1]; j <= g1.hi[1]; ++j)
+    for (int i = g1.lo[0]; i <= g1.hi[0]; ++i) {
+        const int32_t w = FLG(G, i, j, k);
+        if (fl_single(w)) {
+            double vtot = 0.0, divnc = 0.0;
+            for (int kk = -1; kk <= 1; kk++)
+            for (int jj = -1; jj <= 1; jj++)
+            for (int ii = -1; ii <= 1; ii++)
+                if ((ii != 0 || jj != 0 || kk != 0) && fl_conn(w, ii, jj, kk) &&
+                    IN2(i + ii, j + jj, k + kk)) {
+                    const double wf = FA(vf, i + ii, j + jj, k + kk, 0) * 1.0;
+                    vtot += wf;
+                    divnc += wf * FA(divc, i + ii, j + jj, k + kk, n);
+                }
+            divnc /= vtot;
+            const double mask = IN2(i, j, k) ? 1.0 : 0.0;
+            FA(&optmp, i, j, k, n) = (1 - FA(vf, i, j, k, 0)) * (divnc - FA(divc, i, j, k, n)) * mask;
+            FA(&delm, i, j, k, n) = -(FA(vf, i, j, k, 0)) * FA(&optmp, i, j, k, n);
+        } else {
+            FA(&delm, i, j, k, n) = 0;
+        }
+    }
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = g1.lo[2]; k <= g1.hi[2]; ++k)
+    for (int j = g1.lo[1]; j <= g1.hi[1]; ++j)
+    for (int i = g1.lo[0]; i <= g1.hi[0]; ++i) {
+        const int32_t w = FLG(G, i, j, k);
+        if (!fl_single(w)) continue;
+        double wtot = 0.0;
+        for (int kk = -1; kk <= 1; kk++)
+        for (int jj = -1; jj <= 1; jj++)
+        for (int ii = -1; ii <= 1; ii++)
+            if ((ii != 0 || jj != 0 || kk != 0) && fl_conn(w, ii, jj, kk))
+                wtot += FA(wt, i + ii, j + jj, k + kk, 0) * FA(vf, i + ii, j + jj, k + kk, 0) *
+                        (IN2(i + ii, j + jj, k + kk) ? 1.0 : 0.0);
+        wtot = 1.0 / wtot;
+        for (int kk = -1; kk <= 1; kk++)
+        for (int jj = -1; jj <= 1; jj++)
+        for (int ii = -1; ii <= 1; ii++)
+            if ((ii != 0 || jj != 0 || kk != 0) && fl_conn(w, ii, jj, kk)) {
+                const double drho = FA(&delm, i, j, k, n) * wtot * FA(wt, i + ii, j + jj, k + kk, 0) *
+                                    (IN2(i + ii, j + jj, k + kk) ? 1.0 : 0.0);
+                FA(&optmp, i + ii, j + jj, k + kk, n) += drho;
+            }
+    }
+#undef IN2
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i)
+        if (!fl_covered(FLG(G, i, j, k)))
+            FA(dsdt, i, j, k, n) = FA(divc, i, j, k, n) + FA(&optmp, i, j, k, n);
+    ora_fab_free(&optmp);
+    ora_fab_free(&delm);
+}
+
+/* hydro_umdrv_eb, Source/Hydro/Hydro_umdrv_eb.cpp:14-191 (Godunov branch,
+ * FluxRedist): q/qaux/srcq on grow(bx, 6), uin (Sborder) likewise; dsdt on
+ * bx is overwritten on non-covered cells; flx[d] (faces of bx) receive the
+ * face-centroid fluxes (extensive, for flux registers). */
+void ora_hydro_umdrv_eb(const int lo[3], const int hi[3], const ora_fab *uin,
+                        ora_fab *dsdt, const ora_fab *q, const ora_fab *qa,
+                        const ora_fab *srcq, ora_fab flx[3], double dt,
+                        const ora_params *P, const ora_ebgeom *G) {
+    const obox bx = mkbox(lo, hi);
+    const obox bxg_i = bgrow(bx, 2), bxg_ii = bgrow(bxg_i, 1);
+    const double *dx = P->dx;
+    const double dxinv[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
+    ora_fab ftmp[3], qec[3];
+    for (int d = 0; d < 3; ++d) {
+        fab_alloc_box(&ftmp[d], bnodes(bxg_ii, d), O_NVAR);     /* setVal(0) */
+        /* qec is not initialised in the reference; values of closed faces
+         * are never used except through the connectivity-weighted
+         * interpolation of eb_pdivu, which can reach them: zero here */
+        fab_alloc_box(&qec[d], bnodes(bgrow(bxg_ii, 1), d), O_NGDNV);
+    }
+    godunov_umeth_eb(bxg_i, P, G, q, qa, srcq, ftmp, qec, dt);
+
+    /* adjust_fluxes_eb, Hydro_adjust_fluxes.cpp:48-170 */
+    {
+        obox qb = {{q->lo[0], q->lo[1], q->lo[2]}, {q->hi[0], q->hi[1], q->hi[2]}};
+        obox nb = bnodes(bnodes(bnodes(bgrow(qb, -1), 0), 1), 2);
+        for (int d = 0; d < 3; ++d) {
+            const int g = P->is_periodic[d] ? 16 : 0;
+            const int dl = P->domlo[d] - g, dh = P->domhi[d] + 1 + g;
+            if (nb.lo[d] < dl) nb.lo[d] = dl;
+            if (nb.hi[d] > dh) nb.hi[d] = dh;
+        }
+        ora_fab divu;
+        obox db = bnodes(bnodes(bnodes(bgrow(qb, -1), 0), 1), 2);
+        fab_alloc_box(&divu, db, 1);
+        for (int k = nb.lo[2]; k <= nb.hi[2]; ++k)
+        for (int j = nb.lo[1]; j <= nb.hi[1]; ++j)
+        for (int i = nb.lo[0]; i <= nb.hi[0]; ++i)
+            FA(&divu, i, j, k, 0) = eb_divu_node(q, &G->vfrac, i, j, k, dxinv);
+        const double areafac[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[1] * dx[0]};
+        for (int d = 0; d < 3; ++d) {
+            const obox fb = bnodes(bxg_ii, d);
+            for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+            for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+            for (int i = fb.lo[0]; i <= fb.hi[0]; ++i) {
+                artif_visc(i, j, k, &ftmp[d], &divu, uin, dx[d], P->difmag, d,
+                           P->domlo[d], P->domhi[d], P->bclo[d], P->bchi[d]);
+                if (FA(&G->ap[d], i, j, k, 0) > 0.) norm_spec_flx(i, j, k, &ftmp[d]);
+                if (FA(&G->ap[d], i, j, k, 0) > 0.)
+                    for (int n = 0; n < O_NVAR; ++n) FA(&ftmp[d], i, j, k, n) *= areafac[d];
+            }
+        }
+        ora_fab_free(&divu);
+    }
+
+    /* hydro_consup_eb, Hydro_consup.cpp:29-98: eb_compute_div + eb_pdivu */
+    ora_fab divc, wt;
+    fab_alloc_box(&divc, bxg_i, O_NVAR);
+    fab_alloc_box(&wt, bxg_i, 1);
+    const double cell_vol_inv = dxinv[0] * dxinv[1] * dxinv[2];
+    const ora_fab *ap = G->ap, *fc = G->fc;
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = bxg_i.lo[2]; k <= bxg_i.hi[2]; ++k)
+    for (int j = bxg_i.lo[1]; j <= bxg_i.hi[1]; ++j)
+    for (int i = bxg_i.lo[0]; i <= bxg_i.hi[0]; ++i) {
+        const int32_t w = FLG(G, i, j, k);
+        if (fl_covered(w)) continue;
+        const int valid = i >= bx.lo[0] && i <= bx.hi[0] && j >= bx.lo[1] && j <= bx.hi[1] &&
+                          k >= bx.lo[2] && k <= bx.hi[2];
+        const int xh = (i == bx.hi[0]), yh = (j == bx.hi[1]), zh = (k == bx.hi[2]);
+        double dv;
+        if (fl_regular(w)) {
+            dv = cell_vol_inv * (FA(&ftmp[0], i + 1, j, k, n) - FA(&ftmp[0], i, j, k, n)) +
+                 cell_vol_inv * (FA(&ftmp[1], i, j + 1, k, n) - FA(&ftmp[1], i, j, k, n)) +
+                 cell_vol_inv * (FA(&ftmp[2], i, j, k + 1, n) - FA(&ftmp[2], i, j, k, n));
+            if (valid) {
+                FA(&flx[0], i, j, k, n) = FA(&ftmp[0], i, j, k, n);
+                FA(&flx[1], i, j, k, n) = FA(&ftmp[1], i, j, k, n);
+                FA(&flx[2], i, j, k, n) = FA(&ftmp[2], i, j, k, n);
+                if (xh) FA(&flx[0], i + 1, j, k, n) = FA(&ftmp[0], i + 1, j, k, n);
+                if (yh) FA(&flx[1], i, j + 1, k, n) = FA(&ftmp[1], i, j + 1, k, n);
+                if (zh) FA(&flx[2], i, j, k + 1, n) = FA(&ftmp[2], i, j, k + 1, n);
+            }
+        } else {
+            const double fxm = face_centroid_value(&ftmp[0], n, &fc[0], &ap[0], G, 0, i, j, k);
+            const double fxp = face_centroid_value(&ftmp[0], n, &fc[0], &ap[0], G, 0, i + 1, j, k);
+            const double fym = face_centroid_value(&ftmp[1], n, &fc[1], &ap[1], G, 1, i, j, k);
+            const double fyp = face_centroid_value(&ftmp[1], n, &fc[1], &ap[1], G, 1, i, j + 1, k);
+            const double fzm = face_centroid_value(&ftmp[2], n, &fc[2], &ap[2], G, 2, i, j, k);
+            const double fzp = face_centroid_value(&ftmp[2], n, &fc[2], &ap[2], G, 2, i, j, k + 1);
+            if (valid) {
+                FA(&flx[0], i, j, k, n) = fxm;
+                FA(&flx[1], i, j, k, n) = fym;
+                FA(&flx[2], i, j, k, n) = fzm;
+                if (xh) FA(&flx[0], i + 1, j, k, n) = fxp;
+                if (yh) FA(&flx[1], i, j + 1, k, n) = fyp;
+                if (zh) FA(&flx[2], i, j, k + 1, n) = fzp;
+            }
+            const double axm = FA(&ap[0], i, j, k, 0), axp = FA(&ap[0], i + 1, j, k, 0);
+            const double aym = FA(&ap[1], i, j, k, 0), ayp = FA(&ap[1], i, j + 1, k, 0);
+            const double azm = FA(&ap[2], i, j, k, 0), azp = FA(&ap[2], i, j, k + 1, 0);
+            double fw[O_NVAR];
+            hyp_wallflux(q, qa, i, j, k, axm, axp, aym, ayp, azm, azp, P, fw);
+            const double vfr = FA(&G->vfrac, i, j, k, 0);
+            dv = cell_vol_inv / vfr *
+                 (axp * fxp - axm * fxm + ayp * fyp - aym * fym + azp * fzp - azm * fzm);
+            dv += fw[n] * dxinv[0] / vfr;
+        }
+        FA(&divc, i, j, k, n) = dv * -1.0;
+        FA(&wt, i, j, k, 0) = FA(q, i, j, k, O_QRHO); /* eb_weights_type 2 */
+    }
+    for (int k = bxg_i.lo[2]; k <= bxg_i.hi[2]; ++k)
+    for (int j = bxg_i.lo[1]; j <= bxg_i.hi[1]; ++j)
+    for (int i = bxg_i.lo[0]; i <= bxg_i.hi[0]; ++i) {
+        const int32_t w = FLG(G, i, j, k);
+        if (fl_covered(w)) continue;
+        if (fl_regular(w)) {
+            const double pdivu =
+                0.5 * ((FA(&qec[0], i + 1, j, k, O_GDPRES) + FA(&qec[0], i, j, k, O_GDPRES)) *
+                           (FA(&qec[0], i + 1, j, k, O_GDU) - FA(&qec[0], i, j, k, O_GDU)) * dxinv[0] +
+                       (FA(&qec[1], i, j + 1, k, O_GDPRES) + FA(&qec[1], i, j, k, O_GDPRES)) *
+                           (FA(&qec[1], i, j + 1, k, O_GDV) - FA(&qec[1], i, j, k, O_GDV)) * dxinv[1] +
+                       (FA(&qec[2], i, j, k + 1, O_GDPRES) + FA(&qec[2], i, j, k, O_GDPRES)) *
+                           (FA(&qec[2], i, j, k + 1, O_GDW) - FA(&qec[2], i, j, k, O_GDW)) * dxinv[2]);
+            FA(&divc, i, j, k, O_UEINT) -= pdivu;
+        } else {
+            double fm[3], fp[3];
+            const int gdn[3] = {O_GDU, O_GDV, O_GDW};
+            for (int d = 0; d < 3; ++d) {
+                int ip[3] = {i, j, k};
+                ip[d] += 1;
+                fm[d] = (FA(&ap[d], i, j, k, 0) > 0.0)
+                            ? face_centroid_value(&qec[d], gdn[d], &fc[d], &ap[d], G, d, i, j, k)
+                            : 0.0;
+                fp[d] = (FA(&ap[d], ip[0], ip[1], ip[2], 0) > 0.0)
+                            ? face_centroid_value(&qec[d], gdn[d], &fc[d], &ap[d], G, d, ip[0], ip[1], ip[2])
+                            : 0.0;
+            }
+            FA(&divc, i, j, k, O_UEINT) -=
+                (1.0 / FA(&G->vfrac, i, j, k, 0)) * FA(q, i, j, k, O_QPRES) *
+                (dxinv[0] * (FA(&ap[0], i + 1, j, k, 0) * fp[0] - FA(&ap[0], i, j, k, 0) * fm[0]) +
+                 dxinv[1] * (FA(&ap[1], i, j + 1, k, 0) * fp[1] - FA(&ap[1], i, j, k, 0) * fm[1]) +
+                 dxinv[2] * (FA(&ap[2], i, j, k + 1, 0) * fp[2] - FA(&ap[2], i, j, k, 0) * fm[2]));
+        }
+    }
+    flux_redistribute(bx, P, G, dsdt, &divc, &wt);
+    ora_fab_free(&divc);
+    ora_fab_free(&wt);
+    for (int d = 0; d < 3; ++d) { ora_fab_free(&ftmp[d]); ora_fab_free(&qec[d]); }
+}
+
+static void geom_strides(ora_ebgeom *G) {
+    G->fjs = (int64_t)(G->fhi[0] - G->flo[0] + 1);
+    G->fks = G->fjs * (int64_t)(G->fhi[1] - G->flo[1] + 1);
+}
+
+/* Switch a freshly created level to the EB build: ghost width 6
+ * (CAMR::numGrow, CAMR.H:455-483), the flags over flo..fhi (which must hold
+ * the domain grown by 7), and the per-box path of
+ * CAMR_construct_hydro_source.cpp:80-83,150-151. */
+int ora_level_set_eb(ora_level *L, const int32_t *flags, const int flo[3], const int fhi[3]) {
+    int64_t n = 1;
+    for (int d = 0; d < 3; ++d) n *= (int64_t)(fhi[d] - flo[d] + 1);
+    L->flags = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    memcpy(L->flags, flags, sizeof(int32_t) * (size_t)n);
+    L->P.eb = 1;
+    L->ng = EB_NGROW;
+    L->geom = (ora_ebgeom *)calloc(L->nbox, sizeof(ora_ebgeom));
+    L->kind = (int *)calloc(L->nbox, sizeof(int));
+    for (int b = 0; b < L->nbox; ++b) {
+        const obox B = L->box[b];
+        ora_ebgeom *G = &L->geom[b];
+        G->flags = L->flags;
+        for (int d = 0; d < 3; ++d) { G->flo[d] = flo[d]; G->fhi[d] = fhi[d]; }
+        geom_strides(G);
+        ora_fab_free(&L->Sborder[b]); fab_alloc_box(&L->Sborder[b], bgrow(B, L->ng), O_NVAR);
+        ora_fab_free(&L->hsrc[b]); fab_alloc_box(&L->hsrc[b], bgrow(B, L->ng), O_NVAR);
+        ora_fab_free(&L->oldsrc[b]); fab_alloc_box(&L->oldsrc[b], bgrow(B, L->ng), O_NVAR);
+        ora_fab_free(&L->srchyd[b]); fab_alloc_box(&L->srchyd[b], bgrow(B, L->ng), O_NVAR);
+        int all_cov = 1, any_nonreg = 0;
+        for (int k = B.lo[2] - 2; k <= B.hi[2] + 2; ++k)
+        for (int j = B.lo[1] - 2; j <= B.hi[1] + 2; ++j)
+        for (int i = B.lo[0] - 2; i <= B.hi[0] + 2; ++i) {
+            const int32_t w = FLG(G, i, j, k);
+            const int valid = i >= B.lo[0] && i <= B.hi[0] && j >= B.lo[1] && j <= B.hi[1] &&
+                              k >= B.lo[2] && k <= B.hi[2];
+            if (valid && !fl_covered(w)) all_cov = 0;
+            if (!fl_regular(w)) any_nonreg = 1;
+        }
+        L->kind[b] = all_cov ? 2 : (any_nonreg ? 1 : 0);
+    }
+    return 0;
+}
+
+int ora_level_box_kind(const ora_level *L, int b) { return L->kind ? L->kind[b] : 0; }
+
+/* Cut-cell data of box b on grow(bx, 7): vfrac (cells), apertures ap[d]
+ * (faces of direction d), face centroids fc[d] (2 components). */
+void ora_level_set_box_eb(ora_level *L, int b, const double *vfrac, const double *apx,
+                          const double *apy, const double *apz, const double *fcx,
+                          const double *fcy, const double *fcz) {
+    ora_ebgeom *G = &L->geom[b];
+    const obox g = bgrow(L->box[b], 7);
+    const double *ap[3] = {apx, apy, apz}, *fc[3] = {fcx, fcy, fcz};
+    ora_fab_free(&G->vfrac);
+    fab_alloc_box(&G->vfrac, g, 1);
+    memcpy(G->vfrac.p, vfrac, sizeof(double) * (size_t)G->vfrac.ns);
+    for (int d = 0; d < 3; ++d) {
+        ora_fab_free(&G->ap[d]); ora_fab_free(&G->fc[d]);
+        fab_alloc_box(&G->ap[d], bnodes(g, d), 1);
+        fab_alloc_box(&G->fc[d], bnodes(g, d), 2);
+        memcpy(G->ap[d].p, ap[d], sizeof(double) * (size_t)G->ap[d].ns);
+        memcpy(G->fc[d].p, fc[d], sizeof(double) * (size_t)(2 * G->fc[d].ns));
     }
 }

@@ -58,7 +58,19 @@ typedef struct ora_params {
     double dx[3];
     double prob_lo[3];
     int tile[3]; /* hydro_tile_size used by construct_hydro_source */
+    int eb;      /* 1: the guards of an AMREX_USE_EB build (covered cells) */
 } ora_params;
+
+/* Cut-cell geometry of one box (the EBFArrayBoxFactory arrays of an MFIter):
+ * flags over flo..fhi (a view; see camr_oracle.c for the word layout),
+ * vfrac on grow(bx, 7), apertures ap[d] and face centroids fc[d] (2
+ * components, the tangential offsets in [-1/2, 1/2]) on its faces. */
+typedef struct ora_ebgeom {
+    const int32_t *flags;
+    int flo[3], fhi[3];
+    int64_t fjs, fks;
+    ora_fab vfrac, ap[3], fc[3];
+} ora_ebgeom;
 
 void ora_default_params(ora_params *p);
 
@@ -93,6 +105,12 @@ void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
                      const ora_params *p);
 double ora_estdt_box(const int lo[3], const int hi[3], const ora_fab *u,
                      const double dx[3]);
+/* hydro_umdrv_eb (Godunov, FluxRedist): q/qa/srcq and uin on grow(bx, 6);
+ * dsdt on bx overwritten on non-covered cells; flx[d] on faces of bx. */
+void ora_hydro_umdrv_eb(const int lo[3], const int hi[3], const ora_fab *uin,
+                        ora_fab *dsdt, const ora_fab *q, const ora_fab *qa,
+                        const ora_fab *srcq, ora_fab flx[3], double dt,
+                        const ora_params *p, const ora_ebgeom *geom);
 
 /* ---- level driver (single level, boxes covering the domain) ---- */
 typedef struct ora_level ora_level;
@@ -117,6 +135,16 @@ double ora_level_time(const ora_level *L);
 /* Run nsteps with the reference's dt sequence; dts (may be NULL) receives
  * the dt of every step. Returns 0 or error code. */
 int    ora_level_run(ora_level *L, int nsteps, double *dts);
+/* EB build of the level (call right after create, before set_box): flags
+ * over flo..fhi = the domain grown by 7 cells; then the cut-cell data of
+ * every box whose kind is 1.  Kinds: 0 regular, 1 hydro_umdrv_eb, 2 covered. */
+int    ora_level_set_eb(ora_level *L, const int32_t *flags, const int flo[3],
+                        const int fhi[3]);
+int    ora_level_box_kind(const ora_level *L, int b);
+void   ora_level_set_box_eb(ora_level *L, int b, const double *vfrac,
+                            const double *apx, const double *apy,
+                            const double *apz, const double *fcx,
+                            const double *fcy, const double *fcz);
 /* Sum of each component over valid cells (for conservation checks). */
 void   ora_level_sums(const ora_level *L, double sums[O_NVAR]);
 

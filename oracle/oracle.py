@@ -44,6 +44,7 @@ class OraParams(C.Structure):
         ("domlo", C.c_int * 3), ("domhi", C.c_int * 3),
         ("is_periodic", C.c_int * 3), ("dx", C.c_double * 3),
         ("prob_lo", C.c_double * 3), ("tile", C.c_int * 3),
+        ("eb", C.c_int),
     ]
 
 
@@ -98,6 +99,12 @@ def lib():
         L.ora_level_time.argtypes = [C.c_void_p]
         L.ora_level_time.restype = C.c_double
         L.ora_level_sums.argtypes = [C.c_void_p, dp]
+        ip32 = C.POINTER(C.c_int32)
+        L.ora_level_set_eb.argtypes = [C.c_void_p, ip32, ip, ip]
+        L.ora_level_set_eb.restype = C.c_int
+        L.ora_level_box_kind.argtypes = [C.c_void_p, C.c_int]
+        L.ora_level_box_kind.restype = C.c_int
+        L.ora_level_set_box_eb.argtypes = [C.c_void_p, C.c_int] + [dp] * 7
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_get_threads.restype = C.c_int
         _lib = L
@@ -190,6 +197,24 @@ class Level:
         a = np.empty(self.shape(b))
         lib().ora_level_get_box(self._h, b, _dp(a))
         return a
+
+    def set_eb(self, geom):
+        """EB build of the level with the cut-cell data of `geom`
+        (``camr_amd.eb.Geometry``); call before set_box."""
+        flags = np.ascontiguousarray(geom.flags, dtype=np.int32)
+        flo = np.array(geom.lo, dtype=np.int32)
+        fhi = np.array(geom.hi, dtype=np.int32)
+        ipt = C.POINTER(C.c_int)
+        lib().ora_level_set_eb(self._h, flags.ctypes.data_as(C.POINTER(C.c_int32)),
+                               flo.ctypes.data_as(ipt), fhi.ctypes.data_as(ipt))
+        self.kinds = [lib().ora_level_box_kind(self._h, b) for b in range(len(self.boxes))]
+        for b, k in enumerate(self.kinds):
+            if k == 1:
+                g = geom.box_geom(b)
+                arrs = [np.ascontiguousarray(g[n], dtype=np.float64)
+                        for n in ("vfrac", "apx", "apy", "apz", "fcx", "fcy", "fcz")]
+                lib().ora_level_set_box_eb(self._h, b, *[_dp(a) for a in arrs])
+        self._geom_keep = geom
 
     def set_exterior(self, data):
         n = [self.p.domhi[d] - self.p.domlo[d] + 1 for d in range(3)]
