@@ -79,7 +79,8 @@ EXPORTS = [
     "camr_level_time", "camr_level_profile", "camr_level_profile_report",
     "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
-    "camr_level_adjoint_backward",
+    "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
+    "camr_level_box_kind", "camr_level_set_box_eb",
 ]
 
 _lib = None
@@ -134,6 +135,10 @@ def lib():
         L.camr_level_adj_upload_box.argtypes = [vp, C.c_int32, dp]
         L.camr_level_adj_download_box.argtypes = [vp, C.c_int32, dp]
         L.camr_level_adjoint_backward.argtypes = [vp]
+        i32p = C.POINTER(C.c_int32)
+        L.camr_level_set_eb.argtypes = [vp, i32p, i32p, i32p]
+        L.camr_level_box_kind.argtypes = [vp, C.c_int32, i32p]
+        L.camr_level_set_box_eb.argtypes = [vp, C.c_int32] + [dp] * 7
         for name in EXPORTS:
             f = getattr(L, name)
             if f.restype is C.c_int:  # default restype

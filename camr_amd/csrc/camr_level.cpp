@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -85,6 +86,7 @@ HydroConst make_const(const camr_hydro_params &p) {
         h.domhi[d] = p.domhi[d];
         h.dx[d] = p.dx[d];
     }
+    h.eb = 0;
     return h;
 }
 
@@ -129,6 +131,10 @@ class HydroLevel {
     void set_stream(hipStream_t s) { stream_ = s; }
     double time() const { return time_; }
     Prof &prof() { return prof_; }
+    int set_eb(const int32_t *flags, const int32_t flo[3], const int32_t fhi[3]);
+    int set_box_eb(int b, const double *vf, const double *const ap[3],
+                   const double *const fc[3]);
+    int box_kind(int b, int32_t *kind) const;
 
   private:
     int est_time_step(double *est);
@@ -164,6 +170,14 @@ class HydroLevel {
     Prof prof_;
     bool fused_ = true;
     FusedEngine engine_;
+    // embedded boundary (AMREX_USE_EB build): ghost width 6, flags over the
+    // domain grown by 7, per-box path (0 regular, 1 cut cells, 2 covered)
+    int ng_ = CAMR_NGROW;
+    int32_t *ebflags_ = nullptr;
+    EbFlags ebf_{};
+    std::vector<int> kind_;
+    std::vector<EbGeom> ebg_;
+    std::vector<double *> ebbuf_;
     bool pending_ = false;   // a fused step awaits resolve()
     bool needs_ext_ = false; // some ghost cell has no owning box
     bool dt_ready_ = false;  // dtmin_ holds the CFL estimate of S_new
@@ -280,6 +294,9 @@ void HydroLevel::release() {
     gstream_ = nullptr;
     fr(q_); fr(qaux_); fr(srcq_);
     for (auto &f : flx_) { fr(f); f = nullptr; }
+    fr(ebflags_); ebflags_ = nullptr;
+    for (auto p : ebbuf_) fr(p);
+    ebbuf_.clear();
     if (h_status_) (void)hipHostFree(h_status_);
     if (h_dtmin_) (void)hipHostFree(h_dtmin_);
     tab_old_ = tab_sb_ = tab_src_ = nullptr;
@@ -463,6 +480,119 @@ int HydroLevel::create() {
     return CAMR_OK;
 }
 
+// Embedded boundary (the AMREX_USE_EB build of the level): ghost width 6
+// (CAMR::numGrow for Godunov + FluxRedist, CAMR.H:455-483), the cut-cell
+// flags over flo..fhi (which must hold the domain grown by 7) and the box
+// paths of CAMR_construct_hydro_source.cpp:80-83,150-151.  The level runs
+// on the staged engine; every box needs the cut-cell data of set_box_eb when
+// its kind is 1.
+int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t fhi[3]) {
+    if (!flags || !flo || !fhi) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    if (fused_) { set_error("embedded boundaries run on the staged engine"); return CAMR_ERR_PARAM; }
+    if (ebflags_) { set_error("embedded boundary already set"); return CAMR_ERR_PARAM; }
+    if (desc_.hp.ppm_type != 0 || desc_.hp.use_pslope != 0) {
+        set_error("embedded boundaries need ppm_type = 0 and use_pslope = 0 (CAMR.cpp:186-190)");
+        return CAMR_ERR_PARAM;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (std::fabs(desc_.hp.dx[d] - desc_.hp.dx[0]) > 1e-12 * desc_.hp.dx[0]) {
+            set_error("embedded boundaries need dx == dy == dz (CAMR.cpp:305-313)");
+            return CAMR_ERR_PARAM;
+        }
+    const Box fb = make_box(flo, fhi);
+    const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
+    if (!fb.ok() || !fb.contains(grow(dom, 7))) {
+        set_error("EB flags must cover the domain grown by 7 cells");
+        return CAMR_ERR_SHAPE;
+    }
+    if (needs_ext_) { set_error("embedded boundaries need boxes that cover the domain"); return CAMR_ERR_SHAPE; }
+    int rc = resolve();
+    if (rc) return rc;
+    CAMR_HIP(hipStreamSynchronize(stream_));
+    const size_t nf = fb.npts();
+    CAMR_HIP(hipMalloc(&ebflags_, nf * sizeof(int32_t)));
+    CAMR_HIP(hipMemcpy(ebflags_, flags, nf * sizeof(int32_t), hipMemcpyHostToDevice));
+    ebf_.p = ebflags_;
+    ebf_.lo0 = fb.lo[0]; ebf_.lo1 = fb.lo[1]; ebf_.lo2 = fb.lo[2];
+    ebf_.js = fb.len(0);
+    ebf_.ks = (int64_t)fb.len(0) * fb.len(1);
+    EbFlags hf = ebf_;
+    hf.p = flags;
+    ng_ = 6;
+    hc_.eb = 1;
+    const int nb = (int)box_.size();
+    kind_.assign(nb, 0);
+    ebg_.assign(nb, EbGeom{});
+    size_t max_g = 0, sc = 0;
+    for (int b = 0; b < nb; ++b) {
+        const Box B = box_[b];
+        bool all_cov = true, nonreg = false;
+        const Box G2 = grow(B, 2);
+        for (int k = G2.lo[2]; k <= G2.hi[2]; ++k)
+            for (int j = G2.lo[1]; j <= G2.hi[1]; ++j)
+                for (int i = G2.lo[0]; i <= G2.hi[0]; ++i) {
+                    const int t = eb_type(hf(i, j, k));
+                    if (B.contains(i, j, k) && t != kEbCovered) all_cov = false;
+                    if (t != kEbRegular) nonreg = true;
+                }
+        kind_[b] = all_cov ? 2 : (nonreg ? 1 : 0);
+        ebg_[b].f = ebf_;
+        max_g = std::max<size_t>(max_g, grow(B, ng_).npts());
+        sc = std::max(sc, umdrv_staged_scratch_doubles(B));
+        if (kind_[b] == 1) sc = std::max(sc, umdrv_eb_scratch_doubles(B, grow(B, ng_)));
+        // Sborder with the wider ghost region
+        (void)hipFree(sb_[b]);
+        sb_[b] = nullptr;
+        CAMR_HIP(hipMalloc(&sb_[b], NVAR * grow(B, ng_).npts() * sizeof(double)));
+        sbv_[b] = dense(sb_[b], grow(B, ng_));
+    }
+    std::vector<DevFab> t(nb);
+    for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
+    CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    auto fr = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
+    fr(q_); fr(qaux_); fr(srcq_);
+    CAMR_HIP(hipMalloc(&q_, QVAR * max_g * sizeof(double)));
+    CAMR_HIP(hipMalloc(&qaux_, NQAUX * max_g * sizeof(double)));
+    CAMR_HIP(hipMalloc(&srcq_, QVAR * max_g * sizeof(double)));
+    if (!scratch_.reserve(sc * sizeof(double))) {
+        set_error("scratch allocation of %zu bytes failed", sc * sizeof(double));
+        return CAMR_ERR_ALLOC;
+    }
+    return CAMR_OK;
+}
+
+int HydroLevel::set_box_eb(int b, const double *vf, const double *const ap[3],
+                           const double *const fc[3]) {
+    if (!ebflags_) { set_error("set the embedded boundary first"); return CAMR_ERR_PARAM; }
+    if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
+    if (!vf || !ap[0] || !ap[1] || !ap[2] || !fc[0] || !fc[1] || !fc[2]) {
+        set_error("null argument");
+        return CAMR_ERR_PARAM;
+    }
+    const Box G = grow(box_[b], 7);
+    auto up = [&](const double *h, const Box &bb, int nc, Arr *out) -> int {
+        double *p = nullptr;
+        const size_t n = (size_t)nc * bb.npts();
+        CAMR_HIP(hipMalloc(&p, n * sizeof(double)));
+        ebbuf_.push_back(p);
+        CAMR_HIP(hipMemcpy(p, h, n * sizeof(double), hipMemcpyHostToDevice));
+        *out = dense(p, bb);
+        return CAMR_OK;
+    };
+    EbGeom &g = ebg_[b];
+    int rc = up(vf, G, 1, &g.vf);
+    for (int d = 0; d < 3 && !rc; ++d) rc = up(ap[d], nodes(G, d), 1, &g.ap[d]);
+    for (int d = 0; d < 3 && !rc; ++d) rc = up(fc[d], nodes(G, d), 2, &g.fc[d]);
+    g.f = ebf_;
+    return rc;
+}
+
+int HydroLevel::box_kind(int b, int32_t *kind) const {
+    if (b < 0 || b >= (int)box_.size() || !kind) { set_error("bad argument"); return CAMR_ERR_PARAM; }
+    *kind = kind_.empty() ? 0 : kind_[b];
+    return CAMR_OK;
+}
+
 int HydroLevel::upload(int b, const double *h) {
     if (b < 0 || b >= (int)box_.size()) { set_error("bad box index"); return CAMR_ERR_PARAM; }
     int rc = resolve();
@@ -540,8 +670,8 @@ int HydroLevel::init_post() {
 // enforce_min_density -> normalize_species -> computeTemp.
 int HydroLevel::clean_state(std::vector<Arr> &S, bool launch_detect) {
     for (size_t b = 0; b < box_.size(); ++b) {
-        if (launch_detect) launch_on(k_detect, box_[b], stream_, S[b], hc_.small_dens, status_);
-        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], S[b], hc_, status_);
+        if (launch_detect) launch_on(k_detect, box_[b], stream_, S[b], hc_.small_dens, status_, ebf_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], S[b], hc_, status_, ebf_);
         launch_on(k_clean_point, box_[b], stream_, S[b], hc_);
     }
     return CAMR_OK;
@@ -554,21 +684,21 @@ int HydroLevel::expand_state() {
     for (size_t b = 0; b < box_.size(); ++b) { t[b].a = sold_[b]; t[b].b = box_[b]; }
     CAMR_HIP(hipMemcpyAsync(tab_old_, t.data(), t.size() * sizeof(DevFab), hipMemcpyHostToDevice, stream_));
     for (size_t b = 0; b < box_.size(); ++b) {
-        const Box G = grow(box_[b], CAMR_NGROW);
+        const Box G = grow(box_[b], ng_);
         launch_on(k_fill_raw, G, stream_, sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_,
                   extv_, ext_ ? 1 : 0, status_);
     }
     for (size_t b = 0; b < box_.size(); ++b) {
-        launch_on(k_detect, box_[b], stream_, sbv_[b], hc_.small_dens, status_);
-        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], sbv_[b], hc_, status_);
+        launch_on(k_detect, box_[b], stream_, sbv_[b], hc_.small_dens, status_, ebf_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], sbv_[b], hc_, status_, ebf_);
         launch_on(k_normalize, box_[b], stream_, sbv_[b]);
     }
     for (size_t b = 0; b < box_.size(); ++b) {
-        const Box G = grow(box_[b], CAMR_NGROW);
+        const Box G = grow(box_[b], ng_);
         launch_on(k_resync, G, stream_, box_[b], sbv_[b], (const DevFab *)tab_sb_, (const int *)owner_, dom_);
     }
     for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_resetT, grow(box_[b], CAMR_NGROW), stream_, sbv_[b], hc_);
+        launch_on(k_resetT, grow(box_[b], ng_), stream_, sbv_[b], hc_);
     // hipMemcpyAsync from pageable host memory is synchronous w.r.t. the
     // host buffer, so `t` may go out of scope here.
     return CAMR_OK;
@@ -600,13 +730,38 @@ int HydroLevel::construct_hydro_source(double dt) {
         return CAMR_OK;
     } else {
         for (size_t b = 0; b < box_.size(); ++b) {
-            const Box B = box_[b], G = grow(B, CAMR_NGROW);
+            const Box B = box_[b], G = grow(B, ng_);
+            Arr ds = dense(dsdt_[b], B);
+            launch_on(k_fill_value, B, stream_, ds, (int)NVAR, 0.0);
+            const int kind = kind_.empty() ? 0 : kind_[b];
+            if (kind == 2) continue;  // covered box (CAMR_construct_hydro_source.cpp:80)
+            if (kind == 1 && !ebg_[b].vf.p) {
+                set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", (int)b);
+                return CAMR_ERR_PARAM;
+            }
             Arr q = dense(q_, G), qa = dense(qaux_, G), sq = dense(srcq_, G);
             launch_on(k_ctoprim, G, stream_, sbv_[b], q, qa, hc_, status_);
             if (grav_src_values_)
                 launch_on(k_srctoprim, G, stream_, q, qa, dense(srchyd_[b], G), sq);
-            Arr ds = dense(dsdt_[b], B);
-            launch_on(k_fill_value, B, stream_, ds, (int)NVAR, 0.0);
+            if (kind == 1) {
+                UmdrvEbArgs E;
+                E.bx = B;
+                E.qbox = G;
+                E.hc = hc_;
+                for (int d = 0; d < 3; ++d) E.periodic[d] = desc_.is_periodic[d];
+                E.dt = dt;
+                E.uin = sbv_[b];
+                E.dsdt = ds;
+                E.q = q;
+                E.qaux = qa;
+                E.srcq = sq;
+                E.has_src = grav_src_values_ ? 1 : 0;
+                for (int d = 0; d < 3; ++d) E.flx[d] = dense(flx_[d], nodes(B, d));
+                E.g = ebg_[b];
+                int rc = umdrv_eb(E, scratch_.base(), stream_, &prof_);
+                if (rc) { set_error("kernel launch failed"); return rc; }
+                continue;
+            }
             UmdrvArgs A;
             A.bx = B;
             A.qbox = G;
@@ -635,7 +790,7 @@ int HydroLevel::construct_hydro_source(double dt) {
     }
     for (size_t b = 0; b < box_.size(); ++b)
         launch_on(k_add_hydro, box_[b], stream_, snew_[b], dense(dsdt_[b], box_[b]), dt,
-                  hc_.small_dens, status_);
+                  hc_.small_dens, status_, ebf_);
     return CAMR_OK;
 }
 
@@ -688,7 +843,7 @@ int HydroLevel::advance_exact(double dt) {
         // new-time source correction + clean_state (:150-154)
         for (size_t b = 0; b < box_.size(); ++b)
             launch_on(k_add_newsrc, box_[b], stream_, snew_[b], sold_[b], dt, desc_.const_grav,
-                      hc_.small_dens, status_);
+                      hc_.small_dens, status_, ebf_);
         clean_state(snew_, false);
     }
     // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
@@ -842,7 +997,7 @@ int HydroLevel::run_graph(int nsteps, double *dts) {
             launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
             for (size_t b = 0; b < box_.size(); ++b)
                 launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2],
-                          dtmin_, status_);
+                          dtmin_, status_, ebf_);
             CAMR_HIP(hipGetLastError());
         }
         StepCtl c{};
@@ -1158,7 +1313,7 @@ int HydroLevel::est_time_step(double *est) {
         const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
         launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
         for (size_t b = 0; b < box_.size(); ++b)
-            launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_);
+            launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_, ebf_);
         CAMR_HIP(hipGetLastError());
         dt_ready_ = true;
     }
@@ -1351,6 +1506,69 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     return CAMR_OK;
 }
 
+int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
+                        const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
+                        const camr_array4 flx[3], const camr_eb_geom *geom, double dt,
+                        const camr_hydro_params *p, const int32_t is_periodic[3], void *stream) {
+    if (!bx || !uin || !dsdt || !q || !qaux || !flx || !geom || !p || !is_periodic || !geom->flags) {
+        set_error("null argument");
+        return CAMR_ERR_PARAM;
+    }
+    int rc = check_params(*p);
+    if (rc) return rc;
+    if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    if (p->ppm_type != 0 || p->use_pslope != 0) {
+        set_error("hydro_umdrv_eb needs ppm_type = 0 and use_pslope = 0 (CAMR.cpp:186-190)");
+        return CAMR_ERR_PARAM;
+    }
+    const Box B = make_box(bx->lo, bx->hi);
+    const Box G = grow(B, 6), G7 = grow(B, 7);
+    if (!B.ok() || !covers(*q, G, QVAR) || !covers(*qaux, G, NQAUX) || !covers(*uin, G, NVAR) ||
+        !covers(*dsdt, B, NVAR) || !make_box(geom->flo, geom->fhi).contains(G7) ||
+        !covers(geom->vfrac, G7, 1)) {
+        set_error("camr_hydro_umdrv_eb: q/qaux/uin must cover grow(bx,6), dsdt bx, the geometry grow(bx,7)");
+        return CAMR_ERR_SHAPE;
+    }
+    const bool has_src = srcq && srcq->p;
+    if (has_src && !covers(*srcq, G, QVAR)) { set_error("srcq must cover grow(bx,6)"); return CAMR_ERR_SHAPE; }
+    for (int d = 0; d < 3; ++d) {
+        if (!covers(flx[d], nodes(B, d), NVAR)) { set_error("flx[%d] must cover the faces of bx", d); return CAMR_ERR_SHAPE; }
+        if (!covers(geom->ap[d], nodes(G7, d), 1) || !covers(geom->fc[d], nodes(G7, d), 2)) {
+            set_error("ap[%d] / fc[%d] must cover the faces of grow(bx,7)", d, d);
+            return CAMR_ERR_SHAPE;
+        }
+    }
+    UmdrvEbArgs A;
+    A.bx = B;
+    A.qbox = box_of(*q);
+    A.hc = make_const(*p);
+    A.hc.eb = 1;
+    for (int d = 0; d < 3; ++d) A.periodic[d] = is_periodic[d];
+    A.dt = dt;
+    A.uin = view(*uin);
+    A.dsdt = view(*dsdt);
+    A.q = view(*q);
+    A.qaux = view(*qaux);
+    A.srcq = has_src ? view(*srcq) : view(*q);
+    A.has_src = has_src ? 1 : 0;
+    for (int d = 0; d < 3; ++d) A.flx[d] = view(flx[d]);
+    const Box fb = make_box(geom->flo, geom->fhi);
+    A.g.f.p = geom->flags;
+    A.g.f.lo0 = fb.lo[0]; A.g.f.lo1 = fb.lo[1]; A.g.f.lo2 = fb.lo[2];
+    A.g.f.js = fb.len(0);
+    A.g.f.ks = (int64_t)fb.len(0) * fb.len(1);
+    A.g.vf = view(geom->vfrac);
+    for (int d = 0; d < 3; ++d) { A.g.ap[d] = view(geom->ap[d]); A.g.fc[d] = view(geom->fc[d]); }
+    hipStream_t s = (hipStream_t)stream;
+    double *scr = nullptr;
+    const size_t n = umdrv_eb_scratch_doubles(B, A.qbox);
+    CAMR_HIP(hipMallocAsync((void **)&scr, n * sizeof(double), s));
+    rc = umdrv_eb(A, scr, s);
+    CAMR_HIP(hipFreeAsync(scr, s));
+    if (rc) { set_error("kernel launch failed"); return rc; }
+    return CAMR_OK;
+}
+
 int camr_estdt_hydro(const camr_box *bx, const camr_array4 *u, const double dx[3],
                      double *dt_out, void *stream) {
     if (!bx || !u || !dx || !dt_out) { set_error("null argument"); return CAMR_ERR_PARAM; }
@@ -1361,7 +1579,8 @@ int camr_estdt_hydro(const camr_box *bx, const camr_array4 *u, const double dx[3
     char *buf = nullptr;
     CAMR_HIP(hipMallocAsync((void **)&buf, 16, s));
     CAMR_HIP(hipMemcpyAsync(buf, &h, 16, hipMemcpyHostToDevice, s));
-    launch_on(k_estdt, B, s, view(*u), dx[0], dx[1], dx[2], (unsigned long long *)buf, (int *)(buf + 8));
+    launch_on(k_estdt, B, s, view(*u), dx[0], dx[1], dx[2], (unsigned long long *)buf, (int *)(buf + 8),
+              EbFlags{});
     CAMR_HIP(hipMemcpyAsync(&h, buf, 16, hipMemcpyDeviceToHost, s));
     CAMR_HIP(hipFreeAsync(buf, s));
     CAMR_HIP(hipStreamSynchronize(s));
@@ -1427,6 +1646,20 @@ int camr_level_adjoint_backward(camr_level *L) {
 }
 int camr_level_set_exterior(camr_level *L, const double *host) {
     return L ? L->impl.set_exterior(host) : CAMR_ERR_PARAM;
+}
+int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
+                      const int32_t fhi[3]) {
+    return L ? L->impl.set_eb(flags, flo, fhi) : CAMR_ERR_PARAM;
+}
+int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac, const double *apx,
+                          const double *apy, const double *apz, const double *fcx,
+                          const double *fcy, const double *fcz) {
+    if (!L) return CAMR_ERR_PARAM;
+    const double *ap[3] = {apx, apy, apz}, *fc[3] = {fcx, fcy, fcz};
+    return L->impl.set_box_eb(b, vfrac, ap, fc);
+}
+int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind) {
+    return L ? L->impl.box_kind(b, kind) : CAMR_ERR_PARAM;
 }
 int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order) {
     if (!L || !block) { set_error("null argument"); return CAMR_ERR_PARAM; }

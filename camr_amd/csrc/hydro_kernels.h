@@ -94,10 +94,13 @@ __device__ __forceinline__ double divu_node(const QA &q, int i, int j, int k,
 // ---- adjust_fluxes for one face: hydro_artif_visc (hydro_artif_visc.H:7-64),
 // hydro_norm_spec_flx (Hydro_utils_K.H:207-219), hydro_ext_flx (:224-234).
 // f holds the NVAR fluxes of the face, u the conserved state (Sborder).
+// open == false (a closed cut-cell face, adjust_fluxes_eb): the species
+// normalisation and the area factor are skipped.
 template <int D, class DA, class UA>
 __device__ __forceinline__ void adjust_face(double *f, const DA &divu,
                                             const UA &u, int i, int j, int k,
-                                            double area, const HydroConst &hc) {
+                                            double area, const HydroConst &hc,
+                                            bool open = true) {
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int L1 = (D == 0) ? 1 : 0;
     constexpr int L2 = (D == 2) ? 1 : 2;
@@ -119,6 +122,7 @@ __device__ __forceinline__ void adjust_face(double *f, const DA &divu,
             f[n] += dxd * div * (u(i, j, k, n) - u(i - ex, j - ey, k - ez, n));
     }
     f[UTEMP] = 0.0;
+    if (!open) return;
     double summ = 0.0;
     summ += f[UFS];
     summ += f[UFS + 1];

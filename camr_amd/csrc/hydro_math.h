@@ -50,6 +50,7 @@ struct HydroConst {
     double eta1, eta2;
     int bclo[3], bchi[3], domlo[3], domhi[3];
     double dx[3];
+    int eb;  // 1: the covered-cell guards of an AMREX_USE_EB build
 };
 
 // ---- EOS (Source/EOS/GammaLaw/EOS.H) ---------------------------------------

@@ -1,7 +1,7 @@
 // hydro_state.h -- declarations of the level-step state kernels.
 #pragma once
 
-#include "hydro_kernels.h"
+#include "hydro_eb.h"
 
 namespace camr {
 
@@ -34,9 +34,9 @@ __global__ void k_step_dt(StepCtl *c, unsigned long long *dtmin, int *status, Dt
 __global__ void k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status);
 __global__ void k_srctoprim(Range r, Arr q, Arr qa, Arr src, Arr srcq);
 __global__ void k_estdt(Range r, Arr u, double dx, double dy, double dz,
-                        unsigned long long *out, int *status);
-__global__ void k_detect(Range r, Arr S, double small_dens, int *flags);
-__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags);
+                        unsigned long long *out, int *status, EbFlags eb);
+__global__ void k_detect(Range r, Arr S, double small_dens, int *flags, EbFlags eb);
+__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags, EbFlags eb);
 __global__ void k_clean_point(Range r, Arr S, HydroConst hc);
 __global__ void k_normalize(Range r, Arr S);
 __global__ void k_resetT(Range r, Arr S, HydroConst hc);
@@ -48,9 +48,9 @@ __global__ void k_resync(Range r, Box valid, Arr dst, const DevFab *src,
 __global__ void k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt,
                               double g, int add_grav);
 __global__ void k_add_hydro(Range r, Arr snew, Arr dsdt, double dt,
-                            double small_dens, int *flags);
+                            double small_dens, int *flags, EbFlags eb);
 __global__ void k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g,
-                             double small_dens, int *flags);
+                             double small_dens, int *flags, EbFlags eb);
 __global__ void k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g);
 __global__ void k_fill_value(Range r, Arr a, int nc, double v);
 __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v);

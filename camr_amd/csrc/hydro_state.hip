@@ -16,7 +16,13 @@ k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const double rho = u(i, j, k, URHO);
-    if (!(rho > 0.)) return;
+    if (!(rho > 0.)) {
+        if (hc.eb) {  // covered cell: q = 0 (CAMR_construct_hydro_source.cpp:116-118)
+            for (int n = 0; n < QVAR; ++n) q(i, j, k, n) = 0.;
+            for (int n = 0; n < NQAUX; ++n) qa(i, j, k, n) = 0.;
+        }
+        return;
+    }
     const double rhoinv = 1.0 / rho;
     const double vx = u(i, j, k, UMX) * rhoinv;
     const double vy = u(i, j, k, UMY) * rhoinv;
@@ -83,30 +89,33 @@ k_srctoprim(Range r, Arr q, Arr qa, Arr src, Arr srcq) {
 // so the block minimum is combined with a 64-bit integer atomicMin.
 __global__ void __launch_bounds__(kBlock)
 k_estdt(Range r, Arr u, double dx, double dy, double dz, unsigned long long *out,
-        int *status) {
+        int *status, EbFlags eb) {
     int i, j, k;
     double v = 1.7976931348623157e308;
-    if (range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+    if (range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k) &&
+        !eb.covered(i, j, k))  // Timestep.H:32-33
         v = estdt_point(u, i, j, k, dx, dy, dz);
     block_min_commit(v, out, status);
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_detect(Range r, Arr S, double small_dens, int *flags) {
+k_detect(Range r, Arr S, double small_dens, int *flags, EbFlags eb) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (eb.covered(i, j, k)) return;  // CAMR.cpp:987-988
     floor_detect(S(i, j, k, URHO), small_dens, flags);
 }
 
 // enforce_min_density (CAMR.cpp:941-1045), exact sequential order of the
 // reference CPU loop over one box; runs only if a floor was detected.
-__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags) {
+__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags, EbFlags eb) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (!(*flags & kFloorFlag)) return;
     const double sd = hc.small_dens;
     for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
     for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
     for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        if (eb.covered(i, j, k)) continue;
         const double r0 = S(i, j, k, URHO);
         if (r0 == 0. || !(r0 < sd)) continue;
         double max_dens = r0;
@@ -184,9 +193,11 @@ k_init_post(Range r, Arr S, HydroConst hc) {
     double s[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
-    const double rhoInv = 1.0 / s[URHO];
-    const double u = s[UMX] * rhoInv, v = s[UMY] * rhoInv, w = s[UMZ] * rhoInv;
-    s[UEDEN] = s[UEINT] + 0.5 * s[URHO] * (u * u + v * v + w * w);
+    if (!hc.eb || s[URHO] > 0.) {  // EB build: CAMR.cpp:921-923
+        const double rhoInv = 1.0 / s[URHO];
+        const double u = s[UMX] * rhoInv, v = s[UMY] * rhoInv, w = s[UMZ] * rhoInv;
+        s[UEDEN] = s[UEINT] + 0.5 * s[URHO] * (u * u + v * v + w * w);
+    }
     reset_e_and_T(s, hc);
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) S(i, j, k, n) = s[n];
@@ -254,18 +265,20 @@ k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt, double g, int add_
 // S_new += dt * dSdt (CAMR_advance.cpp:141) + floor detection for the
 // following clean_state.
 __global__ void __launch_bounds__(kBlock)
-k_add_hydro(Range r, Arr snew, Arr dsdt, double dt, double small_dens, int *flags) {
+k_add_hydro(Range r, Arr snew, Arr dsdt, double dt, double small_dens, int *flags,
+            EbFlags eb) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) += dt * dsdt(i, j, k, n);
-    floor_detect(snew(i, j, k, URHO), small_dens, flags);
+    if (!eb.covered(i, j, k)) floor_detect(snew(i, j, k, URHO), small_dens, flags);
 }
 
 // New-time gravity correction source (GravitySrc.cpp:81-125) Saxpy
 // (CAMR_advance.cpp:150-152) + floor detection.
 __global__ void __launch_bounds__(kBlock)
-k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens, int *flags) {
+k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens, int *flags,
+             EbFlags eb) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const double s_umz = 0.5 * g * (snew(i, j, k, URHO) - sold(i, j, k, URHO));
@@ -275,7 +288,7 @@ k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens
         const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
         snew(i, j, k, n) += dt * src;
     }
-    floor_detect(snew(i, j, k, URHO), small_dens, flags);
+    if (!eb.covered(i, j, k)) floor_detect(snew(i, j, k, URHO), small_dens, flags);
 }
 
 // sources_for_hydro = 0 + 1.0 * old source on valid cells

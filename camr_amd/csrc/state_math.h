@@ -54,6 +54,9 @@ __device__ __forceinline__ void block_min_commit(double v, unsigned long long *o
 // of one cell.
 CAMR_HD void reset_e_and_T(double *s, const HydroConst &hc) {
     const double rho = s[URHO];
+    // EB build: covered cells (rho = 0) keep their zeros, T = 0
+    // (CAMR_reset_internal_e.H:36-40, CAMR.cpp:1166-1169)
+    if (hc.eb && !(rho > 0.)) { s[UTEMP] = 0.0; return; }
     const double rhoInv = 1.0 / rho;
     const double Up = s[UMX] * rhoInv;
     const double Vp = s[UMY] * rhoInv;

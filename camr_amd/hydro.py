@@ -108,6 +108,28 @@ class Level:
             self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
         return a
 
+    def set_eb(self, geom):
+        """Embedded-boundary build of the level with the cut-cell data of
+        `geom` (``camr_amd.eb.Geometry``).  The level must have been created
+        with ``engine=ENGINE_STAGED``; call before the first step."""
+        i32 = C.POINTER(C.c_int32)
+        flags = np.ascontiguousarray(geom.flags, dtype=np.int32)
+        flo = np.ascontiguousarray(geom.lo, dtype=np.int32)
+        fhi = np.ascontiguousarray(geom.hi, dtype=np.int32)
+        L.check(L.lib().camr_level_set_eb(self._h, flags.ctypes.data_as(i32),
+                                          flo.ctypes.data_as(i32), fhi.ctypes.data_as(i32)))
+        self.kinds = []
+        for b in range(len(self.prob.boxes)):
+            k = C.c_int32()
+            L.check(L.lib().camr_level_box_kind(self._h, b, C.byref(k)))
+            self.kinds.append(k.value)
+            if k.value == 1:
+                g = geom.box_geom(b)
+                arrs = [np.ascontiguousarray(g[n], dtype=np.float64)
+                        for n in ("vfrac", "apx", "apy", "apz", "fcx", "fcy", "fcz")]
+                L.check(L.lib().camr_level_set_box_eb(
+                    self._h, b, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]))
+
     def state_array(self, b) -> L.Array4:
         a = L.Array4()
         L.check(L.lib().camr_level_state_array(self._h, b, C.byref(a)))

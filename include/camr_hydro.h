@@ -125,6 +125,39 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin,
                      const camr_array4 *vol, double dt,
                      const camr_hydro_params *p, void *stream);
 
+/* Cut-cell geometry of one box: the EBFArrayBoxFactory arrays that
+ * hydro_umdrv_eb reads (Source/Hydro/Hydro_umdrv_eb.cpp:55-64) with the
+ * EBCellFlag array replaced by an int32 word per cell: bits 0..26 say
+ * whether the neighbour at offset (ii,jj,kk) is connected, bit
+ * (ii+1) + 3 (jj+1) + 9 (kk+1); bits 27..28 hold the type (0 regular, 1
+ * single-valued cut cell, 2 covered).  flags (device) covers flo..fhi;
+ * vfrac, ap[d] (area fractions of the d-faces) and fc[d] (their centroid
+ * offsets, 2 components: the two tangential directions in increasing
+ * order, in [-1/2, 1/2]) cover grow(bx, 7) and its faces. */
+typedef struct camr_eb_geom {
+    const int32_t *flags;
+    int32_t flo[3], fhi[3];
+    camr_array4 vfrac;
+    camr_array4 ap[3];
+    camr_array4 fc[3];
+} camr_eb_geom;
+
+/* hydro_umdrv_eb, Godunov branch with redistribution_type "FluxRedist" and
+ * eb_weights_type 2 on a single level (Source/Hydro/Hydro.H:56-91,
+ * Hydro_umdrv_eb.cpp:14-191; replaces the call at
+ * CAMR_construct_hydro_source.cpp:195-204).  q, qaux, srcq and uin cover
+ * grow(bx,6); dsdt (zeroed by the caller, as CAMR_construct_hydro_source.cpp:14
+ * does) receives the redistributed update on the non-covered cells of bx;
+ * flx[d] receives the face-centroid fluxes of the d-faces of bx (for flux
+ * registers).  Needs ppm_type 0 and dx == dy == dz.  No flux-register
+ * arguments: the single-level redistribution has none. */
+int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin,
+                        const camr_array4 *dsdt, const camr_array4 *q,
+                        const camr_array4 *qaux, const camr_array4 *srcq,
+                        const camr_array4 flx[3], const camr_eb_geom *geom,
+                        double dt, const camr_hydro_params *p,
+                        const int32_t is_periodic[3], void *stream);
+
 /* CAMR_estdt_hydro (Source/Utils/Timestep.H:18-61) min-reduced over bx.
  * Synchronous: *dt_out receives the minimum on return. */
 int camr_estdt_hydro(const camr_box *bx, const camr_array4 *u,
@@ -186,6 +219,23 @@ double camr_level_time(const camr_level *L);
  * the coarser level (FillPatch, CAMR.cpp:1247; CAMR_setup.cpp:141-144).
  * Without it such ghost cells are an error (CAMR_ERR_SHAPE). */
 int camr_level_set_exterior(camr_level *L, const double *host);
+
+/* The embedded-boundary build of a level (AMREX_USE_EB: CAMR.cpp:303-329,
+ * 419-421; CAMR_construct_hydro_source.cpp:77-204): ghost width 6, covered
+ * cells skipped by clean_state and the dt estimate, and per box the path of
+ * CAMR_construct_hydro_source.cpp:80-83,150-151 -- kind 2 (every valid cell
+ * covered) no hydro, kind 1 (a cut cell within 2 cells) hydro_umdrv_eb,
+ * kind 0 hydro_umdrv.  Call once, before the first step, on a level created
+ * with the staged engine whose boxes cover the domain; flags (host, same
+ * word as camr_eb_geom) cover flo..fhi, which must hold the domain grown by
+ * 7 cells.  Then give every kind-1 box its cut-cell data (host arrays on
+ * grow(bx,7), faces nodal, fc with 2 components). */
+int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
+                      const int32_t fhi[3]);
+int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind);
+int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac,
+                          const double *apx, const double *apy, const double *apz,
+                          const double *fcx, const double *fcy, const double *fcz);
 
 /* Launch configuration of the fused engine: block size (64, 128 or 256) of
  * each of the CAMR_NSTAGE stage ranges (grow(bx,4), grow(bx,2), first-flux
