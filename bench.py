@@ -224,6 +224,82 @@ def bench_sod(args) -> int:
     return 0
 
 
+def bench_eb(args) -> int:
+    """Sod past a sphere (embedded boundary, FluxRedist): (2n) x n x n cells
+    in 32^3 boxes, PLM, CFL 0.3.  The level takes the regular path in boxes
+    without cut cells and hydro_umdrv_eb in the others.  The first two
+    steps are checked bit for bit against the oracle (timed on the host
+    cores as the CPU baseline), then `warmup` + `steps` steps are timed."""
+    import numpy as np
+    import torch
+    from camr_amd import _lib, workloads
+    from camr_amd.hydro import Level
+    from oracle import oracle as O
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n = args.eb_n
+    prob = workloads.sod_sphere(n=n, max_grid=32)
+    t0 = time.perf_counter()
+    geom = workloads.sod_sphere_geometry(prob)
+    t_geom = time.perf_counter() - t0
+    cells = 2 * n * n * n
+    init = [workloads.sod_sphere_init(prob, b, geom) for b in range(len(prob.boxes))]
+    L = Level(prob, engine=_lib.ENGINE_STAGED)
+    L.set_eb(geom)
+    kinds = list(L.kinds)
+    for b, U in enumerate(init):
+        L.set_box(b, U)
+    L.init_post()
+    gdts = L.run(2)
+    identical = None
+    cpu = None
+    if not args.no_cpu_baseline:
+        O.build()
+        p = O.params(ppm_type=0, cfl=prob.value("cfl"), bclo=prob.bclo, bchi=prob.bchi,
+                     domlo=prob.domlo, domhi=prob.domhi, is_periodic=prob.is_periodic,
+                     dx=prob.dx, tile=(1 << 20,) * 3)
+        OL = O.Level(p, prob.boxes)
+        OL.set_eb(geom)
+        for b, U in enumerate(init):
+            OL.set_box(b, U)
+        OL.init_post()
+        t0 = time.perf_counter()
+        err, odts = OL.run(2)
+        el = time.perf_counter() - t0
+        assert err == 0
+        identical = bool(np.array_equal(np.asarray(gdts), odts) and
+                         all(np.array_equal(L.get_box(b), OL.get_box(b))
+                             for b in range(len(prob.boxes))))
+        OL.close()
+        cpu = {"value": cells * 2 / el, "unit": "cell-updates/s",
+               "cores": O.lib().ora_get_threads(), "kind": "port",
+               "sample": f"the first 2 steps of the same workload, {el:.2f} s wall"}
+    L.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L.run(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = cells * args.steps / el
+    out = {"metric": f"FP64 cell-updates/s, Sod past a sphere {2 * n}x{n}x{n} (EB, FluxRedist)",
+           "unit": "cell-updates/s", "value": value, "higher_is_better": True, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: Sod states around a sphere",
+           "config": {"workload": f"{2 * n}x{n}x{n} on [0,2]x[0,1]^2, sphere r=0.15 at "
+                                  "(0.6,0.5,0.5), Sod split at x=0.3, PLM, CFL 0.3, outflow x, "
+                                  "slip walls y/z, 32^3 boxes",
+                      "boxes": {"regular": kinds.count(0), "cut_cell": kinds.count(1),
+                                "covered": kinds.count(2)},
+                      "geometry_setup_s": round(t_geom, 2)},
+           "identical_to_oracle_2_steps": identical,
+           "roofline": {"bound": "hbm", "achieved": 160.0 * value / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": 160.0 * value / 8e12, "traffic": None,
+                        "kernel": "whole step, 160 B per cell-update"},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_adjoint(args) -> int:
     """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
     wall time of the backward pass (recomputation from the checkpoints
@@ -340,13 +416,16 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb"], default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
                          "AMR levels against per-box launches; adjoint: "
                          "backward/forward cost of the level-step adjoint; sod: "
-                         "the 256x4x4 Sod tube, GPU beside the host oracle")
+                         "the 256x4x4 Sod tube, GPU beside the host oracle; eb: "
+                         "Sod past a sphere, embedded boundary with FluxRedist")
     ap.add_argument("--amr-n0", type=int, default=128)
     ap.add_argument("--adj-n", type=int, default=128)
+    ap.add_argument("--eb-n", type=int, default=128,
+                    help="eb workload: n of the (2n) x n x n grid")
     ap.add_argument("--adj-dirs", type=int, default=16,
                     help="finite-difference directions of the adjoint check")
     ap.add_argument("--adj-reps", type=int, default=3)
@@ -366,6 +445,8 @@ def main() -> int:
         return bench_amr(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "eb":
+        return bench_eb(args)
     if args.workload == "sod":
         return bench_sod(args)
 
