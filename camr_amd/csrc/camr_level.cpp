@@ -139,6 +139,7 @@ class HydroLevel {
   private:
     int est_time_step(double *est);
     int advance_exact(double dt);
+    int advance_exact_passes(double dt);
     int advance_fused(double dt);
     int resolve();
     int clean_state(std::vector<Arr> &S, bool launch_detect);
@@ -173,6 +174,22 @@ class HydroLevel {
     // embedded boundary (AMREX_USE_EB build): ghost width 6, flags over the
     // domain grown by 7, per-box path (0 regular, 1 cut cells, 2 covered)
     int ng_ = CAMR_NGROW;
+    // staged engine: per-box work spread over worker streams (default
+    // priority), each with its own q / qaux / srcq and scratch; events join
+    // them where boxes read each other's data
+    static constexpr int kWorkers = 4;
+    int nws_ = 0;
+    hipStream_t wst_[kWorkers] = {};
+    hipEvent_t wev_[kWorkers] = {}, mev_ = nullptr;
+    size_t qstride_ = 0, sstride_ = 0;
+    hipStream_t ws(size_t b) const { return nws_ ? wst_[b % nws_] : stream_; }
+    bool in_ws_ = false;  // inside a step spread over the worker streams
+    hipStream_t cur_ws(size_t b) const { return in_ws_ ? ws(b) : stream_; }
+    size_t fstride_ = 0;
+    int wslot(size_t b) const { return nws_ ? (int)(b % nws_) : 0; }
+    int fork();
+    int join();
+    int setup_workers(size_t max_g, size_t sc);
     int32_t *ebflags_ = nullptr;
     EbFlags ebf_{};
     std::vector<int> kind_;
@@ -295,6 +312,15 @@ void HydroLevel::release() {
     fr(q_); fr(qaux_); fr(srcq_);
     for (auto &f : flx_) { fr(f); f = nullptr; }
     fr(ebflags_); ebflags_ = nullptr;
+    for (int w = 0; w < kWorkers; ++w) {
+        if (wst_[w]) (void)hipStreamDestroy(wst_[w]);
+        if (wev_[w]) (void)hipEventDestroy(wev_[w]);
+        wst_[w] = nullptr;
+        wev_[w] = nullptr;
+    }
+    if (mev_) (void)hipEventDestroy(mev_);
+    mev_ = nullptr;
+    nws_ = 0;
     for (auto p : ebbuf_) fr(p);
     ebbuf_.clear();
     if (h_status_) (void)hipHostFree(h_status_);
@@ -465,17 +491,12 @@ int HydroLevel::create() {
         else if (rc) return rc;
     }
     if (!fused_) {
-        if ((rc = alloc(&q_, QVAR * max_g4))) return rc;
-        if ((rc = alloc(&qaux_, NQAUX * max_g4))) return rc;
-        if ((rc = alloc(&srcq_, QVAR * max_g4))) return rc;
+        fstride_ = max_nodes;
         for (int d = 0; d < 3; ++d)
-            if ((rc = alloc(&flx_[d], NVAR * max_nodes))) return rc;
+            if ((rc = alloc(&flx_[d], NVAR * max_nodes * kWorkers))) return rc;
         size_t sc = 0;
         for (const Box &B : box_) sc = std::max(sc, umdrv_staged_scratch_doubles(B));
-        if (!scratch_.reserve(sc * sizeof(double))) {
-            set_error("scratch allocation of %zu bytes failed", sc * sizeof(double));
-            return CAMR_ERR_ALLOC;
-        }
+        if ((rc = setup_workers(max_g4, sc))) return rc;
     }
     return CAMR_OK;
 }
@@ -549,14 +570,48 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     std::vector<DevFab> t(nb);
     for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
     CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    return setup_workers(max_g, sc);
+}
+
+// q / qaux / srcq (max_g points each) and scratch (sc doubles) per worker
+// stream; with a single box the level's own stream does all the work.
+int HydroLevel::setup_workers(size_t max_g, size_t sc) {
     auto fr = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
     fr(q_); fr(qaux_); fr(srcq_);
-    CAMR_HIP(hipMalloc(&q_, QVAR * max_g * sizeof(double)));
-    CAMR_HIP(hipMalloc(&qaux_, NQAUX * max_g * sizeof(double)));
-    CAMR_HIP(hipMalloc(&srcq_, QVAR * max_g * sizeof(double)));
-    if (!scratch_.reserve(sc * sizeof(double))) {
-        set_error("scratch allocation of %zu bytes failed", sc * sizeof(double));
+    const int nw = (box_.size() > 1 && !grav_src_values_) ? kWorkers : 1;
+    qstride_ = max_g;
+    sstride_ = (sc + 63) / 64 * 64;
+    CAMR_HIP(hipMalloc(&q_, QVAR * max_g * nw * sizeof(double)));
+    CAMR_HIP(hipMalloc(&qaux_, NQAUX * max_g * nw * sizeof(double)));
+    CAMR_HIP(hipMalloc(&srcq_, QVAR * max_g * nw * sizeof(double)));
+    if (!scratch_.reserve(sstride_ * nw * sizeof(double))) {
+        set_error("scratch allocation of %zu bytes failed", sstride_ * nw * sizeof(double));
         return CAMR_ERR_ALLOC;
+    }
+    if (nw > 1 && !nws_) {
+        for (int w = 0; w < nw; ++w) {
+            CAMR_HIP(hipStreamCreateWithFlags(&wst_[w], hipStreamNonBlocking));
+            CAMR_HIP(hipEventCreateWithFlags(&wev_[w], hipEventDisableTiming));
+        }
+        CAMR_HIP(hipEventCreateWithFlags(&mev_, hipEventDisableTiming));
+        nws_ = nw;
+    }
+    return CAMR_OK;
+}
+
+// Worker streams wait for the level's stream / the level's stream waits for
+// every worker.
+int HydroLevel::fork() {
+    if (!nws_) return CAMR_OK;
+    CAMR_HIP(hipEventRecord(mev_, stream_));
+    for (int w = 0; w < nws_; ++w) CAMR_HIP(hipStreamWaitEvent(wst_[w], mev_, 0));
+    return CAMR_OK;
+}
+int HydroLevel::join() {
+    if (!nws_) return CAMR_OK;
+    for (int w = 0; w < nws_; ++w) {
+        CAMR_HIP(hipEventRecord(wev_[w], wst_[w]));
+        CAMR_HIP(hipStreamWaitEvent(stream_, wev_[w], 0));
     }
     return CAMR_OK;
 }
@@ -670,9 +725,10 @@ int HydroLevel::init_post() {
 // enforce_min_density -> normalize_species -> computeTemp.
 int HydroLevel::clean_state(std::vector<Arr> &S, bool launch_detect) {
     for (size_t b = 0; b < box_.size(); ++b) {
-        if (launch_detect) launch_on(k_detect, box_[b], stream_, S[b], hc_.small_dens, status_, ebf_);
-        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], S[b], hc_, status_, ebf_);
-        launch_on(k_clean_point, box_[b], stream_, S[b], hc_);
+        const hipStream_t s = cur_ws(b);
+        if (launch_detect) launch_on(k_detect, box_[b], s, S[b], hc_.small_dens, status_, ebf_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, s, box_[b], S[b], hc_, status_, ebf_);
+        launch_on(k_clean_point, box_[b], s, S[b], hc_);
     }
     return CAMR_OK;
 }
@@ -682,23 +738,28 @@ int HydroLevel::clean_state(std::vector<Arr> &S, bool launch_detect) {
 int HydroLevel::expand_state() {
     std::vector<DevFab> t(box_.size());
     for (size_t b = 0; b < box_.size(); ++b) { t[b].a = sold_[b]; t[b].b = box_[b]; }
+    // every box's S_old is final (the caller joined the workers)
     CAMR_HIP(hipMemcpyAsync(tab_old_, t.data(), t.size() * sizeof(DevFab), hipMemcpyHostToDevice, stream_));
+    if (in_ws_) fork();
     for (size_t b = 0; b < box_.size(); ++b) {
         const Box G = grow(box_[b], ng_);
-        launch_on(k_fill_raw, G, stream_, sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_,
+        launch_on(k_fill_raw, G, cur_ws(b), sbv_[b], (const DevFab *)tab_old_, (const int *)owner_, dom_,
                   extv_, ext_ ? 1 : 0, status_);
     }
     for (size_t b = 0; b < box_.size(); ++b) {
-        launch_on(k_detect, box_[b], stream_, sbv_[b], hc_.small_dens, status_, ebf_);
-        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, stream_, box_[b], sbv_[b], hc_, status_, ebf_);
-        launch_on(k_normalize, box_[b], stream_, sbv_[b]);
+        const hipStream_t s = cur_ws(b);
+        launch_on(k_detect, box_[b], s, sbv_[b], hc_.small_dens, status_, ebf_);
+        hipLaunchKernelGGL(k_floor_seq, dim3(1), dim3(1), 0, s, box_[b], sbv_[b], hc_, status_, ebf_);
+        launch_on(k_normalize, box_[b], s, sbv_[b]);
     }
+    // FillBoundary reads the other boxes' cleaned valid cells
+    if (in_ws_) { join(); fork(); }
     for (size_t b = 0; b < box_.size(); ++b) {
         const Box G = grow(box_[b], ng_);
-        launch_on(k_resync, G, stream_, box_[b], sbv_[b], (const DevFab *)tab_sb_, (const int *)owner_, dom_);
+        launch_on(k_resync, G, cur_ws(b), box_[b], sbv_[b], (const DevFab *)tab_sb_, (const int *)owner_, dom_);
     }
     for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_resetT, grow(box_[b], ng_), stream_, sbv_[b], hc_);
+        launch_on(k_resetT, grow(box_[b], ng_), cur_ws(b), sbv_[b], hc_);
     // hipMemcpyAsync from pageable host memory is synchronous w.r.t. the
     // host buffer, so `t` may go out of scope here.
     return CAMR_OK;
@@ -731,18 +792,23 @@ int HydroLevel::construct_hydro_source(double dt) {
     } else {
         for (size_t b = 0; b < box_.size(); ++b) {
             const Box B = box_[b], G = grow(B, ng_);
+            const hipStream_t bs = cur_ws(b);
+            const int w = in_ws_ ? wslot(b) : 0;
+            double *wq = q_ + (size_t)w * QVAR * qstride_, *wqa = qaux_ + (size_t)w * NQAUX * qstride_;
+            double *wsq = srcq_ + (size_t)w * QVAR * qstride_;
+            double *wscr = scratch_.base() + (size_t)w * sstride_;
             Arr ds = dense(dsdt_[b], B);
-            launch_on(k_fill_value, B, stream_, ds, (int)NVAR, 0.0);
+            launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
             const int kind = kind_.empty() ? 0 : kind_[b];
             if (kind == 2) continue;  // covered box (CAMR_construct_hydro_source.cpp:80)
             if (kind == 1 && !ebg_[b].vf.p) {
                 set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", (int)b);
                 return CAMR_ERR_PARAM;
             }
-            Arr q = dense(q_, G), qa = dense(qaux_, G), sq = dense(srcq_, G);
-            launch_on(k_ctoprim, G, stream_, sbv_[b], q, qa, hc_, status_);
+            Arr q = dense(wq, G), qa = dense(wqa, G), sq = dense(wsq, G);
+            launch_on(k_ctoprim, G, bs, sbv_[b], q, qa, hc_, status_);
             if (grav_src_values_)
-                launch_on(k_srctoprim, G, stream_, q, qa, dense(srchyd_[b], G), sq);
+                launch_on(k_srctoprim, G, bs, q, qa, dense(srchyd_[b], G), sq);
             if (kind == 1) {
                 UmdrvEbArgs E;
                 E.bx = B;
@@ -756,9 +822,10 @@ int HydroLevel::construct_hydro_source(double dt) {
                 E.qaux = qa;
                 E.srcq = sq;
                 E.has_src = grav_src_values_ ? 1 : 0;
-                for (int d = 0; d < 3; ++d) E.flx[d] = dense(flx_[d], nodes(B, d));
+                for (int d = 0; d < 3; ++d)
+                    E.flx[d] = dense(flx_[d] + (size_t)w * NVAR * fstride_, nodes(B, d));
                 E.g = ebg_[b];
-                int rc = umdrv_eb(E, scratch_.base(), stream_, &prof_);
+                int rc = umdrv_eb(E, wscr, bs, &prof_);
                 if (rc) { set_error("kernel launch failed"); return rc; }
                 continue;
             }
@@ -774,7 +841,7 @@ int HydroLevel::construct_hydro_source(double dt) {
             A.srcq = sq;
             A.has_src = grav_src_values_ ? 1 : 0;
             for (int d = 0; d < 3; ++d) {
-                A.flx[d] = dense(flx_[d], nodes(B, d));
+                A.flx[d] = dense(flx_[d] + (size_t)w * NVAR * fstride_, nodes(B, d));
                 A.area[d] = A.flx[d];
             }
             A.has_area = 0;
@@ -784,12 +851,12 @@ int HydroLevel::construct_hydro_source(double dt) {
             A.vol = ds;
             A.has_vol = 0;
             A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
-            int rc = umdrv_staged(A, scratch_.base(), stream_, &prof_);
+            int rc = umdrv_staged(A, wscr, bs, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
     }
     for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_add_hydro, box_[b], stream_, snew_[b], dense(dsdt_[b], box_[b]), dt,
+        launch_on(k_add_hydro, box_[b], cur_ws(b), snew_[b], dense(dsdt_[b], box_[b]), dt,
                   hc_.small_dens, status_, ebf_);
     return CAMR_OK;
 }
@@ -824,14 +891,29 @@ int HydroLevel::advance(double dt) {
 // The reference sequence pass by pass (CAMR_advance.cpp:72-164), starting
 // after the swap; S_old holds the raw previous state.
 int HydroLevel::advance_exact(double dt) {
+    // the staged engine spreads the per-box passes over its worker streams;
+    // the boxes meet only where one reads another's cells (FillPatch,
+    // FillBoundary), which the joins below order
+    in_ws_ = nws_ > 0 && !fused_;
+    int rc = fork();
+    if (!rc) rc = advance_exact_passes(dt);
+    const int rj = join();
+    in_ws_ = false;
+    return rc ? rc : rj;
+}
+
+int HydroLevel::advance_exact_passes(double dt) {
     // clean_state(S_old) (:83)
     clean_state(sold_, true);
-    // expand_state(Sborder, prev_time, numGrow()) (:99)
-    int rc = expand_state();
+    // expand_state(Sborder, prev_time, numGrow()) (:99): FillPatch reads
+    // every box's S_old
+    int rc = join();
+    if (rc) return rc;
+    rc = expand_state();
     if (rc) return rc;
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
     for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_copy_oldsrc, box_[b], stream_, snew_[b], sbv_[b], sold_[b], dt,
+        launch_on(k_copy_oldsrc, box_[b], cur_ws(b), snew_[b], sbv_[b], sold_[b], dt,
                   desc_.const_grav, desc_.add_grav_src);
     // construct_hydro_source + Saxpy (:138-141)
     scratch_.reset();
@@ -842,7 +924,7 @@ int HydroLevel::advance_exact(double dt) {
     if (desc_.add_grav_src) {
         // new-time source correction + clean_state (:150-154)
         for (size_t b = 0; b < box_.size(); ++b)
-            launch_on(k_add_newsrc, box_[b], stream_, snew_[b], sold_[b], dt, desc_.const_grav,
+            launch_on(k_add_newsrc, box_[b], cur_ws(b), snew_[b], sold_[b], dt, desc_.const_grav,
                       hc_.small_dens, status_, ebf_);
         clean_state(snew_, false);
     }
