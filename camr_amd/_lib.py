@@ -34,6 +34,11 @@ class Array4(C.Structure):
                 ("ncomp", C.c_int32)]
 
 
+class EbGeom(C.Structure):
+    _fields_ = [("flags", C.c_void_p), ("flo", C.c_int32 * 3), ("fhi", C.c_int32 * 3),
+                ("vfrac", Array4), ("ap", Array4 * 3), ("fc", Array4 * 3)]
+
+
 class HydroParams(C.Structure):
     _fields_ = [
         ("ppm_type", C.c_int32), ("plm_iorder", C.c_int32),
@@ -98,6 +103,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise OSError(f"{LIB_PATH} is not built; run camr_amd/build.py "
                           "(there is no CPU fallback for the hydro path)")
+        # one HIP runtime per process: when PyTorch is installed its bundled
+        # runtime is loaded first, so that the library and torch's device
+        # tensors share it
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
         L.camr_default_params.argtypes = [C.POINTER(HydroParams)]
@@ -137,6 +149,9 @@ def lib():
         L.camr_level_adjoint_backward.argtypes = [vp]
         i32p = C.POINTER(C.c_int32)
         L.camr_level_set_eb.argtypes = [vp, i32p, i32p, i32p]
+        L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,
+                                          C.POINTER(EbGeom), C.c_double,
+                                          C.POINTER(HydroParams), i32p, vp]
         L.camr_level_box_kind.argtypes = [vp, C.c_int32, i32p]
         L.camr_level_set_box_eb.argtypes = [vp, C.c_int32] + [dp] * 7
         for name in EXPORTS:

@@ -282,6 +282,32 @@ def hydro_umdrv(lo, hi, uin, uin_lo, dsdt, q, qa, srcq, flx, dt, params,
         dt, C.byref(params), _stream(stream)))
 
 
+def hydro_umdrv_eb(lo, hi, uin, dsdt, q, qa, flx, geom, dt, params, is_periodic,
+                   stream=0):
+    """hydro_umdrv_eb (Hydro.H:56-91, Godunov + FluxRedist) on device
+    tensors: uin / q / qa start at lo-6, dsdt and flx[d] at lo.  `geom` is
+    a dict of device tensors: flags (int32, from geom['flo']), vfrac, apx..,
+    fcx.. (from lo-7)."""
+    g6 = [x - 6 for x in lo]
+    g7 = [x - 7 for x in lo]
+    fl = (L.Array4 * 3)(*[array4_of(f, lo) for f in flx])
+    G = L.EbGeom()
+    G.flags = geom["flags"].data_ptr()
+    fs = geom["flags"].shape
+    for d in range(3):
+        G.flo[d] = int(geom["flo"][d])
+        G.fhi[d] = int(geom["flo"][d]) + fs[2 - d] - 1
+    G.vfrac = array4_of(geom["vfrac"][None], g7)
+    for d, n in enumerate("xyz"):
+        G.ap[d] = array4_of(geom["ap" + n][None], g7)
+        G.fc[d] = array4_of(geom["fc" + n], g7)
+    per = (C.c_int32 * 3)(*[int(x) for x in is_periodic])
+    L.check(L.lib().camr_hydro_umdrv_eb(
+        C.byref(_box(lo, hi)), C.byref(array4_of(uin, g6)), C.byref(array4_of(dsdt, lo)),
+        C.byref(array4_of(q, g6)), C.byref(array4_of(qa, g6)), C.byref(L.Array4()), fl,
+        C.byref(G), dt, C.byref(params), per, _stream(stream)))
+
+
 def estdt_hydro(lo, hi, u, ulo, dx, stream=0):
     out = C.c_double()
     dxa = (C.c_double * 3)(*dx)

@@ -48,6 +48,12 @@ class OraParams(C.Structure):
     ]
 
 
+class OraEbGeom(C.Structure):
+    _fields_ = [("flags", C.POINTER(C.c_int32)), ("flo", C.c_int * 3), ("fhi", C.c_int * 3),
+                ("fjs", C.c_int64), ("fks", C.c_int64), ("vfrac", OraFab),
+                ("ap", OraFab * 3), ("fc", OraFab * 3)]
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the committed Makefile."""
     if force or not os.path.exists(_LIB):
@@ -102,6 +108,8 @@ def lib():
         ip32 = C.POINTER(C.c_int32)
         L.ora_level_set_eb.argtypes = [C.c_void_p, ip32, ip, ip]
         L.ora_level_set_eb.restype = C.c_int
+        L.ora_hydro_umdrv_eb.argtypes = [ip, ip] + [C.POINTER(OraFab)] * 6 + [
+            C.c_double, C.POINTER(OraParams), C.POINTER(OraEbGeom)]
         L.ora_level_box_kind.argtypes = [C.c_void_p, C.c_int]
         L.ora_level_box_kind.restype = C.c_int
         L.ora_level_set_box_eb.argtypes = [C.c_void_p, C.c_int] + [dp] * 7

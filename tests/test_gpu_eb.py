@@ -15,6 +15,8 @@ from camr_amd.hydro import Level
 
 pytestmark = pytest.mark.gpu
 
+torch = pytest.importorskip("torch")
+
 UNTILED = (1 << 20,) * 3
 
 
@@ -72,3 +74,93 @@ def test_eb_level_rejects_fused_engine_and_ppm():
     gl = Level(prob, engine=_lib.ENGINE_STAGED)
     with pytest.raises(_lib.CamrError):
         gl.set_eb(geom)
+
+
+def test_umdrv_eb_per_box_bitwise(oracle):
+    """camr_hydro_umdrv_eb through the C ABI against the oracle's
+    hydro_umdrv_eb on one cut-cell box of Sod past a sphere, with a state
+    that varies in all three directions (shock plus a smooth field)."""
+    import ctypes as C
+    from camr_amd import eb
+    from camr_amd.hydro import hydro_ctoprim, hydro_params, hydro_umdrv_eb
+    prob = workloads.sod_sphere(n=16, max_grid=16, interface=0.4)
+    geom = workloads.sod_sphere_geometry(prob)
+    b = 0
+    lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+    g6lo = tuple(x - 6 for x in lo)
+    g6hi = tuple(x + 6 for x in hi)
+    assert geom.box_kind(b) == eb.BOX_EB
+    # state on grow(bx, 6): Sod split in x + a smooth perturbation, covered cells zero
+    x, y, z = [(np.arange(g6lo[d], g6hi[d] + 1) + 0.5) * prob.dx[d] for d in range(3)]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    rho = np.where(X < 0.4, 1.0, 0.125) * (1.0 + 0.05 * np.sin(2 * np.pi * (X + 2 * Y + 3 * Z)))
+    p = np.where(X < 0.4, 1.0, 0.1) * (1.0 + 0.05 * np.cos(2 * np.pi * (2 * X - Y + Z)))
+    u = 0.1 * np.sin(2 * np.pi * (Y + Z))
+    v = 0.1 * np.cos(2 * np.pi * (X - Z))
+    w = 0.1 * np.sin(2 * np.pi * (X + Y))
+    S = np.zeros((10,) + rho.shape)
+    S[0], S[1], S[2], S[3] = rho, rho * u, rho * v, rho * w
+    S[5] = p / 0.4
+    S[4] = S[5] + 0.5 * rho * (u * u + v * v + w * w)
+    S[7] = rho
+    cov = geom.covered_mask(g6lo, g6hi)
+    S[:, cov] = 0.0
+    S = np.ascontiguousarray(S)
+    hp = hydro_params(prob)
+    op = _oracle_params(oracle, prob)
+    op.eb = 1
+    dt = 0.2 * prob.dx[0]
+    L3 = C.c_int * 3
+    # oracle
+    q = np.zeros((11,) + rho.shape)
+    qa = np.zeros((5,) + rho.shape)
+    err = C.c_int(0)
+    oracle.lib().ora_ctoprim_box(L3(*g6lo), L3(*g6hi), C.byref(oracle.fab(S, g6lo)),
+                                 C.byref(oracle.fab(q, g6lo)), C.byref(oracle.fab(qa, g6lo)),
+                                 C.byref(op), C.byref(err))
+    assert err.value == 0
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    dsdt = np.zeros((10, n[2], n[1], n[0]))
+    flx = []
+    for d in range(3):
+        s = [n[2], n[1], n[0]]
+        s[2 - d] += 1
+        flx.append(np.zeros([10] + s))
+    fl = (oracle.OraFab * 3)(*[oracle.fab(f, lo) for f in flx])
+    bg = geom.box_geom(b)
+    g7 = tuple(x - 7 for x in lo)
+    G = oracle.OraEbGeom()
+    flags = np.ascontiguousarray(geom.flags, dtype=np.int32)
+    G.flags = flags.ctypes.data_as(C.POINTER(C.c_int32))
+    for d in range(3):
+        G.flo[d], G.fhi[d] = int(geom.lo[d]), int(geom.hi[d])
+    G.fjs = flags.shape[2]
+    G.fks = flags.shape[2] * flags.shape[1]
+    keep = {k: np.ascontiguousarray(v[None] if v.ndim == 3 else v) for k, v in bg.items()}
+    G.vfrac = oracle.fab(keep["vfrac"], g7)
+    for d, nm in enumerate("xyz"):
+        G.ap[d] = oracle.fab(keep["ap" + nm], g7)
+        G.fc[d] = oracle.fab(keep["fc" + nm], g7)
+    srcq = np.zeros_like(q)
+    oracle.lib().ora_hydro_umdrv_eb(L3(*lo), L3(*hi), C.byref(oracle.fab(S, g6lo)),
+                                    C.byref(oracle.fab(dsdt, lo)), C.byref(oracle.fab(q, g6lo)),
+                                    C.byref(oracle.fab(qa, g6lo)), C.byref(oracle.fab(srcq, g6lo)),
+                                    fl, dt, C.byref(op), C.byref(G))
+    # gpu
+    dev = torch.device("cuda:0")
+    tS = torch.from_numpy(S).to(dev)
+    tq = torch.zeros(q.shape, dtype=torch.float64, device=dev)
+    tqa = torch.zeros(qa.shape, dtype=torch.float64, device=dev)
+    hydro_ctoprim(g6lo, g6hi, tS, g6lo, tq, tqa, hp)
+    tds = torch.zeros(dsdt.shape, dtype=torch.float64, device=dev)
+    tfl = [torch.zeros(f.shape, dtype=torch.float64, device=dev) for f in flx]
+    tg = {k: torch.from_numpy(v if v.ndim == 4 else v).to(dev) for k, v in bg.items()}
+    tg["flags"] = torch.from_numpy(flags).to(dev)
+    tg["flo"] = tuple(int(v) for v in geom.lo)
+    hydro_umdrv_eb(lo, hi, tS, tds, tq, tqa, tfl, tg, dt, hp, prob.is_periodic)
+    torch.cuda.synchronize()
+    assert np.array_equal(tq.cpu().numpy(), q)
+    for d in range(3):
+        assert np.array_equal(tfl[d].cpu().numpy(), flx[d]), d
+    assert np.array_equal(tds.cpu().numpy(), dsdt)
+    assert np.abs(dsdt).max() > 0.0
