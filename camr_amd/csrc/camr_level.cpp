@@ -139,12 +139,11 @@ class HydroLevel {
   private:
     int est_time_step(double *est);
     int advance_exact(double dt);
-    int advance_exact_passes(double dt);
     int advance_fused(double dt);
     int resolve();
     int clean_state(std::vector<Arr> &S, bool launch_detect);
     int expand_state();
-    int construct_hydro_source(double dt);
+    int construct_hydro_source(double dt, bool add = true);
     int check_status();
     int status_error(int st);
     void release();
@@ -200,6 +199,11 @@ class HydroLevel {
     bool dt_ready_ = false;  // dtmin_ holds the CFL estimate of S_new
     double last_dt_ = 0.0;
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
+    DevFab *tab_dsdt_ = nullptr;                  // staged engine: dS/dt of every box
+    int advance_staged(double dt);
+    int clean_all(const DevFab *tab, bool launch_detect);
+    template <typename K, typename... A>
+    void launch_all(K kern, int g, A... args);
     double *dt_dev_ = nullptr;  // the step's dt, read by the kernels
     double *ext_ = nullptr;     // exterior data (whole domain), when set
     Arr extv_{};
@@ -292,7 +296,8 @@ void HydroLevel::release() {
     for (auto p : dsdt_) fr(p);
     for (auto p : srchyd_) fr(p);
     bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear();
-    fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_);
+    fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
+    tab_dsdt_ = nullptr;
     tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
     fr(dt_dev_); fr(ext_); dt_dev_ = nullptr; ext_ = nullptr;
     release_graphs();
@@ -492,6 +497,12 @@ int HydroLevel::create() {
     }
     if (!fused_) {
         fstride_ = max_nodes;
+        {
+            std::vector<DevFab> td(nb);
+            for (int b = 0; b < nb; ++b) { td[b].a = dense(dsdt_[b], box_[b]); td[b].b = box_[b]; }
+            CAMR_HIP(hipMalloc(&tab_dsdt_, nb * sizeof(DevFab)));
+            CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+        }
         for (int d = 0; d < 3; ++d)
             if ((rc = alloc(&flx_[d], NVAR * max_nodes * kWorkers))) return rc;
         size_t sc = 0;
@@ -767,7 +778,7 @@ int HydroLevel::expand_state() {
 
 // construct_hydro_source (Hydro/CAMR_construct_hydro_source.cpp:9-267):
 // per box ctoprim, srctoprim, hydro_umdrv, then S_new += dt * dSdt.
-int HydroLevel::construct_hydro_source(double dt) {
+int HydroLevel::construct_hydro_source(double dt, bool add) {
     if (grav_src_values_) {
         for (size_t b = 0; b < box_.size(); ++b)
             launch_on(k_srchyd, grow(box_[b], CAMR_NGROW), stream_, box_[b],
@@ -855,9 +866,10 @@ int HydroLevel::construct_hydro_source(double dt) {
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
     }
-    for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_add_hydro, box_[b], cur_ws(b), snew_[b], dense(dsdt_[b], box_[b]), dt,
-                  hc_.small_dens, status_, ebf_);
+    if (add)
+        for (size_t b = 0; b < box_.size(); ++b)
+            launch_on(k_add_hydro, box_[b], cur_ws(b), snew_[b], dense(dsdt_[b], box_[b]), dt,
+                      hc_.small_dens, status_, ebf_);
     return CAMR_OK;
 }
 
@@ -891,25 +903,11 @@ int HydroLevel::advance(double dt) {
 // The reference sequence pass by pass (CAMR_advance.cpp:72-164), starting
 // after the swap; S_old holds the raw previous state.
 int HydroLevel::advance_exact(double dt) {
-    // the staged engine spreads the per-box passes over its worker streams;
-    // the boxes meet only where one reads another's cells (FillPatch,
-    // FillBoundary), which the joins below order
-    in_ws_ = nws_ > 0 && !fused_;
-    int rc = fork();
-    if (!rc) rc = advance_exact_passes(dt);
-    const int rj = join();
-    in_ws_ = false;
-    return rc ? rc : rj;
-}
-
-int HydroLevel::advance_exact_passes(double dt) {
+    if (!fused_) return advance_staged(dt);
     // clean_state(S_old) (:83)
     clean_state(sold_, true);
-    // expand_state(Sborder, prev_time, numGrow()) (:99): FillPatch reads
-    // every box's S_old
-    int rc = join();
-    if (rc) return rc;
-    rc = expand_state();
+    // expand_state(Sborder, prev_time, numGrow()) (:99)
+    int rc = expand_state();
     if (rc) return rc;
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
     for (size_t b = 0; b < box_.size(); ++b)
@@ -930,6 +928,69 @@ int HydroLevel::advance_exact_passes(double dt) {
     }
     // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
     clean_state(snew_, true);
+    return CAMR_OK;
+}
+
+template <typename K, typename... A>
+void HydroLevel::launch_all(K kern, int g, A... args) {
+    int64_t n = 1;
+    for (const Box &B : box_) n = std::max<int64_t>(n, grow(B, g).npts());
+    hipLaunchKernelGGL(kern, dim3(grid_for(n), (unsigned)box_.size()), dim3(kBlock), 0, stream_,
+                       args...);
+}
+
+// clean_state on the valid cells of every box of `tab` (CAMR.cpp:1253-1262,
+// ng = 0): three launches for the whole level.
+int HydroLevel::clean_all(const DevFab *tab, bool launch_detect) {
+    if (launch_detect) launch_all(kb_detect, 0, tab, hc_.small_dens, status_, ebf_);
+    hipLaunchKernelGGL(kb_floor_seq, dim3(1), dim3(1), 0, stream_, tab, (int)box_.size(), hc_,
+                       status_, ebf_);
+    launch_all(kb_clean_point, 0, tab, hc_);
+    return CAMR_OK;
+}
+
+// The staged engine's step, pass by pass as in CAMR_advance.cpp:72-164:
+// the pointwise passes run as one launch over all boxes on the level's
+// stream, the per-box hydro (hydro_umdrv / hydro_umdrv_eb) on the worker
+// streams between a fork and a join.
+int HydroLevel::advance_staged(double dt) {
+    const DevFab *tnew = a_is_new_ ? tab_a_ : tab_b_, *told = a_is_new_ ? tab_b_ : tab_a_;
+    // clean_state(S_old) (:83)
+    clean_all(told, true);
+    // expand_state (:99): FillPatch from S_old, clean_state(Sborder) on the
+    // valid cells, FillBoundary, computeTemp on the grown boxes
+    launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, told, (const int *)owner_, dom_,
+               extv_, ext_ ? 1 : 0, status_);
+    launch_all(kb_detect, 0, (const DevFab *)tab_sb_, hc_.small_dens, status_, ebf_);
+    hipLaunchKernelGGL(kb_floor_seq, dim3(1), dim3(1), 0, stream_, (const DevFab *)tab_sb_,
+                       (int)box_.size(), hc_, status_, ebf_);
+    launch_all(kb_normalize, 0, (const DevFab *)tab_sb_);
+    launch_all(kb_resync, ng_, (const DevFab *)tab_sb_, ng_, (const DevFab *)tab_sb_,
+               (const int *)owner_, dom_);
+    launch_all(kb_resetT, ng_, (const DevFab *)tab_sb_, ng_, hc_);
+    // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
+    launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, dt, desc_.const_grav,
+               desc_.add_grav_src);
+    // construct_hydro_source (:138) per box on the worker streams
+    scratch_.reset();
+    int rc = fork();
+    if (rc) return rc;
+    in_ws_ = true;
+    rc = construct_hydro_source(dt, false);
+    in_ws_ = false;
+    const int rj = join();
+    if (rc) return rc;
+    if (rj) return rj;
+    // Saxpy (:141), clean_state(S_new) (:146)
+    launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, dt, hc_.small_dens, status_, ebf_);
+    clean_all(tnew, false);
+    if (desc_.add_grav_src) {
+        // new-time source correction + clean_state (:150-154)
+        launch_all(kb_add_newsrc, 0, tnew, told, dt, desc_.const_grav, hc_.small_dens, status_, ebf_);
+        clean_all(tnew, false);
+    }
+    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
+    clean_all(tnew, true);
     return CAMR_OK;
 }
 

@@ -54,6 +54,21 @@ __global__ void k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g,
 __global__ void k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g);
 __global__ void k_fill_value(Range r, Arr a, int nc, double v);
 __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v);
+// the passes above over every box of a table in one launch (grid.y = box)
+__global__ void kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb);
+__global__ void kb_floor_seq(const DevFab *S, int nbox, HydroConst hc, int *flags, EbFlags eb);
+__global__ void kb_clean_point(const DevFab *S, HydroConst hc);
+__global__ void kb_normalize(const DevFab *S);
+__global__ void kb_resetT(const DevFab *S, int g, HydroConst hc);
+__global__ void kb_fill_raw(const DevFab *dst, int g, const DevFab *src, const int *owner,
+                            DomInfo D, Arr ext, int has_ext, int *status);
+__global__ void kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D);
+__global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, double dt,
+                               double g, int add_grav);
+__global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, double dt, double small_dens,
+                             int *flags, EbFlags eb);
+__global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, double dt, double g,
+                              double small_dens, int *flags, EbFlags eb);
 inline Box make_box1() { Box b; for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = 0; } return b; }
 
 template <typename K, typename... A>

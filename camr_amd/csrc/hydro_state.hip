@@ -98,19 +98,21 @@ k_estdt(Range r, Arr u, double dx, double dy, double dz, unsigned long long *out
     block_min_commit(v, out, status);
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_detect(Range r, Arr S, double small_dens, int *flags, EbFlags eb) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void pt_detect(Arr S, double small_dens, int *flags, EbFlags eb, int i, int j, int k) {
     if (eb.covered(i, j, k)) return;  // CAMR.cpp:987-988
     floor_detect(S(i, j, k, URHO), small_dens, flags);
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_detect(Range r, Arr S, double small_dens, int *flags, EbFlags eb) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_detect(S, small_dens, flags, eb, i, j, k);
+}
+
 // enforce_min_density (CAMR.cpp:941-1045), exact sequential order of the
 // reference CPU loop over one box; runs only if a floor was detected.
-__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags, EbFlags eb) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (!(*flags & kFloorFlag)) return;
+__device__ void floor_seq_box(Box bx, Arr S, HydroConst hc, EbFlags eb) {
     const double sd = hc.small_dens;
     for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
     for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
@@ -145,11 +147,14 @@ __global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags, EbFlags eb
     }
 }
 
+__global__ void k_floor_seq(Box bx, Arr S, HydroConst hc, int *flags, EbFlags eb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!(*flags & kFloorFlag)) return;
+    floor_seq_box(bx, S, hc, eb);
+}
+
 // normalize_species + computeTemp on valid cells (after the floor pass).
-__global__ void __launch_bounds__(kBlock)
-k_clean_point(Range r, Arr S, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void pt_clean_point(Arr S, HydroConst hc, int i, int j, int k) {
     double s[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
@@ -160,9 +165,13 @@ k_clean_point(Range r, Arr S, HydroConst hc) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_normalize(Range r, Arr S) {
+k_clean_point(Range r, Arr S, HydroConst hc) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_clean_point(S, hc, i, j, k);
+}
+
+__device__ __forceinline__ void pt_normalize(Arr S, int i, int j, int k) {
     double s[NVAR];
     s[URHO] = S(i, j, k, URHO);
 #pragma unroll
@@ -173,9 +182,13 @@ k_normalize(Range r, Arr S) {
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_resetT(Range r, Arr S, HydroConst hc) {
+k_normalize(Range r, Arr S) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_normalize(S, i, j, k);
+}
+
+__device__ __forceinline__ void pt_resetT(Arr S, HydroConst hc, int i, int j, int k) {
     double s[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) s[n] = S(i, j, k, n);
@@ -183,6 +196,13 @@ k_resetT(Range r, Arr S, HydroConst hc) {
     S(i, j, k, UEDEN) = s[UEDEN];
     S(i, j, k, UEINT) = s[UEINT];
     S(i, j, k, UTEMP) = s[UTEMP];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_resetT(Range r, Arr S, HydroConst hc) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_resetT(S, hc, i, j, k);
 }
 
 // enforce_consistent_e (CAMR.cpp:911-938) + computeTemp
@@ -205,11 +225,8 @@ k_init_post(Range r, Arr S, HydroConst hc) {
 
 // FillPatchSingleLevel of every cell of the grown box of `dst` from the
 // valid data of the level's boxes (`src` table).
-__global__ void __launch_bounds__(kBlock)
-k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D, Arr ext,
-           int has_ext, int *status) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void pt_fill_raw(Arr dst, const DevFab *src, const int *owner, DomInfo D, Arr ext,
+           int has_ext, int *status, int i, int j, int k) {
     int m[3];
     unsigned odd;
     map_cell(D, i, j, k, m, &odd);
@@ -224,12 +241,17 @@ k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D, Arr
     }
 }
 
-// FillBoundary: ghost cells of box b that are periodic / interior images of
-// valid cells get those cells' values (physical-boundary ghosts untouched).
 __global__ void __launch_bounds__(kBlock)
-k_resync(Range r, Box valid, Arr dst, const DevFab *src, const int *owner, DomInfo D) {
+k_fill_raw(Range r, Arr dst, const DevFab *src, const int *owner, DomInfo D, Arr ext,
+           int has_ext, int *status) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_fill_raw(dst, src, owner, D, ext, has_ext, status, i, j, k);
+}
+
+// FillBoundary: ghost cells of box b that are periodic / interior images of
+// valid cells get those cells' values (physical-boundary ghosts untouched).
+__device__ __forceinline__ void pt_resync(Box valid, Arr dst, const DevFab *src, const int *owner, DomInfo D, int i, int j, int k) {
     if (valid.contains(i, j, k)) return;
     int m[3];
     unsigned odd;
@@ -241,14 +263,18 @@ k_resync(Range r, Box valid, Arr dst, const DevFab *src, const int *owner, DomIn
     for (int n = 0; n < NVAR; ++n) dst(i, j, k, n) = s(m[0], m[1], m[2], n);
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_resync(Range r, Box valid, Arr dst, const DevFab *src, const int *owner, DomInfo D) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_resync(valid, dst, src, owner, D, i, j, k);
+}
+
 // ----------------------------------------------------- Saxpy with sources
 // S_new = Sborder (MultiFab::Copy, CAMR_advance.cpp:112), then the old
 // gravity source Saxpy (:115-118; GravitySrc.cpp:36-78).  Zero-valued
 // sources still change -0.0 into +0.0, so they are applied literally.
-__global__ void __launch_bounds__(kBlock)
-k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt, double g, int add_grav) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void pt_copy_oldsrc(Arr snew, Arr sb, Arr sold, double dt, double g, int add_grav, int i, int j, int k) {
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
         double v = sb(i, j, k, n);
@@ -262,25 +288,34 @@ k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt, double g, int add_
     }
 }
 
-// S_new += dt * dSdt (CAMR_advance.cpp:141) + floor detection for the
-// following clean_state.
 __global__ void __launch_bounds__(kBlock)
-k_add_hydro(Range r, Arr snew, Arr dsdt, double dt, double small_dens, int *flags,
-            EbFlags eb) {
+k_copy_oldsrc(Range r, Arr snew, Arr sb, Arr sold, double dt, double g, int add_grav) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_copy_oldsrc(snew, sb, sold, dt, g, add_grav, i, j, k);
+}
+
+// S_new += dt * dSdt (CAMR_advance.cpp:141) + floor detection for the
+// following clean_state.
+__device__ __forceinline__ void pt_add_hydro(Arr snew, Arr dsdt, double dt, double small_dens, int *flags,
+            EbFlags eb, int i, int j, int k) {
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) snew(i, j, k, n) += dt * dsdt(i, j, k, n);
     if (!eb.covered(i, j, k)) floor_detect(snew(i, j, k, URHO), small_dens, flags);
 }
 
-// New-time gravity correction source (GravitySrc.cpp:81-125) Saxpy
-// (CAMR_advance.cpp:150-152) + floor detection.
 __global__ void __launch_bounds__(kBlock)
-k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens, int *flags,
-             EbFlags eb) {
+k_add_hydro(Range r, Arr snew, Arr dsdt, double dt, double small_dens, int *flags,
+            EbFlags eb) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_add_hydro(snew, dsdt, dt, small_dens, flags, eb, i, j, k);
+}
+
+// New-time gravity correction source (GravitySrc.cpp:81-125) Saxpy
+// (CAMR_advance.cpp:150-152) + floor detection.
+__device__ __forceinline__ void pt_add_newsrc(Arr snew, Arr sold, double dt, double g, double small_dens, int *flags,
+             EbFlags eb, int i, int j, int k) {
     const double s_umz = 0.5 * g * (snew(i, j, k, URHO) - sold(i, j, k, URHO));
     const double s_ued = 0.5 * g * (snew(i, j, k, UMZ) - sold(i, j, k, UMZ));
 #pragma unroll
@@ -289,6 +324,14 @@ k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens
         snew(i, j, k, n) += dt * src;
     }
     if (!eb.covered(i, j, k)) floor_detect(snew(i, j, k, URHO), small_dens, flags);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g, double small_dens, int *flags,
+             EbFlags eb) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    pt_add_newsrc(snew, sold, dt, g, small_dens, flags, eb, i, j, k);
 }
 
 // sources_for_hydro = 0 + 1.0 * old source on valid cells
@@ -368,6 +411,86 @@ __global__ void k_step_dt(StepCtl *c, unsigned long long *dtmin, int *status, Dt
     c->nstep += 1;
     c->time += dt;
     *dtmin = 0x7fefffffffffffffULL;  // DBL_MAX for this step's estimate
+}
+
+
+// ---------------------------------------------- one launch over all boxes
+// The same passes over every box of a table in one launch: blockIdx.y is
+// the box, blockIdx.x walks the points of the box grown by g.
+__device__ __forceinline__ bool batch_cell(const DevFab *tab, int g, int &b, int &i, int &j,
+                                           int &k) {
+    b = blockIdx.y;
+    const Range r = range_of(grow(tab[b].b, g));
+    return range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb) {
+    int b, i, j, k;
+    if (batch_cell(S, 0, b, i, j, k)) pt_detect(S[b].a, small_dens, flags, eb, i, j, k);
+}
+
+// enforce_min_density box after box (each box independent of the others)
+__global__ void kb_floor_seq(const DevFab *S, int nbox, HydroConst hc, int *flags, EbFlags eb) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || blockIdx.y != 0) return;
+    if (!(*flags & kFloorFlag)) return;
+    for (int b = 0; b < nbox; ++b) floor_seq_box(S[b].b, S[b].a, hc, eb);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_clean_point(const DevFab *S, HydroConst hc) {
+    int b, i, j, k;
+    if (batch_cell(S, 0, b, i, j, k)) pt_clean_point(S[b].a, hc, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_normalize(const DevFab *S) {
+    int b, i, j, k;
+    if (batch_cell(S, 0, b, i, j, k)) pt_normalize(S[b].a, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_resetT(const DevFab *S, int g, HydroConst hc) {
+    int b, i, j, k;
+    if (batch_cell(S, g, b, i, j, k)) pt_resetT(S[b].a, hc, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_fill_raw(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D, Arr ext,
+            int has_ext, int *status) {
+    int b, i, j, k;
+    if (batch_cell(dst, g, b, i, j, k))
+        pt_fill_raw(dst[b].a, src, owner, D, ext, has_ext, status, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D) {
+    int b, i, j, k;
+    if (batch_cell(dst, g, b, i, j, k)) pt_resync(dst[b].b, dst[b].a, src, owner, D, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, double dt, double g,
+               int add_grav) {
+    int b, i, j, k;
+    if (batch_cell(snew, 0, b, i, j, k))
+        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, dt, g, add_grav, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_add_hydro(const DevFab *snew, const DevFab *dsdt, double dt, double small_dens, int *flags,
+             EbFlags eb) {
+    int b, i, j, k;
+    if (batch_cell(snew, 0, b, i, j, k))
+        pt_add_hydro(snew[b].a, dsdt[b].a, dt, small_dens, flags, eb, i, j, k);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_add_newsrc(const DevFab *snew, const DevFab *sold, double dt, double g, double small_dens,
+              int *flags, EbFlags eb) {
+    int b, i, j, k;
+    if (batch_cell(snew, 0, b, i, j, k))
+        pt_add_newsrc(snew[b].a, sold[b].a, dt, g, small_dens, flags, eb, i, j, k);
 }
 
 } // namespace camr
