@@ -243,7 +243,7 @@ def bench_eb(args) -> int:
     t_geom = time.perf_counter() - t0
     cells = 2 * n * n * n
     init = [workloads.sod_sphere_init(prob, b, geom) for b in range(len(prob.boxes))]
-    L = Level(prob, engine=_lib.ENGINE_STAGED)
+    L = Level(prob, engine=_lib.ENGINE_STAGED, graph=True)
     L.set_eb(geom)
     kinds = list(L.kinds)
     for b, U in enumerate(init):
@@ -287,7 +287,7 @@ def bench_eb(args) -> int:
            "data": "synthetic: Sod states around a sphere",
            "config": {"workload": f"{2 * n}x{n}x{n} on [0,2]x[0,1]^2, sphere r=0.15 at "
                                   "(0.6,0.5,0.5), Sod split at x=0.3, PLM, CFL 0.3, outflow x, "
-                                  "slip walls y/z, 32^3 boxes",
+                                  "slip walls y/z, 32^3 boxes, one captured graph per step",
                       "boxes": {"regular": kinds.count(0), "cut_cell": kinds.count(1),
                                 "covered": kinds.count(2)},
                       "geometry_setup_s": round(t_geom, 2)},

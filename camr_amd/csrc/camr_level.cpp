@@ -109,6 +109,13 @@ static int check_params(const camr_hydro_params &p) {
     return CAMR_OK;
 }
 
+// Write a double into device memory in stream order (no host staging).
+static void put_double(double *dst, double v, hipStream_t s) {
+    unsigned long long bits;
+    memcpy(&bits, &v, sizeof(bits));
+    launch_on(k_fill_u64, make_box1(), s, (unsigned long long *)dst, bits);
+}
+
 static bool covers(const camr_array4 &a, const Box &b, int nc) {
     return a.p != nullptr && box_of(a).contains(b) && a.ncomp >= nc;
 }
@@ -201,6 +208,12 @@ class HydroLevel {
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
     DevFab *tab_dsdt_ = nullptr;                  // staged engine: dS/dt of every box
     int advance_staged(double dt);
+    // staged engine: the step captured once per buffer parity, replayed
+    // with the step's dt in dt_dev_
+    hipGraph_t sgraph_[2] = {nullptr, nullptr};
+    hipGraphExec_t sgexec_[2] = {nullptr, nullptr};
+    bool use_staged_graph() const { return desc_.graph != 0 && !fused_ && !grav_src_values_; }
+    int replay_staged(double dt);
     int clean_all(const DevFab *tab, bool launch_detect);
     template <typename K, typename... A>
     void launch_all(K kern, int g, A... args);
@@ -268,7 +281,10 @@ int HydroLevel::set_exterior(const double *h) {
     int rc = resolve();
     if (rc) return rc;
     const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
-    if (!ext_) CAMR_HIP(hipMalloc(&ext_, NVAR * dom.npts() * sizeof(double)));
+    if (!ext_) {
+        release_graphs();  // captured steps were recorded without exterior data
+        CAMR_HIP(hipMalloc(&ext_, NVAR * dom.npts() * sizeof(double)));
+    }
     CAMR_HIP(hipMemcpy(ext_, h, NVAR * dom.npts() * sizeof(double), hipMemcpyHostToDevice));
     extv_ = dense(ext_, dom);
     return CAMR_OK;
@@ -541,6 +557,7 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     int rc = resolve();
     if (rc) return rc;
     CAMR_HIP(hipStreamSynchronize(stream_));
+    release_graphs();
     const size_t nf = fb.npts();
     CAMR_HIP(hipMalloc(&ebflags_, nf * sizeof(int32_t)));
     CAMR_HIP(hipMemcpy(ebflags_, flags, nf * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -636,6 +653,7 @@ int HydroLevel::set_box_eb(int b, const double *vf, const double *const ap[3],
         return CAMR_ERR_PARAM;
     }
     const Box G = grow(box_[b], 7);
+    release_graphs();  // captured steps hold the geometry pointers
     auto up = [&](const double *h, const Box &bb, int nc, Arr *out) -> int {
         double *p = nullptr;
         const size_t n = (size_t)nc * bb.npts();
@@ -826,7 +844,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
                 E.qbox = G;
                 E.hc = hc_;
                 for (int d = 0; d < 3; ++d) E.periodic[d] = desc_.is_periodic[d];
-                E.dt = dt;
+                E.dtp = dt_dev_;
                 E.uin = sbv_[b];
                 E.dsdt = ds;
                 E.q = q;
@@ -844,7 +862,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             A.bx = B;
             A.qbox = G;
             A.hc = hc_;
-            A.dt = dt;
+            A.dtp = dt_dev_;
             A.uin = sbv_[b];
             A.dsdt = ds;
             A.q = q;
@@ -890,6 +908,9 @@ int HydroLevel::advance(double dt) {
         rc = advance_fused(dt);
         pending_ = true;
         last_dt_ = dt;
+    } else if (use_staged_graph() && !prof_.enabled()) {
+        rc = replay_staged(dt);
+        dt_ready_ = false;
     } else {
         rc = advance_exact(dt);
         dt_ready_ = false;
@@ -969,8 +990,8 @@ int HydroLevel::advance_staged(double dt) {
                (const int *)owner_, dom_);
     launch_all(kb_resetT, ng_, (const DevFab *)tab_sb_, ng_, hc_);
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
-    launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, dt, desc_.const_grav,
-               desc_.add_grav_src);
+    launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, (const double *)dt_dev_,
+               desc_.const_grav, desc_.add_grav_src);
     // construct_hydro_source (:138) per box on the worker streams
     scratch_.reset();
     int rc = fork();
@@ -982,11 +1003,13 @@ int HydroLevel::advance_staged(double dt) {
     if (rc) return rc;
     if (rj) return rj;
     // Saxpy (:141), clean_state(S_new) (:146)
-    launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, dt, hc_.small_dens, status_, ebf_);
+    launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, (const double *)dt_dev_,
+               hc_.small_dens, status_, ebf_);
     clean_all(tnew, false);
     if (desc_.add_grav_src) {
         // new-time source correction + clean_state (:150-154)
-        launch_all(kb_add_newsrc, 0, tnew, told, dt, desc_.const_grav, hc_.small_dens, status_, ebf_);
+        launch_all(kb_add_newsrc, 0, tnew, told, (const double *)dt_dev_, desc_.const_grav,
+                   hc_.small_dens, status_, ebf_);
         clean_all(tnew, false);
     }
     // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
@@ -1035,7 +1058,40 @@ void HydroLevel::release_graphs() {
         if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
         gexec_[p] = nullptr;
         graph_[p] = nullptr;
+        if (sgexec_[p]) (void)hipGraphExecDestroy(sgexec_[p]);
+        if (sgraph_[p]) (void)hipGraphDestroy(sgraph_[p]);
+        sgexec_[p] = nullptr;
+        sgraph_[p] = nullptr;
     }
+}
+
+// One graph launch per step on the staged engine: the whole pass sequence
+// of advance_staged (every launch reads dt from dt_dev_, the tables and
+// scratch are fixed), captured on first use for the current buffer parity.
+int HydroLevel::replay_staged(double dt) {
+    const int p = parity();
+    if (!sgexec_[p]) {
+        // the legacy default stream cannot be captured: record on the
+        // level's capture stream, replay on its own stream
+        if (!gstream_) CAMR_HIP(hipStreamCreateWithFlags(&gstream_, hipStreamNonBlocking));
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        const hipStream_t own = stream_;
+        stream_ = gstream_;
+        hipError_t e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+        int rc = e == hipSuccess ? advance_staged(dt) : CAMR_ERR_HIP;
+        hipGraph_t g = nullptr;
+        if (e == hipSuccess) e = hipStreamEndCapture(stream_, &g);
+        stream_ = own;
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            if (!rc) set_error("capture of the level step failed: %s", hipGetErrorString(e));
+            return rc ? rc : CAMR_ERR_HIP;
+        }
+        sgraph_[p] = g;
+        CAMR_HIP(hipGraphInstantiate(&sgexec_[p], g, nullptr, nullptr, 0));
+    }
+    CAMR_HIP(hipGraphLaunch(sgexec_[p], stream_));
+    return CAMR_OK;
 }
 
 // Capture one level step per buffer parity: device dt, fill, the hydro
@@ -1621,7 +1677,6 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     A.bx = B;
     A.qbox = box_of(*q);
     A.hc = make_const(*p);
-    A.dt = dt;
     A.uin = view(*uin);
     A.dsdt = view(*dsdt);
     A.q = view(*q);
@@ -1642,7 +1697,9 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     hipStream_t s = (hipStream_t)stream;
     double *scr = nullptr;
     const size_t n = umdrv_staged_scratch_doubles(B);
-    CAMR_HIP(hipMallocAsync((void **)&scr, n * sizeof(double), s));
+    CAMR_HIP(hipMallocAsync((void **)&scr, (n + 1) * sizeof(double), s));
+    A.dtp = scr + n;
+    put_double(scr + n, dt, s);
     rc = umdrv_staged(A, scr, s);
     CAMR_HIP(hipFreeAsync(scr, s));
     if (rc) { set_error("kernel launch failed"); return rc; }
@@ -1687,7 +1744,6 @@ int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_a
     A.hc = make_const(*p);
     A.hc.eb = 1;
     for (int d = 0; d < 3; ++d) A.periodic[d] = is_periodic[d];
-    A.dt = dt;
     A.uin = view(*uin);
     A.dsdt = view(*dsdt);
     A.q = view(*q);
@@ -1705,7 +1761,9 @@ int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_a
     hipStream_t s = (hipStream_t)stream;
     double *scr = nullptr;
     const size_t n = umdrv_eb_scratch_doubles(B, A.qbox);
-    CAMR_HIP(hipMallocAsync((void **)&scr, n * sizeof(double), s));
+    CAMR_HIP(hipMallocAsync((void **)&scr, (n + 1) * sizeof(double), s));
+    A.dtp = scr + n;
+    put_double(scr + n, dt, s);
     rc = umdrv_eb(A, scr, s);
     CAMR_HIP(hipFreeAsync(scr, s));
     if (rc) { set_error("kernel launch failed"); return rc; }

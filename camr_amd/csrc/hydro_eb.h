@@ -42,7 +42,7 @@ struct UmdrvEbArgs {
     Box bx, qbox;
     HydroConst hc;
     int periodic[3];
-    double dt;
+    const double *dtp;  // device: the step's dt
     Arr uin, dsdt, q, qaux, srcq;
     int has_src;
     Arr flx[3];

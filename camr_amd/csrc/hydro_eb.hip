@@ -29,7 +29,7 @@ __device__ __forceinline__ int dv(int d, int e) { return d == e ? 1 : 0; }
 // (PLM_eb.H:18-54), hydro_plm_d_eb (PLM_eb.H:103-287); covered cells skip.
 __global__ void __launch_bounds__(kBlock)
 k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1,
-         Arr qp1, Arr qm2, Arr qp2, HydroConst hc, double dt) {
+         Arr qp1, Arr qm2, Arr qp2, HydroConst hc, const double *dtp) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (eb_type(g.f(i, j, k)) == kEbCovered) return;
@@ -79,7 +79,7 @@ k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1
             sl[s] = ord ? mc_slope(st, fl, ord) : 0.0;
         }
         double qp[NS], qm[NS];
-        plm_trace(d, c, sl, cc, dt / hc.dx[d], hc.small_dens, hc.small_pres, qp, qm);
+        plm_trace(d, c, sl, cc, *dtp / hc.dx[d], hc.small_dens, hc.small_pres, qp, qm);
         const bool open_lo = g.ap[d](i, j, k, 0) > 0.0;
         const bool open_hi = g.ap[d](i + ex, j + ey, k + ez, 0) > 0.0;
 #pragma unroll
@@ -141,12 +141,13 @@ k_riemann_eb(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, Arr ap, Hydr
 template <int D, int T>
 __global__ void __launch_bounds__(kBlock)
 k_transdo_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT,
-             EbFlags fl, Arr qmo, Arr qpo, double cdt, HydroConst hc) {
+             EbFlags fl, Arr qmo, Arr qpo, const double *dtp, HydroConst hc) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (fl.covered(i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int tx = (T == 0), ty = (T == 1), tz = (T == 2);
+    const double cdt = 1.0 / 3.0 * *dtp / hc.dx[T];  // Godunov_3D_eb.cpp:59-61
     const bool no_cov = apT(i + tx, j + ty, k + tz, 0) > 0.0 && apT(i, j, k, 0) > 0.0;
     const bool lo_open = apD(i, j, k, 0) > 0.0;
     const bool hi_open = apD(i + ex, j + ey, k + ez, 0) > 0.0;
@@ -180,8 +181,8 @@ k_transdo_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT
 template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_transdd_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa, Arr ga,
-             Arr fb, Arr gb, Arr apD, Arr apA, Arr apB, EbFlags fl, Arr qm, Arr qp, double ca,
-             double cb, double hdt, HydroConst hc) {
+             Arr fb, Arr gb, Arr apD, Arr apA, Arr apB, EbFlags fl, Arr qm, Arr qp,
+             const double *dtp, HydroConst hc) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (fl.covered(i, j, k)) return;
@@ -190,6 +191,8 @@ k_transdd_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa,
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int ax = (T0 == 0), ay = (T0 == 1), az = (T0 == 2);
     constexpr int bx = (T1 == 0), by = (T1 == 1), bz = (T1 == 2);
+    const double hdt = 0.5 * *dtp;  // Godunov_3D_eb.cpp:53-58
+    const double ca = hdt / hc.dx[T0], cb = hdt / hc.dx[T1];
     const bool no_cov = apA(i + ax, j + ay, k + az, 0) > 0.0 && apB(i + bx, j + by, k + bz, 0) > 0.0 &&
                         apA(i, j, k, 0) > 0.0 && apB(i, j, k, 0) > 0.0;
     const bool lo_open = apD(i, j, k, 0) > 0.0;
@@ -554,10 +557,7 @@ int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) 
         cur += (size_t)nc * b.npts();
         return a;
     };
-    const double dt = A.dt, hdt = 0.5 * dt;
     const double dxi[3] = {1.0 / hc.dx[0], 1.0 / hc.dx[1], 1.0 / hc.dx[2]};
-    const double cdt[3] = {1.0 / 3.0 * dt / hc.dx[0], 1.0 / 3.0 * dt / hc.dx[1], 1.0 / 3.0 * dt / hc.dx[2]};
-    const double hdtd[3] = {hdt / hc.dx[0], hdt / hc.dx[1], hdt / hc.dx[2]};
 
     Arr QM[3], QP[3], F1[3], G1[3], F2[3][3], G2[3][3], FT[3], QE[3];
     for (int d = 0; d < 3; ++d) {
@@ -590,7 +590,7 @@ int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) 
 
     g_name = "eb_trace_plm";
     launch(k_plm_eb, L.bxg2, st, A.q, A.qbox, A.qaux, g, QM[0], QP[0], QM[1], QP[1], QM[2],
-           QP[2], hc, dt);
+           QP[2], hc, A.dtp);
     g_name = "eb_riemann_1";
     launch(k_riemann_eb<0, 0>, L.f1[0], st, QM[0], QP[0], A.qaux, F1[0], G1[0], g.ap[0], hc);
     launch(k_riemann_eb<1, 0>, L.f1[1], st, QM[1], QP[1], A.qaux, F1[1], G1[1], g.ap[1], hc);
@@ -603,7 +603,7 @@ int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) 
         Arr qmT = dense(tq0, L.mbx[D]), qpT = dense(tq1, L.mbx[D]);                          \
         g_name = "eb_transverse_1";                                                          \
         launch(k_transdo_eb<D, T>, grow_dir(L.bxg2, T, -1), st, QM[D], QP[D], A.qaux, F1[T], \
-               G1[T], g.ap[D], g.ap[T], g.f, qmT, qpT, cdt[T], hc);                         \
+               G1[T], g.ap[D], g.ap[T], g.f, qmT, qpT, A.dtp, hc);                          \
         g_name = "eb_riemann_2";                                                             \
         launch(k_riemann_eb<D, 1>, grow_dir(nodes(L.bxg1, D), O, 1), st, qmT, qpT, A.qaux,  \
                F2[D][T], G2[D][T], g.ap[D], hc);                                             \
@@ -623,7 +623,7 @@ int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) 
         g_name = "eb_transverse_2";                                                           \
         launch(k_transdd_eb<D>, L.bxg1, st, QM[D], QP[D], A.qaux, A.srcq, A.has_src,          \
                F2[T0][T1], G2[T0][T1], F2[T1][T0], G2[T1][T0], g.ap[D], g.ap[T0], g.ap[T1],   \
-               g.f, qmF, qpF, hdtd[T0], hdtd[T1], hdt, hc);                                   \
+               g.f, qmF, qpF, A.dtp, hc);                                                     \
         g_name = "eb_riemann_3";                                                              \
         launch(k_riemann_eb<D, 2>, grow_dir(grow_dir(nodes(L.btf, D), T0, 1), T1, 1), st, qmF, \
                qpF, A.qaux, FT[D], QE[D], g.ap[D], hc);                                       \

@@ -12,7 +12,7 @@ struct UmdrvArgs {
     Box bx;        // valid region
     Box qbox;      // box of q / qaux / srcq (flatten's q.contains test)
     HydroConst hc;
-    double dt;
+    const double *dtp;  // device: the step's dt (read by the kernels)
     Arr uin, dsdt, q, qaux, srcq;
     int has_src;   // 0: identically +0 primitive sources
     Arr flx[3];

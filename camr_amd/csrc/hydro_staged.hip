@@ -51,7 +51,7 @@ k_flat(Range r, Arr q, Box qbox, Arr flat, int use_flattening) {
 // hydro_plm_d for all three directions (Godunov_3D.cpp:104-152).
 __global__ void __launch_bounds__(kBlock)
 k_plm(Range r, Arr q, Arr qaux, Arr srcq, int has_src, Arr flat, Arr qm0,
-      Arr qp0, Arr qm1, Arr qp1, Arr qm2, Arr qp2, HydroConst hc, double dt) {
+      Arr qp0, Arr qm1, Arr qp1, Arr qm2, Arr qp2, HydroConst hc, const double *dtp) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const double fl = flat(i, j, k, 0);
@@ -84,7 +84,7 @@ k_plm(Range r, Arr q, Arr qaux, Arr srcq, int has_src, Arr flat, Arr qm0,
             }
         }
         double qp[NS], qm[NS];
-        plm_trace(d, c, sl, cc, dt / hc.dx[d], hc.small_dens, hc.small_pres, qp, qm);
+        plm_trace(d, c, sl, cc, *dtp / hc.dx[d], hc.small_dens, hc.small_pres, qp, qm);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             QP[d](i, j, k, s) = qp[s];
@@ -97,11 +97,11 @@ k_plm(Range r, Arr q, Arr qaux, Arr srcq, int has_src, Arr flat, Arr qm0,
 // trace_ppm for one direction (PPM.cpp:11-411).
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_ppm(Range r, Arr q, Arr qaux, Arr flat, Arr qm, Arr qp, HydroConst hc, double dt) {
+k_ppm(Range r, Arr q, Arr qaux, Arr flat, Arr qm, Arr qp, HydroConst hc, const double *dtp) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
-    const double dtdx = dt / hc.dx[D];
+    const double dtdx = *dtp / hc.dx[D];
     const double cc = sqrt(qaux(i, j, k, QGAMC) * q(i, j, k, QPRES) / q(i, j, k, QRHO));
     const double un = q(i, j, k, QU + D);
     const double fl = flat(i, j, k, 0);
@@ -221,7 +221,7 @@ k_riemann(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, HydroConst hc) 
 template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_transdo(Range r, Arr qnm, Arr qnp, Arr qaux, Arr fa, Arr ga, Arr fb, Arr gb,
-          Arr qmA, Arr qpA, Arr qmB, Arr qpB, double cda, double cdb, HydroConst hc) {
+          Arr qmA, Arr qpA, Arr qmB, Arr qpB, const double *dtp, HydroConst hc) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     constexpr int T0 = (D == 0) ? 1 : 0;
@@ -239,7 +239,8 @@ k_transdo(Range r, Arr qnm, Arr qnp, Arr qaux, Arr fa, Arr ga, Arr fb, Arr gb,
         const int t = pair ? T1 : T0;
         const Arr &F = pair ? fb : fa;
         const Arr &G = pair ? gb : ga;
-        const double cdt = pair ? cdb : cda;
+        // cdtdx = dt/(3 dx_t), Godunov_3D.cpp:50-56
+        const double cdt = 1.0 / 3.0 * *dtp / hc.dx[t];
         const int tx = (t == 0), ty = (t == 1), tz = (t == 2);
         double fl[NF], fr[NF], gl[NG], gr[NG];
 #pragma unroll
@@ -268,10 +269,13 @@ k_transdo(Range r, Arr qnm, Arr qnp, Arr qaux, Arr fa, Arr ga, Arr fb, Arr gb,
 template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_transdd(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa,
-          Arr ga, Arr fb, Arr gb, Arr qm, Arr qp, double ca, double cb,
-          double hdt, HydroConst hc) {
+          Arr ga, Arr fb, Arr gb, Arr qm, Arr qp, const double *dtp, HydroConst hc) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    // hdt = dt/2 and hdt/dx of the two transverse directions (Godunov_3D.cpp:50-56)
+    const double hdt = 0.5 * *dtp;
+    const double ca = 0.5 * *dtp / hc.dx[(D == 0) ? 1 : 0];
+    const double cb = 0.5 * *dtp / hc.dx[(D == 2) ? 1 : 2];
     constexpr int T0 = (D == 0) ? 1 : 0;
     constexpr int T1 = (D == 2) ? 1 : 2;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
@@ -421,7 +425,6 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         cur += (size_t)nc * b.npts();
         return a;
     };
-    const double dt = A.dt;
 
     Arr flat = take(bxg2, 1);
     NAMED("flatten");
@@ -435,12 +438,12 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     if (hc.ppm_type == 0) {
         NAMED("trace_plm");
         launch(k_plm, bxg2, st, A.q, A.qaux, A.srcq, A.has_src, flat, qdm[0],
-               qdp[0], qdm[1], qdp[1], qdm[2], qdp[2], hc, dt);
+               qdp[0], qdm[1], qdp[1], qdm[2], qdp[2], hc, A.dtp);
     } else {
         NAMED("trace_ppm");
-        launch(k_ppm<0>, bxg2, st, A.q, A.qaux, flat, qdm[0], qdp[0], hc, dt);
-        launch(k_ppm<1>, bxg2, st, A.q, A.qaux, flat, qdm[1], qdp[1], hc, dt);
-        launch(k_ppm<2>, bxg2, st, A.q, A.qaux, flat, qdm[2], qdp[2], hc, dt);
+        launch(k_ppm<0>, bxg2, st, A.q, A.qaux, flat, qdm[0], qdp[0], hc, A.dtp);
+        launch(k_ppm<1>, bxg2, st, A.q, A.qaux, flat, qdm[1], qdp[1], hc, A.dtp);
+        launch(k_ppm<2>, bxg2, st, A.q, A.qaux, flat, qdm[2], qdp[2], hc, A.dtp);
     }
 
     // first fluxes
@@ -456,8 +459,6 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     launch(k_riemann<2, 0>, nodes(grow_dir(bxg2, 2, -1), 2), st, qdm[2], qdp[2], A.qaux, F1[2], G1[2], hc);
 
     // transverse corrections + Riemann solves of the 6 pairs
-    const double cdt[3] = {1.0 / 3.0 * dt / hc.dx[0], 1.0 / 3.0 * dt / hc.dx[1],
-                           1.0 / 3.0 * dt / hc.dx[2]};
     Arr F2[3][3], G2[3][3];
     for (int d = 0; d < 3; ++d)
         for (int t = 0; t < 3; ++t)
@@ -475,23 +476,21 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         Arr qmB = take(grow_hi(tb, d, 1), NS), qpB = take(tb, NS);
         const Box fb = nodes(bxg1, d);
         if (d == 0) {
-            launch(k_transdo<0>, tb, st, qdm[0], qdp[0], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, cdt[t0], cdt[t1], hc);
+            launch(k_transdo<0>, tb, st, qdm[0], qdp[0], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
             launch(k_riemann<0, 1>, fb, st, qmA, qpA, A.qaux, F2[0][t0], G2[0][t0], hc);
             launch(k_riemann<0, 1>, fb, st, qmB, qpB, A.qaux, F2[0][t1], G2[0][t1], hc);
         } else if (d == 1) {
-            launch(k_transdo<1>, tb, st, qdm[1], qdp[1], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, cdt[t0], cdt[t1], hc);
+            launch(k_transdo<1>, tb, st, qdm[1], qdp[1], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
             launch(k_riemann<1, 1>, fb, st, qmA, qpA, A.qaux, F2[1][t0], G2[1][t0], hc);
             launch(k_riemann<1, 1>, fb, st, qmB, qpB, A.qaux, F2[1][t1], G2[1][t1], hc);
         } else {
-            launch(k_transdo<2>, tb, st, qdm[2], qdp[2], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, cdt[t0], cdt[t1], hc);
+            launch(k_transdo<2>, tb, st, qdm[2], qdp[2], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
             launch(k_riemann<2, 1>, fb, st, qmA, qpA, A.qaux, F2[2][t0], G2[2][t0], hc);
             launch(k_riemann<2, 1>, fb, st, qmB, qpB, A.qaux, F2[2][t1], G2[2][t1], hc);
         }
     }
 
     // final states and fluxes
-    const double hdt = 0.5 * dt;
-    const double hdtd[3] = {0.5 * dt / hc.dx[0], 0.5 * dt / hc.dx[1], 0.5 * dt / hc.dx[2]};
     Arr G3[3];
     double *fq_base = cur;
     // G3 and divu stay live until the update: allocate before the reused region
@@ -507,13 +506,13 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         Arr qm = take(grow_hi(tb, d, 1), NS), qp = take(tb, NS);
         const Box fb = nodes(bx, d);
         if (d == 0) {
-            launch(k_transdd<0>, tb, st, qdm[0], qdp[0], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, hdtd[t0], hdtd[t1], hdt, hc);
+            launch(k_transdd<0>, tb, st, qdm[0], qdp[0], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
             launch(k_riemann<0, 2>, fb, st, qm, qp, A.qaux, A.flx[0], G3[0], hc);
         } else if (d == 1) {
-            launch(k_transdd<1>, tb, st, qdm[1], qdp[1], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, hdtd[t0], hdtd[t1], hdt, hc);
+            launch(k_transdd<1>, tb, st, qdm[1], qdp[1], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
             launch(k_riemann<1, 2>, fb, st, qm, qp, A.qaux, A.flx[1], G3[1], hc);
         } else {
-            launch(k_transdd<2>, tb, st, qdm[2], qdp[2], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, hdtd[t0], hdtd[t1], hdt, hc);
+            launch(k_transdd<2>, tb, st, qdm[2], qdp[2], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
             launch(k_riemann<2, 2>, fb, st, qm, qp, A.qaux, A.flx[2], G3[2], hc);
         }
     }

@@ -63,11 +63,11 @@ __global__ void kb_resetT(const DevFab *S, int g, HydroConst hc);
 __global__ void kb_fill_raw(const DevFab *dst, int g, const DevFab *src, const int *owner,
                             DomInfo D, Arr ext, int has_ext, int *status);
 __global__ void kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D);
-__global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, double dt,
-                               double g, int add_grav);
-__global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, double dt, double small_dens,
-                             int *flags, EbFlags eb);
-__global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, double dt, double g,
+__global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold,
+                               const double *dtp, double g, int add_grav);
+__global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp,
+                             double small_dens, int *flags, EbFlags eb);
+__global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 inline Box make_box1() { Box b; for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = 0; } return b; }
 

@@ -470,27 +470,27 @@ kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo
 }
 
 __global__ void __launch_bounds__(kBlock)
-kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, double dt, double g,
-               int add_grav) {
+kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, const double *dtp,
+               double g, int add_grav) {
     int b, i, j, k;
     if (batch_cell(snew, 0, b, i, j, k))
-        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, dt, g, add_grav, i, j, k);
+        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, *dtp, g, add_grav, i, j, k);
 }
 
 __global__ void __launch_bounds__(kBlock)
-kb_add_hydro(const DevFab *snew, const DevFab *dsdt, double dt, double small_dens, int *flags,
-             EbFlags eb) {
+kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double small_dens,
+             int *flags, EbFlags eb) {
     int b, i, j, k;
     if (batch_cell(snew, 0, b, i, j, k))
-        pt_add_hydro(snew[b].a, dsdt[b].a, dt, small_dens, flags, eb, i, j, k);
+        pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
 __global__ void __launch_bounds__(kBlock)
-kb_add_newsrc(const DevFab *snew, const DevFab *sold, double dt, double g, double small_dens,
-              int *flags, EbFlags eb) {
+kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
+              double small_dens, int *flags, EbFlags eb) {
     int b, i, j, k;
     if (batch_cell(snew, 0, b, i, j, k))
-        pt_add_newsrc(snew[b].a, sold[b].a, dt, g, small_dens, flags, eb, i, j, k);
+        pt_add_newsrc(snew[b].a, sold[b].a, *dtp, g, small_dens, flags, eb, i, j, k);
 }
 
 } // namespace camr

@@ -64,6 +64,33 @@ def test_eb_level_matches_oracle(oracle, n, max_grid, nsteps):
     assert moved > 1e-3
 
 
+def test_eb_level_graph_replays_match_oracle(oracle):
+    """The EB level step captured in a graph (one per buffer parity) and
+    replayed step after step, with host syncs and dt computations between
+    the replays: every step's state equals the oracle's."""
+    prob = workloads.sod_sphere(n=32, max_grid=16, interface=0.4)
+    geom = workloads.sod_sphere_geometry(prob)
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    ol.set_eb(geom)
+    gl = Level(prob, engine=_lib.ENGINE_STAGED, graph=True)
+    gl.set_eb(geom)
+    for b in range(len(prob.boxes)):
+        U = workloads.sod_sphere_init(prob, b, geom)
+        ol.set_box(b, U)
+        gl.set_box(b, U)
+    ol.init_post()
+    gl.init_post()
+    odt = gdt = None
+    for s in range(5):
+        odt = ol.initial_dt() if s == 0 else ol.new_dt(odt)
+        gdt = gl.initial_dt() if s == 0 else gl.new_dt(gdt)
+        assert odt == gdt
+        assert ol.advance(odt) == 0
+        gl.advance(gdt)
+        for b in range(len(prob.boxes)):
+            assert np.array_equal(gl.get_box(b), ol.get_box(b)), (s, b)
+
+
 def test_eb_level_rejects_fused_engine_and_ppm():
     prob = workloads.sod_sphere(n=16, max_grid=16)
     geom = workloads.sod_sphere_geometry(prob)
