@@ -85,7 +85,7 @@ EXPORTS = [
     "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
-    "camr_level_box_kind", "camr_level_set_box_eb",
+    "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
 ]
 
 _lib = None
@@ -149,6 +149,7 @@ def lib():
         L.camr_level_adjoint_backward.argtypes = [vp]
         i32p = C.POINTER(C.c_int32)
         L.camr_level_set_eb.argtypes = [vp, i32p, i32p, i32p]
+        L.camr_level_exterior_from_coarse.argtypes = [vp, vp, C.c_int32]
         L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,
                                           C.POINTER(EbGeom), C.c_double,
                                           C.POINTER(HydroParams), i32p, vp]

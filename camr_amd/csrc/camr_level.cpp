@@ -262,6 +262,7 @@ class HydroLevel {
         return desc_.graph != 0 && fused_ && batched_ && !grav_src_values_;
     }
     int set_exterior(const double *h);
+    int exterior_from(HydroLevel &coarse, int ratio);
     int set_tuning(const Tuning &t);
     int get_tuning(Tuning *t) const;
     bool fused() const { return fused_; }
@@ -287,6 +288,37 @@ int HydroLevel::set_exterior(const double *h) {
     }
     CAMR_HIP(hipMemcpy(ext_, h, NVAR * dom.npts() * sizeof(double), hipMemcpyHostToDevice));
     extv_ = dense(ext_, dom);
+    return CAMR_OK;
+}
+
+// Exterior data from the coarser level: FillPatch's coarse-fine
+// interpolation (k_ext_from_coarse) at every cell of this level's domain,
+// from the coarse level's current S_new.
+int HydroLevel::exterior_from(HydroLevel &C, int ratio) {
+    if (ratio < 2) { set_error("refinement ratio must be at least 2"); return CAMR_ERR_PARAM; }
+    for (int d = 0; d < 3; ++d)
+        if (desc_.hp.domlo[d] != C.desc_.hp.domlo[d] * ratio ||
+            desc_.hp.domhi[d] + 1 != (C.desc_.hp.domhi[d] + 1) * ratio) {
+            set_error("the fine domain is not the coarse domain refined by %d", ratio);
+            return CAMR_ERR_SHAPE;
+        }
+    int rc = C.resolve();
+    if (rc) return rc;
+    CAMR_HIP(hipStreamSynchronize(C.stream_));
+    rc = resolve();
+    if (rc) return rc;
+    const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
+    if (!ext_) {
+        release_graphs();  // captured steps were recorded without exterior data
+        CAMR_HIP(hipMalloc(&ext_, NVAR * dom.npts() * sizeof(double)));
+    }
+    extv_ = dense(ext_, dom);
+    const DevFab *ct = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
+    launch_on(k_ext_from_coarse, dom, stream_, extv_, ct, (const int *)C.owner_, C.dom_, ratio,
+              dom.lo[0], dom.lo[1], dom.lo[2]);
+    CAMR_HIP(hipGetLastError());
+    // the coarse level may step next: finish reading its state first
+    CAMR_HIP(hipStreamSynchronize(stream_));
     return CAMR_OK;
 }
 
@@ -1847,6 +1879,10 @@ int camr_level_adjoint_backward(camr_level *L) {
 }
 int camr_level_set_exterior(camr_level *L, const double *host) {
     return L ? L->impl.set_exterior(host) : CAMR_ERR_PARAM;
+}
+int camr_level_exterior_from_coarse(camr_level *fine, camr_level *coarse, int32_t ratio) {
+    if (!fine || !coarse || fine == coarse) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
+    return fine->impl.exterior_from(coarse->impl, ratio);
 }
 int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
                       const int32_t fhi[3]) {

@@ -493,5 +493,78 @@ kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double 
         pt_add_newsrc(snew[b].a, sold[b].a, *dtp, g, small_dens, flags, eb, i, j, k);
 }
 
+
+// ------------------------------------------------ coarse-to-fine ghost data
+// FillPatch's interpolation from the coarser level (AmrLevel::FillPatch,
+// CAMR.cpp:1247; interpolater cell_cons_interp, Source/Utils/
+// CAMR_setup.cpp:141-144): AMReX's CellConservativeLinear with linear
+// limiting, restated -- per direction the monotonized-central slope of the
+// coarse data, then one factor alpha in [0,1] for the slope vector so that
+// every child lies within the min / max of the 27 coarse neighbours.  One
+// thread per fine cell of the domain, all NVAR components, into the fine
+// level's exterior array.
+__global__ void __launch_bounds__(kBlock)
+k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomInfo CD,
+                  int ratio, int flo0, int flo1, int flo2) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int f[3] = {i, j, k}, flo[3] = {flo0, flo1, flo2};
+    const double half = 0.5 * (ratio - 1);
+    int cc[3];
+    double off[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        cc[d] = (f[d] - flo[d]) / ratio + CD.domlo[d];
+        const int sub = (f[d] - flo[d]) - (cc[d] - CD.domlo[d]) * ratio;
+        off[d] = (sub - half) / ratio;
+    }
+    const double edge = half / ratio;
+    // the 27 coarse neighbours: source cell, owner and sign mask
+    int mm[27][3], ob[27];
+    unsigned odd[27];
+    for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+            for (int d = -1; d <= 1; ++d) {
+                const int t = (a + 1) * 9 + (b + 1) * 3 + (d + 1);
+                map_cell(CD, cc[0] + d, cc[1] + b, cc[2] + a, mm[t], &odd[t]);
+                ob[t] = owner_of(CD, cowner, mm[t]);
+            }
+#pragma unroll 1
+    for (int n = 0; n < NVAR; ++n) {
+        double u[27];
+        for (int t = 0; t < 27; ++t) {
+            const double v = ob[t] >= 0 ? ctab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n) : 0.0;
+            u[t] = ((odd[t] >> n) & 1u) ? -v : v;
+        }
+        const double u0 = u[13];
+        const double um[3] = {u[12], u[10], u[4]};
+        const double up[3] = {u[14], u[16], u[22]};
+        double s[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double cen = 0.5 * (up[d] - um[d]);
+            const double forw = 2.0 * (up[d] - u0);
+            const double back = 2.0 * (u0 - um[d]);
+            const double slp = (forw * back >= 0.0) ? vmin(fabs(forw), fabs(back)) : 0.0;
+            s[d] = copysign(1.0, cen) * vmin(slp, fabs(cen));
+        }
+        double cmn = u0, cmx = u0;
+        for (int t = 0; t < 27; ++t) {
+            cmn = vmin(cmn, u[t]);
+            cmx = vmax(cmx, u[t]);
+        }
+        const double dmn = cmn - u0, dmx = cmx - u0;
+        double alpha = 1.0;
+        for (int cz = -1; cz <= 1; cz += 2)
+            for (int cy = -1; cy <= 1; cy += 2)
+                for (int cx = -1; cx <= 1; cx += 2) {
+                    const double dd = cx * edge * s[0] + cy * edge * s[1] + cz * edge * s[2];
+                    if (dd > dmx) alpha = vmin(alpha, dmx / dd);
+                    if (dd < dmn) alpha = vmin(alpha, dmn / dd);
+                }
+        ext(i, j, k, n) = u0 + off[0] * (alpha * s[0]) + off[1] * (alpha * s[1]) + off[2] * (alpha * s[2]);
+    }
+}
+
 } // namespace camr
 

@@ -108,6 +108,12 @@ class Level:
             self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
         return a
 
+    def exterior_from_coarse(self, coarse: "Level", ratio: int):
+        """Ghost data outside the level's boxes from the coarser level
+        `coarse` (FillPatch's conservative linear interpolation), computed on
+        the device from its current state."""
+        L.check(L.lib().camr_level_exterior_from_coarse(self._h, coarse._h, int(ratio)))
+
     def set_eb(self, geom):
         """Embedded-boundary build of the level with the cut-cell data of
         `geom` (``camr_amd.eb.Geometry``).  The level must have been created

@@ -220,6 +220,15 @@ double camr_level_time(const camr_level *L);
  * Without it such ghost cells are an error (CAMR_ERR_SHAPE). */
 int camr_level_set_exterior(camr_level *L, const double *host);
 
+/* The same exterior data computed on the device from the coarser level's
+ * current state: FillPatch's coarse-to-fine interpolation (AmrLevel::
+ * FillPatch, CAMR.cpp:1247, with the State_Type interpolater cell_cons_interp,
+ * Source/Utils/CAMR_setup.cpp:141-144: conservative linear, MC-limited,
+ * linearly limited) at every cell of the fine domain.  The fine domain must
+ * be the coarse domain refined by `ratio`.  Single time level: the coarse
+ * data of the moment (no time interpolation between coarse old and new). */
+int camr_level_exterior_from_coarse(camr_level *fine, camr_level *coarse, int32_t ratio);
+
 /* The embedded-boundary build of a level (AMREX_USE_EB: CAMR.cpp:303-329,
  * 419-421; CAMR_construct_hydro_source.cpp:77-204): ghost width 6, covered
  * cells skipped by clean_state and the dt estimate, and per box the path of

@@ -2785,3 +2785,95 @@ void ora_level_set_box_eb(ora_level *L, int b, const double *vfrac, const double
         memcpy(G->fc[d].p, fc[d], sizeof(double) * (size_t)(2 * G->fc[d].ns));
     }
 }
+
+/* ======================================================================== */
+/* Coarse-to-fine ghost data: FillPatch's interpolation from the coarser     */
+/* level (AmrLevel::FillPatch, CAMR.cpp:1247, with the State_Type            */
+/* interpolater cell_cons_interp, Source/Utils/CAMR_setup.cpp:141-144)       */
+/* ======================================================================== */
+/* Value of component n of the coarse level at cell c (any cell: ghost cells
+ * map through the domain BCs as FillPatch does). */
+static double level_value(const ora_level *L, const int c[3], int n) {
+    int m[3];
+    double s;
+    map_cell(&L->P, c, n, m, &s);
+    const int ob = L->owner[dom_index(L, m)];
+    const double v = ob >= 0 ? FA(&L->Snew[ob], m[0], m[1], m[2], n) : 0.0;
+    return (s < 0.0) ? -v : v;
+}
+
+/* AMReX's CellConservativeLinear (linear limiting), restated: per
+ * direction the monotonized-central slope of the coarse data, then one
+ * factor alpha in [0,1] for the slope vector so that every child value lies
+ * within the min / max of the 27 coarse neighbours.  The children of a
+ * coarse cell average back to it (the offsets are symmetric). */
+static double cons_interp(const ora_level *C, const int cc[3], int n, const double off[3],
+                          double edge) {
+    double u[3][3][3];
+    for (int a = -1; a <= 1; ++a)
+    for (int b = -1; b <= 1; ++b)
+    for (int d = -1; d <= 1; ++d) {
+        const int c[3] = {cc[0] + d, cc[1] + b, cc[2] + a};
+        u[a + 1][b + 1][d + 1] = level_value(C, c, n);
+    }
+    const double u0 = u[1][1][1];
+    const double um[3] = {u[1][1][0], u[1][0][1], u[0][1][1]};
+    const double up[3] = {u[1][1][2], u[1][2][1], u[2][1][1]};
+    double s[3];
+    for (int d = 0; d < 3; ++d) {
+        const double cen = 0.5 * (up[d] - um[d]);
+        const double forw = 2.0 * (up[d] - u0);
+        const double back = 2.0 * (u0 - um[d]);
+        const double slp = (forw * back >= 0.0) ? smin(fabs(forw), fabs(back)) : 0.0;
+        s[d] = copysign(1.0, cen) * smin(slp, fabs(cen));
+    }
+    double cmn = u0, cmx = u0;
+    for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+    for (int d = 0; d < 3; ++d) {
+        cmn = smin(cmn, u[a][b][d]);
+        cmx = smax(cmx, u[a][b][d]);
+    }
+    const double dmn = cmn - u0, dmx = cmx - u0;
+    double alpha = 1.0;
+    for (int cz = -1; cz <= 1; cz += 2)
+    for (int cy = -1; cy <= 1; cy += 2)
+    for (int cx = -1; cx <= 1; cx += 2) {
+        /* the extreme children: the outermost fine cells, at +-edge */
+        const double dd = cx * edge * s[0] + cy * edge * s[1] + cz * edge * s[2];
+        if (dd > dmx) alpha = smin(alpha, dmx / dd);
+        if (dd < dmn) alpha = smin(alpha, dmn / dd);
+    }
+    return u0 + off[0] * (alpha * s[0]) + off[1] * (alpha * s[1]) + off[2] * (alpha * s[2]);
+}
+
+/* The exterior data of fine level F (every cell of its domain, NVAR
+ * components) interpolated from coarse level C with refinement `ratio`. */
+int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) {
+    for (int d = 0; d < 3; ++d)
+        if (F->P.domlo[d] != C->P.domlo[d] * ratio ||
+            F->P.domhi[d] + 1 != (C->P.domhi[d] + 1) * ratio) return 1;
+    const int64_t nc = F->dn[0] * F->dn[1] * F->dn[2];
+    if (!F->ext) F->ext = (double *)malloc(sizeof(double) * (size_t)(nc * O_NVAR));
+    const double half = 0.5 * (ratio - 1);
+    for (int n = 0; n < O_NVAR; ++n)
+    for (int k = F->P.domlo[2]; k <= F->P.domhi[2]; ++k)
+    for (int j = F->P.domlo[1]; j <= F->P.domhi[1]; ++j)
+    for (int i = F->P.domlo[0]; i <= F->P.domhi[0]; ++i) {
+        const int f[3] = {i, j, k};
+        int cc[3];
+        double off[3];
+        for (int d = 0; d < 3; ++d) {
+            cc[d] = (f[d] - F->P.domlo[d]) / ratio + C->P.domlo[d];
+            const int sub = (f[d] - F->P.domlo[d]) - (cc[d] - C->P.domlo[d]) * ratio;
+            off[d] = (sub - half) / ratio; /* fine-centre offset in coarse cells */
+        }
+        F->ext[dom_index(F, f) + (int64_t)n * nc] = cons_interp(C, cc, n, off, half / ratio);
+    }
+    return 0;
+}
+
+void ora_level_get_exterior(const ora_level *L, double *out) {
+    if (L->ext)
+        memcpy(out, L->ext, sizeof(double) * (size_t)(L->dn[0] * L->dn[1] * L->dn[2] * O_NVAR));
+}

@@ -145,6 +145,11 @@ void   ora_level_set_box_eb(ora_level *L, int b, const double *vfrac,
                             const double *apx, const double *apy,
                             const double *apz, const double *fcx,
                             const double *fcy, const double *fcz);
+/* Exterior data of fine level F from coarse level C (refinement `ratio`):
+ * FillPatch's conservative linear interpolation (cell_cons_interp) at every
+ * cell of F's domain.  Returns 1 if the domains do not nest. */
+int    ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio);
+void   ora_level_get_exterior(const ora_level *L, double *out);
 /* Sum of each component over valid cells (for conservation checks). */
 void   ora_level_sums(const ora_level *L, double sums[O_NVAR]);
 

@@ -113,6 +113,9 @@ def lib():
         L.ora_level_box_kind.argtypes = [C.c_void_p, C.c_int]
         L.ora_level_box_kind.restype = C.c_int
         L.ora_level_set_box_eb.argtypes = [C.c_void_p, C.c_int] + [dp] * 7
+        L.ora_level_exterior_from_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.ora_level_exterior_from_coarse.restype = C.c_int
+        L.ora_level_get_exterior.argtypes = [C.c_void_p, dp]
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_get_threads.restype = C.c_int
         _lib = L
@@ -223,6 +226,17 @@ class Level:
                         for n in ("vfrac", "apx", "apy", "apz", "fcx", "fcy", "fcz")]
                 lib().ora_level_set_box_eb(self._h, b, *[_dp(a) for a in arrs])
         self._geom_keep = geom
+
+    def exterior_from_coarse(self, coarse, ratio):
+        """Exterior data interpolated from level `coarse` (FillPatch's
+        coarse-fine interpolation)."""
+        assert lib().ora_level_exterior_from_coarse(self._h, coarse._h, int(ratio)) == 0
+
+    def get_exterior(self):
+        n = [self.p.domhi[d] - self.p.domlo[d] + 1 for d in range(3)]
+        a = np.empty((NVAR, n[2], n[1], n[0]))
+        lib().ora_level_get_exterior(self._h, _dp(a))
+        return a
 
     def set_exterior(self, data):
         n = [self.p.domhi[d] - self.p.domlo[d] + 1 for d in range(3)]

@@ -1,0 +1,93 @@
+"""Coarse-to-fine ghost data: FillPatch's interpolation from the coarser level
+(AmrLevel::FillPatch, CAMR.cpp:1247; interpolater cell_cons_interp,
+Source/Utils/CAMR_setup.cpp:141-144).  AMReX is absent, so the oracle
+restates CellConservativeLinear with linear limiting; what the mathematics
+fixes pins it:
+
+* conservation: the children of every coarse cell average to it;
+* linear data is reproduced (the limiters do not act on it);
+* no new extrema: every child lies within its coarse 27-neighbourhood;
+* the same for refinement ratio 4.
+"""
+import numpy as np
+import pytest
+
+from camr_amd import problems
+
+NV = 10
+
+
+def _levels(O, nc, ratio, periodic=True, field=None):
+    bc = 0 if periodic else 2
+    per = (1, 1, 1) if periodic else (0, 0, 0)
+    pc = O.params(domlo=(0, 0, 0), domhi=(nc - 1,) * 3, dx=(1.0 / nc,) * 3, bclo=(bc,) * 3,
+                  bchi=(bc,) * 3, is_periodic=per)
+    nf = nc * ratio
+    pf = O.params(domlo=(0, 0, 0), domhi=(nf - 1,) * 3, dx=(1.0 / nf,) * 3, bclo=(bc,) * 3,
+                  bchi=(bc,) * 3, is_periodic=per)
+    h = nc // 2
+    boxes = [(i, j, k, i + h - 1, j + h - 1, k + h - 1)
+             for k in (0, h) for j in (0, h) for i in (0, h)]
+    C = O.Level(pc, boxes)
+    U = field(nc)
+    for b, (i0, j0, k0, i1, j1, k1) in enumerate(boxes):
+        C.set_box(b, np.ascontiguousarray(U[:, k0:k1 + 1, j0:j1 + 1, i0:i1 + 1]))
+    F = O.Level(pf, [(0, 0, 0, 7, 7, 7)])
+    F.exterior_from_coarse(C, ratio)
+    return U, F.get_exterior()
+
+
+def _smooth(nc):
+    rng = np.random.default_rng(problems.SEED_FIELD)
+    x = (np.arange(nc) + 0.5) / nc
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    U = np.empty((NV, nc, nc, nc))
+    for n in range(NV):
+        a, b, c = rng.uniform(0, 2 * np.pi, 3)
+        U[n] = 1.0 + 0.3 * np.sin(2 * np.pi * X + a) * np.cos(2 * np.pi * Y + b) + \
+            0.2 * np.sin(4 * np.pi * Z + c) + (np.abs(X - 0.5) < 0.2) * 0.5  # a jump too
+    return U
+
+
+def _children(E, ratio):
+    nf = E.shape[1]
+    nc = nf // ratio
+    return E.reshape(NV, nc, ratio, nc, ratio, nc, ratio)
+
+
+@pytest.mark.parametrize("ratio", [2, 4])
+def test_children_average_to_the_coarse_cell(oracle, ratio):
+    U, E = _levels(oracle, 8, ratio, field=_smooth)
+    mean = _children(E, ratio).mean(axis=(2, 4, 6))
+    assert np.abs(mean - U).max() <= 1e-14 * np.abs(U).max()
+
+
+def test_no_new_extrema(oracle):
+    U, E = _levels(oracle, 8, 2, field=_smooth)
+    P = np.pad(U, ((0, 0), (1, 1), (1, 1), (1, 1)), mode="wrap")
+    lo = np.full_like(U, np.inf)
+    hi = np.full_like(U, -np.inf)
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                s = P[:, a:a + 8, b:b + 8, c:c + 8]
+                lo, hi = np.minimum(lo, s), np.maximum(hi, s)
+    ch = _children(E, 2)
+    eps = 1e-14 * np.abs(U).max()
+    assert np.all(ch >= lo[:, :, None, :, None, :, None] - eps)
+    assert np.all(ch <= hi[:, :, None, :, None, :, None] + eps)
+
+
+def test_linear_data_is_reproduced(oracle):
+    def linear(nc):
+        x = (np.arange(nc) + 0.5) / nc
+        Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+        return np.stack([1.0 + 0.5 * X - 0.25 * Y + 0.125 * Z] * NV)
+    U, E = _levels(oracle, 8, 2, periodic=False, field=linear)
+    nf = 16
+    xf = (np.arange(nf) + 0.5) / nf
+    Z, Y, X = np.meshgrid(xf, xf, xf, indexing="ij")
+    want = 1.0 + 0.5 * X - 0.25 * Y + 0.125 * Z
+    # coarse cells away from the domain faces (their neighbours are linear)
+    inner = (slice(None), slice(2, nf - 2), slice(2, nf - 2), slice(2, nf - 2))
+    assert np.abs(E[inner] - want[None][inner]).max() <= 1e-14
