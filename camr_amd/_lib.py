@@ -54,7 +54,7 @@ class HydroParams(C.Structure):
         ("dual_energy_eta2", C.c_double),
         ("bclo", C.c_int32 * 3), ("bchi", C.c_int32 * 3),
         ("domlo", C.c_int32 * 3), ("domhi", C.c_int32 * 3),
-        ("dx", C.c_double * 3),
+        ("dx", C.c_double * 3), ("do_mol", C.c_int32),
     ]
 
 

@@ -91,8 +91,12 @@ HydroConst make_const(const camr_hydro_params &p) {
 }
 
 static int check_params(const camr_hydro_params &p) {
-    if (p.ppm_type != 0 && p.ppm_type != 1) {
-        set_error("ppm_type must be 0 (PLM) or 1 (PPM)");  // Godunov_3D.cpp:174-176
+    if (p.do_mol != 0 && p.do_mol != 1) {
+        set_error("do_mol must be 0 (Godunov) or 1 (method of lines)");
+        return CAMR_ERR_PARAM;
+    }
+    if (!p.do_mol && p.ppm_type != 0 && p.ppm_type != 1) {
+        set_error("ppm_type must be 0 (PLM) or 1 (PPM)");  // CAMR.cpp:180-184
         return CAMR_ERR_PARAM;
     }
     if (p.plm_iorder != 1 && p.plm_iorder != 2 && p.plm_iorder != 4) {
@@ -208,6 +212,8 @@ class HydroLevel {
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
     DevFab *tab_dsdt_ = nullptr;                  // staged engine: dS/dt of every box
     int advance_staged(double dt);
+    void expand_all(const DevFab *src);
+    int hydro_on_workers(double dt);
     // staged engine: the step captured once per buffer parity, replayed
     // with the step's dt in dt_dev_
     hipGraph_t sgraph_[2] = {nullptr, nullptr};
@@ -394,6 +400,14 @@ int HydroLevel::create() {
         set_error("Invalid CFL factor; must be between zero and one.");  // CAMR.cpp:173-174
         return CAMR_ERR_PARAM;
     }
+    if (desc_.hp.do_mol && desc_.cfl > 0.3 && desc_.fixed_dt <= 0.0) {
+        set_error("Invalid CFL factor; must be <= 0.3 when using MOL hydro");  // CAMR.cpp:175-177
+        return CAMR_ERR_PARAM;
+    }
+    if (desc_.hp.do_mol && desc_.engine == 2) {
+        set_error("the method of lines runs on the staged engine");
+        return CAMR_ERR_PARAM;
+    }
     hc_ = make_const(desc_.hp);
     for (int d = 0; d < 3; ++d) {
         dom_.domlo[d] = desc_.hp.domlo[d];
@@ -529,7 +543,8 @@ int HydroLevel::create() {
     CAMR_HIP(hipMemset(status_, 0, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_status_, sizeof(int)));
     CAMR_HIP(hipHostMalloc(&h_dtmin_, sizeof(unsigned long long)));
-    fused_ = (desc_.engine != 1);
+    // the fused engine implements the Godunov integrator only
+    fused_ = (desc_.engine != 1) && !desc_.hp.do_mol;
     batched_ = desc_.batched != 0;
     if (fused_) {
         std::vector<Arr> va(nb), vb(nb);
@@ -569,6 +584,10 @@ int HydroLevel::create() {
 int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t fhi[3]) {
     if (!flags || !flo || !fhi) { set_error("null argument"); return CAMR_ERR_PARAM; }
     if (fused_) { set_error("embedded boundaries run on the staged engine"); return CAMR_ERR_PARAM; }
+    if (desc_.hp.do_mol) {
+        set_error("embedded boundaries with the method of lines (MOL_umeth_eb) are not built");
+        return CAMR_ERR_PARAM;
+    }
     if (ebflags_) { set_error("embedded boundary already set"); return CAMR_ERR_PARAM; }
     if (desc_.hp.ppm_type != 0 || desc_.hp.use_pslope != 0) {
         set_error("embedded boundaries need ppm_type = 0 and use_pslope = 0 (CAMR.cpp:186-190)");
@@ -868,7 +887,8 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             }
             Arr q = dense(wq, G), qa = dense(wqa, G), sq = dense(wsq, G);
             launch_on(k_ctoprim, G, bs, sbv_[b], q, qa, hc_, status_);
-            if (grav_src_values_)
+            // primitive sources feed the Godunov predictor only (:129-143)
+            if (grav_src_values_ && !desc_.hp.do_mol)
                 launch_on(k_srctoprim, G, bs, q, qa, dense(srchyd_[b], G), sq);
             if (kind == 1) {
                 UmdrvEbArgs E;
@@ -912,7 +932,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             A.vol = ds;
             A.has_vol = 0;
             A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
-            int rc = umdrv_staged(A, wscr, bs, &prof_);
+            int rc = desc_.hp.do_mol ? umdrv_mol(A, wscr, bs, &prof_) : umdrv_staged(A, wscr, bs, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
     }
@@ -1010,33 +1030,30 @@ int HydroLevel::advance_staged(double dt) {
     const DevFab *tnew = a_is_new_ ? tab_a_ : tab_b_, *told = a_is_new_ ? tab_b_ : tab_a_;
     // clean_state(S_old) (:83)
     clean_all(told, true);
-    // expand_state (:99): FillPatch from S_old, clean_state(Sborder) on the
-    // valid cells, FillBoundary, computeTemp on the grown boxes
-    launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, told, (const int *)owner_, dom_,
-               extv_, ext_ ? 1 : 0, status_);
-    launch_all(kb_detect, 0, (const DevFab *)tab_sb_, hc_.small_dens, status_, ebf_);
-    hipLaunchKernelGGL(kb_floor_seq, dim3(1), dim3(1), 0, stream_, (const DevFab *)tab_sb_,
-                       (int)box_.size(), hc_, status_, ebf_);
-    launch_all(kb_normalize, 0, (const DevFab *)tab_sb_);
-    launch_all(kb_resync, ng_, (const DevFab *)tab_sb_, ng_, (const DevFab *)tab_sb_,
-               (const int *)owner_, dom_);
-    launch_all(kb_resetT, ng_, (const DevFab *)tab_sb_, ng_, hc_);
+    // expand_state(Sborder, prev_time) (:99)
+    expand_all(told);
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
     launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, (const double *)dt_dev_,
                desc_.const_grav, desc_.add_grav_src);
-    // construct_hydro_source (:138) per box on the worker streams
+    // construct_hydro_source (:125 / :138) per box on the worker streams
     scratch_.reset();
-    int rc = fork();
+    int rc = hydro_on_workers(dt);
     if (rc) return rc;
-    in_ws_ = true;
-    rc = construct_hydro_source(dt, false);
-    in_ws_ = false;
-    const int rj = join();
-    if (rc) return rc;
-    if (rj) return rj;
-    // Saxpy (:141), clean_state(S_new) (:146)
+    // Saxpy (:128 / :141)
     launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, (const double *)dt_dev_,
                hc_.small_dens, status_, ebf_);
+    if (desc_.hp.do_mol) {
+        // second stage (:130-134): expand_state(Sborder, cur_time) from
+        // S_new, the hydro source of that state, then S_new = 0.5 Sborder
+        // + 0.5 S_old + 0.5 dt dSdt
+        expand_all(tnew);
+        rc = hydro_on_workers(dt);
+        if (rc) return rc;
+        launch_all(kb_mol_combine, 0, tnew, (const DevFab *)tab_sb_, told,
+                   (const DevFab *)tab_dsdt_, (const double *)dt_dev_, hc_.small_dens, status_,
+                   ebf_);
+    }
+    // clean_state(S_new) (:146)
     clean_all(tnew, false);
     if (desc_.add_grav_src) {
         // new-time source correction + clean_state (:150-154)
@@ -1047,6 +1064,33 @@ int HydroLevel::advance_staged(double dt) {
     // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
     clean_all(tnew, true);
     return CAMR_OK;
+}
+
+// expand_state (CAMR.cpp:1243-1250) from the states of `src`: FillPatch,
+// clean_state(Sborder) on the valid cells, FillBoundary, computeTemp on
+// the grown boxes.
+void HydroLevel::expand_all(const DevFab *src) {
+    launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_, dom_,
+               extv_, ext_ ? 1 : 0, status_);
+    launch_all(kb_detect, 0, (const DevFab *)tab_sb_, hc_.small_dens, status_, ebf_);
+    hipLaunchKernelGGL(kb_floor_seq, dim3(1), dim3(1), 0, stream_, (const DevFab *)tab_sb_,
+                       (int)box_.size(), hc_, status_, ebf_);
+    launch_all(kb_normalize, 0, (const DevFab *)tab_sb_);
+    launch_all(kb_resync, ng_, (const DevFab *)tab_sb_, ng_, (const DevFab *)tab_sb_,
+               (const int *)owner_, dom_);
+    launch_all(kb_resetT, ng_, (const DevFab *)tab_sb_, ng_, hc_);
+}
+
+// construct_hydro_source into dSdt of every box, spread over the worker
+// streams between a fork and a join.
+int HydroLevel::hydro_on_workers(double dt) {
+    int rc = fork();
+    if (rc) return rc;
+    in_ws_ = true;
+    rc = construct_hydro_source(dt, false);
+    in_ws_ = false;
+    const int rj = join();
+    return rc ? rc : rj;
 }
 
 // Same step with the pointwise passes fused (level_fused.hip); exact unless a
@@ -1728,11 +1772,11 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     A.vol_c = p->dx[0] * p->dx[1] * p->dx[2];
     hipStream_t s = (hipStream_t)stream;
     double *scr = nullptr;
-    const size_t n = umdrv_staged_scratch_doubles(B);
+    const size_t n = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
     CAMR_HIP(hipMallocAsync((void **)&scr, (n + 1) * sizeof(double), s));
     A.dtp = scr + n;
     put_double(scr + n, dt, s);
-    rc = umdrv_staged(A, scr, s);
+    rc = p->do_mol ? umdrv_mol(A, scr, s) : umdrv_staged(A, scr, s);
     CAMR_HIP(hipFreeAsync(scr, s));
     if (rc) { set_error("kernel launch failed"); return rc; }
     return CAMR_OK;
@@ -1751,6 +1795,10 @@ int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_a
     if (!(dt > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
     if (p->ppm_type != 0 || p->use_pslope != 0) {
         set_error("hydro_umdrv_eb needs ppm_type = 0 and use_pslope = 0 (CAMR.cpp:186-190)");
+        return CAMR_ERR_PARAM;
+    }
+    if (p->do_mol) {
+        set_error("hydro_umdrv_eb with the method of lines (MOL_umeth_eb) is not built");
         return CAMR_ERR_PARAM;
     }
     const Box B = make_box(bx->lo, bx->hi);

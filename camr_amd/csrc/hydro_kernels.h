@@ -29,6 +29,10 @@ HydroConst make_const(const camr_hydro_params &p);
 size_t umdrv_staged_scratch_doubles(const Box &bx);
 int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st,
                  Prof *prof = nullptr);
+// hydro_umdrv with do_mol (MOL_umeth): q / qaux must cover grow(bx, 3);
+// srcq, has_src and dtp are not read.
+size_t umdrv_mol_scratch_doubles(const Box &bx);
+int umdrv_mol(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof = nullptr);
 
 // ---- hydro_divu (Source/Hydro/Hydro_utils_K.H:12-169), node (i,j,k) -------
 template <class QA>

@@ -242,6 +242,86 @@ CAMR_HD void face_flux(int d, int idx, const HydroConst &hc, double *ql,
     }
 }
 
+// ---- method of lines (Source/Hydro/MOL/) ------------------------------------
+// limiter, MOL_hydro_K.H:10-20 (plm_theta = 2, van Leer's MC).
+CAMR_HD double mol_limiter(double dlft, double drgt) {
+    const double plm_theta = 2.0;
+    const double dcen = 0.5 * (dlft + drgt);
+    const double dsgn = copysign(1.0, dcen);
+    const double slop = plm_theta * vmin(fabs(dlft), fabs(drgt));
+    const double dlim = (dlft * drgt >= 0.0) ? slop : 0.0;
+    return dsgn * vmin(dlim, fabs(dcen));
+}
+
+// Characteristic slopes of one cell along direction d, mol_slope_x/y/z
+// (MOL_hydro_K.H:22-256), from the compact states of the cell (c) and its
+// neighbours below (m) and above (p) and the cell's sound speed.  Output
+// (compact): [0] / [1] the acoustic waves (reference dq 0 / 2), [2] / [3]
+// the transverse velocities (dq 3 / 4), [4] the rho*e wave (dq QREINT),
+// [5..7] the species waves.  The density wave (dq 1) is read by no face
+// state and is not formed.
+enum { MS_AM = 0, MS_AP = 1, MS_T1 = 2, MS_T2 = 3, MS_E = 4, MS_Y = 5, NMS = 8 };
+CAMR_HD void mol_slopes(int d, const double *m, const double *c, const double *p, double cc,
+                        int plm_iorder, double small_dens, double *dq) {
+    if (plm_iorder == 1) {
+        for (int n = 0; n < NMS; ++n) dq[n] = 0.0;
+        return;
+    }
+    const int un = vel_n(d), t1 = vel_t1(d), t2 = vel_t2(d);
+    double dlft = 0.5 * (c[SP] - m[SP]) / cc - 0.5 * c[SR] * (c[un] - m[un]);
+    double drgt = 0.5 * (p[SP] - c[SP]) / cc - 0.5 * c[SR] * (p[un] - c[un]);
+    dq[MS_AM] = mol_limiter(dlft, drgt);
+    const double cs2 = cc * cc;
+    dlft = 0.5 * (c[SP] - m[SP]) / cc + 0.5 * c[SR] * (c[un] - m[un]);
+    drgt = 0.5 * (p[SP] - c[SP]) / cc + 0.5 * c[SR] * (p[un] - c[un]);
+    dq[MS_AP] = mol_limiter(dlft, drgt);
+    dq[MS_T1] = mol_limiter(c[t1] - m[t1], p[t1] - c[t1]);
+    dq[MS_T2] = mol_limiter(c[t2] - m[t2], p[t2] - c[t2]);
+    const double enth = (c[SE] + c[SP]) / vmax(c[SR], small_dens);
+    dlft = (c[SE] - m[SE]) - enth * (c[SP] - m[SP]) / cs2;
+    drgt = (p[SE] - c[SE]) - enth * (p[SP] - c[SP]) / cs2;
+    dq[MS_E] = mol_limiter(dlft, drgt);
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) {
+        const int f = SY + n;
+        dlft = c[SR] * c[f] - m[SR] * m[f] - c[f] * (c[SP] - m[SP]) / cs2;
+        drgt = p[SR] * p[f] - c[SR] * c[f] - c[f] * (p[SP] - c[SP]) / cs2;
+        dq[MS_Y + n] = mol_limiter(dlft, drgt);
+    }
+}
+
+// Face state on one side of a face of direction d, mol_riemann_x/y/z
+// (MOL_riemann_K.H:12-168): side = +1 extrapolates cell c (below the face)
+// up, side = -1 cell c (above) down.  ya holds the mass fractions of the
+// cell above the face, which both sides take when the reconstructed
+// partial densities sum below small_dens.
+CAMR_HD void mol_face_state(int d, double side, const double *c, const double *dq, double cc,
+                            const double *ya, double small_dens, double small_pres,
+                            double *out) {
+    const int un = vel_n(d), t1 = vel_t1(d), t2 = vel_t2(d);
+    const double enth = (c[SE] + c[SP]) / vmax(c[SR], small_dens);
+    const double ac = dq[MS_AM] + dq[MS_AP];
+    double pv = (side > 0.0) ? c[SP] + 0.5 * ac * cc : c[SP] - 0.5 * ac * cc;
+    out[SP] = vmax(pv, small_pres);
+    const double du = (dq[MS_AP] - dq[MS_AM]) / c[SR];
+    out[un] = (side > 0.0) ? c[un] + 0.5 * du : c[un] - 0.5 * du;
+    out[t1] = (side > 0.0) ? c[t1] + 0.5 * dq[MS_T1] : c[t1] - 0.5 * dq[MS_T1];
+    out[t2] = (side > 0.0) ? c[t2] + 0.5 * dq[MS_T2] : c[t2] - 0.5 * dq[MS_T2];
+    double sum = 0.;
+    double sd[NSPEC];
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) {
+        const double h = 0.5 * (dq[MS_Y + n] + c[SY + n] * ac / cc);
+        sd[n] = (side > 0.0) ? c[SR] * c[SY + n] + h : c[SR] * c[SY + n] - h;
+        sum += sd[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NSPEC; ++n) out[SY + n] = (sum < small_dens) ? ya[n] : sd[n] / sum;
+    out[SR] = vmax(sum, small_dens);
+    const double he = 0.5 * (dq[MS_E] + enth * ac / cc);
+    out[SE] = (side > 0.0) ? c[SE] + he : c[SE] - he;
+}
+
 // ---- flatten (Source/Hydro/Godunov/flatten.H:19-99) ------------------------
 // p[o], un[o] hold pressure / normal velocity at offsets o-3, o = 0..6.
 CAMR_HD double flatten_1d(const double *p, const double *un) {

@@ -314,6 +314,49 @@ k_transdd(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa,
     for (int s = 0; s < NS; ++s) qm(i + ex, j + ey, k + ez, s) = o[s];
 }
 
+// ---------------------------------------------------- method of lines face
+// MOL_umeth's per-direction pair (MOL_umeth.cpp:74-139): the slopes of the
+// two cells beside the face (mol_slope_*), their face states
+// (mol_riemann_*) and hydro_cmpflx, one face per thread.  The slope of a
+// cell is formed by both faces that read it, with the same operations, so
+// no slope array is kept.
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_mol_face(Range r, Arr q, Arr qaux, Arr fout, Arr gout, HydroConst hc) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    // compact states of the cells i-2 .. i+1 along D
+    double c[4][NS];
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            c[o][s] = q(i + (o - 2) * ex, j + (o - 2) * ey, k + (o - 2) * ez, kQcomp[s]);
+    const double cl = qaux(i - ex, j - ey, k - ez, QC), cr = qaux(i, j, k, QC);
+    double dql[NMS], dqr[NMS];
+    mol_slopes(D, c[0], c[1], c[2], cl, hc.plm_iorder, hc.small_dens, dql);
+    mol_slopes(D, c[1], c[2], c[3], cr, hc.plm_iorder, hc.small_dens, dqr);
+    double ql[NS], qr[NS];
+    mol_face_state(D, 1.0, c[1], dql, cl, &c[2][SY], hc.small_dens, hc.small_pres, ql);
+    mol_face_state(D, -1.0, c[2], dqr, cr, &c[2][SY], hc.small_dens, hc.small_pres, qr);
+    FaceAux a;
+    a.c_l = cl;
+    a.c_r = cr;
+    a.csml_l = qaux(i - ex, j - ey, k - ez, QCSML);
+    a.csml_r = qaux(i, j, k, QCSML);
+    a.gam_l = qaux(i - ex, j - ey, k - ez, QGAMC);
+    a.gam_r = qaux(i, j, k, QGAMC);
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    double f[NF], g[NG];
+    int mut;
+    face_flux(D, idx, hc, ql, qr, a, f, g, &mut);
+#pragma unroll
+    for (int s = 0; s < NF; ++s) fout(i, j, k, kUcomp[s]) = f[s];
+    gout(i, j, k, GUN) = g[GUN];
+    gout(i, j, k, GP) = g[GP];
+}
+
 } // namespace
 
 // ---------------------------------------------------------------- divu
@@ -525,6 +568,48 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     launch(k_adjust<2>, nodes(bx, 2), st, A.flx[2], divu, A.uin, A.area[2], A.has_area, A.area_c[2], hc);
     NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G3[0], G3[1], G3[2],
+           A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
+           A.area_c[2], A.vol, A.has_vol, A.vol_c);
+    g_prof = nullptr;
+    return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
+}
+
+// hydro_umdrv with do_mol (Hydro_umdrv.cpp:57-63): MOL_umeth's fluxes and
+// Godunov face states, then the same divu / adjust_fluxes / consup as the
+// Godunov path.  The face states reach two cells past bx along the face
+// direction; divu one cell past it.
+size_t umdrv_mol_scratch_doubles(const Box &bx) {
+    size_t n = 0;
+    for (int d = 0; d < 3; ++d) n += (size_t)NG * nodes(bx, d).npts();
+    n += nodes(nodes(nodes(bx, 0), 1), 2).npts();
+    return n + 64;
+}
+
+int umdrv_mol(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof) {
+    g_prof = prof;
+    const HydroConst &hc = A.hc;
+    const Box bx = A.bx;
+    double *cur = scratch;
+    auto take = [&](const Box &b, int nc) {
+        Arr a = dense(cur, b);
+        cur += (size_t)nc * b.npts();
+        return a;
+    };
+    Arr G[3];
+    for (int d = 0; d < 3; ++d) G[d] = take(nodes(bx, d), NG);
+    Arr divu = take(nodes(nodes(nodes(bx, 0), 1), 2), 1);
+    NAMED("mol_faces");
+    launch(k_mol_face<0>, nodes(bx, 0), st, A.q, A.qaux, A.flx[0], G[0], hc);
+    launch(k_mol_face<1>, nodes(bx, 1), st, A.q, A.qaux, A.flx[1], G[1], hc);
+    launch(k_mol_face<2>, nodes(bx, 2), st, A.q, A.qaux, A.flx[2], G[2], hc);
+    NAMED("divu");
+    launch(k_divu, nodes(nodes(nodes(bx, 0), 1), 2), st, A.q, divu, hc);
+    NAMED("adjust_fluxes");
+    launch(k_adjust<0>, nodes(bx, 0), st, A.flx[0], divu, A.uin, A.area[0], A.has_area, A.area_c[0], hc);
+    launch(k_adjust<1>, nodes(bx, 1), st, A.flx[1], divu, A.uin, A.area[1], A.has_area, A.area_c[1], hc);
+    launch(k_adjust<2>, nodes(bx, 2), st, A.flx[2], divu, A.uin, A.area[2], A.has_area, A.area_c[2], hc);
+    NAMED("consup");
+    launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G[0], G[1], G[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
            A.area_c[2], A.vol, A.has_vol, A.vol_c);
     g_prof = nullptr;

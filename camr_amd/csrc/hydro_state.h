@@ -69,6 +69,9 @@ __global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const doubl
                              double small_dens, int *flags, EbFlags eb);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
+__global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,
+                               const DevFab *dsdt, const double *dtp, double small_dens,
+                               int *flags, EbFlags eb);
 __global__ void k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner,
                                   DomInfo CD, int ratio, int flo0, int flo1, int flo2);
 inline Box make_box1() { Box b; for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = 0; } return b; }

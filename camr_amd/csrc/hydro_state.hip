@@ -485,6 +485,25 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
         pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
+// Second stage of the method of lines (CAMR_advance.cpp:133-134):
+// LinComb(S_new, 0.5, Sborder, 0.5, S_old), then Saxpy(S_new, 0.5 dt,
+// new_hydro_source), + floor detection for the following clean_state.
+__global__ void __launch_bounds__(kBlock)
+kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
+               const double *dtp, double small_dens, int *flags, EbFlags eb) {
+    int b, i, j, k;
+    if (!batch_cell(snew, 0, b, i, j, k)) return;
+    const Arr S = snew[b].a, B = sb[b].a, O = sold[b].a, D = dsdt[b].a;
+    const double a = 0.5 * *dtp;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double v = 0.5 * B(i, j, k, n) + 0.5 * O(i, j, k, n);
+        v += a * D(i, j, k, n);
+        S(i, j, k, n) = v;
+    }
+    if (!eb.covered(i, j, k)) floor_detect(S(i, j, k, URHO), small_dens, flags);
+}
+
 __global__ void __launch_bounds__(kBlock)
 kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
               double small_dens, int *flags, EbFlags eb) {

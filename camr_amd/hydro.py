@@ -19,7 +19,8 @@ DEFAULTS = dict(ppm_type=0, plm_iorder=4, use_pslope=0, use_flattening=1,
                 allow_small_energy=1, dual_energy_update_E_from_e=1,
                 small=1.e-8, small_dens=1.e-7, small_pres=1.e-200,
                 small_temp=1.e2, smallu=1.e-12, small_num=1.e-8,
-                difmag=0.1, dual_energy_eta1=1.0, dual_energy_eta2=1.0e-4)
+                difmag=0.1, dual_energy_eta1=1.0, dual_energy_eta2=1.0e-4,
+                do_mol=0)
 LEVEL_DEFAULTS = dict(add_grav_src=1, const_grav=0.0, cfl=0.8,
                       change_max=1.1, init_shrink=1.0, max_dt=1.e200,
                       fixed_dt=-1.0, stop_time=-1.0)

@@ -85,6 +85,7 @@ typedef struct camr_hydro_params {
     int32_t bclo[3], bchi[3];   /* physical BC codes (camr_bc) */
     int32_t domlo[3], domhi[3]; /* problem-domain cell bounds */
     double dx[3];
+    int32_t do_mol;             /* 0 Godunov (CTU), 1 method of lines (MOL_umeth) */
 } camr_hydro_params;
 
 /* Defaults of Source/Params/_cpp_parameters and CAMR_Constants.H. */
@@ -110,8 +111,10 @@ int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q,
                          const camr_array4 *qaux, const camr_array4 *src,
                          const camr_array4 *srcq, void *stream);
 
-/* hydro_umdrv (Godunov branch), Source/Hydro/Hydro.H:11-35 /
- * Hydro_umdrv.cpp:11-93.  Same contract: q, qaux, srcq cover grow(bx,4);
+/* hydro_umdrv, Source/Hydro/Hydro.H:11-35 / Hydro_umdrv.cpp:11-93, its
+ * do_mol argument carried by p->do_mol (Godunov_umeth or MOL_umeth; the
+ * method of lines reads neither srcq nor dt's value beyond dt > 0).
+ * Same contract: q, qaux, srcq cover grow(bx,4);
  * uin covers grow(bx,1); dsdt is ACCUMULATED (+=) on bx; flx[d] covers the
  * d-faces of bx and is overwritten with extensive (area-weighted) fluxes.
  * area[d].p / vol->p may be NULL for a uniform Cartesian grid (then

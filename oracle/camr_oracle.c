@@ -1423,6 +1423,27 @@ static void norm_spec_flx(int i, int j, int k, ora_fab *flx) {
     for (int n = O_UFS; n < num; ++n) FA(flx, i, j, k, n) *= fac;
 }
 
+/* p div(u) from the Godunov face states, hydro_pdivu Hydro_utils_K.H:175-202 */
+static void pdivu_box(obox bx, const ora_fab *q1, const ora_fab *q2, const ora_fab *q3,
+                      const double area[3], ora_fab *pdivu, double vol) {
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        FA(pdivu, i, j, k, 0) =
+            0.5 *
+            ((FA(q1, i + 1, j, k, O_GDPRES) + FA(q1, i, j, k, O_GDPRES)) *
+                 (FA(q1, i + 1, j, k, O_GDU) * area[0] -
+                  FA(q1, i, j, k, O_GDU) * area[0]) +
+             (FA(q2, i, j + 1, k, O_GDPRES) + FA(q2, i, j, k, O_GDPRES)) *
+                 (FA(q2, i, j + 1, k, O_GDV) * area[1] -
+                  FA(q2, i, j, k, O_GDV) * area[1]) +
+             (FA(q3, i, j, k + 1, O_GDPRES) + FA(q3, i, j, k, O_GDPRES)) *
+                 (FA(q3, i, j, k + 1, O_GDW) * area[2] -
+                  FA(q3, i, j, k, O_GDW) * area[2])) /
+            vol;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Godunov_umeth (3-D), Source/Hydro/Godunov/Godunov_3D.cpp:12-461           */
 /* ------------------------------------------------------------------------ */
@@ -1570,27 +1591,182 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
             if (t != d) { ora_fab_free(&fl2[d][t]); ora_fab_free(&qg2[d][t]); }
     for (int d = 0; d < 3; ++d) { ora_fab_free(&qdm[d]); ora_fab_free(&qdp[d]); }
 
-    /* p div(u), Hydro_utils_K.H:175-202 */
-    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
-    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
-    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
-        FA(pdivu, i, j, k, 0) =
-            0.5 *
-            ((FA(q1, i + 1, j, k, O_GDPRES) + FA(q1, i, j, k, O_GDPRES)) *
-                 (FA(q1, i + 1, j, k, O_GDU) * area[0] -
-                  FA(q1, i, j, k, O_GDU) * area[0]) +
-             (FA(q2, i, j + 1, k, O_GDPRES) + FA(q2, i, j, k, O_GDPRES)) *
-                 (FA(q2, i, j + 1, k, O_GDV) * area[1] -
-                  FA(q2, i, j, k, O_GDV) * area[1]) +
-             (FA(q3, i, j, k + 1, O_GDPRES) + FA(q3, i, j, k, O_GDPRES)) *
-                 (FA(q3, i, j, k + 1, O_GDW) * area[2] -
-                  FA(q3, i, j, k, O_GDW) * area[2])) /
-            vol;
-    }
+    pdivu_box(bx, q1, q2, q3, area, pdivu, vol);
 }
 
 /* ------------------------------------------------------------------------ */
-/* hydro_umdrv, Source/Hydro/Hydro_umdrv.cpp:11-93                           */
+/* Method of lines: MOL_umeth (3-D), Source/Hydro/MOL/MOL_umeth.cpp:15-142;  */
+/* slopes MOL_hydro_K.H:10-256, face states MOL_riemann_K.H:12-491           */
+/* ------------------------------------------------------------------------ */
+/* limiter, MOL_hydro_K.H:10-20 (plm_theta 2: van Leer's MC) */
+static double mol_limiter(double dlft, double drgt, double plm_theta) {
+    const double dcen = 0.5 * (dlft + drgt);
+    const double dsgn = copysign(1.0, dcen);
+    const double slop = plm_theta * smin(fabs(dlft), fabs(drgt));
+    const double dlim = (dlft * drgt >= 0.0) ? slop : 0.0;
+    return dsgn * smin(dlim, fabs(dcen));
+}
+
+/* Characteristic slopes of cell (i,j,k) along direction d: mol_slope_x/y/z
+ * (MOL_hydro_K.H:22-256).  dq[0] / dq[2] the acoustic waves, dq[3] / dq[4]
+ * the transverse velocities (x: v, w; y: u, w; z: u, v), dq[QREINT] the
+ * entropy-like wave, dq[QFS+n] the species waves.  dq[1] (the density
+ * wave) is not computed by the x variant and no face state reads it; it is
+ * left 0 here for every direction. */
+static void mol_slope(const ora_fab *q, const ora_fab *qa, int i, int j, int k, int d,
+                      int plm_iorder, double plm_theta, double small_dens, double dq[O_QVAR]) {
+    int m[3] = {i, j, k}, p[3] = {i, j, k};
+    m[d] -= 1;
+    p[d] += 1;
+    const int un = O_QU + d;
+    const int t1 = (d == 0) ? O_QV : O_QU;
+    const int t2 = (d == 2) ? O_QV : O_QW;
+    for (int n = 0; n < O_QVAR; ++n) dq[n] = 0.0;
+    if (plm_iorder == 1) return;
+#define QC_(n) FA(q, i, j, k, n)
+#define QM_(n) FV(q, m, n)
+#define QP_(n) FV(q, p, n)
+    const double c = FA(qa, i, j, k, O_QC);
+    double dlft = 0.5 * (QC_(O_QPRES) - QM_(O_QPRES)) / c - 0.5 * QC_(O_QRHO) * (QC_(un) - QM_(un));
+    double drgt = 0.5 * (QP_(O_QPRES) - QC_(O_QPRES)) / c - 0.5 * QC_(O_QRHO) * (QP_(un) - QC_(un));
+    dq[0] = mol_limiter(dlft, drgt, plm_theta);
+    const double cs2 = c * c;
+    dlft = 0.5 * (QC_(O_QPRES) - QM_(O_QPRES)) / c + 0.5 * QC_(O_QRHO) * (QC_(un) - QM_(un));
+    drgt = 0.5 * (QP_(O_QPRES) - QC_(O_QPRES)) / c + 0.5 * QC_(O_QRHO) * (QP_(un) - QC_(un));
+    dq[2] = mol_limiter(dlft, drgt, plm_theta);
+    dlft = QC_(t1) - QM_(t1);
+    drgt = QP_(t1) - QC_(t1);
+    dq[3] = mol_limiter(dlft, drgt, plm_theta);
+    dlft = QC_(t2) - QM_(t2);
+    drgt = QP_(t2) - QC_(t2);
+    dq[4] = mol_limiter(dlft, drgt, plm_theta);
+    const double enth = (QC_(O_QREINT) + QC_(O_QPRES)) / smax(QC_(O_QRHO), small_dens);
+    dlft = (QC_(O_QREINT) - QM_(O_QREINT)) - enth * (QC_(O_QPRES) - QM_(O_QPRES)) / cs2;
+    drgt = (QP_(O_QREINT) - QC_(O_QREINT)) - enth * (QP_(O_QPRES) - QC_(O_QPRES)) / cs2;
+    dq[O_QREINT] = mol_limiter(dlft, drgt, plm_theta);
+    for (int n = 0; n < NSPEC; ++n) {
+        const int f = O_QFS + n;
+        dlft = QC_(O_QRHO) * QC_(f) - QM_(O_QRHO) * QM_(f) - QC_(f) * (QC_(O_QPRES) - QM_(O_QPRES)) / cs2;
+        drgt = QP_(O_QRHO) * QP_(f) - QC_(O_QRHO) * QC_(f) - QC_(f) * (QP_(O_QPRES) - QC_(O_QPRES)) / cs2;
+        dq[f] = mol_limiter(dlft, drgt, plm_theta);
+    }
+#undef QC_
+#undef QM_
+#undef QP_
+}
+
+/* Left / right states at the face (i,j,k) of direction d from the slopes
+ * dqm of the cell below and dqc of the cell above: mol_riemann_x/y/z before
+ * their hydro_cmpflx call (MOL_riemann_K.H:12-168, 171-326, 329-488).
+ * Species are reconstructed as partial densities and renormalised; a
+ * vanishing total falls back to the mass fractions of the cell ABOVE the
+ * face on both sides, as the reference does. */
+static void mol_face_states(const ora_fab *q, const ora_fab *qa, int i, int j, int k, int d,
+                            const double *dqm, const double *dqc, double small_dens,
+                            double small_pres, double *ql, double *qr) {
+    int m[3] = {i, j, k};
+    m[d] -= 1;
+    const int un = O_QU + d;
+    const int t1 = (d == 0) ? O_QV : O_QU;
+    const int t2 = (d == 2) ? O_QV : O_QW;
+    double cspeed = FV(qa, m, O_QC);
+    double enth = (FV(q, m, O_QREINT) + FV(q, m, O_QPRES)) / smax(FV(q, m, O_QRHO), small_dens);
+    double pl = FV(q, m, O_QPRES) + 0.5 * (dqm[0] + dqm[2]) * cspeed;
+    pl = smax(pl, small_pres);
+    ql[O_QPRES] = pl;
+    ql[un] = FV(q, m, un) + 0.5 * ((dqm[2] - dqm[0]) / FV(q, m, O_QRHO));
+    ql[t1] = FV(q, m, t1) + 0.5 * dqm[3];
+    ql[t2] = FV(q, m, t2) + 0.5 * dqm[4];
+    double sum = 0.;
+    for (int n = 0; n < NSPEC; ++n) {
+        const int f = O_QFS + n;
+        const double specden = FV(q, m, O_QRHO) * FV(q, m, f) +
+                               0.5 * (dqm[f] + FV(q, m, f) * (dqm[0] + dqm[2]) / cspeed);
+        sum += specden;
+        ql[f] = specden;
+    }
+    if (sum < small_dens) {
+        for (int n = 0; n < NSPEC; ++n) ql[O_QFS + n] = FA(q, i, j, k, O_QFS + n);
+    } else {
+        for (int n = 0; n < NSPEC; ++n) ql[O_QFS + n] = ql[O_QFS + n] / sum;
+    }
+    ql[O_QRHO] = smax(sum, small_dens);
+    ql[O_QREINT] = FV(q, m, O_QREINT) + 0.5 * (dqm[O_QREINT] + enth * (dqm[0] + dqm[2]) / cspeed);
+
+    cspeed = FA(qa, i, j, k, O_QC);
+    enth = (FA(q, i, j, k, O_QREINT) + FA(q, i, j, k, O_QPRES)) / smax(FA(q, i, j, k, O_QRHO), small_dens);
+    double pr = FA(q, i, j, k, O_QPRES) - 0.5 * (dqc[0] + dqc[2]) * cspeed;
+    pr = smax(pr, small_pres);
+    qr[O_QPRES] = pr;
+    qr[un] = FA(q, i, j, k, un) - 0.5 * ((dqc[2] - dqc[0]) / FA(q, i, j, k, O_QRHO));
+    qr[t1] = FA(q, i, j, k, t1) - 0.5 * dqc[3];
+    qr[t2] = FA(q, i, j, k, t2) - 0.5 * dqc[4];
+    sum = 0.;
+    for (int n = 0; n < NSPEC; ++n) {
+        const int f = O_QFS + n;
+        const double specden = FA(q, i, j, k, O_QRHO) * FA(q, i, j, k, f) -
+                               0.5 * (dqc[f] + FA(q, i, j, k, f) * (dqc[0] + dqc[2]) / cspeed);
+        sum += specden;
+        qr[f] = specden;
+    }
+    if (sum < small_dens) {
+        for (int n = 0; n < NSPEC; ++n) qr[O_QFS + n] = FA(q, i, j, k, O_QFS + n);
+    } else {
+        for (int n = 0; n < NSPEC; ++n) qr[O_QFS + n] = qr[O_QFS + n] / sum;
+    }
+    qr[O_QRHO] = smax(sum, small_dens);
+    qr[O_QREINT] = FA(q, i, j, k, O_QREINT) - 0.5 * (dqc[O_QREINT] + enth * (dqc[0] + dqc[2]) / cspeed);
+}
+
+/* MOL_umeth: per direction, slopes on grow(bx, d, 1), then the face states
+ * and hydro_cmpflx on the faces of bx; then p div(u). */
+static void mol_umeth(obox bx, const ora_params *P, const ora_fab *q, const ora_fab *qa,
+                      ora_fab *flx[3], ora_fab *qg[3], const double area[3], ora_fab *pdivu,
+                      double vol) {
+    const double plm_theta = 2.0;
+    ora_fab slope, ql, qr;
+    fab_alloc_box(&slope, bgrow(bx, 1), O_QVAR);
+    for (int d = 0; d < 3; ++d) {
+        const obox sb = bgrowd(bx, d, 1);
+        for (int k = sb.lo[2]; k <= sb.hi[2]; ++k)
+        for (int j = sb.lo[1]; j <= sb.hi[1]; ++j)
+        for (int i = sb.lo[0]; i <= sb.hi[0]; ++i) {
+            double dq[O_QVAR];
+            mol_slope(q, qa, i, j, k, d, P->plm_iorder, plm_theta, P->small_dens, dq);
+            for (int n = 0; n < O_QVAR; ++n) FA(&slope, i, j, k, n) = dq[n];
+        }
+        const obox fb = bnodes(bx, d);
+        fab_alloc_box(&ql, fb, O_QVAR);
+        fab_alloc_box(&qr, fb, O_QVAR);
+        for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
+        for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
+        for (int i = fb.lo[0]; i <= fb.hi[0]; ++i) {
+            int m[3] = {i, j, k};
+            m[d] -= 1;
+            double dqm[O_QVAR], dqc[O_QVAR], l[O_QVAR], r[O_QVAR];
+            for (int n = 0; n < O_QVAR; ++n) {
+                dqm[n] = FV(&slope, m, n);
+                dqc[n] = FA(&slope, i, j, k, n);
+                l[n] = r[n] = 0.0;
+            }
+            mol_face_states(q, qa, i, j, k, d, dqm, dqc, P->small_dens, P->small_pres, l, r);
+            for (int n = 0; n < O_QVAR; ++n) {
+                FA(&ql, i, j, k, n) = l[n];
+                FA(&qr, i, j, k, n) = r[n];
+            }
+            cmpflx(i, j, k, P->bclo[d], P->bchi[d], P->domlo[d], P->domhi[d], &ql, &qr,
+                   flx[d], qg[d], qa, d, P->small, P->small_dens, P->small_pres, P->smallu);
+        }
+        ora_fab_free(&ql);
+        ora_fab_free(&qr);
+    }
+    ora_fab_free(&slope);
+    pdivu_box(bx, qg[0], qg[1], qg[2], area, pdivu, vol);
+}
+
+/* ------------------------------------------------------------------------ */
+/* hydro_umdrv, Source/Hydro/Hydro_umdrv.cpp:11-93 (Godunov or, with        */
+/* do_mol, MOL_umeth)                                                         */
 /* + adjust_fluxes Hydro_adjust_fluxes.cpp:13-44 + hydro_consup              */
 /* Hydro_consup.cpp:13-25                                                     */
 /* ------------------------------------------------------------------------ */
@@ -1611,8 +1787,13 @@ void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
     const double vol = dx[0] * dx[1] * dx[2];
     const double area[3] = {dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1]};
 
-    godunov_umeth(bx, P, q, qa, srcq, &flx[0], &flx[1], &flx[2], &qec[0],
-                  &qec[1], &qec[2], area, &pdivu, vol, dt);
+    if (P->do_mol) {
+        ora_fab *fl[3] = {&flx[0], &flx[1], &flx[2]}, *qg[3] = {&qec[0], &qec[1], &qec[2]};
+        mol_umeth(bx, P, q, qa, fl, qg, area, &pdivu, vol);
+    } else {
+        godunov_umeth(bx, P, q, qa, srcq, &flx[0], &flx[1], &flx[2], &qec[0],
+                      &qec[1], &qec[2], area, &pdivu, vol, dt);
+    }
 
     for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
     for (int j = bxg2.lo[1]; j <= bxg2.hi[1]; ++j)
@@ -2142,7 +2323,8 @@ static void construct_hydro_source(ora_level *L, double dt) {
         fab_alloc_box(&qaux, qbx, O_NQAUX);
         fab_alloc_box(&srcq, qbx, O_QVAR);
         ora_ctoprim_box(qbx.lo, qbx.hi, &L->Sborder[b], &q, &qaux, P, &err);
-        ora_srctoprim_box(qbx.lo, qbx.hi, &q, &qaux, &L->srchyd[b], &srcq);
+        if (!P->do_mol) /* CAMR_construct_hydro_source.cpp:129-143 */
+            ora_srctoprim_box(qbx.lo, qbx.hi, &q, &qaux, &L->srchyd[b], &srcq);
         if (L->kind && L->kind[b] == 1)
             ora_hydro_umdrv_eb(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
                                &srcq, flux, dt, P, &L->geom[b]);
@@ -2217,6 +2399,25 @@ int ora_level_advance(ora_level *L, double dt) {
     set_val(L->srchyd, L->nbox, 0.0);
     construct_hydro_source(L, dt);
     saxpy_valid(L, L->Snew, dt, L->hsrc);
+    if (P->do_mol) {
+        /* second stage (CAMR_advance.cpp:124-135): expand_state(Sborder,
+         * cur_time) = FillPatch from S_new + clean_state, the hydro source
+         * of that state, LinComb(S_new, 0.5, Sborder, 0.5, S_old) and
+         * Saxpy(S_new, 0.5 dt, new_hydro_source) */
+        for (int b = 0; b < L->nbox; ++b) fill_patch(L, &L->Sborder[b], b, L->Snew);
+        clean_state(L, L->Sborder, L->ng);
+        construct_hydro_source(L, dt);
+        for (int b = 0; b < L->nbox; ++b) {
+            obox B = L->box[b];
+            for (int n = 0; n < O_NVAR; ++n)
+            for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+            for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+            for (int i = B.lo[0]; i <= B.hi[0]; ++i)
+                FA(&L->Snew[b], i, j, k, n) = 0.5 * FA(&L->Sborder[b], i, j, k, n) +
+                                              0.5 * FA(&L->Sold[b], i, j, k, n);
+        }
+        saxpy_valid(L, L->Snew, 0.5 * dt, L->hsrc);
+    }
     clean_state(L, L->Snew, 0);
     if (nsrc) {
         /* construct_new_grav_source, GravitySrc.cpp:20-33,81-125 */

@@ -59,6 +59,7 @@ typedef struct ora_params {
     double prob_lo[3];
     int tile[3]; /* hydro_tile_size used by construct_hydro_source */
     int eb;      /* 1: the guards of an AMREX_USE_EB build (covered cells) */
+    int do_mol;  /* 1: method-of-lines integrator (two-stage RK, MOL_umeth) */
 } ora_params;
 
 /* Cut-cell geometry of one box (the EBFArrayBoxFactory arrays of an MFIter):

@@ -44,7 +44,7 @@ class OraParams(C.Structure):
         ("domlo", C.c_int * 3), ("domhi", C.c_int * 3),
         ("is_periodic", C.c_int * 3), ("dx", C.c_double * 3),
         ("prob_lo", C.c_double * 3), ("tile", C.c_int * 3),
-        ("eb", C.c_int),
+        ("eb", C.c_int), ("do_mol", C.c_int),
     ]
 
 
