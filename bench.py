@@ -39,6 +39,10 @@ NVAR = 10
 #     cells (2); writes the compact flux + Godunov (10) = 30 doubles.
 ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
                 "riemann_normal": 30}
+# mol_faces (one launch per direction d, method of lines): per d-face, the 9
+# primitives and 3 aux values (c, csmall, gamma) of one cell read, the 9
+# fluxes (NVAR without UTEMP) and Godunov (u, p) written.
+NS_MOL_READ, NF_MOL_WRITE = 12, 11
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # for the fused engine at 256^3 PPM (profiles/hbm_traffic_fused_256cube_ppm.md,
@@ -305,6 +309,95 @@ def bench_eb(args) -> int:
     return 0
 
 
+def bench_mol(args) -> int:
+    """The method-of-lines integrator (do_mol = 1) on the n^3 periodic
+    smooth field, CFL 0.3 (the MOL limit, CAMR.cpp:175-177), one box on
+    the staged engine.  The first step is checked bit for bit against the
+    oracle (timed on the host cores as the CPU baseline); the dominant
+    kernel is timed with HIP events on the level's stream during the timed
+    steps."""
+    import numpy as np
+    import torch
+    from camr_amd import _lib, workloads
+    from camr_amd.hydro import Level
+    from oracle import oracle as O
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n, cfl = args.n, 0.3
+    prob = workloads.smooth_periodic(n=n, ppm_type=0, cfl=cfl)
+    prob.params["do_mol"] = 1
+    U0 = workloads.smooth_init(prob, 0)
+    L = Level(prob, engine=_lib.ENGINE_STAGED)
+    L.set_box(0, U0)
+    L.init_post()
+    dt = L.initial_dt()
+    L.advance(dt)
+    L.sync()
+    identical, cpu = None, None
+    if not args.no_cpu_baseline:
+        O.build()
+        p = O.params(do_mol=1, cfl=cfl, bclo=prob.bclo, bchi=prob.bchi, domlo=prob.domlo,
+                     domhi=prob.domhi, is_periodic=prob.is_periodic, dx=prob.dx)
+        OL = O.Level(p, prob.boxes)
+        OL.set_box(0, U0)
+        OL.init_post()
+        odt = OL.initial_dt()
+        t0 = time.perf_counter()
+        err = OL.advance(odt)
+        el = time.perf_counter() - t0
+        assert err == 0
+        identical = bool(odt == dt and np.array_equal(OL.get_box(0), L.get_box(0)))
+        OL.close()
+        cpu = {"value": n ** 3 / el, "unit": "cell-updates/s",
+               "cores": O.lib().ora_get_threads(), "kind": "port",
+               "sample": f"the first step of the same {n}^3 MOL workload, {el:.1f} s wall"}
+    for _ in range(max(args.warmup - 1, 0)):
+        dt = L.new_dt(dt)
+        L.advance(dt)
+    L.sync()
+    L.profile(True)
+    dt = L.new_dt(dt)
+    L.advance(dt)
+    L.sync()
+    rep = L.profile_report()
+    kern = max((k for k in rep if k != "advance"), key=lambda k: rep[k]["ms"])
+    L.profile(True, kern)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = L.new_dt(dt)
+        L.advance(dt)
+    L.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rep = L.profile_report()[kern]
+    L.profile(False)
+    cups = n ** 3 * args.steps / el
+    avg_ms = rep["ms"] / rep["count"]
+    # algorithmic bytes per launch of the dominant kernel
+    if kern == "mol_faces":
+        # per face: 9 primitives + 3 aux values of one cell read, 9 fluxes
+        # + 2 Godunov values written
+        algo = (NS_MOL_READ + NF_MOL_WRITE) * 8 * (n + 1) * n * n
+    else:
+        algo = ALGO_DOUBLES.get(kern, 2 * NVAR) * 8 * n ** 3
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    out = {"metric": f"FP64 cell-updates/s at {n}^3, method of lines (do_mol = 1)",
+           "value": cups, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: seeded smooth periodic field (as the default workload)",
+           "config": {"workload": f"{n}^3 periodic single-level MOL advance (two-stage RK, "
+                                  f"characteristic MC slopes), CFL {cfl}, one box, staged engine",
+                      "cells": n ** 3, "cfl": cfl},
+           "identical_to_oracle_1_step": identical,
+           "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": None, "avg_launch_ms": avg_ms, "launches": rep["count"]},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_adjoint(args) -> int:
     """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
     wall time of the backward pass (recomputation from the checkpoints
@@ -421,12 +514,14 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb", "mol"],
+                    default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
                          "AMR levels against per-box launches; adjoint: "
                          "backward/forward cost of the level-step adjoint; sod: "
                          "the 256x4x4 Sod tube, GPU beside the host oracle; eb: "
-                         "Sod past a sphere, embedded boundary with FluxRedist")
+                         "Sod past a sphere, embedded boundary with FluxRedist; "
+                         "mol: the n^3 periodic field with the method of lines")
     ap.add_argument("--amr-n0", type=int, default=128)
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--eb-n", type=int, default=128,
@@ -454,6 +549,8 @@ def main() -> int:
         return bench_eb(args)
     if args.workload == "sod":
         return bench_sod(args)
+    if args.workload == "mol":
+        return bench_mol(args)
 
     import torch
     from camr_amd import _lib, workloads

@@ -569,7 +569,9 @@ int HydroLevel::create() {
         for (int d = 0; d < 3; ++d)
             if ((rc = alloc(&flx_[d], NVAR * max_nodes * kWorkers))) return rc;
         size_t sc = 0;
-        for (const Box &B : box_) sc = std::max(sc, umdrv_staged_scratch_doubles(B));
+        for (const Box &B : box_)
+            sc = std::max(sc, desc_.hp.do_mol ? umdrv_mol_scratch_doubles(B)
+                                              : umdrv_staged_scratch_doubles(B));
         if ((rc = setup_workers(max_g4, sc))) return rc;
     }
     return CAMR_OK;
