@@ -1887,6 +1887,8 @@ struct ora_level {
     int32_t *flags;      /* EB flags over the domain grown by 7 (or NULL) */
     ora_ebgeom *geom;    /* per box: flags view + cut-cell data on grow(bx,7) */
     int *kind;           /* per box: 0 regular, 1 EB path, 2 all covered */
+    int defer_post;      /* 1: the step leaves post_timestep's clean_state to the caller */
+    struct ora_fluxreg *reg_crse, *reg_fine; /* flux registers: this level as coarse / fine */
 };
 
 static int64_t dom_index(const ora_level *L, const int c[3]) {
@@ -2271,6 +2273,10 @@ double ora_level_new_dt(ora_level *L, double dt_old) {
 typedef struct { int b; obox t; } otile;
 
 /* construct_hydro_source, Hydro/CAMR_construct_hydro_source.cpp:9-267 */
+typedef struct ora_fluxreg ora_fluxreg;
+static void fluxreg_crse_add(ora_fluxreg *R, int b, obox bx, ora_fab flux[3], double dt);
+static void fluxreg_fine_add(ora_fluxreg *R, obox fbx, ora_fab flux[3], double dt);
+
 static void construct_hydro_source(ora_level *L, double dt) {
     const ora_params *P = &L->P;
     for (int b = 0; b < L->nbox; ++b) {
@@ -2331,6 +2337,11 @@ static void construct_hydro_source(ora_level *L, double dt) {
         else
             ora_hydro_umdrv(bx.lo, bx.hi, &L->Sborder[b], &L->hsrc[b], &q, &qaux,
                             &srcq, flux, dt, P);
+        /* do_reflux: the tile's fluxes into the flux registers
+         * (CAMR_construct_hydro_source.cpp:243-254), fac_for_reflux * dt */
+        const double fdt = (P->do_mol ? 0.5 : 1.0) * dt;
+        if (L->reg_crse) fluxreg_crse_add(L->reg_crse, b, bx, flux, fdt);
+        if (L->reg_fine) fluxreg_fine_add(L->reg_fine, bx, flux, fdt);
         ora_fab_free(&q); ora_fab_free(&qaux); ora_fab_free(&srcq);
         for (int d = 0; d < 3; ++d) ora_fab_free(&flux[d]);
     }
@@ -2438,8 +2449,9 @@ int ora_level_advance(ora_level *L, double dt) {
         saxpy_valid(L, L->Snew, dt, L->newsrc);
         clean_state(L, L->Snew, 0);
     }
-    /* post_timestep: clean_state(S_new) */
-    clean_state(L, L->Snew, 0);
+    /* post_timestep: clean_state(S_new); a coarse level of a two-level
+     * step runs it after reflux and avgDown instead */
+    if (!L->defer_post) clean_state(L, L->Snew, 0);
     L->time += dt;
     return L->err;
 }
@@ -2994,12 +3006,21 @@ void ora_level_set_box_eb(ora_level *L, int b, const double *vfrac, const double
 /* ======================================================================== */
 /* Value of component n of the coarse level at cell c (any cell: ghost cells
  * map through the domain BCs as FillPatch does). */
-static double level_value(const ora_level *L, const int c[3], int n) {
+/* Coarse data at the fill time: new only (tw.mode 0), old only (1), or
+ * alpha * old + beta * new (2), StateData::InterpAddBox's choice. */
+typedef struct { int mode; double alpha, beta; } otimew;
+
+static double level_value(const ora_level *L, const int c[3], int n, otimew tw) {
     int m[3];
     double s;
     map_cell(&L->P, c, n, m, &s);
     const int ob = L->owner[dom_index(L, m)];
-    const double v = ob >= 0 ? FA(&L->Snew[ob], m[0], m[1], m[2], n) : 0.0;
+    double v = 0.0;
+    if (ob >= 0) {
+        const double vn = FA(&L->Snew[ob], m[0], m[1], m[2], n);
+        const double vo = FA(&L->Sold[ob], m[0], m[1], m[2], n);
+        v = tw.mode == 0 ? vn : (tw.mode == 1 ? vo : tw.alpha * vo + tw.beta * vn);
+    }
     return (s < 0.0) ? -v : v;
 }
 
@@ -3009,13 +3030,13 @@ static double level_value(const ora_level *L, const int c[3], int n) {
  * within the min / max of the 27 coarse neighbours.  The children of a
  * coarse cell average back to it (the offsets are symmetric). */
 static double cons_interp(const ora_level *C, const int cc[3], int n, const double off[3],
-                          double edge) {
+                          double edge, otimew tw) {
     double u[3][3][3];
     for (int a = -1; a <= 1; ++a)
     for (int b = -1; b <= 1; ++b)
     for (int d = -1; d <= 1; ++d) {
         const int c[3] = {cc[0] + d, cc[1] + b, cc[2] + a};
-        u[a + 1][b + 1][d + 1] = level_value(C, c, n);
+        u[a + 1][b + 1][d + 1] = level_value(C, c, n, tw);
     }
     const double u0 = u[1][1][1];
     const double um[3] = {u[1][1][0], u[1][0][1], u[0][1][1]};
@@ -3050,7 +3071,7 @@ static double cons_interp(const ora_level *C, const int cc[3], int n, const doub
 
 /* The exterior data of fine level F (every cell of its domain, NVAR
  * components) interpolated from coarse level C with refinement `ratio`. */
-int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) {
+static int exterior_from_coarse_t(ora_level *F, const ora_level *C, int ratio, otimew tw) {
     for (int d = 0; d < 3; ++d)
         if (F->P.domlo[d] != C->P.domlo[d] * ratio ||
             F->P.domhi[d] + 1 != (C->P.domhi[d] + 1) * ratio) return 1;
@@ -3069,12 +3090,284 @@ int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) 
             const int sub = (f[d] - F->P.domlo[d]) - (cc[d] - C->P.domlo[d]) * ratio;
             off[d] = (sub - half) / ratio; /* fine-centre offset in coarse cells */
         }
-        F->ext[dom_index(F, f) + (int64_t)n * nc] = cons_interp(C, cc, n, off, half / ratio);
+        F->ext[dom_index(F, f) + (int64_t)n * nc] = cons_interp(C, cc, n, off, half / ratio, tw);
     }
     return 0;
+}
+
+int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) {
+    const otimew tw = {0, 0.0, 0.0};
+    return exterior_from_coarse_t(F, C, ratio, tw);
 }
 
 void ora_level_get_exterior(const ora_level *L, double *out) {
     if (L->ext)
         memcpy(out, L->ext, sizeof(double) * (size_t)(L->dn[0] * L->dn[1] * L->dn[2] * O_NVAR));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coarse-fine synchronisation: the flux register, reflux, average_down and */
+/* the two-level subcycled step.  CAMR uses AMReX's YAFluxRegister          */
+/* (CAMR.H:310, CrseAdd / FineAdd at CAMR_construct_hydro_source.cpp:       */
+/* 243-254, Reflux at CAMR.cpp:803-851), average_down (CAMR.cpp:1047-1066)  */
+/* and Amr::timeStep; AMReX is absent, so these are restated from their     */
+/* published definitions:                                                   */
+/*  - coarse cells are flagged fine (covered by the coarsened fine boxes),  */
+/*    boundary (not covered, within one cell of a covered one) or neither;  */
+/*  - CrseAdd: a boundary cell takes -/+ dt/vol_c times the coarse flux     */
+/*    through each face it shares with a covered cell;                      */
+/*  - FineAdd: the same cell takes +/- dt_f/(vol_f r^3) times each fine     */
+/*    flux on that face, one addition per fine face, in the order (outer,   */
+/*    inner) = (k, j) for x faces, (k, i) for y, (j, i) for z;              */
+/*  - Reflux: crse_data += cfp, then S_new += crse_data on every valid cell;*/
+/*  - average_down: a covered coarse cell = (1/r^3) * the sum of its        */
+/*    children in (k, j, i) order.                                          */
+/* Coarse boundary cells bordering two different fine boxes are not        */
+/* supported (where AMReX's coarse-fine patches overlap it weighs them).    */
+/* ------------------------------------------------------------------------ */
+enum { FR_NONE = 0, FR_BND = 1, FR_FINE = 2 };
+struct ora_fluxreg {
+    ora_level *C, *F;
+    int ratio;
+    unsigned char *flag; /* per coarse domain cell */
+    ora_fab *crse, *cfp; /* per coarse box, NVAR components */
+};
+
+/* periodic image of a coarse index; 0 outside a non-periodic domain */
+static int fr_map(const ora_level *C, const int c[3], int m[3]) {
+    for (int d = 0; d < 3; ++d) {
+        const int lo = C->P.domlo[d], n = C->P.domhi[d] - lo + 1;
+        int x = c[d];
+        if (x < lo || x > lo + n - 1) {
+            if (!C->P.is_periodic[d]) return 0;
+            x = lo + ((x - lo) % n + n) % n;
+        }
+        m[d] = x;
+    }
+    return 1;
+}
+
+static int fr_flag(const ora_fluxreg *R, const int c[3]) {
+    int m[3];
+    if (!fr_map(R->C, c, m)) return FR_NONE;
+    return R->flag[dom_index(R->C, m)];
+}
+
+void ora_fluxreg_destroy(ora_fluxreg *R) {
+    if (!R) return;
+    if (R->C->reg_crse == R) R->C->reg_crse = NULL;
+    if (R->F->reg_fine == R) R->F->reg_fine = NULL;
+    for (int b = 0; b < R->C->nbox; ++b) { ora_fab_free(&R->crse[b]); ora_fab_free(&R->cfp[b]); }
+    free(R->crse); free(R->cfp); free(R->flag); free(R);
+}
+
+ora_fluxreg *ora_fluxreg_create(ora_level *C, ora_level *F, int ratio) {
+    if (ratio < 2) return NULL;
+    for (int d = 0; d < 3; ++d)
+        if (F->P.domlo[d] != C->P.domlo[d] * ratio ||
+            F->P.domhi[d] + 1 != (C->P.domhi[d] + 1) * ratio) return NULL;
+    for (int b = 0; b < F->nbox; ++b)
+        for (int d = 0; d < 3; ++d)
+            if (F->box[b].lo[d] % ratio || (F->box[b].hi[d] + 1) % ratio) return NULL;
+    ora_fluxreg *R = (ora_fluxreg *)calloc(1, sizeof(ora_fluxreg));
+    R->C = C; R->F = F; R->ratio = ratio;
+    const int64_t nc = C->dn[0] * C->dn[1] * C->dn[2];
+    R->flag = (unsigned char *)calloc((size_t)nc, 1);
+    int *cov = (int *)malloc(sizeof(int) * (size_t)nc); /* covering fine box */
+    for (int64_t c = 0; c < nc; ++c) cov[c] = -1;
+    for (int b = 0; b < F->nbox; ++b) {
+        const obox fb = F->box[b];
+        for (int k = fb.lo[2] / ratio; k <= fb.hi[2] / ratio; ++k)
+        for (int j = fb.lo[1] / ratio; j <= fb.hi[1] / ratio; ++j)
+        for (int i = fb.lo[0] / ratio; i <= fb.hi[0] / ratio; ++i) {
+            const int c[3] = {i, j, k};
+            R->flag[dom_index(C, c)] = FR_FINE;
+            cov[dom_index(C, c)] = b;
+        }
+    }
+    int bad = 0;
+    for (int b = 0; b < F->nbox; ++b) {
+        const obox fb = F->box[b];
+        for (int k = fb.lo[2] / ratio - 1; k <= fb.hi[2] / ratio + 1; ++k)
+        for (int j = fb.lo[1] / ratio - 1; j <= fb.hi[1] / ratio + 1; ++j)
+        for (int i = fb.lo[0] / ratio - 1; i <= fb.hi[0] / ratio + 1; ++i) {
+            const int c[3] = {i, j, k};
+            int m[3];
+            if (!fr_map(C, c, m)) continue;
+            if (R->flag[dom_index(C, m)] != FR_FINE) R->flag[dom_index(C, m)] = FR_BND;
+        }
+    }
+    for (int k = C->P.domlo[2]; k <= C->P.domhi[2]; ++k)
+    for (int j = C->P.domlo[1]; j <= C->P.domhi[1]; ++j)
+    for (int i = C->P.domlo[0]; i <= C->P.domhi[0]; ++i) {
+        const int c[3] = {i, j, k};
+        if (R->flag[dom_index(C, c)] != FR_BND) continue;
+        int seen = -1;
+        for (int d = 0; d < 3; ++d)
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                int nb[3] = {i, j, k}, m[3];
+                nb[d] += sgn;
+                if (!fr_map(C, nb, m)) continue;
+                const int fbx = cov[dom_index(C, m)];
+                if (fbx < 0) continue;
+                if (seen >= 0 && fbx != seen) bad = 1;
+                seen = fbx;
+            }
+    }
+    free(cov);
+    if (bad) { free(R->flag); free(R); return NULL; }
+    R->crse = (ora_fab *)calloc(C->nbox, sizeof(ora_fab));
+    R->cfp = (ora_fab *)calloc(C->nbox, sizeof(ora_fab));
+    for (int b = 0; b < C->nbox; ++b) {
+        fab_alloc_box(&R->crse[b], C->box[b], O_NVAR);
+        fab_alloc_box(&R->cfp[b], C->box[b], O_NVAR);
+    }
+    C->reg_crse = R;
+    F->reg_fine = R;
+    return R;
+}
+
+/* YAFluxRegister::CrseAdd over the tile bx of coarse box b */
+static void fluxreg_crse_add(ora_fluxreg *R, int b, obox bx, ora_fab flux[3], double dt) {
+    const double *dx = R->C->P.dx;
+    double dx1 = dx[0];
+    dx1 *= dx[1];
+    dx1 *= dx[2];
+    const double dtdx = dt / dx1;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+    for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+    for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+        const int c[3] = {i, j, k};
+        if (fr_flag(R, c) != FR_BND) continue;
+        for (int d = 0; d < 3; ++d) {
+            int cm[3] = {i, j, k}, cp[3] = {i, j, k};
+            cm[d] -= 1;
+            cp[d] += 1;
+            if (fr_flag(R, cm) == FR_FINE) {
+                for (int n = 0; n < O_NVAR; ++n)
+                    FA(&R->crse[b], i, j, k, n) -= dtdx * FA(&flux[d], i, j, k, n);
+            } else if (fr_flag(R, cp) == FR_FINE) {
+                for (int n = 0; n < O_NVAR; ++n)
+                    FA(&R->crse[b], i, j, k, n) += dtdx * FV(&flux[d], cp, n);
+            }
+        }
+    }
+}
+
+/* YAFluxRegister::FineAdd over the fine tile fbx (aligned to the ratio) */
+static void fluxreg_fine_add(ora_fluxreg *R, obox fbx, ora_fab flux[3], double dt) {
+    const ora_level *C = R->C;
+    const int r = R->ratio;
+    const double *dx = R->F->P.dx;
+    double dx1 = dx[0];
+    dx1 *= dx[1];
+    dx1 *= dx[2];
+    const double dtdx = dt / (dx1 * (double)(r * r * r));
+    obox cb;
+    for (int d = 0; d < 3; ++d) { cb.lo[d] = fbx.lo[d] / r; cb.hi[d] = fbx.hi[d] / r; }
+    for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+            obox sl = cb;
+            sl.lo[d] = sl.hi[d] = side ? cb.hi[d] + 1 : cb.lo[d] - 1;
+            const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2; /* inner, outer */
+            for (int k = sl.lo[2]; k <= sl.hi[2]; ++k)
+            for (int j = sl.lo[1]; j <= sl.hi[1]; ++j)
+            for (int i = sl.lo[0]; i <= sl.hi[0]; ++i) {
+                const int c[3] = {i, j, k};
+                int m[3];
+                if (!fr_map(C, c, m) || R->flag[dom_index(C, m)] != FR_BND) continue;
+                /* the coarse box holding the cell */
+                const int cbx = C->owner[dom_index(C, m)];
+                if (cbx < 0) continue;
+                for (int n = 0; n < O_NVAR; ++n) {
+                    double *dp = &FA(&R->cfp[cbx], m[0], m[1], m[2], n);
+                    for (int o1 = 0; o1 < r; ++o1)
+                        for (int o0 = 0; o0 < r; ++o0) {
+                            int f[3];
+                            f[d] = side ? c[d] * r : (c[d] + 1) * r;
+                            f[t1] = c[t1] * r + o1;
+                            f[t0] = c[t0] * r + o0;
+                            const double tmp = side ? dtdx * FV(&flux[d], f, n)
+                                                    : -dtdx * FV(&flux[d], f, n);
+                            *dp += tmp;
+                        }
+                }
+            }
+        }
+}
+
+/* average_down of fine level F onto coarse level C (non-EB, Cartesian) */
+int ora_level_average_down(ora_level *C, const ora_level *F, int ratio) {
+    for (int d = 0; d < 3; ++d)
+        if (F->P.domlo[d] != C->P.domlo[d] * ratio ||
+            F->P.domhi[d] + 1 != (C->P.domhi[d] + 1) * ratio) return 1;
+    const double volfrac = 1.0 / (double)(ratio * ratio * ratio);
+    for (int b = 0; b < C->nbox; ++b) {
+        const obox B = C->box[b];
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            const int f0[3] = {i * ratio, j * ratio, k * ratio};
+            const int fo = F->owner[dom_index(F, f0)];
+            if (fo < 0) continue; /* not covered (fine boxes are aligned to the ratio) */
+            for (int n = 0; n < O_NVAR; ++n) {
+                double c = 0.0;
+                for (int kr = 0; kr < ratio; ++kr)
+                for (int jr = 0; jr < ratio; ++jr)
+                for (int ir = 0; ir < ratio; ++ir) {
+                    const int fc[3] = {f0[0] + ir, f0[1] + jr, f0[2] + kr};
+                    c += FV(&F->Snew[F->owner[dom_index(F, fc)]], fc, n);
+                }
+                FA(&C->Snew[b], i, j, k, n) = volfrac * c;
+            }
+        }
+    }
+    return 0;
+}
+
+/* One coarse step of a two-level hierarchy with subcycling (Amr::timeStep
+ * as CAMR drives it): reset the register and advance level 0 (CAMR_advance,
+ * CrseAdd), then `ratio` steps of level 1 whose ghost data come from level
+ * 0 interpolated in time (StateData::InterpAddBox: old / new within
+ * (t_new - t_old) 1e-3 of either end, else alpha old + beta new) and in
+ * space (FillPatch), with FineAdd; then level 0's post_timestep (CAMR.cpp:
+ * 679-699): reflux, avgDown, clean_state. */
+int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
+    if (!R || R->C != C || R->F != F) return 1;
+    const int r = R->ratio;
+    for (int b = 0; b < C->nbox; ++b) {
+        memset(R->crse[b].p, 0, sizeof(double) * (size_t)(R->crse[b].ns * O_NVAR));
+        memset(R->cfp[b].p, 0, sizeof(double) * (size_t)(R->cfp[b].ns * O_NVAR));
+    }
+    const double t_old = C->time;
+    C->defer_post = 1;
+    int err = ora_level_advance(C, dtc);
+    C->defer_post = 0;
+    if (err) return err;
+    const double t_new = C->time;
+    const double dtf = dtc / r;
+    const double teps = (t_new - t_old) * 1.e-3;
+    for (int s = 0; s < r; ++s) {
+        const double t = F->time;
+        otimew tw = {2, (t_new - t) / (t_new - t_old), (t - t_old) / (t_new - t_old)};
+        if (t > t_old - teps && t < t_old + teps) tw.mode = 1;
+        else if (t > t_new - teps && t < t_new + teps) tw.mode = 0;
+        if (exterior_from_coarse_t(F, C, r, tw)) return 1;
+        err = ora_level_advance(F, dtf);
+        if (err) return err;
+    }
+    for (int b = 0; b < C->nbox; ++b) {
+        const obox B = C->box[b];
+        for (int n = 0; n < O_NVAR; ++n)
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+            double *d = &FA(&R->crse[b], i, j, k, n);
+            *d += FA(&R->cfp[b], i, j, k, n);
+            FA(&C->Snew[b], i, j, k, n) += *d;
+        }
+    }
+    ora_level_average_down(C, F, r);
+    clean_state(C, C->Snew, 0);
+    return C->err;
 }

@@ -151,6 +151,13 @@ void   ora_level_set_box_eb(ora_level *L, int b, const double *vfrac,
  * cell of F's domain.  Returns 1 if the domains do not nest. */
 int    ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio);
 void   ora_level_get_exterior(const ora_level *L, double *out);
+
+/* ---- coarse-fine synchronisation (flux register, reflux, average_down) -- */
+typedef struct ora_fluxreg ora_fluxreg;
+ora_fluxreg *ora_fluxreg_create(ora_level *C, ora_level *F, int ratio);
+void   ora_fluxreg_destroy(ora_fluxreg *R);
+int    ora_level_average_down(ora_level *C, const ora_level *F, int ratio);
+int    ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dt_coarse);
 /* Sum of each component over valid cells (for conservation checks). */
 void   ora_level_sums(const ora_level *L, double sums[O_NVAR]);
 

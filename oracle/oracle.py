@@ -116,6 +116,13 @@ def lib():
         L.ora_level_exterior_from_coarse.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.ora_level_exterior_from_coarse.restype = C.c_int
         L.ora_level_get_exterior.argtypes = [C.c_void_p, dp]
+        L.ora_fluxreg_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.ora_fluxreg_create.restype = C.c_void_p
+        L.ora_fluxreg_destroy.argtypes = [C.c_void_p]
+        L.ora_level_average_down.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.ora_level_average_down.restype = C.c_int
+        L.ora_amr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
+        L.ora_amr_step.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_get_threads.restype = C.c_int
         _lib = L
@@ -269,6 +276,37 @@ class Level:
     @property
     def time(self):
         return lib().ora_level_time(self._h)
+
+
+class FluxRegister:
+    """Flux register between coarse level C and fine level F (YAFluxRegister
+    restated); attaches itself to both levels."""
+
+    def __init__(self, C: Level, F: Level, ratio: int):
+        self.C, self.F, self.ratio = C, F, ratio
+        self._h = lib().ora_fluxreg_create(C._h, F._h, ratio)
+        if not self._h:
+            raise ValueError("flux register: unsupported level pair")
+
+    def close(self):
+        if self._h:
+            lib().ora_fluxreg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def average_down(C: Level, F: Level, ratio: int):
+    assert lib().ora_level_average_down(C._h, F._h, ratio) == 0
+
+
+def amr_step(C: Level, F: Level, R: FluxRegister, dt: float) -> int:
+    """One subcycled two-level step of coarse size dt (fine steps dt/ratio)."""
+    return lib().ora_amr_step(C._h, F._h, R._h, dt)
 
 
 def set_threads(n: int):

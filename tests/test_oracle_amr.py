@@ -1,0 +1,180 @@
+"""The oracle's coarse-fine synchronisation: the flux register (AMReX's
+YAFluxRegister as CAMR uses it, CAMR.H:310, CrseAdd / FineAdd at
+CAMR_construct_hydro_source.cpp:243-254, Reflux at CAMR.cpp:803-851),
+average_down (CAMR.cpp:1047-1066) and the two-level subcycled step
+(Amr::timeStep with CAMR::post_timestep, CAMR.cpp:679-699).  AMReX is
+absent, so the mathematics pins them:
+
+* the composite state (uncovered coarse cells + fine cells) conserves
+  mass, momentum and energy to rounding on a periodic domain, for Godunov
+  PPM and PLM, the method of lines, and ratios 2 and 4 -- a flux register
+  with a wrong sign, factor or face fails this by orders of magnitude;
+* average_down sets every covered coarse cell to the mean of its children;
+* a uniform flow stays uniform through composite steps (the fine level bit
+  for bit, the refluxed coarse cells to rounding);
+* a layout where a coarse cell borders two fine boxes is refused.
+"""
+import numpy as np
+import pytest
+
+from camr_amd import problems
+
+G = 1.4
+NV = 10
+
+
+def _params(O, n, periodic=True, **kw):
+    bc = 0 if periodic else 4
+    per = (1, 1, 1) if periodic else (0, 0, 0)
+    kw.setdefault("cfl", 0.3)
+    return O.params(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(1.0 / n,) * 3, bclo=(bc,) * 3,
+                    bchi=(bc,) * 3, is_periodic=per, **kw)
+
+
+def _field(n, lo, hi, blast=True):
+    U = problems.SmoothField(kmax=3, mach=0.3).state(n, lo, hi)
+    if blast:
+        x = [(np.arange(lo[d], hi[d] + 1) + 0.5) / n for d in range(3)]
+        Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+        r2 = (X - 0.27) ** 2 + (Y - 0.5) ** 2 + (Z - 0.45) ** 2
+        p = 1.0 + 4.0 * np.exp(-r2 / 0.004)
+        ke = U[4] - U[5]
+        U[5] = p / (G - 1)
+        U[4] = U[5] + ke
+    return np.ascontiguousarray(U)
+
+
+def _boxes(nc, mg):
+    return [(i, j, k, min(i + mg, nc) - 1, min(j + mg, nc) - 1, min(k + mg, nc) - 1)
+            for k in range(0, nc, mg) for j in range(0, nc, mg) for i in range(0, nc, mg)]
+
+
+FINE2 = [(8, 8, 8, 15, 23, 23), (16, 8, 8, 23, 23, 23)]  # coarse [4, 11]^3 at ratio 2
+
+
+def hierarchy(O, nc=16, ratio=2, fine=None, periodic=True, blast=True, **kw):
+    """Coarse level nc^3 in 8^3 boxes, fine level over `fine` boxes at
+    `ratio`, both from the same continuous field, coarse covered cells set
+    by average_down."""
+    fine = fine or FINE2
+    C = O.Level(_params(O, nc, periodic, **kw), _boxes(nc, 8))
+    F = O.Level(_params(O, nc * ratio, periodic, **kw), fine)
+    for b, bb in enumerate(C.boxes):
+        C.set_box(b, _field(nc, bb[:3], bb[3:], blast))
+    for b, bb in enumerate(F.boxes):
+        F.set_box(b, _field(nc * ratio, bb[:3], bb[3:], blast))
+    R = O.FluxRegister(C, F, ratio)
+    O.average_down(C, F, ratio)
+    C.init_post()
+    F.init_post()
+    return C, F, R
+
+
+def composite_sums(C, F, ratio):
+    """Sum of each component over the composite grid in fine-cell volumes."""
+    nf = F.p.domhi[0] + 1
+    cov = np.zeros((nf // ratio,) * 3, dtype=bool)
+    tot = np.zeros(NV)
+    for b, bb in enumerate(F.boxes):
+        tot += F.get_box(b).reshape(NV, -1).sum(axis=1)
+        cov[bb[2] // ratio:(bb[5] + 1) // ratio, bb[1] // ratio:(bb[4] + 1) // ratio,
+            bb[0] // ratio:(bb[3] + 1) // ratio] = True
+    for b, bb in enumerate(C.boxes):
+        S = C.get_box(b)
+        m = ~cov[bb[2]:bb[5] + 1, bb[1]:bb[4] + 1, bb[0]:bb[3] + 1]
+        tot += ratio ** 3 * S[:, m].sum(axis=1)
+    return tot
+
+
+def run(O, C, F, R, ratio, nsteps):
+    dt = None
+    for s in range(nsteps):
+        if s == 0:
+            dt = min(C.initial_dt(), ratio * F.initial_dt())
+        else:
+            dt = min(C.new_dt(dt), ratio * F.new_dt(dt / ratio))
+        assert O.amr_step(C, F, R, dt) == 0
+    return dt
+
+
+@pytest.mark.parametrize("ratio,kw", [(2, dict(ppm_type=1)), (2, dict(ppm_type=0)),
+                                      (2, dict(do_mol=1)), (4, dict(ppm_type=1))])
+def test_composite_conservation(oracle, ratio, kw):
+    fine = FINE2 if ratio == 2 else [(16, 16, 16, 31, 47, 47), (32, 16, 16, 47, 47, 47)]
+    C, F, R = hierarchy(oracle, ratio=ratio, fine=fine, **kw)
+    s0 = composite_sums(C, F, ratio)
+    S0 = [C.get_box(b) for b in range(len(C.boxes))]
+    run(oracle, C, F, R, ratio, 3)
+    s1 = composite_sums(C, F, ratio)
+    scale = np.abs(np.concatenate([F.get_box(b).reshape(NV, -1) for b in range(2)], axis=1)).sum(axis=1)
+    for c in (0, 1, 2, 3, 4, 7, 8, 9):
+        assert abs(s1[c] - s0[c]) <= 1e-12 * max(scale[c], 1.0), (c, s1[c] - s0[c], scale[c])
+    # the fine level times follow the coarse one, and the coarse state moved
+    assert abs(C.time - F.time) <= 1e-15 * C.time
+    assert max(np.abs(C.get_box(b) - S0[b]).max() for b in range(len(S0))) > 1e-3
+
+
+def test_average_down_is_the_mean_of_the_children(oracle):
+    C, F, R = hierarchy(oracle)
+    run(oracle, C, F, R, 2, 1)
+    nf = 32
+    fine = np.zeros((NV, nf, nf, nf))
+    for b, bb in enumerate(F.boxes):
+        fine[:, bb[2]:bb[5] + 1, bb[1]:bb[4] + 1, bb[0]:bb[3] + 1] = F.get_box(b)
+    want = fine.reshape(NV, nf // 2, 2, nf // 2, 2, nf // 2, 2).mean(axis=(2, 4, 6))
+    scale = np.abs(want).reshape(NV, -1).max(axis=1)
+    seen = 0
+    for b, bb in enumerate(C.boxes):
+        S = C.get_box(b)
+        for k in range(bb[2], bb[5] + 1):
+            for j in range(bb[1], bb[4] + 1):
+                for i in range(bb[0], bb[3] + 1):
+                    if 4 <= i <= 11 and 4 <= j <= 11 and 4 <= k <= 11:
+                        got = S[:, k - bb[2], j - bb[1], i - bb[0]]
+                        # post_timestep's clean_state re-derives rho e and T and
+                        # renormalises species; rho, rho u and rho E are the averages
+                        for c in (0, 1, 2, 3, 4):
+                            assert abs(got[c] - want[c, k, j, i]) <= 1e-14 * scale[c]
+                        seen += 1
+    assert seen == 8 ** 3
+
+
+def test_uniform_flow_stays_uniform(oracle):
+    C = oracle.Level(_params(oracle, 16), _boxes(16, 8))
+    F = oracle.Level(_params(oracle, 32), FINE2)
+    rho, u = 1.3, (0.4, -0.2, 0.1)
+
+    def uni(shape):
+        U = np.zeros(shape)
+        U[0] = rho
+        for d in range(3):
+            U[1 + d] = rho * u[d]
+        U[5] = 2.0
+        U[4] = 2.0 + 0.5 * rho * sum(v * v for v in u)
+        U[7] = rho
+        return U
+
+    for L in (C, F):
+        for b in range(len(L.boxes)):
+            L.set_box(b, uni(L.shape(b)))
+    R = oracle.FluxRegister(C, F, 2)
+    C.init_post()
+    F.init_post()
+    S0c = [C.get_box(b) for b in range(len(C.boxes))]
+    S0f = [F.get_box(b) for b in range(len(F.boxes))]
+    run(oracle, C, F, R, 2, 2)
+    # the fine level bit for bit; on the coarse level the register's
+    # -dt/vol_c F_c + sum dt_f/vol_c F_f cancels to rounding
+    for b in range(len(F.boxes)):
+        assert np.array_equal(F.get_box(b), S0f[b])
+    for b in range(len(C.boxes)):
+        d = np.abs(C.get_box(b) - S0c[b]).reshape(NV, -1).max(axis=1)
+        assert np.all(d <= 4e-16 * np.abs(S0c[b]).reshape(NV, -1).max(axis=1) + 1e-16), d
+
+
+def test_coarse_cell_bordering_two_fine_boxes_is_refused(oracle):
+    # an L of two fine boxes: coarse cell (12, 12, *) borders both
+    C = oracle.Level(_params(oracle, 16), _boxes(16, 8))
+    F = oracle.Level(_params(oracle, 32), [(8, 8, 8, 23, 23, 23), (24, 8, 8, 31, 15, 23)])
+    with pytest.raises(ValueError):
+        oracle.FluxRegister(C, F, 2)
