@@ -86,6 +86,8 @@ EXPORTS = [
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
+    "camr_flux_register_create", "camr_flux_register_destroy", "camr_level_average_down",
+    "camr_amr_step",
 ]
 
 _lib = None
@@ -150,6 +152,10 @@ def lib():
         i32p = C.POINTER(C.c_int32)
         L.camr_level_set_eb.argtypes = [vp, i32p, i32p, i32p]
         L.camr_level_exterior_from_coarse.argtypes = [vp, vp, C.c_int32]
+        L.camr_flux_register_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
+        L.camr_flux_register_destroy.argtypes = [vp]
+        L.camr_level_average_down.argtypes = [vp, vp, C.c_int32]
+        L.camr_amr_step.argtypes = [vp, vp, vp, C.c_double]
         L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,
                                           C.POINTER(EbGeom), C.c_double,
                                           C.POINTER(HydroParams), i32p, vp]

@@ -125,6 +125,7 @@ static bool covers(const camr_array4 &a, const Box &b, int nc) {
 }
 
 // ============================================================ level driver
+struct FluxReg;
 class HydroLevel {
   public:
     explicit HydroLevel(const camr_level_desc &d) : desc_(d) {}
@@ -218,7 +219,9 @@ class HydroLevel {
     // with the step's dt in dt_dev_
     hipGraph_t sgraph_[2] = {nullptr, nullptr};
     hipGraphExec_t sgexec_[2] = {nullptr, nullptr};
-    bool use_staged_graph() const { return desc_.graph != 0 && !fused_ && !grav_src_values_; }
+    bool use_staged_graph() const {
+        return desc_.graph != 0 && !fused_ && !grav_src_values_ && !reg_crse_ && !reg_fine_;
+    }
     int replay_staged(double dt);
     int clean_all(const DevFab *tab, bool launch_detect);
     template <typename K, typename... A>
@@ -268,10 +271,43 @@ class HydroLevel {
         return desc_.graph != 0 && fused_ && batched_ && !grav_src_values_;
     }
     int set_exterior(const double *h);
-    int exterior_from(HydroLevel &coarse, int ratio);
+    int exterior_from(HydroLevel &coarse, int ratio, int tmode = 0, double ta = 0.0,
+                      double tb = 0.0);
     int set_tuning(const Tuning &t);
     int get_tuning(Tuning *t) const;
     bool fused() const { return fused_; }
+
+  private:
+    // coarse-fine synchronisation: the register this level feeds as the
+    // coarse level (CrseAdd) and as the fine level (FineAdd)
+    FluxReg *reg_crse_ = nullptr, *reg_fine_ = nullptr;
+    bool defer_post_ = false;  // leave post_timestep's clean_state to amr_step
+    void reg_add(size_t b, const Arr *flx, hipStream_t s);
+    friend struct FluxReg;
+    friend int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out);
+    friend int average_down(HydroLevel &C, HydroLevel &F, int ratio);
+    friend int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc);
+};
+
+// The flux register between a coarse and a fine level (see the
+// coarse-fine synchronisation section below).
+struct FluxReg {
+    HydroLevel *C = nullptr, *F = nullptr;
+    int ratio = 2;
+    uint8_t *flag = nullptr;  // device, per coarse domain cell
+    std::vector<double *> crse, cfp;
+    DevFab *tab_crse = nullptr, *tab_cfp = nullptr;
+    double dx1c = 0.0, dx1f = 0.0;
+    ~FluxReg() {
+        if (C && C->reg_crse_ == this) C->reg_crse_ = nullptr;
+        if (F && F->reg_fine_ == this) F->reg_fine_ = nullptr;
+        auto fr = [](void *p) { if (p) (void)hipFree(p); };
+        for (double *p : crse) fr(p);
+        for (double *p : cfp) fr(p);
+        fr(flag);
+        fr(tab_crse);
+        fr(tab_cfp);
+    }
 };
 
 void HydroLevel::set_dt_dev(double dt) {
@@ -300,7 +336,7 @@ int HydroLevel::set_exterior(const double *h) {
 // Exterior data from the coarser level: FillPatch's coarse-fine
 // interpolation (k_ext_from_coarse) at every cell of this level's domain,
 // from the coarse level's current S_new.
-int HydroLevel::exterior_from(HydroLevel &C, int ratio) {
+int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, double tb) {
     if (ratio < 2) { set_error("refinement ratio must be at least 2"); return CAMR_ERR_PARAM; }
     for (int d = 0; d < 3; ++d)
         if (desc_.hp.domlo[d] != C.desc_.hp.domlo[d] * ratio ||
@@ -320,8 +356,9 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio) {
     }
     extv_ = dense(ext_, dom);
     const DevFab *ct = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
+    const DevFab *co = C.a_is_new_ ? C.tab_b_ : C.tab_a_;
     launch_on(k_ext_from_coarse, dom, stream_, extv_, ct, (const int *)C.owner_, C.dom_, ratio,
-              dom.lo[0], dom.lo[1], dom.lo[2]);
+              dom.lo[0], dom.lo[1], dom.lo[2], co, tmode, ta, tb);
     CAMR_HIP(hipGetLastError());
     // the coarse level may step next: finish reading its state first
     CAMR_HIP(hipStreamSynchronize(stream_));
@@ -343,6 +380,10 @@ int HydroLevel::get_tuning(Tuning *t) const {
 }
 
 void HydroLevel::release() {
+    // a register outliving its levels no longer reaches them
+    if (reg_crse_) reg_crse_->C = nullptr;
+    if (reg_fine_) reg_fine_->F = nullptr;
+    reg_crse_ = reg_fine_ = nullptr;
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     for (auto p : bufA_) fr(p);
     for (auto p : bufB_) fr(p);
@@ -936,6 +977,8 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
             int rc = desc_.hp.do_mol ? umdrv_mol(A, wscr, bs, &prof_) : umdrv_staged(A, wscr, bs, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
+            // do_reflux: the box's fluxes into the flux registers (:243-254)
+            if (reg_crse_ || reg_fine_) reg_add(b, A.flx, bs);
         }
     }
     if (add)
@@ -1063,8 +1106,9 @@ int HydroLevel::advance_staged(double dt) {
                    hc_.small_dens, status_, ebf_);
         clean_all(tnew, false);
     }
-    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
-    clean_all(tnew, true);
+    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699); the coarse
+    // level of a two-level step runs it after reflux and avgDown
+    if (!defer_post_) clean_all(tnew, true);
     return CAMR_OK;
 }
 
@@ -1638,12 +1682,257 @@ int HydroLevel::new_dt(double dt_old, double *dt) {
 
 } // namespace camr
 
+// ============================================ coarse-fine synchronisation
+// The flux register between two levels (AMReX's YAFluxRegister as CAMR uses
+// it, CAMR.H:310), average_down (CAMR.cpp:1047-1066) and the two-level
+// subcycled step (Amr::timeStep with CAMR::post_timestep, CAMR.cpp:
+// 679-699).  Coarse boundary cells that border two different fine boxes
+// are not supported (AMReX weighs its overlapping coarse-fine patches
+// there); the register refuses such layouts.
+namespace camr {
+
+static double cell_volume(const double *dx) {
+    double v = dx[0];
+    v *= dx[1];
+    v *= dx[2];
+    return v;
+}
+
+// periodic image of a coarse index; false outside a non-periodic domain
+static bool host_reg_map(const DomInfo &D, const int *c, int *m) {
+    for (int d = 0; d < 3; ++d) {
+        const int lo = D.domlo[d], n = D.domhi[d] - lo + 1;
+        int x = c[d];
+        if (x < lo || x > lo + n - 1) {
+            if (!D.periodic[d]) return false;
+            x = lo + ((x - lo) % n + n) % n;
+        }
+        m[d] = x;
+    }
+    return true;
+}
+
+static int check_refines(const camr_level_desc &f, const camr_level_desc &c, int ratio) {
+    if (ratio < 2) { set_error("refinement ratio must be at least 2"); return CAMR_ERR_PARAM; }
+    for (int d = 0; d < 3; ++d)
+        if (f.hp.domlo[d] != c.hp.domlo[d] * ratio ||
+            f.hp.domhi[d] + 1 != (c.hp.domhi[d] + 1) * ratio) {
+            set_error("the fine domain is not the coarse domain refined by %d", ratio);
+            return CAMR_ERR_SHAPE;
+        }
+    return CAMR_OK;
+}
+
+int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out) {
+    int rc = check_refines(F.desc_, C.desc_, ratio);
+    if (rc) return rc;
+    if (C.fused_ || F.fused_) { set_error("the flux register needs the staged engine on both levels"); return CAMR_ERR_PARAM; }
+    if (C.ebflags_ || F.ebflags_) { set_error("flux registers with embedded boundaries are not built"); return CAMR_ERR_PARAM; }
+    if (C.reg_crse_ || F.reg_fine_) { set_error("a flux register is already attached"); return CAMR_ERR_PARAM; }
+    if (C.desc_.hp.do_mol != F.desc_.hp.do_mol) { set_error("both levels need the same integrator"); return CAMR_ERR_PARAM; }
+    for (const Box &B : F.box_)
+        for (int d = 0; d < 3; ++d)
+            if (B.lo[d] % ratio || (B.hi[d] + 1) % ratio) {
+                set_error("fine boxes must be aligned to the refinement ratio");
+                return CAMR_ERR_SHAPE;
+            }
+    const DomInfo &D = C.dom_;
+    const int64_t n0 = D.domhi[0] - D.domlo[0] + 1, n1 = D.domhi[1] - D.domlo[1] + 1,
+                  n2 = D.domhi[2] - D.domlo[2] + 1;
+    auto idx = [&](const int *m) {
+        return (int64_t)(m[0] - D.domlo[0]) + (int64_t)(m[1] - D.domlo[1]) * n0 +
+               (int64_t)(m[2] - D.domlo[2]) * n0 * n1;
+    };
+    std::vector<uint8_t> flag((size_t)(n0 * n1 * n2), 0);
+    std::vector<int> cov(flag.size(), -1);
+    for (size_t fb = 0; fb < F.box_.size(); ++fb) {
+        const Box &B = F.box_[fb];
+        for (int k = B.lo[2] / ratio; k <= B.hi[2] / ratio; ++k)
+            for (int j = B.lo[1] / ratio; j <= B.hi[1] / ratio; ++j)
+                for (int i = B.lo[0] / ratio; i <= B.hi[0] / ratio; ++i) {
+                    const int c[3] = {i, j, k};
+                    flag[idx(c)] = kRegFine;
+                    cov[idx(c)] = (int)fb;
+                }
+    }
+    for (const Box &B : F.box_)
+        for (int k = B.lo[2] / ratio - 1; k <= B.hi[2] / ratio + 1; ++k)
+            for (int j = B.lo[1] / ratio - 1; j <= B.hi[1] / ratio + 1; ++j)
+                for (int i = B.lo[0] / ratio - 1; i <= B.hi[0] / ratio + 1; ++i) {
+                    const int c[3] = {i, j, k};
+                    int m[3];
+                    if (host_reg_map(D, c, m) && flag[idx(m)] != kRegFine) flag[idx(m)] = kRegBnd;
+                }
+    for (int k = D.domlo[2]; k <= D.domhi[2]; ++k)
+        for (int j = D.domlo[1]; j <= D.domhi[1]; ++j)
+            for (int i = D.domlo[0]; i <= D.domhi[0]; ++i) {
+                const int c[3] = {i, j, k};
+                if (flag[idx(c)] != kRegBnd) continue;
+                int seen = -1;
+                for (int d = 0; d < 3; ++d)
+                    for (int sg = -1; sg <= 1; sg += 2) {
+                        int nb[3] = {i, j, k}, m[3];
+                        nb[d] += sg;
+                        if (!host_reg_map(D, nb, m)) continue;
+                        const int fb = cov[idx(m)];
+                        if (fb < 0) continue;
+                        if (seen >= 0 && fb != seen) {
+                            set_error("coarse cell (%d,%d,%d) borders two fine boxes", i, j, k);
+                            return CAMR_ERR_SHAPE;
+                        }
+                        seen = fb;
+                    }
+            }
+    if ((rc = C.resolve()) || (rc = F.resolve())) return rc;
+    CAMR_HIP(hipStreamSynchronize(C.stream_));
+    CAMR_HIP(hipStreamSynchronize(F.stream_));
+    std::unique_ptr<FluxReg> R(new FluxReg());
+    R->C = &C;
+    R->F = &F;
+    R->ratio = ratio;
+    R->dx1c = cell_volume(C.desc_.hp.dx);
+    R->dx1f = cell_volume(F.desc_.hp.dx);
+    CAMR_HIP(hipMalloc(&R->flag, flag.size()));
+    CAMR_HIP(hipMemcpy(R->flag, flag.data(), flag.size(), hipMemcpyHostToDevice));
+    const size_t nb = C.box_.size();
+    std::vector<DevFab> tc(nb), tf(nb);
+    R->crse.assign(nb, nullptr);
+    R->cfp.assign(nb, nullptr);
+    for (size_t b = 0; b < nb; ++b) {
+        const size_t n = (size_t)NVAR * C.box_[b].npts() * sizeof(double);
+        CAMR_HIP(hipMalloc(&R->crse[b], n));
+        CAMR_HIP(hipMalloc(&R->cfp[b], n));
+        CAMR_HIP(hipMemset(R->crse[b], 0, n));
+        CAMR_HIP(hipMemset(R->cfp[b], 0, n));
+        tc[b].a = dense(R->crse[b], C.box_[b]);
+        tc[b].b = C.box_[b];
+        tf[b].a = dense(R->cfp[b], C.box_[b]);
+        tf[b].b = C.box_[b];
+    }
+    CAMR_HIP(hipMalloc(&R->tab_crse, nb * sizeof(DevFab)));
+    CAMR_HIP(hipMalloc(&R->tab_cfp, nb * sizeof(DevFab)));
+    CAMR_HIP(hipMemcpy(R->tab_crse, tc.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    CAMR_HIP(hipMemcpy(R->tab_cfp, tf.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    C.release_graphs();
+    F.release_graphs();
+    C.reg_crse_ = R.get();
+    F.reg_fine_ = R.get();
+    *out = R.release();
+    return CAMR_OK;
+}
+
+// CrseAdd / FineAdd of box b's fluxes, on the stream that computed them,
+// with fac_for_reflux * dt (CAMR_construct_hydro_source.cpp:24)
+void HydroLevel::reg_add(size_t b, const Arr *flx, hipStream_t s) {
+    const double fac = desc_.hp.do_mol ? 0.5 : 1.0;
+    if (reg_crse_) {
+        FluxReg &R = *reg_crse_;
+        launch_on(k_reg_crse_add, box_[b], s, dense(R.crse[b], box_[b]), flx[0], flx[1], flx[2],
+                  (const uint8_t *)R.flag, dom_, (const double *)dt_dev_, fac, R.dx1c);
+    }
+    if (reg_fine_) {
+        FluxReg &R = *reg_fine_;
+        const int r = R.ratio;
+        Box cb = box_[b];
+        for (int d = 0; d < 3; ++d) { cb.lo[d] /= r; cb.hi[d] = (cb.hi[d] + 1) / r - 1; }
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                Box sl = cb;
+                sl.lo[d] = sl.hi[d] = side ? cb.hi[d] + 1 : cb.lo[d] - 1;
+                if (d == 0)
+                    launch_on(k_reg_fine_add<0>, sl, s, (const DevFab *)R.tab_cfp, (const int *)R.C->owner_,
+                              R.C->dom_, (const uint8_t *)R.flag, flx[0], side, r,
+                              (const double *)dt_dev_, fac, R.dx1f);
+                else if (d == 1)
+                    launch_on(k_reg_fine_add<1>, sl, s, (const DevFab *)R.tab_cfp, (const int *)R.C->owner_,
+                              R.C->dom_, (const uint8_t *)R.flag, flx[1], side, r,
+                              (const double *)dt_dev_, fac, R.dx1f);
+                else
+                    launch_on(k_reg_fine_add<2>, sl, s, (const DevFab *)R.tab_cfp, (const int *)R.C->owner_,
+                              R.C->dom_, (const uint8_t *)R.flag, flx[2], side, r,
+                              (const double *)dt_dev_, fac, R.dx1f);
+            }
+    }
+}
+
+// average_down of the fine level's S_new onto the coarse level's
+int average_down(HydroLevel &C, HydroLevel &F, int ratio) {
+    int rc = check_refines(F.desc_, C.desc_, ratio);
+    if (rc) return rc;
+    for (const Box &B : F.box_)
+        for (int d = 0; d < 3; ++d)
+            if (B.lo[d] % ratio || (B.hi[d] + 1) % ratio) {
+                set_error("fine boxes must be aligned to the refinement ratio");
+                return CAMR_ERR_SHAPE;
+            }
+    if ((rc = C.resolve()) || (rc = F.resolve())) return rc;
+    CAMR_HIP(hipStreamSynchronize(F.stream_));
+    const DevFab *ctn = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
+    const DevFab *ftn = F.a_is_new_ ? F.tab_a_ : F.tab_b_;
+    C.launch_all(kb_avgdown, 0, ctn, ftn, (const int *)F.owner_, F.dom_, ratio);
+    C.dt_ready_ = false;
+    CAMR_HIP(hipGetLastError());
+    return CAMR_OK;
+}
+
+// One coarse step of the two-level hierarchy with subcycling: reset the
+// register, CAMR_advance on the coarse level (CrseAdd), `ratio` fine steps
+// with ghost data interpolated from the coarse level in time
+// (StateData::InterpAddBox: old / new within (t_new - t_old) 1e-3 of either
+// end, else alpha old + beta new) and space (FillPatch), FineAdd; then the
+// coarse post_timestep: reflux, avgDown, clean_state.
+int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc) {
+    if (R.C != &C || R.F != &F) { set_error("the flux register belongs to other levels"); return CAMR_ERR_PARAM; }
+    if (!(dtc > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    if (std::fabs(C.time_ - F.time_) > 1e-12 * std::fmax(1.0, std::fabs(C.time_))) {
+        set_error("the levels are at different times");
+        return CAMR_ERR_PARAM;
+    }
+    int rc = C.resolve();
+    if (rc) return rc;
+    // reset (CAMR_advance.cpp:64-66)
+    for (size_t b = 0; b < C.box_.size(); ++b) {
+        launch_on(k_fill_value, C.box_[b], C.stream_, dense(R.crse[b], C.box_[b]), (int)NVAR, 0.0);
+        launch_on(k_fill_value, C.box_[b], C.stream_, dense(R.cfp[b], C.box_[b]), (int)NVAR, 0.0);
+    }
+    const double t_old = C.time_;
+    C.defer_post_ = true;
+    rc = C.advance(dtc);
+    C.defer_post_ = false;
+    if (rc) return rc;
+    const double t_new = C.time_;
+    const double dtf = dtc / R.ratio;
+    const double teps = (t_new - t_old) * 1.e-3;
+    for (int s = 0; s < R.ratio; ++s) {
+        const double t = F.time_;
+        int tmode = 2;
+        if (t > t_old - teps && t < t_old + teps) tmode = 1;
+        else if (t > t_new - teps && t < t_new + teps) tmode = 0;
+        const double ta = (t_new - t) / (t_new - t_old), tb = (t - t_old) / (t_new - t_old);
+        if ((rc = F.exterior_from(C, R.ratio, tmode, ta, tb))) return rc;
+        if ((rc = F.advance(dtf))) return rc;
+    }
+    if ((rc = F.resolve())) return rc;
+    CAMR_HIP(hipStreamSynchronize(F.stream_));
+    // post_timestep of the coarse level (CAMR.cpp:685-699)
+    const DevFab *ctn = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
+    C.launch_all(kb_reflux, 0, ctn, (const DevFab *)R.tab_crse, (const DevFab *)R.tab_cfp);
+    if ((rc = average_down(C, F, R.ratio))) return rc;
+    C.clean_all(ctn, true);
+    CAMR_HIP(hipGetLastError());
+    return CAMR_OK;
+}
+} // namespace camr
+
 // ================================================================ C-ABI
 using namespace camr;
 
 struct camr_level {
     HydroLevel impl;
     explicit camr_level(const camr_level_desc &d) : impl(d) {}
+};
+struct camr_flux_register {
+    FluxReg *impl;
 };
 
 extern "C" {
@@ -1933,6 +2222,29 @@ int camr_level_set_exterior(camr_level *L, const double *host) {
 int camr_level_exterior_from_coarse(camr_level *fine, camr_level *coarse, int32_t ratio) {
     if (!fine || !coarse || fine == coarse) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
     return fine->impl.exterior_from(coarse->impl, ratio);
+}
+int camr_flux_register_create(camr_level *coarse, camr_level *fine, int32_t ratio,
+                              camr_flux_register **out) {
+    if (!coarse || !fine || coarse == fine || !out) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
+    FluxReg *R = nullptr;
+    int rc = flux_register_create(coarse->impl, fine->impl, ratio, &R);
+    if (rc) return rc;
+    *out = new camr_flux_register{R};
+    return CAMR_OK;
+}
+int camr_flux_register_destroy(camr_flux_register *r) {
+    if (!r) return CAMR_ERR_PARAM;
+    delete r->impl;
+    delete r;
+    return CAMR_OK;
+}
+int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio) {
+    if (!coarse || !fine || coarse == fine) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
+    return average_down(coarse->impl, fine->impl, ratio);
+}
+int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r, double dt_coarse) {
+    if (!coarse || !fine || coarse == fine || !r || !r->impl) { set_error("bad arguments"); return CAMR_ERR_PARAM; }
+    return amr_step(coarse->impl, fine->impl, *r->impl, dt_coarse);
 }
 int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
                       const int32_t fhi[3]) {

@@ -73,7 +73,19 @@ __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFa
                                const DevFab *dsdt, const double *dtp, double small_dens,
                                int *flags, EbFlags eb);
 __global__ void k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner,
-                                  DomInfo CD, int ratio, int flo0, int flo1, int flo2);
+                                  DomInfo CD, int ratio, int flo0, int flo1, int flo2,
+                                  const DevFab *otab, int tmode, double ta, double tb);
+// coarse-fine synchronisation (flux register, reflux, average_down)
+constexpr int kRegBnd = 1, kRegFine = 2;
+__global__ void k_reg_crse_add(Range r, Arr d, Arr f0, Arr f1, Arr f2, const uint8_t *flag,
+                               DomInfo CD, const double *dtp, double fac, double dx1);
+template <int D>
+__global__ void k_reg_fine_add(Range r, const DevFab *cfp, const int *cowner, DomInfo CD,
+                               const uint8_t *flag, Arr f, int side, int ratio,
+                               const double *dtp, double fac, double dx1f);
+__global__ void kb_reflux(const DevFab *snew, const DevFab *crse, const DevFab *cfp);
+__global__ void kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo FD,
+                           int ratio);
 inline Box make_box1() { Box b; for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = 0; } return b; }
 
 template <typename K, typename... A>

@@ -485,6 +485,131 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
         pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
+// ------------------------------------------- coarse-fine synchronisation
+// AMReX's YAFluxRegister as CAMR uses it (CAMR.H:310; CrseAdd / FineAdd at
+// CAMR_construct_hydro_source.cpp:243-254; Reflux at CAMR.cpp:803-851) and
+// average_down (CAMR.cpp:1047-1066), restated.  Coarse cells carry a flag
+// over the coarse domain: kRegFine (covered by the fine level), kRegBnd
+// (not covered, within one cell of a covered cell) or 0.
+__device__ __forceinline__ bool reg_map(const DomInfo &D, const int *c, int *m) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int lo = D.domlo[d], n = D.domhi[d] - lo + 1;
+        int x = c[d];
+        if (x < lo || x > lo + n - 1) {
+            if (!D.periodic[d]) return false;
+            x = lo + ((x - lo) % n + n) % n;
+        }
+        m[d] = x;
+    }
+    return true;
+}
+__device__ __forceinline__ int reg_flag(const DomInfo &D, const uint8_t *flag, const int *c) {
+    int m[3];
+    if (!reg_map(D, c, m)) return 0;
+    const int64_t n0 = D.domhi[0] - D.domlo[0] + 1, n1 = D.domhi[1] - D.domlo[1] + 1;
+    return flag[(int64_t)(m[0] - D.domlo[0]) + (int64_t)(m[1] - D.domlo[1]) * n0 +
+                (int64_t)(m[2] - D.domlo[2]) * n0 * n1];
+}
+
+// CrseAdd over the coarse box of `d`: a boundary cell takes -/+ dt/vol_c
+// times the coarse flux through each face it shares with a covered cell
+// (x, then y, then z).
+__global__ void __launch_bounds__(kBlock)
+k_reg_crse_add(Range r, Arr d, Arr f0, Arr f1, Arr f2, const uint8_t *flag, DomInfo CD,
+               const double *dtp, double fac, double dx1) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    if (reg_flag(CD, flag, c) != kRegBnd) return;
+    const double dtdx = (fac * *dtp) / dx1;
+    const Arr f[3] = {f0, f1, f2};
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) {
+        int cm[3] = {i, j, k}, cp[3] = {i, j, k};
+        cm[dd] -= 1;
+        cp[dd] += 1;
+        if (reg_flag(CD, flag, cm) == kRegFine) {
+            for (int n = 0; n < NVAR; ++n) d(i, j, k, n) -= dtdx * f[dd](i, j, k, n);
+        } else if (reg_flag(CD, flag, cp) == kRegFine) {
+            for (int n = 0; n < NVAR; ++n) d(i, j, k, n) += dtdx * f[dd](cp[0], cp[1], cp[2], n);
+        }
+    }
+}
+
+// FineAdd of one fine box's D-faces on one side: r is the slab of coarse
+// cells (unwrapped index) next to the coarsened box; each boundary cell
+// takes +/- dt_f/(vol_f r^3) times every fine flux on its face, one
+// addition per fine face, (outer, inner) = (k, j) / (k, i) / (j, i).
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_reg_fine_add(Range r, const DevFab *cfp, const int *cowner, DomInfo CD, const uint8_t *flag,
+               Arr f, int side, int ratio, const double *dtp, double fac, double dx1f) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    int m[3];
+    if (!reg_map(CD, c, m)) return;
+    if (reg_flag(CD, flag, c) != kRegBnd) return;
+    const int cb = owner_of(CD, cowner, m);
+    if (cb < 0) return;
+    const Arr d = cfp[cb].a;
+    const double dtdx = (fac * *dtp) / (dx1f * (double)(ratio * ratio * ratio));
+    constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;  // inner, outer
+    for (int n = 0; n < NVAR; ++n) {
+        double acc = d(m[0], m[1], m[2], n);
+        for (int o1 = 0; o1 < ratio; ++o1)
+            for (int o0 = 0; o0 < ratio; ++o0) {
+                int fi[3];
+                fi[D] = side ? c[D] * ratio : (c[D] + 1) * ratio;
+                fi[T1] = c[T1] * ratio + o1;
+                fi[T0] = c[T0] * ratio + o0;
+                const double v = f(fi[0], fi[1], fi[2], n);
+                acc += side ? dtdx * v : -dtdx * v;
+            }
+        d(m[0], m[1], m[2], n) = acc;
+    }
+}
+template __global__ void k_reg_fine_add<0>(Range, const DevFab *, const int *, DomInfo, const uint8_t *, Arr, int, int, const double *, double, double);
+template __global__ void k_reg_fine_add<1>(Range, const DevFab *, const int *, DomInfo, const uint8_t *, Arr, int, int, const double *, double, double);
+template __global__ void k_reg_fine_add<2>(Range, const DevFab *, const int *, DomInfo, const uint8_t *, Arr, int, int, const double *, double, double);
+
+// Reflux: crse_data += cfp, then S_new += crse_data, every valid cell.
+__global__ void __launch_bounds__(kBlock)
+kb_reflux(const DevFab *snew, const DevFab *crse, const DevFab *cfp) {
+    int b, i, j, k;
+    if (!batch_cell(snew, 0, b, i, j, k)) return;
+    const Arr S = snew[b].a, Cr = crse[b].a, Fp = cfp[b].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        const double dd = Cr(i, j, k, n) + Fp(i, j, k, n);
+        Cr(i, j, k, n) = dd;
+        S(i, j, k, n) += dd;
+    }
+}
+
+// average_down: a covered coarse cell = (1/r^3) * the sum of its children
+// in (k, j, i) order (fine boxes are aligned to the ratio).
+__global__ void __launch_bounds__(kBlock)
+kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo FD, int ratio) {
+    int b, i, j, k;
+    if (!batch_cell(csnew, 0, b, i, j, k)) return;
+    const int f0[3] = {i * ratio, j * ratio, k * ratio};
+    const int fo = owner_of(FD, fowner, f0);
+    if (fo < 0) return;
+    const double volfrac = 1.0 / (double)(ratio * ratio * ratio);
+    for (int n = 0; n < NVAR; ++n) {
+        double c = 0.0;
+        for (int kr = 0; kr < ratio; ++kr)
+            for (int jr = 0; jr < ratio; ++jr)
+                for (int ir = 0; ir < ratio; ++ir) {
+                    const int fc[3] = {f0[0] + ir, f0[1] + jr, f0[2] + kr};
+                    c += ftab[owner_of(FD, fowner, fc)].a(fc[0], fc[1], fc[2], n);
+                }
+        csnew[b].a(i, j, k, n) = volfrac * c;
+    }
+}
+
 // Second stage of the method of lines (CAMR_advance.cpp:133-134):
 // LinComb(S_new, 0.5, Sborder, 0.5, S_old), then Saxpy(S_new, 0.5 dt,
 // new_hydro_source), + floor detection for the following clean_state.
@@ -524,7 +649,8 @@ kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double 
 // level's exterior array.
 __global__ void __launch_bounds__(kBlock)
 k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomInfo CD,
-                  int ratio, int flo0, int flo1, int flo2) {
+                  int ratio, int flo0, int flo1, int flo2, const DevFab *otab, int tmode,
+                  double ta, double tb) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const int f[3] = {i, j, k}, flo[3] = {flo0, flo1, flo2};
@@ -552,7 +678,17 @@ k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomIn
     for (int n = 0; n < NVAR; ++n) {
         double u[27];
         for (int t = 0; t < 27; ++t) {
-            const double v = ob[t] >= 0 ? ctab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n) : 0.0;
+            double v = 0.0;
+            if (ob[t] >= 0) {
+                // coarse data at the fill time: new (tmode 0), old (1) or
+                // ta * old + tb * new (StateData::InterpAddBox)
+                const double vn = ctab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n);
+                if (tmode == 0) v = vn;
+                else {
+                    const double vo = otab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n);
+                    v = (tmode == 1) ? vo : ta * vo + tb * vn;
+                }
+            }
             u[t] = ((odd[t] >> n) & 1u) ? -v : v;
         }
         const double u0 = u[13];

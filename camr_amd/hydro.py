@@ -103,6 +103,10 @@ class Level:
         L.check(L.lib().camr_level_upload_box(
             self._h, b, a.ctypes.data_as(C.POINTER(C.c_double))))
 
+    @property
+    def boxes(self):
+        return self.prob.boxes
+
     def get_box(self, b):
         a = np.empty(self.shape(b))
         L.check(L.lib().camr_level_download_box(
@@ -114,6 +118,11 @@ class Level:
         `coarse` (FillPatch's conservative linear interpolation), computed on
         the device from its current state."""
         L.check(L.lib().camr_level_exterior_from_coarse(self._h, coarse._h, int(ratio)))
+
+    def average_down(self, fine: "Level", ratio: int):
+        """CAMR::avgDown: the coarse cells covered by `fine` become the mean
+        of their children."""
+        L.check(L.lib().camr_level_average_down(self._h, fine._h, int(ratio)))
 
     def set_eb(self, geom):
         """Embedded-boundary build of the level with the cut-cell data of
@@ -321,3 +330,32 @@ def estdt_hydro(lo, hi, u, ulo, dx, stream=0):
     L.check(L.lib().camr_estdt_hydro(C.byref(_box(lo, hi)), C.byref(array4_of(u, ulo)),
                                      dxa, C.byref(out), _stream(stream)))
     return out.value
+
+
+class FluxRegister:
+    """The flux register between `coarse` and `fine` (AMReX's YAFluxRegister
+    as CAMR uses it); attaching it makes both levels add their fluxes on
+    every step.  Destroy it (close) before its levels."""
+
+    def __init__(self, coarse: Level, fine: Level, ratio: int):
+        self.coarse, self.fine, self.ratio = coarse, fine, int(ratio)
+        h = C.c_void_p()
+        L.check(L.lib().camr_flux_register_create(coarse._h, fine._h, self.ratio, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().camr_flux_register_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def amr_step(coarse: Level, fine: Level, reg: FluxRegister, dt: float):
+    """One subcycled two-level step of coarse size dt (Amr::timeStep with
+    CAMR's post_timestep: reflux, avgDown, clean_state)."""
+    L.check(L.lib().camr_amr_step(coarse._h, fine._h, reg._h, float(dt)))

@@ -232,6 +232,39 @@ int camr_level_set_exterior(camr_level *L, const double *host);
  * data of the moment (no time interpolation between coarse old and new). */
 int camr_level_exterior_from_coarse(camr_level *fine, camr_level *coarse, int32_t ratio);
 
+/*
+ * Coarse-fine synchronisation of a two-level hierarchy (both levels on the
+ * staged engine, no embedded boundary).
+ */
+typedef struct camr_flux_register camr_flux_register;
+
+/* The flux register of the pair: AMReX's YAFluxRegister as CAMR holds it
+ * on the fine level (CAMR.H:310-313, defined at CAMR.cpp:287-290) and
+ * fills it from the per-box fluxes
+ * (CrseAdd / FineAdd, CAMR_construct_hydro_source.cpp:243-254, with
+ * fac_for_reflux * dt, :24).  Attaching it makes both levels add their
+ * fluxes on every step.  Fine boxes must be aligned to `ratio`, and no
+ * coarse cell may border two different fine boxes (CAMR_ERR_SHAPE). */
+int camr_flux_register_create(camr_level *coarse, camr_level *fine, int32_t ratio,
+                              camr_flux_register **out);
+/* Detaches the register from its levels and frees it. */
+int camr_flux_register_destroy(camr_flux_register *r);
+
+/* CAMR::avgDown (CAMR.cpp:1047-1066, non-EB amrex::average_down): every
+ * coarse cell covered by the fine level becomes the mean of its children. */
+int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio);
+
+/* One coarse step of the hierarchy with subcycling, Amr::timeStep as CAMR
+ * drives it: reset the register (CAMR_advance.cpp:64-66), CAMR_advance on
+ * the coarse level, `ratio` steps of dt_coarse / ratio on the fine level,
+ * each with its exterior ghost data interpolated from the coarse old and
+ * new states in time (StateData's linear interpolation) and then in space
+ * (as camr_level_exterior_from_coarse), then the coarse post_timestep
+ * (CAMR.cpp:679-699): reflux (CAMR.cpp:803-851), avgDown, clean_state.
+ * Both levels must be at the same time. */
+int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r,
+                  double dt_coarse);
+
 /* The embedded-boundary build of a level (AMREX_USE_EB: CAMR.cpp:303-329,
  * 419-421; CAMR_construct_hydro_source.cpp:77-204): ghost width 6, covered
  * cells skipped by clean_state and the dt estimate, and per box the path of

@@ -1,0 +1,102 @@
+"""Two-level subcycled steps with flux registers on the GPU against the
+oracle, bit for bit: CrseAdd on the coarse level's per-box fluxes, FineAdd
+on the fine level's, the fine ghost data interpolated from the coarse
+level in time and space, reflux, average_down and the coarse
+post_timestep clean (camr_amr_step), on Godunov PPM / PLM and the method
+of lines, periodic and walled domains, ratios 2 and 4."""
+import numpy as np
+import pytest
+
+from camr_amd import _lib
+from camr_amd.hydro import FluxRegister, Level, Problem, amr_step
+from tests.test_oracle_amr import FINE2, _boxes, _field, composite_sums
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FINE4 = [(16, 16, 16, 31, 47, 47), (32, 16, 16, 47, 47, 47)]
+
+
+def _pair(O, n, boxes, periodic, kw):
+    bc = 0 if periodic else 4
+    per = (1, 1, 1) if periodic else (0, 0, 0)
+    params = dict(cfl=0.3, **kw)
+    op = O.params(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(1.0 / n,) * 3, bclo=(bc,) * 3,
+                  bchi=(bc,) * 3, is_periodic=per, **params)
+    prob = Problem(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(1.0 / n,) * 3, bclo=(bc,) * 3,
+                   bchi=(bc,) * 3, is_periodic=per, boxes=boxes, params=params)
+    ol = O.Level(op, boxes)
+    gl = Level(prob, engine=_lib.ENGINE_STAGED)
+    for b, bb in enumerate(boxes):
+        U = _field(n, bb[:3], bb[3:])
+        ol.set_box(b, U)
+        gl.set_box(b, U)
+    return ol, gl
+
+
+@pytest.mark.parametrize("ratio,periodic,kw", [
+    (2, True, dict(ppm_type=1)),
+    (2, False, dict(ppm_type=0)),
+    (2, True, dict(do_mol=1)),
+    (4, True, dict(ppm_type=1)),
+])
+def test_amr_step_matches_oracle(oracle, ratio, periodic, kw):
+    print(f"two-level step ratio={ratio} periodic={periodic} {kw}", flush=True)
+    nc = 16
+    fine = FINE2 if ratio == 2 else FINE4
+    oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw)
+    of, gf = _pair(oracle, nc * ratio, fine, periodic, kw)
+    oR = oracle.FluxRegister(oc, of, ratio)
+    gR = FluxRegister(gc, gf, ratio)
+    oracle.average_down(oc, of, ratio)
+    gc.average_down(gf, ratio)
+    for L in (oc, of, gc, gf):
+        L.init_post()
+    s0 = composite_sums(gc, gf, ratio, nc * ratio)
+    dt = None
+    for s in range(3):
+        if s == 0:
+            odt = min(oc.initial_dt(), ratio * of.initial_dt())
+            gdt = min(gc.initial_dt(), ratio * gf.initial_dt())
+        else:
+            odt = min(oc.new_dt(dt), ratio * of.new_dt(dt / ratio))
+            gdt = min(gc.new_dt(dt), ratio * gf.new_dt(dt / ratio))
+        assert odt == gdt, (s, odt, gdt)
+        dt = odt
+        assert oracle.amr_step(oc, of, oR, dt) == 0
+        amr_step(gc, gf, gR, dt)
+        for b in range(len(oc.boxes)):
+            assert np.array_equal(gc.get_box(b), oc.get_box(b)), ("coarse", s, b)
+        for b in range(len(of.boxes)):
+            assert np.array_equal(gf.get_box(b), of.get_box(b)), ("fine", s, b)
+    if periodic:
+        s1 = composite_sums(gc, gf, ratio, nc * ratio)
+        for c in (0, 1, 2, 3, 4):
+            assert abs(s1[c] - s0[c]) <= 1e-12 * max(abs(s0[c]), 1.0)
+    gR.close()
+    oR.close()
+
+
+def test_register_refuses_unsupported_layouts():
+    prob_c = Problem(domlo=(0, 0, 0), domhi=(15,) * 3, dx=(1 / 16,) * 3, bclo=(0,) * 3,
+                     bchi=(0,) * 3, is_periodic=(1, 1, 1), boxes=_boxes(16, 8),
+                     params=dict(cfl=0.3))
+
+    def fine(boxes):
+        return Problem(domlo=(0, 0, 0), domhi=(31,) * 3, dx=(1 / 32,) * 3, bclo=(0,) * 3,
+                       bchi=(0,) * 3, is_periodic=(1, 1, 1), boxes=boxes, params=dict(cfl=0.3))
+
+    gc = Level(prob_c, engine=_lib.ENGINE_STAGED)
+    # a coarse cell bordering two fine boxes
+    gf = Level(fine([(8, 8, 8, 23, 23, 23), (24, 8, 8, 31, 15, 23)]), engine=_lib.ENGINE_STAGED)
+    with pytest.raises(_lib.CamrError, match="borders two fine boxes"):
+        FluxRegister(gc, gf, 2)
+    # fine boxes not aligned to the ratio
+    gf = Level(fine([(9, 8, 8, 23, 23, 23)]), engine=_lib.ENGINE_STAGED)
+    with pytest.raises(_lib.CamrError, match="aligned"):
+        FluxRegister(gc, gf, 2)
+    # the fused engine
+    gf = Level(fine(FINE2), engine=_lib.ENGINE_FUSED)
+    with pytest.raises(_lib.CamrError, match="staged"):
+        FluxRegister(gc, gf, 2)

@@ -70,9 +70,9 @@ def hierarchy(O, nc=16, ratio=2, fine=None, periodic=True, blast=True, **kw):
     return C, F, R
 
 
-def composite_sums(C, F, ratio):
+def composite_sums(C, F, ratio, nf=None):
     """Sum of each component over the composite grid in fine-cell volumes."""
-    nf = F.p.domhi[0] + 1
+    nf = nf or F.p.domhi[0] + 1
     cov = np.zeros((nf // ratio,) * 3, dtype=bool)
     tot = np.zeros(NV)
     for b, bb in enumerate(F.boxes):
