@@ -398,6 +398,109 @@ def bench_mol(args) -> int:
     return 0
 
 
+def bench_amr_sync(args) -> int:
+    """Two-level subcycled step with flux registers (camr_amr_step): a
+    periodic n0^3 coarse level and a fine block over the middle half of the
+    domain in each direction at ratio 2, both in 32^3 boxes, PPM at CFL
+    0.4.  One coarse step = the coarse advance (CrseAdd), two fine steps
+    with time-interpolated coarse-fine ghosts (FineAdd), reflux,
+    average_down and the coarse clean.  The first step is checked bit for
+    bit against the oracle (timed on the host cores as the CPU baseline)."""
+    import numpy as np
+    import torch
+    from camr_amd import _lib, workloads
+    from camr_amd.hydro import FluxRegister, Level, Problem, amr_step
+    from oracle import oracle as O
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    n0, r, mg, cfl = args.amr_n0, 2, 32, 0.4
+    coarse = workloads.smooth_periodic(n=n0, ppm_type=1, cfl=cfl, max_grid=mg)
+    nf = n0 * r
+    lo, hi = nf // 4, 3 * nf // 4
+    fboxes = [(i, j, k, i + mg - 1, j + mg - 1, k + mg - 1) for k in range(lo, hi, mg)
+              for j in range(lo, hi, mg) for i in range(lo, hi, mg)]
+    fine = Problem(domlo=(0, 0, 0), domhi=(nf - 1,) * 3, dx=(1.0 / nf,) * 3, bclo=coarse.bclo,
+                   bchi=coarse.bchi, is_periodic=coarse.is_periodic, boxes=fboxes,
+                   params=dict(ppm_type=1, cfl=cfl))
+    cells_c, cells_f = n0 ** 3, len(fboxes) * mg ** 3
+    ups = cells_c + r * cells_f  # cell-updates per coarse step
+
+    def gpu_pair():
+        C = Level(coarse, engine=_lib.ENGINE_STAGED)
+        F = Level(fine, engine=_lib.ENGINE_STAGED)
+        for b in range(len(coarse.boxes)):
+            C.set_box(b, workloads.smooth_init(coarse, b))
+        for b in range(len(fboxes)):
+            F.set_box(b, workloads.smooth_init(fine, b))
+        R = FluxRegister(C, F, r)
+        C.average_down(F, r)
+        C.init_post()
+        F.init_post()
+        return C, F, R
+
+    C, F, R = gpu_pair()
+    dt = min(C.initial_dt(), r * F.initial_dt())
+    amr_step(C, F, R, dt)
+    identical, cpu = None, None
+    if not args.no_cpu_baseline:
+        O.build()
+
+        def oparams(prob):
+            return O.params(ppm_type=1, cfl=cfl, bclo=prob.bclo, bchi=prob.bchi,
+                            domlo=prob.domlo, domhi=prob.domhi, is_periodic=prob.is_periodic,
+                            dx=prob.dx)
+        OC, OF = O.Level(oparams(coarse), coarse.boxes), O.Level(oparams(fine), fboxes)
+        for b in range(len(coarse.boxes)):
+            OC.set_box(b, workloads.smooth_init(coarse, b))
+        for b in range(len(fboxes)):
+            OF.set_box(b, workloads.smooth_init(fine, b))
+        OR = O.FluxRegister(OC, OF, r)
+        O.average_down(OC, OF, r)
+        OC.init_post()
+        OF.init_post()
+        odt = min(OC.initial_dt(), r * OF.initial_dt())
+        t0 = time.perf_counter()
+        assert O.amr_step(OC, OF, OR, odt) == 0
+        el = time.perf_counter() - t0
+        identical = bool(odt == dt and
+                         all(np.array_equal(C.get_box(b), OC.get_box(b)) for b in range(len(coarse.boxes))) and
+                         all(np.array_equal(F.get_box(b), OF.get_box(b)) for b in range(len(fboxes))))
+        OR.close()
+        cpu = {"value": ups / el, "unit": "cell-updates/s", "cores": O.lib().ora_get_threads(),
+               "kind": "port",
+               "sample": f"the first composite step of the same hierarchy, {el:.1f} s wall"}
+    for _ in range(args.warmup):
+        dt = min(C.new_dt(dt), r * F.new_dt(dt / r))
+        amr_step(C, F, R, dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = min(C.new_dt(dt), r * F.new_dt(dt / r))
+        amr_step(C, F, R, dt)
+    C.sync()
+    F.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = ups * args.steps / el
+    out = {"metric": "FP64 cell-updates/s of the two-level subcycled step with reflux (PPM)",
+           "unit": "cell-updates/s", "value": value, "higher_is_better": True, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic: seeded smooth periodic field on both levels",
+           "config": {"workload": f"{n0}^3 periodic coarse level + fine block "
+                                  f"[{lo},{hi - 1}]^3 at ratio 2, 32^3 boxes, PPM, CFL {cfl}; "
+                                  "one step = coarse step + 2 fine steps + reflux + average_down",
+                      "coarse_cells": cells_c, "fine_cells": cells_f,
+                      "cell_updates_per_step": ups, "boxes": [len(coarse.boxes), len(fboxes)]},
+           "identical_to_oracle_1_step": identical,
+           "roofline": {"bound": "hbm", "achieved": 160.0 * value / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": 160.0 * value / 8e12, "traffic": None,
+                        "kernel": "whole composite step, 160 B per cell-update"},
+           "cpu_baseline": cpu}
+    R.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_adjoint(args) -> int:
     """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
     wall time of the backward pass (recomputation from the checkpoints
@@ -514,14 +617,15 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb", "mol"],
+    ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb", "mol", "amr_sync"],
                     default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
                          "AMR levels against per-box launches; adjoint: "
                          "backward/forward cost of the level-step adjoint; sod: "
                          "the 256x4x4 Sod tube, GPU beside the host oracle; eb: "
                          "Sod past a sphere, embedded boundary with FluxRedist; "
-                         "mol: the n^3 periodic field with the method of lines")
+                         "mol: the n^3 periodic field with the method of lines; "
+                         "amr_sync: two-level subcycled steps with flux registers")
     ap.add_argument("--amr-n0", type=int, default=128)
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--eb-n", type=int, default=128,
@@ -551,6 +655,8 @@ def main() -> int:
         return bench_sod(args)
     if args.workload == "mol":
         return bench_mol(args)
+    if args.workload == "amr_sync":
+        return bench_amr_sync(args)
 
     import torch
     from camr_amd import _lib, workloads

@@ -272,7 +272,7 @@ class HydroLevel {
     }
     int set_exterior(const double *h);
     int exterior_from(HydroLevel &coarse, int ratio, int tmode = 0, double ta = 0.0,
-                      double tb = 0.0);
+                      double tb = 0.0, bool ghosts_only = false);
     int set_tuning(const Tuning &t);
     int get_tuning(Tuning *t) const;
     bool fused() const { return fused_; }
@@ -298,6 +298,8 @@ struct FluxReg {
     std::vector<double *> crse, cfp;
     DevFab *tab_crse = nullptr, *tab_cfp = nullptr;
     double dx1c = 0.0, dx1f = 0.0;
+    // per fine box, bit 2d+side: its slab on that side holds boundary cells
+    std::vector<int> slabs;
     ~FluxReg() {
         if (C && C->reg_crse_ == this) C->reg_crse_ = nullptr;
         if (F && F->reg_fine_ == this) F->reg_fine_ = nullptr;
@@ -336,7 +338,8 @@ int HydroLevel::set_exterior(const double *h) {
 // Exterior data from the coarser level: FillPatch's coarse-fine
 // interpolation (k_ext_from_coarse) at every cell of this level's domain,
 // from the coarse level's current S_new.
-int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, double tb) {
+int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, double tb,
+                              bool ghosts_only) {
     if (ratio < 2) { set_error("refinement ratio must be at least 2"); return CAMR_ERR_PARAM; }
     for (int d = 0; d < 3; ++d)
         if (desc_.hp.domlo[d] != C.desc_.hp.domlo[d] * ratio ||
@@ -357,8 +360,27 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
     extv_ = dense(ext_, dom);
     const DevFab *ct = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
     const DevFab *co = C.a_is_new_ ? C.tab_b_ : C.tab_a_;
-    launch_on(k_ext_from_coarse, dom, stream_, extv_, ct, (const int *)C.owner_, C.dom_, ratio,
-              dom.lo[0], dom.lo[1], dom.lo[2], co, tmode, ta, tb);
+    // one thread per cell and component (grid.y = NVAR)
+    if (ghosts_only) {
+        // just the cells a fill reads from the exterior: the domain images
+        // of the boxes' ghost regions that no box owns, over the bounding
+        // box of the grown boxes
+        Box bb = grow(box_[0], ng_);
+        for (const Box &B : box_)
+            for (int d = 0; d < 3; ++d) {
+                bb.lo[d] = std::min(bb.lo[d], B.lo[d] - ng_);
+                bb.hi[d] = std::max(bb.hi[d], B.hi[d] + ng_);
+            }
+        const Range r = range_of(bb);
+        hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
+                           r, extv_, ct, (const int *)C.owner_, C.dom_, ratio, dom.lo[0], dom.lo[1],
+                           dom.lo[2], co, tmode, ta, tb, (const int *)owner_, dom_);
+    } else {
+        const Range r = range_of(dom);
+        hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
+                           r, extv_, ct, (const int *)C.owner_, C.dom_, ratio, dom.lo[0], dom.lo[1],
+                           dom.lo[2], co, tmode, ta, tb, (const int *)nullptr, dom_);
+    }
     CAMR_HIP(hipGetLastError());
     // the coarse level may step next: finish reading its state first
     CAMR_HIP(hipStreamSynchronize(stream_));
@@ -1783,10 +1805,31 @@ int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out)
                         seen = fb;
                     }
             }
+    // which slabs of which fine box border coarse boundary cells
+    std::vector<int> slabs(F.box_.size(), 0);
+    for (size_t fb = 0; fb < F.box_.size(); ++fb) {
+        Box cb = F.box_[fb];
+        for (int d = 0; d < 3; ++d) { cb.lo[d] /= ratio; cb.hi[d] = (cb.hi[d] + 1) / ratio - 1; }
+        for (int d = 0; d < 3; ++d)
+            for (int side = 0; side < 2; ++side) {
+                Box sl = cb;
+                sl.lo[d] = sl.hi[d] = side ? cb.hi[d] + 1 : cb.lo[d] - 1;
+                bool any = false;
+                for (int k = sl.lo[2]; k <= sl.hi[2] && !any; ++k)
+                    for (int j = sl.lo[1]; j <= sl.hi[1] && !any; ++j)
+                        for (int i = sl.lo[0]; i <= sl.hi[0] && !any; ++i) {
+                            const int c[3] = {i, j, k};
+                            int m[3];
+                            any = host_reg_map(D, c, m) && flag[idx(m)] == kRegBnd;
+                        }
+                if (any) slabs[fb] |= 1 << (2 * d + side);
+            }
+    }
     if ((rc = C.resolve()) || (rc = F.resolve())) return rc;
     CAMR_HIP(hipStreamSynchronize(C.stream_));
     CAMR_HIP(hipStreamSynchronize(F.stream_));
     std::unique_ptr<FluxReg> R(new FluxReg());
+    R->slabs = slabs;
     R->C = &C;
     R->F = &F;
     R->ratio = ratio;
@@ -1837,6 +1880,7 @@ void HydroLevel::reg_add(size_t b, const Arr *flx, hipStream_t s) {
         for (int d = 0; d < 3; ++d) { cb.lo[d] /= r; cb.hi[d] = (cb.hi[d] + 1) / r - 1; }
         for (int d = 0; d < 3; ++d)
             for (int side = 0; side < 2; ++side) {
+                if (!(R.slabs[b] >> (2 * d + side) & 1)) continue;
                 Box sl = cb;
                 sl.lo[d] = sl.hi[d] = side ? cb.hi[d] + 1 : cb.lo[d] - 1;
                 if (d == 0)
@@ -1909,7 +1953,7 @@ int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc) {
         if (t > t_old - teps && t < t_old + teps) tmode = 1;
         else if (t > t_new - teps && t < t_new + teps) tmode = 0;
         const double ta = (t_new - t) / (t_new - t_old), tb = (t - t_old) / (t_new - t_old);
-        if ((rc = F.exterior_from(C, R.ratio, tmode, ta, tb))) return rc;
+        if ((rc = F.exterior_from(C, R.ratio, tmode, ta, tb, true))) return rc;
         if ((rc = F.advance(dtf))) return rc;
     }
     if ((rc = F.resolve())) return rc;

@@ -485,12 +485,8 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
         pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
-// ------------------------------------------- coarse-fine synchronisation
-// AMReX's YAFluxRegister as CAMR uses it (CAMR.H:310; CrseAdd / FineAdd at
-// CAMR_construct_hydro_source.cpp:243-254; Reflux at CAMR.cpp:803-851) and
-// average_down (CAMR.cpp:1047-1066), restated.  Coarse cells carry a flag
-// over the coarse domain: kRegFine (covered by the fine level), kRegBnd
-// (not covered, within one cell of a covered cell) or 0.
+// periodic image of an index in domain D (false outside a non-periodic
+// domain) and the coarse-fine flag of a cell
 __device__ __forceinline__ bool reg_map(const DomInfo &D, const int *c, int *m) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -512,6 +508,12 @@ __device__ __forceinline__ int reg_flag(const DomInfo &D, const uint8_t *flag, c
                 (int64_t)(m[2] - D.domlo[2]) * n0 * n1];
 }
 
+// ------------------------------------------- coarse-fine synchronisation
+// AMReX's YAFluxRegister as CAMR uses it (CAMR.H:310; CrseAdd / FineAdd at
+// CAMR_construct_hydro_source.cpp:243-254; Reflux at CAMR.cpp:803-851) and
+// average_down (CAMR.cpp:1047-1066), restated.  Coarse cells carry a flag
+// over the coarse domain: kRegFine (covered by the fine level), kRegBnd
+// (not covered, within one cell of a covered cell) or 0.
 // CrseAdd over the coarse box of `d`: a boundary cell takes -/+ dt/vol_c
 // times the coarse flux through each face it shares with a covered cell
 // (x, then y, then z).
@@ -650,9 +652,18 @@ kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double 
 __global__ void __launch_bounds__(kBlock)
 k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomInfo CD,
                   int ratio, int flo0, int flo1, int flo2, const DevFab *otab, int tmode,
-                  double ta, double tb) {
+                  double ta, double tb, const int *fowner, DomInfo FD) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    if (fowner) {
+        // ghost-shell mode: r is a grown fine box; take the cell's domain
+        // image and skip it when outside a non-periodic domain or owned
+        int c[3] = {i, j, k}, m[3];
+        if (!reg_map(FD, c, m) || owner_of(FD, fowner, m) >= 0) return;
+        i = m[0];
+        j = m[1];
+        k = m[2];
+    }
     const int f[3] = {i, j, k}, flo[3] = {flo0, flo1, flo2};
     const double half = 0.5 * (ratio - 1);
     int cc[3];
@@ -674,8 +685,11 @@ k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomIn
                 map_cell(CD, cc[0] + d, cc[1] + b, cc[2] + a, mm[t], &odd[t]);
                 ob[t] = owner_of(CD, cowner, mm[t]);
             }
+    // grid.y > 1: one component per block row
+    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
+    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
 #pragma unroll 1
-    for (int n = 0; n < NVAR; ++n) {
+    for (int n = nlo; n < nhi; ++n) {
         double u[27];
         for (int t = 0; t < 27; ++t) {
             double v = 0.0;

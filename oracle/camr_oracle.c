@@ -3100,6 +3100,47 @@ int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) 
     return exterior_from_coarse_t(F, C, ratio, tw);
 }
 
+/* The same interpolation at just the cells a fill of F reads from the
+ * exterior: the domain images of grow(box, ng) that no box of F owns. */
+static int exterior_ghosts_from_coarse_t(ora_level *F, const ora_level *C, int ratio, otimew tw) {
+    for (int d = 0; d < 3; ++d)
+        if (F->P.domlo[d] != C->P.domlo[d] * ratio ||
+            F->P.domhi[d] + 1 != (C->P.domhi[d] + 1) * ratio) return 1;
+    const int64_t nc = F->dn[0] * F->dn[1] * F->dn[2];
+    if (!F->ext) F->ext = (double *)calloc((size_t)(nc * O_NVAR), sizeof(double));
+    const double half = 0.5 * (ratio - 1);
+    for (int b = 0; b < F->nbox; ++b) {
+        const obox G = bgrow(F->box[b], F->ng);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(ora_get_threads())
+#endif
+        for (int k = G.lo[2]; k <= G.hi[2]; ++k)
+        for (int j = G.lo[1]; j <= G.hi[1]; ++j)
+        for (int i = G.lo[0]; i <= G.hi[0]; ++i) {
+            int f[3] = {i, j, k};
+            int ok = 1;
+            for (int d = 0; d < 3; ++d) {
+                const int lo = F->P.domlo[d], n = F->P.domhi[d] - lo + 1;
+                if (f[d] < lo || f[d] > lo + n - 1) {
+                    if (!F->P.is_periodic[d]) { ok = 0; break; }
+                    f[d] = lo + ((f[d] - lo) % n + n) % n;
+                }
+            }
+            if (!ok || F->owner[dom_index(F, f)] >= 0) continue;
+            int cc[3];
+            double off[3];
+            for (int d = 0; d < 3; ++d) {
+                cc[d] = (f[d] - F->P.domlo[d]) / ratio + C->P.domlo[d];
+                const int sub = (f[d] - F->P.domlo[d]) - (cc[d] - C->P.domlo[d]) * ratio;
+                off[d] = (sub - half) / ratio;
+            }
+            for (int n = 0; n < O_NVAR; ++n)
+                F->ext[dom_index(F, f) + (int64_t)n * nc] = cons_interp(C, cc, n, off, half / ratio, tw);
+        }
+    }
+    return 0;
+}
+
 void ora_level_get_exterior(const ora_level *L, double *out) {
     if (L->ext)
         memcpy(out, L->ext, sizeof(double) * (size_t)(L->dn[0] * L->dn[1] * L->dn[2] * O_NVAR));
@@ -3352,7 +3393,7 @@ int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
         otimew tw = {2, (t_new - t) / (t_new - t_old), (t - t_old) / (t_new - t_old)};
         if (t > t_old - teps && t < t_old + teps) tw.mode = 1;
         else if (t > t_new - teps && t < t_new + teps) tw.mode = 0;
-        if (exterior_from_coarse_t(F, C, r, tw)) return 1;
+        if (exterior_ghosts_from_coarse_t(F, C, r, tw)) return 1;
         err = ora_level_advance(F, dtf);
         if (err) return err;
     }
