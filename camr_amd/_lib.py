@@ -39,6 +39,13 @@ class EbGeom(C.Structure):
                 ("vfrac", Array4), ("ap", Array4 * 3), ("fc", Array4 * 3)]
 
 
+class PlotDesc(C.Structure):
+    _fields_ = [("nlev", C.c_int32), ("levels", C.POINTER(C.c_void_p)),
+                ("ref_ratio", C.POINTER(C.c_int32)), ("level_steps", C.POINTER(C.c_int32)),
+                ("prob_lo", C.c_double * 3), ("prob_hi", C.c_double * 3),
+                ("plot_rhoy", C.c_int32), ("derive_vars", C.c_char_p)]
+
+
 class HydroParams(C.Structure):
     _fields_ = [
         ("ppm_type", C.c_int32), ("plm_iorder", C.c_int32),
@@ -87,7 +94,7 @@ EXPORTS = [
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
     "camr_flux_register_create", "camr_flux_register_destroy", "camr_level_average_down",
-    "camr_amr_step",
+    "camr_amr_step", "camr_write_plotfile",
 ]
 
 _lib = None
@@ -156,6 +163,7 @@ def lib():
         L.camr_flux_register_destroy.argtypes = [vp]
         L.camr_level_average_down.argtypes = [vp, vp, C.c_int32]
         L.camr_amr_step.argtypes = [vp, vp, vp, C.c_double]
+        L.camr_write_plotfile.argtypes = [C.c_char_p, C.POINTER(PlotDesc)]
         L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,
                                           C.POINTER(EbGeom), C.c_double,
                                           C.POINTER(HydroParams), i32p, vp]

@@ -12,7 +12,12 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <memory>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
 #include <vector>
 
 #include "hydro_adjoint.h"
@@ -142,6 +147,9 @@ class HydroLevel {
     int sync();
     void set_stream(hipStream_t s) { stream_ = s; }
     double time() const { return time_; }
+    const camr_level_desc &desc() const { return desc_; }
+    const std::vector<Box> &boxes() const { return box_; }
+    bool is_eb() const { return ebflags_ != nullptr; }
     Prof &prof() { return prof_; }
     int set_eb(const int32_t *flags, const int32_t flo[3], const int32_t fhi[3]);
     int set_box_eb(int b, const double *vf, const double *const ap[3],
@@ -1968,6 +1976,218 @@ int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc) {
 }
 } // namespace camr
 
+// ================================================================ plotfile
+// AMReX plotfile of a level hierarchy as CAMR writes it
+// (CAMR::writePlotFile, Source/Utils/CAMR_io.cpp:407-591, with Amr's
+// header lines and VisMF's one-file-per-level data).  AMReX is absent; the
+// layout is restated from its published format:
+//   dir/Header            plotfile type, variables, geometry, per-level grids
+//   dir/Level_<l>/Cell_H  VisMF header: boxes, byte offsets, per-box min/max
+//   dir/Level_<l>/Cell_D_00000  per box a FAB header line, then the data,
+//                         component after component, x fastest, native
+//                         little-endian IEEE doubles
+//   dir/job_info          a short description of the run
+namespace camr {
+namespace {
+
+constexpr double kSmallDen = 1.e-20;  // Derive.cpp local_small_den (EB builds)
+
+// Pointwise derived variables of Source/Utils/Derive.cpp, non-EB forms
+// (EB levels guard the density as the EB build does).
+double derive_value(const std::string &name, const double *u, bool eb) {
+    const double rho = u[URHO];
+    if (name == "x_velocity" || name == "y_velocity" || name == "z_velocity") {
+        const int c = UMX + (name[0] - 'x');
+        return eb ? u[c] / vmax(rho, kSmallDen) : u[c] / rho;  // CAMR_dervelx/y/z
+    }
+    if (name == "magvel") {  // CAMR_dermagvel
+        const double rhoInv = eb ? 1.0 / vmax(rho, kSmallDen) : 1.0 / rho;
+        const double d1 = u[UMX] * rhoInv, d2 = u[UMY] * rhoInv, d3 = u[UMZ] * rhoInv;
+        return std::sqrt((d1 * d1) + (d2 * d2) + (d3 * d3));
+    }
+    if (name == "kineng") {  // CAMR_derkineng
+        const double sq = u[UMX] * u[UMX] + u[UMY] * u[UMY] + u[UMZ] * u[UMZ];
+        return eb ? 0.5 * sq / vmax(rho, kSmallDen) : 0.5 * sq / rho;
+    }
+    // EOS::REY2P / REY2Gam of the gamma law (EOS.H:135-145)
+    if (name == "pressure") {  // CAMR_derpres
+        if (eb && rho <= 0.) return 0.;
+        const double rhoInv = 1.0 / rho;
+        const double e = u[UEINT] * rhoInv;
+        return (kGamma - 1.0) * rho * e;
+    }
+    if (name == "soundspeed" || name == "MachNumber") {  // CAMR_dersoundspeed / dermachnumber
+        if (rho <= 0.) return 0.;
+        const double rhoInv = 1.0 / rho;
+        const double e = u[UEINT] * rhoInv;
+        const double pres = (kGamma - 1.0) * rho * e;
+        const double c = std::sqrt(kGamma * pres / rho);
+        if (name == "soundspeed") return c;
+        const double sq = u[UMX] * u[UMX] + u[UMY] * u[UMY] + u[UMZ] * u[UMZ];
+        return std::sqrt(sq) / rho / c;
+    }
+    return 0.0;
+}
+
+// derive_lst order of CAMR_setup.cpp:211-300 for the pointwise variables
+const char *const kDeriveOrder[] = {"pressure", "kineng", "soundspeed", "MachNumber",
+                                    "x_velocity", "y_velocity", "z_velocity", "magvel"};
+
+std::string box_str(const Box &b) {
+    std::ostringstream o;
+    o << "((" << b.lo[0] << ',' << b.lo[1] << ',' << b.lo[2] << ") (" << b.hi[0] << ','
+      << b.hi[1] << ',' << b.hi[2] << ") (0,0,0))";
+    return o.str();
+}
+
+bool make_dir(const std::string &d) {
+    struct stat st;
+    if (stat(d.c_str(), &st) == 0) return S_ISDIR(st.st_mode);
+    return mkdir(d.c_str(), 0755) == 0;
+}
+
+}  // namespace
+
+int write_plotfile(const char *dir, const camr_plot_desc &pd, const std::vector<HydroLevel *> &lev) {
+    if (pd.nlev > 1 && !pd.ref_ratio) { set_error("plotfile: ref_ratio needed"); return CAMR_ERR_PARAM; }
+    // variables: the state components (CAMR_setup.cpp:159-198) that are plot
+    // variables -- all but rho_<species> unless plot_rhoy (CAMR_io.cpp:
+    // 100-111) -- then the requested derived variables in derive_lst order
+    static const char *const kState[NVAR] = {"density", "xmom", "ymom", "zmom", "rho_E",
+                                             "rho_e", "Temp", "rho_F", "rho_A", "rho_P"};
+    std::vector<int> comps;
+    std::vector<std::string> names;
+    for (int n = 0; n < NVAR; ++n) {
+        if (n >= UFS && !pd.plot_rhoy) continue;
+        comps.push_back(n);
+        names.push_back(kState[n]);
+    }
+    std::vector<std::string> want;
+    if (pd.derive_vars) {
+        std::istringstream in(pd.derive_vars);
+        std::string w;
+        while (in >> w) {
+            bool ok = false;
+            for (const char *d : kDeriveOrder) ok = ok || w == d;
+            if (!ok) { set_error("plotfile: derived variable '%s' is not built", w.c_str()); return CAMR_ERR_PARAM; }
+            want.push_back(w);
+        }
+    }
+    std::vector<std::string> derived;
+    for (const char *d : kDeriveOrder)
+        for (const std::string &w : want)
+            if (w == d) { derived.push_back(d); names.push_back(d); break; }
+    const int nvars = (int)names.size();
+
+    const std::string root(dir);
+    if (!make_dir(root)) { set_error("plotfile: cannot create %s", dir); return CAMR_ERR_PARAM; }
+    std::ofstream H(root + "/Header");
+    if (!H) { set_error("plotfile: cannot write %s/Header", dir); return CAMR_ERR_PARAM; }
+    H.precision(17);
+    const int f_lev = pd.nlev - 1;
+    H << "HyperCLaw-V1.1\n" << nvars << '\n';
+    for (const std::string &n : names) H << n << '\n';
+    H << 3 << '\n' << lev[0]->time() << '\n' << f_lev << '\n';
+    for (int d = 0; d < 3; ++d) H << pd.prob_lo[d] << ' ';
+    H << '\n';
+    for (int d = 0; d < 3; ++d) H << pd.prob_hi[d] << ' ';
+    H << '\n';
+    for (int l = 0; l < f_lev; ++l) H << pd.ref_ratio[l] << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l)
+        H << box_str(make_box(lev[l]->desc().hp.domlo, lev[l]->desc().hp.domhi)) << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l) H << (pd.level_steps ? pd.level_steps[l] : 0) << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l) {
+        for (int d = 0; d < 3; ++d) H << lev[l]->desc().hp.dx[d] << ' ';
+        H << '\n';
+    }
+    H << 0 << '\n' << "0\n";  // Cartesian coordinates; no boundary data
+
+    for (int l = 0; l <= f_lev; ++l) {
+        HydroLevel &L = *lev[l];
+        const std::vector<Box> &bx = L.boxes();
+        const double *dx = L.desc().hp.dx;
+        char lname[32];
+        snprintf(lname, sizeof lname, "Level_%d", l);
+        const std::string ldir = root + "/" + lname;
+        if (!make_dir(ldir)) { set_error("plotfile: cannot create %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+        H << l << ' ' << bx.size() << ' ' << L.time() << '\n';
+        H << (pd.level_steps ? pd.level_steps[l] : 0) << '\n';
+        for (const Box &b : bx)
+            for (int d = 0; d < 3; ++d)
+                H << pd.prob_lo[d] + dx[d] * b.lo[d] << ' ' << pd.prob_lo[d] + dx[d] * (b.hi[d] + 1) << '\n';
+        H << lname << "/Cell\n";
+
+        std::ofstream D(ldir + "/Cell_D_00000", std::ios::binary);
+        if (!D) { set_error("plotfile: cannot write %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+        std::vector<long long> offs;
+        std::vector<std::vector<double>> mn, mx;
+        std::vector<double> u, out;
+        for (size_t b = 0; b < bx.size(); ++b) {
+            const size_t np = bx[b].npts();
+            u.resize(NVAR * np);
+            int rc = L.download((int)b, u.data());
+            if (rc) return rc;
+            out.assign((size_t)nvars * np, 0.0);
+            int c = 0;
+            for (int n : comps) {
+                std::copy(u.begin() + (size_t)n * np, u.begin() + (size_t)(n + 1) * np,
+                          out.begin() + (size_t)c * np);
+                ++c;
+            }
+            double cell[NVAR];
+            for (const std::string &dn : derived) {
+                for (size_t p = 0; p < np; ++p) {
+                    for (int n = 0; n < NVAR; ++n) cell[n] = u[(size_t)n * np + p];
+                    out[(size_t)c * np + p] = derive_value(dn, cell, L.is_eb());
+                }
+                ++c;
+            }
+            offs.push_back((long long)D.tellp());
+            D << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(bx[b]) << ' '
+              << nvars << '\n';
+            D.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)(out.size() * sizeof(double)));
+            std::vector<double> lo(nvars), hi(nvars);
+            for (int v = 0; v < nvars; ++v) {
+                double a = out[(size_t)v * np], z = a;
+                for (size_t p = 1; p < np; ++p) {
+                    a = std::min(a, out[(size_t)v * np + p]);
+                    z = std::max(z, out[(size_t)v * np + p]);
+                }
+                lo[v] = a;
+                hi[v] = z;
+            }
+            mn.push_back(lo);
+            mx.push_back(hi);
+        }
+        if (!D) { set_error("plotfile: write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+        std::ofstream V(ldir + "/Cell_H");
+        V.precision(17);
+        V << 1 << '\n' << 1 << '\n' << nvars << '\n' << 0 << '\n';
+        V << '(' << bx.size() << " 0\n";
+        for (const Box &b : bx) V << box_str(b) << '\n';
+        V << ")\n" << bx.size() << '\n';
+        for (size_t b = 0; b < bx.size(); ++b) V << "FabOnDisk: Cell_D_00000 " << offs[b] << '\n';
+        V << '\n' << bx.size() << ',' << nvars << '\n';
+        for (auto &r : mn) { for (double v : r) V << v << ','; V << '\n'; }
+        V << '\n' << bx.size() << ',' << nvars << '\n';
+        for (auto &r : mx) { for (double v : r) V << v << ','; V << '\n'; }
+        if (!V) { set_error("plotfile: write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+    }
+    std::ofstream J(root + "/job_info");
+    J << "===============================================================================\n"
+      << " CAMR Job Information\n"
+      << "===============================================================================\n"
+      << "code: camr_amd (MI355X, FP64)\n"
+      << "levels: " << pd.nlev << "\n"
+      << "time: " << std::setprecision(17) << lev[0]->time() << "\n";
+    if (!H) { set_error("plotfile: header write failed"); return CAMR_ERR_PARAM; }
+    return CAMR_OK;
+}
+}  // namespace camr
+
 // ================================================================ C-ABI
 using namespace camr;
 
@@ -2285,6 +2505,15 @@ int camr_flux_register_destroy(camr_flux_register *r) {
 int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio) {
     if (!coarse || !fine || coarse == fine) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
     return average_down(coarse->impl, fine->impl, ratio);
+}
+int camr_write_plotfile(const char *dir, const camr_plot_desc *d) {
+    if (!d || !dir || d->nlev < 1 || !d->levels) { set_error("plotfile: bad arguments"); return CAMR_ERR_PARAM; }
+    std::vector<HydroLevel *> lev(d->nlev);
+    for (int l = 0; l < d->nlev; ++l) {
+        if (!d->levels[l]) { set_error("plotfile: null level"); return CAMR_ERR_PARAM; }
+        lev[l] = &d->levels[l]->impl;
+    }
+    return write_plotfile(dir, *d, lev);
 }
 int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r, double dt_coarse) {
     if (!coarse || !fine || coarse == fine || !r || !r->impl) { set_error("bad arguments"); return CAMR_ERR_PARAM; }

@@ -1,0 +1,94 @@
+"""AMReX plotfiles of level hierarchies: the writer is the library's
+``camr_write_plotfile`` (CAMR::writePlotFile, Source/Utils/CAMR_io.cpp:
+407-591); the reader here parses the same layout (Header, Level_<l>/Cell_H,
+Level_<l>/Cell_D_*) into numpy arrays, for checks and post-processing."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from . import _lib as L
+
+
+def write_plotfile(path, levels, ref_ratio=(), level_steps=None, prob_lo=(0.0, 0.0, 0.0),
+                   prob_hi=None, plot_rhoy=False, derive_vars=()):
+    """Write `levels` (``camr_amd.hydro.Level``, coarse first) to the
+    plotfile directory `path`.  prob_hi defaults to prob_lo + the coarse
+    domain size in cells times its dx."""
+    n = len(levels)
+    if prob_hi is None:
+        p0 = levels[0].prob
+        prob_hi = tuple(prob_lo[d] + (p0.domhi[d] - p0.domlo[d] + 1) * p0.dx[d] for d in range(3))
+    hs = (C.c_void_p * n)(*[lv._h for lv in levels])
+    rr = (C.c_int32 * max(n - 1, 1))(*(list(ref_ratio) or [0]))
+    st = (C.c_int32 * n)(*(level_steps or [0] * n))
+    d = L.PlotDesc()
+    d.nlev = n
+    d.levels = C.cast(hs, C.POINTER(C.c_void_p))
+    d.ref_ratio = C.cast(rr, C.POINTER(C.c_int32))
+    d.level_steps = C.cast(st, C.POINTER(C.c_int32))
+    for k in range(3):
+        d.prob_lo[k], d.prob_hi[k] = prob_lo[k], prob_hi[k]
+    d.plot_rhoy = int(bool(plot_rhoy))
+    dv = " ".join(derive_vars).encode()
+    d.derive_vars = dv or None
+    L.check(L.lib().camr_write_plotfile(os.fsencode(path), C.byref(d)))
+
+
+_BOX = re.compile(r"\(\((-?\d+),(-?\d+),(-?\d+)\) \((-?\d+),(-?\d+),(-?\d+)\) \((\d+),(\d+),(\d+)\)\)")
+
+
+def _boxes(text):
+    return [tuple(int(x) for x in m.groups()[:6]) for m in _BOX.finditer(text)]
+
+
+def read_plotfile(path):
+    """Parse a plotfile into a dict: names, time, finest_level, prob_lo,
+    prob_hi, ref_ratio, domains, level_steps, dx, and per level its time,
+    boxes and the data of every box as an array (nvars, nz, ny, nx)."""
+    with open(os.path.join(path, "Header")) as f:
+        lines = f.read().split("\n")
+    it = iter(lines)
+    out = {"type": next(it)}
+    nv = int(next(it))
+    out["names"] = [next(it) for _ in range(nv)]
+    assert int(next(it)) == 3
+    out["time"] = float(next(it))
+    fl = int(next(it))
+    out["finest_level"] = fl
+    out["prob_lo"] = [float(x) for x in next(it).split()]
+    out["prob_hi"] = [float(x) for x in next(it).split()]
+    out["ref_ratio"] = [int(x) for x in next(it).split()]
+    out["domains"] = _boxes(next(it))
+    out["level_steps"] = [int(x) for x in next(it).split()]
+    out["dx"] = [[float(x) for x in next(it).split()] for _ in range(fl + 1)]
+    out["coord"] = int(next(it))
+    next(it)  # boundary data flag
+    levels = []
+    for _ in range(fl + 1):
+        lev, nbox, t = next(it).split()
+        steps = int(next(it))
+        locs = [[float(x) for x in next(it).split()] for _ in range(3 * int(nbox))]
+        rel = next(it)
+        with open(os.path.join(path, rel + "_H")) as f:
+            vh = f.read()
+        boxes = _boxes(vh)
+        offs = [(m.group(1), int(m.group(2))) for m in re.finditer(r"FabOnDisk: (\S+) (\d+)", vh)]
+        data = []
+        for (fname, off), b in zip(offs, boxes):
+            with open(os.path.join(path, os.path.dirname(rel), fname), "rb") as f:
+                f.seek(off)
+                head = f.readline().decode()
+                fb = _boxes(head)[0]
+                ncomp = int(head.rsplit(")", 1)[1])
+                assert fb == b and ncomp == nv
+                n = [b[3 + d] - b[d] + 1 for d in range(3)]
+                a = np.fromfile(f, dtype="<f8", count=ncomp * n[0] * n[1] * n[2])
+                data.append(a.reshape(ncomp, n[2], n[1], n[0]))
+        levels.append({"level": int(lev), "time": float(t), "steps": steps, "boxes": boxes,
+                       "grid_locations": locs, "data": data})
+    out["levels"] = levels
+    return out

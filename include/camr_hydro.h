@@ -265,6 +265,27 @@ int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio)
 int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r,
                   double dt_coarse);
 
+/*
+ * AMReX plotfile of a level hierarchy (CAMR::writePlotFile, Source/Utils/
+ * CAMR_io.cpp:407-591, with Amr's header lines): dir/Header,
+ * dir/Level_<l>/Cell_H, dir/Level_<l>/Cell_D_00000 and dir/job_info.
+ * Variables: density, xmom, ymom, zmom, rho_E, rho_e, Temp, the rho_<species>
+ * when plot_rhoy is set (CAMR.plot_rhoy, CAMR_io.cpp:100-111), then the
+ * requested derived variables among pressure, kineng, soundspeed,
+ * MachNumber, x_velocity, y_velocity, z_velocity, magvel (Derive.cpp), in
+ * the order CAMR registers them.  Data are the levels' current S_new.
+ */
+typedef struct camr_plot_desc {
+    int32_t nlev;                  /* levels, coarse first */
+    camr_level *const *levels;
+    const int32_t *ref_ratio;      /* nlev - 1 ratios */
+    const int32_t *level_steps;    /* nlev step counts, or NULL (0) */
+    double prob_lo[3], prob_hi[3]; /* geometry.prob_lo / prob_hi */
+    int32_t plot_rhoy;
+    const char *derive_vars;       /* space-separated amr.derive_plot_vars, or NULL */
+} camr_plot_desc;
+int camr_write_plotfile(const char *dir, const camr_plot_desc *d);
+
 /* The embedded-boundary build of a level (AMREX_USE_EB: CAMR.cpp:303-329,
  * 419-421; CAMR_construct_hydro_source.cpp:77-204): ghost width 6, covered
  * cells skipped by clean_state and the dt estimate, and per box the path of
