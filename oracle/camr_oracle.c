@@ -3100,6 +3100,23 @@ int ora_level_exterior_from_coarse(ora_level *F, const ora_level *C, int ratio) 
     return exterior_from_coarse_t(F, C, ratio, tw);
 }
 
+/* The exterior of F from C at time t between C's old state (t_old) and new
+ * state (t_new), StateData::InterpAddBox's rule, over F's whole domain. */
+int ora_level_exterior_from_coarse_time(ora_level *F, const ora_level *C, int ratio,
+                                        double t_old, double t_new, double t) {
+    const double teps = (t_new - t_old) * 1.e-3;
+    otimew tw = {2, (t_new - t) / (t_new - t_old), (t - t_old) / (t_new - t_old)};
+    if (t > t_old - teps && t < t_old + teps) tw.mode = 1;
+    else if (t > t_new - teps && t < t_new + teps) tw.mode = 0;
+    return exterior_from_coarse_t(F, C, ratio, tw);
+}
+
+/* Set a level's old state (the time-interpolation source of its finer
+ * level's ghost data). */
+void ora_level_set_old_box(ora_level *L, int b, const double *data) {
+    memcpy(L->Sold[b].p, data, sizeof(double) * (size_t)(L->Sold[b].ns * O_NVAR));
+}
+
 /* The same interpolation at just the cells a fill of F reads from the
  * exterior: the domain images of grow(box, ng) that no box of F owns. */
 static int exterior_ghosts_from_coarse_t(ora_level *F, const ora_level *C, int ratio, otimew tw) {

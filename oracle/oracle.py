@@ -123,6 +123,9 @@ def lib():
         L.ora_level_average_down.restype = C.c_int
         L.ora_amr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
         L.ora_amr_step.restype = C.c_int
+        L.ora_level_exterior_from_coarse_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_double] * 3
+        L.ora_level_exterior_from_coarse_time.restype = C.c_int
+        L.ora_level_set_old_box.argtypes = [C.c_void_p, C.c_int, dp]
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_get_threads.restype = C.c_int
         _lib = L
@@ -238,6 +241,14 @@ class Level:
         """Exterior data interpolated from level `coarse` (FillPatch's
         coarse-fine interpolation)."""
         assert lib().ora_level_exterior_from_coarse(self._h, coarse._h, int(ratio)) == 0
+
+    def exterior_from_coarse_time(self, coarse, ratio, t_old, t_new, t):
+        assert lib().ora_level_exterior_from_coarse_time(self._h, coarse._h, ratio, t_old, t_new, t) == 0
+
+    def set_old_box(self, b, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        assert data.shape == self.shape(b)
+        lib().ora_level_set_old_box(self._h, b, _dp(data))
 
     def get_exterior(self):
         n = [self.p.domhi[d] - self.p.domlo[d] + 1 for d in range(3)]

@@ -7,7 +7,11 @@ fixes pins it:
 * conservation: the children of every coarse cell average to it;
 * linear data is reproduced (the limiters do not act on it);
 * no new extrema: every child lies within its coarse 27-neighbourhood;
-* the same for refinement ratio 4.
+* the same for refinement ratio 4;
+* in time (the two-level step's ghost data): at t between the coarse old
+  and new states the result equals the spatial interpolation of
+  alpha old + beta new, alpha = (t_new - t) / (t_new - t_old), and it is the
+  old or the new state's within (t_new - t_old) 1e-3 of either end.
 """
 import numpy as np
 import pytest
@@ -91,3 +95,34 @@ def test_linear_data_is_reproduced(oracle):
     # coarse cells away from the domain faces (their neighbours are linear)
     inner = (slice(None), slice(2, nf - 2), slice(2, nf - 2), slice(2, nf - 2))
     assert np.abs(E[inner] - want[None][inner]).max() <= 1e-14
+
+
+@pytest.mark.parametrize("t,wa,wb", [(0.25, 0.75, 0.25), (0.5, 0.5, 0.5), (1.0e-4, 1.0, 0.0),
+                                     (1.0 - 1.0e-4, 0.0, 1.0)])
+def test_time_interpolation(oracle, t, wa, wb):
+    nc, ratio = 8, 2
+    U_old = _smooth(nc)
+    U_new = _smooth(nc)[::-1].copy() * 1.3 + 0.1  # a different field
+    want_field = wa * U_old + wb * U_new if wa and wb else (U_old if wa else U_new)
+
+    def level_with(old, new):
+        pc = oracle.params(domlo=(0, 0, 0), domhi=(nc - 1,) * 3, dx=(1.0 / nc,) * 3,
+                           bclo=(0,) * 3, bchi=(0,) * 3, is_periodic=(1, 1, 1))
+        h = nc // 2
+        boxes = [(i, j, k, i + h - 1, j + h - 1, k + h - 1)
+                 for k in (0, h) for j in (0, h) for i in (0, h)]
+        C = oracle.Level(pc, boxes)
+        for b, (i0, j0, k0, i1, j1, k1) in enumerate(boxes):
+            sl = (slice(None), slice(k0, k1 + 1), slice(j0, j1 + 1), slice(i0, i1 + 1))
+            C.set_box(b, np.ascontiguousarray(new[sl]))
+            if old is not None:
+                C.set_old_box(b, np.ascontiguousarray(old[sl]))
+        return C
+
+    pf = oracle.params(domlo=(0, 0, 0), domhi=(nc * ratio - 1,) * 3, dx=(1.0 / (nc * ratio),) * 3,
+                       bclo=(0,) * 3, bchi=(0,) * 3, is_periodic=(1, 1, 1))
+    F1 = oracle.Level(pf, [(0, 0, 0, 3, 3, 3)])
+    F1.exterior_from_coarse_time(level_with(U_old, U_new), ratio, 0.0, 1.0, t)
+    F2 = oracle.Level(pf, [(0, 0, 0, 3, 3, 3)])
+    F2.exterior_from_coarse(level_with(None, want_field), ratio)
+    assert np.array_equal(F1.get_exterior(), F2.get_exterior())
