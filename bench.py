@@ -39,10 +39,11 @@ NVAR = 10
 #     cells (2); writes the compact flux + Godunov (10) = 30 doubles.
 ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
                 "riemann_normal": 30}
-# mol_faces (one launch per direction d, method of lines): per d-face, the 9
-# primitives and 3 aux values (c, csmall, gamma) of one cell read, the 9
-# fluxes (NVAR without UTEMP) and Godunov (u, p) written.
-NS_MOL_READ, NF_MOL_WRITE = 12, 11
+# mol_faces (one launch per direction d, method of lines, flux adjustment
+# fused): per d-face, the 9 primitives and 3 aux values (c, csmall, gamma)
+# of one cell, its 10 conserved values (artificial viscosity) and one divu
+# node read; the NVAR fluxes and Godunov (u, p) written.
+NS_MOL_READ, NF_MOL_WRITE = 23, 12
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # for the fused engine at 256^3 PPM (profiles/hbm_traffic_fused_256cube_ppm.md,

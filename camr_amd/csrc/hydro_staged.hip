@@ -317,12 +317,15 @@ k_transdd(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa,
 // ---------------------------------------------------- method of lines face
 // MOL_umeth's per-direction pair (MOL_umeth.cpp:74-139): the slopes of the
 // two cells beside the face (mol_slope_*), their face states
-// (mol_riemann_*) and hydro_cmpflx, one face per thread.  The slope of a
-// cell is formed by both faces that read it, with the same operations, so
-// no slope array is kept.
+// (mol_riemann_*) and hydro_cmpflx, one face per thread, followed in
+// registers by the face's adjust_fluxes (artificial viscosity from the
+// divu nodes, species normalisation, area).  The slope of a cell is formed
+// by both faces that read it, with the same operations, so no slope array
+// is kept.
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_mol_face(Range r, Arr q, Arr qaux, Arr fout, Arr gout, HydroConst hc) {
+k_mol_face(Range r, Arr q, Arr qaux, Arr fout, Arr gout, HydroConst hc, Arr divu, Arr u,
+           Arr area, int has_area, double area_c) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
@@ -351,8 +354,13 @@ k_mol_face(Range r, Arr q, Arr qaux, Arr fout, Arr gout, HydroConst hc) {
     double f[NF], g[NG];
     int mut;
     face_flux(D, idx, hc, ql, qr, a, f, g, &mut);
+    double fa[NVAR];
+    fa[UTEMP] = 0.0;
 #pragma unroll
-    for (int s = 0; s < NF; ++s) fout(i, j, k, kUcomp[s]) = f[s];
+    for (int s = 0; s < NF; ++s) fa[kUcomp[s]] = f[s];
+    adjust_face<D>(fa, divu, u, i, j, k, has_area ? area(i, j, k, 0) : area_c, hc);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) fout(i, j, k, n) = fa[n];
     gout(i, j, k, GUN) = g[GUN];
     gout(i, j, k, GP) = g[GP];
 }
@@ -574,9 +582,9 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
 }
 
-// hydro_umdrv with do_mol (Hydro_umdrv.cpp:57-63): MOL_umeth's fluxes and
-// Godunov face states, then the same divu / adjust_fluxes / consup as the
-// Godunov path.  The face states reach two cells past bx along the face
+// hydro_umdrv with do_mol (Hydro_umdrv.cpp:57-63): divu, then MOL_umeth's
+// fluxes and Godunov face states with adjust_fluxes applied in the same
+// kernel, then the same consup as the Godunov path.  The face states reach two cells past bx along the face
 // direction; divu one cell past it.
 size_t umdrv_mol_scratch_doubles(const Box &bx) {
     size_t n = 0;
@@ -598,16 +606,15 @@ int umdrv_mol(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof) {
     Arr G[3];
     for (int d = 0; d < 3; ++d) G[d] = take(nodes(bx, d), NG);
     Arr divu = take(nodes(nodes(nodes(bx, 0), 1), 2), 1);
-    NAMED("mol_faces");
-    launch(k_mol_face<0>, nodes(bx, 0), st, A.q, A.qaux, A.flx[0], G[0], hc);
-    launch(k_mol_face<1>, nodes(bx, 1), st, A.q, A.qaux, A.flx[1], G[1], hc);
-    launch(k_mol_face<2>, nodes(bx, 2), st, A.q, A.qaux, A.flx[2], G[2], hc);
     NAMED("divu");
     launch(k_divu, nodes(nodes(nodes(bx, 0), 1), 2), st, A.q, divu, hc);
-    NAMED("adjust_fluxes");
-    launch(k_adjust<0>, nodes(bx, 0), st, A.flx[0], divu, A.uin, A.area[0], A.has_area, A.area_c[0], hc);
-    launch(k_adjust<1>, nodes(bx, 1), st, A.flx[1], divu, A.uin, A.area[1], A.has_area, A.area_c[1], hc);
-    launch(k_adjust<2>, nodes(bx, 2), st, A.flx[2], divu, A.uin, A.area[2], A.has_area, A.area_c[2], hc);
+    NAMED("mol_faces");
+    launch(k_mol_face<0>, nodes(bx, 0), st, A.q, A.qaux, A.flx[0], G[0], hc, divu, A.uin,
+           A.area[0], A.has_area, A.area_c[0]);
+    launch(k_mol_face<1>, nodes(bx, 1), st, A.q, A.qaux, A.flx[1], G[1], hc, divu, A.uin,
+           A.area[1], A.has_area, A.area_c[1]);
+    launch(k_mol_face<2>, nodes(bx, 2), st, A.q, A.qaux, A.flx[2], G[2], hc, divu, A.uin,
+           A.area[2], A.has_area, A.area_c[2]);
     NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G[0], G[1], G[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
