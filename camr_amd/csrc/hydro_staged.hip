@@ -173,10 +173,18 @@ k_ppm(Range r, Arr q, Arr qaux, Arr flat, Arr qm, Arr qp, HydroConst hc, const d
 // hydro_cmpflx over faces of direction D.  mode 0: compact flux + GD out,
 // writing the boundary-mutated states back (first solve); mode 1: compact
 // out without write-back (temporaries); mode 2: final solve into the
-// caller's NVAR flux array + GD out.
+// caller's NVAR flux array + GD out, with the face's adjust_fluxes
+// (artificial viscosity from the divu nodes, species normalisation, area)
+// applied in registers.
+struct AdjustIn {
+    Arr divu, u, area;
+    int has_area;
+    double area_c;
+};
+
 template <int D, int MODE>
 __global__ void __launch_bounds__(kBlock)
-k_riemann(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, HydroConst hc) {
+k_riemann(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, HydroConst hc, AdjustIn adj) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
@@ -206,8 +214,14 @@ k_riemann(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, HydroConst hc) 
         }
     }
     if (MODE == 2) {
+        double fa[NVAR];
+        fa[UTEMP] = 0.0;
 #pragma unroll
-        for (int s = 0; s < NF; ++s) fout(i, j, k, kUcomp[s]) = f[s];
+        for (int s = 0; s < NF; ++s) fa[kUcomp[s]] = f[s];
+        adjust_face<D>(fa, adj.divu, adj.u, i, j, k,
+                       adj.has_area ? adj.area(i, j, k, 0) : adj.area_c, hc);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) fout(i, j, k, n) = fa[n];
     } else {
 #pragma unroll
         for (int s = 0; s < NF; ++s) fout(i, j, k, s) = f[s];
@@ -376,24 +390,6 @@ k_divu(Range r, Arr q, Arr divu, HydroConst hc) {
     divu(i, j, k, 0) = divu_node(q, i, j, k, hc);
 }
 
-// ----------------------------------------------- adjust_fluxes (per face)
-// hydro_artif_visc + hydro_norm_spec_flx + hydro_ext_flx
-// (Hydro_adjust_fluxes.cpp:13-44).
-template <int D>
-__global__ void __launch_bounds__(kBlock)
-k_adjust(Range r, Arr flx, Arr divu, Arr u, Arr area, int has_area,
-         double area_c, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    double f[NVAR];
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) f[n] = flx(i, j, k, n);
-    const double a = has_area ? area(i, j, k, 0) : area_c;
-    adjust_face<D>(f, divu, u, i, j, k, a, hc);
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) flx(i, j, k, n) = f[n];
-}
-
 // ------------------------------------------------------ pdivu + update
 // hydro_pdivu (Hydro_utils_K.H:175-202) + hydro_update (:239-257).
 __global__ void __launch_bounds__(kBlock)
@@ -505,9 +501,9 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         G1[d] = take(fb, NG);
     }
     NAMED("riemann_normal");
-    launch(k_riemann<0, 0>, nodes(grow_dir(bxg2, 0, -1), 0), st, qdm[0], qdp[0], A.qaux, F1[0], G1[0], hc);
-    launch(k_riemann<1, 0>, nodes(grow_dir(bxg2, 1, -1), 1), st, qdm[1], qdp[1], A.qaux, F1[1], G1[1], hc);
-    launch(k_riemann<2, 0>, nodes(grow_dir(bxg2, 2, -1), 2), st, qdm[2], qdp[2], A.qaux, F1[2], G1[2], hc);
+    launch(k_riemann<0, 0>, nodes(grow_dir(bxg2, 0, -1), 0), st, qdm[0], qdp[0], A.qaux, F1[0], G1[0], hc, AdjustIn{});
+    launch(k_riemann<1, 0>, nodes(grow_dir(bxg2, 1, -1), 1), st, qdm[1], qdp[1], A.qaux, F1[1], G1[1], hc, AdjustIn{});
+    launch(k_riemann<2, 0>, nodes(grow_dir(bxg2, 2, -1), 2), st, qdm[2], qdp[2], A.qaux, F1[2], G1[2], hc, AdjustIn{});
 
     // transverse corrections + Riemann solves of the 6 pairs
     Arr F2[3][3], G2[3][3];
@@ -528,16 +524,16 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         const Box fb = nodes(bxg1, d);
         if (d == 0) {
             launch(k_transdo<0>, tb, st, qdm[0], qdp[0], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
-            launch(k_riemann<0, 1>, fb, st, qmA, qpA, A.qaux, F2[0][t0], G2[0][t0], hc);
-            launch(k_riemann<0, 1>, fb, st, qmB, qpB, A.qaux, F2[0][t1], G2[0][t1], hc);
+            launch(k_riemann<0, 1>, fb, st, qmA, qpA, A.qaux, F2[0][t0], G2[0][t0], hc, AdjustIn{});
+            launch(k_riemann<0, 1>, fb, st, qmB, qpB, A.qaux, F2[0][t1], G2[0][t1], hc, AdjustIn{});
         } else if (d == 1) {
             launch(k_transdo<1>, tb, st, qdm[1], qdp[1], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
-            launch(k_riemann<1, 1>, fb, st, qmA, qpA, A.qaux, F2[1][t0], G2[1][t0], hc);
-            launch(k_riemann<1, 1>, fb, st, qmB, qpB, A.qaux, F2[1][t1], G2[1][t1], hc);
+            launch(k_riemann<1, 1>, fb, st, qmA, qpA, A.qaux, F2[1][t0], G2[1][t0], hc, AdjustIn{});
+            launch(k_riemann<1, 1>, fb, st, qmB, qpB, A.qaux, F2[1][t1], G2[1][t1], hc, AdjustIn{});
         } else {
             launch(k_transdo<2>, tb, st, qdm[2], qdp[2], A.qaux, F1[t0], G1[t0], F1[t1], G1[t1], qmA, qpA, qmB, qpB, A.dtp, hc);
-            launch(k_riemann<2, 1>, fb, st, qmA, qpA, A.qaux, F2[2][t0], G2[2][t0], hc);
-            launch(k_riemann<2, 1>, fb, st, qmB, qpB, A.qaux, F2[2][t1], G2[2][t1], hc);
+            launch(k_riemann<2, 1>, fb, st, qmA, qpA, A.qaux, F2[2][t0], G2[2][t0], hc, AdjustIn{});
+            launch(k_riemann<2, 1>, fb, st, qmB, qpB, A.qaux, F2[2][t1], G2[2][t1], hc, AdjustIn{});
         }
     }
 
@@ -549,6 +545,11 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     for (int d = 0; d < 3; ++d) G3[d] = take(nodes(bx, d), NG);
     Arr divu = take(nodes(nodes(nodes(bx, 0), 1), 2), 1);
     double *qf_base = cur;
+    // divu first: the final solves apply adjust_fluxes in registers
+    NAMED("divu");
+    launch(k_divu, nodes(nodes(nodes(bx, 0), 1), 2), st, A.q, divu, hc);
+    AdjustIn adj[3];
+    for (int d = 0; d < 3; ++d) adj[d] = AdjustIn{divu, A.uin, A.area[d], A.has_area, A.area_c[d]};
     NAMED("transverse_2");
     for (int d = 0; d < 3; ++d) {
         cur = qf_base;
@@ -558,22 +559,16 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
         const Box fb = nodes(bx, d);
         if (d == 0) {
             launch(k_transdd<0>, tb, st, qdm[0], qdp[0], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
-            launch(k_riemann<0, 2>, fb, st, qm, qp, A.qaux, A.flx[0], G3[0], hc);
+            launch(k_riemann<0, 2>, fb, st, qm, qp, A.qaux, A.flx[0], G3[0], hc, adj[0]);
         } else if (d == 1) {
             launch(k_transdd<1>, tb, st, qdm[1], qdp[1], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
-            launch(k_riemann<1, 2>, fb, st, qm, qp, A.qaux, A.flx[1], G3[1], hc);
+            launch(k_riemann<1, 2>, fb, st, qm, qp, A.qaux, A.flx[1], G3[1], hc, adj[1]);
         } else {
             launch(k_transdd<2>, tb, st, qdm[2], qdp[2], A.qaux, A.srcq, A.has_src, F2[t0][t1], G2[t0][t1], F2[t1][t0], G2[t1][t0], qm, qp, A.dtp, hc);
-            launch(k_riemann<2, 2>, fb, st, qm, qp, A.qaux, A.flx[2], G3[2], hc);
+            launch(k_riemann<2, 2>, fb, st, qm, qp, A.qaux, A.flx[2], G3[2], hc, adj[2]);
         }
     }
 
-    NAMED("divu");
-    launch(k_divu, nodes(nodes(nodes(bx, 0), 1), 2), st, A.q, divu, hc);
-    NAMED("adjust_fluxes");
-    launch(k_adjust<0>, nodes(bx, 0), st, A.flx[0], divu, A.uin, A.area[0], A.has_area, A.area_c[0], hc);
-    launch(k_adjust<1>, nodes(bx, 1), st, A.flx[1], divu, A.uin, A.area[1], A.has_area, A.area_c[1], hc);
-    launch(k_adjust<2>, nodes(bx, 2), st, A.flx[2], divu, A.uin, A.area[2], A.has_area, A.area_c[2], hc);
     NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G3[0], G3[1], G3[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
