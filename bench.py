@@ -290,6 +290,7 @@ def bench_eb(args) -> int:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     value = cells * args.steps / el
+    force = [float(x) for x in L.eb_force()]
     out = {"metric": f"FP64 cell-updates/s, Sod past a sphere {2 * n}x{n}x{n} (EB, FluxRedist)",
            "unit": "cell-updates/s", "value": value, "higher_is_better": True, "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
@@ -302,6 +303,7 @@ def bench_eb(args) -> int:
                                 "covered": kinds.count(2)},
                       "geometry_setup_s": round(t_geom, 2)},
            "identical_to_oracle_2_steps": identical,
+           "force_on_sphere_after_run": force,
            "roofline": {"bound": "hbm", "achieved": 160.0 * value / 1e9, "peak": 8000.0,
                         "unit": "GB/s", "frac": 160.0 * value / 8e12, "traffic": None,
                         "kernel": "whole step, 160 B per cell-update"},

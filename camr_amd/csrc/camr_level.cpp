@@ -155,6 +155,7 @@ class HydroLevel {
     int set_box_eb(int b, const double *vf, const double *const ap[3],
                    const double *const fc[3]);
     int box_kind(int b, int32_t *kind) const;
+    int eb_force(double f[3]);
 
   private:
     int est_time_step(double *est);
@@ -723,6 +724,36 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
     CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
     return setup_workers(max_g, sc);
+}
+
+// The force on the embedded body, F = sum over the cut cells of the wall
+// momentum flux (compute_hyp_wallflux's fluxw[UMX..UMZ],
+// Hydro_hypflux_eb_K.H:65) times dx^2, from the current S_new.  CAMR itself
+// computes no force; this is the functional of the Sod-past-a-sphere
+// workload.  Summed on the host box by box, cell by cell (x fastest).
+int HydroLevel::eb_force(double f[3]) {
+    if (!ebflags_) { set_error("the level has no embedded boundary"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
+    f[0] = f[1] = f[2] = 0.0;
+    const double dx2 = desc_.hp.dx[0] * desc_.hp.dx[0];
+    std::vector<double> h;
+    for (size_t b = 0; b < box_.size(); ++b) {
+        if (kind_[b] != 1) continue;
+        if (!ebg_[b].vf.p) { set_error("box %d has cut cells but no cut-cell data", (int)b); return CAMR_ERR_PARAM; }
+        const Box B = box_[b];
+        const size_t np = B.npts();
+        if (sstride_ < 3 * np) { set_error("scratch too small for the force"); return CAMR_ERR_PARAM; }
+        Arr q = dense(q_, B), qa = dense(qaux_, B), out = dense(scratch_.base(), B);
+        launch_on(k_ctoprim, B, stream_, snew_[b], q, qa, hc_, status_);
+        if ((rc = eb_force_box(B, q, qa, ebg_[b], hc_, out, dx2, stream_))) return rc;
+        h.resize(3 * np);
+        CAMR_HIP(hipStreamSynchronize(stream_));
+        CAMR_HIP(hipMemcpy(h.data(), scratch_.base(), 3 * np * sizeof(double), hipMemcpyDeviceToHost));
+        for (int d = 0; d < 3; ++d)
+            for (size_t c = 0; c < np; ++c) f[d] += h[(size_t)d * np + c];
+    }
+    return check_status();
 }
 
 // q / qaux / srcq (max_g points each) and scratch (sc doubles) per worker
@@ -2532,6 +2563,10 @@ int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac, const d
 }
 int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind) {
     return L ? L->impl.box_kind(b, kind) : CAMR_ERR_PARAM;
+}
+int camr_level_eb_force(camr_level *L, double force[3]) {
+    if (!L || !force) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    return L->impl.eb_force(force);
 }
 int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order) {
     if (!L || !block) { set_error("null argument"); return CAMR_ERR_PARAM; }

@@ -51,5 +51,9 @@ struct UmdrvEbArgs {
 
 size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox);
 int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof = nullptr);
+// Per valid cell of bx: the wall momentum flux of its cut cell times dx2
+// (3 components, 0 for other cells) into out; q / qaux cover bx.
+int eb_force_box(const Box &bx, Arr q, Arr qaux, const EbGeom &g, const HydroConst &hc, Arr out,
+                 double dx2, hipStream_t st);
 
 } // namespace camr

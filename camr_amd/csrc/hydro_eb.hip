@@ -306,6 +306,36 @@ __device__ __forceinline__ double centroid_value(const Arr &u, int n, const EbGe
     return v;
 }
 
+// compute_hyp_wallflux (Hydro_hypflux_eb_K.H:12-68): the flux through the
+// embedded wall of cut cell (i,j,k), a Riemann problem of the cell state
+// against its mirror across the wall normal from the apertures.
+__device__ __forceinline__ void eb_wallflux(const Arr &q, const Arr &qaux, const EbGeom &g, int i,
+                                            int j, int k, const HydroConst &hc, double *fw) {
+    const double axm = g.ap[0](i, j, k, 0), axp = g.ap[0](i + 1, j, k, 0);
+    const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
+    const double azm = g.ap[2](i, j, k, 0), azp = g.ap[2](i, j, k + 1, 0);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) fw[n] = 0.;
+    const double apnorm = sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) +
+                               (azm - azp) * (azm - azp));
+    const double apnorminv = 1. / apnorm;
+    const double un = q(i, j, k, QU) * (axm - axp) * apnorminv +
+                      q(i, j, k, QV) * (aym - ayp) * apnorminv +
+                      q(i, j, k, QW) * (azm - azp) * apnorminv;
+    const double rho = q(i, j, k, QRHO), p = q(i, j, k, QPRES), qrei = q(i, j, k, QREINT);
+    const double gamc = qaux(i, j, k, QGAMC);
+    RiemannOut o;
+    riemann_two_shock(rho, un, 0., 0., p, qrei, rho, -un, 0., 0., p, qrei, qaux(i, j, k, QC),
+                      gamc, gamc, hc.small, qaux(i, j, k, QCSML), hc.small_dens,
+                      hc.small_pres, hc.smallu, o);
+    fw[URHO] = o.f_rho;
+    fw[UEDEN] = o.f_eden;
+    fw[UEINT] = o.f_eint;
+    fw[UMX] = (axm - axp) * o.f_un;
+    fw[UMY] = (aym - ayp) * o.f_un;
+    fw[UMZ] = (azm - azp) * o.f_un;
+}
+
 // -------------------------------------------- hydro_consup_eb (one cell)
 // eb_compute_div (Hydro_utils_eb_K.H:283-575) for all NVAR components, the
 // redistribution weight (eb_weights_type 2: rho), the face-centroid fluxes
@@ -347,30 +377,8 @@ k_eb_div(Range r, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, 
         const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
         const double azm = g.ap[2](i, j, k, 0), azp = g.ap[2](i, j, k + 1, 0);
         const double vfr = g.vf(i, j, k, 0);
-        // compute_hyp_wallflux (Hydro_hypflux_eb_K.H:12-68)
         double fw[NVAR];
-#pragma unroll
-        for (int n = 0; n < NVAR; ++n) fw[n] = 0.;
-        {
-            const double apnorm = sqrt((axm - axp) * (axm - axp) + (aym - ayp) * (aym - ayp) +
-                                       (azm - azp) * (azm - azp));
-            const double apnorminv = 1. / apnorm;
-            const double un = q(i, j, k, QU) * (axm - axp) * apnorminv +
-                              q(i, j, k, QV) * (aym - ayp) * apnorminv +
-                              q(i, j, k, QW) * (azm - azp) * apnorminv;
-            const double rho = q(i, j, k, QRHO), p = q(i, j, k, QPRES), qrei = q(i, j, k, QREINT);
-            const double gamc = qaux(i, j, k, QGAMC);
-            RiemannOut o;
-            riemann_two_shock(rho, un, 0., 0., p, qrei, rho, -un, 0., 0., p, qrei, qaux(i, j, k, QC),
-                              gamc, gamc, hc.small, qaux(i, j, k, QCSML), hc.small_dens,
-                              hc.small_pres, hc.smallu, o);
-            fw[URHO] = o.f_rho;
-            fw[UEDEN] = o.f_eden;
-            fw[UEINT] = o.f_eint;
-            fw[UMX] = (axm - axp) * o.f_un;
-            fw[UMY] = (aym - ayp) * o.f_un;
-            fw[UMZ] = (azm - azp) * o.f_un;
-        }
+        eb_wallflux(q, qaux, g, i, j, k, hc, fw);
 #pragma unroll 1
         for (int n = 0; n < NVAR; ++n) {
             const double fxm = centroid_value<0>(ft0, n, g, i, j, k);
@@ -663,6 +671,36 @@ int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) 
     launch(k_redist_dst, A.bx, st, g, dbox2, divc, wt, optmp0, delm, winv, A.dsdt);
     g_prof = nullptr;
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
+
+// The force on the embedded body from one box's cut cells: per valid cell
+// the wall momentum flux times dx^2 (fluxw[UMX..UMZ] of
+// compute_hyp_wallflux, Hydro_hypflux_eb_K.H:65), 0 elsewhere; the caller
+// sums the cells in a fixed order.
+__global__ void __launch_bounds__(kBlock)
+k_eb_force(Range r, Arr q, Arr qaux, EbGeom g, HydroConst hc, Arr out, double dx2) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (eb_type(g.f(i, j, k)) == kEbSingle) {
+        double fw[NVAR];
+        eb_wallflux(q, qaux, g, i, j, k, hc, fw);
+        f[0] = fw[UMX] * dx2;
+        f[1] = fw[UMY] * dx2;
+        f[2] = fw[UMZ] * dx2;
+    }
+    out(i, j, k, 0) = f[0];
+    out(i, j, k, 1) = f[1];
+    out(i, j, k, 2) = f[2];
+}
+
+int eb_force_box(const Box &bx, Arr q, Arr qaux, const EbGeom &g, const HydroConst &hc, Arr out,
+                 double dx2, hipStream_t st) {
+    Range r = range_of(bx);
+    hipLaunchKernelGGL(k_eb_force, dim3(grid_for(r.n)), dim3(kBlock), 0, st, r, q, qaux, g, hc,
+                       out, dx2);
+    return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
 }
 
 } // namespace camr

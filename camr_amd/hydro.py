@@ -119,6 +119,13 @@ class Level:
         the device from its current state."""
         L.check(L.lib().camr_level_exterior_from_coarse(self._h, coarse._h, int(ratio)))
 
+    def eb_force(self):
+        """The force on the embedded body from the current state: the wall
+        momentum flux of every cut cell times dx^2, summed (x, y, z)."""
+        f = (C.c_double * 3)()
+        L.check(L.lib().camr_level_eb_force(self._h, f))
+        return np.array(list(f))
+
     def average_down(self, fine: "Level", ratio: int):
         """CAMR::avgDown: the coarse cells covered by `fine` become the mean
         of their children."""

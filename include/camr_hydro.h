@@ -299,6 +299,14 @@ int camr_write_plotfile(const char *dir, const camr_plot_desc *d);
 int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
                       const int32_t fhi[3]);
 int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind);
+
+/* The force on the embedded body from the level's current state: the sum
+ * over all cut cells of the wall momentum flux (compute_hyp_wallflux's
+ * fluxw[UMX], fluxw[UMY], fluxw[UMZ], Hydro_hypflux_eb_K.H:65) times dx^2,
+ * summed box by box and cell by cell (x fastest).  CAMR computes no force;
+ * this is the x-force functional of the Sod-past-a-sphere workload (and
+ * its y and z analogues). */
+int camr_level_eb_force(camr_level *L, double force[3]);
 int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac,
                           const double *apx, const double *apy, const double *apz,
                           const double *fcx, const double *fcy, const double *fcz);

@@ -3163,6 +3163,48 @@ void ora_level_get_exterior(const ora_level *L, double *out) {
         memcpy(out, L->ext, sizeof(double) * (size_t)(L->dn[0] * L->dn[1] * L->dn[2] * O_NVAR));
 }
 
+/* The force on the embedded body from the level's current state: the sum
+ * over the cut cells of the wall momentum flux (compute_hyp_wallflux's
+ * fluxw[UMX..UMZ], Hydro_hypflux_eb_K.H:65) times dx^2, summed box by box,
+ * cell by cell (x fastest).  CAMR computes no force; this is the x-force
+ * functional of the Sod-past-a-sphere workload (and its y, z analogues). */
+int ora_level_eb_force(ora_level *L, double f[3]) {
+    if (!L->kind || !L->geom) return 1;
+    f[0] = f[1] = f[2] = 0.0;
+    const double dx2 = L->P.dx[0] * L->P.dx[0];
+    for (int b = 0; b < L->nbox; ++b) {
+        if (L->kind[b] != 1) continue;
+        const obox B = L->box[b];
+        const ora_ebgeom *G = &L->geom[b];
+        ora_fab q, qa;
+        fab_alloc_box(&q, B, O_QVAR);
+        fab_alloc_box(&qa, B, O_NQAUX);
+        int err = 0;
+        ora_ctoprim_box(B.lo, B.hi, &L->Snew[b], &q, &qa, &L->P, &err);
+        const int64_t np = q.ns;
+        double *c = (double *)calloc((size_t)(3 * np), sizeof(double));
+        int64_t idx = 0;
+        for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+        for (int i = B.lo[0]; i <= B.hi[0]; ++i, ++idx) {
+            if (!fl_single(FLG(G, i, j, k))) continue;
+            const double axm = FA(&G->ap[0], i, j, k, 0), axp = FA(&G->ap[0], i + 1, j, k, 0);
+            const double aym = FA(&G->ap[1], i, j, k, 0), ayp = FA(&G->ap[1], i, j + 1, k, 0);
+            const double azm = FA(&G->ap[2], i, j, k, 0), azp = FA(&G->ap[2], i, j, k + 1, 0);
+            double fw[O_NVAR];
+            hyp_wallflux(&q, &qa, i, j, k, axm, axp, aym, ayp, azm, azp, &L->P, fw);
+            for (int d = 0; d < 3; ++d) c[d * np + idx] = fw[O_UMX + d] * dx2;
+        }
+        for (int d = 0; d < 3; ++d)
+            for (int64_t p = 0; p < np; ++p) f[d] += c[d * np + p];
+        free(c);
+        ora_fab_free(&q);
+        ora_fab_free(&qa);
+        if (err) return err;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Coarse-fine synchronisation: the flux register, reflux, average_down and */
 /* the two-level subcycled step.  CAMR uses AMReX's YAFluxRegister          */

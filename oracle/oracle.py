@@ -123,6 +123,8 @@ def lib():
         L.ora_level_average_down.restype = C.c_int
         L.ora_amr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
         L.ora_amr_step.restype = C.c_int
+        L.ora_level_eb_force.argtypes = [C.c_void_p, dp]
+        L.ora_level_eb_force.restype = C.c_int
         L.ora_level_exterior_from_coarse_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_double] * 3
         L.ora_level_exterior_from_coarse_time.restype = C.c_int
         L.ora_level_set_old_box.argtypes = [C.c_void_p, C.c_int, dp]
@@ -241,6 +243,11 @@ class Level:
         """Exterior data interpolated from level `coarse` (FillPatch's
         coarse-fine interpolation)."""
         assert lib().ora_level_exterior_from_coarse(self._h, coarse._h, int(ratio)) == 0
+
+    def eb_force(self):
+        f = np.zeros(3)
+        assert lib().ora_level_eb_force(self._h, _dp(f)) == 0
+        return f
 
     def exterior_from_coarse_time(self, coarse, ratio, t_old, t_new, t):
         assert lib().ora_level_exterior_from_coarse_time(self._h, coarse._h, ratio, t_old, t_new, t) == 0

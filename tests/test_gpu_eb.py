@@ -57,6 +57,11 @@ def test_eb_level_matches_oracle(oracle, n, max_grid, nsteps):
             worst = max(worst, d)
             print(f"box {b} kind {gl.kinds[b]} differs: max rel {d:.3e}", flush=True)
     assert worst == 0.0
+    # the force on the sphere from the final state (the wall momentum flux
+    # of every cut cell times dx^2), bit for bit
+    gf, of = gl.eb_force(), ol.eb_force()
+    assert np.array_equal(gf, of), (gf, of)
+    assert abs(gf[0]) > 0.0
     # the shock reached the cut cells
     init = [workloads.sod_sphere_init(prob, b, geom) for b in range(len(prob.boxes))]
     moved = max(float(np.abs(gl.get_box(b)[0] - init[b][0]).max())

@@ -11,7 +11,11 @@ geometry it consumes (camr_amd/eb.py), pinned by what the mathematics fixes:
   vfrac * U over the valid cells change only by the domain-boundary fluxes
   (zero here: slip walls and a quiescent outflow);
 * a level of the EB build with no cut cell reproduces the plain level bit
-  for bit.
+  for bit;
+* the force on the body (the wall momentum flux of the cut cells times
+  dx^2): zero for a uniform pressure (a closed surface), -b V_body for the
+  pressure 1 + b x (Archimedes) to first order in dx, no y / z force by
+  symmetry.
 
 AMReX's EB2 geometry and its redistribution routine are not available, so
 their numbers are parity unpinned; these checks pin the restatement.
@@ -147,3 +151,46 @@ def test_eb_level_without_cut_cells_matches_plain(oracle):
     assert np.array_equal(da, db)
     for b in range(len(prob.boxes)):
         assert np.array_equal(A.get_box(b), B.get_box(b))
+
+
+def _pressure_field_level(oracle, n, pfun):
+    """Oracle EB level of Sod past a sphere at rest with pressure pfun(X)."""
+    import numpy as np
+    prob = workloads.sod_sphere(n=n, max_grid=16)
+    geom = workloads.sod_sphere_geometry(prob)
+    L = oracle.Level(_params(oracle, prob), prob.boxes)
+    L.set_eb(geom)
+    for b, bb in enumerate(prob.boxes):
+        lo, hi = bb[:3], bb[3:]
+        x = [(np.arange(lo[d], hi[d] + 1) + 0.5) * prob.dx[d] for d in range(3)]
+        Z, Y, X = np.meshgrid(x[2], x[1], x[0], indexing="ij")
+        U = np.zeros((10,) + X.shape)
+        U[0] = 1.0
+        U[5] = pfun(X) / 0.4
+        U[4] = U[5]
+        U[7] = 1.0
+        U[:, geom.covered_mask(lo, hi)] = 0.0
+        L.set_box(b, np.ascontiguousarray(U))
+    L.init_post()
+    vbody = (1.0 - geom.vfrac[7:-7, 7:-7, 7:-7]).sum() * prob.dx[0] ** 3
+    return L, vbody
+
+
+def test_force_of_a_uniform_pressure_vanishes(oracle):
+    """The body's surface is closed: the aperture differences of its cut
+    cells sum to zero, so a uniform pressure exerts no net force."""
+    L, vbody = _pressure_field_level(oracle, 32, lambda X: 1.0 + 0.0 * X)
+    f = L.eb_force()
+    assert abs(f).max() <= 1e-13 * 3.14159 * 0.15 ** 2
+
+
+def test_force_of_a_linear_pressure_is_archimedes(oracle):
+    """p = 1 + b x: the force is -b V_body (V from the cut-cell volume
+    fractions), to first order in dx; y and z vanish by symmetry."""
+    errs = []
+    for n in (32, 64):
+        L, vbody = _pressure_field_level(oracle, n, lambda X: 1.0 + 0.5 * X)
+        f = L.eb_force()
+        errs.append(abs(f[0] + 0.5 * vbody) / (0.5 * vbody))
+        assert abs(f[1]) <= 1e-13 and abs(f[2]) <= 1e-13
+    assert errs[0] < 2e-2 and errs[1] < 0.5 * errs[0], errs
