@@ -237,6 +237,8 @@ class HydroLevel {
     void launch_all(K kern, int g, A... args);
     double *dt_dev_ = nullptr;  // the step's dt, read by the kernels
     double *ext_ = nullptr;     // exterior data (whole domain), when set
+    double *cstage_ = nullptr;  // coarse values staged for exterior_from
+    size_t cstage_n_ = 0;
     Arr extv_{};
     bool batched_ = true;       // one launch per stage over all boxes
     int parity() const { return a_is_new_ ? 0 : 1; }
@@ -369,7 +371,22 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
     extv_ = dense(ext_, dom);
     const DevFab *ct = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
     const DevFab *co = C.a_is_new_ ? C.tab_b_ : C.tab_a_;
-    // one thread per cell and component (grid.y = NVAR)
+    // pass 1: the coarse values the interpolation reads (source cell, fill
+    // time, reflection sign) staged densely over the coarse domain grown by 1
+    const Box cst = grow(make_box(C.desc_.hp.domlo, C.desc_.hp.domhi), 1);
+    if (!cstage_ || cstage_n_ < NVAR * cst.npts()) {
+        if (cstage_) (void)hipFree(cstage_);
+        cstage_ = nullptr;
+        CAMR_HIP(hipMalloc(&cstage_, NVAR * cst.npts() * sizeof(double)));
+        cstage_n_ = NVAR * cst.npts();
+    }
+    const Arr stv = dense(cstage_, cst);
+    {
+        const Range r = range_of(cst);
+        hipLaunchKernelGGL(k_coarse_stage, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_, r,
+                           stv, ct, (const int *)C.owner_, C.dom_, co, tmode, ta, tb);
+    }
+    // pass 2: one thread per fine cell and component (grid.y = NVAR)
     if (ghosts_only) {
         // just the cells a fill reads from the exterior: the domain images
         // of the boxes' ghost regions that no box owns, over the bounding
@@ -382,13 +399,13 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
             }
         const Range r = range_of(bb);
         hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
-                           r, extv_, ct, (const int *)C.owner_, C.dom_, ratio, dom.lo[0], dom.lo[1],
-                           dom.lo[2], co, tmode, ta, tb, (const int *)owner_, dom_);
+                           r, extv_, stv, C.dom_, ratio, dom.lo[0], dom.lo[1], dom.lo[2],
+                           (const int *)owner_, dom_);
     } else {
         const Range r = range_of(dom);
         hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
-                           r, extv_, ct, (const int *)C.owner_, C.dom_, ratio, dom.lo[0], dom.lo[1],
-                           dom.lo[2], co, tmode, ta, tb, (const int *)nullptr, dom_);
+                           r, extv_, stv, C.dom_, ratio, dom.lo[0], dom.lo[1], dom.lo[2],
+                           (const int *)nullptr, dom_);
     }
     CAMR_HIP(hipGetLastError());
     // the coarse level may step next: finish reading its state first
@@ -416,6 +433,8 @@ void HydroLevel::release() {
     if (reg_fine_) reg_fine_->F = nullptr;
     reg_crse_ = reg_fine_ = nullptr;
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
+    fr(cstage_);
+    cstage_ = nullptr;
     for (auto p : bufA_) fr(p);
     for (auto p : bufB_) fr(p);
     for (auto p : sb_) fr(p);

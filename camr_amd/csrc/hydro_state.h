@@ -72,10 +72,10 @@ __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const doub
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,
                                const DevFab *dsdt, const double *dtp, double small_dens,
                                int *flags, EbFlags eb);
-__global__ void k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner,
-                                  DomInfo CD, int ratio, int flo0, int flo1, int flo2,
-                                  const DevFab *otab, int tmode, double ta, double tb,
-                                  const int *fowner, DomInfo FD);
+__global__ void k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
+                               const DevFab *otab, int tmode, double ta, double tb);
+__global__ void k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0,
+                                  int flo1, int flo2, const int *fowner, DomInfo FD);
 // coarse-fine synchronisation (flux register, reflux, average_down)
 constexpr int kRegBnd = 1, kRegFine = 2;
 __global__ void k_reg_crse_add(Range r, Arr d, Arr f0, Arr f1, Arr f2, const uint8_t *flag,

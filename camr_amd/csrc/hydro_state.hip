@@ -650,9 +650,37 @@ kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double 
 // thread per fine cell of the domain, all NVAR components, into the fine
 // level's exterior array.
 __global__ void __launch_bounds__(kBlock)
-k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomInfo CD,
-                  int ratio, int flo0, int flo1, int flo2, const DevFab *otab, int tmode,
-                  double ta, double tb, const int *fowner, DomInfo FD) {
+k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
+               const DevFab *otab, int tmode, double ta, double tb) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    // the coarse value FillPatch sees at (i, j, k): its source cell through
+    // the domain's periodicity / BCs, at the fill time -- new (tmode 0),
+    // old (1) or ta * old + tb * new (StateData::InterpAddBox) -- with the
+    // reflection sign; 0 where no box owns the source
+    int m[3];
+    unsigned odd;
+    map_cell(CD, i, j, k, m, &odd);
+    const int ob = owner_of(CD, cowner, m);
+    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
+    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
+    for (int n = nlo; n < nhi; ++n) {
+        double v = 0.0;
+        if (ob >= 0) {
+            const double vn = ctab[ob].a(m[0], m[1], m[2], n);
+            if (tmode == 0) v = vn;
+            else {
+                const double vo = otab[ob].a(m[0], m[1], m[2], n);
+                v = (tmode == 1) ? vo : ta * vo + tb * vn;
+            }
+        }
+        st(i, j, k, n) = ((odd >> n) & 1u) ? -v : v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0, int flo1, int flo2,
+                  const int *fowner, DomInfo FD) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (fowner) {
@@ -675,36 +703,16 @@ k_ext_from_coarse(Range r, Arr ext, const DevFab *ctab, const int *cowner, DomIn
         off[d] = (sub - half) / ratio;
     }
     const double edge = half / ratio;
-    // the 27 coarse neighbours: source cell, owner and sign mask
-    int mm[27][3], ob[27];
-    unsigned odd[27];
-    for (int a = -1; a <= 1; ++a)
-        for (int b = -1; b <= 1; ++b)
-            for (int d = -1; d <= 1; ++d) {
-                const int t = (a + 1) * 9 + (b + 1) * 3 + (d + 1);
-                map_cell(CD, cc[0] + d, cc[1] + b, cc[2] + a, mm[t], &odd[t]);
-                ob[t] = owner_of(CD, cowner, mm[t]);
-            }
     // grid.y > 1: one component per block row
     const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
     const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
 #pragma unroll 1
     for (int n = nlo; n < nhi; ++n) {
         double u[27];
-        for (int t = 0; t < 27; ++t) {
-            double v = 0.0;
-            if (ob[t] >= 0) {
-                // coarse data at the fill time: new (tmode 0), old (1) or
-                // ta * old + tb * new (StateData::InterpAddBox)
-                const double vn = ctab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n);
-                if (tmode == 0) v = vn;
-                else {
-                    const double vo = otab[ob[t]].a(mm[t][0], mm[t][1], mm[t][2], n);
-                    v = (tmode == 1) ? vo : ta * vo + tb * vn;
-                }
-            }
-            u[t] = ((odd[t] >> n) & 1u) ? -v : v;
-        }
+        for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b)
+                for (int d = -1; d <= 1; ++d)
+                    u[(a + 1) * 9 + (b + 1) * 3 + (d + 1)] = st(cc[0] + d, cc[1] + b, cc[2] + a, n);
         const double u0 = u[13];
         const double um[3] = {u[12], u[10], u[4]};
         const double up[3] = {u[14], u[16], u[22]};
