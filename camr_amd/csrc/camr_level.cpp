@@ -226,11 +226,10 @@ class HydroLevel {
     int hydro_on_workers(double dt);
     // staged engine: the step captured once per buffer parity, replayed
     // with the step's dt in dt_dev_
-    hipGraph_t sgraph_[2] = {nullptr, nullptr};
-    hipGraphExec_t sgexec_[2] = {nullptr, nullptr};
-    bool use_staged_graph() const {
-        return desc_.graph != 0 && !fused_ && !grav_src_values_ && !reg_crse_ && !reg_fine_;
-    }
+    // [buffer parity][post_timestep clean deferred]
+    hipGraph_t sgraph_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraphExec_t sgexec_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    bool use_staged_graph() const { return desc_.graph != 0 && !fused_ && !grav_src_values_; }
     int replay_staged(double dt);
     int clean_all(const DevFab *tab, bool launch_detect);
     template <typename K, typename... A>
@@ -312,8 +311,9 @@ struct FluxReg {
     // per fine box, bit 2d+side: its slab on that side holds boundary cells
     std::vector<int> slabs;
     ~FluxReg() {
-        if (C && C->reg_crse_ == this) C->reg_crse_ = nullptr;
-        if (F && F->reg_fine_ == this) F->reg_fine_ = nullptr;
+        // captured level steps hold the register's launches
+        if (C && C->reg_crse_ == this) { C->release_graphs(); C->reg_crse_ = nullptr; }
+        if (F && F->reg_fine_ == this) { F->release_graphs(); F->reg_fine_ = nullptr; }
         auto fr = [](void *p) { if (p) (void)hipFree(p); };
         for (double *p : crse) fr(p);
         for (double *p : cfp) fr(p);
@@ -1260,10 +1260,12 @@ void HydroLevel::release_graphs() {
         if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
         gexec_[p] = nullptr;
         graph_[p] = nullptr;
-        if (sgexec_[p]) (void)hipGraphExecDestroy(sgexec_[p]);
-        if (sgraph_[p]) (void)hipGraphDestroy(sgraph_[p]);
-        sgexec_[p] = nullptr;
-        sgraph_[p] = nullptr;
+        for (int q = 0; q < 2; ++q) {
+            if (sgexec_[p][q]) (void)hipGraphExecDestroy(sgexec_[p][q]);
+            if (sgraph_[p][q]) (void)hipGraphDestroy(sgraph_[p][q]);
+            sgexec_[p][q] = nullptr;
+            sgraph_[p][q] = nullptr;
+        }
     }
 }
 
@@ -1272,7 +1274,8 @@ void HydroLevel::release_graphs() {
 // scratch are fixed), captured on first use for the current buffer parity.
 int HydroLevel::replay_staged(double dt) {
     const int p = parity();
-    if (!sgexec_[p]) {
+    const int q = defer_post_ ? 1 : 0;
+    if (!sgexec_[p][q]) {
         // the legacy default stream cannot be captured: record on the
         // level's capture stream, replay on its own stream
         if (!gstream_) CAMR_HIP(hipStreamCreateWithFlags(&gstream_, hipStreamNonBlocking));
@@ -1289,10 +1292,10 @@ int HydroLevel::replay_staged(double dt) {
             if (!rc) set_error("capture of the level step failed: %s", hipGetErrorString(e));
             return rc ? rc : CAMR_ERR_HIP;
         }
-        sgraph_[p] = g;
-        CAMR_HIP(hipGraphInstantiate(&sgexec_[p], g, nullptr, nullptr, 0));
+        sgraph_[p][q] = g;
+        CAMR_HIP(hipGraphInstantiate(&sgexec_[p][q], g, nullptr, nullptr, 0));
     }
-    CAMR_HIP(hipGraphLaunch(sgexec_[p], stream_));
+    CAMR_HIP(hipGraphLaunch(sgexec_[p][q], stream_));
     return CAMR_OK;
 }
 

@@ -18,7 +18,7 @@ torch = pytest.importorskip("torch")
 FINE4 = [(16, 16, 16, 31, 47, 47), (32, 16, 16, 47, 47, 47)]
 
 
-def _pair(O, n, boxes, periodic, kw):
+def _pair(O, n, boxes, periodic, kw, graph=False):
     bc = 0 if periodic else 4
     per = (1, 1, 1) if periodic else (0, 0, 0)
     params = dict(cfl=0.3, **kw)
@@ -27,7 +27,7 @@ def _pair(O, n, boxes, periodic, kw):
     prob = Problem(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(1.0 / n,) * 3, bclo=(bc,) * 3,
                    bchi=(bc,) * 3, is_periodic=per, boxes=boxes, params=params)
     ol = O.Level(op, boxes)
-    gl = Level(prob, engine=_lib.ENGINE_STAGED)
+    gl = Level(prob, engine=_lib.ENGINE_STAGED, graph=graph)
     for b, bb in enumerate(boxes):
         U = _field(n, bb[:3], bb[3:])
         ol.set_box(b, U)
@@ -35,18 +35,22 @@ def _pair(O, n, boxes, periodic, kw):
     return ol, gl
 
 
-@pytest.mark.parametrize("ratio,periodic,kw", [
-    (2, True, dict(ppm_type=1)),
-    (2, False, dict(ppm_type=0)),
-    (2, True, dict(do_mol=1)),
-    (4, True, dict(ppm_type=1)),
+@pytest.mark.parametrize("ratio,periodic,kw,graph", [
+    (2, True, dict(ppm_type=1), False),
+    (2, False, dict(ppm_type=0), False),
+    (2, True, dict(do_mol=1), False),
+    (4, True, dict(ppm_type=1), False),
+    (2, True, dict(ppm_type=1), True),
+    (2, True, dict(do_mol=1), True),
 ])
-def test_amr_step_matches_oracle(oracle, ratio, periodic, kw):
-    print(f"two-level step ratio={ratio} periodic={periodic} {kw}", flush=True)
+def test_amr_step_matches_oracle(oracle, ratio, periodic, kw, graph):
+    """graph: every level step captured (per buffer parity and deferred
+    clean) and replayed, the registers' adds inside the graphs."""
+    print(f"two-level step ratio={ratio} periodic={periodic} {kw} graph={graph}", flush=True)
     nc = 16
     fine = FINE2 if ratio == 2 else FINE4
-    oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw)
-    of, gf = _pair(oracle, nc * ratio, fine, periodic, kw)
+    oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw, graph)
+    of, gf = _pair(oracle, nc * ratio, fine, periodic, kw, graph)
     oR = oracle.FluxRegister(oc, of, ratio)
     gR = FluxRegister(gc, gf, ratio)
     oracle.average_down(oc, of, ratio)
@@ -55,7 +59,7 @@ def test_amr_step_matches_oracle(oracle, ratio, periodic, kw):
         L.init_post()
     s0 = composite_sums(gc, gf, ratio, nc * ratio)
     dt = None
-    for s in range(3):
+    for s in range(4):
         if s == 0:
             odt = min(oc.initial_dt(), ratio * of.initial_dt())
             gdt = min(gc.initial_dt(), ratio * gf.initial_dt())
