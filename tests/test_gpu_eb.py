@@ -202,7 +202,7 @@ def test_eb_level_without_cut_cells_matches_plain_level():
     """A level of the EB build whose geometry has no cut cell (the sphere
     lies outside the domain) steps bit for bit like the plain level: the
     covered-cell guards and the 6-cell ghost region leave smooth data
-    alone (SURVEY's "regular boxes must match the non-EB path")."""
+    alone: regular boxes match the non-EB path."""
     from camr_amd import eb
     prob = workloads.smooth_periodic(n=16, ppm_type=0, cfl=0.4, max_grid=8)
     geom = eb.Geometry(prob, eb.Sphere((10.0, 10.0, 10.0), 0.1))

@@ -1,6 +1,6 @@
 """Parity of the gfx950 path with the CPU oracle, through the C-ABI.
 
-Bar: bit-identical FP64 results (the north-star tolerance is 1e-12 relative
+Bar: bit-identical FP64 results (the path's stated tolerance is 1e-12 relative
 L-inf per conserved field; the kernels reproduce the oracle's operation
 order with FMA contraction disabled, so equality is exact).  Reported on
 failure: max relative difference per component.
