@@ -253,7 +253,7 @@ def bench_eb(args) -> int:
     t_geom = time.perf_counter() - t0
     cells = 2 * n * n * n
     init = [workloads.sod_sphere_init(prob, b, geom) for b in range(len(prob.boxes))]
-    L = Level(prob, engine=_lib.ENGINE_STAGED, graph=True)
+    L = Level(prob, engine=_lib.ENGINE_STAGED, graph=True, worker_streams=args.workers)
     L.set_eb(geom)
     kinds = list(L.kinds)
     for b, U in enumerate(init):
@@ -428,8 +428,8 @@ def bench_amr_sync(args) -> int:
     ups = cells_c + r * cells_f  # cell-updates per coarse step
 
     def gpu_pair():
-        C = Level(coarse, engine=_lib.ENGINE_STAGED)
-        F = Level(fine, engine=_lib.ENGINE_STAGED)
+        C = Level(coarse, engine=_lib.ENGINE_STAGED, worker_streams=args.workers)
+        F = Level(fine, engine=_lib.ENGINE_STAGED, worker_streams=args.workers)
         for b in range(len(coarse.boxes)):
             C.set_box(b, workloads.smooth_init(coarse, b))
         for b in range(len(fboxes)):
@@ -630,6 +630,9 @@ def main() -> int:
                          "mol: the n^3 periodic field with the method of lines; "
                          "amr_sync: two-level subcycled steps with flux registers")
     ap.add_argument("--amr-n0", type=int, default=128)
+    ap.add_argument("--workers", type=int, default=0,
+                    help="eb / amr_sync: worker streams of the staged engine (0: the "
+                         "library's default of 4)")
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--eb-n", type=int, default=128,
                     help="eb workload: n of the (2n) x n x n grid")

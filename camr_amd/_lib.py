@@ -74,6 +74,7 @@ class LevelDesc(C.Structure):
         ("fixed_dt", C.c_double), ("stop_time", C.c_double),
         ("nbox", C.c_int32), ("box_lohi", C.POINTER(C.c_int32)),
         ("engine", C.c_int32), ("batched", C.c_int32), ("graph", C.c_int32),
+        ("worker_streams", C.c_int32),
     ]
 
 NSTAGE = 13  # CAMR_NSTAGE: stage ranges of the fused engine

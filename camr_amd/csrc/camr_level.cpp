@@ -197,7 +197,8 @@ class HydroLevel {
     // staged engine: per-box work spread over worker streams (default
     // priority), each with its own q / qaux / srcq and scratch; events join
     // them where boxes read each other's data
-    static constexpr int kWorkers = 4;
+    static constexpr int kWorkers = 16;  // most worker streams
+    int nworkers() const { return desc_.worker_streams > 0 ? desc_.worker_streams : 4; }
     int nws_ = 0;
     hipStream_t wst_[kWorkers] = {};
     hipEvent_t wev_[kWorkers] = {}, mev_ = nullptr;
@@ -491,6 +492,10 @@ int HydroLevel::create() {
         set_error("Invalid CFL factor; must be between zero and one.");  // CAMR.cpp:173-174
         return CAMR_ERR_PARAM;
     }
+    if (desc_.worker_streams < 0 || desc_.worker_streams > kWorkers) {
+        set_error("worker_streams must be 0 (default 4) .. %d", kWorkers);
+        return CAMR_ERR_PARAM;
+    }
     if (desc_.hp.do_mol && desc_.cfl > 0.3 && desc_.fixed_dt <= 0.0) {
         set_error("Invalid CFL factor; must be <= 0.3 when using MOL hydro");  // CAMR.cpp:175-177
         return CAMR_ERR_PARAM;
@@ -658,7 +663,7 @@ int HydroLevel::create() {
             CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
         }
         for (int d = 0; d < 3; ++d)
-            if ((rc = alloc(&flx_[d], NVAR * max_nodes * kWorkers))) return rc;
+            if ((rc = alloc(&flx_[d], NVAR * max_nodes * nworkers()))) return rc;
         size_t sc = 0;
         for (const Box &B : box_)
             sc = std::max(sc, desc_.hp.do_mol ? umdrv_mol_scratch_doubles(B)
@@ -780,7 +785,7 @@ int HydroLevel::eb_force(double f[3]) {
 int HydroLevel::setup_workers(size_t max_g, size_t sc) {
     auto fr = [](double *&p) { if (p) (void)hipFree(p); p = nullptr; };
     fr(q_); fr(qaux_); fr(srcq_);
-    const int nw = (box_.size() > 1 && !grav_src_values_) ? kWorkers : 1;
+    const int nw = (box_.size() > 1 && !grav_src_values_) ? nworkers() : 1;
     qstride_ = max_g;
     sstride_ = (sc + 63) / 64 * 64;
     CAMR_HIP(hipMalloc(&q_, QVAR * max_g * nw * sizeof(double)));
@@ -2290,6 +2295,7 @@ void camr_default_level_desc(camr_level_desc *d) {
     d->engine = 0;
     d->batched = 1;
     d->graph = 0;
+    d->worker_streams = 0;
 }
 
 const char *camr_last_error(void) { return g_err; }

@@ -61,7 +61,7 @@ class Level:
     """One AMR level advanced on the GPU by the C-ABI level driver."""
 
     def __init__(self, prob: Problem, engine: int = L.ENGINE_AUTO, batched: bool = True,
-                 graph: bool = False):
+                 graph: bool = False, worker_streams: int = 0):
         self.prob = prob
         d = L.LevelDesc()
         L.lib().camr_default_level_desc(C.byref(d))
@@ -77,6 +77,8 @@ class Level:
         d.engine = engine
         d.batched = int(bool(batched))
         d.graph = int(bool(graph))
+        if worker_streams:
+            d.worker_streams = int(worker_streams)
         self._desc = d
         h = C.c_void_p()
         L.check(L.lib().camr_level_create(C.byref(d), C.byref(h)))

@@ -193,6 +193,10 @@ typedef struct camr_level_desc {
                                 step (device-side dt, no host round trip per step);
                                 used with the batched fused engine and zero gravity,
                                 otherwise camr_level_run steps on the host */
+    int32_t worker_streams;  /* staged engine: streams the per-box hydro of a
+                                multi-box level is spread over (default 4, at most
+                                16); more than the process's hardware queues
+                                (GPU_MAX_HW_QUEUES) gain nothing */
 } camr_level_desc;
 
 void camr_default_level_desc(camr_level_desc *d);
