@@ -95,7 +95,7 @@ EXPORTS = [
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
     "camr_flux_register_create", "camr_flux_register_destroy", "camr_level_average_down",
-    "camr_amr_step", "camr_write_plotfile", "camr_level_eb_force",
+    "camr_amr_step", "camr_write_plotfile", "camr_level_eb_force", "camr_amr_advance",
 ]
 
 _lib = None
@@ -164,6 +164,7 @@ def lib():
         L.camr_flux_register_destroy.argtypes = [vp]
         L.camr_level_average_down.argtypes = [vp, vp, C.c_int32]
         L.camr_amr_step.argtypes = [vp, vp, vp, C.c_double]
+        L.camr_amr_advance.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int32, C.c_double]
         L.camr_write_plotfile.argtypes = [C.c_char_p, C.POINTER(PlotDesc)]
         L.camr_level_eb_force.argtypes = [vp, C.POINTER(C.c_double)]
         L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,

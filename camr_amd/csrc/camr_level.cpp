@@ -298,6 +298,10 @@ class HydroLevel {
     friend int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out);
     friend int average_down(HydroLevel &C, HydroLevel &F, int ratio);
     friend int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc);
+    friend int amr_advance_level(const std::vector<HydroLevel *> &L, const std::vector<FluxReg *> &R,
+                                 size_t l, double dt);
+    friend int amr_advance(const std::vector<HydroLevel *> &L, const std::vector<FluxReg *> &R,
+                           double dt0);
 };
 
 // The flux register between a coarse and a fine level (see the
@@ -1985,52 +1989,78 @@ int average_down(HydroLevel &C, HydroLevel &F, int ratio) {
     return CAMR_OK;
 }
 
-// One coarse step of the two-level hierarchy with subcycling: reset the
-// register, CAMR_advance on the coarse level (CrseAdd), `ratio` fine steps
-// with ghost data interpolated from the coarse level in time
-// (StateData::InterpAddBox: old / new within (t_new - t_old) 1e-3 of either
-// end, else alpha old + beta new) and space (FillPatch), FineAdd; then the
-// coarse post_timestep: reflux, avgDown, clean_state.
-int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc) {
-    if (R.C != &C || R.F != &F) { set_error("the flux register belongs to other levels"); return CAMR_ERR_PARAM; }
-    if (!(dtc > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
-    if (std::fabs(C.time_ - F.time_) > 1e-12 * std::fmax(1.0, std::fabs(C.time_))) {
-        set_error("the levels are at different times");
-        return CAMR_ERR_PARAM;
-    }
+// One step of level l of a hierarchy with subcycling (Amr::timeStep as
+// CAMR drives it): reset the register to the finer level, CAMR_advance on
+// level l (CrseAdd / FineAdd), then `ratio` steps of level l+1 --
+// recursively, with their own finer levels -- each with its exterior ghost
+// data interpolated from level l in time (StateData::InterpAddBox: old /
+// new within (t_new - t_old) 1e-3 of either end, else alpha old + beta
+// new) and space (FillPatch); then level l's post_timestep (CAMR.cpp:
+// 685-699): reflux, avgDown, clean_state.
+int amr_advance_level(const std::vector<HydroLevel *> &L, const std::vector<FluxReg *> &R,
+                             size_t l, double dt) {
+    HydroLevel &C = *L[l];
+    const bool has_fine = l + 1 < L.size();
     int rc = C.resolve();
     if (rc) return rc;
-    // reset (CAMR_advance.cpp:64-66)
-    for (size_t b = 0; b < C.box_.size(); ++b) {
-        launch_on(k_fill_value, C.box_[b], C.stream_, dense(R.crse[b], C.box_[b]), (int)NVAR, 0.0);
-        launch_on(k_fill_value, C.box_[b], C.stream_, dense(R.cfp[b], C.box_[b]), (int)NVAR, 0.0);
+    if (has_fine) {
+        // reset (CAMR_advance.cpp:64-66)
+        FluxReg &Rg = *R[l];
+        for (size_t b = 0; b < C.box_.size(); ++b) {
+            launch_on(k_fill_value, C.box_[b], C.stream_, dense(Rg.crse[b], C.box_[b]), (int)NVAR, 0.0);
+            launch_on(k_fill_value, C.box_[b], C.stream_, dense(Rg.cfp[b], C.box_[b]), (int)NVAR, 0.0);
+        }
     }
     const double t_old = C.time_;
-    C.defer_post_ = true;
-    rc = C.advance(dtc);
+    C.defer_post_ = has_fine;
+    rc = C.advance(dt);
     C.defer_post_ = false;
-    if (rc) return rc;
+    if (rc || !has_fine) return rc;
+    FluxReg &Rg = *R[l];
+    HydroLevel &F = *L[l + 1];
     const double t_new = C.time_;
-    const double dtf = dtc / R.ratio;
+    const double dtf = dt / Rg.ratio;
     const double teps = (t_new - t_old) * 1.e-3;
-    for (int s = 0; s < R.ratio; ++s) {
+    for (int s = 0; s < Rg.ratio; ++s) {
         const double t = F.time_;
         int tmode = 2;
         if (t > t_old - teps && t < t_old + teps) tmode = 1;
         else if (t > t_new - teps && t < t_new + teps) tmode = 0;
         const double ta = (t_new - t) / (t_new - t_old), tb = (t - t_old) / (t_new - t_old);
-        if ((rc = F.exterior_from(C, R.ratio, tmode, ta, tb, true))) return rc;
-        if ((rc = F.advance(dtf))) return rc;
+        if ((rc = F.exterior_from(C, Rg.ratio, tmode, ta, tb, true))) return rc;
+        if ((rc = amr_advance_level(L, R, l + 1, dtf))) return rc;
     }
     if ((rc = F.resolve())) return rc;
     CAMR_HIP(hipStreamSynchronize(F.stream_));
-    // post_timestep of the coarse level (CAMR.cpp:685-699)
+    // post_timestep of level l (CAMR.cpp:685-699)
     const DevFab *ctn = C.a_is_new_ ? C.tab_a_ : C.tab_b_;
-    C.launch_all(kb_reflux, 0, ctn, (const DevFab *)R.tab_crse, (const DevFab *)R.tab_cfp);
-    if ((rc = average_down(C, F, R.ratio))) return rc;
+    C.launch_all(kb_reflux, 0, ctn, (const DevFab *)Rg.tab_crse, (const DevFab *)Rg.tab_cfp);
+    if ((rc = average_down(C, F, Rg.ratio))) return rc;
     C.clean_all(ctn, true);
     CAMR_HIP(hipGetLastError());
     return CAMR_OK;
+}
+
+// One coarse step of an nlev-level hierarchy; R[l] is the register between
+// levels l and l+1, and all levels must be at the same time.
+int amr_advance(const std::vector<HydroLevel *> &L, const std::vector<FluxReg *> &R, double dt0) {
+    if (L.empty() || R.size() + 1 != L.size()) { set_error("need nlev - 1 flux registers"); return CAMR_ERR_PARAM; }
+    if (!(dt0 > 0.0)) { set_error("dt must be positive"); return CAMR_ERR_DT; }
+    for (size_t l = 0; l + 1 < L.size(); ++l)
+        if (!R[l] || R[l]->C != L[l] || R[l]->F != L[l + 1]) {
+            set_error("flux register %d does not join levels %d and %d", (int)l, (int)l, (int)l + 1);
+            return CAMR_ERR_PARAM;
+        }
+    for (size_t l = 1; l < L.size(); ++l)
+        if (std::fabs(L[l]->time_ - L[0]->time_) > 1e-12 * std::fmax(1.0, std::fabs(L[0]->time_))) {
+            set_error("the levels are at different times");
+            return CAMR_ERR_PARAM;
+        }
+    return amr_advance_level(L, R, 0, dt0);
+}
+
+int amr_step(HydroLevel &C, HydroLevel &F, FluxReg &R, double dtc) {
+    return amr_advance({&C, &F}, {&R}, dtc);
 }
 } // namespace camr
 
@@ -2573,6 +2603,21 @@ int camr_write_plotfile(const char *dir, const camr_plot_desc *d) {
         lev[l] = &d->levels[l]->impl;
     }
     return write_plotfile(dir, *d, lev);
+}
+int camr_amr_advance(camr_level *const *levels, camr_flux_register *const *regs, int32_t nlev,
+                     double dt0) {
+    if (!levels || nlev < 1 || (nlev > 1 && !regs)) { set_error("bad arguments"); return CAMR_ERR_PARAM; }
+    std::vector<HydroLevel *> L(nlev);
+    std::vector<FluxReg *> R(nlev - 1);
+    for (int l = 0; l < nlev; ++l) {
+        if (!levels[l]) { set_error("null level"); return CAMR_ERR_PARAM; }
+        L[l] = &levels[l]->impl;
+    }
+    for (int l = 0; l + 1 < nlev; ++l) {
+        if (!regs[l] || !regs[l]->impl) { set_error("null flux register"); return CAMR_ERR_PARAM; }
+        R[l] = regs[l]->impl;
+    }
+    return amr_advance(L, R, dt0);
 }
 int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r, double dt_coarse) {
     if (!coarse || !fine || coarse == fine || !r || !r->impl) { set_error("bad arguments"); return CAMR_ERR_PARAM; }

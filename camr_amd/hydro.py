@@ -368,3 +368,12 @@ def amr_step(coarse: Level, fine: Level, reg: FluxRegister, dt: float):
     """One subcycled two-level step of coarse size dt (Amr::timeStep with
     CAMR's post_timestep: reflux, avgDown, clean_state)."""
     L.check(L.lib().camr_amr_step(coarse._h, fine._h, reg._h, float(dt)))
+
+
+def amr_advance(levels, regs, dt: float):
+    """One coarse step of a hierarchy (levels coarse first, regs[l] joining
+    levels l and l+1) with subcycling at every level."""
+    n = len(levels)
+    hl = (C.c_void_p * n)(*[lv._h.value for lv in levels])
+    hr = (C.c_void_p * max(n - 1, 1))(*([r._h.value for r in regs] or [None]))
+    L.check(L.lib().camr_amr_advance(hl, hr, n, float(dt)))

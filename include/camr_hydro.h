@@ -269,6 +269,15 @@ int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio)
 int camr_amr_step(camr_level *coarse, camr_level *fine, camr_flux_register *r,
                   double dt_coarse);
 
+/* The same for a hierarchy of nlev levels (coarse first, regs[l] joining
+ * levels l and l+1): one step of level 0 and, recursively, `ratio` steps
+ * of each finer level per step of its coarser one, every level's
+ * post_timestep after its finer levels' steps (Amr::timeStep's
+ * subcycling).  A level with a finer and a coarser neighbour feeds both
+ * registers. */
+int camr_amr_advance(camr_level *const *levels, camr_flux_register *const *regs, int32_t nlev,
+                     double dt0);
+
 /*
  * AMReX plotfile of a level hierarchy (CAMR::writePlotFile, Source/Utils/
  * CAMR_io.cpp:407-591, with Amr's header lines): dir/Header,

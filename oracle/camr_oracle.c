@@ -3425,27 +3425,34 @@ int ora_level_average_down(ora_level *C, const ora_level *F, int ratio) {
     return 0;
 }
 
-/* One coarse step of a two-level hierarchy with subcycling (Amr::timeStep
- * as CAMR drives it): reset the register and advance level 0 (CAMR_advance,
- * CrseAdd), then `ratio` steps of level 1 whose ghost data come from level
- * 0 interpolated in time (StateData::InterpAddBox: old / new within
+/* One step of level l of a hierarchy with subcycling (Amr::timeStep as CAMR
+ * drives it): reset the register to the finer level and advance level l
+ * (CAMR_advance, CrseAdd / FineAdd), then `ratio` steps of level l+1 --
+ * recursively, with their own finer levels -- whose ghost data come from
+ * level l interpolated in time (StateData::InterpAddBox: old / new within
  * (t_new - t_old) 1e-3 of either end, else alpha old + beta new) and in
- * space (FillPatch), with FineAdd; then level 0's post_timestep (CAMR.cpp:
- * 679-699): reflux, avgDown, clean_state. */
-int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
-    if (!R || R->C != C || R->F != F) return 1;
-    const int r = R->ratio;
-    for (int b = 0; b < C->nbox; ++b) {
-        memset(R->crse[b].p, 0, sizeof(double) * (size_t)(R->crse[b].ns * O_NVAR));
-        memset(R->cfp[b].p, 0, sizeof(double) * (size_t)(R->cfp[b].ns * O_NVAR));
+ * space (FillPatch); then level l's post_timestep (CAMR.cpp:679-699):
+ * reflux, avgDown, clean_state. */
+static int amr_advance(ora_level **L, ora_fluxreg **R, int nlev, int l, double dt) {
+    ora_level *C = L[l];
+    const int has_fine = l + 1 < nlev;
+    if (has_fine) {
+        ora_fluxreg *Rg = R[l];
+        for (int b = 0; b < C->nbox; ++b) {
+            memset(Rg->crse[b].p, 0, sizeof(double) * (size_t)(Rg->crse[b].ns * O_NVAR));
+            memset(Rg->cfp[b].p, 0, sizeof(double) * (size_t)(Rg->cfp[b].ns * O_NVAR));
+        }
     }
     const double t_old = C->time;
-    C->defer_post = 1;
-    int err = ora_level_advance(C, dtc);
+    C->defer_post = has_fine;
+    int err = ora_level_advance(C, dt);
     C->defer_post = 0;
-    if (err) return err;
+    if (err || !has_fine) return err;
+    ora_fluxreg *Rg = R[l];
+    ora_level *F = L[l + 1];
+    const int r = Rg->ratio;
     const double t_new = C->time;
-    const double dtf = dtc / r;
+    const double dtf = dt / r;
     const double teps = (t_new - t_old) * 1.e-3;
     for (int s = 0; s < r; ++s) {
         const double t = F->time;
@@ -3453,7 +3460,7 @@ int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
         if (t > t_old - teps && t < t_old + teps) tw.mode = 1;
         else if (t > t_new - teps && t < t_new + teps) tw.mode = 0;
         if (exterior_ghosts_from_coarse_t(F, C, r, tw)) return 1;
-        err = ora_level_advance(F, dtf);
+        err = amr_advance(L, R, nlev, l + 1, dtf);
         if (err) return err;
     }
     for (int b = 0; b < C->nbox; ++b) {
@@ -3462,12 +3469,26 @@ int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
         for (int k = B.lo[2]; k <= B.hi[2]; ++k)
         for (int j = B.lo[1]; j <= B.hi[1]; ++j)
         for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
-            double *d = &FA(&R->crse[b], i, j, k, n);
-            *d += FA(&R->cfp[b], i, j, k, n);
+            double *d = &FA(&Rg->crse[b], i, j, k, n);
+            *d += FA(&Rg->cfp[b], i, j, k, n);
             FA(&C->Snew[b], i, j, k, n) += *d;
         }
     }
     ora_level_average_down(C, F, r);
     clean_state(C, C->Snew, 0);
     return C->err;
+}
+
+/* One coarse step of an nlev-level hierarchy; R[l] is the register between
+ * levels l and l+1. */
+int ora_amr_advance(ora_level **L, ora_fluxreg **R, int nlev, double dt0) {
+    for (int l = 0; l + 1 < nlev; ++l)
+        if (!R[l] || R[l]->C != L[l] || R[l]->F != L[l + 1]) return 1;
+    return amr_advance(L, R, nlev, 0, dt0);
+}
+
+int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
+    ora_level *L[2] = {C, F};
+    ora_fluxreg *Rs[1] = {R};
+    return ora_amr_advance(L, Rs, 2, dtc);
 }

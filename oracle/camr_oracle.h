@@ -158,6 +158,7 @@ ora_fluxreg *ora_fluxreg_create(ora_level *C, ora_level *F, int ratio);
 void   ora_fluxreg_destroy(ora_fluxreg *R);
 int    ora_level_average_down(ora_level *C, const ora_level *F, int ratio);
 int    ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dt_coarse);
+int    ora_amr_advance(ora_level **L, ora_fluxreg **R, int nlev, double dt0);
 int    ora_level_eb_force(ora_level *L, double f[3]);
 int    ora_level_exterior_from_coarse_time(ora_level *F, const ora_level *C, int ratio,
                                            double t_old, double t_new, double t);

@@ -123,6 +123,9 @@ def lib():
         L.ora_level_average_down.restype = C.c_int
         L.ora_amr_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]
         L.ora_amr_step.restype = C.c_int
+        L.ora_amr_advance.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int,
+                                      C.c_double]
+        L.ora_amr_advance.restype = C.c_int
         L.ora_level_eb_force.argtypes = [C.c_void_p, dp]
         L.ora_level_eb_force.restype = C.c_int
         L.ora_level_exterior_from_coarse_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_double] * 3
@@ -325,6 +328,15 @@ def average_down(C: Level, F: Level, ratio: int):
 def amr_step(C: Level, F: Level, R: FluxRegister, dt: float) -> int:
     """One subcycled two-level step of coarse size dt (fine steps dt/ratio)."""
     return lib().ora_amr_step(C._h, F._h, R._h, dt)
+
+
+def amr_advance(levels, regs, dt0: float) -> int:
+    """One coarse step of a hierarchy (levels coarse first, regs[l] between
+    levels l and l+1) with subcycling at every level."""
+    n = len(levels)
+    hl = (C.c_void_p * n)(*[lv._h for lv in levels])
+    hr = (C.c_void_p * max(n - 1, 1))(*([r._h for r in regs] or [None]))
+    return lib().ora_amr_advance(hl, hr, n, dt0)
 
 
 def set_threads(n: int):

@@ -178,3 +178,80 @@ def test_coarse_cell_bordering_two_fine_boxes_is_refused(oracle):
     F = oracle.Level(_params(oracle, 32), [(8, 8, 8, 23, 23, 23), (24, 8, 8, 31, 15, 23)])
     with pytest.raises(ValueError):
         oracle.FluxRegister(C, F, 2)
+
+
+FINE3 = [(24, 24, 24, 31, 39, 39), (32, 24, 24, 39, 39, 39)]  # level-1 cells [12, 19]^3
+
+
+def three_levels(O, **kw):
+    """Levels 0 (16^3, 8^3 boxes), 1 (FINE2 at ratio 2) and 2 (FINE3 at
+    ratio 2, nested four level-1 cells inside level 1)."""
+    L0 = O.Level(_params(O, 16, **kw), _boxes(16, 8))
+    L1 = O.Level(_params(O, 32, **kw), FINE2)
+    L2 = O.Level(_params(O, 64, **kw), FINE3)
+    for L, n in ((L0, 16), (L1, 32), (L2, 64)):
+        for b, bb in enumerate(L.boxes):
+            L.set_box(b, _field(n, bb[:3], bb[3:]))
+    R0 = O.FluxRegister(L0, L1, 2)
+    R1 = O.FluxRegister(L1, L2, 2)
+    O.average_down(L1, L2, 2)
+    O.average_down(L0, L1, 2)
+    for L in (L0, L1, L2):
+        L.init_post()
+    return [L0, L1, L2], [R0, R1]
+
+
+def composite_sums3(levels):
+    """Composite sums over three levels in finest-cell volumes."""
+    tot = np.zeros(NV)
+    covered = [None] + [np.zeros((16 * 2 ** (l - 1),) * 3, dtype=bool) for l in (1, 2)]
+    for l in (1, 2):
+        for bb in levels[l].boxes:
+            covered[l][bb[2] // 2:(bb[5] + 1) // 2, bb[1] // 2:(bb[4] + 1) // 2,
+                       bb[0] // 2:(bb[3] + 1) // 2] = True
+    for l, L in enumerate(levels):
+        w = 8 ** (2 - l)
+        for b, bb in enumerate(L.boxes):
+            S = L.get_box(b)
+            if l < 2:
+                m = ~covered[l + 1][bb[2]:bb[5] + 1, bb[1]:bb[4] + 1, bb[0]:bb[3] + 1]
+                tot += w * S[:, m].sum(axis=1)
+            else:
+                tot += S.reshape(NV, -1).sum(axis=1)
+    return tot
+
+
+def run3(O, levels, regs, nsteps):
+    dt = None
+    for s in range(nsteps):
+        ests = [L.initial_dt() if s == 0 else L.new_dt(dt / 2 ** l) for l, L in enumerate(levels)]
+        dt = min(e * 2 ** l for l, e in enumerate(ests))
+        assert O.amr_advance(levels, regs, dt) == 0
+    return dt
+
+
+def test_three_level_composite_conservation(oracle):
+    levels, regs = three_levels(oracle, ppm_type=1)
+    s0 = composite_sums3(levels)
+    run3(oracle, levels, regs, 2)
+    s1 = composite_sums3(levels)
+    scale = np.abs(levels[2].get_box(0)).reshape(NV, -1).sum(axis=1) * 8
+    for c in (0, 1, 2, 3, 4):
+        assert abs(s1[c] - s0[c]) <= 1e-12 * max(scale[c], 1.0), (c, s1[c] - s0[c])
+    t = [L.time for L in levels]
+    assert abs(t[0] - t[1]) <= 1e-15 * t[0] and abs(t[0] - t[2]) <= 1e-15 * t[0]
+
+
+def test_two_level_advance_equals_amr_step(oracle):
+    """amr_advance over two levels is the two-level step, bit for bit."""
+    outs = []
+    for use_general in (False, True):
+        C, F, R = hierarchy(oracle)
+        dt = min(C.initial_dt(), 2 * F.initial_dt())
+        if use_general:
+            assert oracle.amr_advance([C, F], [R], dt) == 0
+        else:
+            assert oracle.amr_step(C, F, R, dt) == 0
+        outs.append([C.get_box(b) for b in range(len(C.boxes))] +
+                    [F.get_box(b) for b in range(len(F.boxes))])
+    assert all(np.array_equal(a, b) for a, b in zip(*outs))
