@@ -46,6 +46,12 @@ class PlotDesc(C.Structure):
                 ("plot_rhoy", C.c_int32), ("derive_vars", C.c_char_p)]
 
 
+class CheckpointDesc(C.Structure):
+    _fields_ = [("nlev", C.c_int32), ("levels", C.POINTER(C.c_void_p)),
+                ("ref_ratio", C.POINTER(C.c_int32)), ("level_steps", C.POINTER(C.c_int32)),
+                ("dt_level", C.POINTER(C.c_double))]
+
+
 class HydroParams(C.Structure):
     _fields_ = [
         ("ppm_type", C.c_int32), ("plm_iorder", C.c_int32),
@@ -96,6 +102,7 @@ EXPORTS = [
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
     "camr_flux_register_create", "camr_flux_register_destroy", "camr_level_average_down",
     "camr_amr_step", "camr_write_plotfile", "camr_level_eb_force", "camr_amr_advance",
+    "camr_write_checkpoint", "camr_level_set_time",
 ]
 
 _lib = None
@@ -166,6 +173,8 @@ def lib():
         L.camr_amr_step.argtypes = [vp, vp, vp, C.c_double]
         L.camr_amr_advance.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int32, C.c_double]
         L.camr_write_plotfile.argtypes = [C.c_char_p, C.POINTER(PlotDesc)]
+        L.camr_write_checkpoint.argtypes = [C.c_char_p, C.POINTER(CheckpointDesc)]
+        L.camr_level_set_time.argtypes = [vp, C.c_double]
         L.camr_level_eb_force.argtypes = [vp, C.POINTER(C.c_double)]
         L.camr_hydro_umdrv_eb.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4,
                                           C.POINTER(EbGeom), C.c_double,

@@ -147,6 +147,12 @@ class HydroLevel {
     int sync();
     void set_stream(hipStream_t s) { stream_ = s; }
     double time() const { return time_; }
+    int set_time(double t) {
+        int rc = resolve();
+        if (rc) return rc;
+        time_ = t;
+        return CAMR_OK;
+    }
     const camr_level_desc &desc() const { return desc_; }
     const std::vector<Box> &boxes() const { return box_; }
     bool is_eb() const { return ebflags_ != nullptr; }
@@ -2134,6 +2140,59 @@ bool make_dir(const std::string &d) {
     return mkdir(d.c_str(), 0755) == 0;
 }
 
+// A level's MultiFab in VisMF's layout: ldir/<name>_D_00000 holds per box
+// a FAB header line and the data (component after component, x fastest,
+// little-endian IEEE doubles); ldir/<name>_H the boxes, byte offsets and
+// per-box min / max.  fill(u, np, out) turns a box's S_new (NVAR
+// components of np cells) into its nvars output components.
+template <typename Fill>
+int write_vismf(const std::string &ldir, const char *name, HydroLevel &L, int nvars, Fill fill) {
+    const std::vector<Box> &bx = L.boxes();
+    const std::string dname = std::string(name) + "_D_00000";
+    std::ofstream D(ldir + "/" + dname, std::ios::binary);
+    if (!D) { set_error("cannot write %s/%s", ldir.c_str(), dname.c_str()); return CAMR_ERR_PARAM; }
+    std::vector<long long> offs;
+    std::vector<std::vector<double>> mn, mx;
+    std::vector<double> u, out;
+    for (size_t b = 0; b < bx.size(); ++b) {
+        const size_t np = bx[b].npts();
+        u.resize(NVAR * np);
+        int rc = L.download((int)b, u.data());
+        if (rc) return rc;
+        fill(u, np, out);
+        offs.push_back((long long)D.tellp());
+        D << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(bx[b]) << ' '
+          << nvars << '\n';
+        D.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)(out.size() * sizeof(double)));
+        std::vector<double> lo(nvars), hi(nvars);
+        for (int v = 0; v < nvars; ++v) {
+            double a = out[(size_t)v * np], z = a;
+            for (size_t p = 1; p < np; ++p) {
+                a = std::min(a, out[(size_t)v * np + p]);
+                z = std::max(z, out[(size_t)v * np + p]);
+            }
+            lo[v] = a;
+            hi[v] = z;
+        }
+        mn.push_back(lo);
+        mx.push_back(hi);
+    }
+    if (!D) { set_error("write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+    std::ofstream V(ldir + "/" + name + "_H");
+    V.precision(17);
+    V << 1 << '\n' << 1 << '\n' << nvars << '\n' << 0 << '\n';
+    V << '(' << bx.size() << " 0\n";
+    for (const Box &b : bx) V << box_str(b) << '\n';
+    V << ")\n" << bx.size() << '\n';
+    for (size_t b = 0; b < bx.size(); ++b) V << "FabOnDisk: " << dname << ' ' << offs[b] << '\n';
+    V << '\n' << bx.size() << ',' << nvars << '\n';
+    for (auto &r : mn) { for (double v : r) V << v << ','; V << '\n'; }
+    V << '\n' << bx.size() << ',' << nvars << '\n';
+    for (auto &r : mx) { for (double v : r) V << v << ','; V << '\n'; }
+    if (!V) { set_error("write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+    return CAMR_OK;
+}
+
 }  // namespace
 
 int write_plotfile(const char *dir, const camr_plot_desc &pd, const std::vector<HydroLevel *> &lev) {
@@ -2208,16 +2267,8 @@ int write_plotfile(const char *dir, const camr_plot_desc &pd, const std::vector<
                 H << pd.prob_lo[d] + dx[d] * b.lo[d] << ' ' << pd.prob_lo[d] + dx[d] * (b.hi[d] + 1) << '\n';
         H << lname << "/Cell\n";
 
-        std::ofstream D(ldir + "/Cell_D_00000", std::ios::binary);
-        if (!D) { set_error("plotfile: cannot write %s", ldir.c_str()); return CAMR_ERR_PARAM; }
-        std::vector<long long> offs;
-        std::vector<std::vector<double>> mn, mx;
-        std::vector<double> u, out;
-        for (size_t b = 0; b < bx.size(); ++b) {
-            const size_t np = bx[b].npts();
-            u.resize(NVAR * np);
-            int rc = L.download((int)b, u.data());
-            if (rc) return rc;
+        int rc = write_vismf(ldir, "Cell", L, nvars, [&](const std::vector<double> &u, size_t np,
+                                                         std::vector<double> &out) {
             out.assign((size_t)nvars * np, 0.0);
             int c = 0;
             for (int n : comps) {
@@ -2233,36 +2284,8 @@ int write_plotfile(const char *dir, const camr_plot_desc &pd, const std::vector<
                 }
                 ++c;
             }
-            offs.push_back((long long)D.tellp());
-            D << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(bx[b]) << ' '
-              << nvars << '\n';
-            D.write(reinterpret_cast<const char *>(out.data()), (std::streamsize)(out.size() * sizeof(double)));
-            std::vector<double> lo(nvars), hi(nvars);
-            for (int v = 0; v < nvars; ++v) {
-                double a = out[(size_t)v * np], z = a;
-                for (size_t p = 1; p < np; ++p) {
-                    a = std::min(a, out[(size_t)v * np + p]);
-                    z = std::max(z, out[(size_t)v * np + p]);
-                }
-                lo[v] = a;
-                hi[v] = z;
-            }
-            mn.push_back(lo);
-            mx.push_back(hi);
-        }
-        if (!D) { set_error("plotfile: write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
-        std::ofstream V(ldir + "/Cell_H");
-        V.precision(17);
-        V << 1 << '\n' << 1 << '\n' << nvars << '\n' << 0 << '\n';
-        V << '(' << bx.size() << " 0\n";
-        for (const Box &b : bx) V << box_str(b) << '\n';
-        V << ")\n" << bx.size() << '\n';
-        for (size_t b = 0; b < bx.size(); ++b) V << "FabOnDisk: Cell_D_00000 " << offs[b] << '\n';
-        V << '\n' << bx.size() << ',' << nvars << '\n';
-        for (auto &r : mn) { for (double v : r) V << v << ','; V << '\n'; }
-        V << '\n' << bx.size() << ',' << nvars << '\n';
-        for (auto &r : mx) { for (double v : r) V << v << ','; V << '\n'; }
-        if (!V) { set_error("plotfile: write failed in %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+        });
+        if (rc) return rc;
     }
     std::ofstream J(root + "/job_info");
     J << "===============================================================================\n"
@@ -2272,6 +2295,49 @@ int write_plotfile(const char *dir, const camr_plot_desc &pd, const std::vector<
       << "levels: " << pd.nlev << "\n"
       << "time: " << std::setprecision(17) << lev[0]->time() << "\n";
     if (!H) { set_error("plotfile: header write failed"); return CAMR_ERR_PARAM; }
+    return CAMR_OK;
+}
+
+// A checkpoint of a level hierarchy for restart (CAMR::checkPoint,
+// Source/Utils/CAMR_io.cpp:45-84: AmrLevel's state data plus a CPUtime
+// file), in the directory layout of AMReX checkpoints: dir/Header with the
+// hierarchy (time, ratios, per-level dt and steps, boxes), the new-time
+// state of level l in VisMF layout as dir/Level_<l>/SD_0_New_H / _D_00000,
+// all NVAR components bit for bit, and dir/CPUtime.
+int write_checkpoint(const char *dir, const camr_checkpoint_desc &cd,
+                     const std::vector<HydroLevel *> &lev) {
+    const std::string root(dir);
+    if (!make_dir(root)) { set_error("checkpoint: cannot create %s", dir); return CAMR_ERR_PARAM; }
+    std::ofstream H(root + "/Header");
+    if (!H) { set_error("checkpoint: cannot write %s/Header", dir); return CAMR_ERR_PARAM; }
+    H.precision(17);
+    const int f_lev = cd.nlev - 1;
+    H << "CheckPointVersion_1.0\n" << 3 << '\n' << lev[0]->time() << '\n' << f_lev << '\n';
+    for (int l = 0; l <= f_lev; ++l)
+        H << box_str(make_box(lev[l]->desc().hp.domlo, lev[l]->desc().hp.domhi)) << ' ';
+    H << '\n';
+    for (int l = 0; l < f_lev; ++l) H << cd.ref_ratio[l] << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l) H << (cd.dt_level ? cd.dt_level[l] : 0.0) << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l) H << (cd.level_steps ? cd.level_steps[l] : 0) << ' ';
+    H << '\n';
+    for (int l = 0; l <= f_lev; ++l) {
+        HydroLevel &L = *lev[l];
+        char lname[32];
+        snprintf(lname, sizeof lname, "Level_%d", l);
+        const std::string ldir = root + "/" + lname;
+        if (!make_dir(ldir)) { set_error("checkpoint: cannot create %s", ldir.c_str()); return CAMR_ERR_PARAM; }
+        H << l << ' ' << L.boxes().size() << ' ' << L.time() << '\n';
+        for (const Box &b : L.boxes()) H << box_str(b) << '\n';
+        H << lname << "/SD_0_New\n";
+        int rc = write_vismf(ldir, "SD_0_New", L, NVAR,
+                             [](const std::vector<double> &u, size_t, std::vector<double> &out) { out = u; });
+        if (rc) return rc;
+    }
+    std::ofstream T(root + "/CPUtime");
+    T << std::setprecision(15) << 0.0;
+    if (!H || !T) { set_error("checkpoint: write failed"); return CAMR_ERR_PARAM; }
     return CAMR_OK;
 }
 }  // namespace camr
@@ -2594,6 +2660,20 @@ int camr_flux_register_destroy(camr_flux_register *r) {
 int camr_level_average_down(camr_level *coarse, camr_level *fine, int32_t ratio) {
     if (!coarse || !fine || coarse == fine) { set_error("bad level arguments"); return CAMR_ERR_PARAM; }
     return average_down(coarse->impl, fine->impl, ratio);
+}
+int camr_write_checkpoint(const char *dir, const camr_checkpoint_desc *d) {
+    if (!d || !dir || d->nlev < 1 || !d->levels) { set_error("checkpoint: bad arguments"); return CAMR_ERR_PARAM; }
+    if (d->nlev > 1 && !d->ref_ratio) { set_error("checkpoint: ref_ratio needed"); return CAMR_ERR_PARAM; }
+    std::vector<HydroLevel *> lev(d->nlev);
+    for (int l = 0; l < d->nlev; ++l) {
+        if (!d->levels[l]) { set_error("checkpoint: null level"); return CAMR_ERR_PARAM; }
+        lev[l] = &d->levels[l]->impl;
+    }
+    return write_checkpoint(dir, *d, lev);
+}
+int camr_level_set_time(camr_level *L, double t) {
+    if (!L) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    return L->impl.set_time(t);
 }
 int camr_write_plotfile(const char *dir, const camr_plot_desc *d) {
     if (!d || !dir || d->nlev < 1 || !d->levels) { set_error("plotfile: bad arguments"); return CAMR_ERR_PARAM; }

@@ -1,7 +1,9 @@
-"""AMReX plotfiles of level hierarchies: the writer is the library's
-``camr_write_plotfile`` (CAMR::writePlotFile, Source/Utils/CAMR_io.cpp:
-407-591); the reader here parses the same layout (Header, Level_<l>/Cell_H,
-Level_<l>/Cell_D_*) into numpy arrays, for checks and post-processing."""
+"""AMReX plotfiles and checkpoints of level hierarchies: the writers are the
+library's ``camr_write_plotfile`` (CAMR::writePlotFile, Source/Utils/
+CAMR_io.cpp:407-591) and ``camr_write_checkpoint`` (CAMR::checkPoint,
+:45-84); the readers here parse the same layouts (Header, VisMF
+``<name>_H`` / ``<name>_D_*``) into numpy arrays, and ``restart`` loads a
+checkpoint back into levels."""
 from __future__ import annotations
 
 import ctypes as C
@@ -45,6 +47,27 @@ def _boxes(text):
     return [tuple(int(x) for x in m.groups()[:6]) for m in _BOX.finditer(text)]
 
 
+def _read_vismf(path, rel, ncomp_expected):
+    """Boxes and per-box arrays (ncomp, nz, ny, nx) of the VisMF MultiFab
+    `rel` (relative to `path`)."""
+    with open(os.path.join(path, rel + "_H")) as f:
+        vh = f.read()
+    boxes = _boxes(vh)
+    offs = [(m.group(1), int(m.group(2))) for m in re.finditer(r"FabOnDisk: (\S+) (\d+)", vh)]
+    data = []
+    for (fname, off), b in zip(offs, boxes):
+        with open(os.path.join(path, os.path.dirname(rel), fname), "rb") as f:
+            f.seek(off)
+            head = f.readline().decode()
+            fb = _boxes(head)[0]
+            ncomp = int(head.rsplit(")", 1)[1])
+            assert fb == b and ncomp == ncomp_expected
+            n = [b[3 + d] - b[d] + 1 for d in range(3)]
+            a = np.fromfile(f, dtype="<f8", count=ncomp * n[0] * n[1] * n[2])
+            data.append(a.reshape(ncomp, n[2], n[1], n[0]))
+    return boxes, data
+
+
 def read_plotfile(path):
     """Parse a plotfile into a dict: names, time, finest_level, prob_lo,
     prob_hi, ref_ratio, domains, level_steps, dx, and per level its time,
@@ -73,22 +96,63 @@ def read_plotfile(path):
         steps = int(next(it))
         locs = [[float(x) for x in next(it).split()] for _ in range(3 * int(nbox))]
         rel = next(it)
-        with open(os.path.join(path, rel + "_H")) as f:
-            vh = f.read()
-        boxes = _boxes(vh)
-        offs = [(m.group(1), int(m.group(2))) for m in re.finditer(r"FabOnDisk: (\S+) (\d+)", vh)]
-        data = []
-        for (fname, off), b in zip(offs, boxes):
-            with open(os.path.join(path, os.path.dirname(rel), fname), "rb") as f:
-                f.seek(off)
-                head = f.readline().decode()
-                fb = _boxes(head)[0]
-                ncomp = int(head.rsplit(")", 1)[1])
-                assert fb == b and ncomp == nv
-                n = [b[3 + d] - b[d] + 1 for d in range(3)]
-                a = np.fromfile(f, dtype="<f8", count=ncomp * n[0] * n[1] * n[2])
-                data.append(a.reshape(ncomp, n[2], n[1], n[0]))
+        boxes, data = _read_vismf(path, rel, nv)
         levels.append({"level": int(lev), "time": float(t), "steps": steps, "boxes": boxes,
                        "grid_locations": locs, "data": data})
     out["levels"] = levels
     return out
+
+
+def write_checkpoint(path, levels, ref_ratio=(), level_steps=None, dt_level=None):
+    """Write a restart checkpoint of `levels` (coarse first)."""
+    n = len(levels)
+    hs = (C.c_void_p * n)(*[lv._h for lv in levels])
+    rr = (C.c_int32 * max(n - 1, 1))(*(list(ref_ratio) or [0]))
+    st = (C.c_int32 * n)(*(level_steps or [0] * n))
+    dts = (C.c_double * n)(*(dt_level or [0.0] * n))
+    d = L.CheckpointDesc()
+    d.nlev = n
+    d.levels = C.cast(hs, C.POINTER(C.c_void_p))
+    d.ref_ratio = C.cast(rr, C.POINTER(C.c_int32))
+    d.level_steps = C.cast(st, C.POINTER(C.c_int32))
+    d.dt_level = C.cast(dts, C.POINTER(C.c_double))
+    L.check(L.lib().camr_write_checkpoint(os.fsencode(path), C.byref(d)))
+
+
+def read_checkpoint(path):
+    """Parse a checkpoint: time, finest_level, domains, ref_ratio, dt_level,
+    level_steps, and per level its time, boxes and state arrays."""
+    with open(os.path.join(path, "Header")) as f:
+        it = iter(f.read().split("\n"))
+    out = {"version": next(it)}
+    assert int(next(it)) == 3
+    out["time"] = float(next(it))
+    fl = int(next(it))
+    out["finest_level"] = fl
+    out["domains"] = _boxes(next(it))
+    out["ref_ratio"] = [int(x) for x in next(it).split()]
+    out["dt_level"] = [float(x) for x in next(it).split()]
+    out["level_steps"] = [int(x) for x in next(it).split()]
+    levels = []
+    for _ in range(fl + 1):
+        lev, nbox, t = next(it).split()
+        hdr_boxes = [_boxes(next(it))[0] for _ in range(int(nbox))]
+        boxes, data = _read_vismf(path, next(it), 10)
+        assert boxes == hdr_boxes
+        levels.append({"level": int(lev), "time": float(t), "boxes": boxes, "data": data})
+    out["levels"] = levels
+    return out
+
+
+def restart(levels, path):
+    """Load checkpoint `path` into `levels` (created with the same boxes):
+    every box's state and each level's time.  Returns the checkpoint's
+    dt_level and level_steps."""
+    ck = read_checkpoint(path)
+    assert len(ck["levels"]) == len(levels)
+    for lv, rec in zip(levels, ck["levels"]):
+        assert [tuple(b) for b in lv.boxes] == rec["boxes"]
+        for b, a in enumerate(rec["data"]):
+            lv.set_box(b, a)
+        L.check(L.lib().camr_level_set_time(lv._h, rec["time"]))
+    return ck["dt_level"], ck["level_steps"]

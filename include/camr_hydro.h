@@ -299,6 +299,26 @@ typedef struct camr_plot_desc {
 } camr_plot_desc;
 int camr_write_plotfile(const char *dir, const camr_plot_desc *d);
 
+/*
+ * Checkpoint of a level hierarchy for restart (CAMR::checkPoint,
+ * Source/Utils/CAMR_io.cpp:45-84: AmrLevel's state data and a CPUtime
+ * file), in the directory layout of AMReX checkpoints: dir/Header (time,
+ * ratios, per-level dt and steps, domains and boxes), the new-time state of
+ * each level in VisMF layout (dir/Level_<l>/SD_0_New_H and _D_00000, all
+ * NVAR components bit for bit) and dir/CPUtime.  Restart: upload each box's
+ * state (camr_level_upload_box) and the level time (camr_level_set_time).
+ */
+typedef struct camr_checkpoint_desc {
+    int32_t nlev;
+    camr_level *const *levels;     /* coarse first */
+    const int32_t *ref_ratio;      /* nlev - 1 ratios */
+    const int32_t *level_steps;    /* nlev, or NULL */
+    const double *dt_level;        /* nlev, the last dt of each level, or NULL */
+} camr_checkpoint_desc;
+int camr_write_checkpoint(const char *dir, const camr_checkpoint_desc *d);
+/* Set a level's time (restart). */
+int camr_level_set_time(camr_level *L, double t);
+
 /* The embedded-boundary build of a level (AMREX_USE_EB: CAMR.cpp:303-329,
  * 419-421; CAMR_construct_hydro_source.cpp:77-204): ghost width 6, covered
  * cells skipped by clean_state and the dt estimate, and per box the path of
