@@ -405,7 +405,8 @@ def bench_amr_sync(args) -> int:
     """Two-level subcycled step with flux registers (camr_amr_step): a
     periodic n0^3 coarse level and a fine block over the middle half of the
     domain in each direction at ratio 2, both in 32^3 boxes, PPM at CFL
-    0.4.  One coarse step = the coarse advance (CrseAdd), two fine steps
+    0.4, on the fused engine (--engine staged for the per-stage kernels).
+    One coarse step = the coarse advance (CrseAdd), two fine steps
     with time-interpolated coarse-fine ghosts (FineAdd), reflux,
     average_down and the coarse clean.  The first step is checked bit for
     bit against the oracle (timed on the host cores as the CPU baseline)."""
@@ -428,8 +429,9 @@ def bench_amr_sync(args) -> int:
     ups = cells_c + r * cells_f  # cell-updates per coarse step
 
     def gpu_pair():
-        C = Level(coarse, engine=_lib.ENGINE_STAGED, worker_streams=args.workers)
-        F = Level(fine, engine=_lib.ENGINE_STAGED, worker_streams=args.workers)
+        eng = _lib.ENGINE_FUSED if args.engine == "fused" else _lib.ENGINE_STAGED
+        C = Level(coarse, engine=eng, worker_streams=args.workers)
+        F = Level(fine, engine=eng, worker_streams=args.workers)
         for b in range(len(coarse.boxes)):
             C.set_box(b, workloads.smooth_init(coarse, b))
         for b in range(len(fboxes)):
@@ -492,7 +494,7 @@ def bench_amr_sync(args) -> int:
            "config": {"workload": f"{n0}^3 periodic coarse level + fine block "
                                   f"[{lo},{hi - 1}]^3 at ratio 2, 32^3 boxes, PPM, CFL {cfl}; "
                                   "one step = coarse step + 2 fine steps + reflux + average_down",
-                      "coarse_cells": cells_c, "fine_cells": cells_f,
+                      "engine": args.engine, "coarse_cells": cells_c, "fine_cells": cells_f,
                       "cell_updates_per_step": ups, "boxes": [len(coarse.boxes), len(fboxes)]},
            "identical_to_oracle_1_step": identical,
            "roofline": {"bound": "hbm", "achieved": 160.0 * value / 1e9, "peak": HBM_PEAK_GBS,

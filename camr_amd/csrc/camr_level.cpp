@@ -300,6 +300,9 @@ class HydroLevel {
     FluxReg *reg_crse_ = nullptr, *reg_fine_ = nullptr;
     bool defer_post_ = false;  // leave post_timestep's clean_state to amr_step
     void reg_add(size_t b, const Arr *flx, hipStream_t s);
+    // the fused engine keeps the step's fluxes in its canonical arrays: add
+    // them once the step is final (after resolve)
+    int reg_add_fused();
     friend struct FluxReg;
     friend int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out);
     friend int average_down(HydroLevel &C, HydroLevel &F, int ratio);
@@ -321,6 +324,12 @@ struct FluxReg {
     double dx1c = 0.0, dx1f = 0.0;
     // per fine box, bit 2d+side: its slab on that side holds boundary cells
     std::vector<int> slabs;
+    // one-launch add records for fused-engine levels (their fluxes live in
+    // fixed canonical arrays): CrseAdd per coarse box, FineAdd per slab
+    RegCrseRec *crse_recs = nullptr;
+    RegFineRec *fine_recs = nullptr;
+    int ncrse_recs = 0, nfine_recs = 0;
+    int64_t crse_pts = 0, fine_pts = 0;  // largest record range
     ~FluxReg() {
         // captured level steps hold the register's launches
         if (C && C->reg_crse_ == this) { C->release_graphs(); C->reg_crse_ = nullptr; }
@@ -331,6 +340,8 @@ struct FluxReg {
         fr(flag);
         fr(tab_crse);
         fr(tab_cfp);
+        fr(crse_recs);
+        fr(fine_recs);
     }
 };
 
@@ -1139,8 +1150,9 @@ int HydroLevel::advance_exact(double dt) {
                       hc_.small_dens, status_, ebf_);
         clean_state(snew_, false);
     }
-    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699)
-    clean_state(snew_, true);
+    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699); a coarse level
+    // of a subcycled step runs it after reflux and avgDown
+    if (!defer_post_) clean_state(snew_, true);
     return CAMR_OK;
 }
 
@@ -1256,6 +1268,7 @@ int HydroLevel::advance_fused(double dt) {
     pa.dtp = dt_dev_;
     pa.g = desc_.const_grav;
     pa.add_grav = desc_.add_grav_src;
+    pa.defer_post = defer_post_ ? 1 : 0;
     pa.dtmin = dtmin_;
     const int nb = batched_ ? 1 : (int)box_.size();
     for (int i = 0; i < nb; ++i) {
@@ -1345,6 +1358,7 @@ int HydroLevel::capture_graphs() {
         pa.dtp = dtp;
         pa.g = desc_.const_grav;
         pa.add_grav = desc_.add_grav_src;
+        pa.defer_post = 0;
         pa.dtmin = dtmin_;
         if (hipStreamBeginCapture(gstream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             set_error("stream capture failed to start");
@@ -1824,7 +1838,6 @@ static int check_refines(const camr_level_desc &f, const camr_level_desc &c, int
 int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out) {
     int rc = check_refines(F.desc_, C.desc_, ratio);
     if (rc) return rc;
-    if (C.fused_ || F.fused_) { set_error("the flux register needs the staged engine on both levels"); return CAMR_ERR_PARAM; }
     if (C.ebflags_ || F.ebflags_) { set_error("flux registers with embedded boundaries are not built"); return CAMR_ERR_PARAM; }
     if (C.reg_crse_ || F.reg_fine_) { set_error("a flux register is already attached"); return CAMR_ERR_PARAM; }
     if (C.desc_.hp.do_mol != F.desc_.hp.do_mol) { set_error("both levels need the same integrator"); return CAMR_ERR_PARAM; }
@@ -1975,6 +1988,64 @@ void HydroLevel::reg_add(size_t b, const Arr *flx, hipStream_t s) {
     }
 }
 
+// The fused engine's adds, each one launch over a table of records built
+// on first use (the canonical flux arrays and the register never move).
+int HydroLevel::reg_add_fused() {
+    const double fac = desc_.hp.do_mol ? 0.5 : 1.0;
+    if (reg_crse_) {
+        FluxReg &R = *reg_crse_;
+        if (!R.crse_recs) {
+            std::vector<RegCrseRec> v(box_.size());
+            for (size_t b = 0; b < box_.size(); ++b) {
+                v[b].r = range_of(box_[b]);
+                v[b].d = dense(R.crse[b], box_[b]);
+                for (int d = 0; d < 3; ++d) v[b].f[d] = engine_.flux_view((int)b, d);
+                R.crse_pts = std::max<int64_t>(R.crse_pts, box_[b].npts());
+            }
+            CAMR_HIP(hipMalloc(&R.crse_recs, v.size() * sizeof(RegCrseRec)));
+            CAMR_HIP(hipMemcpy(R.crse_recs, v.data(), v.size() * sizeof(RegCrseRec), hipMemcpyHostToDevice));
+            R.ncrse_recs = (int)v.size();
+        }
+        hipLaunchKernelGGL(kb_reg_crse_add, dim3(grid_for(R.crse_pts), R.ncrse_recs), dim3(kBlock), 0,
+                           stream_, (const RegCrseRec *)R.crse_recs, (const uint8_t *)R.flag, dom_,
+                           (const double *)dt_dev_, fac, R.dx1c);
+    }
+    if (reg_fine_) {
+        FluxReg &R = *reg_fine_;
+        const int r = R.ratio;
+        if (!R.fine_recs) {
+            std::vector<RegFineRec> v;
+            for (size_t b = 0; b < box_.size(); ++b) {
+                Box cb = box_[b];
+                for (int d = 0; d < 3; ++d) { cb.lo[d] /= r; cb.hi[d] = (cb.hi[d] + 1) / r - 1; }
+                for (int d = 0; d < 3; ++d)
+                    for (int side = 0; side < 2; ++side) {
+                        if (!(R.slabs[b] >> (2 * d + side) & 1)) continue;
+                        Box sl = cb;
+                        sl.lo[d] = sl.hi[d] = side ? cb.hi[d] + 1 : cb.lo[d] - 1;
+                        RegFineRec e;
+                        e.r = range_of(sl);
+                        e.f = engine_.flux_view((int)b, d);
+                        e.d = d;
+                        e.side = side;
+                        v.push_back(e);
+                        R.fine_pts = std::max<int64_t>(R.fine_pts, sl.npts());
+                    }
+            }
+            if (v.empty()) return CAMR_OK;
+            CAMR_HIP(hipMalloc(&R.fine_recs, v.size() * sizeof(RegFineRec)));
+            CAMR_HIP(hipMemcpy(R.fine_recs, v.data(), v.size() * sizeof(RegFineRec), hipMemcpyHostToDevice));
+            R.nfine_recs = (int)v.size();
+        }
+        hipLaunchKernelGGL(kb_reg_fine_add, dim3(grid_for(R.fine_pts), R.nfine_recs), dim3(kBlock), 0,
+                           stream_, (const RegFineRec *)R.fine_recs, (const DevFab *)R.tab_cfp,
+                           (const int *)R.C->owner_, R.C->dom_, (const uint8_t *)R.flag, r,
+                           (const double *)dt_dev_, fac, R.dx1f);
+    }
+    CAMR_HIP(hipGetLastError());
+    return CAMR_OK;
+}
+
 // average_down of the fine level's S_new onto the coarse level's
 int average_down(HydroLevel &C, HydroLevel &F, int ratio) {
     int rc = check_refines(F.desc_, C.desc_, ratio);
@@ -2020,6 +2091,9 @@ int amr_advance_level(const std::vector<HydroLevel *> &L, const std::vector<Flux
     const double t_old = C.time_;
     C.defer_post_ = has_fine;
     rc = C.advance(dt);
+    // a fused step that trips a density floor is redone here, still deferred
+    if (!rc) rc = C.resolve();
+    if (!rc && C.fused_ && (C.reg_crse_ || C.reg_fine_)) rc = C.reg_add_fused();
     C.defer_post_ = false;
     if (rc || !has_fine) return rc;
     FluxReg &Rg = *R[l];

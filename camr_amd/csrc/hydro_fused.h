@@ -221,6 +221,7 @@ struct PostArgs {
     double g;
     int add_grav;
     unsigned long long *dtmin;
+    int defer_post;  // 1: leave post_timestep's clean_state to the caller
 };
 
 // Grid of one stage launch over a box set.
@@ -285,6 +286,12 @@ class FusedEngine {
     // launches made while set read this word and skip their work when it is
     // non-zero (the halt flag of captured step graphs)
     void set_halt(const int *h) { halt_ = h; }
+    // the final (adjusted, area-weighted) d-fluxes of box b as an Array4
+    // view over the box's canonical box; components 0..NVAR-1 are the fluxes
+    Arr flux_view(int b, int d) const {
+        const Box cb = canon_box(boxes_[b]);
+        return dense(scratch_.base() + off_[b] + (size_t)slot_off(S_FLX + d) * cb.npts(), cb);
+    }
 
   private:
     template <typename K, typename... A>

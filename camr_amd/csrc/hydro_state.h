@@ -84,6 +84,22 @@ template <int D>
 __global__ void k_reg_fine_add(Range r, const DevFab *cfp, const int *cowner, DomInfo CD,
                                const uint8_t *flag, Arr f, int side, int ratio,
                                const double *dtp, double fac, double dx1f);
+// flux-register records of one-launch adds (fused-engine levels)
+struct RegCrseRec {
+    Range r;   // the coarse box
+    Arr d;     // its register data
+    Arr f[3];  // its final fluxes
+};
+struct RegFineRec {
+    Range r;   // a slab of coarse cells beside one side of a fine box
+    Arr f;     // that box's final fluxes of direction d
+    int d, side;
+};
+__global__ void kb_reg_crse_add(const RegCrseRec *recs, const uint8_t *flag, DomInfo CD,
+                                const double *dtp, double fac, double dx1);
+__global__ void kb_reg_fine_add(const RegFineRec *recs, const DevFab *cfp, const int *cowner,
+                                DomInfo CD, const uint8_t *flag, int ratio, const double *dtp,
+                                double fac, double dx1f);
 __global__ void kb_reflux(const DevFab *snew, const DevFab *crse, const DevFab *cfp);
 __global__ void kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo FD,
                            int ratio);

@@ -576,6 +576,63 @@ template __global__ void k_reg_fine_add<0>(Range, const DevFab *, const int *, D
 template __global__ void k_reg_fine_add<1>(Range, const DevFab *, const int *, DomInfo, const uint8_t *, Arr, int, int, const double *, double, double);
 template __global__ void k_reg_fine_add<2>(Range, const DevFab *, const int *, DomInfo, const uint8_t *, Arr, int, int, const double *, double, double);
 
+// The same two adds over a table of records, one launch for every box
+// (grid.y = record): CrseAdd per coarse box, FineAdd per bordering side of
+// every fine box.  Each coarse cell still takes its contributions from one
+// record, in the same order.
+__global__ void __launch_bounds__(kBlock)
+kb_reg_crse_add(const RegCrseRec *recs, const uint8_t *flag, DomInfo CD, const double *dtp,
+                double fac, double dx1) {
+    const RegCrseRec &R = recs[blockIdx.y];
+    int i, j, k;
+    if (!range_point(R.r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    if (reg_flag(CD, flag, c) != kRegBnd) return;
+    const double dtdx = (fac * *dtp) / dx1;
+#pragma unroll
+    for (int dd = 0; dd < 3; ++dd) {
+        int cm[3] = {i, j, k}, cp[3] = {i, j, k};
+        cm[dd] -= 1;
+        cp[dd] += 1;
+        if (reg_flag(CD, flag, cm) == kRegFine) {
+            for (int n = 0; n < NVAR; ++n) R.d(i, j, k, n) -= dtdx * R.f[dd](i, j, k, n);
+        } else if (reg_flag(CD, flag, cp) == kRegFine) {
+            for (int n = 0; n < NVAR; ++n) R.d(i, j, k, n) += dtdx * R.f[dd](cp[0], cp[1], cp[2], n);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_reg_fine_add(const RegFineRec *recs, const DevFab *cfp, const int *cowner, DomInfo CD,
+                const uint8_t *flag, int ratio, const double *dtp, double fac, double dx1f) {
+    const RegFineRec &R = recs[blockIdx.y];
+    int i, j, k;
+    if (!range_point(R.r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int c[3] = {i, j, k};
+    int m[3];
+    if (!reg_map(CD, c, m)) return;
+    if (reg_flag(CD, flag, c) != kRegBnd) return;
+    const int cb = owner_of(CD, cowner, m);
+    if (cb < 0) return;
+    const Arr d = cfp[cb].a;
+    const double dtdx = (fac * *dtp) / (dx1f * (double)(ratio * ratio * ratio));
+    const int D = R.d, side = R.side;
+    const int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;  // inner, outer
+    for (int n = 0; n < NVAR; ++n) {
+        double acc = d(m[0], m[1], m[2], n);
+        for (int o1 = 0; o1 < ratio; ++o1)
+            for (int o0 = 0; o0 < ratio; ++o0) {
+                int fi[3];
+                fi[D] = side ? c[D] * ratio : (c[D] + 1) * ratio;
+                fi[T1] = c[T1] * ratio + o1;
+                fi[T0] = c[T0] * ratio + o0;
+                const double v = R.f(fi[0], fi[1], fi[2], n);
+                acc += side ? dtdx * v : -dtdx * v;
+            }
+        d(m[0], m[1], m[2], n) = acc;
+    }
+}
+
 // Reflux: crse_data += cfp, then S_new += crse_data, every valid cell.
 __global__ void __launch_bounds__(kBlock)
 kb_reflux(const DevFab *snew, const DevFab *crse, const DevFab *cfp) {

@@ -116,7 +116,9 @@ __device__ __forceinline__ double post_values(const BoxEnt &E, int i, int j, int
         }
         clean_point(v, a.hc);
     }
-    clean_point(v, a.hc);
+    // post_timestep's clean_state (CAMR.cpp:698-699), unless a finer level's
+    // steps come first (then the caller cleans after reflux and avgDown)
+    if (!a.defer_post) clean_point(v, a.hc);
     // estTimeStep on the new state
     const double rho = v[URHO];
     const double rhoInv = 1.0 / rho;

@@ -18,7 +18,7 @@ torch = pytest.importorskip("torch")
 FINE4 = [(16, 16, 16, 31, 47, 47), (32, 16, 16, 47, 47, 47)]
 
 
-def _pair(O, n, boxes, periodic, kw, graph=False):
+def _pair(O, n, boxes, periodic, kw, graph=False, engine=_lib.ENGINE_STAGED):
     bc = 0 if periodic else 4
     per = (1, 1, 1) if periodic else (0, 0, 0)
     params = dict(cfl=0.3, **kw)
@@ -27,7 +27,7 @@ def _pair(O, n, boxes, periodic, kw, graph=False):
     prob = Problem(domlo=(0, 0, 0), domhi=(n - 1,) * 3, dx=(1.0 / n,) * 3, bclo=(bc,) * 3,
                    bchi=(bc,) * 3, is_periodic=per, boxes=boxes, params=params)
     ol = O.Level(op, boxes)
-    gl = Level(prob, engine=_lib.ENGINE_STAGED, graph=graph)
+    gl = Level(prob, engine=engine, graph=graph)
     for b, bb in enumerate(boxes):
         U = _field(n, bb[:3], bb[3:])
         ol.set_box(b, U)
@@ -35,22 +35,30 @@ def _pair(O, n, boxes, periodic, kw, graph=False):
     return ol, gl
 
 
-@pytest.mark.parametrize("ratio,periodic,kw,graph", [
-    (2, True, dict(ppm_type=1), False),
-    (2, False, dict(ppm_type=0), False),
-    (2, True, dict(do_mol=1), False),
-    (4, True, dict(ppm_type=1), False),
-    (2, True, dict(ppm_type=1), True),
-    (2, True, dict(do_mol=1), True),
+FUSED = _lib.ENGINE_FUSED
+
+
+@pytest.mark.parametrize("ratio,periodic,kw,graph,engine", [
+    (2, True, dict(ppm_type=1), False, _lib.ENGINE_STAGED),
+    (2, False, dict(ppm_type=0), False, _lib.ENGINE_STAGED),
+    (2, True, dict(do_mol=1), False, _lib.ENGINE_STAGED),
+    (4, True, dict(ppm_type=1), False, _lib.ENGINE_STAGED),
+    (2, True, dict(ppm_type=1), True, _lib.ENGINE_STAGED),
+    (2, True, dict(do_mol=1), True, _lib.ENGINE_STAGED),
+    (2, True, dict(ppm_type=1), False, FUSED),
+    (2, False, dict(ppm_type=0), False, FUSED),
+    (4, True, dict(ppm_type=1), False, FUSED),
 ])
-def test_amr_step_matches_oracle(oracle, ratio, periodic, kw, graph):
+def test_amr_step_matches_oracle(oracle, ratio, periodic, kw, graph, engine):
     """graph: every level step captured (per buffer parity and deferred
-    clean) and replayed, the registers' adds inside the graphs."""
-    print(f"two-level step ratio={ratio} periodic={periodic} {kw} graph={graph}", flush=True)
+    clean) and replayed, the registers' adds inside the graphs.  FUSED: the
+    fused engine's canonical flux arrays feed the registers."""
+    print(f"two-level step ratio={ratio} periodic={periodic} {kw} graph={graph} "
+          f"engine={engine}", flush=True)
     nc = 16
     fine = FINE2 if ratio == 2 else FINE4
-    oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw, graph)
-    of, gf = _pair(oracle, nc * ratio, fine, periodic, kw, graph)
+    oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw, graph, engine)
+    of, gf = _pair(oracle, nc * ratio, fine, periodic, kw, graph, engine)
     oR = oracle.FluxRegister(oc, of, ratio)
     gR = FluxRegister(gc, gf, ratio)
     oracle.average_down(oc, of, ratio)
@@ -100,21 +108,19 @@ def test_register_refuses_unsupported_layouts():
     gf = Level(fine([(9, 8, 8, 23, 23, 23)]), engine=_lib.ENGINE_STAGED)
     with pytest.raises(_lib.CamrError, match="aligned"):
         FluxRegister(gc, gf, 2)
-    # the fused engine
-    gf = Level(fine(FINE2), engine=_lib.ENGINE_FUSED)
-    with pytest.raises(_lib.CamrError, match="staged"):
-        FluxRegister(gc, gf, 2)
 
 
-def test_three_level_advance_matches_oracle(oracle):
+@pytest.mark.parametrize("engine", [_lib.ENGINE_STAGED, FUSED])
+def test_three_level_advance_matches_oracle(oracle, engine):
     """Three levels (16^3, a 2x block at ratio 2, a nested block at ratio 2
     inside it), two coarse steps of recursive subcycling: every level bit
     for bit, the composite sums conserved."""
     from camr_amd.hydro import amr_advance
     from tests.test_oracle_amr import FINE3, composite_sums3
     kw = dict(ppm_type=1)
-    pairs = [_pair(oracle, 16, _boxes(16, 8), True, kw), _pair(oracle, 32, FINE2, True, kw),
-             _pair(oracle, 64, FINE3, True, kw)]
+    pairs = [_pair(oracle, 16, _boxes(16, 8), True, kw, engine=engine),
+             _pair(oracle, 32, FINE2, True, kw, engine=engine),
+             _pair(oracle, 64, FINE3, True, kw, engine=engine)]
     ol = [p[0] for p in pairs]
     gl = [p[1] for p in pairs]
     oR = [oracle.FluxRegister(ol[0], ol[1], 2), oracle.FluxRegister(ol[1], ol[2], 2)]
