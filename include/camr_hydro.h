@@ -237,8 +237,8 @@ int camr_level_set_exterior(camr_level *L, const double *host);
 int camr_level_exterior_from_coarse(camr_level *fine, camr_level *coarse, int32_t ratio);
 
 /*
- * Coarse-fine synchronisation of a two-level hierarchy (both levels on the
- * staged engine, no embedded boundary).
+ * Coarse-fine synchronisation of level hierarchies (either engine, no
+ * embedded boundary).
  */
 typedef struct camr_flux_register camr_flux_register;
 
