@@ -148,3 +148,53 @@ def test_three_level_advance_matches_oracle(oracle, engine):
         assert abs(s1[c] - s0[c]) <= 1e-12 * max(abs(s0[c]), 1.0)
     for r in gR + oR:
         r.close()
+
+
+@pytest.mark.parametrize("engine", [_lib.ENGINE_STAGED, FUSED])
+def test_density_floor_inside_a_two_level_step(oracle, engine):
+    """Cells below small_dens on both levels: enforce_min_density fires in
+    the composite step (on the fused engine the step is redone exactly
+    before the register adds, so they are counted once)."""
+    kw = dict(ppm_type=0)
+    oc, gc = _pair(oracle, 16, _boxes(16, 8), True, kw, engine=engine)
+    of, gf = _pair(oracle, 32, FINE2, True, kw, engine=engine)
+
+    def plant(L, b, cells):
+        U = L.get_box(b)
+        bb = L.boxes[b]
+        for (i, j, k), rho in cells:
+            ii, jj, kk = i - bb[0], j - bb[1], k - bb[2]
+            f = rho / U[0, kk, jj, ii]
+            U[[0, 1, 2, 3, 5, 7, 8, 9], kk, jj, ii] *= f
+            U[4, kk, jj, ii] = U[5, kk, jj, ii] + 0.5 * (U[1:4, kk, jj, ii] ** 2).sum() / U[0, kk, jj, ii]
+        return U
+
+    for (o, g) in ((oc, gc), (of, gf)):
+        b = 0
+        cells = [((o.boxes[b][0] + 1, o.boxes[b][1] + 1, o.boxes[b][2] + 1), 5e-8),
+                 ((o.boxes[b][0] + 2, o.boxes[b][1] + 1, o.boxes[b][2] + 1), 2e-8)]
+        U = plant(o, b, cells)
+        o.set_box(b, U)
+        g.set_box(b, U)
+    oR = oracle.FluxRegister(oc, of, 2)
+    gR = FluxRegister(gc, gf, 2)
+    oracle.average_down(oc, of, 2)
+    gc.average_down(gf, 2)
+    for L in (oc, of, gc, gf):
+        L.init_post()
+    dt = None
+    for s in range(2):
+        odt = min(oc.initial_dt(), 2 * of.initial_dt()) if s == 0 else \
+            min(oc.new_dt(dt), 2 * of.new_dt(dt / 2))
+        gdt = min(gc.initial_dt(), 2 * gf.initial_dt()) if s == 0 else \
+            min(gc.new_dt(dt), 2 * gf.new_dt(dt / 2))
+        assert odt == gdt
+        dt = odt
+        assert oracle.amr_step(oc, of, oR, dt) == 0
+        amr_step(gc, gf, gR, dt)
+        for b in range(len(oc.boxes)):
+            assert np.array_equal(gc.get_box(b), oc.get_box(b)), ("coarse", s, b)
+        for b in range(len(of.boxes)):
+            assert np.array_equal(gf.get_box(b), of.get_box(b)), ("fine", s, b)
+    gR.close()
+    oR.close()
