@@ -245,6 +245,11 @@ class HydroLevel {
     double *ext_ = nullptr;     // exterior data (whole domain), when set
     double *cstage_ = nullptr;  // coarse values staged for exterior_from
     size_t cstage_n_ = 0;
+    // the domain cells, in domain order, that the boxes' ghost regions map
+    // to and no box owns (x, y, z triples; built on the first ghost fill)
+    int *shell_cells_ = nullptr;
+    int64_t shell_n_ = -1;
+    int build_shell_cells();
     Arr extv_{};
     bool batched_ = true;       // one launch per stage over all boxes
     int parity() const { return a_is_new_ ? 0 : 1; }
@@ -319,7 +324,10 @@ struct FluxReg {
     HydroLevel *C = nullptr, *F = nullptr;
     int ratio = 2;
     uint8_t *flag = nullptr;  // device, per coarse domain cell
+    // crse / cfp per coarse box, views into one pool (reset in one launch)
     std::vector<double *> crse, cfp;
+    double *pool = nullptr;
+    int64_t pool_n = 0;
     DevFab *tab_crse = nullptr, *tab_cfp = nullptr;
     double dx1c = 0.0, dx1f = 0.0;
     // per fine box, bit 2d+side: its slab on that side holds boundary cells
@@ -335,8 +343,7 @@ struct FluxReg {
         if (C && C->reg_crse_ == this) { C->release_graphs(); C->reg_crse_ = nullptr; }
         if (F && F->reg_fine_ == this) { F->release_graphs(); F->reg_fine_ = nullptr; }
         auto fr = [](void *p) { if (p) (void)hipFree(p); };
-        for (double *p : crse) fr(p);
-        for (double *p : cfp) fr(p);
+        fr(pool);
         fr(flag);
         fr(tab_crse);
         fr(tab_cfp);
@@ -413,16 +420,14 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
         // just the cells a fill reads from the exterior: the domain images
         // of the boxes' ghost regions that no box owns, over the bounding
         // box of the grown boxes
-        Box bb = grow(box_[0], ng_);
-        for (const Box &B : box_)
-            for (int d = 0; d < 3; ++d) {
-                bb.lo[d] = std::min(bb.lo[d], B.lo[d] - ng_);
-                bb.hi[d] = std::max(bb.hi[d], B.hi[d] + ng_);
-            }
-        const Range r = range_of(bb);
-        hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
-                           r, extv_, stv, C.dom_, ratio, dom.lo[0], dom.lo[1], dom.lo[2],
-                           (const int *)owner_, dom_);
+        if (shell_n_ < 0) {
+            rc = build_shell_cells();
+            if (rc) return rc;
+        }
+        if (shell_n_ > 0)
+            hipLaunchKernelGGL(k_ext_cells, dim3(grid_for(shell_n_), NVAR), dim3(kBlock), 0,
+                               stream_, (const int *)shell_cells_, shell_n_, extv_, stv, C.dom_,
+                               ratio, dom.lo[0], dom.lo[1], dom.lo[2]);
     } else {
         const Range r = range_of(dom);
         hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
@@ -432,6 +437,52 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
     CAMR_HIP(hipGetLastError());
     // the coarse level may step next: finish reading its state first
     CAMR_HIP(hipStreamSynchronize(stream_));
+    return CAMR_OK;
+}
+
+int HydroLevel::build_shell_cells() {
+    const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
+    const int64_t n0 = dom.len(0), n1 = dom.len(1), n2 = dom.len(2);
+    std::vector<int> own((size_t)dom.npts());
+    CAMR_HIP(hipMemcpy(own.data(), owner_, own.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> mark(own.size(), 0);
+    const int64_t len[3] = {n0, n1, n2};
+    for (const Box &B0 : box_) {
+        const Box G = grow(B0, ng_);
+        for (int k = G.lo[2]; k <= G.hi[2]; ++k)
+            for (int j = G.lo[1]; j <= G.hi[1]; ++j)
+                for (int i = G.lo[0]; i <= G.hi[0]; ++i) {
+                    const int c[3] = {i, j, k};
+                    int64_t m[3];
+                    bool in = true;
+                    for (int d = 0; d < 3 && in; ++d) {
+                        int64_t x = c[d] - dom.lo[d];
+                        if (x < 0 || x >= len[d]) {
+                            if (!desc_.is_periodic[d]) in = false;
+                            else x = (x % len[d] + len[d]) % len[d];
+                        }
+                        m[d] = x;
+                    }
+                    if (!in) continue;
+                    const int64_t at = m[0] + m[1] * n0 + m[2] * n0 * n1;
+                    if (own[(size_t)at] < 0) mark[(size_t)at] = 1;
+                }
+    }
+    std::vector<int> cells;
+    for (int64_t k = 0; k < n2; ++k)
+        for (int64_t j = 0; j < n1; ++j)
+            for (int64_t i = 0; i < n0; ++i)
+                if (mark[(size_t)(i + j * n0 + k * n0 * n1)]) {
+                    cells.push_back((int)i + dom.lo[0]);
+                    cells.push_back((int)j + dom.lo[1]);
+                    cells.push_back((int)k + dom.lo[2]);
+                }
+    shell_n_ = (int64_t)(cells.size() / 3);
+    if (shell_n_ > 0) {
+        CAMR_HIP(hipMalloc(&shell_cells_, cells.size() * sizeof(int)));
+        CAMR_HIP(hipMemcpy(shell_cells_, cells.data(), cells.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+    }
     return CAMR_OK;
 }
 
@@ -457,6 +508,9 @@ void HydroLevel::release() {
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     fr(cstage_);
     cstage_ = nullptr;
+    fr(shell_cells_);
+    shell_cells_ = nullptr;
+    shell_n_ = -1;
     for (auto p : bufA_) fr(p);
     for (auto p : bufB_) fr(p);
     for (auto p : sb_) fr(p);
@@ -738,6 +792,10 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     EbFlags hf = ebf_;
     hf.p = flags;
     ng_ = 6;
+    // the ghost regions grew: rebuild the exterior cell list on the next fill
+    if (shell_cells_) (void)hipFree(shell_cells_);
+    shell_cells_ = nullptr;
+    shell_n_ = -1;
     hc_.eb = 1;
     const int nb = (int)box_.size();
     kind_.assign(nb, 0);
@@ -1930,12 +1988,16 @@ int flux_register_create(HydroLevel &C, HydroLevel &F, int ratio, FluxReg **out)
     std::vector<DevFab> tc(nb), tf(nb);
     R->crse.assign(nb, nullptr);
     R->cfp.assign(nb, nullptr);
+    int64_t half = 0;
+    for (size_t b = 0; b < nb; ++b) half += (int64_t)NVAR * C.box_[b].npts();
+    R->pool_n = 2 * half;
+    CAMR_HIP(hipMalloc(&R->pool, R->pool_n * sizeof(double)));
+    CAMR_HIP(hipMemset(R->pool, 0, R->pool_n * sizeof(double)));
+    int64_t off = 0;
     for (size_t b = 0; b < nb; ++b) {
-        const size_t n = (size_t)NVAR * C.box_[b].npts() * sizeof(double);
-        CAMR_HIP(hipMalloc(&R->crse[b], n));
-        CAMR_HIP(hipMalloc(&R->cfp[b], n));
-        CAMR_HIP(hipMemset(R->crse[b], 0, n));
-        CAMR_HIP(hipMemset(R->cfp[b], 0, n));
+        R->crse[b] = R->pool + off;
+        R->cfp[b] = R->pool + half + off;
+        off += (int64_t)NVAR * C.box_[b].npts();
         tc[b].a = dense(R->crse[b], C.box_[b]);
         tc[b].b = C.box_[b];
         tf[b].a = dense(R->cfp[b], C.box_[b]);
@@ -2083,10 +2145,8 @@ int amr_advance_level(const std::vector<HydroLevel *> &L, const std::vector<Flux
     if (has_fine) {
         // reset (CAMR_advance.cpp:64-66)
         FluxReg &Rg = *R[l];
-        for (size_t b = 0; b < C.box_.size(); ++b) {
-            launch_on(k_fill_value, C.box_[b], C.stream_, dense(Rg.crse[b], C.box_[b]), (int)NVAR, 0.0);
-            launch_on(k_fill_value, C.box_[b], C.stream_, dense(Rg.cfp[b], C.box_[b]), (int)NVAR, 0.0);
-        }
+        hipLaunchKernelGGL(k_fill_flat, dim3(grid_for(Rg.pool_n)), dim3(kBlock), 0, C.stream_,
+                           Rg.pool, Rg.pool_n, 0.0);
     }
     const double t_old = C.time_;
     C.defer_post_ = has_fine;

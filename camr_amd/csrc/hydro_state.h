@@ -53,6 +53,7 @@ __global__ void k_add_newsrc(Range r, Arr snew, Arr sold, double dt, double g,
                              double small_dens, int *flags, EbFlags eb);
 __global__ void k_srchyd(Range r, Box valid, Arr dst, Arr sold, double g);
 __global__ void k_fill_value(Range r, Arr a, int nc, double v);
+__global__ void k_fill_flat(double *p, int64_t n, double v);
 __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v);
 // the passes above over every box of a table in one launch (grid.y = box)
 __global__ void kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb);
@@ -76,6 +77,8 @@ __global__ void k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *c
                                const DevFab *otab, int tmode, double ta, double tb);
 __global__ void k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0,
                                   int flo1, int flo2, const int *fowner, DomInfo FD);
+__global__ void k_ext_cells(const int *cells, int64_t ncells, Arr ext, Arr st, DomInfo CD,
+                            int ratio, int flo0, int flo1, int flo2);
 // coarse-fine synchronisation (flux register, reflux, average_down)
 constexpr int kRegBnd = 1, kRegFine = 2;
 __global__ void k_reg_crse_add(Range r, Arr d, Arr f0, Arr f1, Arr f2, const uint8_t *flag,

@@ -360,6 +360,12 @@ k_fill_value(Range r, Arr a, int nc, double v) {
     for (int n = 0; n < nc; ++n) a(i, j, k, n) = v;
 }
 
+__global__ void __launch_bounds__(kBlock)
+k_fill_flat(double *p, int64_t n, double v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 // one-element initialiser (a kernel rather than a memset so that the step
 // stays capturable in a HIP graph)
 __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v) {
@@ -735,6 +741,12 @@ k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo C
     }
 }
 
+// FillPatch's conservative linear interpolation of fine cell (i,j,k),
+// components [nlo, nhi), from the staged coarse values.
+__device__ __forceinline__ void ext_point(int i, int j, int k, const Arr &ext, const Arr &st,
+                                          const DomInfo &CD, int ratio, const int *flo, int nlo,
+                                          int nhi);
+
 __global__ void __launch_bounds__(kBlock)
 k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0, int flo1, int flo2,
                   const int *fowner, DomInfo FD) {
@@ -749,7 +761,29 @@ k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0, int
         j = m[1];
         k = m[2];
     }
-    const int f[3] = {i, j, k}, flo[3] = {flo0, flo1, flo2};
+    const int flo[3] = {flo0, flo1, flo2};
+    // grid.y > 1: one component per block row
+    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
+    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
+    ext_point(i, j, k, ext, st, CD, ratio, flo, nlo, nhi);
+}
+
+// The same interpolation over a list of fine cells (x, y, z triples, each
+// a domain cell no box owns), one component per block row.
+__global__ void __launch_bounds__(kBlock)
+k_ext_cells(const int *cells, int64_t ncells, Arr ext, Arr st, DomInfo CD, int ratio, int flo0,
+            int flo1, int flo2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int flo[3] = {flo0, flo1, flo2};
+    const int n = (int)blockIdx.y;
+    ext_point(cells[3 * t], cells[3 * t + 1], cells[3 * t + 2], ext, st, CD, ratio, flo, n, n + 1);
+}
+
+__device__ __forceinline__ void ext_point(int i, int j, int k, const Arr &ext, const Arr &st,
+                                          const DomInfo &CD, int ratio, const int *flo, int nlo,
+                                          int nhi) {
+    const int f[3] = {i, j, k};
     const double half = 0.5 * (ratio - 1);
     int cc[3];
     double off[3];
@@ -760,9 +794,6 @@ k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0, int
         off[d] = (sub - half) / ratio;
     }
     const double edge = half / ratio;
-    // grid.y > 1: one component per block row
-    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
-    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
 #pragma unroll 1
     for (int n = nlo; n < nhi; ++n) {
         double u[27];
