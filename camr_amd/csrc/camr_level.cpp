@@ -249,7 +249,13 @@ class HydroLevel {
     // to and no box owns (x, y, z triples; built on the first ghost fill)
     int *shell_cells_ = nullptr;
     int64_t shell_n_ = -1;
-    int build_shell_cells();
+    // the coarse cells (grown coarse domain) their interpolation reads, for
+    // the refinement ratio shell_ratio_
+    int *cstage_cells_ = nullptr;
+    int64_t cstage_cells_n_ = 0;
+    int shell_ratio_ = 0;
+    int build_shell_cells(int ratio, const Box &cdom);
+    void drop_shell_cells();
     Arr extv_{};
     bool batched_ = true;       // one launch per stage over all boxes
     int parity() const { return a_is_new_ ? 0 : 1; }
@@ -410,25 +416,30 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
         cstage_n_ = NVAR * cst.npts();
     }
     const Arr stv = dense(cstage_, cst);
-    {
-        const Range r = range_of(cst);
-        hipLaunchKernelGGL(k_coarse_stage, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_, r,
-                           stv, ct, (const int *)C.owner_, C.dom_, co, tmode, ta, tb);
-    }
-    // pass 2: one thread per fine cell and component (grid.y = NVAR)
     if (ghosts_only) {
         // just the cells a fill reads from the exterior: the domain images
-        // of the boxes' ghost regions that no box owns, over the bounding
-        // box of the grown boxes
-        if (shell_n_ < 0) {
-            rc = build_shell_cells();
+        // of the boxes' ghost regions that no box owns, and the coarse
+        // cells their interpolation reads
+        if (shell_n_ < 0 || shell_ratio_ != ratio) {
+            rc = build_shell_cells(ratio, make_box(C.desc_.hp.domlo, C.desc_.hp.domhi));
             if (rc) return rc;
         }
+        if (cstage_cells_n_ > 0)
+            hipLaunchKernelGGL(k_coarse_stage_cells, dim3(grid_for(cstage_cells_n_), NVAR),
+                               dim3(kBlock), 0, stream_, (const int *)cstage_cells_,
+                               cstage_cells_n_, stv, ct, (const int *)C.owner_, C.dom_, co, tmode,
+                               ta, tb);
+        // pass 2: one thread per fine cell and component (grid.y = NVAR)
         if (shell_n_ > 0)
             hipLaunchKernelGGL(k_ext_cells, dim3(grid_for(shell_n_), NVAR), dim3(kBlock), 0,
                                stream_, (const int *)shell_cells_, shell_n_, extv_, stv, C.dom_,
                                ratio, dom.lo[0], dom.lo[1], dom.lo[2]);
     } else {
+        {
+            const Range r = range_of(cst);
+            hipLaunchKernelGGL(k_coarse_stage, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
+                               r, stv, ct, (const int *)C.owner_, C.dom_, co, tmode, ta, tb);
+        }
         const Range r = range_of(dom);
         hipLaunchKernelGGL(k_ext_from_coarse, dim3(grid_for(r.n), NVAR), dim3(kBlock), 0, stream_,
                            r, extv_, stv, C.dom_, ratio, dom.lo[0], dom.lo[1], dom.lo[2],
@@ -440,7 +451,17 @@ int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, do
     return CAMR_OK;
 }
 
-int HydroLevel::build_shell_cells() {
+void HydroLevel::drop_shell_cells() {
+    if (shell_cells_) (void)hipFree(shell_cells_);
+    if (cstage_cells_) (void)hipFree(cstage_cells_);
+    shell_cells_ = cstage_cells_ = nullptr;
+    shell_n_ = -1;
+    cstage_cells_n_ = 0;
+    shell_ratio_ = 0;
+}
+
+int HydroLevel::build_shell_cells(int ratio, const Box &cdom) {
+    drop_shell_cells();
     const Box dom = make_box(desc_.hp.domlo, desc_.hp.domhi);
     const int64_t n0 = dom.len(0), n1 = dom.len(1), n2 = dom.len(2);
     std::vector<int> own((size_t)dom.npts());
@@ -468,6 +489,11 @@ int HydroLevel::build_shell_cells() {
                     if (own[(size_t)at] < 0) mark[(size_t)at] = 1;
                 }
     }
+    // the coarse parents of the listed cells and their 26 neighbours, over
+    // the coarse domain grown by one (the staged array's box)
+    const Box cst = grow(cdom, 1);
+    const int64_t c0 = cst.len(0), c1 = cst.len(1), c2 = cst.len(2);
+    std::vector<uint8_t> cmark((size_t)cst.npts(), 0);
     std::vector<int> cells;
     for (int64_t k = 0; k < n2; ++k)
         for (int64_t j = 0; j < n1; ++j)
@@ -476,6 +502,12 @@ int HydroLevel::build_shell_cells() {
                     cells.push_back((int)i + dom.lo[0]);
                     cells.push_back((int)j + dom.lo[1]);
                     cells.push_back((int)k + dom.lo[2]);
+                    // parent in the staged box's index space (cst.lo = cdom.lo - 1)
+                    const int64_t pi = i / ratio + 1, pj = j / ratio + 1, pk = k / ratio + 1;
+                    for (int64_t dk = -1; dk <= 1; ++dk)
+                        for (int64_t dj = -1; dj <= 1; ++dj)
+                            for (int64_t di = -1; di <= 1; ++di)
+                                cmark[(size_t)((pi + di) + (pj + dj) * c0 + (pk + dk) * c0 * c1)] = 1;
                 }
     shell_n_ = (int64_t)(cells.size() / 3);
     if (shell_n_ > 0) {
@@ -483,6 +515,22 @@ int HydroLevel::build_shell_cells() {
         CAMR_HIP(hipMemcpy(shell_cells_, cells.data(), cells.size() * sizeof(int),
                            hipMemcpyHostToDevice));
     }
+    std::vector<int> ccells;
+    for (int64_t k = 0; k < c2; ++k)
+        for (int64_t j = 0; j < c1; ++j)
+            for (int64_t i = 0; i < c0; ++i)
+                if (cmark[(size_t)(i + j * c0 + k * c0 * c1)]) {
+                    ccells.push_back((int)i + cst.lo[0]);
+                    ccells.push_back((int)j + cst.lo[1]);
+                    ccells.push_back((int)k + cst.lo[2]);
+                }
+    cstage_cells_n_ = (int64_t)(ccells.size() / 3);
+    if (cstage_cells_n_ > 0) {
+        CAMR_HIP(hipMalloc(&cstage_cells_, ccells.size() * sizeof(int)));
+        CAMR_HIP(hipMemcpy(cstage_cells_, ccells.data(), ccells.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+    }
+    shell_ratio_ = ratio;
     return CAMR_OK;
 }
 
@@ -508,9 +556,7 @@ void HydroLevel::release() {
     auto fr = [](void *p) { if (p) (void)hipFree(p); };
     fr(cstage_);
     cstage_ = nullptr;
-    fr(shell_cells_);
-    shell_cells_ = nullptr;
-    shell_n_ = -1;
+    drop_shell_cells();
     for (auto p : bufA_) fr(p);
     for (auto p : bufB_) fr(p);
     for (auto p : sb_) fr(p);
@@ -792,10 +838,8 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     EbFlags hf = ebf_;
     hf.p = flags;
     ng_ = 6;
-    // the ghost regions grew: rebuild the exterior cell list on the next fill
-    if (shell_cells_) (void)hipFree(shell_cells_);
-    shell_cells_ = nullptr;
-    shell_n_ = -1;
+    // the ghost regions grew: rebuild the exterior cell lists on the next fill
+    drop_shell_cells();
     hc_.eb = 1;
     const int nb = (int)box_.size();
     kind_.assign(nb, 0);

@@ -75,6 +75,9 @@ __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFa
                                int *flags, EbFlags eb);
 __global__ void k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
                                const DevFab *otab, int tmode, double ta, double tb);
+__global__ void k_coarse_stage_cells(const int *cells, int64_t ncells, Arr st, const DevFab *ctab,
+                                     const int *cowner, DomInfo CD, const DevFab *otab, int tmode,
+                                     double ta, double tb);
 __global__ void k_ext_from_coarse(Range r, Arr ext, Arr st, DomInfo CD, int ratio, int flo0,
                                   int flo1, int flo2, const int *fowner, DomInfo FD);
 __global__ void k_ext_cells(const int *cells, int64_t ncells, Arr ext, Arr st, DomInfo CD,

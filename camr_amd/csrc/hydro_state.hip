@@ -709,24 +709,24 @@ kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double 
 // CAMR_setup.cpp:141-144): AMReX's CellConservativeLinear with linear
 // limiting, restated -- per direction the monotonized-central slope of the
 // coarse data, then one factor alpha in [0,1] for the slope vector so that
-// every child lies within the min / max of the 27 coarse neighbours.  One
-// thread per fine cell of the domain, all NVAR components, into the fine
-// level's exterior array.
-__global__ void __launch_bounds__(kBlock)
-k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
-               const DevFab *otab, int tmode, double ta, double tb) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    // the coarse value FillPatch sees at (i, j, k): its source cell through
-    // the domain's periodicity / BCs, at the fill time -- new (tmode 0),
-    // old (1) or ta * old + tb * new (StateData::InterpAddBox) -- with the
-    // reflection sign; 0 where no box owns the source
+// every child lies within the min / max of the 27 coarse neighbours, into
+// the fine level's exterior array.  Two passes: the coarse values staged
+// densely (k_coarse_stage*), then one thread per fine cell and component
+// (k_ext_*).
+
+// The coarse value FillPatch sees at (i, j, k), components [nlo, nhi):
+// its source cell through the domain's periodicity / BCs, at the fill
+// time -- new (tmode 0), old (1) or ta * old + tb * new
+// (StateData::InterpAddBox) -- with the reflection sign; 0 where no box
+// owns the source.
+__device__ __forceinline__ void stage_point(int i, int j, int k, const Arr &st, const DevFab *ctab,
+                                            const int *cowner, const DomInfo &CD,
+                                            const DevFab *otab, int tmode, double ta, double tb,
+                                            int nlo, int nhi) {
     int m[3];
     unsigned odd;
     map_cell(CD, i, j, k, m, &odd);
     const int ob = owner_of(CD, cowner, m);
-    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
-    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
     for (int n = nlo; n < nhi; ++n) {
         double v = 0.0;
         if (ob >= 0) {
@@ -739,6 +739,29 @@ k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo C
         }
         st(i, j, k, n) = ((odd >> n) & 1u) ? -v : v;
     }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
+               const DevFab *otab, int tmode, double ta, double tb) {
+    int i, j, k;
+    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+    const int nlo = gridDim.y > 1 ? (int)blockIdx.y : 0;
+    const int nhi = gridDim.y > 1 ? (int)blockIdx.y + 1 : NVAR;
+    stage_point(i, j, k, st, ctab, cowner, CD, otab, tmode, ta, tb, nlo, nhi);
+}
+
+// The same staging over a list of coarse cells (x, y, z triples), one
+// component per block row.
+__global__ void __launch_bounds__(kBlock)
+k_coarse_stage_cells(const int *cells, int64_t ncells, Arr st, const DevFab *ctab,
+                     const int *cowner, DomInfo CD, const DevFab *otab, int tmode, double ta,
+                     double tb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncells) return;
+    const int n = (int)blockIdx.y;
+    stage_point(cells[3 * t], cells[3 * t + 1], cells[3 * t + 2], st, ctab, cowner, CD, otab,
+                tmode, ta, tb, n, n + 1);
 }
 
 // FillPatch's conservative linear interpolation of fine cell (i,j,k),
