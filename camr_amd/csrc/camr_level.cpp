@@ -1847,8 +1847,8 @@ int HydroLevel::est_time_step(double *est) {
     if (!dt_ready_) {
         const unsigned long long init = (unsigned long long)0x7fefffffffffffffULL;  // DBL_MAX
         launch_on(k_fill_u64, make_box1(), stream_, dtmin_, init);
-        for (size_t b = 0; b < box_.size(); ++b)
-            launch_on(k_estdt, box_[b], stream_, snew_[b], hc_.dx[0], hc_.dx[1], hc_.dx[2], dtmin_, status_, ebf_);
+        launch_all(kb_estdt, 0, (const DevFab *)(a_is_new_ ? tab_a_ : tab_b_), hc_.dx[0],
+                   hc_.dx[1], hc_.dx[2], dtmin_, status_, ebf_);
         CAMR_HIP(hipGetLastError());
         dt_ready_ = true;
     }

@@ -56,6 +56,8 @@ __global__ void k_fill_value(Range r, Arr a, int nc, double v);
 __global__ void k_fill_flat(double *p, int64_t n, double v);
 __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v);
 // the passes above over every box of a table in one launch (grid.y = box)
+__global__ void kb_estdt(const DevFab *S, double dx, double dy, double dz,
+                         unsigned long long *out, int *status, EbFlags eb);
 __global__ void kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb);
 __global__ void kb_floor_seq(const DevFab *S, int nbox, HydroConst hc, int *flags, EbFlags eb);
 __global__ void kb_clean_point(const DevFab *S, HydroConst hc);

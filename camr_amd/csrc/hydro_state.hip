@@ -430,6 +430,18 @@ __device__ __forceinline__ bool batch_cell(const DevFab *tab, int g, int &b, int
     return range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
 }
 
+// k_estdt over every box of the table in one launch (the minimum is exact,
+// so the box order does not matter)
+__global__ void __launch_bounds__(kBlock)
+kb_estdt(const DevFab *S, double dx, double dy, double dz, unsigned long long *out, int *status,
+         EbFlags eb) {
+    int b, i, j, k;
+    double v = 1.7976931348623157e308;
+    if (batch_cell(S, 0, b, i, j, k) && !eb.covered(i, j, k))
+        v = estdt_point(S[b].a, i, j, k, dx, dy, dz);
+    block_min_commit(v, out, status);
+}
+
 __global__ void __launch_bounds__(kBlock)
 kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb) {
     int b, i, j, k;
