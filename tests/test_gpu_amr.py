@@ -42,12 +42,15 @@ FUSED = _lib.ENGINE_FUSED
     (2, True, dict(ppm_type=1), False, _lib.ENGINE_STAGED),
     (2, False, dict(ppm_type=0), False, _lib.ENGINE_STAGED),
     (2, True, dict(do_mol=1), False, _lib.ENGINE_STAGED),
+    (2, False, dict(do_mol=1), False, _lib.ENGINE_STAGED),
     (4, True, dict(ppm_type=1), False, _lib.ENGINE_STAGED),
+    (4, False, dict(ppm_type=1), False, _lib.ENGINE_STAGED),
     (2, True, dict(ppm_type=1), True, _lib.ENGINE_STAGED),
     (2, True, dict(do_mol=1), True, _lib.ENGINE_STAGED),
     (2, True, dict(ppm_type=1), False, FUSED),
     (2, False, dict(ppm_type=0), False, FUSED),
     (4, True, dict(ppm_type=1), False, FUSED),
+    (4, False, dict(ppm_type=1), False, FUSED),
 ])
 def test_amr_step_matches_oracle(oracle, ratio, periodic, kw, graph, engine):
     """graph: every level step captured (per buffer parity and deferred
