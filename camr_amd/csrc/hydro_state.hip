@@ -666,7 +666,8 @@ kb_reflux(const DevFab *snew, const DevFab *crse, const DevFab *cfp) {
 }
 
 // average_down: a covered coarse cell = (1/r^3) * the sum of its children
-// in (k, j, i) order (fine boxes are aligned to the ratio).
+// in (k, j, i) order.  Fine boxes are aligned to the ratio (checked by the
+// host), so all children of a coarse cell lie in the box owning the first.
 __global__ void __launch_bounds__(kBlock)
 kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo FD, int ratio) {
     int b, i, j, k;
@@ -674,15 +675,14 @@ kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo F
     const int f0[3] = {i * ratio, j * ratio, k * ratio};
     const int fo = owner_of(FD, fowner, f0);
     if (fo < 0) return;
+    const Arr F = ftab[fo].a;
     const double volfrac = 1.0 / (double)(ratio * ratio * ratio);
     for (int n = 0; n < NVAR; ++n) {
         double c = 0.0;
         for (int kr = 0; kr < ratio; ++kr)
             for (int jr = 0; jr < ratio; ++jr)
-                for (int ir = 0; ir < ratio; ++ir) {
-                    const int fc[3] = {f0[0] + ir, f0[1] + jr, f0[2] + kr};
-                    c += ftab[owner_of(FD, fowner, fc)].a(fc[0], fc[1], fc[2], n);
-                }
+                for (int ir = 0; ir < ratio; ++ir)
+                    c += F(f0[0] + ir, f0[1] + jr, f0[2] + kr, n);
         csnew[b].a(i, j, k, n) = volfrac * c;
     }
 }
