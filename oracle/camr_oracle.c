@@ -42,6 +42,13 @@ static const int QPASS[NSPEC] = {8, 9, 10};
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 
+/* Planted mistakes for the mutation tests (tests/test_oracle_plants.py): a
+ * build with -DORA_PLANT=n carries slip n, so that the tests can show that
+ * their pins catch it.  The default build (ORA_PLANT 0) carries none. */
+#ifndef ORA_PLANT
+#define ORA_PLANT 0
+#endif
+
 #define FA(f, i, j, k, n)                                                   \
     ((f)->p[(int64_t)((i) - (f)->lo[0]) + (int64_t)((j) - (f)->lo[1]) * (f)->js + \
             (int64_t)((k) - (f)->lo[2]) * (f)->ks + (int64_t)(n) * (f)->ns])
@@ -923,7 +930,11 @@ static void trace_ppm(obox bx, int idir, const ora_fab *q, const ora_fab *qa,
             else if (un < 0.) { alpha0r *= -1.0; alpha0e_g *= -1.0; }
             else { alpha0r *= -0.5; alpha0e_g *= -0.5; }
             FV(qp, iv, O_QRHO) = rho_ref + alphap + alpham + alpha0r;
+#if ORA_PLANT == 4
+            FV(qp, iv, QUN) = un_ref + (alpham - alphap) * cc_ref * rho_ref_inv;
+#else
             FV(qp, iv, QUN) = un_ref + (alphap - alpham) * cc_ref * rho_ref_inv;
+#endif
             FV(qp, iv, O_QPRES) = p_ref + (alphap + alpham) * csq_ref;
             FV(qp, iv, O_QRHO) = smax(FV(qp, iv, O_QRHO), small_dens);
             FV(qp, iv, O_QPRES) = smax(FV(qp, iv, O_QPRES), small_pres);
@@ -997,7 +1008,11 @@ static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
     int ivpn[3] = {i, j, k}, ivpt[3] = {i, j, k};
     ivpn[dir] += 1;
     ivpt[other_dir] += 1;
+#if ORA_PLANT == 7
+    const int qvidx = (dir == 0) ? O_GDU : ((dir == 1) ? O_GDV : O_GDW);
+#else
     const int qvidx = (other_dir == 0) ? O_GDU : ((other_dir == 1) ? O_GDV : O_GDW);
+#endif
     int no_cov_face = 1, lo_open = 1, hi_open = 1;
     if (trans_area)
         no_cov_face = (FV(trans_area, ivpt, 0) > 0.0) && (FV(trans_area, iv, 0) > 0.0);
@@ -1082,8 +1097,13 @@ static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
             0.5 * (runewr * runewr + rvnewr * rvnewr + rwnewr * rwnewr) / rrnewr;
         FV(qp, iv, O_QREINT) = renewr - rhoekinr;
         if (!reset_state) {
+#if ORA_PLANT == 1
+            double pnewr = FV(qnormp, iv, O_QPRES) +
+                           cdtdx * (dAup + pav * dAu * (c - 1.0));
+#else
             double pnewr = FV(qnormp, iv, O_QPRES) -
                            cdtdx * (dAup + pav * dAu * (c - 1.0));
+#endif
             FV(qp, iv, O_QPRES) = smax(pnewr, small_pres);
         } else {
             FV(qp, iv, O_QPRES) = FV(qnormp, iv, O_QPRES);
@@ -1124,8 +1144,13 @@ static void transdo(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
             0.5 * (runewl * runewl + rvnewl * rvnewl + rwnewl * rwnewl) / rrnewl;
         FV(qm, ivpn, O_QREINT) = renewl - rhoekinl;
         if (!reset_state) {
+#if ORA_PLANT == 1
+            double pnewl = FV(qnormm, ivpn, O_QPRES) +
+                           cdtdx * (dAup + pav * dAu * (c - 1.0));
+#else
             double pnewl = FV(qnormm, ivpn, O_QPRES) -
                            cdtdx * (dAup + pav * dAu * (c - 1.0));
+#endif
             FV(qm, ivpn, O_QPRES) = smax(pnewl, small_pres);
         } else {
             FV(qm, ivpn, O_QPRES) = FV(qnormm, ivpn, O_QPRES);
@@ -1217,7 +1242,11 @@ static void transdd(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
         dAuy = ugpy - ugmy;
     }
     const double pxnew = cdtdx0 * (dAupx + pavx * dAux * (c - 1.0));
+#if ORA_PLANT == 8
+    const double pynew = 0.0 * cdtdx1 * (dAupy + pavy * dAuy * (c - 1.0));
+#else
     const double pynew = cdtdx1 * (dAupy + pavy * dAuy * (c - 1.0));
+#endif
 
     if (!lo_open) {
         FV(qp, iv, O_QRHO) = FV(qnormp, iv, O_QRHO);
@@ -1394,8 +1423,13 @@ static void artif_visc(int i, int j, int k, ora_fab *flx, const ora_fab *divu,
     ivpp[l1] += 1;
     ivpp[l2] += 1;
     const double div =
+#if ORA_PLANT == 5
+        difmag * smax(0.0, 0.25 * (FV(divu, iv, 0) + FV(divu, ivpj, 0) +
+                                   FV(divu, ivpk, 0) + FV(divu, ivpp, 0)));
+#else
         difmag * smin(0.0, 0.25 * (FV(divu, iv, 0) + FV(divu, ivpj, 0) +
                                    FV(divu, ivpk, 0) + FV(divu, ivpp, 0)));
+#endif
     int at_bndry = ((dir == 0) && (i == domlo) && (bclo == O_NoSlipWall)) ||
                    ((dir == 0) && (i == domhi + 1) && (bchi == O_NoSlipWall)) ||
                    ((dir == 1) && (j == domlo) && (bclo == O_NoSlipWall)) ||
@@ -1522,7 +1556,11 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
 
     /* transverse corrections and Riemann solves for the pairs d|t.
      * Pair order follows the reference: X|Y, X|Z, Y|X, Y|Z, Z|X, Z|Y. */
+#if ORA_PLANT == 2
+    const double cdt[3] = {0.5 * dt / dx, 0.5 * dt / dy, 0.5 * dt / dz};
+#else
     const double cdt[3] = {cdtdx, cdtdy, cdtdz};
+#endif
     ora_fab fl2[3][3], qg2[3][3]; /* fl2[d][t]: flux of direction d corrected by t */
     for (int d = 0; d < 3; ++d) {
         const int t0 = (d == 0) ? 1 : 0;
@@ -1575,9 +1613,15 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
         for (int k = tb.lo[2]; k <= tb.hi[2]; ++k)
         for (int j = tb.lo[1]; j <= tb.hi[1]; ++j)
         for (int i = tb.lo[0]; i <= tb.hi[0]; ++i)
+#if ORA_PLANT == 3
+            transdd(i, j, k, d, &qmf, &qpf, &qdm[d], &qdp[d], &fl2[t0][d],
+                    &fl2[t1][d], &qg2[t0][d], &qg2[t1][d], qaux, srcQ, hdt,
+                    hdtd[t0], hdtd[t1], trd, sp, NULL, NULL, NULL);
+#else
             transdd(i, j, k, d, &qmf, &qpf, &qdm[d], &qdp[d], &fl2[t0][t1],
                     &fl2[t1][t0], &qg2[t0][t1], &qg2[t1][t0], qaux, srcQ, hdt,
                     hdtd[t0], hdtd[t1], trd, sp, NULL, NULL, NULL);
+#endif
         const obox fb = bnodes(bx, d);
         for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
         for (int j = fb.lo[1]; j <= fb.hi[1]; ++j)
@@ -2073,7 +2117,11 @@ static void enforce_min_density(ora_fab *S, obox bx, const ora_params *P,
             for (int ii = -1; ii <= 1; ii++) {
                 if ((i + ii >= bx.lo[0]) && (j + jj >= bx.lo[1]) && (k + kk >= bx.lo[2]) &&
                     (i + ii <= bx.hi[0]) && (j + jj <= bx.hi[1]) && (k + kk <= bx.hi[2])) {
+#if ORA_PLANT == 6
+                    if (FA(S, i + ii, j + jj, k + kk, O_URHO) >= max_dens) {
+#else
                     if (FA(S, i + ii, j + jj, k + kk, O_URHO) > max_dens) {
+#endif
                         is = i + ii; js = j + jj; ks = k + kk;
                         max_dens = FA(S, is, js, ks, O_URHO);
                     }
@@ -3491,4 +3539,54 @@ int ora_amr_step(ora_level *C, ora_level *F, ora_fluxreg *R, double dtc) {
     ora_level *L[2] = {C, F};
     ora_fluxreg *Rs[1] = {R};
     return ora_amr_advance(L, Rs, 2, dtc);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Single-point entry points of the known-answer tests (tests/               */
+/* test_oracle_godunov.py): the stage functions above, on caller arrays.     */
+/* ------------------------------------------------------------------------ */
+void ora_transdo_point(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
+                       ora_fab *qp, const ora_fab *qnormm, const ora_fab *qnormp,
+                       const ora_fab *flxx, const ora_fab *qa, const ora_fab *qint,
+                       double cdtdx, int trd, double small_pres) {
+    transdo(i, j, k, dir, other_dir, qm, qp, qnormm, qnormp, flxx, qa, qint, cdtdx, trd,
+            small_pres, NULL, NULL);
+}
+
+void ora_transdd_point(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
+                       const ora_fab *qnormm, const ora_fab *qnormp, const ora_fab *flxx,
+                       const ora_fab *flxy, const ora_fab *qx, const ora_fab *qy,
+                       const ora_fab *qa, const ora_fab *srcq, double hdt, double cdtdx0,
+                       double cdtdx1, int trd, double small_pres) {
+    transdd(i, j, k, dir, qm, qp, qnormm, qnormp, flxx, flxy, qx, qy, qa, srcq, hdt, cdtdx0,
+            cdtdx1, trd, small_pres, NULL, NULL, NULL);
+}
+
+void ora_trace_ppm_point(int i, int j, int k, int idir, const ora_fab *q, const ora_fab *qa,
+                         ora_fab *qm, ora_fab *qp, double dt, const double dx[3],
+                         int use_flattening, double small_dens, double small_pres) {
+    obox b;
+    b.lo[0] = b.hi[0] = i;
+    b.lo[1] = b.hi[1] = j;
+    b.lo[2] = b.hi[2] = k;
+    trace_ppm(b, idir, q, qa, qm, qp, b, dt, dx, use_flattening, small_dens, small_pres);
+}
+
+void ora_artif_visc_point(int i, int j, int k, ora_fab *flx, const ora_fab *divu,
+                          const ora_fab *u, double dx, double difmag, int dir, int domlo,
+                          int domhi, int bclo, int bchi) {
+    artif_visc(i, j, k, flx, divu, u, dx, difmag, dir, domlo, domhi, bclo, bchi);
+}
+
+void ora_divu_node_point(int i, int j, int k, const ora_fab *q, ora_fab *divu,
+                         const ora_params *P) {
+    divu_node(i, j, k, q, P->dx[0], P->dx[1], P->dx[2], divu, P->domlo, P->domhi, P->bclo,
+              P->bchi);
+}
+
+int ora_enforce_min_density_box(ora_fab *S, const int lo[3], const int hi[3],
+                                const ora_params *P) {
+    int err = 0;
+    enforce_min_density(S, mkbox(lo, hi), P, &err, NULL);
+    return err;
 }

@@ -166,6 +166,28 @@ void   ora_level_set_old_box(ora_level *L, int b, const double *data);
 /* Sum of each component over valid cells (for conservation checks). */
 void   ora_level_sums(const ora_level *L, double sums[O_NVAR]);
 
+/* ---- single stage functions at one point (known-answer tests) ---- */
+void   ora_transdo_point(int i, int j, int k, int dir, int other_dir, ora_fab *qm,
+                         ora_fab *qp, const ora_fab *qnormm, const ora_fab *qnormp,
+                         const ora_fab *flxx, const ora_fab *qa, const ora_fab *qint,
+                         double cdtdx, int trd, double small_pres);
+void   ora_transdd_point(int i, int j, int k, int dir, ora_fab *qm, ora_fab *qp,
+                         const ora_fab *qnormm, const ora_fab *qnormp, const ora_fab *flxx,
+                         const ora_fab *flxy, const ora_fab *qx, const ora_fab *qy,
+                         const ora_fab *qa, const ora_fab *srcq, double hdt, double cdtdx0,
+                         double cdtdx1, int trd, double small_pres);
+void   ora_trace_ppm_point(int i, int j, int k, int idir, const ora_fab *q, const ora_fab *qa,
+                           ora_fab *qm, ora_fab *qp, double dt, const double dx[3],
+                           int use_flattening, double small_dens, double small_pres);
+void   ora_artif_visc_point(int i, int j, int k, ora_fab *flx, const ora_fab *divu,
+                            const ora_fab *u, double dx, double difmag, int dir, int domlo,
+                            int domhi, int bclo, int bchi);
+void   ora_divu_node_point(int i, int j, int k, const ora_fab *q, ora_fab *divu,
+                           const ora_params *P);
+/* enforce_min_density over lo..hi of S in place; returns the error bits */
+int    ora_enforce_min_density_box(ora_fab *S, const int lo[3], const int hi[3],
+                                   const ora_params *P);
+
 /* OpenMP thread count used by the tiled loops (0 = library default). */
 void   ora_set_threads(int n);
 int    ora_get_threads(void);

@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# CAMR_ORACLE_LIB selects another build of the oracle (the mutation tests
+# load builds that carry a planted slip); the default is the committed recipe's.
+_LIB = os.environ.get("CAMR_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 NVAR, QVAR, NQAUX, NGDNV = 10, 11, 5, 6
 
@@ -131,6 +133,15 @@ def lib():
         L.ora_level_exterior_from_coarse_time.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_double] * 3
         L.ora_level_exterior_from_coarse_time.restype = C.c_int
         L.ora_level_set_old_box.argtypes = [C.c_void_p, C.c_int, dp]
+        F = C.POINTER(OraFab)
+        L.ora_transdo_point.argtypes = [C.c_int] * 5 + [F] * 7 + [C.c_double, C.c_int, C.c_double]
+        L.ora_transdd_point.argtypes = [C.c_int] * 4 + [F] * 10 + [C.c_double] * 3 + [C.c_int, C.c_double]
+        L.ora_trace_ppm_point.argtypes = [C.c_int] * 4 + [F] * 4 + [C.c_double, dp, C.c_int,
+                                                                    C.c_double, C.c_double]
+        L.ora_artif_visc_point.argtypes = [C.c_int] * 3 + [F] * 3 + [C.c_double, C.c_double] + [C.c_int] * 5
+        L.ora_divu_node_point.argtypes = [C.c_int] * 3 + [F, F, C.POINTER(OraParams)]
+        L.ora_enforce_min_density_box.argtypes = [F, ip, ip, C.POINTER(OraParams)]
+        L.ora_enforce_min_density_box.restype = C.c_int
         L.ora_set_threads.argtypes = [C.c_int]
         L.ora_get_threads.restype = C.c_int
         _lib = L
