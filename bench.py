@@ -55,7 +55,20 @@ MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15783877140}
 # lane; FMAs only from the division / square-root expansions, contraction is
 # off), summed over the step's kernels from rocprofv3 SQ_INSTS_VALU_*_F64
 # counters at 256^3 (profiles/valu_fused_256cube_ppm.md).
-FP64_FLOP_PER_CELL_UPDATE = 10114
+FP64_EXECUTED_PER_CELL_UPDATE = 10114
+
+# Algorithmic FP64 operations of the reference's path per cell-update, by
+# stage (add/sub, mul, div, sqrt = 1 each; no FMA): the oracle run with an
+# operation-counting real on the 256^3 PPM workload, untiled
+# (`python oracle/count_ops.py --n 256`, profiles/fp64_opcount_256cube_ppm.md).
+# Keys are the profiler regions of the fused engine; a face stage's count
+# covers its three launches (one per direction).
+ALGO_FLOP = {"fill_clean_prim": 169.0, "flatten": 59.3, "trace_ppm": 3501.9,
+             "riemann_normal": 300.6, "transverse_1": 1285.1, "divu": 30.4,
+             "transverse_2": 989.9, "consup_clean_estdt": 111.0 + 129.0}
+ALGO_FLOP_PER_CELL_UPDATE = sum(ALGO_FLOP.values())
+LAUNCHES_PER_STEP = {"trace_ppm": 3, "riemann_normal": 3, "transverse_1": 3,
+                     "transverse_2": 3}
 
 
 def unit_count(name: str, n: int) -> int:
@@ -71,10 +84,24 @@ def unit_count(name: str, n: int) -> int:
     return n ** 3
 
 
-def cpu_baseline(n: int, cfl: float, U0, nsteps: int = 2):
-    """`nsteps` steps of the same workload with the CPU oracle (the
-    reference's stages restated in C, OpenMP over hydro tiles), on this
-    host's cores."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n: int, cfl: float, U0, warmup: int = 1, nsteps: int = 5):
+    """The same workload with the CPU oracle (the reference's stages
+    restated in C, OpenMP over hydro tiles) on this host's cores: `warmup`
+    untimed steps, then the median of `nsteps` timed steps, each step
+    computeInitialDt / computeNewDt + CAMR_advance.  Returns the baseline and
+    the oracle's state and dt sequence after all its steps."""
+    import numpy as np
     from oracle import oracle as O
     O.build()
     from camr_amd import workloads
@@ -85,19 +112,28 @@ def cpu_baseline(n: int, cfl: float, U0, nsteps: int = 2):
     L = O.Level(p, prob.boxes)
     L.set_box(0, U0)
     L.init_post()
-    dt = L.initial_dt()
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
+    dts, times = [], []
+    dt = None
+    for s in range(warmup + nsteps):
+        t0 = time.perf_counter()
+        dt = L.initial_dt() if s == 0 else L.new_dt(dt)
         err = L.advance(dt)
+        el = time.perf_counter() - t0
         assert err == 0
-        dt = L.new_dt(dt)  # the dt reduction belongs to the step
-    el = time.perf_counter() - t0
+        dts.append(dt)
+        if s >= warmup:
+            times.append(el)
+    state = L.get_box(0)
     L.close()
-    return {"value": n ** 3 * nsteps / el, "unit": "cell-updates/s",
-            "cores": O.lib().ora_get_threads(), "kind": "port",
-            "sample": f"{nsteps} full steps (advance + new dt) of the same "
-                      f"{n}^3 PPM workload from the same initial state, "
-                      f"{el:.1f} s wall on {O.lib().ora_get_threads()} threads"}
+    med = float(np.median(times))
+    threads = O.lib().ora_get_threads()
+    return ({"value": n ** 3 / med, "unit": "cell-updates/s", "cores": threads,
+             "kind": "port", "cpu_model": cpu_model(),
+             "sample": f"the same {n}^3 PPM workload from the same initial state: "
+                       f"{warmup} warm-up step, then the median of {nsteps} timed steps "
+                       f"(computeNewDt + CAMR_advance), {med:.2f} s per step on "
+                       f"{threads} OpenMP threads, hydro tiles (1024,16,16)"},
+            state, np.array(dts))
 
 
 def _amr_level(prob, mode, coarse, init_field):
@@ -147,7 +183,7 @@ def bench_amr(args) -> int:
                 T = _amr_level(prob, mode, src, init)
                 res["tuning"] = autotune.tune(T, steps=2)
                 T.close()
-                L.tuning = res["tuning"]
+                autotune.apply(L, *res["tuning"])
             L.run(args.warmup)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -506,17 +542,16 @@ def bench_amr_sync(args) -> int:
     return 0
 
 
-def bench_adjoint(args) -> int:
-    """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
-    wall time of the backward pass (recomputation from the checkpoints
-    included) over that of the forward steps, and the gradient checked
-    against central differences along seeded unit directions."""
+def adjoint_measure(n: int, nsteps: int, every: int, reps: int, ndirs: int) -> dict:
+    """Adjoint of `nsteps` PPM level steps at n^3 (checkpoints every
+    `every`): wall time of the backward pass (recomputation from the
+    checkpoints included) over that of the forward steps, best of `reps`;
+    with `ndirs` > 0 the gradient is checked against central differences
+    along seeded unit directions."""
     import numpy as np
     import torch
     from camr_amd import problems, workloads
     from camr_amd.hydro import Level
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
-    n, nsteps, every = args.adj_n, args.steps, 5
     prob = workloads.smooth_periodic(n=n, ppm_type=1, cfl=0.4)
 
     def level_from(seed):
@@ -551,7 +586,7 @@ def bench_adjoint(args) -> int:
 
     gradient()  # warm-up: allocates the checkpoints and adjoint scratch
     times = {"forward": [], "forward_checkpointed": [], "backward": []}
-    for _ in range(max(1, args.adj_reps)):
+    for _ in range(max(1, reps)):
         L.set_box(0, S0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -574,42 +609,54 @@ def bench_adjoint(args) -> int:
         torch.cuda.synchronize()
         times["backward"].append(time.perf_counter() - t0)
     ms = {k: 1e3 * min(v) for k, v in times.items()}
-    grad = L.get_adjoint(0)
-    gnorm = float(np.sqrt((grad * grad).sum()))
-    rng = np.random.default_rng(problems.SEED_DIRECTIONS)
-    eps, fd_rows = 1e-6, []
-    l0 = 0.5 * float(((forward(S0)[0] - rho_t) ** 2).sum()) if args.adj_dirs else 0.0
-    for _ in range(args.adj_dirs):
-        v = rng.standard_normal(S0.shape)
-        v /= np.sqrt((v * v).sum())
-        lp = 0.5 * float(((forward(S0 + eps * v)[0] - rho_t) ** 2).sum())
-        lm = 0.5 * float(((forward(S0 - eps * v)[0] - rho_t) ** 2).sum())
-        fd = (lp - lm) / (2 * eps)
-        # one-sided slopes: they differ where the step crosses a limiter or
-        # upwind switch within +-eps, which the central difference averages
-        fr, fl = (lp - l0) / eps, (l0 - lm) / eps
-        ad = float((grad * v).sum())
-        fd_rows.append((fd, ad, abs(fr - fl)))
-    err_g = max((abs(f - a) / gnorm for f, a, _ in fd_rows), default=None)
-    err_d = max((abs(f - a) / abs(f) for f, a, _ in fd_rows), default=None)
-    kink = max((k / abs(f) for f, _, k in fd_rows), default=None)
-    ratio = ms["backward"] / ms["forward"]
-    out = {"metric": "adjoint/forward cost ratio", "value": ratio, "unit": "x",
-           "higher_is_better": False, "n_gpus": 1, "steps": nsteps, "warmup": 1,
-           "dtype": "f64", "data": "synthetic", "vs_baseline": None,
-           "config": {"workload": f"adjoint of {nsteps} periodic PPM level steps at {n}^3, "
-                                  f"checkpoints every {every} steps, frozen dt sequence, "
-                                  "L = 1/2 sum (rho(T) - rho_target)^2 over all initial "
-                                  "conserved components; smooth field with a smooth "
-                                  "pressure variation"},
+    out = {"n": n, "steps": nsteps, "checkpoint_every": every,
            "forward_ms": ms["forward"], "forward_checkpointed_ms": ms["forward_checkpointed"],
-           "backward_ms": ms["backward"],
-           "backward_over_checkpointed_forward": ms["backward"] / ms["forward_checkpointed"],
-           "fd_check": {"directions": args.adj_dirs, "eps": eps, "grad_norm": gnorm,
-                        "max_abs_err_over_grad_norm": err_g,
-                        "max_abs_err_over_directional": err_d,
-                        "max_one_sided_slope_gap_over_directional": kink,
-                        "directions_fd_adjoint_slope_gap": [[f, a, k] for f, a, k in fd_rows]}}
+           "backward_ms": ms["backward"], "ratio": ms["backward"] / ms["forward"],
+           "backward_over_checkpointed_forward": ms["backward"] / ms["forward_checkpointed"]}
+    if ndirs > 0:
+        grad = L.get_adjoint(0)
+        gnorm = float(np.sqrt((grad * grad).sum()))
+        rng = np.random.default_rng(problems.SEED_DIRECTIONS)
+        eps, fd_rows = 1e-6, []
+        l0 = 0.5 * float(((forward(S0)[0] - rho_t) ** 2).sum())
+        for _ in range(ndirs):
+            v = rng.standard_normal(S0.shape)
+            v /= np.sqrt((v * v).sum())
+            lp = 0.5 * float(((forward(S0 + eps * v)[0] - rho_t) ** 2).sum())
+            lm = 0.5 * float(((forward(S0 - eps * v)[0] - rho_t) ** 2).sum())
+            fd = (lp - lm) / (2 * eps)
+            # one-sided slopes: they differ where the step crosses a limiter
+            # or upwind switch within +-eps, which the central difference
+            # averages
+            fr, fl = (lp - l0) / eps, (l0 - lm) / eps
+            ad = float((grad * v).sum())
+            fd_rows.append((fd, ad, abs(fr - fl)))
+        out["fd_check"] = {
+            "directions": ndirs, "eps": eps, "grad_norm": gnorm,
+            "max_abs_err_over_grad_norm": max(abs(f - a) / gnorm for f, a, _ in fd_rows),
+            "max_abs_err_over_directional": max(abs(f - a) / abs(f) for f, a, _ in fd_rows),
+            "max_one_sided_slope_gap_over_directional": max(k / abs(f) for f, _, k in fd_rows),
+            "directions_fd_adjoint_slope_gap": [[f, a, k] for f, a, k in fd_rows]}
+    L.close()
+    return out
+
+
+def bench_adjoint(args) -> int:
+    """Adjoint of `steps` PPM level steps at n^3 (checkpoints every 5):
+    backward over forward wall time, and the gradient checked against
+    central differences along seeded unit directions."""
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    r = adjoint_measure(args.adj_n, args.steps, 5, args.adj_reps, args.adj_dirs)
+    out = {"metric": "adjoint/forward cost ratio", "value": r["ratio"], "unit": "x",
+           "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "dtype": "f64", "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": f"adjoint of {args.steps} periodic PPM level steps at "
+                                  f"{args.adj_n}^3, checkpoints every 5 steps, frozen dt "
+                                  "sequence, L = 1/2 sum (rho(T) - rho_target)^2 over all "
+                                  "initial conserved components; smooth field with a "
+                                  "smooth pressure variation"}}
+    out.update({k: v for k, v in r.items() if k != "ratio"})
     print(json.dumps(out), flush=True)
     return 0
 
@@ -622,6 +669,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="c1: leave out the adjoint/forward ratio sub-measurement")
     ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb", "mol", "amr_sync"],
                     default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
@@ -666,6 +715,7 @@ def main() -> int:
     if args.workload == "amr_sync":
         return bench_amr_sync(args)
 
+    import numpy as np
     import torch
     from camr_amd import _lib, workloads
     from camr_amd.hydro import Level
@@ -724,8 +774,29 @@ def main() -> int:
 
     cups = n ** 3 * args.steps / el
     avg_ms = rep["ms"] / rep["count"]
-    algo = ALGO_DOUBLES.get(kern, 2 * NVAR) * 8 * unit_count(kern, n)
-    achieved = algo / (avg_ms * 1e-3) / 1e9
+    # algorithmic FP64 operations of one launch of the dominant kernel: its
+    # stage's count per cell-update over the launches of that stage per step
+    flop_launch = ALGO_FLOP.get(kern, 0.0) * n ** 3 / LAUNCHES_PER_STEP.get(kern, 1)
+    achieved = flop_launch / (avg_ms * 1e-3) / 1e12
+    algo_bytes = ALGO_DOUBLES.get(kern, 2 * NVAR) * 8 * unit_count(kern, n)
+    tiles = L.tiles if args.engine == "fused" else None
+
+    # parity at full size: the cpu baseline's steps from U0 on the GPU
+    n_check = 6
+    L.set_box(0, U0)
+    L.init_post()
+    gdts = []
+    dt = None
+    for s_ in range(n_check):
+        dt = L.initial_dt() if s_ == 0 else L.new_dt(dt)
+        L.advance(dt)
+        gdts.append(dt)
+    L.sync()
+    U_gpu = L.get_box(0)
+    L.close()
+    del L
+    torch.cuda.synchronize()
+
     out = {
         "metric": "FP64 cell-updates/s at 256³ PPM; % HBM peak; adjoint/forward cost ratio",
         "value": cups,
@@ -743,22 +814,39 @@ def main() -> int:
         "config": {"workload": f"{n}^3 periodic single-level Godunov PPM advance "
                                f"(CAMR_advance + computeNewDt), CFL {cfl}",
                    "cells": n ** 3, "ppm_type": 1, "cfl": cfl,
-                   "engine": args.engine, "tuning": tuning},
-        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
+                   "engine": args.engine, "tuning": tuning, "tiles": tiles},
+        "roofline": {"bound": "fp64", "kernel": kern, "achieved": achieved,
+                     "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFS,
                      "traffic": MEASURED_TRAFFIC.get((args.engine, n, kern)),
+                     "algorithmic_flop_per_launch": flop_launch,
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "hbm_frac_of_algorithmic_bytes":
+                         algo_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "avg_launch_ms": avg_ms, "launches": rep["count"]},
-        "step_hbm": {"bytes_per_cell_update": 2 * NVAR * 8,
-                     "achieved_GBs": cups * 2 * NVAR * 8 / 1e9,
-                     "frac": cups * 2 * NVAR * 8 / 1e9 / HBM_PEAK_GBS},
-        "step_fp64": {"flop_per_cell_update": FP64_FLOP_PER_CELL_UPDATE,
-                      "achieved_TFs": cups * FP64_FLOP_PER_CELL_UPDATE / 1e12,
+        "step_fp64": {"algorithmic_flop_per_cell_update": ALGO_FLOP_PER_CELL_UPDATE,
+                      "achieved_TFs": cups * ALGO_FLOP_PER_CELL_UPDATE / 1e12,
                       "peak_TFs": FP64_PEAK_TFS,
-                      "frac": cups * FP64_FLOP_PER_CELL_UPDATE / 1e12 / FP64_PEAK_TFS},
+                      "frac": cups * ALGO_FLOP_PER_CELL_UPDATE / 1e12 / FP64_PEAK_TFS,
+                      "executed_flop_per_cell_update": FP64_EXECUTED_PER_CELL_UPDATE},
+        "step_hbm": {"bytes_per_cell_update": {"NC6": 96, "NC10": 2 * NVAR * 8},
+                     "frac_NC6": cups * 96 / 1e9 / HBM_PEAK_GBS,
+                     "frac_NC10": cups * 2 * NVAR * 8 / 1e9 / HBM_PEAK_GBS},
     }
+    if not args.no_adjoint:
+        # the metric's third part: the adjoint of 20 PPM steps at 128^3 with
+        # checkpoints every 5 steps, backward over forward wall time
+        r = adjoint_measure(128, 20, 5, 2, 0)
+        out["adjoint"] = {"ratio": r["ratio"], "forward_ms": r["forward_ms"],
+                          "backward_ms": r["backward_ms"],
+                          "workload": "adjoint of 20 periodic PPM level steps at 128^3, "
+                                      "checkpoints every 5, L = 1/2 sum (rho(T) - rho_t)^2"}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, cfl, U0)
+        base, U_ora, odts = cpu_baseline(n, cfl, U0, warmup=1, nsteps=n_check - 1)
+        out["cpu_baseline"] = base
+        out["identical_to_oracle"] = bool(np.array_equal(U_gpu, U_ora) and
+                                          np.array_equal(np.array(gdts), odts))
+        out["oracle_check"] = (f"state and dt sequence after {n_check} steps from the same "
+                               "initial state, GPU against the CPU oracle, bit for bit")
     print(json.dumps(out), flush=True)
     return 0
 

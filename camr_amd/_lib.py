@@ -89,7 +89,7 @@ NSTAGE = 13  # CAMR_NSTAGE: stage ranges of the fused engine
 # Every extern "C" symbol declared in include/camr_hydro.h.
 EXPORTS = [
     "camr_default_params", "camr_last_error", "camr_version",
-    "camr_hydro_ctoprim", "camr_hydro_srctoprim", "camr_hydro_umdrv",
+    "camr_hydro_ctoprim", "camr_hydro_srctoprim", "camr_hydro_umdrv", "camr_scratch_bytes",
     "camr_estdt_hydro", "camr_default_level_desc", "camr_level_create",
     "camr_level_destroy", "camr_level_set_stream", "camr_level_upload_box",
     "camr_level_download_box", "camr_level_state_array",
@@ -97,6 +97,7 @@ EXPORTS = [
     "camr_level_advance", "camr_level_run", "camr_level_sync",
     "camr_level_time", "camr_level_profile", "camr_level_profile_report",
     "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
+    "camr_level_set_tiles", "camr_level_get_tiles", "camr_level_set_xcd", "camr_level_get_xcd",
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
@@ -140,6 +141,8 @@ def lib():
         L.camr_hydro_srctoprim.argtypes = [C.POINTER(Box), A4, A4, A4, A4, vp]
         L.camr_hydro_umdrv.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4, A4, A4,
                                        C.c_double, C.POINTER(HydroParams), vp]
+        L.camr_scratch_bytes.argtypes = [C.POINTER(Box), C.POINTER(HydroParams),
+                                         C.POINTER(C.c_size_t)]
         L.camr_estdt_hydro.argtypes = [C.POINTER(Box), A4, dp, dp, vp]
         L.camr_level_create.argtypes = [C.POINTER(LevelDesc), C.POINTER(vp)]
         L.camr_level_destroy.argtypes = [vp]
@@ -160,6 +163,10 @@ def lib():
         L.camr_level_set_exterior.argtypes = [vp, dp]
         L.camr_level_set_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.c_int32]
         L.camr_level_get_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.camr_level_set_tiles.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.camr_level_get_tiles.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.camr_level_set_xcd.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.camr_level_get_xcd.argtypes = [vp, C.POINTER(C.c_int32)]
         L.camr_level_adjoint_forward.argtypes = [vp, C.c_int32, dp, C.c_int32]
         L.camr_level_adj_upload_box.argtypes = [vp, C.c_int32, dp]
         L.camr_level_adj_download_box.argtypes = [vp, C.c_int32, dp]

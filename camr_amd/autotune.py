@@ -1,22 +1,29 @@
 """Autotuner for the batched level engine's launch shapes.
 
-What is tuned (``Level.tuning``):
+What is tuned (``Level.tuning`` and ``Level.tiles``):
 
-* the block size (64, 128 or 256 threads: one to four wavefronts) of each of
-  the 13 stage ranges of the fused engine (``CAMR_NSTAGE``): ghost fill /
-  primitives on grow(bx,4), flattening / traces on grow(bx,2), first-flux
-  faces (x, y, z), transdo faces (x, y, z), nodes, final faces (x, y, z)
-  and the valid box;
+* the block of each of the 13 stage ranges of the fused engine
+  (``CAMR_NSTAGE``: ghost fill / primitives on grow(bx,4), flattening /
+  traces on grow(bx,2), first-flux faces (x, y, z), transdo faces (x, y, z),
+  nodes, final faces (x, y, z) and the valid box): either a run of 64, 128
+  or 256 consecutive points in x-fastest order, or a 3-D tile of points
+  (e.g. 16 x 4 x 4) that keeps the y and z stencil neighbours of a block's
+  points inside the block, so that they are re-read from its own caches
+  rather than from HBM;
+* the XCD chunk of each stage range: the hardware hands consecutive blocks
+  to the eight XCDs in turn, so a block's neighbouring rows sit in other
+  L2 caches; with chunk K every XCD takes K consecutive blocks of each
+  window of 8K;
 * the order of the boxes in the batched launches (level order, or largest
   box first so that the longest boxes start first and the tail is short).
 
 How: every stage kernel is timed with the level's hipEvent profiler over a
-few steps for each candidate block size (one run per size covers every
+few steps for each candidate block (one run per candidate covers every
 stage, since a stage's cost depends only on its own launch shape); the
-fastest size per stage range is kept (the three directions of a face stage
-share one profiler name, so they share one size).  The combined choice is
-then checked against the default on whole steps and kept only if faster.
-Results are cached in a JSON file keyed by the device name and the
+fastest candidate per stage range is kept (the three directions of a face
+stage share one profiler name, so they share one candidate).  The combined
+choice is then checked against the default on whole steps and kept only if
+faster.  Results are cached in a JSON file keyed by the device name and the
 histogram of box sizes, so a level with the same box population reuses
 them without timing anything.
 """
@@ -32,6 +39,15 @@ import numpy as np
 from . import _lib
 
 BLOCKS = (64, 128, 256)
+LINEAR = (0, 0, 0)
+# 3-D tiles (x, y, z) of 256 or 128 points; x at least 8 for whole 64-byte
+# row segments
+TILES = ((64, 4, 1), (32, 8, 1), (32, 4, 2), (16, 4, 4), (64, 2, 2), (16, 8, 2),
+         (8, 8, 4), (32, 2, 4), (16, 4, 2), (32, 2, 2))
+XCD_CHUNKS = (0, 16)
+# candidates: (block size, tile, XCD chunk); linear blocks carry LINEAR
+CANDIDATES = tuple((bs, LINEAR, k) for bs in BLOCKS for k in XCD_CHUNKS) + tuple(
+    (t[0] * t[1] * t[2], t, 0) for t in TILES[:4])
 # profiler region name -> stage-range indices (CAMR_NSTAGE order)
 STAGES = {
     "fill_clean_prim": (0,), "prim": (0,),
@@ -84,10 +100,20 @@ def _step_time(level, steps: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
+def apply(level, blocks, order, tiles, xcd=None):
+    """Set per-stage blocks (linear block sizes or tiles), XCD chunks and
+    the box order."""
+    level.tuning = ([b if t == LINEAR else 256 for b, t in zip(blocks, tiles)], order)
+    level.tiles = tiles
+    level.xcd = xcd if xcd is not None else [0] * _lib.NSTAGE
+
+
 def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
     """Choose and apply the launch shapes of `level` (which must hold a
     valid state; it is advanced by the timing runs).  Returns the tuning
-    (blocks, order) now set on the level."""
+    (blocks, order, tiles, xcd) now set on the level: per stage range its
+    block size, its tile ((0, 0, 0) for linear blocks) and its XCD chunk,
+    and the box order."""
     device = _device_name()
     key = key_of(level, device)
     path = cache_path()
@@ -98,43 +124,48 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if key in cache:
+    if key in cache and "xcd" in cache[key]:
         t = cache[key]
-        level.tuning = (t["blocks"], t["order"])
-        return level.tuning
+        tiles = [tuple(x) for x in t["tiles"]]
+        apply(level, t["blocks"], t["order"], tiles, t["xcd"])
+        return list(t["blocks"]), t["order"], tiles, list(t["xcd"])
 
-    default = level.tuning
+    default = (list(level.tuning[0]), level.tuning[1], list(level.tiles), list(level.xcd))
     level.run(1)  # warm-up (graph capture, first launches)
-    # per-stage times for each uniform block size, both box orders
-    best_blocks = list(default[0])
+    # per-stage times for each uniform candidate, both box orders
     times = {}
     for order in (0, 1):
-        for bs in BLOCKS:
-            level.tuning = ([bs] * _lib.NSTAGE, order)
-            times[(order, bs)] = _stage_times(level, steps)
+        for cand in CANDIDATES:
+            bs, tile, xk = cand
+            apply(level, [bs] * _lib.NSTAGE, order, [tile] * _lib.NSTAGE, [xk] * _lib.NSTAGE)
+            times[(order, cand)] = _stage_times(level, steps)
     results = []
     for order in (0, 1):
         blocks = [256] * _lib.NSTAGE
+        tiles = [LINEAR] * _lib.NSTAGE
+        xcd = [0] * _lib.NSTAGE
         for grp in GROUPS:
             names = [n for n, st in STAGES.items() if st == grp or set(st) & set(grp)]
-            cost = {bs: sum(times[(order, bs)].get(n, 0.0) for n in names) for bs in BLOCKS}
+            cost = {c: sum(times[(order, c)].get(n, 0.0) for n in names) for c in CANDIDATES}
             pick = min(cost, key=cost.get)
             for g in grp:
-                blocks[g] = pick
-        level.tuning = (blocks, order)
-        results.append((_step_time(level, steps), blocks, order))
-    level.tuning = default
+                blocks[g], tiles[g], xcd[g] = pick
+        apply(level, blocks, order, tiles, xcd)
+        results.append((_step_time(level, steps), blocks, order, tiles, xcd))
+    apply(level, *default)
     t_def = _step_time(level, steps)
     results.sort(key=lambda r: r[0])
-    t_best, blocks, order = results[0]
+    t_best, blocks, order, tiles, xcd = results[0]
     if t_best >= t_def:
-        blocks, order = list(default[0]), default[1]
+        blocks, order, tiles, xcd = default
         t_best = t_def
-    level.tuning = (blocks, order)
+    apply(level, blocks, order, tiles, xcd)
     if verbose:
         print(f"autotune {key}: default {t_def * 1e3:.3f} ms/step, chosen "
-              f"{t_best * 1e3:.3f} ms/step, blocks {blocks}, order {order}", flush=True)
+              f"{t_best * 1e3:.3f} ms/step, blocks {blocks}, tiles {tiles}, xcd {xcd}, "
+              f"order {order}", flush=True)
     cache[key] = {"blocks": list(map(int, blocks)), "order": int(order),
+                  "tiles": [list(map(int, t)) for t in tiles], "xcd": list(map(int, xcd)),
                   "ms_per_step": t_best * 1e3, "default_ms_per_step": t_def * 1e3}
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -142,4 +173,4 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
             json.dump(cache, f, indent=1, sort_keys=True)
     except OSError:
         pass
-    return level.tuning
+    return list(blocks), order, [tuple(t) for t in tiles], list(xcd)

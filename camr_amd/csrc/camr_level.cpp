@@ -18,6 +18,7 @@
 #include <sstream>
 #include <string>
 #include <sys/stat.h>
+#include <mutex>
 #include <vector>
 
 #include "hydro_adjoint.h"
@@ -387,6 +388,12 @@ int HydroLevel::set_exterior(const double *h) {
 int HydroLevel::exterior_from(HydroLevel &C, int ratio, int tmode, double ta, double tb,
                               bool ghosts_only) {
     if (ratio < 2) { set_error("refinement ratio must be at least 2"); return CAMR_ERR_PARAM; }
+    // EB builds interpolate with eb_cell_cons_interp (CAMR_setup.cpp:139-143),
+    // which weighs by volume fractions and skips covered cells: not built
+    if (ebflags_ || C.ebflags_) {
+        set_error("coarse-fine interpolation with embedded boundaries is not built");
+        return CAMR_ERR_PARAM;
+    }
     for (int d = 0; d < 3; ++d)
         if (desc_.hp.domlo[d] != C.desc_.hp.domlo[d] * ratio ||
             desc_.hp.domhi[d] + 1 != (C.desc_.hp.domhi[d] + 1) * ratio) {
@@ -2156,6 +2163,12 @@ int HydroLevel::reg_add_fused() {
 int average_down(HydroLevel &C, HydroLevel &F, int ratio) {
     int rc = check_refines(F.desc_, C.desc_, ratio);
     if (rc) return rc;
+    // EB builds average with EB_average_down (CAMR.cpp:1060), volume-fraction
+    // weighted over non-covered children: not built
+    if (C.ebflags_ || F.ebflags_) {
+        set_error("average_down with embedded boundaries is not built");
+        return CAMR_ERR_PARAM;
+    }
     for (const Box &B : F.box_)
         for (int d = 0; d < 3; ++d)
             if (B.lo[d] % ratio || (B.hi[d] + 1) % ratio) {
@@ -2613,10 +2626,13 @@ int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q, const camr_a
     return CAMR_OK;
 }
 
-int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
-                     const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
-                     const camr_array4 flx[3], const camr_array4 area[3], const camr_array4 *vol,
-                     double dt, const camr_hydro_params *p, void *stream) {
+// Arguments of one hydro_umdrv call, checked; *n_scratch receives the
+// scratch doubles the call needs (the dt word not included).
+static int umdrv_args(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
+                      const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
+                      const camr_array4 flx[3], const camr_array4 area[3],
+                      const camr_array4 *vol, double dt, const camr_hydro_params *p,
+                      UmdrvArgs *A, size_t *n_scratch) {
     if (!bx || !uin || !dsdt || !q || !qaux || !flx || !p) { set_error("null argument"); return CAMR_ERR_PARAM; }
     int rc = check_params(*p);
     if (rc) return rc;
@@ -2637,37 +2653,95 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     }
     const bool has_vol = vol && vol->p;
     if (has_vol && !covers(*vol, B, 1)) { set_error("vol must cover bx"); return CAMR_ERR_SHAPE; }
-    UmdrvArgs A;
-    A.bx = B;
-    A.qbox = box_of(*q);
-    A.hc = make_const(*p);
-    A.uin = view(*uin);
-    A.dsdt = view(*dsdt);
-    A.q = view(*q);
-    A.qaux = view(*qaux);
-    A.srcq = has_src ? view(*srcq) : view(*q);
-    A.has_src = has_src ? 1 : 0;
+    A->bx = B;
+    A->qbox = box_of(*q);
+    A->hc = make_const(*p);
+    A->uin = view(*uin);
+    A->dsdt = view(*dsdt);
+    A->q = view(*q);
+    A->qaux = view(*qaux);
+    A->srcq = has_src ? view(*srcq) : view(*q);
+    A->has_src = has_src ? 1 : 0;
     for (int d = 0; d < 3; ++d) {
-        A.flx[d] = view(flx[d]);
-        A.area[d] = has_area ? view(area[d]) : A.flx[d];
+        A->flx[d] = view(flx[d]);
+        A->area[d] = has_area ? view(area[d]) : A->flx[d];
     }
-    A.has_area = has_area ? 1 : 0;
-    A.area_c[0] = p->dx[1] * p->dx[2];
-    A.area_c[1] = p->dx[0] * p->dx[2];
-    A.area_c[2] = p->dx[0] * p->dx[1];
-    A.vol = has_vol ? view(*vol) : A.dsdt;
-    A.has_vol = has_vol ? 1 : 0;
-    A.vol_c = p->dx[0] * p->dx[1] * p->dx[2];
-    hipStream_t s = (hipStream_t)stream;
-    double *scr = nullptr;
-    const size_t n = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
-    CAMR_HIP(hipMallocAsync((void **)&scr, (n + 1) * sizeof(double), s));
+    A->has_area = has_area ? 1 : 0;
+    A->area_c[0] = p->dx[1] * p->dx[2];
+    A->area_c[1] = p->dx[0] * p->dx[2];
+    A->area_c[2] = p->dx[0] * p->dx[1];
+    A->vol = has_vol ? view(*vol) : A->dsdt;
+    A->has_vol = has_vol ? 1 : 0;
+    A->vol_c = p->dx[0] * p->dx[1] * p->dx[2];
+    *n_scratch = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
+    return CAMR_OK;
+}
+
+// The kernels of one hydro_umdrv call on `scr` (n doubles of scratch, then
+// the dt word).
+static int umdrv_run(UmdrvArgs &A, double *scr, size_t n, double dt, int do_mol,
+                     hipStream_t s) {
     A.dtp = scr + n;
     put_double(scr + n, dt, s);
-    rc = p->do_mol ? umdrv_mol(A, scr, s) : umdrv_staged(A, scr, s);
-    CAMR_HIP(hipFreeAsync(scr, s));
+    const int rc = do_mol ? umdrv_mol(A, scr, s) : umdrv_staged(A, scr, s);
     if (rc) { set_error("kernel launch failed"); return rc; }
     return CAMR_OK;
+}
+
+int camr_scratch_bytes(const camr_box *bx, const camr_hydro_params *p, size_t *bytes) {
+    if (!bx || !p || !bytes) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    const Box B = make_box(bx->lo, bx->hi);
+    if (!B.ok()) { set_error("empty box"); return CAMR_ERR_SHAPE; }
+    const size_t n = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
+    *bytes = (n + 1) * sizeof(double);
+    return CAMR_OK;
+}
+
+// camr_hydro_umdrv's scratch: a stream-ordered memory pool per device,
+// owned by the library and created on the device's first call.  Its release
+// threshold is unbounded, so memory freed at the end of a call stays in the
+// pool and later calls of the same or smaller size are served without
+// going to the driver; concurrent calls on distinct streams each get their
+// own block.
+static hipMemPool_t umdrv_pool(int dev) {
+    static std::mutex mu;
+    static std::vector<hipMemPool_t> pools;
+    std::lock_guard<std::mutex> g(mu);
+    if ((int)pools.size() <= dev) pools.resize(dev + 1, nullptr);
+    if (!pools[dev]) {
+        hipMemPoolProps props;
+        memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        if (hipMemPoolCreate(&pool, &props) != hipSuccess) return nullptr;
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        pools[dev] = pool;
+    }
+    return pools[dev];
+}
+
+int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
+                     const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
+                     const camr_array4 flx[3], const camr_array4 area[3], const camr_array4 *vol,
+                     double dt, const camr_hydro_params *p, void *stream) {
+    UmdrvArgs A;
+    size_t n = 0;
+    int rc = umdrv_args(bx, uin, dsdt, q, qaux, srcq, flx, area, vol, dt, p, &A, &n);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0;
+    CAMR_HIP(hipGetDevice(&dev));
+    hipMemPool_t pool = umdrv_pool(dev);
+    if (!pool) { set_error("scratch pool creation failed on device %d", dev); return CAMR_ERR_ALLOC; }
+    double *scr = nullptr;
+    CAMR_HIP(hipMallocFromPoolAsync((void **)&scr, (n + 1) * sizeof(double), pool, s));
+    rc = umdrv_run(A, scr, n, dt, p->do_mol, s);
+    CAMR_HIP(hipFreeAsync(scr, s));
+    return rc;
 }
 
 int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
@@ -2902,10 +2976,61 @@ int camr_level_eb_force(camr_level *L, double force[3]) {
 int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order) {
     if (!L || !block) { set_error("null argument"); return CAMR_ERR_PARAM; }
     static_assert(CAMR_NSTAGE == ST_COUNT, "stage kinds");
-    Tuning t;
+    Tuning t = default_tuning();
     for (int k = 0; k < ST_COUNT; ++k) t.block[k] = block[k];
     t.order = order;
     return L->impl.set_tuning(t);
+}
+int camr_level_set_tiles(camr_level *L, const int32_t tile[3 * CAMR_NSTAGE]) {
+    if (!L || !tile) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) {
+        const int32_t *x = tile + 3 * k;
+        if (x[0] == 0 && x[1] == 0 && x[2] == 0) {
+            t.tile[k][0] = -1;
+            t.tile[k][1] = t.tile[k][2] = 0;
+            continue;
+        }
+        int l[3];
+        for (int d = 0; d < 3; ++d) {
+            if (x[d] <= 0 || (x[d] & (x[d] - 1))) {
+                set_error("stage %d: tile sizes must be powers of two", k);
+                return CAMR_ERR_PARAM;
+            }
+            l[d] = __builtin_ctz((unsigned)x[d]);
+            t.tile[k][d] = l[d];
+        }
+        t.block[k] = x[0] * x[1] * x[2];
+    }
+    return L->impl.set_tuning(t);
+}
+int camr_level_set_xcd(camr_level *L, const int32_t chunk[CAMR_NSTAGE]) {
+    if (!L || !chunk) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) t.xcd[k] = chunk[k];
+    return L->impl.set_tuning(t);
+}
+int camr_level_get_xcd(camr_level *L, int32_t chunk[CAMR_NSTAGE]) {
+    if (!L || !chunk) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) chunk[k] = t.xcd[k];
+    return CAMR_OK;
+}
+int camr_level_get_tiles(camr_level *L, int32_t tile[3 * CAMR_NSTAGE]) {
+    if (!L || !tile) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k)
+        for (int d = 0; d < 3; ++d)
+            tile[3 * k + d] = t.tile[k][0] < 0 ? 0 : (1 << t.tile[k][d]);
+    return CAMR_OK;
 }
 int camr_level_get_tuning(camr_level *L, int32_t block[CAMR_NSTAGE], int32_t *order) {
     if (!L || !block || !order) { set_error("null argument"); return CAMR_ERR_PARAM; }

@@ -137,11 +137,38 @@ struct Batch {
     int nbox;
     int first;                    // block id of this launch's first block
     const int *halt;              // graph replays: non-zero = skip (or NULL)
+    // block shape: lx < 0 = the stage range in linear x-fastest order;
+    // else a tile of 2^lx x 2^ly x 2^lz points per block
+    int lx, ly, lz;
+    // XCD chunk: 0, or K such that every K consecutive logical blocks run
+    // on one XCD (see batch_box)
+    int xk;
 };
 
+// Number of blocks of one box's stage range sb under the block shape.
+__host__ __device__ inline int64_t batch_blocks(const Box &sb, int block, int lx, int ly,
+                                                int lz) {
+    if (lx < 0) return (sb.npts() + block - 1) / block;
+    return (int64_t)((sb.len(0) + (1 << lx) - 1) >> lx) * ((sb.len(1) + (1 << ly) - 1) >> ly) *
+           ((sb.len(2) + (1 << lz) - 1) >> lz);
+}
+
 // Box of this block and the block's index within the box's stage range.
+// The hardware hands workgroup w to XCD w % 8, so consecutive blocks (the
+// neighbouring rows a stencil re-reads) land in eight different L2 caches.
+// With B.xk = K the launch's blocks are permuted within groups of 8K: the
+// workgroups of one XCD take K consecutive logical blocks, while all XCDs
+// stay within the same window of 8K blocks (the same few planes).
 __device__ __forceinline__ int batch_box(const Batch &B, int &lblock) {
-    const int bid = (int)blockIdx.x + B.first;
+    int w = (int)blockIdx.x;
+    if (B.xk > 0) {
+        const int g = 8 * B.xk;
+        if (w < ((int)gridDim.x / g) * g) {
+            const int r = w % g;
+            w = (w - r) + (r & 7) * B.xk + (r >> 3);
+        }
+    }
+    const int bid = w + B.first;
     int lo = 0, hi = B.nbox - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -152,6 +179,27 @@ __device__ __forceinline__ int batch_box(const Batch &B, int &lblock) {
     return lo;
 }
 
+// The point (i,j,k) of stage range sb that this thread covers as the
+// lblk-th block of its box; false past the range's end.  Tiled blocks take
+// their tiles x fastest; threads fill a tile x fastest, so a wavefront
+// spans whole x rows of the tile.
+__device__ __forceinline__ bool batch_point(const Batch &B, const Box &sb, int lblk, int &i,
+                                            int &j, int &k) {
+    if (B.lx < 0)
+        return range_point(range_of(sb), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k);
+    const int ntx = (sb.len(0) + (1 << B.lx) - 1) >> B.lx;
+    const int nty = (sb.len(1) + (1 << B.ly) - 1) >> B.ly;
+    const int bz = lblk / (ntx * nty);
+    const int rem = lblk - bz * ntx * nty;
+    const int by = rem / ntx;
+    const int bx = rem - by * ntx;
+    const int t = (int)threadIdx.x;
+    i = sb.lo[0] + (bx << B.lx) + (t & ((1 << B.lx) - 1));
+    j = sb.lo[1] + (by << B.ly) + ((t >> B.lx) & ((1 << B.ly) - 1));
+    k = sb.lo[2] + (bz << B.lz) + (t >> (B.lx + B.ly));
+    return i <= sb.hi[0] && j <= sb.hi[1] && k <= sb.hi[2];
+}
+
 // First statements of every kernel: the block's box E (and its layout L)
 // and the point (i,j,k) of the stage range this thread covers.
 #define BATCH_POINT(KIND)                                                      \
@@ -160,9 +208,7 @@ __device__ __forceinline__ int batch_box(const Batch &B, int &lblock) {
     const BoxEnt &E = B.tab[batch_box(B, lblk_)];                              \
     const Canon L = E.L;                                                       \
     int i, j, k;                                                               \
-    if (!range_point(range_of(stage_box((KIND), E.bx)),                        \
-                     (int64_t)lblk_ * blockDim.x + threadIdx.x, i, j, k))      \
-        return;
+    if (!batch_point(B, stage_box((KIND), E.bx), lblk_, i, j, k)) return;
 
 // Slot S of box E (S a compile-time constant, so the address stays scalar).
 template <int S>
@@ -237,10 +283,15 @@ void launch_post_fused(const LaunchCfg &c, const PostArgs &a, hipStream_t st, in
 void launch_update_post_fused(const LaunchCfg &c, const HydroConst &hc, const double *dtp,
                               const PostArgs &a, hipStream_t st, int *status);
 
-// Per-stage launch configuration (block size per stage kind).
+// Per-stage launch configuration (block size and shape per stage kind).
 struct Tuning {
     int block[ST_COUNT];
     int order;  // 0: boxes in level order; 1: largest first
+    // tile of each stage's blocks as log2 sizes (x, y, z) whose product is
+    // the block size; tile[k][0] < 0: linear x-fastest blocks
+    int tile[ST_COUNT][3];
+    // XCD chunk of each stage (0: hardware order)
+    int xcd[ST_COUNT];
 };
 
 class FusedEngine {

@@ -463,7 +463,12 @@ k_update_c(Batch B, HydroConst hc, const double *dtp, int *flags) {
 // --------------------------------------------------------------- engine
 Tuning default_tuning() {
     Tuning t;
-    for (int k = 0; k < ST_COUNT; ++k) t.block[k] = kBlock;
+    for (int k = 0; k < ST_COUNT; ++k) {
+        t.block[k] = kBlock;
+        t.tile[k][0] = -1;
+        t.tile[k][1] = t.tile[k][2] = 0;
+        t.xcd[k] = 0;
+    }
     t.order = 0;
     return t;
 }
@@ -528,6 +533,20 @@ int FusedEngine::set_tuning(const Tuning &t) {
         set_error("box order must be 0 (level order) or 1 (largest first)");
         return CAMR_ERR_PARAM;
     }
+    for (int k = 0; k < ST_COUNT; ++k) {
+        if (t.xcd[k] < 0 || t.xcd[k] > 4096) {
+            set_error("stage %d: XCD chunk must be 0..4096", k);
+            return CAMR_ERR_PARAM;
+        }
+        const int *l = t.tile[k];
+        if (l[0] < 0) continue;
+        if (l[0] > 8 || l[1] < 0 || l[1] > 8 || l[2] < 0 || l[2] > 8 ||
+            (1 << (l[0] + l[1] + l[2])) != t.block[k] || l[0] < 3) {
+            set_error("stage %d: a tile is 2^lx x 2^ly x 2^lz points with lx >= 3 and "
+                      "2^(lx+ly+lz) equal to its block size", k);
+            return CAMR_ERR_PARAM;
+        }
+    }
     tune_ = t;
     return build_tables();
 }
@@ -562,8 +581,9 @@ int FusedEngine::build_tables() {
     for (int k = 0; k < ST_COUNT; ++k) {
         int *row = &blk_host_[(size_t)k * (n + 1)];
         for (int p = 0; p < n; ++p) {
-            const int64_t pts = stage_box(k, boxes_[order_[p]]).npts();
-            const int64_t nb = (pts + tune_.block[k] - 1) / tune_.block[k];
+            const int *l = tune_.tile[k];
+            const int64_t nb = batch_blocks(stage_box(k, boxes_[order_[p]]), tune_.block[k],
+                                            l[0], l[1], l[2]);
             if ((int64_t)row[p] + nb > 0x7fffffff) {
                 set_error("stage grid exceeds 2^31 blocks");
                 return CAMR_ERR_SHAPE;
@@ -609,6 +629,10 @@ LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {
     }
     c.block = tune_.block[kind];
     c.B.halt = halt_;
+    c.B.lx = tune_.tile[kind][0];
+    c.B.ly = tune_.tile[kind][1];
+    c.B.lz = tune_.tile[kind][2];
+    c.B.xk = tune_.xcd[kind];
     return c;
 }
 

@@ -140,7 +140,7 @@ k_post_fused(Batch B, PostArgs a, int *status) {
     int i, j, k;
     double dtv = 1.7976931348623157e308;
     // every thread reaches the block reduction below
-    if (range_point(range_of(E.bx), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k)) {
+    if (batch_point(B, E.bx, lblk, i, j, k)) {
         const Arr snew = E.snew;
         double v[NVAR];
 #pragma unroll
@@ -162,7 +162,7 @@ k_update_post(Batch B, HydroConst hc, const double *dtp, PostArgs a, int *status
     const BoxEnt &E = B.tab[batch_box(B, lblk)];
     int i, j, k;
     double dtv = 1.7976931348623157e308;
-    if (range_point(range_of(E.bx), (int64_t)lblk * blockDim.x + threadIdx.x, i, j, k)) {
+    if (batch_point(B, E.bx, lblk, i, j, k)) {
         const Arr snew = E.snew;
         double v[NVAR];
 #pragma unroll

@@ -238,6 +238,30 @@ class Level:
         arr = (C.c_int32 * L.NSTAGE)(*blk)
         L.check(L.lib().camr_level_set_tuning(self._h, arr, int(order)))
 
+    @property
+    def tiles(self):
+        """Block shape (tx, ty, tz) of each stage range; (0, 0, 0) = blocks of
+        consecutive points in x-fastest order."""
+        t = (C.c_int32 * (3 * L.NSTAGE))()
+        L.check(L.lib().camr_level_get_tiles(self._h, t))
+        return [tuple(t[3 * s:3 * s + 3]) for s in range(L.NSTAGE)]
+
+    @tiles.setter
+    def tiles(self, tiles):
+        flat = [int(x) for tt in tiles for x in tt]
+        L.check(L.lib().camr_level_set_tiles(self._h, (C.c_int32 * (3 * L.NSTAGE))(*flat)))
+
+    @property
+    def xcd(self):
+        """XCD chunk of each stage range (0: the hardware's block order)."""
+        t = (C.c_int32 * L.NSTAGE)()
+        L.check(L.lib().camr_level_get_xcd(self._h, t))
+        return list(t)
+
+    @xcd.setter
+    def xcd(self, chunks):
+        L.check(L.lib().camr_level_set_xcd(self._h, (C.c_int32 * L.NSTAGE)(*map(int, chunks))))
+
     def profile(self, enable=True, kernel=None):
         """Time launches (all, or only those named `kernel`) with hipEvents."""
         L.check(L.lib().camr_level_profile(
@@ -288,6 +312,13 @@ def hydro_srctoprim(qlo, qhi, q, qa, src, srcq, stream=0):
         C.byref(_box(qlo, qhi)), C.byref(array4_of(q, qlo)),
         C.byref(array4_of(qa, qlo)), C.byref(array4_of(src, qlo)),
         C.byref(array4_of(srcq, qlo)), _stream(stream)))
+
+
+def scratch_bytes(lo, hi, params):
+    """Bytes of scratch one hydro_umdrv call on box lo..hi takes."""
+    n = C.c_size_t()
+    L.check(L.lib().camr_scratch_bytes(C.byref(_box(lo, hi)), C.byref(params), C.byref(n)))
+    return n.value
 
 
 def hydro_umdrv(lo, hi, uin, uin_lo, dsdt, q, qa, srcq, flx, dt, params,

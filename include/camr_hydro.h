@@ -22,6 +22,7 @@
 #ifndef CAMR_HYDRO_H
 #define CAMR_HYDRO_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -120,13 +121,22 @@ int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q,
  * area[d].p / vol->p may be NULL for a uniform Cartesian grid (then
  * area = product of the two other dx, vol = dx*dy*dz as geom.GetFaceArea /
  * GetVolume give); srcq->p may be NULL for identically +0 sources.
- * Scratch memory comes from a per-device pool owned by the library. */
+ * Scratch memory (the reference's The_Async_Arena temporaries,
+ * Godunov_3D.cpp:79-98) comes from a stream-ordered memory pool per device
+ * owned by the library: freed in stream order at the end of the call and
+ * kept in the pool, so repeated calls do not go back to the driver.  Calls
+ * on distinct streams may overlap. */
 int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin,
                      const camr_array4 *dsdt, const camr_array4 *q,
                      const camr_array4 *qaux, const camr_array4 *srcq,
                      const camr_array4 flx[3], const camr_array4 area[3],
                      const camr_array4 *vol, double dt,
                      const camr_hydro_params *p, void *stream);
+
+/* Bytes of scratch one camr_hydro_umdrv call on box bx takes with params p
+ * (the Godunov or MOL temporaries of Godunov_3D.cpp:79-98 /
+ * MOL_umeth.cpp, plus the device word of dt). */
+int camr_scratch_bytes(const camr_box *bx, const camr_hydro_params *p, size_t *bytes);
 
 /* Cut-cell geometry of one box: the EBFArrayBoxFactory arrays that
  * hydro_umdrv_eb reads (Source/Hydro/Hydro_umdrv_eb.cpp:55-64) with the
@@ -352,6 +362,23 @@ int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac,
 #define CAMR_NSTAGE 13
 int camr_level_set_tuning(camr_level *L, const int32_t block[CAMR_NSTAGE], int32_t order);
 int camr_level_get_tuning(camr_level *L, int32_t block[CAMR_NSTAGE], int32_t *order);
+/* Block shapes of the stage ranges: tile[3*s .. 3*s+2] = (tx, ty, tz) points
+ * per block of stage range s, powers of two with tx >= 8 and tx*ty*tz in
+ * {64, 128, 256} (that product becomes the stage's block size), or (0, 0, 0)
+ * for blocks of consecutive points in x-fastest order.  A tile keeps the
+ * y and z stencil neighbours of a block's points inside the block.
+ * camr_level_set_tuning resets every stage to (0, 0, 0).  No reference
+ * counterpart: set by the autotuner. */
+int camr_level_set_tiles(camr_level *L, const int32_t tile[3 * CAMR_NSTAGE]);
+int camr_level_get_tiles(camr_level *L, int32_t tile[3 * CAMR_NSTAGE]);
+/* XCD chunk of each stage range: 0 keeps the hardware's round-robin of
+ * workgroups over the eight XCDs; K > 0 permutes each launch's blocks so
+ * that every XCD takes K consecutive blocks of each window of 8K, keeping a
+ * block's neighbouring rows in its own XCD's L2.  Results do not depend on
+ * it.  camr_level_set_tuning resets it to 0.  No reference counterpart: set
+ * by the autotuner. */
+int camr_level_set_xcd(camr_level *L, const int32_t chunk[CAMR_NSTAGE]);
+int camr_level_get_xcd(camr_level *L, int32_t chunk[CAMR_NSTAGE]);
 
 /* Adjoint (reverse mode) of level steps, for gradients of a loss on the
  * final state with respect to the initial state.  No reference counterpart

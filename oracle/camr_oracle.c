@@ -49,6 +49,14 @@ static inline double smin(double a, double b) { return (b < a) ? b : a; }
 #define ORA_PLANT 0
 #endif
 
+/* Stage markers of the operation-counting build (oracle/opcount.cpp), which
+ * attributes every FP64 operation to the stage running; no-ops otherwise. */
+#ifdef ORA_OPCOUNT
+#define ORA_STAGE(s) (ora_stage = (s))
+#else
+#define ORA_STAGE(s) ((void)0)
+#endif
+
 #define FA(f, i, j, k, n)                                                   \
     ((f)->p[(int64_t)((i) - (f)->lo[0]) + (int64_t)((j) - (f)->lo[1]) * (f)->js + \
             (int64_t)((k) - (f)->lo[2]) * (f)->ks + (int64_t)(n) * (f)->ns])
@@ -868,9 +876,11 @@ static void trace_ppm(obox bx, int idir, const ora_fab *q, const ora_fab *qa,
                          FV(q, iv, O_QRHO));
         double un = FV(q, iv, QUN);
         double flat = 1.0;
+        ORA_STAGE(8);
         if (use_flattening == 1)
             for (int dir_flat = 0; dir_flat < 3; dir_flat++)
                 flat = smin(flat, ora_flatten(q, i, j, k, dir_flat));
+        ORA_STAGE(1);
 
         double Ip[O_QVAR + 1][3];
         double Im[O_QVAR + 1][3];
@@ -1512,6 +1522,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
         flxbx[d] = bnodes(bgrowd(bxg2, d, -1), d);
     }
 
+    ORA_STAGE(1);
     if (P->ppm_type == 0) {
         const double del[3] = {dx, dy, dz};
         for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
@@ -1519,9 +1530,11 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
         for (int i = bxg2.lo[0]; i <= bxg2.hi[0]; ++i) {
             double slope[O_QVAR];
             double flat = 1.0;
+            ORA_STAGE(8);
             if (P->use_flattening == 1)
                 for (int dir_flat = 0; dir_flat < 3; dir_flat++)
                     flat = smin(flat, ora_flatten(q, i, j, k, dir_flat));
+            ORA_STAGE(1);
             for (int idir = 0; idir < 3; ++idir) {
                 for (int n = 0; n < O_QVAR; ++n) {
                     if (n == O_QPRES && P->use_pslope)
@@ -1542,6 +1555,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
     }
 
     /* first flux estimates */
+    ORA_STAGE(2);
     ora_fab f[3], qgd[3];
     for (int d = 0; d < 3; ++d) {
         fab_alloc_box(&f[d], flxbx[d], O_NVAR);
@@ -1561,6 +1575,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
 #else
     const double cdt[3] = {cdtdx, cdtdy, cdtdz};
 #endif
+    ORA_STAGE(3);
     ora_fab fl2[3][3], qg2[3][3]; /* fl2[d][t]: flux of direction d corrected by t */
     for (int d = 0; d < 3; ++d) {
         const int t0 = (d == 0) ? 1 : 0;
@@ -1599,6 +1614,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
     for (int d = 0; d < 3; ++d) { ora_fab_free(&f[d]); ora_fab_free(&qgd[d]); }
 
     /* final states and fluxes */
+    ORA_STAGE(5);
     ora_fab qmf, qpf;
     fab_alloc_box(&qmf, bxg2, O_QVAR);
     fab_alloc_box(&qpf, bxg1, O_QVAR);
@@ -1635,6 +1651,7 @@ static void godunov_umeth(obox bx, const ora_params *P, const ora_fab *q,
             if (t != d) { ora_fab_free(&fl2[d][t]); ora_fab_free(&qg2[d][t]); }
     for (int d = 0; d < 3; ++d) { ora_fab_free(&qdm[d]); ora_fab_free(&qdp[d]); }
 
+    ORA_STAGE(6);
     pdivu_box(bx, q1, q2, q3, area, pdivu, vol);
 }
 
@@ -1839,12 +1856,14 @@ void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
                       &qec[1], &qec[2], area, &pdivu, vol, dt);
     }
 
+    ORA_STAGE(4);
     for (int k = bxg2.lo[2]; k <= bxg2.hi[2]; ++k)
     for (int j = bxg2.lo[1]; j <= bxg2.hi[1]; ++j)
     for (int i = bxg2.lo[0]; i <= bxg2.hi[0]; ++i)
         divu_node(i, j, k, q, dx[0], dx[1], dx[2], &divu, P->domlo, P->domhi,
                   P->bclo, P->bchi);
 
+    ORA_STAGE(5);
     for (int d = 0; d < 3; ++d) {
         const obox fb = bnodes(bx, d);
         for (int k = fb.lo[2]; k <= fb.hi[2]; ++k)
@@ -1858,6 +1877,7 @@ void ora_hydro_umdrv(const int lo[3], const int hi[3], const ora_fab *uin,
         }
     }
 
+    ORA_STAGE(6);
     for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
     for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
     for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
@@ -2280,6 +2300,7 @@ void ora_level_init_post(ora_level *L) {
 /* estTimeStep, CAMR.cpp:490-563 */
 static double est_time_step(ora_level *L) {
     const ora_params *P = &L->P;
+    ORA_STAGE(7);
     if (P->fixed_dt > 0.0) return P->fixed_dt;
     double estdt = P->max_dt;
     const double max_dt_over_cfl = P->max_dt / P->cfl;
@@ -2376,6 +2397,7 @@ static void construct_hydro_source(ora_level *L, double dt) {
         fab_alloc_box(&q, qbx, O_QVAR);
         fab_alloc_box(&qaux, qbx, O_NQAUX);
         fab_alloc_box(&srcq, qbx, O_QVAR);
+        ORA_STAGE(0);
         ora_ctoprim_box(qbx.lo, qbx.hi, &L->Sborder[b], &q, &qaux, P, &err);
         if (!P->do_mol) /* CAMR_construct_hydro_source.cpp:129-143 */
             ora_srctoprim_box(qbx.lo, qbx.hi, &q, &qaux, &L->srchyd[b], &srcq);
@@ -2421,6 +2443,7 @@ static void set_val(ora_fab *f, int nbox, double v) {
 int ora_level_advance(ora_level *L, double dt) {
     const ora_params *P = &L->P;
     L->err = 0;
+    ORA_STAGE(0);
     /* swapTimeLevels: old <- new */
     for (int b = 0; b < L->nbox; ++b) {
         ora_fab t = L->Sold[b]; L->Sold[b] = L->Snew[b]; L->Snew[b] = t;
@@ -2477,6 +2500,7 @@ int ora_level_advance(ora_level *L, double dt) {
         }
         saxpy_valid(L, L->Snew, 0.5 * dt, L->hsrc);
     }
+    ORA_STAGE(7);
     clean_state(L, L->Snew, 0);
     if (nsrc) {
         /* construct_new_grav_source, GravitySrc.cpp:20-33,81-125 */

@@ -17,7 +17,7 @@ def declared_functions():
             continue
         text = open(os.path.join(inc, f)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"\b(?:int|void|double|const char\s*\*)\s*\*?\s*(camr_\w+)\s*\(", text):
+        for m in re.finditer(r"\b(?:int|void|double|size_t|const char\s*\*)\s*\*?\s*(camr_\w+)\s*\(", text):
             names.add(m.group(1))
     return names
 
@@ -64,3 +64,22 @@ def test_invalid_parameters_are_rejected_without_gpu_work():
         assert e.code == _lib.ERR_PARAM
     else:
         raise AssertionError("cfl=1.5 accepted")
+
+
+def test_scratch_query_grows_with_the_box_and_refuses_empty_boxes():
+    # host-side size query of the per-box drop-in's temporaries
+    from camr_amd.hydro import Problem, hydro_params, scratch_bytes
+    sizes = []
+    for n in (8, 16, 32):
+        lo, hi = (0, 0, 0), (n - 1,) * 3
+        prob = Problem(domlo=lo, domhi=hi, dx=(1 / 64,) * 3, bclo=(0,) * 3, bchi=(0,) * 3,
+                       is_periodic=(0, 0, 0), boxes=[lo + hi], params=dict(ppm_type=1))
+        sizes.append(scratch_bytes(lo, hi, hydro_params(prob)))
+    assert sizes[0] < sizes[1] < sizes[2] < 96 << 20
+    mol = Problem(domlo=(0, 0, 0), domhi=(31,) * 3, dx=(1 / 64,) * 3, bclo=(0,) * 3,
+                  bchi=(0,) * 3, is_periodic=(0, 0, 0), boxes=[(0, 0, 0, 31, 31, 31)],
+                  params=dict(do_mol=1, cfl=0.3))
+    assert scratch_bytes((0, 0, 0), (31, 31, 31), hydro_params(mol)) < sizes[2] // 10
+    import pytest
+    with pytest.raises(_lib.CamrError):
+        scratch_bytes((0, 0, 0), (-1, 0, 0), hydro_params(mol))

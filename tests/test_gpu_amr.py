@@ -16,6 +16,10 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 FINE4 = [(16, 16, 16, 31, 47, 47), (32, 16, 16, 47, 47, 47)]
+# fine boxes on the domain faces (x low and z high): the ghost shell wraps
+# into periodic images or leaves a walled domain, and the registers border
+# the faces
+FACE = {2: [(0, 8, 8, 15, 23, 31)], 4: [(0, 16, 16, 31, 47, 63)]}
 
 
 def _pair(O, n, boxes, periodic, kw, graph=False, engine=_lib.ENGINE_STAGED):
@@ -51,15 +55,28 @@ FUSED = _lib.ENGINE_FUSED
     (2, False, dict(ppm_type=0), False, FUSED),
     (4, True, dict(ppm_type=1), False, FUSED),
     (4, False, dict(ppm_type=1), False, FUSED),
+    (2, True, dict(ppm_type=1, face=1), False, _lib.ENGINE_STAGED),
+    (2, False, dict(ppm_type=0, face=1), False, _lib.ENGINE_STAGED),
+    (4, True, dict(ppm_type=1, face=1), False, _lib.ENGINE_STAGED),
+    (4, False, dict(ppm_type=1, face=1), False, _lib.ENGINE_STAGED),
+    (2, True, dict(ppm_type=1, face=1), False, FUSED),
+    (2, False, dict(ppm_type=0, face=1), False, FUSED),
+    (4, True, dict(ppm_type=1, face=1), False, FUSED),
+    (4, False, dict(ppm_type=1, face=1), False, FUSED),
 ])
 def test_amr_step_matches_oracle(oracle, ratio, periodic, kw, graph, engine):
     """graph: every level step captured (per buffer parity and deferred
     clean) and replayed, the registers' adds inside the graphs.  FUSED: the
-    fused engine's canonical flux arrays feed the registers."""
+    fused engine's canonical flux arrays feed the registers.  face: the fine
+    boxes touch domain faces."""
     print(f"two-level step ratio={ratio} periodic={periodic} {kw} graph={graph} "
           f"engine={engine}", flush=True)
     nc = 16
-    fine = FINE2 if ratio == 2 else FINE4
+    kw = dict(kw)
+    if kw.pop("face", 0):
+        fine = FACE[ratio]
+    else:
+        fine = FINE2 if ratio == 2 else FINE4
     oc, gc = _pair(oracle, nc, _boxes(nc, 8), periodic, kw, graph, engine)
     of, gf = _pair(oracle, nc * ratio, fine, periodic, kw, graph, engine)
     oR = oracle.FluxRegister(oc, of, ratio)

@@ -96,6 +96,22 @@ def test_eb_level_graph_replays_match_oracle(oracle):
             assert np.array_equal(gl.get_box(b), ol.get_box(b)), (s, b)
 
 
+def test_eb_levels_are_refused_by_coarse_fine_transfers():
+    """average_down and the coarse-fine ghost interpolation of EB builds
+    (EB_average_down, eb_cell_cons_interp) are not built: the plain ones
+    must refuse EB levels instead of averaging covered cells in."""
+    crse = workloads.sod_sphere(n=16, max_grid=16)
+    fine = workloads.sod_sphere(n=32, max_grid=16)
+    C = Level(crse, engine=_lib.ENGINE_STAGED)
+    C.set_eb(workloads.sod_sphere_geometry(crse))
+    F = Level(fine, engine=_lib.ENGINE_STAGED)
+    F.set_eb(workloads.sod_sphere_geometry(fine))
+    with pytest.raises(_lib.CamrError):
+        C.average_down(F, 2)
+    with pytest.raises(_lib.CamrError):
+        F.exterior_from_coarse(C, 2)
+
+
 def test_eb_level_rejects_fused_engine_and_ppm():
     prob = workloads.sod_sphere(n=16, max_grid=16)
     geom = workloads.sod_sphere_geometry(prob)

@@ -47,7 +47,11 @@ def _run_both(O, prob, init, nsteps, engine=_lib.ENGINE_AUTO, batched=True, ext=
     ol = O.Level(_oracle_params(O, prob), prob.boxes)
     gl = Level(prob, engine=engine, batched=batched, graph=graph)
     if tuning is not None:
-        gl.tuning = tuning
+        if len(tuning) > 2:
+            from camr_amd import autotune
+            autotune.apply(gl, *tuning)
+        else:
+            gl.tuning = tuning
     if ext is not None:
         ol.set_exterior(ext)
         gl.set_exterior(ext)
@@ -89,25 +93,40 @@ CASES = [
     dict(ppm=0, order=4, flat=1, bclo=(0, 0, 0), bchi=(0, 0, 0)),
     dict(ppm=0, order=2, flat=0, bclo=(4, 2, 5), bchi=(3, 4, 2)),
     dict(ppm=1, order=4, flat=1, bclo=(5, 3, 2), bchi=(2, 5, 4)),
+    # plm_pslope (PLM.H:84-165) with a non-zero primitive source
+    dict(ppm=0, order=4, flat=1, pslope=1, src=True, bclo=(0, 0, 0), bchi=(0, 0, 0)),
+    dict(ppm=0, order=2, flat=1, pslope=1, src=True, trd=0, bclo=(4, 2, 5), bchi=(3, 4, 2)),
+    # transverse_reset_density = 0, sources through transdd's hdt * srcQ
+    dict(ppm=1, order=4, flat=1, src=True, trd=0, bclo=(0, 0, 0), bchi=(0, 0, 0)),
+    # a density band whose thin x-face states empty under the y flux
+    # difference: the transdo density reset (Godunov_utils_3D.H:148-155)
+    # fires
+    dict(ppm=0, order=4, flat=1, pocket=True, trd=1, bclo=(0, 0, 0), bchi=(0, 0, 0)),
+    dict(ppm=1, order=4, flat=1, pocket=True, trd=1, bclo=(0, 0, 0), bchi=(0, 0, 0)),
 ]
 
 
-@pytest.mark.parametrize("case", CASES)
-def test_umdrv_per_box_bitwise(oracle, case):
-    lo, hi = (3, 5, 7), (3 + 15, 5 + 11, 7 + 13)
-    G = 4
-    glo = tuple(l - G for l in lo)
-    ghi = tuple(h + G for h in hi)
-    U = _box_state(glo, ghi, seed=11)
-    dx = (1 / 64,) * 3
-    prob = Problem(domlo=lo, domhi=hi, dx=dx, bclo=case["bclo"], bchi=case["bchi"],
-                   is_periodic=(0, 0, 0), boxes=[lo + hi],
-                   params=dict(ppm_type=case["ppm"], plm_iorder=case["order"],
-                               use_flattening=case["flat"]))
-    hp = hydro_params(prob)
-    op = _oracle_params(oracle, prob)
-    dt = 0.3 * dx[0]
-    # oracle
+def _thinning_band(U):
+    """Density falling tenfold per cell along x (to 1e-3), uniform pressure,
+    in a flow that diverges along y: the limited slopes put a tenth of a
+    cell's density on its thin x-face, and the y mass-flux difference of
+    the cell exceeds that over one transverse correction."""
+    nz, ny, nx = U.shape[1:]
+    i, j = np.arange(nx), np.arange(ny)
+    rho = np.clip(10.0 ** (-(i - 10.0)), 1e-3, 1.0)[None, None, :] * np.ones((nz, ny, 1))
+    v = np.tanh((j - ny / 2 + 0.5) / 1.5)[None, :, None] * np.ones((nz, 1, nx))
+    u, w = 0.2 * U[1] / U[0], 0.2 * U[3] / U[0]
+    V = np.zeros_like(U)
+    V[0], V[1], V[2], V[3] = rho, rho * u, rho * v, rho * w
+    V[5] = 1.0 / 0.4
+    V[7] = rho
+    V[4] = V[5] + 0.5 * (V[1] ** 2 + V[2] ** 2 + V[3] ** 2) / V[0]
+    V[6] = U[6]
+    return np.ascontiguousarray(V)
+
+
+def _oracle_umdrv(oracle, U, glo, ghi, lo, hi, op, dt, src):
+    """ctoprim (+ srctoprim) and hydro_umdrv of the oracle on one box."""
     q = np.zeros((11,) + U.shape[1:])
     qa = np.zeros((5,) + U.shape[1:])
     srcq = np.zeros_like(q)
@@ -117,6 +136,11 @@ def test_umdrv_per_box_bitwise(oracle, case):
                                  C.byref(oracle.fab(q, glo)), C.byref(oracle.fab(qa, glo)),
                                  C.byref(op), C.byref(err))
     assert err.value == 0
+    if src is not None:
+        oracle.lib().ora_srctoprim_box(L3(*glo), L3(*ghi), C.byref(oracle.fab(q, glo)),
+                                       C.byref(oracle.fab(qa, glo)),
+                                       C.byref(oracle.fab(src, glo)),
+                                       C.byref(oracle.fab(srcq, glo)))
     n = [hi[d] - lo[d] + 1 for d in range(3)]
     dsdt = np.zeros((10, n[2], n[1], n[0]))
     flx = []
@@ -129,8 +153,45 @@ def test_umdrv_per_box_bitwise(oracle, case):
                                  C.byref(oracle.fab(dsdt, lo)), C.byref(oracle.fab(q, glo)),
                                  C.byref(oracle.fab(qa, glo)), C.byref(oracle.fab(srcq, glo)),
                                  fl, dt, C.byref(op))
+    return q, qa, srcq, dsdt, flx
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_umdrv_per_box_bitwise(oracle, case):
+    lo, hi = (3, 5, 7), (3 + 15, 5 + 11, 7 + 13)
+    G = 4
+    glo = tuple(l - G for l in lo)
+    ghi = tuple(h + G for h in hi)
+    U = _box_state(glo, ghi, seed=11)
+    if case.get("pocket"):
+        U = _thinning_band(U)
+    src = None
+    if case.get("src"):
+        # a gravity-like conserved source (rho g along z, its work) plus a
+        # smooth internal-energy source
+        g = -0.7
+        src = np.zeros_like(U)
+        src[3] = U[0] * g
+        src[4] = U[3] * g
+        src[5] = 0.05 * np.sin(np.arange(U.shape[3]) / 3.0)[None, None, :] * U[0]
+    dx = (1 / 64,) * 3
+    prob = Problem(domlo=lo, domhi=hi, dx=dx, bclo=case["bclo"], bchi=case["bchi"],
+                   is_periodic=(0, 0, 0), boxes=[lo + hi],
+                   params=dict(ppm_type=case["ppm"], plm_iorder=case["order"],
+                               use_flattening=case["flat"], use_pslope=case.get("pslope", 0),
+                               transverse_reset_density=case.get("trd", 1)))
+    hp = hydro_params(prob)
+    op = _oracle_params(oracle, prob)
+    dt = 0.3 * dx[0]
+    q, qa, srcq, dsdt, flx = _oracle_umdrv(oracle, U, glo, ghi, lo, hi, op, dt, src)
+    if case.get("pocket"):
+        # the reset changes the result, so it fires on this state
+        op0 = _oracle_params(oracle, prob)
+        op0.transverse_reset_density = 0
+        assert not np.array_equal(_oracle_umdrv(oracle, U, glo, ghi, lo, hi, op0, dt, src)[3],
+                                  dsdt)
     # gpu
-    from camr_amd.hydro import hydro_ctoprim, hydro_umdrv
+    from camr_amd.hydro import hydro_ctoprim, hydro_srctoprim, hydro_umdrv
     dev = torch.device("cuda:0")
     tU = torch.from_numpy(U).to(dev)
     tq = torch.zeros((11,) + U.shape[1:], dtype=torch.float64, device=dev)
@@ -139,9 +200,15 @@ def test_umdrv_per_box_bitwise(oracle, case):
     torch.cuda.synchronize()
     assert_bitwise(tq.cpu().numpy(), q, "ctoprim q")
     assert_bitwise(tqa.cpu().numpy(), qa, "ctoprim qaux")
+    tsq = None
+    if src is not None:
+        tsq = torch.zeros_like(tq)
+        hydro_srctoprim(glo, ghi, tq, tqa, torch.from_numpy(src).to(dev), tsq)
+        torch.cuda.synchronize()
+        assert_bitwise(tsq.cpu().numpy(), srcq, "srctoprim")
     tds = torch.zeros(dsdt.shape, dtype=torch.float64, device=dev)
     tfl = [torch.full(f.shape, float("nan"), dtype=torch.float64, device=dev) for f in flx]
-    hydro_umdrv(lo, hi, tU, glo, tds, tq, tqa, None, tfl, dt, hp)
+    hydro_umdrv(lo, hi, tU, glo, tds, tq, tqa, tsq, tfl, dt, hp)
     torch.cuda.synchronize()
     for d in range(3):
         assert_bitwise(tfl[d].cpu().numpy(), flx[d], f"flux {d}")
@@ -189,21 +256,24 @@ def test_multibox_level_bitwise(oracle, batched, graph):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"box {b}")
 
 
-@pytest.mark.parametrize("grav", [0.0, -0.7])
-def test_walls_and_sources_bitwise(oracle, grav):
+@pytest.mark.parametrize("engine", [_lib.ENGINE_AUTO, _lib.ENGINE_STAGED])
+@pytest.mark.parametrize("grav,pslope,trd", [(0.0, 0, 1), (-0.7, 0, 1), (-0.7, 1, 1),
+                                             (-0.7, 1, 0)])
+def test_walls_and_sources_bitwise(oracle, engine, grav, pslope, trd):
     n = (24, 20, 16)
     dx = (1 / 24,) * 3
     prob = Problem(domlo=(0, 0, 0), domhi=tuple(x - 1 for x in n), dx=dx,
                    bclo=(2, 4, 5), bchi=(3, 5, 4), is_periodic=(0, 0, 0),
                    boxes=[(0, 0, 0, 11, 19, 15), (12, 0, 0, 23, 19, 15)],
-                   params=dict(ppm_type=0, cfl=0.4, const_grav=grav))
+                   params=dict(ppm_type=0, cfl=0.4, const_grav=grav, use_pslope=pslope,
+                               transverse_reset_density=trd))
     full = problems.SmoothField(seed=3, kmax=3, mach=0.4).state(48, (0, 0, 0), (23, 19, 15))
 
     def init(p, b):
         lo, hi = p.boxes[b][:3], p.boxes[b][3:]
         return np.ascontiguousarray(full[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
 
-    ol, gl, odts, gdts = _run_both(oracle, prob, init, 4)
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 4, engine)
     assert np.array_equal(odts, gdts)
     for b in range(2):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"walls box {b}")
@@ -277,6 +347,54 @@ def test_refined_level_with_exterior_bitwise(oracle, batched, tuning, graph):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"refined box {b}")
 
 
+TILE_SETS = {
+    "16x4x4": [(16, 4, 4)] * 13,
+    "mixed": [(64, 4, 1), (32, 4, 2), (16, 8, 2), (8, 8, 4), (32, 2, 4), (16, 4, 4),
+              (0, 0, 0), (32, 8, 1), (64, 2, 2), (16, 4, 2), (32, 2, 2), (8, 8, 2), (32, 4, 1)],
+}
+
+
+@pytest.mark.parametrize("tiles", sorted(TILE_SETS))
+@pytest.mark.parametrize("batched,graph", [(True, False), (False, False), (True, True)])
+def test_tiled_blocks_bitwise(oracle, tiles, batched, graph):
+    """3-D tiled blocks (the autotuner's candidates, the last partial tiles
+    of uneven boxes included) change which thread computes a point, not
+    its value: a refined level of uneven boxes, bit for bit."""
+    prob = workloads.refined_level(n=64, radius=0.3, blocking_factor=8, max_size=32,
+                                   efficiency=0.8)
+    ext = workloads.exterior(prob)
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gl = Level(prob, batched=batched, graph=graph)
+    gl.tiles = TILE_SETS[tiles]
+    assert gl.tiles == TILE_SETS[tiles]
+    # block permutations over the XCDs, chunks of 1 to 64 blocks
+    gl.xcd = [0, 1, 2, 3, 4, 8, 16, 32, 64, 5, 7, 0, 12]
+    for L in (ol, gl):
+        L.set_exterior(ext)
+        for b in range(len(prob.boxes)):
+            L.set_box(b, init(prob, b))
+        L.init_post()
+    err, odts = ol.run(3)
+    assert err == 0
+    gdts = gl.run(3)
+    assert np.array_equal(odts, gdts)
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"tiled box {b}")
+
+
+def test_tile_shapes_are_validated():
+    prob = workloads.smooth_periodic(n=16, ppm_type=1)
+    gl = Level(prob)
+    for bad in ([(12, 4, 4)] * 13, [(4, 8, 8)] * 13, [(64, 4, 4)] * 13):
+        with pytest.raises(_lib.CamrError):
+            gl.tiles = bad
+
+
 def test_unowned_ghosts_without_exterior_are_an_error():
     prob = workloads.refined_level(n=32, radius=0.3, blocking_factor=8, max_size=16)
     gl = Level(prob)
@@ -332,13 +450,15 @@ def test_autotuned_level_stays_bitwise(oracle, tmp_path, monkeypatch):
     for b in range(len(prob.boxes)):
         T.set_box(b, init(prob, b))
     T.init_post()
-    blocks, order = autotune.tune(T, steps=1)
+    blocks, order, tiles, xcd = autotune.tune(T, steps=1)
     assert len(blocks) == _lib.NSTAGE and all(x in (64, 128, 256) for x in blocks)
+    assert all(t == (0, 0, 0) or t[0] * t[1] * t[2] == b for b, t in zip(blocks, tiles))
+    assert len(xcd) == _lib.NSTAGE
     assert (tmp_path / "tuning.json").exists()
     # a second level with the same boxes reads the cache
-    assert autotune.tune(Level(prob), steps=1) == (blocks, order)
-    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, ext=ext, tuning=(blocks, order),
-                                   graph=True)
+    assert autotune.tune(Level(prob), steps=1) == (blocks, order, tiles, xcd)
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, ext=ext,
+                                   tuning=(blocks, order, tiles, xcd), graph=True)
     assert np.array_equal(odts, gdts)
     for b in range(len(prob.boxes)):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"tuned box {b}")
