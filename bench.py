@@ -542,21 +542,25 @@ def bench_amr_sync(args) -> int:
     return 0
 
 
-def adjoint_measure(n: int, nsteps: int, every: int, reps: int, ndirs: int) -> dict:
+def adjoint_measure(n: int, nsteps: int, every: int, reps: int, ndirs: int,
+                    field: str = "varp") -> dict:
     """Adjoint of `nsteps` PPM level steps at n^3 (checkpoints every
     `every`): wall time of the backward pass (recomputation from the
     checkpoints included) over that of the forward steps, best of `reps`;
     with `ndirs` > 0 the gradient is checked against central differences
-    along seeded unit directions."""
+    along seeded unit directions.  field "varp": the seeded smooth field
+    with a smooth pressure variation; "c1": the same field at p = 1."""
     import numpy as np
     import torch
     from camr_amd import problems, workloads
     from camr_amd.hydro import Level
     prob = workloads.smooth_periodic(n=n, ppm_type=1, cfl=0.4)
 
+    init = workloads.smooth_init_varp if field == "varp" else workloads.smooth_init
+
     def level_from(seed):
         L = Level(prob)
-        L.set_box(0, workloads.smooth_init_varp(prob, 0, seed))
+        L.set_box(0, init(prob, 0, seed))
         L.init_post()
         return L
 
@@ -647,15 +651,18 @@ def bench_adjoint(args) -> int:
     central differences along seeded unit directions."""
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    r = adjoint_measure(args.adj_n, args.steps, 5, args.adj_reps, args.adj_dirs)
+    r = adjoint_measure(args.adj_n, args.steps, 5, args.adj_reps, args.adj_dirs,
+                        args.adj_field)
     out = {"metric": "adjoint/forward cost ratio", "value": r["ratio"], "unit": "x",
            "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": 1,
            "dtype": "f64", "data": "synthetic", "vs_baseline": None,
            "config": {"workload": f"adjoint of {args.steps} periodic PPM level steps at "
                                   f"{args.adj_n}^3, checkpoints every 5 steps, frozen dt "
                                   "sequence, L = 1/2 sum (rho(T) - rho_target)^2 over all "
-                                  "initial conserved components; smooth field with a "
-                                  "smooth pressure variation"}}
+                                  "initial conserved components; "
+                                  + ("smooth field with a smooth pressure variation"
+                                     if args.adj_field == "varp" else
+                                     "the C1 field (p = 1)")}}
     out.update({k: v for k, v in r.items() if k != "ratio"})
     print(json.dumps(out), flush=True)
     return 0
@@ -690,6 +697,8 @@ def main() -> int:
     ap.add_argument("--adj-dirs", type=int, default=16,
                     help="finite-difference directions of the adjoint check")
     ap.add_argument("--adj-reps", type=int, default=3)
+    ap.add_argument("--adj-field", choices=["varp", "c1"], default="varp",
+                    help="adjoint workload field: smooth pressure variation, or p = 1")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of the autotuner's")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",

@@ -3022,6 +3022,22 @@ int camr_level_get_xcd(camr_level *L, int32_t chunk[CAMR_NSTAGE]) {
     for (int k = 0; k < ST_COUNT; ++k) chunk[k] = t.xcd[k];
     return CAMR_OK;
 }
+int camr_level_set_zmarch(camr_level *L, const int32_t planes[CAMR_NSTAGE]) {
+    if (!L || !planes) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) t.zm[k] = planes[k];
+    return L->impl.set_tuning(t);
+}
+int camr_level_get_zmarch(camr_level *L, int32_t planes[CAMR_NSTAGE]) {
+    if (!L || !planes) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    for (int k = 0; k < ST_COUNT; ++k) planes[k] = t.zm[k];
+    return CAMR_OK;
+}
 int camr_level_get_tiles(camr_level *L, int32_t tile[3 * CAMR_NSTAGE]) {
     if (!L || !tile) { set_error("null argument"); return CAMR_ERR_PARAM; }
     Tuning t;

@@ -723,7 +723,7 @@ __global__ void k_zero(double *p, size_t n) {
 template <typename K>
 void go(const FusedEngine &eng, const char *name, Prof *prof, K kern, int kind, int parity,
         hipStream_t st, const AdjArgs &a) {
-    const LaunchCfg c = eng.cfg(kind, -1, parity);
+    const LaunchCfg c = eng.cfg(kind, -1, parity, /*linear=*/true);
     if (c.grid == 0) return;
     Region reg(prof, name, st);
     hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.block), 0, st, c.B, a);

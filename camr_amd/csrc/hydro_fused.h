@@ -143,11 +143,17 @@ struct Batch {
     // XCD chunk: 0, or K such that every K consecutive logical blocks run
     // on one XCD (see batch_box)
     int xk;
+    // z-march: 0, or the number of consecutive z planes each thread walks
+    // (the column kernels of batch_column)
+    int zm;
 };
 
 // Number of blocks of one box's stage range sb under the block shape.
 __host__ __device__ inline int64_t batch_blocks(const Box &sb, int block, int lx, int ly,
-                                                int lz) {
+                                                int lz, int zm = 0) {
+    if (zm > 0)
+        return (((int64_t)sb.len(0) * sb.len(1) + block - 1) / block) *
+               ((sb.len(2) + zm - 1) / zm);
     if (lx < 0) return (sb.npts() + block - 1) / block;
     return (int64_t)((sb.len(0) + (1 << lx) - 1) >> lx) * ((sb.len(1) + (1 << ly) - 1) >> ly) *
            ((sb.len(2) + (1 << lz) - 1) >> lz);
@@ -199,6 +205,34 @@ __device__ __forceinline__ bool batch_point(const Batch &B, const Box &sb, int l
     k = sb.lo[2] + (bz << B.lz) + (t >> (B.lx + B.ly));
     return i <= sb.hi[0] && j <= sb.hi[1] && k <= sb.hi[2];
 }
+
+// Column of a z-marching launch: the thread's (i, j) of the stage range's
+// x-y plane and its run of planes k0..k1 (at most B.zm of them); false past
+// the range's end.  A block's threads take consecutive points of the plane,
+// x fastest, so each plane's loads stay row-coalesced.
+__device__ __forceinline__ bool batch_column(const Batch &B, const Box &sb, int lblk, int &i,
+                                             int &j, int &k0, int &k1) {
+    const int n0 = sb.len(0);
+    const int64_t np = (int64_t)n0 * sb.len(1);
+    const int nbp = (int)((np + blockDim.x - 1) / blockDim.x);
+    const int zc = lblk / nbp;
+    const int64_t t = (int64_t)(lblk - zc * nbp) * blockDim.x + threadIdx.x;
+    if (t >= np) return false;
+    const int jj = (int)(t / n0);
+    i = sb.lo[0] + (int)(t - (int64_t)jj * n0);
+    j = sb.lo[1] + jj;
+    k0 = sb.lo[2] + zc * B.zm;
+    k1 = k0 + B.zm - 1 < sb.hi[2] ? k0 + B.zm - 1 : sb.hi[2];
+    return true;
+}
+
+#define BATCH_COLUMN(KIND)                                                     \
+    if (B.halt && *B.halt) return;                                             \
+    int lblk_;                                                                 \
+    const BoxEnt &E = B.tab[batch_box(B, lblk_)];                              \
+    const Canon L = E.L;                                                       \
+    int i, j, k0, k1;                                                          \
+    if (!batch_column(B, stage_box((KIND), E.bx), lblk_, i, j, k0, k1)) return;
 
 // First statements of every kernel: the block's box E (and its layout L)
 // and the point (i,j,k) of the stage range this thread covers.
@@ -292,6 +326,9 @@ struct Tuning {
     int tile[ST_COUNT][3];
     // XCD chunk of each stage (0: hardware order)
     int xcd[ST_COUNT];
+    // z-march planes per thread (0: one point per thread); only the z-face
+    // transverse stages (ST_T1 + 2, ST_T2 + 2) have marching kernels
+    int zm[ST_COUNT];
 };
 
 class FusedEngine {
@@ -333,7 +370,9 @@ class FusedEngine {
     double *adjoint_base() const { return ascratch_.base(); }
     size_t scratch_total() const { return total_; }
 
-    LaunchCfg cfg(int kind, int b, int parity) const;
+    // linear = true: the stage's blocks without z-marching (kernels that
+    // take one point per thread over a stage that marches when tuned)
+    LaunchCfg cfg(int kind, int b, int parity, bool linear = false) const;
     // launches made while set read this word and skip their work when it is
     // non-zero (the halt flag of captured step graphs)
     void set_halt(const int *h) { halt_ = h; }
@@ -353,7 +392,7 @@ class FusedEngine {
     std::vector<Arr> bufA_, bufB_;
     std::vector<size_t> off_;      // scratch offset of each box (doubles)
     std::vector<int> order_;       // box order of the tables
-    std::vector<int> blk_host_;    // ST_COUNT x (nbox + 1)
+    std::vector<int> blk_host_;    // 2 x ST_COUNT x (nbox + 1): as tuned, without z-march
     std::vector<int> pos_;         // table position of box b
     Tuning tune_{};
     BoxEnt *tab_[2] = {nullptr, nullptr};

@@ -430,6 +430,200 @@ k_trans2(Batch B, HydroConst hc, const double *dtp) {
     flx.st(v0, XGP, g[GP]);
 }
 
+// ------------------------------------------------ z-marching z-face kernels
+// The z-face transverse stages with each thread walking a column of faces
+// k0..k1: the minus side of face k belongs to cell k-1, the plus side of
+// face k-1, so its transverse differences, sound speeds and the values the
+// flux adjustment reads come from the previous iteration's registers
+// instead of a second load of the plane below.  Same per-point arithmetic
+// as k_trans1<2> / k_trans2<2, S>, hence the same results.
+
+// adjust_face<2> (hydro_artif_visc, hydro_norm_spec_flx, hydro_ext_flx) on
+// values: d4 the face's four nodal divergences in the order (i,j), (i+1,j),
+// (i,j+1), (i+1,j+1); uc/um the conserved state of the cells above and
+// below the face.
+__device__ __forceinline__ void adjust_zface_vals(double *f, const double *d4,
+                                                  const double *uc, const double *um, int k,
+                                                  double area, const HydroConst &hc) {
+    const double div = hc.difmag * vmin(0.0, 0.25 * (d4[0] + d4[1] + d4[2] + d4[3]));
+    const bool at_bndry = (k == hc.domlo[2] && hc.bclo[2] == BC_NOSLIPWALL) ||
+                          (k == hc.domhi[2] + 1 && hc.bchi[2] == BC_NOSLIPWALL);
+    const double dxd = hc.dx[2];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        if (n == UTEMP) continue;
+        const bool tang = (n == UMX || n == UMY);
+        if (!(at_bndry && tang)) f[n] += dxd * div * (uc[n] - um[n]);
+    }
+    f[UTEMP] = 0.0;
+    double summ = 0.0;
+    summ += f[UFS];
+    summ += f[UFS + 1];
+    summ += f[UFS + 2];
+    const double fac = (summ != 0.0) ? f[URHO] / summ : 1.0;
+    f[UFS] *= fac;
+    f[UFS + 1] *= fac;
+    f[UFS + 2] *= fac;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) f[n] *= area;
+}
+
+// transdo differences of the cell at offset v in direction T (faces v and
+// v + e_T of F)
+template <int T>
+__device__ __forceinline__ void trans1_diff(const CView &F, unsigned v, const Canon &L,
+                                            double cdt, TransDiff &t) {
+    double fl[NF], fr[NF], gl[NG], gr[NG];
+    load_tflux(F, v, fl, gl);
+    load_tflux(F, v + shift<T>(L), fr, gr);
+    trans_diff_one(fl, fr, gl, gr, cdt, t);
+}
+
+// waves per SIMD requested for the z-marched transdo kernel: 2 keeps every
+// value in registers; 3 trades a few spilled bytes per lane for occupancy
+#ifndef CAMR_T1ZM_MINW
+#define CAMR_T1ZM_MINW 2
+#endif
+__global__ void __launch_bounds__(kBlock, CAMR_T1ZM_MINW)
+k_trans1_zm(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_COLUMN(ST_T1 + 2)
+    const double cda = 1.0 / 3.0 * *dtp / hc.dx[0];
+    const double cdb = 1.0 / 3.0 * *dtp / hc.dx[1];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC);
+    const CView qm = cview(L, slot_ptr<S_QM0 + 2>(E), NS), qp = cview(L, slot_ptr<S_QP0 + 2>(E), NS);
+    const CView Fx = cview(L, slot_ptr<S_F1 + 0>(E), NTF), Fy = cview(L, slot_ptr<S_F1 + 1>(E), NTF);
+    const CView Ox = cview(L, slot_ptr<f2_slot(2, 0)>(E), NTF);
+    const CView Oy = cview(L, slot_ptr<f2_slot(2, 1)>(E), NTF);
+    // the cell below the first face
+    TransDiff tmx, tmy;
+    unsigned v = L.off(i, j, k0 - 1);
+    trans1_diff<0>(Fx, v, L, cda, tmx);
+    trans1_diff<1>(Fy, v, L, cdb, tmy);
+    double cm = q.ld(v, SC), csm = q.ld(v, SCS);
+    for (int k = k0; k <= k1; ++k) {
+        v = L.off(i, j, k);
+        double np[NS], nm[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            np[s] = qp.ld(v, s);
+            nm[s] = qm.ld(v, s);
+        }
+        FaceAux a;
+        a.c_l = cm;
+        a.csml_l = csm;
+        a.c_r = q.ld(v, SC);
+        a.csml_r = q.ld(v, SCS);
+        a.gam_l = kGamma;
+        a.gam_r = kGamma;
+        {
+            double sp[NS], sm[NS], f[NF], g[NG];
+            int mut;
+            TransDiff tp;
+            trans1_diff<0>(Fx, v, L, cda, tp);
+            transverse_one_d(np, tp, cda, kGamma, hc.trd, hc.small_pres, sp);
+            transverse_one_d(nm, tmx, cda, kGamma, hc.trd, hc.small_pres, sm);
+            tmx = tp;
+            face_flux(2, k, hc, sm, sp, a, f, g, &mut);
+            store_tflux(Ox, v, f, g);
+        }
+        {
+            double sp[NS], sm[NS], f[NF], g[NG];
+            int mut;
+            TransDiff tp;
+            trans1_diff<1>(Fy, v, L, cdb, tp);
+            transverse_one_d(np, tp, cdb, kGamma, hc.trd, hc.small_pres, sp);
+            transverse_one_d(nm, tmy, cdb, kGamma, hc.trd, hc.small_pres, sm);
+            tmy = tp;
+            face_flux(2, k, hc, sm, sp, a, f, g, &mut);
+            store_tflux(Oy, v, f, g);
+        }
+        cm = a.c_r;
+        csm = a.csml_r;
+    }
+}
+
+template <int HAS_SRC>
+__global__ void TRANS_BOUNDS
+k_trans2_zm(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_COLUMN(ST_T2 + 2)
+    const double dt = *dtp;
+    const double hdt = 0.5 * dt;
+    const double ca = 0.5 * dt / hc.dx[0], cb = 0.5 * dt / hc.dx[1];
+    const double area = hc.dx[0] * hc.dx[1];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+    const CView qm = cview(L, slot_ptr<S_QM0 + 2>(E), NS), qp = cview(L, slot_ptr<S_QP0 + 2>(E), NS);
+    const CView Fa = cview(L, slot_ptr<f2_slot(0, 1)>(E), NTF);
+    const CView Fb = cview(L, slot_ptr<f2_slot(1, 0)>(E), NTF);
+    const CView dv = cview(L, slot_ptr<S_DIVU>(E), 1), uc = cview(L, slot_ptr<S_U>(E), NVAR);
+    const CView flx = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
+    const unsigned sx = 8u, sy = L.sy();
+    // the cell below the first face
+    TransDiff tm;
+    double srcm[NS], um[NVAR];
+    unsigned v = L.off(i, j, k0 - 1);
+    trans2_diff<2>(v, L, Fa, Fb, ca, cb, tm);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) srcm[s] = HAS_SRC ? sq.ld(v, s) : 0.0;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) um[n] = uc.ld(v, n);
+    double cm = q.ld(v, SC), csm = q.ld(v, SCS);
+    for (int k = k0; k <= k1; ++k) {
+        v = L.off(i, j, k);
+        TransDiff tp;
+        trans2_diff<2>(v, L, Fa, Fb, ca, cb, tp);
+        double srcp[NS], sp[NS], sm[NS];
+        {
+            double n[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                n[s] = qp.ld(v, s);
+                srcp[s] = HAS_SRC ? sq.ld(v, s) : 0.0;
+            }
+            transverse_two_d(n, tp, hdt, srcp, hc.trd, hc.small_pres, sp);
+        }
+        {
+            double n[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) n[s] = qm.ld(v, s);
+            transverse_two_d(n, tm, hdt, srcm, hc.trd, hc.small_pres, sm);
+        }
+        FaceAux a;
+        a.c_l = cm;
+        a.csml_l = csm;
+        a.c_r = q.ld(v, SC);
+        a.csml_r = q.ld(v, SCS);
+        a.gam_l = kGamma;
+        a.gam_r = kGamma;
+        double f[NF], g[NG];
+        int mut;
+        face_flux(2, k, hc, sm, sp, a, f, g, &mut);
+        double fu[NVAR];
+        fu[URHO] = f[FR]; fu[UMX] = f[FMX]; fu[UMY] = f[FMY]; fu[UMZ] = f[FMZ];
+        fu[UEDEN] = f[FE]; fu[UEINT] = f[FEI]; fu[UTEMP] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSPEC; ++s) fu[UFS + s] = f[FY + s];
+        double up[NVAR], d4[4];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) up[n] = uc.ld(v, n);
+        d4[0] = dv.ld(v, 0);
+        d4[1] = dv.ld(v + sx, 0);
+        d4[2] = dv.ld(v + sy, 0);
+        d4[3] = dv.ld(v + sx + sy, 0);
+        adjust_zface_vals(fu, d4, up, um, k, area, hc);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) flx.st(v, n, fu[n]);
+        flx.st(v, XGU, g[GUN]);
+        flx.st(v, XGP, g[GP]);
+        tm = tp;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) srcm[s] = srcp[s];
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) um[n] = up[n];
+        cm = a.c_r;
+        csm = a.csml_r;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_divu_c(Batch B, HydroConst hc) {
     BATCH_POINT(ST_DIVU)
@@ -468,6 +662,7 @@ Tuning default_tuning() {
         t.tile[k][0] = -1;
         t.tile[k][1] = t.tile[k][2] = 0;
         t.xcd[k] = 0;
+        t.zm[k] = 0;
     }
     t.order = 0;
     return t;
@@ -510,7 +705,7 @@ int FusedEngine::setup(const std::vector<Box> &boxes, const std::vector<Arr> &bu
     }
     if (blk_dev_) (void)hipFree(blk_dev_);
     blk_dev_ = nullptr;
-    if (hipMalloc(&blk_dev_, (size_t)ST_COUNT * (n + 1) * sizeof(int)) != hipSuccess) {
+    if (hipMalloc(&blk_dev_, (size_t)2 * ST_COUNT * (n + 1) * sizeof(int)) != hipSuccess) {
         set_error("block table allocation failed");
         return CAMR_ERR_ALLOC;
     }
@@ -536,6 +731,12 @@ int FusedEngine::set_tuning(const Tuning &t) {
     for (int k = 0; k < ST_COUNT; ++k) {
         if (t.xcd[k] < 0 || t.xcd[k] > 4096) {
             set_error("stage %d: XCD chunk must be 0..4096", k);
+            return CAMR_ERR_PARAM;
+        }
+        if (t.zm[k] < 0 || t.zm[k] > 1024 ||
+            (t.zm[k] > 0 && ((k != ST_T1 + 2 && k != ST_T2 + 2) || t.tile[k][0] >= 0))) {
+            set_error("stage %d: z-march (1..1024 planes) exists for the linear blocks of "
+                      "the z-face transverse stages only", k);
             return CAMR_ERR_PARAM;
         }
         const int *l = t.tile[k];
@@ -577,13 +778,15 @@ int FusedEngine::build_tables() {
         e.sold = bufA_[b];
         t1[p] = e;
     }
-    blk_host_.assign((size_t)ST_COUNT * (n + 1), 0);
-    for (int k = 0; k < ST_COUNT; ++k) {
+    blk_host_.assign((size_t)2 * ST_COUNT * (n + 1), 0);
+    for (int k = 0; k < 2 * ST_COUNT; ++k) {
         int *row = &blk_host_[(size_t)k * (n + 1)];
+        const int kind = k % ST_COUNT;
+        const int zm = k < ST_COUNT ? tune_.zm[kind] : 0;
         for (int p = 0; p < n; ++p) {
-            const int *l = tune_.tile[k];
-            const int64_t nb = batch_blocks(stage_box(k, boxes_[order_[p]]), tune_.block[k],
-                                            l[0], l[1], l[2]);
+            const int *l = tune_.tile[kind];
+            const int64_t nb = batch_blocks(stage_box(kind, boxes_[order_[p]]),
+                                            tune_.block[kind], l[0], l[1], l[2], zm);
             if ((int64_t)row[p] + nb > 0x7fffffff) {
                 set_error("stage grid exceeds 2^31 blocks");
                 return CAMR_ERR_SHAPE;
@@ -612,12 +815,13 @@ int FusedEngine::enable_adjoint() {
     return build_tables();
 }
 
-LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {
+LaunchCfg FusedEngine::cfg(int kind, int b, int parity, bool linear) const {
     const int n = (int)boxes_.size();
-    const int *row = &blk_host_[(size_t)kind * (n + 1)];
+    const int rk = kind + (linear ? ST_COUNT : 0);
+    const int *row = &blk_host_[(size_t)rk * (n + 1)];
     LaunchCfg c;
     c.B.tab = tab_[parity];
-    c.B.blk = blk_dev_ + (size_t)kind * (n + 1);
+    c.B.blk = blk_dev_ + (size_t)rk * (n + 1);
     c.B.nbox = n;
     if (b < 0) {
         c.B.first = 0;
@@ -633,6 +837,7 @@ LaunchCfg FusedEngine::cfg(int kind, int b, int parity) const {
     c.B.ly = tune_.tile[kind][1];
     c.B.lz = tune_.tile[kind][2];
     c.B.xk = tune_.xcd[kind];
+    c.B.zm = linear ? 0 : tune_.zm[kind];
     return c;
 }
 
@@ -666,16 +871,26 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
     launch("riemann_normal", prof, k_riemann1<2>, ST_R1 + 2, b, parity, st, hc);
     launch("transverse_1", prof, k_trans1<0>, ST_T1 + 0, b, parity, st, hc, dtp);
     launch("transverse_1", prof, k_trans1<1>, ST_T1 + 1, b, parity, st, hc, dtp);
-    launch("transverse_1", prof, k_trans1<2>, ST_T1 + 2, b, parity, st, hc, dtp);
+    if (tune_.zm[ST_T1 + 2] > 0)
+        launch("transverse_1", prof, k_trans1_zm, ST_T1 + 2, b, parity, st, hc, dtp);
+    else
+        launch("transverse_1", prof, k_trans1<2>, ST_T1 + 2, b, parity, st, hc, dtp);
     launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
+    const bool zm2 = tune_.zm[ST_T2 + 2] > 0;
     if (has_src) {
         launch("transverse_2", prof, k_trans2<0, 1>, ST_T2 + 0, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<1, 1>, ST_T2 + 1, b, parity, st, hc, dtp);
-        launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
+        if (zm2)
+            launch("transverse_2", prof, k_trans2_zm<1>, ST_T2 + 2, b, parity, st, hc, dtp);
+        else
+            launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
     } else {
         launch("transverse_2", prof, k_trans2<0, 0>, ST_T2 + 0, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
-        launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
+        if (zm2)
+            launch("transverse_2", prof, k_trans2_zm<0>, ST_T2 + 2, b, parity, st, hc, dtp);
+        else
+            launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
     }
     if (no_update) {
         // the caller applies the update itself

@@ -262,6 +262,17 @@ class Level:
     def xcd(self, chunks):
         L.check(L.lib().camr_level_set_xcd(self._h, (C.c_int32 * L.NSTAGE)(*map(int, chunks))))
 
+    @property
+    def zmarch(self):
+        """z planes each thread walks, per stage range (0: one point)."""
+        t = (C.c_int32 * L.NSTAGE)()
+        L.check(L.lib().camr_level_get_zmarch(self._h, t))
+        return list(t)
+
+    @zmarch.setter
+    def zmarch(self, planes):
+        L.check(L.lib().camr_level_set_zmarch(self._h, (C.c_int32 * L.NSTAGE)(*map(int, planes))))
+
     def profile(self, enable=True, kernel=None):
         """Time launches (all, or only those named `kernel`) with hipEvents."""
         L.check(L.lib().camr_level_profile(

@@ -379,6 +379,14 @@ int camr_level_get_tiles(camr_level *L, int32_t tile[3 * CAMR_NSTAGE]);
  * by the autotuner. */
 int camr_level_set_xcd(camr_level *L, const int32_t chunk[CAMR_NSTAGE]);
 int camr_level_get_xcd(camr_level *L, int32_t chunk[CAMR_NSTAGE]);
+/* z-march of each stage range: 0 = one point per thread; P > 0 = each
+ * thread walks P consecutive z planes of its (x, y) column, carrying what a
+ * face shares with the face below in registers.  Only the z-face transverse
+ * stages (transdo pairs and transdd, stage ranges 7 and 11) with linear
+ * blocks take it.  Results do not depend on it.  camr_level_set_tuning resets
+ * it to 0.  No reference counterpart: set by the autotuner. */
+int camr_level_set_zmarch(camr_level *L, const int32_t planes[CAMR_NSTAGE]);
+int camr_level_get_zmarch(camr_level *L, int32_t planes[CAMR_NSTAGE]);
 
 /* Adjoint (reverse mode) of level steps, for gradients of a loss on the
  * final state with respect to the initial state.  No reference counterpart

@@ -387,12 +387,81 @@ def test_tiled_blocks_bitwise(oracle, tiles, batched, graph):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"tiled box {b}")
 
 
+@pytest.mark.parametrize("planes", [1, 3, 8, 64])
+@pytest.mark.parametrize("batched,graph", [(True, False), (False, False), (True, True)])
+def test_zmarch_bitwise(oracle, planes, batched, graph):
+    """z-face transverse stages with every thread walking `planes` z planes
+    (the last runs of uneven boxes shorter), values carried between faces in
+    registers: a refined level of uneven boxes, bit for bit."""
+    prob = workloads.refined_level(n=64, radius=0.3, blocking_factor=8, max_size=32,
+                                   efficiency=0.8)
+    ext = workloads.exterior(prob)
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gl = Level(prob, batched=batched, graph=graph)
+    zm = [0] * 13
+    zm[7] = zm[11] = planes
+    gl.zmarch = zm
+    gl.xcd = [16] * 13
+    assert gl.zmarch == zm
+    for L in (ol, gl):
+        L.set_exterior(ext)
+        for b in range(len(prob.boxes)):
+            L.set_box(b, init(prob, b))
+        L.init_post()
+    err, odts = ol.run(3)
+    assert err == 0
+    gdts = gl.run(3)
+    assert np.array_equal(odts, gdts)
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"z-marched box {b}")
+
+
+@pytest.mark.parametrize("ppm", [0, 1])
+def test_zmarch_with_sources_and_walls_bitwise(oracle, ppm):
+    n = (24, 20, 16)
+    dx = (1 / 24,) * 3
+    prob = Problem(domlo=(0, 0, 0), domhi=tuple(x - 1 for x in n), dx=dx,
+                   bclo=(2, 4, 5), bchi=(3, 5, 4), is_periodic=(0, 0, 0),
+                   boxes=[(0, 0, 0, 11, 19, 15), (12, 0, 0, 23, 19, 15)],
+                   params=dict(ppm_type=ppm, cfl=0.4, const_grav=-0.7))
+    full = problems.SmoothField(seed=3, kmax=3, mach=0.4).state(48, (0, 0, 0), (23, 19, 15))
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gl = Level(prob, engine=_lib.ENGINE_FUSED)
+    zm = [0] * 13
+    zm[7] = zm[11] = 5
+    gl.zmarch = zm
+    for L in (ol, gl):
+        for b in range(2):
+            lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+            L.set_box(b, np.ascontiguousarray(full[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1,
+                                                   lo[0]:hi[0] + 1]))
+        L.init_post()
+    err, odts = ol.run(4)
+    assert err == 0
+    gdts = gl.run(4)
+    assert np.array_equal(odts, gdts)
+    for b in range(2):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"z-marched walls box {b}")
+
+
 def test_tile_shapes_are_validated():
     prob = workloads.smooth_periodic(n=16, ppm_type=1)
     gl = Level(prob)
     for bad in ([(12, 4, 4)] * 13, [(4, 8, 8)] * 13, [(64, 4, 4)] * 13):
         with pytest.raises(_lib.CamrError):
             gl.tiles = bad
+    with pytest.raises(_lib.CamrError):
+        gl.zmarch = [4] * 13          # only the z-face transverse stages march
+    gl.tiles = [(16, 4, 4)] * 13
+    zm = [0] * 13
+    zm[7] = 4
+    with pytest.raises(_lib.CamrError):
+        gl.zmarch = zm                # marching needs linear blocks
 
 
 def test_unowned_ghosts_without_exterior_are_an_error():
