@@ -90,6 +90,7 @@ NSTAGE = 13  # CAMR_NSTAGE: stage ranges of the fused engine
 EXPORTS = [
     "camr_default_params", "camr_last_error", "camr_version",
     "camr_hydro_ctoprim", "camr_hydro_srctoprim", "camr_hydro_umdrv", "camr_scratch_bytes",
+    "camr_hydro_umdrv_noalloc",
     "camr_estdt_hydro", "camr_default_level_desc", "camr_level_create",
     "camr_level_destroy", "camr_level_set_stream", "camr_level_upload_box",
     "camr_level_download_box", "camr_level_state_array",
@@ -142,6 +143,7 @@ def lib():
         L.camr_hydro_srctoprim.argtypes = [C.POINTER(Box), A4, A4, A4, A4, vp]
         L.camr_hydro_umdrv.argtypes = [C.POINTER(Box), A4, A4, A4, A4, A4, A4, A4, A4,
                                        C.c_double, C.POINTER(HydroParams), vp]
+        L.camr_hydro_umdrv_noalloc.argtypes = L.camr_hydro_umdrv.argtypes
         L.camr_scratch_bytes.argtypes = [C.POINTER(Box), C.POINTER(HydroParams),
                                          C.POINTER(C.c_size_t)]
         L.camr_estdt_hydro.argtypes = [C.POINTER(Box), A4, dp, dp, vp]

@@ -2744,6 +2744,60 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     return rc;
 }
 
+// ---------------------------------------- allocation-free hydro_umdrv
+// Scratch of camr_hydro_umdrv_noalloc: a file-scope device array, one copy
+// per GPU, held from module load to process end.  12 Mi doubles (96 MiB)
+// hold the Godunov temporaries of boxes up to 32^3 cells.  In host code
+// `scratch` names a host-side stand-in, never a device address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress on that GPU.
+constexpr size_t kStaticScratchDoubles = size_t(12) << 20;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
+// Device address of this GPU's `scratch`, looked up on the device's first
+// call (with the device current) and kept per device.
+static double *static_scratch(int dev) {
+    static std::mutex mu;
+    static std::vector<double *> ptr;
+    std::lock_guard<std::mutex> g(mu);
+    if ((int)ptr.size() <= dev) ptr.resize(dev + 1, nullptr);
+    if (!ptr[dev]) {
+        void *a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !a) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return nullptr;
+        }
+        ptr[dev] = (double *)a;
+    }
+    return ptr[dev];
+}
+
+int camr_hydro_umdrv_noalloc(const camr_box *bx, const camr_array4 *uin,
+                             const camr_array4 *dsdt, const camr_array4 *q,
+                             const camr_array4 *qaux, const camr_array4 *srcq,
+                             const camr_array4 flx[3], const camr_array4 area[3],
+                             const camr_array4 *vol, double dt, const camr_hydro_params *p,
+                             void *stream) {
+    UmdrvArgs A;
+    size_t n = 0;
+    int rc = umdrv_args(bx, uin, dsdt, q, qaux, srcq, flx, area, vol, dt, p, &A, &n);
+    if (rc) return rc;
+    if (n + 1 > kStaticScratchDoubles) {
+        set_error("camr_hydro_umdrv_noalloc: the box needs %zu bytes of scratch, more than "
+                  "the %zu-byte static maximum", (n + 1) * sizeof(double),
+                  kStaticScratchDoubles * sizeof(double));
+        return CAMR_ERR_SHAPE;
+    }
+    int dev = 0;
+    CAMR_HIP(hipGetDevice(&dev));
+    double *scr = static_scratch(dev);
+    if (!scr) return CAMR_ERR_HIP;
+    // every scratch element the kernels read is written earlier in the same
+    // call, so the previous call's contents need no clearing
+    return umdrv_run(A, scr, n, dt, p->do_mol, (hipStream_t)stream);
+}
+
 int camr_hydro_umdrv_eb(const camr_box *bx, const camr_array4 *uin, const camr_array4 *dsdt,
                         const camr_array4 *q, const camr_array4 *qaux, const camr_array4 *srcq,
                         const camr_array4 flx[3], const camr_eb_geom *geom, double dt,

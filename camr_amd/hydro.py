@@ -333,14 +333,16 @@ def scratch_bytes(lo, hi, params):
 
 
 def hydro_umdrv(lo, hi, uin, uin_lo, dsdt, q, qa, srcq, flx, dt, params,
-                area=None, vol=None, stream=0):
+                area=None, vol=None, stream=0, noalloc=False):
     """hydro_umdrv (Hydro.H:11-35) on device tensors; q/qa/srcq start at
-    lo-4, flx[d] at lo, dsdt at lo.  srcq None = identically +0 sources."""
+    lo-4, flx[d] at lo, dsdt at lo.  srcq None = identically +0 sources.
+    noalloc: the allocation-free entry point (static device scratch)."""
     g = [x - L.NGROW for x in lo]
     fl = (L.Array4 * 3)(*[array4_of(f, lo) for f in flx])
     ar = (L.Array4 * 3)(*([array4_of(a, lo) for a in area] if area else
                           [L.Array4() for _ in range(3)]))
-    L.check(L.lib().camr_hydro_umdrv(
+    fn = L.lib().camr_hydro_umdrv_noalloc if noalloc else L.lib().camr_hydro_umdrv
+    L.check(fn(
         C.byref(_box(lo, hi)), C.byref(array4_of(uin, uin_lo)),
         C.byref(array4_of(dsdt, lo)), C.byref(array4_of(q, g)),
         C.byref(array4_of(qa, g)),

@@ -133,6 +133,21 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin,
                      const camr_array4 *vol, double dt,
                      const camr_hydro_params *p, void *stream);
 
+/* camr_hydro_umdrv with no allocation of any kind: the scratch is a
+ * file-scope device array of the library (96 MiB per GPU, held until the
+ * process ends), enough for boxes up to 32^3 cells; larger boxes
+ * (camr_scratch_bytes above 96 MiB) are refused with CAMR_ERR_SHAPE.  No
+ * counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Calls share their GPU's array: calls that overlap on one GPU
+ * (on different streams) race. */
+int camr_hydro_umdrv_noalloc(const camr_box *bx, const camr_array4 *uin,
+                             const camr_array4 *dsdt, const camr_array4 *q,
+                             const camr_array4 *qaux, const camr_array4 *srcq,
+                             const camr_array4 flx[3], const camr_array4 area[3],
+                             const camr_array4 *vol, double dt,
+                             const camr_hydro_params *p, void *stream);
+
 /* Bytes of scratch one camr_hydro_umdrv call on box bx takes with params p
  * (the Godunov or MOL temporaries of Godunov_3D.cpp:79-98 /
  * MOL_umeth.cpp, plus the device word of dt). */
