@@ -2692,7 +2692,8 @@ int camr_scratch_bytes(const camr_box *bx, const camr_hydro_params *p, size_t *b
     if (!bx || !p || !bytes) { set_error("null argument"); return CAMR_ERR_PARAM; }
     const Box B = make_box(bx->lo, bx->hi);
     if (!B.ok()) { set_error("empty box"); return CAMR_ERR_SHAPE; }
-    const size_t n = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
+    size_t n = p->do_mol ? umdrv_mol_scratch_doubles(B) : umdrv_staged_scratch_doubles(B);
+    if (!p->do_mol && canon_fits(B)) n = std::max(n, fused_umdrv_scratch_doubles(B));
     *bytes = (n + 1) * sizeof(double);
     return CAMR_OK;
 }
@@ -2733,13 +2734,24 @@ int camr_hydro_umdrv(const camr_box *bx, const camr_array4 *uin, const camr_arra
     int rc = umdrv_args(bx, uin, dsdt, q, qaux, srcq, flx, area, vol, dt, p, &A, &n);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    // Godunov boxes on a uniform grid with q over exactly grow(bx, 4) run on
+    // the fused engine's kernels; others on the staged ones (same results)
+    const bool fused = fused_umdrv_applies(A, p->do_mol);
+    if (fused) n = fused_umdrv_scratch_doubles(A.bx);
     int dev = 0;
     CAMR_HIP(hipGetDevice(&dev));
     hipMemPool_t pool = umdrv_pool(dev);
     if (!pool) { set_error("scratch pool creation failed on device %d", dev); return CAMR_ERR_ALLOC; }
     double *scr = nullptr;
     CAMR_HIP(hipMallocFromPoolAsync((void **)&scr, (n + 1) * sizeof(double), pool, s));
-    rc = umdrv_run(A, scr, n, dt, p->do_mol, s);
+    if (fused) {
+        A.dtp = scr + n;
+        put_double(scr + n, dt, s);
+        rc = fused_umdrv(A, scr, s);
+        if (rc) set_error("kernel launch failed");
+    } else {
+        rc = umdrv_run(A, scr, n, dt, p->do_mol, s);
+    }
     CAMR_HIP(hipFreeAsync(scr, s));
     return rc;
 }

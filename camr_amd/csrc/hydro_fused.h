@@ -404,4 +404,16 @@ class FusedEngine {
 
 Tuning default_tuning();
 
+// hydro_umdrv of one box (Hydro_umdrv.cpp:11-93, Godunov) on the fused
+// engine's kernels, for callers that hold the reference's arrays: q / qaux
+// (QVAR / NQAUX layout over exactly grow(bx, 4)), srcq, uin are converted
+// into the canonical arrays, the fused stages run over a one-box table, and
+// the fluxes go to flx[d] and the divergence is added to dsdt with the
+// operation order of hydro_update.  Applies without area / volume arrays
+// (uniform Cartesian grid) and without the method of lines.
+struct UmdrvArgs;
+bool fused_umdrv_applies(const UmdrvArgs &A, int do_mol);
+size_t fused_umdrv_scratch_doubles(const Box &bx);
+int fused_umdrv(const UmdrvArgs &A, double *scratch, hipStream_t st);
+
 } // namespace camr

@@ -21,6 +21,9 @@
 // neighbours through uniform offsets.
 #include "canon.h"
 #include "hydro_fused.h"
+#include "hydro_kernels.h"
+
+#include <cstring>
 
 namespace camr {
 
@@ -624,6 +627,87 @@ k_trans2_zm(Batch B, HydroConst hc, const double *dtp) {
     }
 }
 
+// ------------------------------------------------ one-box route (per-box ABI)
+// The table of a one-box launch: its entry and a block prefix of {0}.
+struct OneBox {
+    BoxEnt e;
+    int blk0;
+};
+__global__ void k_put_onebox(OneBox *dst, OneBox v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// The caller's primitives (QVAR / NQAUX layout), primitive sources and
+// conserved state into the canonical compact arrays.
+__global__ void __launch_bounds__(kBlock)
+k_box_in(Batch B, Arr q, Arr qa, Arr srcq, int has_src, Arr uin, Box ubox) {
+    BATCH_POINT(ST_G4)
+    const unsigned v0 = L.off(i, j, k);
+    const CView qc = cview(L, slot_ptr<S_Q>(E), NQC);
+    qc.st(v0, SR, q(i, j, k, QRHO));
+    qc.st(v0, SU, q(i, j, k, QU));
+    qc.st(v0, SV, q(i, j, k, QV));
+    qc.st(v0, SW, q(i, j, k, QW));
+    qc.st(v0, SP, q(i, j, k, QPRES));
+    qc.st(v0, SE, q(i, j, k, QREINT));
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) qc.st(v0, SY + s, q(i, j, k, QFS + s));
+    qc.st(v0, SC, qa(i, j, k, QC));
+    qc.st(v0, SCS, qa(i, j, k, QCSML));
+    if (has_src) {
+        const CView sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+        sq.st(v0, SR, srcq(i, j, k, QRHO));
+        sq.st(v0, SU, srcq(i, j, k, QU));
+        sq.st(v0, SV, srcq(i, j, k, QV));
+        sq.st(v0, SW, srcq(i, j, k, QW));
+        sq.st(v0, SP, srcq(i, j, k, QPRES));
+        sq.st(v0, SE, srcq(i, j, k, QREINT));
+#pragma unroll
+        for (int s = 0; s < NSPEC; ++s) sq.st(v0, SY + s, srcq(i, j, k, QFS + s));
+    }
+    if (ubox.contains(i, j, k)) {
+        const CView uc = cview(L, slot_ptr<S_U>(E), NVAR);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) uc.st(v0, n, uin(i, j, k, n));
+    }
+}
+
+// The final d-fluxes of the canonical array into the caller's flx[d].
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_box_flux_out(Batch B, Arr flx) {
+    BATCH_POINT(ST_T2 + D)
+    const CView f = cview(L, slot_ptr<S_FLX + D>(E), NXF);
+    const unsigned v0 = L.off(i, j, k);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) flx(i, j, k, n) = f.ld(v0, n);
+}
+
+// hydro_pdivu (Hydro_utils_K.H:175-202) and hydro_update (:239-257) into
+// dsdt (+=), in the operation order of the staged k_update with the
+// uniform-grid area and volume.
+__global__ void __launch_bounds__(kBlock)
+k_box_update(Batch B, Arr dsdt, double ac0, double ac1, double ac2, double volc) {
+    BATCH_POINT(ST_VALID)
+    const CView f0 = cview(L, slot_ptr<S_FLX + 0>(E), NXF), f1 = cview(L, slot_ptr<S_FLX + 1>(E), NXF),
+                f2 = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
+    const unsigned v0 = L.off(i, j, k), sx = 8u, sy = L.sy(), sz = L.sz();
+    const double pdivu =
+        0.5 *
+        ((f0.ld(v0 + sx, XGP) + f0.ld(v0, XGP)) * (f0.ld(v0 + sx, XGU) * ac0 - f0.ld(v0, XGU) * ac0) +
+         (f1.ld(v0 + sy, XGP) + f1.ld(v0, XGP)) * (f1.ld(v0 + sy, XGU) * ac1 - f1.ld(v0, XGU) * ac1) +
+         (f2.ld(v0 + sz, XGP) + f2.ld(v0, XGP)) * (f2.ld(v0 + sz, XGU) * ac2 - f2.ld(v0, XGU) * ac2)) /
+        volc;
+    const double volinv = 1.0 / volc;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        dsdt(i, j, k, n) += (f0.ld(v0, n) - f0.ld(v0 + sx, n) + f1.ld(v0, n) - f1.ld(v0 + sy, n) +
+                             f2.ld(v0, n) - f2.ld(v0 + sz, n)) *
+                            volinv;
+    }
+    dsdt(i, j, k, UEINT) -= pdivu;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_divu_c(Batch B, HydroConst hc) {
     BATCH_POINT(ST_DIVU)
@@ -917,6 +1001,91 @@ int FusedEngine::post(int b, int parity, const PostArgs &a, hipStream_t st, int 
     const LaunchCfg c = cfg(ST_VALID, b, parity);
     Region reg(prof, "clean_estdt", st);
     launch_post_fused(c, a, st, status);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
+// ------------------------------------------------------- one-box route
+namespace {
+constexpr size_t kOneBoxHeader = 64;  // doubles ahead of the canonical arrays (512 bytes)
+static_assert(sizeof(OneBox) <= kOneBoxHeader * sizeof(double), "one-box header");
+
+template <typename K, typename... A>
+void launch_one(K kern, int kind, const Box &bx, const Batch &B0, hipStream_t st, A... args) {
+    const int64_t nb = batch_blocks(stage_box(kind, bx), kBlock, -1, 0, 0);
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kBlock), 0, st, B0, args...);
+}
+} // namespace
+
+bool fused_umdrv_applies(const UmdrvArgs &A, int do_mol) {
+    const Box g = grow(A.bx, CAMR_NGROW);
+    bool same = true;
+    for (int d = 0; d < 3; ++d) same = same && A.qbox.lo[d] == g.lo[d] && A.qbox.hi[d] == g.hi[d];
+    return !do_mol && !A.has_area && !A.has_vol && same && canon_fits(A.bx);
+}
+
+size_t fused_umdrv_scratch_doubles(const Box &bx) {
+    return kOneBoxHeader + FusedEngine::scratch_doubles(bx);
+}
+
+int fused_umdrv(const UmdrvArgs &A, double *scratch, hipStream_t st) {
+    const Box bx = A.bx;
+    const HydroConst &hc = A.hc;
+    OneBox ob;
+    memset(&ob, 0, sizeof(ob));
+    ob.e.id = 0;
+    ob.e.bx = bx;
+    ob.e.L = canon_of(bx);
+    ob.e.base = scratch + kOneBoxHeader;
+    ob.e.abase = nullptr;
+    ob.blk0 = 0;
+    OneBox *dev = (OneBox *)scratch;
+    hipLaunchKernelGGL(k_put_onebox, dim3(1), dim3(64), 0, st, dev, ob);
+    Batch B;
+    memset(&B, 0, sizeof(B));
+    B.tab = &dev->e;
+    B.blk = &dev->blk0;
+    B.nbox = 1;
+    B.first = 0;
+    B.halt = nullptr;
+    B.lx = -1;
+    B.xk = 0;
+    B.zm = 0;
+    const Box ubox = grow(bx, 1);
+    launch_one(k_box_in, ST_G4, bx, B, st, A.q, A.qaux, A.srcq, A.has_src, A.uin, ubox);
+    const double *dtp = A.dtp;
+    const int has_src = A.has_src;
+    launch_one(k_flat_c, ST_G2, bx, B, st, hc.use_flattening);
+    if (hc.ppm_type == 1) {
+        launch_one(k_ppm_c<0>, ST_G2, bx, B, st, hc, dtp);
+        launch_one(k_ppm_c<1>, ST_G2, bx, B, st, hc, dtp);
+        launch_one(k_ppm_c<2>, ST_G2, bx, B, st, hc, dtp);
+    } else {
+        launch_one(k_plm_c<0>, ST_G2, bx, B, st, has_src, hc, dtp);
+        launch_one(k_plm_c<1>, ST_G2, bx, B, st, has_src, hc, dtp);
+        launch_one(k_plm_c<2>, ST_G2, bx, B, st, has_src, hc, dtp);
+    }
+    launch_one(k_riemann1<0>, ST_R1 + 0, bx, B, st, hc);
+    launch_one(k_riemann1<1>, ST_R1 + 1, bx, B, st, hc);
+    launch_one(k_riemann1<2>, ST_R1 + 2, bx, B, st, hc);
+    launch_one(k_trans1<0>, ST_T1 + 0, bx, B, st, hc, dtp);
+    launch_one(k_trans1<1>, ST_T1 + 1, bx, B, st, hc, dtp);
+    launch_one(k_trans1<2>, ST_T1 + 2, bx, B, st, hc, dtp);
+    launch_one(k_divu_c, ST_DIVU, bx, B, st, hc);
+    if (has_src) {
+        launch_one(k_trans2<0, 1>, ST_T2 + 0, bx, B, st, hc, dtp);
+        launch_one(k_trans2<1, 1>, ST_T2 + 1, bx, B, st, hc, dtp);
+        launch_one(k_trans2<2, 1>, ST_T2 + 2, bx, B, st, hc, dtp);
+    } else {
+        launch_one(k_trans2<0, 0>, ST_T2 + 0, bx, B, st, hc, dtp);
+        launch_one(k_trans2<1, 0>, ST_T2 + 1, bx, B, st, hc, dtp);
+        launch_one(k_trans2<2, 0>, ST_T2 + 2, bx, B, st, hc, dtp);
+    }
+    launch_one(k_box_flux_out<0>, ST_T2 + 0, bx, B, st, A.flx[0]);
+    launch_one(k_box_flux_out<1>, ST_T2 + 1, bx, B, st, A.flx[1]);
+    launch_one(k_box_flux_out<2>, ST_T2 + 2, bx, B, st, A.flx[2]);
+    launch_one(k_box_update, ST_VALID, bx, B, st, A.dsdt, A.area_c[0], A.area_c[1], A.area_c[2],
+               A.vol_c);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 

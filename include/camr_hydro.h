@@ -121,6 +121,10 @@ int camr_hydro_srctoprim(const camr_box *qbx, const camr_array4 *q,
  * area[d].p / vol->p may be NULL for a uniform Cartesian grid (then
  * area = product of the two other dx, vol = dx*dy*dz as geom.GetFaceArea /
  * GetVolume give); srcq->p may be NULL for identically +0 sources.
+ * Godunov boxes on a uniform grid (area and vol NULL) whose q covers
+ * exactly grow(bx,4), as construct_hydro_source allocates it, run on the
+ * fused engine's kernels; other calls on the stage-by-stage kernels; both
+ * give the same results.
  * Scratch memory (the reference's The_Async_Arena temporaries,
  * Godunov_3D.cpp:79-98) comes from a stream-ordered memory pool per device
  * owned by the library: freed in stream order at the end of the call and
@@ -148,8 +152,8 @@ int camr_hydro_umdrv_noalloc(const camr_box *bx, const camr_array4 *uin,
                              const camr_array4 *vol, double dt,
                              const camr_hydro_params *p, void *stream);
 
-/* Bytes of scratch one camr_hydro_umdrv call on box bx takes with params p
- * (the Godunov or MOL temporaries of Godunov_3D.cpp:79-98 /
+/* Bytes of scratch one camr_hydro_umdrv call on box bx takes at most with
+ * params p (the Godunov or MOL temporaries of Godunov_3D.cpp:79-98 /
  * MOL_umeth.cpp, plus the device word of dt). */
 int camr_scratch_bytes(const camr_box *bx, const camr_hydro_params *p, size_t *bytes);
 

@@ -215,6 +215,62 @@ def test_umdrv_per_box_bitwise(oracle, case):
     assert_bitwise(tds.cpu().numpy(), dsdt, "dsdt")
 
 
+@pytest.mark.parametrize("route", ["area_vol_arrays", "larger_q_box"])
+def test_umdrv_stage_route_bitwise(oracle, route):
+    """Calls that leave the fused route (area / volume arrays given, or q
+    over more than grow(bx,4)) run the stage-by-stage kernels, with the
+    same results."""
+    lo, hi = (3, 5, 7), (3 + 15, 5 + 11, 7 + 13)
+    G = 5 if route == "larger_q_box" else 4
+    glo = tuple(l - G for l in lo)
+    ghi = tuple(h + G for h in hi)
+    U = _box_state(glo, ghi, seed=13)
+    dx = (1 / 64,) * 3
+    prob = Problem(domlo=lo, domhi=hi, dx=dx, bclo=(0, 0, 0), bchi=(0, 0, 0),
+                   is_periodic=(0, 0, 0), boxes=[lo + hi], params=dict(ppm_type=1))
+    hp = hydro_params(prob)
+    op = _oracle_params(oracle, prob)
+    dt = 0.3 * dx[0]
+    q, qa, srcq, dsdt, flx = _oracle_umdrv(oracle, U, glo, ghi, lo, hi, op, dt, None)
+    from camr_amd.hydro import hydro_ctoprim, hydro_umdrv
+    dev = torch.device("cuda:0")
+    tU = torch.from_numpy(U).to(dev)
+    tq = torch.zeros((11,) + U.shape[1:], dtype=torch.float64, device=dev)
+    tqa = torch.zeros((5,) + U.shape[1:], dtype=torch.float64, device=dev)
+    hydro_ctoprim(glo, ghi, tU, glo, tq, tqa, hp)
+    area = vol = None
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    if route == "area_vol_arrays":
+        area = []
+        for d in range(3):
+            sh = [1, n[2], n[1], n[0]]
+            sh[3 - d] += 1
+            a = dx[(d + 1) % 3] * dx[(d + 2) % 3] if d != 1 else dx[0] * dx[2]
+            area.append(torch.full(sh, a, dtype=torch.float64, device=dev))
+        vol = torch.full([1, n[2], n[1], n[0]], dx[0] * dx[1] * dx[2], dtype=torch.float64,
+                         device=dev)
+    tds = torch.zeros(dsdt.shape, dtype=torch.float64, device=dev)
+    tfl = [torch.full(f.shape, float("nan"), dtype=torch.float64, device=dev) for f in flx]
+    # q and qaux start at lo - 4 for the call: pass the grow(bx,4) window
+    # unless the route under test is the larger box
+    if route == "larger_q_box":
+        from camr_amd import _lib as LL
+        from camr_amd.hydro import array4_of, _box, _stream
+        import ctypes as CC
+        fl = (LL.Array4 * 3)(*[array4_of(f, lo) for f in tfl])
+        ar = (LL.Array4 * 3)(*[LL.Array4() for _ in range(3)])
+        LL.check(LL.lib().camr_hydro_umdrv(
+            CC.byref(_box(lo, hi)), CC.byref(array4_of(tU, glo)), CC.byref(array4_of(tds, lo)),
+            CC.byref(array4_of(tq, glo)), CC.byref(array4_of(tqa, glo)), CC.byref(LL.Array4()),
+            fl, ar, CC.byref(LL.Array4()), dt, CC.byref(hp), _stream(0)))
+    else:
+        hydro_umdrv(lo, hi, tU, glo, tds, tq, tqa, None, tfl, dt, hp, area=area, vol=vol)
+    torch.cuda.synchronize()
+    for d in range(3):
+        assert_bitwise(tfl[d].cpu().numpy(), flx[d], f"flux {d}")
+    assert_bitwise(tds.cpu().numpy(), dsdt, "dsdt")
+
+
 def test_estdt_matches_oracle(oracle):
     lo, hi = (0, 0, 0), (20, 17, 9)
     U = _box_state(lo, hi, seed=5)
