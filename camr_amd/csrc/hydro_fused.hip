@@ -441,21 +441,22 @@ k_trans2(Batch B, HydroConst hc, const double *dtp) {
 // instead of a second load of the plane below.  Same per-point arithmetic
 // as k_trans1<2> / k_trans2<2, S>, hence the same results.
 
-// adjust_face<2> (hydro_artif_visc, hydro_norm_spec_flx, hydro_ext_flx) on
-// values: d4 the face's four nodal divergences in the order (i,j), (i+1,j),
-// (i,j+1), (i+1,j+1); uc/um the conserved state of the cells above and
-// below the face.
-__device__ __forceinline__ void adjust_zface_vals(double *f, const double *d4,
-                                                  const double *uc, const double *um, int k,
-                                                  double area, const HydroConst &hc) {
+// adjust_face<D> (hydro_artif_visc, hydro_norm_spec_flx, hydro_ext_flx) on
+// values: d4 the face's four nodal divergences in adjust_face's order (the
+// face's node, +e_T0, +e_T1, +e_T0+e_T1); uc/um the conserved state of the
+// cells above and below the face.
+template <int D>
+__device__ __forceinline__ void adjust_face_vals(double *f, const double *d4,
+                                                 const double *uc, const double *um, int idx,
+                                                 double area, const HydroConst &hc) {
     const double div = hc.difmag * vmin(0.0, 0.25 * (d4[0] + d4[1] + d4[2] + d4[3]));
-    const bool at_bndry = (k == hc.domlo[2] && hc.bclo[2] == BC_NOSLIPWALL) ||
-                          (k == hc.domhi[2] + 1 && hc.bchi[2] == BC_NOSLIPWALL);
-    const double dxd = hc.dx[2];
+    const bool at_bndry = (idx == hc.domlo[D] && hc.bclo[D] == BC_NOSLIPWALL) ||
+                          (idx == hc.domhi[D] + 1 && hc.bchi[D] == BC_NOSLIPWALL);
+    const double dxd = hc.dx[D];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
         if (n == UTEMP) continue;
-        const bool tang = (n == UMX || n == UMY);
+        const bool tang = (n == UMX || n == UMY || n == UMZ) && (n != UMX + D);
         if (!(at_bndry && tang)) f[n] += dxd * div * (uc[n] - um[n]);
     }
     f[UTEMP] = 0.0;
@@ -612,7 +613,7 @@ k_trans2_zm(Batch B, HydroConst hc, const double *dtp) {
         d4[1] = dv.ld(v + sx, 0);
         d4[2] = dv.ld(v + sy, 0);
         d4[3] = dv.ld(v + sx + sy, 0);
-        adjust_zface_vals(fu, d4, up, um, k, area, hc);
+        adjust_face_vals<2>(fu, d4, up, um, k, area, hc);
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) flx.st(v, n, fu[n]);
         flx.st(v, XGU, g[GUN]);
@@ -625,6 +626,186 @@ k_trans2_zm(Batch B, HydroConst hc, const double *dtp) {
         cm = a.c_r;
         csm = a.csml_r;
     }
+}
+
+// ------------------------------------------------ x-face kernels, lane-shared
+// A wavefront covers consecutive x points of a row, so the cell left of
+// lane l's x-face is the cell right of lane l-1's: its transverse
+// differences, sound speeds, sources and state come from lane l-1 through a
+// cross-lane move instead of a second load.  The first lane of a wavefront
+// and the first face of a row load them.  Same per-point arithmetic as
+// k_trans1<0> / k_trans2<0, S>, hence the same results.
+
+__device__ __forceinline__ double lane_up(double x) { return __shfl_up(x, 1, 64); }
+
+// True when lane l-1 holds the point (i-1, j, k): the blocks take
+// consecutive x points (linear blocks, or the rows of a tile of width
+// 2^lx >= 8), except at a wavefront's first lane, a row's first point and
+// a tile row's first point.
+__device__ __forceinline__ bool left_in_lane(const Batch &B, int i, int lo0) {
+    const int t = (int)threadIdx.x;
+    if ((t & 63) == 0 || i == lo0) return false;
+    return B.lx < 0 || (t & ((1 << B.lx) - 1)) != 0;
+}
+
+__device__ __forceinline__ void lane_up_td1(const TransDiff &a, TransDiff &b) {
+    b.rho = lane_up(a.rho);
+    b.mu = lane_up(a.mu);
+    b.mv = lane_up(a.mv);
+    b.mw = lane_up(a.mw);
+    b.e = lane_up(a.e);
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) b.y[s] = lane_up(a.y[s]);
+    b.dAup = lane_up(a.dAup);
+    b.pav = lane_up(a.pav);
+    b.dAu = lane_up(a.dAu);
+}
+// the fields transverse_two_d reads: the flux differences and the two
+// pressure corrections (dAup holds the first, pnew the second)
+__device__ __forceinline__ void lane_up_td2(const TransDiff &a, TransDiff &b) {
+    b.rho = lane_up(a.rho);
+    b.mu = lane_up(a.mu);
+    b.mv = lane_up(a.mv);
+    b.mw = lane_up(a.mw);
+    b.e = lane_up(a.e);
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) b.y[s] = lane_up(a.y[s]);
+    b.dAup = lane_up(a.dAup);
+    b.pnew = lane_up(a.pnew);
+}
+
+__global__ void TRANS_BOUNDS
+k_trans1_x(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_T1 + 0)
+    const double cda = 1.0 / 3.0 * *dtp / hc.dx[1];
+    const double cdb = 1.0 / 3.0 * *dtp / hc.dx[2];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC);
+    const CView qm = cview(L, slot_ptr<S_QM0 + 0>(E), NS), qp = cview(L, slot_ptr<S_QP0 + 0>(E), NS);
+    const CView Fy = cview(L, slot_ptr<S_F1 + 1>(E), NTF), Fz = cview(L, slot_ptr<S_F1 + 2>(E), NTF);
+    const CView Oy = cview(L, slot_ptr<f2_slot(0, 1)>(E), NTF);
+    const CView Oz = cview(L, slot_ptr<f2_slot(0, 2)>(E), NTF);
+    const unsigned v0 = L.off(i, j, k), vm = v0 - 8u;
+    const bool own = !left_in_lane(B, i, stage_box(ST_T1 + 0, E.bx).lo[0]);
+    double np[NS], nm[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        np[s] = qp.ld(v0, s);
+        nm[s] = qm.ld(v0, s);
+    }
+    FaceAux a;
+    a.c_r = q.ld(v0, SC);
+    a.csml_r = q.ld(v0, SCS);
+    a.c_l = lane_up(a.c_r);
+    a.csml_l = lane_up(a.csml_r);
+    if (own) {
+        a.c_l = q.ld(vm, SC);
+        a.csml_l = q.ld(vm, SCS);
+    }
+    a.gam_l = kGamma;
+    a.gam_r = kGamma;
+    {
+        double sp[NS], sm[NS], f[NF], g[NG];
+        int mut;
+        TransDiff tp, tm;
+        trans1_diff<1>(Fy, v0, L, cda, tp);
+        lane_up_td1(tp, tm);
+        if (own) trans1_diff<1>(Fy, vm, L, cda, tm);
+        transverse_one_d(np, tp, cda, kGamma, hc.trd, hc.small_pres, sp);
+        transverse_one_d(nm, tm, cda, kGamma, hc.trd, hc.small_pres, sm);
+        face_flux(0, i, hc, sm, sp, a, f, g, &mut);
+        store_tflux(Oy, v0, f, g);
+    }
+    {
+        double sp[NS], sm[NS], f[NF], g[NG];
+        int mut;
+        TransDiff tp, tm;
+        trans1_diff<2>(Fz, v0, L, cdb, tp);
+        lane_up_td1(tp, tm);
+        if (own) trans1_diff<2>(Fz, vm, L, cdb, tm);
+        transverse_one_d(np, tp, cdb, kGamma, hc.trd, hc.small_pres, sp);
+        transverse_one_d(nm, tm, cdb, kGamma, hc.trd, hc.small_pres, sm);
+        face_flux(0, i, hc, sm, sp, a, f, g, &mut);
+        store_tflux(Oz, v0, f, g);
+    }
+}
+
+template <int HAS_SRC>
+__global__ void TRANS_BOUNDS
+k_trans2_x(Batch B, HydroConst hc, const double *dtp) {
+    BATCH_POINT(ST_T2 + 0)
+    const double dt = *dtp;
+    const double hdt = 0.5 * dt;
+    const double ca = 0.5 * dt / hc.dx[1], cb = 0.5 * dt / hc.dx[2];
+    const double area = hc.dx[1] * hc.dx[2];
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+    const CView qm = cview(L, slot_ptr<S_QM0 + 0>(E), NS), qp = cview(L, slot_ptr<S_QP0 + 0>(E), NS);
+    const CView Fa = cview(L, slot_ptr<f2_slot(1, 2)>(E), NTF);
+    const CView Fb = cview(L, slot_ptr<f2_slot(2, 1)>(E), NTF);
+    const CView dv = cview(L, slot_ptr<S_DIVU>(E), 1), uc = cview(L, slot_ptr<S_U>(E), NVAR);
+    const unsigned v0 = L.off(i, j, k), vm = v0 - 8u, sy = L.sy(), sz = L.sz();
+    const bool own = !left_in_lane(B, i, stage_box(ST_T2 + 0, E.bx).lo[0]);
+    double sp[NS], sm[NS];
+    {
+        double n[NS], srcp[NS], srcm[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            srcp[s] = HAS_SRC ? sq.ld(v0, s) : 0.0;
+            srcm[s] = HAS_SRC ? lane_up(srcp[s]) : 0.0;
+        }
+        if (HAS_SRC && own) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) srcm[s] = sq.ld(vm, s);
+        }
+        TransDiff tp, tm;
+        trans2_diff<0>(v0, L, Fa, Fb, ca, cb, tp);
+        lane_up_td2(tp, tm);
+        if (own) trans2_diff<0>(vm, L, Fa, Fb, ca, cb, tm);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) n[s] = qp.ld(v0, s);
+        transverse_two_d(n, tp, hdt, srcp, hc.trd, hc.small_pres, sp);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) n[s] = qm.ld(v0, s);
+        transverse_two_d(n, tm, hdt, srcm, hc.trd, hc.small_pres, sm);
+    }
+    FaceAux a;
+    a.c_r = q.ld(v0, SC);
+    a.csml_r = q.ld(v0, SCS);
+    a.c_l = lane_up(a.c_r);
+    a.csml_l = lane_up(a.csml_r);
+    if (own) {
+        a.c_l = q.ld(vm, SC);
+        a.csml_l = q.ld(vm, SCS);
+    }
+    a.gam_l = kGamma;
+    a.gam_r = kGamma;
+    double f[NF], g[NG];
+    int mut;
+    face_flux(0, i, hc, sm, sp, a, f, g, &mut);
+    double fu[NVAR];
+    fu[URHO] = f[FR]; fu[UMX] = f[FMX]; fu[UMY] = f[FMY]; fu[UMZ] = f[FMZ];
+    fu[UEDEN] = f[FE]; fu[UEINT] = f[FEI]; fu[UTEMP] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) fu[UFS + s] = f[FY + s];
+    double up[NVAR], um[NVAR], d4[4];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        up[n] = uc.ld(v0, n);
+        um[n] = lane_up(up[n]);
+    }
+    if (own) {
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) um[n] = uc.ld(vm, n);
+    }
+    d4[0] = dv.ld(v0, 0);
+    d4[1] = dv.ld(v0 + sy, 0);
+    d4[2] = dv.ld(v0 + sz, 0);
+    d4[3] = dv.ld(v0 + sy + sz, 0);
+    adjust_face_vals<0>(fu, d4, up, um, i, area, hc);
+    const CView flx = cview(L, slot_ptr<S_FLX + 0>(E), NXF);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) flx.st(v0, n, fu[n]);
+    flx.st(v0, XGU, g[GUN]);
+    flx.st(v0, XGP, g[GP]);
 }
 
 // ------------------------------------------------ one-box route (per-box ABI)
@@ -953,7 +1134,7 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
     launch("riemann_normal", prof, k_riemann1<0>, ST_R1 + 0, b, parity, st, hc);
     launch("riemann_normal", prof, k_riemann1<1>, ST_R1 + 1, b, parity, st, hc);
     launch("riemann_normal", prof, k_riemann1<2>, ST_R1 + 2, b, parity, st, hc);
-    launch("transverse_1", prof, k_trans1<0>, ST_T1 + 0, b, parity, st, hc, dtp);
+    launch("transverse_1", prof, k_trans1_x, ST_T1 + 0, b, parity, st, hc, dtp);
     launch("transverse_1", prof, k_trans1<1>, ST_T1 + 1, b, parity, st, hc, dtp);
     if (tune_.zm[ST_T1 + 2] > 0)
         launch("transverse_1", prof, k_trans1_zm, ST_T1 + 2, b, parity, st, hc, dtp);
@@ -962,14 +1143,14 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
     launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
     const bool zm2 = tune_.zm[ST_T2 + 2] > 0;
     if (has_src) {
-        launch("transverse_2", prof, k_trans2<0, 1>, ST_T2 + 0, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2_x<1>, ST_T2 + 0, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<1, 1>, ST_T2 + 1, b, parity, st, hc, dtp);
         if (zm2)
             launch("transverse_2", prof, k_trans2_zm<1>, ST_T2 + 2, b, parity, st, hc, dtp);
         else
             launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
     } else {
-        launch("transverse_2", prof, k_trans2<0, 0>, ST_T2 + 0, b, parity, st, hc, dtp);
+        launch("transverse_2", prof, k_trans2_x<0>, ST_T2 + 0, b, parity, st, hc, dtp);
         launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
         if (zm2)
             launch("transverse_2", prof, k_trans2_zm<0>, ST_T2 + 2, b, parity, st, hc, dtp);
@@ -1068,16 +1249,16 @@ int fused_umdrv(const UmdrvArgs &A, double *scratch, hipStream_t st) {
     launch_one(k_riemann1<0>, ST_R1 + 0, bx, B, st, hc);
     launch_one(k_riemann1<1>, ST_R1 + 1, bx, B, st, hc);
     launch_one(k_riemann1<2>, ST_R1 + 2, bx, B, st, hc);
-    launch_one(k_trans1<0>, ST_T1 + 0, bx, B, st, hc, dtp);
+    launch_one(k_trans1_x, ST_T1 + 0, bx, B, st, hc, dtp);
     launch_one(k_trans1<1>, ST_T1 + 1, bx, B, st, hc, dtp);
     launch_one(k_trans1<2>, ST_T1 + 2, bx, B, st, hc, dtp);
     launch_one(k_divu_c, ST_DIVU, bx, B, st, hc);
     if (has_src) {
-        launch_one(k_trans2<0, 1>, ST_T2 + 0, bx, B, st, hc, dtp);
+        launch_one(k_trans2_x<1>, ST_T2 + 0, bx, B, st, hc, dtp);
         launch_one(k_trans2<1, 1>, ST_T2 + 1, bx, B, st, hc, dtp);
         launch_one(k_trans2<2, 1>, ST_T2 + 2, bx, B, st, hc, dtp);
     } else {
-        launch_one(k_trans2<0, 0>, ST_T2 + 0, bx, B, st, hc, dtp);
+        launch_one(k_trans2_x<0>, ST_T2 + 0, bx, B, st, hc, dtp);
         launch_one(k_trans2<1, 0>, ST_T2 + 1, bx, B, st, hc, dtp);
         launch_one(k_trans2<2, 0>, ST_T2 + 2, bx, B, st, hc, dtp);
     }
