@@ -99,7 +99,8 @@ EXPORTS = [
     "camr_level_time", "camr_level_profile", "camr_level_profile_report",
     "camr_level_set_exterior", "camr_level_set_tuning", "camr_level_get_tuning",
     "camr_level_set_tiles", "camr_level_get_tiles", "camr_level_set_xcd", "camr_level_get_xcd",
-    "camr_level_set_zmarch", "camr_level_get_zmarch",
+    "camr_level_set_zmarch", "camr_level_get_zmarch", "camr_level_set_ctu",
+    "camr_level_get_ctu",
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
     "camr_level_adjoint_backward", "camr_hydro_umdrv_eb", "camr_level_set_eb",
     "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
@@ -172,6 +173,8 @@ def lib():
         L.camr_level_get_xcd.argtypes = [vp, C.POINTER(C.c_int32)]
         L.camr_level_set_zmarch.argtypes = [vp, C.POINTER(C.c_int32)]
         L.camr_level_get_zmarch.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.camr_level_set_ctu.argtypes = [vp, C.c_int32, C.c_int32]
+        L.camr_level_get_ctu.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.camr_level_adjoint_forward.argtypes = [vp, C.c_int32, dp, C.c_int32]
         L.camr_level_adj_upload_box.argtypes = [vp, C.c_int32, dp]
         L.camr_level_adj_download_box.argtypes = [vp, C.c_int32, dp]

@@ -15,7 +15,9 @@ What is tuned (``Level.tuning`` and ``Level.tiles``):
   L2 caches; with chunk K every XCD takes K consecutive blocks of each
   window of 8K;
 * the order of the boxes in the batched launches (level order, or largest
-  box first so that the longest boxes start first and the tail is short).
+  box first so that the longest boxes start first and the tail is short);
+* the transverse stages as the staged transdo / transdd launches or as the
+  LDS-staged CTU face kernels (``Level.ctu``), a whole-step choice.
 
 How: every stage kernel is timed with the level's hipEvent profiler over a
 few steps for each candidate block (one run per candidate covers every
@@ -100,20 +102,27 @@ def _step_time(level, steps: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def apply(level, blocks, order, tiles, xcd=None):
-    """Set per-stage blocks (linear block sizes or tiles), XCD chunks and
-    the box order."""
+CTU_PLANES = 8
+
+
+def apply(level, blocks, order, tiles, xcd=None, ctu=None):
+    """Set per-stage blocks (linear block sizes or tiles), XCD chunks, the
+    box order and the CTU switch; unnamed settings keep the library's
+    defaults."""
     level.tuning = ([b if t == LINEAR else 256 for b, t in zip(blocks, tiles)], order)
     level.tiles = tiles
-    level.xcd = xcd if xcd is not None else [0] * _lib.NSTAGE
+    if xcd is not None:
+        level.xcd = xcd
+    if ctu is not None:
+        level.ctu = tuple(ctu)
 
 
 def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
     """Choose and apply the launch shapes of `level` (which must hold a
     valid state; it is advanced by the timing runs).  Returns the tuning
-    (blocks, order, tiles, xcd) now set on the level: per stage range its
-    block size, its tile ((0, 0, 0) for linear blocks) and its XCD chunk,
-    and the box order."""
+    (blocks, order, tiles, xcd, ctu) now set on the level: per stage range
+    its block size, its tile ((0, 0, 0) for linear blocks) and its XCD
+    chunk, the box order and the CTU switch (on, planes)."""
     device = _device_name()
     key = key_of(level, device)
     path = cache_path()
@@ -124,20 +133,23 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if key in cache and "xcd" in cache[key]:
+    if key in cache and "ctu" in cache[key]:
         t = cache[key]
         tiles = [tuple(x) for x in t["tiles"]]
-        apply(level, t["blocks"], t["order"], tiles, t["xcd"])
-        return list(t["blocks"]), t["order"], tiles, list(t["xcd"])
+        ctu = tuple(t["ctu"])
+        apply(level, t["blocks"], t["order"], tiles, t["xcd"], ctu)
+        return list(t["blocks"]), t["order"], tiles, list(t["xcd"]), ctu
 
-    default = (list(level.tuning[0]), level.tuning[1], list(level.tiles), list(level.xcd))
+    default = (list(level.tuning[0]), level.tuning[1], list(level.tiles), list(level.xcd),
+               tuple(level.ctu))
     level.run(1)  # warm-up (graph capture, first launches)
     # per-stage times for each uniform candidate, both box orders
     times = {}
     for order in (0, 1):
         for cand in CANDIDATES:
             bs, tile, xk = cand
-            apply(level, [bs] * _lib.NSTAGE, order, [tile] * _lib.NSTAGE, [xk] * _lib.NSTAGE)
+            apply(level, [bs] * _lib.NSTAGE, order, [tile] * _lib.NSTAGE, [xk] * _lib.NSTAGE,
+                  (0, CTU_PLANES))
             times[(order, cand)] = _stage_times(level, steps)
     results = []
     for order in (0, 1):
@@ -150,22 +162,24 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
             pick = min(cost, key=cost.get)
             for g in grp:
                 blocks[g], tiles[g], xcd[g] = pick
-        apply(level, blocks, order, tiles, xcd)
-        results.append((_step_time(level, steps), blocks, order, tiles, xcd))
+        for ctu in ((0, CTU_PLANES), (1, CTU_PLANES)):
+            apply(level, blocks, order, tiles, xcd, ctu)
+            results.append((_step_time(level, steps), blocks, order, tiles, xcd, ctu))
     apply(level, *default)
     t_def = _step_time(level, steps)
     results.sort(key=lambda r: r[0])
-    t_best, blocks, order, tiles, xcd = results[0]
+    t_best, blocks, order, tiles, xcd, ctu = results[0]
     if t_best >= t_def:
-        blocks, order, tiles, xcd = default
+        blocks, order, tiles, xcd, ctu = default
         t_best = t_def
-    apply(level, blocks, order, tiles, xcd)
+    apply(level, blocks, order, tiles, xcd, ctu)
     if verbose:
         print(f"autotune {key}: default {t_def * 1e3:.3f} ms/step, chosen "
               f"{t_best * 1e3:.3f} ms/step, blocks {blocks}, tiles {tiles}, xcd {xcd}, "
-              f"order {order}", flush=True)
+              f"order {order}, ctu {ctu}", flush=True)
     cache[key] = {"blocks": list(map(int, blocks)), "order": int(order),
                   "tiles": [list(map(int, t)) for t in tiles], "xcd": list(map(int, xcd)),
+                  "ctu": list(map(int, ctu)),
                   "ms_per_step": t_best * 1e3, "default_ms_per_step": t_def * 1e3}
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -173,4 +187,4 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
             json.dump(cache, f, indent=1, sort_keys=True)
     except OSError:
         pass
-    return list(blocks), order, [tuple(t) for t in tiles], list(xcd)
+    return list(blocks), order, [tuple(t) for t in tiles], list(xcd), tuple(ctu)

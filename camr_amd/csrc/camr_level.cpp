@@ -3069,6 +3069,7 @@ int camr_level_set_tiles(camr_level *L, const int32_t tile[3 * CAMR_NSTAGE]) {
             t.tile[k][d] = l[d];
         }
         t.block[k] = x[0] * x[1] * x[2];
+        t.zm[k] = 0;  // z-march takes linear blocks
     }
     return L->impl.set_tuning(t);
 }
@@ -3095,6 +3096,24 @@ int camr_level_set_zmarch(camr_level *L, const int32_t planes[CAMR_NSTAGE]) {
     if (rc) return rc;
     for (int k = 0; k < ST_COUNT; ++k) t.zm[k] = planes[k];
     return L->impl.set_tuning(t);
+}
+int camr_level_set_ctu(camr_level *L, int32_t on, int32_t planes) {
+    if (!L) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    t.ctu = on;
+    t.ctu_planes = planes;
+    return L->impl.set_tuning(t);
+}
+int camr_level_get_ctu(camr_level *L, int32_t *on, int32_t *planes) {
+    if (!L || !on || !planes) { set_error("null argument"); return CAMR_ERR_PARAM; }
+    Tuning t;
+    int rc = L->impl.get_tuning(&t);
+    if (rc) return rc;
+    *on = t.ctu;
+    *planes = t.ctu_planes;
+    return CAMR_OK;
 }
 int camr_level_get_zmarch(camr_level *L, int32_t planes[CAMR_NSTAGE]) {
     if (!L || !planes) { set_error("null argument"); return CAMR_ERR_PARAM; }

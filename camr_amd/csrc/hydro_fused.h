@@ -226,6 +226,41 @@ __device__ __forceinline__ bool batch_column(const Batch &B, const Box &sb, int 
     return true;
 }
 
+// Tiles of the LDS-staged CTU face kernels (k_ctu_x/y/z): kCtuX x kCtuY
+// points of the x-y plane per block (two x rows per wavefront), each block
+// walking `planes` z planes.  With merge (the x-face stage), a single point
+// left over past the full x tiles joins the last full tile, which then is
+// kCtuX + 1 points wide: the one face past a box's last cell would
+// otherwise take a whole block.
+constexpr int kCtuX = 32, kCtuY = 8, kCtuBlock = kCtuX * kCtuY;
+__host__ __device__ inline int ctu_ntx(int n, bool merge) {
+    return (merge && n > kCtuX && n % kCtuX == 1) ? n / kCtuX : (n + kCtuX - 1) / kCtuX;
+}
+__host__ __device__ inline int64_t ctu_blocks(const Box &sb, int planes, bool merge) {
+    return (int64_t)ctu_ntx(sb.len(0), merge) * ((sb.len(1) + kCtuY - 1) / kCtuY) *
+           ((sb.len(2) + planes - 1) / planes);
+}
+struct CtuTile {
+    int i0, j0;  // first point of the tile
+    int k0, k1;  // the block's planes
+    int wx;      // points in x: kCtuX, fewer in a partial tile, kCtuX + 1 merged
+};
+__device__ __forceinline__ CtuTile ctu_tile(const Box &sb, int lblk, int planes, bool merge) {
+    const int n0 = sb.len(0);
+    const int ntx = ctu_ntx(n0, merge), nty = (sb.len(1) + kCtuY - 1) / kCtuY;
+    const int bz = lblk / (ntx * nty);
+    const int rem = lblk - bz * ntx * nty;
+    const int by = rem / ntx, bx = rem - by * ntx;
+    CtuTile t;
+    t.i0 = sb.lo[0] + bx * kCtuX;
+    t.j0 = sb.lo[1] + by * kCtuY;
+    t.k0 = sb.lo[2] + bz * planes;
+    t.k1 = t.k0 + planes - 1 < sb.hi[2] ? t.k0 + planes - 1 : sb.hi[2];
+    const int rest = n0 - bx * kCtuX;
+    t.wx = (bx == ntx - 1) ? rest : kCtuX;
+    return t;
+}
+
 #define BATCH_COLUMN(KIND)                                                     \
     if (B.halt && *B.halt) return;                                             \
     int lblk_;                                                                 \
@@ -329,6 +364,11 @@ struct Tuning {
     // z-march planes per thread (0: one point per thread); only the z-face
     // transverse stages (ST_T1 + 2, ST_T2 + 2) have marching kernels
     int zm[ST_COUNT];
+    // 1: the transverse stages run as the LDS-staged CTU face kernels
+    // (k_ctu_x/y/z, the transdo fluxes never stored); 0: the staged
+    // k_trans1 / k_trans2 launches
+    int ctu;
+    int ctu_planes;  // z planes each CTU block walks
 };
 
 class FusedEngine {
@@ -387,12 +427,18 @@ class FusedEngine {
     template <typename K, typename... A>
     void launch(const char *name, Prof *prof, K kern, int kind, int b, int parity,
                 hipStream_t st, A... args);
+    // grid of the CTU face kernel of direction d
+    LaunchCfg cfg_ctu(int d, int b, int parity) const;
+    template <typename K, typename... A>
+    void launch_ctu(const char *name, Prof *prof, K kern, int d, int b, int parity,
+                    hipStream_t st, A... args);
     int build_tables();
     std::vector<Box> boxes_;
     std::vector<Arr> bufA_, bufB_;
     std::vector<size_t> off_;      // scratch offset of each box (doubles)
     std::vector<int> order_;       // box order of the tables
-    std::vector<int> blk_host_;    // 2 x ST_COUNT x (nbox + 1): as tuned, without z-march
+    // (2 x ST_COUNT + 3) x (nbox + 1): as tuned, without z-march, CTU tiles
+    std::vector<int> blk_host_;
     std::vector<int> pos_;         // table position of box b
     Tuning tune_{};
     BoxEnt *tab_[2] = {nullptr, nullptr};

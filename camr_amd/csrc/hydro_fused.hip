@@ -808,6 +808,433 @@ k_trans2_x(Batch B, HydroConst hc, const double *dtp) {
     flx.st(v0, XGP, g[GP]);
 }
 
+// ------------------------------------------------ LDS-staged CTU face kernels
+// k_ctu_x / k_ctu_y / k_ctu_z: the transdo pairs, transdd and the final
+// Riemann solve of one face direction D in one kernel, so that the
+// transdo-corrected fluxes (the S_F2 slots) never go to memory.  A block
+// owns a tile of kCtuX x kCtuY points of the x-y plane and walks B.zm
+// planes in z.  Per plane:
+//   1. each thread solves the transdo pairs on the low faces of its cell:
+//      the in-plane transverse direction into LDS, the z direction in
+//      registers (face k+1 of one plane is face k of the next); the faces
+//      on the tile's far edge and the halo cells' faces go to a few
+//      designated threads;
+//   2. after a barrier, each thread forms the transdd differences of its
+//      cell from the pair fluxes on the cell's four transverse faces;
+//   3. after a barrier, the D-face between the thread's cell and the cell
+//      below it in D (LDS for x and y, the previous plane for z) is finished
+//      as in k_trans2: transdd states, Riemann solve, adjust_fluxes, store.
+// Every value is the same per-point arithmetic as k_trans1 + k_trans2 on
+// the same inputs, hence the same results.
+
+// transdo pair D|T at the D-face of offset v (index idx along D): the
+// arithmetic of trans1_pair without the store; f/g the compact flux
+__device__ __forceinline__ void rec_put(double *s, int n, int p, const double *f,
+                                        const double *g) {
+    s[TR * n + p] = f[FR];
+    s[TMX * n + p] = f[FMX];
+    s[TMY * n + p] = f[FMY];
+    s[TMZ * n + p] = f[FMZ];
+    s[TE * n + p] = f[FE];
+#pragma unroll
+    for (int c = 0; c < NSPEC; ++c) s[(TY + c) * n + p] = f[FY + c];
+    s[TGU * n + p] = g[GUN];
+    s[TGP * n + p] = g[GP];
+}
+__device__ __forceinline__ void rec_get(const double *s, int n, int p, double *f, double *g) {
+    f[FR] = s[TR * n + p];
+    f[FMX] = s[TMX * n + p];
+    f[FMY] = s[TMY * n + p];
+    f[FMZ] = s[TMZ * n + p];
+    f[FE] = s[TE * n + p];
+    f[FEI] = 0.0;
+#pragma unroll
+    for (int c = 0; c < NSPEC; ++c) f[FY + c] = s[(TY + c) * n + p];
+    g[GUN] = s[TGU * n + p];
+    g[GP] = s[TGP * n + p];
+}
+// the transdd fields of a TransDiff (what transverse_two_d reads)
+enum { NTD = 10 };
+__device__ __forceinline__ void td_put(double *s, int n, int p, const TransDiff &t) {
+    s[0 * n + p] = t.rho;
+    s[1 * n + p] = t.mu;
+    s[2 * n + p] = t.mv;
+    s[3 * n + p] = t.mw;
+    s[4 * n + p] = t.e;
+#pragma unroll
+    for (int c = 0; c < NSPEC; ++c) s[(5 + c) * n + p] = t.y[c];
+    s[8 * n + p] = t.dAup;
+    s[9 * n + p] = t.pnew;
+}
+__device__ __forceinline__ void td_get(const double *s, int n, int p, TransDiff &t) {
+    t.rho = s[0 * n + p];
+    t.mu = s[1 * n + p];
+    t.mv = s[2 * n + p];
+    t.mw = s[3 * n + p];
+    t.e = s[4 * n + p];
+#pragma unroll
+    for (int c = 0; c < NSPEC; ++c) t.y[c] = s[(5 + c) * n + p];
+    t.dAup = s[8 * n + p];
+    t.pnew = s[9 * n + p];
+}
+
+template <int D, int T>
+__device__ __forceinline__ void ctu_pair(const BoxEnt &E, const Canon &L, unsigned v, int idx,
+                                         double cdt, const HydroConst &hc, double *f,
+                                         double *g) {
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const CView F = cview(L, slot_ptr<S_F1 + T>(E), NTF);
+    const unsigned sd = shift<D>(L);
+    double sp[NS], sm[NS];
+    {
+        double n[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) n[s] = qp.ld(v, s);
+        TransDiff t;
+        trans1_diff<T>(F, v, L, cdt, t);
+        transverse_one_d(n, t, cdt, kGamma, hc.trd, hc.small_pres, sp);
+    }
+    {
+        double n[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) n[s] = qm.ld(v, s);
+        TransDiff t;
+        trans1_diff<T>(F, v - sd, L, cdt, t);
+        transverse_one_d(n, t, cdt, kGamma, hc.trd, hc.small_pres, sm);
+    }
+    const FaceAux a = face_aux(q, v - sd, v);
+    int mut;
+    face_flux(D, idx, hc, sm, sp, a, f, g, &mut);
+}
+
+// transdd states of the D-face at offset v of point (i,j,k) from the
+// differences tp of the cell above the face and tm of the cell below it,
+// then the Riemann solve, adjust_fluxes and the store of the final flux
+// (the tail of k_trans2)
+template <int D, int HAS_SRC>
+__device__ __forceinline__ void ctu_face(const BoxEnt &E, const Canon &L, unsigned v, int i,
+                                         int j, int k, const TransDiff &tp,
+                                         const TransDiff &tm, double hdt, double area,
+                                         const HydroConst &hc) {
+    const CView q = cview(L, slot_ptr<S_Q>(E), NQC), sq = cview(L, slot_ptr<S_SRCQ>(E), NS);
+    const CView qm = cview(L, slot_ptr<S_QM0 + D>(E), NS), qp = cview(L, slot_ptr<S_QP0 + D>(E), NS);
+    const unsigned sd = shift<D>(L);
+    double sp[NS], sm[NS];
+    {
+        double n[NS], src[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            n[s] = qp.ld(v, s);
+            src[s] = HAS_SRC ? sq.ld(v, s) : 0.0;
+        }
+        transverse_two_d(n, tp, hdt, src, hc.trd, hc.small_pres, sp);
+    }
+    {
+        double n[NS], src[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            n[s] = qm.ld(v, s);
+            src[s] = HAS_SRC ? sq.ld(v - sd, s) : 0.0;
+        }
+        transverse_two_d(n, tm, hdt, src, hc.trd, hc.small_pres, sm);
+    }
+    const FaceAux a = face_aux(q, v - sd, v);
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    double f[NF], g[NG];
+    int mut;
+    face_flux(D, idx, hc, sm, sp, a, f, g, &mut);
+    double fu[NVAR];
+    fu[URHO] = f[FR]; fu[UMX] = f[FMX]; fu[UMY] = f[FMY]; fu[UMZ] = f[FMZ];
+    fu[UEDEN] = f[FE]; fu[UEINT] = f[FEI]; fu[UTEMP] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) fu[UFS + s] = f[FY + s];
+    adjust_face<D>(fu, cview(L, slot_ptr<S_DIVU>(E), 1), cview(L, slot_ptr<S_U>(E), NVAR), i, j,
+                   k, area, hc);
+    const CView flx = cview(L, slot_ptr<S_FLX + D>(E), NXF);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) flx.st(v, n, fu[n]);
+    flx.st(v, XGU, g[GUN]);
+    flx.st(v, XGP, g[GP]);
+}
+
+#ifndef CAMR_CTU_MINW
+#define CAMR_CTU_MINW 3
+#endif
+#define CTU_BOUNDS __launch_bounds__(kCtuBlock, CAMR_CTU_MINW)
+
+// First statements of the three kernels: the block's box, stage range sb
+// and tile T; (tx, ty) the thread's point in the tile.
+#define CTU_TILE(D)                                                            \
+    if (B.halt && *B.halt) return;                                             \
+    int lblk_;                                                                 \
+    const BoxEnt &E = B.tab[batch_box(B, lblk_)];                              \
+    const Canon L = E.L;                                                       \
+    const Box sb = stage_box(ST_T2 + (D), E.bx);                               \
+    const CtuTile T = ctu_tile(sb, lblk_, B.zm, (D) == 0);                     \
+    const int t = (int)threadIdx.x, tx = t % kCtuX, ty = t / kCtuX;            \
+    const int i = T.i0 + tx, j = T.j0 + ty;                                    \
+    const int hx = sb.hi[0], hy = sb.hi[1];
+
+// z-faces.  Pair records: x|y on x-faces (columns 0..kCtuX of each row),
+// y|x on y-faces (rows 0..kCtuY of each column); the transdd differences
+// of the cell below the face come from the previous plane.
+template <int HAS_SRC>
+__global__ void CTU_BOUNDS k_ctu_z(Batch B, HydroConst hc, const double *dtp) {
+    CTU_TILE(2)
+    constexpr int NA = kCtuY * (kCtuX + 1), NB = (kCtuY + 1) * kCtuX;
+    __shared__ double sA[NTF * NA], sB[NTF * NB];
+    const double dt = *dtp;
+    const double cdx = 1.0 / 3.0 * dt / hc.dx[0], cdy = 1.0 / 3.0 * dt / hc.dx[1];
+    const double ca = 0.5 * dt / hc.dx[0], cb = 0.5 * dt / hc.dx[1];
+    const double hdt = 0.5 * dt, area = hc.dx[0] * hc.dx[1];
+    const bool cell = i <= hx && j <= hy;
+    const bool ownA = i <= hx + 1 && j <= hy;  // x|y on the cell's low x-face
+    const bool ownB = i <= hx && j <= hy + 1;  // y|x on the cell's low y-face
+    // far edges: y|x on row kCtuY (wavefront 0), x|y on column kCtuX (wavefront 1)
+    const int eB = t, eA = t - 64;
+    const bool extB = eB < kCtuX && T.i0 + eB <= hx && T.j0 + kCtuY <= hy + 1;
+    const bool extA = eA >= 0 && eA < kCtuY && T.i0 + kCtuX <= hx + 1 && T.j0 + eA <= hy;
+    TransDiff tm = {};
+    for (int k = T.k0 - 1; k <= T.k1; ++k) {
+        if (ownA) {
+            double f[NF], g[NG];
+            ctu_pair<0, 1>(E, L, L.off(i, j, k), i, cdy, hc, f, g);
+            rec_put(sA, NA, ty * (kCtuX + 1) + tx, f, g);
+        }
+        if (ownB) {
+            double f[NF], g[NG];
+            ctu_pair<1, 0>(E, L, L.off(i, j, k), j, cdx, hc, f, g);
+            rec_put(sB, NB, ty * kCtuX + tx, f, g);
+        }
+        if (extB) {
+            double f[NF], g[NG];
+            const int jb = T.j0 + kCtuY;
+            ctu_pair<1, 0>(E, L, L.off(T.i0 + eB, jb, k), jb, cdx, hc, f, g);
+            rec_put(sB, NB, kCtuY * kCtuX + eB, f, g);
+        }
+        if (extA) {
+            double f[NF], g[NG];
+            const int ia = T.i0 + kCtuX;
+            ctu_pair<0, 1>(E, L, L.off(ia, T.j0 + eA, k), ia, cdy, hc, f, g);
+            rec_put(sA, NA, eA * (kCtuX + 1) + kCtuX, f, g);
+        }
+        __syncthreads();
+        TransDiff tp = {};
+        if (cell) {
+            double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
+            rec_get(sA, NA, ty * (kCtuX + 1) + tx, fal, gal);
+            rec_get(sA, NA, ty * (kCtuX + 1) + tx + 1, far_, gar);
+            rec_get(sB, NB, ty * kCtuX + tx, fbl, gbl);
+            rec_get(sB, NB, (ty + 1) * kCtuX + tx, fbr, gbr);
+            trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, kGamma, tp);
+        }
+        __syncthreads();
+        if (cell && k >= T.k0)
+            ctu_face<2, HAS_SRC>(E, L, L.off(i, j, k), i, j, k, tp, tm, hdt, area, hc);
+        tm = tp;
+    }
+}
+
+// x-faces.  The thread at column tx owns face i = i0 + tx and the cell
+// right of it; the cell left of the tile's first face (column 0 of the
+// LDS arrays, the cells sitting at columns 1..kCtuX) is a halo cell, and
+// a merged last tile (T.wx = kCtuX + 1) has a second one right of its
+// last cell (column kCtuX + 1) with one more face.  y|z pairs on y-faces
+// in LDS (rows 0..kCtuY), z|y pairs on z-faces in registers (halo cells:
+// in LDS, two planes), transdd differences in LDS.
+template <int HAS_SRC>
+__global__ void CTU_BOUNDS k_ctu_x(Batch B, HydroConst hc, const double *dtp) {
+    CTU_TILE(0)
+    constexpr int NC = kCtuX + 2;
+    constexpr int NY = (kCtuY + 1) * NC, NT = kCtuY * NC, NH = 2 * kCtuY;
+    __shared__ double sY[NTF * NY], sT[NTD * NT], sH[2 * NTF * NH];
+    const double dt = *dtp;
+    const double cdy = 1.0 / 3.0 * dt / hc.dx[1], cdz = 1.0 / 3.0 * dt / hc.dx[2];
+    const double ca = 0.5 * dt / hc.dx[1], cb = 0.5 * dt / hc.dx[2];
+    const double hdt = 0.5 * dt, area = hc.dx[1] * hc.dx[2];
+    const int wx = T.wx < kCtuX ? T.wx : kCtuX;  // own columns of this tile
+    const bool face = tx < wx && j <= hy;          // face (i, j) and the cell right of it
+    const bool ownY = tx < wx && j <= hy + 1;      // y|z on the cell's low y-face
+    // y|z on far or halo faces (wavefront 0): row kCtuY of the own columns,
+    // then the left halo column (rows 0..kCtuY), then the right one
+    int ye_i = 0, ye_j = 0, ye_p = 0;
+    bool extY = false;
+    if (t < kCtuX) {
+        ye_i = T.i0 + t; ye_j = T.j0 + kCtuY; ye_p = kCtuY * NC + t + 1;
+        extY = t < wx && ye_j <= hy + 1;
+    } else if (t < kCtuX + kCtuY + 1) {
+        const int r = t - kCtuX;
+        ye_i = T.i0 - 1; ye_j = T.j0 + r; ye_p = r * NC;
+        extY = ye_j <= hy + 1;
+    } else if (t < kCtuX + 2 * (kCtuY + 1)) {
+        const int r = t - kCtuX - (kCtuY + 1);
+        ye_i = T.i0 + kCtuX; ye_j = T.j0 + r; ye_p = r * NC + kCtuX + 1;
+        extY = T.wx == kCtuX + 1 && ye_j <= hy + 1;
+    }
+    // halo cells (wavefront 1): left column rows 0..kCtuY-1, then the right one
+    const int h = t - 64;
+    const bool halo = h >= 0 && h < 2 * kCtuY &&
+                      (h < kCtuY ? T.j0 + h <= hy : (T.wx == kCtuX + 1 && T.j0 + h - kCtuY <= hy));
+    const int hc_i = h < kCtuY ? T.i0 - 1 : T.i0 + kCtuX;
+    const int hc_j = T.j0 + (h < kCtuY ? h : h - kCtuY);
+    const int hc_col = h < kCtuY ? 0 : kCtuX + 1;
+    double zlo[NF], zglo[NG];
+    if (face) ctu_pair<2, 1>(E, L, L.off(i, j, T.k0), T.k0, cdy, hc, zlo, zglo);
+    if (halo) {
+        double f[NF], g[NG];
+        ctu_pair<2, 1>(E, L, L.off(hc_i, hc_j, T.k0), T.k0, cdy, hc, f, g);
+        rec_put(sH + (T.k0 & 1) * NTF * NH, NH, h, f, g);
+    }
+    for (int k = T.k0; k <= T.k1; ++k) {
+        double zhi[NF], zghi[NG];
+        if (face) ctu_pair<2, 1>(E, L, L.off(i, j, k + 1), k + 1, cdy, hc, zhi, zghi);
+        if (ownY) {
+            double f[NF], g[NG];
+            ctu_pair<1, 2>(E, L, L.off(i, j, k), j, cdz, hc, f, g);
+            rec_put(sY, NY, ty * NC + tx + 1, f, g);
+        }
+        if (extY) {
+            double f[NF], g[NG];
+            ctu_pair<1, 2>(E, L, L.off(ye_i, ye_j, k), ye_j, cdz, hc, f, g);
+            rec_put(sY, NY, ye_p, f, g);
+        }
+        if (halo) {
+            double f[NF], g[NG];
+            ctu_pair<2, 1>(E, L, L.off(hc_i, hc_j, k + 1), k + 1, cdy, hc, f, g);
+            rec_put(sH + ((k + 1) & 1) * NTF * NH, NH, h, f, g);
+        }
+        __syncthreads();
+        TransDiff tp = {};
+        if (face) {
+            double fal[NF], far_[NF], gal[NG], gar[NG];
+            rec_get(sY, NY, ty * NC + tx + 1, fal, gal);
+            rec_get(sY, NY, (ty + 1) * NC + tx + 1, far_, gar);
+            trans_diff_two(fal, far_, gal, gar, zlo, zhi, zglo, zghi, ca, cb, kGamma, tp);
+            td_put(sT, NT, ty * NC + tx + 1, tp);
+        }
+        if (halo) {
+            const int r = hc_j - T.j0;
+            double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
+            rec_get(sY, NY, r * NC + hc_col, fal, gal);
+            rec_get(sY, NY, (r + 1) * NC + hc_col, far_, gar);
+            rec_get(sH + (k & 1) * NTF * NH, NH, h, fbl, gbl);
+            rec_get(sH + ((k + 1) & 1) * NTF * NH, NH, h, fbr, gbr);
+            TransDiff th;
+            trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, kGamma, th);
+            td_put(sT, NT, r * NC + hc_col, th);
+        }
+        __syncthreads();
+        if (face) {
+            TransDiff tm;
+            td_get(sT, NT, ty * NC + tx, tm);
+            ctu_face<0, HAS_SRC>(E, L, L.off(i, j, k), i, j, k, tp, tm, hdt, area, hc);
+        }
+        if (halo && h >= kCtuY) {
+            // the merged tile's last face, between its last cell and the halo cell
+            const int r = hc_j - T.j0;
+            TransDiff tl, tr;
+            td_get(sT, NT, r * NC + kCtuX, tl);
+            td_get(sT, NT, r * NC + kCtuX + 1, tr);
+            ctu_face<0, HAS_SRC>(E, L, L.off(hc_i, hc_j, k), hc_i, hc_j, k, tr, tl, hdt, area, hc);
+        }
+#pragma unroll
+        for (int n = 0; n < NF; ++n) zlo[n] = zhi[n];
+        zglo[GUN] = zghi[GUN];
+        zglo[GP] = zghi[GP];
+    }
+}
+
+// y-faces.  The thread at row ty owns face j = j0 + ty and the cell above
+// it; row 0 of the LDS arrays is the halo row of cells below the tile's
+// first face (rows 1..kCtuY the own cells).  x|z pairs on x-faces in LDS
+// (columns 0..kCtuX), z|x pairs in registers (halo cells: in LDS), transdd
+// differences in LDS.
+template <int HAS_SRC>
+__global__ void CTU_BOUNDS k_ctu_y(Batch B, HydroConst hc, const double *dtp) {
+    CTU_TILE(1)
+    constexpr int NC = kCtuX + 1;
+    constexpr int NX = (kCtuY + 1) * NC, NT = (kCtuY + 1) * kCtuX, NH = kCtuX;
+    __shared__ double sX[NTF * NX], sT[NTD * NT], sH[2 * NTF * NH];
+    const double dt = *dtp;
+    const double cdx = 1.0 / 3.0 * dt / hc.dx[0], cdz = 1.0 / 3.0 * dt / hc.dx[2];
+    const double ca = 0.5 * dt / hc.dx[0], cb = 0.5 * dt / hc.dx[2];
+    const double hdt = 0.5 * dt, area = hc.dx[0] * hc.dx[2];
+    const bool face = i <= hx && j <= hy;        // face (i, j) and the cell above it
+    const bool ownX = i <= hx + 1 && j <= hy;    // x|z on the cell's low x-face
+    // x|z on far or halo faces (wavefront 0): column kCtuX of the own rows,
+    // then the halo row (columns 0..kCtuX)
+    int xe_i = 0, xe_j = 0, xe_p = 0;
+    bool extX = false;
+    if (t < kCtuY) {
+        xe_i = T.i0 + kCtuX; xe_j = T.j0 + t; xe_p = (t + 1) * NC + kCtuX;
+        extX = xe_i <= hx + 1 && xe_j <= hy;
+    } else if (t < kCtuY + kCtuX + 1) {
+        const int c = t - kCtuY;
+        xe_i = T.i0 + c; xe_j = T.j0 - 1; xe_p = c;
+        extX = xe_i <= hx + 1;
+    }
+    // halo cells (wavefront 1): the row below the tile
+    const int h = t - 64;
+    const bool halo = h >= 0 && h < kCtuX && T.i0 + h <= hx;
+    const int hc_i = T.i0 + h, hc_j = T.j0 - 1;
+    double zlo[NF], zglo[NG];
+    if (face) ctu_pair<2, 0>(E, L, L.off(i, j, T.k0), T.k0, cdx, hc, zlo, zglo);
+    if (halo) {
+        double f[NF], g[NG];
+        ctu_pair<2, 0>(E, L, L.off(hc_i, hc_j, T.k0), T.k0, cdx, hc, f, g);
+        rec_put(sH + (T.k0 & 1) * NTF * NH, NH, h, f, g);
+    }
+    for (int k = T.k0; k <= T.k1; ++k) {
+        double zhi[NF], zghi[NG];
+        if (face) ctu_pair<2, 0>(E, L, L.off(i, j, k + 1), k + 1, cdx, hc, zhi, zghi);
+        if (ownX) {
+            double f[NF], g[NG];
+            ctu_pair<0, 2>(E, L, L.off(i, j, k), i, cdz, hc, f, g);
+            rec_put(sX, NX, (ty + 1) * NC + tx, f, g);
+        }
+        if (extX) {
+            double f[NF], g[NG];
+            ctu_pair<0, 2>(E, L, L.off(xe_i, xe_j, k), xe_i, cdz, hc, f, g);
+            rec_put(sX, NX, xe_p, f, g);
+        }
+        if (halo) {
+            double f[NF], g[NG];
+            ctu_pair<2, 0>(E, L, L.off(hc_i, hc_j, k + 1), k + 1, cdx, hc, f, g);
+            rec_put(sH + ((k + 1) & 1) * NTF * NH, NH, h, f, g);
+        }
+        __syncthreads();
+        TransDiff tp = {};
+        if (face) {
+            double fal[NF], far_[NF], gal[NG], gar[NG];
+            rec_get(sX, NX, (ty + 1) * NC + tx, fal, gal);
+            rec_get(sX, NX, (ty + 1) * NC + tx + 1, far_, gar);
+            trans_diff_two(fal, far_, gal, gar, zlo, zhi, zglo, zghi, ca, cb, kGamma, tp);
+            td_put(sT, NT, (ty + 1) * kCtuX + tx, tp);
+        }
+        if (halo) {
+            double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
+            rec_get(sX, NX, h, fal, gal);
+            rec_get(sX, NX, h + 1, far_, gar);
+            rec_get(sH + (k & 1) * NTF * NH, NH, h, fbl, gbl);
+            rec_get(sH + ((k + 1) & 1) * NTF * NH, NH, h, fbr, gbr);
+            TransDiff th;
+            trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, kGamma, th);
+            td_put(sT, NT, h, th);
+        }
+        __syncthreads();
+        if (face) {
+            TransDiff tm;
+            td_get(sT, NT, ty * kCtuX + tx, tm);
+            ctu_face<1, HAS_SRC>(E, L, L.off(i, j, k), i, j, k, tp, tm, hdt, area, hc);
+        }
+#pragma unroll
+        for (int n = 0; n < NF; ++n) zlo[n] = zhi[n];
+        zglo[GUN] = zghi[GUN];
+        zglo[GP] = zghi[GP];
+    }
+}
+
 // ------------------------------------------------ one-box route (per-box ABI)
 // The table of a one-box launch: its entry and a block prefix of {0}.
 struct OneBox {
@@ -920,16 +1347,23 @@ k_update_c(Batch B, HydroConst hc, const double *dtp, int *flags) {
 } // namespace
 
 // --------------------------------------------------------------- engine
+// block prefix rows: the stage kinds as tuned, the same without z-march
+// (LaunchCfg linear), then the CTU face kernels of the three directions
+constexpr int kCtuRow0 = 2 * ST_COUNT, kBlkRows = 2 * ST_COUNT + 3;
+
 Tuning default_tuning() {
     Tuning t;
     for (int k = 0; k < ST_COUNT; ++k) {
         t.block[k] = kBlock;
         t.tile[k][0] = -1;
         t.tile[k][1] = t.tile[k][2] = 0;
-        t.xcd[k] = 0;
+        t.xcd[k] = 16;
         t.zm[k] = 0;
     }
+    t.zm[ST_T1 + 2] = t.zm[ST_T2 + 2] = 2;
     t.order = 0;
+    t.ctu = 0;
+    t.ctu_planes = 8;
     return t;
 }
 
@@ -970,7 +1404,7 @@ int FusedEngine::setup(const std::vector<Box> &boxes, const std::vector<Arr> &bu
     }
     if (blk_dev_) (void)hipFree(blk_dev_);
     blk_dev_ = nullptr;
-    if (hipMalloc(&blk_dev_, (size_t)2 * ST_COUNT * (n + 1) * sizeof(int)) != hipSuccess) {
+    if (hipMalloc(&blk_dev_, (size_t)kBlkRows * (n + 1) * sizeof(int)) != hipSuccess) {
         set_error("block table allocation failed");
         return CAMR_ERR_ALLOC;
     }
@@ -991,6 +1425,10 @@ int FusedEngine::set_tuning(const Tuning &t) {
         }
     if (t.order != 0 && t.order != 1) {
         set_error("box order must be 0 (level order) or 1 (largest first)");
+        return CAMR_ERR_PARAM;
+    }
+    if ((t.ctu != 0 && t.ctu != 1) || t.ctu_planes < 1 || t.ctu_planes > 1024) {
+        set_error("CTU face kernels: switch 0 or 1, 1..1024 planes per block");
         return CAMR_ERR_PARAM;
     }
     for (int k = 0; k < ST_COUNT; ++k) {
@@ -1043,15 +1481,18 @@ int FusedEngine::build_tables() {
         e.sold = bufA_[b];
         t1[p] = e;
     }
-    blk_host_.assign((size_t)2 * ST_COUNT * (n + 1), 0);
-    for (int k = 0; k < 2 * ST_COUNT; ++k) {
+    blk_host_.assign((size_t)kBlkRows * (n + 1), 0);
+    for (int k = 0; k < kBlkRows; ++k) {
         int *row = &blk_host_[(size_t)k * (n + 1)];
-        const int kind = k % ST_COUNT;
+        const bool ctu = k >= kCtuRow0;
+        const int kind = ctu ? ST_T2 + (k - kCtuRow0) : k % ST_COUNT;
         const int zm = k < ST_COUNT ? tune_.zm[kind] : 0;
         for (int p = 0; p < n; ++p) {
             const int *l = tune_.tile[kind];
-            const int64_t nb = batch_blocks(stage_box(kind, boxes_[order_[p]]),
-                                            tune_.block[kind], l[0], l[1], l[2], zm);
+            const Box sb = stage_box(kind, boxes_[order_[p]]);
+            const int64_t nb =
+                ctu ? ctu_blocks(sb, tune_.ctu_planes, kind == ST_T2)
+                    : batch_blocks(sb, tune_.block[kind], l[0], l[1], l[2], zm);
             if ((int64_t)row[p] + nb > 0x7fffffff) {
                 set_error("stage grid exceeds 2^31 blocks");
                 return CAMR_ERR_SHAPE;
@@ -1115,6 +1556,40 @@ void FusedEngine::launch(const char *name, Prof *prof, K kern, int kind, int b, 
     hipLaunchKernelGGL(kern, dim3(c.grid), dim3(c.block), 0, st, c.B, args...);
 }
 
+LaunchCfg FusedEngine::cfg_ctu(int d, int b, int parity) const {
+    const int n = (int)boxes_.size();
+    const int rk = kCtuRow0 + d;
+    const int *row = &blk_host_[(size_t)rk * (n + 1)];
+    LaunchCfg c;
+    c.B.tab = tab_[parity];
+    c.B.blk = blk_dev_ + (size_t)rk * (n + 1);
+    c.B.nbox = n;
+    if (b < 0) {
+        c.B.first = 0;
+        c.grid = (unsigned)row[n];
+    } else {
+        const int p = pos_[b];
+        c.B.first = row[p];
+        c.grid = (unsigned)(row[p + 1] - row[p]);
+    }
+    c.block = kCtuBlock;
+    c.B.halt = halt_;
+    c.B.lx = -1;
+    c.B.ly = c.B.lz = 0;
+    c.B.xk = tune_.xcd[ST_T2 + d];
+    c.B.zm = tune_.ctu_planes;
+    return c;
+}
+
+template <typename K, typename... A>
+void FusedEngine::launch_ctu(const char *name, Prof *prof, K kern, int d, int b, int parity,
+                             hipStream_t st, A... args) {
+    const LaunchCfg c = cfg_ctu(d, b, parity);
+    if (c.grid == 0) return;
+    Region reg(prof, name, st);
+    hipLaunchKernelGGL(kern, dim3(c.grid), dim3(kCtuBlock), 0, st, c.B, args...);
+}
+
 int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
                        const DevFab *src_tab, int has_src, const HydroConst &hc,
                        const double *dtp, hipStream_t st, int *status, Prof *prof,
@@ -1134,28 +1609,43 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
     launch("riemann_normal", prof, k_riemann1<0>, ST_R1 + 0, b, parity, st, hc);
     launch("riemann_normal", prof, k_riemann1<1>, ST_R1 + 1, b, parity, st, hc);
     launch("riemann_normal", prof, k_riemann1<2>, ST_R1 + 2, b, parity, st, hc);
-    launch("transverse_1", prof, k_trans1_x, ST_T1 + 0, b, parity, st, hc, dtp);
-    launch("transverse_1", prof, k_trans1<1>, ST_T1 + 1, b, parity, st, hc, dtp);
-    if (tune_.zm[ST_T1 + 2] > 0)
-        launch("transverse_1", prof, k_trans1_zm, ST_T1 + 2, b, parity, st, hc, dtp);
-    else
-        launch("transverse_1", prof, k_trans1<2>, ST_T1 + 2, b, parity, st, hc, dtp);
-    launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
-    const bool zm2 = tune_.zm[ST_T2 + 2] > 0;
-    if (has_src) {
-        launch("transverse_2", prof, k_trans2_x<1>, ST_T2 + 0, b, parity, st, hc, dtp);
-        launch("transverse_2", prof, k_trans2<1, 1>, ST_T2 + 1, b, parity, st, hc, dtp);
-        if (zm2)
-            launch("transverse_2", prof, k_trans2_zm<1>, ST_T2 + 2, b, parity, st, hc, dtp);
-        else
-            launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
+    // the adjoint's recompute (no_update) reads the transdo fluxes
+    const bool ctu = tune_.ctu && !no_update;
+    if (ctu) {
+        launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
+        if (has_src) {
+            launch_ctu("transverse_ctu", prof, k_ctu_x<1>, 0, b, parity, st, hc, dtp);
+            launch_ctu("transverse_ctu", prof, k_ctu_y<1>, 1, b, parity, st, hc, dtp);
+            launch_ctu("transverse_ctu", prof, k_ctu_z<1>, 2, b, parity, st, hc, dtp);
+        } else {
+            launch_ctu("transverse_ctu", prof, k_ctu_x<0>, 0, b, parity, st, hc, dtp);
+            launch_ctu("transverse_ctu", prof, k_ctu_y<0>, 1, b, parity, st, hc, dtp);
+            launch_ctu("transverse_ctu", prof, k_ctu_z<0>, 2, b, parity, st, hc, dtp);
+        }
     } else {
-        launch("transverse_2", prof, k_trans2_x<0>, ST_T2 + 0, b, parity, st, hc, dtp);
-        launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
-        if (zm2)
-            launch("transverse_2", prof, k_trans2_zm<0>, ST_T2 + 2, b, parity, st, hc, dtp);
+        launch("transverse_1", prof, k_trans1_x, ST_T1 + 0, b, parity, st, hc, dtp);
+        launch("transverse_1", prof, k_trans1<1>, ST_T1 + 1, b, parity, st, hc, dtp);
+        if (tune_.zm[ST_T1 + 2] > 0)
+            launch("transverse_1", prof, k_trans1_zm, ST_T1 + 2, b, parity, st, hc, dtp);
         else
-            launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
+            launch("transverse_1", prof, k_trans1<2>, ST_T1 + 2, b, parity, st, hc, dtp);
+        launch("divu", prof, k_divu_c, ST_DIVU, b, parity, st, hc);
+        const bool zm2 = tune_.zm[ST_T2 + 2] > 0;
+        if (has_src) {
+            launch("transverse_2", prof, k_trans2_x<1>, ST_T2 + 0, b, parity, st, hc, dtp);
+            launch("transverse_2", prof, k_trans2<1, 1>, ST_T2 + 1, b, parity, st, hc, dtp);
+            if (zm2)
+                launch("transverse_2", prof, k_trans2_zm<1>, ST_T2 + 2, b, parity, st, hc, dtp);
+            else
+                launch("transverse_2", prof, k_trans2<2, 1>, ST_T2 + 2, b, parity, st, hc, dtp);
+        } else {
+            launch("transverse_2", prof, k_trans2_x<0>, ST_T2 + 0, b, parity, st, hc, dtp);
+            launch("transverse_2", prof, k_trans2<1, 0>, ST_T2 + 1, b, parity, st, hc, dtp);
+            if (zm2)
+                launch("transverse_2", prof, k_trans2_zm<0>, ST_T2 + 2, b, parity, st, hc, dtp);
+            else
+                launch("transverse_2", prof, k_trans2<2, 0>, ST_T2 + 2, b, parity, st, hc, dtp);
+        }
     }
     if (no_update) {
         // the caller applies the update itself

@@ -273,6 +273,19 @@ class Level:
     def zmarch(self, planes):
         L.check(L.lib().camr_level_set_zmarch(self._h, (C.c_int32 * L.NSTAGE)(*map(int, planes))))
 
+    @property
+    def ctu(self):
+        """(on, planes): the LDS-staged CTU face kernels (1) or the staged
+        transdo / transdd launches (0), and the z planes per CTU block."""
+        on, planes = C.c_int32(), C.c_int32()
+        L.check(L.lib().camr_level_get_ctu(self._h, C.byref(on), C.byref(planes)))
+        return on.value, planes.value
+
+    @ctu.setter
+    def ctu(self, value):
+        on, planes = value
+        L.check(L.lib().camr_level_set_ctu(self._h, int(on), int(planes)))
+
     def profile(self, enable=True, kernel=None):
         """Time launches (all, or only those named `kernel`) with hipEvents."""
         L.check(L.lib().camr_level_profile(

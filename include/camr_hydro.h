@@ -394,18 +394,29 @@ int camr_level_get_tiles(camr_level *L, int32_t tile[3 * CAMR_NSTAGE]);
  * workgroups over the eight XCDs; K > 0 permutes each launch's blocks so
  * that every XCD takes K consecutive blocks of each window of 8K, keeping a
  * block's neighbouring rows in its own XCD's L2.  Results do not depend on
- * it.  camr_level_set_tuning resets it to 0.  No reference counterpart: set
- * by the autotuner. */
+ * it.  camr_level_set_tuning resets it to 16.  No reference counterpart:
+ * set by the autotuner. */
 int camr_level_set_xcd(camr_level *L, const int32_t chunk[CAMR_NSTAGE]);
 int camr_level_get_xcd(camr_level *L, int32_t chunk[CAMR_NSTAGE]);
 /* z-march of each stage range: 0 = one point per thread; P > 0 = each
  * thread walks P consecutive z planes of its (x, y) column, carrying what a
  * face shares with the face below in registers.  Only the z-face transverse
  * stages (transdo pairs and transdd, stage ranges 7 and 11) with linear
- * blocks take it.  Results do not depend on it.  camr_level_set_tuning resets
- * it to 0.  No reference counterpart: set by the autotuner. */
+ * blocks take it; camr_level_set_tiles clears it for tiled stages.  Results
+ * do not depend on it.  camr_level_set_tuning resets it to 2 for those two
+ * stages and 0 elsewhere.  No reference counterpart: set by the autotuner. */
 int camr_level_set_zmarch(camr_level *L, const int32_t planes[CAMR_NSTAGE]);
 int camr_level_get_zmarch(camr_level *L, int32_t planes[CAMR_NSTAGE]);
+/* LDS-staged CTU face kernels: on = 1 runs the transverse stages of each
+ * face direction as one kernel that keeps the transdo pair fluxes in LDS
+ * and registers (tiles of 32 x 8 points of the x-y plane, each block
+ * walking `planes` z planes, 1..1024); on = 0 (the default, planes 8) runs
+ * the staged transdo and transdd launches through memory, which measure
+ * faster on MI355X.  Results do not depend on it.  camr_level_set_tuning
+ * resets it to the default.  No reference counterpart: set by the
+ * autotuner. */
+int camr_level_set_ctu(camr_level *L, int32_t on, int32_t planes);
+int camr_level_get_ctu(camr_level *L, int32_t *on, int32_t *planes);
 
 /* Adjoint (reverse mode) of level steps, for gradients of a loss on the
  * final state with respect to the initial state.  No reference counterpart

@@ -428,6 +428,8 @@ def test_tiled_blocks_bitwise(oracle, tiles, batched, graph):
     gl = Level(prob, batched=batched, graph=graph)
     gl.tiles = TILE_SETS[tiles]
     assert gl.tiles == TILE_SETS[tiles]
+    if graph:
+        gl.ctu = (0, 32)              # tiles of the staged transverse launches too
     # block permutations over the XCDs, chunks of 1 to 64 blocks
     gl.xcd = [0, 1, 2, 3, 4, 8, 16, 32, 64, 5, 7, 0, 12]
     for L in (ol, gl):
@@ -461,6 +463,7 @@ def test_zmarch_bitwise(oracle, planes, batched, graph):
     gl = Level(prob, batched=batched, graph=graph)
     zm = [0] * 13
     zm[7] = zm[11] = planes
+    gl.ctu = (0, 32)                  # the staged transverse launches
     gl.zmarch = zm
     gl.xcd = [16] * 13
     assert gl.zmarch == zm
@@ -488,6 +491,7 @@ def test_zmarch_with_sources_and_walls_bitwise(oracle, ppm):
     full = problems.SmoothField(seed=3, kmax=3, mach=0.4).state(48, (0, 0, 0), (23, 19, 15))
     ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
     gl = Level(prob, engine=_lib.ENGINE_FUSED)
+    gl.ctu = (0, 32)
     zm = [0] * 13
     zm[7] = zm[11] = 5
     gl.zmarch = zm
@@ -503,6 +507,76 @@ def test_zmarch_with_sources_and_walls_bitwise(oracle, ppm):
     assert np.array_equal(odts, gdts)
     for b in range(2):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"z-marched walls box {b}")
+
+
+@pytest.mark.parametrize("planes", [1, 5, 32, 300])
+@pytest.mark.parametrize("batched,graph", [(True, False), (False, False), (True, True)])
+def test_ctu_kernels_bitwise(oracle, planes, batched, graph):
+    """The LDS-staged CTU face kernels (transdo pairs, transdd and the final
+    Riemann solve of a face direction in one kernel) with blocks walking
+    `planes` z planes: a refined level of uneven boxes (partial tiles, merged
+    x tiles, boxes shorter than a block's run), bit for bit."""
+    prob = workloads.refined_level(n=64, radius=0.3, blocking_factor=8, max_size=32,
+                                   efficiency=0.8)
+    ext = workloads.exterior(prob)
+
+    def init(p, b):
+        lo, hi = p.boxes[b][:3], p.boxes[b][3:]
+        return np.ascontiguousarray(ext[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gl = Level(prob, batched=batched, graph=graph)
+    gl.ctu = (1, planes)
+    gl.xcd = [8] * 13
+    assert gl.ctu == (1, planes)
+    for L in (ol, gl):
+        L.set_exterior(ext)
+        for b in range(len(prob.boxes)):
+            L.set_box(b, init(prob, b))
+        L.init_post()
+    err, odts = ol.run(3)
+    assert err == 0
+    gdts = gl.run(3)
+    assert np.array_equal(odts, gdts)
+    for b in range(len(prob.boxes)):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"CTU box {b}")
+
+
+@pytest.mark.parametrize("ppm,ctu", [(0, 1), (1, 1), (1, 0)])
+def test_ctu_with_sources_and_walls_bitwise(oracle, ppm, ctu):
+    """Walls, outflow and gravity (the transdd source terms) through the CTU
+    face kernels and, for comparison, the staged launches; boxes of 40 x 33
+    x 17 points leave partial tiles in every direction."""
+    n = (80, 33, 17)
+    dx = (1 / 80,) * 3
+    prob = Problem(domlo=(0, 0, 0), domhi=tuple(x - 1 for x in n), dx=dx,
+                   bclo=(2, 4, 5), bchi=(3, 5, 4), is_periodic=(0, 0, 0),
+                   boxes=[(0, 0, 0, 39, 32, 16), (40, 0, 0, 79, 32, 16)],
+                   params=dict(ppm_type=ppm, cfl=0.4, const_grav=-0.7))
+    full = problems.SmoothField(seed=5, kmax=3, mach=0.4).state(80, (0, 0, 0), (79, 32, 16))
+    ol = oracle.Level(_oracle_params(oracle, prob), prob.boxes)
+    gl = Level(prob, engine=_lib.ENGINE_FUSED)
+    gl.ctu = (ctu, 7)
+    for L in (ol, gl):
+        for b in range(2):
+            lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+            L.set_box(b, np.ascontiguousarray(full[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1,
+                                                   lo[0]:hi[0] + 1]))
+        L.init_post()
+    err, odts = ol.run(4)
+    assert err == 0
+    gdts = gl.run(4)
+    assert np.array_equal(odts, gdts)
+    for b in range(2):
+        assert_bitwise(gl.get_box(b), ol.get_box(b), f"CTU walls box {b}")
+
+
+def test_ctu_settings_are_validated():
+    gl = Level(workloads.smooth_periodic(n=16, ppm_type=1))
+    assert gl.ctu == (0, 8)
+    for bad in ((2, 32), (1, 0), (1, 1025)):
+        with pytest.raises(_lib.CamrError):
+            gl.ctu = bad
 
 
 def test_tile_shapes_are_validated():
@@ -575,15 +649,16 @@ def test_autotuned_level_stays_bitwise(oracle, tmp_path, monkeypatch):
     for b in range(len(prob.boxes)):
         T.set_box(b, init(prob, b))
     T.init_post()
-    blocks, order, tiles, xcd = autotune.tune(T, steps=1)
+    res = autotune.tune(T, steps=1)
+    blocks, order, tiles, xcd, ctu = res
     assert len(blocks) == _lib.NSTAGE and all(x in (64, 128, 256) for x in blocks)
     assert all(t == (0, 0, 0) or t[0] * t[1] * t[2] == b for b, t in zip(blocks, tiles))
     assert len(xcd) == _lib.NSTAGE
+    assert ctu[0] in (0, 1) and ctu[1] >= 1
     assert (tmp_path / "tuning.json").exists()
     # a second level with the same boxes reads the cache
-    assert autotune.tune(Level(prob), steps=1) == (blocks, order, tiles, xcd)
-    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, ext=ext,
-                                   tuning=(blocks, order, tiles, xcd), graph=True)
+    assert autotune.tune(Level(prob), steps=1) == res
+    ol, gl, odts, gdts = _run_both(oracle, prob, init, 3, ext=ext, tuning=res, graph=True)
     assert np.array_equal(odts, gdts)
     for b in range(len(prob.boxes)):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"tuned box {b}")
