@@ -221,6 +221,12 @@ class HydroLevel {
     int32_t *ebflags_ = nullptr;
     EbFlags ebf_{};
     std::vector<int> kind_;
+    // EB level: the all-regular boxes (kind 0) as one box batch of the
+    // fused engine's kernels (hydro_umdrv of CAMR_construct_hydro_source.cpp:
+    // 150-151 for each, in batched launches); the cut-cell boxes keep the
+    // per-box EB path
+    FusedEngine reg_engine_;
+    bool eb_batched_ = false;
     std::vector<EbGeom> ebg_;
     std::vector<double *> ebbuf_;
     bool pending_ = false;   // a fused step awaits resolve()
@@ -877,6 +883,25 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     std::vector<DevFab> t(nb);
     for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
     CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+    {
+        std::vector<Box> rb;
+        std::vector<Arr> va, vb;
+        std::vector<int> ids;
+        for (int b = 0; b < nb; ++b) {
+            if (kind_[b] != 0) continue;
+            rb.push_back(box_[b]);
+            va.push_back(dense(bufA_[b], box_[b]));
+            vb.push_back(dense(bufB_[b], box_[b]));
+            ids.push_back(b);
+        }
+        eb_batched_ = false;
+        if (!rb.empty()) {
+            rc = reg_engine_.setup(rb, va, vb, default_tuning(), ids);
+            reg_engine_.set_qgrow(ng_);
+            if (rc == CAMR_OK) eb_batched_ = true;
+            else if (rc != CAMR_ERR_SHAPE) return rc;  // too large: per-box path
+        }
+    }
     return setup_workers(max_g, sc);
 }
 
@@ -1128,6 +1153,18 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
         }
         return CAMR_OK;
     } else {
+        if (eb_batched_) {
+            // the regular boxes of an EB level in batched launches, on the
+            // first worker stream like the per-box work, so that the join
+            // orders them before the update
+            const hipStream_t es = cur_ws(0);
+            const int has_src = grav_src_values_ ? 1 : 0;
+            const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
+            int rc = reg_engine_.hydro(-1, parity(), false, tab_sb_, src, has_src, hc_, dt_dev_,
+                                       es, status_, &prof_, nullptr, true);
+            if (!rc) rc = reg_engine_.dsdt(-1, parity(), tab_dsdt_, hc_, es, &prof_);
+            if (rc) { set_error("kernel launch failed"); return rc; }
+        }
         for (size_t b = 0; b < box_.size(); ++b) {
             const Box B = box_[b], G = grow(B, ng_);
             const hipStream_t bs = cur_ws(b);
@@ -1136,8 +1173,10 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             double *wsq = srcq_ + (size_t)w * QVAR * qstride_;
             double *wscr = scratch_.base() + (size_t)w * sstride_;
             Arr ds = dense(dsdt_[b], B);
-            launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
             const int kind = kind_.empty() ? 0 : kind_[b];
+            // regular box of an EB level: reg_engine_ has written its dS/dt
+            if (kind == 0 && eb_batched_) continue;
+            launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
             if (kind == 2) continue;  // covered box (CAMR_construct_hydro_source.cpp:80)
             if (kind == 1 && !ebg_[b].vf.p) {
                 set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", (int)b);

@@ -380,9 +380,11 @@ class FusedEngine {
     static size_t scratch_doubles(const Box &bx);
     static bool fits(const Box &bx);
     // Canonical arrays for every box and the two box tables (state buffers
-    // A/B as new/old and swapped).
+    // A/B as new/old and swapped).  ids: the level's index of each box (the
+    // row of the level tables sb_tab / src_tab / dsdt_tab), default 0..n-1.
     int setup(const std::vector<Box> &boxes, const std::vector<Arr> &bufA,
-              const std::vector<Arr> &bufB, const Tuning &t);
+              const std::vector<Arr> &bufB, const Tuning &t,
+              const std::vector<int> &ids = std::vector<int>());
     int set_tuning(const Tuning &t);
     const Tuning &tuning() const { return tune_; }
 
@@ -403,7 +405,16 @@ class FusedEngine {
     int fill(int b, int parity, const FillArgs &a, hipStream_t st, int *status, Prof *prof);
     // clean_state passes after the update + next dt estimate
     int post(int b, int parity, const PostArgs &a, hipStream_t st, int *status, Prof *prof);
+    // after hydro(..., no_update = true): hydro_pdivu + hydro_update of
+    // every box into dsdt_tab[id] (overwritten, the staged path's zero fill
+    // and += order), for callers that apply dS/dt themselves
+    int dsdt(int b, int parity, const DevFab *dsdt_tab, const HydroConst &hc, hipStream_t st,
+             Prof *prof);
     int nbox() const { return (int)boxes_.size(); }
+    // ghost width of the primitives the flattening stencil may read (q of
+    // CAMR_construct_hydro_source.cpp, grow(bx, numGrow)): 4, or 6 on an EB
+    // level, where points beyond the canonical grow(bx, 4) come from Sborder
+    void set_qgrow(int g) { qgrow_ = g; }
     // adjoint arrays with the canonical layout of every box (allocated on
     // first use); total doubles of one scratch set
     int enable_adjoint();
@@ -434,6 +445,7 @@ class FusedEngine {
                     hipStream_t st, A... args);
     int build_tables();
     std::vector<Box> boxes_;
+    std::vector<int> ids_;
     std::vector<Arr> bufA_, bufB_;
     std::vector<size_t> off_;      // scratch offset of each box (doubles)
     std::vector<int> order_;       // box order of the tables
@@ -444,6 +456,7 @@ class FusedEngine {
     BoxEnt *tab_[2] = {nullptr, nullptr};
     int *blk_dev_ = nullptr;
     const int *halt_ = nullptr;
+    int qgrow_ = CAMR_NGROW;
     size_t total_ = 0;
     Scratch scratch_, ascratch_;
 };

@@ -67,7 +67,24 @@ k_prim_c(Batch B, const DevFab *sb_tab, const DevFab *src_tab, int has_src, Hydr
     const double rho = v[URHO];
     double o[NQC];
     double e;
-    if (!prim_compact(v, o, e, hc, status)) return;
+    if (!prim_compact(v, o, e, hc, status)) {
+        if (hc.eb) {
+            // covered cell of an EB level: q = 0 (CAMR_construct_hydro_source.cpp:
+            // 116-118), and hydro_srctoprim's zeros for such cells
+#pragma unroll
+            for (int n = 0; n < NQC; ++n) q.st(v0, n, 0.0);
+            if (has_src) {
+                const CView sq = cview(L, srcq_, NS);
+                sq.st(v0, SR, 0.0);
+                sq.st(v0, SU, 0.0);
+                sq.st(v0, SV, 0.0);
+                sq.st(v0, SW, 0.0);
+                sq.st(v0, SP, 0.0);
+                sq.st(v0, SE, 0.0);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int n = 0; n < NQC; ++n) q.st(v0, n, o[n]);
     if (has_src) {
@@ -91,11 +108,16 @@ k_prim_c(Batch B, const DevFab *sb_tab, const DevFab *src_tab, int has_src, Hydr
 
 // ------------------------------------------------------------ flattening
 // min over directions of flatten (flatten.H:19-99); 1 when the +-3 stencil
-// leaves q's box.
+// leaves q's box grow(bx, qgrow).  The canonical primitives cover
+// grow(bx, 4); with qgrow > 4 (the 6-ghost q of an EB level) the pressure
+// and normal velocity of a stencil point beyond them come from hydro_ctoprim
+// of Sborder (u), covered cells (rho <= 0) giving 0 as in k_prim_c.
 template <int D>
 __device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsigned v0,
-                                           int i, int j, int k, const Box &qbox) {
+                                           int i, int j, int k, const Box &bx, int qgrow,
+                                           const Arr &u, const HydroConst &hc, int *status) {
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
+    const Box qbox = grow(bx, qgrow), cbox = grow(bx, CAMR_NGROW);
     if (!(qbox.contains(i + 3 * ex, j + 3 * ey, k + 3 * ez) &&
           qbox.contains(i - 3 * ex, j - 3 * ey, k - 3 * ez)))
         return 1.0;
@@ -103,24 +125,39 @@ __device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsig
     double p[7], un[7];
 #pragma unroll
     for (int o = -3; o <= 3; ++o) {
-        const unsigned vo = v0 + (unsigned)o * sd;
-        p[o + 3] = q.ld(vo, SP);
-        un[o + 3] = q.ld(vo, SU + D);
+        const int ii = i + o * ex, jj = j + o * ey, kk = k + o * ez;
+        if (cbox.contains(ii, jj, kk)) {
+            const unsigned vo = v0 + (unsigned)o * sd;
+            p[o + 3] = q.ld(vo, SP);
+            un[o + 3] = q.ld(vo, SU + D);
+        } else {
+            double v[NVAR], w[NQC], e;
+#pragma unroll
+            for (int n = 0; n < NVAR; ++n) v[n] = u(ii, jj, kk, n);
+            if (prim_compact(v, w, e, hc, status)) {
+                p[o + 3] = w[SP];
+                un[o + 3] = w[SU + D];
+            } else {
+                p[o + 3] = 0.0;
+                un[o + 3] = 0.0;
+            }
+        }
     }
     return flatten_1d(p, un);
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_flat_c(Batch B, int use_flattening) {
+k_flat_c(Batch B, int use_flattening, int qgrow, const DevFab *sb_tab, HydroConst hc,
+         int *status) {
     BATCH_POINT(ST_G2)
-    const Box qbox = grow(E.bx, CAMR_NGROW);
     const CView q = cview(L, slot_ptr<S_Q>(E), NQC), flat = cview(L, slot_ptr<S_FLAT>(E), 1);
+    const Arr u = qgrow > CAMR_NGROW ? sb_tab[E.id].a : Arr{};
     const unsigned v0 = L.off(i, j, k);
     double f = 1.0;
     if (use_flattening == 1) {
-        f = vmin(f, flat_dir<0>(q, L, v0, i, j, k, qbox));
-        f = vmin(f, flat_dir<1>(q, L, v0, i, j, k, qbox));
-        f = vmin(f, flat_dir<2>(q, L, v0, i, j, k, qbox));
+        f = vmin(f, flat_dir<0>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
+        f = vmin(f, flat_dir<1>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
+        f = vmin(f, flat_dir<2>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
     }
     flat.st(v0, 0, f);
 }
@@ -1316,6 +1353,33 @@ k_box_update(Batch B, Arr dsdt, double ac0, double ac1, double ac2, double volc)
     dsdt(i, j, k, UEINT) -= pdivu;
 }
 
+// The same over a box batch into dsdt_tab[E.id], overwriting: 0 + the
+// divergence, then - pdivu, as hydro_update's += on the zeroed dS/dt.
+__global__ void __launch_bounds__(kBlock)
+k_dsdt_c(Batch B, const DevFab *dsdt_tab, double ac0, double ac1, double ac2, double volc) {
+    BATCH_POINT(ST_VALID)
+    const Arr dsdt = dsdt_tab[E.id].a;
+    const CView f0 = cview(L, slot_ptr<S_FLX + 0>(E), NXF), f1 = cview(L, slot_ptr<S_FLX + 1>(E), NXF),
+                f2 = cview(L, slot_ptr<S_FLX + 2>(E), NXF);
+    const unsigned v0 = L.off(i, j, k), sx = 8u, sy = L.sy(), sz = L.sz();
+    const double pdivu =
+        0.5 *
+        ((f0.ld(v0 + sx, XGP) + f0.ld(v0, XGP)) * (f0.ld(v0 + sx, XGU) * ac0 - f0.ld(v0, XGU) * ac0) +
+         (f1.ld(v0 + sy, XGP) + f1.ld(v0, XGP)) * (f1.ld(v0 + sy, XGU) * ac1 - f1.ld(v0, XGU) * ac1) +
+         (f2.ld(v0 + sz, XGP) + f2.ld(v0, XGP)) * (f2.ld(v0 + sz, XGU) * ac2 - f2.ld(v0, XGU) * ac2)) /
+        volc;
+    const double volinv = 1.0 / volc;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double d = 0.0;
+        d += (f0.ld(v0, n) - f0.ld(v0 + sx, n) + f1.ld(v0, n) - f1.ld(v0 + sy, n) +
+              f2.ld(v0, n) - f2.ld(v0 + sz, n)) *
+             volinv;
+        if (n == UEINT) d -= pdivu;
+        dsdt(i, j, k, n) = d;
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_divu_c(Batch B, HydroConst hc) {
     BATCH_POINT(ST_DIVU)
@@ -1373,8 +1437,14 @@ size_t FusedEngine::scratch_doubles(const Box &bx) {
 bool FusedEngine::fits(const Box &bx) { return canon_fits(bx); }
 
 int FusedEngine::setup(const std::vector<Box> &boxes, const std::vector<Arr> &bufA,
-                       const std::vector<Arr> &bufB, const Tuning &t) {
+                       const std::vector<Arr> &bufB, const Tuning &t,
+                       const std::vector<int> &ids) {
     boxes_ = boxes;
+    ids_ = ids;
+    if (!ids_.empty() && ids_.size() != boxes_.size()) {
+        set_error("fused engine: one level index per box");
+        return CAMR_ERR_PARAM;
+    }
     bufA_ = bufA;
     bufB_ = bufB;
     const int n = (int)boxes_.size();
@@ -1469,7 +1539,7 @@ int FusedEngine::build_tables() {
     for (int p = 0; p < n; ++p) {
         const int b = order_[p];
         BoxEnt e;
-        e.id = b;
+        e.id = ids_.empty() ? b : ids_[b];
         e.bx = boxes_[b];
         e.L = canon_of(boxes_[b]);
         e.base = scratch_.base() + off_[b];
@@ -1596,7 +1666,10 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
                        const PostArgs *post, bool no_update) {
     if (!have_prim || has_src)
         launch("prim", prof, k_prim_c, ST_G4, b, parity, st, sb_tab, src_tab, has_src, hc, status);
-    launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening);
+    // q beyond the canonical grow(bx, 4) comes from sb_tab (given with !have_prim)
+    const int qgrow = (!have_prim && sb_tab) ? qgrow_ : CAMR_NGROW;
+    launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening, qgrow, sb_tab, hc,
+           status);
     if (hc.ppm_type == 1) {
         launch("trace_ppm", prof, k_ppm_c<0>, ST_G2, b, parity, st, hc, dtp);
         launch("trace_ppm", prof, k_ppm_c<1>, ST_G2, b, parity, st, hc, dtp);
@@ -1667,6 +1740,13 @@ int FusedEngine::fill(int b, int parity, const FillArgs &a, hipStream_t st, int 
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 
+int FusedEngine::dsdt(int b, int parity, const DevFab *dsdt_tab, const HydroConst &hc,
+                      hipStream_t st, Prof *prof) {
+    launch("consup", prof, k_dsdt_c, ST_VALID, b, parity, st, dsdt_tab, hc.dx[1] * hc.dx[2],
+           hc.dx[0] * hc.dx[2], hc.dx[0] * hc.dx[1], hc.dx[0] * hc.dx[1] * hc.dx[2]);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
 int FusedEngine::post(int b, int parity, const PostArgs &a, hipStream_t st, int *status,
                       Prof *prof) {
     const LaunchCfg c = cfg(ST_VALID, b, parity);
@@ -1726,7 +1806,8 @@ int fused_umdrv(const UmdrvArgs &A, double *scratch, hipStream_t st) {
     launch_one(k_box_in, ST_G4, bx, B, st, A.q, A.qaux, A.srcq, A.has_src, A.uin, ubox);
     const double *dtp = A.dtp;
     const int has_src = A.has_src;
-    launch_one(k_flat_c, ST_G2, bx, B, st, hc.use_flattening);
+    launch_one(k_flat_c, ST_G2, bx, B, st, hc.use_flattening, (int)CAMR_NGROW,
+               (const DevFab *)nullptr, hc, (int *)nullptr);
     if (hc.ppm_type == 1) {
         launch_one(k_ppm_c<0>, ST_G2, bx, B, st, hc, dtp);
         launch_one(k_ppm_c<1>, ST_G2, bx, B, st, hc, dtp);

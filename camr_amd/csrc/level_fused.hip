@@ -155,7 +155,12 @@ k_post_fused(Batch B, PostArgs a, int *status) {
 // k_update_c followed by k_post_fused, with S_new kept in registers in
 // between (the same arithmetic: S_new is rounded once by the update either
 // way).
-__global__ void __launch_bounds__(kBlock)
+#ifdef CAMR_UPD_MINW
+#define UPD_BOUNDS __launch_bounds__(kBlock, CAMR_UPD_MINW)
+#else
+#define UPD_BOUNDS __launch_bounds__(kBlock)
+#endif
+__global__ void UPD_BOUNDS
 k_update_post(Batch B, HydroConst hc, const double *dtp, PostArgs a, int *status) {
     if (B.halt && *B.halt) return;  // uniform: the whole block leaves
     int lblk;
