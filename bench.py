@@ -37,8 +37,13 @@ NVAR = 10
 #     writes the two face states (2 x 9) = 29 doubles.
 #   riemann_normal: per face: qm, qp (18), sound speed / floor of both
 #     cells (2); writes the compact flux + Godunov (10) = 30 doubles.
+#   transverse_ctu (the CTU face kernels, transdo pairs + transdd + final
+#     solve of one direction): per d-face: qm, qp of all three directions
+#     (54), the first fluxes + Godunov of the two transverse directions
+#     (20), sound speed / floor (2), conserved state (10), one divu node;
+#     writes the final flux and Godunov (12) = 99 doubles.
 ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
-                "riemann_normal": 30}
+                "riemann_normal": 30, "transverse_ctu": 99}
 # mol_faces (one launch per direction d, method of lines, flux adjustment
 # fused): per d-face, the 9 primitives and 3 aux values (c, csmall, gamma)
 # of one cell, its 10 conserved values (artificial viscosity) and one divu
@@ -46,16 +51,18 @@ ALGO_DOUBLES = {"transverse_2": 63, "transverse_1": 60, "trace_ppm": 29,
 NS_MOL_READ, NF_MOL_WRITE = 23, 12
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-# for the fused engine at 256^3 PPM (profiles/hbm_traffic_fused_256cube_ppm.md,
-# FETCH_SIZE scaled by 1.9692, WRITE_SIZE by 1.0, the factors an 8-byte-lane
-# copy of 1 GiB reads back).  Reported only for that engine and size.
-MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 15783877140}
+# for the fused engine at 256^3 PPM with the default launch shapes
+# (profiles/hbm_traffic_fused_256cube_ppm.md, profiles/ctu_faces_256cube_ppm.md;
+# FETCH_SIZE scaled by 1.9692, WRITE_SIZE by 0.9961, the factors an
+# 8-byte-lane copy of 1 GiB reads back).  Reported only for that engine and size.
+MEASURED_TRAFFIC = {("fused", 256, "transverse_2"): 10447411888,
+                    ("fused", 256, "transverse_ctu"): 28527364887}
 
 # Executed FP64 work of one PPM cell-update (ADD, MUL, TRANS = 1, FMA = 2 per
 # lane; FMAs only from the division / square-root expansions, contraction is
 # off), summed over the step's kernels from rocprofv3 SQ_INSTS_VALU_*_F64
 # counters at 256^3 (profiles/valu_fused_256cube_ppm.md).
-FP64_EXECUTED_PER_CELL_UPDATE = 10114
+FP64_EXECUTED_PER_CELL_UPDATE = 10069
 
 # Algorithmic FP64 operations of the reference's path per cell-update, by
 # stage (add/sub, mul, div, sqrt = 1 each; no FMA): the oracle run with an
@@ -67,13 +74,15 @@ ALGO_FLOP = {"fill_clean_prim": 169.0, "flatten": 59.3, "trace_ppm": 3501.9,
              "riemann_normal": 300.6, "transverse_1": 1285.1, "divu": 30.4,
              "transverse_2": 989.9, "consup_clean_estdt": 111.0 + 129.0}
 ALGO_FLOP_PER_CELL_UPDATE = sum(ALGO_FLOP.values())
+# the CTU face kernels do the work of both transverse stages
+ALGO_FLOP["transverse_ctu"] = ALGO_FLOP["transverse_1"] + ALGO_FLOP["transverse_2"]
 LAUNCHES_PER_STEP = {"trace_ppm": 3, "riemann_normal": 3, "transverse_1": 3,
-                     "transverse_2": 3}
+                     "transverse_2": 3, "transverse_ctu": 3}
 
 
 def unit_count(name: str, n: int) -> int:
     """Faces or cells one launch of kernel `name` covers for an n^3 box."""
-    if name.startswith("transverse_2"):
+    if name.startswith("transverse_2") or name.startswith("transverse_ctu"):
         return (n + 1) * n * n
     if name.startswith("transverse_1"):
         return (n + 3) * (n + 2) * (n + 2)
@@ -699,6 +708,9 @@ def main() -> int:
     ap.add_argument("--adj-reps", type=int, default=3)
     ap.add_argument("--adj-field", choices=["varp", "c1"], default="varp",
                     help="adjoint workload field: smooth pressure variation, or p = 1")
+    ap.add_argument("--ctu", type=int, choices=[0, 1], default=None,
+                    help="C1: force the CTU face kernels on (1) or off (0)")
+    ap.add_argument("--ctu-planes", type=int, default=8)
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch shapes instead of the autotuner's")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -743,6 +755,10 @@ def main() -> int:
         # its timing steps advance this level before the warm-up
         from camr_amd import autotune
         tuning = autotune.tune(L, steps=2)
+    if args.ctu is not None and args.engine == "fused":
+        # transverse stages as the LDS-staged CTU face kernels (1) or the
+        # staged transdo / transdd launches (0)
+        L.ctu = (args.ctu, args.ctu_planes)
 
     # warm-up steps follow the reference dt sequence
     dt = L.initial_dt()
@@ -789,6 +805,7 @@ def main() -> int:
     achieved = flop_launch / (avg_ms * 1e-3) / 1e12
     algo_bytes = ALGO_DOUBLES.get(kern, 2 * NVAR) * 8 * unit_count(kern, n)
     tiles = L.tiles if args.engine == "fused" else None
+    ctu = L.ctu if args.engine == "fused" else None
 
     # parity at full size: the cpu baseline's steps from U0 on the GPU
     n_check = 6
@@ -823,7 +840,8 @@ def main() -> int:
         "config": {"workload": f"{n}^3 periodic single-level Godunov PPM advance "
                                f"(CAMR_advance + computeNewDt), CFL {cfl}",
                    "cells": n ** 3, "ppm_type": 1, "cfl": cfl,
-                   "engine": args.engine, "tuning": tuning, "tiles": tiles},
+                   "engine": args.engine, "tuning": tuning, "tiles": tiles,
+                   "ctu": list(ctu) if ctu else None},
         "roofline": {"bound": "fp64", "kernel": kern, "achieved": achieved,
                      "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFS,
                      "traffic": MEASURED_TRAFFIC.get((args.engine, n, kern)),

@@ -112,12 +112,12 @@ k_prim_c(Batch B, const DevFab *sb_tab, const DevFab *src_tab, int has_src, Hydr
 // grow(bx, 4); with qgrow > 4 (the 6-ghost q of an EB level) the pressure
 // and normal velocity of a stencil point beyond them come from hydro_ctoprim
 // of Sborder (u), covered cells (rho <= 0) giving 0 as in k_prim_c.
-template <int D>
+template <int D, bool WIDE>
 __device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsigned v0,
                                            int i, int j, int k, const Box &bx, int qgrow,
                                            const Arr &u, const HydroConst &hc, int *status) {
     constexpr int ex = ex_(D, 0), ey = ex_(D, 1), ez = ex_(D, 2);
-    const Box qbox = grow(bx, qgrow), cbox = grow(bx, CAMR_NGROW);
+    const Box qbox = grow(bx, WIDE ? qgrow : CAMR_NGROW), cbox = grow(bx, CAMR_NGROW);
     if (!(qbox.contains(i + 3 * ex, j + 3 * ey, k + 3 * ez) &&
           qbox.contains(i - 3 * ex, j - 3 * ey, k - 3 * ez)))
         return 1.0;
@@ -126,7 +126,7 @@ __device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsig
 #pragma unroll
     for (int o = -3; o <= 3; ++o) {
         const int ii = i + o * ex, jj = j + o * ey, kk = k + o * ez;
-        if (cbox.contains(ii, jj, kk)) {
+        if (!WIDE || cbox.contains(ii, jj, kk)) {
             const unsigned vo = v0 + (unsigned)o * sd;
             p[o + 3] = q.ld(vo, SP);
             un[o + 3] = q.ld(vo, SU + D);
@@ -146,18 +146,21 @@ __device__ __forceinline__ double flat_dir(const CView &q, const Canon &L, unsig
     return flatten_1d(p, un);
 }
 
+// WIDE: q's box is grow(bx, qgrow) with qgrow > 4 (points beyond the
+// canonical primitives from sb_tab); else grow(bx, 4)
+template <bool WIDE>
 __global__ void __launch_bounds__(kBlock)
 k_flat_c(Batch B, int use_flattening, int qgrow, const DevFab *sb_tab, HydroConst hc,
          int *status) {
     BATCH_POINT(ST_G2)
     const CView q = cview(L, slot_ptr<S_Q>(E), NQC), flat = cview(L, slot_ptr<S_FLAT>(E), 1);
-    const Arr u = qgrow > CAMR_NGROW ? sb_tab[E.id].a : Arr{};
+    const Arr u = WIDE ? sb_tab[E.id].a : Arr{};
     const unsigned v0 = L.off(i, j, k);
     double f = 1.0;
     if (use_flattening == 1) {
-        f = vmin(f, flat_dir<0>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
-        f = vmin(f, flat_dir<1>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
-        f = vmin(f, flat_dir<2>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
+        f = vmin(f, flat_dir<0, WIDE>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
+        f = vmin(f, flat_dir<1, WIDE>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
+        f = vmin(f, flat_dir<2, WIDE>(q, L, v0, i, j, k, E.bx, qgrow, u, hc, status));
     }
     flat.st(v0, 0, f);
 }
@@ -1668,8 +1671,12 @@ int FusedEngine::hydro(int b, int parity, bool have_prim, const DevFab *sb_tab,
         launch("prim", prof, k_prim_c, ST_G4, b, parity, st, sb_tab, src_tab, has_src, hc, status);
     // q beyond the canonical grow(bx, 4) comes from sb_tab (given with !have_prim)
     const int qgrow = (!have_prim && sb_tab) ? qgrow_ : CAMR_NGROW;
-    launch("flatten", prof, k_flat_c, ST_G2, b, parity, st, hc.use_flattening, qgrow, sb_tab, hc,
-           status);
+    if (qgrow > CAMR_NGROW)
+        launch("flatten", prof, k_flat_c<true>, ST_G2, b, parity, st, hc.use_flattening, qgrow,
+               sb_tab, hc, status);
+    else
+        launch("flatten", prof, k_flat_c<false>, ST_G2, b, parity, st, hc.use_flattening, qgrow,
+               sb_tab, hc, status);
     if (hc.ppm_type == 1) {
         launch("trace_ppm", prof, k_ppm_c<0>, ST_G2, b, parity, st, hc, dtp);
         launch("trace_ppm", prof, k_ppm_c<1>, ST_G2, b, parity, st, hc, dtp);
@@ -1806,7 +1813,7 @@ int fused_umdrv(const UmdrvArgs &A, double *scratch, hipStream_t st) {
     launch_one(k_box_in, ST_G4, bx, B, st, A.q, A.qaux, A.srcq, A.has_src, A.uin, ubox);
     const double *dtp = A.dtp;
     const int has_src = A.has_src;
-    launch_one(k_flat_c, ST_G2, bx, B, st, hc.use_flattening, (int)CAMR_NGROW,
+    launch_one(k_flat_c<false>, ST_G2, bx, B, st, hc.use_flattening, (int)CAMR_NGROW,
                (const DevFab *)nullptr, hc, (int *)nullptr);
     if (hc.ppm_type == 1) {
         launch_one(k_ppm_c<0>, ST_G2, bx, B, st, hc, dtp);
