@@ -331,11 +331,15 @@ int camr_write_plotfile(const char *dir, const camr_plot_desc *d);
 /*
  * Checkpoint of a level hierarchy for restart (CAMR::checkPoint,
  * Source/Utils/CAMR_io.cpp:45-84: AmrLevel's state data and a CPUtime
- * file), in the directory layout of AMReX checkpoints: dir/Header (time,
- * ratios, per-level dt and steps, domains and boxes), the new-time state of
- * each level in VisMF layout (dir/Level_<l>/SD_0_New_H and _D_00000, all
- * NVAR components bit for bit) and dir/CPUtime.  Restart: upload each box's
- * state (camr_level_upload_box) and the level time (camr_level_set_time).
+ * file), in a directory layout modelled on AMReX checkpoints: dir/Header
+ * (time, ratios, per-level dt and steps, domains and boxes), the new-time
+ * state of each level in VisMF layout (dir/Level_<l>/SD_0_New_H and
+ * _D_00000, all NVAR components bit for bit) and dir/CPUtime (written as 0:
+ * no CPU-time accounting).  The Header is this library's own, without
+ * Geometry, Amr's cycle counts or StateData records: only camr_amd reads it
+ * back (camr_amd.plotfile.restart), not an AMReX build.  Restart: upload each
+ * box's state (camr_level_upload_box) and the level time
+ * (camr_level_set_time).
  */
 typedef struct camr_checkpoint_desc {
     int32_t nlev;
