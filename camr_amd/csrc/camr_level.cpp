@@ -210,11 +210,16 @@ class HydroLevel {
     hipStream_t wst_[kWorkers] = {};
     hipEvent_t wev_[kWorkers] = {}, mev_ = nullptr;
     size_t qstride_ = 0, sstride_ = 0;
-    hipStream_t ws(size_t b) const { return nws_ ? wst_[b % nws_] : stream_; }
+    // worker of box b: b mod nws_, or wmap_[b] when set (an EB level keeps its
+    // cut-cell boxes off worker 0, which runs the batch of regular boxes)
+    std::vector<int> wmap_;
+    int wslot(size_t b) const {
+        return nws_ ? (wmap_.empty() ? (int)(b % nws_) : wmap_[b]) : 0;
+    }
+    hipStream_t ws(size_t b) const { return nws_ ? wst_[wslot(b)] : stream_; }
     bool in_ws_ = false;  // inside a step spread over the worker streams
     hipStream_t cur_ws(size_t b) const { return in_ws_ ? ws(b) : stream_; }
     size_t fstride_ = 0;
-    int wslot(size_t b) const { return nws_ ? (int)(b % nws_) : 0; }
     int fork();
     int join();
     int setup_workers(size_t max_g, size_t sc);
@@ -902,7 +907,18 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
             else if (rc != CAMR_ERR_SHAPE) return rc;  // too large: per-box path
         }
     }
-    return setup_workers(max_g, sc);
+    rc = setup_workers(max_g, sc);
+    if (rc) return rc;
+    wmap_.clear();
+    if (eb_batched_ && nws_ > 1) {
+        // the regular batch runs on worker 0: spread the cut-cell boxes over
+        // the other workers
+        wmap_.assign(nb, 0);
+        int r = 0;
+        for (int b = 0; b < nb; ++b)
+            wmap_[b] = kind_[b] == 1 ? 1 + (r++ % (nws_ - 1)) : b % nws_;
+    }
+    return CAMR_OK;
 }
 
 // The force on the embedded body, F = sum over the cut cells of the wall
@@ -1157,7 +1173,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             // the regular boxes of an EB level in batched launches, on the
             // first worker stream like the per-box work, so that the join
             // orders them before the update
-            const hipStream_t es = cur_ws(0);
+            const hipStream_t es = in_ws_ && nws_ ? wst_[0] : stream_;
             const int has_src = grav_src_values_ ? 1 : 0;
             const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
             int rc = reg_engine_.hydro(-1, parity(), false, tab_sb_, src, has_src, hc_, dt_dev_,
