@@ -58,7 +58,7 @@ STAGES = {
     "transverse_1": (5, 6, 7),
     "divu": (8,),
     "transverse_2": (9, 10, 11),
-    "consup": (12,), "clean_estdt": (12,),
+    "consup": (12,), "clean_estdt": (12,), "consup_clean_estdt": (12,),
 }
 # groups of stage ranges that must share a block size (one profiler name)
 GROUPS = [(0,), (1,), (2, 3, 4), (5, 6, 7), (8,), (9, 10, 11), (12,)]
@@ -117,6 +117,30 @@ def apply(level, blocks, order, tiles, xcd=None, ctu=None):
         level.ctu = tuple(ctu)
 
 
+def _cand_name(c) -> str:
+    bs, tile, xk = c
+    shape = f"{bs} linear" if tile == LINEAR else "x".join(map(str, tile))
+    return f"{shape}, xcd {xk}"
+
+
+def report_markdown(table, results, t_def) -> str:
+    """The tuner's measurements as markdown: per stage group the time of
+    every candidate (ms per step, level order), then the whole-step checks."""
+    out = ["| stage ranges | " + " | ".join(_cand_name(c) for c in CANDIDATES) + " |",
+           "|---|" + "---:|" * len(CANDIDATES)]
+    for order, grp, cost in table:
+        if order != 0:
+            continue
+        best = min(cost.values())
+        cells = [(f"**{cost[c]:.3f}**" if cost[c] == best else f"{cost[c]:.3f}") for c in CANDIDATES]
+        out.append(f"| {','.join(map(str, grp))} | " + " | ".join(cells) + " |")
+    out += ["", "| whole step | ms |", "|---|---:|", f"| defaults | {t_def * 1e3:.3f} |"]
+    for t, blocks, order, tiles, xcd, ctu in results:
+        out.append(f"| per-stage picks, box order {order}, CTU {'on' if ctu[0] else 'off'} | "
+                   f"{t * 1e3:.3f} |")
+    return "\n".join(out)
+
+
 def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
     """Choose and apply the launch shapes of `level` (which must hold a
     valid state; it is advanced by the timing runs).  Returns the tuning
@@ -152,6 +176,7 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
                   (0, CTU_PLANES))
             times[(order, cand)] = _stage_times(level, steps)
     results = []
+    table = []  # (order, group, {candidate: ms per step})
     for order in (0, 1):
         blocks = [256] * _lib.NSTAGE
         tiles = [LINEAR] * _lib.NSTAGE
@@ -159,6 +184,7 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
         for grp in GROUPS:
             names = [n for n, st in STAGES.items() if st == grp or set(st) & set(grp)]
             cost = {c: sum(times[(order, c)].get(n, 0.0) for n in names) for c in CANDIDATES}
+            table.append((order, grp, cost))
             pick = min(cost, key=cost.get)
             for g in grp:
                 blocks[g], tiles[g], xcd[g] = pick
@@ -177,6 +203,7 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
         print(f"autotune {key}: default {t_def * 1e3:.3f} ms/step, chosen "
               f"{t_best * 1e3:.3f} ms/step, blocks {blocks}, tiles {tiles}, xcd {xcd}, "
               f"order {order}, ctu {ctu}", flush=True)
+        print(report_markdown(table, results, t_def), flush=True)
     cache[key] = {"blocks": list(map(int, blocks)), "order": int(order),
                   "tiles": [list(map(int, t)) for t in tiles], "xcd": list(map(int, xcd)),
                   "ctu": list(map(int, ctu)),
@@ -188,3 +215,47 @@ def tune(level, steps: int = 3, use_cache: bool = True, verbose: bool = False):
     except OSError:
         pass
     return list(blocks), order, [tuple(t) for t in tiles], list(xcd), tuple(ctu)
+
+
+def _main() -> int:
+    """Tune the C1 level (256^3 periodic PPM, one box) and the two levels of
+    the C2 hierarchy without the cache, printing every measurement."""
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--which", choices=["c1", "c2", "all"], default="all")
+    args = ap.parse_args()
+    from . import workloads
+    from .hydro import Level
+    if args.which in ("c1", "all"):
+        prob = workloads.smooth_periodic(n=256, ppm_type=1, cfl=0.4)
+        L = Level(prob)
+        L.set_box(0, workloads.smooth_init(prob, 0))
+        L.init_post()
+        print("## C1: 256^3 periodic PPM, one box\n", flush=True)
+        tune(L, steps=args.steps, use_cache=False, verbose=True)
+        L.close()
+    if args.which in ("c2", "all"):
+        lev0, lev1 = workloads.amr_levels(128)
+        C = Level(lev0)
+        for b in range(len(lev0.boxes)):
+            C.set_box(b, workloads.smooth_init(lev0, b))
+        C.init_post()
+        print("\n## C2 level 0: 128^3 in 64 boxes of 32^3\n", flush=True)
+        tune(C, steps=args.steps, use_cache=False, verbose=True)
+        ext1 = workloads.exterior(lev1)
+        F = Level(lev1)
+        F.set_exterior(ext1)
+        for b in range(len(lev1.boxes)):
+            lo, hi = lev1.boxes[b][:3], lev1.boxes[b][3:]
+            F.set_box(b, ext1[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].copy())
+        F.init_post()
+        print(f"\n## C2 level 1: {len(lev1.boxes)} boxes of 8^3 to 64^3\n", flush=True)
+        tune(F, steps=args.steps, use_cache=False, verbose=True)
+        F.close()
+        C.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
