@@ -234,6 +234,15 @@ class HydroLevel {
     bool eb_batched_ = false;
     std::vector<EbGeom> ebg_;
     std::vector<double *> ebbuf_;
+    // the cut-cell boxes (kind 1) as one table of the batched EB path, each
+    // with its own q / qaux / srcq, fluxes and scratch in cut_buf_ (freed
+    // with ebbuf_)
+    std::vector<int> cut_ids_;
+    std::vector<size_t> cut_off_;
+    std::vector<EbBox> cut_htab_;
+    EbBox *cut_tab_ = nullptr;  // device copy of cut_htab_
+    double *cut_buf_ = nullptr;
+    int cut_entry(int c);
     bool pending_ = false;   // a fused step awaits resolve()
     bool needs_ext_ = false; // some ghost cell has no owning box
     bool dt_ready_ = false;  // dtmin_ holds the CFL estimate of S_new
@@ -826,6 +835,11 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
         return CAMR_ERR_PARAM;
     }
     if (ebflags_) { set_error("embedded boundary already set"); return CAMR_ERR_PARAM; }
+    if (grav_src_values_) {
+        // the hydro source would need the 6 ghost cells of the cut-cell path
+        set_error("embedded boundaries with a constant-gravity source are not built");
+        return CAMR_ERR_PARAM;
+    }
     if (desc_.hp.ppm_type != 0 || desc_.hp.use_pslope != 0) {
         set_error("embedded boundaries need ppm_type = 0 and use_pslope = 0 (CAMR.cpp:186-190)");
         return CAMR_ERR_PARAM;
@@ -878,7 +892,6 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
         ebg_[b].f = ebf_;
         max_g = std::max<size_t>(max_g, grow(B, ng_).npts());
         sc = std::max(sc, umdrv_staged_scratch_doubles(B));
-        if (kind_[b] == 1) sc = std::max(sc, umdrv_eb_scratch_doubles(B, grow(B, ng_)));
         // Sborder with the wider ghost region
         (void)hipFree(sb_[b]);
         sb_[b] = nullptr;
@@ -909,6 +922,30 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     }
     rc = setup_workers(max_g, sc);
     if (rc) return rc;
+    // the cut-cell boxes: arrays per box, table entries rewritten when a
+    // box's geometry arrives
+    cut_ids_.clear();
+    cut_off_.clear();
+    size_t tot = 0;
+    auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
+    for (int b = 0; b < nb; ++b) {
+        if (kind_[b] != 1) continue;
+        const Box B = box_[b], G = grow(B, ng_);
+        cut_ids_.push_back(b);
+        cut_off_.push_back(tot);
+        tot += r32((size_t)(2 * QVAR + NQAUX) * G.npts());
+        for (int d = 0; d < 3; ++d) tot += r32((size_t)NVAR * nodes(B, d).npts());
+        tot += r32(umdrv_eb_scratch_doubles(B, G));
+    }
+    if (!cut_ids_.empty()) {
+        CAMR_HIP(hipMalloc(&cut_buf_, tot * sizeof(double)));
+        ebbuf_.push_back(cut_buf_);
+        CAMR_HIP(hipMalloc(&cut_tab_, cut_ids_.size() * sizeof(EbBox)));
+        ebbuf_.push_back(reinterpret_cast<double *>(cut_tab_));
+        cut_htab_.assign(cut_ids_.size(), EbBox{});
+        for (size_t c = 0; c < cut_ids_.size(); ++c)
+            if ((rc = cut_entry((int)c))) return rc;
+    }
     wmap_.clear();
     if (eb_batched_ && nws_ > 1) {
         // the regular batch runs on worker 0: spread the cut-cell boxes over
@@ -1018,7 +1055,40 @@ int HydroLevel::set_box_eb(int b, const double *vf, const double *const ap[3],
     for (int d = 0; d < 3 && !rc; ++d) rc = up(ap[d], nodes(G, d), 1, &g.ap[d]);
     for (int d = 0; d < 3 && !rc; ++d) rc = up(fc[d], nodes(G, d), 2, &g.fc[d]);
     g.f = ebf_;
+    for (size_t c = 0; c < cut_ids_.size() && !rc; ++c)
+        if (cut_ids_[c] == b) rc = cut_entry((int)c);
     return rc;
+}
+
+// Table entry c of the cut-cell batch: box cut_ids_[c] with its arrays in
+// cut_buf_ (q, qaux, srcq over grow(bx, 6), the face fluxes, the scratch),
+// its Sborder, dS/dt and geometry; copied to the device table.
+int HydroLevel::cut_entry(int c) {
+    const int b = cut_ids_[c];
+    const Box B = box_[b], G = grow(B, ng_);
+    auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
+    double *p = cut_buf_ + cut_off_[c];
+    UmdrvEbArgs A;
+    A.bx = B;
+    A.qbox = G;
+    A.hc = hc_;
+    for (int d = 0; d < 3; ++d) A.periodic[d] = desc_.is_periodic[d];
+    A.dtp = dt_dev_;
+    A.uin = sbv_[b];
+    A.dsdt = dense(dsdt_[b], B);
+    A.q = dense(p, G);
+    A.srcq = dense(p + (size_t)QVAR * G.npts(), G);
+    A.qaux = dense(p + (size_t)2 * QVAR * G.npts(), G);
+    p += r32((size_t)(2 * QVAR + NQAUX) * G.npts());
+    for (int d = 0; d < 3; ++d) {
+        A.flx[d] = dense(p, nodes(B, d));
+        p += r32((size_t)NVAR * nodes(B, d).npts());
+    }
+    A.has_src = 0;
+    A.g = ebg_[b];
+    eb_box_setup(A, p, &cut_htab_[c]);
+    CAMR_HIP(hipMemcpy(cut_tab_ + c, &cut_htab_[c], sizeof(EbBox), hipMemcpyHostToDevice));
+    return CAMR_OK;
 }
 
 int HydroLevel::box_kind(int b, int32_t *kind) const {
@@ -1181,6 +1251,28 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             if (!rc) rc = reg_engine_.dsdt(-1, parity(), tab_dsdt_, hc_, es, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
+        if (!cut_ids_.empty()) {
+            // the cut-cell boxes: primitives per box, then hydro_umdrv_eb
+            // (:182-226) in batched launches over all of them, on a worker
+            // stream of their own
+            const hipStream_t cs = in_ws_ && nws_ ? wst_[nws_ > 1 ? 1 : 0] : stream_;
+            for (size_t c = 0; c < cut_ids_.size(); ++c) {
+                const int b = cut_ids_[c];
+                if (!ebg_[b].vf.p) {
+                    set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", b);
+                    return CAMR_ERR_PARAM;
+                }
+                const EbBox &E = cut_htab_[c];
+                launch_on(k_fill_value, E.bx, cs, E.dsdt, (int)NVAR, 0.0);
+                launch_on(k_ctoprim, E.qbox, cs, sbv_[b], E.q, E.qaux, hc_, status_);
+                if (grav_src_values_ && !desc_.hp.do_mol)
+                    launch_on(k_srctoprim, E.qbox, cs, E.q, E.qaux, dense(srchyd_[b], E.qbox),
+                              E.srcq);
+            }
+            int rc = umdrv_eb_batch(cut_tab_, cut_htab_.data(), (int)cut_ids_.size(), hc_,
+                                    dt_dev_, grav_src_values_ ? 1 : 0, cs, &prof_);
+            if (rc) { set_error("kernel launch failed"); return rc; }
+        }
         for (size_t b = 0; b < box_.size(); ++b) {
             const Box B = box_[b], G = grow(B, ng_);
             const hipStream_t bs = cur_ws(b);
@@ -1190,39 +1282,16 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             double *wscr = scratch_.base() + (size_t)w * sstride_;
             Arr ds = dense(dsdt_[b], B);
             const int kind = kind_.empty() ? 0 : kind_[b];
-            // regular box of an EB level: reg_engine_ has written its dS/dt
-            if (kind == 0 && eb_batched_) continue;
+            // regular box of an EB level: reg_engine_ has written its dS/dt;
+            // cut-cell box: the batch above has
+            if ((kind == 0 && eb_batched_) || kind == 1) continue;
             launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
             if (kind == 2) continue;  // covered box (CAMR_construct_hydro_source.cpp:80)
-            if (kind == 1 && !ebg_[b].vf.p) {
-                set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", (int)b);
-                return CAMR_ERR_PARAM;
-            }
             Arr q = dense(wq, G), qa = dense(wqa, G), sq = dense(wsq, G);
             launch_on(k_ctoprim, G, bs, sbv_[b], q, qa, hc_, status_);
             // primitive sources feed the Godunov predictor only (:129-143)
             if (grav_src_values_ && !desc_.hp.do_mol)
                 launch_on(k_srctoprim, G, bs, q, qa, dense(srchyd_[b], G), sq);
-            if (kind == 1) {
-                UmdrvEbArgs E;
-                E.bx = B;
-                E.qbox = G;
-                E.hc = hc_;
-                for (int d = 0; d < 3; ++d) E.periodic[d] = desc_.is_periodic[d];
-                E.dtp = dt_dev_;
-                E.uin = sbv_[b];
-                E.dsdt = ds;
-                E.q = q;
-                E.qaux = qa;
-                E.srcq = sq;
-                E.has_src = grav_src_values_ ? 1 : 0;
-                for (int d = 0; d < 3; ++d)
-                    E.flx[d] = dense(flx_[d] + (size_t)w * NVAR * fstride_, nodes(B, d));
-                E.g = ebg_[b];
-                int rc = umdrv_eb(E, wscr, bs, &prof_);
-                if (rc) { set_error("kernel launch failed"); return rc; }
-                continue;
-            }
             UmdrvArgs A;
             A.bx = B;
             A.qbox = G;

@@ -49,7 +49,29 @@ struct UmdrvEbArgs {
     EbGeom g;
 };
 
+// One box of a cut-cell batch: its inputs and outputs (those of
+// UmdrvEbArgs) and its scratch arrays, carved by eb_box_setup.
+struct EbBox {
+    Box bx, qbox, nb, dbox2;  // nb: divu nodes clamped to the domain; dbox2: redistribution domain
+    EbGeom g;
+    Arr q, qaux, srcq, uin, dsdt, flx[3];
+    Arr QM[3], QP[3], F1[3], G1[3], F2[3][3], G2[3][3], FT[3], QE[3];
+    Arr qmT[3], qpT[3], qmF, qpF, divu, divc, wt, optmp0, delm, winv;
+};
+
+// Doubles of scratch one box needs (the one-box call's EbBox record included).
 size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox);
+// Carves the scratch arrays of box A.bx out of `scratch` (device memory of
+// umdrv_eb_scratch_doubles(A.bx, A.qbox) doubles) into *out (host memory).
+void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out);
+// hydro_umdrv_eb over the n boxes of tab (device copy) / htab (the same
+// records in host memory, read for the grid sizes): one launch per stage,
+// every box of the table in the launch.  All boxes share hc, dtp and has_src
+// (srcq holds primitive sources).
+int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
+                   const double *dtp, int has_src, hipStream_t st, Prof *prof = nullptr);
+// One box: its EbBox record goes into the head of scratch (written by a
+// kernel, so the call can be captured into a graph).
 int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof = nullptr);
 // Per valid cell of bx: the wall momentum flux of its cut cell times dx2
 // (3 components, 0 for other cells) into out; q / qaux cover bx.

@@ -6,14 +6,18 @@
 // 29-98: the face-centroid divergence, the wall flux and p div u); and
 // AMReX's single-level flux redistribution.
 //
-// One kernel per stage over the points of one box, intermediates in HBM
-// scratch in the compact layouts of the staged engine.  The per-point
+// One kernel per stage over the points of a table of boxes (grid.y = box),
+// intermediates in HBM scratch per box in the compact layouts of the staged
+// engine, so that a level's cut-cell boxes share each launch.  The per-point
 // arithmetic is hydro_math.h's, so a cut-cell box reproduces the regular
 // path's operation order wherever every face is open.  The redistribution
 // scatter of the reference is a gather here: each cell adds the shares of
 // its cut neighbours in the order the sequential scatter would, so the sum
 // is bit-identical and needs no atomics.
 #include "hydro_eb.h"
+
+#include <cstring>
+#include <utility>
 
 namespace camr {
 
@@ -24,17 +28,48 @@ __device__ __constant__ int kUc[NF] = {URHO, UMX, UMY, UMZ, UEDEN, UEINT, UFS, U
 
 __device__ __forceinline__ int dv(int d, int e) { return d == e ? 1 : 0; }
 
+// One direction of the cut-cell PLM trace: slopes of order 4 / 2 / 0 by the
+// covered neighbours, the trace, states kept on open faces only.
+template <int D>
+__device__ __forceinline__ void plm_eb_dir(int i, int j, int k, const Arr &q, const EbGeom &g,
+                                           const double *c, double cc, double fl, const Arr &QM,
+                                           const Arr &QP, const HydroConst &hc,
+                                           const double *dtp) {
+    constexpr int ex = D == 0, ey = D == 1, ez = D == 2;
+    const bool cm1 = g.f.covered(i - ex, j - ey, k - ez), cm2 = g.f.covered(i - 2 * ex, j - 2 * ey, k - 2 * ez);
+    const bool cp1 = g.f.covered(i + ex, j + ey, k + ez), cp2 = g.f.covered(i + 2 * ex, j + 2 * ey, k + 2 * ez);
+    int ord = 0;
+    if (hc.plm_iorder == 4 && !cm1 && !cm2 && !cp1 && !cp2) ord = 4;
+    else if (hc.plm_iorder > 1 && !cm1 && !cp1) ord = 2;
+    double sl[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double st[5];
+#pragma unroll
+        for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, kQc[s]);
+        sl[s] = ord ? mc_slope(st, fl, ord) : 0.0;
+    }
+    double qp[NS], qm[NS];
+    plm_trace(D, c, sl, cc, *dtp / hc.dx[D], hc.small_dens, hc.small_pres, qp, qm);
+    const bool open_lo = g.ap[D](i, j, k, 0) > 0.0;
+    const bool open_hi = g.ap[D](i + ex, j + ey, k + ez, 0) > 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        QP(i, j, k, s) = open_lo ? qp[s] : 0.0;
+        QM(i + ex, j + ey, k + ez, s) = open_hi ? qm[s] : 0.0;
+    }
+}
+
 // ---------------------------------------------- PLM trace (cut-cell form)
 // Godunov_3D_eb.cpp:94-152: flatten_eb (flatten.H:104-134), plm_slope_eb
 // (PLM_eb.H:18-54), hydro_plm_d_eb (PLM_eb.H:103-287); covered cells skip.
-__global__ void __launch_bounds__(kBlock)
-k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1,
+__device__ __forceinline__ void
+b_plm_eb(int i, int j, int k, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1,
          Arr qp1, Arr qm2, Arr qp2, HydroConst hc, const double *dtp) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (eb_type(g.f(i, j, k)) == kEbCovered) return;
     double fl = 1.0;
     if (hc.use_flattening == 1) {
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int ex = dv(d, 0), ey = dv(d, 1), ez = dv(d, 2);
             double fd = 1.0;
@@ -46,6 +81,7 @@ k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1
                           eb_type(g.f(i + n * ex, j + n * ey, k + n * ez)) == kEbSingle;
                 if (!cut) {
                     double p[7], un[7];
+#pragma unroll
                     for (int o = -3; o <= 3; ++o) {
                         p[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QPRES);
                         un[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QU + d);
@@ -60,34 +96,9 @@ k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1
 #pragma unroll
     for (int s = 0; s < NS; ++s) c[s] = q(i, j, k, kQc[s]);
     const double cc = qaux(i, j, k, QC);
-    Arr QM[3] = {qm0, qm1, qm2};
-    Arr QP[3] = {qp0, qp1, qp2};
-#pragma unroll 1
-    for (int d = 0; d < 3; ++d) {
-        const int ex = dv(d, 0), ey = dv(d, 1), ez = dv(d, 2);
-        const bool cm1 = g.f.covered(i - ex, j - ey, k - ez), cm2 = g.f.covered(i - 2 * ex, j - 2 * ey, k - 2 * ez);
-        const bool cp1 = g.f.covered(i + ex, j + ey, k + ez), cp2 = g.f.covered(i + 2 * ex, j + 2 * ey, k + 2 * ez);
-        int ord = 0;
-        if (hc.plm_iorder == 4 && !cm1 && !cm2 && !cp1 && !cp2) ord = 4;
-        else if (hc.plm_iorder > 1 && !cm1 && !cp1) ord = 2;
-        double sl[NS];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            double st[5];
-#pragma unroll
-            for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, kQc[s]);
-            sl[s] = ord ? mc_slope(st, fl, ord) : 0.0;
-        }
-        double qp[NS], qm[NS];
-        plm_trace(d, c, sl, cc, *dtp / hc.dx[d], hc.small_dens, hc.small_pres, qp, qm);
-        const bool open_lo = g.ap[d](i, j, k, 0) > 0.0;
-        const bool open_hi = g.ap[d](i + ex, j + ey, k + ez, 0) > 0.0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            QP[d](i, j, k, s) = open_lo ? qp[s] : 0.0;
-            QM[d](i + ex, j + ey, k + ez, s) = open_hi ? qm[s] : 0.0;
-        }
-    }
+    plm_eb_dir<0>(i, j, k, q, g, c, cc, fl, qm0, qp0, hc, dtp);
+    plm_eb_dir<1>(i, j, k, q, g, c, cc, fl, qm1, qp1, hc, dtp);
+    plm_eb_dir<2>(i, j, k, q, g, c, cc, fl, qm2, qp2, hc, dtp);
 }
 
 // ------------------------------------------------ Riemann solve, open faces
@@ -95,10 +106,8 @@ k_plm_eb(Range r, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1
 // boundary-mutated states written back); 1: transverse solve (compact out);
 // 2: final solve into the NVAR flux array.
 template <int D, int MODE>
-__global__ void __launch_bounds__(kBlock)
-k_riemann_eb(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, Arr ap, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void
+b_riemann_eb(int i, int j, int k, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, Arr ap, HydroConst hc) {
     if (!(ap(i, j, k, 0) > 0.)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     double ql[NS], qr[NS];
@@ -139,11 +148,9 @@ k_riemann_eb(Range r, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, Arr ap, Hydr
 // differences (the same arithmetic as zero fluxes), a closed normal face
 // passes the predictor state through.
 template <int D, int T>
-__global__ void __launch_bounds__(kBlock)
-k_transdo_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT,
+__device__ __forceinline__ void
+b_transdo_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT,
              EbFlags fl, Arr qmo, Arr qpo, const double *dtp, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (fl.covered(i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int tx = (T == 0), ty = (T == 1), tz = (T == 2);
@@ -179,12 +186,10 @@ k_transdo_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT
 // hydro_transdd with norm_area = ap[D], trans0_area = ap[T0], trans1_area =
 // ap[T1] (Godunov_utils_3D.H:245-482).
 template <int D>
-__global__ void __launch_bounds__(kBlock)
-k_transdd_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa, Arr ga,
+__device__ __forceinline__ void
+b_transdd_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa, Arr ga,
              Arr fb, Arr gb, Arr apD, Arr apA, Arr apB, EbFlags fl, Arr qm, Arr qp,
              const double *dtp, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (fl.covered(i, j, k)) return;
     constexpr int T0 = (D == 0) ? 1 : 0;
     constexpr int T1 = (D == 2) ? 1 : 2;
@@ -238,10 +243,8 @@ k_transdd_eb(Range r, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa,
 // ------------------------------------------------------------- eb_divu
 // Hydro_utils_eb_K.H:31-78 on nodes: zero unless the 8 cells around the
 // node are all full.
-__global__ void __launch_bounds__(kBlock)
-k_eb_divu(Range r, Arr q, Arr vf, Arr divu, double dxi0, double dxi1, double dxi2) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void
+b_eb_divu(int i, int j, int k, Arr q, Arr vf, Arr divu, double dxi0, double dxi1, double dxi2) {
     const double sv = vf(i, j, k, 0) + vf(i - 1, j, k, 0) + vf(i, j - 1, k, 0) +
                       vf(i - 1, j - 1, k, 0) + vf(i, j, k - 1, 0) + vf(i - 1, j, k - 1, 0) +
                       vf(i, j - 1, k - 1, 0) + vf(i - 1, j - 1, k - 1, 0);
@@ -271,10 +274,8 @@ k_eb_divu(Range r, Arr q, Arr vf, Arr divu, double dxi0, double dxi1, double dxi
 // area factor only where the aperture is open (Hydro_adjust_fluxes.cpp:
 // 96-165, hydro_ext_flx_eb Hydro_utils_eb_K.H:12-27).
 template <int D>
-__global__ void __launch_bounds__(kBlock)
-k_adjust_eb(Range r, Arr flx, Arr divu, Arr u, Arr ap, double area_c, HydroConst hc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void
+b_adjust_eb(int i, int j, int k, Arr flx, Arr divu, Arr u, Arr ap, double area_c, HydroConst hc) {
     double f[NVAR];
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) f[n] = flx(i, j, k, n);
@@ -340,18 +341,21 @@ __device__ __forceinline__ void eb_wallflux(const Arr &q, const Arr &qaux, const
 // eb_compute_div (Hydro_utils_eb_K.H:283-575) for all NVAR components, the
 // redistribution weight (eb_weights_type 2: rho), the face-centroid fluxes
 // of the valid box, then eb_pdivu (:82-280).
-__global__ void __launch_bounds__(kBlock)
-k_eb_div(Range r, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, EbGeom g,
+template <bool CUT>
+__device__ __forceinline__ void
+b_eb_div(int i, int j, int k, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, EbGeom g,
          Arr q, Arr qaux, Arr divc, Arr wt, Arr fo0, Arr fo1, Arr fo2, HydroConst hc,
          double dxi0, double dxi1, double dxi2) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const int32_t w = g.f(i, j, k);
     if (eb_type(w) == kEbCovered) return;
+    // CUT: the cut cells only; otherwise the regular cells and the weight of
+    // every cell (two launches: the cut-cell form's registers would hold
+    // the regular cells' occupancy down)
+    if (CUT != (eb_type(w) == kEbSingle)) return;
     const bool valid = bx.contains(i, j, k);
     const bool xh = i == bx.hi[0], yh = j == bx.hi[1], zh = k == bx.hi[2];
     const double cvi = dxi0 * dxi1 * dxi2;
-    if (eb_type(w) == kEbRegular) {
+    if constexpr (!CUT) {
 #pragma unroll
         for (int n = 0; n < NVAR; ++n) {
             const double dvv = cvi * (ft0(i + 1, j, k, n) - ft0(i, j, k, n)) +
@@ -372,6 +376,7 @@ k_eb_div(Range r, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, 
                    (ge1(i, j + 1, k, GP) + ge1(i, j, k, GP)) * (ge1(i, j + 1, k, GUN) - ge1(i, j, k, GUN)) * dxi1 +
                    (ge2(i, j, k + 1, GP) + ge2(i, j, k, GP)) * (ge2(i, j, k + 1, GUN) - ge2(i, j, k, GUN)) * dxi2);
         divc(i, j, k, UEINT) -= pdivu;
+        wt(i, j, k, 0) = q(i, j, k, QRHO);
     } else {
         const double axm = g.ap[0](i, j, k, 0), axp = g.ap[0](i + 1, j, k, 0);
         const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
@@ -408,8 +413,8 @@ k_eb_div(Range r, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, 
         divc(i, j, k, UEINT) -= (1.0 / vfr) * q(i, j, k, QPRES) *
                                 (dxi0 * (axp * fxp - axm * fxm) + dxi1 * (ayp * fyp - aym * fym) +
                                  dxi2 * (azp * fzp - azm * fzm));
+        wt(i, j, k, 0) = q(i, j, k, QRHO);
     }
-    wt(i, j, k, 0) = q(i, j, k, QRHO);
 }
 
 // ------------------------------------------------- flux redistribution
@@ -417,10 +422,8 @@ k_eb_div(Range r, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, 
 // registers, use_wts_in_divnc false), over grow(bx, 1): the cut cell's
 // own share optmp0 and the mass deficit delm it hands out, and 1 / (sum of
 // its neighbours' weight * vfrac).
-__global__ void __launch_bounds__(kBlock)
-k_redist_src(Range r, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void
+b_redist_src(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv) {
     const int32_t w = g.f(i, j, k);
     if (eb_type(w) != kEbSingle) {
 #pragma unroll
@@ -430,84 +433,92 @@ k_redist_src(Range r, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr del
     }
     const double vfc = g.vf(i, j, k, 0);
     const double mask = dbox2.contains(i, j, k) ? 1.0 : 0.0;
+    // connected neighbours (bit (ii+1) + 3 (jj+1) + 9 (kk+1), the centre
+    // cleared), and those of them inside dbox2; visited in increasing bit
+    // order, i.e. k, then j, then i, as the reference's loops
+    uint32_t con = (uint32_t)w & ((1u << 27) - 1) & ~(1u << 13), in = 0;
+    for (uint32_t m = con; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        if (dbox2.contains(i + t % 3 - 1, j + (t / 3) % 3 - 1, k + t / 9 - 1)) in |= 1u << t;
+    }
+    double vtot = 0.0;
+    for (uint32_t m = in; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        vtot += g.vf(i + t % 3 - 1, j + (t / 3) % 3 - 1, k + t / 9 - 1, 0) * 1.0;
+    }
 #pragma unroll 1
     for (int n = 0; n < NVAR; ++n) {
-        double vtot = 0.0, divnc = 0.0;
-        for (int kk = -1; kk <= 1; kk++)
-        for (int jj = -1; jj <= 1; jj++)
-        for (int ii = -1; ii <= 1; ii++)
-            if ((ii != 0 || jj != 0 || kk != 0) && eb_conn(w, ii, jj, kk) &&
-                dbox2.contains(i + ii, j + jj, k + kk)) {
-                const double wf = g.vf(i + ii, j + jj, k + kk, 0) * 1.0;
-                vtot += wf;
-                divnc += wf * divc(i + ii, j + jj, k + kk, n);
-            }
+        double divnc = 0.0;
+        for (uint32_t m = in; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+            const double wf = g.vf(i + ii, j + jj, k + kk, 0) * 1.0;
+            divnc += wf * divc(i + ii, j + jj, k + kk, n);
+        }
         divnc /= vtot;
         const double op = (1 - vfc) * (divnc - divc(i, j, k, n)) * mask;
         optmp0(i, j, k, n) = op;
         delm(i, j, k, n) = -(vfc) * op;
     }
     double wtot = 0.0;
-    for (int kk = -1; kk <= 1; kk++)
-    for (int jj = -1; jj <= 1; jj++)
-    for (int ii = -1; ii <= 1; ii++)
-        if ((ii != 0 || jj != 0 || kk != 0) && eb_conn(w, ii, jj, kk))
-            wtot += wt(i + ii, j + jj, k + kk, 0) * g.vf(i + ii, j + jj, k + kk, 0) *
-                    (dbox2.contains(i + ii, j + jj, k + kk) ? 1.0 : 0.0);
+    for (uint32_t m = con; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+        wtot += wt(i + ii, j + jj, k + kk, 0) * g.vf(i + ii, j + jj, k + kk, 0) *
+                (dbox2.contains(i + ii, j + jj, k + kk) ? 1.0 : 0.0);
+    }
     winv(i, j, k, 0) = 1.0 / wtot;
 }
 
 // Receiving side over bx: the shares of the cut neighbours in the order of
 // the reference's sequential scatter (source cells in increasing k, j, i),
 // then dsdt = divc + optmp on non-covered cells.
-__global__ void __launch_bounds__(kBlock)
-k_redist_dst(Range r, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv,
+__device__ __forceinline__ void
+b_redist_dst(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv,
              Arr dsdt) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     if (g.f.covered(i, j, k)) return;
     const double mask = dbox2.contains(i, j, k) ? 1.0 : 0.0;
     const double wtt = wt(i, j, k, 0);
+    // the sending neighbours, bit (oi+1) + 3 (oj+1) + 9 (ok+1)
+    uint32_t src = 0;
+#pragma unroll
+    for (int ok = -1; ok <= 1; ok++)
+#pragma unroll
+    for (int oj = -1; oj <= 1; oj++)
+#pragma unroll
+    for (int oi = -1; oi <= 1; oi++) {
+        if (oi == 0 && oj == 0 && ok == 0) continue;
+        const int32_t ws = g.f(i + oi, j + oj, k + ok);
+        if (eb_type(ws) == kEbSingle && eb_conn(ws, -oi, -oj, -ok))
+            src |= 1u << ((oi + 1) + 3 * (oj + 1) + 9 * (ok + 1));
+    }
 #pragma unroll 1
     for (int n = 0; n < NVAR; ++n) {
         double acc = optmp0(i, j, k, n);
-        for (int ok = -1; ok <= 1; ok++)
-        for (int oj = -1; oj <= 1; oj++)
-        for (int oi = -1; oi <= 1; oi++) {
-            if (oi == 0 && oj == 0 && ok == 0) continue;
-            const int32_t ws = g.f(i + oi, j + oj, k + ok);
-            if (eb_type(ws) == kEbSingle && eb_conn(ws, -oi, -oj, -ok)) {
-                const double drho = delm(i + oi, j + oj, k + ok, n) * winv(i + oi, j + oj, k + ok, 0) * wtt * mask;
-                acc += drho;
-            }
+        // in increasing bit order: k, then j, then i of the sender
+        for (uint32_t m = src; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ok = t / 9 - 1, oj = (t / 3) % 3 - 1, oi = t % 3 - 1;
+            const double drho = delm(i + oi, j + oj, k + ok, n) * winv(i + oi, j + oj, k + ok, 0) * wtt * mask;
+            acc += drho;
         }
         dsdt(i, j, k, n) = divc(i, j, k, n) + acc;
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_zero(Range r, Arr a, int nc) {
-    int i, j, k;
-    if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void
+b_zero(int i, int j, int k, Arr a, int nc) {
     for (int n = 0; n < nc; ++n) a(i, j, k, n) = 0.0;
 }
 
-Prof *g_prof = nullptr;
-const char *g_name = "";
-
-template <typename K, typename... A>
-void launch(K kern, const Box &b, hipStream_t s, A... args) {
-    Range r = range_of(b);
-    Region reg(g_prof, g_name, s);
-    hipLaunchKernelGGL(kern, dim3(grid_for(r.n)), dim3(kBlock), 0, s, r, args...);
-}
-
+// ---------------------------------------------------------------- driver
+// The index boxes of one box's stages.
 struct Layout {
     Box bx, btf, bxg1, bxg2, g1;
     Box qm[3], f1[3], f2[3], ft[3], qe[3], nb, mbx[3];
 };
 
-Layout layout_of(const Box &bx, const Box &qbox) {
+__host__ __device__ inline Layout layout_of(const Box &bx, const Box &qbox) {
     Layout L;
     L.bx = bx;
     L.btf = grow(bx, 2);
@@ -527,11 +538,172 @@ Layout layout_of(const Box &bx, const Box &qbox) {
     const Box a = nodes(nodes(nodes(grow(qbox, -1), 0), 1), 2);
     const Box b = nodes(nodes(nodes(grow(bx, 5), 0), 1), 2);
     for (int d = 0; d < 3; ++d) {
-        L.nb.lo[d] = std::max(a.lo[d], b.lo[d]);
-        L.nb.hi[d] = std::min(a.hi[d], b.hi[d]);
+        L.nb.lo[d] = a.lo[d] > b.lo[d] ? a.lo[d] : b.lo[d];
+        L.nb.hi[d] = a.hi[d] < b.hi[d] ? a.hi[d] : b.hi[d];
     }
     return L;
 }
+
+// The stages of hydro_umdrv_eb in launch order: one launch site each, run
+// over every box of a table (grid.y = box).  The transverse pairs D|T go in
+// the order of Godunov_3D_eb.cpp:214-455, (0|1, 0|2, 1|0, 1|2, 2|0, 2|1),
+// each as transdo then Riemann; the final states per direction as transdd
+// then Riemann.
+enum EbSite {
+    ES_ZERO = 0,                    // the accumulated arrays, all in one launch
+    ES_PLM,
+    ES_R1,                          // + d (3)
+    ES_PAIR = ES_R1 + 3,            // + 2 p (transdo), + 2 p + 1 (Riemann), p < 6
+    ES_FINAL = ES_PAIR + 12,        // + 2 d (transdd), + 2 d + 1 (Riemann)
+    ES_DIVU = ES_FINAL + 6,
+    ES_ADJ,                         // + d (3)
+    ES_DIV = ES_ADJ + 3,
+    ES_DIVCUT,
+    ES_RSRC,
+    ES_RDST,
+    ES_COUNT
+};
+// pair p: (D, T, O) with O the third direction
+__host__ __device__ constexpr int pair_d(int p) { return p / 2; }
+__host__ __device__ constexpr int pair_t(int p) {
+    return p / 2 == 0 ? (p % 2 ? 2 : 1) : p / 2 == 1 ? (p % 2 ? 2 : 0) : (p % 2 ? 1 : 0);
+}
+__host__ __device__ constexpr int pair_o(int p) { return 3 - pair_d(p) - pair_t(p); }
+__host__ __device__ constexpr int t0_of(int d) { return d == 0 ? 1 : 0; }
+__host__ __device__ constexpr int t1_of(int d) { return d == 2 ? 1 : 2; }
+
+template <int S>
+__host__ __device__ inline Box site_box(const EbBox &B, const Layout &L) {
+    // ES_ZERO: the nodes of grow(bx, 5), which hold ft, qe, nb and btf
+    if constexpr (S == ES_ZERO) return nodes(nodes(nodes(grow(B.bx, 5), 0), 1), 2);
+    else if constexpr (S == ES_PLM) return L.bxg2;
+    else if constexpr (S >= ES_R1 && S < ES_R1 + 3) return L.f1[S - ES_R1];
+    else if constexpr (S >= ES_PAIR && S < ES_PAIR + 12) {
+        constexpr int p = (S - ES_PAIR) / 2;
+        if constexpr ((S - ES_PAIR) % 2 == 0) return grow_dir(L.bxg2, pair_t(p), -1);
+        else return grow_dir(nodes(L.bxg1, pair_d(p)), pair_o(p), 1);
+    } else if constexpr (S >= ES_FINAL && S < ES_FINAL + 6) {
+        constexpr int d = (S - ES_FINAL) / 2;
+        if constexpr ((S - ES_FINAL) % 2 == 0) return L.bxg1;
+        else return grow_dir(grow_dir(nodes(L.btf, d), t0_of(d), 1), t1_of(d), 1);
+    } else if constexpr (S == ES_DIVU) return B.nb;
+    else if constexpr (S >= ES_ADJ && S < ES_ADJ + 3) return L.ft[S - ES_ADJ];
+    else if constexpr (S == ES_DIV || S == ES_DIVCUT) return L.btf;
+    else if constexpr (S == ES_RSRC) return L.g1;
+    else return B.bx;
+}
+
+// The stage's per-point work with box B's arrays (the arguments of the
+// former per-box launches, unchanged).
+template <int S>
+__device__ __forceinline__ void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
+                                          const HydroConst &hc, const double *dtp, int has_src) {
+    if constexpr (S == ES_ZERO) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (L.ft[d].contains(i, j, k)) b_zero(i, j, k, B.FT[d], (int)NVAR);
+            if (L.qe[d].contains(i, j, k)) b_zero(i, j, k, B.QE[d], (int)NG);
+        }
+        if (L.nb.contains(i, j, k)) b_zero(i, j, k, B.divu, 1);
+        if (L.btf.contains(i, j, k)) {
+            b_zero(i, j, k, B.divc, (int)NVAR);
+            b_zero(i, j, k, B.wt, 1);
+        }
+    } else if constexpr (S == ES_PLM) {
+        b_plm_eb(i, j, k, B.q, B.qbox, B.qaux, B.g, B.QM[0], B.QP[0], B.QM[1], B.QP[1], B.QM[2],
+                 B.QP[2], hc, dtp);
+    } else if constexpr (S >= ES_R1 && S < ES_R1 + 3) {
+        constexpr int d = S - ES_R1;
+        b_riemann_eb<d, 0>(i, j, k, B.QM[d], B.QP[d], B.qaux, B.F1[d], B.G1[d], B.g.ap[d], hc);
+    } else if constexpr (S >= ES_PAIR && S < ES_PAIR + 12) {
+        constexpr int p = (S - ES_PAIR) / 2, D = pair_d(p), T = pair_t(p);
+        if constexpr ((S - ES_PAIR) % 2 == 0)
+            b_transdo_eb<D, T>(i, j, k, B.QM[D], B.QP[D], B.qaux, B.F1[T], B.G1[T], B.g.ap[D],
+                               B.g.ap[T], B.g.f, B.qmT[D], B.qpT[D], dtp, hc);
+        else
+            b_riemann_eb<D, 1>(i, j, k, B.qmT[D], B.qpT[D], B.qaux, B.F2[D][T], B.G2[D][T],
+                               B.g.ap[D], hc);
+    } else if constexpr (S >= ES_FINAL && S < ES_FINAL + 6) {
+        constexpr int D = (S - ES_FINAL) / 2, T0 = t0_of(D), T1 = t1_of(D);
+        if constexpr ((S - ES_FINAL) % 2 == 0)
+            b_transdd_eb<D>(i, j, k, B.QM[D], B.QP[D], B.qaux, B.srcq, has_src, B.F2[T0][T1],
+                            B.G2[T0][T1], B.F2[T1][T0], B.G2[T1][T0], B.g.ap[D], B.g.ap[T0],
+                            B.g.ap[T1], B.g.f, B.qmF, B.qpF, dtp, hc);
+        else
+            b_riemann_eb<D, 2>(i, j, k, B.qmF, B.qpF, B.qaux, B.FT[D], B.QE[D], B.g.ap[D], hc);
+    } else if constexpr (S == ES_DIVU) {
+        b_eb_divu(i, j, k, B.q, B.g.vf, B.divu, 1.0 / hc.dx[0], 1.0 / hc.dx[1], 1.0 / hc.dx[2]);
+    } else if constexpr (S >= ES_ADJ && S < ES_ADJ + 3) {
+        constexpr int D = S - ES_ADJ;
+        const double area = D == 0 ? hc.dx[1] * hc.dx[2]
+                          : D == 1 ? hc.dx[0] * hc.dx[2] : hc.dx[1] * hc.dx[0];
+        b_adjust_eb<D>(i, j, k, B.FT[D], B.divu, B.uin, B.g.ap[D], area, hc);
+    } else if constexpr (S == ES_DIV || S == ES_DIVCUT) {
+        b_eb_div<S == ES_DIVCUT>(i, j, k, B.bx, B.FT[0], B.FT[1], B.FT[2], B.QE[0], B.QE[1], B.QE[2], B.g, B.q,
+                 B.qaux, B.divc, B.wt, B.flx[0], B.flx[1], B.flx[2], hc, 1.0 / hc.dx[0],
+                 1.0 / hc.dx[1], 1.0 / hc.dx[2]);
+    } else if constexpr (S == ES_RSRC) {
+        b_redist_src(i, j, k, B.g, B.dbox2, B.divc, B.wt, B.optmp0, B.delm, B.winv);
+    } else {
+        b_redist_dst(i, j, k, B.g, B.dbox2, B.divc, B.wt, B.optmp0, B.delm, B.winv, B.dsdt);
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(kBlock)
+k_eb_site(const EbBox *tab, HydroConst hc, const double *dtp, int has_src) {
+    const EbBox &B = tab[blockIdx.y];
+    const Layout L = layout_of(B.bx, B.qbox);
+    const Box sb = site_box<S>(B, L);
+    if (!sb.ok()) return;
+    int i, j, k;
+    if (!range_point(range_of(sb), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+        return;
+    site_body<S>(B, L, i, j, k, hc, dtp, has_src);
+}
+
+constexpr const char *site_name(int s) {
+    return s < ES_PLM ? "eb_zero"
+         : s == ES_PLM ? "eb_trace_plm"
+         : s < ES_PAIR ? "eb_riemann_1"
+         : s < ES_FINAL ? ((s - ES_PAIR) % 2 ? "eb_riemann_2" : "eb_transverse_1")
+         : s < ES_DIVU ? ((s - ES_FINAL) % 2 ? "eb_riemann_3" : "eb_transverse_2")
+         : s == ES_DIVU ? "eb_divu"
+         : s < ES_DIV ? "eb_adjust"
+         : s <= ES_DIVCUT ? "eb_divergence" : "eb_redistribute";
+}
+
+template <int S>
+void launch_site(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
+                 const double *dtp, int has_src, hipStream_t st, Prof *prof) {
+    int64_t nblk = 0;
+    for (int b = 0; b < n; ++b) {
+        const Box sb = site_box<S>(htab[b], layout_of(htab[b].bx, htab[b].qbox));
+        if (sb.ok()) nblk = std::max<int64_t>(nblk, (sb.npts() + kBlock - 1) / kBlock);
+    }
+    if (nblk == 0) return;
+    Region reg(prof, site_name(S), st);
+    hipLaunchKernelGGL(k_eb_site<S>, dim3((unsigned)nblk, (unsigned)n), dim3(kBlock), 0, st, tab,
+                       hc, dtp, has_src);
+}
+
+template <int... S>
+void launch_sites(std::integer_sequence<int, S...>, const EbBox *tab, const EbBox *htab, int n,
+                  const HydroConst &hc, const double *dtp, int has_src, hipStream_t st,
+                  Prof *prof) {
+    (launch_site<S>(tab, htab, n, hc, dtp, has_src, st, prof), ...);
+}
+
+// The EbBox of a one-box call, written into device memory by kernel
+// arguments (no host buffer whose lifetime a captured copy would outlive).
+struct Words {
+    uint64_t w[240];
+};
+__global__ void k_put_words(uint64_t *dst, Words w, int n) {
+    for (int t = (int)threadIdx.x; t < n; t += (int)blockDim.x) dst[t] = w.w[t];
+}
+
+constexpr size_t kEbHead = (sizeof(EbBox) + 255) / 256 * 32;  // doubles, 256-byte multiple
 
 } // namespace
 
@@ -551,128 +723,92 @@ size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox) {
     n += L.nb.npts();                                                 // divu
     n += (size_t)(NVAR + 1) * L.btf.npts();                           // divc, wt
     n += (size_t)(2 * NVAR + 1) * L.g1.npts();                        // optmp0, delm, winv
-    return n + 64;
+    return n + 64 + kEbHead;
 }
 
-int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) {
-    g_prof = prof;
-    const HydroConst &hc = A.hc;
+void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out) {
     const Layout L = layout_of(A.bx, A.qbox);
-    const EbGeom &g = A.g;
+    EbBox B;
+    memset(&B, 0, sizeof(B));
+    B.bx = A.bx;
+    B.qbox = A.qbox;
+    B.g = A.g;
+    B.q = A.q;
+    B.qaux = A.qaux;
+    B.srcq = A.srcq;
+    B.uin = A.uin;
+    B.dsdt = A.dsdt;
+    for (int d = 0; d < 3; ++d) B.flx[d] = A.flx[d];
     double *cur = scratch;
     auto take = [&](const Box &b, int nc) {
         Arr a = dense(cur, b);
         cur += (size_t)nc * b.npts();
         return a;
     };
-    const double dxi[3] = {1.0 / hc.dx[0], 1.0 / hc.dx[1], 1.0 / hc.dx[2]};
-
-    Arr QM[3], QP[3], F1[3], G1[3], F2[3][3], G2[3][3], FT[3], QE[3];
     for (int d = 0; d < 3; ++d) {
-        QM[d] = take(L.qm[d], NS);
-        QP[d] = take(L.bxg2, NS);
-        F1[d] = take(L.f1[d], NF);
-        G1[d] = take(L.f1[d], NG);
+        B.QM[d] = take(L.qm[d], NS);
+        B.QP[d] = take(L.bxg2, NS);
+        B.F1[d] = take(L.f1[d], NF);
+        B.G1[d] = take(L.f1[d], NG);
         for (int t = 0; t < 3; ++t)
-            if (t != d) { F2[d][t] = take(L.f2[d], NF); G2[d][t] = take(L.f2[d], NG); }
-        FT[d] = take(L.ft[d], NVAR);
-        QE[d] = take(L.qe[d], NG);
+            if (t != d) { B.F2[d][t] = take(L.f2[d], NF); B.G2[d][t] = take(L.f2[d], NG); }
+        B.FT[d] = take(L.ft[d], NVAR);
+        B.QE[d] = take(L.qe[d], NG);
     }
     size_t tmax = 0;
     for (int d = 0; d < 3; ++d) tmax = std::max<size_t>(tmax, L.mbx[d].npts());
     double *tq0 = cur; cur += (size_t)NS * tmax;
     double *tq1 = cur; cur += (size_t)NS * tmax;
-    Arr qmF = take(L.bxg2, NS), qpF = take(L.bxg1, NS);
-    Arr divu = take(L.nb, 1);
-    Arr divc = take(L.btf, NVAR), wt = take(L.btf, 1);
-    Arr optmp0 = take(L.g1, NVAR), delm = take(L.g1, NVAR), winv = take(L.g1, 1);
-
-    g_name = "eb_zero";
-    for (int d = 0; d < 3; ++d) {
-        launch(k_zero, L.ft[d], st, FT[d], (int)NVAR);
-        launch(k_zero, L.qe[d], st, QE[d], (int)NG);
-    }
-    launch(k_zero, L.nb, st, divu, 1);
-    launch(k_zero, L.btf, st, divc, (int)NVAR);
-    launch(k_zero, L.btf, st, wt, 1);
-
-    g_name = "eb_trace_plm";
-    launch(k_plm_eb, L.bxg2, st, A.q, A.qbox, A.qaux, g, QM[0], QP[0], QM[1], QP[1], QM[2],
-           QP[2], hc, A.dtp);
-    g_name = "eb_riemann_1";
-    launch(k_riemann_eb<0, 0>, L.f1[0], st, QM[0], QP[0], A.qaux, F1[0], G1[0], g.ap[0], hc);
-    launch(k_riemann_eb<1, 0>, L.f1[1], st, QM[1], QP[1], A.qaux, F1[1], G1[1], g.ap[1], hc);
-    launch(k_riemann_eb<2, 0>, L.f1[2], st, QM[2], QP[2], A.qaux, F1[2], G1[2], g.ap[2], hc);
-
-    // transverse pairs D|T over bxg2 shrunk in T; Riemann over the faces of
-    // bxg1 in D grown by one in the third direction (Godunov_3D_eb.cpp:214-455)
-#define CAMR_EB_PAIR(D, T, O)                                                                \
-    {                                                                                        \
-        Arr qmT = dense(tq0, L.mbx[D]), qpT = dense(tq1, L.mbx[D]);                          \
-        g_name = "eb_transverse_1";                                                          \
-        launch(k_transdo_eb<D, T>, grow_dir(L.bxg2, T, -1), st, QM[D], QP[D], A.qaux, F1[T], \
-               G1[T], g.ap[D], g.ap[T], g.f, qmT, qpT, A.dtp, hc);                          \
-        g_name = "eb_riemann_2";                                                             \
-        launch(k_riemann_eb<D, 1>, grow_dir(nodes(L.bxg1, D), O, 1), st, qmT, qpT, A.qaux,  \
-               F2[D][T], G2[D][T], g.ap[D], hc);                                             \
-    }
-    CAMR_EB_PAIR(0, 1, 2)
-    CAMR_EB_PAIR(0, 2, 1)
-    CAMR_EB_PAIR(1, 0, 2)
-    CAMR_EB_PAIR(1, 2, 0)
-    CAMR_EB_PAIR(2, 0, 1)
-    CAMR_EB_PAIR(2, 1, 0)
-#undef CAMR_EB_PAIR
-
-    // final states on bxg1, fluxes on faces of btf grown tangentially
-    // (Godunov_3D_eb.cpp:457-565)
-#define CAMR_EB_FINAL(D, T0, T1)                                                              \
-    {                                                                                         \
-        g_name = "eb_transverse_2";                                                           \
-        launch(k_transdd_eb<D>, L.bxg1, st, QM[D], QP[D], A.qaux, A.srcq, A.has_src,          \
-               F2[T0][T1], G2[T0][T1], F2[T1][T0], G2[T1][T0], g.ap[D], g.ap[T0], g.ap[T1],   \
-               g.f, qmF, qpF, A.dtp, hc);                                                     \
-        g_name = "eb_riemann_3";                                                              \
-        launch(k_riemann_eb<D, 2>, grow_dir(grow_dir(nodes(L.btf, D), T0, 1), T1, 1), st, qmF, \
-               qpF, A.qaux, FT[D], QE[D], g.ap[D], hc);                                       \
-    }
-    CAMR_EB_FINAL(0, 1, 2)
-    CAMR_EB_FINAL(1, 0, 2)
-    CAMR_EB_FINAL(2, 0, 1)
-#undef CAMR_EB_FINAL
-
+    // the pair states of direction D: one buffer, the layout of mbx[D]
+    for (int d = 0; d < 3; ++d) { B.qmT[d] = dense(tq0, L.mbx[d]); B.qpT[d] = dense(tq1, L.mbx[d]); }
+    B.qmF = take(L.bxg2, NS);
+    B.qpF = take(L.bxg1, NS);
+    B.divu = take(L.nb, 1);
+    B.divc = take(L.btf, NVAR);
+    B.wt = take(L.btf, 1);
+    B.optmp0 = take(L.g1, NVAR);
+    B.delm = take(L.g1, NVAR);
+    B.winv = take(L.g1, 1);
     // adjust_fluxes_eb: divu on the nodes of grow(qbox, -1) that lie in the
     // domain grown by 16 in periodic directions (Hydro_adjust_fluxes.cpp:73-84)
-    Box nb = L.nb;
+    B.nb = L.nb;
     for (int d = 0; d < 3; ++d) {
         const int gp = A.periodic[d] ? 16 : 0;
-        nb.lo[d] = std::max(nb.lo[d], hc.domlo[d] - gp);
-        nb.hi[d] = std::min(nb.hi[d], hc.domhi[d] + 1 + gp);
+        B.nb.lo[d] = std::max(B.nb.lo[d], A.hc.domlo[d] - gp);
+        B.nb.hi[d] = std::min(B.nb.hi[d], A.hc.domhi[d] + 1 + gp);
     }
-    g_name = "eb_divu";
-    if (nb.ok()) launch(k_eb_divu, nb, st, A.q, g.vf, divu, dxi[0], dxi[1], dxi[2]);
-    g_name = "eb_adjust";
-    launch(k_adjust_eb<0>, L.ft[0], st, FT[0], divu, A.uin, g.ap[0], hc.dx[1] * hc.dx[2], hc);
-    launch(k_adjust_eb<1>, L.ft[1], st, FT[1], divu, A.uin, g.ap[1], hc.dx[0] * hc.dx[2], hc);
-    launch(k_adjust_eb<2>, L.ft[2], st, FT[2], divu, A.uin, g.ap[2], hc.dx[1] * hc.dx[0], hc);
-
-    g_name = "eb_divergence";
-    launch(k_eb_div, L.btf, st, A.bx, FT[0], FT[1], FT[2], QE[0], QE[1], QE[2], g, A.q, A.qaux,
-           divc, wt, A.flx[0], A.flx[1], A.flx[2], hc, dxi[0], dxi[1], dxi[2]);
-
-    Box dbox2;
     for (int d = 0; d < 3; ++d) {
         const int gp = A.periodic[d] ? 2 : 0;
-        dbox2.lo[d] = hc.domlo[d] - gp;
-        dbox2.hi[d] = hc.domhi[d] + gp;
+        B.dbox2.lo[d] = A.hc.domlo[d] - gp;
+        B.dbox2.hi[d] = A.hc.domhi[d] + gp;
     }
-    g_name = "eb_redistribute";
-    launch(k_redist_src, L.g1, st, g, dbox2, divc, wt, optmp0, delm, winv);
-    launch(k_redist_dst, A.bx, st, g, dbox2, divc, wt, optmp0, delm, winv, A.dsdt);
-    g_prof = nullptr;
+    *out = B;
+}
+
+int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
+                   const double *dtp, int has_src, hipStream_t st, Prof *prof) {
+    if (n <= 0) return CAMR_OK;
+    launch_sites(std::make_integer_sequence<int, ES_COUNT>{}, tab, htab, n, hc, dtp, has_src, st,
+                 prof);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 
+int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof) {
+    EbBox hb;
+    eb_box_setup(A, scratch + kEbHead, &hb);
+    static_assert(sizeof(EbBox) % 8 == 0, "EbBox words");
+    const int nw = (int)(sizeof(EbBox) / 8);
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(&hb);
+    for (int w0 = 0; w0 < nw; w0 += 240) {
+        Words w;
+        const int m = std::min(240, nw - w0);
+        memcpy(w.w, src + w0, (size_t)m * 8);
+        hipLaunchKernelGGL(k_put_words, dim3(1), dim3(256), 0, st,
+                           reinterpret_cast<uint64_t *>(scratch) + w0, w, m);
+    }
+    return umdrv_eb_batch(reinterpret_cast<const EbBox *>(scratch), &hb, 1, A.hc, A.dtp,
+                          A.has_src, st, prof);
+}
 
 // The force on the embedded body from one box's cut cells: per valid cell
 // the wall momentum flux times dx^2 (fluxw[UMX..UMZ] of

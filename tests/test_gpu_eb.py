@@ -112,7 +112,7 @@ def test_eb_levels_are_refused_by_coarse_fine_transfers():
         F.exterior_from_coarse(C, 2)
 
 
-def test_eb_level_rejects_fused_engine_and_ppm():
+def test_eb_level_rejects_fused_engine_ppm_and_gravity():
     prob = workloads.sod_sphere(n=16, max_grid=16)
     geom = workloads.sod_sphere_geometry(prob)
     gl = Level(prob, engine=_lib.ENGINE_FUSED)
@@ -121,6 +121,12 @@ def test_eb_level_rejects_fused_engine_and_ppm():
     prob.params["ppm_type"] = 1
     gl = Level(prob, engine=_lib.ENGINE_STAGED)
     with pytest.raises(_lib.CamrError):
+        gl.set_eb(geom)
+    # a constant-gravity source would need the cut-cell path's 6 ghost cells
+    prob.params["ppm_type"] = 0
+    prob.params["const_grav"] = -0.5
+    gl = Level(prob, engine=_lib.ENGINE_STAGED)
+    with pytest.raises(_lib.CamrError, match="gravity"):
         gl.set_eb(geom)
 
 
