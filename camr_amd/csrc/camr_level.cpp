@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "hydro_adjoint.h"
+#include "hydro_eb_adjoint.h"
 #include "hydro_fused.h"
 #include "hydro_state.h"
 
@@ -302,6 +303,22 @@ class HydroLevel {
     bool adj_tail_ready_ = false;  // blk_ holds the last block's states from the forward run
     double adj_time0_ = 0.0;
     size_t state_doubles_ = 0;
+    // EB level: the step's adjoint through the cut-cell stages, for every
+    // box that is not covered (the all-regular boxes with regular geometry,
+    // which the cut-cell stages then compute as the regular path does)
+    std::vector<int> eadj_ids_;
+    std::vector<EbBox> eadj_fh_;
+    EbBox *eadj_ftab_ = nullptr;
+    std::vector<EbAdjEnt> eadj_h_;
+    EbAdjEnt *eadj_tab_ = nullptr;
+    double *eadj_fbuf_ = nullptr, *eadj_abuf_ = nullptr;  // forward / adjoint arrays
+    size_t eadj_an_ = 0;                                   // doubles of eadj_abuf_
+    DevFab *tab_ub_ = nullptr, *tab_sbb_ = nullptr;
+    EbGeom *geom_dev_ = nullptr;
+    int *kinds_dev_ = nullptr;
+    EbGeom unit_geom_{};
+    int eb_adjoint_setup();
+    int adjoint_step_eb(double dt, bool check);
     int adjoint_scope() const;
     int adjoint_alloc(int nck, int every);
     int save_state(double *dst);
@@ -311,6 +328,7 @@ class HydroLevel {
 
   public:
     int adjoint_forward(int nsteps, const double *dts, int every);
+    int eb_force_seed(const double w[3]);
     int adjoint_backward();
     int adj_upload(int b, const double *h);
     int adj_download(int b, double *h);
@@ -1719,6 +1737,22 @@ int HydroLevel::run_graph(int nsteps, double *dts) {
 // ------------------------------------------------------------- adjoint
 int HydroLevel::adjoint_scope() const {
     const char *why = nullptr;
+    if (ebflags_) {
+        // the embedded-boundary level (staged engine, PLM; set_eb refuses
+        // gravity sources and the method of lines)
+        bool nsw = false;
+        for (int d = 0; d < 3; ++d)
+            nsw |= desc_.hp.bclo[d] == BC_NOSLIPWALL || desc_.hp.bchi[d] == BC_NOSLIPWALL;
+        if (desc_.hp.allow_small_energy != 1 || desc_.hp.allow_negative_energy != 0)
+            why = "the adjoint needs allow_small_energy = 1, allow_negative_energy = 0";
+        else if (nsw) why = "the embedded-boundary adjoint does not model NoSlipWall faces";
+        else if (ext_ || needs_ext_) why = "the adjoint needs a level covering its domain";
+        if (why) {
+            set_error("%s", why);
+            return CAMR_ERR_ADJOINT;
+        }
+        return CAMR_OK;
+    }
     if (!fused_) why = "the adjoint runs on the fused engine";
     else if (desc_.hp.ppm_type != 1) why = "the adjoint differentiates the PPM path";
     else if (grav_src_values_) why = "the adjoint needs const_grav = 0";
@@ -1758,7 +1792,7 @@ int HydroLevel::load_state(const double *src) {
 }
 
 int HydroLevel::adjoint_alloc(int nck, int every) {
-    int rc = engine_.enable_adjoint();
+    int rc = ebflags_ ? eb_adjoint_setup() : engine_.enable_adjoint();
     if (rc) return rc;
     state_doubles_ = 0;
     for (const Box &B : box_) state_doubles_ += NVAR * B.npts();
@@ -1829,6 +1863,7 @@ int HydroLevel::adjoint_forward(int nsteps, const double *dts, int every) {
 // then run the adjoint kernels from abar_[acur_] (dL/dS_{s+1}) into the
 // other set (dL/dS_s).  The level's state is S_s again on return.
 int HydroLevel::adjoint_step(double dt, bool check) {
+    if (ebflags_) return adjoint_step_eb(dt, check);
     int rc = resolve();
     if (rc) return rc;
     std::swap(snew_, sold_);
@@ -1889,6 +1924,205 @@ int HydroLevel::adjoint_flags() {
     return status_error(*h_status_);
 }
 
+// The arrays of the embedded-boundary adjoint (allocated once) and the
+// box tables (rebuilt on every call, so that geometry set later is seen).
+int HydroLevel::eb_adjoint_setup() {
+    const int nb = (int)box_.size();
+    if (!eadj_fbuf_) {
+        // regular geometry for the all-regular boxes: constant rows indexed
+        // by i only (zero j / k / component strides)
+        // (the flag box's x extent, plus the high nodes)
+        const int64_t lo0 = ebf_.lo0, n0 = ebf_.js + 2;
+        std::vector<double> ones((size_t)n0, 1.0), zeros((size_t)n0, 0.0);
+        std::vector<int32_t> regf((size_t)n0, (int32_t)((1u << 27) - 1));
+        double *po = nullptr, *pz = nullptr;
+        int32_t *pf = nullptr;
+        CAMR_HIP(hipMalloc(&po, n0 * sizeof(double)));
+        CAMR_HIP(hipMalloc(&pz, n0 * sizeof(double)));
+        CAMR_HIP(hipMalloc(&pf, n0 * sizeof(int32_t)));
+        ebbuf_.push_back(po);
+        ebbuf_.push_back(pz);
+        ebbuf_.push_back(reinterpret_cast<double *>(pf));
+        CAMR_HIP(hipMemcpy(po, ones.data(), n0 * sizeof(double), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(pz, zeros.data(), n0 * sizeof(double), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(pf, regf.data(), n0 * sizeof(int32_t), hipMemcpyHostToDevice));
+        auto row = [&](double *p) {
+            Arr a{};
+            a.p = p;
+            a.js = a.ks = a.ns = 0;
+            a.lo0 = (int)lo0; a.lo1 = 0; a.lo2 = 0;
+            return a;
+        };
+        unit_geom_.f.p = pf;
+        unit_geom_.f.lo0 = (int)lo0; unit_geom_.f.lo1 = 0; unit_geom_.f.lo2 = 0;
+        unit_geom_.f.js = 0; unit_geom_.f.ks = 0;
+        unit_geom_.vf = row(po);
+        for (int d = 0; d < 3; ++d) { unit_geom_.ap[d] = row(po); unit_geom_.fc[d] = row(pz); }
+        eadj_ids_.clear();
+        for (int b = 0; b < nb; ++b)
+            if (kind_[b] != 2) eadj_ids_.push_back(b);
+        auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
+        size_t fn = 0, an = 0;
+        for (int b : eadj_ids_) {
+            const Box B = box_[b], G = grow(B, ng_);
+            fn += r32((size_t)(QVAR + NQAUX) * G.npts());
+            for (int d = 0; d < 3; ++d) fn += r32((size_t)NVAR * nodes(B, d).npts());
+            fn += r32(umdrv_eb_scratch_doubles(B, G, true));
+            an += r32((size_t)(QVAR + NQAUX) * G.npts());
+            an += r32(umdrv_eb_scratch_doubles(B, G, true));
+        }
+        for (int b = 0; b < nb; ++b)
+            an += r32((size_t)NVAR * box_[b].npts()) + r32((size_t)NVAR * grow(box_[b], ng_).npts());
+        CAMR_HIP(hipMalloc(&eadj_fbuf_, fn * sizeof(double)));
+        ebbuf_.push_back(eadj_fbuf_);
+        CAMR_HIP(hipMalloc(&eadj_abuf_, an * sizeof(double)));
+        ebbuf_.push_back(eadj_abuf_);
+        eadj_an_ = an;
+        const size_t n = eadj_ids_.size();
+        CAMR_HIP(hipMalloc(&eadj_ftab_, std::max<size_t>(n, 1) * sizeof(EbBox)));
+        ebbuf_.push_back(reinterpret_cast<double *>(eadj_ftab_));
+        CAMR_HIP(hipMalloc(&eadj_tab_, std::max<size_t>(n, 1) * sizeof(EbAdjEnt)));
+        ebbuf_.push_back(reinterpret_cast<double *>(eadj_tab_));
+        CAMR_HIP(hipMalloc(&tab_ub_, nb * sizeof(DevFab)));
+        ebbuf_.push_back(reinterpret_cast<double *>(tab_ub_));
+        CAMR_HIP(hipMalloc(&tab_sbb_, nb * sizeof(DevFab)));
+        ebbuf_.push_back(reinterpret_cast<double *>(tab_sbb_));
+        CAMR_HIP(hipMalloc(&geom_dev_, nb * sizeof(EbGeom)));
+        ebbuf_.push_back(reinterpret_cast<double *>(geom_dev_));
+        CAMR_HIP(hipMalloc(&kinds_dev_, nb * sizeof(int)));
+        ebbuf_.push_back(reinterpret_cast<double *>(kinds_dev_));
+    }
+    // tables
+    auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
+    double *fp = eadj_fbuf_, *ap = eadj_abuf_;
+    std::vector<Arr> ubv(box_.size()), sbbv(box_.size());
+    std::vector<EbGeom> geo(box_.size());
+    eadj_fh_.assign(eadj_ids_.size(), EbBox{});
+    eadj_h_.assign(eadj_ids_.size(), EbAdjEnt{});
+    // the state-sized adjoint arrays of every box come first
+    for (size_t b = 0; b < box_.size(); ++b) {
+        const Box B = box_[b], G = grow(B, ng_);
+        ubv[b] = dense(ap, B);
+        ap += r32((size_t)NVAR * B.npts());
+        sbbv[b] = dense(ap, G);
+        ap += r32((size_t)NVAR * G.npts());
+        geo[b] = kind_[b] == 1 ? ebg_[b] : unit_geom_;
+        if (kind_[b] == 1 && !ebg_[b].vf.p) {
+            set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", (int)b);
+            return CAMR_ERR_PARAM;
+        }
+    }
+    for (size_t c = 0; c < eadj_ids_.size(); ++c) {
+        const int b = eadj_ids_[c];
+        const Box B = box_[b], G = grow(B, ng_);
+        UmdrvEbArgs A;
+        A.bx = B;
+        A.qbox = G;
+        A.hc = hc_;
+        for (int d = 0; d < 3; ++d) A.periodic[d] = desc_.is_periodic[d];
+        A.dtp = dt_dev_;
+        A.uin = sbv_[b];
+        A.dsdt = dense(dsdt_[b], B);
+        A.q = dense(fp, G);
+        A.qaux = dense(fp + (size_t)QVAR * G.npts(), G);
+        A.srcq = A.q;  // no sources
+        A.has_src = 0;
+        fp += r32((size_t)(QVAR + NQAUX) * G.npts());
+        for (int d = 0; d < 3; ++d) {
+            A.flx[d] = dense(fp, nodes(B, d));
+            fp += r32((size_t)NVAR * nodes(B, d).npts());
+        }
+        A.g = geo[b];
+        eb_box_setup(A, fp, &eadj_fh_[c], true);
+        fp += r32(umdrv_eb_scratch_doubles(B, G, true));
+        UmdrvEbArgs Ab = A;
+        Ab.q = dense(ap, G);
+        Ab.qaux = dense(ap + (size_t)QVAR * G.npts(), G);
+        Ab.srcq = Ab.q;
+        ap += r32((size_t)(QVAR + NQAUX) * G.npts());
+        Ab.uin = sbbv[b];
+        Ab.dsdt = ubv[b];
+        eb_box_setup(Ab, ap, &eadj_h_[c].A, true);
+        ap += r32(umdrv_eb_scratch_doubles(B, G, true));
+        eadj_h_[c].F = eadj_fh_[c];
+    }
+    std::vector<DevFab> tu(box_.size()), ts(box_.size());
+    std::vector<int> kd(box_.size());
+    for (size_t b = 0; b < box_.size(); ++b) {
+        tu[b].a = ubv[b]; tu[b].b = box_[b];
+        ts[b].a = sbbv[b]; ts[b].b = box_[b];
+        kd[b] = kind_[b];
+    }
+    const size_t n = eadj_ids_.size();
+    if (n) {
+        CAMR_HIP(hipMemcpy(eadj_ftab_, eadj_fh_.data(), n * sizeof(EbBox), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(eadj_tab_, eadj_h_.data(), n * sizeof(EbAdjEnt), hipMemcpyHostToDevice));
+    }
+    CAMR_HIP(hipMemcpy(tab_ub_, tu.data(), tu.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+    CAMR_HIP(hipMemcpy(tab_sbb_, ts.data(), ts.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+    CAMR_HIP(hipMemcpy(geom_dev_, geo.data(), geo.size() * sizeof(EbGeom), hipMemcpyHostToDevice));
+    CAMR_HIP(hipMemcpy(kinds_dev_, kd.data(), kd.size() * sizeof(int), hipMemcpyHostToDevice));
+    return CAMR_OK;
+}
+
+// One step's adjoint on an embedded-boundary level: recompute the step from
+// S_s (clean_state, FillPatch, primitives, the cut-cell stages of every box
+// that is not covered), then the adjoint kernels from abar_[acur_]
+// (dL/dS_{s+1}) into the other set (dL/dS_s).
+int HydroLevel::adjoint_step_eb(double dt, bool check) {
+    int rc = resolve();
+    if (rc) return rc;
+    std::swap(snew_, sold_);
+    a_is_new_ = !a_is_new_;
+    set_dt_dev(dt);
+    const DevFab *told = a_is_new_ ? tab_b_ : tab_a_;
+    clean_all(told, true);
+    expand_all(told);
+    for (size_t b = 0; b < box_.size(); ++b)
+        launch_on(k_fill_value, box_[b], stream_, dense(dsdt_[b], box_[b]), (int)NVAR, 0.0);
+    for (size_t c = 0; c < eadj_ids_.size(); ++c) {
+        const EbBox &F = eadj_fh_[c];
+        launch_on(k_ctoprim, F.qbox, stream_, F.uin, F.q, F.qaux, hc_, status_);
+    }
+    const int n = (int)eadj_ids_.size();
+    rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
+    if (!rc) {
+        const int out = 1 - acur_;
+        zero_doubles(eadj_abuf_, eadj_an_, stream_);
+        for (size_t b = 0; b < box_.size(); ++b)
+            zero_doubles(abar_[out][b], NVAR * box_[b].npts(), stream_);
+        launch_all(kb_ebadj_post, 0, (const DevFab *)tab_sb_, (const DevFab *)tab_dsdt_,
+                   (const DevFab *)tab_abar_[acur_], (const DevFab *)tab_ub_,
+                   (const DevFab *)tab_sbb_, (const double *)dt_dev_, hc_,
+                   desc_.add_grav_src, status_, ebf_);
+        rc = eb_adjoint_batch(eadj_tab_, eadj_h_.data(), n, hc_, dt_dev_, stream_, &prof_);
+        launch_all(kb_ebadj_fill, ng_, (const DevFab *)tab_sbb_, ng_, told,
+                   (const DevFab *)tab_abar_[out], (const int *)owner_, dom_, hc_, status_);
+        launch_all(kb_ebadj_old, 0, (const DevFab *)tab_abar_[out], told, hc_, status_);
+        acur_ = out;
+    }
+    std::swap(snew_, sold_);
+    a_is_new_ = !a_is_new_;
+    dt_ready_ = false;
+    if (rc) { set_error("kernel launch failed"); return rc; }
+    return check ? adjoint_flags() : CAMR_OK;
+}
+
+// abar_[acur_] = d(w . F)/dS of the force on the embedded body (eb_force)
+// at the level's current state: the seed of the force's adjoint.
+int HydroLevel::eb_force_seed(const double w[3]) {
+    if (!ebflags_) { set_error("the level has no embedded boundary"); return CAMR_ERR_PARAM; }
+    if (abar_[0].empty() || !geom_dev_) { set_error("no adjoint forward run"); return CAMR_ERR_PARAM; }
+    int rc = resolve();
+    if (rc) return rc;
+    const double dx2 = desc_.hp.dx[0] * desc_.hp.dx[0];
+    launch_all(kb_ebadj_force, 0, (const DevFab *)(a_is_new_ ? tab_a_ : tab_b_),
+               (const DevFab *)tab_abar_[acur_], (const EbGeom *)geom_dev_,
+               (const int *)kinds_dev_, w[0], w[1], w[2], dx2, hc_);
+    CAMR_HIP(hipGetLastError());
+    return sync();
+}
+
 int HydroLevel::adjoint_backward() {
     const int nsteps = (int)adj_dts_.size();
     if (nsteps == 0 || abar_[0].empty()) { set_error("no adjoint forward run"); return CAMR_ERR_PARAM; }
@@ -1896,7 +2130,7 @@ int HydroLevel::adjoint_backward() {
     const int nck = (nsteps + every - 1) / every;
     int rc = resolve();
     if (rc) return rc;
-    zero_doubles(engine_.adjoint_base(), engine_.scratch_total(), stream_);
+    if (!ebflags_) zero_doubles(engine_.adjoint_base(), engine_.scratch_total(), stream_);
     for (int c = nck - 1; c >= 0 && !rc; --c) {
         const int start = c * every;
         const int len = std::min(every, nsteps - start);
@@ -3073,6 +3307,10 @@ int camr_level_adj_upload_box(camr_level *L, int32_t b, const double *host) {
 }
 int camr_level_adj_download_box(camr_level *L, int32_t b, double *host) {
     return L ? L->impl.adj_download(b, host) : CAMR_ERR_PARAM;
+}
+int camr_level_adj_seed_eb_force(camr_level *L, const double w[3]) {
+    if (!L || !w) return CAMR_ERR_PARAM;
+    return L->impl.eb_force_seed(w);
 }
 int camr_level_adjoint_backward(camr_level *L) {
     return L ? L->impl.adjoint_backward() : CAMR_ERR_PARAM;

@@ -35,6 +35,29 @@ struct EbGeom {
     Arr vf, ap[3], fc[3];
 };
 
+// Value of face array `u` (component n) of direction d at face (i,j,k),
+// moved to the face centroid (eb_compute_div, Hydro_utils_eb_K.H:410-520).
+template <int D>
+CAMR_HD double centroid_value(const Arr &u, int n, const EbGeom &g, int i,
+                                                 int j, int k) {
+    double v = u(i, j, k, n);
+    const double a = g.ap[D](i, j, k, 0);
+    if (a != 0.0 && a != 1.0) {
+        constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+        const double c0 = g.fc[D](i, j, k, 0), c1 = g.fc[D](i, j, k, 1);
+        const int s0 = (int)copysign(1.0, c0), s1 = (int)copysign(1.0, c1);
+        const int o0x = T0 == 0 ? s0 : 0, o0y = T0 == 1 ? s0 : 0, o0z = T0 == 2 ? s0 : 0;
+        const int o1x = T1 == 0 ? s1 : 0, o1y = T1 == 1 ? s1 : 0, o1z = T1 == 2 ? s1 : 0;
+        const int32_t w = g.f(i, j, k);
+        const double f0 = eb_conn(w, o0x, o0y, o0z) ? fabs(c0) : 0.0;
+        const double f1 = eb_conn(w, o1x, o1y, o1z) ? fabs(c1) : 0.0;
+        v = (1.0 - f0) * (1.0 - f1) * v + f0 * (1.0 - f1) * u(i + o0x, j + o0y, k + o0z, n) +
+            f1 * (1.0 - f0) * u(i + o1x, j + o1y, k + o1z, n) +
+            f0 * f1 * u(i + o0x + o1x, j + o0y + o1y, k + o0z + o1z, n);
+    }
+    return v;
+}
+
 // Arguments of one hydro_umdrv_eb call (Source/Hydro/Hydro_umdrv_eb.cpp:
 // 14-191, Godunov branch, FluxRedist).  q / qaux / srcq and uin cover
 // grow(bx, 6); dsdt (bx) must hold zeros on entry; flx[d] on the faces of bx.
@@ -49,6 +72,47 @@ struct UmdrvEbArgs {
     EbGeom g;
 };
 
+// The index boxes of one box's stages.
+struct Layout {
+    Box bx, btf, bxg1, bxg2, g1;
+    Box qm[3], f1[3], f2[3], ft[3], qe[3], nb, mbx[3];
+};
+
+__host__ __device__ inline Layout layout_of(const Box &bx, const Box &qbox) {
+    Layout L;
+    L.bx = bx;
+    L.btf = grow(bx, 2);
+    L.bxg1 = grow(L.btf, 1);
+    L.bxg2 = grow(L.btf, 2);
+    L.g1 = grow(bx, 1);
+    for (int d = 0; d < 3; ++d) {
+        L.qm[d] = grow_hi(L.bxg2, d, 1);
+        L.f1[d] = nodes(grow_dir(L.bxg2, d, -1), d);
+        L.f2[d] = nodes(L.bxg2, d);
+        L.ft[d] = nodes(grow(bx, 3), d);
+        L.qe[d] = nodes(grow(bx, 4), d);
+        L.mbx[d] = grow_hi(L.bxg2, d, 1);
+    }
+    // divu nodes: those of grow(qbox, -1) (Hydro_adjust_fluxes.cpp:73), no
+    // further out than the nodes of grow(bx, 5) that the face stencils reach
+    const Box a = nodes(nodes(nodes(grow(qbox, -1), 0), 1), 2);
+    const Box b = nodes(nodes(nodes(grow(bx, 5), 0), 1), 2);
+    for (int d = 0; d < 3; ++d) {
+        L.nb.lo[d] = a.lo[d] > b.lo[d] ? a.lo[d] : b.lo[d];
+        L.nb.hi[d] = a.hi[d] < b.hi[d] ? a.hi[d] : b.hi[d];
+    }
+    return L;
+}
+
+// pair p: (D, T, O) with O the third direction
+__host__ __device__ constexpr int pair_d(int p) { return p / 2; }
+__host__ __device__ constexpr int pair_t(int p) {
+    return p / 2 == 0 ? (p % 2 ? 2 : 1) : p / 2 == 1 ? (p % 2 ? 2 : 0) : (p % 2 ? 1 : 0);
+}
+__host__ __device__ constexpr int pair_o(int p) { return 3 - pair_d(p) - pair_t(p); }
+__host__ __device__ constexpr int t0_of(int d) { return d == 0 ? 1 : 0; }
+__host__ __device__ constexpr int t1_of(int d) { return d == 2 ? 1 : 2; }
+
 // One box of a cut-cell batch: its inputs and outputs (those of
 // UmdrvEbArgs) and its scratch arrays, carved by eb_box_setup.
 struct EbBox {
@@ -56,14 +120,21 @@ struct EbBox {
     EbGeom g;
     Arr q, qaux, srcq, uin, dsdt, flx[3];
     Arr QM[3], QP[3], F1[3], G1[3], F2[3][3], G2[3][3], FT[3], QE[3];
-    Arr qmT[3], qpT[3], qmF, qpF, divu, divc, wt, optmp0, delm, winv;
+    // the corrected states of transverse pair p (D|T in the order 0|1, 0|2,
+    // 1|0, 1|2, 2|0, 2|1) and the final states of direction d: one buffer
+    // for all pairs / all directions, or one each when the adjoint reads
+    // them back (`keep`)
+    Arr qmT[6], qpT[6], qmF[3], qpF[3];
+    Arr divu, divc, wt, optmp0, delm, winv;
 };
 
-// Doubles of scratch one box needs (the one-box call's EbBox record included).
-size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox);
+// Doubles of scratch one box needs (the one-box call's EbBox record
+// included); keep: every pair's and direction's states in their own arrays.
+size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox, bool keep = false);
 // Carves the scratch arrays of box A.bx out of `scratch` (device memory of
-// umdrv_eb_scratch_doubles(A.bx, A.qbox) doubles) into *out (host memory).
-void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out);
+// umdrv_eb_scratch_doubles(A.bx, A.qbox, keep) doubles) into *out (host
+// memory).
+void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep = false);
 // hydro_umdrv_eb over the n boxes of tab (device copy) / htab (the same
 // records in host memory, read for the grid sizes): one launch per stage,
 // every box of the table in the launch.  All boxes share hc, dtp and has_src

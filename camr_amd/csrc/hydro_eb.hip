@@ -23,15 +23,16 @@ namespace camr {
 
 namespace {
 
-__device__ __constant__ int kQc[NS] = {QRHO, QU, QV, QW, QPRES, QREINT, QFS, QFS + 1, QFS + 2};
-__device__ __constant__ int kUc[NF] = {URHO, UMX, UMY, UMZ, UEDEN, UEINT, UFS, UFS + 1, UFS + 2};
+// compact state / flux component -> QVAR / NVAR component
+CAMR_HD constexpr int kQc(int s) { return s < 4 ? s : s == 4 ? QPRES : s == 5 ? QREINT : QFS + s - 6; }
+CAMR_HD constexpr int kUc(int s) { return s < 6 ? s : UFS + s - 6; }
 
-__device__ __forceinline__ int dv(int d, int e) { return d == e ? 1 : 0; }
+CAMR_HD int dv(int d, int e) { return d == e ? 1 : 0; }
 
 // One direction of the cut-cell PLM trace: slopes of order 4 / 2 / 0 by the
 // covered neighbours, the trace, states kept on open faces only.
 template <int D>
-__device__ __forceinline__ void plm_eb_dir(int i, int j, int k, const Arr &q, const EbGeom &g,
+CAMR_HD void plm_eb_dir(int i, int j, int k, const Arr &q, const EbGeom &g,
                                            const double *c, double cc, double fl, const Arr &QM,
                                            const Arr &QP, const HydroConst &hc,
                                            const double *dtp) {
@@ -46,7 +47,7 @@ __device__ __forceinline__ void plm_eb_dir(int i, int j, int k, const Arr &q, co
     for (int s = 0; s < NS; ++s) {
         double st[5];
 #pragma unroll
-        for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, kQc[s]);
+        for (int o = -2; o <= 2; ++o) st[o + 2] = q(i + o * ex, j + o * ey, k + o * ez, kQc(s));
         sl[s] = ord ? mc_slope(st, fl, ord) : 0.0;
     }
     double qp[NS], qm[NS];
@@ -63,7 +64,7 @@ __device__ __forceinline__ void plm_eb_dir(int i, int j, int k, const Arr &q, co
 // ---------------------------------------------- PLM trace (cut-cell form)
 // Godunov_3D_eb.cpp:94-152: flatten_eb (flatten.H:104-134), plm_slope_eb
 // (PLM_eb.H:18-54), hydro_plm_d_eb (PLM_eb.H:103-287); covered cells skip.
-__device__ __forceinline__ void
+CAMR_HD void
 b_plm_eb(int i, int j, int k, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr qp0, Arr qm1,
          Arr qp1, Arr qm2, Arr qp2, HydroConst hc, const double *dtp) {
     if (eb_type(g.f(i, j, k)) == kEbCovered) return;
@@ -94,7 +95,7 @@ b_plm_eb(int i, int j, int k, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr 
     }
     double c[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) c[s] = q(i, j, k, kQc[s]);
+    for (int s = 0; s < NS; ++s) c[s] = q(i, j, k, kQc(s));
     const double cc = qaux(i, j, k, QC);
     plm_eb_dir<0>(i, j, k, q, g, c, cc, fl, qm0, qp0, hc, dtp);
     plm_eb_dir<1>(i, j, k, q, g, c, cc, fl, qm1, qp1, hc, dtp);
@@ -106,7 +107,7 @@ b_plm_eb(int i, int j, int k, Arr q, Box qbox, Arr qaux, EbGeom g, Arr qm0, Arr 
 // boundary-mutated states written back); 1: transverse solve (compact out);
 // 2: final solve into the NVAR flux array.
 template <int D, int MODE>
-__device__ __forceinline__ void
+CAMR_HD void
 b_riemann_eb(int i, int j, int k, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, Arr ap, HydroConst hc) {
     if (!(ap(i, j, k, 0) > 0.)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
@@ -133,7 +134,7 @@ b_riemann_eb(int i, int j, int k, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, 
     }
     if (MODE == 2) {
 #pragma unroll
-        for (int s = 0; s < NF; ++s) fout(i, j, k, kUc[s]) = f[s];
+        for (int s = 0; s < NF; ++s) fout(i, j, k, kUc(s)) = f[s];
     } else {
 #pragma unroll
         for (int s = 0; s < NF; ++s) fout(i, j, k, s) = f[s];
@@ -148,7 +149,7 @@ b_riemann_eb(int i, int j, int k, Arr qL, Arr qR, Arr qaux, Arr fout, Arr gout, 
 // differences (the same arithmetic as zero fluxes), a closed normal face
 // passes the predictor state through.
 template <int D, int T>
-__device__ __forceinline__ void
+CAMR_HD void
 b_transdo_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr apD, Arr apT,
              EbFlags fl, Arr qmo, Arr qpo, const double *dtp, HydroConst hc) {
     if (fl.covered(i, j, k)) return;
@@ -186,7 +187,7 @@ b_transdo_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr F, Arr G, Arr 
 // hydro_transdd with norm_area = ap[D], trans0_area = ap[T0], trans1_area =
 // ap[T1] (Godunov_utils_3D.H:245-482).
 template <int D>
-__device__ __forceinline__ void
+CAMR_HD void
 b_transdd_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_src, Arr fa, Arr ga,
              Arr fb, Arr gb, Arr apD, Arr apA, Arr apB, EbFlags fl, Arr qm, Arr qp,
              const double *dtp, HydroConst hc) {
@@ -221,7 +222,7 @@ b_transdd_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_
     double src[NS];
     if (has_src) {
 #pragma unroll
-        for (int s = 0; s < NS; ++s) src[s] = srcq(i, j, k, kQc[s]);
+        for (int s = 0; s < NS; ++s) src[s] = srcq(i, j, k, kQc(s));
     }
     double n[NS], o[NS];
 #pragma unroll
@@ -243,7 +244,7 @@ b_transdd_eb(int i, int j, int k, Arr qnm, Arr qnp, Arr qaux, Arr srcq, int has_
 // ------------------------------------------------------------- eb_divu
 // Hydro_utils_eb_K.H:31-78 on nodes: zero unless the 8 cells around the
 // node are all full.
-__device__ __forceinline__ void
+CAMR_HD void
 b_eb_divu(int i, int j, int k, Arr q, Arr vf, Arr divu, double dxi0, double dxi1, double dxi2) {
     const double sv = vf(i, j, k, 0) + vf(i - 1, j, k, 0) + vf(i, j - 1, k, 0) +
                       vf(i - 1, j - 1, k, 0) + vf(i, j, k - 1, 0) + vf(i - 1, j, k - 1, 0) +
@@ -274,7 +275,7 @@ b_eb_divu(int i, int j, int k, Arr q, Arr vf, Arr divu, double dxi0, double dxi1
 // area factor only where the aperture is open (Hydro_adjust_fluxes.cpp:
 // 96-165, hydro_ext_flx_eb Hydro_utils_eb_K.H:12-27).
 template <int D>
-__device__ __forceinline__ void
+CAMR_HD void
 b_adjust_eb(int i, int j, int k, Arr flx, Arr divu, Arr u, Arr ap, double area_c, HydroConst hc) {
     double f[NVAR];
 #pragma unroll
@@ -284,33 +285,10 @@ b_adjust_eb(int i, int j, int k, Arr flx, Arr divu, Arr u, Arr ap, double area_c
     for (int n = 0; n < NVAR; ++n) flx(i, j, k, n) = f[n];
 }
 
-// Value of face array `u` (component n) of direction d at face (i,j,k),
-// moved to the face centroid (eb_compute_div, Hydro_utils_eb_K.H:410-520).
-template <int D>
-__device__ __forceinline__ double centroid_value(const Arr &u, int n, const EbGeom &g, int i,
-                                                 int j, int k) {
-    double v = u(i, j, k, n);
-    const double a = g.ap[D](i, j, k, 0);
-    if (a != 0.0 && a != 1.0) {
-        constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
-        const double c0 = g.fc[D](i, j, k, 0), c1 = g.fc[D](i, j, k, 1);
-        const int s0 = (int)copysign(1.0, c0), s1 = (int)copysign(1.0, c1);
-        const int o0x = T0 == 0 ? s0 : 0, o0y = T0 == 1 ? s0 : 0, o0z = T0 == 2 ? s0 : 0;
-        const int o1x = T1 == 0 ? s1 : 0, o1y = T1 == 1 ? s1 : 0, o1z = T1 == 2 ? s1 : 0;
-        const int32_t w = g.f(i, j, k);
-        const double f0 = eb_conn(w, o0x, o0y, o0z) ? fabs(c0) : 0.0;
-        const double f1 = eb_conn(w, o1x, o1y, o1z) ? fabs(c1) : 0.0;
-        v = (1.0 - f0) * (1.0 - f1) * v + f0 * (1.0 - f1) * u(i + o0x, j + o0y, k + o0z, n) +
-            f1 * (1.0 - f0) * u(i + o1x, j + o1y, k + o1z, n) +
-            f0 * f1 * u(i + o0x + o1x, j + o0y + o1y, k + o0z + o1z, n);
-    }
-    return v;
-}
-
 // compute_hyp_wallflux (Hydro_hypflux_eb_K.H:12-68): the flux through the
 // embedded wall of cut cell (i,j,k), a Riemann problem of the cell state
 // against its mirror across the wall normal from the apertures.
-__device__ __forceinline__ void eb_wallflux(const Arr &q, const Arr &qaux, const EbGeom &g, int i,
+CAMR_HD void eb_wallflux(const Arr &q, const Arr &qaux, const EbGeom &g, int i,
                                             int j, int k, const HydroConst &hc, double *fw) {
     const double axm = g.ap[0](i, j, k, 0), axp = g.ap[0](i + 1, j, k, 0);
     const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
@@ -342,7 +320,7 @@ __device__ __forceinline__ void eb_wallflux(const Arr &q, const Arr &qaux, const
 // redistribution weight (eb_weights_type 2: rho), the face-centroid fluxes
 // of the valid box, then eb_pdivu (:82-280).
 template <bool CUT>
-__device__ __forceinline__ void
+CAMR_HD void
 b_eb_div(int i, int j, int k, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, EbGeom g,
          Arr q, Arr qaux, Arr divc, Arr wt, Arr fo0, Arr fo1, Arr fo2, HydroConst hc,
          double dxi0, double dxi1, double dxi2) {
@@ -422,7 +400,7 @@ b_eb_div(int i, int j, int k, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge
 // registers, use_wts_in_divnc false), over grow(bx, 1): the cut cell's
 // own share optmp0 and the mass deficit delm it hands out, and 1 / (sum of
 // its neighbours' weight * vfrac).
-__device__ __forceinline__ void
+CAMR_HD void
 b_redist_src(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv) {
     const int32_t w = g.f(i, j, k);
     if (eb_type(w) != kEbSingle) {
@@ -473,7 +451,7 @@ b_redist_src(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr opt
 // Receiving side over bx: the shares of the cut neighbours in the order of
 // the reference's sequential scatter (source cells in increasing k, j, i),
 // then dsdt = divc + optmp on non-covered cells.
-__device__ __forceinline__ void
+CAMR_HD void
 b_redist_dst(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr optmp0, Arr delm, Arr winv,
              Arr dsdt) {
     if (g.f.covered(i, j, k)) return;
@@ -506,44 +484,12 @@ b_redist_dst(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr opt
     }
 }
 
-__device__ __forceinline__ void
+CAMR_HD void
 b_zero(int i, int j, int k, Arr a, int nc) {
     for (int n = 0; n < nc; ++n) a(i, j, k, n) = 0.0;
 }
 
 // ---------------------------------------------------------------- driver
-// The index boxes of one box's stages.
-struct Layout {
-    Box bx, btf, bxg1, bxg2, g1;
-    Box qm[3], f1[3], f2[3], ft[3], qe[3], nb, mbx[3];
-};
-
-__host__ __device__ inline Layout layout_of(const Box &bx, const Box &qbox) {
-    Layout L;
-    L.bx = bx;
-    L.btf = grow(bx, 2);
-    L.bxg1 = grow(L.btf, 1);
-    L.bxg2 = grow(L.btf, 2);
-    L.g1 = grow(bx, 1);
-    for (int d = 0; d < 3; ++d) {
-        L.qm[d] = grow_hi(L.bxg2, d, 1);
-        L.f1[d] = nodes(grow_dir(L.bxg2, d, -1), d);
-        L.f2[d] = nodes(L.bxg2, d);
-        L.ft[d] = nodes(grow(bx, 3), d);
-        L.qe[d] = nodes(grow(bx, 4), d);
-        L.mbx[d] = grow_hi(L.bxg2, d, 1);
-    }
-    // divu nodes: those of grow(qbox, -1) (Hydro_adjust_fluxes.cpp:73), no
-    // further out than the nodes of grow(bx, 5) that the face stencils reach
-    const Box a = nodes(nodes(nodes(grow(qbox, -1), 0), 1), 2);
-    const Box b = nodes(nodes(nodes(grow(bx, 5), 0), 1), 2);
-    for (int d = 0; d < 3; ++d) {
-        L.nb.lo[d] = a.lo[d] > b.lo[d] ? a.lo[d] : b.lo[d];
-        L.nb.hi[d] = a.hi[d] < b.hi[d] ? a.hi[d] : b.hi[d];
-    }
-    return L;
-}
-
 // The stages of hydro_umdrv_eb in launch order: one launch site each, run
 // over every box of a table (grid.y = box).  The transverse pairs D|T go in
 // the order of Godunov_3D_eb.cpp:214-455, (0|1, 0|2, 1|0, 1|2, 2|0, 2|1),
@@ -563,15 +509,6 @@ enum EbSite {
     ES_RDST,
     ES_COUNT
 };
-// pair p: (D, T, O) with O the third direction
-__host__ __device__ constexpr int pair_d(int p) { return p / 2; }
-__host__ __device__ constexpr int pair_t(int p) {
-    return p / 2 == 0 ? (p % 2 ? 2 : 1) : p / 2 == 1 ? (p % 2 ? 2 : 0) : (p % 2 ? 1 : 0);
-}
-__host__ __device__ constexpr int pair_o(int p) { return 3 - pair_d(p) - pair_t(p); }
-__host__ __device__ constexpr int t0_of(int d) { return d == 0 ? 1 : 0; }
-__host__ __device__ constexpr int t1_of(int d) { return d == 2 ? 1 : 2; }
-
 template <int S>
 __host__ __device__ inline Box site_box(const EbBox &B, const Layout &L) {
     // ES_ZERO: the nodes of grow(bx, 5), which hold ft, qe, nb and btf
@@ -596,7 +533,7 @@ __host__ __device__ inline Box site_box(const EbBox &B, const Layout &L) {
 // The stage's per-point work with box B's arrays (the arguments of the
 // former per-box launches, unchanged).
 template <int S>
-__device__ __forceinline__ void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
+CAMR_HD void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
                                           const HydroConst &hc, const double *dtp, int has_src) {
     if constexpr (S == ES_ZERO) {
 #pragma unroll
@@ -619,18 +556,19 @@ __device__ __forceinline__ void site_body(const EbBox &B, const Layout &L, int i
         constexpr int p = (S - ES_PAIR) / 2, D = pair_d(p), T = pair_t(p);
         if constexpr ((S - ES_PAIR) % 2 == 0)
             b_transdo_eb<D, T>(i, j, k, B.QM[D], B.QP[D], B.qaux, B.F1[T], B.G1[T], B.g.ap[D],
-                               B.g.ap[T], B.g.f, B.qmT[D], B.qpT[D], dtp, hc);
+                               B.g.ap[T], B.g.f, B.qmT[p], B.qpT[p], dtp, hc);
         else
-            b_riemann_eb<D, 1>(i, j, k, B.qmT[D], B.qpT[D], B.qaux, B.F2[D][T], B.G2[D][T],
+            b_riemann_eb<D, 1>(i, j, k, B.qmT[p], B.qpT[p], B.qaux, B.F2[D][T], B.G2[D][T],
                                B.g.ap[D], hc);
     } else if constexpr (S >= ES_FINAL && S < ES_FINAL + 6) {
         constexpr int D = (S - ES_FINAL) / 2, T0 = t0_of(D), T1 = t1_of(D);
         if constexpr ((S - ES_FINAL) % 2 == 0)
             b_transdd_eb<D>(i, j, k, B.QM[D], B.QP[D], B.qaux, B.srcq, has_src, B.F2[T0][T1],
                             B.G2[T0][T1], B.F2[T1][T0], B.G2[T1][T0], B.g.ap[D], B.g.ap[T0],
-                            B.g.ap[T1], B.g.f, B.qmF, B.qpF, dtp, hc);
+                            B.g.ap[T1], B.g.f, B.qmF[D], B.qpF[D], dtp, hc);
         else
-            b_riemann_eb<D, 2>(i, j, k, B.qmF, B.qpF, B.qaux, B.FT[D], B.QE[D], B.g.ap[D], hc);
+            b_riemann_eb<D, 2>(i, j, k, B.qmF[D], B.qpF[D], B.qaux, B.FT[D], B.QE[D], B.g.ap[D],
+                               hc);
     } else if constexpr (S == ES_DIVU) {
         b_eb_divu(i, j, k, B.q, B.g.vf, B.divu, 1.0 / hc.dx[0], 1.0 / hc.dx[1], 1.0 / hc.dx[2]);
     } else if constexpr (S >= ES_ADJ && S < ES_ADJ + 3) {
@@ -707,7 +645,7 @@ constexpr size_t kEbHead = (sizeof(EbBox) + 255) / 256 * 32;  // doubles, 256-by
 
 } // namespace
 
-size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox) {
+size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox, bool keep) {
     const Layout L = layout_of(bx, qbox);
     size_t n = 0;
     for (int d = 0; d < 3; ++d) {
@@ -718,15 +656,15 @@ size_t umdrv_eb_scratch_doubles(const Box &bx, const Box &qbox) {
     }
     size_t t = 0;
     for (int d = 0; d < 3; ++d) t = std::max<size_t>(t, L.mbx[d].npts());
-    n += (size_t)2 * NS * t;                                          // qmT, qpT
-    n += (size_t)NS * (L.bxg2.npts() + L.bxg1.npts());               // final qm, qp
+    n += (size_t)(keep ? 6 : 1) * 2 * NS * t;                         // qmT, qpT
+    n += (size_t)(keep ? 3 : 1) * NS * (L.bxg2.npts() + L.bxg1.npts());  // final qm, qp
     n += L.nb.npts();                                                 // divu
     n += (size_t)(NVAR + 1) * L.btf.npts();                           // divc, wt
     n += (size_t)(2 * NVAR + 1) * L.g1.npts();                        // optmp0, delm, winv
     return n + 64 + kEbHead;
 }
 
-void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out) {
+void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep) {
     const Layout L = layout_of(A.bx, A.qbox);
     EbBox B;
     memset(&B, 0, sizeof(B));
@@ -757,12 +695,28 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out) {
     }
     size_t tmax = 0;
     for (int d = 0; d < 3; ++d) tmax = std::max<size_t>(tmax, L.mbx[d].npts());
-    double *tq0 = cur; cur += (size_t)NS * tmax;
-    double *tq1 = cur; cur += (size_t)NS * tmax;
-    // the pair states of direction D: one buffer, the layout of mbx[D]
-    for (int d = 0; d < 3; ++d) { B.qmT[d] = dense(tq0, L.mbx[d]); B.qpT[d] = dense(tq1, L.mbx[d]); }
-    B.qmF = take(L.bxg2, NS);
-    B.qpF = take(L.bxg1, NS);
+    // the pair states of pair p: the layout of mbx[D] (D the pair's normal
+    // direction) in one buffer for all pairs, or one buffer each
+    for (int p = 0; p < 6; ++p) {
+        if (p == 0 || keep) {
+            double *tq0 = cur; cur += (size_t)NS * tmax;
+            double *tq1 = cur; cur += (size_t)NS * tmax;
+            B.qmT[p] = dense(tq0, L.mbx[pair_d(p)]);
+            B.qpT[p] = dense(tq1, L.mbx[pair_d(p)]);
+        } else {
+            B.qmT[p] = dense(B.qmT[0].p, L.mbx[pair_d(p)]);
+            B.qpT[p] = dense(B.qpT[0].p, L.mbx[pair_d(p)]);
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (d == 0 || keep) {
+            B.qmF[d] = take(L.bxg2, NS);
+            B.qpF[d] = take(L.bxg1, NS);
+        } else {
+            B.qmF[d] = B.qmF[0];
+            B.qpF[d] = B.qpF[0];
+        }
+    }
     B.divu = take(L.nb, 1);
     B.divc = take(L.btf, NVAR);
     B.wt = take(L.btf, 1);

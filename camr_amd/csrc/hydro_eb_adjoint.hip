@@ -1,0 +1,1024 @@
+// hydro_eb_adjoint.hip -- reverse-mode adjoint of the embedded-boundary
+// level step: hydro_umdrv_eb (Hydro_umdrv_eb.cpp:14-191; Godunov_umeth_eb,
+// Godunov_3D_eb.cpp:16-567; adjust_fluxes_eb, Hydro_adjust_fluxes.cpp:
+// 48-170; hydro_consup_eb, Hydro_consup.cpp:29-98; AMReX's flux
+// redistribution) stage by stage in reverse order, then the update, the
+// clean_state passes and FillPatch of CAMR_advance (CAMR_advance.cpp:
+// 83-154).
+//
+// The primal intermediates are those of a recompute of the step through the
+// batched cut-cell stages (hydro_eb.hip) with every transverse pair's and
+// direction's states in their own arrays; the adjoint of each intermediate
+// lives in an array of the same layout.  Each kernel recomputes its own
+// per-point forward values and applies the per-point adjoints of
+// adjoint_math.h / eb_adjoint_math.h; values shared between threads
+// (stencil neighbours, fluxes read by several cells, ghost images) receive
+// FP64 atomics.  The hydrodynamic variables (rho, rho u, rho E, rho e) carry
+// the adjoint; species and T are passive (nothing on the path reads them
+// back into those variables).  Covered cells and closed faces contribute
+// nothing, as they do not in the forward step.
+#include "eb_adjoint_math.h"
+#include "hydro_eb_adjoint.h"
+#include "state_math.h"
+
+#include <utility>
+
+namespace camr {
+
+namespace {
+
+constexpr int NH = 6;  // hydro components: URHO..UEINT == SR..SE == FR..FEI
+
+// compact hydro component -> QVAR component
+CAMR_HD constexpr int kQa(int s) { return s < 4 ? s : s == 4 ? QPRES : QREINT; }
+
+// Accumulation into an adjoint that other threads also update: an FP64
+// atomic on the device (a plain add in a host build of these functions).
+CAMR_HD void aadd(double *p, double v) {
+    if (v == 0.0) return;
+#ifdef __HIP_DEVICE_COMPILE__
+    unsafeAtomicAdd(p, v);
+#else
+    *p += v;
+#endif
+}
+CAMR_HD void aadd(const Arr &a, int i, int j, int k, int n, double v) {
+    aadd(&a(i, j, k, n), v);
+}
+
+// ------------------------------------------------------------ helpers
+CAMR_HD bool tdb_any(const TransDiffBar &t) {
+    return t.rho != 0.0 || t.mu != 0.0 || t.mv != 0.0 || t.mw != 0.0 || t.e != 0.0 ||
+           t.dAup != 0.0 || t.pav != 0.0 || t.dAu != 0.0 || t.pnew != 0.0;
+}
+
+template <int D>
+CAMR_HD FaceAux face_aux(const Arr &qaux, int i, int j, int k) {
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    FaceAux a;
+    a.c_l = qaux(i - ex, j - ey, k - ez, QC);
+    a.c_r = qaux(i, j, k, QC);
+    a.csml_l = qaux(i - ex, j - ey, k - ez, QCSML);
+    a.csml_r = qaux(i, j, k, QCSML);
+    a.gam_l = qaux(i - ex, j - ey, k - ez, QGAMC);
+    a.gam_r = qaux(i, j, k, QGAMC);
+    return a;
+}
+template <int D>
+CAMR_HD void face_aux_adj(const Arr &qab, int i, int j, int k,
+                                             const FaceAuxBar &ab) {
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    aadd(qab, i - ex, j - ey, k - ez, QC, ab.c_l);
+    aadd(qab, i - ex, j - ey, k - ez, QCSML, ab.csml_l);
+    aadd(qab, i, j, k, QC, ab.c_r);
+    aadd(qab, i, j, k, QCSML, ab.csml_r);
+}
+
+// Riemann solve of face (i,j,k) in reverse: fb / gb the adjoints of its
+// compact fluxes and Godunov (u, p); qLb / qRb (face arrays) and the sound
+// speeds' adjoints accumulate.
+template <int D>
+CAMR_HD void riemann_face_adj(int i, int j, int k, const Arr &qL, const Arr &qR,
+                                                 const Arr &qaux, const Arr &qLb, const Arr &qRb,
+                                                 const Arr &qab, const double *fb,
+                                                 const double *gb, const HydroConst &hc) {
+    double ql[NS], qr[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        ql[s] = qL(i, j, k, s);
+        qr[s] = qR(i, j, k, s);
+    }
+    const FaceAux a = face_aux<D>(qaux, i, j, k);
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    double qlb[NS] = {}, qrb[NS] = {};
+    FaceAuxBar ab{};
+    face_flux_bc_adj(D, idx, hc, ql, qr, a, fb, gb, qlb, qrb, ab);
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+        aadd(qLb, i, j, k, s, qlb[s]);
+        aadd(qRb, i, j, k, s, qrb[s]);
+    }
+    face_aux_adj<D>(qab, i, j, k, ab);
+}
+
+// centroid_value<D> in reverse: yb to the (up to four) face values it
+// interpolates (eb_compute_div, Hydro_utils_eb_K.H:410-520).
+template <int D>
+CAMR_HD void centroid_adj(const Arr &ub, int n, const EbGeom &g, int i, int j,
+                                             int k, double yb) {
+    if (yb == 0.0) return;
+    const double a = g.ap[D](i, j, k, 0);
+    if (a != 0.0 && a != 1.0) {
+        constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+        const double c0 = g.fc[D](i, j, k, 0), c1 = g.fc[D](i, j, k, 1);
+        const int s0 = (int)copysign(1.0, c0), s1 = (int)copysign(1.0, c1);
+        const int o0x = T0 == 0 ? s0 : 0, o0y = T0 == 1 ? s0 : 0, o0z = T0 == 2 ? s0 : 0;
+        const int o1x = T1 == 0 ? s1 : 0, o1y = T1 == 1 ? s1 : 0, o1z = T1 == 2 ? s1 : 0;
+        const int32_t w = g.f(i, j, k);
+        const double f0 = eb_conn(w, o0x, o0y, o0z) ? fabs(c0) : 0.0;
+        const double f1 = eb_conn(w, o1x, o1y, o1z) ? fabs(c1) : 0.0;
+        aadd(ub, i, j, k, n, (1.0 - f0) * (1.0 - f1) * yb);
+        aadd(ub, i + o0x, j + o0y, k + o0z, n, f0 * (1.0 - f1) * yb);
+        aadd(ub, i + o1x, j + o1y, k + o1z, n, f1 * (1.0 - f0) * yb);
+        aadd(ub, i + o0x + o1x, j + o0y + o1y, k + o0z + o1z, n, f0 * f1 * yb);
+    } else {
+        aadd(ub, i, j, k, n, yb);
+    }
+}
+
+// ---------------------------------------------------------------- sites
+// In launch order (the reverse of the forward stages).
+enum AdjSite {
+    AS_RDST = 0,
+    AS_RSRC,
+    AS_DIVCUT,
+    AS_DIV,
+    AS_ADJ,                 // + (2 - D): D = 2, 1, 0
+    AS_DIVU = AS_ADJ + 3,
+    AS_FINAL,               // + 2 (2 - D) Riemann, + 2 (2 - D) + 1 transdd
+    AS_PAIR = AS_FINAL + 6, // + 2 (5 - p) Riemann, + 2 (5 - p) + 1 transdo
+    AS_R1 = AS_PAIR + 12,   // + (2 - D)
+    AS_PLM = AS_R1 + 3,
+    AS_PRIM,
+    AS_COUNT
+};
+
+template <int S>
+__host__ __device__ inline Box adj_box(const EbBox &B, const Layout &L) {
+    if constexpr (S == AS_RDST) return B.bx;
+    else if constexpr (S == AS_RSRC) return L.g1;
+    else if constexpr (S == AS_DIVCUT || S == AS_DIV) return L.btf;
+    else if constexpr (S >= AS_ADJ && S < AS_ADJ + 3) return L.ft[2 - (S - AS_ADJ)];
+    else if constexpr (S == AS_DIVU) return B.nb;
+    else if constexpr (S >= AS_FINAL && S < AS_FINAL + 6) {
+        constexpr int d = 2 - (S - AS_FINAL) / 2;
+        if constexpr ((S - AS_FINAL) % 2 == 0)
+            return grow_dir(grow_dir(nodes(L.btf, d), t0_of(d), 1), t1_of(d), 1);
+        else return L.bxg1;
+    } else if constexpr (S >= AS_PAIR && S < AS_PAIR + 12) {
+        constexpr int p = 5 - (S - AS_PAIR) / 2;
+        if constexpr ((S - AS_PAIR) % 2 == 0)
+            return grow_dir(nodes(L.bxg1, pair_d(p)), pair_o(p), 1);
+        else return grow_dir(L.bxg2, pair_t(p), -1);
+    } else if constexpr (S >= AS_R1 && S < AS_R1 + 3) return L.f1[2 - (S - AS_R1)];
+    else if constexpr (S == AS_PLM) return L.bxg2;
+    else return B.qbox;
+}
+
+// -------------------------------------------------- flux redistribution
+// b_redist_dst in reverse: dsdt = divc + optmp0 + the senders' shares.
+CAMR_HD void adj_rdst(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                         double dt) {
+    const EbGeom &g = F.g;
+    if (g.f.covered(i, j, k)) return;
+    const double mask = F.dbox2.contains(i, j, k) ? 1.0 : 0.0;
+    const double wtt = F.wt(i, j, k, 0);
+    uint32_t src = 0;
+#pragma unroll
+    for (int ok = -1; ok <= 1; ok++)
+#pragma unroll
+        for (int oj = -1; oj <= 1; oj++)
+#pragma unroll
+            for (int oi = -1; oi <= 1; oi++) {
+                if (oi == 0 && oj == 0 && ok == 0) continue;
+                const int32_t ws = g.f(i + oi, j + oj, k + ok);
+                if (eb_type(ws) == kEbSingle && eb_conn(ws, -oi, -oj, -ok))
+                    src |= 1u << ((oi + 1) + 3 * (oj + 1) + 9 * (ok + 1));
+            }
+    double wttb = 0.0;
+    for (int n = 0; n < NH; ++n) {
+        const double db = dt * A.dsdt(i, j, k, n);
+        if (db == 0.0) continue;
+        A.divc(i, j, k, n) += db;
+        A.optmp0(i, j, k, n) += db;
+        for (uint32_t m = src; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ok = t / 9 - 1, oj = (t / 3) % 3 - 1, oi = t % 3 - 1;
+            const double dl = F.delm(i + oi, j + oj, k + ok, n);
+            const double wi = F.winv(i + oi, j + oj, k + ok, 0);
+            aadd(A.delm, i + oi, j + oj, k + ok, n, db * wi * wtt * mask);
+            aadd(A.winv, i + oi, j + oj, k + ok, 0, db * dl * wtt * mask);
+            wttb += db * dl * wi * mask;
+        }
+    }
+    aadd(A.wt, i, j, k, 0, wttb);
+}
+
+// b_redist_src in reverse (single cells only; the others wrote zeros).
+CAMR_HD void adj_rsrc(const EbBox &F, const EbBox &A, int i, int j, int k) {
+    const EbGeom &g = F.g;
+    const int32_t w = g.f(i, j, k);
+    if (eb_type(w) != kEbSingle) return;
+    const Box &dbox2 = F.dbox2;
+    const double vfc = g.vf(i, j, k, 0);
+    const double mask = dbox2.contains(i, j, k) ? 1.0 : 0.0;
+    const uint32_t con = (uint32_t)w & ((1u << 27) - 1) & ~(1u << 13);
+    uint32_t in = 0;
+    for (uint32_t m = con; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        if (dbox2.contains(i + t % 3 - 1, j + (t / 3) % 3 - 1, k + t / 9 - 1)) in |= 1u << t;
+    }
+    double vtot = 0.0;
+    for (uint32_t m = in; m; m &= m - 1) {
+        const int t = __builtin_ctz(m);
+        vtot += g.vf(i + t % 3 - 1, j + (t / 3) % 3 - 1, k + t / 9 - 1, 0) * 1.0;
+    }
+    for (int n = 0; n < NH; ++n) {
+        const double opb = A.optmp0(i, j, k, n) + A.delm(i, j, k, n) * -(vfc);
+        if (opb == 0.0) continue;
+        const double xb = opb * (1 - vfc) * mask;  // of divnc - divc(i)
+        aadd(A.divc, i, j, k, n, -xb);
+        const double sb = xb / vtot;
+        for (uint32_t m = in; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+            const double wf = g.vf(i + ii, j + jj, k + kk, 0) * 1.0;
+            aadd(A.divc, i + ii, j + jj, k + kk, n, sb * wf);
+        }
+    }
+    const double wib = A.winv(i, j, k, 0);
+    if (wib != 0.0) {
+        double wtot = 0.0;
+        for (uint32_t m = con; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+            wtot += F.wt(i + ii, j + jj, k + kk, 0) * g.vf(i + ii, j + jj, k + kk, 0) *
+                    (dbox2.contains(i + ii, j + jj, k + kk) ? 1.0 : 0.0);
+        }
+        const double wtb = -wib / (wtot * wtot);
+        for (uint32_t m = con; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int ii = t % 3 - 1, jj = (t / 3) % 3 - 1, kk = t / 9 - 1;
+            aadd(A.wt, i + ii, j + jj, k + kk, 0,
+                 wtb * g.vf(i + ii, j + jj, k + kk, 0) *
+                     (dbox2.contains(i + ii, j + jj, k + kk) ? 1.0 : 0.0));
+        }
+    }
+}
+
+// ------------------------------------------------------- hydro_consup_eb
+// b_eb_div in reverse.  CUT: the cut cells; otherwise the regular ones.
+template <bool CUT>
+CAMR_HD void adj_div(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                        const HydroConst &hc) {
+    const EbGeom &g = F.g;
+    const int32_t w = g.f(i, j, k);
+    if (eb_type(w) == kEbCovered) return;
+    if (CUT != (eb_type(w) == kEbSingle)) return;
+    const double dxi0 = 1.0 / hc.dx[0], dxi1 = 1.0 / hc.dx[1], dxi2 = 1.0 / hc.dx[2];
+    const double cvi = dxi0 * dxi1 * dxi2;
+    // wt = rho
+    aadd(A.q, i, j, k, QRHO, A.wt(i, j, k, 0));
+    if constexpr (!CUT) {
+        for (int n = 0; n < NH; ++n) {
+            const double vb = -A.divc(i, j, k, n);
+            if (vb == 0.0) continue;
+            aadd(A.FT[0], i + 1, j, k, n, cvi * vb);
+            aadd(A.FT[0], i, j, k, n, -cvi * vb);
+            aadd(A.FT[1], i, j + 1, k, n, cvi * vb);
+            aadd(A.FT[1], i, j, k, n, -cvi * vb);
+            aadd(A.FT[2], i, j, k + 1, n, cvi * vb);
+            aadd(A.FT[2], i, j, k, n, -cvi * vb);
+        }
+        // pdivu = 0.5 (sum_d (p(+) + p(-)) (u(+) - u(-)) dxi_d)
+        const double pb = 0.5 * -A.divc(i, j, k, UEINT);
+        if (pb != 0.0) {
+            const Arr *GE[3] = {&F.QE[0], &F.QE[1], &F.QE[2]};
+            const Arr *GEb[3] = {&A.QE[0], &A.QE[1], &A.QE[2]};
+            const double dxi[3] = {dxi0, dxi1, dxi2};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int ex = d == 0, ey = d == 1, ez = d == 2;
+                const Arr &ge = *GE[d], &geb = *GEb[d];
+                const double P = ge(i + ex, j + ey, k + ez, GP) + ge(i, j, k, GP);
+                const double U = ge(i + ex, j + ey, k + ez, GUN) - ge(i, j, k, GUN);
+                aadd(geb, i + ex, j + ey, k + ez, GP, pb * U * dxi[d]);
+                aadd(geb, i, j, k, GP, pb * U * dxi[d]);
+                aadd(geb, i + ex, j + ey, k + ez, GUN, pb * P * dxi[d]);
+                aadd(geb, i, j, k, GUN, -pb * P * dxi[d]);
+            }
+        }
+    } else {
+        const double axm = g.ap[0](i, j, k, 0), axp = g.ap[0](i + 1, j, k, 0);
+        const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
+        const double azm = g.ap[2](i, j, k, 0), azp = g.ap[2](i, j, k + 1, 0);
+        const double vfr = g.vf(i, j, k, 0);
+        double fwb[NVAR] = {};
+        for (int n = 0; n < NH; ++n) {
+            const double vb = -A.divc(i, j, k, n);
+            if (vb == 0.0) continue;
+            fwb[n] = vb * dxi0 / vfr;
+            const double sb = vb * (cvi / vfr);
+            centroid_adj<0>(A.FT[0], n, g, i, j, k, -sb * axm);
+            centroid_adj<0>(A.FT[0], n, g, i + 1, j, k, sb * axp);
+            centroid_adj<1>(A.FT[1], n, g, i, j, k, -sb * aym);
+            centroid_adj<1>(A.FT[1], n, g, i, j + 1, k, sb * ayp);
+            centroid_adj<2>(A.FT[2], n, g, i, j, k, -sb * azm);
+            centroid_adj<2>(A.FT[2], n, g, i, j, k + 1, sb * azp);
+        }
+        // wall flux of the cell
+        {
+            const double wv[6] = {F.q(i, j, k, QRHO), F.q(i, j, k, QU), F.q(i, j, k, QV),
+                                  F.q(i, j, k, QW), F.q(i, j, k, QPRES), F.q(i, j, k, QREINT)};
+            double wb[6] = {};
+            double cb = 0.0, csb = 0.0;
+            wallflux_adj(wv, F.qaux(i, j, k, QC), F.qaux(i, j, k, QCSML), axm - axp, aym - ayp,
+                         azm - azp, hc, fwb, wb, cb, csb);
+#pragma unroll
+            for (int s = 0; s < NH; ++s) aadd(A.q, i, j, k, kQa(s), wb[s]);
+            aadd(A.qaux, i, j, k, QC, cb);
+            aadd(A.qaux, i, j, k, QCSML, csb);
+        }
+        // eb_pdivu: divc(UEINT) -= (1 / vfr) p (sum_d dxi_d (a+ u+ - a- u-))
+        const double Pb = -A.divc(i, j, k, UEINT);
+        if (Pb != 0.0) {
+            const double fxm = axm > 0.0 ? centroid_value<0>(F.QE[0], GUN, g, i, j, k) : 0.0;
+            const double fxp = axp > 0.0 ? centroid_value<0>(F.QE[0], GUN, g, i + 1, j, k) : 0.0;
+            const double fym = aym > 0.0 ? centroid_value<1>(F.QE[1], GUN, g, i, j, k) : 0.0;
+            const double fyp = ayp > 0.0 ? centroid_value<1>(F.QE[1], GUN, g, i, j + 1, k) : 0.0;
+            const double fzm = azm > 0.0 ? centroid_value<2>(F.QE[2], GUN, g, i, j, k) : 0.0;
+            const double fzp = azp > 0.0 ? centroid_value<2>(F.QE[2], GUN, g, i, j, k + 1) : 0.0;
+            const double X = dxi0 * (axp * fxp - axm * fxm) + dxi1 * (ayp * fyp - aym * fym) +
+                             dxi2 * (azp * fzp - azm * fzm);
+            const double p = F.q(i, j, k, QPRES);
+            aadd(A.q, i, j, k, QPRES, Pb * (1.0 / vfr) * X);
+            const double Xb = Pb * (1.0 / vfr) * p;
+            if (axm > 0.0) centroid_adj<0>(A.QE[0], GUN, g, i, j, k, -Xb * dxi0 * axm);
+            if (axp > 0.0) centroid_adj<0>(A.QE[0], GUN, g, i + 1, j, k, Xb * dxi0 * axp);
+            if (aym > 0.0) centroid_adj<1>(A.QE[1], GUN, g, i, j, k, -Xb * dxi1 * aym);
+            if (ayp > 0.0) centroid_adj<1>(A.QE[1], GUN, g, i, j + 1, k, Xb * dxi1 * ayp);
+            if (azm > 0.0) centroid_adj<2>(A.QE[2], GUN, g, i, j, k, -Xb * dxi2 * azm);
+            if (azp > 0.0) centroid_adj<2>(A.QE[2], GUN, g, i, j, k + 1, Xb * dxi2 * azp);
+        }
+    }
+}
+
+// ------------------------------------------------- adjust_fluxes_eb
+// b_adjust_eb<D> in reverse: f_out = (f_in + artificial viscosity) times
+// the area on an open face.  The flux adjoint is transformed in place.
+template <int D>
+CAMR_HD void adj_adjust(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                           const HydroConst &hc) {
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    constexpr int L1 = (D == 0) ? 1 : 0, L2 = (D == 2) ? 1 : 2;
+    constexpr int ax = (L1 == 0), ay = (L1 == 1), az = (L1 == 2);
+    constexpr int bx = (L2 == 0), by = (L2 == 1), bz = (L2 == 2);
+    const bool open = F.g.ap[D](i, j, k, 0) > 0.;
+    const double area = D == 0 ? hc.dx[1] * hc.dx[2] : D == 1 ? hc.dx[0] * hc.dx[2]
+                                                              : hc.dx[1] * hc.dx[0];
+    const Arr &fb = A.FT[D];
+    double pre[NH];
+    bool any = false;
+#pragma unroll
+    for (int n = 0; n < NH; ++n) {
+        const double b = fb(i, j, k, n);
+        pre[n] = open ? b * area : b;
+        any |= pre[n] != 0.0;
+    }
+    // species and T carry no adjoint
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) fb(i, j, k, n) = n < NH ? pre[n] : 0.0;
+    if (!any) return;
+    const Arr &dv = F.divu, &u = F.uin;
+    const double x = 0.25 * (dv(i, j, k, 0) + dv(i + ax, j + ay, k + az, 0) +
+                             dv(i + bx, j + by, k + bz, 0) +
+                             dv(i + ax + bx, j + ay + by, k + az + bz, 0));
+    const double div = hc.difmag * vmin(0.0, x);
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    const bool at_bndry = (idx == hc.domlo[D] && hc.bclo[D] == BC_NOSLIPWALL) ||
+                          (idx == hc.domhi[D] + 1 && hc.bchi[D] == BC_NOSLIPWALL);
+    const double dxd = hc.dx[D];
+    double divb = 0.0;
+#pragma unroll
+    for (int n = 0; n < NH; ++n) {
+        const bool tang = (n == UMX || n == UMY || n == UMZ) && (n != UMX + D);
+        if (at_bndry && tang) continue;
+        divb += pre[n] * dxd * (u(i, j, k, n) - u(i - ex, j - ey, k - ez, n));
+        aadd(A.uin, i, j, k, n, pre[n] * dxd * div);
+        aadd(A.uin, i - ex, j - ey, k - ez, n, -pre[n] * dxd * div);
+    }
+    if (x < 0.0) {
+        const double xb = 0.25 * hc.difmag * divb;
+        aadd(A.divu, i, j, k, 0, xb);
+        aadd(A.divu, i + ax, j + ay, k + az, 0, xb);
+        aadd(A.divu, i + bx, j + by, k + bz, 0, xb);
+        aadd(A.divu, i + ax + bx, j + ay + by, k + az + bz, 0, xb);
+    }
+}
+
+// b_eb_divu in reverse.
+CAMR_HD void adj_divu(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                         const HydroConst &hc) {
+    const double db = A.divu(i, j, k, 0);
+    if (db == 0.0) return;
+    const Arr &vf = F.g.vf;
+    const double sv = vf(i, j, k, 0) + vf(i - 1, j, k, 0) + vf(i, j - 1, k, 0) +
+                      vf(i - 1, j - 1, k, 0) + vf(i, j, k - 1, 0) + vf(i - 1, j, k - 1, 0) +
+                      vf(i, j - 1, k - 1, 0) + vf(i - 1, j - 1, k - 1, 0);
+    if (!(sv >= 8.0 * (1.0 - 1.e-12))) return;
+    // divu = ux + vy + wz, each 0.25 (sum of the 8 cells' signed values) / dx
+    const double cx = 0.25 * db * (1.0 / hc.dx[0]);
+    const double by = 0.25 * db * (1.0 / hc.dx[1]);
+    const double bz = 0.25 * db * (1.0 / hc.dx[2]);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int oi = c & 1, oj = (c >> 1) & 1, ok = (c >> 2) & 1;  // cell (i - 1 + oi, ...)
+        const int ci = i - 1 + oi, cj = j - 1 + oj, ck = k - 1 + ok;
+        aadd(A.q, ci, cj, ck, QU, oi ? cx : -cx);
+        aadd(A.q, ci, cj, ck, QV, oj ? by : -by);
+        aadd(A.q, ci, cj, ck, QW, ok ? bz : -bz);
+    }
+}
+
+// ------------------------------------------------------ Riemann solves
+// b_riemann_eb<D, MODE> in reverse for MODE 1 (pair p) and 2 (final).
+template <int D>
+CAMR_HD void adj_riemann(const Arr &qL, const Arr &qR, const Arr &qLb,
+                                            const Arr &qRb, const Arr &qaux, const Arr &qab,
+                                            const Arr &fb_arr, const Arr &gb_arr, const Arr &ap,
+                                            int i, int j, int k, const HydroConst &hc) {
+    if (!(ap(i, j, k, 0) > 0.)) return;
+    double fb[NF] = {}, gb[NG];
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+        fb[s] = fb_arr(i, j, k, s);
+        any |= fb[s] != 0.0;
+    }
+    gb[GUN] = gb_arr(i, j, k, GUN);
+    gb[GP] = gb_arr(i, j, k, GP);
+    any |= gb[GUN] != 0.0 || gb[GP] != 0.0;
+    if (!any) return;
+    riemann_face_adj<D>(i, j, k, qL, qR, qaux, qLb, qRb, qab, fb, gb, hc);
+}
+
+// b_riemann_eb<D, 0> in reverse: the first solve, whose domain-boundary
+// override wrote the other side's rho, p, rho e back into the replaced side.
+template <int D>
+CAMR_HD void adj_riemann1(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                             const HydroConst &hc) {
+    if (!(F.g.ap[D](i, j, k, 0) > 0.)) return;
+    const int idx = (D == 0) ? i : ((D == 1) ? j : k);
+    const Arr &QMb = A.QM[D], &QPb = A.QP[D];
+    const int bl = hc.bclo[D], bh = hc.bchi[D];
+    const bool lo = idx == hc.domlo[D] && (bl == BC_SYMMETRY || bl == BC_SLIPWALL ||
+                                           bl == BC_NOSLIPWALL || bl == BC_OUTFLOW);
+    const bool hi = idx == hc.domhi[D] + 1 && (bh == BC_SYMMETRY || bh == BC_SLIPWALL ||
+                                               bh == BC_NOSLIPWALL || bh == BC_OUTFLOW);
+    const int W[3] = {SR, SP, SE};
+    if (hi) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            QMb(i, j, k, W[c]) += QPb(i, j, k, W[c]);
+            QPb(i, j, k, W[c]) = 0.0;
+        }
+    }
+    if (lo) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            QPb(i, j, k, W[c]) += QMb(i, j, k, W[c]);
+            QMb(i, j, k, W[c]) = 0.0;
+        }
+    }
+    adj_riemann<D>(F.QM[D], F.QP[D], QMb, QPb, F.qaux, A.qaux, A.F1[D], A.G1[D], F.g.ap[D], i, j,
+                   k, hc);
+}
+
+// ------------------------------------------------- transverse predictors
+// b_transdo_eb<D, T> in reverse.
+template <int D, int T>
+CAMR_HD void adj_transdo(const EbBox &F, const EbBox &A, int p, int i, int j,
+                                            int k, double dt, const HydroConst &hc) {
+    if (F.g.f.covered(i, j, k)) return;
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    constexpr int tx = (T == 0), ty = (T == 1), tz = (T == 2);
+    const double cdt = 1.0 / 3.0 * dt / hc.dx[T];
+    const Arr &apD = F.g.ap[D], &apT = F.g.ap[T];
+    const bool no_cov = apT(i + tx, j + ty, k + tz, 0) > 0.0 && apT(i, j, k, 0) > 0.0;
+    const bool lo_open = apD(i, j, k, 0) > 0.0;
+    const bool hi_open = apD(i + ex, j + ey, k + ez, 0) > 0.0;
+    const Arr &Fl = F.F1[T], &Gl = F.G1[T];
+    double fl_[NF], fr[NF], gl[NG], gr[NG];
+#pragma unroll
+    for (int s = 0; s < NF; ++s) {
+        fl_[s] = no_cov ? Fl(i, j, k, s) : 0.0;
+        fr[s] = no_cov ? Fl(i + tx, j + ty, k + tz, s) : 0.0;
+    }
+    gl[GUN] = no_cov ? Gl(i, j, k, GUN) : 0.0;
+    gl[GP] = no_cov ? Gl(i, j, k, GP) : 0.0;
+    gr[GUN] = no_cov ? Gl(i + tx, j + ty, k + tz, GUN) : 0.0;
+    gr[GP] = no_cov ? Gl(i + tx, j + ty, k + tz, GP) : 0.0;
+    const double gamc = F.qaux(i, j, k, QGAMC);
+    TransDiff t;
+    trans_diff_one(fl_, fr, gl, gr, cdt, t);
+    TransDiffBar tb{};
+    // the plus state of face i (from qp(i)), the minus state of face i + e_D
+    for (int side = 0; side < 2; ++side) {
+        const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+        const Arr &qn = side ? F.QM[D] : F.QP[D];
+        const Arr &qnb = side ? A.QM[D] : A.QP[D];
+        const Arr &qob_a = side ? A.qmT[p] : A.qpT[p];
+        const bool open = side ? hi_open : lo_open;
+        double qob[NS] = {}, qnb_l[NS] = {};
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < NH; ++s) {
+            qob[s] = qob_a(i + oi, j + oj, k + ok, s);
+            any |= qob[s] != 0.0;
+        }
+        if (!any) continue;
+        if (open) {
+            double n[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) n[s] = qn(i + oi, j + oj, k + ok, s);
+            transverse_one_d_adj(n, t, cdt, gamc, hc.trd, hc.small_pres, qob, qnb_l, tb);
+        } else {
+#pragma unroll
+            for (int s = 0; s < NH; ++s) qnb_l[s] = qob[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NH; ++s) aadd(qnb, i + oi, j + oj, k + ok, s, qnb_l[s]);
+    }
+    // (a zero adjoint is not pushed through: forward values outside the
+    // domain of dependence of the valid cells need not be finite)
+    if (!no_cov || !tdb_any(tb)) return;
+    double flb[NF] = {}, frb[NF] = {}, glb[NG] = {}, grb[NG] = {};
+    trans_diff_one_adj(fl_, fr, gl, gr, cdt, tb, flb, frb, glb, grb);
+    const Arr &Flb = A.F1[T], &Glb = A.G1[T];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        aadd(Flb, i, j, k, s, flb[s]);
+        aadd(Flb, i + tx, j + ty, k + tz, s, frb[s]);
+    }
+    aadd(Glb, i, j, k, GUN, glb[GUN]);
+    aadd(Glb, i, j, k, GP, glb[GP]);
+    aadd(Glb, i + tx, j + ty, k + tz, GUN, grb[GUN]);
+    aadd(Glb, i + tx, j + ty, k + tz, GP, grb[GP]);
+}
+
+// b_transdd_eb<D> in reverse (no sources).
+template <int D>
+CAMR_HD void adj_transdd(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                            double dt, const HydroConst &hc) {
+    if (F.g.f.covered(i, j, k)) return;
+    constexpr int T0 = (D == 0) ? 1 : 0, T1 = (D == 2) ? 1 : 2;
+    constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    constexpr int ax = (T0 == 0), ay = (T0 == 1), az = (T0 == 2);
+    constexpr int bx = (T1 == 0), by = (T1 == 1), bz = (T1 == 2);
+    const double hdt = 0.5 * dt;
+    const double ca = hdt / hc.dx[T0], cb = hdt / hc.dx[T1];
+    const Arr &apD = F.g.ap[D], &apA = F.g.ap[T0], &apB = F.g.ap[T1];
+    const bool no_cov = apA(i + ax, j + ay, k + az, 0) > 0.0 && apB(i + bx, j + by, k + bz, 0) > 0.0 &&
+                        apA(i, j, k, 0) > 0.0 && apB(i, j, k, 0) > 0.0;
+    const bool lo_open = apD(i, j, k, 0) > 0.0;
+    const bool hi_open = apD(i + ex, j + ey, k + ez, 0) > 0.0;
+    const Arr &fa = F.F2[T0][T1], &ga = F.G2[T0][T1], &fbA = F.F2[T1][T0], &gbA = F.G2[T1][T0];
+    double fal[NF], far_[NF], fbl[NF], fbr[NF], gal[NG], gar[NG], gbl[NG], gbr[NG];
+#pragma unroll
+    for (int s = 0; s < NF; ++s) {
+        fal[s] = no_cov ? fa(i, j, k, s) : 0.0;
+        far_[s] = no_cov ? fa(i + ax, j + ay, k + az, s) : 0.0;
+        fbl[s] = no_cov ? fbA(i, j, k, s) : 0.0;
+        fbr[s] = no_cov ? fbA(i + bx, j + by, k + bz, s) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < NG; ++s) {
+        gal[s] = no_cov ? ga(i, j, k, s) : 0.0;
+        gar[s] = no_cov ? ga(i + ax, j + ay, k + az, s) : 0.0;
+        gbl[s] = no_cov ? gbA(i, j, k, s) : 0.0;
+        gbr[s] = no_cov ? gbA(i + bx, j + by, k + bz, s) : 0.0;
+    }
+    const double gamc = F.qaux(i, j, k, QGAMC);
+    TransDiff t;
+    trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, gamc, t);
+    TransDiffBar tb{};
+    for (int side = 0; side < 2; ++side) {
+        const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+        const Arr &qn = side ? F.QM[D] : F.QP[D];
+        const Arr &qnb = side ? A.QM[D] : A.QP[D];
+        const Arr &qob_a = side ? A.qmF[D] : A.qpF[D];
+        const bool open = side ? hi_open : lo_open;
+        double qob[NS] = {}, qnb_l[NS] = {};
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < NH; ++s) {
+            qob[s] = qob_a(i + oi, j + oj, k + ok, s);
+            any |= qob[s] != 0.0;
+        }
+        if (!any) continue;
+        if (open) {
+            double n[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) n[s] = qn(i + oi, j + oj, k + ok, s);
+            transverse_two_d_adj(n, t, hc.trd, hc.small_pres, qob, qnb_l, tb);
+        } else {
+#pragma unroll
+            for (int s = 0; s < NH; ++s) qnb_l[s] = qob[s];
+        }
+#pragma unroll
+        for (int s = 0; s < NH; ++s) aadd(qnb, i + oi, j + oj, k + ok, s, qnb_l[s]);
+    }
+    if (!no_cov || !tdb_any(tb)) return;
+    double falb[NF] = {}, farb[NF] = {}, fblb[NF] = {}, fbrb[NF] = {};
+    double galb[NG] = {}, garb[NG] = {}, gblb[NG] = {}, gbrb[NG] = {};
+    trans_diff_two_adj(gal, gar, gbl, gbr, ca, cb, gamc, tb, falb, farb, galb, garb, fblb, fbrb,
+                       gblb, gbrb);
+    const Arr &fab = A.F2[T0][T1], &gab = A.G2[T0][T1], &fbb = A.F2[T1][T0], &gbb = A.G2[T1][T0];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        aadd(fab, i, j, k, s, falb[s]);
+        aadd(fab, i + ax, j + ay, k + az, s, farb[s]);
+        aadd(fbb, i, j, k, s, fblb[s]);
+        aadd(fbb, i + bx, j + by, k + bz, s, fbrb[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NG; ++s) {
+        aadd(gab, i, j, k, s, galb[s]);
+        aadd(gab, i + ax, j + ay, k + az, s, garb[s]);
+        aadd(gbb, i, j, k, s, gblb[s]);
+        aadd(gbb, i + bx, j + by, k + bz, s, gbrb[s]);
+    }
+}
+
+// ------------------------------------------------------------ PLM trace
+// b_plm_eb in reverse: the trace of each direction, its limited slopes and
+// the flattening coefficient.
+template <int D>
+CAMR_HD void adj_plm_dir(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                            const double *c, double cc, double fl, double *cb,
+                                            double &ccb, double &flb, double dt,
+                                            const HydroConst &hc) {
+    constexpr int ex = D == 0, ey = D == 1, ez = D == 2;
+    const EbGeom &g = F.g;
+    const bool open_lo = g.ap[D](i, j, k, 0) > 0.0;
+    const bool open_hi = g.ap[D](i + ex, j + ey, k + ez, 0) > 0.0;
+    double qpb[NS] = {}, qmb[NS] = {};
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+        qpb[s] = open_lo ? A.QP[D](i, j, k, s) : 0.0;
+        qmb[s] = open_hi ? A.QM[D](i + ex, j + ey, k + ez, s) : 0.0;
+        any |= qpb[s] != 0.0 || qmb[s] != 0.0;
+    }
+    if (!any) return;
+    const bool cm1 = g.f.covered(i - ex, j - ey, k - ez), cm2 = g.f.covered(i - 2 * ex, j - 2 * ey, k - 2 * ez);
+    const bool cp1 = g.f.covered(i + ex, j + ey, k + ez), cp2 = g.f.covered(i + 2 * ex, j + 2 * ey, k + 2 * ez);
+    int ord = 0;
+    if (hc.plm_iorder == 4 && !cm1 && !cm2 && !cp1 && !cp2) ord = 4;
+    else if (hc.plm_iorder > 1 && !cm1 && !cp1) ord = 2;
+    const Arr &q = F.q;
+    double st[NH][5], sl[NS] = {};
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+#pragma unroll
+        for (int o = -2; o <= 2; ++o) st[s][o + 2] = q(i + o * ex, j + o * ey, k + o * ez, kQa(s));
+        sl[s] = ord ? mc_slope(st[s], fl, ord) : 0.0;
+    }
+    double slb[NS] = {};
+    plm_trace_adj(D, c, sl, cc, dt / hc.dx[D], hc.small_dens, hc.small_pres, qpb, qmb, cb, slb,
+                  ccb);
+    if (!ord) return;
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+        double sb[5] = {};
+        mc_slope_adj(st[s], fl, ord, slb[s], sb, flb);
+#pragma unroll
+        for (int o = -2; o <= 2; ++o) aadd(A.q, i + o * ex, j + o * ey, k + o * ez, kQa(s), sb[o + 2]);
+    }
+}
+
+CAMR_HD void adj_plm(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                        double dt, const HydroConst &hc) {
+    const EbGeom &g = F.g;
+    if (eb_type(g.f(i, j, k)) == kEbCovered) return;
+    const Arr &q = F.q;
+    const Box &qbox = F.qbox;
+    double fd[3] = {1.0, 1.0, 1.0};
+    bool fcalc[3] = {false, false, false};
+    double fl = 1.0;
+    if (hc.use_flattening == 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            if (qbox.contains(i + 3 * ex, j + 3 * ey, k + 3 * ez) &&
+                qbox.contains(i - 3 * ex, j - 3 * ey, k - 3 * ez)) {
+                bool cut = eb_type(g.f(i, j, k)) == kEbSingle;
+                for (int n = 1; n <= 3; ++n)
+                    cut = cut || eb_type(g.f(i - n * ex, j - n * ey, k - n * ez)) == kEbSingle ||
+                          eb_type(g.f(i + n * ex, j + n * ey, k + n * ez)) == kEbSingle;
+                if (!cut) {
+                    double p[7], un[7];
+#pragma unroll
+                    for (int o = -3; o <= 3; ++o) {
+                        p[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QPRES);
+                        un[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QU + d);
+                    }
+                    fd[d] = flatten_1d(p, un);
+                    fcalc[d] = true;
+                }
+            }
+        }
+        fl = vmin(vmin(vmin(1.0, fd[0]), fd[1]), fd[2]);
+    }
+    double c[NS];
+#pragma unroll
+    for (int s = 0; s < NH; ++s) c[s] = q(i, j, k, kQa(s));
+#pragma unroll
+    for (int s = NH; s < NS; ++s) c[s] = 0.0;
+    const double cc = F.qaux(i, j, k, QC);
+    double cb[NS] = {}, ccb = 0.0, flb = 0.0;
+    adj_plm_dir<0>(F, A, i, j, k, c, cc, fl, cb, ccb, flb, dt, hc);
+    adj_plm_dir<1>(F, A, i, j, k, c, cc, fl, cb, ccb, flb, dt, hc);
+    adj_plm_dir<2>(F, A, i, j, k, c, cc, fl, cb, ccb, flb, dt, hc);
+#pragma unroll
+    for (int s = 0; s < NH; ++s) aadd(A.q, i, j, k, kQa(s), cb[s]);
+    aadd(A.qaux, i, j, k, QC, ccb);
+    if (hc.use_flattening == 1 && flb != 0.0) {
+        // fl = vmin(vmin(vmin(1, fd0), fd1), fd2)
+        const double f1 = vmin(1.0, fd[0]), f2 = vmin(f1, fd[1]);
+        double b2 = 0.0, b1 = 0.0, b0 = 0.0, fdb[3] = {0.0, 0.0, 0.0}, one = 0.0;
+        vmin_adj(f2, fd[2], flb, b2, fdb[2]);
+        vmin_adj(f1, fd[1], b2, b1, fdb[1]);
+        vmin_adj(1.0, fd[0], b1, one, fdb[0]);
+        (void)b0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (!fcalc[d] || fdb[d] == 0.0) continue;
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            double p[7], un[7], pb[7] = {};
+#pragma unroll
+            for (int o = -3; o <= 3; ++o) {
+                p[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QPRES);
+                un[o + 3] = q(i + o * ex, j + o * ey, k + o * ez, QU + d);
+            }
+            flatten_1d_adj(p, un, fdb[d], pb);
+#pragma unroll
+            for (int o = -3; o <= 3; ++o) aadd(A.q, i + o * ex, j + o * ey, k + o * ez, QPRES, pb[o + 3]);
+        }
+    }
+}
+
+// hydro_ctoprim in reverse (cells with rho > 0; covered cells have q = 0).
+CAMR_HD void adj_prim(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                         const HydroConst &hc) {
+    const Arr &u = F.uin;
+    if (!(u(i, j, k, URHO) > 0.)) return;
+    double ob[NQC] = {};
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < NH; ++s) {
+        ob[s] = A.q(i, j, k, kQa(s));
+        any |= ob[s] != 0.0;
+    }
+    ob[SC] = A.qaux(i, j, k, QC);
+    ob[SCS] = A.qaux(i, j, k, QCSML);
+    any |= ob[SC] != 0.0 || ob[SCS] != 0.0;
+    if (!any) return;
+    double v[NVAR], vb[NVAR] = {};
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) v[n] = u(i, j, k, n);
+    prim_compact_adj(v, ob, hc.eta1, hc.small_num, vb);
+#pragma unroll
+    for (int n = 0; n < NH; ++n) aadd(A.uin, i, j, k, n, vb[n]);
+}
+
+template <int S>
+CAMR_HD void adj_body(const EbBox &F, const EbBox &A, int i, int j, int k,
+                                         const HydroConst &hc, double dt) {
+    if constexpr (S == AS_RDST) adj_rdst(F, A, i, j, k, dt);
+    else if constexpr (S == AS_RSRC) adj_rsrc(F, A, i, j, k);
+    else if constexpr (S == AS_DIVCUT) adj_div<true>(F, A, i, j, k, hc);
+    else if constexpr (S == AS_DIV) adj_div<false>(F, A, i, j, k, hc);
+    else if constexpr (S >= AS_ADJ && S < AS_ADJ + 3) adj_adjust<2 - (S - AS_ADJ)>(F, A, i, j, k, hc);
+    else if constexpr (S == AS_DIVU) adj_divu(F, A, i, j, k, hc);
+    else if constexpr (S >= AS_FINAL && S < AS_FINAL + 6) {
+        constexpr int D = 2 - (S - AS_FINAL) / 2;
+        if constexpr ((S - AS_FINAL) % 2 == 0)
+            adj_riemann<D>(F.qmF[D], F.qpF[D], A.qmF[D], A.qpF[D], F.qaux, A.qaux, A.FT[D],
+                           A.QE[D], F.g.ap[D], i, j, k, hc);
+        else adj_transdd<D>(F, A, i, j, k, dt, hc);
+    } else if constexpr (S >= AS_PAIR && S < AS_PAIR + 12) {
+        constexpr int p = 5 - (S - AS_PAIR) / 2, D = pair_d(p), T = pair_t(p);
+        if constexpr ((S - AS_PAIR) % 2 == 0)
+            adj_riemann<D>(F.qmT[p], F.qpT[p], A.qmT[p], A.qpT[p], F.qaux, A.qaux, A.F2[D][T],
+                           A.G2[D][T], F.g.ap[D], i, j, k, hc);
+        else adj_transdo<D, T>(F, A, p, i, j, k, dt, hc);
+    } else if constexpr (S >= AS_R1 && S < AS_R1 + 3) adj_riemann1<2 - (S - AS_R1)>(F, A, i, j, k, hc);
+    else if constexpr (S == AS_PLM) adj_plm(F, A, i, j, k, dt, hc);
+    else adj_prim(F, A, i, j, k, hc);
+}
+
+template <int S>
+__global__ void __launch_bounds__(kBlock)
+k_eb_adj_site(const EbAdjEnt *tab, HydroConst hc, const double *dtp) {
+    const EbAdjEnt &E = tab[blockIdx.y];
+    const Layout L = layout_of(E.F.bx, E.F.qbox);
+    const Box sb = adj_box<S>(E.F, L);
+    if (!sb.ok()) return;
+    int i, j, k;
+    if (!range_point(range_of(sb), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+        return;
+    adj_body<S>(E.F, E.A, i, j, k, hc, *dtp);
+}
+
+constexpr const char *adj_name(int s) {
+    return s <= AS_RSRC ? "ebadj_redistribute"
+         : s <= AS_DIV ? "ebadj_divergence"
+         : s < AS_DIVU ? "ebadj_adjust"
+         : s == AS_DIVU ? "ebadj_divu"
+         : s < AS_PAIR ? "ebadj_final"
+         : s < AS_R1 ? "ebadj_pairs"
+         : s < AS_PLM ? "ebadj_riemann_1"
+         : s == AS_PLM ? "ebadj_trace_plm" : "ebadj_ctoprim";
+}
+
+template <int S>
+void launch_adj(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
+                const double *dtp, hipStream_t st, Prof *prof) {
+    int64_t nblk = 0;
+    for (int b = 0; b < n; ++b) {
+        const Box sb = adj_box<S>(htab[b].F, layout_of(htab[b].F.bx, htab[b].F.qbox));
+        if (sb.ok()) nblk = std::max<int64_t>(nblk, (sb.npts() + kBlock - 1) / kBlock);
+    }
+    if (nblk == 0) return;
+    Region reg(prof, adj_name(S), st);
+    hipLaunchKernelGGL(k_eb_adj_site<S>, dim3((unsigned)nblk, (unsigned)n), dim3(kBlock), 0, st,
+                       tab, hc, dtp);
+}
+
+template <int... S>
+void launch_adjs(std::integer_sequence<int, S...>, const EbAdjEnt *tab, const EbAdjEnt *htab,
+                 int n, const HydroConst &hc, const double *dtp, hipStream_t st, Prof *prof) {
+    (launch_adj<S>(tab, htab, n, hc, dtp, st, prof), ...);
+}
+
+__device__ __forceinline__ bool batch_pt(const DevFab *tab, int g, int &b, int &i, int &j,
+                                         int &k) {
+    b = blockIdx.y;
+    const Range r = range_of(grow(tab[b].b, g));
+    return range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+}
+
+// reset_e_adj, with the covered cells of an EB level (rho = 0) left alone
+// as reset_e_and_T leaves them
+CAMR_HD bool reset_adj_eb(const double *s, const HydroConst &hc, double *b) {
+    if (hc.eb && !(s[URHO] > 0.)) return true;
+    return reset_e_adj(s, hc, b);
+}
+
+} // namespace
+
+int eb_adjoint_batch(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
+                     const double *dtp, hipStream_t st, Prof *prof) {
+    if (n <= 0) return CAMR_OK;
+    launch_adjs(std::make_integer_sequence<int, AS_COUNT>{}, tab, htab, n, hc, dtp, st, prof);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_ebadj_post(const DevFab *sb, const DevFab *dsdt, const DevFab *sbar_in, const DevFab *ub,
+              const DevFab *sbb, const double *dtp, HydroConst hc, int add_grav, int *status,
+              EbFlags eb) {
+    int b, i, j, k;
+    if (!batch_pt(sb, 0, b, i, j, k)) return;
+    const Arr s = sb[b].a, ds = dsdt[b].a, bin = sbar_in[b].a;
+    const double dt = *dtp;
+    double x0[NVAR], x1[NVAR], x2[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) x0[n] = s(i, j, k, n) + dt * ds(i, j, k, n);
+    if (!eb.covered(i, j, k)) {
+        if (x0[URHO] == 0.) atomicOr(status, 1 << CAMR_ERR_ZERO_DENSITY);
+        else if (x0[URHO] < hc.small_dens) atomicOr(status, kFloorFlag);
+    }
+    normalize_species(x0);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) x1[n] = x0[n];
+    reset_e_and_T(x1, hc);
+    normalize_species(x1);
+    if (add_grav) {
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) x2[n] = x1[n];
+        reset_e_and_T(x2, hc);
+        normalize_species(x2);
+    }
+    double bb[NVAR] = {};
+#pragma unroll
+    for (int n = 0; n < NH; ++n) bb[n] = bin(i, j, k, n);
+    bool ok = true;
+    if (add_grav) ok &= reset_adj_eb(x2, hc, bb);
+    ok &= reset_adj_eb(x1, hc, bb);
+    ok &= reset_adj_eb(x0, hc, bb);
+    if (!ok) atomicOr(status, 1 << CAMR_ERR_ADJOINT);
+    const Arr u = ub[b].a, sa = sbb[b].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        const double v = n < NH ? bb[n] : 0.0;
+        u(i, j, k, n) = v;
+        sa(i, j, k, n) = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_ebadj_fill(const DevFab *sbb, int g, const DevFab *sclean, const DevFab *gout,
+              const int *owner, DomInfo D, HydroConst hc, int *status) {
+    int b, i, j, k;
+    if (!batch_pt(sbb, g, b, i, j, k)) return;
+    const Arr sa = sbb[b].a;
+    double bb[NVAR] = {};
+    bool any = false;
+#pragma unroll
+    for (int n = 0; n < NH; ++n) {
+        bb[n] = sa(i, j, k, n);
+        any |= bb[n] != 0.0;
+    }
+    if (!any) return;
+    int m[3];
+    unsigned odd;
+    map_cell(D, i, j, k, m, &odd);
+    const int ob = owner_of(D, owner, m);
+    if (ob < 0) { atomicOr(status, 1 << CAMR_ERR_SHAPE); return; }
+    const Arr sc = sclean[ob].a;
+    double x[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        const double v = sc(m[0], m[1], m[2], n);
+        x[n] = ((odd >> n) & 1u) ? -v : v;
+    }
+    // the energy reset of Sborder (kb_resetT), at the copied state
+    if (!reset_adj_eb(x, hc, bb)) atomicOr(status, 1 << CAMR_ERR_ADJOINT);
+    const Arr go = gout[ob].a;
+#pragma unroll
+    for (int n = 0; n < NH; ++n)
+        aadd(go, m[0], m[1], m[2], n, ((odd >> n) & 1u) ? -bb[n] : bb[n]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_ebadj_old(const DevFab *gout, const DevFab *sclean, HydroConst hc, int *status) {
+    int b, i, j, k;
+    if (!batch_pt(gout, 0, b, i, j, k)) return;
+    const Arr go = gout[b].a, sc = sclean[b].a;
+    double bb[NVAR] = {}, x[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) x[n] = sc(i, j, k, n);
+#pragma unroll
+    for (int n = 0; n < NH; ++n) bb[n] = go(i, j, k, n);
+    // clean_state(S_old) reset; its input differs from sclean in rho e only,
+    // which the modelled branches do not read
+    if (!reset_adj_eb(x, hc, bb)) atomicOr(status, 1 << CAMR_ERR_ADJOINT);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) go(i, j, k, n) = n < NH ? bb[n] : 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_ebadj_force(const DevFab *S, const DevFab *gout, const EbGeom *geoms, const int *kinds,
+               double w0, double w1, double w2, double dx2, HydroConst hc) {
+    int b, i, j, k;
+    if (!batch_pt(S, 0, b, i, j, k)) return;
+    const Arr go = gout[b].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) go(i, j, k, n) = 0.0;
+    if (kinds[b] != 1) return;
+    const EbGeom &g = geoms[b];
+    if (eb_type(g.f(i, j, k)) != kEbSingle) return;
+    const Arr s = S[b].a;
+    double v[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) v[n] = s(i, j, k, n);
+    if (!(v[URHO] > 0.)) return;
+    // primitives of the cell (prim_compact), then the wall flux
+    double o[NQC], e;
+    int st_dummy = 0;
+    {
+        const double rho = v[URHO], rhoinv = 1.0 / rho;
+        const double vx = v[UMX] * rhoinv, vy = v[UMY] * rhoinv, vz = v[UMZ] * rhoinv;
+        const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+        e = ((v[UEDEN] - kineng) / v[UEDEN] > hc.eta1) ? (v[UEDEN] - kineng) * rhoinv
+                                                       : v[UEINT] * rhoinv;
+        const double p = (kGamma - 1.0) * rho * e;
+        const double cs = sqrt(kGamma * p / rho);
+        o[SR] = rho; o[SU] = vx; o[SV] = vy; o[SW] = vz; o[SP] = p; o[SE] = e * rho;
+        o[SC] = cs;
+        o[SCS] = vmax(hc.small_num, hc.small_num * cs);
+    }
+    (void)st_dummy;
+    const double axm = g.ap[0](i, j, k, 0), axp = g.ap[0](i + 1, j, k, 0);
+    const double aym = g.ap[1](i, j, k, 0), ayp = g.ap[1](i, j + 1, k, 0);
+    const double azm = g.ap[2](i, j, k, 0), azp = g.ap[2](i, j, k + 1, 0);
+    double fwb[NVAR] = {};
+    fwb[UMX] = w0 * dx2;
+    fwb[UMY] = w1 * dx2;
+    fwb[UMZ] = w2 * dx2;
+    double wb[6] = {}, cb = 0.0, csb = 0.0;
+    wallflux_adj(o, o[SC], o[SCS], axm - axp, aym - ayp, azm - azp, hc, fwb, wb, cb, csb);
+    double ob[NQC] = {};
+#pragma unroll
+    for (int s = 0; s < NH; ++s) ob[s] = wb[s];
+    ob[SC] = cb;
+    ob[SCS] = csb;
+    double vb[NVAR] = {};
+    prim_compact_adj(v, ob, hc.eta1, hc.small_num, vb);
+#pragma unroll
+    for (int n = 0; n < NH; ++n) go(i, j, k, n) = vb[n];
+}
+
+} // namespace camr

@@ -101,7 +101,7 @@ __device__ __forceinline__ double divu_node(const QA &q, int i, int j, int k,
 // open == false (a closed cut-cell face, adjust_fluxes_eb): the species
 // normalisation and the area factor are skipped.
 template <int D, class DA, class UA>
-__device__ __forceinline__ void adjust_face(double *f, const DA &divu,
+CAMR_HD void adjust_face(double *f, const DA &divu,
                                             const UA &u, int i, int j, int k,
                                             double area, const HydroConst &hc,
                                             bool open = true) {

@@ -217,6 +217,13 @@ class Level:
         """dL/dS_T (set_adjoint) -> dL/dS_0 (get_adjoint); the level ends at S_0."""
         L.check(L.lib().camr_level_adjoint_backward(self._h))
 
+    def seed_force_adjoint(self, w=(1.0, 0.0, 0.0)):
+        """After adjoint_forward on an embedded-boundary level: dL/dS_T for
+        L = w . F, F the force on the body at the final state (eb_force)."""
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        L.check(L.lib().camr_level_adj_seed_eb_force(
+            self._h, a.ctypes.data_as(C.POINTER(C.c_double))))
+
     def set_exterior(self, data):
         """Values for ghost cells no box owns: (NVAR, nz, ny, nx) over the domain."""
         n = [self.prob.domhi[d] - self.prob.domlo[d] + 1 for d in range(3)]

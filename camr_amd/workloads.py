@@ -145,6 +145,40 @@ def sod_sphere_geometry(prob):
     return eb.Geometry(prob, eb.Sphere(c, r))
 
 
+def sod_sphere_smooth_init(prob, b, geom=None, amp=0.2):
+    """A smooth state on the Sod-past-a-sphere grid for gradient checks:
+    density, pressure and the three velocities are fixed sine modes of the
+    cell centre along oblique wave vectors (a flow of Mach ~0.4 in +x),
+    covered cells zero.  The
+    Sod states' discontinuity would put limiters on their switching points,
+    where finite differences measure secants across kinks."""
+    import numpy as np
+    lo, hi = prob.boxes[b][:3], prob.boxes[b][3:]
+    x, y, z = [prob.prob_lo[d] + (np.arange(lo[d], hi[d] + 1) + 0.5) * prob.dx[d]
+               for d in range(3)]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    tp = 2 * np.pi
+    # every field varies along every axis: one constant along a direction
+    # would put its slopes there exactly on the limiter's kink
+    rho = (1.0 + amp * np.sin(tp * (0.5 * X + 0.3 * Y + 0.2 * Z) + 0.4)
+           + 0.5 * amp * np.cos(tp * (0.35 * X + Z - Y) + 0.2))
+    u = 0.4 + 0.5 * amp * np.sin(tp * (0.25 * X + Y + Z) + 0.3)
+    v = 0.5 * amp * np.sin(tp * (0.5 * X + 0.4 * Y + Z) + 0.7)
+    w = 0.5 * amp * np.cos(tp * (0.5 * X - Y + 0.3 * Z) + 0.1)
+    p = 1.0 + amp * np.sin(tp * (0.5 * X + Y + 0.5 * Z) + 1.1)
+    P = problems
+    U = np.zeros((P.NVAR,) + rho.shape)
+    U[P.URHO] = rho
+    U[P.UMX], U[P.UMY], U[P.UMZ] = rho * u, rho * v, rho * w
+    U[P.UEINT] = p / (P.GAMMA - 1.0)
+    U[P.UEDEN] = U[P.UEINT] + 0.5 * rho * (u * u + v * v + w * w)
+    U[P.UFS] = rho
+    if geom is not None:
+        cov = geom.covered_mask(lo, hi)
+        U[:, cov] = 0.0
+    return U
+
+
 def sod_sphere_init(prob, b, geom=None):
     """Left state (rho, p) = (1, 1) for x < interface, right (0.125, 0.1),
     at rest; covered cells zero (EB_set_covered, CAMR.cpp:419-421)."""

@@ -426,8 +426,10 @@ int camr_level_get_ctu(camr_level *L, int32_t *on, int32_t *planes);
  * final state with respect to the initial state.  No reference counterpart
  * (CAMR has no adjoint); for inverse and sensitivity problems.  Scope: the
  * fused engine, periodic (Interior) domain, PPM, const_grav = 0, the
- * default energy branches; rho, rho u, rho E, rho e carry the adjoint,
- * species and temperature are passive (zero gradient).  dt is frozen.
+ * default energy branches; or a level with an embedded boundary (staged
+ * engine, PLM, FluxRedist; any boundary condition but NoSlipWall).  rho,
+ * rho u, rho E, rho e carry the adjoint, species and temperature are
+ * passive (zero gradient).  dt is frozen.
  *
  * camr_level_adjoint_forward: from the level's state S_0, nsteps steps with
  * the given dts, keeping device checkpoints every ckpt_every steps; the
@@ -440,6 +442,12 @@ int camr_level_adjoint_forward(camr_level *L, int32_t nsteps, const double *dts,
 int camr_level_adj_upload_box(camr_level *L, int32_t b, const double *host);
 int camr_level_adj_download_box(camr_level *L, int32_t b, double *host);
 int camr_level_adjoint_backward(camr_level *L);
+/* After camr_level_adjoint_forward on a level with an embedded boundary:
+ * set dL/dS_T to the gradient of L = w . F, F the force on the body at the
+ * final state (camr_level_eb_force: the wall momentum flux of every cut cell
+ * times dx^2, Hydro_hypflux_eb_K.H:65), in place of
+ * camr_level_adj_upload_box.  No reference counterpart. */
+int camr_level_adj_seed_eb_force(camr_level *L, const double w[3]);
 
 /* Named hipEvent timers around kernel launches on the level's stream -- the
  * counterpart of the reference's BL_PROFILE regions (TinyProfiler).  With a

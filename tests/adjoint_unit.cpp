@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "adjoint_math.h"
+#include "eb_adjoint_math.h"
 #include "state_math.h"
 
 using namespace camr;
@@ -451,6 +452,150 @@ int main() {
             for (int i = 0; i < 10; ++i) g[i] = (i < 6) ? sb[i] : 0.0;
         };
         check("reset_e", 10, 6, gen, f, a, 300, st);
+        done(st);
+    }
+
+    for (int order : {2, 4}) {   // limited PLM slope (mc_slope)
+        Stat st;
+        auto gen = [] {
+            std::vector<double> x(6);
+            for (int i = 0; i < 5; ++i) x[i] = U(-1, 1);
+            x[5] = U(0.2, 1.0);  // flattening coefficient
+            return x;
+        };
+        auto f = [order](const std::vector<double> &x, std::vector<double> &y) {
+            y[0] = mc_slope(x.data(), x[5], order);
+        };
+        auto a = [order](const std::vector<double> &x, const std::vector<double> &w,
+                         std::vector<double> &g) {
+            double sb[5] = {}, flb = 0.0;
+            mc_slope_adj(x.data(), x[5], order, w[0], sb, flb);
+            for (int i = 0; i < 5; ++i) g[i] = sb[i];
+            g[5] = flb;
+        };
+        check(order == 2 ? "mc_slope order 2" : "mc_slope order 4", 6, 1, gen, f, a, 3000, st);
+        done(st);
+    }
+
+    for (int d = 0; d < 3; ++d) {   // PLM characteristic trace (plm_trace)
+        Stat st;
+        const double dtdx = 0.37;
+        auto gen = [&] {
+            std::vector<double> x = state6();
+            for (int i = 0; i < 6; ++i) x.push_back(U(-0.2, 0.2));  // slopes
+            x.push_back(U(0.8, 2.0));                                // sound speed
+            return x;
+        };
+        auto unpack = [](const std::vector<double> &x, double *c, double *sl, double &cc) {
+            for (int i = 0; i < NS; ++i) { c[i] = 0.0; sl[i] = 0.0; }
+            for (int i = 0; i < 6; ++i) { c[i] = x[i]; sl[i] = x[6 + i]; }
+            cc = x[12];
+        };
+        auto f = [&, d](const std::vector<double> &x, std::vector<double> &y) {
+            double c[NS], sl[NS], cc, qp[NS], qm[NS];
+            unpack(x, c, sl, cc);
+            plm_trace(d, c, sl, cc, dtdx, hc.small_dens, hc.small_pres, qp, qm);
+            for (int i = 0; i < 6; ++i) { y[i] = qp[i]; y[6 + i] = qm[i]; }
+        };
+        auto a = [&, d](const std::vector<double> &x, const std::vector<double> &w,
+                        std::vector<double> &g) {
+            double c[NS], sl[NS], cc, qpb[NS] = {}, qmb[NS] = {}, cb[NS] = {}, slb[NS] = {};
+            double ccb = 0.0;
+            unpack(x, c, sl, cc);
+            for (int i = 0; i < 6; ++i) { qpb[i] = w[i]; qmb[i] = w[6 + i]; }
+            plm_trace_adj(d, c, sl, cc, dtdx, hc.small_dens, hc.small_pres, qpb, qmb, cb, slb, ccb);
+            for (int i = 0; i < 6; ++i) { g[i] = cb[i]; g[6 + i] = slb[i]; }
+            g[12] = ccb;
+        };
+        char nm[32];
+        snprintf(nm, sizeof nm, "plm_trace d=%d", d);
+        check(nm, 13, 12, gen, f, a, 400, st);
+        done(st);
+    }
+
+    // face_flux at domain faces: wall (reflecting) and outflow overrides on
+    // either side
+    for (int bc : {BC_SLIPWALL, BC_OUTFLOW})
+        for (int side = 0; side < 2; ++side) {
+            Stat st;
+            const int d = 1;
+            HydroConst h2 = hc;
+            for (int k = 0; k < 3; ++k) {
+                h2.domlo[k] = 0; h2.domhi[k] = 9;
+                h2.bclo[k] = bc; h2.bchi[k] = bc;
+            }
+            const int idx = side == 0 ? 0 : 10;
+            auto gen = [&] {
+                std::vector<double> x = state6(), r = state6();
+                x.insert(x.end(), r.begin(), r.end());
+                x.push_back(U(0.5, 2)); x.push_back(U(0.5, 2));
+                x.push_back(U(0.5, 2) * 1e-8); x.push_back(U(0.5, 2) * 1e-8);
+                return x;
+            };
+            auto unpack = [](const std::vector<double> &x, double *ql, double *qr, FaceAux &a) {
+                for (int i = 0; i < NS; ++i) { ql[i] = 0.0; qr[i] = 0.0; }
+                for (int i = 0; i < 6; ++i) { ql[i] = x[i]; qr[i] = x[6 + i]; }
+                a.c_l = x[12]; a.c_r = x[13]; a.csml_l = x[14]; a.csml_r = x[15];
+                a.gam_l = kGamma; a.gam_r = kGamma;
+            };
+            auto f = [&](const std::vector<double> &x, std::vector<double> &y) {
+                double ql[NS], qr[NS], fl[NF], g[NG];
+                FaceAux a;
+                unpack(x, ql, qr, a);
+                int mut;
+                face_flux(d, idx, h2, ql, qr, a, fl, g, &mut);
+                for (int i = 0; i < 6; ++i) y[i] = fl[i];
+                y[6] = g[GUN]; y[7] = g[GP];
+            };
+            auto a = [&](const std::vector<double> &x, const std::vector<double> &w,
+                         std::vector<double> &gr) {
+                double ql[NS], qr[NS], qlb[NS] = {}, qrb[NS] = {}, fb[NF] = {}, gb[NG];
+                FaceAux a;
+                unpack(x, ql, qr, a);
+                for (int i = 0; i < 6; ++i) fb[i] = w[i];
+                gb[GUN] = w[6]; gb[GP] = w[7];
+                FaceAuxBar ab{};
+                face_flux_bc_adj(d, idx, h2, ql, qr, a, fb, gb, qlb, qrb, ab);
+                for (int i = 0; i < 6; ++i) { gr[i] = qlb[i]; gr[6 + i] = qrb[i]; }
+                gr[12] = ab.c_l; gr[13] = ab.c_r; gr[14] = ab.csml_l; gr[15] = ab.csml_r;
+            };
+            char nm[40];
+            snprintf(nm, sizeof nm, "face_flux %s %s", bc == BC_OUTFLOW ? "outflow" : "wall",
+                     side ? "hi" : "lo");
+            check(nm, 16, 8, gen, f, a, 150, st);
+            done(st);
+        }
+
+    {   // wall flux of a cut cell (compute_hyp_wallflux)
+        Stat st;
+        auto gen = [&] {
+            std::vector<double> x = state6();
+            x.push_back(U(0.8, 2.0));        // c
+            x.push_back(U(0.5, 2) * 1e-8);   // csml
+            return x;
+        };
+        const double n0 = 0.3, n1 = -0.5, n2 = 0.2;
+        auto f = [&](const std::vector<double> &x, std::vector<double> &y) {
+            const double apnorm = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+            const double apnorminv = 1. / apnorm;
+            const double un = x[1] * n0 * apnorminv + x[2] * n1 * apnorminv + x[3] * n2 * apnorminv;
+            RiemannOut o;
+            riemann_two_shock(x[0], un, 0., 0., x[4], x[5], x[0], -un, 0., 0., x[4], x[5], x[6],
+                              kGamma, kGamma, hc.small, x[7], hc.small_dens, hc.small_pres,
+                              hc.smallu, o);
+            y[0] = o.f_rho; y[1] = n0 * o.f_un; y[2] = n1 * o.f_un; y[3] = n2 * o.f_un;
+            y[4] = o.f_eden; y[5] = o.f_eint;
+        };
+        auto a = [&](const std::vector<double> &x, const std::vector<double> &w,
+                     std::vector<double> &g) {
+            double fwb[NVAR] = {};
+            for (int i = 0; i < 6; ++i) fwb[i] = w[i];
+            double wb[6] = {}, cb = 0.0, csb = 0.0;
+            wallflux_adj(x.data(), x[6], x[7], n0, n1, n2, hc, fwb, wb, cb, csb);
+            for (int i = 0; i < 6; ++i) g[i] = wb[i];
+            g[6] = cb; g[7] = csb;
+        };
+        check("wallflux", 8, 6, gen, f, a, 400, st);
         done(st);
     }
 
