@@ -353,8 +353,129 @@ def bench_eb(args) -> int:
                         "unit": "GB/s", "frac": 160.0 * value / 8e12, "traffic": None,
                         "kernel": "whole step, 160 B per cell-update"},
            "cpu_baseline": cpu}
+    if not args.no_adjoint and args.eb_adj_steps > 0:
+        L.close()
+        out["adjoint"] = eb_adjoint_measure(prob, geom, init, args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def eb_adjoint_measure(prob, geom, init, args) -> dict:
+    """Adjoint of the x-force on the sphere after `eb_adj_steps` steps from
+    the Sod state (checkpoints every 5, dt frozen from the primal run):
+    backward over forward wall time; then the gradient of the same
+    functional after `eb_fd_steps` steps from a smooth state on the same grid
+    (``workloads.sod_sphere_smooth_init``) against central differences along
+    seeded unit directions over the hydro components of the non-covered
+    cells.  The Sod state's discontinuity puts limiters on their switches,
+    where differences measure secants; its one-sided slope gaps are
+    reported beside."""
+    import numpy as np
+    import torch
+    from camr_amd import _lib, problems, workloads
+    from camr_amd.hydro import Level
+    nb = len(prob.boxes)
+
+    def level(states):
+        L = Level(prob, engine=_lib.ENGINE_STAGED, graph=True, worker_streams=args.workers)
+        L.set_eb(geom)
+        for b, U in enumerate(states):
+            L.set_box(b, U)
+        L.init_post()
+        return L
+
+    res = {}
+    nsteps, every = args.eb_adj_steps, 5
+    L = level(init)
+    S0 = [L.get_box(b) for b in range(nb)]
+    dts = L.run(nsteps)
+    force_T = [float(x) for x in L.eb_force()]
+
+    def run_from(states, dts_):
+        for b in range(nb):
+            L.set_box(b, states[b])
+        for dt in dts_:
+            L.advance(dt)
+
+    def grad_of(states, dts_):
+        for b in range(nb):
+            L.set_box(b, states[b])
+        L.adjoint_forward(dts_, every)
+        L.seed_force_adjoint((1.0, 0.0, 0.0))
+        L.adjoint_backward()
+        return [L.get_adjoint(b) for b in range(nb)]
+
+    grad_of(S0, dts)  # warm-up: allocates the checkpoints and the adjoint arrays
+    times = {"forward": [], "forward_checkpointed": [], "backward": []}
+    for _ in range(max(1, args.adj_reps)):
+        torch.cuda.synchronize()
+        for b in range(nb):
+            L.set_box(b, S0[b])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for dt in dts:
+            L.advance(dt)
+        torch.cuda.synchronize()
+        times["forward"].append(time.perf_counter() - t0)
+        for b in range(nb):
+            L.set_box(b, S0[b])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.adjoint_forward(dts, every)
+        torch.cuda.synchronize()
+        times["forward_checkpointed"].append(time.perf_counter() - t0)
+        L.seed_force_adjoint((1.0, 0.0, 0.0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L.adjoint_backward()
+        torch.cuda.synchronize()
+        times["backward"].append(time.perf_counter() - t0)
+    ms = {k: 1e3 * min(v) for k, v in times.items()}
+    grad = [L.get_adjoint(b) for b in range(nb)]
+    res.update({"functional": "x-force on the sphere at the final state",
+                "steps": nsteps, "checkpoint_every": every, "force_at_T": force_T,
+                "forward_ms": ms["forward"], "forward_checkpointed_ms": ms["forward_checkpointed"],
+                "backward_ms": ms["backward"], "ratio": ms["backward"] / ms["forward"],
+                "grad_norm_sod": float(np.sqrt(sum(float((g * g).sum()) for g in grad)))})
+    # finite differences, smooth state
+    cov = [geom.covered_mask(prob.boxes[b][:3], prob.boxes[b][3:]) for b in range(nb)]
+    L.close()
+    if args.eb_fd_dirs <= 0:
+        return res
+    L = level([workloads.sod_sphere_smooth_init(prob, b, geom) for b in range(nb)])
+    S1 = [L.get_box(b) for b in range(nb)]
+    fdts = L.run(args.eb_fd_steps)
+    g = grad_of(S1, fdts)
+    gnorm = float(np.sqrt(sum(float((x * x).sum()) for x in g)))
+    rng = np.random.default_rng(problems.SEED_DIRECTIONS)
+    eps, rows = 1e-6, []
+    run_from(S1, fdts)
+    l0 = float(L.eb_force()[0])
+    for _ in range(args.eb_fd_dirs):
+        v = []
+        for s, c in zip(S1, cov):
+            x = np.zeros_like(s)
+            x[:6] = rng.standard_normal(x[:6].shape)
+            x[:, c] = 0.0
+            v.append(x)
+        nv = np.sqrt(sum(float((x * x).sum()) for x in v))
+        v = [x / nv for x in v]
+        run_from([s + eps * x for s, x in zip(S1, v)], fdts)
+        lp = float(L.eb_force()[0])
+        run_from([s - eps * x for s, x in zip(S1, v)], fdts)
+        lm = float(L.eb_force()[0])
+        fd = (lp - lm) / (2 * eps)
+        ad = sum(float((a * x).sum()) for a, x in zip(g, v))
+        rows.append((fd, ad, abs((lp - l0) / eps - (l0 - lm) / eps)))
+    res["fd_check"] = {
+        "state": "smooth state on the same grid", "steps": args.eb_fd_steps,
+        "directions": args.eb_fd_dirs, "eps": eps, "grad_norm": gnorm,
+        "max_abs_err_over_grad_norm": max(abs(f - a) / gnorm for f, a, _ in rows),
+        "max_abs_err_over_directional": max(abs(f - a) / abs(f) for f, a, _ in rows),
+        "max_one_sided_slope_gap_over_directional": max(k / abs(f) for f, _, k in rows),
+        "directions_fd_adjoint_slope_gap": [[f, a, k] for f, a, k in rows]}
+    L.close()
+    return res
 
 
 def bench_mol(args) -> int:
@@ -686,7 +807,7 @@ def main() -> int:
     ap.add_argument("--engine", choices=["fused", "staged"], default="fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
-                    help="c1: leave out the adjoint/forward ratio sub-measurement")
+                    help="c1 / eb: leave out the adjoint sub-measurement")
     ap.add_argument("--workload", choices=["c1", "amr", "adjoint", "sod", "eb", "mol", "amr_sync"],
                     default="c1",
                     help="c1: 256^3 PPM level step (default); amr: batched "
@@ -703,6 +824,12 @@ def main() -> int:
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--eb-n", type=int, default=128,
                     help="eb workload: n of the (2n) x n x n grid")
+    ap.add_argument("--eb-adj-steps", type=int, default=50,
+                    help="eb workload: steps of the x-force adjoint (0: left out)")
+    ap.add_argument("--eb-fd-steps", type=int, default=10,
+                    help="eb workload: steps of the adjoint's finite-difference check")
+    ap.add_argument("--eb-fd-dirs", type=int, default=4,
+                    help="eb workload: directions of the adjoint's finite-difference check")
     ap.add_argument("--adj-dirs", type=int, default=16,
                     help="finite-difference directions of the adjoint check")
     ap.add_argument("--adj-reps", type=int, default=3)
