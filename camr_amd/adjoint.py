@@ -8,6 +8,11 @@ frozen dt sequence; its backward runs the hand-written adjoint kernels,
 recomputing the steps from device checkpoints taken every `every` steps.
 The dt sequence is frozen: derivatives do not flow through the CFL
 estimate.  Use ``primal_dts`` to get the reference dt sequence of a run.
+
+``EbForce.apply(U0, level, dts, every, w)`` maps the initial state of an
+embedded-boundary level to w . F, F the force on the body after the steps
+(``Level.eb_force``); its backward seeds the adjoint with the force's own
+(``Level.seed_force_adjoint``).
 """
 from __future__ import annotations
 
@@ -58,3 +63,25 @@ class LevelSteps(torch.autograd.Function):
         boxes = [level.get_adjoint(b) for b in range(len(level.prob.boxes))]
         g = boxes[0] if ctx.like.dim() == 4 else np.stack(boxes)
         return torch.as_tensor(g, dtype=ctx.like.dtype, device=ctx.like.device), None, None, None
+
+
+class EbForce(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, level: Level, dts, every: int = 5, w=(1.0, 0.0, 0.0)):
+        _set_state(level, U0)
+        level.adjoint_forward(np.asarray(dts, dtype=np.float64), every)
+        f = level.eb_force()
+        ctx.level = level
+        ctx.like = U0
+        ctx.w = tuple(float(x) for x in w)
+        return torch.tensor(float(np.dot(ctx.w, f)), dtype=U0.dtype, device=U0.device)
+
+    @staticmethod
+    def backward(ctx, gF):
+        level = ctx.level
+        level.seed_force_adjoint(ctx.w)
+        level.adjoint_backward()
+        boxes = [level.get_adjoint(b) for b in range(len(level.prob.boxes))]
+        g = boxes[0] if ctx.like.dim() == 4 else np.stack(boxes)
+        g = torch.as_tensor(g, dtype=ctx.like.dtype, device=ctx.like.device) * gF
+        return g, None, None, None, None

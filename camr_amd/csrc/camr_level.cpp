@@ -303,6 +303,15 @@ class HydroLevel {
     bool adj_tail_ready_ = false;  // blk_ holds the last block's states from the forward run
     double adj_time0_ = 0.0;
     size_t state_doubles_ = 0;
+    int64_t *state_off_ = nullptr;  // device: each box's offset in a flat state copy
+    // One launch over every box's state (grid.y = box), a grid-stride x.
+    template <typename K, typename... A>
+    void launch_flat(K kern, A... args) {
+        int64_t n = 1;
+        for (const Box &B : box_) n = std::max<int64_t>(n, (int64_t)NVAR * B.npts());
+        const unsigned gx = (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 2048);
+        hipLaunchKernelGGL(kern, dim3(gx, (unsigned)box_.size()), dim3(kBlock), 0, stream_, args...);
+    }
     // EB level: the step's adjoint through the cut-cell stages, for every
     // box that is not covered (the all-regular boxes with regular geometry,
     // which the cut-cell stages then compute as the regular path does)
@@ -622,6 +631,8 @@ void HydroLevel::release() {
     }
     for (auto p : ckpt_) fr(p);
     for (auto p : blk_) fr(p);
+    fr(state_off_);
+    state_off_ = nullptr;
     ckpt_.clear();
     blk_.clear();
     if (gstream_) (void)hipStreamDestroy(gstream_);
@@ -1273,6 +1284,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             // the cut-cell boxes: primitives per box, then hydro_umdrv_eb
             // (:182-226) in batched launches over all of them, on a worker
             // stream of their own
+            // (an EB level has no gravity source: set_eb refuses it)
             const hipStream_t cs = in_ws_ && nws_ ? wst_[nws_ > 1 ? 1 : 0] : stream_;
             for (size_t c = 0; c < cut_ids_.size(); ++c) {
                 const int b = cut_ids_[c];
@@ -1280,15 +1292,10 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
                     set_error("box %d has cut cells but no cut-cell data (camr_level_set_box_eb)", b);
                     return CAMR_ERR_PARAM;
                 }
-                const EbBox &E = cut_htab_[c];
-                launch_on(k_fill_value, E.bx, cs, E.dsdt, (int)NVAR, 0.0);
-                launch_on(k_ctoprim, E.qbox, cs, sbv_[b], E.q, E.qaux, hc_, status_);
-                if (grav_src_values_ && !desc_.hp.do_mol)
-                    launch_on(k_srctoprim, E.qbox, cs, E.q, E.qaux, dense(srchyd_[b], E.qbox),
-                              E.srcq);
             }
-            int rc = umdrv_eb_batch(cut_tab_, cut_htab_.data(), (int)cut_ids_.size(), hc_,
-                                    dt_dev_, grav_src_values_ ? 1 : 0, cs, &prof_);
+            const int nc = (int)cut_ids_.size();
+            int rc = eb_prim_batch(cut_tab_, cut_htab_.data(), nc, hc_, status_, cs, &prof_);
+            if (!rc) rc = umdrv_eb_batch(cut_tab_, cut_htab_.data(), nc, hc_, dt_dev_, 0, cs, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
         for (size_t b = 0; b < box_.size(); ++b) {
@@ -1768,25 +1775,21 @@ int HydroLevel::adjoint_scope() const {
     return CAMR_OK;
 }
 
+// The state of every box to / from a flat buffer (boxes one after another,
+// state_off_ holding each box's offset), one launch over all boxes.
 int HydroLevel::save_state(double *dst) {
-    size_t o = 0;
-    for (size_t b = 0; b < box_.size(); ++b) {
-        const size_t n = NVAR * box_[b].npts();
-        CAMR_HIP(hipMemcpyAsync(dst + o, snew_[b].p, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        o += n;
-    }
+    if (!state_off_) return CAMR_ERR_PARAM;
+    launch_flat(kb_state_flat, (const DevFab *)(a_is_new_ ? tab_a_ : tab_b_), dst,
+                (const int64_t *)state_off_, 1);
     return CAMR_OK;
 }
 
 int HydroLevel::load_state(const double *src) {
     int rc = resolve();
     if (rc) return rc;
-    size_t o = 0;
-    for (size_t b = 0; b < box_.size(); ++b) {
-        const size_t n = NVAR * box_[b].npts();
-        CAMR_HIP(hipMemcpyAsync(snew_[b].p, src + o, n * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        o += n;
-    }
+    if (!state_off_) return CAMR_ERR_PARAM;
+    launch_flat(kb_state_flat, (const DevFab *)(a_is_new_ ? tab_a_ : tab_b_),
+                const_cast<double *>(src), (const int64_t *)state_off_, 0);
     dt_ready_ = false;
     return CAMR_OK;
 }
@@ -1795,7 +1798,15 @@ int HydroLevel::adjoint_alloc(int nck, int every) {
     int rc = ebflags_ ? eb_adjoint_setup() : engine_.enable_adjoint();
     if (rc) return rc;
     state_doubles_ = 0;
-    for (const Box &B : box_) state_doubles_ += NVAR * B.npts();
+    std::vector<int64_t> off;
+    for (const Box &B : box_) {
+        off.push_back((int64_t)state_doubles_);
+        state_doubles_ += NVAR * B.npts();
+    }
+    if (!state_off_) {
+        CAMR_HIP(hipMalloc(&state_off_, off.size() * sizeof(int64_t)));
+        CAMR_HIP(hipMemcpy(state_off_, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
     if (abar_[0].empty()) {
         for (int q = 0; q < 2; ++q) {
             std::vector<DevFab> t(box_.size());
@@ -2078,19 +2089,16 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     const DevFab *told = a_is_new_ ? tab_b_ : tab_a_;
     clean_all(told, true);
     expand_all(told);
-    for (size_t b = 0; b < box_.size(); ++b)
-        launch_on(k_fill_value, box_[b], stream_, dense(dsdt_[b], box_[b]), (int)NVAR, 0.0);
-    for (size_t c = 0; c < eadj_ids_.size(); ++c) {
-        const EbBox &F = eadj_fh_[c];
-        launch_on(k_ctoprim, F.qbox, stream_, F.uin, F.q, F.qaux, hc_, status_);
-    }
+    for (size_t b = 0; b < box_.size(); ++b)  // covered boxes: no hydro, dS/dt = 0
+        if (kind_[b] == 2)
+            launch_on(k_fill_value, box_[b], stream_, dense(dsdt_[b], box_[b]), (int)NVAR, 0.0);
     const int n = (int)eadj_ids_.size();
-    rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
+    rc = eb_prim_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, status_, stream_, &prof_);
+    if (!rc) rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
     if (!rc) {
         const int out = 1 - acur_;
         zero_doubles(eadj_abuf_, eadj_an_, stream_);
-        for (size_t b = 0; b < box_.size(); ++b)
-            zero_doubles(abar_[out][b], NVAR * box_[b].npts(), stream_);
+        launch_all(kb_zero_state, 0, (const DevFab *)tab_abar_[out]);
         launch_all(kb_ebadj_post, 0, (const DevFab *)tab_sb_, (const DevFab *)tab_dsdt_,
                    (const DevFab *)tab_abar_[acur_], (const DevFab *)tab_ub_,
                    (const DevFab *)tab_sbb_, (const double *)dt_dev_, hc_,

@@ -141,6 +141,10 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep =
 // (srcq holds primitive sources).
 int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
                    const double *dtp, int has_src, hipStream_t st, Prof *prof = nullptr);
+// hydro_ctoprim of every box of the table over its qbox (Sborder in uin) and
+// the zero fill of its dS/dt, in one launch.
+int eb_prim_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc, int *status,
+                  hipStream_t st, Prof *prof = nullptr);
 // One box: its EbBox record goes into the head of scratch (written by a
 // kernel, so the call can be captured into a graph).
 int umdrv_eb(const UmdrvEbArgs &A, double *scratch, hipStream_t st, Prof *prof = nullptr);

@@ -15,6 +15,7 @@
 // its cut neighbours in the order the sequential scatter would, so the sum
 // is bit-identical and needs no atomics.
 #include "hydro_eb.h"
+#include "state_math.h"
 
 #include <cstring>
 #include <utility>
@@ -737,6 +738,31 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep) 
         B.dbox2.hi[d] = A.hc.domhi[d] + gp;
     }
     *out = B;
+}
+
+// hydro_ctoprim over qbox and the zero fill of dS/dt over bx
+// (CAMR_construct_hydro_source.cpp:108-120, the hyd_src setVal), every box
+// of the table in one launch.
+__global__ void __launch_bounds__(kBlock)
+k_eb_prim(const EbBox *tab, HydroConst hc, int *status) {
+    const EbBox &B = tab[blockIdx.y];
+    int i, j, k;
+    if (!range_point(range_of(B.qbox), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+        return;
+    pt_ctoprim(B.uin, B.q, B.qaux, hc, status, i, j, k);
+    if (B.bx.contains(i, j, k))
+        for (int n = 0; n < NVAR; ++n) B.dsdt(i, j, k, n) = 0.0;
+}
+
+int eb_prim_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc, int *status,
+                  hipStream_t st, Prof *prof) {
+    if (n <= 0) return CAMR_OK;
+    int64_t np = 0;
+    for (int b = 0; b < n; ++b) np = std::max<int64_t>(np, htab[b].qbox.npts());
+    Region reg(prof, "eb_ctoprim", st);
+    hipLaunchKernelGGL(k_eb_prim, dim3(grid_for(np), (unsigned)n), dim3(kBlock), 0, st, tab, hc,
+                       status);
+    return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 
 int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,

@@ -59,6 +59,8 @@ __global__ void k_fill_u64(Range r, unsigned long long *p, unsigned long long v)
 __global__ void kb_estdt(const DevFab *S, double dx, double dy, double dz,
                          unsigned long long *out, int *status, EbFlags eb);
 __global__ void kb_detect(const DevFab *S, double small_dens, int *flags, EbFlags eb);
+__global__ void kb_state_flat(const DevFab *S, double *flat, const int64_t *off, int to_flat);
+__global__ void kb_zero_state(const DevFab *S);
 __global__ void kb_floor_seq(const DevFab *S, int nbox, HydroConst hc, int *flags, EbFlags eb);
 __global__ void kb_clean_point(const DevFab *S, HydroConst hc);
 __global__ void kb_normalize(const DevFab *S);

@@ -15,42 +15,7 @@ __global__ void __launch_bounds__(kBlock)
 k_ctoprim(Range r, Arr u, Arr q, Arr qa, HydroConst hc, int *status) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
-    const double rho = u(i, j, k, URHO);
-    if (!(rho > 0.)) {
-        if (hc.eb) {  // covered cell: q = 0 (CAMR_construct_hydro_source.cpp:116-118)
-            for (int n = 0; n < QVAR; ++n) q(i, j, k, n) = 0.;
-            for (int n = 0; n < NQAUX; ++n) qa(i, j, k, n) = 0.;
-        }
-        return;
-    }
-    const double rhoinv = 1.0 / rho;
-    const double vx = u(i, j, k, UMX) * rhoinv;
-    const double vy = u(i, j, k, UMY) * rhoinv;
-    const double vz = u(i, j, k, UMZ) * rhoinv;
-    const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-    q(i, j, k, QRHO) = rho;
-    q(i, j, k, QU) = vx;
-    q(i, j, k, QV) = vy;
-    q(i, j, k, QW) = vz;
-    const double ueden = u(i, j, k, UEDEN);
-    const double ueint = u(i, j, k, UEINT);
-    if (!hc.allow_negative_energy && !(ueden > 0. && ueint > 0.))
-        atomicOr(status, 1 << CAMR_ERR_NEG_ENERGY);
-    const double e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv
-                                                         : ueint * rhoinv;
-#pragma unroll
-    for (int s = 0; s < NSPEC; ++s) q(i, j, k, QFS + s) = u(i, j, k, UFS + s) * rhoinv;
-    const double p = (kGamma - 1.0) * rho * e;
-    q(i, j, k, QTEMP) = eos_T_from_e(e);
-    q(i, j, k, QREINT) = e * rho;
-    q(i, j, k, QPRES) = p;
-    q(i, j, k, QGAME) = p / (e * rho) + 1.0;
-    qa(i, j, k, QDPDR) = (kGamma - 1.0) * e;
-    qa(i, j, k, QDPDE) = (kGamma - 1.0) * rho;
-    qa(i, j, k, QGAMC) = kGamma;
-    const double cs = sqrt(kGamma * p / rho);
-    qa(i, j, k, QC) = cs;
-    qa(i, j, k, QCSML) = vmax(hc.small_num, hc.small_num * cs);
+    pt_ctoprim(u, q, qa, hc, status, i, j, k);
 }
 
 // hydro_srctoprim (Hydro_ctoprim.H:106-157)
@@ -432,6 +397,31 @@ __device__ __forceinline__ bool batch_cell(const DevFab *tab, int g, int &b, int
 
 // k_estdt over every box of the table in one launch (the minimum is exact,
 // so the box order does not matter)
+// The state of every box (NVAR dense components) to or from a flat buffer
+// that holds the boxes one after another (off[b]: box b's first double).
+__global__ void __launch_bounds__(kBlock)
+kb_state_flat(const DevFab *S, double *flat, const int64_t *off, int to_flat) {
+    const int b = blockIdx.y;
+    double *p = S[b].a.p;
+    double *f = flat + off[b];
+    const int64_t n = (int64_t)NVAR * S[b].b.npts();
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        if (to_flat) f[t] = p[t];
+        else p[t] = f[t];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_zero_state(const DevFab *S) {
+    const int b = blockIdx.y;
+    double *p = S[b].a.p;
+    const int64_t n = (int64_t)NVAR * S[b].b.npts();
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
 __global__ void __launch_bounds__(kBlock)
 kb_estdt(const DevFab *S, double dx, double dy, double dz, unsigned long long *out, int *status,
          EbFlags eb) {

@@ -127,6 +127,51 @@ __device__ __forceinline__ void floor_detect(double rho, double small_dens, int 
 }
 
 
+// ---------------------------------------------------------------- ctoprim
+// hydro_ctoprim (Source/Hydro/Hydro_ctoprim.H:11-101), all QVAR / NQAUX
+// components of cell (i,j,k).  Cells with rho <= 0 are left untouched as in
+// the reference (q = 0 in an EB build: covered cells).
+__device__ __forceinline__ void pt_ctoprim(const Arr &u, const Arr &q, const Arr &qa,
+                                           const HydroConst &hc, int *status, int i, int j,
+                                           int k) {
+    const double rho = u(i, j, k, URHO);
+    if (!(rho > 0.)) {
+        if (hc.eb) {  // covered cell: q = 0 (CAMR_construct_hydro_source.cpp:116-118)
+            for (int n = 0; n < QVAR; ++n) q(i, j, k, n) = 0.;
+            for (int n = 0; n < NQAUX; ++n) qa(i, j, k, n) = 0.;
+        }
+        return;
+    }
+    const double rhoinv = 1.0 / rho;
+    const double vx = u(i, j, k, UMX) * rhoinv;
+    const double vy = u(i, j, k, UMY) * rhoinv;
+    const double vz = u(i, j, k, UMZ) * rhoinv;
+    const double kineng = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    q(i, j, k, QRHO) = rho;
+    q(i, j, k, QU) = vx;
+    q(i, j, k, QV) = vy;
+    q(i, j, k, QW) = vz;
+    const double ueden = u(i, j, k, UEDEN);
+    const double ueint = u(i, j, k, UEINT);
+    if (!hc.allow_negative_energy && !(ueden > 0. && ueint > 0.))
+        atomicOr(status, 1 << CAMR_ERR_NEG_ENERGY);
+    const double e = ((ueden - kineng) / ueden > hc.eta1) ? (ueden - kineng) * rhoinv
+                                                         : ueint * rhoinv;
+#pragma unroll
+    for (int s = 0; s < NSPEC; ++s) q(i, j, k, QFS + s) = u(i, j, k, UFS + s) * rhoinv;
+    const double p = (kGamma - 1.0) * rho * e;
+    q(i, j, k, QTEMP) = eos_T_from_e(e);
+    q(i, j, k, QREINT) = e * rho;
+    q(i, j, k, QPRES) = p;
+    q(i, j, k, QGAME) = p / (e * rho) + 1.0;
+    qa(i, j, k, QDPDR) = (kGamma - 1.0) * e;
+    qa(i, j, k, QDPDE) = (kGamma - 1.0) * rho;
+    qa(i, j, k, QGAMC) = kGamma;
+    const double cs = sqrt(kGamma * p / rho);
+    qa(i, j, k, QC) = cs;
+    qa(i, j, k, QCSML) = vmax(hc.small_num, hc.small_num * cs);
+}
+
 // ---------------------------------------------------------- ghost filling
 // Source index and sign of component n for a cell c of the grown box:
 // periodic wrap, foextrap (Outflow; Inflow with the identity bcnormal of

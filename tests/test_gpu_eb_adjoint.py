@@ -127,3 +127,23 @@ def test_eb_force_seed_needs_an_adjoint_run():
     L.set_eb(geom)
     with pytest.raises(_lib.CamrError):
         L.seed_force_adjoint((1.0, 0.0, 0.0))
+
+
+def test_eb_force_autograd_function_matches_direct_adjoint():
+    torch = pytest.importorskip("torch")
+    from camr_amd.adjoint import EbForce, LevelSteps
+    nsteps = 3
+    prob, L, S0, dts, cov = _setup(16, 8, nsteps)
+    grad = _grad(L, prob, S0, dts, 2, lambda lv: lv.seed_force_adjoint((1.0, 0.0, 0.0)))
+    U0 = torch.tensor(np.stack(S0), dtype=torch.float64, requires_grad=True)
+    F = EbForce.apply(U0, L, dts, 2)
+    F.backward()
+    # atomics: two backward passes agree to rounding, not bit for bit
+    g = U0.grad.numpy()
+    scale = max(float(np.abs(x).max()) for x in grad)
+    assert np.abs(g - np.stack(grad)).max() <= 1e-12 * scale
+    # the state map of the same level: a linear loss through LevelSteps
+    U1 = torch.tensor(np.stack(S0), dtype=torch.float64, requires_grad=True)
+    UT = LevelSteps.apply(U1, L, dts, 2)
+    UT[:, 0].sum().backward()
+    assert np.all(np.isfinite(U1.grad.numpy())) and float(U1.grad.abs().max()) > 0.0
