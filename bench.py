@@ -299,8 +299,9 @@ def bench_eb(args) -> int:
     cells = 2 * n * n * n
     init = [workloads.sod_sphere_init(prob, b, geom) for b in range(len(prob.boxes))]
     L = Level(prob, engine=_lib.ENGINE_STAGED, graph=True, worker_streams=args.workers)
-    L.set_eb(geom)
+    L.set_eb(geom, merge=not args.no_eb_merge)
     kinds = list(L.kinds)
+    nvirt, nmerged = L.merge_info()
     for b, U in enumerate(init):
         L.set_box(b, U)
     L.init_post()
@@ -346,6 +347,8 @@ def bench_eb(args) -> int:
                                   "slip walls y/z, 32^3 boxes, one captured graph per step",
                       "boxes": {"regular": kinds.count(0), "cut_cell": kinds.count(1),
                                 "covered": kinds.count(2)},
+                      "regular_batch": {"virtual_boxes": nvirt, "boxes_merged": nmerged,
+                                        "boxes_alone": kinds.count(0) - nmerged},
                       "geometry_setup_s": round(t_geom, 2)},
            "identical_to_oracle_2_steps": identical,
            "force_on_sphere_after_run": force,
@@ -824,6 +827,9 @@ def main() -> int:
     ap.add_argument("--adj-n", type=int, default=128)
     ap.add_argument("--eb-n", type=int, default=128,
                     help="eb workload: n of the (2n) x n x n grid")
+    ap.add_argument("--no-eb-merge", action="store_true",
+                    help="eb workload: run every regular box as itself in the batch (no "
+                         "virtual boxes)")
     ap.add_argument("--eb-adj-steps", type=int, default=50,
                     help="eb workload: steps of the x-force adjoint (0: left out)")
     ap.add_argument("--eb-fd-steps", type=int, default=10,

@@ -103,7 +103,7 @@ EXPORTS = [
     "camr_level_get_ctu",
     "camr_level_adjoint_forward", "camr_level_adj_upload_box", "camr_level_adj_download_box",
     "camr_level_adjoint_backward", "camr_level_adj_seed_eb_force", "camr_hydro_umdrv_eb", "camr_level_set_eb",
-    "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
+    "camr_level_set_eb_merge", "camr_level_eb_merge_info", "camr_level_box_kind", "camr_level_set_box_eb", "camr_level_exterior_from_coarse",
     "camr_flux_register_create", "camr_flux_register_destroy", "camr_level_average_down",
     "camr_amr_step", "camr_write_plotfile", "camr_level_eb_force", "camr_amr_advance",
     "camr_write_checkpoint", "camr_level_set_time",
@@ -182,6 +182,8 @@ def lib():
         L.camr_level_adj_seed_eb_force.argtypes = [vp, dp]
         i32p = C.POINTER(C.c_int32)
         L.camr_level_set_eb.argtypes = [vp, i32p, i32p, i32p]
+        L.camr_level_set_eb_merge.argtypes = [vp, C.c_int32]
+        L.camr_level_eb_merge_info.argtypes = [vp, i32p, i32p]
         L.camr_level_exterior_from_coarse.argtypes = [vp, vp, C.c_int32]
         L.camr_flux_register_create.argtypes = [vp, vp, C.c_int32, C.POINTER(vp)]
         L.camr_flux_register_destroy.argtypes = [vp]

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -14,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -163,6 +165,17 @@ class HydroLevel {
     int set_box_eb(int b, const double *vf, const double *const ap[3],
                    const double *const fc[3]);
     int box_kind(int b, int32_t *kind) const;
+    int set_eb_merge(int on) {
+        if (ebflags_) { set_error("set the merge of regular boxes before camr_level_set_eb"); return CAMR_ERR_PARAM; }
+        eb_merge_ = on != 0;
+        return CAMR_OK;
+    }
+    int eb_merge_info(int32_t *nvirtual, int32_t *nmerged) const {
+        if (!nvirtual || !nmerged) { set_error("null argument"); return CAMR_ERR_PARAM; }
+        *nvirtual = (int32_t)vbox_.size();
+        *nmerged = nmerged_;
+        return CAMR_OK;
+    }
     int eb_force(double f[3]);
 
   private:
@@ -183,6 +196,7 @@ class HydroLevel {
     DomInfo dom_{};
     std::vector<Box> box_;
     std::vector<double *> bufA_, bufB_, sb_, dsdt_, srchyd_;
+    std::vector<Arr> dsv_;  // dS/dt of every box (a view into its virtual box's on an EB level)
     bool a_is_new_ = true;
     std::vector<Arr> snew_, sold_, sbv_;
     DevFab *tab_old_ = nullptr, *tab_sb_ = nullptr, *tab_src_ = nullptr;
@@ -233,6 +247,17 @@ class HydroLevel {
     // per-box EB path
     FusedEngine reg_engine_;
     bool eb_batched_ = false;
+    // regular boxes of equal shape that tile a box in blocks of 2x2x2, 2x2
+    // or 2 run in that batch as the one virtual box they tile (fewer ghost
+    // cells): its Sborder gathered from theirs (kb_vgather over vg_dev_), its
+    // dS/dt one array that their rows of tab_dsdt_ view.  tab_sbx_ /
+    // tab_dsdtx_: the level's rows, then one per virtual box
+    bool eb_merge_ = true;
+    std::vector<Box> vbox_;
+    int nmerged_ = 0;
+    VGather *vg_dev_ = nullptr;
+    DevFab *tab_sbx_ = nullptr, *tab_dsdtx_ = nullptr;
+    int eb_regular_batch();
     std::vector<EbGeom> ebg_;
     std::vector<double *> ebbuf_;
     // the cut-cell boxes (kind 1) as one table of the batched EB path, each
@@ -616,7 +641,11 @@ void HydroLevel::release() {
     for (auto p : sb_) fr(p);
     for (auto p : dsdt_) fr(p);
     for (auto p : srchyd_) fr(p);
-    bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear();
+    bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear(); dsv_.clear();
+    vbox_.clear();
+    nmerged_ = 0;
+    vg_dev_ = nullptr;  // freed with ebbuf_
+    tab_sbx_ = tab_dsdtx_ = nullptr;
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
     tab_dsdt_ = nullptr;
     tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
@@ -729,6 +758,7 @@ int HydroLevel::create() {
         sb_.push_back(p);
         if ((rc = alloc(&p, NVAR * B.npts()))) return rc;
         dsdt_.push_back(p);
+        dsv_.push_back(dense(p, B));
         if (grav_src_values_) {
             if ((rc = alloc(&p, NVAR * grow(B, CAMR_NGROW).npts()))) return rc;
             srchyd_.push_back(p);
@@ -835,7 +865,7 @@ int HydroLevel::create() {
         fstride_ = max_nodes;
         {
             std::vector<DevFab> td(nb);
-            for (int b = 0; b < nb; ++b) { td[b].a = dense(dsdt_[b], box_[b]); td[b].b = box_[b]; }
+            for (int b = 0; b < nb; ++b) { td[b].a = dsv_[b]; td[b].b = box_[b]; }
             CAMR_HIP(hipMalloc(&tab_dsdt_, nb * sizeof(DevFab)));
             CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
         }
@@ -930,25 +960,7 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     std::vector<DevFab> t(nb);
     for (int b = 0; b < nb; ++b) { t[b].a = sbv_[b]; t[b].b = box_[b]; }
     CAMR_HIP(hipMemcpy(tab_sb_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
-    {
-        std::vector<Box> rb;
-        std::vector<Arr> va, vb;
-        std::vector<int> ids;
-        for (int b = 0; b < nb; ++b) {
-            if (kind_[b] != 0) continue;
-            rb.push_back(box_[b]);
-            va.push_back(dense(bufA_[b], box_[b]));
-            vb.push_back(dense(bufB_[b], box_[b]));
-            ids.push_back(b);
-        }
-        eb_batched_ = false;
-        if (!rb.empty()) {
-            rc = reg_engine_.setup(rb, va, vb, default_tuning(), ids);
-            reg_engine_.set_qgrow(ng_);
-            if (rc == CAMR_OK) eb_batched_ = true;
-            else if (rc != CAMR_ERR_SHAPE) return rc;  // too large: per-box path
-        }
-    }
+    if ((rc = eb_regular_batch())) return rc;
     rc = setup_workers(max_g, sc);
     if (rc) return rc;
     // the cut-cell boxes: arrays per box, table entries rewritten when a
@@ -983,6 +995,150 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
         int r = 0;
         for (int b = 0; b < nb; ++b)
             wmap_[b] = kind_[b] == 1 ? 1 + (r++ % (nws_ - 1)) : b % nws_;
+    }
+    return CAMR_OK;
+}
+
+// The regular boxes of an EB level (kind 0) as one batch of the fused
+// kernels.  Scanning them in (z, y, x) order of their low corners, each box
+// not yet taken starts the first block of 2x2x2, 2x2x1, 2x1x2, 1x2x2,
+// 2x1x1, 1x2x1 or 1x1x2 untaken regular boxes of its own shape that tile a
+// box; the batch runs that box instead of them.  A box's results depend
+// only on Sborder over grow(bx, 6), whose ghost cells hold the neighbours'
+// valid data (FillPatch's resync), so the virtual box computes the same
+// fluxes and dS/dt at every cell.
+int HydroLevel::eb_regular_batch() {
+    const int nb = (int)box_.size();
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool merge = eb_merge_ && pass == 0;
+        std::vector<int> vof(nb, -1);
+        std::vector<std::vector<int>> groups;
+        std::vector<Box> vb;
+        if (merge) {
+            std::map<std::array<int, 3>, int> at;
+            std::vector<int> ord;
+            for (int b = 0; b < nb; ++b)
+                if (kind_[b] == 0) {
+                    at[{box_[b].lo[0], box_[b].lo[1], box_[b].lo[2]}] = b;
+                    ord.push_back(b);
+                }
+            std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+                const Box &X = box_[x], &Y = box_[y];
+                return std::array<int, 3>{X.lo[2], X.lo[1], X.lo[0]} <
+                       std::array<int, 3>{Y.lo[2], Y.lo[1], Y.lo[0]};
+            });
+            static const int shapes[7][3] = {{2, 2, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2},
+                                             {2, 1, 1}, {1, 2, 1}, {1, 1, 2}};
+            for (int b : ord) {
+                if (vof[b] >= 0) continue;
+                const Box B = box_[b];
+                const int len[3] = {B.len(0), B.len(1), B.len(2)};
+                for (const auto &sh : shapes) {
+                    std::vector<int> g;
+                    bool ok = true;
+                    for (int c = 0; c < sh[2] && ok; ++c)
+                        for (int r = 0; r < sh[1] && ok; ++r)
+                            for (int a = 0; a < sh[0] && ok; ++a) {
+                                auto it = at.find({B.lo[0] + a * len[0], B.lo[1] + r * len[1],
+                                                   B.lo[2] + c * len[2]});
+                                ok = it != at.end() && vof[it->second] < 0;
+                                if (!ok) break;
+                                const Box M = box_[it->second];
+                                for (int d = 0; d < 3; ++d) ok = ok && M.len(d) == len[d];
+                                g.push_back(it->second);
+                            }
+                    Box V = B;
+                    for (int d = 0; d < 3; ++d) V.hi[d] = B.lo[d] + sh[d] * len[d] - 1;
+                    if (!ok || !FusedEngine::fits(V)) continue;
+                    for (int m : g) vof[m] = (int)groups.size();
+                    groups.push_back(g);
+                    vb.push_back(V);
+                    break;
+                }
+            }
+        }
+        // the virtual boxes' Sborder and dS/dt
+        const int nv = (int)groups.size();
+        auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
+        size_t tot = 0;
+        for (const Box &V : vb) tot += r32((size_t)NVAR * grow(V, ng_).npts()) + r32((size_t)NVAR * V.npts());
+        double *vbuf = nullptr;
+        if (nv) {
+            CAMR_HIP(hipMalloc(&vbuf, tot * sizeof(double)));
+            ebbuf_.push_back(vbuf);
+        }
+        std::vector<VGather> vg(nv);
+        std::vector<Arr> vds(nv);
+        double *p = vbuf;
+        for (int v = 0; v < nv; ++v) {
+            const Box V = vb[v];
+            vg[v].v = V;
+            vg[v].sb = dense(p, grow(V, ng_));
+            p += r32((size_t)NVAR * grow(V, ng_).npts());
+            vds[v] = dense(p, V);
+            p += r32((size_t)NVAR * V.npts());
+            vg[v].n = (int)groups[v].size();
+            for (int t = 0; t < 8; ++t) vg[v].id[t] = t < vg[v].n ? groups[v][t] : groups[v][0];
+        }
+        for (int b = 0; b < nb; ++b) dsv_[b] = vof[b] >= 0 ? vds[vof[b]] : dense(dsdt_[b], box_[b]);
+        // the batch: virtual boxes (rows nb + v of the extended tables), then
+        // the regular boxes left alone
+        std::vector<Box> rb;
+        std::vector<Arr> va, vbv;
+        std::vector<int> ids;
+        for (int v = 0; v < nv; ++v) {
+            rb.push_back(vb[v]);
+            va.push_back(vds[v]);  // the state views are read by the update only, which the
+            vbv.push_back(vds[v]); // batch leaves to the level
+            ids.push_back(nb + v);
+        }
+        for (int b = 0; b < nb; ++b) {
+            if (kind_[b] != 0 || vof[b] >= 0) continue;
+            rb.push_back(box_[b]);
+            va.push_back(dense(bufA_[b], box_[b]));
+            vbv.push_back(dense(bufB_[b], box_[b]));
+            ids.push_back(b);
+        }
+        eb_batched_ = false;
+        vbox_.clear();
+        nmerged_ = 0;
+        if (rb.empty()) break;
+        int rc = reg_engine_.setup(rb, va, vbv, default_tuning(), ids);
+        reg_engine_.set_qgrow(ng_);
+        if (rc == CAMR_ERR_SHAPE && merge) continue;  // retry box by box
+        if (rc == CAMR_ERR_SHAPE) break;              // too large: per-box path
+        if (rc) return rc;
+        eb_batched_ = true;
+        vbox_ = vb;
+        for (int b = 0; b < nb; ++b) nmerged_ += vof[b] >= 0;
+        std::vector<DevFab> ts(nb + nv), td(nb + nv);
+        for (int b = 0; b < nb; ++b) {
+            ts[b].a = sbv_[b]; ts[b].b = box_[b];
+            td[b].a = dsv_[b]; td[b].b = box_[b];
+        }
+        for (int v = 0; v < nv; ++v) {
+            ts[nb + v].a = vg[v].sb; ts[nb + v].b = vb[v];
+            td[nb + v].a = vds[v]; td[nb + v].b = vb[v];
+        }
+        CAMR_HIP(hipMalloc(&tab_sbx_, (nb + nv) * sizeof(DevFab)));
+        ebbuf_.push_back(reinterpret_cast<double *>(tab_sbx_));
+        CAMR_HIP(hipMalloc(&tab_dsdtx_, (nb + nv) * sizeof(DevFab)));
+        ebbuf_.push_back(reinterpret_cast<double *>(tab_dsdtx_));
+        CAMR_HIP(hipMemcpy(tab_sbx_, ts.data(), ts.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(tab_dsdtx_, td.data(), td.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+        if (nv) {
+            CAMR_HIP(hipMalloc(&vg_dev_, nv * sizeof(VGather)));
+            ebbuf_.push_back(reinterpret_cast<double *>(vg_dev_));
+            CAMR_HIP(hipMemcpy(vg_dev_, vg.data(), nv * sizeof(VGather), hipMemcpyHostToDevice));
+        }
+        break;
+    }
+    if (!eb_batched_) {
+        for (int b = 0; b < nb; ++b) dsv_[b] = dense(dsdt_[b], box_[b]);
+        std::vector<DevFab> td(nb);
+        for (int b = 0; b < nb; ++b) { td[b].a = dsv_[b]; td[b].b = box_[b]; }
+        CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
     }
     return CAMR_OK;
 }
@@ -1104,7 +1260,7 @@ int HydroLevel::cut_entry(int c) {
     for (int d = 0; d < 3; ++d) A.periodic[d] = desc_.is_periodic[d];
     A.dtp = dt_dev_;
     A.uin = sbv_[b];
-    A.dsdt = dense(dsdt_[b], B);
+    A.dsdt = dsv_[b];
     A.q = dense(p, G);
     A.srcq = dense(p + (size_t)QVAR * G.npts(), G);
     A.qaux = dense(p + (size_t)2 * QVAR * G.npts(), G);
@@ -1275,9 +1431,17 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             const hipStream_t es = in_ws_ && nws_ ? wst_[0] : stream_;
             const int has_src = grav_src_values_ ? 1 : 0;
             const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
-            int rc = reg_engine_.hydro(-1, parity(), false, tab_sb_, src, has_src, hc_, dt_dev_,
+            if (!vbox_.empty()) {
+                Region reg(&prof_, "vgather", es);
+                int64_t n = 1;
+                for (const Box &V : vbox_) n = std::max<int64_t>(n, grow(V, ng_).npts());
+                hipLaunchKernelGGL(kb_vgather, dim3(grid_for(n), (unsigned)vbox_.size()),
+                                   dim3(kBlock), 0, es, (const VGather *)vg_dev_,
+                                   (const DevFab *)tab_sb_, ng_);
+            }
+            int rc = reg_engine_.hydro(-1, parity(), false, tab_sbx_, src, has_src, hc_, dt_dev_,
                                        es, status_, &prof_, nullptr, true);
-            if (!rc) rc = reg_engine_.dsdt(-1, parity(), tab_dsdt_, hc_, es, &prof_);
+            if (!rc) rc = reg_engine_.dsdt(-1, parity(), tab_dsdtx_, hc_, es, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
         if (!cut_ids_.empty()) {
@@ -1305,7 +1469,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             double *wq = q_ + (size_t)w * QVAR * qstride_, *wqa = qaux_ + (size_t)w * NQAUX * qstride_;
             double *wsq = srcq_ + (size_t)w * QVAR * qstride_;
             double *wscr = scratch_.base() + (size_t)w * sstride_;
-            Arr ds = dense(dsdt_[b], B);
+            Arr ds = dsv_[b];
             const int kind = kind_.empty() ? 0 : kind_[b];
             // regular box of an EB level: reg_engine_ has written its dS/dt;
             // cut-cell box: the batch above has
@@ -1347,7 +1511,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
     }
     if (add)
         for (size_t b = 0; b < box_.size(); ++b)
-            launch_on(k_add_hydro, box_[b], cur_ws(b), snew_[b], dense(dsdt_[b], box_[b]), dt,
+            launch_on(k_add_hydro, box_[b], cur_ws(b), snew_[b], dsv_[b], dt,
                       hc_.small_dens, status_, ebf_);
     return CAMR_OK;
 }
@@ -2033,7 +2197,7 @@ int HydroLevel::eb_adjoint_setup() {
         for (int d = 0; d < 3; ++d) A.periodic[d] = desc_.is_periodic[d];
         A.dtp = dt_dev_;
         A.uin = sbv_[b];
-        A.dsdt = dense(dsdt_[b], B);
+        A.dsdt = dsv_[b];
         A.q = dense(fp, G);
         A.qaux = dense(fp + (size_t)QVAR * G.npts(), G);
         A.srcq = A.q;  // no sources
@@ -2091,7 +2255,7 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     expand_all(told);
     for (size_t b = 0; b < box_.size(); ++b)  // covered boxes: no hydro, dS/dt = 0
         if (kind_[b] == 2)
-            launch_on(k_fill_value, box_[b], stream_, dense(dsdt_[b], box_[b]), (int)NVAR, 0.0);
+            launch_on(k_fill_value, box_[b], stream_, dsv_[b], (int)NVAR, 0.0);
     const int n = (int)eadj_ids_.size();
     rc = eb_prim_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, status_, stream_, &prof_);
     if (!rc) rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
@@ -3401,6 +3565,12 @@ int camr_level_set_box_eb(camr_level *L, int32_t b, const double *vfrac, const d
     if (!L) return CAMR_ERR_PARAM;
     const double *ap[3] = {apx, apy, apz}, *fc[3] = {fcx, fcy, fcz};
     return L->impl.set_box_eb(b, vfrac, ap, fc);
+}
+int camr_level_set_eb_merge(camr_level *L, int32_t on) {
+    return L ? L->impl.set_eb_merge(on) : CAMR_ERR_PARAM;
+}
+int camr_level_eb_merge_info(camr_level *L, int32_t *nvirtual, int32_t *nmerged) {
+    return L ? L->impl.eb_merge_info(nvirtual, nmerged) : CAMR_ERR_PARAM;
 }
 int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind) {
     return L ? L->impl.box_kind(b, kind) : CAMR_ERR_PARAM;

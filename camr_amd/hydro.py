@@ -133,11 +133,14 @@ class Level:
         of their children."""
         L.check(L.lib().camr_level_average_down(self._h, fine._h, int(ratio)))
 
-    def set_eb(self, geom):
+    def set_eb(self, geom, merge: bool = True):
         """Embedded-boundary build of the level with the cut-cell data of
         `geom` (``camr_amd.eb.Geometry``).  The level must have been created
-        with ``engine=ENGINE_STAGED``; call before the first step."""
+        with ``engine=ENGINE_STAGED``; call before the first step.  `merge`:
+        equal regular boxes that tile a box in blocks of 2x2x2, 2x2 or 2 run
+        as that box (same results, fewer ghost cells)."""
         i32 = C.POINTER(C.c_int32)
+        L.check(L.lib().camr_level_set_eb_merge(self._h, 1 if merge else 0))
         flags = np.ascontiguousarray(geom.flags, dtype=np.int32)
         flo = np.ascontiguousarray(geom.lo, dtype=np.int32)
         fhi = np.ascontiguousarray(geom.hi, dtype=np.int32)
@@ -154,6 +157,13 @@ class Level:
                         for n in ("vfrac", "apx", "apy", "apz", "fcx", "fcy", "fcz")]
                 L.check(L.lib().camr_level_set_box_eb(
                     self._h, b, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs]))
+
+    def merge_info(self):
+        """(virtual boxes, level boxes they hold) of an EB level's batch of
+        regular boxes."""
+        nv, nm = C.c_int32(), C.c_int32()
+        L.check(L.lib().camr_level_eb_merge_info(self._h, C.byref(nv), C.byref(nm)))
+        return nv.value, nm.value
 
     def state_array(self, b) -> L.Array4:
         a = L.Array4()

@@ -365,6 +365,15 @@ int camr_level_set_time(camr_level *L, double t);
 int camr_level_set_eb(camr_level *L, const int32_t *flags, const int32_t flo[3],
                       const int32_t fhi[3]);
 int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind);
+/* Launch shape of the regular boxes of an EB level, which run as one batch
+ * of the fused kernels (the per-box loop of
+ * CAMR_construct_hydro_source.cpp:146-238): with merging on (the default),
+ * equal regular boxes that tile a box in blocks of 2x2x2, 2x2 or 2 run as
+ * that one box, with fewer ghost cells; results are identical either way.
+ * Set before camr_level_set_eb.  _info: the number of such virtual boxes
+ * and of the level boxes they hold. */
+int camr_level_set_eb_merge(camr_level *L, int32_t on);
+int camr_level_eb_merge_info(camr_level *L, int32_t *nvirtual, int32_t *nmerged);
 
 /* The force on the embedded body from the level's current state: the sum
  * over all cut cells of the wall momentum flux (compute_hyp_wallflux's

@@ -240,3 +240,30 @@ def test_eb_level_without_cut_cells_matches_plain_level():
     assert np.array_equal(np.asarray(A.run(3)), np.asarray(B.run(3)))
     for b in range(len(prob.boxes)):
         assert np.array_equal(A.get_box(b), B.get_box(b))
+
+
+def test_eb_merged_regular_boxes_match_box_by_box():
+    """The regular boxes of an EB level run as virtual boxes (blocks of equal
+    boxes that tile a box, their Sborder gathered from the boxes' own) and
+    box by box: the same dt sequence and the same state bit for bit, with
+    the 8^3 boxes of a 64x32x32 Sod-past-a-sphere grid merged into blocks."""
+    print("EB merge: 64x32x32, 8^3 boxes, merged vs box by box", flush=True)
+    prob = workloads.sod_sphere(n=32, max_grid=8, interface=0.4)
+    geom = workloads.sod_sphere_geometry(prob)
+    A = Level(prob, engine=_lib.ENGINE_STAGED)
+    A.set_eb(geom)
+    B = Level(prob, engine=_lib.ENGINE_STAGED)
+    B.set_eb(geom, merge=False)
+    nv, nm = A.merge_info()
+    assert nv > 0 and nm >= 2 * nv
+    assert B.merge_info() == (0, 0)
+    for b in range(len(prob.boxes)):
+        U = workloads.sod_sphere_init(prob, b, geom)
+        A.set_box(b, U)
+        B.set_box(b, U)
+    A.init_post()
+    B.init_post()
+    assert np.array_equal(np.asarray(A.run(5)), np.asarray(B.run(5)))
+    for b in range(len(prob.boxes)):
+        assert np.array_equal(A.get_box(b), B.get_box(b)), b
+    assert np.array_equal(A.eb_force(), B.eb_force())
