@@ -247,8 +247,8 @@ class HydroLevel {
     // per-box EB path
     FusedEngine reg_engine_;
     bool eb_batched_ = false;
-    // regular boxes of equal shape that tile a box in blocks of 2x2x2, 2x2
-    // or 2 run in that batch as the one virtual box they tile (fewer ghost
+    // regular boxes of equal shape that tile a box in blocks of up to
+    // 4x4x4 run in that batch as the one virtual box they tile (fewer ghost
     // cells): its Sborder gathered from theirs (kb_vgather over vg_dev_), its
     // dS/dt one array that their rows of tab_dsdt_ view.  tab_sbx_ /
     // tab_dsdtx_: the level's rows, then one per virtual box
@@ -256,6 +256,7 @@ class HydroLevel {
     std::vector<Box> vbox_;
     int nmerged_ = 0;
     VGather *vg_dev_ = nullptr;
+    int *vg_ids_ = nullptr;
     DevFab *tab_sbx_ = nullptr, *tab_dsdtx_ = nullptr;
     int eb_regular_batch();
     std::vector<EbGeom> ebg_;
@@ -645,6 +646,7 @@ void HydroLevel::release() {
     vbox_.clear();
     nmerged_ = 0;
     vg_dev_ = nullptr;  // freed with ebbuf_
+    vg_ids_ = nullptr;
     tab_sbx_ = tab_dsdtx_ = nullptr;
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
     tab_dsdt_ = nullptr;
@@ -1001,9 +1003,9 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
 
 // The regular boxes of an EB level (kind 0) as one batch of the fused
 // kernels.  Scanning them in (z, y, x) order of their low corners, each box
-// not yet taken starts the first block of 2x2x2, 2x2x1, 2x1x2, 1x2x2,
-// 2x1x1, 1x2x1 or 1x1x2 untaken regular boxes of its own shape that tile a
-// box; the batch runs that box instead of them.  A box's results depend
+// not yet taken starts the largest block of sx x sy x sz (each 1..4, most
+// boxes first) untaken regular boxes of its own shape that tile a box the
+// fused engine's layout holds; the batch runs that box instead of them.  A box's results depend
 // only on Sborder over grow(bx, 6), whose ghost cells hold the neighbours'
 // valid data (FillPatch's resync), so the virtual box computes the same
 // fluxes and dS/dt at every cell.
@@ -1012,8 +1014,9 @@ int HydroLevel::eb_regular_batch() {
     for (int pass = 0; pass < 2; ++pass) {
         const bool merge = eb_merge_ && pass == 0;
         std::vector<int> vof(nb, -1);
-        std::vector<std::vector<int>> groups;
+        std::vector<std::vector<int>> groups;  // boxes of each block, x fastest
         std::vector<Box> vb;
+        std::vector<std::array<int, 3>> vsh;
         if (merge) {
             std::map<std::array<int, 3>, int> at;
             std::vector<int> ord;
@@ -1027,8 +1030,15 @@ int HydroLevel::eb_regular_batch() {
                 return std::array<int, 3>{X.lo[2], X.lo[1], X.lo[0]} <
                        std::array<int, 3>{Y.lo[2], Y.lo[1], Y.lo[0]};
             });
-            static const int shapes[7][3] = {{2, 2, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2},
-                                             {2, 1, 1}, {1, 2, 1}, {1, 1, 2}};
+            std::vector<std::array<int, 3>> shapes;
+            for (int sz = 1; sz <= 4; ++sz)
+                for (int sy = 1; sy <= 4; ++sy)
+                    for (int sx = 1; sx <= 4; ++sx)
+                        if (sx * sy * sz > 1) shapes.push_back({sx, sy, sz});
+            std::stable_sort(shapes.begin(), shapes.end(),
+                             [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
+                                 return x[0] * x[1] * x[2] > y[0] * y[1] * y[2];
+                             });
             for (int b : ord) {
                 if (vof[b] >= 0) continue;
                 const Box B = box_[b];
@@ -1053,6 +1063,7 @@ int HydroLevel::eb_regular_batch() {
                     for (int m : g) vof[m] = (int)groups.size();
                     groups.push_back(g);
                     vb.push_back(V);
+                    vsh.push_back(sh);
                     break;
                 }
             }
@@ -1069,6 +1080,7 @@ int HydroLevel::eb_regular_batch() {
         }
         std::vector<VGather> vg(nv);
         std::vector<Arr> vds(nv);
+        std::vector<int> vids;
         double *p = vbuf;
         for (int v = 0; v < nv; ++v) {
             const Box V = vb[v];
@@ -1077,8 +1089,12 @@ int HydroLevel::eb_regular_batch() {
             p += r32((size_t)NVAR * grow(V, ng_).npts());
             vds[v] = dense(p, V);
             p += r32((size_t)NVAR * V.npts());
-            vg[v].n = (int)groups[v].size();
-            for (int t = 0; t < 8; ++t) vg[v].id[t] = t < vg[v].n ? groups[v][t] : groups[v][0];
+            for (int d = 0; d < 3; ++d) {
+                vg[v].s[d] = vsh[v][d];
+                vg[v].len[d] = box_[groups[v][0]].len(d);
+            }
+            vg[v].off = (int)vids.size();
+            vids.insert(vids.end(), groups[v].begin(), groups[v].end());
         }
         for (int b = 0; b < nb; ++b) dsv_[b] = vof[b] >= 0 ? vds[vof[b]] : dense(dsdt_[b], box_[b]);
         // the batch: virtual boxes (rows nb + v of the extended tables), then
@@ -1131,6 +1147,9 @@ int HydroLevel::eb_regular_batch() {
             CAMR_HIP(hipMalloc(&vg_dev_, nv * sizeof(VGather)));
             ebbuf_.push_back(reinterpret_cast<double *>(vg_dev_));
             CAMR_HIP(hipMemcpy(vg_dev_, vg.data(), nv * sizeof(VGather), hipMemcpyHostToDevice));
+            CAMR_HIP(hipMalloc(&vg_ids_, vids.size() * sizeof(int)));
+            ebbuf_.push_back(reinterpret_cast<double *>(vg_ids_));
+            CAMR_HIP(hipMemcpy(vg_ids_, vids.data(), vids.size() * sizeof(int), hipMemcpyHostToDevice));
         }
         break;
     }
@@ -1437,7 +1456,7 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
                 for (const Box &V : vbox_) n = std::max<int64_t>(n, grow(V, ng_).npts());
                 hipLaunchKernelGGL(kb_vgather, dim3(grid_for(n), (unsigned)vbox_.size()),
                                    dim3(kBlock), 0, es, (const VGather *)vg_dev_,
-                                   (const DevFab *)tab_sb_, ng_);
+                                   (const int *)vg_ids_, (const DevFab *)tab_sb_, ng_);
             }
             int rc = reg_engine_.hydro(-1, parity(), false, tab_sbx_, src, has_src, hc_, dt_dev_,
                                        es, status_, &prof_, nullptr, true);

@@ -72,17 +72,19 @@ __global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFa
                                const double *dtp, double g, int add_grav);
 __global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp,
                              double small_dens, int *flags, EbFlags eb);
-// A virtual box: up to eight boxes of a level that tile the box v, and the
-// Sborder of v over grow(v, g) gathered from theirs.
+// A virtual box: s[0] x s[1] x s[2] boxes of a level, each len[0] x len[1]
+// x len[2] cells, that tile the box v, and the Sborder of v over grow(v, g)
+// gathered from theirs.  ids[off + (c s[1] + r) s[0] + a]: the row in the
+// level's Sborder table of the box at block position (a, r, c).
 struct VGather {
     Box v;
     Arr sb;
-    int n;
-    int id[8];  // rows of the level's Sborder table
+    int len[3], s[3];
+    int off;
 };
 // Every point of grow(v, g) of each virtual box of vt (grid.y = virtual
 // box) from the Sborder of the box that holds the point clamped to v.
-__global__ void kb_vgather(const VGather *vt, const DevFab *sb_tab, int g);
+__global__ void kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,

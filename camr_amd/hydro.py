@@ -137,8 +137,8 @@ class Level:
         """Embedded-boundary build of the level with the cut-cell data of
         `geom` (``camr_amd.eb.Geometry``).  The level must have been created
         with ``engine=ENGINE_STAGED``; call before the first step.  `merge`:
-        equal regular boxes that tile a box in blocks of 2x2x2, 2x2 or 2 run
-        as that box (same results, fewer ghost cells)."""
+        equal regular boxes that tile a box in blocks of up to 4x4x4 run as
+        that box (same results, fewer ghost cells)."""
         i32 = C.POINTER(C.c_int32)
         L.check(L.lib().camr_level_set_eb_merge(self._h, 1 if merge else 0))
         flags = np.ascontiguousarray(geom.flags, dtype=np.int32)

@@ -368,8 +368,8 @@ int camr_level_box_kind(camr_level *L, int32_t b, int32_t *kind);
 /* Launch shape of the regular boxes of an EB level, which run as one batch
  * of the fused kernels (the per-box loop of
  * CAMR_construct_hydro_source.cpp:146-238): with merging on (the default),
- * equal regular boxes that tile a box in blocks of 2x2x2, 2x2 or 2 run as
- * that one box, with fewer ghost cells; results are identical either way.
+ * equal regular boxes that tile a box in blocks of up to 4x4x4 run as that
+ * one box, with fewer ghost cells; results are identical either way.
  * Set before camr_level_set_eb.  _info: the number of such virtual boxes
  * and of the level boxes they hold. */
 int camr_level_set_eb_merge(camr_level *L, int32_t on);
