@@ -257,8 +257,18 @@ class HydroLevel {
     int nmerged_ = 0;
     VGather *vg_dev_ = nullptr;
     int *vg_ids_ = nullptr;
+    std::vector<int> vof_;       // virtual box of each level box, or -1
+    std::vector<VGather> vgh_;   // host copy of vg_dev_
+    std::vector<Arr> vds_;       // dS/dt of each virtual box
+    VGather *vga_dev_ = nullptr; // the adjoint's: sb = dL/dSborder of each virtual box
     DevFab *tab_sbx_ = nullptr, *tab_dsdtx_ = nullptr;
     int eb_regular_batch();
+    // points of the largest grow(v, 6) (0: no virtual box)
+    int64_t vgather_points() const {
+        int64_t n = 0;
+        for (const Box &V : vbox_) n = std::max<int64_t>(n, grow(V, ng_).npts());
+        return n;
+    }
     std::vector<EbGeom> ebg_;
     std::vector<double *> ebbuf_;
     // the cut-cell boxes (kind 1) as one table of the batched EB path, each
@@ -647,6 +657,10 @@ void HydroLevel::release() {
     nmerged_ = 0;
     vg_dev_ = nullptr;  // freed with ebbuf_
     vg_ids_ = nullptr;
+    vga_dev_ = nullptr;
+    vof_.clear();
+    vgh_.clear();
+    vds_.clear();
     tab_sbx_ = tab_dsdtx_ = nullptr;
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
     tab_dsdt_ = nullptr;
@@ -1117,6 +1131,9 @@ int HydroLevel::eb_regular_batch() {
         }
         eb_batched_ = false;
         vbox_.clear();
+        vof_.assign(nb, -1);
+        vgh_.clear();
+        vds_.clear();
         nmerged_ = 0;
         if (rb.empty()) break;
         int rc = reg_engine_.setup(rb, va, vbv, default_tuning(), ids);
@@ -1126,6 +1143,9 @@ int HydroLevel::eb_regular_batch() {
         if (rc) return rc;
         eb_batched_ = true;
         vbox_ = vb;
+        vof_ = vof;
+        vgh_ = vg;
+        vds_ = vds;
         for (int b = 0; b < nb; ++b) nmerged_ += vof[b] >= 0;
         std::vector<DevFab> ts(nb + nv), td(nb + nv);
         for (int b = 0; b < nb; ++b) {
@@ -1450,10 +1470,8 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             const hipStream_t es = in_ws_ && nws_ ? wst_[0] : stream_;
             const int has_src = grav_src_values_ ? 1 : 0;
             const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
-            if (!vbox_.empty()) {
+            if (const int64_t n = vgather_points()) {
                 Region reg(&prof_, "vgather", es);
-                int64_t n = 1;
-                for (const Box &V : vbox_) n = std::max<int64_t>(n, grow(V, ng_).npts());
                 hipLaunchKernelGGL(kb_vgather, dim3(grid_for(n), (unsigned)vbox_.size()),
                                    dim3(kBlock), 0, es, (const VGather *)vg_dev_,
                                    (const int *)vg_ids_, (const DevFab *)tab_sb_, ng_);
@@ -2152,13 +2170,16 @@ int HydroLevel::eb_adjoint_setup() {
         unit_geom_.f.js = 0; unit_geom_.f.ks = 0;
         unit_geom_.vf = row(po);
         for (int d = 0; d < 3; ++d) { unit_geom_.ap[d] = row(po); unit_geom_.fc[d] = row(pz); }
+        // the boxes of the recompute: every box with cut cells or regular
+        // alone (its level index), then the virtual boxes (-1 - index)
         eadj_ids_.clear();
         for (int b = 0; b < nb; ++b)
-            if (kind_[b] != 2) eadj_ids_.push_back(b);
+            if (kind_[b] != 2 && (vof_.empty() || vof_[b] < 0)) eadj_ids_.push_back(b);
+        for (size_t v = 0; v < vbox_.size(); ++v) eadj_ids_.push_back(-1 - (int)v);
         auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
         size_t fn = 0, an = 0;
-        for (int b : eadj_ids_) {
-            const Box B = box_[b], G = grow(B, ng_);
+        for (int id : eadj_ids_) {
+            const Box B = id >= 0 ? box_[id] : vbox_[-1 - id], G = grow(B, ng_);
             fn += r32((size_t)(QVAR + NQAUX) * G.npts());
             for (int d = 0; d < 3; ++d) fn += r32((size_t)NVAR * nodes(B, d).npts());
             fn += r32(umdrv_eb_scratch_doubles(B, G, true));
@@ -2167,6 +2188,12 @@ int HydroLevel::eb_adjoint_setup() {
         }
         for (int b = 0; b < nb; ++b)
             an += r32((size_t)NVAR * box_[b].npts()) + r32((size_t)NVAR * grow(box_[b], ng_).npts());
+        for (const Box &V : vbox_)
+            an += r32((size_t)NVAR * V.npts()) + r32((size_t)NVAR * grow(V, ng_).npts());
+        if (!vbox_.empty()) {
+            CAMR_HIP(hipMalloc(&vga_dev_, vbox_.size() * sizeof(VGather)));
+            ebbuf_.push_back(reinterpret_cast<double *>(vga_dev_));
+        }
         CAMR_HIP(hipMalloc(&eadj_fbuf_, fn * sizeof(double)));
         ebbuf_.push_back(eadj_fbuf_);
         CAMR_HIP(hipMalloc(&eadj_abuf_, an * sizeof(double)));
@@ -2206,17 +2233,30 @@ int HydroLevel::eb_adjoint_setup() {
             return CAMR_ERR_PARAM;
         }
     }
+    // each virtual box's dL/dS_new (the rows of its boxes view it) and
+    // dL/dSborder over grow(v, 6), added into its boxes' after the stages
+    std::vector<Arr> vub(vbox_.size());
+    std::vector<VGather> vga = vgh_;
+    for (size_t v = 0; v < vbox_.size(); ++v) {
+        const Box V = vbox_[v], G = grow(V, ng_);
+        vub[v] = dense(ap, V);
+        ap += r32((size_t)NVAR * V.npts());
+        vga[v].sb = dense(ap, G);
+        ap += r32((size_t)NVAR * G.npts());
+    }
+    for (size_t b = 0; b < box_.size(); ++b)
+        if (!vof_.empty() && vof_[b] >= 0) ubv[b] = vub[vof_[b]];
     for (size_t c = 0; c < eadj_ids_.size(); ++c) {
-        const int b = eadj_ids_[c];
-        const Box B = box_[b], G = grow(B, ng_);
+        const int id = eadj_ids_[c], v = -1 - id;
+        const Box B = id >= 0 ? box_[id] : vbox_[v], G = grow(B, ng_);
         UmdrvEbArgs A;
         A.bx = B;
         A.qbox = G;
         A.hc = hc_;
         for (int d = 0; d < 3; ++d) A.periodic[d] = desc_.is_periodic[d];
         A.dtp = dt_dev_;
-        A.uin = sbv_[b];
-        A.dsdt = dsv_[b];
+        A.uin = id >= 0 ? sbv_[id] : vgh_[v].sb;
+        A.dsdt = id >= 0 ? dsv_[id] : vds_[v];
         A.q = dense(fp, G);
         A.qaux = dense(fp + (size_t)QVAR * G.npts(), G);
         A.srcq = A.q;  // no sources
@@ -2226,7 +2266,7 @@ int HydroLevel::eb_adjoint_setup() {
             A.flx[d] = dense(fp, nodes(B, d));
             fp += r32((size_t)NVAR * nodes(B, d).npts());
         }
-        A.g = geo[b];
+        A.g = id >= 0 ? geo[id] : unit_geom_;
         eb_box_setup(A, fp, &eadj_fh_[c], true);
         fp += r32(umdrv_eb_scratch_doubles(B, G, true));
         UmdrvEbArgs Ab = A;
@@ -2234,8 +2274,8 @@ int HydroLevel::eb_adjoint_setup() {
         Ab.qaux = dense(ap + (size_t)QVAR * G.npts(), G);
         Ab.srcq = Ab.q;
         ap += r32((size_t)(QVAR + NQAUX) * G.npts());
-        Ab.uin = sbbv[b];
-        Ab.dsdt = ubv[b];
+        Ab.uin = id >= 0 ? sbbv[id] : vga[v].sb;
+        Ab.dsdt = id >= 0 ? ubv[id] : vub[v];
         eb_box_setup(Ab, ap, &eadj_h_[c].A, true);
         ap += r32(umdrv_eb_scratch_doubles(B, G, true));
         eadj_h_[c].F = eadj_fh_[c];
@@ -2254,6 +2294,8 @@ int HydroLevel::eb_adjoint_setup() {
     }
     CAMR_HIP(hipMemcpy(tab_ub_, tu.data(), tu.size() * sizeof(DevFab), hipMemcpyHostToDevice));
     CAMR_HIP(hipMemcpy(tab_sbb_, ts.data(), ts.size() * sizeof(DevFab), hipMemcpyHostToDevice));
+    if (!vga.empty())
+        CAMR_HIP(hipMemcpy(vga_dev_, vga.data(), vga.size() * sizeof(VGather), hipMemcpyHostToDevice));
     CAMR_HIP(hipMemcpy(geom_dev_, geo.data(), geo.size() * sizeof(EbGeom), hipMemcpyHostToDevice));
     CAMR_HIP(hipMemcpy(kinds_dev_, kd.data(), kd.size() * sizeof(int), hipMemcpyHostToDevice));
     return CAMR_OK;
@@ -2272,6 +2314,11 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     const DevFab *told = a_is_new_ ? tab_b_ : tab_a_;
     clean_all(told, true);
     expand_all(told);
+    const int64_t vn = vgather_points();
+    if (vn)
+        hipLaunchKernelGGL(kb_vgather, dim3(grid_for(vn), (unsigned)vbox_.size()), dim3(kBlock), 0,
+                           stream_, (const VGather *)vg_dev_, (const int *)vg_ids_,
+                           (const DevFab *)tab_sb_, ng_);
     for (size_t b = 0; b < box_.size(); ++b)  // covered boxes: no hydro, dS/dt = 0
         if (kind_[b] == 2)
             launch_on(k_fill_value, box_[b], stream_, dsv_[b], (int)NVAR, 0.0);
@@ -2287,6 +2334,10 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
                    (const DevFab *)tab_sbb_, (const double *)dt_dev_, hc_,
                    desc_.add_grav_src, status_, ebf_);
         rc = eb_adjoint_batch(eadj_tab_, eadj_h_.data(), n, hc_, dt_dev_, stream_, &prof_);
+        if (vn)
+            hipLaunchKernelGGL(kb_vscatter, dim3(grid_for(vn), (unsigned)vbox_.size()), dim3(kBlock),
+                               0, stream_, (const VGather *)vga_dev_, (const int *)vg_ids_,
+                               (const DevFab *)tab_sbb_, ng_);
         launch_all(kb_ebadj_fill, ng_, (const DevFab *)tab_sbb_, ng_, told,
                    (const DevFab *)tab_abar_[out], (const int *)owner_, dom_, hc_, status_);
         launch_all(kb_ebadj_old, 0, (const DevFab *)tab_abar_[out], told, hc_, status_);

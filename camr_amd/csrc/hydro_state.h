@@ -85,6 +85,9 @@ struct VGather {
 // Every point of grow(v, g) of each virtual box of vt (grid.y = virtual
 // box) from the Sborder of the box that holds the point clamped to v.
 __global__ void kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
+// Its adjoint: each point of grow(v, g) of vt's sb (here dL/dSborder of the
+// virtual box) added into sb_tab's array of the box kb_vgather read it from.
+__global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,

@@ -512,6 +512,25 @@ kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
     for (int n = 0; n < NVAR; ++n) V.sb(i, j, k, n) = s(i, j, k, n);
 }
 
+// Every point of grow(b, g) is the image of at most one point of the
+// virtual box (the clamp owner is unique), so the adds do not collide.
+__global__ void __launch_bounds__(kBlock)
+kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
+    const VGather &V = vt[blockIdx.y];
+    int i, j, k;
+    if (!range_point(range_of(grow(V.v, g)), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i,
+                     j, k))
+        return;
+    const int ci = min(max(i, V.v.lo[0]), V.v.hi[0]);
+    const int cj = min(max(j, V.v.lo[1]), V.v.hi[1]);
+    const int ck = min(max(k, V.v.lo[2]), V.v.hi[2]);
+    const int a = (ci - V.v.lo[0]) / V.len[0], r = (cj - V.v.lo[1]) / V.len[1],
+              c = (ck - V.v.lo[2]) / V.len[2];
+    const Arr s = sb_tab[ids[V.off + (c * V.s[1] + r) * V.s[0] + a]].a;
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) s(i, j, k, n) += V.sb(i, j, k, n);
+}
+
 // periodic image of an index in domain D (false outside a non-periodic
 // domain) and the coarse-fine flag of a cell
 __device__ __forceinline__ bool reg_map(const DomInfo &D, const int *c, int *m) {
