@@ -202,6 +202,7 @@ class HydroLevel {
     DevFab *tab_old_ = nullptr, *tab_sb_ = nullptr, *tab_src_ = nullptr;
     int *owner_ = nullptr;
     int *status_ = nullptr;              // device: error bits | floor flag
+    int *redo_ = nullptr;                // device: the staged tail must be repeated pass by pass
     unsigned long long *dtmin_ = nullptr;  // device: bit pattern of min dt
     int *h_status_ = nullptr;            // pinned host mirrors
     unsigned long long *h_dtmin_ = nullptr;
@@ -664,7 +665,8 @@ void HydroLevel::release() {
     tab_sbx_ = tab_dsdtx_ = nullptr;
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
     tab_dsdt_ = nullptr;
-    tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(dtmin_);
+    tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(redo_); fr(dtmin_);
+    redo_ = nullptr;
     fr(dt_dev_); fr(ext_); dt_dev_ = nullptr; ext_ = nullptr;
     release_graphs();
     fr(ctl_); fr(ctl_dts_); ctl_ = nullptr; ctl_dts_ = nullptr; ctl_cap_ = 0;
@@ -857,6 +859,8 @@ int HydroLevel::create() {
         CAMR_HIP(hipMemcpy(tab_src_, t.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
     }
     CAMR_HIP(hipMalloc(&status_, sizeof(int)));
+    CAMR_HIP(hipMalloc(&redo_, sizeof(int)));
+    CAMR_HIP(hipMemset(redo_, 0, sizeof(int)));
     CAMR_HIP(hipMalloc(&dtmin_, sizeof(unsigned long long)));
     CAMR_HIP(hipMalloc(&dt_dev_, sizeof(double)));
     CAMR_HIP(hipMemset(status_, 0, sizeof(int)));
@@ -1643,6 +1647,25 @@ int HydroLevel::advance_staged(double dt) {
     clean_all(told, true);
     // expand_state(Sborder, prev_time) (:99)
     expand_all(told);
+    if (!desc_.hp.do_mol) {
+        // construct_hydro_source (:138) per box on the worker streams (it
+        // reads Sborder, not S_new), then S_new = Sborder + old source +
+        // dt dS/dt and the clean_state passes up to post_timestep's
+        // (:112-154, CAMR.cpp:698-699) in one pass, repeated pass by pass
+        // by kb_tail_redo where a density floor would act
+        scratch_.reset();
+        int rc = hydro_on_workers(dt);
+        if (rc) return rc;
+        const int post = defer_post_ ? 0 : 1;
+        launch_all(kb_tail_fused, 0, tnew, (const DevFab *)tab_sb_, told,
+                   (const DevFab *)tab_dsdt_, (const double *)dt_dev_, desc_.const_grav,
+                   desc_.add_grav_src, post, hc_, redo_, ebf_);
+        hipLaunchKernelGGL(kb_tail_redo, dim3(1), dim3(1024), 0, stream_, tnew,
+                           (const DevFab *)tab_sb_, told, (const DevFab *)tab_dsdt_,
+                           (int)box_.size(), (const double *)dt_dev_, desc_.const_grav,
+                           desc_.add_grav_src, post, hc_, status_, redo_, ebf_);
+        return CAMR_OK;
+    }
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
     launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, (const double *)dt_dev_,
                desc_.const_grav, desc_.add_grav_src);

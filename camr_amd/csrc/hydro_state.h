@@ -88,6 +88,17 @@ __global__ void kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_t
 // Its adjoint: each point of grow(v, g) of vt's sb (here dL/dSborder of the
 // virtual box) added into sb_tab's array of the box kb_vgather read it from.
 __global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
+// The staged step's pointwise tail after the hydro source in one pass
+// (S_new = Sborder + sources + dt dS/dt and the clean_state passes), and
+// its pass-by-pass repeat where a density floor would act (one workgroup of
+// up to 1024 threads; a no-op unless kb_tail_fused set *redo).
+__global__ void kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold,
+                              const DevFab *dsdt, const double *dtp, double g, int add_grav,
+                              int post, HydroConst hc, int *redo, EbFlags eb);
+__global__ void kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold,
+                             const DevFab *dsdt, int nbox, const double *dtp, double g,
+                             int add_grav, int post, HydroConst hc, int *flags, int *redo,
+                             EbFlags eb);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,

@@ -531,6 +531,118 @@ kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
     for (int n = 0; n < NVAR; ++n) s(i, j, k, n) += V.sb(i, j, k, n);
 }
 
+// ------------------------------------------- the staged step's tail fused
+// S_new = Sborder + old source (CAMR_advance.cpp:112-118), += dt dS/dt
+// (:141), clean_state (:146), the new-time source and its clean_state
+// (:150-154) and post_timestep's clean_state (CAMR.cpp:698-699, when
+// `post`) on one cell in registers, the arithmetic of pt_copy_oldsrc,
+// pt_add_hydro, pt_clean_point and pt_add_newsrc in their order.  The
+// enforce_min_density passes between them act only on cells below
+// small_dens; where a floor check of the sequence would fire, the cell
+// sets *redo and kb_tail_redo repeats the tail pass by pass.
+__device__ __forceinline__ bool below_floor(double rho, double small_dens) {
+    return rho == 0. || rho < small_dens;
+}
+
+__global__ void __launch_bounds__(kBlock)
+kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
+              const double *dtp, double g, int add_grav, int post, HydroConst hc, int *redo,
+              EbFlags eb) {
+    int b, i, j, k;
+    if (!batch_cell(snew, 0, b, i, j, k)) return;
+    const Arr sn = snew[b].a, u = sb[b].a, so = sold[b].a, ds = dsdt[b].a;
+    const double dt = *dtp;
+    const bool cov = eb.covered(i, j, k);
+    double v[NVAR];
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) {
+        double x = u(i, j, k, n);
+        if (add_grav) {
+            double src = 0.0;
+            if (n == UMZ) src = g * so(i, j, k, URHO);
+            else if (n == UEDEN) src = g * so(i, j, k, UMZ);
+            x += dt * src;
+        }
+        v[n] = x;
+    }
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) v[n] += dt * ds(i, j, k, n);
+    bool fire = !cov && below_floor(v[URHO], hc.small_dens);
+    normalize_species(v);
+    reset_e_and_T(v, hc);
+    if (add_grav) {
+        const double s_umz = 0.5 * g * (v[URHO] - so(i, j, k, URHO));
+        const double s_ued = 0.5 * g * (v[UMZ] - so(i, j, k, UMZ));
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) {
+            const double src = (n == UMZ) ? s_umz : ((n == UEDEN) ? s_ued : 0.0);
+            v[n] += dt * src;
+        }
+        fire = fire || (!cov && below_floor(v[URHO], hc.small_dens));
+        normalize_species(v);
+        reset_e_and_T(v, hc);
+    }
+    if (post) {
+        fire = fire || (!cov && below_floor(v[URHO], hc.small_dens));
+        normalize_species(v);
+        reset_e_and_T(v, hc);
+    }
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) sn(i, j, k, n) = v[n];
+    if (fire) atomicOr(redo, 1);
+}
+
+// The same tail pass by pass (floor detection and enforce_min_density
+// included), by one workgroup whose barriers order the passes; returns at
+// once unless kb_tail_fused set *redo, which it clears.
+__global__ void __launch_bounds__(1024)
+kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
+             int nbox, const double *dtp, double g, int add_grav, int post, HydroConst hc,
+             int *flags, int *redo, EbFlags eb) {
+    if (!*redo) return;
+    const double dt = *dtp;
+    auto each = [&](auto f) {
+        for (int b = 0; b < nbox; ++b) {
+            const Range r = range_of(snew[b].b);
+            for (int64_t t = threadIdx.x; t < r.n; t += blockDim.x) {
+                int i, j, k;
+                range_point(r, t, i, j, k);
+                f(b, i, j, k);
+            }
+        }
+        __syncthreads();
+    };
+    auto floor = [&]() {
+        if (threadIdx.x == 0 &&
+            (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFloorFlag))
+            for (int b = 0; b < nbox; ++b) floor_seq_box(snew[b].b, snew[b].a, hc, eb);
+        __syncthreads();
+    };
+    each([&](int b, int i, int j, int k) {
+        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, dt, g, add_grav, i, j, k);
+    });
+    each([&](int b, int i, int j, int k) {
+        pt_add_hydro(snew[b].a, dsdt[b].a, dt, hc.small_dens, flags, eb, i, j, k);
+    });
+    floor();
+    each([&](int b, int i, int j, int k) { pt_clean_point(snew[b].a, hc, i, j, k); });
+    if (add_grav) {
+        each([&](int b, int i, int j, int k) {
+            pt_add_newsrc(snew[b].a, sold[b].a, dt, g, hc.small_dens, flags, eb, i, j, k);
+        });
+        floor();
+        each([&](int b, int i, int j, int k) { pt_clean_point(snew[b].a, hc, i, j, k); });
+    }
+    if (post) {
+        each([&](int b, int i, int j, int k) {
+            pt_detect(snew[b].a, hc.small_dens, flags, eb, i, j, k);
+        });
+        floor();
+        each([&](int b, int i, int j, int k) { pt_clean_point(snew[b].a, hc, i, j, k); });
+    }
+    if (threadIdx.x == 0) *redo = 0;
+}
+
 // periodic image of an index in domain D (false outside a non-periodic
 // domain) and the coarse-fine flag of a cell
 __device__ __forceinline__ bool reg_map(const DomInfo &D, const int *c, int *m) {
