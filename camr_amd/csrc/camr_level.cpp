@@ -288,7 +288,8 @@ class HydroLevel {
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
     DevFab *tab_dsdt_ = nullptr;                  // staged engine: dS/dt of every box
     int advance_staged(double dt);
-    void expand_all(const DevFab *src);
+    // cleaned: src went through clean_all (kb_expand_cleaned, one pass)
+    void expand_all(const DevFab *src, bool cleaned = false);
     int hydro_on_workers(double dt);
     // staged engine: the step captured once per buffer parity, replayed
     // with the step's dt in dt_dev_
@@ -1646,7 +1647,7 @@ int HydroLevel::advance_staged(double dt) {
     // clean_state(S_old) (:83)
     clean_all(told, true);
     // expand_state(Sborder, prev_time) (:99)
-    expand_all(told);
+    expand_all(told, true);
     if (!desc_.hp.do_mol) {
         // construct_hydro_source (:138) per box on the worker streams (it
         // reads Sborder, not S_new), then S_new = Sborder + old source +
@@ -1704,7 +1705,12 @@ int HydroLevel::advance_staged(double dt) {
 // expand_state (CAMR.cpp:1243-1250) from the states of `src`: FillPatch,
 // clean_state(Sborder) on the valid cells, FillBoundary, computeTemp on
 // the grown boxes.
-void HydroLevel::expand_all(const DevFab *src) {
+void HydroLevel::expand_all(const DevFab *src, bool cleaned) {
+    if (cleaned) {
+        launch_all(kb_expand_cleaned, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_,
+                   dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_);
+        return;
+    }
     launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_, dom_,
                extv_, ext_ ? 1 : 0, status_);
     launch_all(kb_detect, 0, (const DevFab *)tab_sb_, hc_.small_dens, status_, ebf_);
@@ -2336,7 +2342,7 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     set_dt_dev(dt);
     const DevFab *told = a_is_new_ ? tab_b_ : tab_a_;
     clean_all(told, true);
-    expand_all(told);
+    expand_all(told, true);
     const int64_t vn = vgather_points();
     if (vn)
         hipLaunchKernelGGL(kb_vgather, dim3(grid_for(vn), (unsigned)vbox_.size()), dim3(kBlock), 0,

@@ -88,6 +88,12 @@ __global__ void kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_t
 // Its adjoint: each point of grow(v, g) of vt's sb (here dL/dSborder of the
 // virtual box) added into sb_tab's array of the box kb_vgather read it from.
 __global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
+// expand_state of every box of dst (grow g) from the cleaned states of src:
+// kb_fill_raw, the detection, normalize_species, kb_resync and kb_resetT
+// in one pass (src must have been through clean_state).
+__global__ void kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner,
+                                  DomInfo D, Arr ext, int has_ext, HydroConst hc, int *status,
+                                  EbFlags eb);
 // The staged step's pointwise tail after the hydro source in one pass
 // (S_new = Sborder + sources + dt dS/dt and the clean_state passes), and
 // its pass-by-pass repeat where a density floor would act (one workgroup of

@@ -531,6 +531,50 @@ kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
     for (int n = 0; n < NVAR; ++n) s(i, j, k, n) += V.sb(i, j, k, n);
 }
 
+// ------------------------------------- expand_state after clean_state(S_old)
+// FillPatch of every point of grow(b, g) from the level's cleaned S_old
+// (pt_fill_raw), the floor detection and normalize_species of the valid
+// cells, FillBoundary (pt_resync: interior and periodic ghost cells take
+// their owner's normalized valid cell) and computeTemp (pt_resetT) on one
+// point in registers.  Exact when src was cleaned: enforce_min_density
+// leaves no cell with 0 < rho < small_dens (a fixed cell takes a
+// neighbour's state of density >= small_dens, or the small state), so the
+// floor pass between the detection and normalize_species finds nothing.
+__global__ void __launch_bounds__(kBlock)
+kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D,
+                  Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb) {
+    int b, i, j, k;
+    if (!batch_cell(dst, g, b, i, j, k)) return;
+    const Arr S = dst[b].a;
+    int m[3];
+    unsigned odd;
+    const bool phys = map_cell(D, i, j, k, m, &odd);
+    const int ob = owner_of(D, owner, m);
+    double v[NVAR];
+    if (ob < 0 && !has_ext) {
+        // pt_fill_raw leaves the point as it was; computeTemp still acts on it
+        atomicOr(status, 1 << CAMR_ERR_SHAPE);
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) v[n] = S(i, j, k, n);
+    } else {
+        const Arr s = ob >= 0 ? src[ob].a : ext;
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) {
+            const double x = s(m[0], m[1], m[2], n);
+            v[n] = ((odd >> n) & 1u) ? -x : x;
+        }
+        if (dst[b].b.contains(i, j, k)) {
+            if (!eb.covered(i, j, k)) floor_detect(v[URHO], hc.small_dens, status);
+            normalize_species(v);
+        } else if (!phys && ob >= 0) {
+            normalize_species(v);
+        }
+    }
+    reset_e_and_T(v, hc);
+#pragma unroll
+    for (int n = 0; n < NVAR; ++n) S(i, j, k, n) = v[n];
+}
+
 // ------------------------------------------- the staged step's tail fused
 // S_new = Sborder + old source (CAMR_advance.cpp:112-118), += dt dS/dt
 // (:141), clean_state (:146), the new-time source and its clean_state
