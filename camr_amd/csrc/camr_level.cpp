@@ -250,22 +250,25 @@ class HydroLevel {
     bool eb_batched_ = false;
     // regular boxes of equal shape that tile a box in blocks of up to
     // 4x4x4 run in that batch as the one virtual box they tile (fewer ghost
-    // cells): its Sborder gathered from theirs (kb_vgather over vg_dev_), its
-    // dS/dt one array that their rows of tab_dsdt_ view.  tab_sbx_ /
-    // tab_dsdtx_: the level's rows, then one per virtual box
+    // cells): its Sborder over grow(v, 6) and its dS/dt are one array each,
+    // which their rows of tab_sb_ / tab_dsdt_ view; expand_state fills the
+    // virtual box's Sborder as one row of tab_exp_ (exp_box_: the rows'
+    // boxes).  tab_sbx_ / tab_dsdtx_: the level's rows, then one per
+    // virtual box
     bool eb_merge_ = true;
     std::vector<Box> vbox_;
     int nmerged_ = 0;
-    VGather *vg_dev_ = nullptr;
     int *vg_ids_ = nullptr;
     std::vector<int> vof_;       // virtual box of each level box, or -1
-    std::vector<VGather> vgh_;   // host copy of vg_dev_
+    std::vector<VGather> vgh_;   // each virtual box's block and Sborder
     std::vector<Arr> vds_;       // dS/dt of each virtual box
     VGather *vga_dev_ = nullptr; // the adjoint's: sb = dL/dSborder of each virtual box
     DevFab *tab_sbx_ = nullptr, *tab_dsdtx_ = nullptr;
+    DevFab *tab_exp_ = nullptr;
+    std::vector<Box> exp_box_;
     int eb_regular_batch();
     // points of the largest grow(v, 6) (0: no virtual box)
-    int64_t vgather_points() const {
+    int64_t vbox_points() const {
         int64_t n = 0;
         for (const Box &V : vbox_) n = std::max<int64_t>(n, grow(V, ng_).npts());
         return n;
@@ -657,13 +660,14 @@ void HydroLevel::release() {
     bufA_.clear(); bufB_.clear(); sb_.clear(); dsdt_.clear(); srchyd_.clear(); dsv_.clear();
     vbox_.clear();
     nmerged_ = 0;
-    vg_dev_ = nullptr;  // freed with ebbuf_
-    vg_ids_ = nullptr;
+    vg_ids_ = nullptr;  // freed with ebbuf_
     vga_dev_ = nullptr;
     vof_.clear();
     vgh_.clear();
     vds_.clear();
     tab_sbx_ = tab_dsdtx_ = nullptr;
+    tab_exp_ = nullptr;
+    exp_box_.clear();
     fr(tab_old_); fr(tab_sb_); fr(tab_src_); fr(tab_a_); fr(tab_b_); fr(tab_dsdt_);
     tab_dsdt_ = nullptr;
     tab_a_ = tab_b_ = nullptr; fr(owner_); fr(status_); fr(redo_); fr(dtmin_);
@@ -1152,6 +1156,9 @@ int HydroLevel::eb_regular_batch() {
         vgh_ = vg;
         vds_ = vds;
         for (int b = 0; b < nb; ++b) nmerged_ += vof[b] >= 0;
+        // the boxes of a virtual box read and write its Sborder
+        for (int b = 0; b < nb; ++b)
+            if (vof[b] >= 0) sbv_[b] = vg[vof[b]].sb;
         std::vector<DevFab> ts(nb + nv), td(nb + nv);
         for (int b = 0; b < nb; ++b) {
             ts[b].a = sbv_[b]; ts[b].b = box_[b];
@@ -1168,10 +1175,16 @@ int HydroLevel::eb_regular_batch() {
         CAMR_HIP(hipMemcpy(tab_sbx_, ts.data(), ts.size() * sizeof(DevFab), hipMemcpyHostToDevice));
         CAMR_HIP(hipMemcpy(tab_dsdtx_, td.data(), td.size() * sizeof(DevFab), hipMemcpyHostToDevice));
         CAMR_HIP(hipMemcpy(tab_dsdt_, td.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
+        CAMR_HIP(hipMemcpy(tab_sb_, ts.data(), nb * sizeof(DevFab), hipMemcpyHostToDevice));
         if (nv) {
-            CAMR_HIP(hipMalloc(&vg_dev_, nv * sizeof(VGather)));
-            ebbuf_.push_back(reinterpret_cast<double *>(vg_dev_));
-            CAMR_HIP(hipMemcpy(vg_dev_, vg.data(), nv * sizeof(VGather), hipMemcpyHostToDevice));
+            // expand_state's rows: the boxes left alone, then the virtual boxes
+            std::vector<DevFab> te;
+            for (int b = 0; b < nb; ++b)
+                if (vof[b] < 0) { te.push_back(ts[b]); exp_box_.push_back(box_[b]); }
+            for (int v = 0; v < nv; ++v) { te.push_back(ts[nb + v]); exp_box_.push_back(vb[v]); }
+            CAMR_HIP(hipMalloc(&tab_exp_, te.size() * sizeof(DevFab)));
+            ebbuf_.push_back(reinterpret_cast<double *>(tab_exp_));
+            CAMR_HIP(hipMemcpy(tab_exp_, te.data(), te.size() * sizeof(DevFab), hipMemcpyHostToDevice));
             CAMR_HIP(hipMalloc(&vg_ids_, vids.size() * sizeof(int)));
             ebbuf_.push_back(reinterpret_cast<double *>(vg_ids_));
             CAMR_HIP(hipMemcpy(vg_ids_, vids.data(), vids.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -1475,12 +1488,6 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             const hipStream_t es = in_ws_ && nws_ ? wst_[0] : stream_;
             const int has_src = grav_src_values_ ? 1 : 0;
             const DevFab *src = grav_src_values_ ? (const DevFab *)tab_src_ : nullptr;
-            if (const int64_t n = vgather_points()) {
-                Region reg(&prof_, "vgather", es);
-                hipLaunchKernelGGL(kb_vgather, dim3(grid_for(n), (unsigned)vbox_.size()),
-                                   dim3(kBlock), 0, es, (const VGather *)vg_dev_,
-                                   (const int *)vg_ids_, (const DevFab *)tab_sb_, ng_);
-            }
             int rc = reg_engine_.hydro(-1, parity(), false, tab_sbx_, src, has_src, hc_, dt_dev_,
                                        es, status_, &prof_, nullptr, true);
             if (!rc) rc = reg_engine_.dsdt(-1, parity(), tab_dsdtx_, hc_, es, &prof_);
@@ -1706,6 +1713,15 @@ int HydroLevel::advance_staged(double dt) {
 // clean_state(Sborder) on the valid cells, FillBoundary, computeTemp on
 // the grown boxes.
 void HydroLevel::expand_all(const DevFab *src, bool cleaned) {
+    if (cleaned && tab_exp_) {
+        // virtual boxes as one row each
+        int64_t n = 1;
+        for (const Box &B : exp_box_) n = std::max<int64_t>(n, grow(B, ng_).npts());
+        hipLaunchKernelGGL(kb_expand_cleaned, dim3(grid_for(n), (unsigned)exp_box_.size()),
+                           dim3(kBlock), 0, stream_, (const DevFab *)tab_exp_, ng_, src,
+                           (const int *)owner_, dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_);
+        return;
+    }
     if (cleaned) {
         launch_all(kb_expand_cleaned, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_,
                    dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_);
@@ -2343,11 +2359,7 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     const DevFab *told = a_is_new_ ? tab_b_ : tab_a_;
     clean_all(told, true);
     expand_all(told, true);
-    const int64_t vn = vgather_points();
-    if (vn)
-        hipLaunchKernelGGL(kb_vgather, dim3(grid_for(vn), (unsigned)vbox_.size()), dim3(kBlock), 0,
-                           stream_, (const VGather *)vg_dev_, (const int *)vg_ids_,
-                           (const DevFab *)tab_sb_, ng_);
+    const int64_t vn = vbox_points();
     for (size_t b = 0; b < box_.size(); ++b)  // covered boxes: no hydro, dS/dt = 0
         if (kind_[b] == 2)
             launch_on(k_fill_value, box_[b], stream_, dsv_[b], (int)NVAR, 0.0);

@@ -73,20 +73,20 @@ __global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFa
 __global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp,
                              double small_dens, int *flags, EbFlags eb);
 // A virtual box: s[0] x s[1] x s[2] boxes of a level, each len[0] x len[1]
-// x len[2] cells, that tile the box v, and the Sborder of v over grow(v, g)
-// gathered from theirs.  ids[off + (c s[1] + r) s[0] + a]: the row in the
-// level's Sborder table of the box at block position (a, r, c).
+// x len[2] cells, that tile the box v, and an array over grow(v, g) (its
+// Sborder, or the adjoint of it).  ids[off + (c s[1] + r) s[0] + a]: the
+// level index of the box at block position (a, r, c).  The Sborder of v
+// holds at each point what the Sborder of the box holding the point
+// clamped to v holds there (a point beyond v lies in grow(that box, 6)).
 struct VGather {
     Box v;
     Arr sb;
     int len[3], s[3];
     int off;
 };
-// Every point of grow(v, g) of each virtual box of vt (grid.y = virtual
-// box) from the Sborder of the box that holds the point clamped to v.
-__global__ void kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
-// Its adjoint: each point of grow(v, g) of vt's sb (here dL/dSborder of the
-// virtual box) added into sb_tab's array of the box kb_vgather read it from.
+// dL/dSborder of each virtual box of vt (grid.y = virtual box): every
+// point of grow(v, g) added into sb_tab's array of the box that holds the
+// point clamped to v.
 __global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
 // expand_state of every box of dst (grow g) from the cleaned states of src:
 // kb_fill_raw, the detection, normalize_species, kb_resync and kb_resetT

@@ -493,25 +493,6 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
         pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
-// A point of grow(v, g) beyond v lies in grow(b, g) of the box b that holds
-// its clamp to v: the ghost cells of b there hold what v's would.
-__global__ void __launch_bounds__(kBlock)
-kb_vgather(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
-    const VGather &V = vt[blockIdx.y];
-    int i, j, k;
-    if (!range_point(range_of(grow(V.v, g)), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i,
-                     j, k))
-        return;
-    const int ci = min(max(i, V.v.lo[0]), V.v.hi[0]);
-    const int cj = min(max(j, V.v.lo[1]), V.v.hi[1]);
-    const int ck = min(max(k, V.v.lo[2]), V.v.hi[2]);
-    const int a = (ci - V.v.lo[0]) / V.len[0], r = (cj - V.v.lo[1]) / V.len[1],
-              c = (ck - V.v.lo[2]) / V.len[2];
-    const Arr s = sb_tab[ids[V.off + (c * V.s[1] + r) * V.s[0] + a]].a;
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) V.sb(i, j, k, n) = s(i, j, k, n);
-}
-
 // Every point of grow(b, g) is the image of at most one point of the
 // virtual box (the clamp owner is unique), so the adds do not collide.
 __global__ void __launch_bounds__(kBlock)
