@@ -88,6 +88,47 @@ __host__ __device__ inline Range range_of(const Box &b) {
     r.n = b.npts();
     return r;
 }
+// The blocks of one launch over a table of boxes of uneven sizes, in one
+// dimension: entry u owns blocks [off[u], off[u + 1]).  n = 0: a 2-D grid,
+// blockIdx.y the entry and blockIdx.x its block (for longer tables).
+constexpr int kFlatEntries = 64;
+struct FlatBlocks {
+    int n;
+    int off[kFlatEntries + 1];
+};
+// entry u and block lb within it of this block
+__device__ __forceinline__ void flat_entry(const FlatBlocks &f, int &u, int &lb) {
+    if (f.n == 0) {
+        u = blockIdx.y;
+        lb = blockIdx.x;
+        return;
+    }
+    const int b = blockIdx.x;
+    int lo = 0, hi = f.n;  // the last u with off[u] <= b (entries of no block skipped)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (f.off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    u = lo;
+    lb = b - f.off[lo];
+}
+// Host: the flattened blocks of n entries with cnt(u) blocks each, or a 2-D
+// grid of the largest count when n exceeds kFlatEntries; returns the grid.
+template <typename C>
+inline dim3 flat_blocks(int n, C cnt, FlatBlocks &f) {
+    if (n <= kFlatEntries) {
+        f.n = n;
+        f.off[0] = 0;
+        for (int u = 0; u < n; ++u) f.off[u + 1] = f.off[u] + (int)cnt(u);
+        return dim3((unsigned)f.off[n], 1);
+    }
+    f.n = 0;
+    int64_t m = 0;
+    for (int u = 0; u < n; ++u) m = m > cnt(u) ? m : cnt(u);
+    return dim3((unsigned)m, (unsigned)n);
+}
+
 __device__ __forceinline__ bool range_point(const Range &r, int64_t t, int &i,
                                             int &j, int &k) {
     if (t >= r.n) return false;

@@ -363,6 +363,10 @@ class HydroLevel {
     EbAdjEnt *eadj_tab_ = nullptr;
     double *eadj_fbuf_ = nullptr, *eadj_abuf_ = nullptr;  // forward / adjoint arrays
     size_t eadj_an_ = 0;                                   // doubles of eadj_abuf_
+    // the parts of eadj_abuf_ cleared before each adjoint step: all but the
+    // pair and final face-state adjoints, which their Riemann stages set
+    // (range starts and lengths in doubles)
+    std::vector<size_t> eadj_zs_, eadj_zl_;
     DevFab *tab_ub_ = nullptr, *tab_sbb_ = nullptr;
     EbGeom *geom_dev_ = nullptr;
     int *kinds_dev_ = nullptr;
@@ -1714,17 +1718,21 @@ int HydroLevel::advance_staged(double dt) {
 // the grown boxes.
 void HydroLevel::expand_all(const DevFab *src, bool cleaned) {
     if (cleaned && tab_exp_) {
-        // virtual boxes as one row each
-        int64_t n = 1;
-        for (const Box &B : exp_box_) n = std::max<int64_t>(n, grow(B, ng_).npts());
-        hipLaunchKernelGGL(kb_expand_cleaned, dim3(grid_for(n), (unsigned)exp_box_.size()),
-                           dim3(kBlock), 0, stream_, (const DevFab *)tab_exp_, ng_, src,
-                           (const int *)owner_, dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_);
+        // virtual boxes as one row each, the rows' blocks in one dimension
+        FlatBlocks fb;
+        const dim3 grid = flat_blocks(
+            (int)exp_box_.size(),
+            [&](int r) -> int64_t { return grid_for(grow(exp_box_[r], ng_).npts()); }, fb);
+        hipLaunchKernelGGL(kb_expand_cleaned, grid, dim3(kBlock), 0, stream_,
+                           (const DevFab *)tab_exp_, ng_, src, (const int *)owner_, dom_, extv_,
+                           ext_ ? 1 : 0, hc_, status_, ebf_, fb);
         return;
     }
     if (cleaned) {
+        FlatBlocks fb;
+        fb.n = 0;  // boxes of the level: blockIdx.y
         launch_all(kb_expand_cleaned, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_,
-                   dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_);
+                   dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_, fb);
         return;
     }
     launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_, dom_,
@@ -2229,7 +2237,7 @@ int HydroLevel::eb_adjoint_setup() {
             for (int d = 0; d < 3; ++d) fn += r32((size_t)NVAR * nodes(B, d).npts());
             fn += r32(umdrv_eb_scratch_doubles(B, G, true));
             an += r32((size_t)(QVAR + NQAUX) * G.npts());
-            an += r32(umdrv_eb_scratch_doubles(B, G, true));
+            an += r32(umdrv_eb_scratch_doubles(B, G, false));
         }
         for (int b = 0; b < nb; ++b)
             an += r32((size_t)NVAR * box_[b].npts()) + r32((size_t)NVAR * grow(box_[b], ng_).npts());
@@ -2244,6 +2252,7 @@ int HydroLevel::eb_adjoint_setup() {
         CAMR_HIP(hipMalloc(&eadj_abuf_, an * sizeof(double)));
         ebbuf_.push_back(eadj_abuf_);
         eadj_an_ = an;
+
         const size_t n = eadj_ids_.size();
         CAMR_HIP(hipMalloc(&eadj_ftab_, std::max<size_t>(n, 1) * sizeof(EbBox)));
         ebbuf_.push_back(reinterpret_cast<double *>(eadj_ftab_));
@@ -2291,6 +2300,7 @@ int HydroLevel::eb_adjoint_setup() {
     }
     for (size_t b = 0; b < box_.size(); ++b)
         if (!vof_.empty() && vof_[b] >= 0) ubv[b] = vub[vof_[b]];
+    std::vector<std::pair<size_t, size_t>> skip;  // face-state buffers, in address order
     for (size_t c = 0; c < eadj_ids_.size(); ++c) {
         const int id = eadj_ids_[c], v = -1 - id;
         const Box B = id >= 0 ? box_[id] : vbox_[v], G = grow(B, ng_);
@@ -2312,6 +2322,7 @@ int HydroLevel::eb_adjoint_setup() {
             fp += r32((size_t)NVAR * nodes(B, d).npts());
         }
         A.g = id >= 0 ? geo[id] : unit_geom_;
+        A.regular = (id < 0 || kind_[id] == 0) ? 1 : 0;
         eb_box_setup(A, fp, &eadj_fh_[c], true);
         fp += r32(umdrv_eb_scratch_doubles(B, G, true));
         UmdrvEbArgs Ab = A;
@@ -2321,9 +2332,27 @@ int HydroLevel::eb_adjoint_setup() {
         ap += r32((size_t)(QVAR + NQAUX) * G.npts());
         Ab.uin = id >= 0 ? sbbv[id] : vga[v].sb;
         Ab.dsdt = id >= 0 ? ubv[id] : vub[v];
-        eb_box_setup(Ab, ap, &eadj_h_[c].A, true);
-        ap += r32(umdrv_eb_scratch_doubles(B, G, true));
+        // the adjoint side shares one buffer of face states among the pairs
+        // and one among the final solves (set, not accumulated)
+        eb_box_setup(Ab, ap, &eadj_h_[c].A, false);
+        ap += r32(umdrv_eb_scratch_doubles(B, G, false));
         eadj_h_[c].F = eadj_fh_[c];
+        const EbBox &Ad = eadj_h_[c].A;
+        const size_t x0 = (size_t)(Ad.qmT[0].p - eadj_abuf_);
+        const size_t x1 = (size_t)(Ad.qpF[0].p - eadj_abuf_) +
+                          (size_t)NS * layout_of(B, G).bxg1.npts();
+        skip.push_back({x0, x1});
+    }
+    {
+        // clear the complement of the face-state buffers
+        eadj_zs_.clear();
+        eadj_zl_.clear();
+        size_t at = 0;
+        for (const auto &x : skip) {
+            if (x.first > at) { eadj_zs_.push_back(at); eadj_zl_.push_back(x.first - at); }
+            at = x.second;
+        }
+        if (eadj_an_ > at) { eadj_zs_.push_back(at); eadj_zl_.push_back(eadj_an_ - at); }
     }
     std::vector<DevFab> tu(box_.size()), ts(box_.size());
     std::vector<int> kd(box_.size());
@@ -2368,17 +2397,22 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
     if (!rc) rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
     if (!rc) {
         const int out = 1 - acur_;
-        zero_doubles(eadj_abuf_, eadj_an_, stream_);
+        zero_ranges(eadj_abuf_, eadj_zs_, eadj_zl_, stream_);
         launch_all(kb_zero_state, 0, (const DevFab *)tab_abar_[out]);
         launch_all(kb_ebadj_post, 0, (const DevFab *)tab_sb_, (const DevFab *)tab_dsdt_,
                    (const DevFab *)tab_abar_[acur_], (const DevFab *)tab_ub_,
                    (const DevFab *)tab_sbb_, (const double *)dt_dev_, hc_,
                    desc_.add_grav_src, status_, ebf_);
         rc = eb_adjoint_batch(eadj_tab_, eadj_h_.data(), n, hc_, dt_dev_, stream_, &prof_);
-        if (vn)
-            hipLaunchKernelGGL(kb_vscatter, dim3(grid_for(vn), (unsigned)vbox_.size()), dim3(kBlock),
-                               0, stream_, (const VGather *)vga_dev_, (const int *)vg_ids_,
-                               (const DevFab *)tab_sbb_, ng_);
+        if (vn) {
+            FlatBlocks fb;
+            const dim3 grid = flat_blocks(
+                (int)vbox_.size(),
+                [&](int v) -> int64_t { return grid_for(grow(vbox_[v], ng_).npts()); }, fb);
+            hipLaunchKernelGGL(kb_vscatter, grid, dim3(kBlock), 0, stream_,
+                               (const VGather *)vga_dev_, (const int *)vg_ids_,
+                               (const DevFab *)tab_sbb_, ng_, fb);
+        }
         launch_all(kb_ebadj_fill, ng_, (const DevFab *)tab_sbb_, ng_, told,
                    (const DevFab *)tab_abar_[out], (const int *)owner_, dom_, hc_, status_);
         launch_all(kb_ebadj_old, 0, (const DevFab *)tab_abar_[out], told, hc_, status_);

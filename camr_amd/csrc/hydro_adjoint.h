@@ -1,6 +1,8 @@
 // hydro_adjoint.h -- adjoint (reverse mode) of the fused level step.
 #pragma once
 
+#include <vector>
+
 #include "hydro_fused.h"
 
 namespace camr {
@@ -24,5 +26,9 @@ struct AdjArgs {
 int adjoint_step_kernels(const FusedEngine &eng, int parity, const AdjArgs &a, hipStream_t st,
                          Prof *prof);
 void zero_doubles(double *p, size_t n, hipStream_t st);
+// zero the ranges [start[r], start[r] + len[r]) of p, in one launch for up
+// to 64 ranges
+void zero_ranges(double *p, const std::vector<size_t> &start, const std::vector<size_t> &len,
+                 hipStream_t st);
 
 } // namespace camr

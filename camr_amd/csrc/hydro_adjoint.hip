@@ -731,6 +731,49 @@ void go(const FusedEngine &eng, const char *name, Prof *prof, K kern, int kind, 
 
 } // namespace
 
+// Zero ranges in one launch: block b clears kZeroChunk doubles of range u,
+// where off[u] <= b < off[u + 1].
+constexpr int kZeroChunk = 4096;
+constexpr int kZeroRanges = 64;
+struct ZeroList {
+    int n;
+    int off[kZeroRanges + 1];
+    size_t start[kZeroRanges], len[kZeroRanges];
+};
+
+__global__ void __launch_bounds__(kBlock) k_zero_list(double *p, ZeroList z) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = z.n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (z.off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const size_t beg = (size_t)(b - z.off[lo]) * kZeroChunk;
+    const size_t end = beg + kZeroChunk < z.len[lo] ? beg + kZeroChunk : z.len[lo];
+    double *q = p + z.start[lo];
+    for (size_t t = beg + threadIdx.x; t < end; t += blockDim.x) q[t] = 0.0;
+}
+
+void zero_ranges(double *p, const std::vector<size_t> &start, const std::vector<size_t> &len,
+                 hipStream_t st) {
+    const size_t n = start.size();
+    if (n > (size_t)kZeroRanges) {
+        for (size_t r = 0; r < n; ++r) zero_doubles(p + start[r], len[r], st);
+        return;
+    }
+    ZeroList z;
+    z.n = (int)n;
+    z.off[0] = 0;
+    for (size_t r = 0; r < n; ++r) {
+        z.start[r] = start[r];
+        z.len[r] = len[r];
+        z.off[r + 1] = z.off[r] + (int)((len[r] + kZeroChunk - 1) / kZeroChunk);
+    }
+    if (z.off[n] == 0) return;
+    hipLaunchKernelGGL(k_zero_list, dim3((unsigned)z.off[n]), dim3(kBlock), 0, st, p, z);
+}
+
 void zero_doubles(double *p, size_t n, hipStream_t st) {
     if (n == 0) return;
     const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 65536);

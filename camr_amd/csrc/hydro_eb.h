@@ -70,6 +70,10 @@ struct UmdrvEbArgs {
     int has_src;
     Arr flx[3];
     EbGeom g;
+    // 1: every cell of the geometry is regular (constant rows), so the
+    // cut-cell divergence and the redistribution's source side have nothing
+    // to do and its receiving side is dsdt = divc + 0
+    int regular = 0;
 };
 
 // The index boxes of one box's stages.
@@ -126,6 +130,7 @@ struct EbBox {
     // them back (`keep`)
     Arr qmT[6], qpT[6], qmF[3], qpF[3];
     Arr divu, divc, wt, optmp0, delm, winv;
+    int regular, pad_;  // UmdrvEbArgs::regular
 };
 
 // Doubles of scratch one box needs (the one-box call's EbBox record

@@ -578,11 +578,19 @@ CAMR_HD void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
                           : D == 1 ? hc.dx[0] * hc.dx[2] : hc.dx[1] * hc.dx[0];
         b_adjust_eb<D>(i, j, k, B.FT[D], B.divu, B.uin, B.g.ap[D], area, hc);
     } else if constexpr (S == ES_DIV || S == ES_DIVCUT) {
+        if (S == ES_DIVCUT && B.regular) return;  // no cut cell
         b_eb_div<S == ES_DIVCUT>(i, j, k, B.bx, B.FT[0], B.FT[1], B.FT[2], B.QE[0], B.QE[1], B.QE[2], B.g, B.q,
                  B.qaux, B.divc, B.wt, B.flx[0], B.flx[1], B.flx[2], hc, 1.0 / hc.dx[0],
                  1.0 / hc.dx[1], 1.0 / hc.dx[2]);
     } else if constexpr (S == ES_RSRC) {
+        // a regular box sends nothing (the receiving side below does not
+        // read optmp0 / delm / winv there)
+        if (B.regular) return;
         b_redist_src(i, j, k, B.g, B.dbox2, B.divc, B.wt, B.optmp0, B.delm, B.winv);
+    } else if (B.regular) {
+        // b_redist_dst with optmp0 = 0 and no sending neighbour
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) B.dsdt(i, j, k, n) = B.divc(i, j, k, n) + 0.0;
     } else {
         b_redist_dst(i, j, k, B.g, B.dbox2, B.divc, B.wt, B.optmp0, B.delm, B.winv, B.dsdt);
     }
@@ -590,13 +598,17 @@ CAMR_HD void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
 
 template <int S>
 __global__ void __launch_bounds__(kBlock)
-k_eb_site(const EbBox *tab, HydroConst hc, const double *dtp, int has_src) {
-    const EbBox &B = tab[blockIdx.y];
+k_eb_site(const EbBox *tab, HydroConst hc, const double *dtp, int has_src, FlatBlocks fb) {
+    int u, lb;
+    flat_entry(fb, u, lb);
+    const EbBox &B = tab[u];
+    if constexpr (S == ES_DIVCUT || S == ES_RSRC)
+        if (B.regular) return;
     const Layout L = layout_of(B.bx, B.qbox);
     const Box sb = site_box<S>(B, L);
     if (!sb.ok()) return;
     int i, j, k;
-    if (!range_point(range_of(sb), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+    if (!range_point(range_of(sb), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
         return;
     site_body<S>(B, L, i, j, k, hc, dtp, has_src);
 }
@@ -615,15 +627,17 @@ constexpr const char *site_name(int s) {
 template <int S>
 void launch_site(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
                  const double *dtp, int has_src, hipStream_t st, Prof *prof) {
-    int64_t nblk = 0;
-    for (int b = 0; b < n; ++b) {
+    // blocks per box: none where the stage has no point or nothing to do
+    auto cnt = [&](int b) -> int64_t {
+        if ((S == ES_DIVCUT || S == ES_RSRC) && htab[b].regular) return 0;
         const Box sb = site_box<S>(htab[b], layout_of(htab[b].bx, htab[b].qbox));
-        if (sb.ok()) nblk = std::max<int64_t>(nblk, (sb.npts() + kBlock - 1) / kBlock);
-    }
-    if (nblk == 0) return;
+        return sb.ok() ? (sb.npts() + kBlock - 1) / kBlock : 0;
+    };
+    FlatBlocks fb;
+    const dim3 grid = flat_blocks(n, cnt, fb);
+    if (grid.x == 0) return;
     Region reg(prof, site_name(S), st);
-    hipLaunchKernelGGL(k_eb_site<S>, dim3((unsigned)nblk, (unsigned)n), dim3(kBlock), 0, st, tab,
-                       hc, dtp, has_src);
+    hipLaunchKernelGGL(k_eb_site<S>, grid, dim3(kBlock), 0, st, tab, hc, dtp, has_src, fb);
 }
 
 template <int... S>
@@ -672,6 +686,7 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep) 
     B.bx = A.bx;
     B.qbox = A.qbox;
     B.g = A.g;
+    B.regular = A.regular;
     B.q = A.q;
     B.qaux = A.qaux;
     B.srcq = A.srcq;
@@ -744,10 +759,12 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep) 
 // (CAMR_construct_hydro_source.cpp:108-120, the hyd_src setVal), every box
 // of the table in one launch.
 __global__ void __launch_bounds__(kBlock)
-k_eb_prim(const EbBox *tab, HydroConst hc, int *status) {
-    const EbBox &B = tab[blockIdx.y];
+k_eb_prim(const EbBox *tab, HydroConst hc, int *status, FlatBlocks fb) {
+    int u, lb;
+    flat_entry(fb, u, lb);
+    const EbBox &B = tab[u];
     int i, j, k;
-    if (!range_point(range_of(B.qbox), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+    if (!range_point(range_of(B.qbox), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
         return;
     pt_ctoprim(B.uin, B.q, B.qaux, hc, status, i, j, k);
     if (B.bx.contains(i, j, k))
@@ -757,11 +774,11 @@ k_eb_prim(const EbBox *tab, HydroConst hc, int *status) {
 int eb_prim_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc, int *status,
                   hipStream_t st, Prof *prof) {
     if (n <= 0) return CAMR_OK;
-    int64_t np = 0;
-    for (int b = 0; b < n; ++b) np = std::max<int64_t>(np, htab[b].qbox.npts());
+    FlatBlocks fb;
+    const dim3 grid =
+        flat_blocks(n, [&](int b) -> int64_t { return grid_for(htab[b].qbox.npts()); }, fb);
     Region reg(prof, "eb_ctoprim", st);
-    hipLaunchKernelGGL(k_eb_prim, dim3(grid_for(np), (unsigned)n), dim3(kBlock), 0, st, tab, hc,
-                       status);
+    hipLaunchKernelGGL(k_eb_prim, grid, dim3(kBlock), 0, st, tab, hc, status, fb);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 

@@ -75,9 +75,10 @@ CAMR_HD void face_aux_adj(const Arr &qab, int i, int j, int k,
 }
 
 // Riemann solve of face (i,j,k) in reverse: fb / gb the adjoints of its
-// compact fluxes and Godunov (u, p); qLb / qRb (face arrays) and the sound
-// speeds' adjoints accumulate.
-template <int D>
+// compact fluxes and Godunov (u, p); qLb / qRb (face arrays) accumulate,
+// or with ASSIGN are set (their hydro components: this face is their only
+// writer), and the sound speeds' adjoints accumulate.
+template <int D, bool ASSIGN = false>
 CAMR_HD void riemann_face_adj(int i, int j, int k, const Arr &qL, const Arr &qR,
                                                  const Arr &qaux, const Arr &qLb, const Arr &qRb,
                                                  const Arr &qab, const double *fb,
@@ -95,8 +96,13 @@ CAMR_HD void riemann_face_adj(int i, int j, int k, const Arr &qL, const Arr &qR,
     face_flux_bc_adj(D, idx, hc, ql, qr, a, fb, gb, qlb, qrb, ab);
 #pragma unroll
     for (int s = 0; s < NH; ++s) {
-        aadd(qLb, i, j, k, s, qlb[s]);
-        aadd(qRb, i, j, k, s, qrb[s]);
+        if constexpr (ASSIGN) {
+            qLb(i, j, k, s) = qlb[s];
+            qRb(i, j, k, s) = qrb[s];
+        } else {
+            aadd(qLb, i, j, k, s, qlb[s]);
+            aadd(qRb, i, j, k, s, qrb[s]);
+        }
     }
     face_aux_adj<D>(qab, i, j, k, ab);
 }
@@ -170,6 +176,14 @@ __host__ __device__ inline Box adj_box(const EbBox &B, const Layout &L) {
 CAMR_HD void adj_rdst(const EbBox &F, const EbBox &A, int i, int j, int k,
                                          double dt) {
     const EbGeom &g = F.g;
+    if (F.regular) {
+        // dsdt = divc + 0: no sender, and optmp0 carries nothing back
+        for (int n = 0; n < NH; ++n) {
+            const double db = dt * A.dsdt(i, j, k, n);
+            if (db != 0.0) A.divc(i, j, k, n) += db;
+        }
+        return;
+    }
     if (g.f.covered(i, j, k)) return;
     const double mask = F.dbox2.contains(i, j, k) ? 1.0 : 0.0;
     const double wtt = F.wt(i, j, k, 0);
@@ -206,6 +220,7 @@ CAMR_HD void adj_rdst(const EbBox &F, const EbBox &A, int i, int j, int k,
 
 // b_redist_src in reverse (single cells only; the others wrote zeros).
 CAMR_HD void adj_rsrc(const EbBox &F, const EbBox &A, int i, int j, int k) {
+    if (F.regular) return;
     const EbGeom &g = F.g;
     const int32_t w = g.f(i, j, k);
     if (eb_type(w) != kEbSingle) return;
@@ -432,12 +447,24 @@ CAMR_HD void adj_divu(const EbBox &F, const EbBox &A, int i, int j, int k,
 
 // ------------------------------------------------------ Riemann solves
 // b_riemann_eb<D, MODE> in reverse for MODE 1 (pair p) and 2 (final).
-template <int D>
+// ASSIGN: the face states' adjoints are set at every face of the stage (to
+// zero where the face is closed or carries no adjoint), so their arrays
+// need no clearing and can be shared by the pairs.
+template <int D, bool ASSIGN = false>
 CAMR_HD void adj_riemann(const Arr &qL, const Arr &qR, const Arr &qLb,
                                             const Arr &qRb, const Arr &qaux, const Arr &qab,
                                             const Arr &fb_arr, const Arr &gb_arr, const Arr &ap,
                                             int i, int j, int k, const HydroConst &hc) {
-    if (!(ap(i, j, k, 0) > 0.)) return;
+    auto none = [&]() {
+        if constexpr (ASSIGN) {
+#pragma unroll
+            for (int s = 0; s < NH; ++s) {
+                qLb(i, j, k, s) = 0.0;
+                qRb(i, j, k, s) = 0.0;
+            }
+        }
+    };
+    if (!(ap(i, j, k, 0) > 0.)) { none(); return; }
     double fb[NF] = {}, gb[NG];
     bool any = false;
 #pragma unroll
@@ -448,8 +475,8 @@ CAMR_HD void adj_riemann(const Arr &qL, const Arr &qR, const Arr &qLb,
     gb[GUN] = gb_arr(i, j, k, GUN);
     gb[GP] = gb_arr(i, j, k, GP);
     any |= gb[GUN] != 0.0 || gb[GP] != 0.0;
-    if (!any) return;
-    riemann_face_adj<D>(i, j, k, qL, qR, qaux, qLb, qRb, qab, fb, gb, hc);
+    if (!any) { none(); return; }
+    riemann_face_adj<D, ASSIGN>(i, j, k, qL, qR, qaux, qLb, qRb, qab, fb, gb, hc);
 }
 
 // b_riemann_eb<D, 0> in reverse: the first solve, whose domain-boundary
@@ -512,9 +539,12 @@ CAMR_HD void adj_transdo(const EbBox &F, const EbBox &A, int p, int i, int j,
     TransDiff t;
     trans_diff_one(fl_, fr, gl, gr, cdt, t);
     TransDiffBar tb{};
+    // faces of the pair's Riemann stage, which set their states' adjoints
+    const Box rf = grow_dir(nodes(layout_of(F.bx, F.qbox).bxg1, D), 3 - D - T, 1);
     // the plus state of face i (from qp(i)), the minus state of face i + e_D
     for (int side = 0; side < 2; ++side) {
         const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+        if (!rf.contains(i + oi, j + oj, k + ok)) continue;  // no adjoint there
         const Arr &qn = side ? F.QM[D] : F.QP[D];
         const Arr &qnb = side ? A.QM[D] : A.QP[D];
         const Arr &qob_a = side ? A.qmT[p] : A.qpT[p];
@@ -592,8 +622,11 @@ CAMR_HD void adj_transdd(const EbBox &F, const EbBox &A, int i, int j, int k,
     TransDiff t;
     trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, gamc, t);
     TransDiffBar tb{};
+    // faces of the final Riemann stage, which set their states' adjoints
+    const Box rf = grow_dir(grow_dir(nodes(layout_of(F.bx, F.qbox).btf, D), T0, 1), T1, 1);
     for (int side = 0; side < 2; ++side) {
         const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+        if (!rf.contains(i + oi, j + oj, k + ok)) continue;  // no adjoint there
         const Arr &qn = side ? F.QM[D] : F.QP[D];
         const Arr &qnb = side ? A.QM[D] : A.QP[D];
         const Arr &qob_a = side ? A.qmF[D] : A.qpF[D];
@@ -787,20 +820,20 @@ CAMR_HD void adj_body(const EbBox &F, const EbBox &A, int i, int j, int k,
                                          const HydroConst &hc, double dt) {
     if constexpr (S == AS_RDST) adj_rdst(F, A, i, j, k, dt);
     else if constexpr (S == AS_RSRC) adj_rsrc(F, A, i, j, k);
-    else if constexpr (S == AS_DIVCUT) adj_div<true>(F, A, i, j, k, hc);
+    else if constexpr (S == AS_DIVCUT) { if (!F.regular) adj_div<true>(F, A, i, j, k, hc); }
     else if constexpr (S == AS_DIV) adj_div<false>(F, A, i, j, k, hc);
     else if constexpr (S >= AS_ADJ && S < AS_ADJ + 3) adj_adjust<2 - (S - AS_ADJ)>(F, A, i, j, k, hc);
     else if constexpr (S == AS_DIVU) adj_divu(F, A, i, j, k, hc);
     else if constexpr (S >= AS_FINAL && S < AS_FINAL + 6) {
         constexpr int D = 2 - (S - AS_FINAL) / 2;
         if constexpr ((S - AS_FINAL) % 2 == 0)
-            adj_riemann<D>(F.qmF[D], F.qpF[D], A.qmF[D], A.qpF[D], F.qaux, A.qaux, A.FT[D],
+            adj_riemann<D, true>(F.qmF[D], F.qpF[D], A.qmF[D], A.qpF[D], F.qaux, A.qaux, A.FT[D],
                            A.QE[D], F.g.ap[D], i, j, k, hc);
         else adj_transdd<D>(F, A, i, j, k, dt, hc);
     } else if constexpr (S >= AS_PAIR && S < AS_PAIR + 12) {
         constexpr int p = 5 - (S - AS_PAIR) / 2, D = pair_d(p), T = pair_t(p);
         if constexpr ((S - AS_PAIR) % 2 == 0)
-            adj_riemann<D>(F.qmT[p], F.qpT[p], A.qmT[p], A.qpT[p], F.qaux, A.qaux, A.F2[D][T],
+            adj_riemann<D, true>(F.qmT[p], F.qpT[p], A.qmT[p], A.qpT[p], F.qaux, A.qaux, A.F2[D][T],
                            A.G2[D][T], F.g.ap[D], i, j, k, hc);
         else adj_transdo<D, T>(F, A, p, i, j, k, dt, hc);
     } else if constexpr (S >= AS_R1 && S < AS_R1 + 3) adj_riemann1<2 - (S - AS_R1)>(F, A, i, j, k, hc);
@@ -810,13 +843,17 @@ CAMR_HD void adj_body(const EbBox &F, const EbBox &A, int i, int j, int k,
 
 template <int S>
 __global__ void __launch_bounds__(kBlock)
-k_eb_adj_site(const EbAdjEnt *tab, HydroConst hc, const double *dtp) {
-    const EbAdjEnt &E = tab[blockIdx.y];
+k_eb_adj_site(const EbAdjEnt *tab, HydroConst hc, const double *dtp, FlatBlocks fb) {
+    int u, lb;
+    flat_entry(fb, u, lb);
+    const EbAdjEnt &E = tab[u];
+    if constexpr (S == AS_DIVCUT || S == AS_RSRC)
+        if (E.F.regular) return;
     const Layout L = layout_of(E.F.bx, E.F.qbox);
     const Box sb = adj_box<S>(E.F, L);
     if (!sb.ok()) return;
     int i, j, k;
-    if (!range_point(range_of(sb), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k))
+    if (!range_point(range_of(sb), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
         return;
     adj_body<S>(E.F, E.A, i, j, k, hc, *dtp);
 }
@@ -835,15 +872,16 @@ constexpr const char *adj_name(int s) {
 template <int S>
 void launch_adj(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
                 const double *dtp, hipStream_t st, Prof *prof) {
-    int64_t nblk = 0;
-    for (int b = 0; b < n; ++b) {
+    auto cnt = [&](int b) -> int64_t {
+        if ((S == AS_DIVCUT || S == AS_RSRC) && htab[b].F.regular) return 0;
         const Box sb = adj_box<S>(htab[b].F, layout_of(htab[b].F.bx, htab[b].F.qbox));
-        if (sb.ok()) nblk = std::max<int64_t>(nblk, (sb.npts() + kBlock - 1) / kBlock);
-    }
-    if (nblk == 0) return;
+        return sb.ok() ? (sb.npts() + kBlock - 1) / kBlock : 0;
+    };
+    FlatBlocks fb;
+    const dim3 grid = flat_blocks(n, cnt, fb);
+    if (grid.x == 0) return;
     Region reg(prof, adj_name(S), st);
-    hipLaunchKernelGGL(k_eb_adj_site<S>, dim3((unsigned)nblk, (unsigned)n), dim3(kBlock), 0, st,
-                       tab, hc, dtp);
+    hipLaunchKernelGGL(k_eb_adj_site<S>, grid, dim3(kBlock), 0, st, tab, hc, dtp, fb);
 }
 
 template <int... S>

@@ -87,13 +87,14 @@ struct VGather {
 // dL/dSborder of each virtual box of vt (grid.y = virtual box): every
 // point of grow(v, g) added into sb_tab's array of the box that holds the
 // point clamped to v.
-__global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g);
+__global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g,
+                            FlatBlocks fb);
 // expand_state of every box of dst (grow g) from the cleaned states of src:
 // kb_fill_raw, the detection, normalize_species, kb_resync and kb_resetT
 // in one pass (src must have been through clean_state).
 __global__ void kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner,
                                   DomInfo D, Arr ext, int has_ext, HydroConst hc, int *status,
-                                  EbFlags eb);
+                                  EbFlags eb, FlatBlocks fb);
 // The staged step's pointwise tail after the hydro source in one pass
 // (S_new = Sborder + sources + dt dS/dt and the clean_state passes), and
 // its pass-by-pass repeat where a density floor would act (one workgroup of

@@ -496,11 +496,12 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
 // Every point of grow(b, g) is the image of at most one point of the
 // virtual box (the clamp owner is unique), so the adds do not collide.
 __global__ void __launch_bounds__(kBlock)
-kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
-    const VGather &V = vt[blockIdx.y];
+kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g, FlatBlocks fb) {
+    int u, lb;
+    flat_entry(fb, u, lb);
+    const VGather &V = vt[u];
     int i, j, k;
-    if (!range_point(range_of(grow(V.v, g)), (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i,
-                     j, k))
+    if (!range_point(range_of(grow(V.v, g)), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
         return;
     const int ci = min(max(i, V.v.lo[0]), V.v.hi[0]);
     const int cj = min(max(j, V.v.lo[1]), V.v.hi[1]);
@@ -523,9 +524,11 @@ kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g) {
 // floor pass between the detection and normalize_species finds nothing.
 __global__ void __launch_bounds__(kBlock)
 kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D,
-                  Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb) {
-    int b, i, j, k;
-    if (!batch_cell(dst, g, b, i, j, k)) return;
+                  Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb, FlatBlocks fb) {
+    int b, lb, i, j, k;
+    flat_entry(fb, b, lb);
+    if (!range_point(range_of(grow(dst[b].b, g)), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
+        return;
     const Arr S = dst[b].a;
     int m[3];
     unsigned odd;

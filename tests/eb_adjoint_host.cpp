@@ -265,6 +265,7 @@ static int run_case(const Case &cs, int samples) {
     A.has_src = 0;
     for (int d = 0; d < 3; ++d) A.flx[d] = dense(flx[d].data(), nodes(bx, d));
     A.g = G.g;
+    A.regular = cs.plane ? 0 : 1;  // the shortcut of all-regular geometry
     EbBox F;
     eb_box_setup(A, scr.data(), &F, true);
     std::vector<double> W(NVAR * bx.npts(), 0.0);
@@ -300,7 +301,9 @@ static int run_case(const Case &cs, int samples) {
     std::vector<double> qb(QVAR * nq, 0.0), qab(NQAUX * nq, 0.0), sbb(NVAR * nq, 0.0);
     std::vector<double> ub(NVAR * bx.npts());
     for (size_t e = 0; e < ub.size(); ++e) ub[e] = W[e] / dt;
-    std::vector<double> ascr(umdrv_eb_scratch_doubles(bx, qbox, true), 0.0);
+    // the adjoint side shares the pairs' and the final solves' face-state
+    // buffers, as the level does
+    std::vector<double> ascr(umdrv_eb_scratch_doubles(bx, qbox, false), 0.0);
     UmdrvEbArgs Ab = A;
     Ab.q = dense(qb.data(), qbox);
     Ab.qaux = dense(qab.data(), qbox);
@@ -308,7 +311,7 @@ static int run_case(const Case &cs, int samples) {
     Ab.uin = dense(sbb.data(), qbox);
     Ab.dsdt = dense(ub.data(), bx);
     EbBox Ad;
-    eb_box_setup(Ab, ascr.data(), &Ad, true);
+    eb_box_setup(Ab, ascr.data(), &Ad, false);
     adj_all(std::make_integer_sequence<int, AS_COUNT>{}, F, Ad, hc, dt);
     const Arr Gv = dense(sbb.data(), qbox);
     // finite differences at sampled Sborder entries
