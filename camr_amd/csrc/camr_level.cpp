@@ -2411,7 +2411,7 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
                 [&](int v) -> int64_t { return grid_for(grow(vbox_[v], ng_).npts()); }, fb);
             hipLaunchKernelGGL(kb_vscatter, grid, dim3(kBlock), 0, stream_,
                                (const VGather *)vga_dev_, (const int *)vg_ids_,
-                               (const DevFab *)tab_sbb_, ng_, fb);
+                               (const DevFab *)tab_sbb_, ng_, kEbAdjComps, fb);
         }
         launch_all(kb_ebadj_fill, ng_, (const DevFab *)tab_sbb_, ng_, told,
                    (const DevFab *)tab_abar_[out], (const int *)owner_, dom_, hc_, status_);

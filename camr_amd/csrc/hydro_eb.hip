@@ -543,7 +543,8 @@ CAMR_HD void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
             if (L.qe[d].contains(i, j, k)) b_zero(i, j, k, B.QE[d], (int)NG);
         }
         if (L.nb.contains(i, j, k)) b_zero(i, j, k, B.divu, 1);
-        if (L.btf.contains(i, j, k)) {
+        // (a regular box's divergence stage sets every divc and wt)
+        if (!B.regular && L.btf.contains(i, j, k)) {
             b_zero(i, j, k, B.divc, (int)NVAR);
             b_zero(i, j, k, B.wt, 1);
         }

@@ -8,6 +8,10 @@
 
 namespace camr {
 
+// The components of the state whose adjoints the EB kernels write
+// (URHO..UEINT); the others stay zero.
+constexpr int kEbAdjComps = UEINT + 1;
+
 // One box of the adjoint batch: the forward arrays of the recomputed step
 // (every pair's and direction's states kept) and their adjoints in arrays
 // of the same layouts.  In A: q / qaux hold the adjoints of the primitives,

@@ -84,11 +84,11 @@ struct VGather {
     int len[3], s[3];
     int off;
 };
-// dL/dSborder of each virtual box of vt (grid.y = virtual box): every
-// point of grow(v, g) added into sb_tab's array of the box that holds the
-// point clamped to v.
+// dL/dSborder of each virtual box of vt: components 0..nc-1 of every point
+// of grow(v, g) added into sb_tab's array of the box that holds the point
+// clamped to v.
 __global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g,
-                            FlatBlocks fb);
+                            int nc, FlatBlocks fb);
 // expand_state of every box of dst (grow g) from the cleaned states of src:
 // kb_fill_raw, the detection, normalize_species, kb_resync and kb_resetT
 // in one pass (src must have been through clean_state).

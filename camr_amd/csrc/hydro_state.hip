@@ -496,7 +496,8 @@ kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double s
 // Every point of grow(b, g) is the image of at most one point of the
 // virtual box (the clamp owner is unique), so the adds do not collide.
 __global__ void __launch_bounds__(kBlock)
-kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g, FlatBlocks fb) {
+kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g, int nc,
+            FlatBlocks fb) {
     int u, lb;
     flat_entry(fb, u, lb);
     const VGather &V = vt[u];
@@ -509,8 +510,7 @@ kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g, Flat
     const int a = (ci - V.v.lo[0]) / V.len[0], r = (cj - V.v.lo[1]) / V.len[1],
               c = (ck - V.v.lo[2]) / V.len[2];
     const Arr s = sb_tab[ids[V.off + (c * V.s[1] + r) * V.s[0] + a]].a;
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) s(i, j, k, n) += V.sb(i, j, k, n);
+    for (int n = 0; n < nc; ++n) s(i, j, k, n) += V.sb(i, j, k, n);
 }
 
 // ------------------------------------- expand_state after clean_state(S_old)
