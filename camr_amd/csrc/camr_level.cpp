@@ -2323,6 +2323,7 @@ int HydroLevel::eb_adjoint_setup() {
         }
         A.g = id >= 0 ? geo[id] : unit_geom_;
         A.regular = (id < 0 || kind_[id] == 0) ? 1 : 0;
+        A.noflx = 1;  // the adjoint reads no output flux
         eb_box_setup(A, fp, &eadj_fh_[c], true);
         fp += r32(umdrv_eb_scratch_doubles(B, G, true));
         UmdrvEbArgs Ab = A;

@@ -74,6 +74,9 @@ struct UmdrvEbArgs {
     // cut-cell divergence and the redistribution's source side have nothing
     // to do and its receiving side is dsdt = divc + 0
     int regular = 0;
+    // 1: leave flx unwritten (a recompute that needs only dS/dt and the
+    // stages' arrays)
+    int noflx = 0;
 };
 
 // The index boxes of one box's stages.
@@ -130,7 +133,8 @@ struct EbBox {
     // them back (`keep`)
     Arr qmT[6], qpT[6], qmF[3], qpF[3];
     Arr divu, divc, wt, optmp0, delm, winv;
-    int regular, pad_;  // UmdrvEbArgs::regular
+    int regular;  // UmdrvEbArgs::regular
+    int noflx;    // 1: the face fluxes of bx are not copied into flx (a recompute)
 };
 
 // Doubles of scratch one box needs (the one-box call's EbBox record

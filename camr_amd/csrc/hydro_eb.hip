@@ -324,14 +324,15 @@ template <bool CUT>
 CAMR_HD void
 b_eb_div(int i, int j, int k, Box bx, Arr ft0, Arr ft1, Arr ft2, Arr ge0, Arr ge1, Arr ge2, EbGeom g,
          Arr q, Arr qaux, Arr divc, Arr wt, Arr fo0, Arr fo1, Arr fo2, HydroConst hc,
-         double dxi0, double dxi1, double dxi2) {
+         double dxi0, double dxi1, double dxi2, bool out = true) {
     const int32_t w = g.f(i, j, k);
     if (eb_type(w) == kEbCovered) return;
     // CUT: the cut cells only; otherwise the regular cells and the weight of
     // every cell (two launches: the cut-cell form's registers would hold
     // the regular cells' occupancy down)
     if (CUT != (eb_type(w) == kEbSingle)) return;
-    const bool valid = bx.contains(i, j, k);
+    // out: copy the face fluxes of the valid box into fo0..fo2
+    const bool valid = out && bx.contains(i, j, k);
     const bool xh = i == bx.hi[0], yh = j == bx.hi[1], zh = k == bx.hi[2];
     const double cvi = dxi0 * dxi1 * dxi2;
     if constexpr (!CUT) {
@@ -582,7 +583,7 @@ CAMR_HD void site_body(const EbBox &B, const Layout &L, int i, int j, int k,
         if (S == ES_DIVCUT && B.regular) return;  // no cut cell
         b_eb_div<S == ES_DIVCUT>(i, j, k, B.bx, B.FT[0], B.FT[1], B.FT[2], B.QE[0], B.QE[1], B.QE[2], B.g, B.q,
                  B.qaux, B.divc, B.wt, B.flx[0], B.flx[1], B.flx[2], hc, 1.0 / hc.dx[0],
-                 1.0 / hc.dx[1], 1.0 / hc.dx[2]);
+                 1.0 / hc.dx[1], 1.0 / hc.dx[2], !B.noflx);
     } else if constexpr (S == ES_RSRC) {
         // a regular box sends nothing (the receiving side below does not
         // read optmp0 / delm / winv there)
@@ -688,6 +689,7 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep) 
     B.qbox = A.qbox;
     B.g = A.g;
     B.regular = A.regular;
+    B.noflx = A.noflx;
     B.q = A.q;
     B.qaux = A.qaux;
     B.srcq = A.srcq;

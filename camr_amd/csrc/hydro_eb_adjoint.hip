@@ -518,6 +518,21 @@ CAMR_HD void adj_transdo(const EbBox &F, const EbBox &A, int p, int i, int j,
                                             int k, double dt, const HydroConst &hc) {
     if (F.g.f.covered(i, j, k)) return;
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
+    // faces of the pair's Riemann stage, which set their states' adjoints;
+    // without an adjoint on either of the cell's two states, nothing to do
+    const Box rf = grow_dir(nodes(layout_of(F.bx, F.qbox).bxg1, D), 3 - D - T, 1);
+    {
+        bool any = false;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+            if (!rf.contains(i + oi, j + oj, k + ok)) continue;
+            const Arr &qob_a = side ? A.qmT[p] : A.qpT[p];
+#pragma unroll
+            for (int s = 0; s < NH; ++s) any |= qob_a(i + oi, j + oj, k + ok, s) != 0.0;
+        }
+        if (!any) return;
+    }
     constexpr int tx = (T == 0), ty = (T == 1), tz = (T == 2);
     const double cdt = 1.0 / 3.0 * dt / hc.dx[T];
     const Arr &apD = F.g.ap[D], &apT = F.g.ap[T];
@@ -539,8 +554,6 @@ CAMR_HD void adj_transdo(const EbBox &F, const EbBox &A, int p, int i, int j,
     TransDiff t;
     trans_diff_one(fl_, fr, gl, gr, cdt, t);
     TransDiffBar tb{};
-    // faces of the pair's Riemann stage, which set their states' adjoints
-    const Box rf = grow_dir(nodes(layout_of(F.bx, F.qbox).bxg1, D), 3 - D - T, 1);
     // the plus state of face i (from qp(i)), the minus state of face i + e_D
     for (int side = 0; side < 2; ++side) {
         const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
@@ -595,6 +608,21 @@ CAMR_HD void adj_transdd(const EbBox &F, const EbBox &A, int i, int j, int k,
     constexpr int ex = (D == 0), ey = (D == 1), ez = (D == 2);
     constexpr int ax = (T0 == 0), ay = (T0 == 1), az = (T0 == 2);
     constexpr int bx = (T1 == 0), by = (T1 == 1), bz = (T1 == 2);
+    // faces of the final Riemann stage, which set their states' adjoints;
+    // without an adjoint on either of the cell's two states, nothing to do
+    const Box rf = grow_dir(grow_dir(nodes(layout_of(F.bx, F.qbox).btf, D), T0, 1), T1, 1);
+    {
+        bool any = false;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
+            if (!rf.contains(i + oi, j + oj, k + ok)) continue;
+            const Arr &qob_a = side ? A.qmF[D] : A.qpF[D];
+#pragma unroll
+            for (int s = 0; s < NH; ++s) any |= qob_a(i + oi, j + oj, k + ok, s) != 0.0;
+        }
+        if (!any) return;
+    }
     const double hdt = 0.5 * dt;
     const double ca = hdt / hc.dx[T0], cb = hdt / hc.dx[T1];
     const Arr &apD = F.g.ap[D], &apA = F.g.ap[T0], &apB = F.g.ap[T1];
@@ -622,8 +650,6 @@ CAMR_HD void adj_transdd(const EbBox &F, const EbBox &A, int i, int j, int k,
     TransDiff t;
     trans_diff_two(fal, far_, gal, gar, fbl, fbr, gbl, gbr, ca, cb, gamc, t);
     TransDiffBar tb{};
-    // faces of the final Riemann stage, which set their states' adjoints
-    const Box rf = grow_dir(grow_dir(nodes(layout_of(F.bx, F.qbox).btf, D), T0, 1), T1, 1);
     for (int side = 0; side < 2; ++side) {
         const int oi = side ? ex : 0, oj = side ? ey : 0, ok = side ? ez : 0;
         if (!rf.contains(i + oi, j + oj, k + ok)) continue;  // no adjoint there
@@ -724,6 +750,21 @@ CAMR_HD void adj_plm(const EbBox &F, const EbBox &A, int i, int j, int k,
                                         double dt, const HydroConst &hc) {
     const EbGeom &g = F.g;
     if (eb_type(g.f(i, j, k)) == kEbCovered) return;
+    {
+        // no face state of the cell carries an adjoint: nothing to do (the
+        // flattening below is needed only by the directions that do)
+        bool any = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            const bool lo = g.ap[d](i, j, k, 0) > 0.0, hi = g.ap[d](i + ex, j + ey, k + ez, 0) > 0.0;
+#pragma unroll
+            for (int s = 0; s < NH; ++s)
+                any |= (lo && A.QP[d](i, j, k, s) != 0.0) ||
+                       (hi && A.QM[d](i + ex, j + ey, k + ez, s) != 0.0);
+        }
+        if (!any) return;
+    }
     const Arr &q = F.q;
     const Box &qbox = F.qbox;
     double fd[3] = {1.0, 1.0, 1.0};
