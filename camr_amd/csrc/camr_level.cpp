@@ -291,6 +291,9 @@ class HydroLevel {
     DevFab *tab_a_ = nullptr, *tab_b_ = nullptr;  // raw-state tables of buffers A / B
     DevFab *tab_dsdt_ = nullptr;                  // staged engine: dS/dt of every box
     int advance_staged(double dt);
+    // the staged step leaves the CFL estimate of S_new in dtmin_ (its
+    // one-pass tail with post_timestep's clean)
+    bool staged_dt() const { return !desc_.hp.do_mol && !defer_post_; }
     // cleaned: src went through clean_all (kb_expand_cleaned, one pass)
     void expand_all(const DevFab *src, bool cleaned = false);
     int hydro_on_workers(double dt);
@@ -1588,10 +1591,10 @@ int HydroLevel::advance(double dt) {
         last_dt_ = dt;
     } else if (use_staged_graph() && !prof_.enabled()) {
         rc = replay_staged(dt);
-        dt_ready_ = false;
+        dt_ready_ = staged_dt();
     } else {
         rc = advance_exact(dt);
-        dt_ready_ = false;
+        dt_ready_ = !fused_ && staged_dt();
     }
     if (rc) return rc;
     CAMR_HIP(hipGetLastError());
@@ -1669,13 +1672,19 @@ int HydroLevel::advance_staged(double dt) {
         int rc = hydro_on_workers(dt);
         if (rc) return rc;
         const int post = defer_post_ ? 0 : 1;
+        // with post_timestep's clean the state is final: its CFL estimate
+        // (estTimeStep's minimum) comes out of the same pass
+        unsigned long long *dtm = post ? dtmin_ : nullptr;
+        if (dtm)
+            launch_on(k_fill_u64, make_box1(), stream_, dtmin_,
+                      (unsigned long long)0x7fefffffffffffffULL);
         launch_all(kb_tail_fused, 0, tnew, (const DevFab *)tab_sb_, told,
                    (const DevFab *)tab_dsdt_, (const double *)dt_dev_, desc_.const_grav,
-                   desc_.add_grav_src, post, hc_, redo_, ebf_);
+                   desc_.add_grav_src, post, hc_, redo_, ebf_, dtm, status_);
         hipLaunchKernelGGL(kb_tail_redo, dim3(1), dim3(1024), 0, stream_, tnew,
                            (const DevFab *)tab_sb_, told, (const DevFab *)tab_dsdt_,
                            (int)box_.size(), (const double *)dt_dev_, desc_.const_grav,
-                           desc_.add_grav_src, post, hc_, status_, redo_, ebf_);
+                           desc_.add_grav_src, post, hc_, status_, redo_, ebf_, dtm);
         return CAMR_OK;
     }
     // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)

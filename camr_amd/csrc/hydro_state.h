@@ -101,11 +101,15 @@ __global__ void kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, c
 // up to 1024 threads; a no-op unless kb_tail_fused set *redo).
 __global__ void kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold,
                               const DevFab *dsdt, const double *dtp, double g, int add_grav,
-                              int post, HydroConst hc, int *redo, EbFlags eb);
+                              int post, HydroConst hc, int *redo, EbFlags eb,
+                              unsigned long long *dtmin, int *status);
+// With dtmin: the CFL estimate of the new state into *dtmin (kb_estdt's),
+// set by kb_tail_fused (which expects DBL_MAX there) and, after a repeat,
+// by kb_tail_redo.
 __global__ void kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold,
                              const DevFab *dsdt, int nbox, const double *dtp, double g,
                              int add_grav, int post, HydroConst hc, int *flags, int *redo,
-                             EbFlags eb);
+                             EbFlags eb, unsigned long long *dtmin);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,

@@ -572,14 +572,11 @@ __device__ __forceinline__ bool below_floor(double rho, double small_dens) {
     return rho == 0. || rho < small_dens;
 }
 
-__global__ void __launch_bounds__(kBlock)
-kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
-              const double *dtp, double g, int add_grav, int post, HydroConst hc, int *redo,
-              EbFlags eb) {
-    int b, i, j, k;
-    if (!batch_cell(snew, 0, b, i, j, k)) return;
-    const Arr sn = snew[b].a, u = sb[b].a, so = sold[b].a, ds = dsdt[b].a;
-    const double dt = *dtp;
+__device__ __forceinline__ double tail_point(const Arr sn, const Arr u, const Arr so,
+                                             const Arr ds, double dt, double g, int add_grav,
+                                             int post, const HydroConst &hc, int *redo,
+                                             const EbFlags &eb, bool want_dt, int i, int j,
+                                             int k) {
     const bool cov = eb.covered(i, j, k);
     double v[NVAR];
 #pragma unroll
@@ -618,7 +615,27 @@ kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const De
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) sn(i, j, k, n) = v[n];
     if (fire) atomicOr(redo, 1);
+    // a cell no floor acts on ends as the pass-by-pass tail leaves it (the
+    // floors change only the cells they act on); kb_tail_redo estimates
+    // the others
+    return (want_dt && !cov && !fire) ? estdt_vals(v, hc.dx[0], hc.dx[1], hc.dx[2])
+                                      : 1.7976931348623157e308;
 }
+
+// With dtmin (post only), also the CFL estimate of the new state
+// (kb_estdt's minimum, exact: the minimum does not depend on the order).
+__global__ void __launch_bounds__(kBlock)
+kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
+              const double *dtp, double g, int add_grav, int post, HydroConst hc, int *redo,
+              EbFlags eb, unsigned long long *dtmin, int *status) {
+    int b, i, j, k;
+    const bool in = batch_cell(snew, 0, b, i, j, k);
+    double est = 1.7976931348623157e308;
+    if (in) est = tail_point(snew[b].a, sb[b].a, sold[b].a, dsdt[b].a, *dtp, g, add_grav, post,
+                             hc, redo, eb, dtmin != nullptr, i, j, k);
+    if (dtmin) block_min_commit(est, dtmin, status);
+}
+
 
 // The same tail pass by pass (floor detection and enforce_min_density
 // included), by one workgroup whose barriers order the passes; returns at
@@ -626,7 +643,7 @@ kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const De
 __global__ void __launch_bounds__(1024)
 kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
              int nbox, const double *dtp, double g, int add_grav, int post, HydroConst hc,
-             int *flags, int *redo, EbFlags eb) {
+             int *flags, int *redo, EbFlags eb, unsigned long long *dtmin) {
     if (!*redo) return;
     const double dt = *dtp;
     auto each = [&](auto f) {
@@ -667,6 +684,22 @@ kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold, const Dev
         });
         floor();
         each([&](int b, int i, int j, int k) { pt_clean_point(snew[b].a, hc, i, j, k); });
+    }
+    if (dtmin) {
+        // the CFL estimate of the repeated state, as kb_estdt
+        if (threadIdx.x == 0) *dtmin = (unsigned long long)0x7fefffffffffffffULL;
+        __syncthreads();
+        double m = 1.7976931348623157e308;
+        each([&](int b, int i, int j, int k) {
+            if (eb.covered(i, j, k)) return;
+            double v = estdt_point(snew[b].a, i, j, k, hc.dx[0], hc.dx[1], hc.dx[2]);
+            if (!(v > 0.0) && v == v) atomicOr(flags, 1 << CAMR_ERR_DT);
+            if (!(v == v)) v = 1.7976931348623157e308;
+            m = vmin(m, v);
+        });
+        for (int off = 32; off > 0; off >>= 1) m = vmin(m, __shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0 && m > 0.0)
+            atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
     }
     if (threadIdx.x == 0) *redo = 0;
 }

@@ -10,21 +10,32 @@ namespace camr {
 // CAMR_estdt_hydro (Source/Utils/Timestep.H:18-61) for one cell.  The min
 // over cells is order independent; non-negative doubles order like their
 // bit patterns, so block minima are combined with a 64-bit atomicMin.
-__device__ __forceinline__ double estdt_point(const Arr &u, int i, int j, int k,
-                                              double dx, double dy, double dz) {
-    const double rho = u(i, j, k, URHO);
+// the same on a register copy of the cell
+__device__ __forceinline__ double estdt_vals(const double *v, double dx, double dy, double dz) {
+    const double rho = v[URHO];
     const double rhoInv = 1.0 / rho;
-    const double eint = u(i, j, k, UEINT) / rho;
+    const double eint = v[UEINT] / rho;
     const double pres = (kGamma - 1.0) * rho * eint;
     const double c = sqrt(kGamma * pres / rho);
     double dt = 1.7976931348623157e308;
-    const double ux = u(i, j, k, UMX) * rhoInv;
+    const double ux = v[UMX] * rhoInv;
     dt = vmin(dt, dx / (c + fabs(ux)));
-    const double uy = u(i, j, k, UMY) * rhoInv;
+    const double uy = v[UMY] * rhoInv;
     dt = vmin(dt, dy / (c + fabs(uy)));
-    const double uz = u(i, j, k, UMZ) * rhoInv;
+    const double uz = v[UMZ] * rhoInv;
     dt = vmin(dt, dz / (c + fabs(uz)));
     return dt;
+}
+
+__device__ __forceinline__ double estdt_point(const Arr &u, int i, int j, int k,
+                                              double dx, double dy, double dz) {
+    double w[NVAR] = {};
+    w[URHO] = u(i, j, k, URHO);
+    w[UMX] = u(i, j, k, UMX);
+    w[UMY] = u(i, j, k, UMY);
+    w[UMZ] = u(i, j, k, UMZ);
+    w[UEINT] = u(i, j, k, UEINT);
+    return estdt_vals(w, dx, dy, dz);
 }
 
 __device__ __forceinline__ void block_min_commit(double v, unsigned long long *out,
