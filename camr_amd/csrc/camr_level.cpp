@@ -283,6 +283,8 @@ class HydroLevel {
     std::vector<EbBox> cut_htab_;
     EbBox *cut_tab_ = nullptr;  // device copy of cut_htab_
     double *cut_buf_ = nullptr;
+    CutCell *cut_cells_ = nullptr;  // the cut cells of grow(bx, 2) of the table's boxes
+    int ncut_cells_ = 0;
     int cut_entry(int c);
     bool pending_ = false;   // a fused step awaits resolve()
     bool needs_ext_ = false; // some ghost cell has no owning box
@@ -636,17 +638,21 @@ int HydroLevel::build_shell_cells(int ratio, const Box &cdom) {
     return CAMR_OK;
 }
 
+// The launch shapes of the level's fused kernels: the fused engine, or on
+// an EB level the batch of its regular boxes.
 int HydroLevel::set_tuning(const Tuning &t) {
-    if (!fused_) { set_error("tuning applies to the fused engine"); return CAMR_ERR_PARAM; }
+    FusedEngine *e = fused_ ? &engine_ : (eb_batched_ ? &reg_engine_ : nullptr);
+    if (!e) { set_error("tuning applies to the fused engine or an EB level's regular boxes"); return CAMR_ERR_PARAM; }
     int rc = resolve();
     if (rc) return rc;
     CAMR_HIP(hipStreamSynchronize(stream_));
     release_graphs();
-    return engine_.set_tuning(t);
+    return e->set_tuning(t);
 }
 int HydroLevel::get_tuning(Tuning *t) const {
-    if (!fused_) { set_error("tuning applies to the fused engine"); return CAMR_ERR_PARAM; }
-    *t = engine_.tuning();
+    const FusedEngine *e = fused_ ? &engine_ : (eb_batched_ ? &reg_engine_ : nullptr);
+    if (!e) { set_error("tuning applies to the fused engine or an EB level's regular boxes"); return CAMR_ERR_PARAM; }
+    *t = e->tuning();
     return CAMR_OK;
 }
 
@@ -999,6 +1005,8 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     // box's geometry arrives
     cut_ids_.clear();
     cut_off_.clear();
+    cut_cells_ = nullptr;  // (an earlier list is freed with ebbuf_)
+    ncut_cells_ = 0;
     size_t tot = 0;
     auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
     for (int b = 0; b < nb; ++b) {
@@ -1018,6 +1026,23 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
         cut_htab_.assign(cut_ids_.size(), EbBox{});
         for (size_t c = 0; c < cut_ids_.size(); ++c)
             if ((rc = cut_entry((int)c))) return rc;
+        // the cut cells, for the cut-cell divergence (the cells where
+        // b_eb_div<true> acts)
+        std::vector<CutCell> cells;
+        for (size_t c = 0; c < cut_ids_.size(); ++c) {
+            const Box G = grow(box_[cut_ids_[c]], 2);
+            for (int k = G.lo[2]; k <= G.hi[2]; ++k)
+                for (int j = G.lo[1]; j <= G.hi[1]; ++j)
+                    for (int i = G.lo[0]; i <= G.hi[0]; ++i)
+                        if (eb_type(hf(i, j, k)) == kEbSingle) cells.push_back({(int)c, i, j, k});
+        }
+        ncut_cells_ = (int)cells.size();
+        if (ncut_cells_) {
+            CAMR_HIP(hipMalloc(&cut_cells_, cells.size() * sizeof(CutCell)));
+            ebbuf_.push_back(reinterpret_cast<double *>(cut_cells_));
+            CAMR_HIP(hipMemcpy(cut_cells_, cells.data(), cells.size() * sizeof(CutCell),
+                               hipMemcpyHostToDevice));
+        }
     }
     wmap_.clear();
     if (eb_batched_ && nws_ > 1) {
@@ -1515,7 +1540,9 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             }
             const int nc = (int)cut_ids_.size();
             int rc = eb_prim_batch(cut_tab_, cut_htab_.data(), nc, hc_, status_, cs, &prof_);
-            if (!rc) rc = umdrv_eb_batch(cut_tab_, cut_htab_.data(), nc, hc_, dt_dev_, 0, cs, &prof_);
+            if (!rc)
+                rc = umdrv_eb_batch(cut_tab_, cut_htab_.data(), nc, hc_, dt_dev_, 0, cs, &prof_,
+                                    cut_cells_, ncut_cells_);
             if (rc) { set_error("kernel launch failed"); return rc; }
         }
         for (size_t b = 0; b < box_.size(); ++b) {

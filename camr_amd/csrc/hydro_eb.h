@@ -148,8 +148,16 @@ void eb_box_setup(const UmdrvEbArgs &A, double *scratch, EbBox *out, bool keep =
 // records in host memory, read for the grid sizes): one launch per stage,
 // every box of the table in the launch.  All boxes share hc, dtp and has_src
 // (srcq holds primitive sources).
+// A cut cell (eb_type kEbSingle) of grow(bx, 2) of entry e of a table.
+struct CutCell {
+    int e, i, j, k;
+};
+// cells (device, ncells): every cut cell of grow(bx, 2) of the table's
+// boxes; the cut-cell divergence then runs over them alone (one thread per
+// cut cell) instead of over grow(bx, 2).
 int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
-                   const double *dtp, int has_src, hipStream_t st, Prof *prof = nullptr);
+                   const double *dtp, int has_src, hipStream_t st, Prof *prof = nullptr,
+                   const CutCell *cells = nullptr, int ncells = 0);
 // hydro_ctoprim of every box of the table over its qbox (Sborder in uin) and
 // the zero fill of its dS/dt, in one launch.
 int eb_prim_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc, int *status,

@@ -626,9 +626,29 @@ constexpr const char *site_name(int s) {
          : s <= ES_DIVCUT ? "eb_divergence" : "eb_redistribute";
 }
 
+// The cut-cell divergence over a list of the cut cells.
+__global__ void __launch_bounds__(kBlock)
+k_eb_divcut_list(const EbBox *tab, const CutCell *cells, int n, HydroConst hc, const double *dtp,
+                 int has_src) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CutCell c = cells[t];
+    const EbBox &B = tab[c.e];
+    const Layout L = layout_of(B.bx, B.qbox);
+    site_body<ES_DIVCUT>(B, L, c.i, c.j, c.k, hc, dtp, has_src);
+}
+
 template <int S>
 void launch_site(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
-                 const double *dtp, int has_src, hipStream_t st, Prof *prof) {
+                 const double *dtp, int has_src, hipStream_t st, Prof *prof,
+                 const CutCell *cells, int ncells) {
+    if (S == ES_DIVCUT && cells) {
+        if (ncells == 0) return;
+        Region reg(prof, site_name(S), st);
+        hipLaunchKernelGGL(k_eb_divcut_list, dim3(grid_for(ncells)), dim3(kBlock), 0, st, tab,
+                           cells, ncells, hc, dtp, has_src);
+        return;
+    }
     // blocks per box: none where the stage has no point or nothing to do
     auto cnt = [&](int b) -> int64_t {
         if ((S == ES_DIVCUT || S == ES_RSRC) && htab[b].regular) return 0;
@@ -645,8 +665,8 @@ void launch_site(const EbBox *tab, const EbBox *htab, int n, const HydroConst &h
 template <int... S>
 void launch_sites(std::integer_sequence<int, S...>, const EbBox *tab, const EbBox *htab, int n,
                   const HydroConst &hc, const double *dtp, int has_src, hipStream_t st,
-                  Prof *prof) {
-    (launch_site<S>(tab, htab, n, hc, dtp, has_src, st, prof), ...);
+                  Prof *prof, const CutCell *cells, int ncells) {
+    (launch_site<S>(tab, htab, n, hc, dtp, has_src, st, prof, cells, ncells), ...);
 }
 
 // The EbBox of a one-box call, written into device memory by kernel
@@ -786,10 +806,11 @@ int eb_prim_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &
 }
 
 int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
-                   const double *dtp, int has_src, hipStream_t st, Prof *prof) {
+                   const double *dtp, int has_src, hipStream_t st, Prof *prof,
+                   const CutCell *cells, int ncells) {
     if (n <= 0) return CAMR_OK;
     launch_sites(std::make_integer_sequence<int, ES_COUNT>{}, tab, htab, n, hc, dtp, has_src, st,
-                 prof);
+                 prof, cells, ncells);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 
