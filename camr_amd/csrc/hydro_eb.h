@@ -153,8 +153,9 @@ struct CutCell {
     int e, i, j, k;
 };
 // cells (device, ncells): every cut cell of grow(bx, 2) of the table's
-// boxes; the cut-cell divergence then runs over them alone (one thread per
-// cut cell) instead of over grow(bx, 2).
+// boxes; the cut-cell divergence and the redistribution's source side then
+// run over them alone (one thread per cut cell) instead of over grow(bx, 2)
+// and grow(bx, 1).
 int umdrv_eb_batch(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
                    const double *dtp, int has_src, hipStream_t st, Prof *prof = nullptr,
                    const CutCell *cells = nullptr, int ncells = 0);

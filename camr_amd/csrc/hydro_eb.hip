@@ -472,9 +472,12 @@ b_redist_dst(int i, int j, int k, EbGeom g, Box dbox2, Arr divc, Arr wt, Arr opt
         if (eb_type(ws) == kEbSingle && eb_conn(ws, -oi, -oj, -ok))
             src |= 1u << ((oi + 1) + 3 * (oj + 1) + 9 * (ok + 1));
     }
+    // the cell's own share (zero unless it is a cut cell; b_redist_src
+    // writes that zero when it runs over every cell, not over a cell list)
+    const bool own = eb_type(g.f(i, j, k)) == kEbSingle;
 #pragma unroll 1
     for (int n = 0; n < NVAR; ++n) {
-        double acc = optmp0(i, j, k, n);
+        double acc = own ? optmp0(i, j, k, n) : 0.0;
         // in increasing bit order: k, then j, then i of the sender
         for (uint32_t m = src; m; m &= m - 1) {
             const int t = __builtin_ctz(m);
@@ -638,15 +641,31 @@ k_eb_divcut_list(const EbBox *tab, const CutCell *cells, int n, HydroConst hc, c
     site_body<ES_DIVCUT>(B, L, c.i, c.j, c.k, hc, dtp, has_src);
 }
 
+// The redistribution's source side over the cut cells of grow(bx, 1) in
+// the list (the other cells send nothing).
+__global__ void __launch_bounds__(kBlock)
+k_eb_rsrc_list(const EbBox *tab, const CutCell *cells, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CutCell c = cells[t];
+    const EbBox &B = tab[c.e];
+    if (!grow(B.bx, 1).contains(c.i, c.j, c.k)) return;
+    b_redist_src(c.i, c.j, c.k, B.g, B.dbox2, B.divc, B.wt, B.optmp0, B.delm, B.winv);
+}
+
 template <int S>
 void launch_site(const EbBox *tab, const EbBox *htab, int n, const HydroConst &hc,
                  const double *dtp, int has_src, hipStream_t st, Prof *prof,
                  const CutCell *cells, int ncells) {
-    if (S == ES_DIVCUT && cells) {
+    if ((S == ES_DIVCUT || S == ES_RSRC) && cells) {
         if (ncells == 0) return;
         Region reg(prof, site_name(S), st);
-        hipLaunchKernelGGL(k_eb_divcut_list, dim3(grid_for(ncells)), dim3(kBlock), 0, st, tab,
-                           cells, ncells, hc, dtp, has_src);
+        if (S == ES_DIVCUT)
+            hipLaunchKernelGGL(k_eb_divcut_list, dim3(grid_for(ncells)), dim3(kBlock), 0, st, tab,
+                               cells, ncells, hc, dtp, has_src);
+        else
+            hipLaunchKernelGGL(k_eb_rsrc_list, dim3(grid_for(ncells)), dim3(kBlock), 0, st, tab,
+                               cells, ncells);
         return;
     }
     // blocks per box: none where the stage has no point or nothing to do
