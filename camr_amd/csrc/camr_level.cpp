@@ -285,6 +285,9 @@ class HydroLevel {
     double *cut_buf_ = nullptr;
     CutCell *cut_cells_ = nullptr;  // the cut cells of grow(bx, 2) of the table's boxes
     int ncut_cells_ = 0;
+    std::vector<CutCell> cut_cells_h_;  // host copy (entries: cut-table positions)
+    CutCell *eadj_cells_ = nullptr;     // the same for the adjoint's table
+    int neadj_cells_ = 0;
     int cut_entry(int c);
     bool pending_ = false;   // a fused step awaits resolve()
     bool needs_ext_ = false; // some ghost cell has no owning box
@@ -1007,6 +1010,7 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
     cut_off_.clear();
     cut_cells_ = nullptr;  // (an earlier list is freed with ebbuf_)
     ncut_cells_ = 0;
+    cut_cells_h_.clear();
     size_t tot = 0;
     auto r32 = [](size_t n) { return (n + 31) / 32 * 32; };
     for (int b = 0; b < nb; ++b) {
@@ -1037,6 +1041,7 @@ int HydroLevel::set_eb(const int32_t *flags, const int32_t flo[3], const int32_t
                         if (eb_type(hf(i, j, k)) == kEbSingle) cells.push_back({(int)c, i, j, k});
         }
         ncut_cells_ = (int)cells.size();
+        cut_cells_h_ = cells;
         if (ncut_cells_) {
             CAMR_HIP(hipMalloc(&cut_cells_, cells.size() * sizeof(CutCell)));
             ebbuf_.push_back(reinterpret_cast<double *>(cut_cells_));
@@ -2407,6 +2412,19 @@ int HydroLevel::eb_adjoint_setup() {
     CAMR_HIP(hipMemcpy(tab_sbb_, ts.data(), ts.size() * sizeof(DevFab), hipMemcpyHostToDevice));
     if (!vga.empty())
         CAMR_HIP(hipMemcpy(vga_dev_, vga.data(), vga.size() * sizeof(VGather), hipMemcpyHostToDevice));
+    if (!eadj_cells_ && !cut_cells_h_.empty()) {
+        // the cut cells with the adjoint table's entries
+        std::vector<int> pos(box_.size(), -1);
+        for (size_t c = 0; c < eadj_ids_.size(); ++c)
+            if (eadj_ids_[c] >= 0) pos[eadj_ids_[c]] = (int)c;
+        std::vector<CutCell> cells = cut_cells_h_;
+        for (CutCell &x : cells) x.e = pos[cut_ids_[x.e]];
+        neadj_cells_ = (int)cells.size();
+        CAMR_HIP(hipMalloc(&eadj_cells_, cells.size() * sizeof(CutCell)));
+        ebbuf_.push_back(reinterpret_cast<double *>(eadj_cells_));
+        CAMR_HIP(hipMemcpy(eadj_cells_, cells.data(), cells.size() * sizeof(CutCell),
+                           hipMemcpyHostToDevice));
+    }
     CAMR_HIP(hipMemcpy(geom_dev_, geo.data(), geo.size() * sizeof(EbGeom), hipMemcpyHostToDevice));
     CAMR_HIP(hipMemcpy(kinds_dev_, kd.data(), kd.size() * sizeof(int), hipMemcpyHostToDevice));
     return CAMR_OK;
@@ -2431,7 +2449,9 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
             launch_on(k_fill_value, box_[b], stream_, dsv_[b], (int)NVAR, 0.0);
     const int n = (int)eadj_ids_.size();
     rc = eb_prim_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, status_, stream_, &prof_);
-    if (!rc) rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_);
+    if (!rc)
+        rc = umdrv_eb_batch(eadj_ftab_, eadj_fh_.data(), n, hc_, dt_dev_, 0, stream_, &prof_,
+                            eadj_cells_, neadj_cells_);
     if (!rc) {
         const int out = 1 - acur_;
         zero_ranges(eadj_abuf_, eadj_zs_, eadj_zl_, stream_);
@@ -2440,7 +2460,8 @@ int HydroLevel::adjoint_step_eb(double dt, bool check) {
                    (const DevFab *)tab_abar_[acur_], (const DevFab *)tab_ub_,
                    (const DevFab *)tab_sbb_, (const double *)dt_dev_, hc_,
                    desc_.add_grav_src, status_, ebf_);
-        rc = eb_adjoint_batch(eadj_tab_, eadj_h_.data(), n, hc_, dt_dev_, stream_, &prof_);
+        rc = eb_adjoint_batch(eadj_tab_, eadj_h_.data(), n, hc_, dt_dev_, stream_, &prof_,
+                              eadj_cells_, neadj_cells_);
         if (vn) {
             FlatBlocks fb;
             const dim3 grid = flat_blocks(

@@ -25,8 +25,12 @@ struct EbAdjEnt {
 // htab (host copy, for the grid sizes), from A.dsdt (dL/dS_new before the
 // cleaning) down to A.uin (dL/dSborder).  Every adjoint array must be zero
 // on entry except A.dsdt and the valid cells of A.uin.
+// cells (device, ncells): the cut cells of grow(bx, 2) of the table's boxes
+// (CutCell), over which the cut-cell divergence and the redistribution's
+// source side then run.
 int eb_adjoint_batch(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
-                     const double *dtp, hipStream_t st, Prof *prof = nullptr);
+                     const double *dtp, hipStream_t st, Prof *prof = nullptr,
+                     const CutCell *cells = nullptr, int ncells = 0);
 
 // Level part, over every box of the tables (grid.y = box):
 // the update S_new = Sborder + dt dS/dt, the clean_state passes after it

@@ -910,9 +910,32 @@ constexpr const char *adj_name(int s) {
          : s == AS_PLM ? "ebadj_trace_plm" : "ebadj_ctoprim";
 }
 
+// AS_DIVCUT / AS_RSRC over a list of the cut cells
+template <int S>
+__global__ void __launch_bounds__(kBlock)
+k_eb_adj_list(const EbAdjEnt *tab, const CutCell *cells, int n, HydroConst hc,
+              const double *dtp) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const CutCell c = cells[t];
+    const EbAdjEnt &E = tab[c.e];
+    if (S == AS_RSRC && !grow(E.F.bx, 1).contains(c.i, c.j, c.k)) return;
+    adj_body<S>(E.F, E.A, c.i, c.j, c.k, hc, *dtp);
+}
+
 template <int S>
 void launch_adj(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
-                const double *dtp, hipStream_t st, Prof *prof) {
+                const double *dtp, hipStream_t st, Prof *prof, const CutCell *cells,
+                int ncells) {
+    if constexpr (S == AS_DIVCUT || S == AS_RSRC) {
+        if (cells) {
+            if (ncells == 0) return;
+            Region reg(prof, adj_name(S), st);
+            hipLaunchKernelGGL(k_eb_adj_list<S>, dim3(grid_for(ncells)), dim3(kBlock), 0, st,
+                               tab, cells, ncells, hc, dtp);
+            return;
+        }
+    }
     auto cnt = [&](int b) -> int64_t {
         if ((S == AS_DIVCUT || S == AS_RSRC) && htab[b].F.regular) return 0;
         const Box sb = adj_box<S>(htab[b].F, layout_of(htab[b].F.bx, htab[b].F.qbox));
@@ -927,8 +950,9 @@ void launch_adj(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroCon
 
 template <int... S>
 void launch_adjs(std::integer_sequence<int, S...>, const EbAdjEnt *tab, const EbAdjEnt *htab,
-                 int n, const HydroConst &hc, const double *dtp, hipStream_t st, Prof *prof) {
-    (launch_adj<S>(tab, htab, n, hc, dtp, st, prof), ...);
+                 int n, const HydroConst &hc, const double *dtp, hipStream_t st, Prof *prof,
+                 const CutCell *cells, int ncells) {
+    (launch_adj<S>(tab, htab, n, hc, dtp, st, prof, cells, ncells), ...);
 }
 
 __device__ __forceinline__ bool batch_pt(const DevFab *tab, int g, int &b, int &i, int &j,
@@ -948,9 +972,11 @@ CAMR_HD bool reset_adj_eb(const double *s, const HydroConst &hc, double *b) {
 } // namespace
 
 int eb_adjoint_batch(const EbAdjEnt *tab, const EbAdjEnt *htab, int n, const HydroConst &hc,
-                     const double *dtp, hipStream_t st, Prof *prof) {
+                     const double *dtp, hipStream_t st, Prof *prof, const CutCell *cells,
+                     int ncells) {
     if (n <= 0) return CAMR_OK;
-    launch_adjs(std::make_integer_sequence<int, AS_COUNT>{}, tab, htab, n, hc, dtp, st, prof);
+    launch_adjs(std::make_integer_sequence<int, AS_COUNT>{}, tab, htab, n, hc, dtp, st, prof,
+                cells, ncells);
     return hipGetLastError() == hipSuccess ? CAMR_OK : CAMR_ERR_HIP;
 }
 
