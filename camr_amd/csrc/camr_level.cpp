@@ -298,7 +298,8 @@ class HydroLevel {
     int advance_staged(double dt);
     // the staged step leaves the CFL estimate of S_new in dtmin_ (its
     // one-pass tail with post_timestep's clean)
-    bool staged_dt() const { return !desc_.hp.do_mol && !defer_post_; }
+    bool staged_dt() const { return !defer_post_; }
+    void staged_tail(const DevFab *tnew, const DevFab *told, int mol);
     // cleaned: src went through clean_all (kb_expand_cleaned, one pass)
     void expand_all(const DevFab *src, bool cleaned = false);
     int hydro_on_workers(double dt);
@@ -1694,64 +1695,54 @@ int HydroLevel::advance_staged(double dt) {
     clean_all(told, true);
     // expand_state(Sborder, prev_time) (:99)
     expand_all(told, true);
-    if (!desc_.hp.do_mol) {
-        // construct_hydro_source (:138) per box on the worker streams (it
-        // reads Sborder, not S_new), then S_new = Sborder + old source +
-        // dt dS/dt and the clean_state passes up to post_timestep's
-        // (:112-154, CAMR.cpp:698-699) in one pass, repeated pass by pass
-        // by kb_tail_redo where a density floor would act
-        scratch_.reset();
+    scratch_.reset();
+    if (desc_.hp.do_mol) {
+        // first stage: MultiFab::Copy(S_new, Sborder) + old-source Saxpy
+        // (:112-118), the hydro source, S_new += dt dS/dt (:128)
+        launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told,
+                   (const double *)dt_dev_, desc_.const_grav, desc_.add_grav_src);
         int rc = hydro_on_workers(dt);
         if (rc) return rc;
-        const int post = defer_post_ ? 0 : 1;
-        // with post_timestep's clean the state is final: its CFL estimate
-        // (estTimeStep's minimum) comes out of the same pass
-        unsigned long long *dtm = post ? dtmin_ : nullptr;
-        if (dtm)
-            launch_on(k_fill_u64, make_box1(), stream_, dtmin_,
-                      (unsigned long long)0x7fefffffffffffffULL);
-        launch_all(kb_tail_fused, 0, tnew, (const DevFab *)tab_sb_, told,
-                   (const DevFab *)tab_dsdt_, (const double *)dt_dev_, desc_.const_grav,
-                   desc_.add_grav_src, post, hc_, redo_, ebf_, dtm, status_);
-        hipLaunchKernelGGL(kb_tail_redo, dim3(1), dim3(1024), 0, stream_, tnew,
-                           (const DevFab *)tab_sb_, told, (const DevFab *)tab_dsdt_,
-                           (int)box_.size(), (const double *)dt_dev_, desc_.const_grav,
-                           desc_.add_grav_src, post, hc_, status_, redo_, ebf_, dtm);
-        return CAMR_OK;
-    }
-    // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118)
-    launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told, (const double *)dt_dev_,
-               desc_.const_grav, desc_.add_grav_src);
-    // construct_hydro_source (:125 / :138) per box on the worker streams
-    scratch_.reset();
-    int rc = hydro_on_workers(dt);
-    if (rc) return rc;
-    // Saxpy (:128 / :141)
-    launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, (const double *)dt_dev_,
-               hc_.small_dens, status_, ebf_);
-    if (desc_.hp.do_mol) {
+        launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, (const double *)dt_dev_,
+                   hc_.small_dens, status_, ebf_);
         // second stage (:130-134): expand_state(Sborder, cur_time) from
-        // S_new, the hydro source of that state, then S_new = 0.5 Sborder
-        // + 0.5 S_old + 0.5 dt dSdt
+        // S_new and the hydro source of that state; the combination
+        // S_new = 0.5 Sborder + 0.5 S_old + 0.5 dt dSdt opens the tail
         expand_all(tnew);
         rc = hydro_on_workers(dt);
         if (rc) return rc;
-        launch_all(kb_mol_combine, 0, tnew, (const DevFab *)tab_sb_, told,
-                   (const DevFab *)tab_dsdt_, (const double *)dt_dev_, hc_.small_dens, status_,
-                   ebf_);
+        staged_tail(tnew, told, 1);
+        return CAMR_OK;
     }
-    // clean_state(S_new) (:146)
-    clean_all(tnew, false);
-    if (desc_.add_grav_src) {
-        // new-time source correction + clean_state (:150-154)
-        launch_all(kb_add_newsrc, 0, tnew, told, (const double *)dt_dev_, desc_.const_grav,
-                   hc_.small_dens, status_, ebf_);
-        clean_all(tnew, false);
-    }
-    // post_timestep: clean_state(S_new) (CAMR.cpp:698-699); the coarse
-    // level of a two-level step runs it after reflux and avgDown
-    if (!defer_post_) clean_all(tnew, true);
+    // construct_hydro_source (:138) per box on the worker streams (it reads
+    // Sborder, not S_new)
+    int rc = hydro_on_workers(dt);
+    if (rc) return rc;
+    staged_tail(tnew, told, 0);
     return CAMR_OK;
+}
+
+// The step's pointwise tail after the hydro source in one pass: S_new
+// (Sborder + old source + dt dS/dt, :112-141; with mol the second MOL
+// stage, :134), clean_state (:146), the new-time source and its
+// clean_state (:150-154) and post_timestep's clean_state (CAMR.cpp:
+// 698-699), then the CFL estimate of the result; repeated pass by pass by
+// kb_tail_redo where a density floor would act.
+void HydroLevel::staged_tail(const DevFab *tnew, const DevFab *told, int mol) {
+    const int post = defer_post_ ? 0 : 1;
+    // with post_timestep's clean the state is final: its CFL estimate
+    // (estTimeStep's minimum) comes out of the same pass
+    unsigned long long *dtm = post ? dtmin_ : nullptr;
+    if (dtm)
+        launch_on(k_fill_u64, make_box1(), stream_, dtmin_,
+                  (unsigned long long)0x7fefffffffffffffULL);
+    launch_all(kb_tail_fused, 0, tnew, (const DevFab *)tab_sb_, told, (const DevFab *)tab_dsdt_,
+               (const double *)dt_dev_, desc_.const_grav, desc_.add_grav_src, post, mol, hc_,
+               redo_, ebf_, dtm, status_);
+    hipLaunchKernelGGL(kb_tail_redo, dim3(1), dim3(1024), 0, stream_, tnew,
+                       (const DevFab *)tab_sb_, told, (const DevFab *)tab_dsdt_,
+                       (int)box_.size(), (const double *)dt_dev_, desc_.const_grav,
+                       desc_.add_grav_src, post, mol, hc_, status_, redo_, ebf_, dtm);
 }
 
 // expand_state (CAMR.cpp:1243-1250) from the states of `src`: FillPatch,

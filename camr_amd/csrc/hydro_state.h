@@ -96,20 +96,21 @@ __global__ void kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, c
                                   DomInfo D, Arr ext, int has_ext, HydroConst hc, int *status,
                                   EbFlags eb, FlatBlocks fb);
 // The staged step's pointwise tail after the hydro source in one pass
-// (S_new = Sborder + sources + dt dS/dt and the clean_state passes), and
+// (S_new = Sborder + sources + dt dS/dt, or with mol the second stage's
+// 0.5 Sborder + 0.5 S_old + 0.5 dt dS/dt, and the clean_state passes), and
 // its pass-by-pass repeat where a density floor would act (one workgroup of
 // up to 1024 threads; a no-op unless kb_tail_fused set *redo).
 __global__ void kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold,
                               const DevFab *dsdt, const double *dtp, double g, int add_grav,
-                              int post, HydroConst hc, int *redo, EbFlags eb,
+                              int post, int mol, HydroConst hc, int *redo, EbFlags eb,
                               unsigned long long *dtmin, int *status);
 // With dtmin: the CFL estimate of the new state into *dtmin (kb_estdt's),
 // set by kb_tail_fused (which expects DBL_MAX there) and, after a repeat,
 // by kb_tail_redo.
 __global__ void kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold,
                              const DevFab *dsdt, int nbox, const double *dtp, double g,
-                             int add_grav, int post, HydroConst hc, int *flags, int *redo,
-                             EbFlags eb, unsigned long long *dtmin);
+                             int add_grav, int post, int mol, HydroConst hc, int *flags,
+                             int *redo, EbFlags eb, unsigned long long *dtmin);
 __global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
                               double small_dens, int *flags, EbFlags eb);
 __global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,

@@ -574,24 +574,36 @@ __device__ __forceinline__ bool below_floor(double rho, double small_dens) {
 
 __device__ __forceinline__ double tail_point(const Arr sn, const Arr u, const Arr so,
                                              const Arr ds, double dt, double g, int add_grav,
-                                             int post, const HydroConst &hc, int *redo,
+                                             int post, int mol, const HydroConst &hc, int *redo,
                                              const EbFlags &eb, bool want_dt, int i, int j,
                                              int k) {
     const bool cov = eb.covered(i, j, k);
     double v[NVAR];
+    if (mol) {
+        // the second stage of the method of lines (kb_mol_combine):
+        // 0.5 Sborder + 0.5 S_old + 0.5 dt dS/dt
+        const double a = 0.5 * dt;
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) {
-        double x = u(i, j, k, n);
-        if (add_grav) {
-            double src = 0.0;
-            if (n == UMZ) src = g * so(i, j, k, URHO);
-            else if (n == UEDEN) src = g * so(i, j, k, UMZ);
-            x += dt * src;
+        for (int n = 0; n < NVAR; ++n) {
+            double x = 0.5 * u(i, j, k, n) + 0.5 * so(i, j, k, n);
+            x += a * ds(i, j, k, n);
+            v[n] = x;
         }
-        v[n] = x;
-    }
+    } else {
 #pragma unroll
-    for (int n = 0; n < NVAR; ++n) v[n] += dt * ds(i, j, k, n);
+        for (int n = 0; n < NVAR; ++n) {
+            double x = u(i, j, k, n);
+            if (add_grav) {
+                double src = 0.0;
+                if (n == UMZ) src = g * so(i, j, k, URHO);
+                else if (n == UEDEN) src = g * so(i, j, k, UMZ);
+                x += dt * src;
+            }
+            v[n] = x;
+        }
+#pragma unroll
+        for (int n = 0; n < NVAR; ++n) v[n] += dt * ds(i, j, k, n);
+    }
     bool fire = !cov && below_floor(v[URHO], hc.small_dens);
     normalize_species(v);
     reset_e_and_T(v, hc);
@@ -626,13 +638,13 @@ __device__ __forceinline__ double tail_point(const Arr sn, const Arr u, const Ar
 // (kb_estdt's minimum, exact: the minimum does not depend on the order).
 __global__ void __launch_bounds__(kBlock)
 kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
-              const double *dtp, double g, int add_grav, int post, HydroConst hc, int *redo,
-              EbFlags eb, unsigned long long *dtmin, int *status) {
+              const double *dtp, double g, int add_grav, int post, int mol, HydroConst hc,
+              int *redo, EbFlags eb, unsigned long long *dtmin, int *status) {
     int b, i, j, k;
     const bool in = batch_cell(snew, 0, b, i, j, k);
     double est = 1.7976931348623157e308;
     if (in) est = tail_point(snew[b].a, sb[b].a, sold[b].a, dsdt[b].a, *dtp, g, add_grav, post,
-                             hc, redo, eb, dtmin != nullptr, i, j, k);
+                             mol, hc, redo, eb, dtmin != nullptr, i, j, k);
     if (dtmin) block_min_commit(est, dtmin, status);
 }
 
@@ -642,8 +654,8 @@ kb_tail_fused(const DevFab *snew, const DevFab *sb, const DevFab *sold, const De
 // once unless kb_tail_fused set *redo, which it clears.
 __global__ void __launch_bounds__(1024)
 kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
-             int nbox, const double *dtp, double g, int add_grav, int post, HydroConst hc,
-             int *flags, int *redo, EbFlags eb, unsigned long long *dtmin) {
+             int nbox, const double *dtp, double g, int add_grav, int post, int mol,
+             HydroConst hc, int *flags, int *redo, EbFlags eb, unsigned long long *dtmin) {
     if (!*redo) return;
     const double dt = *dtp;
     auto each = [&](auto f) {
@@ -663,12 +675,25 @@ kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab *sold, const Dev
             for (int b = 0; b < nbox; ++b) floor_seq_box(snew[b].b, snew[b].a, hc, eb);
         __syncthreads();
     };
-    each([&](int b, int i, int j, int k) {
-        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, dt, g, add_grav, i, j, k);
-    });
-    each([&](int b, int i, int j, int k) {
-        pt_add_hydro(snew[b].a, dsdt[b].a, dt, hc.small_dens, flags, eb, i, j, k);
-    });
+    if (mol) {
+        each([&](int b, int i, int j, int k) {
+            const Arr S = snew[b].a, B = sb[b].a, O = sold[b].a, D = dsdt[b].a;
+            const double a = 0.5 * dt;
+            for (int n = 0; n < NVAR; ++n) {
+                double v = 0.5 * B(i, j, k, n) + 0.5 * O(i, j, k, n);
+                v += a * D(i, j, k, n);
+                S(i, j, k, n) = v;
+            }
+            if (!eb.covered(i, j, k)) floor_detect(S(i, j, k, URHO), hc.small_dens, flags);
+        });
+    } else {
+        each([&](int b, int i, int j, int k) {
+            pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, dt, g, add_grav, i, j, k);
+        });
+        each([&](int b, int i, int j, int k) {
+            pt_add_hydro(snew[b].a, dsdt[b].a, dt, hc.small_dens, flags, eb, i, j, k);
+        });
+    }
     floor();
     each([&](int b, int i, int j, int k) { pt_clean_point(snew[b].a, hc, i, j, k); });
     if (add_grav) {

@@ -664,15 +664,23 @@ def test_autotuned_level_stays_bitwise(oracle, tmp_path, monkeypatch):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"tuned box {b}")
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_staged_tail_floors_in_new_state_bitwise(oracle, graph):
+@pytest.mark.parametrize("graph,mol", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_staged_tail_floors_in_new_state_bitwise(oracle, graph, mol):
     # small_dens inside the density range of the field: enforce_min_density
     # acts on S_new after the update every step, where the staged engine's
-    # one-pass tail hands over to its pass-by-pass repeat
+    # one-pass tail (with mol: the second stage of the method of lines)
+    # hands over to its pass-by-pass repeat
     prob = workloads.smooth_periodic(n=16, ppm_type=0, max_grid=8)
-    prob.params["small_dens"] = 0.95
+    # (the method of lines at its CFL limit 0.3 moves the density less per
+    # step: a floor at the field's mean density acts after its update)
+    sd = 1.0 if mol else 0.95
+    prob.params["small_dens"] = sd
+    prob.params["do_mol"] = mol
+    if mol:
+        prob.params["cfl"] = 0.3  # the MOL integrator's limit
     p = _oracle_params(oracle, prob)
-    p.small_dens = 0.95
+    p.small_dens = sd
+    p.do_mol = mol
     ol = oracle.Level(p, prob.boxes)
     gl = Level(prob, engine=_lib.ENGINE_STAGED, graph=graph)
     for b in range(len(prob.boxes)):
@@ -689,7 +697,17 @@ def test_staged_tail_floors_in_new_state_bitwise(oracle, graph):
     for b in range(len(prob.boxes)):
         S = gl.get_box(b)
         assert_bitwise(S, ol.get_box(b), f"floor tail box {b}")
-        # a floor copies a neighbour's whole state: equal neighbours in the
-        # final S_new show that floors acted after the update
-        copied += int(np.all(S[:, :, :, 1:] == S[:, :, :, :-1], axis=0).sum())
+        # a floor copies a neighbour's whole state: equal neighbours (of the
+        # 26) in the final S_new show that floors acted after the update
+        n = S.shape[1:]
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dx == dy == dz == 0:
+                        continue
+                    a = S[:, max(dz, 0):n[0] + min(dz, 0), max(dy, 0):n[1] + min(dy, 0),
+                          max(dx, 0):n[2] + min(dx, 0)]
+                    c = S[:, max(-dz, 0):n[0] + min(-dz, 0), max(-dy, 0):n[1] + min(-dy, 0),
+                          max(-dx, 0):n[2] + min(-dx, 0)]
+                    copied += int(np.all(a == c, axis=0).sum())
     assert copied > 0
