@@ -1697,14 +1697,14 @@ int HydroLevel::advance_staged(double dt) {
     expand_all(told, true);
     scratch_.reset();
     if (desc_.hp.do_mol) {
-        // first stage: MultiFab::Copy(S_new, Sborder) + old-source Saxpy
-        // (:112-118), the hydro source, S_new += dt dS/dt (:128)
-        launch_all(kb_copy_oldsrc, 0, tnew, (const DevFab *)tab_sb_, told,
-                   (const double *)dt_dev_, desc_.const_grav, desc_.add_grav_src);
+        // first stage: the hydro source (it reads Sborder, not S_new), then
+        // MultiFab::Copy(S_new, Sborder) + old-source Saxpy (:112-118) and
+        // S_new += dt dS/dt (:128) in one pass
         int rc = hydro_on_workers(dt);
         if (rc) return rc;
-        launch_all(kb_add_hydro, 0, tnew, (const DevFab *)tab_dsdt_, (const double *)dt_dev_,
-                   hc_.small_dens, status_, ebf_);
+        launch_all(kb_copy_add_hydro, 0, tnew, (const DevFab *)tab_sb_, told,
+                   (const DevFab *)tab_dsdt_, (const double *)dt_dev_, desc_.const_grav,
+                   desc_.add_grav_src, hc_.small_dens, status_, ebf_);
         // second stage (:130-134): expand_state(Sborder, cur_time) from
         // S_new and the hydro source of that state; the combination
         // S_new = 0.5 Sborder + 0.5 S_old + 0.5 dt dSdt opens the tail

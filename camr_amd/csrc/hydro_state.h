@@ -68,10 +68,9 @@ __global__ void kb_resetT(const DevFab *S, int g, HydroConst hc);
 __global__ void kb_fill_raw(const DevFab *dst, int g, const DevFab *src, const int *owner,
                             DomInfo D, Arr ext, int has_ext, int *status);
 __global__ void kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D);
-__global__ void kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold,
-                               const double *dtp, double g, int add_grav);
-__global__ void kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp,
-                             double small_dens, int *flags, EbFlags eb);
+__global__ void kb_copy_add_hydro(const DevFab *snew, const DevFab *sb, const DevFab *sold,
+                                  const DevFab *dsdt, const double *dtp, double g, int add_grav,
+                                  double small_dens, int *flags, EbFlags eb);
 // A virtual box: s[0] x s[1] x s[2] boxes of a level, each len[0] x len[1]
 // x len[2] cells, that tile the box v, and an array over grow(v, g) (its
 // Sborder, or the adjoint of it).  ids[off + (c s[1] + r) s[0] + a]: the
@@ -111,11 +110,6 @@ __global__ void kb_tail_redo(const DevFab *snew, const DevFab *sb, const DevFab 
                              const DevFab *dsdt, int nbox, const double *dtp, double g,
                              int add_grav, int post, int mol, HydroConst hc, int *flags,
                              int *redo, EbFlags eb, unsigned long long *dtmin);
-__global__ void kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
-                              double small_dens, int *flags, EbFlags eb);
-__global__ void kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold,
-                               const DevFab *dsdt, const double *dtp, double small_dens,
-                               int *flags, EbFlags eb);
 __global__ void k_coarse_stage(Range r, Arr st, const DevFab *ctab, const int *cowner, DomInfo CD,
                                const DevFab *otab, int tmode, double ta, double tb);
 __global__ void k_coarse_stage_cells(const int *cells, int64_t ncells, Arr st, const DevFab *ctab,

@@ -477,20 +477,16 @@ kb_resync(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo
     if (batch_cell(dst, g, b, i, j, k)) pt_resync(dst[b].b, dst[b].a, src, owner, D, i, j, k);
 }
 
+// MultiFab::Copy(S_new, Sborder) + old-source Saxpy (pt_copy_oldsrc),
+// then S_new += dt dS/dt with floor detection (pt_add_hydro), on one cell
 __global__ void __launch_bounds__(kBlock)
-kb_copy_oldsrc(const DevFab *snew, const DevFab *sb, const DevFab *sold, const double *dtp,
-               double g, int add_grav) {
+kb_copy_add_hydro(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
+                  const double *dtp, double g, int add_grav, double small_dens, int *flags,
+                  EbFlags eb) {
     int b, i, j, k;
-    if (batch_cell(snew, 0, b, i, j, k))
-        pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, *dtp, g, add_grav, i, j, k);
-}
-
-__global__ void __launch_bounds__(kBlock)
-kb_add_hydro(const DevFab *snew, const DevFab *dsdt, const double *dtp, double small_dens,
-             int *flags, EbFlags eb) {
-    int b, i, j, k;
-    if (batch_cell(snew, 0, b, i, j, k))
-        pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
+    if (!batch_cell(snew, 0, b, i, j, k)) return;
+    pt_copy_oldsrc(snew[b].a, sb[b].a, sold[b].a, *dtp, g, add_grav, i, j, k);
+    pt_add_hydro(snew[b].a, dsdt[b].a, *dtp, small_dens, flags, eb, i, j, k);
 }
 
 // Every point of grow(b, g) is the image of at most one point of the
@@ -580,7 +576,7 @@ __device__ __forceinline__ double tail_point(const Arr sn, const Arr u, const Ar
     const bool cov = eb.covered(i, j, k);
     double v[NVAR];
     if (mol) {
-        // the second stage of the method of lines (kb_mol_combine):
+        // the second stage of the method of lines (CAMR_advance.cpp:134):
         // 0.5 Sborder + 0.5 S_old + 0.5 dt dS/dt
         const double a = 0.5 * dt;
 #pragma unroll
@@ -912,34 +908,6 @@ kb_avgdown(const DevFab *csnew, const DevFab *ftab, const int *fowner, DomInfo F
         csnew[b].a(i, j, k, n) = volfrac * c;
     }
 }
-
-// Second stage of the method of lines (CAMR_advance.cpp:133-134):
-// LinComb(S_new, 0.5, Sborder, 0.5, S_old), then Saxpy(S_new, 0.5 dt,
-// new_hydro_source), + floor detection for the following clean_state.
-__global__ void __launch_bounds__(kBlock)
-kb_mol_combine(const DevFab *snew, const DevFab *sb, const DevFab *sold, const DevFab *dsdt,
-               const double *dtp, double small_dens, int *flags, EbFlags eb) {
-    int b, i, j, k;
-    if (!batch_cell(snew, 0, b, i, j, k)) return;
-    const Arr S = snew[b].a, B = sb[b].a, O = sold[b].a, D = dsdt[b].a;
-    const double a = 0.5 * *dtp;
-#pragma unroll
-    for (int n = 0; n < NVAR; ++n) {
-        double v = 0.5 * B(i, j, k, n) + 0.5 * O(i, j, k, n);
-        v += a * D(i, j, k, n);
-        S(i, j, k, n) = v;
-    }
-    if (!eb.covered(i, j, k)) floor_detect(S(i, j, k, URHO), small_dens, flags);
-}
-
-__global__ void __launch_bounds__(kBlock)
-kb_add_newsrc(const DevFab *snew, const DevFab *sold, const double *dtp, double g,
-              double small_dens, int *flags, EbFlags eb) {
-    int b, i, j, k;
-    if (batch_cell(snew, 0, b, i, j, k))
-        pt_add_newsrc(snew[b].a, sold[b].a, *dtp, g, small_dens, flags, eb, i, j, k);
-}
-
 
 // ------------------------------------------------ coarse-to-fine ghost data
 // FillPatch's interpolation from the coarser level (AmrLevel::FillPatch,
