@@ -1757,14 +1757,22 @@ void HydroLevel::expand_all(const DevFab *src, bool cleaned) {
             [&](int r) -> int64_t { return grid_for(grow(exp_box_[r], ng_).npts()); }, fb);
         hipLaunchKernelGGL(kb_expand_cleaned, grid, dim3(kBlock), 0, stream_,
                            (const DevFab *)tab_exp_, ng_, src, (const int *)owner_, dom_, extv_,
-                           ext_ ? 1 : 0, hc_, status_, ebf_, fb);
+                           ext_ ? 1 : 0, hc_, status_, ebf_, fb, (int *)nullptr);
         return;
     }
-    if (cleaned) {
+    if (!tab_exp_) {
+        // one pass; from an uncleaned src, repeated pass by pass where the
+        // floor pass between the detection and normalize_species would act
         FlatBlocks fb;
         fb.n = 0;  // boxes of the level: blockIdx.y
+        int *rd = cleaned ? nullptr : redo_;
         launch_all(kb_expand_cleaned, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_,
-                   dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_, fb);
+                   dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_, fb, rd);
+        if (rd)
+            hipLaunchKernelGGL(kb_expand_redo, dim3(1), dim3(1024), 0, stream_,
+                               (const DevFab *)tab_sb_, (int)box_.size(), ng_, src,
+                               (const int *)owner_, dom_, extv_, ext_ ? 1 : 0, hc_, status_, ebf_,
+                               rd);
         return;
     }
     launch_all(kb_fill_raw, ng_, (const DevFab *)tab_sb_, ng_, src, (const int *)owner_, dom_,

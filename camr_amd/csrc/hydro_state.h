@@ -91,9 +91,15 @@ __global__ void kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_
 // expand_state of every box of dst (grow g) from the cleaned states of src:
 // kb_fill_raw, the detection, normalize_species, kb_resync and kb_resetT
 // in one pass (src must have been through clean_state).
+// With redo (a source that did not go through clean_state): where a valid
+// cell is below small_dens it sets *redo, and kb_expand_redo repeats the
+// passes one by one (one workgroup of up to 1024 threads).
 __global__ void kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner,
                                   DomInfo D, Arr ext, int has_ext, HydroConst hc, int *status,
-                                  EbFlags eb, FlatBlocks fb);
+                                  EbFlags eb, FlatBlocks fb, int *redo);
+__global__ void kb_expand_redo(const DevFab *dst, int nbox, int g, const DevFab *src,
+                               const int *owner, DomInfo D, Arr ext, int has_ext, HydroConst hc,
+                               int *status, EbFlags eb, int *redo);
 // The staged step's pointwise tail after the hydro source in one pass
 // (S_new = Sborder + sources + dt dS/dt, or with mol the second stage's
 // 0.5 Sborder + 0.5 S_old + 0.5 dt dS/dt, and the clean_state passes), and

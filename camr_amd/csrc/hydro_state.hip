@@ -520,7 +520,8 @@ kb_vscatter(const VGather *vt, const int *ids, const DevFab *sb_tab, int g, int 
 // floor pass between the detection and normalize_species finds nothing.
 __global__ void __launch_bounds__(kBlock)
 kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner, DomInfo D,
-                  Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb, FlatBlocks fb) {
+                  Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb, FlatBlocks fb,
+                  int *redo) {
     int b, lb, i, j, k;
     flat_entry(fb, b, lb);
     if (!range_point(range_of(grow(dst[b].b, g)), (int64_t)lb * blockDim.x + threadIdx.x, i, j, k))
@@ -544,7 +545,11 @@ kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner,
             v[n] = ((odd >> n) & 1u) ? -x : x;
         }
         if (dst[b].b.contains(i, j, k)) {
-            if (!eb.covered(i, j, k)) floor_detect(v[URHO], hc.small_dens, status);
+            if (!eb.covered(i, j, k)) {
+                floor_detect(v[URHO], hc.small_dens, status);
+                // an uncleaned source: the floor pass may act here
+                if (redo && (v[URHO] == 0. || v[URHO] < hc.small_dens)) atomicOr(redo, 1);
+            }
             normalize_species(v);
         } else if (!phys && ob >= 0) {
             normalize_species(v);
@@ -553,6 +558,44 @@ kb_expand_cleaned(const DevFab *dst, int g, const DevFab *src, const int *owner,
     reset_e_and_T(v, hc);
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) S(i, j, k, n) = v[n];
+}
+
+// expand_state pass by pass (kb_fill_raw, kb_detect, kb_floor_seq,
+// kb_normalize, kb_resync, kb_resetT) by one workgroup whose barriers order
+// the passes; returns at once unless kb_expand_cleaned set *redo (which it
+// clears).  For a source that did not go through clean_state.
+__global__ void __launch_bounds__(1024)
+kb_expand_redo(const DevFab *dst, int nbox, int g, const DevFab *src, const int *owner,
+               DomInfo D, Arr ext, int has_ext, HydroConst hc, int *status, EbFlags eb,
+               int *redo) {
+    if (!*redo) return;
+    auto each = [&](int gg, auto f) {
+        for (int b = 0; b < nbox; ++b) {
+            const Range r = range_of(grow(dst[b].b, gg));
+            for (int64_t t = threadIdx.x; t < r.n; t += blockDim.x) {
+                int i, j, k;
+                range_point(r, t, i, j, k);
+                f(b, i, j, k);
+            }
+        }
+        __syncthreads();
+    };
+    each(g, [&](int b, int i, int j, int k) {
+        pt_fill_raw(dst[b].a, src, owner, D, ext, has_ext, status, i, j, k);
+    });
+    each(0, [&](int b, int i, int j, int k) {
+        pt_detect(dst[b].a, hc.small_dens, status, eb, i, j, k);
+    });
+    if (threadIdx.x == 0 &&
+        (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFloorFlag))
+        for (int b = 0; b < nbox; ++b) floor_seq_box(dst[b].b, dst[b].a, hc, eb);
+    __syncthreads();
+    each(0, [&](int b, int i, int j, int k) { pt_normalize(dst[b].a, i, j, k); });
+    each(g, [&](int b, int i, int j, int k) {
+        pt_resync(dst[b].b, dst[b].a, dst, owner, D, i, j, k);
+    });
+    each(g, [&](int b, int i, int j, int k) { pt_resetT(dst[b].a, hc, i, j, k); });
+    if (threadIdx.x == 0) *redo = 0;
 }
 
 // ------------------------------------------- the staged step's tail fused
