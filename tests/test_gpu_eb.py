@@ -29,7 +29,7 @@ def _oracle_params(O, prob):
         domhi=prob.domhi, is_periodic=prob.is_periodic, dx=prob.dx, tile=UNTILED)
 
 
-@pytest.mark.parametrize("n,max_grid,nsteps", [(16, 16, 6), (32, 16, 5), (32, 8, 4)])
+@pytest.mark.parametrize("n,max_grid,nsteps", [(16, 16, 6), (32, 16, 5), (32, 8, 4), (24, 16, 4)])
 def test_eb_level_matches_oracle(oracle, n, max_grid, nsteps):
     print(f"EB level n={n} max_grid={max_grid} steps={nsteps}", flush=True)
     prob = workloads.sod_sphere(n=n, max_grid=max_grid, interface=0.4)
@@ -242,13 +242,15 @@ def test_eb_level_without_cut_cells_matches_plain_level():
         assert np.array_equal(A.get_box(b), B.get_box(b))
 
 
-def test_eb_merged_regular_boxes_match_box_by_box():
+@pytest.mark.parametrize("n,max_grid", [(32, 8), (24, 8)])
+def test_eb_merged_regular_boxes_match_box_by_box(n, max_grid):
     """The regular boxes of an EB level run as virtual boxes (blocks of equal
-    boxes that tile a box, their Sborder gathered from the boxes' own) and
-    box by box: the same dt sequence and the same state bit for bit, with
-    the 8^3 boxes of a 64x32x32 Sod-past-a-sphere grid merged into blocks."""
-    print("EB merge: 64x32x32, 8^3 boxes, merged vs box by box", flush=True)
-    prob = workloads.sod_sphere(n=32, max_grid=8, interface=0.4)
+    boxes that tile a box, their Sborder filled as one row of expand_state)
+    and box by box: the same dt sequence and the same state bit for bit, on
+    Sod-past-a-sphere grids of 8^3 boxes (64x32x32; 48x24x24, whose boxes
+    at the high faces are of other shapes)."""
+    print(f"EB merge: n={n} max_grid={max_grid}, merged vs box by box", flush=True)
+    prob = workloads.sod_sphere(n=n, max_grid=max_grid, interface=0.4)
     geom = workloads.sod_sphere_geometry(prob)
     A = Level(prob, engine=_lib.ENGINE_STAGED)
     A.set_eb(geom)
