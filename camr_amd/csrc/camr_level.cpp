@@ -1563,8 +1563,10 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             // regular box of an EB level: reg_engine_ has written its dS/dt;
             // cut-cell box: the batch above has
             if ((kind == 0 && eb_batched_) || kind == 1) continue;
-            launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
-            if (kind == 2) continue;  // covered box (CAMR_construct_hydro_source.cpp:80)
+            if (kind == 2) {  // covered box: no hydro (CAMR_construct_hydro_source.cpp:80)
+                launch_on(k_fill_value, B, bs, ds, (int)NVAR, 0.0);
+                continue;
+            }
             Arr q = dense(wq, G), qa = dense(wqa, G), sq = dense(wsq, G);
             launch_on(k_ctoprim, G, bs, sbv_[b], q, qa, hc_, status_);
             // primitive sources feed the Godunov predictor only (:129-143)
@@ -1592,6 +1594,9 @@ int HydroLevel::construct_hydro_source(double dt, bool add) {
             A.vol = ds;
             A.has_vol = 0;
             A.vol_c = hc_.dx[0] * hc_.dx[1] * hc_.dx[2];
+            // the hyd_src setVal(0) (:59) and hydro_update's += onto it, as
+            // the update writing 0 + the divergence into every valid cell
+            A.dsdt_zero = 1;
             int rc = desc_.hp.do_mol ? umdrv_mol(A, wscr, bs, &prof_) : umdrv_staged(A, wscr, bs, &prof_);
             if (rc) { set_error("kernel launch failed"); return rc; }
             // do_reflux: the box's fluxes into the flux registers (:243-254)

@@ -22,6 +22,9 @@ struct UmdrvArgs {
     Arr vol;
     int has_vol;
     double vol_c;
+    // 1: dsdt holds nothing yet; hydro_update's += onto its zero fill is
+    // written as 0 + the update (the same values, one pass fewer)
+    int dsdt_zero = 0;
 };
 
 HydroConst make_const(const camr_hydro_params &p);

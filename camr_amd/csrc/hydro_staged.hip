@@ -395,7 +395,7 @@ k_divu(Range r, Arr q, Arr divu, HydroConst hc) {
 __global__ void __launch_bounds__(kBlock)
 k_update(Range r, Arr dsdt, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
          Arr a0, Arr a1, Arr a2, int has_area, double ac0, double ac1,
-         double ac2, Arr vol, int has_vol, double volc) {
+         double ac2, Arr vol, int has_vol, double volc, int from_zero) {
     int i, j, k;
     if (!range_point(r, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const double A0l = has_area ? a0(i, j, k, 0) : ac0;
@@ -414,9 +414,10 @@ k_update(Range r, Arr dsdt, Arr f0, Arr f1, Arr f2, Arr g0, Arr g1, Arr g2,
     const double volinv = 1.0 / v;
 #pragma unroll
     for (int n = 0; n < NVAR; ++n) {
-        dsdt(i, j, k, n) += (f0(i, j, k, n) - f0(i + 1, j, k, n) + f1(i, j, k, n) -
-                             f1(i, j + 1, k, n) + f2(i, j, k, n) - f2(i, j, k + 1, n)) *
-                            volinv;
+        const double base = from_zero ? 0.0 : dsdt(i, j, k, n);
+        dsdt(i, j, k, n) = base + (f0(i, j, k, n) - f0(i + 1, j, k, n) + f1(i, j, k, n) -
+                                   f1(i, j + 1, k, n) + f2(i, j, k, n) - f2(i, j, k + 1, n)) *
+                                      volinv;
     }
     dsdt(i, j, k, UEINT) -= pdivu;
 }
@@ -572,7 +573,7 @@ int umdrv_staged(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof
     NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G3[0], G3[1], G3[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
-           A.area_c[2], A.vol, A.has_vol, A.vol_c);
+           A.area_c[2], A.vol, A.has_vol, A.vol_c, A.dsdt_zero);
     g_prof = nullptr;
     return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
 }
@@ -613,7 +614,7 @@ int umdrv_mol(const UmdrvArgs &A, double *scratch, hipStream_t st, Prof *prof) {
     NAMED("consup");
     launch(k_update, bx, st, A.dsdt, A.flx[0], A.flx[1], A.flx[2], G[0], G[1], G[2],
            A.area[0], A.area[1], A.area[2], A.has_area, A.area_c[0], A.area_c[1],
-           A.area_c[2], A.vol, A.has_vol, A.vol_c);
+           A.area_c[2], A.vol, A.has_vol, A.vol_c, A.dsdt_zero);
     g_prof = nullptr;
     return hipGetLastError() == hipSuccess ? 0 : CAMR_ERR_HIP;
 }
