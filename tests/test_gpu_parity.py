@@ -664,8 +664,9 @@ def test_autotuned_level_stays_bitwise(oracle, tmp_path, monkeypatch):
         assert_bitwise(gl.get_box(b), ol.get_box(b), f"tuned box {b}")
 
 
-@pytest.mark.parametrize("graph,mol", [(False, 0), (True, 0), (False, 1), (True, 1)])
-def test_staged_tail_floors_in_new_state_bitwise(oracle, graph, mol):
+@pytest.mark.parametrize("graph,mol,grav", [(False, 0, 0.0), (True, 0, 0.0), (False, 1, 0.0),
+                                            (True, 1, 0.0), (False, 0, -0.7)])
+def test_staged_tail_floors_in_new_state_bitwise(oracle, graph, mol, grav):
     # small_dens inside the density range of the field: enforce_min_density
     # acts on S_new after the update every step, where the staged engine's
     # one-pass tail (with mol: the second stage of the method of lines)
@@ -676,6 +677,7 @@ def test_staged_tail_floors_in_new_state_bitwise(oracle, graph, mol):
     sd = 1.0 if mol else 0.95
     prob.params["small_dens"] = sd
     prob.params["do_mol"] = mol
+    prob.params["const_grav"] = grav  # with gravity: the new-time source and its clean
     if mol:
         prob.params["cfl"] = 0.3  # the MOL integrator's limit
     p = _oracle_params(oracle, prob)
@@ -710,4 +712,6 @@ def test_staged_tail_floors_in_new_state_bitwise(oracle, graph, mol):
                     c = S[:, max(-dz, 0):n[0] + min(-dz, 0), max(-dy, 0):n[1] + min(-dy, 0),
                           max(-dx, 0):n[2] + min(-dx, 0)]
                     copied += int(np.all(a == c, axis=0).sum())
-    assert copied > 0
+    # (with gravity the new-time source, which differs cell by cell, acts
+    # on the copies after the first floor pass, so they need not survive)
+    assert copied > 0 or grav != 0.0
